@@ -1,0 +1,177 @@
+"""Host-side mirror of HERCULES's time-stepping driver over the C ABI.
+
+``Channel`` owns one device-resident channel (an ``hc_ctx``) and exposes the
+reference's module subroutines by name: ``time_advancement_fd``,
+``poisson_solver_fft``, ``calculate_ubar``, ``screen_div_error`` and the
+drive loop body (src/module_navier_stokes.f90:31-365).  Fields cross as
+interior arrays of shape (nz, ny, nx) — the reference's Fortran (i,j,k)
+interior order — or as the reference's ghosted (nx+4, ny+4, nz+4) arrays.
+
+Every call goes to the HIP library; there is no CPU fallback.
+"""
+from __future__ import annotations
+
+import ctypes as C
+import os
+
+import numpy as np
+
+from .params import Params
+
+_PKG = os.path.dirname(os.path.abspath(__file__))
+LIB_PATH = os.path.join(_PKG, "libhercules_amd.so")
+
+FIELDS = {"u": 0, "v": 1, "w": 2, "p": 3, "t": 4, "ff": 5, "gg": 6, "hh": 7, "ss": 8}
+EQUATIONS = {"u": 0, "v": 1, "w": 2, "p": 3, "t": 4}
+LAYOUT_INTERIOR, LAYOUT_GHOSTED = 0, 1
+ABI_SYMBOLS = ("hc_last_error", "hc_create", "hc_destroy", "hc_set_field", "hc_get_field",
+               "hc_init", "hc_time_advancement_fd", "hc_drive_steps", "hc_poisson_solver_fft",
+               "hc_calculate_ubar", "hc_screen_div_error", "hc_get_scalar", "hc_stream",
+               "hc_field_device_ptr", "hc_synchronize")
+
+
+class HerculesError(RuntimeError):
+    def __init__(self, code, msg):
+        super().__init__(f"[hc status {code}] {msg}")
+        self.code = code
+
+
+_lib = None
+
+
+def load_library(path: str = LIB_PATH):
+    """Load the HIP library; raises if it has not been built."""
+    global _lib
+    if _lib is not None:
+        return _lib
+    if not os.path.exists(path):
+        raise RuntimeError(f"HIP library missing: {path}; run __graft_entry__.build()")
+    L = C.CDLL(path)
+    vp, dp, ip = C.c_void_p, C.POINTER(C.c_double), C.c_int
+    L.hc_last_error.restype = C.c_char_p
+    L.hc_create.argtypes = [vp, C.POINTER(vp)]
+    L.hc_destroy.argtypes = [vp]
+    L.hc_set_field.argtypes = [vp, ip, vp, ip]
+    L.hc_get_field.argtypes = [vp, ip, vp, ip]
+    L.hc_init.argtypes = [vp]
+    L.hc_time_advancement_fd.argtypes = [vp]
+    L.hc_drive_steps.argtypes = [vp, ip]
+    L.hc_poisson_solver_fft.argtypes = [vp, ip, C.c_double, vp, vp, ip]
+    L.hc_calculate_ubar.argtypes = [vp, dp]
+    L.hc_screen_div_error.argtypes = [vp, dp, dp]
+    L.hc_get_scalar.argtypes = [vp, ip, dp]
+    L.hc_stream.restype = vp
+    L.hc_stream.argtypes = [vp]
+    L.hc_field_device_ptr.restype = vp
+    L.hc_field_device_ptr.argtypes = [vp, ip]
+    L.hc_synchronize.argtypes = [vp]
+    _lib = L
+    return L
+
+
+def _ptr(a: np.ndarray):
+    return C.c_void_p(a.ctypes.data)
+
+
+class Channel:
+    """One channel resident on the current HIP device."""
+
+    def __init__(self, prm: Params):
+        self.prm = prm
+        self.L = load_library()
+        self._cp = prm.to_c()
+        h = C.c_void_p()
+        self._check(self.L.hc_create(C.byref(self._cp), C.byref(h)))
+        self.h = h
+        self.shape = (prm.nz, prm.ny, prm.nx)
+        self.ghost_shape = (prm.nz + 4, prm.ny + 4, prm.nx + 4)
+
+    def _check(self, rc):
+        if rc:
+            raise HerculesError(rc, self.L.hc_last_error().decode())
+
+    def close(self):
+        if getattr(self, "h", None):
+            self.L.hc_destroy(self.h)
+            self.h = None
+
+    def __enter__(self):
+        return self
+
+    def __exit__(self, *a):
+        self.close()
+
+    def __del__(self):
+        try:
+            self.close()
+        except Exception:
+            pass
+
+    # ---- fields
+    def set_field(self, name, arr, layout=LAYOUT_INTERIOR):
+        shape = self.shape if layout == LAYOUT_INTERIOR else self.ghost_shape
+        a = np.ascontiguousarray(arr, dtype=np.float64)
+        if a.shape != shape:
+            raise ValueError(f"{name}: shape {a.shape} != {shape}")
+        self._check(self.L.hc_set_field(self.h, FIELDS[name], _ptr(a), layout))
+
+    def get_field(self, name, layout=LAYOUT_INTERIOR, out=None):
+        shape = self.shape if layout == LAYOUT_INTERIOR else self.ghost_shape
+        a = np.zeros(shape) if out is None else out
+        self._check(self.L.hc_get_field(self.h, FIELDS[name], _ptr(a), layout))
+        return a
+
+    def set_state(self, **fields):
+        for k, v in fields.items():
+            self.set_field(k, v)
+
+    # ---- reference subroutines
+    def init(self):
+        self._check(self.L.hc_init(self.h))
+
+    def time_advancement_fd(self):
+        self._check(self.L.hc_time_advancement_fd(self.h))
+
+    def steps(self, n):
+        self._check(self.L.hc_drive_steps(self.h, int(n)))
+
+    def poisson_solver_fft(self, equation, rk_c, src, layout=LAYOUT_INTERIOR):
+        shape = self.shape if layout == LAYOUT_INTERIOR else self.ghost_shape
+        s = np.ascontiguousarray(src, dtype=np.float64)
+        if s.shape != shape:
+            raise ValueError(f"source shape {s.shape} != {shape}")
+        out = np.zeros(shape)
+        self._check(self.L.hc_poisson_solver_fft(self.h, EQUATIONS[equation], float(rk_c),
+                                                 _ptr(s), _ptr(out), layout))
+        return out
+
+    def poisson_solver_fft_device(self, equation, rk_c, src_ptr: int, dst_ptr: int):
+        """Device-resident variant: src/dst are device addresses of (nz,ny,nx) arrays."""
+        self._check(self.L.hc_poisson_solver_fft(self.h, EQUATIONS[equation], float(rk_c),
+                                                 C.c_void_p(src_ptr), C.c_void_p(dst_ptr),
+                                                 LAYOUT_INTERIOR))
+
+    def calculate_ubar(self):
+        v = C.c_double()
+        self._check(self.L.hc_calculate_ubar(self.h, C.byref(v)))
+        return v.value
+
+    def screen_div_error(self):
+        a, m = C.c_double(), C.c_double()
+        self._check(self.L.hc_screen_div_error(self.h, C.byref(a), C.byref(m)))
+        return a.value, m.value
+
+    def scalar(self, name):
+        v = C.c_double()
+        ids = {"dpx_drive": 0, "ubar": 1, "time_sim": 2, "got": 3, "stps": 4}
+        self._check(self.L.hc_get_scalar(self.h, ids[name], C.byref(v)))
+        return v.value
+
+    def stream(self) -> int:
+        return self.L.hc_stream(self.h) or 0
+
+    def device_ptr(self, name) -> int:
+        return self.L.hc_field_device_ptr(self.h, FIELDS[name]) or 0
+
+    def synchronize(self):
+        self._check(self.L.hc_synchronize(self.h))

@@ -1,0 +1,1192 @@
+// MI355X-native HERCULES time-step hot path: kernels and the C ABI
+// declared in include/hercules_amd.h.
+//
+// One substep of the finite-difference RK3 step (time_advancement_fd,
+// src/module_navier_stokes.f90:247-355) is five launches:
+//   k_rhs_mom     right-hand sides ff,gg,hh and qu,qv,qw, fused with the
+//                 x-direction real-to-complex FFT of the q's
+//   k_yfft<-1>    y-direction FFT of the three q spectra
+//   k_column_mom  per (l,m) column: Helmholtz solves for u*, v*, w*, the
+//                 divergence source, the pressure Poisson solve, the
+//                 projection and the pressure increment, all in spectral
+//                 space (exact equivalents of get_p_eqn_src, calculate_uvw
+//                 and calculate_new_p for periodic x, y)
+//   k_yfft<+1>    inverse y FFT of u, v, w and the pressure increment
+//   k_finish      inverse x FFT; stores u, v, w and accumulates p
+// plus, with temperature, the same chain for T.
+#include <hip/hip_runtime.h>
+
+#include <cmath>
+#include <cstdarg>
+#include <cstdio>
+#include <cstring>
+#include <string>
+#include <vector>
+
+#include "../../include/hercules_amd.h"
+#include "hc_kernels.h"
+
+// ---------------------------------------------------------------------------
+// error handling
+static thread_local std::string g_err;
+static int set_err(int code, const char *fmt, ...)
+{
+    char buf[512];
+    va_list ap;
+    va_start(ap, fmt);
+    vsnprintf(buf, sizeof buf, fmt, ap);
+    va_end(ap);
+    g_err = buf;
+    return code;
+}
+#define HIPCHK(x)                                                                          \
+    do {                                                                                   \
+        hipError_t e_ = (x);                                                               \
+        if (e_ != hipSuccess) return set_err(HC_ERR_HIP, "%s: %s", #x, hipGetErrorString(e_)); \
+    } while (0)
+
+extern "C" const char *hc_last_error(void) { return g_err.c_str(); }
+
+// ---------------------------------------------------------------------------
+// kernels
+constexpr int kRhsThreads = 256;
+constexpr int kRowPoints = 1024;  // points of one block of the row kernels
+
+template <int NX>
+constexpr int rows_per_block() { return NX >= kRowPoints ? 1 : kRowPoints / NX; }
+
+// Momentum RHS + x R2C of qu,qv,qw.  Block: rows j0..j0+ROWS-1 of level k.
+template <int CDS, int NX>
+__global__ __launch_bounds__(kRhsThreads) void k_rhs_mom(StepArgs A)
+{
+    constexpr int ROWS = rows_per_block<NX>();
+    constexpr int NL = NX / 2;
+    __shared__ double2 sq_[3 * ROWS * NL];
+    double *sqd = reinterpret_cast<double *>(sq_);
+    const int k = blockIdx.y, j0 = blockIdx.x * ROWS, tid = threadIdx.x;
+    const double dpx = A.dp_opt == 2 ? A.scal[0] : A.dpx;
+    for (int p = tid; p < ROWS * NX; p += kRhsThreads) {
+        int jj = p / NX, i = p - jj * NX, j = j0 + jj;
+        double qu = 0.0, qv = 0.0, qw = 0.0;
+        if (j < A.ny) {
+            Pt q{A, i, j, k};
+            double f, g, h;
+            momentum_point<CDS>(q, dpx, f, g, h, qu, qv, qw);
+            size_t c = ((size_t)k * A.ny + j) * A.nx + i;
+            A.ff[c] = f;
+            A.gg[c] = g;
+            A.hh[c] = h;
+        }
+        sqd[(0 * ROWS + jj) * NX + i] = qu;
+        sqd[(1 * ROWS + jj) * NX + i] = qv;
+        sqd[(2 * ROWS + jj) * NX + i] = qw;
+    }
+    __syncthreads();
+    lds_fft<NL, 3 * ROWS, NL, 1, kRhsThreads, -1>(sq_, A.tw_nl, tid);
+    double2 *outs[3] = {A.s0, A.s1, A.s2};
+    for (int p = tid; p < 3 * ROWS * NL; p += kRhsThreads) {
+        int row = p / NL, l = p - row * NL, f = row / ROWS, jj = row - f * ROWS, j = j0 + jj;
+        if (j < A.ny) outs[f][((size_t)k * A.ny + j) * NL + l] = r2c_post<NL>(sq_ + row * NL, l, A.tw_nx, A.norm);
+    }
+}
+
+// Temperature RHS + x R2C of qt.
+template <int CDS, int NX>
+__global__ __launch_bounds__(kRhsThreads) void k_rhs_t(StepArgs A)
+{
+    constexpr int ROWS = rows_per_block<NX>();
+    constexpr int NL = NX / 2;
+    __shared__ double2 sq_[ROWS * NL];
+    double *sqd = reinterpret_cast<double *>(sq_);
+    const int k = blockIdx.y, j0 = blockIdx.x * ROWS, tid = threadIdx.x;
+    for (int p = tid; p < ROWS * NX; p += kRhsThreads) {
+        int jj = p / NX, i = p - jj * NX, j = j0 + jj;
+        double qt = 0.0;
+        if (j < A.ny) {
+            Pt q{A, i, j, k};
+            double s;
+            temperature_point<CDS>(q, s, qt);
+            A.ss[((size_t)k * A.ny + j) * A.nx + i] = s;
+        }
+        sqd[jj * NX + i] = qt;
+    }
+    __syncthreads();
+    lds_fft<NL, ROWS, NL, 1, kRhsThreads, -1>(sq_, A.tw_nl, tid);
+    for (int p = tid; p < ROWS * NL; p += kRhsThreads) {
+        int jj = p / NL, l = p - jj * NL, j = j0 + jj;
+        if (j < A.ny) A.s0[((size_t)k * A.ny + j) * NL + l] = r2c_post<NL>(sq_ + jj * NL, l, A.tw_nx, A.norm);
+    }
+}
+
+// Plain x R2C of one real field (standalone solve): src -> s0, scaled by norm.
+template <int NX>
+__global__ __launch_bounds__(kRhsThreads) void k_x_r2c(const double *__restrict__ src, double2 *dst, int ny,
+                                                       const double2 *tw_nl, const double2 *tw_nx, double norm)
+{
+    constexpr int ROWS = rows_per_block<NX>();
+    constexpr int NL = NX / 2;
+    __shared__ double2 sq_[ROWS * NL];
+    double *sqd = reinterpret_cast<double *>(sq_);
+    const int k = blockIdx.y, j0 = blockIdx.x * ROWS, tid = threadIdx.x;
+    for (int p = tid; p < ROWS * NX; p += kRhsThreads) {
+        int jj = p / NX, i = p - jj * NX, j = j0 + jj;
+        sqd[jj * NX + i] = j < ny ? src[((size_t)k * ny + j) * NX + i] : 0.0;
+    }
+    __syncthreads();
+    lds_fft<NL, ROWS, NL, 1, kRhsThreads, -1>(sq_, tw_nl, tid);
+    for (int p = tid; p < ROWS * NL; p += kRhsThreads) {
+        int jj = p / NL, l = p - jj * NL, j = j0 + jj;
+        if (j < ny) dst[((size_t)k * ny + j) * NL + l] = r2c_post<NL>(sq_ + jj * NL, l, tw_nx, norm);
+    }
+}
+
+// Inverse x transform (C2R) of F spectra rows into physical rows.
+// mode[f]: 0 store, 1 accumulate (dst += value).
+struct C2RArgs {
+    const double2 *src[4];
+    double *dst[4];
+    int mode[4];
+    int ny;
+    const double2 *tw_nl, *tw_nx;
+};
+template <int NX, int F>
+__global__ __launch_bounds__(kRhsThreads) void k_x_c2r(C2RArgs A)
+{
+    constexpr int ROWS = rows_per_block<NX>() / F > 0 ? rows_per_block<NX>() / F : 1;
+    constexpr int NL = NX / 2;
+    constexpr int B = F * ROWS;
+    __shared__ double2 sq_[B * NL];
+    const int k = blockIdx.y, j0 = blockIdx.x * ROWS, tid = threadIdx.x;
+    for (int p = tid; p < B * NL; p += kRhsThreads) {
+        int row = p / NL, l = p - row * NL, f = row / ROWS, jj = row - f * ROWS, j = j0 + jj;
+        double2 z = czero();
+        if (j < A.ny) {
+            const double2 *s = A.src[f] + ((size_t)k * A.ny + j) * NL;
+            double2 xl = s[l];
+            double2 xt = l == 0 ? czero() : cconj(s[NL - l]);
+            z = c2r_pre(xl, xt, A.tw_nx[l]);
+        }
+        sq_[row * NL + l] = z;
+    }
+    __syncthreads();
+    lds_fft<NL, B, NL, 1, kRhsThreads, +1>(sq_, A.tw_nl, tid);
+    const double *sqd = reinterpret_cast<const double *>(sq_);
+    for (int p = tid; p < B * NX; p += kRhsThreads) {
+        int row = p / NX, i = p - row * NX, f = row / ROWS, jj = row - f * ROWS, j = j0 + jj;
+        if (j < A.ny) {
+            size_t c = ((size_t)k * A.ny + j) * NX + i;
+            double x = sqd[row * NX + i];
+            if (A.mode[f] == 1) A.dst[f][c] += x;
+            else A.dst[f][c] = x;
+        }
+    }
+}
+
+// y-direction FFT of up to 4 spectra, in place.  Block: TL consecutive l of
+// level k, all j.  Forward zeroes the Nyquist row after the transform,
+// inverse before it (src/module_tools.f90:538-572,659-693).
+constexpr int kYTile = 8;
+struct YArgs {
+    double2 *f[4];
+    int nl, nz;
+    const double2 *tw_ny;
+};
+template <int NY, int SIGN>
+__global__ __launch_bounds__(256) void k_yfft(YArgs A)
+{
+    __shared__ double2 tile[NY * kYTile];
+    double2 *fld = A.f[blockIdx.z];
+    const int l0 = blockIdx.x * kYTile, k = blockIdx.y, tid = threadIdx.x;
+    for (int p = tid; p < NY * kYTile; p += 256) {
+        int j = p / kYTile, ll = p - j * kYTile;
+        double2 v = fld[((size_t)k * NY + j) * A.nl + l0 + ll];
+        if (SIGN > 0 && j == NY / 2) v = czero();
+        tile[p] = v;
+    }
+    __syncthreads();
+    lds_fft<NY, kYTile, 1, kYTile, 256, SIGN>(tile, A.tw_ny, tid);
+    for (int p = tid; p < NY * kYTile; p += 256) {
+        int j = p / kYTile, ll = p - j * kYTile;
+        double2 v = tile[p];
+        if (SIGN < 0 && j == NY / 2) v = czero();
+        fld[((size_t)k * NY + j) * A.nl + l0 + ll] = v;
+    }
+}
+
+// ---------------------------------------------------------------------------
+// Column kernels: one (l, m) column per wavefront, eight consecutive l per
+// block, levels spread over the lanes.
+constexpr int kColWaves = 8;
+
+struct ColMap {
+    int lane, cnt, k0;
+};
+HC_DEV ColMap col_map(int nz)
+{
+    ColMap m;
+    m.lane = threadIdx.x & 63;
+    int base = nz / 64, rem = nz % 64;
+    m.cnt = base + (m.lane < rem ? 1 : 0);
+    m.k0 = m.lane * base + (m.lane < rem ? m.lane : rem);
+    return m;
+}
+
+// value of level (k0+cnt-1) from register array v (runtime index)
+template <int L>
+HC_DEV double2 last_of(const double2 (&v)[L], int cnt)
+{
+    double2 r = czero();
+#pragma unroll
+    for (int q = 0; q < L; ++q)
+        if (q == cnt - 1) r = v[q];
+    return r;
+}
+HC_DEV double2 shfl_up_c(double2 v, int d)
+{
+    return make_double2(__shfl_up(v.x, d, 64), __shfl_up(v.y, d, 64));
+}
+HC_DEV double2 shfl_dn_c(double2 v, int d)
+{
+    return make_double2(__shfl_down(v.x, d, 64), __shfl_down(v.y, d, 64));
+}
+
+// Assemble the rows of one equation for this lane's levels.  eq: 0 u/v,
+// 1 w, 2 p, 3 t.  The diagonal base of the (0,0) mode differs at the wall
+// rows because get_{u,w,t}_eqn_coeff zero the 1-D aa(k)/cc(k) while visiting
+// the first mode (src/module_poisson_solver.f90:257-272,361-366,455-469).
+template <int L>
+HC_DEV void assemble(const StepArgs &A, int eq, int l, int jm, bool is00, const ColMap &m, double (&a)[L],
+                     double (&b)[L], double (&c)[L])
+{
+    const int nz = A.nz, nl = A.nl, ny = A.ny;
+    const double rc = A.rc;
+    int ta, tc, td, td00;
+    double hxy;
+    switch (eq) {
+    case 0: ta = LV_AU; tc = LV_CU; td = LV_DU; td00 = LV_DU00;
+        hxy = A.symx[SY_HU * nl + l] + A.symy[SY_HU * ny + jm]; break;
+    case 1: ta = LV_AW; tc = LV_CW; td = LV_DW; td00 = LV_DW00;
+        hxy = A.symx[SY_HU * nl + l] + A.symy[SY_HU * ny + jm]; break;
+    case 2: ta = LV_AP; tc = LV_CP; td = LV_DP; td00 = LV_DP00;
+        hxy = A.symx[SY_PX * nl + l] + A.symy[SY_PX * ny + jm]; break;
+    default: ta = LV_AT; tc = LV_CT; td = LV_DT; td00 = LV_DT00;
+        hxy = A.symx[SY_HT * nl + l] + A.symy[SY_HT * ny + jm]; break;
+    }
+#pragma unroll
+    for (int r = 0; r < L; ++r) {
+        a[r] = 0.0; b[r] = 1.0; c[r] = 0.0;
+        if (r < m.cnt) {
+            int k = m.k0 + r;
+            if (eq == 1 && k == nz - 1) continue;  // w at kend3: trivial row, w = 0
+            a[r] = rc * A.lev[ta * nz + k];
+            c[r] = rc * A.lev[tc * nz + k];
+            double dg = A.lev[(is00 ? td00 : td) * nz + k];
+            if (eq == 2) {
+                b[r] = rc * (hxy + dg);
+                if (is00 && k == 0) { a[r] = 0.0; c[r] = 0.0; }  // delnull (:558-564)
+            } else {
+                b[r] = 1.0 + rc * (-hxy + dg);
+            }
+        }
+    }
+}
+
+// (0,0)-mode right-hand-side fix-ups of poisson_solver_fft (:572-607)
+template <int L>
+HC_DEV void fixups(const StepArgs &A, int eq, const ColMap &m, double2 (&d)[L])
+{
+#pragma unroll
+    for (int r = 0; r < L; ++r) {
+        if (r >= m.cnt) continue;
+        int k = m.k0 + r;
+        if (eq == 0) {  // ubnd
+            if (k == 0) d[r].x += 2.0 * (A.rc * A.aU0) * A.u_mrf;
+            if (k == A.nz - 1 && A.ochannel == 0) d[r].x += 2.0 * (A.rc * A.cUe) * A.u_mrf;
+        } else if (eq == 3) {  // tbnd
+            if (k == 0) d[r].x -= 2.0 * (A.rc * A.aT0) * A.tbot;
+            if (k == A.nz - 1) d[r].x -= 2.0 * (A.rc * A.cTe) * A.ttop;
+        } else if (eq == 2) {  // delnull
+            if (k == 0) d[r] = czero();
+        }
+    }
+}
+
+template <int L>
+__global__ __launch_bounds__(64 * kColWaves) void k_column_mom(StepArgs A)
+{
+    const int l = blockIdx.x * kColWaves + (threadIdx.x >> 6), jm = blockIdx.y;
+    if (l >= A.nl) return;  // whole wavefront leaves; no block barrier below
+    const ColMap m = col_map(A.nz);
+    const int nz = A.nz, nl = A.nl, ny = A.ny;
+    const bool is00 = (l == 0 && jm == 0);
+    const size_t plane = (size_t)ny * nl, col = (size_t)jm * nl + l;
+    double2 duv[2][L], dw[1][L], dp[1][L];
+#pragma unroll
+    for (int r = 0; r < L; ++r) {
+        duv[0][r] = duv[1][r] = dw[0][r] = czero();
+        if (r < m.cnt) {
+            size_t o = (size_t)(m.k0 + r) * plane + col;
+            duv[0][r] = A.s0[o];
+            duv[1][r] = A.s1[o];
+            dw[0][r] = A.s2[o];
+        }
+    }
+    double a[L], b[L], c[L];
+    // u and v share the matrix (assign_abc(aa_u,...) for both, :291-299)
+    assemble<L>(A, 0, l, jm, is00, m, a, b, c);
+    if (is00) fixups<L>(A, 0, m, duv[0]);
+    wave_tridiag<L, 2>(m.cnt, a, b, c, duv);
+    // w (:302-304): kend3 row is trivial and its right-hand side is unused
+#pragma unroll
+    for (int r = 0; r < L; ++r)
+        if (r < m.cnt && m.k0 + r == nz - 1) dw[0][r] = czero();
+    assemble<L>(A, 1, l, jm, is00, m, a, b, c);
+    wave_tridiag<L, 1>(m.cnt, a, b, c, dw);
+    // divergence source /dt (get_p_eqn_src, :127-176) in spectral form
+    const double2 sdx = make_double2(A.symx[SY_DRE * nl + l], A.symx[SY_DIM * nl + l]);
+    const double2 sdy = make_double2(A.symy[SY_DRE * ny + jm], A.symy[SY_DIM * ny + jm]);
+    double2 wprev = shfl_up_c(last_of<L>(dw[0], m.cnt), 1);
+    if (m.lane == 0) wprev = czero();
+#pragma unroll
+    for (int r = 0; r < L; ++r) {
+        dp[0][r] = czero();
+        if (r < m.cnt) {
+            int k = m.k0 + r;
+            double2 wm = r == 0 ? wprev : dw[0][r > 0 ? r - 1 : 0];
+            double dz = A.lev[LV_DZ * nz + k];
+            double2 dv = cadd(cmul(sdx, duv[0][r]), cmul(sdy, duv[1][r]));
+            double2 dwz = csub(dw[0][r], wm);
+            dv.x += dwz.x / dz;
+            dv.y += dwz.y / dz;
+            dp[0][r] = cscale(dv, 1.0 / A.dt);
+        }
+    }
+    // pressure Poisson (:312-314)
+    assemble<L>(A, 2, l, jm, is00, m, a, b, c);
+    if (is00) fixups<L>(A, 2, m, dp[0]);
+    wave_tridiag_refined<L, 1>(m.cnt, a, b, c, dp);
+    // projection (calculate_uvw, :659-734) and pressure increment
+    // (calculate_new_p, :739-808)
+    const double cdt = A.rc * A.dt;
+    const double2 sgx = make_double2(A.symx[SY_GRE * nl + l], A.symx[SY_GIM * nl + l]);
+    const double2 sgy = make_double2(A.symy[SY_GRE * ny + jm], A.symy[SY_GIM * ny + jm]);
+    const double slap = A.symx[SY_L * nl + l] + A.symy[SY_L * ny + jm];
+    double2 pprev = shfl_up_c(last_of<L>(dp[0], m.cnt), 1);
+    double2 pnext = shfl_dn_c(dp[0][0], 1);
+    const double cnu = A.rc * 0.5 * A.dt * A.nu;
+#pragma unroll
+    for (int r = 0; r < L; ++r) {
+        if (r < m.cnt) {
+            int k = m.k0 + r;
+            double2 ph = dp[0][r];
+            double2 pm = r == 0 ? (m.lane == 0 ? ph : pprev) : dp[0][r > 0 ? r - 1 : 0];
+            double2 pn = ph;
+            if (k < nz - 1) {
+                bool nxt_in_lane = (r + 1 < m.cnt);
+                double2 cand = czero();
+#pragma unroll
+                for (int q = 0; q < L; ++q)
+                    if (q == r + 1) cand = dp[0][q];
+                pn = nxt_in_lane ? cand : pnext;
+            }
+            duv[0][r] = csub(duv[0][r], cscale(cmul(sgx, ph), cdt));
+            duv[1][r] = csub(duv[1][r], cscale(cmul(sgy, ph), cdt));
+            double dzt = A.lev[LV_DZT * nz + k], dzb = A.lev[LV_DZB * nz + k], dz = A.lev[LV_DZ * nz + k];
+            if (k < nz - 1) {
+                double2 gz = csub(pn, ph);
+                dw[0][r] = make_double2(dw[0][r].x - cdt * gz.x / dzt, dw[0][r].y - cdt * gz.y / dzt);
+            }
+            double2 lz;
+            lz.x = ((pn.x - ph.x) / dzt - (ph.x - pm.x) / dzb) / dz;
+            lz.y = ((pn.y - ph.y) / dzt - (ph.y - pm.y) / dzb) / dz;
+            double2 lap = make_double2(slap * ph.x + lz.x, slap * ph.y + lz.y);
+            // store
+            size_t o = (size_t)k * plane + col;
+            A.s0[o] = duv[0][r];
+            A.s1[o] = duv[1][r];
+            A.s2[o] = dw[0][r];
+            A.s3[o] = make_double2(ph.x - cnu * lap.x, ph.y - cnu * lap.y);
+        }
+    }
+}
+
+// Temperature Helmholtz solve in place on s0 (:335-337).
+template <int L>
+__global__ __launch_bounds__(64 * kColWaves) void k_column_t(StepArgs A)
+{
+    const int l = blockIdx.x * kColWaves + (threadIdx.x >> 6), jm = blockIdx.y;
+    if (l >= A.nl) return;
+    const ColMap m = col_map(A.nz);
+    const bool is00 = (l == 0 && jm == 0);
+    const size_t plane = (size_t)A.ny * A.nl, col = (size_t)jm * A.nl + l;
+    double2 d[1][L];
+#pragma unroll
+    for (int r = 0; r < L; ++r) d[0][r] = r < m.cnt ? A.s0[(size_t)(m.k0 + r) * plane + col] : czero();
+    double a[L], b[L], c[L];
+    assemble<L>(A, 3, l, jm, is00, m, a, b, c);
+    if (is00) fixups<L>(A, 3, m, d[0]);
+    wave_tridiag<L, 1>(m.cnt, a, b, c, d);
+#pragma unroll
+    for (int r = 0; r < L; ++r)
+        if (r < m.cnt) A.s0[(size_t)(m.k0 + r) * plane + col] = d[0][r];
+}
+
+// Standalone poisson_solver_fft for one equation, in place on s0.
+// eq: 0 u (ubnd), 1 v, 2 w (zstagger), 3 p (delnull), 4 t (tbnd).
+template <int L>
+__global__ __launch_bounds__(64 * kColWaves) void k_column_solve(StepArgs A, int eq)
+{
+    const int l = blockIdx.x * kColWaves + (threadIdx.x >> 6), jm = blockIdx.y;
+    if (l >= A.nl) return;
+    const ColMap m = col_map(A.nz);
+    const bool is00 = (l == 0 && jm == 0);
+    const size_t plane = (size_t)A.ny * A.nl, col = (size_t)jm * A.nl + l;
+    double2 d[1][L];
+#pragma unroll
+    for (int r = 0; r < L; ++r) {
+        d[0][r] = r < m.cnt ? A.s0[(size_t)(m.k0 + r) * plane + col] : czero();
+        if (eq == 2 && r < m.cnt && m.k0 + r == A.nz - 1) d[0][r] = czero();
+    }
+    const int meq = eq == 0 || eq == 1 ? 0 : (eq == 2 ? 1 : (eq == 3 ? 2 : 3));
+    double a[L], b[L], c[L];
+    assemble<L>(A, meq, l, jm, is00, m, a, b, c);
+    if (is00 && eq != 1 && eq != 2) fixups<L>(A, meq, m, d[0]);
+    if (meq == 2) wave_tridiag_refined<L, 1>(m.cnt, a, b, c, d);
+    else wave_tridiag<L, 1>(m.cnt, a, b, c, d);
+#pragma unroll
+    for (int r = 0; r < L; ++r)
+        if (r < m.cnt) A.s0[(size_t)(m.k0 + r) * plane + col] = d[0][r];
+}
+
+// ---------------------------------------------------------------------------
+// small kernels
+// dpx_drive update of dp_opt=2 (update_dpx_drive, src/module_parameters.f90:594-613)
+__global__ void k_dpx_update(double *scal, double u_mrf, double coef)
+{
+    if (threadIdx.x == 0 && blockIdx.x == 0) scal[0] = scal[0] - (scal[1] + u_mrf - 1.0) * coef;
+}
+
+// calculate_ubar (src/module_parameters.f90:513-546): per-level partial sums
+// then a fixed-order final sum, so the result is deterministic.
+__global__ __launch_bounds__(256) void k_ubar_partial(const double *__restrict__ u, const double *lev, int nx,
+                                                      int ny, int nz, double *part)
+{
+    __shared__ double red[256];
+    const int k = blockIdx.x;
+    double s = 0.0;
+    const size_t n = (size_t)nx * ny;
+    for (size_t q = threadIdx.x; q < n; q += 256) s += u[(size_t)k * n + q];
+    red[threadIdx.x] = s;
+    __syncthreads();
+    for (int w = 128; w > 0; w >>= 1) {
+        if (threadIdx.x < w) red[threadIdx.x] += red[threadIdx.x + w];
+        __syncthreads();
+    }
+    if (threadIdx.x == 0) part[k] = red[0] * lev[LV_DZ * nz + k];
+}
+__global__ void k_ubar_final(const double *part, int nz, double lz, int nx, int ny, double *scal)
+{
+    if (threadIdx.x == 0 && blockIdx.x == 0) {
+        double s = 0.0;
+        for (int k = 0; k < nz; ++k) s += part[k];
+        scal[1] = s / lz / nx / ny;
+    }
+}
+
+// screen_div_error values (src/module_io.f90:85-206): per-level sums of
+// |div| and max |div|, NaN flag.  cds as in the reference.
+template <int CDS>
+__global__ __launch_bounds__(256) void k_div_partial(StepArgs A, double *part)
+{
+    __shared__ double rs[256], rm[256];
+    __shared__ int rn[256];
+    const int k = blockIdx.x;
+    double s = 0.0, mx = 0.0;
+    int nan = 0;
+    const size_t n = (size_t)A.nx * A.ny;
+    const double dz = A.lev[LV_DZ * A.nz + k];
+    for (size_t q = threadIdx.x; q < n; q += 256) {
+        int j = (int)(q / A.nx), i = (int)(q - (size_t)j * A.nx);
+        Pt P{A, i, j, k};
+        double dudx, dvdy;
+        if (CDS == 1) {
+            dudx = (P.U(0, 0, 0) - P.U(-1, 0, 0)) / A.dx;
+            dvdy = (P.V(0, 0, 0) - P.V(0, -1, 0)) / A.dy;
+        } else {
+            dudx = (-P.U(1, 0, 0) + 27.0 * P.U(0, 0, 0) - 27.0 * P.U(-1, 0, 0) + P.U(-2, 0, 0)) / 24.0 / A.dx;
+            dvdy = (-P.V(0, 1, 0) + 27.0 * P.V(0, 0, 0) - 27.0 * P.V(0, -1, 0) + P.V(0, -2, 0)) / 24.0 / A.dy;
+        }
+        double w0 = A.w[P.at(0, 0, k)];
+        double wm = k == 0 ? 0.0 : A.w[P.at(0, 0, k - 1)];
+        double d = dudx + dvdy + (w0 - wm) / dz;
+        s += fabs(d);
+        mx = fmax(mx, fabs(d));
+        double u0 = A.u[P.at(0, 0, k)], v0 = A.v[P.at(0, 0, k)];
+        if (u0 != u0 || v0 != v0 || w0 != w0) nan = 1;
+    }
+    rs[threadIdx.x] = s;
+    rm[threadIdx.x] = mx;
+    rn[threadIdx.x] = nan;
+    __syncthreads();
+    for (int w = 128; w > 0; w >>= 1) {
+        if (threadIdx.x < w) {
+            rs[threadIdx.x] += rs[threadIdx.x + w];
+            rm[threadIdx.x] = fmax(rm[threadIdx.x], rm[threadIdx.x + w]);
+            rn[threadIdx.x] |= rn[threadIdx.x + w];
+        }
+        __syncthreads();
+    }
+    if (threadIdx.x == 0) {
+        part[3 * k] = rs[0];
+        part[3 * k + 1] = rm[0];
+        part[3 * k + 2] = rn[0];
+    }
+}
+
+// interior <-> ghosted (nx+4, ny+4, nz+4) Fortran layout
+__global__ void k_pack_ghosted(const double *__restrict__ g, double *dst, int nx, int ny, int nz)
+{
+    size_t n = (size_t)nx * ny * nz;
+    for (size_t q = blockIdx.x * (size_t)blockDim.x + threadIdx.x; q < n; q += (size_t)gridDim.x * blockDim.x) {
+        int i = (int)(q % nx), j = (int)((q / nx) % ny), k = (int)(q / ((size_t)nx * ny));
+        dst[q] = g[((size_t)(k + 2) * (ny + 4) + (j + 2)) * (nx + 4) + (i + 2)];
+    }
+}
+__global__ void k_unpack_ghosted(const double *__restrict__ src, double *g, int nx, int ny, int nz)
+{
+    size_t n = (size_t)nx * ny * nz;
+    for (size_t q = blockIdx.x * (size_t)blockDim.x + threadIdx.x; q < n; q += (size_t)gridDim.x * blockDim.x) {
+        int i = (int)(q % nx), j = (int)((q / nx) % ny), k = (int)(q / ((size_t)nx * ny));
+        g[((size_t)(k + 2) * (ny + 4) + (j + 2)) * (nx + 4) + (i + 2)] = src[q];
+    }
+}
+__global__ void k_zero_level(double *f, size_t off, size_t n)
+{
+    size_t q = blockIdx.x * (size_t)blockDim.x + threadIdx.x;
+    if (q < n) f[off + q] = 0.0;
+}
+
+// ===========================================================================
+// host side
+static const double kPi = 3.1415926535897932;  // src/module_parameters.f90:32
+static const double kRkc1[3] = {8.0 / 15.0, 5.0 / 12.0, 3.0 / 4.0};  // :210-212
+static const double kRkc2[3] = {0.0, -17.0 / 60.0, -5.0 / 12.0};
+static const double kRkc3[3] = {8.0 / 15.0, 2.0 / 15.0, 1.0 / 3.0};
+
+struct hc_ctx {
+    hc_params prm;
+    int nx, ny, nz, nl, Lcol;
+    double dx, dy;
+    int device;
+    hipStream_t stream;
+    double *u, *v, *w, *p, *t;
+    double *fb[2][3];  // ff, gg, hh ping-pong
+    double *sb[2];     // ss ping-pong
+    int fcur, scur;
+    double2 *s[4];
+    double *lev, *symx, *symy, *scal, *part, *stage;
+    double2 *tw_nl, *tw_nx, *tw_ny;
+    double aU0, cUe, aT0, cTe;
+    double time_sim, got;
+    int stps;
+    std::vector<double> h_lev;
+    std::vector<void *> allocs;
+};
+
+static bool fft_len_ok(int n) { return n == 16 || n == 32 || n == 64 || n == 128 || n == 192 || n == 256 || n == 512; }
+
+static int col_levels(int nz)
+{
+    int L = (nz + 63) / 64;
+    if (L <= 2) return 2;
+    if (L <= 3) return 3;
+    if (L <= 4) return 4;
+    if (L <= 5) return 5;
+    if (L <= 8) return 8;
+    if (L <= 16) return 16;
+    return -1;
+}
+
+static int dalloc(hc_ctx *c, void **p, size_t bytes)
+{
+    HIPCHK(hipMalloc(p, bytes));
+    HIPCHK(hipMemsetAsync(*p, 0, bytes, c->stream));
+    c->allocs.push_back(*p);
+    return 0;
+}
+
+// Grid metrics, get_mesh_param (src/module_parameters.f90:432-508), in the
+// reference's 1-based ghosted indexing (k = 1..nz+4).
+static void mesh_1based(const hc_params &P, std::vector<double> &dz, std::vector<double> &lt,
+                        std::vector<double> &lb, std::vector<double> &dzt, std::vector<double> &dzb)
+{
+    const int nka = P.nz + 4, ks = 3, ke = P.nz + 2;
+    std::vector<double> zz(nka + 1, 0.0);
+    dz.assign(nka + 2, 0.0); lt = lb = dzt = dzb = dz;
+    double zmin = 0.0, zmax = P.lz, zmean, a;
+    if (P.ochannel == 0) { zmean = 0.5 * (zmin + zmax); a = (2.0 + P.nz) * 0.5; }
+    else { zmean = zmin + zmax; a = (1.0 + P.nz) * 1.0; }
+    for (int k = 1; k <= P.nz + 1; ++k)
+        zz[k + 2] = zmean + (zmin - zmean) / std::tanh(2.0 * P.zstretch) * std::tanh(2.0 * P.zstretch * (k - a) / (1.0 - a));
+    for (int k = ks; k <= ke; ++k) dz[k] = zz[k + 1] - zz[k];
+    dz[ks - 1] = dz[ks]; dz[ke + 1] = dz[ke]; dz[ks - 2] = dz[ks]; dz[ke + 2] = dz[ke];
+    for (int k = ks - 1; k <= ke + 1; ++k) {
+        lt[k] = dz[k] / (dz[k] + dz[k + 1]);
+        lb[k] = dz[k] / (dz[k] + dz[k - 1]);
+        dzt[k] = 0.5 * (dz[k] + dz[k + 1]);
+        dzb[k] = 0.5 * (dz[k] + dz[k - 1]);
+    }
+}
+
+// Per-level tables: metrics and the 1-D coefficients of get_{u,w,p,t}_eqn_coeff
+// (src/module_poisson_solver.f90:32-477) with their wall rows, for general
+// modes and for the (0,0) mode.
+static void build_levels(hc_ctx *c)
+{
+    const hc_params &P = c->prm;
+    const int nz = P.nz;
+    std::vector<double> dz, lt, lb, dzt, dzb;
+    mesh_1based(P, dz, lt, lb, dzt, dzb);
+    std::vector<double> &T = c->h_lev;
+    T.assign((size_t)LV_COUNT * nz, 0.0);
+    auto at = [&](int tab, int kk) -> double & { return T[(size_t)tab * nz + kk]; };
+    for (int kk = 0; kk < nz; ++kk) {
+        int k = kk + 3;
+        at(LV_DZ, kk) = dz[k]; at(LV_DZT, kk) = dzt[k]; at(LV_DZB, kk) = dzb[k]; at(LV_LT, kk) = lt[k];
+        // u (v): aa, cc of get_u_eqn_coeff (:200-201)
+        double a = -0.5 * P.dt * P.nu / dz[k] / dzb[k], cc = -0.5 * P.dt * P.nu / dz[k] / dzt[k];
+        at(LV_AU, kk) = kk == 0 ? 0.0 : a;
+        at(LV_CU, kk) = kk == nz - 1 ? 0.0 : cc;
+        double d = -a - cc, d00 = -a - cc;
+        if (kk == 0) { d = -cc; d00 = -2.0 * a - cc; }
+        if (kk == nz - 1) { d = -a; d00 = P.ochannel == 0 ? -a - 2.0 * cc : -a; }
+        at(LV_DU, kk) = d; at(LV_DU00, kk) = d00;
+        // t: get_t_eqn_coeff (:398-399), Dirichlet at both walls
+        a = -0.5 * P.dt * P.alpha / dz[k] / dzb[k]; cc = -0.5 * P.dt * P.alpha / dz[k] / dzt[k];
+        at(LV_AT, kk) = kk == 0 ? 0.0 : a;
+        at(LV_CT, kk) = kk == nz - 1 ? 0.0 : cc;
+        d = d00 = -a - cc;
+        if (kk == 0) { d = -cc; d00 = -2.0 * a - cc; }
+        if (kk == nz - 1) { d = -a; d00 = -a - 2.0 * cc; }
+        at(LV_DT, kk) = d; at(LV_DT00, kk) = d00;
+        // p: get_p_eqn_coeff (:51-52), Neumann at both walls
+        a = 1.0 / dz[k] / dzb[k]; cc = 1.0 / dz[k] / dzt[k];
+        at(LV_AP, kk) = kk == 0 ? 0.0 : a;
+        at(LV_CP, kk) = kk == nz - 1 ? 0.0 : cc;
+        d = -a - cc;
+        if (kk == 0) d = -cc;
+        if (kk == nz - 1) d = -a;
+        at(LV_DP, kk) = d; at(LV_DP00, kk) = d;
+        // w: get_w_eqn_coeff (:306-307), levels kstr3..kend3-1
+        if (kk < nz - 1) {
+            a = -0.5 * P.dt * P.nu / dz[k] / dzt[k]; cc = -0.5 * P.dt * P.nu / dz[k + 1] / dzt[k];
+            at(LV_AW, kk) = kk == 0 ? 0.0 : a;
+            at(LV_CW, kk) = kk == nz - 2 ? 0.0 : cc;
+            d = -a - cc;
+            if (kk == 0) d = -cc;
+            if (kk == nz - 2) d = -a;
+            at(LV_DW, kk) = d; at(LV_DW00, kk) = -a - cc;
+        }
+    }
+    int ks = 3, ke = nz + 2;
+    c->aU0 = -0.5 * P.dt * P.nu / dz[ks] / dzb[ks];
+    c->cUe = -0.5 * P.dt * P.nu / dz[ke] / dzt[ke];
+    c->aT0 = -0.5 * P.dt * P.alpha / dz[ks] / dzb[ks];
+    c->cTe = -0.5 * P.dt * P.alpha / dz[ke] / dzt[ke];
+}
+
+// Per-wavenumber symbols of the horizontal FD operators for wavenumber index
+// `m` of an n-point direction with spacing h (src/module_poisson_solver.f90:73-101,
+// 222-246; the stencils of src/module_cnvdiff.f90 and src/module_navier_stokes.f90:669-796).
+static void build_symbols(const hc_params &P, int n, int count, double h, std::vector<double> &S)
+{
+    S.assign((size_t)SY_COUNT * count, 0.0);
+    const double h2 = h * h;
+    for (int q = 0; q < count; ++q) {
+        int m = q < n / 2 ? q : q - n;
+        double th = 2.0 * kPi * m / n;
+        double c1 = std::cos(th), c2 = std::cos(2.0 * th), c3 = std::cos(3.0 * th);
+        double s1 = std::sin(th), s2 = std::sin(2.0 * th);
+        double lap, pois, dre, dim, gre, gim;
+        if (P.cds == 1) {
+            lap = (2.0 * c1 - 2.0) / h2;
+            pois = lap;
+            dre = (1.0 - c1) / h; dim = s1 / h;             // (1 - e^{-i th})/h
+            gre = (c1 - 1.0) / h; gim = s1 / h;             // (e^{i th} - 1)/h
+        } else {
+            lap = (-2.0 * c2 + 32.0 * c1 - 30.0) / 12.0 / h2;
+            pois = (2.0 * c3 - 108.0 * c2 + 1566.0 * c1 - 1460.0) / 576.0 / h2;
+            // (-e^{i th} + 27 - 27 e^{-i th} + e^{-2 i th}) / 24h
+            dre = (-c1 + 27.0 - 27.0 * c1 + c2) / 24.0 / h;
+            dim = (-s1 + 27.0 * s1 - s2) / 24.0 / h;
+            // (-e^{2 i th} + 27 e^{i th} - 27 + e^{-i th}) / 24h
+            gre = (-c2 + 27.0 * c1 - 27.0 + c1) / 24.0 / h;
+            gim = (-s2 + 27.0 * s1 - s1) / 24.0 / h;
+        }
+        S[SY_HU * count + q] = 0.5 * P.dt * P.nu * lap;
+        S[SY_HT * count + q] = 0.5 * P.dt * P.alpha * lap;
+        S[SY_PX * count + q] = pois;
+        S[SY_L * count + q] = lap;
+        S[SY_DRE * count + q] = dre; S[SY_DIM * count + q] = dim;
+        S[SY_GRE * count + q] = gre; S[SY_GIM * count + q] = gim;
+    }
+}
+
+static std::vector<double2> twiddles(int n)
+{
+    std::vector<double2> t(n);
+    for (int e = 0; e < n; ++e) {
+        double a = 2.0 * kPi * (double)e / (double)n;
+        t[e] = make_double2(std::cos(a), -std::sin(a));
+    }
+    return t;
+}
+
+static StepArgs base_args(hc_ctx *c)
+{
+    StepArgs A;
+    std::memset(&A, 0, sizeof A);
+    const hc_params &P = c->prm;
+    A.u = c->u; A.v = c->v; A.w = c->w; A.p = c->p; A.t = c->t;
+    A.s0 = c->s[0]; A.s1 = c->s[1]; A.s2 = c->s[2]; A.s3 = c->s[3];
+    A.lev = c->lev; A.symx = c->symx; A.symy = c->symy;
+    A.tw_nl = c->tw_nl; A.tw_nx = c->tw_nx; A.tw_ny = c->tw_ny;
+    A.scal = c->scal;
+    A.nx = c->nx; A.ny = c->ny; A.nz = c->nz; A.nl = c->nl;
+    A.ochannel = P.ochannel; A.dp_opt = P.dp_opt; A.isscalar = P.isscalar;
+    A.dx = c->dx; A.dy = c->dy; A.dx2 = c->dx * c->dx; A.dy2 = c->dy * c->dy;
+    A.dt = P.dt; A.nu = P.nu; A.alpha = P.alpha; A.got = c->got; A.fc = P.fc; A.ug = P.ug; A.vg = P.vg;
+    A.u_mrf = P.u_mrf; A.t_ref = P.t_ref; A.tbot = P.tbot; A.ttop = P.ttop;
+    A.dpx = 1.0;
+    A.norm = 1.0 / ((double)c->nx * (double)c->ny);
+    A.aU0 = c->aU0; A.cUe = c->cUe; A.aT0 = c->aT0; A.cTe = c->cTe;
+    return A;
+}
+
+// ---- template dispatch ----------------------------------------------------
+#define HC_DISPATCH_N(n, MACRO)                                                          \
+    switch (n) {                                                                          \
+    case 16: MACRO(16); break;                                                            \
+    case 32: MACRO(32); break;                                                            \
+    case 64: MACRO(64); break;                                                            \
+    case 128: MACRO(128); break;                                                          \
+    case 192: MACRO(192); break;                                                          \
+    case 256: MACRO(256); break;                                                          \
+    case 512: MACRO(512); break;                                                          \
+    default: return set_err(HC_ERR_UNSUPPORTED, "unsupported transform length %d", n);   \
+    }
+#define HC_DISPATCH_L(L_, MACRO)                                                         \
+    switch (L_) {                                                                         \
+    case 2: MACRO(2); break;                                                              \
+    case 3: MACRO(3); break;                                                              \
+    case 4: MACRO(4); break;                                                              \
+    case 5: MACRO(5); break;                                                              \
+    case 8: MACRO(8); break;                                                              \
+    case 16: MACRO(16); break;                                                            \
+    default: return set_err(HC_ERR_UNSUPPORTED, "unsupported nz");                       \
+    }
+
+static int rows_for(int nx) { return nx >= kRowPoints ? 1 : kRowPoints / nx; }
+
+static int launch_yfft(hc_ctx *c, int sign, int nf, double2 *f0, double2 *f1, double2 *f2, double2 *f3)
+{
+    YArgs Y;
+    Y.f[0] = f0; Y.f[1] = f1; Y.f[2] = f2; Y.f[3] = f3;
+    Y.nl = c->nl; Y.nz = c->nz; Y.tw_ny = c->tw_ny;
+    dim3 g(c->nl / kYTile, c->nz, nf);
+#define HC_Y(NY_)                                                                                 \
+    if (sign < 0) hipLaunchKernelGGL((k_yfft<NY_, -1>), g, dim3(256), 0, c->stream, Y);          \
+    else hipLaunchKernelGGL((k_yfft<NY_, +1>), g, dim3(256), 0, c->stream, Y);
+    HC_DISPATCH_N(c->ny, HC_Y)
+#undef HC_Y
+    return 0;
+}
+
+static int launch_c2r(hc_ctx *c, int nf, C2RArgs &X)
+{
+    X.ny = c->ny; X.tw_nl = c->tw_nl; X.tw_nx = c->tw_nx;
+    int rows = rows_for(c->nx) / nf > 0 ? rows_for(c->nx) / nf : 1;
+    dim3 g((c->ny + rows - 1) / rows, c->nz);
+#define HC_C(NX_)                                                                                  \
+    if (nf == 4) hipLaunchKernelGGL((k_x_c2r<NX_, 4>), g, dim3(kRhsThreads), 0, c->stream, X);    \
+    else hipLaunchKernelGGL((k_x_c2r<NX_, 1>), g, dim3(kRhsThreads), 0, c->stream, X);
+    HC_DISPATCH_N(c->nx, HC_C)
+#undef HC_C
+    return 0;
+}
+
+static int launch_r2c(hc_ctx *c, const double *src, double2 *dst)
+{
+    int rows = rows_for(c->nx);
+    dim3 g((c->ny + rows - 1) / rows, c->nz);
+    double norm = 1.0 / ((double)c->nx * (double)c->ny);
+#define HC_R(NX_) hipLaunchKernelGGL((k_x_r2c<NX_>), g, dim3(kRhsThreads), 0, c->stream, src, dst, c->ny, \
+                                     c->tw_nl, c->tw_nx, norm);
+    HC_DISPATCH_N(c->nx, HC_R)
+#undef HC_R
+    return 0;
+}
+
+// one substep t_its (1..3 for RK3, 1 for AB2)
+static int substep(hc_ctx *c, int t_its)
+{
+    const hc_params &P = c->prm;
+    StepArgs A = base_args(c);
+    if (P.dts == 2) { A.ra = kRkc1[t_its - 1]; A.rb = kRkc2[t_its - 1]; A.rc = kRkc3[t_its - 1]; }
+    else { A.ra = 1.5; A.rb = -0.5; A.rc = 1.0; }
+    A.use_f0 = A.rb != 0.0;
+    int nxt = c->fcur ^ 1;
+    A.ff0 = c->fb[c->fcur][0]; A.gg0 = c->fb[c->fcur][1]; A.hh0 = c->fb[c->fcur][2];
+    A.ff = c->fb[nxt][0]; A.gg = c->fb[nxt][1]; A.hh = c->fb[nxt][2];
+    if (P.dp_opt == 2) hipLaunchKernelGGL(k_dpx_update, dim3(1), dim3(64), 0, c->stream, c->scal, P.u_mrf, P.nu * 1.0);
+    int rows = rows_for(c->nx);
+    dim3 grow((c->ny + rows - 1) / rows, c->nz);
+#define HC_RHS(NX_)                                                                                 \
+    if (P.cds == 1) hipLaunchKernelGGL((k_rhs_mom<1, NX_>), grow, dim3(kRhsThreads), 0, c->stream, A); \
+    else hipLaunchKernelGGL((k_rhs_mom<2, NX_>), grow, dim3(kRhsThreads), 0, c->stream, A);
+    HC_DISPATCH_N(c->nx, HC_RHS)
+#undef HC_RHS
+    if (int e = launch_yfft(c, -1, 3, c->s[0], c->s[1], c->s[2], c->s[3])) return e;
+    dim3 gcol((c->nl + kColWaves - 1) / kColWaves, c->ny);
+#define HC_COL(L_) hipLaunchKernelGGL((k_column_mom<L_>), gcol, dim3(64 * kColWaves), 0, c->stream, A);
+    HC_DISPATCH_L(c->Lcol, HC_COL)
+#undef HC_COL
+    if (int e = launch_yfft(c, +1, 4, c->s[0], c->s[1], c->s[2], c->s[3])) return e;
+    C2RArgs X;
+    std::memset(&X, 0, sizeof X);
+    X.src[0] = c->s[0]; X.src[1] = c->s[1]; X.src[2] = c->s[2]; X.src[3] = c->s[3];
+    X.dst[0] = c->u; X.dst[1] = c->v; X.dst[2] = c->w; X.dst[3] = c->p;
+    X.mode[3] = 1;
+    if (int e = launch_c2r(c, 4, X)) return e;
+    c->fcur = nxt;
+    if (P.isscalar == 1) {
+        int snx = c->scur ^ 1;
+        A.ss0 = c->sb[c->scur]; A.ss = c->sb[snx];
+#define HC_RT(NX_)                                                                                  \
+    if (P.cds == 1) hipLaunchKernelGGL((k_rhs_t<1, NX_>), grow, dim3(kRhsThreads), 0, c->stream, A);  \
+    else hipLaunchKernelGGL((k_rhs_t<2, NX_>), grow, dim3(kRhsThreads), 0, c->stream, A);
+        HC_DISPATCH_N(c->nx, HC_RT)
+#undef HC_RT
+        if (int e = launch_yfft(c, -1, 1, c->s[0], nullptr, nullptr, nullptr)) return e;
+#define HC_COLT(L_) hipLaunchKernelGGL((k_column_t<L_>), gcol, dim3(64 * kColWaves), 0, c->stream, A);
+        HC_DISPATCH_L(c->Lcol, HC_COLT)
+#undef HC_COLT
+        if (int e = launch_yfft(c, +1, 1, c->s[0], nullptr, nullptr, nullptr)) return e;
+        C2RArgs XT;
+        std::memset(&XT, 0, sizeof XT);
+        XT.src[0] = c->s[0]; XT.dst[0] = c->t;
+        if (int e = launch_c2r(c, 1, XT)) return e;
+        c->scur = snx;
+    }
+    HIPCHK(hipGetLastError());
+    return 0;
+}
+
+static int ubar_launch(hc_ctx *c)
+{
+    hipLaunchKernelGGL(k_ubar_partial, dim3(c->nz), dim3(256), 0, c->stream, c->u, c->lev, c->nx, c->ny, c->nz, c->part);
+    hipLaunchKernelGGL(k_ubar_final, dim3(1), dim3(64), 0, c->stream, c->part, c->nz, c->prm.lz, c->nx, c->ny, c->scal);
+    HIPCHK(hipGetLastError());
+    return 0;
+}
+
+// ---- C ABI ----------------------------------------------------------------
+extern "C" int hc_create(const hc_params *prm, hc_ctx **out)
+{
+    if (!prm || !out) return set_err(HC_ERR_ARG, "null argument");
+    *out = nullptr;
+    const hc_params &P = *prm;
+    // initiate_parameters checks (src/module_parameters.f90:256-270)
+    if (P.nx % 2 != 0 || P.ny % 2 != 0) return set_err(HC_ERR_ARG, "Please use even numbers for nx and ny! Quit!");
+    if (P.nz < 3) return set_err(HC_ERR_ARG, "nz must be at least 3");
+    if (P.cds != 1 && P.cds != 2) return set_err(HC_ERR_UNSUPPORTED, "cds=%d: only the FD paths cds=1,2 are implemented", P.cds);
+    if (P.dts != 1 && P.dts != 2) return set_err(HC_ERR_ARG, "dts must be 1 (AB2) or 2 (RK3)");
+    if (P.dp_opt < 1 || P.dp_opt > 3) return set_err(HC_ERR_ARG, "dp_opt must be 1, 2 or 3");
+    if (P.ochannel != 0 && P.ochannel != 1) return set_err(HC_ERR_ARG, "ochannel must be 0 or 1");
+    if (P.isscalar != 0 && P.isscalar != 1) return set_err(HC_ERR_ARG, "isscalar error!");
+    if (P.isdamp != 0) return set_err(HC_ERR_UNSUPPORTED, "isdamp=1 (Rayleigh damping) is not implemented");
+    if (!fft_len_ok(P.nx) || !fft_len_ok(P.ny)) return set_err(HC_ERR_UNSUPPORTED, "nx, ny must be one of 16,32,64,128,192,256,512");
+    int L = col_levels(P.nz);
+    if (L < 0) return set_err(HC_ERR_UNSUPPORTED, "nz=%d too large", P.nz);
+    hc_ctx *c = new hc_ctx();
+    c->prm = P;
+    c->nx = P.nx; c->ny = P.ny; c->nz = P.nz; c->nl = P.nx / 2; c->Lcol = L;
+    c->dx = P.lx / P.nx; c->dy = P.ly / P.ny;
+    c->got = P.got;
+    c->time_sim = 0.0; c->stps = 0; c->fcur = 0; c->scur = 0;
+    hipError_t he = hipGetDevice(&c->device);
+    if (he == hipSuccess) he = hipStreamCreateWithFlags(&c->stream, hipStreamNonBlocking);
+    if (he != hipSuccess) { delete c; return set_err(HC_ERR_HIP, "HIP device/stream: %s", hipGetErrorString(he)); }
+    const size_t N = (size_t)c->nx * c->ny * c->nz;
+    int e = 0;
+    double **phys[] = {&c->u, &c->v, &c->w, &c->p, &c->t, &c->fb[0][0], &c->fb[0][1], &c->fb[0][2],
+                       &c->fb[1][0], &c->fb[1][1], &c->fb[1][2], &c->sb[0], &c->sb[1]};
+    for (auto pp : phys) if (!e) e = dalloc(c, (void **)pp, N * sizeof(double));
+    for (int q = 0; q < 4 && !e; ++q) e = dalloc(c, (void **)&c->s[q], (size_t)c->nl * c->ny * c->nz * sizeof(double2));
+    build_levels(c);
+    std::vector<double> sx, sy;
+    build_symbols(P, c->nx, c->nl, c->dx, sx);
+    build_symbols(P, c->ny, c->ny, c->dy, sy);
+    auto tnl = twiddles(c->nl), tnx = twiddles(c->nx), tny = twiddles(c->ny);
+    if (!e) e = dalloc(c, (void **)&c->lev, c->h_lev.size() * sizeof(double));
+    if (!e) e = dalloc(c, (void **)&c->symx, sx.size() * sizeof(double));
+    if (!e) e = dalloc(c, (void **)&c->symy, sy.size() * sizeof(double));
+    if (!e) e = dalloc(c, (void **)&c->tw_nl, tnl.size() * sizeof(double2));
+    if (!e) e = dalloc(c, (void **)&c->tw_nx, tnx.size() * sizeof(double2));
+    if (!e) e = dalloc(c, (void **)&c->tw_ny, tny.size() * sizeof(double2));
+    if (!e) e = dalloc(c, (void **)&c->scal, 8 * sizeof(double));
+    if (!e) e = dalloc(c, (void **)&c->part, 3 * (size_t)c->nz * sizeof(double));
+    if (e) { hc_destroy(c); return e; }
+    double scal0[8] = {P.dp_opt == 1 ? 1.0 : 0.0, 0, 0, 0, 0, 0, 0, 0};
+    hipError_t r = hipMemcpyAsync(c->lev, c->h_lev.data(), c->h_lev.size() * sizeof(double), hipMemcpyHostToDevice, c->stream);
+    if (r == hipSuccess) r = hipMemcpyAsync(c->symx, sx.data(), sx.size() * sizeof(double), hipMemcpyHostToDevice, c->stream);
+    if (r == hipSuccess) r = hipMemcpyAsync(c->symy, sy.data(), sy.size() * sizeof(double), hipMemcpyHostToDevice, c->stream);
+    if (r == hipSuccess) r = hipMemcpyAsync(c->tw_nl, tnl.data(), tnl.size() * sizeof(double2), hipMemcpyHostToDevice, c->stream);
+    if (r == hipSuccess) r = hipMemcpyAsync(c->tw_nx, tnx.data(), tnx.size() * sizeof(double2), hipMemcpyHostToDevice, c->stream);
+    if (r == hipSuccess) r = hipMemcpyAsync(c->tw_ny, tny.data(), tny.size() * sizeof(double2), hipMemcpyHostToDevice, c->stream);
+    if (r == hipSuccess) r = hipMemcpyAsync(c->scal, scal0, sizeof scal0, hipMemcpyHostToDevice, c->stream);
+    if (r == hipSuccess) r = hipStreamSynchronize(c->stream);
+    if (r != hipSuccess) { hc_destroy(c); return set_err(HC_ERR_HIP, "table upload: %s", hipGetErrorString(r)); }
+    *out = c;
+    return 0;
+}
+
+extern "C" int hc_destroy(hc_ctx *c)
+{
+    if (!c) return 0;
+    if (c->stream) (void)hipStreamSynchronize(c->stream);
+    for (void *p : c->allocs) (void)hipFree(p);
+    if (c->stage) (void)hipFree(c->stage);
+    if (c->stream) (void)hipStreamDestroy(c->stream);
+    delete c;
+    return 0;
+}
+
+static double *field_ptr(hc_ctx *c, int f)
+{
+    switch (f) {
+    case HC_U: return c->u;
+    case HC_V: return c->v;
+    case HC_W: return c->w;
+    case HC_P: return c->p;
+    case HC_T: return c->t;
+    case HC_FF: return c->fb[c->fcur][0];
+    case HC_GG: return c->fb[c->fcur][1];
+    case HC_HH: return c->fb[c->fcur][2];
+    case HC_SS: return c->sb[c->scur];
+    default: return nullptr;
+    }
+}
+
+static int ensure_stage(hc_ctx *c)
+{
+    if (c->stage) return 0;
+    size_t n = (size_t)(c->nx + 4) * (c->ny + 4) * (c->nz + 4);
+    HIPCHK(hipMalloc((void **)&c->stage, n * sizeof(double)));
+    return 0;
+}
+
+extern "C" int hc_set_field(hc_ctx *c, int field, const double *src, int layout)
+{
+    if (!c || !src) return set_err(HC_ERR_ARG, "null argument");
+    double *d = field_ptr(c, field);
+    if (!d) return set_err(HC_ERR_ARG, "unknown field %d", field);
+    const size_t N = (size_t)c->nx * c->ny * c->nz;
+    if (layout == HC_LAYOUT_INTERIOR) {
+        HIPCHK(hipMemcpyAsync(d, src, N * sizeof(double), hipMemcpyDefault, c->stream));
+    } else if (layout == HC_LAYOUT_GHOSTED) {
+        if (int e = ensure_stage(c)) return e;
+        size_t n = (size_t)(c->nx + 4) * (c->ny + 4) * (c->nz + 4);
+        HIPCHK(hipMemcpyAsync(c->stage, src, n * sizeof(double), hipMemcpyDefault, c->stream));
+        hipLaunchKernelGGL(k_pack_ghosted, dim3(1024), dim3(256), 0, c->stream, c->stage, d, c->nx, c->ny, c->nz);
+    } else {
+        return set_err(HC_ERR_ARG, "unknown layout %d", layout);
+    }
+    HIPCHK(hipStreamSynchronize(c->stream));
+    return 0;
+}
+
+extern "C" int hc_get_field(hc_ctx *c, int field, double *dst, int layout)
+{
+    if (!c || !dst) return set_err(HC_ERR_ARG, "null argument");
+    double *d = field_ptr(c, field);
+    if (!d) return set_err(HC_ERR_ARG, "unknown field %d", field);
+    const size_t N = (size_t)c->nx * c->ny * c->nz;
+    if (layout == HC_LAYOUT_INTERIOR) {
+        HIPCHK(hipMemcpyAsync(dst, d, N * sizeof(double), hipMemcpyDefault, c->stream));
+    } else if (layout == HC_LAYOUT_GHOSTED) {
+        if (int e = ensure_stage(c)) return e;
+        size_t n = (size_t)(c->nx + 4) * (c->ny + 4) * (c->nz + 4);
+        // keep the caller's ghost cells: bring the whole array, overwrite the interior
+        HIPCHK(hipMemcpyAsync(c->stage, dst, n * sizeof(double), hipMemcpyDefault, c->stream));
+        hipLaunchKernelGGL(k_unpack_ghosted, dim3(1024), dim3(256), 0, c->stream, d, c->stage, c->nx, c->ny, c->nz);
+        HIPCHK(hipMemcpyAsync(dst, c->stage, n * sizeof(double), hipMemcpyDefault, c->stream));
+    } else {
+        return set_err(HC_ERR_ARG, "unknown layout %d", layout);
+    }
+    HIPCHK(hipStreamSynchronize(c->stream));
+    return 0;
+}
+
+extern "C" int hc_init(hc_ctx *c)
+{
+    if (!c) return set_err(HC_ERR_ARG, "null context");
+    const hc_params &P = c->prm;
+    if (P.is_ri_var == 1) c->got = P.ri_str;  // update_ri(0)
+    // update_boundary_uvw forces w = 0 on the top face (src/module_boundary.f90:107)
+    size_t plane = (size_t)c->nx * c->ny;
+    hipLaunchKernelGGL(k_zero_level, dim3((unsigned)((plane + 255) / 256)), dim3(256), 0, c->stream, c->w,
+                       (size_t)(c->nz - 1) * plane, plane);
+    if (P.dp_opt == 2)
+        if (int e = ubar_launch(c)) return e;
+    HIPCHK(hipGetLastError());
+    HIPCHK(hipStreamSynchronize(c->stream));
+    return 0;
+}
+
+static int one_step(hc_ctx *c)
+{
+    const hc_params &P = c->prm;
+    c->stps += 1;
+    c->time_sim += P.dt;
+    if (P.is_ri_var == 1) c->got = P.ri_str + (P.ri_end - P.ri_str) * 1.0 / P.imax * c->stps;
+    int nsub = P.dts == 2 ? 3 : 1;
+    for (int s = 1; s <= nsub; ++s)
+        if (int e = substep(c, s)) return e;
+    return 0;
+}
+
+extern "C" int hc_time_advancement_fd(hc_ctx *c)
+{
+    if (!c) return set_err(HC_ERR_ARG, "null context");
+    int nsub = c->prm.dts == 2 ? 3 : 1;
+    for (int s = 1; s <= nsub; ++s)
+        if (int e = substep(c, s)) return e;
+    return 0;
+}
+
+extern "C" int hc_drive_steps(hc_ctx *c, int nsteps)
+{
+    if (!c) return set_err(HC_ERR_ARG, "null context");
+    for (int n = 0; n < nsteps; ++n) {
+        if (int e = one_step(c)) return e;
+        if (c->prm.dp_opt == 2)
+            if (int e = ubar_launch(c)) return e;
+    }
+    HIPCHK(hipGetLastError());
+    return 0;
+}
+
+extern "C" int hc_poisson_solver_fft(hc_ctx *c, int equation, double rk_c, const double *r_src, double *r_var, int layout)
+{
+    if (!c || !r_src || !r_var) return set_err(HC_ERR_ARG, "null argument");
+    if (equation < HC_EQ_U || equation > HC_EQ_T) return set_err(HC_ERR_ARG, "unknown equation %d", equation);
+    if (layout != HC_LAYOUT_INTERIOR && layout != HC_LAYOUT_GHOSTED) return set_err(HC_ERR_ARG, "unknown layout %d", layout);
+    const size_t N = (size_t)c->nx * c->ny * c->nz;
+    // the physical work buffer: the ss ping-pong slot not holding ss
+    double *work = c->sb[c->scur ^ 1];
+    bool dev_io = false;
+    hipPointerAttribute_t at;
+    if (layout == HC_LAYOUT_INTERIOR && hipPointerGetAttributes(&at, r_src) == hipSuccess &&
+        at.type == hipMemoryTypeDevice)
+        dev_io = true;
+    (void)hipGetLastError();
+    const double *src = r_src;
+    if (!dev_io) {
+        if (layout == HC_LAYOUT_INTERIOR) {
+            HIPCHK(hipMemcpyAsync(work, r_src, N * sizeof(double), hipMemcpyDefault, c->stream));
+        } else {
+            if (int e = ensure_stage(c)) return e;
+            size_t n = (size_t)(c->nx + 4) * (c->ny + 4) * (c->nz + 4);
+            HIPCHK(hipMemcpyAsync(c->stage, r_src, n * sizeof(double), hipMemcpyDefault, c->stream));
+            hipLaunchKernelGGL(k_pack_ghosted, dim3(1024), dim3(256), 0, c->stream, c->stage, work, c->nx, c->ny, c->nz);
+        }
+        src = work;
+    }
+    if (int e = launch_r2c(c, src, c->s[0])) return e;
+    if (int e = launch_yfft(c, -1, 1, c->s[0], nullptr, nullptr, nullptr)) return e;
+    StepArgs A = base_args(c);
+    A.rc = c->prm.dts == 2 ? rk_c : 1.0;
+    dim3 gcol((c->nl + kColWaves - 1) / kColWaves, c->ny);
+#define HC_CS(L_) hipLaunchKernelGGL((k_column_solve<L_>), gcol, dim3(64 * kColWaves), 0, c->stream, A, equation);
+    HC_DISPATCH_L(c->Lcol, HC_CS)
+#undef HC_CS
+    if (int e = launch_yfft(c, +1, 1, c->s[0], nullptr, nullptr, nullptr)) return e;
+    C2RArgs X;
+    std::memset(&X, 0, sizeof X);
+    X.src[0] = c->s[0];
+    X.dst[0] = dev_io ? r_var : work;
+    if (int e = launch_c2r(c, 1, X)) return e;
+    if (!dev_io) {
+        if (layout == HC_LAYOUT_INTERIOR) {
+            HIPCHK(hipMemcpyAsync(r_var, work, N * sizeof(double), hipMemcpyDefault, c->stream));
+        } else {
+            size_t n = (size_t)(c->nx + 4) * (c->ny + 4) * (c->nz + 4);
+            HIPCHK(hipMemcpyAsync(c->stage, r_var, n * sizeof(double), hipMemcpyDefault, c->stream));
+            hipLaunchKernelGGL(k_unpack_ghosted, dim3(1024), dim3(256), 0, c->stream, work, c->stage, c->nx, c->ny, c->nz);
+            HIPCHK(hipMemcpyAsync(r_var, c->stage, n * sizeof(double), hipMemcpyDefault, c->stream));
+        }
+        HIPCHK(hipStreamSynchronize(c->stream));
+    }
+    HIPCHK(hipGetLastError());
+    return 0;
+}
+
+extern "C" int hc_calculate_ubar(hc_ctx *c, double *ubar)
+{
+    if (!c || !ubar) return set_err(HC_ERR_ARG, "null argument");
+    if (int e = ubar_launch(c)) return e;
+    HIPCHK(hipMemcpyAsync(ubar, c->scal + 1, sizeof(double), hipMemcpyDeviceToHost, c->stream));
+    HIPCHK(hipStreamSynchronize(c->stream));
+    return 0;
+}
+
+extern "C" int hc_screen_div_error(hc_ctx *c, double *div_avg, double *div_max)
+{
+    if (!c) return set_err(HC_ERR_ARG, "null context");
+    StepArgs A = base_args(c);
+    if (c->prm.cds == 1) hipLaunchKernelGGL((k_div_partial<1>), dim3(c->nz), dim3(256), 0, c->stream, A, c->part);
+    else hipLaunchKernelGGL((k_div_partial<2>), dim3(c->nz), dim3(256), 0, c->stream, A, c->part);
+    std::vector<double> h(3 * (size_t)c->nz);
+    HIPCHK(hipMemcpyAsync(h.data(), c->part, h.size() * sizeof(double), hipMemcpyDeviceToHost, c->stream));
+    HIPCHK(hipStreamSynchronize(c->stream));
+    double s = 0.0, m = 0.0;
+    bool nan = false;
+    for (int k = 0; k < c->nz; ++k) {
+        s += h[3 * k];
+        m = std::fmax(m, h[3 * k + 1]);
+        nan = nan || h[3 * k + 2] != 0.0;
+    }
+    if (div_avg) *div_avg = s / c->nx / c->ny / c->nz;
+    if (div_max) *div_max = m;
+    if (nan) return set_err(HC_ERR_NAN, "NaN error!");
+    return 0;
+}
+
+extern "C" int hc_get_scalar(hc_ctx *c, int id, double *value)
+{
+    if (!c || !value) return set_err(HC_ERR_ARG, "null argument");
+    double sc[2];
+    switch (id) {
+    case 0:
+    case 1:
+        HIPCHK(hipMemcpyAsync(sc, c->scal, 2 * sizeof(double), hipMemcpyDeviceToHost, c->stream));
+        HIPCHK(hipStreamSynchronize(c->stream));
+        *value = sc[id];
+        return 0;
+    case 2: *value = c->time_sim; return 0;
+    case 3: *value = c->got; return 0;
+    case 4: *value = c->stps; return 0;
+    default: return set_err(HC_ERR_ARG, "unknown scalar %d", id);
+    }
+}
+
+extern "C" void *hc_stream(hc_ctx *c) { return c ? (void *)c->stream : nullptr; }
+extern "C" double *hc_field_device_ptr(hc_ctx *c, int field) { return c ? field_ptr(c, field) : nullptr; }
+
+extern "C" int hc_synchronize(hc_ctx *c)
+{
+    if (!c) return set_err(HC_ERR_ARG, "null context");
+    HIPCHK(hipStreamSynchronize(c->stream));
+    return 0;
+}

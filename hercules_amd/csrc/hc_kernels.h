@@ -1,0 +1,345 @@
+// HIP kernels of the HERCULES finite-difference time step (MI355X, gfx950).
+//
+// Data layout in HBM (all FP64):
+//   physical fields   f[k][j][i]        i < nx fastest, k < nz, interior only;
+//                                       z ghost values are derived on the fly
+//                                       from the reference's wall rules and
+//                                       x/y halos by periodic index wrap.
+//   spectra           s[k][j][l]        l < nx/2 complex, j < ny (m = j or
+//                                       j-ny, the Nyquist row j = ny/2 is 0);
+//                                       the x Nyquist bin l = nx/2 is never
+//                                       stored because the reference zeroes it.
+//
+// Transform normalisation follows mpi_2d_fft (src/module_tools.f90:518-700):
+// forward divides by nx*ny, inverse is unnormalised.
+#pragma once
+#include "hc_device.h"
+
+// per-level tables, each nz long, stacked
+enum {
+    LV_DZ = 0, LV_DZT, LV_DZB, LV_LT,
+    LV_AU, LV_CU, LV_DU, LV_DU00,
+    LV_AW, LV_CW, LV_DW, LV_DW00,
+    LV_AP, LV_CP, LV_DP, LV_DP00,
+    LV_AT, LV_CT, LV_DT, LV_DT00,
+    LV_COUNT
+};
+// per-wavenumber tables (x: nx/2 entries, y: ny entries), stacked
+enum {
+    SY_HU = 0,  // Helmholtz symbol with nu:    dt*nu/2 * (second-derivative symbol)
+    SY_HT,      // Helmholtz symbol with alpha
+    SY_PX,      // Poisson symbol: divergence o gradient
+    SY_L,       // second-derivative symbol used by calculate_new_p
+    SY_DRE, SY_DIM,  // divergence symbol (p point from u points)
+    SY_GRE, SY_GIM,  // gradient symbol (u point from p points)
+    SY_COUNT
+};
+
+struct StepArgs {
+    // physical fields
+    const double *u, *v, *w, *p, *t, *ff0, *gg0, *hh0, *ss0;
+    double *ff, *gg, *hh, *ss;
+    double *uo, *vo, *wo, *po, *to;  // outputs of the finish kernels
+    // spectra
+    double2 *s0, *s1, *s2, *s3;
+    // tables
+    const double *lev, *symx, *symy;
+    const double2 *tw_nl, *tw_nx, *tw_ny;
+    const double *scal;  // device scalars: [0] dpx_drive, [1] ubar
+    int nx, ny, nz, nl;
+    int ochannel, dp_opt, isscalar, use_f0;
+    double dx, dy, dx2, dy2, dt, nu, alpha, got, fc, ug, vg, u_mrf, t_ref, tbot, ttop;
+    double ra, rb, rc, dpx, norm;
+    double aU0, cUe, aT0, cTe;  // unmodified wall coefficients for the (0,0)-mode fix-ups
+};
+
+HC_DEV int wrapi(int i, int n) { return i < 0 ? i + n : (i >= n ? i - n : i); }
+HC_DEV double sq(double x) { return x * x; }
+
+// ---------------------------------------------------------------------------
+// Point accessors with the reference's ghost rules
+// (src/module_boundary.f90:33-196) and periodic x/y wrap (comm_bound).
+struct Pt {
+    const StepArgs &A;
+    int i, j, k;
+    HC_DEV size_t at(int di, int dj, int kk) const
+    {
+        return ((size_t)kk * A.ny + wrapi(j + dj, A.ny)) * A.nx + wrapi(i + di, A.nx);
+    }
+    HC_DEV double U(int di, int dj, int dk) const
+    {
+        int kk = k + dk;
+        if (kk < 0) return -A.u[at(di, dj, 0)] - 2.0 * A.u_mrf;
+        if (kk >= A.nz) {
+            double top = A.u[at(di, dj, A.nz - 1)];
+            return A.ochannel == 0 ? -top - 2.0 * A.u_mrf : top;
+        }
+        return A.u[at(di, dj, kk)];
+    }
+    HC_DEV double V(int di, int dj, int dk) const
+    {
+        int kk = k + dk;
+        if (kk < 0) return -A.v[at(di, dj, 0)];
+        if (kk >= A.nz) {
+            double top = A.v[at(di, dj, A.nz - 1)];
+            return A.ochannel == 0 ? -top : top;
+        }
+        return A.v[at(di, dj, kk)];
+    }
+    HC_DEV double W(int di, int dj, int dk) const
+    {
+        int kk = k + dk;
+        if (kk < 0 || kk == A.nz - 1) return 0.0;
+        if (kk >= A.nz) return -A.w[at(di, dj, A.nz - 2)];
+        return A.w[at(di, dj, kk)];
+    }
+    HC_DEV double P(int di, int dj, int dk) const
+    {
+        int kk = k + dk;
+        kk = kk < 0 ? 0 : (kk >= A.nz ? A.nz - 1 : kk);
+        return A.p[at(di, dj, kk)];
+    }
+    HC_DEV double T(int di, int dj, int dk) const
+    {
+        int kk = k + dk;
+        if (kk < 0) return 2.0 * A.tbot - A.t[at(di, dj, 0)];
+        if (kk >= A.nz) return 2.0 * A.ttop - A.t[at(di, dj, A.nz - 1)];
+        return A.t[at(di, dj, kk)];
+    }
+    HC_DEV double lev(int tab, int kk) const { return A.lev[tab * A.nz + kk]; }
+
+    // mid-point interpolants, src/module_tools.f90:410-512, with the ghost
+    // overrides of update_boundary_uvwhxyz / _thxyz (src/module_boundary.f90:202-355)
+    template <int CDS> HC_DEV double uhx(int di, int dj, int dk) const
+    {
+        if (CDS == 1) return 0.5 * (U(di, dj, dk) + U(di - 1, dj, dk));
+        return (-U(di + 1, dj, dk) + 9.0 * U(di, dj, dk) + 9.0 * U(di - 1, dj, dk) - U(di - 2, dj, dk)) / 16.0;
+    }
+    template <int CDS> HC_DEV double uhy(int di, int dj, int dk) const
+    {
+        if (CDS == 1) return 0.5 * (U(di, dj + 1, dk) + U(di, dj, dk));
+        return (-U(di, dj + 2, dk) + 9.0 * U(di, dj + 1, dk) + 9.0 * U(di, dj, dk) - U(di, dj - 1, dk)) / 16.0;
+    }
+    template <int CDS> HC_DEV double vhx(int di, int dj, int dk) const
+    {
+        if (CDS == 1) return 0.5 * (V(di + 1, dj, dk) + V(di, dj, dk));
+        return (-V(di + 2, dj, dk) + 9.0 * V(di + 1, dj, dk) + 9.0 * V(di, dj, dk) - V(di - 1, dj, dk)) / 16.0;
+    }
+    template <int CDS> HC_DEV double vhy(int di, int dj, int dk) const
+    {
+        if (CDS == 1) return 0.5 * (V(di, dj, dk) + V(di, dj - 1, dk));
+        return (-V(di, dj + 1, dk) + 9.0 * V(di, dj, dk) + 9.0 * V(di, dj - 1, dk) - V(di, dj - 2, dk)) / 16.0;
+    }
+    template <int CDS> HC_DEV double whx(int di, int dj, int dk) const
+    {
+        int kk = k + dk;
+        if (kk < 0 || kk == A.nz - 1) return 0.0;
+        if (CDS == 1) return 0.5 * (W(di + 1, dj, dk) + W(di, dj, dk));
+        return (-W(di + 2, dj, dk) + 9.0 * W(di + 1, dj, dk) + 9.0 * W(di, dj, dk) - W(di - 1, dj, dk)) / 16.0;
+    }
+    template <int CDS> HC_DEV double why(int di, int dj, int dk) const
+    {
+        int kk = k + dk;
+        if (kk < 0 || kk == A.nz - 1) return 0.0;
+        if (CDS == 1) return 0.5 * (W(di, dj + 1, dk) + W(di, dj, dk));
+        return (-W(di, dj + 2, dk) + 9.0 * W(di, dj + 1, dk) + 9.0 * W(di, dj, dk) - W(di, dj - 1, dk)) / 16.0;
+    }
+    HC_DEV double uhz(int di, int dj, int dk) const
+    {
+        int kk = k + dk;
+        if (kk < 0) return -A.u_mrf;
+        if (kk == A.nz - 1) return A.ochannel == 0 ? -A.u_mrf : U(di, dj, dk);
+        double lt = lev(LV_LT, kk);
+        return U(di, dj, dk + 1) * lt + U(di, dj, dk) * (1.0 - lt);
+    }
+    HC_DEV double vhz(int di, int dj, int dk) const
+    {
+        int kk = k + dk;
+        if (kk < 0) return 0.0;
+        if (kk == A.nz - 1) return A.ochannel == 0 ? 0.0 : V(di, dj, dk);
+        double lt = lev(LV_LT, kk);
+        return V(di, dj, dk + 1) * lt + V(di, dj, dk) * (1.0 - lt);
+    }
+    template <int CDS> HC_DEV double thx(int di, int dj, int dk) const
+    {
+        if (CDS == 1) return 0.5 * (T(di + 1, dj, dk) + T(di, dj, dk));
+        return (-T(di + 2, dj, dk) + 9.0 * T(di + 1, dj, dk) + 9.0 * T(di, dj, dk) - T(di - 1, dj, dk)) / 16.0;
+    }
+    template <int CDS> HC_DEV double thy(int di, int dj, int dk) const
+    {
+        if (CDS == 1) return 0.5 * (T(di, dj + 1, dk) + T(di, dj, dk));
+        return (-T(di, dj + 2, dk) + 9.0 * T(di, dj + 1, dk) + 9.0 * T(di, dj, dk) - T(di, dj - 1, dk)) / 16.0;
+    }
+    HC_DEV double thz(int dk) const
+    {
+        int kk = k + dk;
+        if (kk < 0) return A.tbot;
+        if (kk == A.nz - 1) return A.ttop;
+        double lt = lev(LV_LT, kk);
+        return T(0, 0, dk + 1) * lt + T(0, 0, dk) * (1.0 - lt);
+    }
+};
+
+// ---------------------------------------------------------------------------
+// Momentum right-hand side at one point, get_momentum_cnvdiff
+// (src/module_cnvdiff.f90:35-374).  Returns ff,gg,hh and qu,qv,qw.
+template <int CDS>
+HC_DEV void momentum_point(const Pt &q, double dpx, double &f, double &g, double &h, double &qu, double &qv, double &qw)
+{
+    const StepArgs &A = q.A;
+    const double dx = A.dx, dy = A.dy, dx2 = A.dx2, dy2 = A.dy2, um = A.u_mrf;
+    const int k = q.k;
+    const double dz = q.lev(LV_DZ, k), dzt = q.lev(LV_DZT, k), dzb = q.lev(LV_DZB, k);
+    size_t c0 = q.at(0, 0, k);
+    // ---- U equation (:55-161)
+    {
+        double uux, uvy, ux, uxx, uyy, dpdx;
+        if (CDS == 1) {
+            uux = (sq(q.uhx<1>(1, 0, 0)) - sq(q.uhx<1>(0, 0, 0))) / dx;
+            uvy = (q.uhy<1>(0, 0, 0) * q.vhx<1>(0, 0, 0) - q.uhy<1>(0, -1, 0) * q.vhx<1>(0, -1, 0)) / dy;
+            ux = (q.uhx<1>(1, 0, 0) - q.uhx<1>(0, 0, 0)) / dx;
+            uxx = (q.U(1, 0, 0) - 2.0 * q.U(0, 0, 0) + q.U(-1, 0, 0)) / dx2;
+            uyy = (q.U(0, 1, 0) - 2.0 * q.U(0, 0, 0) + q.U(0, -1, 0)) / dy2;
+            dpdx = (q.P(1, 0, 0) - q.P(0, 0, 0)) / dx;
+        } else {
+            double h2 = q.uhx<2>(2, 0, 0), h1 = q.uhx<2>(1, 0, 0), h0 = q.uhx<2>(0, 0, 0), hm = q.uhx<2>(-1, 0, 0);
+            uux = (-sq(h2) + 27.0 * sq(h1) - 27.0 * sq(h0) + sq(hm)) / 24.0 / dx;
+            uvy = (-q.uhy<2>(0, 1, 0) * q.vhx<2>(0, 1, 0) + 27.0 * q.uhy<2>(0, 0, 0) * q.vhx<2>(0, 0, 0) -
+                   27.0 * q.uhy<2>(0, -1, 0) * q.vhx<2>(0, -1, 0) + q.uhy<2>(0, -2, 0) * q.vhx<2>(0, -2, 0)) / 24.0 / dy;
+            ux = (-h2 + 27.0 * h1 - 27.0 * h0 + hm) / 24.0 / dx;
+            uxx = (-q.U(2, 0, 0) + 16.0 * q.U(1, 0, 0) - 30.0 * q.U(0, 0, 0) + 16.0 * q.U(-1, 0, 0) - q.U(-2, 0, 0)) / 12.0 / dx2;
+            uyy = (-q.U(0, 2, 0) + 16.0 * q.U(0, 1, 0) - 30.0 * q.U(0, 0, 0) + 16.0 * q.U(0, -1, 0) - q.U(0, -2, 0)) / 12.0 / dy2;
+            dpdx = (-q.P(2, 0, 0) + 27.0 * q.P(1, 0, 0) - 27.0 * q.P(0, 0, 0) + q.P(-1, 0, 0)) / 24.0 / dx;
+        }
+        double uwz = (q.uhz(0, 0, 0) * q.whx<CDS>(0, 0, 0) - q.uhz(0, 0, -1) * q.whx<CDS>(0, 0, -1)) / dz;
+        double u0 = q.U(0, 0, 0);
+        double uzz = ((q.U(0, 0, 1) - u0) / dzt - (u0 - q.U(0, 0, -1)) / dzb) / dz;
+        if (A.dp_opt == 3)
+            f = -uux - uvy - uwz - um * ux + A.fc * (0.5 * (q.vhx<CDS>(0, 0, 0) + q.vhx<CDS>(0, -1, 0)) - A.vg);
+        else
+            f = -uux - uvy - uwz + dpx - um * ux;
+        double f0 = A.use_f0 ? A.ff0[c0] : 0.0;
+        qu = u0 + A.dt * (A.ra * f + A.rb * f0) + A.rc * 0.5 * A.dt * A.nu * (uxx + uyy + uzz) - A.rc * A.dt * dpdx;
+    }
+    // ---- V equation (:166-270)
+    {
+        double vux, vvy, vx, vxx, vyy, dpdy;
+        if (CDS == 1) {
+            vux = (q.vhx<1>(0, 0, 0) * q.uhy<1>(0, 0, 0) - q.vhx<1>(-1, 0, 0) * q.uhy<1>(-1, 0, 0)) / dx;
+            vvy = (sq(q.vhy<1>(0, 1, 0)) - sq(q.vhy<1>(0, 0, 0))) / dy;
+            vx = (q.vhx<1>(0, 0, 0) - q.vhx<1>(-1, 0, 0)) / dx;
+            vxx = (q.V(1, 0, 0) - 2.0 * q.V(0, 0, 0) + q.V(-1, 0, 0)) / dx2;
+            vyy = (q.V(0, 1, 0) - 2.0 * q.V(0, 0, 0) + q.V(0, -1, 0)) / dy2;
+            dpdy = (q.P(0, 1, 0) - q.P(0, 0, 0)) / dy;
+        } else {
+            double x1 = q.vhx<2>(1, 0, 0), x0 = q.vhx<2>(0, 0, 0), xm = q.vhx<2>(-1, 0, 0), xm2 = q.vhx<2>(-2, 0, 0);
+            vux = (-x1 * q.uhy<2>(1, 0, 0) + 27.0 * x0 * q.uhy<2>(0, 0, 0) - 27.0 * xm * q.uhy<2>(-1, 0, 0) +
+                   xm2 * q.uhy<2>(-2, 0, 0)) / 24.0 / dx;
+            vvy = (-sq(q.vhy<2>(0, 2, 0)) + 27.0 * sq(q.vhy<2>(0, 1, 0)) - 27.0 * sq(q.vhy<2>(0, 0, 0)) +
+                   sq(q.vhy<2>(0, -1, 0))) / 24.0 / dy;
+            vx = (-x1 + 27.0 * x0 - 27.0 * xm + xm2) / 24.0 / dx;
+            vxx = (-q.V(2, 0, 0) + 16.0 * q.V(1, 0, 0) - 30.0 * q.V(0, 0, 0) + 16.0 * q.V(-1, 0, 0) - q.V(-2, 0, 0)) / 12.0 / dx2;
+            vyy = (-q.V(0, 2, 0) + 16.0 * q.V(0, 1, 0) - 30.0 * q.V(0, 0, 0) + 16.0 * q.V(0, -1, 0) - q.V(0, -2, 0)) / 12.0 / dy2;
+            dpdy = (-q.P(0, 2, 0) + 27.0 * q.P(0, 1, 0) - 27.0 * q.P(0, 0, 0) + q.P(0, -1, 0)) / 24.0 / dy;
+        }
+        double vwz = (q.vhz(0, 0, 0) * q.why<CDS>(0, 0, 0) - q.vhz(0, 0, -1) * q.why<CDS>(0, 0, -1)) / dz;
+        double v0 = q.V(0, 0, 0);
+        double vzz = ((q.V(0, 0, 1) - v0) / dzt - (v0 - q.V(0, 0, -1)) / dzb) / dz;
+        if (A.dp_opt == 3)
+            g = -vux - vvy - vwz - um * vx + A.fc * (A.ug - 0.5 * (q.uhy<CDS>(0, 0, 0) + q.uhy<CDS>(-1, 0, 0)) - um);
+        else
+            g = -vux - vvy - vwz - um * vx;
+        double g0 = A.use_f0 ? A.gg0[c0] : 0.0;
+        qv = v0 + A.dt * (A.ra * g + A.rb * g0) + A.rc * 0.5 * A.dt * A.nu * (vxx + vyy + vzz) - A.rc * A.dt * dpdy;
+    }
+    // ---- W equation (:274-370), k < kend3
+    if (k < A.nz - 1) {
+        double wux, wvy, wx, wxx, wyy;
+        if (CDS == 1) {
+            wux = (q.whx<1>(0, 0, 0) * q.uhz(0, 0, 0) - q.whx<1>(-1, 0, 0) * q.uhz(-1, 0, 0)) / dx;
+            wvy = (q.why<1>(0, 0, 0) * q.vhz(0, 0, 0) - q.why<1>(0, -1, 0) * q.vhz(0, -1, 0)) / dy;
+            wx = (q.whx<1>(0, 0, 0) - q.whx<1>(-1, 0, 0)) / dx;
+            wxx = (q.W(1, 0, 0) - 2.0 * q.W(0, 0, 0) + q.W(-1, 0, 0)) / dx2;
+            wyy = (q.W(0, 1, 0) - 2.0 * q.W(0, 0, 0) + q.W(0, -1, 0)) / dy2;
+        } else {
+            double x1 = q.whx<2>(1, 0, 0), x0 = q.whx<2>(0, 0, 0), xm = q.whx<2>(-1, 0, 0), xm2 = q.whx<2>(-2, 0, 0);
+            wux = (-x1 * q.uhz(1, 0, 0) + 27.0 * x0 * q.uhz(0, 0, 0) - 27.0 * xm * q.uhz(-1, 0, 0) + xm2 * q.uhz(-2, 0, 0)) / 24.0 / dx;
+            wvy = (-q.why<2>(0, 1, 0) * q.vhz(0, 1, 0) + 27.0 * q.why<2>(0, 0, 0) * q.vhz(0, 0, 0) -
+                   27.0 * q.why<2>(0, -1, 0) * q.vhz(0, -1, 0) + q.why<2>(0, -2, 0) * q.vhz(0, -2, 0)) / 24.0 / dy;
+            wx = (-x1 + 27.0 * x0 - 27.0 * xm + xm2) / 24.0 / dx;
+            wxx = (-q.W(2, 0, 0) + 16.0 * q.W(1, 0, 0) - 30.0 * q.W(0, 0, 0) + 16.0 * q.W(-1, 0, 0) - q.W(-2, 0, 0)) / 12.0 / dx2;
+            wyy = (-q.W(0, 2, 0) + 16.0 * q.W(0, 1, 0) - 30.0 * q.W(0, 0, 0) + 16.0 * q.W(0, -1, 0) - q.W(0, -2, 0)) / 12.0 / dy2;
+        }
+        double w0 = q.W(0, 0, 0), wp = q.W(0, 0, 1), wm = q.W(0, 0, -1);
+        double a1 = sq(0.5 * (w0 + wp)), a2 = sq(0.5 * (w0 + wm));
+        double wwz = (a1 - a2) / dzt;
+        double wzz = ((wp - w0) / q.lev(LV_DZ, k + 1) - (w0 - wm) / dz) / dzt;
+        if (A.isscalar) {
+            double lt = q.lev(LV_LT, k);
+            h = -wux - wvy - wwz - um * wx + A.got * ((1.0 - lt) * q.T(0, 0, 0) + lt * q.T(0, 0, 1) - A.t_ref);
+        } else {
+            h = -wux - wvy - wwz - um * wx;
+        }
+        double dpdz = (q.P(0, 0, 1) - q.P(0, 0, 0)) / dzt;
+        double h0 = A.use_f0 ? A.hh0[c0] : 0.0;
+        qw = w0 + A.dt * (A.ra * h + A.rb * h0) + A.rc * 0.5 * A.dt * A.nu * (wxx + wyy + wzz) - A.rc * A.dt * dpdz;
+    } else {
+        h = 0.0;
+        qw = 0.0;
+    }
+}
+
+// Temperature right-hand side at one point, get_temperature_cnvdiff
+// (src/module_cnvdiff.f90:381-473), evaluated with the updated u, v, w.
+template <int CDS>
+HC_DEV void temperature_point(const Pt &q, double &s, double &qt)
+{
+    const StepArgs &A = q.A;
+    const int k = q.k;
+    const double dx = A.dx, dy = A.dy;
+    double tux, tvy, tx, txx, tyy;
+    if (CDS == 1) {
+        tux = (q.thx<1>(0, 0, 0) * q.U(0, 0, 0) - q.thx<1>(-1, 0, 0) * q.U(-1, 0, 0)) / dx;
+        tvy = (q.thy<1>(0, 0, 0) * q.V(0, 0, 0) - q.thy<1>(0, -1, 0) * q.V(0, -1, 0)) / dy;
+        tx = (q.thx<1>(0, 0, 0) - q.thx<1>(-1, 0, 0)) / dx;
+        txx = (q.T(1, 0, 0) - 2.0 * q.T(0, 0, 0) + q.T(-1, 0, 0)) / A.dx2;
+        tyy = (q.T(0, 1, 0) - 2.0 * q.T(0, 0, 0) + q.T(0, -1, 0)) / A.dy2;
+    } else {
+        double x1 = q.thx<2>(1, 0, 0), x0 = q.thx<2>(0, 0, 0), xm = q.thx<2>(-1, 0, 0), xm2 = q.thx<2>(-2, 0, 0);
+        tux = (-x1 * q.U(1, 0, 0) + 27.0 * x0 * q.U(0, 0, 0) - 27.0 * xm * q.U(-1, 0, 0) + xm2 * q.U(-2, 0, 0)) / 24.0 / dx;
+        tvy = (-q.thy<2>(0, 1, 0) * q.V(0, 1, 0) + 27.0 * q.thy<2>(0, 0, 0) * q.V(0, 0, 0) -
+               27.0 * q.thy<2>(0, -1, 0) * q.V(0, -1, 0) + q.thy<2>(0, -2, 0) * q.V(0, -2, 0)) / 24.0 / dy;
+        tx = (-x1 + 27.0 * x0 - 27.0 * xm + xm2) / 24.0 / dx;
+        txx = (-q.T(2, 0, 0) + 16.0 * q.T(1, 0, 0) - 30.0 * q.T(0, 0, 0) + 16.0 * q.T(-1, 0, 0) - q.T(-2, 0, 0)) / 12.0 / A.dx2;
+        tyy = (-q.T(0, 2, 0) + 16.0 * q.T(0, 1, 0) - 30.0 * q.T(0, 0, 0) + 16.0 * q.T(0, -1, 0) - q.T(0, -2, 0)) / 12.0 / A.dy2;
+    }
+    double twz = (q.thz(0) * q.W(0, 0, 0) - q.thz(-1) * q.W(0, 0, -1)) / q.lev(LV_DZ, k);
+    double t0 = q.T(0, 0, 0);
+    double tzz = ((q.T(0, 0, 1) - t0) / q.lev(LV_DZT, k) - (t0 - q.T(0, 0, -1)) / q.lev(LV_DZB, k)) / q.lev(LV_DZ, k);
+    s = -tux - tvy - twz - A.u_mrf * tx;
+    double s0 = A.use_f0 ? A.ss0[q.at(0, 0, k)] : 0.0;
+    qt = t0 + A.dt * (A.ra * s + A.rb * s0) + A.rc * 0.5 * A.dt * A.alpha * (txx + tyy + tzz);
+}
+
+// ---------------------------------------------------------------------------
+// Row transforms in LDS.  Rows of NX reals are packed as NX/2 complex.
+// R2C post-processing for rows already transformed by an NX/2-point FFT:
+// X(l) = (Z(l)+conj Z(NL-l))/2 + e^{-2 pi i l/NX} (Z(l)-conj Z(NL-l))/(2i)
+template <int NL>
+HC_DEV double2 r2c_post(const double2 *row, int l, const double2 *__restrict__ tw_nx, double norm)
+{
+    double2 z = row[l], zr = cconj(row[(NL - l) % NL]);
+    double2 e = make_double2(0.5 * (z.x + zr.x), 0.5 * (z.y + zr.y));
+    double2 dd = csub(z, zr);
+    double2 o = make_double2(0.5 * dd.y, -0.5 * dd.x);  // (Z - conj Z')/(2i)
+    double2 x = cadd(e, cmul(tw_nx[l], o));
+    return cscale(x, norm);
+}
+// C2R pre-processing: Z(l) = A + i e^{+2 pi i l/NX} B, A = X(l) + X~, B = X(l) - X~,
+// X~ = conj X(NL-l) (zero for l = 0, whose partner is the zeroed Nyquist bin).
+HC_DEV double2 c2r_pre(double2 xl, double2 xt, double2 twl)
+{
+    double2 a = cadd(xl, xt), b = csub(xl, xt);
+    double2 e = cmul(cconj(twl), b);          // e^{+i theta} B
+    return make_double2(a.x - e.y, a.y + e.x);  // A + i e
+}

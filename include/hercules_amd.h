@@ -1,0 +1,129 @@
+/*
+ * hercules_amd.h — C ABI of the MI355X-native HERCULES time-step hot path.
+ *
+ * One hc_ctx holds one channel (the reference's whole single-rank state:
+ * fields, grid metrics and equation coefficients) resident in the HBM of the
+ * current HIP device.  Every entry point returns 0 on success and a nonzero
+ * hc_status otherwise; hc_last_error() gives the message (thread-local).  A
+ * context owns one HIP stream; calls on one context must not overlap.
+ *
+ * Arrays crossing the ABI are FP64.  Two layouts are accepted, chosen per
+ * call by `layout`:
+ *   HC_LAYOUT_INTERIOR  (nx, ny, nz) column-major, x fastest, no ghosts;
+ *   HC_LAYOUT_GHOSTED   (nx+4, ny+4, nz+4) column-major with two ghost cells
+ *                       per side, the reference's own z-pencil arrays
+ *                       (src/module_variables.f90:32-74); only the interior
+ *                       istr3..iend3 x jstr3..jend3 x kstr3..kend3 is read or
+ *                       written.
+ * Pointers may be host or device memory.
+ *
+ * The reference's subroutines each entry point replaces are cited as
+ * path:line relative to the HERCULES source tree.
+ */
+#ifndef HERCULES_AMD_H
+#define HERCULES_AMD_H
+
+#ifdef __cplusplus
+extern "C" {
+#endif
+
+/* Namelist parameters of parameters.input (src/module_parameters.f90:40-157)
+ * that the time step reads.  p_row/p_col are absent: one GPU holds the
+ * whole domain.  I/O cadence parameters stay with the host driver. */
+typedef struct hc_params {
+    int nx, ny, nz;          /* &domain */
+    int ochannel;            /* 0 closed, 1 open channel */
+    int dp_opt;              /* 1 constant dp/dx, 2 constant bulk velocity, 3 Ekman */
+    int cds;                 /* 1 FD2, 2 FD4 (3 spectral: not yet supported) */
+    int dts;                 /* 1 AB2, 2 RK3 */
+    int issk;                /* skew-symmetric form (spectral only) */
+    int isdamp, nzdamp;      /* Rayleigh damping (not yet supported: must be 0) */
+    int isscalar;            /* temperature on/off */
+    int is_ri_var;           /* Richardson number ramp */
+    int imax;                /* number of steps (used by the Ri ramp) */
+    double lx, ly, lz, zstretch, cdamp;
+    double nu, alpha, got, fc, ug, vg, dt, u_mrf, t_ref, tbot, ttop, ri_str, ri_end;
+} hc_params;
+
+typedef struct hc_ctx hc_ctx;
+
+enum hc_status {
+    HC_OK = 0,
+    HC_ERR_ARG = 1,        /* invalid argument or flag (the reference stops) */
+    HC_ERR_UNSUPPORTED = 2,/* parameter combination not implemented */
+    HC_ERR_HIP = 3,        /* HIP runtime failure */
+    HC_ERR_NAN = 4         /* NaN found by the divergence check */
+};
+
+enum hc_field {
+    HC_U = 0, HC_V = 1, HC_W = 2, HC_P = 3, HC_T = 4,
+    HC_FF = 5, HC_GG = 6, HC_HH = 7, HC_SS = 8
+};
+
+enum hc_layout { HC_LAYOUT_INTERIOR = 0, HC_LAYOUT_GHOSTED = 1 };
+
+/* Equation selector for hc_poisson_solver_fft, i.e. the flag tuples
+ * (zstagger, delnull, tbnd, ubnd) used at src/module_navier_stokes.f90:293-337:
+ * u (0,0,0,1), v (0,0,0,0), w (1,0,0,0), p (0,1,0,0), t (0,0,1,0). */
+enum hc_equation { HC_EQ_U = 0, HC_EQ_V = 1, HC_EQ_W = 2, HC_EQ_P = 3, HC_EQ_T = 4 };
+
+/* Message of the last failing call on this thread. */
+const char *hc_last_error(void);
+
+/* Replaces initiate_parameters, initiate_indices, get_mesh_param,
+ * initiate_fields (allocation/zeroing), get_fft_plan and
+ * get_{u,w,p,t}_eqn_coeff: src/module_parameters.f90:242-318,371-508,551-576;
+ * src/module_tools.f90:33-227; src/module_poisson_solver.f90:32-122,181-477.
+ * Validates the parameters as initiate_parameters does (:256-270). */
+int hc_create(const hc_params *prm, hc_ctx **out);
+int hc_destroy(hc_ctx *ctx);
+
+/* Copy a module field in / out: the transfer half of read_init_fields
+ * (src/module_tools.f90:260-316) and output_backup (src/module_io.f90:311-360).
+ * u is the frame-relative module variable; the +-u_mrf shift those two
+ * routines apply at the file boundary stays with the caller. */
+int hc_set_field(hc_ctx *ctx, int field, const double *src, int layout);
+int hc_get_field(hc_ctx *ctx, int field, double *dst, int layout);
+
+/* Start-up part of drive after the fields are read: bulk velocity for
+ * dp_opt=2, Ri ramp start, boundary values (src/module_navier_stokes.f90:60-129). */
+int hc_init(hc_ctx *ctx);
+
+/* time_advancement_fd: one full step, all RK3 (or AB2) substeps
+ * (src/module_navier_stokes.f90:223-365). */
+int hc_time_advancement_fd(hc_ctx *ctx);
+
+/* nsteps passes of the drive loop body: update_ri, time_advancement_fd and
+ * calculate_ubar (src/module_navier_stokes.f90:139-165), output excluded. */
+int hc_drive_steps(hc_ctx *ctx, int nsteps);
+
+/* poisson_solver_fft for one equation (src/module_poisson_solver.f90:483-658)
+ * with its coefficients assigned at rk_c by assign_abc / assign_abc_p
+ * (src/module_navier_stokes.f90:523-619).  r_src and r_var are full fields
+ * in `layout`; r_var's interior is overwritten (level nz of w is set to 0). */
+int hc_poisson_solver_fft(hc_ctx *ctx, int equation, double rk_c, const double *r_src,
+                          double *r_var, int layout);
+
+/* calculate_ubar (src/module_parameters.f90:513-546) on the current u. */
+int hc_calculate_ubar(hc_ctx *ctx, double *ubar);
+
+/* screen_div_error values (src/module_io.f90:85-206): mean |div|, the
+ * reference's signed maximum, and HC_ERR_NAN when u, v or w hold a NaN. */
+int hc_screen_div_error(hc_ctx *ctx, double *div_avg, double *div_max);
+
+/* Scalars of the reference's module state: 0 dpx_drive, 1 ubar, 2 time_sim,
+ * 3 got, 4 number of steps taken. */
+int hc_get_scalar(hc_ctx *ctx, int id, double *value);
+
+/* The HIP stream (hipStream_t) all work of this context is queued on, and
+ * the device address of a field's interior (nx, ny, nz) array. */
+void *hc_stream(hc_ctx *ctx);
+double *hc_field_device_ptr(hc_ctx *ctx, int field);
+
+/* Wait for all queued work of the context. */
+int hc_synchronize(hc_ctx *ctx);
+
+#ifdef __cplusplus
+}
+#endif
+#endif

@@ -1,0 +1,1175 @@
+/*
+ * ORACLE — test infrastructure only.
+ *
+ * A plain-C restatement of HERCULES's finite-difference time step, used as
+ * the correctness checker for the MI355X kernels.  Only tests/, the smoke()
+ * entry and the cpu_baseline leg of bench.py may load this library; nothing
+ * in the product links or calls it.
+ *
+ * It mirrors the reference's algorithm AND its data layout: every field is a
+ * Fortran column-major array (nx+4, ny+4, nz+4) with two ghost cells per side
+ * and 1-based indices, the interior being istr3..iend3 = 3..nx+2 etc.
+ * (src/module_parameters.f90:371-398, src/module_tools.f90:41-89).  One MPI
+ * rank is assumed (p_row = p_col = 1), so 2DECOMP transposes are identity
+ * reshapes and comm_bound is a periodic copy (src/module_boundary.f90:361-487).
+ *
+ * Third-party arithmetic: the reference calls FFTW 3.3.4 (fftw_plan_dft_1d,
+ * FFTW_FORWARD = exp(-2 pi i jk/n), unnormalised; src/module_parameters.f90:565-572).
+ * FFTW is not linked here; a mixed-radix decimation-in-time DFT of the same
+ * definition is used instead (fft_rec below), pinned by the tests against
+ * numpy.fft.
+ *
+ * Each function cites the reference lines it restates.  Pinning of every
+ * function is listed in tests/test_oracle_*.py (closed forms, Fourier-mode
+ * symbols, scipy banded solves, invariants).
+ */
+#include <complex.h>
+#include <math.h>
+#include <stdlib.h>
+#include <string.h>
+#include <stdio.h>
+
+typedef double complex cplx;
+
+/* Same field order and types as hc_params in include/hercules_amd.h. */
+typedef struct ho_params {
+    int nx, ny, nz;
+    int ochannel, dp_opt, cds, dts, issk, isdamp, nzdamp, isscalar, is_ri_var, imax;
+    double lx, ly, lz, zstretch, cdamp;
+    double nu, alpha, got, fc, ug, vg, dt, u_mrf, t_ref, tbot, ttop, ri_str, ri_end;
+} ho_params;
+
+typedef struct ho_state {
+    ho_params prm;
+    int nia, nja, nka, istr3, iend3, jstr3, jend3, kstr3, kend3;
+    double dx, dy, dx2, dy2, dpx_coef, dpx_drive, ubar, got;
+    cplx wx1, wy1;
+    double rk_a, rk_b, rk_c;
+    double *dz, *l_t, *l_b, *dz_t, *dz_b;
+    double *u, *v, *w, *p, *phi, *t;
+    double *ff, *gg, *hh, *ss, *ff0, *gg0, *hh0, *ss0, *qu, *qv, *qw;
+    double *uhx, *uhy, *uhz, *vhx, *vhy, *vhz, *whx, *why, *thx, *thy, *thz;
+    double *aa_u, *cc_u, *aa_w, *cc_w, *aa_p, *cc_p, *aa_t, *cc_t;
+    cplx *bb_u, *bb_w, *bb_p, *bb_t;
+    double *aa, *cc;
+    cplx *bb;
+    cplx *c_src, *c_var, *rowbuf;
+    double div_avg, div_max, div_hist_max, time_sim;
+    int stps;
+    char err[256];
+} ho_state;
+
+static const double PI = 3.1415926535897932; /* src/module_parameters.f90:32 */
+
+/* RK3 coefficients, Orlandi 2000: src/module_parameters.f90:210-212 */
+static const double RKC1[3] = {8.0 / 15.0, 5.0 / 12.0, 3.0 / 4.0};
+static const double RKC2[3] = {0.0, -17.0 / 60.0, -5.0 / 12.0};
+static const double RKC3[3] = {8.0 / 15.0, 2.0 / 15.0, 1.0 / 3.0};
+
+/* ---- index helpers (1-based, Fortran order) ---------------------------- */
+#define F3(S, a, i, j, k) ((a)[((size_t)(k) - 1) * (size_t)(S)->nia * (size_t)(S)->nja + \
+                               ((size_t)(j) - 1) * (size_t)(S)->nia + (size_t)(i) - 1])
+/* (nx, ny, nz+4) coefficient arrays: aa, bb, cc, bb_* */
+#define B3(S, a, i, j, k) ((a)[((size_t)(k) - 1) * (size_t)(S)->prm.nx * (size_t)(S)->prm.ny + \
+                               ((size_t)(j) - 1) * (size_t)(S)->prm.nx + (size_t)(i) - 1])
+/* (nx, ny, nz) complex scratch: c_src, c_var */
+#define C3(S, a, i, j, k) ((a)[((size_t)(k) - 1) * (size_t)(S)->prm.nx * (size_t)(S)->prm.ny + \
+                               ((size_t)(j) - 1) * (size_t)(S)->prm.nx + (size_t)(i) - 1])
+#define V1(a, k) ((a)[(k) - 1])
+
+static size_t nfield(const ho_state *S) { return (size_t)S->nia * S->nja * S->nka; }
+
+/* ---- FFT: unnormalised DFT, X_k = sum_n x_n exp(sign 2 pi i nk/N) ------- */
+static int smallest_factor(int n)
+{
+    for (int f = 2; f * f <= n; ++f)
+        if (n % f == 0) return f;
+    return n;
+}
+
+/* mixed-radix decimation in time; tw[e] = exp(sign 2 pi i e / N) */
+static void fft_rec(const cplx *in, int n, int stride, cplx *out, const cplx *tw, int N)
+{
+    if (n == 1) { out[0] = in[0]; return; }
+    int p = smallest_factor(n), m = n / p;
+    for (int r = 0; r < p; ++r) fft_rec(in + (size_t)r * stride, m, stride * p, out + (size_t)r * m, tw, N);
+    int step = N / n; /* W_n^e = W_N^(e*step) */
+    cplx t[64];
+    if (p > 64) { fprintf(stderr, "oracle fft: prime factor %d too large\n", p); abort(); }
+    for (int k = 0; k < m; ++k) {
+        for (int r = 0; r < p; ++r) t[r] = out[(size_t)r * m + k] * tw[((size_t)r * k * step) % N];
+        for (int q = 0; q < p; ++q) {
+            cplx acc = 0;
+            for (int r = 0; r < p; ++r) acc += t[r] * tw[((size_t)r * q * m * step) % N];
+            out[(size_t)q * m + k] = acc;
+        }
+    }
+}
+
+typedef struct { int n, sign; cplx *tw; } ho_plan;
+
+static ho_plan plan_make(int n, int sign)
+{
+    ho_plan pl = {n, sign, (cplx *)malloc(sizeof(cplx) * n)};
+    for (int e = 0; e < n; ++e) {
+        double a = 2.0 * PI * (double)e / (double)n;
+        pl.tw[e] = cos(a) + (double)sign * sin(a) * I;
+    }
+    return pl;
+}
+
+/* x[0..n-1] with stride xs, in place, via buffers in/out of length n */
+static void plan_exec(const ho_plan *pl, cplx *x, size_t xs, cplx *bin, cplx *bout)
+{
+    for (int e = 0; e < pl->n; ++e) bin[e] = x[(size_t)e * xs];
+    fft_rec(bin, pl->n, 1, bout, pl->tw, pl->n);
+    for (int e = 0; e < pl->n; ++e) x[(size_t)e * xs] = bout[e];
+}
+
+/* exported for pinning against numpy.fft */
+void ho_fft1d(cplx *x, int n, int sign)
+{
+    ho_plan pl = plan_make(n, sign);
+    cplx *a = (cplx *)malloc(sizeof(cplx) * n), *b = (cplx *)malloc(sizeof(cplx) * n);
+    plan_exec(&pl, x, 1, a, b);
+    free(a); free(b); free(pl.tw);
+}
+
+/* ---- state -------------------------------------------------------------- */
+static double *dalloc(size_t n) { return (double *)calloc(n, sizeof(double)); }
+static cplx *calloc_c(size_t n) { return (cplx *)calloc(n, sizeof(cplx)); }
+
+void ho_destroy(ho_state *S);
+
+/* initiate_parameters + initiate_indices + initiate_fields (allocation part)
+ * src/module_parameters.f90:242-318,371-398; src/module_tools.f90:33-227 */
+ho_state *ho_create(const ho_params *P)
+{
+    ho_state *S = (ho_state *)calloc(1, sizeof(ho_state));
+    S->prm = *P;
+    if (P->nx % 2 || P->ny % 2 || P->nz < 3) { ho_destroy(S); return NULL; }
+    S->istr3 = 3; S->iend3 = P->nx + 2;
+    S->jstr3 = 3; S->jend3 = P->ny + 2;
+    S->kstr3 = 3; S->kend3 = P->nz + 2;
+    S->nia = P->nx + 4; S->nja = P->ny + 4; S->nka = P->nz + 4;
+    S->dx = P->lx / P->nx; S->dy = P->ly / P->ny;
+    S->dx2 = S->dx * S->dx; S->dy2 = S->dy * S->dy;
+    S->dpx_coef = P->nu * 1.0;
+    S->wx1 = cexp(I * 2.0 * PI / P->nx);
+    S->wy1 = cexp(I * 2.0 * PI / P->ny);
+    S->got = P->got;
+    if (P->dp_opt == 1) S->dpx_drive = 1.0; else { S->dpx_drive = 0.0; S->ubar = 0.0; }
+    size_t nf = nfield(S), nb = (size_t)P->nx * P->ny * S->nka, nc = (size_t)P->nx * P->ny * P->nz;
+    double **fl[] = {&S->u, &S->v, &S->w, &S->p, &S->phi, &S->t, &S->ff, &S->gg, &S->hh, &S->ss,
+                     &S->ff0, &S->gg0, &S->hh0, &S->ss0, &S->qu, &S->qv, &S->qw, &S->uhx, &S->uhy,
+                     &S->uhz, &S->vhx, &S->vhy, &S->vhz, &S->whx, &S->why, &S->thx, &S->thy, &S->thz};
+    for (size_t q = 0; q < sizeof(fl) / sizeof(fl[0]); ++q) *fl[q] = dalloc(nf);
+    double **v1[] = {&S->dz, &S->l_t, &S->l_b, &S->dz_t, &S->dz_b, &S->aa_u, &S->cc_u, &S->aa_w,
+                     &S->cc_w, &S->aa_p, &S->cc_p, &S->aa_t, &S->cc_t};
+    for (size_t q = 0; q < sizeof(v1) / sizeof(v1[0]); ++q) *v1[q] = dalloc(S->nka);
+    S->bb_u = calloc_c(nb); S->bb_w = calloc_c(nb); S->bb_p = calloc_c(nb); S->bb_t = calloc_c(nb);
+    S->aa = dalloc(nb); S->cc = dalloc(nb); S->bb = calloc_c(nb);
+    S->c_src = calloc_c(nc); S->c_var = calloc_c(nc);
+    int nmax = P->nx > P->ny ? P->nx : P->ny;
+    S->rowbuf = calloc_c((size_t)4 * nmax);
+    return S;
+}
+
+void ho_destroy(ho_state *S)
+{
+    if (!S) return;
+    double *fl[] = {S->u, S->v, S->w, S->p, S->phi, S->t, S->ff, S->gg, S->hh, S->ss, S->ff0, S->gg0,
+                    S->hh0, S->ss0, S->qu, S->qv, S->qw, S->uhx, S->uhy, S->uhz, S->vhx, S->vhy,
+                    S->vhz, S->whx, S->why, S->thx, S->thy, S->thz, S->dz, S->l_t, S->l_b, S->dz_t,
+                    S->dz_b, S->aa_u, S->cc_u, S->aa_w, S->cc_w, S->aa_p, S->cc_p, S->aa_t, S->cc_t,
+                    S->aa, S->cc};
+    for (size_t q = 0; q < sizeof(fl) / sizeof(fl[0]); ++q) free(fl[q]);
+    free(S->bb_u); free(S->bb_w); free(S->bb_p); free(S->bb_t); free(S->bb);
+    free(S->c_src); free(S->c_var); free(S->rowbuf);
+    free(S);
+}
+
+/* ---- A1: get_mesh_param, src/module_parameters.f90:432-508 ------------- */
+void ho_mesh(ho_state *S)
+{
+    const ho_params *P = &S->prm;
+    int nka = S->nka;
+    double zmin = 0.0, zmax = P->lz, zmean = 0.0, coeff_a = 0.0;
+    double *zz = dalloc(nka);
+    if (P->ochannel == 0) { zmean = 0.5 * (zmin + zmax); coeff_a = (2.0 + P->nz) * 0.5; }
+    else                  { zmean = (zmin + zmax);       coeff_a = (1.0 + P->nz) * 1.0; }
+    for (int k = 1; k <= P->nz + 1; ++k)
+        V1(zz, k + 2) = zmean + (zmin - zmean) / tanh(2.0 * P->zstretch) *
+                                tanh(2.0 * P->zstretch * (k - coeff_a) / (1.0 - coeff_a));
+    for (int k = 1; k <= nka; ++k) V1(S->dz, k) = 0.0;
+    for (int k = S->kstr3; k <= S->kend3; ++k) V1(S->dz, k) = V1(zz, k + 1) - V1(zz, k);
+    V1(S->dz, S->kstr3 - 1) = V1(S->dz, S->kstr3);
+    V1(S->dz, S->kend3 + 1) = V1(S->dz, S->kend3);
+    V1(S->dz, S->kstr3 - 2) = V1(S->dz, S->kstr3);
+    V1(S->dz, S->kend3 + 2) = V1(S->dz, S->kend3);
+    for (int k = 1; k <= nka; ++k) { V1(S->l_t, k) = V1(S->l_b, k) = V1(S->dz_t, k) = V1(S->dz_b, k) = 0.0; }
+    for (int k = S->kstr3 - 1; k <= S->kend3 + 1; ++k) {
+        V1(S->l_t, k) = V1(S->dz, k) / (V1(S->dz, k) + V1(S->dz, k + 1));
+        V1(S->l_b, k) = V1(S->dz, k) / (V1(S->dz, k) + V1(S->dz, k - 1));
+        V1(S->dz_t, k) = 0.5 * (V1(S->dz, k) + V1(S->dz, k + 1));
+        V1(S->dz_b, k) = 0.5 * (V1(S->dz, k) + V1(S->dz, k - 1));
+    }
+    free(zz);
+}
+
+/* ---- integer and real powers of a complex number as Fortran evaluates
+ *      them: z**n (integer n) by repeated products, z**x (real x) = exp(x log z) */
+static cplx cipow(cplx z, int n)
+{
+    cplx r = 1.0, b = z;
+    int e = n < 0 ? -n : n;
+    while (e) { if (e & 1) r *= b; b *= b; e >>= 1; }
+    return n < 0 ? 1.0 / r : r;
+}
+static cplx crpow(cplx z, double x) { return x == 0.0 ? 1.0 : cpow(z, x); }
+
+/* wavenumber index as the reference maps it: src/module_poisson_solver.f90:57-69 */
+static int lwave(int i, int n) { return i <= n / 2 ? i - 1 : i - n - 1; }
+
+/* horizontal symbols of the FD second-derivative operators */
+static cplx kx_helm(const ho_state *S, cplx w1, int l, double d2, double coef)
+{
+    /* u/w/t equations: src/module_poisson_solver.f90:222-246 (coef = nu or alpha) */
+    const ho_params *P = &S->prm;
+    if (P->cds == 1) return 0.5 * P->dt * (cipow(w1, l) + cipow(w1, -l) - 2.0) * coef / d2;
+    if (P->cds == 2)
+        return P->dt * (-crpow(w1, 2.0 * l) + 16.0 * cipow(w1, l) - 30.0 + 16.0 * cipow(w1, -l) -
+                        crpow(w1, -2.0 * l)) / 24.0 * coef / d2;
+    return 0; /* cds=3 handled by caller */
+}
+
+static cplx kx_pois(const ho_state *S, cplx w1, int l, double d2)
+{
+    /* p equation: src/module_poisson_solver.f90:73-101 */
+    const ho_params *P = &S->prm;
+    if (P->cds == 1) return (cipow(w1, l) + cipow(w1, -l) - 2.0) / d2;
+    if (P->cds == 2)
+        return (crpow(w1, 3.0 * l) - 54.0 * crpow(w1, 2.0 * l) + 783.0 * cipow(w1, l) - 1460.0 +
+                783.0 * cipow(w1, -l) - 54.0 * crpow(w1, -2.0 * l) + crpow(w1, -3.0 * l)) / 576.0 / d2;
+    return 0;
+}
+
+/* ---- A8: equation coefficients ----------------------------------------- */
+/* get_p_eqn_coeff, src/module_poisson_solver.f90:32-122 */
+static void coeff_p(ho_state *S)
+{
+    const ho_params *P = &S->prm;
+    for (int k = 1; k <= S->nka; ++k) { V1(S->aa_p, k) = 0; V1(S->cc_p, k) = 0; }
+    memset(S->bb_p, 0, sizeof(cplx) * (size_t)P->nx * P->ny * S->nka);
+    for (int k = S->kstr3; k <= S->kend3; ++k) {
+        V1(S->aa_p, k) = 1.0 / V1(S->dz, k) / V1(S->dz_b, k);
+        V1(S->cc_p, k) = 1.0 / V1(S->dz, k) / V1(S->dz_t, k);
+        for (int j = 1; j <= P->ny; ++j)
+            for (int i = 1; i <= P->nx; ++i) {
+                int l = lwave(i, P->nx), m = lwave(j, P->ny);
+                cplx kx, ky;
+                if (P->cds == 3) {
+                    kx = -pow(2.0 * PI / P->lx * l, 2.0);
+                    ky = -pow(2.0 * PI / P->ly * m, 2.0);
+                } else {
+                    kx = kx_pois(S, S->wx1, l, S->dx2);
+                    ky = kx_pois(S, S->wy1, m, S->dy2);
+                }
+                cplx *b = &B3(S, S->bb_p, i, j, k);
+                *b = kx + ky - V1(S->aa_p, k) - V1(S->cc_p, k);
+                if (k == S->kstr3) { *b += V1(S->aa_p, k); V1(S->aa_p, k) = 0.0; }
+                if (k == S->kend3) { *b += V1(S->cc_p, k); V1(S->cc_p, k) = 0.0; }
+            }
+    }
+}
+
+/* get_u_eqn_coeff (src/module_poisson_solver.f90:181-282), get_t_eqn_coeff
+ * (:379-477).  Note the 1-D aa(k)/cc(k) are zeroed inside the (i,j) loop, so
+ * only the first visited mode (i=j=1, i.e. l=m=0) receives the -aa/-cc wall
+ * correction on the diagonal; the order of the loops is kept exactly. */
+static void coeff_uvt(ho_state *S, double *aa, cplx *bb, double *cc, double coef, int is_t)
+{
+    const ho_params *P = &S->prm;
+    for (int k = 1; k <= S->nka; ++k) { V1(aa, k) = 0; V1(cc, k) = 0; }
+    memset(bb, 0, sizeof(cplx) * (size_t)P->nx * P->ny * S->nka);
+    for (int k = S->kstr3; k <= S->kend3; ++k) {
+        V1(aa, k) = -0.5 * P->dt * coef / V1(S->dz, k) / V1(S->dz_b, k);
+        V1(cc, k) = -0.5 * P->dt * coef / V1(S->dz, k) / V1(S->dz_t, k);
+        for (int j = 1; j <= P->ny; ++j)
+            for (int i = 1; i <= P->nx; ++i) {
+                int l = lwave(i, P->nx), m = lwave(j, P->ny);
+                cplx kx, ky;
+                if (P->cds == 3) {
+                    kx = 0.5 * P->dt * coef * (-pow(2.0 * PI / P->lx * l, 2.0));
+                    ky = 0.5 * P->dt * coef * (-pow(2.0 * PI / P->ly * m, 2.0));
+                } else {
+                    kx = kx_helm(S, S->wx1, l, S->dx2, coef);
+                    ky = kx_helm(S, S->wy1, m, S->dy2, coef);
+                }
+                cplx *b = &B3(S, bb, i, j, k);
+                *b = -kx - ky - V1(aa, k) - V1(cc, k);
+                if (k == S->kstr3) { *b -= V1(aa, k); V1(aa, k) = 0.0; }
+                if (k == S->kend3) {
+                    if (is_t || P->ochannel == 0) { *b -= V1(cc, k); V1(cc, k) = 0.0; }
+                    else                          { *b += V1(cc, k); V1(cc, k) = 0.0; }
+                }
+            }
+    }
+}
+
+/* get_w_eqn_coeff, src/module_poisson_solver.f90:287-374 */
+static void coeff_w(ho_state *S)
+{
+    const ho_params *P = &S->prm;
+    double *aa = S->aa_w, *cc = S->cc_w;
+    cplx *bb = S->bb_w;
+    for (int k = 1; k <= S->nka; ++k) { V1(aa, k) = 0; V1(cc, k) = 0; }
+    memset(bb, 0, sizeof(cplx) * (size_t)P->nx * P->ny * S->nka);
+    for (int k = S->kstr3; k <= S->kend3 - 1; ++k) {
+        V1(aa, k) = -0.5 * P->dt * P->nu / V1(S->dz, k) / V1(S->dz_t, k);
+        V1(cc, k) = -0.5 * P->dt * P->nu / V1(S->dz, k + 1) / V1(S->dz_t, k);
+        for (int j = 1; j <= P->ny; ++j)
+            for (int i = 1; i <= P->nx; ++i) {
+                int l = lwave(i, P->nx), m = lwave(j, P->ny);
+                cplx kx, ky;
+                if (P->cds == 3) {
+                    kx = 0.5 * P->dt * P->nu * (-pow(2.0 * PI / P->lx * l, 2.0));
+                    ky = 0.5 * P->dt * P->nu * (-pow(2.0 * PI / P->ly * m, 2.0));
+                } else {
+                    kx = kx_helm(S, S->wx1, l, S->dx2, P->nu);
+                    ky = kx_helm(S, S->wy1, m, S->dy2, P->nu);
+                }
+                B3(S, bb, i, j, k) = -kx - ky - V1(aa, k) - V1(cc, k);
+                if (k == S->kstr3) V1(aa, k) = 0.0;
+                if (k == S->kend3 - 1) V1(cc, k) = 0.0;
+            }
+    }
+}
+
+void ho_coefficients(ho_state *S)
+{
+    coeff_uvt(S, S->aa_u, S->bb_u, S->cc_u, S->prm.nu, 0);
+    coeff_w(S);
+    coeff_p(S);
+    coeff_uvt(S, S->aa_t, S->bb_t, S->cc_t, S->prm.alpha, 1);
+}
+
+/* ---- A5: boundary conditions, src/module_boundary.f90 ------------------ */
+/* comm_bound on one rank: periodic copy of two ghost layers in x then y,
+ * over k = kstr3-1..kend3+1 (src/module_boundary.f90:361-487) */
+static void comm_bound(ho_state *S, double *a)
+{
+    int is = S->istr3, ie = S->iend3, js = S->jstr3, je = S->jend3;
+    for (int k = S->kstr3 - 1; k <= S->kend3 + 1; ++k) {
+        for (int j = js; j <= je; ++j) {
+            F3(S, a, is - 1, j, k) = F3(S, a, ie, j, k);
+            F3(S, a, is - 2, j, k) = F3(S, a, ie - 1, j, k);
+            F3(S, a, ie + 1, j, k) = F3(S, a, is, j, k);
+            F3(S, a, ie + 2, j, k) = F3(S, a, is + 1, j, k);
+        }
+        for (int i = is - 2; i <= ie + 2; ++i) {
+            F3(S, a, i, js - 1, k) = F3(S, a, i, je, k);
+            F3(S, a, i, js - 2, k) = F3(S, a, i, je - 1, k);
+            F3(S, a, i, je + 1, k) = F3(S, a, i, js, k);
+            F3(S, a, i, je + 2, k) = F3(S, a, i, js + 1, k);
+        }
+    }
+}
+
+/* update_boundary_uvw, src/module_boundary.f90:33-128 */
+static void bc_uvw(ho_state *S)
+{
+    const ho_params *P = &S->prm;
+    int ks = S->kstr3, ke = S->kend3;
+    for (int j = S->jstr3; j <= S->jend3; ++j)
+        for (int i = S->istr3; i <= S->iend3; ++i) {
+            if (P->ochannel == 0) {
+                F3(S, S->u, i, j, ke + 1) = -F3(S, S->u, i, j, ke) - 2.0 * P->u_mrf;
+                F3(S, S->v, i, j, ke + 1) = -F3(S, S->v, i, j, ke);
+            } else {
+                F3(S, S->u, i, j, ke + 1) = F3(S, S->u, i, j, ke);
+                F3(S, S->v, i, j, ke + 1) = F3(S, S->v, i, j, ke);
+            }
+            F3(S, S->u, i, j, ks - 1) = -F3(S, S->u, i, j, ks) - 2.0 * P->u_mrf;
+            F3(S, S->v, i, j, ks - 1) = -F3(S, S->v, i, j, ks);
+            F3(S, S->w, i, j, ke) = 0.0;
+            F3(S, S->w, i, j, ke + 1) = -F3(S, S->w, i, j, ke - 1);
+            F3(S, S->w, i, j, ks - 1) = 0.0;
+        }
+    if (P->cds != 3) { comm_bound(S, S->u); comm_bound(S, S->v); comm_bound(S, S->w); }
+}
+
+/* update_boundary_p, src/module_boundary.f90:133-162 */
+static void bc_p(ho_state *S, double *p)
+{
+    for (int j = S->jstr3; j <= S->jend3; ++j)
+        for (int i = S->istr3; i <= S->iend3; ++i) {
+            F3(S, p, i, j, S->kend3 + 1) = F3(S, p, i, j, S->kend3);
+            F3(S, p, i, j, S->kstr3 - 1) = F3(S, p, i, j, S->kstr3);
+        }
+    if (S->prm.cds != 3) comm_bound(S, p);
+}
+
+/* update_boundary_t, src/module_boundary.f90:167-196 */
+static void bc_t(ho_state *S)
+{
+    for (int j = S->jstr3; j <= S->jend3; ++j)
+        for (int i = S->istr3; i <= S->iend3; ++i) {
+            F3(S, S->t, i, j, S->kend3 + 1) = 2.0 * S->prm.ttop - F3(S, S->t, i, j, S->kend3);
+            F3(S, S->t, i, j, S->kstr3 - 1) = 2.0 * S->prm.tbot - F3(S, S->t, i, j, S->kstr3);
+        }
+    if (S->prm.cds != 3) comm_bound(S, S->t);
+}
+
+/* update_boundary_uvwhxyz, src/module_boundary.f90:202-305 */
+static void bc_uvwh(ho_state *S)
+{
+    const ho_params *P = &S->prm;
+    int ks = S->kstr3, ke = S->kend3;
+    double um = P->u_mrf;
+    for (int j = S->jstr3; j <= S->jend3; ++j)
+        for (int i = S->istr3; i <= S->iend3; ++i) {
+            if (P->ochannel == 0) {
+                F3(S, S->uhx, i, j, ke + 1) = -F3(S, S->uhx, i, j, ke) - 2.0 * um;
+                F3(S, S->uhy, i, j, ke + 1) = -F3(S, S->uhy, i, j, ke) - 2.0 * um;
+                F3(S, S->uhz, i, j, ke) = -um;
+                F3(S, S->uhz, i, j, ke + 1) = -F3(S, S->uhz, i, j, ke - 1) - 2.0 * um;
+                F3(S, S->vhx, i, j, ke + 1) = -F3(S, S->vhx, i, j, ke);
+                F3(S, S->vhy, i, j, ke + 1) = -F3(S, S->vhy, i, j, ke);
+                F3(S, S->vhz, i, j, ke) = 0.0;
+                F3(S, S->vhz, i, j, ke + 1) = -F3(S, S->vhz, i, j, ke - 1);
+            } else {
+                F3(S, S->uhx, i, j, ke + 1) = F3(S, S->uhx, i, j, ke);
+                F3(S, S->uhy, i, j, ke + 1) = F3(S, S->uhy, i, j, ke);
+                F3(S, S->uhz, i, j, ke) = F3(S, S->u, i, j, ke);
+                F3(S, S->uhz, i, j, ke + 1) = F3(S, S->uhz, i, j, ke);
+                F3(S, S->vhx, i, j, ke + 1) = F3(S, S->vhx, i, j, ke);
+                F3(S, S->vhy, i, j, ke + 1) = F3(S, S->vhy, i, j, ke);
+                F3(S, S->vhz, i, j, ke) = F3(S, S->v, i, j, ke);
+                F3(S, S->vhz, i, j, ke + 1) = F3(S, S->vhz, i, j, ke);
+            }
+            F3(S, S->whx, i, j, ke) = 0.0;
+            F3(S, S->whx, i, j, ke + 1) = -F3(S, S->whx, i, j, ke - 1);
+            F3(S, S->why, i, j, ke) = 0.0;
+            F3(S, S->why, i, j, ke + 1) = -F3(S, S->why, i, j, ke - 1);
+
+            F3(S, S->uhx, i, j, ks - 1) = -F3(S, S->uhx, i, j, ks) - 2.0 * um;
+            F3(S, S->uhy, i, j, ks - 1) = -F3(S, S->uhy, i, j, ks) - 2.0 * um;
+            F3(S, S->uhz, i, j, ks - 1) = -um;
+            F3(S, S->vhx, i, j, ks - 1) = -F3(S, S->vhx, i, j, ks);
+            F3(S, S->vhy, i, j, ks - 1) = -F3(S, S->vhy, i, j, ks);
+            F3(S, S->vhz, i, j, ks - 1) = 0.0;
+            F3(S, S->whx, i, j, ks - 1) = 0.0;
+            F3(S, S->why, i, j, ks - 1) = 0.0;
+        }
+    if (P->cds != 3) {
+        double *h[] = {S->uhx, S->uhy, S->uhz, S->vhx, S->vhy, S->vhz, S->whx, S->why};
+        for (int q = 0; q < 8; ++q) comm_bound(S, h[q]);
+    }
+}
+
+/* update_boundary_thxyz, src/module_boundary.f90:311-355 */
+static void bc_th(ho_state *S)
+{
+    double tt = S->prm.ttop, tb = S->prm.tbot;
+    int ks = S->kstr3, ke = S->kend3;
+    for (int j = S->jstr3; j <= S->jend3; ++j)
+        for (int i = S->istr3; i <= S->iend3; ++i) {
+            F3(S, S->thx, i, j, ke + 1) = -F3(S, S->thx, i, j, ke) + 2.0 * tt;
+            F3(S, S->thy, i, j, ke + 1) = -F3(S, S->thy, i, j, ke) + 2.0 * tt;
+            F3(S, S->thz, i, j, ke) = tt;
+            F3(S, S->thz, i, j, ke + 1) = -F3(S, S->thz, i, j, ke - 1) + 2.0 * tt;
+            F3(S, S->thx, i, j, ks - 1) = -F3(S, S->thx, i, j, ks) + 2.0 * tb;
+            F3(S, S->thy, i, j, ks - 1) = -F3(S, S->thy, i, j, ks) + 2.0 * tb;
+            F3(S, S->thz, i, j, ks - 1) = tb;
+        }
+    if (S->prm.cds != 3) { comm_bound(S, S->thx); comm_bound(S, S->thy); comm_bound(S, S->thz); }
+}
+
+/* ---- A4: interpolation, src/module_tools.f90:410-512 -------------------- */
+static void interp_uvw(ho_state *S)
+{
+    const ho_params *P = &S->prm;
+    double *u = S->u, *v = S->v, *w = S->w;
+#pragma omp parallel for
+    for (int k = S->kstr3; k <= S->kend3; ++k)
+        for (int j = S->jstr3; j <= S->jend3; ++j)
+            for (int i = S->istr3; i <= S->iend3; ++i) {
+                if (P->cds == 1) {
+                    F3(S, S->uhx, i, j, k) = 0.5 * (F3(S, u, i, j, k) + F3(S, u, i - 1, j, k));
+                    F3(S, S->uhy, i, j, k) = 0.5 * (F3(S, u, i, j + 1, k) + F3(S, u, i, j, k));
+                    F3(S, S->vhx, i, j, k) = 0.5 * (F3(S, v, i + 1, j, k) + F3(S, v, i, j, k));
+                    F3(S, S->vhy, i, j, k) = 0.5 * (F3(S, v, i, j, k) + F3(S, v, i, j - 1, k));
+                    F3(S, S->whx, i, j, k) = 0.5 * (F3(S, w, i + 1, j, k) + F3(S, w, i, j, k));
+                    F3(S, S->why, i, j, k) = 0.5 * (F3(S, w, i, j + 1, k) + F3(S, w, i, j, k));
+                } else if (P->cds == 2) {
+                    F3(S, S->uhx, i, j, k) = (-F3(S, u, i + 1, j, k) + 9.0 * F3(S, u, i, j, k) +
+                                              9.0 * F3(S, u, i - 1, j, k) - F3(S, u, i - 2, j, k)) / 16.0;
+                    F3(S, S->uhy, i, j, k) = (-F3(S, u, i, j + 2, k) + 9.0 * F3(S, u, i, j + 1, k) +
+                                              9.0 * F3(S, u, i, j, k) - F3(S, u, i, j - 1, k)) / 16.0;
+                    F3(S, S->vhx, i, j, k) = (-F3(S, v, i + 2, j, k) + 9.0 * F3(S, v, i + 1, j, k) +
+                                              9.0 * F3(S, v, i, j, k) - F3(S, v, i - 1, j, k)) / 16.0;
+                    F3(S, S->vhy, i, j, k) = (-F3(S, v, i, j + 1, k) + 9.0 * F3(S, v, i, j, k) +
+                                              9.0 * F3(S, v, i, j - 1, k) - F3(S, v, i, j - 2, k)) / 16.0;
+                    F3(S, S->whx, i, j, k) = (-F3(S, w, i + 2, j, k) + 9.0 * F3(S, w, i + 1, j, k) +
+                                              9.0 * F3(S, w, i, j, k) - F3(S, w, i - 1, j, k)) / 16.0;
+                    F3(S, S->why, i, j, k) = (-F3(S, w, i, j + 2, k) + 9.0 * F3(S, w, i, j + 1, k) +
+                                              9.0 * F3(S, w, i, j, k) - F3(S, w, i, j - 1, k)) / 16.0;
+                }
+                double lt = V1(S->l_t, k);
+                F3(S, S->uhz, i, j, k) = F3(S, u, i, j, k + 1) * lt + F3(S, u, i, j, k) * (1.0 - lt);
+                F3(S, S->vhz, i, j, k) = F3(S, v, i, j, k + 1) * lt + F3(S, v, i, j, k) * (1.0 - lt);
+            }
+}
+
+static void interp_t(ho_state *S)
+{
+    const ho_params *P = &S->prm;
+    double *t = S->t;
+#pragma omp parallel for
+    for (int k = S->kstr3; k <= S->kend3; ++k)
+        for (int j = S->jstr3; j <= S->jend3; ++j)
+            for (int i = S->istr3; i <= S->iend3; ++i) {
+                if (P->cds == 1) {
+                    F3(S, S->thx, i, j, k) = 0.5 * (F3(S, t, i + 1, j, k) + F3(S, t, i, j, k));
+                    F3(S, S->thy, i, j, k) = 0.5 * (F3(S, t, i, j + 1, k) + F3(S, t, i, j, k));
+                } else if (P->cds == 2) {
+                    F3(S, S->thx, i, j, k) = (-F3(S, t, i + 2, j, k) + 9.0 * F3(S, t, i + 1, j, k) +
+                                              9.0 * F3(S, t, i, j, k) - F3(S, t, i - 1, j, k)) / 16.0;
+                    F3(S, S->thy, i, j, k) = (-F3(S, t, i, j + 2, k) + 9.0 * F3(S, t, i, j + 1, k) +
+                                              9.0 * F3(S, t, i, j, k) - F3(S, t, i, j - 1, k)) / 16.0;
+                }
+                double lt = V1(S->l_t, k);
+                F3(S, S->thz, i, j, k) = F3(S, t, i, j, k + 1) * lt + F3(S, t, i, j, k) * (1.0 - lt);
+            }
+}
+
+/* ---- A3: update_dpx_drive / calculate_ubar, src/module_parameters.f90:594-613,513-546 */
+static void update_dpx(ho_state *S)
+{
+    if (S->prm.dp_opt == 1) S->dpx_drive = 1.0;
+    else if (S->prm.dp_opt == 2) S->dpx_drive = S->dpx_drive - (S->ubar + S->prm.u_mrf - 1.0) * S->dpx_coef;
+}
+
+void ho_calc_ubar(ho_state *S)
+{
+    double s = 0.0;
+    for (int k = S->kstr3; k <= S->kend3; ++k)
+        for (int j = S->jstr3; j <= S->jend3; ++j)
+            for (int i = S->istr3; i <= S->iend3; ++i) s = s + F3(S, S->u, i, j, k) * V1(S->dz, k);
+    s = s / S->prm.lz / S->prm.nx / S->prm.ny;
+    S->ubar = s; /* one rank: ALLREDUCE sum / nprc is the identity */
+}
+
+
+/* ---- A6: get_momentum_cnvdiff, src/module_cnvdiff.f90:35-374 ------------ */
+#define SQ(x) ((x) * (x))
+static void momentum(ho_state *S)
+{
+    const ho_params *P = &S->prm;
+    double *u = S->u, *v = S->v, *w = S->w, *pp = S->p, *t = S->t;
+    double *uhx = S->uhx, *uhy = S->uhy, *uhz = S->uhz, *vhx = S->vhx, *vhy = S->vhy;
+    double *vhz = S->vhz, *whx = S->whx, *why = S->why;
+    double dx = S->dx, dy = S->dy, dx2 = S->dx2, dy2 = S->dy2, dt = P->dt, nu = P->nu;
+    double ra = S->rk_a, rb = S->rk_b, rc = S->rk_c, um = P->u_mrf;
+    update_dpx(S);                                            /* :51 */
+    double dpx = S->dpx_drive;
+    /* U equation :55-161 */
+#pragma omp parallel for
+    for (int k = S->kstr3; k <= S->kend3; ++k)
+        for (int j = S->jstr3; j <= S->jend3; ++j)
+            for (int i = S->istr3; i <= S->iend3; ++i) {
+                double uux = 0, uvy = 0, ux = 0, uxx = 0, uyy = 0, dpdx = 0;
+                if (P->cds == 1) {
+                    uux = (SQ(F3(S, uhx, i + 1, j, k)) - SQ(F3(S, uhx, i, j, k))) / dx;
+                    uvy = (F3(S, uhy, i, j, k) * F3(S, vhx, i, j, k) - F3(S, uhy, i, j - 1, k) * F3(S, vhx, i, j - 1, k)) / dy;
+                    ux = (F3(S, uhx, i + 1, j, k) - F3(S, uhx, i, j, k)) / dx;
+                    uxx = (F3(S, u, i + 1, j, k) - 2.0 * F3(S, u, i, j, k) + F3(S, u, i - 1, j, k)) / dx2;
+                    uyy = (F3(S, u, i, j + 1, k) - 2.0 * F3(S, u, i, j, k) + F3(S, u, i, j - 1, k)) / dy2;
+                    dpdx = (F3(S, pp, i + 1, j, k) - F3(S, pp, i, j, k)) / dx;
+                } else if (P->cds == 2) {
+                    uux = (-SQ(F3(S, uhx, i + 2, j, k)) + 27.0 * SQ(F3(S, uhx, i + 1, j, k)) -
+                           27.0 * SQ(F3(S, uhx, i, j, k)) + SQ(F3(S, uhx, i - 1, j, k))) / 24.0 / dx;
+                    uvy = (-F3(S, uhy, i, j + 1, k) * F3(S, vhx, i, j + 1, k) + 27.0 * F3(S, uhy, i, j, k) * F3(S, vhx, i, j, k) -
+                           27.0 * F3(S, uhy, i, j - 1, k) * F3(S, vhx, i, j - 1, k) + F3(S, uhy, i, j - 2, k) * F3(S, vhx, i, j - 2, k)) / 24.0 / dy;
+                    ux = (-F3(S, uhx, i + 2, j, k) + 27.0 * F3(S, uhx, i + 1, j, k) - 27.0 * F3(S, uhx, i, j, k) +
+                          F3(S, uhx, i - 1, j, k)) / 24.0 / dx;
+                    uxx = (-F3(S, u, i + 2, j, k) + 16.0 * F3(S, u, i + 1, j, k) - 30.0 * F3(S, u, i, j, k) +
+                           16.0 * F3(S, u, i - 1, j, k) - F3(S, u, i - 2, j, k)) / 12.0 / dx2;
+                    uyy = (-F3(S, u, i, j + 2, k) + 16.0 * F3(S, u, i, j + 1, k) - 30.0 * F3(S, u, i, j, k) +
+                           16.0 * F3(S, u, i, j - 1, k) - F3(S, u, i, j - 2, k)) / 12.0 / dy2;
+                    dpdx = (-F3(S, pp, i + 2, j, k) + 27.0 * F3(S, pp, i + 1, j, k) - 27.0 * F3(S, pp, i, j, k) +
+                            F3(S, pp, i - 1, j, k)) / 24.0 / dx;
+                }
+                double uwz = (F3(S, uhz, i, j, k) * F3(S, whx, i, j, k) - F3(S, uhz, i, j, k - 1) * F3(S, whx, i, j, k - 1)) / V1(S->dz, k);
+                double uzz = ((F3(S, u, i, j, k + 1) - F3(S, u, i, j, k)) / V1(S->dz_t, k) -
+                              (F3(S, u, i, j, k) - F3(S, u, i, j, k - 1)) / V1(S->dz_b, k)) / V1(S->dz, k);
+                double f;
+                if (P->dp_opt == 3)
+                    f = -uux - uvy - uwz - um * ux + P->fc * (0.5 * (F3(S, vhx, i, j, k) + F3(S, vhx, i, j - 1, k)) - P->vg);
+                else
+                    f = -uux - uvy - uwz + dpx - um * ux;
+                F3(S, S->ff, i, j, k) = f;
+                F3(S, S->qu, i, j, k) = F3(S, u, i, j, k) + dt * (ra * f + rb * F3(S, S->ff0, i, j, k)) +
+                                        rc * 0.5 * dt * nu * (uxx + uyy + uzz) - rc * dt * dpdx;
+            }
+    /* V equation :166-270 */
+#pragma omp parallel for
+    for (int k = S->kstr3; k <= S->kend3; ++k)
+        for (int j = S->jstr3; j <= S->jend3; ++j)
+            for (int i = S->istr3; i <= S->iend3; ++i) {
+                double vux = 0, vvy = 0, vx = 0, vxx = 0, vyy = 0, dpdy = 0;
+                if (P->cds == 1) {
+                    vux = (F3(S, vhx, i, j, k) * F3(S, uhy, i, j, k) - F3(S, vhx, i - 1, j, k) * F3(S, uhy, i - 1, j, k)) / dx;
+                    vvy = (SQ(F3(S, vhy, i, j + 1, k)) - SQ(F3(S, vhy, i, j, k))) / dy;
+                    vx = (F3(S, vhx, i, j, k) - F3(S, vhx, i - 1, j, k)) / dx;
+                    vxx = (F3(S, v, i + 1, j, k) - 2.0 * F3(S, v, i, j, k) + F3(S, v, i - 1, j, k)) / dx2;
+                    vyy = (F3(S, v, i, j + 1, k) - 2.0 * F3(S, v, i, j, k) + F3(S, v, i, j - 1, k)) / dy2;
+                    dpdy = (F3(S, pp, i, j + 1, k) - F3(S, pp, i, j, k)) / dy;
+                } else if (P->cds == 2) {
+                    vux = (-F3(S, vhx, i + 1, j, k) * F3(S, uhy, i + 1, j, k) + 27.0 * F3(S, vhx, i, j, k) * F3(S, uhy, i, j, k) -
+                           27.0 * F3(S, vhx, i - 1, j, k) * F3(S, uhy, i - 1, j, k) + F3(S, vhx, i - 2, j, k) * F3(S, uhy, i - 2, j, k)) / 24.0 / dx;
+                    vvy = (-SQ(F3(S, vhy, i, j + 2, k)) + 27.0 * SQ(F3(S, vhy, i, j + 1, k)) -
+                           27.0 * SQ(F3(S, vhy, i, j, k)) + SQ(F3(S, vhy, i, j - 1, k))) / 24.0 / dy;
+                    vx = (-F3(S, vhx, i + 1, j, k) + 27.0 * F3(S, vhx, i, j, k) - 27.0 * F3(S, vhx, i - 1, j, k) +
+                          F3(S, vhx, i - 2, j, k)) / 24.0 / dx;
+                    vxx = (-F3(S, v, i + 2, j, k) + 16.0 * F3(S, v, i + 1, j, k) - 30.0 * F3(S, v, i, j, k) +
+                           16.0 * F3(S, v, i - 1, j, k) - F3(S, v, i - 2, j, k)) / 12.0 / dx2;
+                    vyy = (-F3(S, v, i, j + 2, k) + 16.0 * F3(S, v, i, j + 1, k) - 30.0 * F3(S, v, i, j, k) +
+                           16.0 * F3(S, v, i, j - 1, k) - F3(S, v, i, j - 2, k)) / 12.0 / dy2;
+                    dpdy = (-F3(S, pp, i, j + 2, k) + 27.0 * F3(S, pp, i, j + 1, k) - 27.0 * F3(S, pp, i, j, k) +
+                            F3(S, pp, i, j - 1, k)) / 24.0 / dy;
+                }
+                double vwz = (F3(S, vhz, i, j, k) * F3(S, why, i, j, k) - F3(S, vhz, i, j, k - 1) * F3(S, why, i, j, k - 1)) / V1(S->dz, k);
+                double vzz = ((F3(S, v, i, j, k + 1) - F3(S, v, i, j, k)) / V1(S->dz_t, k) -
+                              (F3(S, v, i, j, k) - F3(S, v, i, j, k - 1)) / V1(S->dz_b, k)) / V1(S->dz, k);
+                double g;
+                if (P->dp_opt == 3)
+                    g = -vux - vvy - vwz - um * vx + P->fc * (P->ug - 0.5 * (F3(S, uhy, i, j, k) + F3(S, uhy, i - 1, j, k)) - um);
+                else
+                    g = -vux - vvy - vwz - um * vx;
+                F3(S, S->gg, i, j, k) = g;
+                F3(S, S->qv, i, j, k) = F3(S, v, i, j, k) + dt * (ra * g + rb * F3(S, S->gg0, i, j, k)) +
+                                        rc * 0.5 * dt * nu * (vxx + vyy + vzz) - rc * dt * dpdy;
+            }
+    /* W equation :274-370 (k < kend3) */
+#pragma omp parallel for
+    for (int k = S->kstr3; k <= S->kend3 - 1; ++k)
+        for (int j = S->jstr3; j <= S->jend3; ++j)
+            for (int i = S->istr3; i <= S->iend3; ++i) {
+                double wux = 0, wvy = 0, wx = 0, wxx = 0, wyy = 0;
+                if (P->cds == 1) {
+                    wux = (F3(S, whx, i, j, k) * F3(S, uhz, i, j, k) - F3(S, whx, i - 1, j, k) * F3(S, uhz, i - 1, j, k)) / dx;
+                    wvy = (F3(S, why, i, j, k) * F3(S, vhz, i, j, k) - F3(S, why, i, j - 1, k) * F3(S, vhz, i, j - 1, k)) / dy;
+                    wx = (F3(S, whx, i, j, k) - F3(S, whx, i - 1, j, k)) / dx;
+                    wxx = (F3(S, w, i + 1, j, k) - 2.0 * F3(S, w, i, j, k) + F3(S, w, i - 1, j, k)) / dx2;
+                    wyy = (F3(S, w, i, j + 1, k) - 2.0 * F3(S, w, i, j, k) + F3(S, w, i, j - 1, k)) / dy2;
+                } else if (P->cds == 2) {
+                    wux = (-F3(S, whx, i + 1, j, k) * F3(S, uhz, i + 1, j, k) + 27.0 * F3(S, whx, i, j, k) * F3(S, uhz, i, j, k) -
+                           27.0 * F3(S, whx, i - 1, j, k) * F3(S, uhz, i - 1, j, k) + F3(S, whx, i - 2, j, k) * F3(S, uhz, i - 2, j, k)) / 24.0 / dx;
+                    wvy = (-F3(S, why, i, j + 1, k) * F3(S, vhz, i, j + 1, k) + 27.0 * F3(S, why, i, j, k) * F3(S, vhz, i, j, k) -
+                           27.0 * F3(S, why, i, j - 1, k) * F3(S, vhz, i, j - 1, k) + F3(S, why, i, j - 2, k) * F3(S, vhz, i, j - 2, k)) / 24.0 / dy;
+                    wx = (-F3(S, whx, i + 1, j, k) + 27.0 * F3(S, whx, i, j, k) - 27.0 * F3(S, whx, i - 1, j, k) +
+                          F3(S, whx, i - 2, j, k)) / 24.0 / dx;
+                    wxx = (-F3(S, w, i + 2, j, k) + 16.0 * F3(S, w, i + 1, j, k) - 30.0 * F3(S, w, i, j, k) +
+                           16.0 * F3(S, w, i - 1, j, k) - F3(S, w, i - 2, j, k)) / 12.0 / dx2;
+                    wyy = (-F3(S, w, i, j + 2, k) + 16.0 * F3(S, w, i, j + 1, k) - 30.0 * F3(S, w, i, j, k) +
+                           16.0 * F3(S, w, i, j - 1, k) - F3(S, w, i, j - 2, k)) / 12.0 / dy2;
+                }
+                double a1 = SQ(0.5 * (F3(S, w, i, j, k) + F3(S, w, i, j, k + 1)));
+                double a2 = SQ(0.5 * (F3(S, w, i, j, k) + F3(S, w, i, j, k - 1)));
+                double wwz = (a1 - a2) / V1(S->dz_t, k);
+                double wzz = ((F3(S, w, i, j, k + 1) - F3(S, w, i, j, k)) / V1(S->dz, k + 1) -
+                              (F3(S, w, i, j, k) - F3(S, w, i, j, k - 1)) / V1(S->dz, k)) / V1(S->dz_t, k);
+                double h;
+                if (P->isscalar == 1)
+                    h = -wux - wvy - wwz - um * wx +
+                        S->got * ((1.0 - V1(S->l_t, k)) * F3(S, t, i, j, k) + V1(S->l_t, k) * F3(S, t, i, j, k + 1) - P->t_ref);
+                else
+                    h = -wux - wvy - wwz - um * wx;
+                double dpdz = (F3(S, pp, i, j, k + 1) - F3(S, pp, i, j, k)) / V1(S->dz_t, k);
+                F3(S, S->hh, i, j, k) = h;
+                F3(S, S->qw, i, j, k) = F3(S, w, i, j, k) + dt * (ra * h + rb * F3(S, S->hh0, i, j, k)) +
+                                        rc * 0.5 * dt * nu * (wxx + wyy + wzz) - rc * dt * dpdz;
+            }
+}
+
+/* ---- A7: get_temperature_cnvdiff, src/module_cnvdiff.f90:381-473 ------- */
+static void temperature(ho_state *S, double *qt)
+{
+    const ho_params *P = &S->prm;
+    double *u = S->u, *v = S->v, *w = S->w, *t = S->t, *thx = S->thx, *thy = S->thy, *thz = S->thz;
+    double dx = S->dx, dy = S->dy, dx2 = S->dx2, dy2 = S->dy2, dt = P->dt;
+#pragma omp parallel for
+    for (int k = S->kstr3; k <= S->kend3; ++k)
+        for (int j = S->jstr3; j <= S->jend3; ++j)
+            for (int i = S->istr3; i <= S->iend3; ++i) {
+                double tux = 0, tvy = 0, tx = 0, txx = 0, tyy = 0;
+                if (P->cds == 1) {
+                    tux = (F3(S, thx, i, j, k) * F3(S, u, i, j, k) - F3(S, thx, i - 1, j, k) * F3(S, u, i - 1, j, k)) / dx;
+                    tvy = (F3(S, thy, i, j, k) * F3(S, v, i, j, k) - F3(S, thy, i, j - 1, k) * F3(S, v, i, j - 1, k)) / dy;
+                    tx = (F3(S, thx, i, j, k) - F3(S, thx, i - 1, j, k)) / dx;
+                    txx = (F3(S, t, i + 1, j, k) - 2.0 * F3(S, t, i, j, k) + F3(S, t, i - 1, j, k)) / dx2;
+                    tyy = (F3(S, t, i, j + 1, k) - 2.0 * F3(S, t, i, j, k) + F3(S, t, i, j - 1, k)) / dy2;
+                } else if (P->cds == 2) {
+                    tux = (-F3(S, thx, i + 1, j, k) * F3(S, u, i + 1, j, k) + 27.0 * F3(S, thx, i, j, k) * F3(S, u, i, j, k) -
+                           27.0 * F3(S, thx, i - 1, j, k) * F3(S, u, i - 1, j, k) + F3(S, thx, i - 2, j, k) * F3(S, u, i - 2, j, k)) / 24.0 / dx;
+                    tvy = (-F3(S, thy, i, j + 1, k) * F3(S, v, i, j + 1, k) + 27.0 * F3(S, thy, i, j, k) * F3(S, v, i, j, k) -
+                           27.0 * F3(S, thy, i, j - 1, k) * F3(S, v, i, j - 1, k) + F3(S, thy, i, j - 2, k) * F3(S, v, i, j - 2, k)) / 24.0 / dy;
+                    tx = (-F3(S, thx, i + 1, j, k) + 27.0 * F3(S, thx, i, j, k) - 27.0 * F3(S, thx, i - 1, j, k) +
+                          F3(S, thx, i - 2, j, k)) / 24.0 / dx;
+                    txx = (-F3(S, t, i + 2, j, k) + 16.0 * F3(S, t, i + 1, j, k) - 30.0 * F3(S, t, i, j, k) +
+                           16.0 * F3(S, t, i - 1, j, k) - F3(S, t, i - 2, j, k)) / 12.0 / dx2;
+                    tyy = (-F3(S, t, i, j + 2, k) + 16.0 * F3(S, t, i, j + 1, k) - 30.0 * F3(S, t, i, j, k) +
+                           16.0 * F3(S, t, i, j - 1, k) - F3(S, t, i, j - 2, k)) / 12.0 / dy2;
+                }
+                double twz = (F3(S, thz, i, j, k) * F3(S, w, i, j, k) - F3(S, thz, i, j, k - 1) * F3(S, w, i, j, k - 1)) / V1(S->dz, k);
+                double tzz = ((F3(S, t, i, j, k + 1) - F3(S, t, i, j, k)) / V1(S->dz_t, k) -
+                              (F3(S, t, i, j, k) - F3(S, t, i, j, k - 1)) / V1(S->dz_b, k)) / V1(S->dz, k);
+                double s = -tux - tvy - twz - P->u_mrf * tx;
+                F3(S, S->ss, i, j, k) = s;
+                F3(S, qt, i, j, k) = F3(S, t, i, j, k) + dt * (S->rk_a * s + S->rk_b * F3(S, S->ss0, i, j, k)) +
+                                     S->rk_c * 0.5 * dt * P->alpha * (txx + tyy + tzz);
+            }
+}
+
+/* ---- A11: mpi_2d_fft on one rank, src/module_tools.f90:518-700 --------- */
+static void zero_nyquist(ho_state *S, cplx *c)
+{
+    const ho_params *P = &S->prm;
+    for (int k = 1; k <= P->nz; ++k)
+        for (int j = 1; j <= P->ny; ++j)
+            for (int i = 1; i <= P->nx; ++i) {
+                int l = lwave(i, P->nx), m = lwave(j, P->ny);
+                if (l == -P->nx / 2) C3(S, c, i, j, k) = 0;
+                if (m == -P->ny / 2) C3(S, c, i, j, k) = 0;
+            }
+}
+
+void ho_fft2d(ho_state *S, cplx *c, int direction)
+{
+    const ho_params *P = &S->prm;
+    if (direction == 1) zero_nyquist(S, c);
+    int sign = direction == -1 ? -1 : +1;  /* FFTW_FORWARD = -1, FFTW_BACKWARD = +1 */
+    ho_plan py = plan_make(P->ny, sign), px = plan_make(P->nx, sign);
+    /* y pencil: FFT along j for every (i,k); forward divides by ny (:581-594) */
+#pragma omp parallel
+    {
+        int nmax = P->nx > P->ny ? P->nx : P->ny;
+        cplx *a = calloc_c(nmax), *b = calloc_c(nmax);
+#pragma omp for
+        for (int k = 1; k <= P->nz; ++k)
+            for (int i = 1; i <= P->nx; ++i) {
+                plan_exec(&py, &C3(S, c, i, 1, k), (size_t)P->nx, a, b);
+                if (direction == -1)
+                    for (int j = 1; j <= P->ny; ++j) C3(S, c, i, j, k) = C3(S, c, i, j, k) / (double)P->ny;
+            }
+        /* x pencil: FFT along i for every (j,k); forward divides by nx (:620-633) */
+#pragma omp for
+        for (int k = 1; k <= P->nz; ++k)
+            for (int j = 1; j <= P->ny; ++j) {
+                plan_exec(&px, &C3(S, c, 1, j, k), 1, a, b);
+                if (direction == -1)
+                    for (int i = 1; i <= P->nx; ++i) C3(S, c, i, j, k) = C3(S, c, i, j, k) / (double)P->nx;
+            }
+        free(a); free(b);
+    }
+    free(py.tw); free(px.tw);
+    if (direction == -1) zero_nyquist(S, c);
+}
+
+/* ---- A9: assign_abc / assign_abc_p, src/module_navier_stokes.f90:523-619 */
+static void assign_abc(ho_state *S, const double *aa_in, const cplx *bb_in, const double *cc_in, int is_p)
+{
+    const ho_params *P = &S->prm;
+    size_t plane = (size_t)P->nx * P->ny;
+    for (int k = 1; k <= S->nka; ++k)
+        for (size_t q = 0; q < plane; ++q) {
+            size_t o = (size_t)(k - 1) * plane + q;
+            if (P->dts == 2) {
+                S->aa[o] = S->rk_c * V1(aa_in, k);
+                S->cc[o] = S->rk_c * V1(cc_in, k);
+                S->bb[o] = is_p ? S->rk_c * bb_in[o] : 1.0 + S->rk_c * bb_in[o];
+            } else {
+                S->aa[o] = V1(aa_in, k);
+                S->cc[o] = V1(cc_in, k);
+                S->bb[o] = is_p ? bb_in[o] : 1.0 + bb_in[o];
+            }
+        }
+}
+
+/* ---- A10: poisson_solver_fft, src/module_poisson_solver.f90:483-658 ---- */
+static int poisson_solver_fft(ho_state *S, const double *r_src, double *r_var,
+                              int zstagger, int delnull, int tbnd, int ubnd)
+{
+    const ho_params *P = &S->prm;
+    int nx = P->nx, ny = P->ny, nz = P->nz, ks = S->kstr3, ke = S->kend3;
+    memset(r_var, 0, sizeof(double) * nfield(S));
+    memset(S->c_var, 0, sizeof(cplx) * (size_t)nx * ny * nz);
+    for (int k = 1; k <= nz; ++k)
+        for (int j = 1; j <= ny; ++j)
+            for (int i = 1; i <= nx; ++i) C3(S, S->c_src, i, j, k) = F3(S, r_src, i + 2, j + 2, k + 2);
+    ho_fft2d(S, S->c_src, -1);
+    int kmax3;
+    if (zstagger == 1) kmax3 = ke - 1;
+    else if (zstagger == 0) kmax3 = ke;
+    else { snprintf(S->err, sizeof S->err, "Invalid zstagger flag!"); return 1; }
+    if (delnull == 1) {
+        B3(S, S->aa, 1, 1, ks) = 0.0;
+        B3(S, S->cc, 1, 1, ks) = 0.0;
+        C3(S, S->c_src, 1, 1, 1) = 0;
+    } else if (delnull != 0) { snprintf(S->err, sizeof S->err, "delnull flag error!"); return 1; }
+    if (tbnd == 1) {
+        double aa_t_zstr = -0.5 * S->rk_c * P->dt * P->alpha / V1(S->dz, ks) / V1(S->dz_b, ks);
+        C3(S, S->c_src, 1, 1, 1) = C3(S, S->c_src, 1, 1, 1) - 2.0 * aa_t_zstr * P->tbot;
+        double cc_t_zend = -0.5 * S->rk_c * P->dt * P->alpha / V1(S->dz, ke) / V1(S->dz_t, ke);
+        C3(S, S->c_src, 1, 1, nz) = C3(S, S->c_src, 1, 1, nz) - 2.0 * cc_t_zend * P->ttop;
+    } else if (tbnd != 0) { snprintf(S->err, sizeof S->err, "tbnd flag error!"); return 1; }
+    if (ubnd == 1) {
+        double aa_u_zstr = -0.5 * S->rk_c * P->dt * P->nu / V1(S->dz, ks) / V1(S->dz_b, ks);
+        C3(S, S->c_src, 1, 1, 1) = C3(S, S->c_src, 1, 1, 1) + 2.0 * aa_u_zstr * P->u_mrf;
+        if (P->ochannel == 0) {
+            double cc_u_zend = -0.5 * S->rk_c * P->dt * P->nu / V1(S->dz, ke) / V1(S->dz_t, ke);
+            C3(S, S->c_src, 1, 1, nz) = C3(S, S->c_src, 1, 1, nz) + 2.0 * cc_u_zend * P->u_mrf;
+        }
+    } else if (ubnd != 0) { snprintf(S->err, sizeof S->err, "ubnd flag error!"); return 1; }
+    /* Thomas algorithm :611-640 */
+#pragma omp parallel for
+    for (int j = 1; j <= ny; ++j)
+        for (int i = 1; i <= nx; ++i) {
+            for (int k = ks + 1; k <= kmax3; ++k) {
+                cplx tmp = B3(S, S->aa, i, j, k) / B3(S, S->bb, i, j, k - 1);
+                B3(S, S->bb, i, j, k) = B3(S, S->bb, i, j, k) - B3(S, S->cc, i, j, k - 1) * tmp;
+                C3(S, S->c_src, i, j, k - 2) = C3(S, S->c_src, i, j, k - 2) - C3(S, S->c_src, i, j, k - 3) * tmp;
+            }
+            C3(S, S->c_var, i, j, kmax3 - 2) = C3(S, S->c_src, i, j, kmax3 - 2) / B3(S, S->bb, i, j, kmax3);
+            for (int k = kmax3 - 1; k >= ks; --k)
+                C3(S, S->c_var, i, j, k - 2) = (C3(S, S->c_src, i, j, k - 2) - B3(S, S->cc, i, j, k) * C3(S, S->c_var, i, j, k - 1)) /
+                                               B3(S, S->bb, i, j, k);
+        }
+    ho_fft2d(S, S->c_var, 1);
+    for (int k = ks; k <= kmax3; ++k)
+        for (int j = 1; j <= ny; ++j)
+            for (int i = 1; i <= nx; ++i) F3(S, r_var, i + 2, j + 2, k) = creal(C3(S, S->c_var, i, j, k - 2));
+    return 0;
+}
+
+/* ---- A12: get_p_eqn_src, src/module_poisson_solver.f90:127-176 -------- */
+static void p_src(ho_state *S, double *qp)
+{
+    const ho_params *P = &S->prm;
+    double *u = S->u, *v = S->v, *w = S->w;
+#pragma omp parallel for
+    for (int k = S->kstr3; k <= S->kend3; ++k)
+        for (int j = S->jstr3; j <= S->jend3; ++j)
+            for (int i = S->istr3; i <= S->iend3; ++i) {
+                double dudx = 0, dvdy = 0;
+                if (P->cds == 1) {
+                    dudx = (F3(S, u, i, j, k) - F3(S, u, i - 1, j, k)) / S->dx;
+                    dvdy = (F3(S, v, i, j, k) - F3(S, v, i, j - 1, k)) / S->dy;
+                } else if (P->cds == 2) {
+                    dudx = (-F3(S, u, i + 1, j, k) + 27.0 * F3(S, u, i, j, k) - 27.0 * F3(S, u, i - 1, j, k) + F3(S, u, i - 2, j, k)) / 24.0 / S->dx;
+                    dvdy = (-F3(S, v, i, j + 1, k) + 27.0 * F3(S, v, i, j, k) - 27.0 * F3(S, v, i, j - 1, k) + F3(S, v, i, j - 2, k)) / 24.0 / S->dy;
+                }
+                double dwdz = (F3(S, w, i, j, k) - F3(S, w, i, j, k - 1)) / V1(S->dz, k);
+                F3(S, qp, i, j, k) = (dudx + dvdy + dwdz) / P->dt;
+            }
+}
+
+/* ---- A13: calculate_uvw, src/module_navier_stokes.f90:659-734 ---------- */
+static void calc_uvw(ho_state *S)
+{
+    const ho_params *P = &S->prm;
+    double *phi = S->phi, c = S->rk_c * P->dt;
+#pragma omp parallel for
+    for (int k = S->kstr3; k <= S->kend3; ++k)
+        for (int j = S->jstr3; j <= S->jend3; ++j)
+            for (int i = S->istr3; i <= S->iend3; ++i) {
+                if (P->cds == 1) {
+                    F3(S, S->u, i, j, k) = F3(S, S->u, i, j, k) - c * (F3(S, phi, i + 1, j, k) - F3(S, phi, i, j, k)) / S->dx;
+                    F3(S, S->v, i, j, k) = F3(S, S->v, i, j, k) - c * (F3(S, phi, i, j + 1, k) - F3(S, phi, i, j, k)) / S->dy;
+                } else if (P->cds == 2) {
+                    F3(S, S->u, i, j, k) = F3(S, S->u, i, j, k) - c * (-F3(S, phi, i + 2, j, k) + 27.0 * F3(S, phi, i + 1, j, k) -
+                                                                      27.0 * F3(S, phi, i, j, k) + F3(S, phi, i - 1, j, k)) / 24.0 / S->dx;
+                    F3(S, S->v, i, j, k) = F3(S, S->v, i, j, k) - c * (-F3(S, phi, i, j + 2, k) + 27.0 * F3(S, phi, i, j + 1, k) -
+                                                                      27.0 * F3(S, phi, i, j, k) + F3(S, phi, i, j - 1, k)) / 24.0 / S->dy;
+                }
+            }
+#pragma omp parallel for
+    for (int k = S->kstr3; k <= S->kend3 - 1; ++k)
+        for (int j = S->jstr3; j <= S->jend3; ++j)
+            for (int i = S->istr3; i <= S->iend3; ++i)
+                F3(S, S->w, i, j, k) = F3(S, S->w, i, j, k) - c * (F3(S, phi, i, j, k + 1) - F3(S, phi, i, j, k)) / V1(S->dz_t, k);
+}
+
+/* ---- A14: calculate_new_p, src/module_navier_stokes.f90:739-808 -------- */
+static void calc_new_p(ho_state *S)
+{
+    const ho_params *P = &S->prm;
+    double *phi = S->phi;
+#pragma omp parallel for
+    for (int k = S->kstr3; k <= S->kend3; ++k)
+        for (int j = S->jstr3; j <= S->jend3; ++j)
+            for (int i = S->istr3; i <= S->iend3; ++i) {
+                double pxx = 0, pyy = 0;
+                if (P->cds == 1) {
+                    pxx = (F3(S, phi, i + 1, j, k) - 2.0 * F3(S, phi, i, j, k) + F3(S, phi, i - 1, j, k)) / S->dx2;
+                    pyy = (F3(S, phi, i, j + 1, k) - 2.0 * F3(S, phi, i, j, k) + F3(S, phi, i, j - 1, k)) / S->dy2;
+                } else if (P->cds == 2) {
+                    pxx = (-F3(S, phi, i + 2, j, k) + 16.0 * F3(S, phi, i + 1, j, k) - 30.0 * F3(S, phi, i, j, k) +
+                           16.0 * F3(S, phi, i - 1, j, k) - F3(S, phi, i - 2, j, k)) / 12.0 / S->dx2;
+                    pyy = (-F3(S, phi, i, j + 2, k) + 16.0 * F3(S, phi, i, j + 1, k) - 30.0 * F3(S, phi, i, j, k) +
+                           16.0 * F3(S, phi, i, j - 1, k) - F3(S, phi, i, j - 2, k)) / 12.0 / S->dy2;
+                }
+                double pzz = ((F3(S, phi, i, j, k + 1) - F3(S, phi, i, j, k)) / V1(S->dz_t, k) -
+                              (F3(S, phi, i, j, k) - F3(S, phi, i, j, k - 1)) / V1(S->dz_b, k)) / V1(S->dz, k);
+                F3(S, S->p, i, j, k) = F3(S, S->p, i, j, k) + F3(S, phi, i, j, k) - S->rk_c * 0.5 * P->dt * P->nu * (pxx + pyy + pzz);
+            }
+}
+
+/* ---- A17: screen_div_error (values only), src/module_io.f90:85-206 ----- */
+static int div_error(ho_state *S, int isfinal)
+{
+    const ho_params *P = &S->prm;
+    double *u = S->u, *v = S->v, *w = S->w, avg = 0.0, mx = 0.0;
+    for (int k = S->kstr3; k <= S->kend3; ++k)
+        for (int j = S->jstr3; j <= S->jend3; ++j)
+            for (int i = S->istr3; i <= S->iend3; ++i) {
+                double dudx = 0, dvdy = 0;
+                if (P->cds == 1) {
+                    dudx = (F3(S, u, i, j, k) - F3(S, u, i - 1, j, k)) / S->dx;
+                    dvdy = (F3(S, v, i, j, k) - F3(S, v, i, j - 1, k)) / S->dy;
+                } else if (P->cds == 2) {
+                    dudx = (-F3(S, u, i + 1, j, k) + 27.0 * F3(S, u, i, j, k) - 27.0 * F3(S, u, i - 1, j, k) + F3(S, u, i - 2, j, k)) / 24.0 / S->dx;
+                    dvdy = (-F3(S, v, i, j + 1, k) + 27.0 * F3(S, v, i, j, k) - 27.0 * F3(S, v, i, j - 1, k) + F3(S, v, i, j - 2, k)) / 24.0 / S->dy;
+                }
+                double d = dudx + dvdy + (F3(S, w, i, j, k) - F3(S, w, i, j, k - 1)) / V1(S->dz, k);
+                avg = avg + fabs(d);
+                if (fabs(d) > mx) mx = d;  /* signed, as stored by the reference (:131-133) */
+                if (F3(S, u, i, j, k) != F3(S, u, i, j, k) || F3(S, v, i, j, k) != F3(S, v, i, j, k) ||
+                    F3(S, w, i, j, k) != F3(S, w, i, j, k)) {
+                    snprintf(S->err, sizeof S->err, "NaN error! %d %d %d", i, j, k);
+                    return 1;
+                }
+                if (isfinal == 1 && fabs(d) > S->div_hist_max) S->div_hist_max = d;
+            }
+    S->div_avg = avg / P->nx / P->ny / P->nz;
+    S->div_max = mx;
+    return 0;
+}
+
+/* ---- assign_rk_coeff, src/module_parameters.f90:618-641 ---------------- */
+static void assign_rk(ho_state *S, int t_its)
+{
+    if (S->prm.dts == 2) { S->rk_a = RKC1[t_its - 1]; S->rk_b = RKC2[t_its - 1]; S->rk_c = RKC3[t_its - 1]; }
+    else { S->rk_a = 1.5; S->rk_b = -0.5; S->rk_c = 1.0; }
+}
+
+/* ---- A16: time_advancement_fd, src/module_navier_stokes.f90:223-365 ---- */
+static int time_advancement_fd(ho_state *S)
+{
+    const ho_params *P = &S->prm;
+    size_t nf = nfield(S);
+    int t_its_max = P->dts == 2 ? 3 : 1;
+    for (int t_its = 1; t_its <= t_its_max; ++t_its) {
+        if (P->isdamp == 1) { snprintf(S->err, sizeof S->err, "isdamp=1 is not restated"); return 1; }
+        memcpy(S->ff0, S->ff, nf * sizeof(double));               /* assign_fgh_old :270 */
+        memcpy(S->gg0, S->gg, nf * sizeof(double));
+        memcpy(S->hh0, S->hh, nf * sizeof(double));
+        interp_uvw(S);                                            /* :274 */
+        bc_uvwh(S);                                               /* :275 */
+        assign_rk(S, t_its);                                      /* :280 */
+        momentum(S);                                              /* :284 */
+        assign_abc(S, S->aa_u, S->bb_u, S->cc_u, 0);              /* :291-293 */
+        if (poisson_solver_fft(S, S->qu, S->u, 0, 0, 0, 1)) return 1;
+        assign_abc(S, S->aa_u, S->bb_u, S->cc_u, 0);              /* :297-299 */
+        if (poisson_solver_fft(S, S->qv, S->v, 0, 0, 0, 0)) return 1;
+        assign_abc(S, S->aa_w, S->bb_w, S->cc_w, 0);              /* :302-304 */
+        if (poisson_solver_fft(S, S->qw, S->w, 1, 0, 0, 0)) return 1;
+        bc_uvw(S);                                                /* :306 */
+        p_src(S, S->qu);                                          /* :310 */
+        assign_abc(S, S->aa_p, S->bb_p, S->cc_p, 1);              /* :312-314 */
+        if (poisson_solver_fft(S, S->qu, S->phi, 0, 1, 0, 0)) return 1;
+        bc_p(S, S->phi);                                          /* :316 */
+        calc_uvw(S);                                              /* :319 */
+        bc_uvw(S);                                                /* :320 */
+        if (P->isscalar == 1) {                                   /* :323-341 */
+            memcpy(S->ss0, S->ss, nf * sizeof(double));
+            interp_t(S);
+            bc_th(S);
+            temperature(S, S->qu);
+            assign_abc(S, S->aa_t, S->bb_t, S->cc_t, 0);
+            if (poisson_solver_fft(S, S->qu, S->t, 0, 0, 1, 0)) return 1;
+            bc_t(S);
+        }
+        if (div_error(S, 1)) return 1;                            /* :344-350 */
+        calc_new_p(S);                                            /* :353 */
+        bc_p(S, S->p);                                            /* :354 */
+    }
+    return 0;
+}
+
+/* ---- drive, src/module_navier_stokes.f90:31-218 (numerics only) -------- */
+/* start-up: mesh, coefficients, bulk velocity and boundary values (:53-129) */
+int ho_init(ho_state *S)
+{
+    ho_mesh(S);
+    if (S->prm.is_ri_var == 1) S->got = S->prm.ri_str;  /* update_ri(0) */
+    if (S->prm.dp_opt == 2) ho_calc_ubar(S);
+    ho_coefficients(S);
+    bc_uvw(S);
+    bc_p(S, S->p);
+    bc_t(S);
+    return 0;
+}
+
+/* nsteps passes of the step loop body (:139-165) */
+int ho_steps(ho_state *S, int nsteps)
+{
+    for (int n = 0; n < nsteps; ++n) {
+        S->stps += 1;
+        S->time_sim += S->prm.dt;
+        if (S->prm.is_ri_var == 1)
+            S->got = S->prm.ri_str + (S->prm.ri_end - S->prm.ri_str) * 1.0 / S->prm.imax * S->stps;
+        if (S->prm.cds == 3) { snprintf(S->err, sizeof S->err, "cds=3 is not restated"); return 1; }
+        if (time_advancement_fd(S)) return 1;
+        if (S->prm.dp_opt == 2) ho_calc_ubar(S);
+    }
+    return 0;
+}
+
+/* ---- accessors for the tests -------------------------------------------- */
+const char *ho_error(ho_state *S) { return S->err; }
+
+static double *field_ptr(ho_state *S, int id)
+{
+    double *f[] = {S->u, S->v, S->w, S->p, S->t, S->phi, S->ff, S->gg, S->hh, S->ss,
+                   S->qu, S->qv, S->qw, S->uhx, S->uhy, S->uhz, S->vhx, S->vhy, S->vhz, S->whx,
+                   S->why, S->thx, S->thy, S->thz};
+    if (id < 0 || id >= (int)(sizeof(f) / sizeof(f[0]))) return NULL;
+    return f[id];
+}
+
+/* interior (nz, ny, nx) C-order <-> ghosted Fortran array */
+int ho_set_field(ho_state *S, int id, const double *src)
+{
+    double *a = field_ptr(S, id);
+    if (!a) return 1;
+    const ho_params *P = &S->prm;
+    for (int k = 1; k <= P->nz; ++k)
+        for (int j = 1; j <= P->ny; ++j)
+            for (int i = 1; i <= P->nx; ++i)
+                F3(S, a, i + 2, j + 2, k + 2) = src[((size_t)(k - 1) * P->ny + (j - 1)) * P->nx + (i - 1)];
+    return 0;
+}
+
+int ho_get_field(ho_state *S, int id, double *dst)
+{
+    double *a = field_ptr(S, id);
+    if (!a) return 1;
+    const ho_params *P = &S->prm;
+    for (int k = 1; k <= P->nz; ++k)
+        for (int j = 1; j <= P->ny; ++j)
+            for (int i = 1; i <= P->nx; ++i)
+                dst[((size_t)(k - 1) * P->ny + (j - 1)) * P->nx + (i - 1)] = F3(S, a, i + 2, j + 2, k + 2);
+    return 0;
+}
+
+/* full ghosted array, Fortran order, (nx+4)*(ny+4)*(nz+4) values */
+int ho_get_ghosted(ho_state *S, int id, double *dst)
+{
+    double *a = field_ptr(S, id);
+    if (!a) return 1;
+    memcpy(dst, a, nfield(S) * sizeof(double));
+    return 0;
+}
+
+void ho_get_mesh(ho_state *S, double *dz, double *l_t, double *l_b, double *dz_t, double *dz_b)
+{
+    memcpy(dz, S->dz, sizeof(double) * S->nka);
+    memcpy(l_t, S->l_t, sizeof(double) * S->nka);
+    memcpy(l_b, S->l_b, sizeof(double) * S->nka);
+    memcpy(dz_t, S->dz_t, sizeof(double) * S->nka);
+    memcpy(dz_b, S->dz_b, sizeof(double) * S->nka);
+}
+
+/* 1-D coefficients (nz+4) and the complex bb (nx, ny, nz+4) of one equation:
+ * eq 0 = u/v, 1 = w, 2 = p, 3 = t */
+int ho_get_coeff(ho_state *S, int eq, double *aa, double *cc, cplx *bb)
+{
+    double *a[] = {S->aa_u, S->aa_w, S->aa_p, S->aa_t}, *c[] = {S->cc_u, S->cc_w, S->cc_p, S->cc_t};
+    cplx *b[] = {S->bb_u, S->bb_w, S->bb_p, S->bb_t};
+    if (eq < 0 || eq > 3) return 1;
+    memcpy(aa, a[eq], sizeof(double) * S->nka);
+    memcpy(cc, c[eq], sizeof(double) * S->nka);
+    memcpy(bb, b[eq], sizeof(cplx) * (size_t)S->prm.nx * S->prm.ny * S->nka);
+    return 0;
+}
+
+/* One standalone poisson_solver_fft call on interior arrays, with the
+ * equation's coefficients assigned by assign_abc/assign_abc_p at rk_c.
+ * eq: 0 = u (flags 0,0,0,1), 1 = v (0,0,0,0), 2 = w (1,0,0,0),
+ *     3 = p (0,1,0,0), 4 = t (0,0,1,0)  (src/module_navier_stokes.f90:291-337) */
+int ho_solve(ho_state *S, int eq, double rk_c, const double *src, double *out)
+{
+    S->rk_c = rk_c;
+    double *r_src = S->qu, *r_var = S->phi;
+    const ho_params *P = &S->prm;
+    for (int k = 1; k <= P->nz; ++k)
+        for (int j = 1; j <= P->ny; ++j)
+            for (int i = 1; i <= P->nx; ++i)
+                F3(S, r_src, i + 2, j + 2, k + 2) = src[((size_t)(k - 1) * P->ny + (j - 1)) * P->nx + (i - 1)];
+    int rc;
+    switch (eq) {
+    case 0: assign_abc(S, S->aa_u, S->bb_u, S->cc_u, 0); rc = poisson_solver_fft(S, r_src, r_var, 0, 0, 0, 1); break;
+    case 1: assign_abc(S, S->aa_u, S->bb_u, S->cc_u, 0); rc = poisson_solver_fft(S, r_src, r_var, 0, 0, 0, 0); break;
+    case 2: assign_abc(S, S->aa_w, S->bb_w, S->cc_w, 0); rc = poisson_solver_fft(S, r_src, r_var, 1, 0, 0, 0); break;
+    case 3: assign_abc(S, S->aa_p, S->bb_p, S->cc_p, 1); rc = poisson_solver_fft(S, r_src, r_var, 0, 1, 0, 0); break;
+    case 4: assign_abc(S, S->aa_t, S->bb_t, S->cc_t, 0); rc = poisson_solver_fft(S, r_src, r_var, 0, 0, 1, 0); break;
+    default: snprintf(S->err, sizeof S->err, "bad equation id"); return 1;
+    }
+    if (rc) return rc;
+    for (int k = 1; k <= P->nz; ++k)
+        for (int j = 1; j <= P->ny; ++j)
+            for (int i = 1; i <= P->nx; ++i)
+                out[((size_t)(k - 1) * P->ny + (j - 1)) * P->nx + (i - 1)] = F3(S, r_var, i + 2, j + 2, k + 2);
+    return 0;
+}
+
+/* Divergence statistics of the current velocity (avg |div|, signed max) */
+int ho_div_stats(ho_state *S, double *avg, double *mx)
+{
+    double keep = S->div_hist_max;
+    int rc = div_error(S, 0);
+    S->div_hist_max = keep;
+    *avg = S->div_avg; *mx = S->div_max;
+    return rc;
+}
+
+/* Stage-level entry points for per-function parity tests.  Each runs the
+ * named reference subroutine on the current state at substep t_its. */
+void ho_stage_interp(ho_state *S) { interp_uvw(S); bc_uvwh(S); }
+void ho_stage_momentum(ho_state *S, int t_its) { assign_rk(S, t_its); interp_uvw(S); bc_uvwh(S); momentum(S); }
+void ho_stage_p_src(ho_state *S) { p_src(S, S->qu); }
+void ho_stage_bc_uvw(ho_state *S) { bc_uvw(S); }
+void ho_stage_temperature(ho_state *S, int t_its) { assign_rk(S, t_its); interp_t(S); bc_th(S); temperature(S, S->qu); }
+double ho_get_scalar(ho_state *S, int id)
+{
+    switch (id) {
+    case 0: return S->dpx_drive;
+    case 1: return S->ubar;
+    case 2: return S->div_hist_max;
+    case 3: return S->time_sim;
+    case 4: return S->got;
+    default: return NAN;
+    }
+}
+void ho_set_scalar(ho_state *S, int id, double v)
+{
+    switch (id) {
+    case 0: S->dpx_drive = v; break;
+    case 1: S->ubar = v; break;
+    case 4: S->got = v; break;
+    default: break;
+    }
+}

@@ -1,0 +1,179 @@
+"""ORACLE — test infrastructure only.
+
+ctypes wrapper around ``oracle/libhc_oracle.so``, the plain-C restatement of
+HERCULES's finite-difference time step (``oracle/hc_oracle.c``).  Only the
+tests, ``__graft_entry__.smoke()`` and the cpu_baseline leg of ``bench.py``
+may import this module; it is the checker, never the product path.
+
+Arrays crossing this wrapper are interior fields in C order, shape
+``(nz, ny, nx)`` (x fastest), which is the reference's Fortran ``(i,j,k)``
+interior order (src/module_tools.f90:273-286 reads them the same way).
+"""
+from __future__ import annotations
+
+import ctypes as C
+import os
+from dataclasses import dataclass, field, fields
+
+import numpy as np
+
+_HERE = os.path.dirname(os.path.abspath(__file__))
+_LIB_PATH = os.path.join(_HERE, "libhc_oracle.so")
+
+_INT_FIELDS = ("nx", "ny", "nz", "ochannel", "dp_opt", "cds", "dts", "issk", "isdamp",
+               "nzdamp", "isscalar", "is_ri_var", "imax")
+_DBL_FIELDS = ("lx", "ly", "lz", "zstretch", "cdamp", "nu", "alpha", "got", "fc", "ug", "vg",
+               "dt", "u_mrf", "t_ref", "tbot", "ttop", "ri_str", "ri_end")
+
+
+class OParams(C.Structure):
+    _fields_ = [(n, C.c_int) for n in _INT_FIELDS] + [(n, C.c_double) for n in _DBL_FIELDS]
+
+
+FIELD_IDS = {"u": 0, "v": 1, "w": 2, "p": 3, "t": 4, "phi": 5, "ff": 6, "gg": 7, "hh": 8,
+             "ss": 9, "qu": 10, "qv": 11, "qw": 12, "uhx": 13, "uhy": 14, "uhz": 15,
+             "vhx": 16, "vhy": 17, "vhz": 18, "whx": 19, "why": 20, "thx": 21, "thy": 22,
+             "thz": 23}
+EQ_IDS = {"u": 0, "v": 1, "w": 2, "p": 3, "t": 4}
+
+_lib = None
+
+
+def lib():
+    global _lib
+    if _lib is None:
+        if not os.path.exists(_LIB_PATH):
+            raise RuntimeError(f"oracle library not built: {_LIB_PATH} (run make -C oracle)")
+        L = C.CDLL(_LIB_PATH)
+        vp, dp, ip = C.c_void_p, C.POINTER(C.c_double), C.c_int
+        L.ho_create.restype = vp
+        L.ho_create.argtypes = [C.POINTER(OParams)]
+        L.ho_destroy.argtypes = [vp]
+        L.ho_init.argtypes = [vp]
+        L.ho_steps.argtypes = [vp, ip]
+        L.ho_error.restype = C.c_char_p
+        L.ho_error.argtypes = [vp]
+        L.ho_set_field.argtypes = [vp, ip, dp]
+        L.ho_get_field.argtypes = [vp, ip, dp]
+        L.ho_get_ghosted.argtypes = [vp, ip, dp]
+        L.ho_get_mesh.argtypes = [vp, dp, dp, dp, dp, dp]
+        L.ho_get_coeff.argtypes = [vp, ip, dp, dp, dp]
+        L.ho_solve.argtypes = [vp, ip, C.c_double, dp, dp]
+        L.ho_div_stats.argtypes = [vp, dp, dp]
+        L.ho_fft1d.argtypes = [dp, ip, ip]
+        L.ho_mesh.argtypes = [vp]
+        L.ho_coefficients.argtypes = [vp]
+        L.ho_calc_ubar.argtypes = [vp]
+        L.ho_stage_interp.argtypes = [vp]
+        L.ho_stage_momentum.argtypes = [vp, ip]
+        L.ho_stage_p_src.argtypes = [vp]
+        L.ho_stage_bc_uvw.argtypes = [vp]
+        L.ho_stage_temperature.argtypes = [vp, ip]
+        L.ho_get_scalar.restype = C.c_double
+        L.ho_get_scalar.argtypes = [vp, ip]
+        L.ho_set_scalar.argtypes = [vp, ip, C.c_double]
+        _lib = L
+    return _lib
+
+
+def _dptr(a):
+    return a.ctypes.data_as(C.POINTER(C.c_double))
+
+
+def make_params(**kw) -> OParams:
+    p = OParams()
+    for n in _INT_FIELDS:
+        setattr(p, n, int(kw.get(n, 0)))
+    for n in _DBL_FIELDS:
+        setattr(p, n, float(kw.get(n, 0.0)))
+    return p
+
+
+class Oracle:
+    """One HERCULES state on the CPU (single rank)."""
+
+    def __init__(self, **params):
+        self.params = dict(params)
+        self._p = make_params(**params)
+        self.h = lib().ho_create(C.byref(self._p))
+        if not self.h:
+            raise ValueError("oracle rejected the parameters")
+        self.nx, self.ny, self.nz = self._p.nx, self._p.ny, self._p.nz
+
+    def close(self):
+        if self.h:
+            lib().ho_destroy(self.h)
+            self.h = None
+
+    def __del__(self):
+        try:
+            self.close()
+        except Exception:
+            pass
+
+    def _check(self, rc):
+        if rc:
+            raise RuntimeError(lib().ho_error(self.h).decode())
+
+    @property
+    def shape(self):
+        return (self.nz, self.ny, self.nx)
+
+    def set(self, name, arr):
+        a = np.ascontiguousarray(arr, dtype=np.float64)
+        assert a.shape == self.shape, (a.shape, self.shape)
+        self._check(lib().ho_set_field(self.h, FIELD_IDS[name], _dptr(a)))
+
+    def get(self, name):
+        a = np.empty(self.shape, dtype=np.float64)
+        self._check(lib().ho_get_field(self.h, FIELD_IDS[name], _dptr(a)))
+        return a
+
+    def get_ghosted(self, name):
+        a = np.empty((self.nz + 4, self.ny + 4, self.nx + 4), dtype=np.float64)
+        self._check(lib().ho_get_ghosted(self.h, FIELD_IDS[name], _dptr(a)))
+        return a
+
+    def init(self):
+        self._check(lib().ho_init(self.h))
+
+    def steps(self, n):
+        self._check(lib().ho_steps(self.h, int(n)))
+
+    def mesh(self):
+        n = self.nz + 4
+        out = [np.zeros(n) for _ in range(5)]
+        lib().ho_get_mesh(self.h, *[_dptr(a) for a in out])
+        return dict(zip(("dz", "l_t", "l_b", "dz_t", "dz_b"), out))
+
+    def coeff(self, eq):
+        n = self.nz + 4
+        aa, cc = np.zeros(n), np.zeros(n)
+        bb = np.zeros((n, self.ny, self.nx), dtype=np.complex128)
+        self._check(lib().ho_get_coeff(self.h, {"u": 0, "w": 1, "p": 2, "t": 3}[eq], _dptr(aa),
+                                       _dptr(cc), bb.ctypes.data_as(C.POINTER(C.c_double))))
+        return aa, bb, cc
+
+    def solve(self, eq, rk_c, src):
+        s = np.ascontiguousarray(src, dtype=np.float64)
+        out = np.zeros(self.shape)
+        self._check(lib().ho_solve(self.h, EQ_IDS[eq], float(rk_c), _dptr(s), _dptr(out)))
+        return out
+
+    def div_stats(self):
+        a, m = C.c_double(), C.c_double()
+        self._check(lib().ho_div_stats(self.h, C.byref(a), C.byref(m)))
+        return a.value, m.value
+
+    def scalar(self, name):
+        return lib().ho_get_scalar(self.h, {"dpx_drive": 0, "ubar": 1, "div_hist_max": 2,
+                                            "time_sim": 3, "got": 4}[name])
+
+    def set_scalar(self, name, v):
+        lib().ho_set_scalar(self.h, {"dpx_drive": 0, "ubar": 1, "got": 4}[name], float(v))
+
+
+def fft1d(x, sign):
+    a = np.ascontiguousarray(x, dtype=np.complex128).copy()
+    lib().ho_fft1d(a.ctypes.data_as(C.POINTER(C.c_double)), a.size, int(sign))
+    return a

@@ -1,0 +1,74 @@
+"""Shared helpers for the parity tests: matching oracle and HIP runs."""
+import numpy as np
+
+from hercules_amd import cases
+from hercules_amd.params import Params
+
+ORACLE_KEYS = ("nx", "ny", "nz", "ochannel", "dp_opt", "cds", "dts", "issk", "isdamp", "nzdamp",
+               "isscalar", "is_ri_var", "imax", "lx", "ly", "lz", "zstretch", "cdamp", "nu",
+               "alpha", "got", "fc", "ug", "vg", "dt", "u_mrf", "t_ref", "tbot", "ttop",
+               "ri_str", "ri_end")
+
+
+def oracle_kwargs(prm: Params):
+    return {k: getattr(prm, k) for k in ORACLE_KEYS}
+
+
+def rel_l2(a, b):
+    a = np.asarray(a)
+    b = np.asarray(b)
+    den = np.linalg.norm(b)
+    return np.linalg.norm(a - b) / (den if den > 0 else 1.0)
+
+
+def make_oracle(prm: Params, state):
+    from oracle.oracle import Oracle
+    o = Oracle(**oracle_kwargs(prm))
+    for k, v in state.items():
+        o.set(k, v)
+    o.init()
+    return o
+
+
+def small_case(kind, nx=32, ny=32, nz=33, **kw):
+    if kind == "closed":
+        prm = cases.closed_channel(nx=nx, ny=ny, nz=nz, **kw)
+        st = cases.laminar_noise(prm, seed=7)
+    elif kind == "open_t":
+        prm = cases.stratified_open_channel(nx=nx, ny=ny, nz=nz, **kw)
+        st = cases.loglaw_noise(prm, seed=11)
+    else:
+        raise ValueError(kind)
+    return prm, st
+
+
+def poisson_residual(prm: Params, rk_c, phi, src):
+    """Relative residual of the pressure equation in physical space: the
+    4th-order staggered divergence of the 4th-order gradient in x, y and
+    the 2nd-order Neumann operator in z (src/module_poisson_solver.f90:32-122),
+    applied to phi, against the source with its (0,0) level-1 value removed
+    (delnull, :558-564).  Normalised by |source| and the operator scale."""
+    from hercules_amd import grid
+    m = grid.metrics(prm.nz, prm.lz, prm.zstretch, prm.ochannel)
+    dz, dzt, dzb = m["dz"][:, None, None], m["dz_t"][:, None, None], m["dz_b"][:, None, None]
+    dx, dy = prm.lx / prm.nx, prm.ly / prm.ny
+
+    def g(f, ax, h):  # gradient at u (v) points
+        return (-np.roll(f, -2, ax) + 27 * np.roll(f, -1, ax) - 27 * f + np.roll(f, 1, ax)) / 24 / h
+
+    def d(f, ax, h):  # divergence at p points
+        return (-np.roll(f, -1, ax) + 27 * f - 27 * np.roll(f, 1, ax) + np.roll(f, 2, ax)) / 24 / h
+
+    if prm.cds == 1:
+        g = lambda f, ax, h: (np.roll(f, -1, ax) - f) / h
+        d = lambda f, ax, h: (f - np.roll(f, 1, ax)) / h
+    ph = np.concatenate([phi[:1], phi, phi[-1:]], 0)  # Neumann ghosts
+    lz = ((ph[2:] - ph[1:-1]) / dzt - (ph[1:-1] - ph[:-2]) / dzb) / dz
+    lap = d(g(phi, 2, dx), 2, dx) + d(g(phi, 1, dy), 1, dy) + lz
+    s = np.array(src, dtype=np.float64)
+    s = s - 0.0
+    # delnull removes the (0,0) mode of level 1 and pins the level-1 mean of phi
+    res = rk_c * lap - s
+    res[0] -= res[0].mean()
+    scale = np.abs(s).max() + np.abs(rk_c * lap).max()
+    return np.abs(res).max() / scale
