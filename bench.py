@@ -1,0 +1,236 @@
+#!/usr/bin/env python3
+"""Throughput of the HERCULES time step on one MI355X.
+
+Workload (default ``fd``): the finite-difference RK3 step of a neutral
+closed channel (cds=2: 4th-order horizontal, 2nd-order vertical; dp_opt=1;
+Re_tau=180; 4pi x 2pi x 2; zstretch=2.0) at 256 x 256 x 257 cells, FP64,
+on a seeded synthetic state (Reichardt log-law mean + 10% Gaussian noise).
+``fd_t`` adds the temperature equation (stratified open channel, got=0.1);
+``poisson`` times the standalone pressure Poisson solve (seeded Gaussian
+right-hand side, plane means removed).
+
+Prints one JSON line.  ``value`` is whole-job Mcell-RK3-steps/s
+(nx*ny*nz*steps / seconds / 1e6) with the state resident in HBM; the
+``roofline`` object is computed for the kernel with the largest share of
+the step time from its algorithmic bytes per launch (fields it must read
+and write, FP64) and its average duration measured with HIP events on the
+library's stream during the timed region; ``cpu_baseline`` times the CPU
+oracle (oracle/hc_oracle.c, OpenMP) on the same workload on this host.
+"""
+from __future__ import annotations
+
+import argparse
+import json
+import os
+import sys
+import time
+
+import numpy as np
+
+ROOT = os.path.dirname(os.path.abspath(__file__))
+sys.path.insert(0, ROOT)
+
+HBM_PEAK_GBS = 8000.0  # MI355X HBM3E, 8 TB/s
+METRIC = "Mcell-RK3-steps/s @256²×257: FP64 vs FP32+FP64-refine; adjoint/forward ratio"
+
+
+def algo_passes(kernel, substep, with_t):
+    """Algorithmic FP64 field passes of one launch (reads + writes of whole
+    fields, a half spectrum counting as one field)."""
+    if kernel == "rhs":
+        reads = 4 + (1 if with_t else 0) + (3 if substep > 1 else 0)
+        return reads + 6
+    return {"yfft_fwd": 6, "column": 7, "yfft_inv": 8, "finish": 9,
+            "rhs_t": None, "yfft_t": 2, "column_t": 2, "finish_t": 2,
+            "solve_xr2c": 2, "solve_yfft": 2, "solve_column": 2, "solve_xc2r": 2}[kernel]
+
+
+def rhs_t_passes(substep):
+    return 4 + (1 if substep > 1 else 0) + 2
+
+
+def build_case(workload):
+    from hercules_amd import cases
+    # wall_rows=1: every mode gets the implicit wall rows that the reference
+    # gives the (0,0) mode only; with the reference's rows this 257-level
+    # grid is unstable (rk_c*dt*nu/(2 dz_min^2) > 1).  Same kernels, same work.
+    if workload == "fd":
+        prm = cases.closed_channel(nx=256, ny=256, nz=257, wall_rows=1)
+        st = cases.loglaw_noise(prm, seed=2024)
+        st.pop("t")
+    elif workload == "fd_t":
+        prm = cases.stratified_open_channel(nx=256, ny=256, nz=257, wall_rows=1)
+        st = cases.loglaw_noise(prm, seed=2024)
+    elif workload == "poisson":
+        prm = cases.closed_channel(nx=256, ny=256, nz=257)
+        st = {"rhs": cases.gaussian_poisson_rhs(prm, seed=2024)}
+    else:
+        raise ValueError(workload)
+    return prm, st
+
+
+def cpu_baseline(prm, st, workload, steps):
+    """Oracle on this host's cores: `steps` full steps (or solves) of the same
+    workload, start-up excluded."""
+    from oracle.oracle import Oracle, lib
+    from tests.helpers import oracle_kwargs
+    o = Oracle(**oracle_kwargs(prm))
+    threads = lib().ho_num_threads()
+    n = prm.nx * prm.ny * prm.nz
+    if workload == "poisson":
+        o.init()
+        t0 = time.perf_counter()
+        for _ in range(steps):
+            o.solve("p", 1.0, st["rhs"])
+        dt = time.perf_counter() - t0
+        val, unit = steps / dt, "Poisson solves/s"
+        sample = f"{steps} pressure solve(s) at {prm.nx}x{prm.ny}x{prm.nz}"
+    else:
+        for k, v in st.items():
+            o.set(k, v)
+        o.init()
+        t0 = time.perf_counter()
+        o.steps(steps)
+        dt = time.perf_counter() - t0
+        val, unit = n * steps / dt / 1e6, "Mcell-RK3-steps/s"
+        sample = f"{steps} RK3 step(s) of the same {prm.nx}x{prm.ny}x{prm.nz} workload, start-up excluded"
+    o.close()
+    return {"value": val, "unit": unit, "cores": int(threads), "kind": "port", "sample": sample,
+            "seconds": dt}
+
+
+def main():
+    ap = argparse.ArgumentParser(description=__doc__.split("\n")[0])
+    ap.add_argument("--gpus", type=int, default=1)
+    ap.add_argument("--steps", type=int, default=50)
+    ap.add_argument("--warmup", type=int, default=5)
+    ap.add_argument("--workload", default="fd", choices=["fd", "fd_t", "poisson"])
+    ap.add_argument("--cpu-steps", type=int, default=1, help="oracle steps for the CPU baseline")
+    ap.add_argument("--no-cpu-baseline", action="store_true")
+    ap.add_argument("--no-profile", action="store_true", help="skip the per-kernel event timing")
+    args = ap.parse_args()
+    if args.gpus != 1:
+        print(f"bench.py: this build runs on one GPU; --gpus {args.gpus} is not supported "
+              "(the channel is resident in one MI355X's HBM)", file=sys.stderr)
+        sys.exit(2)
+
+    from hercules_amd import Channel
+
+    prm, st = build_case(args.workload)
+    n = prm.nx * prm.ny * prm.nz
+    ch = Channel(prm)
+    with_t = args.workload == "fd_t"
+    if args.workload == "poisson":
+        import ctypes as C
+        rhs = np.ascontiguousarray(st["rhs"])
+        # device-resident source and result: reuse two field slots of the context
+        ch.set_field("u", rhs)
+        src_ptr, dst_ptr = ch.device_ptr("u"), ch.device_ptr("v")
+        step = lambda: ch.poisson_solver_fft_device("p", 1.0, src_ptr, dst_ptr)
+    else:
+        for k, v in st.items():
+            ch.set_field(k, v)
+        ch.project()  # seeded noise made discretely divergence-free
+        for k in ("u", "v", "w"):
+            st[k] = ch.get_field(k)
+        ch.init()
+        step = lambda: ch.steps(1)
+
+    for _ in range(args.warmup):
+        step()
+    ch.synchronize()
+    if not args.no_profile:
+        ch.profile(True)
+    t0 = time.perf_counter()
+    for _ in range(args.steps):
+        step()
+    ch.synchronize()
+    elapsed = time.perf_counter() - t0
+    prof = ch.profile_read() if not args.no_profile else {}
+    ch.profile(False)
+    ms_per_step = elapsed / args.steps * 1e3
+
+    # sanity of the final state
+    finite = True
+    div = None
+    if args.workload != "poisson":
+        u = ch.get_field("u")
+        finite = bool(np.isfinite(u).all())
+        try:
+            div = ch.screen_div_error()
+        except Exception as e:  # NaN in the state
+            div = str(e)
+
+    if args.workload == "poisson":
+        value = args.steps / elapsed
+        unit = "Poisson solves/s"
+    else:
+        value = n * args.steps / elapsed / 1e6
+        unit = "Mcell-RK3-steps/s"
+
+    # per-kernel algorithmic bandwidth
+    kernels = {}
+    nsub = 3 if prm.dts == 2 else 1
+    for name, (ms, launches) in prof.items():
+        if name == "rhs":
+            per_step = sum(algo_passes("rhs", s, with_t) for s in range(1, nsub + 1))
+            passes = per_step / nsub
+        elif name == "rhs_t":
+            passes = sum(rhs_t_passes(s) for s in range(1, nsub + 1)) / nsub
+        else:
+            passes = algo_passes(name, 1, with_t)
+        bytes_per_launch = passes * n * 8
+        avg_ms = ms / launches
+        kernels[name] = {"avg_ms": avg_ms, "launches": launches, "share": 0.0,
+                         "algo_bytes": bytes_per_launch,
+                         "GBs": bytes_per_launch / (avg_ms * 1e-3) / 1e9}
+    tot = sum(v["avg_ms"] * v["launches"] for v in kernels.values()) or 1.0
+    for v in kernels.values():
+        v["share"] = v["avg_ms"] * v["launches"] / tot
+    roofline = None
+    if kernels:
+        dom = max(kernels, key=lambda k: kernels[k]["avg_ms"] * kernels[k]["launches"])
+        kd = kernels[dom]
+        roofline = {"bound": "hbm", "kernel": dom, "achieved": round(kd["GBs"], 1),
+                    "peak": HBM_PEAK_GBS, "unit": "GB/s", "frac": round(kd["GBs"] / HBM_PEAK_GBS, 4),
+                    "traffic": None, "avg_ms": round(kd["avg_ms"], 4),
+                    "algo_bytes_per_launch": int(kd["algo_bytes"])}
+    # whole step against the step's algorithmic bytes (1272 B/cell/step for FD
+    # without T, 1608 with T: 159 / 201 FP64 passes per RK3 step)
+    step_passes = {"fd": 159, "fd_t": 201, "poisson": 10}[args.workload]
+    step_gbs = step_passes * n * 8 / (ms_per_step * 1e-3) / 1e9
+    step_roofline = {"passes_per_step": step_passes, "achieved": round(step_gbs, 1),
+                     "peak": HBM_PEAK_GBS, "unit": "GB/s", "frac": round(step_gbs / HBM_PEAK_GBS, 4)}
+
+    cpu = None
+    if not args.no_cpu_baseline:
+        try:
+            cpu = cpu_baseline(prm, st, args.workload, args.cpu_steps)
+        except Exception as e:  # the baseline never gates the GPU number
+            cpu = {"error": str(e)}
+
+    workload_name = {
+        "fd": "FD RK3 step (cds=2), neutral closed channel, Re_tau=180, 256x256x257, zstretch=1, wall_rows=1",
+        "fd_t": "FD RK3 step (cds=2) with temperature, stratified open channel, Re_tau=395, got=0.1, 256x256x257, zstretch=1, wall_rows=1",
+        "poisson": "standalone pressure Poisson solve, 256x256x257, zstretch=1",
+    }[args.workload]
+    out = {
+        "metric": METRIC, "value": round(value, 2), "unit": unit, "n_gpus": 1, "steps": args.steps,
+        "warmup": args.warmup, "ms_per_step": round(ms_per_step, 4), "higher_is_better": True,
+        "scaling": "weak", "vs_baseline": None, "dtype": "f64",
+        "data": "synthetic (seeded log-law + noise state; no checkpoint)",
+        "config": {"workload": workload_name, "nx": prm.nx, "ny": prm.ny, "nz": prm.nz,
+                   "cells": n, "precision": "fp64"},
+        "roofline": roofline, "step_roofline": step_roofline, "cpu_baseline": cpu,
+        "kernels": {k: {kk: (round(vv, 4) if isinstance(vv, float) else vv) for kk, vv in v.items()}
+                    for k, v in kernels.items()},
+        "state_finite": finite, "div": div,
+    }
+    if cpu and "value" in cpu and cpu["value"]:
+        out["gpu_over_cpu"] = round(value / cpu["value"], 1)
+    ch.close()
+    print(json.dumps(out))
+
+
+if __name__ == "__main__":
+    main()

@@ -15,10 +15,16 @@ from .grid import metrics
 from .params import Params
 
 
+# The grid stretching "factor 2.0" of the benchmark configurations is read
+# as the argument of tanh(2*zstretch*...) in get_mesh_param
+# (src/module_parameters.f90:464-465), i.e. zstretch = 1.0.
+ZSTRETCH = 1.0
+
+
 def closed_channel(nx=64, ny=64, nz=65, **kw) -> Params:
     """Neutral closed channel, Re_tau = 180, constant mean pressure gradient,
-    4th-order horizontal FD (cds=2), RK3, dt = 1e-3, zstretch = 2.0."""
-    base = dict(lx=4.0 * np.pi, ly=2.0 * np.pi, lz=2.0, nx=nx, ny=ny, nz=nz, zstretch=2.0,
+    4th-order horizontal FD (cds=2), RK3, dt = 1e-3, tanh(2 zeta) stretching."""
+    base = dict(lx=4.0 * np.pi, ly=2.0 * np.pi, lz=2.0, nx=nx, ny=ny, nz=nz, zstretch=ZSTRETCH,
                 ochannel=0, dp_opt=1, cds=2, dts=2, nu=1.0 / 180.0, alpha=1.0 / 180.0 / 0.71,
                 dt=1e-3, imax=100, isscalar=0, restart=1, isnoise=0, isdamp=0, u_mrf=0.0)
     base.update(kw)
@@ -29,7 +35,7 @@ def stratified_open_channel(nx=256, ny=256, nz=257, **kw) -> Params:
     """Stably stratified open channel (free-slip top, no-slip bottom),
     Re_tau = 395, got = 0.1, Pr = 0.71, 4pi x 2pi x 1."""
     nu = 1.0 / 395.0
-    base = dict(lx=4.0 * np.pi, ly=2.0 * np.pi, lz=1.0, nx=nx, ny=ny, nz=nz, zstretch=2.0,
+    base = dict(lx=4.0 * np.pi, ly=2.0 * np.pi, lz=1.0, nx=nx, ny=ny, nz=nz, zstretch=ZSTRETCH,
                 ochannel=1, dp_opt=1, cds=2, dts=2, nu=nu, alpha=nu / 0.71, got=0.1,
                 dt=1e-3, imax=500, isscalar=1, tbot=0.0, ttop=1.0, t_ref=0.0, restart=1)
     base.update(kw)

@@ -25,9 +25,12 @@ FIELDS = {"u": 0, "v": 1, "w": 2, "p": 3, "t": 4, "ff": 5, "gg": 6, "hh": 7, "ss
 EQUATIONS = {"u": 0, "v": 1, "w": 2, "p": 3, "t": 4}
 LAYOUT_INTERIOR, LAYOUT_GHOSTED = 0, 1
 ABI_SYMBOLS = ("hc_last_error", "hc_create", "hc_destroy", "hc_set_field", "hc_get_field",
-               "hc_init", "hc_time_advancement_fd", "hc_drive_steps", "hc_poisson_solver_fft",
+               "hc_init", "hc_project", "hc_time_advancement_fd", "hc_drive_steps", "hc_poisson_solver_fft",
                "hc_calculate_ubar", "hc_screen_div_error", "hc_get_scalar", "hc_stream",
-               "hc_field_device_ptr", "hc_synchronize")
+               "hc_field_device_ptr", "hc_synchronize", "hc_profile_enable", "hc_profile_read")
+KERNELS = {"rhs": 0, "yfft_fwd": 1, "column": 2, "yfft_inv": 3, "finish": 4, "rhs_t": 5,
+           "yfft_t": 6, "column_t": 7, "finish_t": 8, "solve_xr2c": 9, "solve_yfft": 10,
+           "solve_column": 11, "solve_xc2r": 12}
 
 
 class HerculesError(RuntimeError):
@@ -54,6 +57,7 @@ def load_library(path: str = LIB_PATH):
     L.hc_set_field.argtypes = [vp, ip, vp, ip]
     L.hc_get_field.argtypes = [vp, ip, vp, ip]
     L.hc_init.argtypes = [vp]
+    L.hc_project.argtypes = [vp]
     L.hc_time_advancement_fd.argtypes = [vp]
     L.hc_drive_steps.argtypes = [vp, ip]
     L.hc_poisson_solver_fft.argtypes = [vp, ip, C.c_double, vp, vp, ip]
@@ -65,6 +69,8 @@ def load_library(path: str = LIB_PATH):
     L.hc_field_device_ptr.restype = vp
     L.hc_field_device_ptr.argtypes = [vp, ip]
     L.hc_synchronize.argtypes = [vp]
+    L.hc_profile_enable.argtypes = [vp, ip]
+    L.hc_profile_read.argtypes = [vp, ip, dp, C.POINTER(C.c_long)]
     _lib = L
     return L
 
@@ -129,6 +135,10 @@ class Channel:
     def init(self):
         self._check(self.L.hc_init(self.h))
 
+    def project(self):
+        """Make (u, v, w) discretely divergence-free (initial-state preparation)."""
+        self._check(self.L.hc_project(self.h))
+
     def time_advancement_fd(self):
         self._check(self.L.hc_time_advancement_fd(self.h))
 
@@ -175,3 +185,17 @@ class Channel:
 
     def synchronize(self):
         self._check(self.L.hc_synchronize(self.h))
+
+    # ---- kernel timing
+    def profile(self, on=True):
+        self._check(self.L.hc_profile_enable(self.h, 1 if on else 0))
+
+    def profile_read(self):
+        """{kernel: (total_ms, launches)} accumulated since profile(True)."""
+        out = {}
+        for name, kid in KERNELS.items():
+            ms, n = C.c_double(), C.c_long()
+            self._check(self.L.hc_profile_read(self.h, kid, C.byref(ms), C.byref(n)))
+            if n.value:
+                out[name] = (ms.value, n.value)
+        return out

@@ -311,6 +311,72 @@ HC_DEV void fixups(const StepArgs &A, int eq, const ColMap &m, double2 (&d)[L])
     }
 }
 
+// Spectral divergence of (u, v, w) for this lane's levels, times `scale`:
+// D_x u + D_y v + (w(k) - w(k-1))/dz(k), w(-1) = 0 (get_p_eqn_src,
+// src/module_poisson_solver.f90:141-172, for periodic x, y).
+template <int L>
+HC_DEV void col_divergence(const StepArgs &A, const ColMap &m, int l, int jm, const double2 (&u)[L],
+                           const double2 (&v)[L], const double2 (&w)[L], double scale, double2 (&out)[L])
+{
+    const int nz = A.nz, nl = A.nl, ny = A.ny;
+    const double2 sdx = make_double2(A.symx[SY_DRE * nl + l], A.symx[SY_DIM * nl + l]);
+    const double2 sdy = make_double2(A.symy[SY_DRE * ny + jm], A.symy[SY_DIM * ny + jm]);
+    double2 wprev = shfl_up_c(last_of<L>(w, m.cnt), 1);
+    if (m.lane == 0) wprev = czero();
+#pragma unroll
+    for (int r = 0; r < L; ++r) {
+        out[r] = czero();
+        if (r < m.cnt) {
+            int k = m.k0 + r;
+            double2 wm = r == 0 ? wprev : w[r > 0 ? r - 1 : 0];
+            double dz = A.lev[LV_DZ * nz + k];
+            double2 dv = cadd(cmul(sdx, u[r]), cmul(sdy, v[r]));
+            double2 dwz = csub(w[r], wm);
+            dv.x += dwz.x / dz;
+            dv.y += dwz.y / dz;
+            out[r] = cscale(dv, scale);
+        }
+    }
+}
+
+// phi(k+1) seen from level r of this lane (Neumann copy at the top wall)
+template <int L>
+HC_DEV double2 col_next(const ColMap &m, int nz, const double2 (&ph)[L], int r, double2 pnext)
+{
+    int k = m.k0 + r;
+    if (k >= nz - 1) return ph[r];
+    double2 cand = czero();
+#pragma unroll
+    for (int q = 0; q < L; ++q)
+        if (q == r + 1) cand = ph[q];
+    return r + 1 < m.cnt ? cand : pnext;
+}
+
+// Projection u -= cdt G_x phi, v -= cdt G_y phi, w(k) -= cdt (phi(k+1)-phi(k))/dz_t(k)
+// for k < kend3 (calculate_uvw, src/module_navier_stokes.f90:659-734).
+template <int L>
+HC_DEV void col_project(const StepArgs &A, const ColMap &m, int l, int jm, double cdt, const double2 (&ph)[L],
+                        double2 (&u)[L], double2 (&v)[L], double2 (&w)[L])
+{
+    const int nz = A.nz, nl = A.nl, ny = A.ny;
+    const double2 sgx = make_double2(A.symx[SY_GRE * nl + l], A.symx[SY_GIM * nl + l]);
+    const double2 sgy = make_double2(A.symy[SY_GRE * ny + jm], A.symy[SY_GIM * ny + jm]);
+    double2 pnext = shfl_dn_c(ph[0], 1);
+#pragma unroll
+    for (int r = 0; r < L; ++r) {
+        if (r < m.cnt) {
+            int k = m.k0 + r;
+            u[r] = csub(u[r], cscale(cmul(sgx, ph[r]), cdt));
+            v[r] = csub(v[r], cscale(cmul(sgy, ph[r]), cdt));
+            if (k < nz - 1) {
+                double2 gz = csub(col_next<L>(m, nz, ph, r, pnext), ph[r]);
+                double dzt = A.lev[LV_DZT * nz + k];
+                w[r] = make_double2(w[r].x - cdt * gz.x / dzt, w[r].y - cdt * gz.y / dzt);
+            }
+        }
+    }
+}
+
 template <int L>
 __global__ __launch_bounds__(64 * kColWaves) void k_column_mom(StepArgs A)
 {
@@ -343,24 +409,7 @@ __global__ __launch_bounds__(64 * kColWaves) void k_column_mom(StepArgs A)
     assemble<L>(A, 1, l, jm, is00, m, a, b, c);
     wave_tridiag<L, 1>(m.cnt, a, b, c, dw);
     // divergence source /dt (get_p_eqn_src, :127-176) in spectral form
-    const double2 sdx = make_double2(A.symx[SY_DRE * nl + l], A.symx[SY_DIM * nl + l]);
-    const double2 sdy = make_double2(A.symy[SY_DRE * ny + jm], A.symy[SY_DIM * ny + jm]);
-    double2 wprev = shfl_up_c(last_of<L>(dw[0], m.cnt), 1);
-    if (m.lane == 0) wprev = czero();
-#pragma unroll
-    for (int r = 0; r < L; ++r) {
-        dp[0][r] = czero();
-        if (r < m.cnt) {
-            int k = m.k0 + r;
-            double2 wm = r == 0 ? wprev : dw[0][r > 0 ? r - 1 : 0];
-            double dz = A.lev[LV_DZ * nz + k];
-            double2 dv = cadd(cmul(sdx, duv[0][r]), cmul(sdy, duv[1][r]));
-            double2 dwz = csub(dw[0][r], wm);
-            dv.x += dwz.x / dz;
-            dv.y += dwz.y / dz;
-            dp[0][r] = cscale(dv, 1.0 / A.dt);
-        }
-    }
+    col_divergence<L>(A, m, l, jm, duv[0], duv[1], dw[0], 1.0 / A.dt, dp[0]);
     // pressure Poisson (:312-314)
     assemble<L>(A, 2, l, jm, is00, m, a, b, c);
     if (is00) fixups<L>(A, 2, m, dp[0]);
@@ -368,8 +417,7 @@ __global__ __launch_bounds__(64 * kColWaves) void k_column_mom(StepArgs A)
     // projection (calculate_uvw, :659-734) and pressure increment
     // (calculate_new_p, :739-808)
     const double cdt = A.rc * A.dt;
-    const double2 sgx = make_double2(A.symx[SY_GRE * nl + l], A.symx[SY_GIM * nl + l]);
-    const double2 sgy = make_double2(A.symy[SY_GRE * ny + jm], A.symy[SY_GIM * ny + jm]);
+    col_project<L>(A, m, l, jm, cdt, dp[0], duv[0], duv[1], dw[0]);
     const double slap = A.symx[SY_L * nl + l] + A.symy[SY_L * ny + jm];
     double2 pprev = shfl_up_c(last_of<L>(dp[0], m.cnt), 1);
     double2 pnext = shfl_dn_c(dp[0][0], 1);
@@ -380,22 +428,8 @@ __global__ __launch_bounds__(64 * kColWaves) void k_column_mom(StepArgs A)
             int k = m.k0 + r;
             double2 ph = dp[0][r];
             double2 pm = r == 0 ? (m.lane == 0 ? ph : pprev) : dp[0][r > 0 ? r - 1 : 0];
-            double2 pn = ph;
-            if (k < nz - 1) {
-                bool nxt_in_lane = (r + 1 < m.cnt);
-                double2 cand = czero();
-#pragma unroll
-                for (int q = 0; q < L; ++q)
-                    if (q == r + 1) cand = dp[0][q];
-                pn = nxt_in_lane ? cand : pnext;
-            }
-            duv[0][r] = csub(duv[0][r], cscale(cmul(sgx, ph), cdt));
-            duv[1][r] = csub(duv[1][r], cscale(cmul(sgy, ph), cdt));
+            double2 pn = col_next<L>(m, nz, dp[0], r, pnext);
             double dzt = A.lev[LV_DZT * nz + k], dzb = A.lev[LV_DZB * nz + k], dz = A.lev[LV_DZ * nz + k];
-            if (k < nz - 1) {
-                double2 gz = csub(pn, ph);
-                dw[0][r] = make_double2(dw[0][r].x - cdt * gz.x / dzt, dw[0][r].y - cdt * gz.y / dzt);
-            }
             double2 lz;
             lz.x = ((pn.x - ph.x) / dzt - (ph.x - pm.x) / dzb) / dz;
             lz.y = ((pn.y - ph.y) / dzt - (ph.y - pm.y) / dzb) / dz;
@@ -429,6 +463,45 @@ __global__ __launch_bounds__(64 * kColWaves) void k_column_t(StepArgs A)
 #pragma unroll
     for (int r = 0; r < L; ++r)
         if (r < m.cnt) A.s0[(size_t)(m.k0 + r) * plane + col] = d[0][r];
+}
+
+// Projection of (u, v, w) spectra in s0, s1, s2 onto discretely
+// divergence-free fields with the pressure operator at rk_c = 1:
+// phi = L^{-1} div u, u -= G phi (the operators of get_p_eqn_src,
+// poisson_solver_fft and calculate_uvw).  Used to prepare initial states.
+template <int L>
+__global__ __launch_bounds__(64 * kColWaves) void k_column_project(StepArgs A)
+{
+    const int l = blockIdx.x * kColWaves + (threadIdx.x >> 6), jm = blockIdx.y;
+    if (l >= A.nl) return;
+    const ColMap m = col_map(A.nz);
+    const bool is00 = (l == 0 && jm == 0);
+    const size_t plane = (size_t)A.ny * A.nl, col = (size_t)jm * A.nl + l;
+    double2 u[1][L], v[1][L], w[1][L], ph[1][L];
+#pragma unroll
+    for (int r = 0; r < L; ++r) {
+        u[0][r] = v[0][r] = w[0][r] = czero();
+        if (r < m.cnt) {
+            size_t o = (size_t)(m.k0 + r) * plane + col;
+            u[0][r] = A.s0[o];
+            v[0][r] = A.s1[o];
+            w[0][r] = m.k0 + r == A.nz - 1 ? czero() : A.s2[o];
+        }
+    }
+    col_divergence<L>(A, m, l, jm, u[0], v[0], w[0], 1.0, ph[0]);
+    double a[L], b[L], c[L];
+    assemble<L>(A, 2, l, jm, is00, m, a, b, c);
+    if (is00) fixups<L>(A, 2, m, ph[0]);
+    wave_tridiag_refined<L, 1>(m.cnt, a, b, c, ph);
+    col_project<L>(A, m, l, jm, 1.0, ph[0], u[0], v[0], w[0]);
+#pragma unroll
+    for (int r = 0; r < L; ++r)
+        if (r < m.cnt) {
+            size_t o = (size_t)(m.k0 + r) * plane + col;
+            A.s0[o] = u[0][r];
+            A.s1[o] = v[0][r];
+            A.s2[o] = w[0][r];
+        }
 }
 
 // Standalone poisson_solver_fft for one equation, in place on s0.
@@ -591,7 +664,37 @@ struct hc_ctx {
     int stps;
     std::vector<double> h_lev;
     std::vector<void *> allocs;
+    // optional per-kernel timing with HIP events on the context stream
+    bool prof = false;
+    std::vector<hipEvent_t> ev_pool;
+    std::vector<int> ev_kid;  // kernel id of each recorded (start, stop) pair
+    size_t ev_used = 0;
+    double prof_ms[HC_KERNEL_COUNT] = {};
+    long prof_n[HC_KERNEL_COUNT] = {};
 };
+
+// Record a start (begin=true) or stop event for kernel `kid` when profiling.
+static void prof_mark(hc_ctx *c, int kid, bool begin)
+{
+    if (!c->prof) return;
+    if (begin) {
+        if (c->ev_used + 2 > c->ev_pool.size()) {
+            size_t add = c->ev_pool.size() < 256 ? 256 : c->ev_pool.size();
+            for (size_t q = 0; q < add; ++q) {
+                hipEvent_t e;
+                if (hipEventCreate(&e) != hipSuccess) { c->prof = false; return; }
+                c->ev_pool.push_back(e);
+            }
+        }
+        c->ev_kid.push_back(kid);
+        (void)hipEventRecord(c->ev_pool[c->ev_used], c->stream);
+    } else {
+        (void)hipEventRecord(c->ev_pool[c->ev_used + 1], c->stream);
+        c->ev_used += 2;
+    }
+}
+#define PROF_BEGIN(c, kid) prof_mark(c, kid, true)
+#define PROF_END(c, kid) prof_mark(c, kid, false)
 
 static bool fft_len_ok(int n) { return n == 16 || n == 32 || n == 64 || n == 128 || n == 192 || n == 256 || n == 512; }
 
@@ -660,7 +763,7 @@ static void build_levels(hc_ctx *c)
         double d = -a - cc, d00 = -a - cc;
         if (kk == 0) { d = -cc; d00 = -2.0 * a - cc; }
         if (kk == nz - 1) { d = -a; d00 = P.ochannel == 0 ? -a - 2.0 * cc : -a; }
-        at(LV_DU, kk) = d; at(LV_DU00, kk) = d00;
+        at(LV_DU, kk) = P.wall_rows ? d00 : d; at(LV_DU00, kk) = d00;
         // t: get_t_eqn_coeff (:398-399), Dirichlet at both walls
         a = -0.5 * P.dt * P.alpha / dz[k] / dzb[k]; cc = -0.5 * P.dt * P.alpha / dz[k] / dzt[k];
         at(LV_AT, kk) = kk == 0 ? 0.0 : a;
@@ -668,7 +771,7 @@ static void build_levels(hc_ctx *c)
         d = d00 = -a - cc;
         if (kk == 0) { d = -cc; d00 = -2.0 * a - cc; }
         if (kk == nz - 1) { d = -a; d00 = -a - 2.0 * cc; }
-        at(LV_DT, kk) = d; at(LV_DT00, kk) = d00;
+        at(LV_DT, kk) = P.wall_rows ? d00 : d; at(LV_DT00, kk) = d00;
         // p: get_p_eqn_coeff (:51-52), Neumann at both walls
         a = 1.0 / dz[k] / dzb[k]; cc = 1.0 / dz[k] / dzt[k];
         at(LV_AP, kk) = kk == 0 ? 0.0 : a;
@@ -685,7 +788,7 @@ static void build_levels(hc_ctx *c)
             d = -a - cc;
             if (kk == 0) d = -cc;
             if (kk == nz - 2) d = -a;
-            at(LV_DW, kk) = d; at(LV_DW00, kk) = -a - cc;
+            at(LV_DW, kk) = P.wall_rows ? -a - cc : d; at(LV_DW00, kk) = -a - cc;
         }
     }
     int ks = 3, ke = nz + 2;
@@ -809,6 +912,7 @@ static int launch_c2r(hc_ctx *c, int nf, C2RArgs &X)
     dim3 g((c->ny + rows - 1) / rows, c->nz);
 #define HC_C(NX_)                                                                                  \
     if (nf == 4) hipLaunchKernelGGL((k_x_c2r<NX_, 4>), g, dim3(kRhsThreads), 0, c->stream, X);    \
+    else if (nf == 3) hipLaunchKernelGGL((k_x_c2r<NX_, 3>), g, dim3(kRhsThreads), 0, c->stream, X); \
     else hipLaunchKernelGGL((k_x_c2r<NX_, 1>), g, dim3(kRhsThreads), 0, c->stream, X);
     HC_DISPATCH_N(c->nx, HC_C)
 #undef HC_C
@@ -841,41 +945,61 @@ static int substep(hc_ctx *c, int t_its)
     if (P.dp_opt == 2) hipLaunchKernelGGL(k_dpx_update, dim3(1), dim3(64), 0, c->stream, c->scal, P.u_mrf, P.nu * 1.0);
     int rows = rows_for(c->nx);
     dim3 grow((c->ny + rows - 1) / rows, c->nz);
+    PROF_BEGIN(c, HC_K_RHS);
 #define HC_RHS(NX_)                                                                                 \
     if (P.cds == 1) hipLaunchKernelGGL((k_rhs_mom<1, NX_>), grow, dim3(kRhsThreads), 0, c->stream, A); \
     else hipLaunchKernelGGL((k_rhs_mom<2, NX_>), grow, dim3(kRhsThreads), 0, c->stream, A);
     HC_DISPATCH_N(c->nx, HC_RHS)
 #undef HC_RHS
+    PROF_END(c, HC_K_RHS);
+    PROF_BEGIN(c, HC_K_YFFT_FWD);
     if (int e = launch_yfft(c, -1, 3, c->s[0], c->s[1], c->s[2], c->s[3])) return e;
+    PROF_END(c, HC_K_YFFT_FWD);
     dim3 gcol((c->nl + kColWaves - 1) / kColWaves, c->ny);
+    PROF_BEGIN(c, HC_K_COLUMN);
 #define HC_COL(L_) hipLaunchKernelGGL((k_column_mom<L_>), gcol, dim3(64 * kColWaves), 0, c->stream, A);
     HC_DISPATCH_L(c->Lcol, HC_COL)
 #undef HC_COL
+    PROF_END(c, HC_K_COLUMN);
+    PROF_BEGIN(c, HC_K_YFFT_INV);
     if (int e = launch_yfft(c, +1, 4, c->s[0], c->s[1], c->s[2], c->s[3])) return e;
+    PROF_END(c, HC_K_YFFT_INV);
     C2RArgs X;
     std::memset(&X, 0, sizeof X);
     X.src[0] = c->s[0]; X.src[1] = c->s[1]; X.src[2] = c->s[2]; X.src[3] = c->s[3];
     X.dst[0] = c->u; X.dst[1] = c->v; X.dst[2] = c->w; X.dst[3] = c->p;
     X.mode[3] = 1;
+    PROF_BEGIN(c, HC_K_FINISH);
     if (int e = launch_c2r(c, 4, X)) return e;
+    PROF_END(c, HC_K_FINISH);
     c->fcur = nxt;
     if (P.isscalar == 1) {
         int snx = c->scur ^ 1;
         A.ss0 = c->sb[c->scur]; A.ss = c->sb[snx];
+        PROF_BEGIN(c, HC_K_RHS_T);
 #define HC_RT(NX_)                                                                                  \
     if (P.cds == 1) hipLaunchKernelGGL((k_rhs_t<1, NX_>), grow, dim3(kRhsThreads), 0, c->stream, A);  \
     else hipLaunchKernelGGL((k_rhs_t<2, NX_>), grow, dim3(kRhsThreads), 0, c->stream, A);
         HC_DISPATCH_N(c->nx, HC_RT)
 #undef HC_RT
+        PROF_END(c, HC_K_RHS_T);
+        PROF_BEGIN(c, HC_K_YFFT_T);
         if (int e = launch_yfft(c, -1, 1, c->s[0], nullptr, nullptr, nullptr)) return e;
+        PROF_END(c, HC_K_YFFT_T);
+        PROF_BEGIN(c, HC_K_COLUMN_T);
 #define HC_COLT(L_) hipLaunchKernelGGL((k_column_t<L_>), gcol, dim3(64 * kColWaves), 0, c->stream, A);
         HC_DISPATCH_L(c->Lcol, HC_COLT)
 #undef HC_COLT
+        PROF_END(c, HC_K_COLUMN_T);
+        PROF_BEGIN(c, HC_K_YFFT_T);
         if (int e = launch_yfft(c, +1, 1, c->s[0], nullptr, nullptr, nullptr)) return e;
+        PROF_END(c, HC_K_YFFT_T);
         C2RArgs XT;
         std::memset(&XT, 0, sizeof XT);
         XT.src[0] = c->s[0]; XT.dst[0] = c->t;
+        PROF_BEGIN(c, HC_K_FINISH_T);
         if (int e = launch_c2r(c, 1, XT)) return e;
+        PROF_END(c, HC_K_FINISH_T);
         c->scur = snx;
     }
     HIPCHK(hipGetLastError());
@@ -905,6 +1029,7 @@ extern "C" int hc_create(const hc_params *prm, hc_ctx **out)
     if (P.ochannel != 0 && P.ochannel != 1) return set_err(HC_ERR_ARG, "ochannel must be 0 or 1");
     if (P.isscalar != 0 && P.isscalar != 1) return set_err(HC_ERR_ARG, "isscalar error!");
     if (P.isdamp != 0) return set_err(HC_ERR_UNSUPPORTED, "isdamp=1 (Rayleigh damping) is not implemented");
+    if (P.wall_rows != 0 && P.wall_rows != 1) return set_err(HC_ERR_ARG, "wall_rows must be 0 or 1");
     if (!fft_len_ok(P.nx) || !fft_len_ok(P.ny)) return set_err(HC_ERR_UNSUPPORTED, "nx, ny must be one of 16,32,64,128,192,256,512");
     int L = col_levels(P.nz);
     if (L < 0) return set_err(HC_ERR_UNSUPPORTED, "nz=%d too large", P.nz);
@@ -955,6 +1080,7 @@ extern "C" int hc_destroy(hc_ctx *c)
 {
     if (!c) return 0;
     if (c->stream) (void)hipStreamSynchronize(c->stream);
+    for (hipEvent_t e : c->ev_pool) (void)hipEventDestroy(e);
     for (void *p : c->allocs) (void)hipFree(p);
     if (c->stage) (void)hipFree(c->stage);
     if (c->stream) (void)hipStreamDestroy(c->stream);
@@ -1103,20 +1229,30 @@ extern "C" int hc_poisson_solver_fft(hc_ctx *c, int equation, double rk_c, const
         }
         src = work;
     }
+    PROF_BEGIN(c, HC_K_SOLVE_XR2C);
     if (int e = launch_r2c(c, src, c->s[0])) return e;
+    PROF_END(c, HC_K_SOLVE_XR2C);
+    PROF_BEGIN(c, HC_K_SOLVE_YFFT);
     if (int e = launch_yfft(c, -1, 1, c->s[0], nullptr, nullptr, nullptr)) return e;
+    PROF_END(c, HC_K_SOLVE_YFFT);
     StepArgs A = base_args(c);
     A.rc = c->prm.dts == 2 ? rk_c : 1.0;
     dim3 gcol((c->nl + kColWaves - 1) / kColWaves, c->ny);
+    PROF_BEGIN(c, HC_K_SOLVE_COLUMN);
 #define HC_CS(L_) hipLaunchKernelGGL((k_column_solve<L_>), gcol, dim3(64 * kColWaves), 0, c->stream, A, equation);
     HC_DISPATCH_L(c->Lcol, HC_CS)
 #undef HC_CS
+    PROF_END(c, HC_K_SOLVE_COLUMN);
+    PROF_BEGIN(c, HC_K_SOLVE_YFFT);
     if (int e = launch_yfft(c, +1, 1, c->s[0], nullptr, nullptr, nullptr)) return e;
+    PROF_END(c, HC_K_SOLVE_YFFT);
     C2RArgs X;
     std::memset(&X, 0, sizeof X);
     X.src[0] = c->s[0];
     X.dst[0] = dev_io ? r_var : work;
+    PROF_BEGIN(c, HC_K_SOLVE_XC2R);
     if (int e = launch_c2r(c, 1, X)) return e;
+    PROF_END(c, HC_K_SOLVE_XC2R);
     if (!dev_io) {
         if (layout == HC_LAYOUT_INTERIOR) {
             HIPCHK(hipMemcpyAsync(r_var, work, N * sizeof(double), hipMemcpyDefault, c->stream));
@@ -1187,6 +1323,62 @@ extern "C" double *hc_field_device_ptr(hc_ctx *c, int field) { return c ? field_
 extern "C" int hc_synchronize(hc_ctx *c)
 {
     if (!c) return set_err(HC_ERR_ARG, "null context");
+    HIPCHK(hipStreamSynchronize(c->stream));
+    return 0;
+}
+
+extern "C" int hc_profile_enable(hc_ctx *c, int on)
+{
+    if (!c) return set_err(HC_ERR_ARG, "null context");
+    HIPCHK(hipStreamSynchronize(c->stream));
+    c->prof = on != 0;
+    c->ev_used = 0;
+    c->ev_kid.clear();
+    for (int k = 0; k < HC_KERNEL_COUNT; ++k) { c->prof_ms[k] = 0.0; c->prof_n[k] = 0; }
+    return 0;
+}
+
+extern "C" int hc_profile_read(hc_ctx *c, int kernel, double *total_ms, long *launches)
+{
+    if (!c) return set_err(HC_ERR_ARG, "null context");
+    if (kernel < 0 || kernel >= HC_KERNEL_COUNT) return set_err(HC_ERR_ARG, "unknown kernel id %d", kernel);
+    HIPCHK(hipStreamSynchronize(c->stream));
+    for (size_t q = 0; q < c->ev_kid.size(); ++q) {
+        float ms = 0.0f;
+        HIPCHK(hipEventElapsedTime(&ms, c->ev_pool[2 * q], c->ev_pool[2 * q + 1]));
+        c->prof_ms[c->ev_kid[q]] += ms;
+        c->prof_n[c->ev_kid[q]] += 1;
+    }
+    c->ev_kid.clear();
+    c->ev_used = 0;
+    if (total_ms) *total_ms = c->prof_ms[kernel];
+    if (launches) *launches = c->prof_n[kernel];
+    return 0;
+}
+
+extern "C" int hc_project(hc_ctx *c)
+{
+    if (!c) return set_err(HC_ERR_ARG, "null context");
+    size_t plane = (size_t)c->nx * c->ny;
+    hipLaunchKernelGGL(k_zero_level, dim3((unsigned)((plane + 255) / 256)), dim3(256), 0, c->stream, c->w,
+                       (size_t)(c->nz - 1) * plane, plane);
+    if (int e = launch_r2c(c, c->u, c->s[0])) return e;
+    if (int e = launch_r2c(c, c->v, c->s[1])) return e;
+    if (int e = launch_r2c(c, c->w, c->s[2])) return e;
+    if (int e = launch_yfft(c, -1, 3, c->s[0], c->s[1], c->s[2], nullptr)) return e;
+    StepArgs A = base_args(c);
+    A.rc = 1.0;
+    dim3 gcol((c->nl + kColWaves - 1) / kColWaves, c->ny);
+#define HC_PJ(L_) hipLaunchKernelGGL((k_column_project<L_>), gcol, dim3(64 * kColWaves), 0, c->stream, A);
+    HC_DISPATCH_L(c->Lcol, HC_PJ)
+#undef HC_PJ
+    if (int e = launch_yfft(c, +1, 3, c->s[0], c->s[1], c->s[2], nullptr)) return e;
+    C2RArgs X;
+    std::memset(&X, 0, sizeof X);
+    X.src[0] = c->s[0]; X.src[1] = c->s[1]; X.src[2] = c->s[2];
+    X.dst[0] = c->u; X.dst[1] = c->v; X.dst[2] = c->w;
+    if (int e = launch_c2r(c, 3, X)) return e;
+    HIPCHK(hipGetLastError());
     HIPCHK(hipStreamSynchronize(c->stream));
     return 0;
 }

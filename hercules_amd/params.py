@@ -19,7 +19,8 @@ DBL_FIELDS = ("lx", "ly", "lz", "zstretch", "cdamp", "nu", "alpha", "got", "fc",
 
 class CParams(C.Structure):
     """ctypes image of ``hc_params`` (include/hercules_amd.h)."""
-    _fields_ = [(n, C.c_int) for n in INT_FIELDS] + [(n, C.c_double) for n in DBL_FIELDS]
+    _fields_ = ([(n, C.c_int) for n in INT_FIELDS] + [(n, C.c_double) for n in DBL_FIELDS] +
+                [("wall_rows", C.c_int)])
 
 
 @dataclass
@@ -75,6 +76,9 @@ class Params:
     isyz2d: int = 0
     yz2d_id: list = field(default_factory=list)
     intv_2d: int = 1000
+    # extension, no namelist counterpart: 1 gives every mode the implicit
+    # wall rows the reference gives the (0,0) mode only
+    wall_rows: int = 0
 
     def validate(self):
         """The checks of initiate_parameters (src/module_parameters.f90:256-270)."""
@@ -92,6 +96,7 @@ class Params:
             setattr(p, n, int(getattr(self, n)))
         for n in DBL_FIELDS:
             setattr(p, n, float(getattr(self, n)))
+        p.wall_rows = int(self.wall_rows)
         return p
 
     def as_dict(self):

@@ -43,6 +43,15 @@ typedef struct hc_params {
     int imax;                /* number of steps (used by the Ri ramp) */
     double lx, ly, lz, zstretch, cdamp;
     double nu, alpha, got, fc, ug, vg, dt, u_mrf, t_ref, tbot, ttop, ri_str, ri_end;
+    /* Wall rows of the implicit u, v, w and t systems (no namelist
+     * counterpart).  0: as the reference builds them — get_{u,w,t}_eqn_coeff
+     * zero the 1-D aa(k)/cc(k) while visiting the first mode, so only the
+     * (kx,ky) = (0,0) mode gets the -aa/-cc wall term on its diagonal
+     * (src/module_poisson_solver.f90:257-272,361-366,455-469).  1: every mode
+     * gets the (0,0) mode's rows, which keeps Crank-Nicolson consistent with
+     * the explicit half at the wall; 0 is unstable when
+     * rk_c*dt*nu/(2 dz_min^2) > 1. */
+    int wall_rows;
 } hc_params;
 
 typedef struct hc_ctx hc_ctx;
@@ -104,6 +113,13 @@ int hc_drive_steps(hc_ctx *ctx, int nsteps);
 int hc_poisson_solver_fft(hc_ctx *ctx, int equation, double rk_c, const double *r_src,
                           double *r_var, int layout);
 
+/* Make the current (u, v, w) discretely divergence-free with the scheme's
+ * own operators: phi = L^{-1} div u, u -= G phi, w = 0 on the walls.  No
+ * counterpart in the reference, which reads prepared initial fields; the
+ * operators are those of get_p_eqn_src, poisson_solver_fft (p equation,
+ * rk_c = 1) and calculate_uvw.  Used to build seeded initial states. */
+int hc_project(hc_ctx *ctx);
+
 /* calculate_ubar (src/module_parameters.f90:513-546) on the current u. */
 int hc_calculate_ubar(hc_ctx *ctx, double *ubar);
 
@@ -122,6 +138,30 @@ double *hc_field_device_ptr(hc_ctx *ctx, int field);
 
 /* Wait for all queued work of the context. */
 int hc_synchronize(hc_ctx *ctx);
+
+/* Kernel timing (no counterpart in the reference, which only prints CPU
+ * time per step, src/module_io.f90:72-75).  While enabled, every launch of
+ * the kernels below is bracketed by HIP events on the context stream;
+ * hc_profile_read synchronises and returns the accumulated milliseconds and
+ * launch count of one kernel since hc_profile_enable. */
+enum hc_kernel {
+    HC_K_RHS = 0,        /* momentum right-hand side + x R2C FFT      */
+    HC_K_YFFT_FWD = 1,   /* forward y FFT of the three q spectra       */
+    HC_K_COLUMN = 2,     /* per-column solves, projection, dp          */
+    HC_K_YFFT_INV = 3,   /* inverse y FFT of u, v, w, dp               */
+    HC_K_FINISH = 4,     /* x C2R FFT, store u, v, w, accumulate p     */
+    HC_K_RHS_T = 5,      /* temperature chain ...                      */
+    HC_K_YFFT_T = 6,
+    HC_K_COLUMN_T = 7,
+    HC_K_FINISH_T = 8,
+    HC_K_SOLVE_XR2C = 9, /* standalone poisson_solver_fft chain ...    */
+    HC_K_SOLVE_YFFT = 10,
+    HC_K_SOLVE_COLUMN = 11,
+    HC_K_SOLVE_XC2R = 12,
+    HC_KERNEL_COUNT = 13
+};
+int hc_profile_enable(hc_ctx *ctx, int on);
+int hc_profile_read(hc_ctx *ctx, int kernel, double *total_ms, long *launches);
 
 #ifdef __cplusplus
 }

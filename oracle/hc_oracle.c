@@ -37,6 +37,7 @@ typedef struct ho_params {
     int ochannel, dp_opt, cds, dts, issk, isdamp, nzdamp, isscalar, is_ri_var, imax;
     double lx, ly, lz, zstretch, cdamp;
     double nu, alpha, got, fc, ug, vg, dt, u_mrf, t_ref, tbot, ttop, ri_str, ri_end;
+    int wall_rows; /* 1: every mode gets the wall rows the reference gives mode (0,0) only */
 } ho_params;
 
 typedef struct ho_state {
@@ -254,6 +255,24 @@ static cplx kx_pois(const ho_state *S, cplx w1, int l, double d2)
     return 0;
 }
 
+/* kx(l), ky(m) of the u/w/t equations, evaluated once per wavenumber
+ * (src/module_poisson_solver.f90:222-246 evaluate the same expressions for
+ * every level) */
+static void helm_symbols(const ho_state *S, double coef, cplx *kxv, cplx *kyv)
+{
+    const ho_params *P = &S->prm;
+    for (int i = 1; i <= P->nx; ++i) {
+        int l = lwave(i, P->nx);
+        kxv[i - 1] = P->cds == 3 ? 0.5 * P->dt * coef * (-pow(2.0 * PI / P->lx * l, 2.0))
+                                 : kx_helm(S, S->wx1, l, S->dx2, coef);
+    }
+    for (int j = 1; j <= P->ny; ++j) {
+        int m = lwave(j, P->ny);
+        kyv[j - 1] = P->cds == 3 ? 0.5 * P->dt * coef * (-pow(2.0 * PI / P->ly * m, 2.0))
+                                 : kx_helm(S, S->wy1, m, S->dy2, coef);
+    }
+}
+
 /* ---- A8: equation coefficients ----------------------------------------- */
 /* get_p_eqn_coeff, src/module_poisson_solver.f90:32-122 */
 static void coeff_p(ho_state *S)
@@ -261,26 +280,30 @@ static void coeff_p(ho_state *S)
     const ho_params *P = &S->prm;
     for (int k = 1; k <= S->nka; ++k) { V1(S->aa_p, k) = 0; V1(S->cc_p, k) = 0; }
     memset(S->bb_p, 0, sizeof(cplx) * (size_t)P->nx * P->ny * S->nka);
+    /* kx(l), ky(m) do not depend on k; the reference re-evaluates the same
+     * expressions for every level, here they are evaluated once */
+    cplx *kxv = calloc_c(P->nx), *kyv = calloc_c(P->ny);
+    for (int i = 1; i <= P->nx; ++i) {
+        int l = lwave(i, P->nx);
+        kxv[i - 1] = P->cds == 3 ? -pow(2.0 * PI / P->lx * l, 2.0) : kx_pois(S, S->wx1, l, S->dx2);
+    }
+    for (int j = 1; j <= P->ny; ++j) {
+        int m = lwave(j, P->ny);
+        kyv[j - 1] = P->cds == 3 ? -pow(2.0 * PI / P->ly * m, 2.0) : kx_pois(S, S->wy1, m, S->dy2);
+    }
     for (int k = S->kstr3; k <= S->kend3; ++k) {
         V1(S->aa_p, k) = 1.0 / V1(S->dz, k) / V1(S->dz_b, k);
         V1(S->cc_p, k) = 1.0 / V1(S->dz, k) / V1(S->dz_t, k);
         for (int j = 1; j <= P->ny; ++j)
             for (int i = 1; i <= P->nx; ++i) {
-                int l = lwave(i, P->nx), m = lwave(j, P->ny);
-                cplx kx, ky;
-                if (P->cds == 3) {
-                    kx = -pow(2.0 * PI / P->lx * l, 2.0);
-                    ky = -pow(2.0 * PI / P->ly * m, 2.0);
-                } else {
-                    kx = kx_pois(S, S->wx1, l, S->dx2);
-                    ky = kx_pois(S, S->wy1, m, S->dy2);
-                }
+                cplx kx = kxv[i - 1], ky = kyv[j - 1];
                 cplx *b = &B3(S, S->bb_p, i, j, k);
                 *b = kx + ky - V1(S->aa_p, k) - V1(S->cc_p, k);
                 if (k == S->kstr3) { *b += V1(S->aa_p, k); V1(S->aa_p, k) = 0.0; }
                 if (k == S->kend3) { *b += V1(S->cc_p, k); V1(S->cc_p, k) = 0.0; }
             }
     }
+    free(kxv); free(kyv);
 }
 
 /* get_u_eqn_coeff (src/module_poisson_solver.f90:181-282), get_t_eqn_coeff
@@ -292,21 +315,25 @@ static void coeff_uvt(ho_state *S, double *aa, cplx *bb, double *cc, double coef
     const ho_params *P = &S->prm;
     for (int k = 1; k <= S->nka; ++k) { V1(aa, k) = 0; V1(cc, k) = 0; }
     memset(bb, 0, sizeof(cplx) * (size_t)P->nx * P->ny * S->nka);
+    cplx *kxv = calloc_c(P->nx), *kyv = calloc_c(P->ny);
+    helm_symbols(S, coef, kxv, kyv);
     for (int k = S->kstr3; k <= S->kend3; ++k) {
         V1(aa, k) = -0.5 * P->dt * coef / V1(S->dz, k) / V1(S->dz_b, k);
         V1(cc, k) = -0.5 * P->dt * coef / V1(S->dz, k) / V1(S->dz_t, k);
+        const double a0 = V1(aa, k), c0 = V1(cc, k);
         for (int j = 1; j <= P->ny; ++j)
             for (int i = 1; i <= P->nx; ++i) {
-                int l = lwave(i, P->nx), m = lwave(j, P->ny);
-                cplx kx, ky;
-                if (P->cds == 3) {
-                    kx = 0.5 * P->dt * coef * (-pow(2.0 * PI / P->lx * l, 2.0));
-                    ky = 0.5 * P->dt * coef * (-pow(2.0 * PI / P->ly * m, 2.0));
-                } else {
-                    kx = kx_helm(S, S->wx1, l, S->dx2, coef);
-                    ky = kx_helm(S, S->wy1, m, S->dy2, coef);
-                }
+                cplx kx = kxv[i - 1], ky = kyv[j - 1];
                 cplx *b = &B3(S, bb, i, j, k);
+                if (P->wall_rows == 1) {
+                    /* extension: the first mode's wall rows for every mode */
+                    *b = -kx - ky - a0 - c0;
+                    if (k == S->kstr3) *b -= a0;
+                    if (k == S->kend3) *b += (is_t || P->ochannel == 0) ? -c0 : c0;
+                    if (k == S->kstr3) V1(aa, k) = 0.0;
+                    if (k == S->kend3) V1(cc, k) = 0.0;
+                    continue;
+                }
                 *b = -kx - ky - V1(aa, k) - V1(cc, k);
                 if (k == S->kstr3) { *b -= V1(aa, k); V1(aa, k) = 0.0; }
                 if (k == S->kend3) {
@@ -315,6 +342,7 @@ static void coeff_uvt(ho_state *S, double *aa, cplx *bb, double *cc, double coef
                 }
             }
     }
+    free(kxv); free(kyv);
 }
 
 /* get_w_eqn_coeff, src/module_poisson_solver.f90:287-374 */
@@ -325,25 +353,21 @@ static void coeff_w(ho_state *S)
     cplx *bb = S->bb_w;
     for (int k = 1; k <= S->nka; ++k) { V1(aa, k) = 0; V1(cc, k) = 0; }
     memset(bb, 0, sizeof(cplx) * (size_t)P->nx * P->ny * S->nka);
+    cplx *kxv = calloc_c(P->nx), *kyv = calloc_c(P->ny);
+    helm_symbols(S, P->nu, kxv, kyv);
     for (int k = S->kstr3; k <= S->kend3 - 1; ++k) {
         V1(aa, k) = -0.5 * P->dt * P->nu / V1(S->dz, k) / V1(S->dz_t, k);
         V1(cc, k) = -0.5 * P->dt * P->nu / V1(S->dz, k + 1) / V1(S->dz_t, k);
+        const double a0 = V1(aa, k), c0 = V1(cc, k);
         for (int j = 1; j <= P->ny; ++j)
             for (int i = 1; i <= P->nx; ++i) {
-                int l = lwave(i, P->nx), m = lwave(j, P->ny);
-                cplx kx, ky;
-                if (P->cds == 3) {
-                    kx = 0.5 * P->dt * P->nu * (-pow(2.0 * PI / P->lx * l, 2.0));
-                    ky = 0.5 * P->dt * P->nu * (-pow(2.0 * PI / P->ly * m, 2.0));
-                } else {
-                    kx = kx_helm(S, S->wx1, l, S->dx2, P->nu);
-                    ky = kx_helm(S, S->wy1, m, S->dy2, P->nu);
-                }
-                B3(S, bb, i, j, k) = -kx - ky - V1(aa, k) - V1(cc, k);
+                cplx kx = kxv[i - 1], ky = kyv[j - 1];
+                B3(S, bb, i, j, k) = P->wall_rows == 1 ? -kx - ky - a0 - c0 : -kx - ky - V1(aa, k) - V1(cc, k);
                 if (k == S->kstr3) V1(aa, k) = 0.0;
                 if (k == S->kend3 - 1) V1(cc, k) = 0.0;
             }
     }
+    free(kxv); free(kyv);
 }
 
 void ho_coefficients(ho_state *S)
@@ -1039,6 +1063,12 @@ int ho_steps(ho_state *S, int nsteps)
 }
 
 /* ---- accessors for the tests -------------------------------------------- */
+#ifdef _OPENMP
+#include <omp.h>
+int ho_num_threads(void) { return omp_get_max_threads(); }
+#else
+int ho_num_threads(void) { return 1; }
+#endif
 const char *ho_error(ho_state *S) { return S->err; }
 
 static double *field_ptr(ho_state *S, int id)

@@ -27,7 +27,8 @@ _DBL_FIELDS = ("lx", "ly", "lz", "zstretch", "cdamp", "nu", "alpha", "got", "fc"
 
 
 class OParams(C.Structure):
-    _fields_ = [(n, C.c_int) for n in _INT_FIELDS] + [(n, C.c_double) for n in _DBL_FIELDS]
+    _fields_ = ([(n, C.c_int) for n in _INT_FIELDS] + [(n, C.c_double) for n in _DBL_FIELDS] +
+                [("wall_rows", C.c_int)])
 
 
 FIELD_IDS = {"u": 0, "v": 1, "w": 2, "p": 3, "t": 4, "phi": 5, "ff": 6, "gg": 7, "hh": 8,
@@ -86,6 +87,7 @@ def make_params(**kw) -> OParams:
         setattr(p, n, int(kw.get(n, 0)))
     for n in _DBL_FIELDS:
         setattr(p, n, float(kw.get(n, 0.0)))
+    p.wall_rows = int(kw.get("wall_rows", 0))
     return p
 
 

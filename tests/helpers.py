@@ -7,7 +7,7 @@ from hercules_amd.params import Params
 ORACLE_KEYS = ("nx", "ny", "nz", "ochannel", "dp_opt", "cds", "dts", "issk", "isdamp", "nzdamp",
                "isscalar", "is_ri_var", "imax", "lx", "ly", "lz", "zstretch", "cdamp", "nu",
                "alpha", "got", "fc", "ug", "vg", "dt", "u_mrf", "t_ref", "tbot", "ttop",
-               "ri_str", "ri_end")
+               "ri_str", "ri_end", "wall_rows")
 
 
 def oracle_kwargs(prm: Params):
@@ -65,10 +65,32 @@ def poisson_residual(prm: Params, rk_c, phi, src):
     ph = np.concatenate([phi[:1], phi, phi[-1:]], 0)  # Neumann ghosts
     lz = ((ph[2:] - ph[1:-1]) / dzt - (ph[1:-1] - ph[:-2]) / dzb) / dz
     lap = d(g(phi, 2, dx), 2, dx) + d(g(phi, 1, dy), 1, dy) + lz
-    s = np.array(src, dtype=np.float64)
-    s = s - 0.0
+    # mpi_2d_fft zeroes the Nyquist wavenumbers of the source (src/module_tools.f90:659-693)
+    S = np.fft.fft2(np.array(src, dtype=np.float64))
+    S[:, prm.ny // 2, :] = 0.0
+    S[:, :, prm.nx // 2] = 0.0
+    s = np.fft.ifft2(S).real
     # delnull removes the (0,0) mode of level 1 and pins the level-1 mean of phi
     res = rk_c * lap - s
     res[0] -= res[0].mean()
-    scale = np.abs(s).max() + np.abs(rk_c * lap).max()
+    # backward-error scale: |L|_inf |phi|_inf + |s|_inf
+    lnorm = 2.0 * (1.0 / (dz * dzb) + 1.0 / (dz * dzt)).max() + 6.0 / dx ** 2 + 6.0 / dy ** 2
+    scale = abs(rk_c) * lnorm * np.abs(phi).max() + np.abs(s).max()
     return np.abs(res).max() / scale
+
+
+def divergence(u, v, w, prm: Params):
+    """Discrete divergence of screen_div_error (src/module_io.f90:104-128)
+    and, per cell, the sum of the magnitudes of its three terms."""
+    from hercules_amd import grid
+    dz = grid.metrics(prm.nz, prm.lz, prm.zstretch, prm.ochannel)["dz"][:, None, None]
+    dx, dy = prm.lx / prm.nx, prm.ly / prm.ny
+    if prm.cds == 1:
+        dudx = (u - np.roll(u, 1, 2)) / dx
+        dvdy = (v - np.roll(v, 1, 1)) / dy
+    else:
+        dudx = (-np.roll(u, -1, 2) + 27 * u - 27 * np.roll(u, 1, 2) + np.roll(u, 2, 2)) / 24 / dx
+        dvdy = (-np.roll(v, -1, 1) + 27 * v - 27 * np.roll(v, 1, 1) + np.roll(v, 2, 1)) / 24 / dy
+    wm = np.concatenate([np.zeros_like(w[:1]), w[:-1]], 0)
+    dwdz = (w - wm) / dz
+    return dudx + dvdy + dwdz, np.abs(dudx) + np.abs(dvdy) + np.abs(dwdz)
