@@ -55,38 +55,63 @@ constexpr int kRowPoints = 1024;  // points of one block of the row kernels
 template <int NX>
 constexpr int rows_per_block() { return NX >= kRowPoints ? 1 : kRowPoints / NX; }
 
-// Momentum RHS + x R2C of qu,qv,qw.  Block: rows j0..j0+ROWS-1 of level k.
-template <int CDS, int NX>
-__global__ __launch_bounds__(kRhsThreads) void k_rhs_mom(StepArgs A)
+// Momentum RHS + x R2C of qu,qv,qw.  Block: full rows j0..j0+TY-1 of level
+// k, one point per thread; every row the stencils reach is staged in LDS
+// once (coalesced), then the same LDS holds the q rows for the FFT.
+template <int NX>
+constexpr int rhs_ty()
 {
-    constexpr int ROWS = rows_per_block<NX>();
-    constexpr int NL = NX / 2;
-    __shared__ double2 sq_[3 * ROWS * NL];
-    double *sqd = reinterpret_cast<double *>(sq_);
-    const int k = blockIdx.y, j0 = blockIdx.x * ROWS, tid = threadIdx.x;
+    return NX == 16 ? 16 : NX == 32 ? 32 : NX == 64 ? 16 : NX == 128 ? 8 : NX == 192 ? 4 : NX == 256 ? 4 : 1;
+}
+
+template <int NX>
+constexpr int rhs_threads() { return NX * rhs_ty<NX>() <= 512 ? NX * rhs_ty<NX>() : NX * rhs_ty<NX>() / 2; }
+
+template <int CDS, int NX>
+__global__ __launch_bounds__(rhs_threads<NX>()) void k_rhs_mom(StepArgs A)
+{
+    constexpr int TY = rhs_ty<NX>();
+    constexpr int T = rhs_threads<NX>(), PP = NX * TY / T, NL = NX / 2;
+    using Tl = RhsTile<NX, TY>;
+    static_assert(Tl::NROWS * NX >= 3 * TY * NX, "FFT rows must fit in the staging area");
+    __shared__ double sm[Tl::NROWS * NX];
+    const int tid = threadIdx.x;
+    const int nty = A.ny / TY;
+    const int logical = xcd_remap(blockIdx.x, nty * A.nz);
+    const int jt = logical % nty, k = logical / nty, j0 = jt * TY;
+    for (int q = tid; q < Tl::NROWS * NX; q += T) {
+        int r = q / NX, i = q - r * NX;
+        sm[q] = rhs_stage_value<NX, TY>(A, r, i, j0, k);
+    }
     const double dpx = A.dp_opt == 2 ? A.scal[0] : A.dpx;
-    for (int p = tid; p < ROWS * NX; p += kRhsThreads) {
-        int jj = p / NX, i = p - jj * NX, j = j0 + jj;
-        double qu = 0.0, qv = 0.0, qw = 0.0;
-        if (j < A.ny) {
-            Pt q{A, i, j, k};
-            double f, g, h;
-            momentum_point<CDS>(q, dpx, f, g, h, qu, qv, qw);
-            size_t c = ((size_t)k * A.ny + j) * A.nx + i;
-            A.ff[c] = f;
-            A.gg[c] = g;
-            A.hh[c] = h;
-        }
-        sqd[(0 * ROWS + jj) * NX + i] = qu;
-        sqd[(1 * ROWS + jj) * NX + i] = qv;
-        sqd[(2 * ROWS + jj) * NX + i] = qw;
+    __syncthreads();
+    double qs[PP][3];
+#pragma unroll
+    for (int pp = 0; pp < PP; ++pp) {
+        const int pt = tid + pp * T, i = pt % NX, jj = pt / NX, j = j0 + jj;
+        Ld<NX, TY> q(A, sm, i, jj, j, k);
+        double f, g, h;
+        momentum_point<CDS>(q, dpx, f, g, h, qs[pp][0], qs[pp][1], qs[pp][2]);
+        const size_t c = ((size_t)k * A.ny + j) * NX + i;
+        A.ff[c] = f;
+        A.gg[c] = g;
+        A.hh[c] = h;
+    }
+    __syncthreads();  // staging no longer read: reuse it for the q rows
+#pragma unroll
+    for (int pp = 0; pp < PP; ++pp) {
+        const int pt = tid + pp * T, i = pt % NX, jj = pt / NX;
+        sm[(0 * TY + jj) * NX + i] = qs[pp][0];
+        sm[(1 * TY + jj) * NX + i] = qs[pp][1];
+        sm[(2 * TY + jj) * NX + i] = qs[pp][2];
     }
     __syncthreads();
-    lds_fft<NL, 3 * ROWS, NL, 1, kRhsThreads, -1>(sq_, A.tw_nl, tid);
+    double2 *sq_ = reinterpret_cast<double2 *>(sm);
+    lds_fft<NL, 3 * TY, NL, 1, T, -1>(sq_, A.tw_nl, tid);
     double2 *outs[3] = {A.s0, A.s1, A.s2};
-    for (int p = tid; p < 3 * ROWS * NL; p += kRhsThreads) {
-        int row = p / NL, l = p - row * NL, f = row / ROWS, jj = row - f * ROWS, j = j0 + jj;
-        if (j < A.ny) outs[f][((size_t)k * A.ny + j) * NL + l] = r2c_post<NL>(sq_ + row * NL, l, A.tw_nx, A.norm);
+    for (int p = tid; p < 3 * TY * NL; p += T) {
+        int row = p / NL, l = p - row * NL, fi = row / TY, r = row - fi * TY;
+        outs[fi][((size_t)k * A.ny + j0 + r) * NL + l] = r2c_post<NL>(sq_ + row * NL, l, A.tw_nx, A.norm);
     }
 }
 
@@ -946,9 +971,14 @@ static int substep(hc_ctx *c, int t_its)
     int rows = rows_for(c->nx);
     dim3 grow((c->ny + rows - 1) / rows, c->nz);
     PROF_BEGIN(c, HC_K_RHS);
-#define HC_RHS(NX_)                                                                                 \
-    if (P.cds == 1) hipLaunchKernelGGL((k_rhs_mom<1, NX_>), grow, dim3(kRhsThreads), 0, c->stream, A); \
-    else hipLaunchKernelGGL((k_rhs_mom<2, NX_>), grow, dim3(kRhsThreads), 0, c->stream, A);
+#define HC_RHS(NX_)                                                                                   \
+    {                                                                                                  \
+        constexpr int TY_ = rhs_ty<NX_>();                                                             \
+        dim3 g1((unsigned)(c->ny / TY_ * c->nz));                                                      \
+        constexpr int T_ = rhs_threads<NX_>();                                                         \
+        if (P.cds == 1) hipLaunchKernelGGL((k_rhs_mom<1, NX_>), g1, dim3(T_), 0, c->stream, A);        \
+        else hipLaunchKernelGGL((k_rhs_mom<2, NX_>), g1, dim3(T_), 0, c->stream, A);                   \
+    }
     HC_DISPATCH_N(c->nx, HC_RHS)
 #undef HC_RHS
     PROF_END(c, HC_K_RHS);
@@ -1031,6 +1061,10 @@ extern "C" int hc_create(const hc_params *prm, hc_ctx **out)
     if (P.isdamp != 0) return set_err(HC_ERR_UNSUPPORTED, "isdamp=1 (Rayleigh damping) is not implemented");
     if (P.wall_rows != 0 && P.wall_rows != 1) return set_err(HC_ERR_ARG, "wall_rows must be 0 or 1");
     if (!fft_len_ok(P.nx) || !fft_len_ok(P.ny)) return set_err(HC_ERR_UNSUPPORTED, "nx, ny must be one of 16,32,64,128,192,256,512");
+    {
+        int ty = P.nx == 16 ? 16 : P.nx == 32 ? 32 : P.nx == 64 ? 16 : P.nx == 128 ? 8 : P.nx == 192 ? 4 : P.nx == 256 ? 4 : 1;
+        if (P.ny % ty != 0) return set_err(HC_ERR_UNSUPPORTED, "ny=%d must be a multiple of %d for nx=%d", P.ny, ty, P.nx);
+    }
     int L = col_levels(P.nz);
     if (L < 0) return set_err(HC_ERR_UNSUPPORTED, "nz=%d too large", P.nz);
     hc_ctx *c = new hc_ctx();

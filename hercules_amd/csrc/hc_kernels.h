@@ -59,9 +59,98 @@ HC_DEV double sq(double x) { return x * x; }
 // ---------------------------------------------------------------------------
 // Point accessors with the reference's ghost rules
 // (src/module_boundary.f90:33-196) and periodic x/y wrap (comm_bound).
-struct Pt {
+// Mid-point interpolants, src/module_tools.f90:410-512, with the ghost
+// overrides of update_boundary_uvwhxyz / _thxyz (src/module_boundary.f90:202-355),
+// written once over the raw field accessors U, V, W, T of the derived class.
+template <class D>
+struct Interp {
+    HC_DEV const D &d() const { return static_cast<const D &>(*this); }
+    template <int CDS> HC_DEV double uhx(int di, int dj, int dk) const
+    {
+        const D &q = d();
+        if (CDS == 1) return 0.5 * (q.U(di, dj, dk) + q.U(di - 1, dj, dk));
+        return (-q.U(di + 1, dj, dk) + 9.0 * q.U(di, dj, dk) + 9.0 * q.U(di - 1, dj, dk) - q.U(di - 2, dj, dk)) / 16.0;
+    }
+    template <int CDS> HC_DEV double uhy(int di, int dj, int dk) const
+    {
+        const D &q = d();
+        if (CDS == 1) return 0.5 * (q.U(di, dj + 1, dk) + q.U(di, dj, dk));
+        return (-q.U(di, dj + 2, dk) + 9.0 * q.U(di, dj + 1, dk) + 9.0 * q.U(di, dj, dk) - q.U(di, dj - 1, dk)) / 16.0;
+    }
+    template <int CDS> HC_DEV double vhx(int di, int dj, int dk) const
+    {
+        const D &q = d();
+        if (CDS == 1) return 0.5 * (q.V(di + 1, dj, dk) + q.V(di, dj, dk));
+        return (-q.V(di + 2, dj, dk) + 9.0 * q.V(di + 1, dj, dk) + 9.0 * q.V(di, dj, dk) - q.V(di - 1, dj, dk)) / 16.0;
+    }
+    template <int CDS> HC_DEV double vhy(int di, int dj, int dk) const
+    {
+        const D &q = d();
+        if (CDS == 1) return 0.5 * (q.V(di, dj, dk) + q.V(di, dj - 1, dk));
+        return (-q.V(di, dj + 1, dk) + 9.0 * q.V(di, dj, dk) + 9.0 * q.V(di, dj - 1, dk) - q.V(di, dj - 2, dk)) / 16.0;
+    }
+    template <int CDS> HC_DEV double whx(int di, int dj, int dk) const
+    {
+        const D &q = d();
+        int kk = q.k + dk;
+        if (kk < 0 || kk == q.A.nz - 1) return 0.0;
+        if (CDS == 1) return 0.5 * (q.W(di + 1, dj, dk) + q.W(di, dj, dk));
+        return (-q.W(di + 2, dj, dk) + 9.0 * q.W(di + 1, dj, dk) + 9.0 * q.W(di, dj, dk) - q.W(di - 1, dj, dk)) / 16.0;
+    }
+    template <int CDS> HC_DEV double why(int di, int dj, int dk) const
+    {
+        const D &q = d();
+        int kk = q.k + dk;
+        if (kk < 0 || kk == q.A.nz - 1) return 0.0;
+        if (CDS == 1) return 0.5 * (q.W(di, dj + 1, dk) + q.W(di, dj, dk));
+        return (-q.W(di, dj + 2, dk) + 9.0 * q.W(di, dj + 1, dk) + 9.0 * q.W(di, dj, dk) - q.W(di, dj - 1, dk)) / 16.0;
+    }
+    HC_DEV double uhz(int di, int dj, int dk) const
+    {
+        const D &q = d();
+        int kk = q.k + dk;
+        if (kk < 0) return -q.A.u_mrf;
+        if (kk == q.A.nz - 1) return q.A.ochannel == 0 ? -q.A.u_mrf : q.U(di, dj, dk);
+        double lt = q.lev(LV_LT, kk);
+        return q.U(di, dj, dk + 1) * lt + q.U(di, dj, dk) * (1.0 - lt);
+    }
+    HC_DEV double vhz(int di, int dj, int dk) const
+    {
+        const D &q = d();
+        int kk = q.k + dk;
+        if (kk < 0) return 0.0;
+        if (kk == q.A.nz - 1) return q.A.ochannel == 0 ? 0.0 : q.V(di, dj, dk);
+        double lt = q.lev(LV_LT, kk);
+        return q.V(di, dj, dk + 1) * lt + q.V(di, dj, dk) * (1.0 - lt);
+    }
+    template <int CDS> HC_DEV double thx(int di, int dj, int dk) const
+    {
+        const D &q = d();
+        if (CDS == 1) return 0.5 * (q.T(di + 1, dj, dk) + q.T(di, dj, dk));
+        return (-q.T(di + 2, dj, dk) + 9.0 * q.T(di + 1, dj, dk) + 9.0 * q.T(di, dj, dk) - q.T(di - 1, dj, dk)) / 16.0;
+    }
+    template <int CDS> HC_DEV double thy(int di, int dj, int dk) const
+    {
+        const D &q = d();
+        if (CDS == 1) return 0.5 * (q.T(di, dj + 1, dk) + q.T(di, dj, dk));
+        return (-q.T(di, dj + 2, dk) + 9.0 * q.T(di, dj + 1, dk) + 9.0 * q.T(di, dj, dk) - q.T(di, dj - 1, dk)) / 16.0;
+    }
+    HC_DEV double thz(int dk) const
+    {
+        const D &q = d();
+        int kk = q.k + dk;
+        if (kk < 0) return q.A.tbot;
+        if (kk == q.A.nz - 1) return q.A.ttop;
+        double lt = q.lev(LV_LT, kk);
+        return q.T(0, 0, dk + 1) * lt + q.T(0, 0, dk) * (1.0 - lt);
+    }
+};
+
+// Global-memory accessor: any offset, ghost values by the wall rules.
+struct Pt : Interp<Pt> {
     const StepArgs &A;
     int i, j, k;
+    HC_DEV Pt(const StepArgs &a, int i_, int j_, int k_) : A(a), i(i_), j(j_), k(k_) {}
     HC_DEV size_t at(int di, int dj, int kk) const
     {
         return ((size_t)kk * A.ny + wrapi(j + dj, A.ny)) * A.nx + wrapi(i + di, A.nx);
@@ -107,115 +196,45 @@ struct Pt {
         return A.t[at(di, dj, kk)];
     }
     HC_DEV double lev(int tab, int kk) const { return A.lev[tab * A.nz + kk]; }
-
-    // mid-point interpolants, src/module_tools.f90:410-512, with the ghost
-    // overrides of update_boundary_uvwhxyz / _thxyz (src/module_boundary.f90:202-355)
-    template <int CDS> HC_DEV double uhx(int di, int dj, int dk) const
-    {
-        if (CDS == 1) return 0.5 * (U(di, dj, dk) + U(di - 1, dj, dk));
-        return (-U(di + 1, dj, dk) + 9.0 * U(di, dj, dk) + 9.0 * U(di - 1, dj, dk) - U(di - 2, dj, dk)) / 16.0;
-    }
-    template <int CDS> HC_DEV double uhy(int di, int dj, int dk) const
-    {
-        if (CDS == 1) return 0.5 * (U(di, dj + 1, dk) + U(di, dj, dk));
-        return (-U(di, dj + 2, dk) + 9.0 * U(di, dj + 1, dk) + 9.0 * U(di, dj, dk) - U(di, dj - 1, dk)) / 16.0;
-    }
-    template <int CDS> HC_DEV double vhx(int di, int dj, int dk) const
-    {
-        if (CDS == 1) return 0.5 * (V(di + 1, dj, dk) + V(di, dj, dk));
-        return (-V(di + 2, dj, dk) + 9.0 * V(di + 1, dj, dk) + 9.0 * V(di, dj, dk) - V(di - 1, dj, dk)) / 16.0;
-    }
-    template <int CDS> HC_DEV double vhy(int di, int dj, int dk) const
-    {
-        if (CDS == 1) return 0.5 * (V(di, dj, dk) + V(di, dj - 1, dk));
-        return (-V(di, dj + 1, dk) + 9.0 * V(di, dj, dk) + 9.0 * V(di, dj - 1, dk) - V(di, dj - 2, dk)) / 16.0;
-    }
-    template <int CDS> HC_DEV double whx(int di, int dj, int dk) const
-    {
-        int kk = k + dk;
-        if (kk < 0 || kk == A.nz - 1) return 0.0;
-        if (CDS == 1) return 0.5 * (W(di + 1, dj, dk) + W(di, dj, dk));
-        return (-W(di + 2, dj, dk) + 9.0 * W(di + 1, dj, dk) + 9.0 * W(di, dj, dk) - W(di - 1, dj, dk)) / 16.0;
-    }
-    template <int CDS> HC_DEV double why(int di, int dj, int dk) const
-    {
-        int kk = k + dk;
-        if (kk < 0 || kk == A.nz - 1) return 0.0;
-        if (CDS == 1) return 0.5 * (W(di, dj + 1, dk) + W(di, dj, dk));
-        return (-W(di, dj + 2, dk) + 9.0 * W(di, dj + 1, dk) + 9.0 * W(di, dj, dk) - W(di, dj - 1, dk)) / 16.0;
-    }
-    HC_DEV double uhz(int di, int dj, int dk) const
-    {
-        int kk = k + dk;
-        if (kk < 0) return -A.u_mrf;
-        if (kk == A.nz - 1) return A.ochannel == 0 ? -A.u_mrf : U(di, dj, dk);
-        double lt = lev(LV_LT, kk);
-        return U(di, dj, dk + 1) * lt + U(di, dj, dk) * (1.0 - lt);
-    }
-    HC_DEV double vhz(int di, int dj, int dk) const
-    {
-        int kk = k + dk;
-        if (kk < 0) return 0.0;
-        if (kk == A.nz - 1) return A.ochannel == 0 ? 0.0 : V(di, dj, dk);
-        double lt = lev(LV_LT, kk);
-        return V(di, dj, dk + 1) * lt + V(di, dj, dk) * (1.0 - lt);
-    }
-    template <int CDS> HC_DEV double thx(int di, int dj, int dk) const
-    {
-        if (CDS == 1) return 0.5 * (T(di + 1, dj, dk) + T(di, dj, dk));
-        return (-T(di + 2, dj, dk) + 9.0 * T(di + 1, dj, dk) + 9.0 * T(di, dj, dk) - T(di - 1, dj, dk)) / 16.0;
-    }
-    template <int CDS> HC_DEV double thy(int di, int dj, int dk) const
-    {
-        if (CDS == 1) return 0.5 * (T(di, dj + 1, dk) + T(di, dj, dk));
-        return (-T(di, dj + 2, dk) + 9.0 * T(di, dj + 1, dk) + 9.0 * T(di, dj, dk) - T(di, dj - 1, dk)) / 16.0;
-    }
-    HC_DEV double thz(int dk) const
-    {
-        int kk = k + dk;
-        if (kk < 0) return A.tbot;
-        if (kk == A.nz - 1) return A.ttop;
-        double lt = lev(LV_LT, kk);
-        return T(0, 0, dk + 1) * lt + T(0, 0, dk) * (1.0 - lt);
-    }
+    HC_DEV size_t center() const { return at(0, 0, k); }
 };
 
 // ---------------------------------------------------------------------------
 // Momentum right-hand side at one point, get_momentum_cnvdiff
 // (src/module_cnvdiff.f90:35-374).  Returns ff,gg,hh and qu,qv,qw.
-template <int CDS>
-HC_DEV void momentum_point(const Pt &q, double dpx, double &f, double &g, double &h, double &qu, double &qv, double &qw)
+template <int CDS, class Q>
+HC_DEV void momentum_point(const Q &q, double dpx, double &f, double &g, double &h, double &qu, double &qv, double &qw)
 {
     const StepArgs &A = q.A;
     const double dx = A.dx, dy = A.dy, dx2 = A.dx2, dy2 = A.dy2, um = A.u_mrf;
     const int k = q.k;
     const double dz = q.lev(LV_DZ, k), dzt = q.lev(LV_DZT, k), dzb = q.lev(LV_DZB, k);
-    size_t c0 = q.at(0, 0, k);
+    size_t c0 = q.center();
     // ---- U equation (:55-161)
     {
         double uux, uvy, ux, uxx, uyy, dpdx;
         if (CDS == 1) {
-            uux = (sq(q.uhx<1>(1, 0, 0)) - sq(q.uhx<1>(0, 0, 0))) / dx;
-            uvy = (q.uhy<1>(0, 0, 0) * q.vhx<1>(0, 0, 0) - q.uhy<1>(0, -1, 0) * q.vhx<1>(0, -1, 0)) / dy;
-            ux = (q.uhx<1>(1, 0, 0) - q.uhx<1>(0, 0, 0)) / dx;
+            uux = (sq(q.template uhx<1>(1, 0, 0)) - sq(q.template uhx<1>(0, 0, 0))) / dx;
+            uvy = (q.template uhy<1>(0, 0, 0) * q.template vhx<1>(0, 0, 0) - q.template uhy<1>(0, -1, 0) * q.template vhx<1>(0, -1, 0)) / dy;
+            ux = (q.template uhx<1>(1, 0, 0) - q.template uhx<1>(0, 0, 0)) / dx;
             uxx = (q.U(1, 0, 0) - 2.0 * q.U(0, 0, 0) + q.U(-1, 0, 0)) / dx2;
             uyy = (q.U(0, 1, 0) - 2.0 * q.U(0, 0, 0) + q.U(0, -1, 0)) / dy2;
             dpdx = (q.P(1, 0, 0) - q.P(0, 0, 0)) / dx;
         } else {
-            double h2 = q.uhx<2>(2, 0, 0), h1 = q.uhx<2>(1, 0, 0), h0 = q.uhx<2>(0, 0, 0), hm = q.uhx<2>(-1, 0, 0);
+            double h2 = q.template uhx<2>(2, 0, 0), h1 = q.template uhx<2>(1, 0, 0), h0 = q.template uhx<2>(0, 0, 0), hm = q.template uhx<2>(-1, 0, 0);
             uux = (-sq(h2) + 27.0 * sq(h1) - 27.0 * sq(h0) + sq(hm)) / 24.0 / dx;
-            uvy = (-q.uhy<2>(0, 1, 0) * q.vhx<2>(0, 1, 0) + 27.0 * q.uhy<2>(0, 0, 0) * q.vhx<2>(0, 0, 0) -
-                   27.0 * q.uhy<2>(0, -1, 0) * q.vhx<2>(0, -1, 0) + q.uhy<2>(0, -2, 0) * q.vhx<2>(0, -2, 0)) / 24.0 / dy;
+            uvy = (-q.template uhy<2>(0, 1, 0) * q.template vhx<2>(0, 1, 0) + 27.0 * q.template uhy<2>(0, 0, 0) * q.template vhx<2>(0, 0, 0) -
+                   27.0 * q.template uhy<2>(0, -1, 0) * q.template vhx<2>(0, -1, 0) + q.template uhy<2>(0, -2, 0) * q.template vhx<2>(0, -2, 0)) / 24.0 / dy;
             ux = (-h2 + 27.0 * h1 - 27.0 * h0 + hm) / 24.0 / dx;
             uxx = (-q.U(2, 0, 0) + 16.0 * q.U(1, 0, 0) - 30.0 * q.U(0, 0, 0) + 16.0 * q.U(-1, 0, 0) - q.U(-2, 0, 0)) / 12.0 / dx2;
             uyy = (-q.U(0, 2, 0) + 16.0 * q.U(0, 1, 0) - 30.0 * q.U(0, 0, 0) + 16.0 * q.U(0, -1, 0) - q.U(0, -2, 0)) / 12.0 / dy2;
             dpdx = (-q.P(2, 0, 0) + 27.0 * q.P(1, 0, 0) - 27.0 * q.P(0, 0, 0) + q.P(-1, 0, 0)) / 24.0 / dx;
         }
-        double uwz = (q.uhz(0, 0, 0) * q.whx<CDS>(0, 0, 0) - q.uhz(0, 0, -1) * q.whx<CDS>(0, 0, -1)) / dz;
+        double uwz = (q.uhz(0, 0, 0) * q.template whx<CDS>(0, 0, 0) - q.uhz(0, 0, -1) * q.template whx<CDS>(0, 0, -1)) / dz;
         double u0 = q.U(0, 0, 0);
         double uzz = ((q.U(0, 0, 1) - u0) / dzt - (u0 - q.U(0, 0, -1)) / dzb) / dz;
         if (A.dp_opt == 3)
-            f = -uux - uvy - uwz - um * ux + A.fc * (0.5 * (q.vhx<CDS>(0, 0, 0) + q.vhx<CDS>(0, -1, 0)) - A.vg);
+            f = -uux - uvy - uwz - um * ux + A.fc * (0.5 * (q.template vhx<CDS>(0, 0, 0) + q.template vhx<CDS>(0, -1, 0)) - A.vg);
         else
             f = -uux - uvy - uwz + dpx - um * ux;
         double f0 = A.use_f0 ? A.ff0[c0] : 0.0;
@@ -225,28 +244,28 @@ HC_DEV void momentum_point(const Pt &q, double dpx, double &f, double &g, double
     {
         double vux, vvy, vx, vxx, vyy, dpdy;
         if (CDS == 1) {
-            vux = (q.vhx<1>(0, 0, 0) * q.uhy<1>(0, 0, 0) - q.vhx<1>(-1, 0, 0) * q.uhy<1>(-1, 0, 0)) / dx;
-            vvy = (sq(q.vhy<1>(0, 1, 0)) - sq(q.vhy<1>(0, 0, 0))) / dy;
-            vx = (q.vhx<1>(0, 0, 0) - q.vhx<1>(-1, 0, 0)) / dx;
+            vux = (q.template vhx<1>(0, 0, 0) * q.template uhy<1>(0, 0, 0) - q.template vhx<1>(-1, 0, 0) * q.template uhy<1>(-1, 0, 0)) / dx;
+            vvy = (sq(q.template vhy<1>(0, 1, 0)) - sq(q.template vhy<1>(0, 0, 0))) / dy;
+            vx = (q.template vhx<1>(0, 0, 0) - q.template vhx<1>(-1, 0, 0)) / dx;
             vxx = (q.V(1, 0, 0) - 2.0 * q.V(0, 0, 0) + q.V(-1, 0, 0)) / dx2;
             vyy = (q.V(0, 1, 0) - 2.0 * q.V(0, 0, 0) + q.V(0, -1, 0)) / dy2;
             dpdy = (q.P(0, 1, 0) - q.P(0, 0, 0)) / dy;
         } else {
-            double x1 = q.vhx<2>(1, 0, 0), x0 = q.vhx<2>(0, 0, 0), xm = q.vhx<2>(-1, 0, 0), xm2 = q.vhx<2>(-2, 0, 0);
-            vux = (-x1 * q.uhy<2>(1, 0, 0) + 27.0 * x0 * q.uhy<2>(0, 0, 0) - 27.0 * xm * q.uhy<2>(-1, 0, 0) +
-                   xm2 * q.uhy<2>(-2, 0, 0)) / 24.0 / dx;
-            vvy = (-sq(q.vhy<2>(0, 2, 0)) + 27.0 * sq(q.vhy<2>(0, 1, 0)) - 27.0 * sq(q.vhy<2>(0, 0, 0)) +
-                   sq(q.vhy<2>(0, -1, 0))) / 24.0 / dy;
+            double x1 = q.template vhx<2>(1, 0, 0), x0 = q.template vhx<2>(0, 0, 0), xm = q.template vhx<2>(-1, 0, 0), xm2 = q.template vhx<2>(-2, 0, 0);
+            vux = (-x1 * q.template uhy<2>(1, 0, 0) + 27.0 * x0 * q.template uhy<2>(0, 0, 0) - 27.0 * xm * q.template uhy<2>(-1, 0, 0) +
+                   xm2 * q.template uhy<2>(-2, 0, 0)) / 24.0 / dx;
+            vvy = (-sq(q.template vhy<2>(0, 2, 0)) + 27.0 * sq(q.template vhy<2>(0, 1, 0)) - 27.0 * sq(q.template vhy<2>(0, 0, 0)) +
+                   sq(q.template vhy<2>(0, -1, 0))) / 24.0 / dy;
             vx = (-x1 + 27.0 * x0 - 27.0 * xm + xm2) / 24.0 / dx;
             vxx = (-q.V(2, 0, 0) + 16.0 * q.V(1, 0, 0) - 30.0 * q.V(0, 0, 0) + 16.0 * q.V(-1, 0, 0) - q.V(-2, 0, 0)) / 12.0 / dx2;
             vyy = (-q.V(0, 2, 0) + 16.0 * q.V(0, 1, 0) - 30.0 * q.V(0, 0, 0) + 16.0 * q.V(0, -1, 0) - q.V(0, -2, 0)) / 12.0 / dy2;
             dpdy = (-q.P(0, 2, 0) + 27.0 * q.P(0, 1, 0) - 27.0 * q.P(0, 0, 0) + q.P(0, -1, 0)) / 24.0 / dy;
         }
-        double vwz = (q.vhz(0, 0, 0) * q.why<CDS>(0, 0, 0) - q.vhz(0, 0, -1) * q.why<CDS>(0, 0, -1)) / dz;
+        double vwz = (q.vhz(0, 0, 0) * q.template why<CDS>(0, 0, 0) - q.vhz(0, 0, -1) * q.template why<CDS>(0, 0, -1)) / dz;
         double v0 = q.V(0, 0, 0);
         double vzz = ((q.V(0, 0, 1) - v0) / dzt - (v0 - q.V(0, 0, -1)) / dzb) / dz;
         if (A.dp_opt == 3)
-            g = -vux - vvy - vwz - um * vx + A.fc * (A.ug - 0.5 * (q.uhy<CDS>(0, 0, 0) + q.uhy<CDS>(-1, 0, 0)) - um);
+            g = -vux - vvy - vwz - um * vx + A.fc * (A.ug - 0.5 * (q.template uhy<CDS>(0, 0, 0) + q.template uhy<CDS>(-1, 0, 0)) - um);
         else
             g = -vux - vvy - vwz - um * vx;
         double g0 = A.use_f0 ? A.gg0[c0] : 0.0;
@@ -256,16 +275,16 @@ HC_DEV void momentum_point(const Pt &q, double dpx, double &f, double &g, double
     if (k < A.nz - 1) {
         double wux, wvy, wx, wxx, wyy;
         if (CDS == 1) {
-            wux = (q.whx<1>(0, 0, 0) * q.uhz(0, 0, 0) - q.whx<1>(-1, 0, 0) * q.uhz(-1, 0, 0)) / dx;
-            wvy = (q.why<1>(0, 0, 0) * q.vhz(0, 0, 0) - q.why<1>(0, -1, 0) * q.vhz(0, -1, 0)) / dy;
-            wx = (q.whx<1>(0, 0, 0) - q.whx<1>(-1, 0, 0)) / dx;
+            wux = (q.template whx<1>(0, 0, 0) * q.uhz(0, 0, 0) - q.template whx<1>(-1, 0, 0) * q.uhz(-1, 0, 0)) / dx;
+            wvy = (q.template why<1>(0, 0, 0) * q.vhz(0, 0, 0) - q.template why<1>(0, -1, 0) * q.vhz(0, -1, 0)) / dy;
+            wx = (q.template whx<1>(0, 0, 0) - q.template whx<1>(-1, 0, 0)) / dx;
             wxx = (q.W(1, 0, 0) - 2.0 * q.W(0, 0, 0) + q.W(-1, 0, 0)) / dx2;
             wyy = (q.W(0, 1, 0) - 2.0 * q.W(0, 0, 0) + q.W(0, -1, 0)) / dy2;
         } else {
-            double x1 = q.whx<2>(1, 0, 0), x0 = q.whx<2>(0, 0, 0), xm = q.whx<2>(-1, 0, 0), xm2 = q.whx<2>(-2, 0, 0);
+            double x1 = q.template whx<2>(1, 0, 0), x0 = q.template whx<2>(0, 0, 0), xm = q.template whx<2>(-1, 0, 0), xm2 = q.template whx<2>(-2, 0, 0);
             wux = (-x1 * q.uhz(1, 0, 0) + 27.0 * x0 * q.uhz(0, 0, 0) - 27.0 * xm * q.uhz(-1, 0, 0) + xm2 * q.uhz(-2, 0, 0)) / 24.0 / dx;
-            wvy = (-q.why<2>(0, 1, 0) * q.vhz(0, 1, 0) + 27.0 * q.why<2>(0, 0, 0) * q.vhz(0, 0, 0) -
-                   27.0 * q.why<2>(0, -1, 0) * q.vhz(0, -1, 0) + q.why<2>(0, -2, 0) * q.vhz(0, -2, 0)) / 24.0 / dy;
+            wvy = (-q.template why<2>(0, 1, 0) * q.vhz(0, 1, 0) + 27.0 * q.template why<2>(0, 0, 0) * q.vhz(0, 0, 0) -
+                   27.0 * q.template why<2>(0, -1, 0) * q.vhz(0, -1, 0) + q.template why<2>(0, -2, 0) * q.vhz(0, -2, 0)) / 24.0 / dy;
             wx = (-x1 + 27.0 * x0 - 27.0 * xm + xm2) / 24.0 / dx;
             wxx = (-q.W(2, 0, 0) + 16.0 * q.W(1, 0, 0) - 30.0 * q.W(0, 0, 0) + 16.0 * q.W(-1, 0, 0) - q.W(-2, 0, 0)) / 12.0 / dx2;
             wyy = (-q.W(0, 2, 0) + 16.0 * q.W(0, 1, 0) - 30.0 * q.W(0, 0, 0) + 16.0 * q.W(0, -1, 0) - q.W(0, -2, 0)) / 12.0 / dy2;
@@ -291,24 +310,24 @@ HC_DEV void momentum_point(const Pt &q, double dpx, double &f, double &g, double
 
 // Temperature right-hand side at one point, get_temperature_cnvdiff
 // (src/module_cnvdiff.f90:381-473), evaluated with the updated u, v, w.
-template <int CDS>
-HC_DEV void temperature_point(const Pt &q, double &s, double &qt)
+template <int CDS, class Q>
+HC_DEV void temperature_point(const Q &q, double &s, double &qt)
 {
     const StepArgs &A = q.A;
     const int k = q.k;
     const double dx = A.dx, dy = A.dy;
     double tux, tvy, tx, txx, tyy;
     if (CDS == 1) {
-        tux = (q.thx<1>(0, 0, 0) * q.U(0, 0, 0) - q.thx<1>(-1, 0, 0) * q.U(-1, 0, 0)) / dx;
-        tvy = (q.thy<1>(0, 0, 0) * q.V(0, 0, 0) - q.thy<1>(0, -1, 0) * q.V(0, -1, 0)) / dy;
-        tx = (q.thx<1>(0, 0, 0) - q.thx<1>(-1, 0, 0)) / dx;
+        tux = (q.template thx<1>(0, 0, 0) * q.U(0, 0, 0) - q.template thx<1>(-1, 0, 0) * q.U(-1, 0, 0)) / dx;
+        tvy = (q.template thy<1>(0, 0, 0) * q.V(0, 0, 0) - q.template thy<1>(0, -1, 0) * q.V(0, -1, 0)) / dy;
+        tx = (q.template thx<1>(0, 0, 0) - q.template thx<1>(-1, 0, 0)) / dx;
         txx = (q.T(1, 0, 0) - 2.0 * q.T(0, 0, 0) + q.T(-1, 0, 0)) / A.dx2;
         tyy = (q.T(0, 1, 0) - 2.0 * q.T(0, 0, 0) + q.T(0, -1, 0)) / A.dy2;
     } else {
-        double x1 = q.thx<2>(1, 0, 0), x0 = q.thx<2>(0, 0, 0), xm = q.thx<2>(-1, 0, 0), xm2 = q.thx<2>(-2, 0, 0);
+        double x1 = q.template thx<2>(1, 0, 0), x0 = q.template thx<2>(0, 0, 0), xm = q.template thx<2>(-1, 0, 0), xm2 = q.template thx<2>(-2, 0, 0);
         tux = (-x1 * q.U(1, 0, 0) + 27.0 * x0 * q.U(0, 0, 0) - 27.0 * xm * q.U(-1, 0, 0) + xm2 * q.U(-2, 0, 0)) / 24.0 / dx;
-        tvy = (-q.thy<2>(0, 1, 0) * q.V(0, 1, 0) + 27.0 * q.thy<2>(0, 0, 0) * q.V(0, 0, 0) -
-               27.0 * q.thy<2>(0, -1, 0) * q.V(0, -1, 0) + q.thy<2>(0, -2, 0) * q.V(0, -2, 0)) / 24.0 / dy;
+        tvy = (-q.template thy<2>(0, 1, 0) * q.V(0, 1, 0) + 27.0 * q.template thy<2>(0, 0, 0) * q.V(0, 0, 0) -
+               27.0 * q.template thy<2>(0, -1, 0) * q.V(0, -1, 0) + q.template thy<2>(0, -2, 0) * q.V(0, -2, 0)) / 24.0 / dy;
         tx = (-x1 + 27.0 * x0 - 27.0 * xm + xm2) / 24.0 / dx;
         txx = (-q.T(2, 0, 0) + 16.0 * q.T(1, 0, 0) - 30.0 * q.T(0, 0, 0) + 16.0 * q.T(-1, 0, 0) - q.T(-2, 0, 0)) / 12.0 / A.dx2;
         tyy = (-q.T(0, 2, 0) + 16.0 * q.T(0, 1, 0) - 30.0 * q.T(0, 0, 0) + 16.0 * q.T(0, -1, 0) - q.T(0, -2, 0)) / 12.0 / A.dy2;
@@ -317,7 +336,7 @@ HC_DEV void temperature_point(const Pt &q, double &s, double &qt)
     double t0 = q.T(0, 0, 0);
     double tzz = ((q.T(0, 0, 1) - t0) / q.lev(LV_DZT, k) - (t0 - q.T(0, 0, -1)) / q.lev(LV_DZB, k)) / q.lev(LV_DZ, k);
     s = -tux - tvy - twz - A.u_mrf * tx;
-    double s0 = A.use_f0 ? A.ss0[q.at(0, 0, k)] : 0.0;
+    double s0 = A.use_f0 ? A.ss0[q.center()] : 0.0;
     qt = t0 + A.dt * (A.ra * s + A.rb * s0) + A.rc * 0.5 * A.dt * A.alpha * (txx + tyy + tzz);
 }
 
@@ -342,4 +361,111 @@ HC_DEV double2 c2r_pre(double2 xl, double2 xt, double2 twl)
     double2 a = cadd(xl, xt), b = csub(xl, xt);
     double2 e = cmul(cconj(twl), b);          // e^{+i theta} B
     return make_double2(a.x - e.y, a.y + e.x);  // A + i e
+}
+
+// ---------------------------------------------------------------------------
+// LDS tile of the momentum right-hand side: full x rows j0..j0+TY-1 of
+// level k and the halo rows each stencil reaches, staged once per block.
+template <int NX, int TY>
+struct RhsTile {
+    static constexpr int RU = TY + 6;   // u, v, w at k:   rows j0-3 .. j0+TY+2
+    static constexpr int RP = TY + 3;   // p at k:         rows j0-1 .. j0+TY+1
+    static constexpr int RWM = TY + 3;  // w at k-1:       rows j0-1 .. j0+TY+1
+    static constexpr int RVP = TY + 3;  // v at k+1:       rows j0-2 .. j0+TY
+    static constexpr int RUP = TY;      // u at k+1:       rows j0   .. j0+TY-1
+    static constexpr int OU = 0, OV = RU, OW = 2 * RU, OP = 3 * RU, OWM = OP + RP, OVP = OWM + RWM,
+                         OUP = OVP + RVP, NROWS = OUP + RUP;
+};
+
+template <int NX>
+HC_DEV int wrapx(int i)
+{
+    if constexpr ((NX & (NX - 1)) == 0) return i & (NX - 1);
+    else return i < 0 ? i + NX : (i >= NX ? i - NX : i);
+}
+
+// Value of staged row r (of RhsTile) at column i, with the wall rules of
+// update_boundary_uvw / _p (src/module_boundary.f90:33-162) for the ghost
+// levels k-1 = -1 and k+1 = nz.
+template <int NX, int TY>
+HC_DEV double rhs_stage_value(const StepArgs &A, int r, int i, int j0, int k)
+{
+    using Tl = RhsTile<NX, TY>;
+    const int ny = A.ny, nz = A.nz;
+    auto at = [&](int jj, int kk) -> size_t { return ((size_t)kk * ny + wrapi(jj, ny)) * NX + i; };
+    if (r < Tl::OV) return A.u[at(j0 - 3 + r, k)];
+    if (r < Tl::OW) return A.v[at(j0 - 3 + (r - Tl::OV), k)];
+    if (r < Tl::OP) return A.w[at(j0 - 3 + (r - Tl::OW), k)];
+    if (r < Tl::OWM) return A.p[at(j0 - 1 + (r - Tl::OP), k)];
+    if (r < Tl::OVP) return k == 0 ? 0.0 : A.w[at(j0 - 1 + (r - Tl::OWM), k - 1)];
+    if (r < Tl::OUP) {
+        int jj = j0 - 2 + (r - Tl::OVP);
+        if (k + 1 < nz) return A.v[at(jj, k + 1)];
+        double top = A.v[at(jj, nz - 1)];
+        return A.ochannel == 0 ? -top : top;
+    }
+    int jj = j0 + (r - Tl::OUP);
+    if (k + 1 < nz) return A.u[at(jj, k + 1)];
+    double top = A.u[at(jj, nz - 1)];
+    return A.ochannel == 0 ? -top - 2.0 * A.u_mrf : top;
+}
+
+// LDS accessor: the offsets momentum_point uses, read from the staged rows;
+// the single-point values at k-1 / k+1 come from registers.
+template <int NX, int TY>
+struct Ld : Interp<Ld<NX, TY>> {
+    using Tl = RhsTile<NX, TY>;
+    const StepArgs &A;
+    const double *sm;
+    int i, jj, j, k;
+    double u_km, v_km, w_kp, p_kp, t_k, t_kp;
+    HC_DEV Ld(const StepArgs &a, const double *s, int i_, int jj_, int j_, int k_)
+        : A(a), sm(s), i(i_), jj(jj_), j(j_), k(k_)
+    {
+        Pt g(a, i_, j_, k_);
+        u_km = g.U(0, 0, -1);
+        v_km = g.V(0, 0, -1);
+        const bool in = k_ + 1 < a.nz;
+        w_kp = in ? g.W(0, 0, 1) : 0.0;
+        p_kp = g.P(0, 0, 1);
+        t_k = a.isscalar ? g.T(0, 0, 0) : 0.0;
+        t_kp = (a.isscalar && in) ? g.T(0, 0, 1) : 0.0;
+    }
+    HC_DEV double row(int base, int r, int di) const { return sm[(base + r) * NX + wrapx<NX>(i + di)]; }
+    HC_DEV double U(int di, int dj, int dk) const
+    {
+        if (dk == 0) return row(Tl::OU, jj + 3 + dj, di);
+        if (dk > 0) return row(Tl::OUP, jj + dj, di);
+        return u_km;
+    }
+    HC_DEV double V(int di, int dj, int dk) const
+    {
+        if (dk == 0) return row(Tl::OV, jj + 3 + dj, di);
+        if (dk > 0) return row(Tl::OVP, jj + 2 + dj, di);
+        return v_km;
+    }
+    HC_DEV double W(int di, int dj, int dk) const
+    {
+        if (dk == 0) return row(Tl::OW, jj + 3 + dj, di);
+        if (dk < 0) return row(Tl::OWM, jj + 1 + dj, di);
+        return w_kp;
+    }
+    HC_DEV double P(int di, int dj, int dk) const
+    {
+        if (dk == 0) return row(Tl::OP, jj + 1 + dj, di);
+        return p_kp;
+    }
+    HC_DEV double T(int, int, int dk) const { return dk == 0 ? t_k : t_kp; }
+    HC_DEV double lev(int tab, int kk) const { return A.lev[tab * A.nz + kk]; }
+    HC_DEV size_t center() const { return ((size_t)k * A.ny + j) * NX + i; }
+};
+
+// Block id remap so that each of the 8 XCDs (which receive hardware block
+// ids round-robin) works through one contiguous range of logical tiles and
+// its L2 sees neighbouring rows and levels.  A permutation of [0, n).
+HC_DEV int xcd_remap(int b, int n)
+{
+    constexpr int X = 8;
+    int q = n / X, r = n % X, x = b % X, idx = b / X;
+    return (x < r ? x * (q + 1) : r * (q + 1) + (x - r) * q) + idx;
 }
