@@ -99,12 +99,35 @@ def cpu_baseline(prm, st, workload, steps):
             "seconds": dt}
 
 
+def copy_calibration(args):
+    """Plain device-to-device copy of one 256x256x257 FP64 field (torch's copy
+    kernel): the known-size reference for the FETCH_SIZE / WRITE_SIZE
+    counters and the attainable HBM bandwidth of a streaming kernel."""
+    import torch
+    n = 256 * 256 * 257
+    a = torch.randn(n, dtype=torch.float64, device="cuda")
+    b = torch.empty_like(a)
+    for _ in range(args.warmup):
+        b.copy_(a)
+    torch.cuda.synchronize()
+    e0, e1 = torch.cuda.Event(enable_timing=True), torch.cuda.Event(enable_timing=True)
+    e0.record()
+    for _ in range(args.steps):
+        b.copy_(a)
+    e1.record()
+    torch.cuda.synchronize()
+    ms = e0.elapsed_time(e1) / args.steps
+    gbs = 2 * n * 8 / (ms * 1e-3) / 1e9
+    print(json.dumps({"workload": "copy", "bytes_read": n * 8, "bytes_written": n * 8,
+                      "ms": round(ms, 4), "GBs": round(gbs, 1)}))
+
+
 def main():
     ap = argparse.ArgumentParser(description=__doc__.split("\n")[0])
     ap.add_argument("--gpus", type=int, default=1)
     ap.add_argument("--steps", type=int, default=50)
     ap.add_argument("--warmup", type=int, default=5)
-    ap.add_argument("--workload", default="fd", choices=["fd", "fd_t", "poisson"])
+    ap.add_argument("--workload", default="fd", choices=["fd", "fd_t", "poisson", "copy"])
     ap.add_argument("--cpu-steps", type=int, default=1, help="oracle steps for the CPU baseline")
     ap.add_argument("--no-cpu-baseline", action="store_true")
     ap.add_argument("--no-profile", action="store_true", help="skip the per-kernel event timing")
@@ -113,6 +136,9 @@ def main():
         print(f"bench.py: this build runs on one GPU; --gpus {args.gpus} is not supported "
               "(the channel is resident in one MI355X's HBM)", file=sys.stderr)
         sys.exit(2)
+
+    if args.workload == "copy":
+        return copy_calibration(args)
 
     from hercules_amd import Channel
 

@@ -40,8 +40,29 @@ def build_hip(verbose=False):
     return LIB
 
 
+FC = os.environ.get("HC_FC", shutil.which("amdflang") or "/opt/rocm/bin/amdflang")
+FDIR = os.path.join(PKG, "fortran")
+FEXE = os.path.join(FDIR, "hercules_hip")
+
+
+def build_fortran(verbose=False):
+    """The Fortran host: ISO_C_BINDING module + driver, linked to the HIP library."""
+    out = os.path.join(FDIR, "_mod")
+    os.makedirs(out, exist_ok=True)
+    mod = os.path.join(FDIR, "module_hercules_hip.f90")
+    main = os.path.join(FDIR, "hercules_hip_main.f90")
+    obj = os.path.join(out, "module_hercules_hip.o")
+    _run([FC, "-O2", "-c", mod, "-module-dir", out, "-o", obj])
+    _run([FC, "-O2", main, obj, "-module-dir", out, "-I", out, "-o", FEXE,
+          "-L" + PKG, "-lhercules_amd", "-Wl,-rpath," + PKG, "-Wl,-rpath,$ORIGIN/.."])
+    if verbose:
+        print("built", FEXE)
+    return FEXE
+
+
 def build_all(verbose=False):
     build_hip(verbose)
+    build_fortran(verbose)
 
 
 if __name__ == "__main__":

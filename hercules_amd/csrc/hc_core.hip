@@ -216,23 +216,26 @@ struct YArgs {
     int nl, nz;
     const double2 *tw_ny;
 };
+// LDS rows of the tile are padded by one complex so that the transform's
+// strided accesses spread over the banks.
+constexpr int kYPad = kYTile + 1;
 template <int NY, int SIGN>
 __global__ __launch_bounds__(256) void k_yfft(YArgs A)
 {
-    __shared__ double2 tile[NY * kYTile];
+    __shared__ double2 tile[NY * kYPad];
     double2 *fld = A.f[blockIdx.z];
     const int l0 = blockIdx.x * kYTile, k = blockIdx.y, tid = threadIdx.x;
     for (int p = tid; p < NY * kYTile; p += 256) {
         int j = p / kYTile, ll = p - j * kYTile;
         double2 v = fld[((size_t)k * NY + j) * A.nl + l0 + ll];
         if (SIGN > 0 && j == NY / 2) v = czero();
-        tile[p] = v;
+        tile[j * kYPad + ll] = v;
     }
     __syncthreads();
-    lds_fft<NY, kYTile, 1, kYTile, 256, SIGN>(tile, A.tw_ny, tid);
+    lds_fft<NY, kYTile, 1, kYPad, 256, SIGN>(tile, A.tw_ny, tid);
     for (int p = tid; p < NY * kYTile; p += 256) {
         int j = p / kYTile, ll = p - j * kYTile;
-        double2 v = tile[p];
+        double2 v = tile[j * kYPad + ll];
         if (SIGN < 0 && j == NY / 2) v = czero();
         fld[((size_t)k * NY + j) * A.nl + l0 + ll] = v;
     }
@@ -254,6 +257,44 @@ HC_DEV ColMap col_map(int nz)
     m.cnt = base + (m.lane < rem ? 1 : 0);
     m.k0 = m.lane * base + (m.lane < rem ? m.lane : rem);
     return m;
+}
+
+// The eight columns of a block pass through an LDS tile of nz rows x 8
+// complex (one 128-byte row per level) so that global loads and stores are
+// whole cache lines.  Element (k, w) sits in slot (w + k/4) mod 8 of row k:
+// the lanes of a wavefront, which own consecutive groups of ~4 levels,
+// then hit distinct bank groups.
+HC_DEV int col_tidx(int k, int w) { return k * kColWaves + ((w + (k >> 2)) & (kColWaves - 1)); }
+
+template <int L>
+HC_DEV void col_load(const double2 *__restrict__ src, double2 *tile, int nz, size_t plane, size_t col0,
+                     const ColMap &m, double2 (&d)[L])
+{
+    const int w = threadIdx.x >> 6;
+    for (int q = threadIdx.x; q < nz * kColWaves; q += 64 * kColWaves) {
+        int k = q / kColWaves, c = q - k * kColWaves;
+        tile[col_tidx(k, c)] = src[(size_t)k * plane + col0 + c];
+    }
+    __syncthreads();
+#pragma unroll
+    for (int r = 0; r < L; ++r) d[r] = r < m.cnt ? tile[col_tidx(m.k0 + r, w)] : czero();
+    __syncthreads();
+}
+
+template <int L>
+HC_DEV void col_store(double2 *__restrict__ dst, double2 *tile, int nz, size_t plane, size_t col0,
+                      const ColMap &m, const double2 (&d)[L])
+{
+    const int w = threadIdx.x >> 6;
+#pragma unroll
+    for (int r = 0; r < L; ++r)
+        if (r < m.cnt) tile[col_tidx(m.k0 + r, w)] = d[r];
+    __syncthreads();
+    for (int q = threadIdx.x; q < nz * kColWaves; q += 64 * kColWaves) {
+        int k = q / kColWaves, c = q - k * kColWaves;
+        dst[(size_t)k * plane + col0 + c] = tile[col_tidx(k, c)];
+    }
+    __syncthreads();
 }
 
 // value of level (k0+cnt-1) from register array v (runtime index)
@@ -354,11 +395,11 @@ HC_DEV void col_divergence(const StepArgs &A, const ColMap &m, int l, int jm, co
         if (r < m.cnt) {
             int k = m.k0 + r;
             double2 wm = r == 0 ? wprev : w[r > 0 ? r - 1 : 0];
-            double dz = A.lev[LV_DZ * nz + k];
+            double idz = A.lev[LV_IDZ * nz + k];
             double2 dv = cadd(cmul(sdx, u[r]), cmul(sdy, v[r]));
             double2 dwz = csub(w[r], wm);
-            dv.x += dwz.x / dz;
-            dv.y += dwz.y / dz;
+            dv.x += dwz.x * idz;
+            dv.y += dwz.y * idz;
             out[r] = cscale(dv, scale);
         }
     }
@@ -395,8 +436,8 @@ HC_DEV void col_project(const StepArgs &A, const ColMap &m, int l, int jm, doubl
             v[r] = csub(v[r], cscale(cmul(sgy, ph[r]), cdt));
             if (k < nz - 1) {
                 double2 gz = csub(col_next<L>(m, nz, ph, r, pnext), ph[r]);
-                double dzt = A.lev[LV_DZT * nz + k];
-                w[r] = make_double2(w[r].x - cdt * gz.x / dzt, w[r].y - cdt * gz.y / dzt);
+                double f = cdt * A.lev[LV_IDZT * nz + k];
+                w[r] = make_double2(w[r].x - f * gz.x, w[r].y - f * gz.y);
             }
         }
     }
@@ -405,23 +446,17 @@ HC_DEV void col_project(const StepArgs &A, const ColMap &m, int l, int jm, doubl
 template <int L>
 __global__ __launch_bounds__(64 * kColWaves) void k_column_mom(StepArgs A)
 {
+    // nl is a multiple of kColWaves: every wavefront owns a column
+    extern __shared__ double2 tile[];
     const int l = blockIdx.x * kColWaves + (threadIdx.x >> 6), jm = blockIdx.y;
-    if (l >= A.nl) return;  // whole wavefront leaves; no block barrier below
     const ColMap m = col_map(A.nz);
     const int nz = A.nz, nl = A.nl, ny = A.ny;
     const bool is00 = (l == 0 && jm == 0);
-    const size_t plane = (size_t)ny * nl, col = (size_t)jm * nl + l;
+    const size_t plane = (size_t)ny * nl, col0 = (size_t)jm * nl + (size_t)blockIdx.x * kColWaves;
     double2 duv[2][L], dw[1][L], dp[1][L];
-#pragma unroll
-    for (int r = 0; r < L; ++r) {
-        duv[0][r] = duv[1][r] = dw[0][r] = czero();
-        if (r < m.cnt) {
-            size_t o = (size_t)(m.k0 + r) * plane + col;
-            duv[0][r] = A.s0[o];
-            duv[1][r] = A.s1[o];
-            dw[0][r] = A.s2[o];
-        }
-    }
+    col_load<L>(A.s0, tile, nz, plane, col0, m, duv[0]);
+    col_load<L>(A.s1, tile, nz, plane, col0, m, duv[1]);
+    col_load<L>(A.s2, tile, nz, plane, col0, m, dw[0]);
     double a[L], b[L], c[L];
     // u and v share the matrix (assign_abc(aa_u,...) for both, :291-299)
     assemble<L>(A, 0, l, jm, is00, m, a, b, c);
@@ -434,7 +469,7 @@ __global__ __launch_bounds__(64 * kColWaves) void k_column_mom(StepArgs A)
     assemble<L>(A, 1, l, jm, is00, m, a, b, c);
     wave_tridiag<L, 1>(m.cnt, a, b, c, dw);
     // divergence source /dt (get_p_eqn_src, :127-176) in spectral form
-    col_divergence<L>(A, m, l, jm, duv[0], duv[1], dw[0], 1.0 / A.dt, dp[0]);
+    col_divergence<L>(A, m, l, jm, duv[0], duv[1], dw[0], A.idt, dp[0]);
     // pressure Poisson (:312-314)
     assemble<L>(A, 2, l, jm, is00, m, a, b, c);
     if (is00) fixups<L>(A, 2, m, dp[0]);
@@ -447,47 +482,45 @@ __global__ __launch_bounds__(64 * kColWaves) void k_column_mom(StepArgs A)
     double2 pprev = shfl_up_c(last_of<L>(dp[0], m.cnt), 1);
     double2 pnext = shfl_dn_c(dp[0][0], 1);
     const double cnu = A.rc * 0.5 * A.dt * A.nu;
+    double2 dpn[L];
 #pragma unroll
     for (int r = 0; r < L; ++r) {
+        dpn[r] = czero();
         if (r < m.cnt) {
             int k = m.k0 + r;
             double2 ph = dp[0][r];
             double2 pm = r == 0 ? (m.lane == 0 ? ph : pprev) : dp[0][r > 0 ? r - 1 : 0];
             double2 pn = col_next<L>(m, nz, dp[0], r, pnext);
-            double dzt = A.lev[LV_DZT * nz + k], dzb = A.lev[LV_DZB * nz + k], dz = A.lev[LV_DZ * nz + k];
+            double idzt = A.lev[LV_IDZT * nz + k], idzb = A.lev[LV_IDZB * nz + k], idz = A.lev[LV_IDZ * nz + k];
             double2 lz;
-            lz.x = ((pn.x - ph.x) / dzt - (ph.x - pm.x) / dzb) / dz;
-            lz.y = ((pn.y - ph.y) / dzt - (ph.y - pm.y) / dzb) / dz;
+            lz.x = ((pn.x - ph.x) * idzt - (ph.x - pm.x) * idzb) * idz;
+            lz.y = ((pn.y - ph.y) * idzt - (ph.y - pm.y) * idzb) * idz;
             double2 lap = make_double2(slap * ph.x + lz.x, slap * ph.y + lz.y);
-            // store
-            size_t o = (size_t)k * plane + col;
-            A.s0[o] = duv[0][r];
-            A.s1[o] = duv[1][r];
-            A.s2[o] = dw[0][r];
-            A.s3[o] = make_double2(ph.x - cnu * lap.x, ph.y - cnu * lap.y);
+            dpn[r] = make_double2(ph.x - cnu * lap.x, ph.y - cnu * lap.y);
         }
     }
+    col_store<L>(A.s0, tile, nz, plane, col0, m, duv[0]);
+    col_store<L>(A.s1, tile, nz, plane, col0, m, duv[1]);
+    col_store<L>(A.s2, tile, nz, plane, col0, m, dw[0]);
+    col_store<L>(A.s3, tile, nz, plane, col0, m, dpn);
 }
 
 // Temperature Helmholtz solve in place on s0 (:335-337).
 template <int L>
 __global__ __launch_bounds__(64 * kColWaves) void k_column_t(StepArgs A)
 {
+    extern __shared__ double2 tile[];
     const int l = blockIdx.x * kColWaves + (threadIdx.x >> 6), jm = blockIdx.y;
-    if (l >= A.nl) return;
     const ColMap m = col_map(A.nz);
     const bool is00 = (l == 0 && jm == 0);
-    const size_t plane = (size_t)A.ny * A.nl, col = (size_t)jm * A.nl + l;
+    const size_t plane = (size_t)A.ny * A.nl, col0 = (size_t)jm * A.nl + (size_t)blockIdx.x * kColWaves;
     double2 d[1][L];
-#pragma unroll
-    for (int r = 0; r < L; ++r) d[0][r] = r < m.cnt ? A.s0[(size_t)(m.k0 + r) * plane + col] : czero();
+    col_load<L>(A.s0, tile, A.nz, plane, col0, m, d[0]);
     double a[L], b[L], c[L];
     assemble<L>(A, 3, l, jm, is00, m, a, b, c);
     if (is00) fixups<L>(A, 3, m, d[0]);
     wave_tridiag<L, 1>(m.cnt, a, b, c, d);
-#pragma unroll
-    for (int r = 0; r < L; ++r)
-        if (r < m.cnt) A.s0[(size_t)(m.k0 + r) * plane + col] = d[0][r];
+    col_store<L>(A.s0, tile, A.nz, plane, col0, m, d[0]);
 }
 
 // Projection of (u, v, w) spectra in s0, s1, s2 onto discretely
@@ -497,36 +530,27 @@ __global__ __launch_bounds__(64 * kColWaves) void k_column_t(StepArgs A)
 template <int L>
 __global__ __launch_bounds__(64 * kColWaves) void k_column_project(StepArgs A)
 {
+    extern __shared__ double2 tile[];
     const int l = blockIdx.x * kColWaves + (threadIdx.x >> 6), jm = blockIdx.y;
-    if (l >= A.nl) return;
     const ColMap m = col_map(A.nz);
     const bool is00 = (l == 0 && jm == 0);
-    const size_t plane = (size_t)A.ny * A.nl, col = (size_t)jm * A.nl + l;
+    const size_t plane = (size_t)A.ny * A.nl, col0 = (size_t)jm * A.nl + (size_t)blockIdx.x * kColWaves;
     double2 u[1][L], v[1][L], w[1][L], ph[1][L];
+    col_load<L>(A.s0, tile, A.nz, plane, col0, m, u[0]);
+    col_load<L>(A.s1, tile, A.nz, plane, col0, m, v[0]);
+    col_load<L>(A.s2, tile, A.nz, plane, col0, m, w[0]);
 #pragma unroll
-    for (int r = 0; r < L; ++r) {
-        u[0][r] = v[0][r] = w[0][r] = czero();
-        if (r < m.cnt) {
-            size_t o = (size_t)(m.k0 + r) * plane + col;
-            u[0][r] = A.s0[o];
-            v[0][r] = A.s1[o];
-            w[0][r] = m.k0 + r == A.nz - 1 ? czero() : A.s2[o];
-        }
-    }
+    for (int r = 0; r < L; ++r)
+        if (r < m.cnt && m.k0 + r == A.nz - 1) w[0][r] = czero();
     col_divergence<L>(A, m, l, jm, u[0], v[0], w[0], 1.0, ph[0]);
     double a[L], b[L], c[L];
     assemble<L>(A, 2, l, jm, is00, m, a, b, c);
     if (is00) fixups<L>(A, 2, m, ph[0]);
     wave_tridiag_refined<L, 1>(m.cnt, a, b, c, ph);
     col_project<L>(A, m, l, jm, 1.0, ph[0], u[0], v[0], w[0]);
-#pragma unroll
-    for (int r = 0; r < L; ++r)
-        if (r < m.cnt) {
-            size_t o = (size_t)(m.k0 + r) * plane + col;
-            A.s0[o] = u[0][r];
-            A.s1[o] = v[0][r];
-            A.s2[o] = w[0][r];
-        }
+    col_store<L>(A.s0, tile, A.nz, plane, col0, m, u[0]);
+    col_store<L>(A.s1, tile, A.nz, plane, col0, m, v[0]);
+    col_store<L>(A.s2, tile, A.nz, plane, col0, m, w[0]);
 }
 
 // Standalone poisson_solver_fft for one equation, in place on s0.
@@ -534,26 +558,23 @@ __global__ __launch_bounds__(64 * kColWaves) void k_column_project(StepArgs A)
 template <int L>
 __global__ __launch_bounds__(64 * kColWaves) void k_column_solve(StepArgs A, int eq)
 {
+    extern __shared__ double2 tile[];
     const int l = blockIdx.x * kColWaves + (threadIdx.x >> 6), jm = blockIdx.y;
-    if (l >= A.nl) return;
     const ColMap m = col_map(A.nz);
     const bool is00 = (l == 0 && jm == 0);
-    const size_t plane = (size_t)A.ny * A.nl, col = (size_t)jm * A.nl + l;
+    const size_t plane = (size_t)A.ny * A.nl, col0 = (size_t)jm * A.nl + (size_t)blockIdx.x * kColWaves;
     double2 d[1][L];
+    col_load<L>(A.s0, tile, A.nz, plane, col0, m, d[0]);
 #pragma unroll
-    for (int r = 0; r < L; ++r) {
-        d[0][r] = r < m.cnt ? A.s0[(size_t)(m.k0 + r) * plane + col] : czero();
+    for (int r = 0; r < L; ++r)
         if (eq == 2 && r < m.cnt && m.k0 + r == A.nz - 1) d[0][r] = czero();
-    }
     const int meq = eq == 0 || eq == 1 ? 0 : (eq == 2 ? 1 : (eq == 3 ? 2 : 3));
     double a[L], b[L], c[L];
     assemble<L>(A, meq, l, jm, is00, m, a, b, c);
     if (is00 && eq != 1 && eq != 2) fixups<L>(A, meq, m, d[0]);
     if (meq == 2) wave_tridiag_refined<L, 1>(m.cnt, a, b, c, d);
     else wave_tridiag<L, 1>(m.cnt, a, b, c, d);
-#pragma unroll
-    for (int r = 0; r < L; ++r)
-        if (r < m.cnt) A.s0[(size_t)(m.k0 + r) * plane + col] = d[0][r];
+    col_store<L>(A.s0, tile, A.nz, plane, col0, m, d[0]);
 }
 
 // ---------------------------------------------------------------------------
@@ -781,6 +802,7 @@ static void build_levels(hc_ctx *c)
     for (int kk = 0; kk < nz; ++kk) {
         int k = kk + 3;
         at(LV_DZ, kk) = dz[k]; at(LV_DZT, kk) = dzt[k]; at(LV_DZB, kk) = dzb[k]; at(LV_LT, kk) = lt[k];
+        at(LV_IDZ, kk) = 1.0 / dz[k]; at(LV_IDZT, kk) = 1.0 / dzt[k]; at(LV_IDZB, kk) = 1.0 / dzb[k];
         // u (v): aa, cc of get_u_eqn_coeff (:200-201)
         double a = -0.5 * P.dt * P.nu / dz[k] / dzb[k], cc = -0.5 * P.dt * P.nu / dz[k] / dzt[k];
         at(LV_AU, kk) = kk == 0 ? 0.0 : a;
@@ -888,6 +910,10 @@ static StepArgs base_args(hc_ctx *c)
     A.dpx = 1.0;
     A.norm = 1.0 / ((double)c->nx * (double)c->ny);
     A.aU0 = c->aU0; A.cUe = c->cUe; A.aT0 = c->aT0; A.cTe = c->cTe;
+    A.c24x = 1.0 / (24.0 * c->dx); A.c24y = 1.0 / (24.0 * c->dy);
+    A.c12x2 = 1.0 / (12.0 * c->dx * c->dx); A.c12y2 = 1.0 / (12.0 * c->dy * c->dy);
+    A.idx = 1.0 / c->dx; A.idy = 1.0 / c->dy; A.idx2 = 1.0 / (c->dx * c->dx); A.idy2 = 1.0 / (c->dy * c->dy);
+    A.idt = 1.0 / P.dt;
     return A;
 }
 
@@ -915,6 +941,9 @@ static StepArgs base_args(hc_ctx *c)
     }
 
 static int rows_for(int nx) { return nx >= kRowPoints ? 1 : kRowPoints / nx; }
+
+// dynamic LDS of the column kernels: one nz x 8 complex tile
+static size_t col_lds(const hc_ctx *c) { return (size_t)c->nz * kColWaves * sizeof(double2); }
 
 static int launch_yfft(hc_ctx *c, int sign, int nf, double2 *f0, double2 *f1, double2 *f2, double2 *f3)
 {
@@ -987,7 +1016,7 @@ static int substep(hc_ctx *c, int t_its)
     PROF_END(c, HC_K_YFFT_FWD);
     dim3 gcol((c->nl + kColWaves - 1) / kColWaves, c->ny);
     PROF_BEGIN(c, HC_K_COLUMN);
-#define HC_COL(L_) hipLaunchKernelGGL((k_column_mom<L_>), gcol, dim3(64 * kColWaves), 0, c->stream, A);
+#define HC_COL(L_) hipLaunchKernelGGL((k_column_mom<L_>), gcol, dim3(64 * kColWaves), col_lds(c), c->stream, A);
     HC_DISPATCH_L(c->Lcol, HC_COL)
 #undef HC_COL
     PROF_END(c, HC_K_COLUMN);
@@ -1017,7 +1046,7 @@ static int substep(hc_ctx *c, int t_its)
         if (int e = launch_yfft(c, -1, 1, c->s[0], nullptr, nullptr, nullptr)) return e;
         PROF_END(c, HC_K_YFFT_T);
         PROF_BEGIN(c, HC_K_COLUMN_T);
-#define HC_COLT(L_) hipLaunchKernelGGL((k_column_t<L_>), gcol, dim3(64 * kColWaves), 0, c->stream, A);
+#define HC_COLT(L_) hipLaunchKernelGGL((k_column_t<L_>), gcol, dim3(64 * kColWaves), col_lds(c), c->stream, A);
         HC_DISPATCH_L(c->Lcol, HC_COLT)
 #undef HC_COLT
         PROF_END(c, HC_K_COLUMN_T);
@@ -1273,7 +1302,7 @@ extern "C" int hc_poisson_solver_fft(hc_ctx *c, int equation, double rk_c, const
     A.rc = c->prm.dts == 2 ? rk_c : 1.0;
     dim3 gcol((c->nl + kColWaves - 1) / kColWaves, c->ny);
     PROF_BEGIN(c, HC_K_SOLVE_COLUMN);
-#define HC_CS(L_) hipLaunchKernelGGL((k_column_solve<L_>), gcol, dim3(64 * kColWaves), 0, c->stream, A, equation);
+#define HC_CS(L_) hipLaunchKernelGGL((k_column_solve<L_>), gcol, dim3(64 * kColWaves), col_lds(c), c->stream, A, equation);
     HC_DISPATCH_L(c->Lcol, HC_CS)
 #undef HC_CS
     PROF_END(c, HC_K_SOLVE_COLUMN);
@@ -1403,7 +1432,7 @@ extern "C" int hc_project(hc_ctx *c)
     StepArgs A = base_args(c);
     A.rc = 1.0;
     dim3 gcol((c->nl + kColWaves - 1) / kColWaves, c->ny);
-#define HC_PJ(L_) hipLaunchKernelGGL((k_column_project<L_>), gcol, dim3(64 * kColWaves), 0, c->stream, A);
+#define HC_PJ(L_) hipLaunchKernelGGL((k_column_project<L_>), gcol, dim3(64 * kColWaves), col_lds(c), c->stream, A);
     HC_DISPATCH_L(c->Lcol, HC_PJ)
 #undef HC_PJ
     if (int e = launch_yfft(c, +1, 3, c->s[0], c->s[1], c->s[2], nullptr)) return e;

@@ -17,6 +17,19 @@ HC_DEV double2 czero() { return make_double2(0.0, 0.0); }
 // a - s*b for real s
 HC_DEV double2 caxpy_neg(double2 a, double s, double2 b) { return make_double2(a.x - s * b.x, a.y - s * b.y); }
 
+// 1/x for finite, normal, nonzero x: the hardware reciprocal estimate
+// (v_rcp_f64) refined by two Newton-Raphson steps r <- r + r(1 - x r),
+// which leaves it within an ulp or two of the correctly rounded value at a
+// third of the cost of an IEEE division.
+HC_DEV double rcp64(double x)
+{
+    double r = __builtin_amdgcn_rcp(x);
+    double e = __builtin_fma(-x, r, 1.0);
+    r = __builtin_fma(r, e, r);
+    e = __builtin_fma(-x, r, 1.0);
+    return __builtin_fma(r, e, r);
+}
+
 // ---------------------------------------------------------------------------
 // FFT plans: N = 4^a 2^b 3^c, stages in that order (radix 4 first).
 constexpr int fft_nstages(int n)
@@ -155,7 +168,7 @@ HC_DEV void wave_tridiag(int cnt, const double (&a)[L], const double (&b)[L], co
     for (int r = 1; r < L; ++r) {
         if (r < cnt) {
             double den = b[r] - (r > 1 ? a[r] * cp[r - 1] : 0.0);
-            double iv = 1.0 / den;
+            double iv = rcp64(den);
             cp[r] = c[r] * iv;
             double gr = r == 1 ? -a[1] : -a[r] * g[r - 1];
             g[r] = gr * iv;
@@ -243,7 +256,7 @@ HC_DEV void wave_tridiag(int cnt, const double (&a)[L], const double (&b)[L], co
 #pragma unroll
             for (int q = 0; q < NR; ++q) de_hi[q] = czero();
         }
-        double k1 = al / be_lo, k2 = ga / be_hi;
+        double k1 = al * rcp64(be_lo), k2 = ga * rcp64(be_hi);
         double nal = -al_lo * k1, nga = -ga_hi * k2;
         double nbe = be - ga_lo * k1 - al_hi * k2;
 #pragma unroll
@@ -253,7 +266,7 @@ HC_DEV void wave_tridiag(int cnt, const double (&a)[L], const double (&b)[L], co
         }
         al = nal; be = nbe; ga = nga;
     }
-    double ib = 1.0 / be;
+    double ib = rcp64(be);
     double2 X[NR], Xn[NR];
 #pragma unroll
     for (int q = 0; q < NR; ++q) {

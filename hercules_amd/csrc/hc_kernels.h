@@ -22,6 +22,7 @@ enum {
     LV_AW, LV_CW, LV_DW, LV_DW00,
     LV_AP, LV_CP, LV_DP, LV_DP00,
     LV_AT, LV_CT, LV_DT, LV_DT00,
+    LV_IDZ, LV_IDZT, LV_IDZB,  // 1/dz, 1/dz_t, 1/dz_b
     LV_COUNT
 };
 // per-wavenumber tables (x: nx/2 entries, y: ny entries), stacked
@@ -50,6 +51,8 @@ struct StepArgs {
     int ochannel, dp_opt, isscalar, use_f0;
     double dx, dy, dx2, dy2, dt, nu, alpha, got, fc, ug, vg, u_mrf, t_ref, tbot, ttop;
     double ra, rb, rc, dpx, norm;
+    // reciprocal stencil factors: 1/(24 dx), 1/(12 dx^2), 1/dx, 1/dx^2 (same in y), 1/dt
+    double c24x, c12x2, c24y, c12y2, idx, idx2, idy, idy2, idt;
     double aU0, cUe, aT0, cTe;  // unmodified wall coefficients for the (0,0)-mode fix-ups
 };
 
@@ -206,33 +209,33 @@ template <int CDS, class Q>
 HC_DEV void momentum_point(const Q &q, double dpx, double &f, double &g, double &h, double &qu, double &qv, double &qw)
 {
     const StepArgs &A = q.A;
-    const double dx = A.dx, dy = A.dy, dx2 = A.dx2, dy2 = A.dy2, um = A.u_mrf;
+    const double um = A.u_mrf;
     const int k = q.k;
-    const double dz = q.lev(LV_DZ, k), dzt = q.lev(LV_DZT, k), dzb = q.lev(LV_DZB, k);
+    const double idz = q.lev(LV_IDZ, k), idzt = q.lev(LV_IDZT, k), idzb = q.lev(LV_IDZB, k);
     size_t c0 = q.center();
     // ---- U equation (:55-161)
     {
         double uux, uvy, ux, uxx, uyy, dpdx;
         if (CDS == 1) {
-            uux = (sq(q.template uhx<1>(1, 0, 0)) - sq(q.template uhx<1>(0, 0, 0))) / dx;
-            uvy = (q.template uhy<1>(0, 0, 0) * q.template vhx<1>(0, 0, 0) - q.template uhy<1>(0, -1, 0) * q.template vhx<1>(0, -1, 0)) / dy;
-            ux = (q.template uhx<1>(1, 0, 0) - q.template uhx<1>(0, 0, 0)) / dx;
-            uxx = (q.U(1, 0, 0) - 2.0 * q.U(0, 0, 0) + q.U(-1, 0, 0)) / dx2;
-            uyy = (q.U(0, 1, 0) - 2.0 * q.U(0, 0, 0) + q.U(0, -1, 0)) / dy2;
-            dpdx = (q.P(1, 0, 0) - q.P(0, 0, 0)) / dx;
+            uux = (sq(q.template uhx<1>(1, 0, 0)) - sq(q.template uhx<1>(0, 0, 0))) * A.idx;
+            uvy = (q.template uhy<1>(0, 0, 0) * q.template vhx<1>(0, 0, 0) - q.template uhy<1>(0, -1, 0) * q.template vhx<1>(0, -1, 0)) * A.idy;
+            ux = (q.template uhx<1>(1, 0, 0) - q.template uhx<1>(0, 0, 0)) * A.idx;
+            uxx = (q.U(1, 0, 0) - 2.0 * q.U(0, 0, 0) + q.U(-1, 0, 0)) * A.idx2;
+            uyy = (q.U(0, 1, 0) - 2.0 * q.U(0, 0, 0) + q.U(0, -1, 0)) * A.idy2;
+            dpdx = (q.P(1, 0, 0) - q.P(0, 0, 0)) * A.idx;
         } else {
             double h2 = q.template uhx<2>(2, 0, 0), h1 = q.template uhx<2>(1, 0, 0), h0 = q.template uhx<2>(0, 0, 0), hm = q.template uhx<2>(-1, 0, 0);
-            uux = (-sq(h2) + 27.0 * sq(h1) - 27.0 * sq(h0) + sq(hm)) / 24.0 / dx;
+            uux = (-sq(h2) + 27.0 * sq(h1) - 27.0 * sq(h0) + sq(hm)) * A.c24x;
             uvy = (-q.template uhy<2>(0, 1, 0) * q.template vhx<2>(0, 1, 0) + 27.0 * q.template uhy<2>(0, 0, 0) * q.template vhx<2>(0, 0, 0) -
-                   27.0 * q.template uhy<2>(0, -1, 0) * q.template vhx<2>(0, -1, 0) + q.template uhy<2>(0, -2, 0) * q.template vhx<2>(0, -2, 0)) / 24.0 / dy;
-            ux = (-h2 + 27.0 * h1 - 27.0 * h0 + hm) / 24.0 / dx;
-            uxx = (-q.U(2, 0, 0) + 16.0 * q.U(1, 0, 0) - 30.0 * q.U(0, 0, 0) + 16.0 * q.U(-1, 0, 0) - q.U(-2, 0, 0)) / 12.0 / dx2;
-            uyy = (-q.U(0, 2, 0) + 16.0 * q.U(0, 1, 0) - 30.0 * q.U(0, 0, 0) + 16.0 * q.U(0, -1, 0) - q.U(0, -2, 0)) / 12.0 / dy2;
-            dpdx = (-q.P(2, 0, 0) + 27.0 * q.P(1, 0, 0) - 27.0 * q.P(0, 0, 0) + q.P(-1, 0, 0)) / 24.0 / dx;
+                   27.0 * q.template uhy<2>(0, -1, 0) * q.template vhx<2>(0, -1, 0) + q.template uhy<2>(0, -2, 0) * q.template vhx<2>(0, -2, 0)) * A.c24y;
+            ux = (-h2 + 27.0 * h1 - 27.0 * h0 + hm) * A.c24x;
+            uxx = (-q.U(2, 0, 0) + 16.0 * q.U(1, 0, 0) - 30.0 * q.U(0, 0, 0) + 16.0 * q.U(-1, 0, 0) - q.U(-2, 0, 0)) * A.c12x2;
+            uyy = (-q.U(0, 2, 0) + 16.0 * q.U(0, 1, 0) - 30.0 * q.U(0, 0, 0) + 16.0 * q.U(0, -1, 0) - q.U(0, -2, 0)) * A.c12y2;
+            dpdx = (-q.P(2, 0, 0) + 27.0 * q.P(1, 0, 0) - 27.0 * q.P(0, 0, 0) + q.P(-1, 0, 0)) * A.c24x;
         }
-        double uwz = (q.uhz(0, 0, 0) * q.template whx<CDS>(0, 0, 0) - q.uhz(0, 0, -1) * q.template whx<CDS>(0, 0, -1)) / dz;
+        double uwz = (q.uhz(0, 0, 0) * q.template whx<CDS>(0, 0, 0) - q.uhz(0, 0, -1) * q.template whx<CDS>(0, 0, -1)) * idz;
         double u0 = q.U(0, 0, 0);
-        double uzz = ((q.U(0, 0, 1) - u0) / dzt - (u0 - q.U(0, 0, -1)) / dzb) / dz;
+        double uzz = ((q.U(0, 0, 1) - u0) * idzt - (u0 - q.U(0, 0, -1)) * idzb) * idz;
         if (A.dp_opt == 3)
             f = -uux - uvy - uwz - um * ux + A.fc * (0.5 * (q.template vhx<CDS>(0, 0, 0) + q.template vhx<CDS>(0, -1, 0)) - A.vg);
         else
@@ -244,26 +247,26 @@ HC_DEV void momentum_point(const Q &q, double dpx, double &f, double &g, double 
     {
         double vux, vvy, vx, vxx, vyy, dpdy;
         if (CDS == 1) {
-            vux = (q.template vhx<1>(0, 0, 0) * q.template uhy<1>(0, 0, 0) - q.template vhx<1>(-1, 0, 0) * q.template uhy<1>(-1, 0, 0)) / dx;
-            vvy = (sq(q.template vhy<1>(0, 1, 0)) - sq(q.template vhy<1>(0, 0, 0))) / dy;
-            vx = (q.template vhx<1>(0, 0, 0) - q.template vhx<1>(-1, 0, 0)) / dx;
-            vxx = (q.V(1, 0, 0) - 2.0 * q.V(0, 0, 0) + q.V(-1, 0, 0)) / dx2;
-            vyy = (q.V(0, 1, 0) - 2.0 * q.V(0, 0, 0) + q.V(0, -1, 0)) / dy2;
-            dpdy = (q.P(0, 1, 0) - q.P(0, 0, 0)) / dy;
+            vux = (q.template vhx<1>(0, 0, 0) * q.template uhy<1>(0, 0, 0) - q.template vhx<1>(-1, 0, 0) * q.template uhy<1>(-1, 0, 0)) * A.idx;
+            vvy = (sq(q.template vhy<1>(0, 1, 0)) - sq(q.template vhy<1>(0, 0, 0))) * A.idy;
+            vx = (q.template vhx<1>(0, 0, 0) - q.template vhx<1>(-1, 0, 0)) * A.idx;
+            vxx = (q.V(1, 0, 0) - 2.0 * q.V(0, 0, 0) + q.V(-1, 0, 0)) * A.idx2;
+            vyy = (q.V(0, 1, 0) - 2.0 * q.V(0, 0, 0) + q.V(0, -1, 0)) * A.idy2;
+            dpdy = (q.P(0, 1, 0) - q.P(0, 0, 0)) * A.idy;
         } else {
             double x1 = q.template vhx<2>(1, 0, 0), x0 = q.template vhx<2>(0, 0, 0), xm = q.template vhx<2>(-1, 0, 0), xm2 = q.template vhx<2>(-2, 0, 0);
             vux = (-x1 * q.template uhy<2>(1, 0, 0) + 27.0 * x0 * q.template uhy<2>(0, 0, 0) - 27.0 * xm * q.template uhy<2>(-1, 0, 0) +
-                   xm2 * q.template uhy<2>(-2, 0, 0)) / 24.0 / dx;
+                   xm2 * q.template uhy<2>(-2, 0, 0)) * A.c24x;
             vvy = (-sq(q.template vhy<2>(0, 2, 0)) + 27.0 * sq(q.template vhy<2>(0, 1, 0)) - 27.0 * sq(q.template vhy<2>(0, 0, 0)) +
-                   sq(q.template vhy<2>(0, -1, 0))) / 24.0 / dy;
-            vx = (-x1 + 27.0 * x0 - 27.0 * xm + xm2) / 24.0 / dx;
-            vxx = (-q.V(2, 0, 0) + 16.0 * q.V(1, 0, 0) - 30.0 * q.V(0, 0, 0) + 16.0 * q.V(-1, 0, 0) - q.V(-2, 0, 0)) / 12.0 / dx2;
-            vyy = (-q.V(0, 2, 0) + 16.0 * q.V(0, 1, 0) - 30.0 * q.V(0, 0, 0) + 16.0 * q.V(0, -1, 0) - q.V(0, -2, 0)) / 12.0 / dy2;
-            dpdy = (-q.P(0, 2, 0) + 27.0 * q.P(0, 1, 0) - 27.0 * q.P(0, 0, 0) + q.P(0, -1, 0)) / 24.0 / dy;
+                   sq(q.template vhy<2>(0, -1, 0))) * A.c24y;
+            vx = (-x1 + 27.0 * x0 - 27.0 * xm + xm2) * A.c24x;
+            vxx = (-q.V(2, 0, 0) + 16.0 * q.V(1, 0, 0) - 30.0 * q.V(0, 0, 0) + 16.0 * q.V(-1, 0, 0) - q.V(-2, 0, 0)) * A.c12x2;
+            vyy = (-q.V(0, 2, 0) + 16.0 * q.V(0, 1, 0) - 30.0 * q.V(0, 0, 0) + 16.0 * q.V(0, -1, 0) - q.V(0, -2, 0)) * A.c12y2;
+            dpdy = (-q.P(0, 2, 0) + 27.0 * q.P(0, 1, 0) - 27.0 * q.P(0, 0, 0) + q.P(0, -1, 0)) * A.c24y;
         }
-        double vwz = (q.vhz(0, 0, 0) * q.template why<CDS>(0, 0, 0) - q.vhz(0, 0, -1) * q.template why<CDS>(0, 0, -1)) / dz;
+        double vwz = (q.vhz(0, 0, 0) * q.template why<CDS>(0, 0, 0) - q.vhz(0, 0, -1) * q.template why<CDS>(0, 0, -1)) * idz;
         double v0 = q.V(0, 0, 0);
-        double vzz = ((q.V(0, 0, 1) - v0) / dzt - (v0 - q.V(0, 0, -1)) / dzb) / dz;
+        double vzz = ((q.V(0, 0, 1) - v0) * idzt - (v0 - q.V(0, 0, -1)) * idzb) * idz;
         if (A.dp_opt == 3)
             g = -vux - vvy - vwz - um * vx + A.fc * (A.ug - 0.5 * (q.template uhy<CDS>(0, 0, 0) + q.template uhy<CDS>(-1, 0, 0)) - um);
         else
@@ -275,31 +278,31 @@ HC_DEV void momentum_point(const Q &q, double dpx, double &f, double &g, double 
     if (k < A.nz - 1) {
         double wux, wvy, wx, wxx, wyy;
         if (CDS == 1) {
-            wux = (q.template whx<1>(0, 0, 0) * q.uhz(0, 0, 0) - q.template whx<1>(-1, 0, 0) * q.uhz(-1, 0, 0)) / dx;
-            wvy = (q.template why<1>(0, 0, 0) * q.vhz(0, 0, 0) - q.template why<1>(0, -1, 0) * q.vhz(0, -1, 0)) / dy;
-            wx = (q.template whx<1>(0, 0, 0) - q.template whx<1>(-1, 0, 0)) / dx;
-            wxx = (q.W(1, 0, 0) - 2.0 * q.W(0, 0, 0) + q.W(-1, 0, 0)) / dx2;
-            wyy = (q.W(0, 1, 0) - 2.0 * q.W(0, 0, 0) + q.W(0, -1, 0)) / dy2;
+            wux = (q.template whx<1>(0, 0, 0) * q.uhz(0, 0, 0) - q.template whx<1>(-1, 0, 0) * q.uhz(-1, 0, 0)) * A.idx;
+            wvy = (q.template why<1>(0, 0, 0) * q.vhz(0, 0, 0) - q.template why<1>(0, -1, 0) * q.vhz(0, -1, 0)) * A.idy;
+            wx = (q.template whx<1>(0, 0, 0) - q.template whx<1>(-1, 0, 0)) * A.idx;
+            wxx = (q.W(1, 0, 0) - 2.0 * q.W(0, 0, 0) + q.W(-1, 0, 0)) * A.idx2;
+            wyy = (q.W(0, 1, 0) - 2.0 * q.W(0, 0, 0) + q.W(0, -1, 0)) * A.idy2;
         } else {
             double x1 = q.template whx<2>(1, 0, 0), x0 = q.template whx<2>(0, 0, 0), xm = q.template whx<2>(-1, 0, 0), xm2 = q.template whx<2>(-2, 0, 0);
-            wux = (-x1 * q.uhz(1, 0, 0) + 27.0 * x0 * q.uhz(0, 0, 0) - 27.0 * xm * q.uhz(-1, 0, 0) + xm2 * q.uhz(-2, 0, 0)) / 24.0 / dx;
+            wux = (-x1 * q.uhz(1, 0, 0) + 27.0 * x0 * q.uhz(0, 0, 0) - 27.0 * xm * q.uhz(-1, 0, 0) + xm2 * q.uhz(-2, 0, 0)) * A.c24x;
             wvy = (-q.template why<2>(0, 1, 0) * q.vhz(0, 1, 0) + 27.0 * q.template why<2>(0, 0, 0) * q.vhz(0, 0, 0) -
-                   27.0 * q.template why<2>(0, -1, 0) * q.vhz(0, -1, 0) + q.template why<2>(0, -2, 0) * q.vhz(0, -2, 0)) / 24.0 / dy;
-            wx = (-x1 + 27.0 * x0 - 27.0 * xm + xm2) / 24.0 / dx;
-            wxx = (-q.W(2, 0, 0) + 16.0 * q.W(1, 0, 0) - 30.0 * q.W(0, 0, 0) + 16.0 * q.W(-1, 0, 0) - q.W(-2, 0, 0)) / 12.0 / dx2;
-            wyy = (-q.W(0, 2, 0) + 16.0 * q.W(0, 1, 0) - 30.0 * q.W(0, 0, 0) + 16.0 * q.W(0, -1, 0) - q.W(0, -2, 0)) / 12.0 / dy2;
+                   27.0 * q.template why<2>(0, -1, 0) * q.vhz(0, -1, 0) + q.template why<2>(0, -2, 0) * q.vhz(0, -2, 0)) * A.c24y;
+            wx = (-x1 + 27.0 * x0 - 27.0 * xm + xm2) * A.c24x;
+            wxx = (-q.W(2, 0, 0) + 16.0 * q.W(1, 0, 0) - 30.0 * q.W(0, 0, 0) + 16.0 * q.W(-1, 0, 0) - q.W(-2, 0, 0)) * A.c12x2;
+            wyy = (-q.W(0, 2, 0) + 16.0 * q.W(0, 1, 0) - 30.0 * q.W(0, 0, 0) + 16.0 * q.W(0, -1, 0) - q.W(0, -2, 0)) * A.c12y2;
         }
         double w0 = q.W(0, 0, 0), wp = q.W(0, 0, 1), wm = q.W(0, 0, -1);
         double a1 = sq(0.5 * (w0 + wp)), a2 = sq(0.5 * (w0 + wm));
-        double wwz = (a1 - a2) / dzt;
-        double wzz = ((wp - w0) / q.lev(LV_DZ, k + 1) - (w0 - wm) / dz) / dzt;
+        double wwz = (a1 - a2) * idzt;
+        double wzz = ((wp - w0) * q.lev(LV_IDZ, k + 1) - (w0 - wm) * idz) * idzt;
         if (A.isscalar) {
             double lt = q.lev(LV_LT, k);
             h = -wux - wvy - wwz - um * wx + A.got * ((1.0 - lt) * q.T(0, 0, 0) + lt * q.T(0, 0, 1) - A.t_ref);
         } else {
             h = -wux - wvy - wwz - um * wx;
         }
-        double dpdz = (q.P(0, 0, 1) - q.P(0, 0, 0)) / dzt;
+        double dpdz = (q.P(0, 0, 1) - q.P(0, 0, 0)) * idzt;
         double h0 = A.use_f0 ? A.hh0[c0] : 0.0;
         qw = w0 + A.dt * (A.ra * h + A.rb * h0) + A.rc * 0.5 * A.dt * A.nu * (wxx + wyy + wzz) - A.rc * A.dt * dpdz;
     } else {
@@ -315,26 +318,25 @@ HC_DEV void temperature_point(const Q &q, double &s, double &qt)
 {
     const StepArgs &A = q.A;
     const int k = q.k;
-    const double dx = A.dx, dy = A.dy;
     double tux, tvy, tx, txx, tyy;
     if (CDS == 1) {
-        tux = (q.template thx<1>(0, 0, 0) * q.U(0, 0, 0) - q.template thx<1>(-1, 0, 0) * q.U(-1, 0, 0)) / dx;
-        tvy = (q.template thy<1>(0, 0, 0) * q.V(0, 0, 0) - q.template thy<1>(0, -1, 0) * q.V(0, -1, 0)) / dy;
-        tx = (q.template thx<1>(0, 0, 0) - q.template thx<1>(-1, 0, 0)) / dx;
-        txx = (q.T(1, 0, 0) - 2.0 * q.T(0, 0, 0) + q.T(-1, 0, 0)) / A.dx2;
-        tyy = (q.T(0, 1, 0) - 2.0 * q.T(0, 0, 0) + q.T(0, -1, 0)) / A.dy2;
+        tux = (q.template thx<1>(0, 0, 0) * q.U(0, 0, 0) - q.template thx<1>(-1, 0, 0) * q.U(-1, 0, 0)) * A.idx;
+        tvy = (q.template thy<1>(0, 0, 0) * q.V(0, 0, 0) - q.template thy<1>(0, -1, 0) * q.V(0, -1, 0)) * A.idy;
+        tx = (q.template thx<1>(0, 0, 0) - q.template thx<1>(-1, 0, 0)) * A.idx;
+        txx = (q.T(1, 0, 0) - 2.0 * q.T(0, 0, 0) + q.T(-1, 0, 0)) * A.idx2;
+        tyy = (q.T(0, 1, 0) - 2.0 * q.T(0, 0, 0) + q.T(0, -1, 0)) * A.idy2;
     } else {
         double x1 = q.template thx<2>(1, 0, 0), x0 = q.template thx<2>(0, 0, 0), xm = q.template thx<2>(-1, 0, 0), xm2 = q.template thx<2>(-2, 0, 0);
-        tux = (-x1 * q.U(1, 0, 0) + 27.0 * x0 * q.U(0, 0, 0) - 27.0 * xm * q.U(-1, 0, 0) + xm2 * q.U(-2, 0, 0)) / 24.0 / dx;
+        tux = (-x1 * q.U(1, 0, 0) + 27.0 * x0 * q.U(0, 0, 0) - 27.0 * xm * q.U(-1, 0, 0) + xm2 * q.U(-2, 0, 0)) * A.c24x;
         tvy = (-q.template thy<2>(0, 1, 0) * q.V(0, 1, 0) + 27.0 * q.template thy<2>(0, 0, 0) * q.V(0, 0, 0) -
-               27.0 * q.template thy<2>(0, -1, 0) * q.V(0, -1, 0) + q.template thy<2>(0, -2, 0) * q.V(0, -2, 0)) / 24.0 / dy;
-        tx = (-x1 + 27.0 * x0 - 27.0 * xm + xm2) / 24.0 / dx;
-        txx = (-q.T(2, 0, 0) + 16.0 * q.T(1, 0, 0) - 30.0 * q.T(0, 0, 0) + 16.0 * q.T(-1, 0, 0) - q.T(-2, 0, 0)) / 12.0 / A.dx2;
-        tyy = (-q.T(0, 2, 0) + 16.0 * q.T(0, 1, 0) - 30.0 * q.T(0, 0, 0) + 16.0 * q.T(0, -1, 0) - q.T(0, -2, 0)) / 12.0 / A.dy2;
+               27.0 * q.template thy<2>(0, -1, 0) * q.V(0, -1, 0) + q.template thy<2>(0, -2, 0) * q.V(0, -2, 0)) * A.c24y;
+        tx = (-x1 + 27.0 * x0 - 27.0 * xm + xm2) * A.c24x;
+        txx = (-q.T(2, 0, 0) + 16.0 * q.T(1, 0, 0) - 30.0 * q.T(0, 0, 0) + 16.0 * q.T(-1, 0, 0) - q.T(-2, 0, 0)) * A.c12x2;
+        tyy = (-q.T(0, 2, 0) + 16.0 * q.T(0, 1, 0) - 30.0 * q.T(0, 0, 0) + 16.0 * q.T(0, -1, 0) - q.T(0, -2, 0)) * A.c12y2;
     }
-    double twz = (q.thz(0) * q.W(0, 0, 0) - q.thz(-1) * q.W(0, 0, -1)) / q.lev(LV_DZ, k);
+    double twz = (q.thz(0) * q.W(0, 0, 0) - q.thz(-1) * q.W(0, 0, -1)) * q.lev(LV_IDZ, k);
     double t0 = q.T(0, 0, 0);
-    double tzz = ((q.T(0, 0, 1) - t0) / q.lev(LV_DZT, k) - (t0 - q.T(0, 0, -1)) / q.lev(LV_DZB, k)) / q.lev(LV_DZ, k);
+    double tzz = ((q.T(0, 0, 1) - t0) * q.lev(LV_IDZT, k) - (t0 - q.T(0, 0, -1)) * q.lev(LV_IDZB, k)) * q.lev(LV_IDZ, k);
     s = -tux - tvy - twz - A.u_mrf * tx;
     double s0 = A.use_f0 ? A.ss0[q.center()] : 0.0;
     qt = t0 + A.dt * (A.ra * s + A.rb * s0) + A.rc * 0.5 * A.dt * A.alpha * (txx + tyy + tzz);

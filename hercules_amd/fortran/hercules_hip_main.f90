@@ -1,0 +1,132 @@
+!*************************************************************************
+! Host driver for the MI355X HERCULES time step.
+!
+! Reads parameters.input with the reference's four namelists
+! (src/module_parameters.f90:145-157) and the initial fields
+! results/init_{u,v,w,p,t}.dat (raw FP64, global nx*ny*nz, Fortran order,
+! the 2DECOMP decomp_2d_write_one format read at src/module_tools.f90:273-286),
+! runs imax steps of the drive loop (src/module_navier_stokes.f90:139-210)
+! on the GPU and writes results/bak_{u,v,w,p,t}.dat every ibackup steps
+! (src/module_io.f90:311-360).  Time series, mean fields and slices of the
+! reference are not produced.
+!*************************************************************************
+program hercules_hip
+
+    use, intrinsic :: iso_c_binding
+    use module_hercules_hip
+    implicit none
+
+    ! namelist variables (same names, types and groups as the reference)
+    integer :: p_row, p_col, nx, ny, nz, ochannel
+    real(c_double) :: lx, ly, lz, zstretch
+    integer :: dp_opt, cds, issk, dts, isnoise, isdamp, nzdamp
+    real(c_double) :: noise_mag, cdamp
+    real(c_double) :: nu, alpha, got, fc, ug, vg, dt, u_mrf, t_ref, tbot, ttop, ri_str, ri_end
+    integer :: imax, isscalar, is_ri_var
+    integer :: restart, istmsr, ibackup, iinstfl, imeanfl, isxy2d, isxz2d, isyz2d, intv_2d
+    integer, dimension(50) :: xy2d_id, xz2d_id, yz2d_id
+
+    namelist /domain/ p_row, p_col, lx, ly, lz, nx, ny, nz, zstretch, ochannel
+    namelist /modeling/ dp_opt, cds, dts, isnoise, noise_mag, isdamp, nzdamp, cdamp, issk
+    namelist /constants/ nu, alpha, got, fc, ug, vg, dt, imax, u_mrf, isscalar, t_ref, &
+                         tbot, ttop, is_ri_var, ri_str, ri_end
+    namelist /io/ restart, istmsr, ibackup, iinstfl, imeanfl, &
+                  isxy2d, xy2d_id, isxz2d, xz2d_id, isyz2d, yz2d_id, intv_2d
+
+    type(hc_params) :: prm
+    real(c_double), allocatable, target :: f(:,:,:), g(:,:,:)
+    real(c_double) :: div_avg, div_max
+    integer :: stps, indx_bk, nsteps_env
+    character(len=32) :: arg
+
+    open(10, file='parameters.input')
+    read(10, nml=domain)
+    read(10, nml=modeling)
+    read(10, nml=constants)
+    read(10, nml=io)
+    close(10)
+
+    if (mod(nx, 2) .ne. 0 .or. mod(ny, 2) .ne. 0) then
+        print *, 'Please use even numbers for nx and ny! Quit!'
+        stop
+    endif
+    if (isnoise .ne. 0) then
+        print *, 'isnoise=1 is not supported: prepare seeded initial fields instead'
+        stop
+    endif
+
+    ! optional first argument: number of steps (defaults to imax)
+    nsteps_env = imax
+    if (command_argument_count() >= 1) then
+        call get_command_argument(1, arg)
+        read(arg, *) nsteps_env
+    endif
+
+    prm%nx = nx; prm%ny = ny; prm%nz = nz; prm%ochannel = ochannel; prm%dp_opt = dp_opt
+    prm%cds = cds; prm%dts = dts; prm%issk = issk; prm%isdamp = isdamp; prm%nzdamp = nzdamp
+    prm%isscalar = isscalar; prm%is_ri_var = is_ri_var; prm%imax = imax
+    prm%lx = lx; prm%ly = ly; prm%lz = lz; prm%zstretch = zstretch; prm%cdamp = cdamp
+    prm%nu = nu; prm%alpha = alpha; prm%got = got; prm%fc = fc; prm%ug = ug; prm%vg = vg
+    prm%dt = dt; prm%u_mrf = u_mrf; prm%t_ref = t_ref; prm%tbot = tbot; prm%ttop = ttop
+    prm%ri_str = ri_str; prm%ri_end = ri_end; prm%wall_rows = 0
+    call hip_create(prm)
+
+    allocate(f(nx + 4, ny + 4, nz + 4), g(nx, ny, nz))
+    f(:,:,:) = 0.d0
+    if (restart .eq. 1) then
+        print *, 'Reading Initial Fields.'
+        call read_one('results/init_u.dat', -u_mrf, HC_U)
+        call read_one('results/init_v.dat', 0.d0, HC_V)
+        call read_one('results/init_w.dat', 0.d0, HC_W)
+        call read_one('results/init_p.dat', 0.d0, HC_P)
+        call read_one('results/init_t.dat', 0.d0, HC_T)
+    endif
+    call hip_init()
+
+    indx_bk = 1
+    do stps = 1, nsteps_env
+        call hip_check(hc_drive_steps(hip_ctx, 1_c_int), 'drive step')
+        call hip_check(hc_screen_div_error(hip_ctx, div_avg, div_max), 'screen_div_error')
+        write (*, 105) stps, div_avg, div_max
+105     format(' Step ', i8, ' DIV-ERR FINAL: DIV_AVG = ', es11.4, ';  |DIV|_MAX = ', es11.4)
+        if (indx_bk / ibackup .eq. 1 .or. stps .eq. nsteps_env) then
+            call write_one('results/bak_u.dat', u_mrf, HC_U)
+            call write_one('results/bak_v.dat', 0.d0, HC_V)
+            call write_one('results/bak_w.dat', 0.d0, HC_W)
+            call write_one('results/bak_p.dat', 0.d0, HC_P)
+            call write_one('results/bak_t.dat', 0.d0, HC_T)
+            indx_bk = 0
+        endif
+        indx_bk = indx_bk + 1
+    enddo
+
+    call hip_destroy()
+
+contains
+
+    subroutine read_one(fname, shift, field)
+        character(len=*), intent(in) :: fname
+        real(c_double), intent(in) :: shift
+        integer(c_int), intent(in) :: field
+        integer :: u_
+        open(newunit=u_, file=fname, access='stream', form='unformatted', status='old')
+        read(u_) g
+        close(u_)
+        f(:,:,:) = 0.d0
+        f(3:nx + 2, 3:ny + 2, 3:nz + 2) = g + shift
+        call hip_upload(field, f)
+    end subroutine
+
+    subroutine write_one(fname, shift, field)
+        character(len=*), intent(in) :: fname
+        real(c_double), intent(in) :: shift
+        integer(c_int), intent(in) :: field
+        integer :: u_
+        call hip_download(field, f)
+        g = f(3:nx + 2, 3:ny + 2, 3:nz + 2) + shift
+        open(newunit=u_, file=fname, access='stream', form='unformatted', status='replace')
+        write(u_) g
+        close(u_)
+    end subroutine
+
+end program
