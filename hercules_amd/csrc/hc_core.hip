@@ -84,12 +84,18 @@ __global__ __launch_bounds__(rhs_threads<NX>()) void k_rhs_mom(StepArgs A)
         sm[q] = rhs_stage_value<NX, TY>(A, r, i, j0, k);
     }
     const double dpx = A.dp_opt == 2 ? A.scal[0] : A.dpx;
+    PointVals pv[PP];
+#pragma unroll
+    for (int pp = 0; pp < PP; ++pp) {
+        const int pt = tid + pp * T;
+        pv[pp].load(A, pt % NX, j0 + pt / NX, k);
+    }
     __syncthreads();
     double qs[PP][3];
 #pragma unroll
     for (int pp = 0; pp < PP; ++pp) {
         const int pt = tid + pp * T, i = pt % NX, jj = pt / NX, j = j0 + jj;
-        Ld<NX, TY> q(A, sm, i, jj, j, k);
+        Ld<NX, TY> q(A, sm, i, jj, j, k, pv[pp]);
         double f, g, h;
         momentum_point<CDS>(q, dpx, f, g, h, qs[pp][0], qs[pp][1], qs[pp][2]);
         const size_t c = ((size_t)k * A.ny + j) * NX + i;

@@ -200,6 +200,9 @@ struct Pt : Interp<Pt> {
     }
     HC_DEV double lev(int tab, int kk) const { return A.lev[tab * A.nz + kk]; }
     HC_DEV size_t center() const { return at(0, 0, k); }
+    HC_DEV double f0() const { return A.use_f0 ? A.ff0[center()] : 0.0; }
+    HC_DEV double g0() const { return A.use_f0 ? A.gg0[center()] : 0.0; }
+    HC_DEV double h0() const { return A.use_f0 ? A.hh0[center()] : 0.0; }
 };
 
 // ---------------------------------------------------------------------------
@@ -212,7 +215,6 @@ HC_DEV void momentum_point(const Q &q, double dpx, double &f, double &g, double 
     const double um = A.u_mrf;
     const int k = q.k;
     const double idz = q.lev(LV_IDZ, k), idzt = q.lev(LV_IDZT, k), idzb = q.lev(LV_IDZB, k);
-    size_t c0 = q.center();
     // ---- U equation (:55-161)
     {
         double uux, uvy, ux, uxx, uyy, dpdx;
@@ -240,7 +242,7 @@ HC_DEV void momentum_point(const Q &q, double dpx, double &f, double &g, double 
             f = -uux - uvy - uwz - um * ux + A.fc * (0.5 * (q.template vhx<CDS>(0, 0, 0) + q.template vhx<CDS>(0, -1, 0)) - A.vg);
         else
             f = -uux - uvy - uwz + dpx - um * ux;
-        double f0 = A.use_f0 ? A.ff0[c0] : 0.0;
+        double f0 = q.f0();
         qu = u0 + A.dt * (A.ra * f + A.rb * f0) + A.rc * 0.5 * A.dt * A.nu * (uxx + uyy + uzz) - A.rc * A.dt * dpdx;
     }
     // ---- V equation (:166-270)
@@ -271,7 +273,7 @@ HC_DEV void momentum_point(const Q &q, double dpx, double &f, double &g, double 
             g = -vux - vvy - vwz - um * vx + A.fc * (A.ug - 0.5 * (q.template uhy<CDS>(0, 0, 0) + q.template uhy<CDS>(-1, 0, 0)) - um);
         else
             g = -vux - vvy - vwz - um * vx;
-        double g0 = A.use_f0 ? A.gg0[c0] : 0.0;
+        double g0 = q.g0();
         qv = v0 + A.dt * (A.ra * g + A.rb * g0) + A.rc * 0.5 * A.dt * A.nu * (vxx + vyy + vzz) - A.rc * A.dt * dpdy;
     }
     // ---- W equation (:274-370), k < kend3
@@ -303,7 +305,7 @@ HC_DEV void momentum_point(const Q &q, double dpx, double &f, double &g, double 
             h = -wux - wvy - wwz - um * wx;
         }
         double dpdz = (q.P(0, 0, 1) - q.P(0, 0, 0)) * idzt;
-        double h0 = A.use_f0 ? A.hh0[c0] : 0.0;
+        double h0 = q.h0();
         qw = w0 + A.dt * (A.ra * h + A.rb * h0) + A.rc * 0.5 * A.dt * A.nu * (wxx + wyy + wzz) - A.rc * A.dt * dpdz;
     } else {
         h = 0.0;
@@ -414,25 +416,39 @@ HC_DEV double rhs_stage_value(const StepArgs &A, int r, int i, int j0, int k)
 
 // LDS accessor: the offsets momentum_point uses, read from the staged rows;
 // the single-point values at k-1 / k+1 come from registers.
+// Single-point values of one RHS point at levels k-1 / k+1 and the previous
+// substep's right-hand sides, loaded ahead of the LDS staging barrier.
+struct PointVals {
+    double u_km, v_km, w_kp, p_kp, t_k, t_kp, f0, g0, h0;
+    HC_DEV void load(const StepArgs &a, int i, int j, int k)
+    {
+        Pt g(a, i, j, k);
+        u_km = g.U(0, 0, -1);
+        v_km = g.V(0, 0, -1);
+        const bool in = k + 1 < a.nz;
+        w_kp = in ? g.W(0, 0, 1) : 0.0;
+        p_kp = g.P(0, 0, 1);
+        t_k = a.isscalar ? g.T(0, 0, 0) : 0.0;
+        t_kp = (a.isscalar && in) ? g.T(0, 0, 1) : 0.0;
+        f0 = g.f0();
+        g0 = g.g0();
+        h0 = g.h0();
+    }
+};
+
 template <int NX, int TY>
 struct Ld : Interp<Ld<NX, TY>> {
     using Tl = RhsTile<NX, TY>;
     const StepArgs &A;
     const double *sm;
     int i, jj, j, k;
-    double u_km, v_km, w_kp, p_kp, t_k, t_kp;
-    HC_DEV Ld(const StepArgs &a, const double *s, int i_, int jj_, int j_, int k_)
-        : A(a), sm(s), i(i_), jj(jj_), j(j_), k(k_)
-    {
-        Pt g(a, i_, j_, k_);
-        u_km = g.U(0, 0, -1);
-        v_km = g.V(0, 0, -1);
-        const bool in = k_ + 1 < a.nz;
-        w_kp = in ? g.W(0, 0, 1) : 0.0;
-        p_kp = g.P(0, 0, 1);
-        t_k = a.isscalar ? g.T(0, 0, 0) : 0.0;
-        t_kp = (a.isscalar && in) ? g.T(0, 0, 1) : 0.0;
-    }
+    double u_km, v_km, w_kp, p_kp, t_k, t_kp, f0_, g0_, h0_;
+    HC_DEV Ld(const StepArgs &a, const double *s, int i_, int jj_, int j_, int k_, const PointVals &pv)
+        : A(a), sm(s), i(i_), jj(jj_), j(j_), k(k_), u_km(pv.u_km), v_km(pv.v_km), w_kp(pv.w_kp),
+          p_kp(pv.p_kp), t_k(pv.t_k), t_kp(pv.t_kp), f0_(pv.f0), g0_(pv.g0), h0_(pv.h0) {}
+    HC_DEV double f0() const { return f0_; }
+    HC_DEV double g0() const { return g0_; }
+    HC_DEV double h0() const { return h0_; }
     HC_DEV double row(int base, int r, int di) const { return sm[(base + r) * NX + wrapx<NX>(i + di)]; }
     HC_DEV double U(int di, int dj, int dk) const
     {
