@@ -213,13 +213,19 @@ def main():
     tot = sum(v["avg_ms"] * v["launches"] for v in kernels.values()) or 1.0
     for v in kernels.values():
         v["share"] = v["avg_ms"] * v["launches"] / tot
+    # measured HBM bytes per launch (rocprofv3 FETCH_SIZE / WRITE_SIZE runs,
+    # calibrated against a copy kernel), committed under profiles/
+    traffic = {}
+    tpath = os.path.join(ROOT, "profiles", f"hbm_traffic_{args.workload}_{prm.nx}x{prm.ny}x{prm.nz}.json")
+    if os.path.exists(tpath):
+        traffic = json.load(open(tpath)).get("bytes_per_launch", {})
     roofline = None
     if kernels:
         dom = max(kernels, key=lambda k: kernels[k]["avg_ms"] * kernels[k]["launches"])
         kd = kernels[dom]
         roofline = {"bound": "hbm", "kernel": dom, "achieved": round(kd["GBs"], 1),
                     "peak": HBM_PEAK_GBS, "unit": "GB/s", "frac": round(kd["GBs"] / HBM_PEAK_GBS, 4),
-                    "traffic": None, "avg_ms": round(kd["avg_ms"], 4),
+                    "traffic": traffic.get(dom), "avg_ms": round(kd["avg_ms"], 4),
                     "algo_bytes_per_launch": int(kd["algo_bytes"])}
     # whole step against the step's algorithmic bytes (1272 B/cell/step for FD
     # without T, 1608 with T: 159 / 201 FP64 passes per RK3 step)
