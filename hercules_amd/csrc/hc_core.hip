@@ -121,8 +121,79 @@ __global__ __launch_bounds__(rhs_threads<NX>()) void k_rhs_mom(StepArgs A)
     }
 }
 
-// Temperature RHS + x R2C of qt.
+// Momentum RHS + x R2C of qu,qv,qw through the global-memory accessor of a
+// damped substep (PtD).  Block: rows_per_block full rows of level k.
 template <int CDS, int NX>
+__global__ __launch_bounds__(kRhsThreads) void k_rhs_mom_damped(StepArgs A)
+{
+    constexpr int ROWS = rows_per_block<NX>();
+    constexpr int NL = NX / 2;
+    __shared__ double2 sq_[3 * ROWS * NL];
+    double *sqd = reinterpret_cast<double *>(sq_);
+    const int k = blockIdx.y, j0 = blockIdx.x * ROWS, tid = threadIdx.x;
+    const double dpx = A.dp_opt == 2 ? A.scal[0] : A.dpx;
+    for (int p = tid; p < ROWS * NX; p += kRhsThreads) {
+        int jj = p / NX, i = p - jj * NX, j = j0 + jj;
+        double qu = 0.0, qv = 0.0, qw = 0.0;
+        if (j < A.ny) {
+            PtD q(A, i, j, k);
+            double f, g, h;
+            momentum_point<CDS>(q, dpx, f, g, h, qu, qv, qw);
+            const size_t c = ((size_t)k * A.ny + j) * NX + i;
+            A.ff[c] = f;
+            A.gg[c] = g;
+            A.hh[c] = h;
+        }
+        sqd[(0 * ROWS + jj) * NX + i] = qu;
+        sqd[(1 * ROWS + jj) * NX + i] = qv;
+        sqd[(2 * ROWS + jj) * NX + i] = qw;
+    }
+    __syncthreads();
+    lds_fft<NL, 3 * ROWS, NL, 1, kRhsThreads, -1>(sq_, A.tw_nl, tid);
+    double2 *outs[3] = {A.s0, A.s1, A.s2};
+    for (int p = tid; p < 3 * ROWS * NL; p += kRhsThreads) {
+        int row = p / NL, l = p - row * NL, fi = row / ROWS, jj = row - fi * ROWS, j = j0 + jj;
+        if (j < A.ny) outs[fi][((size_t)k * A.ny + j) * NL + l] = r2c_post<NL>(sq_ + row * NL, l, A.tw_nx, A.norm);
+    }
+}
+
+// Plane means of the damped levels, one block per (level, field):
+// the mean_local of rayleigh_damping (src/module_tools.f90:718-732).
+struct DampFields {
+    const double *f[5];
+};
+__global__ __launch_bounds__(256) void k_damp_means(DampFields F, double *dmp_m, int nx, int ny, int nz, int kdamp0)
+{
+    __shared__ double red[256];
+    const int kk = kdamp0 + blockIdx.x, fid = blockIdx.y;
+    const double *plane = F.f[fid] + (size_t)kk * nx * ny;
+    double s = 0.0;
+    for (int q = threadIdx.x; q < nx * ny; q += 256) s += plane[q];
+    red[threadIdx.x] = s;
+    __syncthreads();
+    for (int w = 128; w > 0; w >>= 1) {
+        if (threadIdx.x < w) red[threadIdx.x] += red[threadIdx.x + w];
+        __syncthreads();
+    }
+    if (threadIdx.x == 0) dmp_m[fid * nz + kk] = red[0] / nx / ny;
+}
+
+// In-place damping of one field's interior (the temperature when no
+// temperature equation overwrites it later in the substep).
+__global__ __launch_bounds__(256) void k_damp_apply(double *f, const double *dmp_s, const double *mean, int nx,
+                                                     int ny, int kdamp0, int nlev)
+{
+    const size_t n = (size_t)nx * ny * nlev;
+    for (size_t q = blockIdx.x * (size_t)blockDim.x + threadIdx.x; q < n; q += (size_t)gridDim.x * blockDim.x) {
+        const int kk = kdamp0 + (int)(q / ((size_t)nx * ny));
+        const size_t c = (size_t)kdamp0 * nx * ny + q;
+        const double v = f[c], s = dmp_s[kk], m = mean[kk];
+        f[c] = v - s * (v - m);
+    }
+}
+
+// Temperature RHS + x R2C of qt.  Acc: Pt, or PtD in a damped substep.
+template <int CDS, int NX, class Acc>
 __global__ __launch_bounds__(kRhsThreads) void k_rhs_t(StepArgs A)
 {
     constexpr int ROWS = rows_per_block<NX>();
@@ -134,7 +205,7 @@ __global__ __launch_bounds__(kRhsThreads) void k_rhs_t(StepArgs A)
         int jj = p / NX, i = p - jj * NX, j = j0 + jj;
         double qt = 0.0;
         if (j < A.ny) {
-            Pt q{A, i, j, k};
+            Acc q(A, i, j, k);
             double s;
             temperature_point<CDS>(q, s, qt);
             A.ss[((size_t)k * A.ny + j) * A.nx + i] = s;
@@ -172,13 +243,16 @@ __global__ __launch_bounds__(kRhsThreads) void k_x_r2c(const double *__restrict_
 }
 
 // Inverse x transform (C2R) of F spectra rows into physical rows.
-// mode[f]: 0 store, 1 accumulate (dst += value).
+// mode[f]: 0 store, 1 accumulate (dst += value), 2 accumulate onto the
+// damped old value on levels k >= kdamp0 (dst = dst - s (dst - mean) + value).
 struct C2RArgs {
     const double2 *src[4];
     double *dst[4];
     int mode[4];
     int ny;
     const double2 *tw_nl, *tw_nx;
+    const double *dmp_s, *dmp_mean;
+    int kdamp0;
 };
 template <int NX, int F>
 __global__ __launch_bounds__(kRhsThreads) void k_x_c2r(C2RArgs A)
@@ -207,8 +281,15 @@ __global__ __launch_bounds__(kRhsThreads) void k_x_c2r(C2RArgs A)
         if (j < A.ny) {
             size_t c = ((size_t)k * A.ny + j) * NX + i;
             double x = sqd[row * NX + i];
-            if (A.mode[f] == 1) A.dst[f][c] += x;
-            else A.dst[f][c] = x;
+            if (A.mode[f] == 1) {
+                A.dst[f][c] += x;
+            } else if (A.mode[f] == 2) {
+                double o = A.dst[f][c];
+                if (k >= A.kdamp0) o = o - A.dmp_s[k] * (o - A.dmp_mean[k]);
+                A.dst[f][c] = o + x;
+            } else {
+                A.dst[f][c] = x;
+            }
         }
     }
 }
@@ -710,6 +791,8 @@ struct hc_ctx {
     int fcur, scur;
     double2 *s[4];
     double *lev, *symx, *symy, *scal, *part, *stage;
+    double *dmp_s, *dmp_m;  // Rayleigh damping: sigz per level, plane means per (field, level)
+    int kdamp0;             // first damped level (nz when isdamp = 0)
     double2 *tw_nl, *tw_nx, *tw_ny;
     double aU0, cUe, aT0, cTe;
     double time_sim, got;
@@ -920,6 +1003,7 @@ static StepArgs base_args(hc_ctx *c)
     A.c12x2 = 1.0 / (12.0 * c->dx * c->dx); A.c12y2 = 1.0 / (12.0 * c->dy * c->dy);
     A.idx = 1.0 / c->dx; A.idy = 1.0 / c->dy; A.idx2 = 1.0 / (c->dx * c->dx); A.idy2 = 1.0 / (c->dy * c->dy);
     A.idt = 1.0 / P.dt;
+    A.dmp_s = c->dmp_s; A.dmp_m = c->dmp_m; A.kdamp0 = c->kdamp0;
     return A;
 }
 
@@ -1005,7 +1089,26 @@ static int substep(hc_ctx *c, int t_its)
     if (P.dp_opt == 2) hipLaunchKernelGGL(k_dpx_update, dim3(1), dim3(64), 0, c->stream, c->scal, P.u_mrf, P.nu * 1.0);
     int rows = rows_for(c->nx);
     dim3 grow((c->ny + rows - 1) / rows, c->nz);
+    const bool damp = P.isdamp == 1;
     PROF_BEGIN(c, HC_K_RHS);
+    if (damp) {
+        // rayleigh_damping of u, v, w, p, t at the start of the substep
+        // (src/module_navier_stokes.f90:258-266): plane means of the undamped
+        // fields; the stencils apply the damping as they read.
+        DampFields F = {{c->u, c->v, c->w, c->p, c->t}};
+        hipLaunchKernelGGL(k_damp_means, dim3(P.nzdamp + 1, 5), dim3(256), 0, c->stream, F, c->dmp_m, c->nx, c->ny,
+                           c->nz, c->kdamp0);
+        if (P.isscalar == 0) {
+            hipLaunchKernelGGL(k_damp_apply, dim3(256), dim3(256), 0, c->stream, c->t, c->dmp_s, c->dmp_m + 4 * c->nz,
+                               c->nx, c->ny, c->kdamp0, P.nzdamp + 1);
+        }
+        A.dmask = 31;
+#define HC_RHSD(NX_)                                                                                         \
+    if (P.cds == 1) hipLaunchKernelGGL((k_rhs_mom_damped<1, NX_>), grow, dim3(kRhsThreads), 0, c->stream, A); \
+    else hipLaunchKernelGGL((k_rhs_mom_damped<2, NX_>), grow, dim3(kRhsThreads), 0, c->stream, A);
+        HC_DISPATCH_N(c->nx, HC_RHSD)
+#undef HC_RHSD
+    } else {
 #define HC_RHS(NX_)                                                                                   \
     {                                                                                                  \
         constexpr int TY_ = rhs_ty<NX_>();                                                             \
@@ -1016,6 +1119,7 @@ static int substep(hc_ctx *c, int t_its)
     }
     HC_DISPATCH_N(c->nx, HC_RHS)
 #undef HC_RHS
+    }
     PROF_END(c, HC_K_RHS);
     PROF_BEGIN(c, HC_K_YFFT_FWD);
     if (int e = launch_yfft(c, -1, 3, c->s[0], c->s[1], c->s[2], c->s[3])) return e;
@@ -1033,7 +1137,8 @@ static int substep(hc_ctx *c, int t_its)
     std::memset(&X, 0, sizeof X);
     X.src[0] = c->s[0]; X.src[1] = c->s[1]; X.src[2] = c->s[2]; X.src[3] = c->s[3];
     X.dst[0] = c->u; X.dst[1] = c->v; X.dst[2] = c->w; X.dst[3] = c->p;
-    X.mode[3] = 1;
+    X.mode[3] = damp ? 2 : 1;  // calculate_new_p adds to the damped p
+    X.dmp_s = c->dmp_s; X.dmp_mean = c->dmp_m + 3 * c->nz; X.kdamp0 = c->kdamp0;
     PROF_BEGIN(c, HC_K_FINISH);
     if (int e = launch_c2r(c, 4, X)) return e;
     PROF_END(c, HC_K_FINISH);
@@ -1042,9 +1147,15 @@ static int substep(hc_ctx *c, int t_its)
         int snx = c->scur ^ 1;
         A.ss0 = c->sb[c->scur]; A.ss = c->sb[snx];
         PROF_BEGIN(c, HC_K_RHS_T);
-#define HC_RT(NX_)                                                                                  \
-    if (P.cds == 1) hipLaunchKernelGGL((k_rhs_t<1, NX_>), grow, dim3(kRhsThreads), 0, c->stream, A);  \
-    else hipLaunchKernelGGL((k_rhs_t<2, NX_>), grow, dim3(kRhsThreads), 0, c->stream, A);
+        A.dmask = 16;  // t read damped, the updated velocities as they are
+#define HC_RT(NX_)                                                                                          \
+    if (damp) {                                                                                             \
+        if (P.cds == 1) hipLaunchKernelGGL((k_rhs_t<1, NX_, PtD>), grow, dim3(kRhsThreads), 0, c->stream, A); \
+        else hipLaunchKernelGGL((k_rhs_t<2, NX_, PtD>), grow, dim3(kRhsThreads), 0, c->stream, A);          \
+    } else {                                                                                                \
+        if (P.cds == 1) hipLaunchKernelGGL((k_rhs_t<1, NX_, Pt>), grow, dim3(kRhsThreads), 0, c->stream, A);  \
+        else hipLaunchKernelGGL((k_rhs_t<2, NX_, Pt>), grow, dim3(kRhsThreads), 0, c->stream, A);           \
+    }
         HC_DISPATCH_N(c->nx, HC_RT)
 #undef HC_RT
         PROF_END(c, HC_K_RHS_T);
@@ -1093,7 +1204,10 @@ extern "C" int hc_create(const hc_params *prm, hc_ctx **out)
     if (P.dp_opt < 1 || P.dp_opt > 3) return set_err(HC_ERR_ARG, "dp_opt must be 1, 2 or 3");
     if (P.ochannel != 0 && P.ochannel != 1) return set_err(HC_ERR_ARG, "ochannel must be 0 or 1");
     if (P.isscalar != 0 && P.isscalar != 1) return set_err(HC_ERR_ARG, "isscalar error!");
-    if (P.isdamp != 0) return set_err(HC_ERR_UNSUPPORTED, "isdamp=1 (Rayleigh damping) is not implemented");
+    if (P.isdamp != 0 && P.isdamp != 1) return set_err(HC_ERR_ARG, "isdamp must be 0 or 1");
+    // the damped levels kend3-nzdamp..kend3 must be interior levels
+    if (P.isdamp == 1 && (P.nzdamp < 1 || P.nzdamp > P.nz - 1))
+        return set_err(HC_ERR_ARG, "nzdamp=%d must be in 1..nz-1 when isdamp=1", P.nzdamp);
     if (P.wall_rows != 0 && P.wall_rows != 1) return set_err(HC_ERR_ARG, "wall_rows must be 0 or 1");
     if (!fft_len_ok(P.nx) || !fft_len_ok(P.ny)) return set_err(HC_ERR_UNSUPPORTED, "nx, ny must be one of 16,32,64,128,192,256,512");
     {
@@ -1130,7 +1244,17 @@ extern "C" int hc_create(const hc_params *prm, hc_ctx **out)
     if (!e) e = dalloc(c, (void **)&c->tw_ny, tny.size() * sizeof(double2));
     if (!e) e = dalloc(c, (void **)&c->scal, 8 * sizeof(double));
     if (!e) e = dalloc(c, (void **)&c->part, 3 * (size_t)c->nz * sizeof(double));
+    if (!e) e = dalloc(c, (void **)&c->dmp_s, (size_t)c->nz * sizeof(double));
+    if (!e) e = dalloc(c, (void **)&c->dmp_m, 5 * (size_t)c->nz * sizeof(double));
     if (e) { hc_destroy(c); return e; }
+    // sponge strength sigz of rayleigh_damping (src/module_tools.f90:735-736)
+    // on the top nzdamp+1 levels, zero below
+    c->kdamp0 = P.isdamp == 1 ? c->nz - 1 - P.nzdamp : c->nz;
+    std::vector<double> hs(c->nz, 0.0);
+    for (int kk = c->kdamp0; kk < c->nz; ++kk) {
+        double zd = (kk - c->kdamp0) * 1.0 / P.nzdamp;
+        hs[kk] = P.cdamp * 0.5 * (1.0 - std::cos(kPi * zd));
+    }
     double scal0[8] = {P.dp_opt == 1 ? 1.0 : 0.0, 0, 0, 0, 0, 0, 0, 0};
     hipError_t r = hipMemcpyAsync(c->lev, c->h_lev.data(), c->h_lev.size() * sizeof(double), hipMemcpyHostToDevice, c->stream);
     if (r == hipSuccess) r = hipMemcpyAsync(c->symx, sx.data(), sx.size() * sizeof(double), hipMemcpyHostToDevice, c->stream);
@@ -1139,6 +1263,7 @@ extern "C" int hc_create(const hc_params *prm, hc_ctx **out)
     if (r == hipSuccess) r = hipMemcpyAsync(c->tw_nx, tnx.data(), tnx.size() * sizeof(double2), hipMemcpyHostToDevice, c->stream);
     if (r == hipSuccess) r = hipMemcpyAsync(c->tw_ny, tny.data(), tny.size() * sizeof(double2), hipMemcpyHostToDevice, c->stream);
     if (r == hipSuccess) r = hipMemcpyAsync(c->scal, scal0, sizeof scal0, hipMemcpyHostToDevice, c->stream);
+    if (r == hipSuccess) r = hipMemcpyAsync(c->dmp_s, hs.data(), hs.size() * sizeof(double), hipMemcpyHostToDevice, c->stream);
     if (r == hipSuccess) r = hipStreamSynchronize(c->stream);
     if (r != hipSuccess) { hc_destroy(c); return set_err(HC_ERR_HIP, "table upload: %s", hipGetErrorString(r)); }
     *out = c;

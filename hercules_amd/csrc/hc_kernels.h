@@ -54,6 +54,11 @@ struct StepArgs {
     // reciprocal stencil factors: 1/(24 dx), 1/(12 dx^2), 1/dx, 1/dx^2 (same in y), 1/dt
     double c24x, c12x2, c24y, c12y2, idx, idx2, idy, idy2, idt;
     double aU0, cUe, aT0, cTe;  // unmodified wall coefficients for the (0,0)-mode fix-ups
+    // Rayleigh damping of the top levels k >= kdamp0: sponge strength per
+    // level and plane means per (field u,v,w,p,t; level); dmask selects the
+    // fields an accessor reads damped (bit f for field f).
+    const double *dmp_s, *dmp_m;
+    int kdamp0, dmask;
 };
 
 HC_DEV int wrapi(int i, int n) { return i < 0 ? i + n : (i >= n ? i - n : i); }
@@ -200,6 +205,100 @@ struct Pt : Interp<Pt> {
     }
     HC_DEV double lev(int tab, int kk) const { return A.lev[tab * A.nz + kk]; }
     HC_DEV size_t center() const { return at(0, 0, k); }
+    HC_DEV double f0() const { return A.use_f0 ? A.ff0[center()] : 0.0; }
+    HC_DEV double g0() const { return A.use_f0 ? A.gg0[center()] : 0.0; }
+    HC_DEV double h0() const { return A.use_f0 ? A.hh0[center()] : 0.0; }
+};
+
+// Global-memory accessor of a substep that starts with rayleigh_damping
+// (src/module_tools.f90:705-749, called at src/module_navier_stokes.f90:258-266).
+// The reference damps the interior of the top levels in place and refreshes
+// no ghost before the stencils run, so a read sees
+//   interior (i,j) of a damped level:  v - sigz*(v - plane mean)
+//   x/y halo (wrapped index):          the undamped value
+//   z ghost:                           the wall rule applied to undamped values.
+// Interpolants are computed on the interior and copied to their halos after
+// (comm_bound in update_boundary_uvwhxyz, src/module_boundary.f90:202-355), so
+// an interpolant at a wrapped offset is evaluated at the wrapped centre.
+struct PtD : Interp<PtD> {
+    const StepArgs &A;
+    int i, j, k;
+    HC_DEV PtD(const StepArgs &a, int i_, int j_, int k_) : A(a), i(i_), j(j_), k(k_) {}
+    HC_DEV double raw(const double *f, int x, int y, int kk) const
+    {
+        return f[((size_t)kk * A.ny + wrapi(y, A.ny)) * A.nx + wrapi(x, A.nx)];
+    }
+    HC_DEV double val(int fid, const double *f, int di, int dj, int kk) const
+    {
+        const int x = i + di, y = j + dj;
+        double v = raw(f, x, y, kk);
+        if (kk >= A.kdamp0 && ((A.dmask >> fid) & 1) && x >= 0 && x < A.nx && y >= 0 && y < A.ny) {
+            const double s = A.dmp_s[kk], m = A.dmp_m[fid * A.nz + kk];
+            v = v - s * (v - m);
+        }
+        return v;
+    }
+    HC_DEV double U(int di, int dj, int dk) const
+    {
+        int kk = k + dk;
+        if (kk < 0) return -raw(A.u, i + di, j + dj, 0) - 2.0 * A.u_mrf;
+        if (kk >= A.nz) {
+            double top = raw(A.u, i + di, j + dj, A.nz - 1);
+            return A.ochannel == 0 ? -top - 2.0 * A.u_mrf : top;
+        }
+        return val(0, A.u, di, dj, kk);
+    }
+    HC_DEV double V(int di, int dj, int dk) const
+    {
+        int kk = k + dk;
+        if (kk < 0) return -raw(A.v, i + di, j + dj, 0);
+        if (kk >= A.nz) {
+            double top = raw(A.v, i + di, j + dj, A.nz - 1);
+            return A.ochannel == 0 ? -top : top;
+        }
+        return val(1, A.v, di, dj, kk);
+    }
+    HC_DEV double W(int di, int dj, int dk) const
+    {
+        int kk = k + dk;
+        if (kk < 0 || kk == A.nz - 1) return 0.0;
+        if (kk >= A.nz) return -raw(A.w, i + di, j + dj, A.nz - 2);
+        return val(2, A.w, di, dj, kk);
+    }
+    HC_DEV double P(int di, int dj, int dk) const
+    {
+        int kk = k + dk;
+        if (kk < 0) return raw(A.p, i + di, j + dj, 0);
+        if (kk >= A.nz) return raw(A.p, i + di, j + dj, A.nz - 1);
+        return val(3, A.p, di, dj, kk);
+    }
+    HC_DEV double T(int di, int dj, int dk) const
+    {
+        int kk = k + dk;
+        if (kk < 0) return 2.0 * A.tbot - raw(A.t, i + di, j + dj, 0);
+        if (kk >= A.nz) return 2.0 * A.ttop - raw(A.t, i + di, j + dj, A.nz - 1);
+        return val(4, A.t, di, dj, kk);
+    }
+    HC_DEV PtD centre(int di, int dj) const { return PtD(A, wrapi(i + di, A.nx), wrapi(j + dj, A.ny), k); }
+#define HC_PTD_INTERP(NAME)                                                                       \
+    template <int CDS> HC_DEV double NAME(int di, int dj, int dk) const                           \
+    {                                                                                             \
+        const PtD c = centre(di, dj);                                                             \
+        return c.Interp<PtD>::template NAME<CDS>(0, 0, dk);                                       \
+    }
+    HC_PTD_INTERP(uhx)
+    HC_PTD_INTERP(uhy)
+    HC_PTD_INTERP(vhx)
+    HC_PTD_INTERP(vhy)
+    HC_PTD_INTERP(whx)
+    HC_PTD_INTERP(why)
+    HC_PTD_INTERP(thx)
+    HC_PTD_INTERP(thy)
+#undef HC_PTD_INTERP
+    HC_DEV double uhz(int di, int dj, int dk) const { return centre(di, dj).Interp<PtD>::uhz(0, 0, dk); }
+    HC_DEV double vhz(int di, int dj, int dk) const { return centre(di, dj).Interp<PtD>::vhz(0, 0, dk); }
+    HC_DEV double lev(int tab, int kk) const { return A.lev[tab * A.nz + kk]; }
+    HC_DEV size_t center() const { return ((size_t)k * A.ny + j) * A.nx + i; }
     HC_DEV double f0() const { return A.use_f0 ? A.ff0[center()] : 0.0; }
     HC_DEV double g0() const { return A.use_f0 ? A.gg0[center()] : 0.0; }
     HC_DEV double h0() const { return A.use_f0 ? A.hh0[center()] : 0.0; }

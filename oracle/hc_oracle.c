@@ -149,6 +149,8 @@ ho_state *ho_create(const ho_params *P)
     ho_state *S = (ho_state *)calloc(1, sizeof(ho_state));
     S->prm = *P;
     if (P->nx % 2 || P->ny % 2 || P->nz < 3) { ho_destroy(S); return NULL; }
+    /* the damped levels kend3-nzdamp..kend3 must be interior levels */
+    if (P->isdamp == 1 && (P->nzdamp < 1 || P->nzdamp > P->nz - 1)) { ho_destroy(S); return NULL; }
     S->istr3 = 3; S->iend3 = P->nx + 2;
     S->jstr3 = 3; S->jend3 = P->ny + 2;
     S->kstr3 = 3; S->kend3 = P->nz + 2;
@@ -983,6 +985,30 @@ static int div_error(ho_state *S, int isfinal)
 }
 
 /* ---- assign_rk_coeff, src/module_parameters.f90:618-641 ---------------- */
+/* ---- A19: rayleigh_damping, src/module_tools.f90:705-749 ---------------- */
+/* Top sponge on levels kend3-nzdamp..kend3: each interior value relaxes
+ * toward its plane mean by sigz = cdamp*(1-cos(pi*zd))/2.  Only the interior
+ * (i,j) is written: the x/y halos and the z ghosts keep the values of the last
+ * boundary update, exactly as in the reference, which calls no boundary
+ * update between the damping and the stencils that read those halos
+ * (src/module_navier_stokes.f90:258-275). */
+static void rayleigh_damping(ho_state *S, double *a)
+{
+    const ho_params *P = &S->prm;
+    for (int k = S->kend3 - P->nzdamp; k <= S->kend3; ++k) {
+        double mean_local = 0.0;
+        for (int j = S->jstr3; j <= S->jend3; ++j)
+            for (int i = S->istr3; i <= S->iend3; ++i) mean_local += F3(S, a, i, j, k);
+        mean_local = mean_local / P->nx / P->ny;
+        double mean_all = mean_local / 1.0; /* MPI_ALLREDUCE over one rank, / nprc */
+        double zd = (k - S->kend3 + P->nzdamp) * 1.0 / P->nzdamp;
+        double sigz = P->cdamp * 0.5 * (1.0 - cos(PI * zd));
+        for (int j = S->jstr3; j <= S->jend3; ++j)
+            for (int i = S->istr3; i <= S->iend3; ++i)
+                F3(S, a, i, j, k) = F3(S, a, i, j, k) - sigz * (F3(S, a, i, j, k) - mean_all);
+    }
+}
+
 static void assign_rk(ho_state *S, int t_its)
 {
     if (S->prm.dts == 2) { S->rk_a = RKC1[t_its - 1]; S->rk_b = RKC2[t_its - 1]; S->rk_c = RKC3[t_its - 1]; }
@@ -996,7 +1022,13 @@ static int time_advancement_fd(ho_state *S)
     size_t nf = nfield(S);
     int t_its_max = P->dts == 2 ? 3 : 1;
     for (int t_its = 1; t_its <= t_its_max; ++t_its) {
-        if (P->isdamp == 1) { snprintf(S->err, sizeof S->err, "isdamp=1 is not restated"); return 1; }
+        if (P->isdamp == 1) {                                     /* :258-266 */
+            rayleigh_damping(S, S->u);
+            rayleigh_damping(S, S->v);
+            rayleigh_damping(S, S->w);
+            rayleigh_damping(S, S->p);
+            rayleigh_damping(S, S->t);
+        }
         memcpy(S->ff0, S->ff, nf * sizeof(double));               /* assign_fgh_old :270 */
         memcpy(S->gg0, S->gg, nf * sizeof(double));
         memcpy(S->hh0, S->hh, nf * sizeof(double));
@@ -1182,6 +1214,11 @@ void ho_stage_interp(ho_state *S) { interp_uvw(S); bc_uvwh(S); }
 void ho_stage_momentum(ho_state *S, int t_its) { assign_rk(S, t_its); interp_uvw(S); bc_uvwh(S); momentum(S); }
 void ho_stage_p_src(ho_state *S) { p_src(S, S->qu); }
 void ho_stage_bc_uvw(ho_state *S) { bc_uvw(S); }
+void ho_stage_damping(ho_state *S, int id)
+{
+    double *f[] = {S->u, S->v, S->w, S->p, S->t};
+    if (id >= 0 && id < 5) rayleigh_damping(S, f[id]);
+}
 void ho_stage_temperature(ho_state *S, int t_its) { assign_rk(S, t_its); interp_t(S); bc_th(S); temperature(S, S->qu); }
 double ho_get_scalar(ho_state *S, int id)
 {

@@ -70,6 +70,7 @@ def lib():
         L.ho_stage_p_src.argtypes = [vp]
         L.ho_stage_bc_uvw.argtypes = [vp]
         L.ho_stage_temperature.argtypes = [vp, ip]
+        L.ho_stage_damping.argtypes = [vp, ip]
         L.ho_get_scalar.restype = C.c_double
         L.ho_get_scalar.argtypes = [vp, ip]
         L.ho_set_scalar.argtypes = [vp, ip, C.c_double]

@@ -72,6 +72,9 @@ def test_steps_shapes_match_oracle(shape):
     ("closed", {"dp_opt": 3, "fc": 0.5, "ug": 1.0, "vg": 0.2}),
     ("closed", {"dts": 1}),
     ("open_t", {}),
+    ("closed", {"isdamp": 1, "nzdamp": 8, "cdamp": 0.3}),
+    ("open_t", {"isdamp": 1, "nzdamp": 10, "cdamp": 0.2}),
+    ("closed", {"isdamp": 1, "nzdamp": 6, "cdamp": 1.0, "cds": 1, "dp_opt": 3, "fc": 0.5, "ug": 1.0}),
 ])
 def test_steps_match_oracle(kind, kw):
     prm, st = small_case(kind, **kw)
@@ -84,6 +87,22 @@ def test_steps_match_oracle(kind, kw):
     for n in names:
         err = rel_l2(ch.get_field(n), o.get(n))
         print(f"{kind} {kw} {n}: rel L2 {err:.3e}")
+        assert err < 1e-10, (n, err)
+    ch.close()
+
+
+def test_damping_of_a_passive_temperature_matches_oracle():
+    """isscalar=0: rayleigh_damping still relaxes t in place every substep
+    (src/module_navier_stokes.f90:258-264) while nothing else updates it."""
+    prm, st = small_case("closed", isdamp=1, nzdamp=9, cdamp=0.4)
+    st["t"] = np.random.default_rng(9).standard_normal(st["u"].shape)
+    o = make_oracle(prm, st)
+    ch = _channel(prm, st)
+    o.steps(2)
+    ch.steps(2)
+    for n in ("u", "v", "w", "p", "t"):
+        err = rel_l2(ch.get_field(n), o.get(n))
+        print(f"damped, passive t: {n} rel L2 {err:.3e}")
         assert err < 1e-10, (n, err)
     ch.close()
 

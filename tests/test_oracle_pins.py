@@ -310,3 +310,62 @@ def test_first_substep_ignores_old_rhs():
     o2.steps(1)
     for n in ("u", "v", "w", "p"):
         assert np.array_equal(o1.get(n), o2.get(n))
+
+
+# ------------------------------------------------- Rayleigh damping (A19)
+@pytest.mark.parametrize("nzdamp,cdamp", [(4, 0.3), (7, 1.0), (16, 0.05)])
+def test_damping_sponge_profile(nzdamp, cdamp):
+    """rayleigh_damping (src/module_tools.f90:705-749) keeps every plane mean
+    and scales the fluctuation of the top nzdamp+1 levels by 1 - sigma with
+    sigma = cdamp sin^2(pi zd / 2), zd = 0 .. 1 from the lowest damped level
+    to the top; lower levels and all ghosts are untouched."""
+    prm = cases.closed_channel(nx=16, ny=8, nz=17, zstretch=1.0, isdamp=1, nzdamp=nzdamp, cdamp=cdamp)
+    st = cases.laminar_noise(prm, seed=21)
+    o = _oracle(prm, st)
+    before = o.get_ghosted("u")
+    from oracle.oracle import lib
+    lib().ho_stage_damping(o.h, 0)
+    after = o.get_ghosted("u")
+    b, a = before[2:-2, 2:-2, 2:-2], after[2:-2, 2:-2, 2:-2]
+    k0 = prm.nz - 1 - nzdamp
+    for k in range(prm.nz):
+        mb, ma = b[k].mean(), a[k].mean()
+        assert abs(ma - mb) < 1e-13 * (abs(mb) + 1.0)
+        if k < k0:
+            assert np.array_equal(a[k], b[k])
+            continue
+        sigma = cdamp * np.sin(0.5 * np.pi * (k - k0) / nzdamp) ** 2
+        fb, fa = b[k] - mb, a[k] - ma
+        assert np.abs(fa - (1.0 - sigma) * fb).max() < 1e-13 * (np.abs(b[k]).max() + 1.0)
+    if cdamp == 1.0:
+        assert np.ptp(a[-1]) < 1e-13 * np.abs(b[-1]).max()  # top level relaxed onto its mean
+    # halos and z ghosts keep their pre-damping values
+    mask = np.ones(after.shape, bool)
+    mask[2:-2, 2:-2, 2:-2] = False
+    assert np.array_equal(after[mask], before[mask])
+
+
+def test_damping_with_zero_strength_is_inert():
+    prm0 = cases.closed_channel(nx=16, ny=16, nz=17, zstretch=1.0)
+    prm1 = cases.closed_channel(nx=16, ny=16, nz=17, zstretch=1.0, isdamp=1, nzdamp=6, cdamp=0.0)
+    st = cases.laminar_noise(prm0, seed=22)
+    o0, o1 = _oracle(prm0, st), _oracle(prm1, st)
+    o0.steps(2)
+    o1.steps(2)
+    for n in ("u", "v", "w", "p"):
+        assert np.array_equal(o0.get(n), o1.get(n))
+
+
+def test_damped_step_stays_divergence_free():
+    prm = cases.closed_channel(nx=16, ny=16, nz=17, zstretch=1.0, isdamp=1, nzdamp=6, cdamp=0.5)
+    st = cases.laminar_noise(prm, seed=23)
+    o = _oracle(prm, st)
+    o.steps(2)
+    d, terms = divergence(o.get("u"), o.get("v"), o.get("w"), prm)
+    assert np.abs(d).max() / terms.max() < 1e-12
+
+
+def test_damping_rejects_levels_outside_the_interior():
+    prm = cases.closed_channel(nx=16, ny=16, nz=9, isdamp=1, nzdamp=9)
+    with pytest.raises(ValueError):
+        Oracle(**oracle_kwargs(prm))
