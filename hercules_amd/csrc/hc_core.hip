@@ -121,6 +121,114 @@ __global__ __launch_bounds__(rhs_threads<NX>()) void k_rhs_mom(StepArgs A)
     }
 }
 
+// Two-phase momentum RHS + x R2C of qu,qv,qw, one point per thread.
+// Staging: raw u, v, w, p rows of level k (Tile2).  Phase 1: every
+// interpolant and flux product of the tile once into LDS.  Phase 2: the
+// point's right-hand side from those rows (momentum_tile).  The staging area
+// then holds the q rows for the x FFT.
+template <int NX>
+constexpr int rhs2_ty() { return NX == 16 ? 16 : NX == 32 ? 32 : NX == 64 ? 16 : NX == 128 ? 8 : 4; }
+template <int NX>
+constexpr int rhs2_threads() { return NX * rhs2_ty<NX>(); }
+
+template <int CDS, int NX>
+__global__ __launch_bounds__(rhs2_threads<NX>()) void k_rhs_mom2(StepArgs A)
+{
+    constexpr int TY = rhs2_ty<NX>();
+    constexpr int T = rhs2_threads<NX>(), NL = NX / 2;
+    using Tl = Tile2<NX, TY>;
+    static_assert(Tl::NRAW >= 3 * TY, "FFT rows must fit in the staging area");
+    constexpr int NFWV = (TY + 3) * NX, PF = (NFWV + T - 1) / T;
+    __shared__ double sm[Tl::NROWS * NX];
+    const int tid = threadIdx.x;
+    const int nty = A.ny / TY;
+    const int logical = xcd_remap(blockIdx.x, nty * A.nz);
+    const int jt = logical % nty, k = logical / nty, j0 = jt * TY;
+    const int i = tid % NX, jj = tid / NX, j = j0 + jj;
+    const bool top = k == A.nz - 1;
+    for (int q = tid; q < Tl::NRAW * NX; q += T) {
+        int r = q / NX, x = q - r * NX;
+        sm[q] = tile2_raw<NX, TY>(A, r, x, j0, k);
+    }
+    PointVals2 pv;
+    pv.load(A, i, j, k);
+    // v(k+1) under the why*vhz rows, loaded with the staging
+    double vkp[PF];
+#pragma unroll
+    for (int m = 0; m < PF; ++m) {
+        const int e = tid + m * T;
+        vkp[m] = 0.0;
+        if ((NFWV % T == 0 || e < NFWV) && !top) {
+            const int r = e / NX, x = e - r * NX;
+            vkp[m] = A.v[((size_t)(k + 1) * A.ny + wrapi(j0 - 2 + r, A.ny)) * NX + x];
+        }
+    }
+    const double dpx = A.dp_opt == 2 ? A.scal[0] : A.dpx;
+    const double um = A.u_mrf;
+    const double lt = top ? 0.0 : A.lev[LV_LT * A.nz + k];
+    __syncthreads();
+    // ---- phase 1: interpolants and flux products of level k
+    {
+        const double *ru = sm + (Tl::OU + jj + 3) * NX, *rw = sm + (Tl::OW + jj + 3) * NX;
+        const double uhx = ixm<CDS, NX>(ru, i);
+        const double whx = top ? 0.0 : ixp<CDS, NX>(rw, i);
+        const double uc = ru[i];
+        const double uhz = top ? (A.ochannel == 0 ? -um : uc) : pv.u_kp * lt + uc * (1.0 - lt);
+        sm[(Tl::OUHX + jj) * NX + i] = uhx;
+        sm[(Tl::OWHX + jj) * NX + i] = whx;
+        sm[(Tl::OFUW + jj) * NX + i] = uhz * whx;
+    }
+    for (int e = tid; e < Tl::NH1 * NX; e += T) {
+        const int r = e / NX, x = e - r * NX;
+        if (r < TY + 1) {  // vhx, rows j0-1+r
+            sm[(Tl::OVHX + r) * NX + x] = ixp<CDS, NX>(sm + (Tl::OV + r + 2) * NX, x);
+        } else {           // vhy, rows j0-1+r'
+            const int r2 = r - (TY + 1);
+            sm[(Tl::OVHY + r2) * NX + x] = iym<CDS, NX>(sm + (Tl::OV + r2 + 2) * NX + x);
+        }
+    }
+    for (int e = tid; e < (TY + 3) * NX; e += T) {  // uhy*vhx, rows j0-2+r
+        const int r = e / NX, x = e - r * NX;
+        const double uhy = iyp<CDS, NX>(sm + (Tl::OU + r + 1) * NX + x);
+        const double vhx = ixp<CDS, NX>(sm + (Tl::OV + r + 1) * NX, x);
+        sm[(Tl::OFUV + r) * NX + x] = uhy * vhx;
+    }
+#pragma unroll
+    for (int m = 0; m < PF; ++m) {  // why*vhz, rows j0-2+r
+        const int e = tid + m * T;
+        if (NFWV % T == 0 || e < NFWV) {
+            const int r = e / NX, x = e - r * NX;
+            const double vc = sm[(Tl::OV + r + 1) * NX + x];
+            const double why = top ? 0.0 : iyp<CDS, NX>(sm + (Tl::OW + r + 1) * NX + x);
+            const double vhz = top ? (A.ochannel == 0 ? 0.0 : vc) : vkp[m] * lt + vc * (1.0 - lt);
+            sm[(Tl::OFWV + r) * NX + x] = vhz * why;
+        }
+    }
+    __syncthreads();
+    // ---- phase 2
+    double qu, qv, qw;
+    {
+        double f, g, h;
+        momentum_tile<CDS, NX, TY>(A, sm, i, jj, k, pv, dpx, f, g, h, qu, qv, qw);
+        const size_t c = ((size_t)k * A.ny + j) * NX + i;
+        A.ff[c] = f;
+        A.gg[c] = g;
+        A.hh[c] = h;
+    }
+    __syncthreads();  // staging no longer read: reuse it for the q rows
+    sm[(0 * TY + jj) * NX + i] = qu;
+    sm[(1 * TY + jj) * NX + i] = qv;
+    sm[(2 * TY + jj) * NX + i] = qw;
+    __syncthreads();
+    double2 *sq_ = reinterpret_cast<double2 *>(sm);
+    lds_fft<NL, 3 * TY, NL, 1, T, -1>(sq_, A.tw_nl, tid);
+    double2 *outs[3] = {A.s0, A.s1, A.s2};
+    for (int p = tid; p < 3 * TY * NL; p += T) {
+        int row = p / NL, l = p - row * NL, fi = row / TY, r = row - fi * TY;
+        outs[fi][((size_t)k * A.ny + j0 + r) * NL + l] = r2c_post<NL>(sq_ + row * NL, l, A.tw_nx, A.norm);
+    }
+}
+
 // Momentum RHS + x R2C of qu,qv,qw through the global-memory accessor of a
 // damped substep (PtD).  Block: rows_per_block full rows of level k.
 template <int CDS, int NX>
@@ -1110,7 +1218,13 @@ static int substep(hc_ctx *c, int t_its)
 #undef HC_RHSD
     } else {
 #define HC_RHS(NX_)                                                                                   \
-    {                                                                                                  \
+    if constexpr (NX_ <= 256) {                                                                        \
+        constexpr int TY_ = rhs2_ty<NX_>();                                                            \
+        dim3 g1((unsigned)(c->ny / TY_ * c->nz));                                                      \
+        constexpr int T_ = rhs2_threads<NX_>();                                                        \
+        if (P.cds == 1) hipLaunchKernelGGL((k_rhs_mom2<1, NX_>), g1, dim3(T_), 0, c->stream, A);       \
+        else hipLaunchKernelGGL((k_rhs_mom2<2, NX_>), g1, dim3(T_), 0, c->stream, A);                  \
+    } else {                                                                                           \
         constexpr int TY_ = rhs_ty<NX_>();                                                             \
         dim3 g1((unsigned)(c->ny / TY_ * c->nz));                                                      \
         constexpr int T_ = rhs_threads<NX_>();                                                         \

@@ -586,3 +586,237 @@ HC_DEV int xcd_remap(int b, int n)
     int q = n / X, r = n % X, x = b % X, idx = b / X;
     return (x < r ? x * (q + 1) : r * (q + 1) + (x - r) * q) + idx;
 }
+
+// ---------------------------------------------------------------------------
+// Two-phase momentum tile.  Level k, full x rows j0..j0+TY-1.  Phase 1
+// evaluates every mid-point interpolant and convective flux product of the
+// tile once into LDS rows; phase 2 forms the flux divergences, Laplacians and
+// pressure gradients of get_momentum_cnvdiff (src/module_cnvdiff.f90:35-374)
+// from those rows.  The interpolants and their wall overrides are the ones of
+// get_interp_fields_uvwhxyz + update_boundary_uvwhxyz
+// (src/module_tools.f90:410-470, src/module_boundary.f90:202-290).
+template <int NX, int TY>
+struct Tile2 {
+    static constexpr int RU = TY + 6;  // raw u, v, w: rows j0-3 .. j0+TY+2
+    static constexpr int RP = TY + 3;  // raw p:       rows j0-1 .. j0+TY+1
+    static constexpr int OU = 0, OV = RU, OW = 2 * RU, OP = 3 * RU, NRAW = 3 * RU + RP;
+    static constexpr int OUHX = NRAW;           // uhx          rows j0   .. j0+TY-1
+    static constexpr int OWHX = OUHX + TY;      // whx          rows j0   .. j0+TY-1
+    static constexpr int OFUW = OWHX + TY;      // uhz*whx      rows j0   .. j0+TY-1
+    static constexpr int OVHX = OFUW + TY;      // vhx          rows j0-1 .. j0+TY-1
+    static constexpr int OVHY = OVHX + TY + 1;  // vhy          rows j0-1 .. j0+TY+1
+    static constexpr int OFUV = OVHY + TY + 3;  // uhy*vhx      rows j0-2 .. j0+TY
+    static constexpr int OFWV = OFUV + TY + 3;  // why*vhz      rows j0-2 .. j0+TY
+    static constexpr int NROWS = OFWV + TY + 3;
+    static constexpr int NH1 = 2 * TY + 4;      // rows of the VHX + VHY block
+};
+
+// Raw row r of a Tile2 at column i (interior values of level k).
+template <int NX, int TY>
+HC_DEV double tile2_raw(const StepArgs &A, int r, int i, int j0, int k)
+{
+    using Tl = Tile2<NX, TY>;
+    auto at = [&](int jj) -> size_t { return ((size_t)k * A.ny + wrapi(jj, A.ny)) * NX + i; };
+    if (r < Tl::OV) return A.u[at(j0 - 3 + r)];
+    if (r < Tl::OW) return A.v[at(j0 - 3 + (r - Tl::OV))];
+    if (r < Tl::OP) return A.w[at(j0 - 3 + (r - Tl::OW))];
+    return A.p[at(j0 - 1 + (r - Tl::OP))];
+}
+
+// x interpolants on one LDS row: to the face between i-1 and i (uhx form)
+// and between i and i+1 (vhx / whx form); y interpolants on a column of rows.
+template <int CDS, int NX>
+HC_DEV double ixm(const double *row, int i)
+{
+    if (CDS == 1) return 0.5 * (row[wrapx<NX>(i)] + row[wrapx<NX>(i - 1)]);
+    return (-row[wrapx<NX>(i + 1)] + 9.0 * row[wrapx<NX>(i)] + 9.0 * row[wrapx<NX>(i - 1)] - row[wrapx<NX>(i - 2)]) / 16.0;
+}
+template <int CDS, int NX>
+HC_DEV double ixp(const double *row, int i)
+{
+    if (CDS == 1) return 0.5 * (row[wrapx<NX>(i + 1)] + row[wrapx<NX>(i)]);
+    return (-row[wrapx<NX>(i + 2)] + 9.0 * row[wrapx<NX>(i + 1)] + 9.0 * row[wrapx<NX>(i)] - row[wrapx<NX>(i - 1)]) / 16.0;
+}
+// col points at row r of the staged block, i at column i: value at row r+d is col[d*NX]
+template <int CDS, int NX>
+HC_DEV double iym(const double *col)  // vhy form: between r-1 and r
+{
+    if (CDS == 1) return 0.5 * (col[0] + col[-NX]);
+    return (-col[NX] + 9.0 * col[0] + 9.0 * col[-NX] - col[-2 * NX]) / 16.0;
+}
+template <int CDS, int NX>
+HC_DEV double iyp(const double *col)  // uhy / why form: between r and r+1
+{
+    if (CDS == 1) return 0.5 * (col[NX] + col[0]);
+    return (-col[2 * NX] + 9.0 * col[NX] + 9.0 * col[0] - col[-NX]) / 16.0;
+}
+
+// Single-point values of one tile point, loaded ahead of the staging barrier:
+// the k-1 / k+1 neighbours, the previous right-hand sides and w at level k-1
+// where the k-1 interpolants whx, why need it.
+struct PointVals2 {
+    double u_km, v_km, u_kp, v_kp, w_kp, p_kp, t_k, t_kp, f0, g0, h0;
+    double wkx[4];  // w(k-1) at x-1, x, x+1, x+2 (row j)
+    double wky[3];  // w(k-1) at rows j-1, j+1, j+2 (column x)
+    HC_DEV void load(const StepArgs &a, int i, int j, int k)
+    {
+        Pt g(a, i, j, k);
+        u_km = g.U(0, 0, -1);
+        v_km = g.V(0, 0, -1);
+        u_kp = g.U(0, 0, 1);
+        v_kp = g.V(0, 0, 1);
+        const bool in = k + 1 < a.nz;
+        w_kp = in ? g.W(0, 0, 1) : 0.0;
+        p_kp = g.P(0, 0, 1);
+        t_k = a.isscalar ? g.T(0, 0, 0) : 0.0;
+        t_kp = (a.isscalar && in) ? g.T(0, 0, 1) : 0.0;
+        f0 = g.f0();
+        g0 = g.g0();
+        h0 = g.h0();
+        if (k > 0) {
+#pragma unroll
+            for (int d = 0; d < 4; ++d) wkx[d] = g.W(d - 1, 0, -1);
+            wky[0] = g.W(0, -1, -1);
+            wky[1] = g.W(0, 1, -1);
+            wky[2] = g.W(0, 2, -1);
+        } else {
+#pragma unroll
+            for (int d = 0; d < 4; ++d) wkx[d] = 0.0;
+            wky[0] = wky[1] = wky[2] = 0.0;
+        }
+    }
+};
+
+// Phase 2 of the two-phase tile at point (i, j0+jj): ff, gg, hh and
+// qu, qv, qw.  Same terms and coefficients as momentum_point.
+template <int CDS, int NX, int TY>
+HC_DEV void momentum_tile(const StepArgs &A, const double *sm, int i, int jj, int k, const PointVals2 &pv, double dpx,
+                          double &f, double &g, double &h, double &qu, double &qv, double &qw)
+{
+    using Tl = Tile2<NX, TY>;
+    auto R = [&](int base, int r, int di) -> double { return sm[(base + r) * NX + wrapx<NX>(i + di)]; };
+    auto U = [&](int di, int dj) { return R(Tl::OU, jj + 3 + dj, di); };
+    auto V = [&](int di, int dj) { return R(Tl::OV, jj + 3 + dj, di); };
+    auto W = [&](int di, int dj) { return R(Tl::OW, jj + 3 + dj, di); };
+    auto P = [&](int di, int dj) { return R(Tl::OP, jj + 1 + dj, di); };
+    auto UHX = [&](int di) { return R(Tl::OUHX, jj, di); };
+    auto WHX = [&](int di) { return R(Tl::OWHX, jj, di); };
+    auto FUW = [&](int di) { return R(Tl::OFUW, jj, di); };
+    auto VHX = [&](int di, int dj) { return R(Tl::OVHX, jj + 1 + dj, di); };
+    auto VHY = [&](int dj) { return R(Tl::OVHY, jj + 1 + dj, 0); };
+    auto FUV = [&](int di, int dj) { return R(Tl::OFUV, jj + 2 + dj, di); };
+    auto FWV = [&](int dj) { return R(Tl::OFWV, jj + 2 + dj, 0); };
+    const double um = A.u_mrf;
+    const double idz = A.lev[LV_IDZ * A.nz + k], idzt = A.lev[LV_IDZT * A.nz + k], idzb = A.lev[LV_IDZB * A.nz + k];
+    const double u0 = U(0, 0), v0 = V(0, 0), w0 = W(0, 0);
+    // k-1 flux products at the point (uhz*whx, vhz*why), wall rules at k-1 = -1
+    double fuw_km, fwv_km;
+    if (k == 0) {
+        fuw_km = -um * 0.0;
+        fwv_km = 0.0 * 0.0;
+    } else {
+        const double lt = A.lev[LV_LT * A.nz + k - 1];
+        const double uhz = u0 * lt + pv.u_km * (1.0 - lt);
+        const double vhz = v0 * lt + pv.v_km * (1.0 - lt);
+        double whx, why;
+        if (CDS == 1) {
+            whx = 0.5 * (pv.wkx[2] + pv.wkx[1]);
+            why = 0.5 * (pv.wky[1] + pv.wkx[1]);
+        } else {
+            whx = (-pv.wkx[3] + 9.0 * pv.wkx[2] + 9.0 * pv.wkx[1] - pv.wkx[0]) / 16.0;
+            why = (-pv.wky[2] + 9.0 * pv.wky[1] + 9.0 * pv.wkx[1] - pv.wky[0]) / 16.0;
+        }
+        fuw_km = uhz * whx;
+        fwv_km = vhz * why;
+    }
+    // ---- U equation (:55-161)
+    {
+        double uux, uvy, ux, uxx, uyy, dpdx;
+        if (CDS == 1) {
+            const double h1 = UHX(1), h0 = UHX(0);
+            uux = (sq(h1) - sq(h0)) * A.idx;
+            uvy = (FUV(0, 0) - FUV(0, -1)) * A.idy;
+            ux = (h1 - h0) * A.idx;
+            uxx = (U(1, 0) - 2.0 * u0 + U(-1, 0)) * A.idx2;
+            uyy = (U(0, 1) - 2.0 * u0 + U(0, -1)) * A.idy2;
+            dpdx = (P(1, 0) - P(0, 0)) * A.idx;
+        } else {
+            const double h2 = UHX(2), h1 = UHX(1), h0 = UHX(0), hm = UHX(-1);
+            uux = (-sq(h2) + 27.0 * sq(h1) - 27.0 * sq(h0) + sq(hm)) * A.c24x;
+            uvy = (-FUV(0, 1) + 27.0 * FUV(0, 0) - 27.0 * FUV(0, -1) + FUV(0, -2)) * A.c24y;
+            ux = (-h2 + 27.0 * h1 - 27.0 * h0 + hm) * A.c24x;
+            uxx = (-U(2, 0) + 16.0 * U(1, 0) - 30.0 * u0 + 16.0 * U(-1, 0) - U(-2, 0)) * A.c12x2;
+            uyy = (-U(0, 2) + 16.0 * U(0, 1) - 30.0 * u0 + 16.0 * U(0, -1) - U(0, -2)) * A.c12y2;
+            dpdx = (-P(2, 0) + 27.0 * P(1, 0) - 27.0 * P(0, 0) + P(-1, 0)) * A.c24x;
+        }
+        const double uwz = (FUW(0) - fuw_km) * idz;
+        const double uzz = ((pv.u_kp - u0) * idzt - (u0 - pv.u_km) * idzb) * idz;
+        if (A.dp_opt == 3)
+            f = -uux - uvy - uwz - um * ux + A.fc * (0.5 * (VHX(0, 0) + VHX(0, -1)) - A.vg);
+        else
+            f = -uux - uvy - uwz + dpx - um * ux;
+        qu = u0 + A.dt * (A.ra * f + A.rb * pv.f0) + A.rc * 0.5 * A.dt * A.nu * (uxx + uyy + uzz) - A.rc * A.dt * dpdx;
+    }
+    // ---- V equation (:166-270)
+    {
+        double vux, vvy, vx, vxx, vyy, dpdy;
+        if (CDS == 1) {
+            vux = (FUV(0, 0) - FUV(-1, 0)) * A.idx;
+            vvy = (sq(VHY(1)) - sq(VHY(0))) * A.idy;
+            vx = (VHX(0, 0) - VHX(-1, 0)) * A.idx;
+            vxx = (V(1, 0) - 2.0 * v0 + V(-1, 0)) * A.idx2;
+            vyy = (V(0, 1) - 2.0 * v0 + V(0, -1)) * A.idy2;
+            dpdy = (P(0, 1) - P(0, 0)) * A.idy;
+        } else {
+            vux = (-FUV(1, 0) + 27.0 * FUV(0, 0) - 27.0 * FUV(-1, 0) + FUV(-2, 0)) * A.c24x;
+            vvy = (-sq(VHY(2)) + 27.0 * sq(VHY(1)) - 27.0 * sq(VHY(0)) + sq(VHY(-1))) * A.c24y;
+            vx = (-VHX(1, 0) + 27.0 * VHX(0, 0) - 27.0 * VHX(-1, 0) + VHX(-2, 0)) * A.c24x;
+            vxx = (-V(2, 0) + 16.0 * V(1, 0) - 30.0 * v0 + 16.0 * V(-1, 0) - V(-2, 0)) * A.c12x2;
+            vyy = (-V(0, 2) + 16.0 * V(0, 1) - 30.0 * v0 + 16.0 * V(0, -1) - V(0, -2)) * A.c12y2;
+            dpdy = (-P(0, 2) + 27.0 * P(0, 1) - 27.0 * P(0, 0) + P(0, -1)) * A.c24y;
+        }
+        const double vwz = (FWV(0) - fwv_km) * idz;
+        const double vzz = ((pv.v_kp - v0) * idzt - (v0 - pv.v_km) * idzb) * idz;
+        if (A.dp_opt == 3) {
+            // uhy at (i, j) and (i-1, j), from the staged u rows
+            const double *cu = sm + (Tl::OU + jj + 3) * NX;
+            const double a0 = iyp<CDS, NX>(cu + wrapx<NX>(i)), a1 = iyp<CDS, NX>(cu + wrapx<NX>(i - 1));
+            g = -vux - vvy - vwz - um * vx + A.fc * (A.ug - 0.5 * (a0 + a1) - um);
+        } else {
+            g = -vux - vvy - vwz - um * vx;
+        }
+        qv = v0 + A.dt * (A.ra * g + A.rb * pv.g0) + A.rc * 0.5 * A.dt * A.nu * (vxx + vyy + vzz) - A.rc * A.dt * dpdy;
+    }
+    // ---- W equation (:274-370), k < kend3
+    if (k < A.nz - 1) {
+        double wux, wvy, wx, wxx, wyy;
+        if (CDS == 1) {
+            wux = (FUW(0) - FUW(-1)) * A.idx;
+            wvy = (FWV(0) - FWV(-1)) * A.idy;
+            wx = (WHX(0) - WHX(-1)) * A.idx;
+            wxx = (W(1, 0) - 2.0 * w0 + W(-1, 0)) * A.idx2;
+            wyy = (W(0, 1) - 2.0 * w0 + W(0, -1)) * A.idy2;
+        } else {
+            wux = (-FUW(1) + 27.0 * FUW(0) - 27.0 * FUW(-1) + FUW(-2)) * A.c24x;
+            wvy = (-FWV(1) + 27.0 * FWV(0) - 27.0 * FWV(-1) + FWV(-2)) * A.c24y;
+            wx = (-WHX(1) + 27.0 * WHX(0) - 27.0 * WHX(-1) + WHX(-2)) * A.c24x;
+            wxx = (-W(2, 0) + 16.0 * W(1, 0) - 30.0 * w0 + 16.0 * W(-1, 0) - W(-2, 0)) * A.c12x2;
+            wyy = (-W(0, 2) + 16.0 * W(0, 1) - 30.0 * w0 + 16.0 * W(0, -1) - W(0, -2)) * A.c12y2;
+        }
+        const double wp = pv.w_kp, wm = pv.wkx[1];
+        const double a1 = sq(0.5 * (w0 + wp)), a2 = sq(0.5 * (w0 + wm));
+        const double wwz = (a1 - a2) * idzt;
+        const double wzz = ((wp - w0) * A.lev[LV_IDZ * A.nz + k + 1] - (w0 - wm) * idz) * idzt;
+        if (A.isscalar) {
+            const double lt = A.lev[LV_LT * A.nz + k];
+            h = -wux - wvy - wwz - um * wx + A.got * ((1.0 - lt) * pv.t_k + lt * pv.t_kp - A.t_ref);
+        } else {
+            h = -wux - wvy - wwz - um * wx;
+        }
+        const double dpdz = (pv.p_kp - P(0, 0)) * idzt;
+        qw = w0 + A.dt * (A.ra * h + A.rb * pv.h0) + A.rc * 0.5 * A.dt * A.nu * (wxx + wyy + wzz) - A.rc * A.dt * dpdz;
+    } else {
+        h = 0.0;
+        qw = 0.0;
+    }
+}
