@@ -55,6 +55,8 @@ typedef struct ho_state {
     double *aa, *cc;
     cplx *bb;
     cplx *c_src, *c_var, *rowbuf;
+    /* spectral path (cds=3): C2C spectra of the fields and right-hand sides */
+    cplx *c_u, *c_v, *c_w, *c_p, *c_t, *c_phi, *c_ff, *c_gg, *c_hh, *c_ss, *c_qu, *c_qv, *c_qw;
     double div_avg, div_max, div_hist_max, time_sim;
     int stps;
     char err[256];
@@ -173,6 +175,11 @@ ho_state *ho_create(const ho_params *P)
     S->bb_u = calloc_c(nb); S->bb_w = calloc_c(nb); S->bb_p = calloc_c(nb); S->bb_t = calloc_c(nb);
     S->aa = dalloc(nb); S->cc = dalloc(nb); S->bb = calloc_c(nb);
     S->c_src = calloc_c(nc); S->c_var = calloc_c(nc);
+    if (P->cds == 3) {
+        cplx **cs[] = {&S->c_u, &S->c_v, &S->c_w, &S->c_p, &S->c_t, &S->c_phi, &S->c_ff, &S->c_gg,
+                       &S->c_hh, &S->c_ss, &S->c_qu, &S->c_qv, &S->c_qw};
+        for (size_t q = 0; q < sizeof(cs) / sizeof(cs[0]); ++q) *cs[q] = calloc_c(nc);
+    }
     int nmax = P->nx > P->ny ? P->nx : P->ny;
     S->rowbuf = calloc_c((size_t)4 * nmax);
     return S;
@@ -189,6 +196,9 @@ void ho_destroy(ho_state *S)
     for (size_t q = 0; q < sizeof(fl) / sizeof(fl[0]); ++q) free(fl[q]);
     free(S->bb_u); free(S->bb_w); free(S->bb_p); free(S->bb_t); free(S->bb);
     free(S->c_src); free(S->c_var); free(S->rowbuf);
+    cplx *cs[] = {S->c_u, S->c_v, S->c_w, S->c_p, S->c_t, S->c_phi, S->c_ff, S->c_gg, S->c_hh, S->c_ss,
+                  S->c_qu, S->c_qv, S->c_qw};
+    for (size_t q = 0; q < sizeof(cs) / sizeof(cs[0]); ++q) free(cs[q]);
     free(S);
 }
 
@@ -825,17 +835,15 @@ static void assign_abc(ho_state *S, const double *aa_in, const cplx *bb_in, cons
 }
 
 /* ---- A10: poisson_solver_fft, src/module_poisson_solver.f90:483-658 ---- */
-static int poisson_solver_fft(ho_state *S, const double *r_src, double *r_var,
-                              int zstagger, int delnull, int tbnd, int ubnd)
+/* Flag handling and Thomas sweeps shared with poisson_solver_fft_spec
+ * (src/module_spectral.f90:602-746, the same statements on a complex
+ * source): S->c_src (z-pencil spectrum) -> S->c_var, with aa/bb/cc set by
+ * assign_abc.  Levels above kmax3 stay zero. */
+static int spectral_thomas(ho_state *S, int zstagger, int delnull, int tbnd, int ubnd, int *kmax3_out)
 {
     const ho_params *P = &S->prm;
     int nx = P->nx, ny = P->ny, nz = P->nz, ks = S->kstr3, ke = S->kend3;
-    memset(r_var, 0, sizeof(double) * nfield(S));
     memset(S->c_var, 0, sizeof(cplx) * (size_t)nx * ny * nz);
-    for (int k = 1; k <= nz; ++k)
-        for (int j = 1; j <= ny; ++j)
-            for (int i = 1; i <= nx; ++i) C3(S, S->c_src, i, j, k) = F3(S, r_src, i + 2, j + 2, k + 2);
-    ho_fft2d(S, S->c_src, -1);
     int kmax3;
     if (zstagger == 1) kmax3 = ke - 1;
     else if (zstagger == 0) kmax3 = ke;
@@ -873,6 +881,22 @@ static int poisson_solver_fft(ho_state *S, const double *r_src, double *r_var,
                 C3(S, S->c_var, i, j, k - 2) = (C3(S, S->c_src, i, j, k - 2) - B3(S, S->cc, i, j, k) * C3(S, S->c_var, i, j, k - 1)) /
                                                B3(S, S->bb, i, j, k);
         }
+    *kmax3_out = kmax3;
+    return 0;
+}
+
+static int poisson_solver_fft(ho_state *S, const double *r_src, double *r_var,
+                              int zstagger, int delnull, int tbnd, int ubnd)
+{
+    const ho_params *P = &S->prm;
+    int nx = P->nx, ny = P->ny, nz = P->nz, ks = S->kstr3;
+    memset(r_var, 0, sizeof(double) * nfield(S));
+    for (int k = 1; k <= nz; ++k)
+        for (int j = 1; j <= ny; ++j)
+            for (int i = 1; i <= nx; ++i) C3(S, S->c_src, i, j, k) = F3(S, r_src, i + 2, j + 2, k + 2);
+    ho_fft2d(S, S->c_src, -1);
+    int kmax3;
+    if (spectral_thomas(S, zstagger, delnull, tbnd, ubnd, &kmax3)) return 1;
     ho_fft2d(S, S->c_var, 1);
     for (int k = ks; k <= kmax3; ++k)
         for (int j = 1; j <= ny; ++j)
@@ -984,6 +1008,328 @@ static int div_error(ho_state *S, int isfinal)
     return 0;
 }
 
+/* ---- A18: spectral path, src/module_spectral.f90 ------------------------- */
+/* Complex (nx, ny, nz) spectra in C3 layout, as the reference holds them
+ * (C2C, forward normalised, Nyquist zeroed by mpi_2d_fft). */
+static size_t ncplx(const ho_state *S) { return (size_t)S->prm.nx * S->prm.ny * S->prm.nz; }
+
+/* dev_fft_linear_cmplx, src/module_spectral.f90:1274-1340 */
+static void dev_linear(ho_state *S, const cplx *in, cplx *out, int ord, int dir)
+{
+    const ho_params *P = &S->prm;
+    for (int k = 1; k <= P->nz; ++k)
+        for (int j = 1; j <= P->ny; ++j)
+            for (int i = 1; i <= P->nx; ++i) {
+                int l = lwave(i, P->nx), m = lwave(j, P->ny);
+                cplx c = C3(S, in, i, j, k);
+                if (ord == 1 && dir == 1) c = c * I * 2.0 * PI / P->lx * l;
+                else if (ord == 1 && dir == 2) c = c * I * 2.0 * PI / P->ly * m;
+                else if (ord == 2 && dir == 1) c = -c * SQ(2.0 * PI / P->lx * l);
+                else c = -c * SQ(2.0 * PI / P->ly * m);
+                if (l == -P->nx / 2 || m == -P->ny / 2) c = 0;
+                C3(S, out, i, j, k) = c;
+            }
+}
+
+/* dev_fft_nonlinear_cmplx, src/module_spectral.f90:1061-1150: FFT of the
+ * product r1*r2 (r1 interpolated to the w level when weqn=1), then the
+ * first derivative in direction dir */
+static void dev_nonlinear(ho_state *S, const double *r1, const double *r2, cplx *out, int dir, int weqn)
+{
+    const ho_params *P = &S->prm;
+    for (int k = 1; k <= P->nz; ++k)
+        for (int j = 1; j <= P->ny; ++j)
+            for (int i = 1; i <= P->nx; ++i) {
+                double a = F3(S, r1, i + 2, j + 2, k + 2);
+                if (weqn == 1) {
+                    double lt = V1(S->l_t, k + 2);
+                    a = F3(S, r1, i + 2, j + 2, k + 3) * lt + F3(S, r1, i + 2, j + 2, k + 2) * (1.0 - lt);
+                }
+                C3(S, out, i, j, k) = a * F3(S, r2, i + 2, j + 2, k + 2);
+            }
+    ho_fft2d(S, out, -1);
+    for (int k = 1; k <= P->nz; ++k)
+        for (int j = 1; j <= P->ny; ++j)
+            for (int i = 1; i <= P->nx; ++i) {
+                int l = lwave(i, P->nx), m = lwave(j, P->ny);
+                cplx c = C3(S, out, i, j, k);
+                c = dir == 1 ? c * I * 2.0 * PI / P->lx * l : c * I * 2.0 * PI / P->ly * m;
+                if (l == -P->nx / 2 || m == -P->ny / 2) c = 0;
+                C3(S, out, i, j, k) = c;
+            }
+}
+
+/* real_2_cmplx / cmplx_2_real, src/module_spectral.f90:1157-1217 */
+static void real_2_cmplx(ho_state *S, const double *r, cplx *c)
+{
+    const ho_params *P = &S->prm;
+    for (int k = 1; k <= P->nz; ++k)
+        for (int j = 1; j <= P->ny; ++j)
+            for (int i = 1; i <= P->nx; ++i) C3(S, c, i, j, k) = F3(S, r, i + 2, j + 2, k + 2);
+    ho_fft2d(S, c, -1);
+}
+static void cmplx_2_real(ho_state *S, const cplx *c, double *r)
+{
+    const ho_params *P = &S->prm;
+    memcpy(S->c_var, c, sizeof(cplx) * ncplx(S));
+    ho_fft2d(S, S->c_var, 1);
+    for (int k = 1; k <= P->nz; ++k)
+        for (int j = 1; j <= P->ny; ++j)
+            for (int i = 1; i <= P->nx; ++i) F3(S, r, i + 2, j + 2, k + 2) = creal(C3(S, S->c_var, i, j, k));
+}
+
+/* spectral_truncation (2/3 rule), src/module_spectral.f90:1222-1268 */
+static void truncation(ho_state *S, cplx *c)
+{
+    const ho_params *P = &S->prm;
+    int lc = (int)(P->nx / 3.0), mc = (int)(P->ny / 3.0);
+    for (int k = 1; k <= P->nz; ++k)
+        for (int j = 1; j <= P->ny; ++j)
+            for (int i = 1; i <= P->nx; ++i) {
+                int l = lwave(i, P->nx), m = lwave(j, P->ny);
+                if (l >= lc || l <= -lc || m >= mc || m <= -mc) C3(S, c, i, j, k) = 0;
+            }
+}
+
+/* out += coef * in over the whole spectrum */
+static void caxpy(ho_state *S, cplx *out, cplx coef, const cplx *in)
+{
+    size_t n = ncplx(S);
+    for (size_t q = 0; q < n; ++q) out[q] = out[q] + coef * in[q];
+}
+
+/* one physical-space term over the interior, stored into c_var and
+ * transformed (the loops of get_*_cnvdiff_spec that end in mpi_2d_fft(c_var,-1)) */
+#define PHYS_TERM(S, k_lo, k_hi, EXPR)                                                   \
+    do {                                                                                 \
+        for (int k = (k_lo); k <= (k_hi); ++k)                                           \
+            for (int j = (S)->jstr3; j <= (S)->jend3; ++j)                               \
+                for (int i = (S)->istr3; i <= (S)->iend3; ++i)                           \
+                    C3(S, (S)->c_var, i - 2, j - 2, k - 2) = (EXPR);                     \
+        ho_fft2d(S, (S)->c_var, -1);                                                     \
+    } while (0)
+
+/* the advective form r * IFFT(i k c) of the skew-symmetric terms: IFFT into
+ * c_src, then the physical product into c_var, transformed */
+static void skew_term(ho_state *S, const cplx *c, int dir, const double *r, int wlev)
+{
+    const ho_params *P = &S->prm;
+    dev_linear(S, c, S->c_src, 1, dir);
+    ho_fft2d(S, S->c_src, 1);
+    for (int k = S->kstr3; k <= S->kend3; ++k)
+        for (int j = S->jstr3; j <= S->jend3; ++j)
+            for (int i = S->istr3; i <= S->iend3; ++i) {
+                double a = F3(S, r, i, j, k);
+                if (wlev) a = V1(S->l_t, k) * F3(S, r, i, j, k + 1) + (1.0 - V1(S->l_t, k)) * F3(S, r, i, j, k);
+                C3(S, S->c_var, i - 2, j - 2, k - 2) = a * creal(C3(S, S->c_src, i - 2, j - 2, k - 2));
+            }
+    ho_fft2d(S, S->c_var, -1);
+    (void)P;
+}
+
+/* get_momentum_cnvdiff_spec, src/module_spectral.f90:35-502 */
+static void momentum_spec(ho_state *S)
+{
+    const ho_params *P = &S->prm;
+    double *u = S->u, *v = S->v, *w = S->w, *p = S->p, *t = S->t;
+    const double coe = P->issk == 1 ? 0.5 : 1.0;
+    const double dif = S->rk_c * 0.5 * P->dt * P->nu;
+    size_t n = ncplx(S);
+    update_dpx(S);
+    /* ---- U (:61-201) */
+    memcpy(S->c_qu, S->c_u, n * sizeof(cplx));
+    dev_linear(S, S->c_u, S->c_var, 2, 1); caxpy(S, S->c_qu, dif, S->c_var);
+    dev_linear(S, S->c_u, S->c_var, 2, 2); caxpy(S, S->c_qu, dif, S->c_var);
+    PHYS_TERM(S, S->kstr3, S->kend3,
+              ((F3(S, u, i, j, k + 1) - F3(S, u, i, j, k)) / V1(S->dz_t, k) -
+               (F3(S, u, i, j, k) - F3(S, u, i, j, k - 1)) / V1(S->dz_b, k)) / V1(S->dz, k));
+    caxpy(S, S->c_qu, dif, S->c_var);
+    dev_linear(S, S->c_p, S->c_var, 1, 1); caxpy(S, S->c_qu, -S->rk_c * P->dt, S->c_var);
+    caxpy(S, S->c_qu, S->rk_b * P->dt, S->c_ff);
+    memset(S->c_ff, 0, n * sizeof(cplx));
+    dev_linear(S, S->c_u, S->c_var, 1, 1); caxpy(S, S->c_ff, -P->u_mrf, S->c_var);
+    dev_nonlinear(S, u, u, S->c_var, 1, 0); caxpy(S, S->c_ff, -coe, S->c_var);
+    dev_nonlinear(S, u, v, S->c_var, 2, 0); caxpy(S, S->c_ff, -coe, S->c_var);
+#define A1U(f) (F3(S, f, i, j, k + 1) * V1(S->l_t, k) + F3(S, f, i, j, k) * (1.0 - V1(S->l_t, k)))
+#define A2U(f) (F3(S, f, i, j, k - 1) * V1(S->l_b, k) + F3(S, f, i, j, k) * (1.0 - V1(S->l_b, k)))
+    PHYS_TERM(S, S->kstr3, S->kend3,
+              1.0 / coe * (P->dp_opt == 3 ? P->fc * (F3(S, v, i, j, k) - P->vg) : S->dpx_drive) -
+                  (A1U(u) * F3(S, w, i, j, k) - A2U(u) * F3(S, w, i, j, k - 1)) / V1(S->dz, k));
+    caxpy(S, S->c_ff, coe, S->c_var);
+    if (P->issk == 1) {
+        skew_term(S, S->c_u, 1, u, 0); caxpy(S, S->c_ff, -coe, S->c_var);
+        skew_term(S, S->c_u, 2, v, 0); caxpy(S, S->c_ff, -coe, S->c_var);
+        PHYS_TERM(S, S->kstr3, S->kend3,
+                  (A1U(u) - A2U(u)) / V1(S->dz, k) * (0.5 * (F3(S, w, i, j, k) + F3(S, w, i, j, k - 1))));
+        caxpy(S, S->c_ff, -coe, S->c_var);
+    }
+    caxpy(S, S->c_qu, S->rk_a * P->dt, S->c_ff);
+    /* ---- V (:207-339) */
+    memcpy(S->c_qv, S->c_v, n * sizeof(cplx));
+    dev_linear(S, S->c_v, S->c_var, 2, 1); caxpy(S, S->c_qv, dif, S->c_var);
+    dev_linear(S, S->c_v, S->c_var, 2, 2); caxpy(S, S->c_qv, dif, S->c_var);
+    PHYS_TERM(S, S->kstr3, S->kend3,
+              ((F3(S, v, i, j, k + 1) - F3(S, v, i, j, k)) / V1(S->dz_t, k) -
+               (F3(S, v, i, j, k) - F3(S, v, i, j, k - 1)) / V1(S->dz_b, k)) / V1(S->dz, k));
+    caxpy(S, S->c_qv, dif, S->c_var);
+    dev_linear(S, S->c_p, S->c_var, 1, 2); caxpy(S, S->c_qv, -S->rk_c * P->dt, S->c_var);
+    caxpy(S, S->c_qv, S->rk_b * P->dt, S->c_gg);
+    memset(S->c_gg, 0, n * sizeof(cplx));
+    dev_linear(S, S->c_v, S->c_var, 1, 1); caxpy(S, S->c_gg, -P->u_mrf, S->c_var);
+    dev_nonlinear(S, u, v, S->c_var, 1, 0); caxpy(S, S->c_gg, -coe, S->c_var);
+    dev_nonlinear(S, v, v, S->c_var, 2, 0); caxpy(S, S->c_gg, -coe, S->c_var);
+    PHYS_TERM(S, S->kstr3, S->kend3,
+              1.0 / coe * (P->dp_opt == 3 ? P->fc * (P->ug - F3(S, u, i, j, k) - P->u_mrf) : 0.0) -
+                  (A1U(v) * F3(S, w, i, j, k) - A2U(v) * F3(S, w, i, j, k - 1)) / V1(S->dz, k));
+    caxpy(S, S->c_gg, coe, S->c_var);
+    if (P->issk == 1) {
+        skew_term(S, S->c_v, 1, u, 0); caxpy(S, S->c_gg, -coe, S->c_var);
+        skew_term(S, S->c_v, 2, v, 0); caxpy(S, S->c_gg, -coe, S->c_var);
+        PHYS_TERM(S, S->kstr3, S->kend3,
+                  (A1U(v) - A2U(v)) / V1(S->dz, k) * (0.5 * (F3(S, w, i, j, k) + F3(S, w, i, j, k - 1))));
+        caxpy(S, S->c_gg, -coe, S->c_var);
+    }
+    caxpy(S, S->c_qv, S->rk_a * P->dt, S->c_gg);
+#undef A1U
+#undef A2U
+    /* ---- W (:344-499) */
+    memcpy(S->c_qw, S->c_w, n * sizeof(cplx));
+    dev_linear(S, S->c_w, S->c_var, 2, 1); caxpy(S, S->c_qw, dif, S->c_var);
+    dev_linear(S, S->c_w, S->c_var, 2, 2); caxpy(S, S->c_qw, dif, S->c_var);
+    PHYS_TERM(S, S->kstr3, S->kend3,
+              ((F3(S, w, i, j, k + 1) - F3(S, w, i, j, k)) / V1(S->dz, k + 1) -
+               (F3(S, w, i, j, k) - F3(S, w, i, j, k - 1)) / V1(S->dz, k)) / V1(S->dz_t, k));
+    caxpy(S, S->c_qw, dif, S->c_var);
+    PHYS_TERM(S, S->kstr3, S->kend3, (F3(S, p, i, j, k + 1) - F3(S, p, i, j, k)) / V1(S->dz_t, k));
+    caxpy(S, S->c_qw, -S->rk_c * P->dt, S->c_var);
+    caxpy(S, S->c_qw, S->rk_b * P->dt, S->c_hh);
+    memset(S->c_hh, 0, n * sizeof(cplx));
+    dev_linear(S, S->c_w, S->c_var, 1, 1); caxpy(S, S->c_hh, -P->u_mrf, S->c_var);
+    dev_nonlinear(S, u, w, S->c_var, 1, 1); caxpy(S, S->c_hh, -coe, S->c_var);
+    dev_nonlinear(S, v, w, S->c_var, 2, 1); caxpy(S, S->c_hh, -coe, S->c_var);
+#define WP (0.5 * (F3(S, w, i, j, k) + F3(S, w, i, j, k + 1)))
+#define WM (0.5 * (F3(S, w, i, j, k) + F3(S, w, i, j, k - 1)))
+    PHYS_TERM(S, S->kstr3, S->kend3,
+              (P->isscalar == 1 ? 1.0 / coe * (S->got * ((1.0 - V1(S->l_t, k)) * F3(S, t, i, j, k) +
+                                                         V1(S->l_t, k) * F3(S, t, i, j, k + 1) - P->t_ref))
+                                : 0.0) -
+                  (WP * WP - WM * WM) / V1(S->dz_t, k));
+    caxpy(S, S->c_hh, coe, S->c_var);
+    if (P->issk == 1) {
+        skew_term(S, S->c_w, 1, u, 1); caxpy(S, S->c_hh, -coe, S->c_var);
+        skew_term(S, S->c_w, 2, v, 1); caxpy(S, S->c_hh, -coe, S->c_var);
+        PHYS_TERM(S, S->kstr3, S->kend3, F3(S, w, i, j, k) * ((WP - WM) / V1(S->dz_t, k)));
+        caxpy(S, S->c_hh, -coe, S->c_var);
+    }
+#undef WP
+#undef WM
+    caxpy(S, S->c_qw, S->rk_a * P->dt, S->c_hh);
+}
+
+/* get_temperature_cnvdiff_spec, src/module_spectral.f90:509-595 (qt into c_qu) */
+static void temperature_spec(ho_state *S)
+{
+    const ho_params *P = &S->prm;
+    double *u = S->u, *v = S->v, *w = S->w, *t = S->t;
+    const double dif = S->rk_c * 0.5 * P->dt * P->alpha;
+    size_t n = ncplx(S);
+    memcpy(S->c_qu, S->c_t, n * sizeof(cplx));
+    dev_linear(S, S->c_t, S->c_var, 2, 1); caxpy(S, S->c_qu, dif, S->c_var);
+    dev_linear(S, S->c_t, S->c_var, 2, 2); caxpy(S, S->c_qu, dif, S->c_var);
+    PHYS_TERM(S, S->kstr3, S->kend3,
+              ((F3(S, t, i, j, k + 1) - F3(S, t, i, j, k)) / V1(S->dz_t, k) -
+               (F3(S, t, i, j, k) - F3(S, t, i, j, k - 1)) / V1(S->dz_b, k)) / V1(S->dz, k));
+    caxpy(S, S->c_qu, dif, S->c_var);
+    caxpy(S, S->c_qu, S->rk_b * P->dt, S->c_ss);
+    memset(S->c_ss, 0, n * sizeof(cplx));
+    dev_linear(S, S->c_t, S->c_var, 1, 1); caxpy(S, S->c_ss, -P->u_mrf, S->c_var);
+    dev_nonlinear(S, u, t, S->c_var, 1, 0); caxpy(S, S->c_ss, -1.0, S->c_var);
+    dev_nonlinear(S, t, v, S->c_var, 2, 0); caxpy(S, S->c_ss, -1.0, S->c_var);
+    PHYS_TERM(S, S->kstr3, S->kend3,
+              ((F3(S, t, i, j, k + 1) * V1(S->l_t, k) + F3(S, t, i, j, k) * (1.0 - V1(S->l_t, k))) * F3(S, w, i, j, k) -
+               (F3(S, t, i, j, k - 1) * V1(S->l_b, k) + F3(S, t, i, j, k) * (1.0 - V1(S->l_b, k))) * F3(S, w, i, j, k - 1)) /
+                  V1(S->dz, k));
+    caxpy(S, S->c_ss, -1.0, S->c_var);
+    caxpy(S, S->c_qu, S->rk_a * P->dt, S->c_ss);
+}
+
+/* poisson_solver_fft_spec, src/module_spectral.f90:602-746 */
+static int poisson_spec(ho_state *S, const cplx *src, cplx *out, int zstagger, int delnull, int tbnd, int ubnd)
+{
+    memcpy(S->c_src, src, sizeof(cplx) * ncplx(S));
+    int kmax3;
+    if (spectral_thomas(S, zstagger, delnull, tbnd, ubnd, &kmax3)) return 1;
+    memcpy(out, S->c_var, sizeof(cplx) * ncplx(S));
+    return 0;
+}
+
+/* get_div_spec, src/module_spectral.f90:752-795 */
+static void div_spec(ho_state *S, const cplx *cu, const cplx *cv, const double *w, cplx *div)
+{
+    size_t n = ncplx(S);
+    dev_linear(S, cu, S->c_var, 1, 1);
+    memcpy(div, S->c_var, n * sizeof(cplx));
+    dev_linear(S, cv, S->c_var, 1, 2);
+    caxpy(S, div, 1.0, S->c_var);
+    PHYS_TERM(S, S->kstr3, S->kend3, (F3(S, w, i, j, k) - F3(S, w, i, j, k - 1)) / V1(S->dz, k));
+    caxpy(S, div, 1.0, S->c_var);
+}
+
+/* calculate_uvw_spec, src/module_spectral.f90:801-849 */
+static void uvw_spec(ho_state *S)
+{
+    const ho_params *P = &S->prm;
+    double *phi = S->phi;
+    dev_linear(S, S->c_phi, S->c_var, 1, 1); caxpy(S, S->c_u, -S->rk_c * P->dt, S->c_var);
+    dev_linear(S, S->c_phi, S->c_var, 1, 2); caxpy(S, S->c_v, -S->rk_c * P->dt, S->c_var);
+    /* levels kstr3..kend3-1 only; c_var keeps its previous content at level nz */
+    PHYS_TERM(S, S->kstr3, S->kend3 - 1, (F3(S, phi, i, j, k + 1) - F3(S, phi, i, j, k)) / V1(S->dz_t, k));
+    for (int k = 1; k <= P->nz - 1; ++k)
+        for (int j = 1; j <= P->ny; ++j)
+            for (int i = 1; i <= P->nx; ++i)
+                C3(S, S->c_w, i, j, k) = C3(S, S->c_w, i, j, k) - S->rk_c * P->dt * C3(S, S->c_var, i, j, k);
+}
+
+/* calculate_new_p_spec, src/module_spectral.f90:854-894 */
+static void new_p_spec(ho_state *S)
+{
+    const ho_params *P = &S->prm;
+    double *phi = S->phi;
+    const double dif = -S->rk_c * 0.5 * P->dt * P->nu;
+    caxpy(S, S->c_p, 1.0, S->c_phi);
+    dev_linear(S, S->c_phi, S->c_var, 2, 1); caxpy(S, S->c_p, dif, S->c_var);
+    dev_linear(S, S->c_phi, S->c_var, 2, 2); caxpy(S, S->c_p, dif, S->c_var);
+    PHYS_TERM(S, S->kstr3, S->kend3,
+              ((F3(S, phi, i, j, k + 1) - F3(S, phi, i, j, k)) / V1(S->dz_t, k) -
+               (F3(S, phi, i, j, k) - F3(S, phi, i, j, k - 1)) / V1(S->dz_b, k)) / V1(S->dz, k));
+    caxpy(S, S->c_p, dif, S->c_var);
+}
+
+/* screen_div_error_spec (values only), src/module_spectral.f90:899-1007 */
+static int div_error_spec(ho_state *S, int isfinal)
+{
+    const ho_params *P = &S->prm;
+    div_spec(S, S->c_u, S->c_v, S->w, S->c_src);
+    ho_fft2d(S, S->c_src, 1);
+    double avg = 0.0, mx = 0.0;
+    for (int k = S->kstr3; k <= S->kend3; ++k)
+        for (int j = S->jstr3; j <= S->jend3; ++j)
+            for (int i = S->istr3; i <= S->iend3; ++i) {
+                double d = creal(C3(S, S->c_src, i - 2, j - 2, k - 2));
+                avg = avg + fabs(d);
+                if (fabs(d) > mx) mx = fabs(d);
+                if (F3(S, S->u, i, j, k) != F3(S, S->u, i, j, k) || F3(S, S->v, i, j, k) != F3(S, S->v, i, j, k) ||
+                    F3(S, S->w, i, j, k) != F3(S, S->w, i, j, k)) {
+                    snprintf(S->err, sizeof S->err, "NaN error! %d %d %d", i, j, k);
+                    return 1;
+                }
+                if (isfinal == 1 && fabs(d) > S->div_hist_max) S->div_hist_max = fabs(d);
+            }
+    S->div_avg = avg / P->nx / P->ny / P->nz;
+    S->div_max = mx;
+    return 0;
+}
+
 /* ---- assign_rk_coeff, src/module_parameters.f90:618-641 ---------------- */
 /* ---- A19: rayleigh_damping, src/module_tools.f90:705-749 ---------------- */
 /* Top sponge on levels kend3-nzdamp..kend3: each interior value relaxes
@@ -1065,6 +1411,59 @@ static int time_advancement_fd(ho_state *S)
     return 0;
 }
 
+/* ---- A18: time_advancement_sp, src/module_navier_stokes.f90:371-517 ---- */
+static int time_advancement_sp(ho_state *S)
+{
+    const ho_params *P = &S->prm;
+    int t_its_max = P->dts == 2 ? 3 : 1;
+    for (int t_its = 1; t_its <= t_its_max; ++t_its) {
+        if (P->isdamp == 1) {
+            /* rayleigh_damping_spec reduces a complex mean as one real
+             * (src/module_spectral.f90:1372): its imaginary part is never set */
+            snprintf(S->err, sizeof S->err, "isdamp=1 with cds=3 has no defined reference result");
+            return 1;
+        }
+        assign_rk(S, t_its);                                              /* :417 */
+        momentum_spec(S);                                                 /* :420-421 */
+        assign_abc(S, S->aa_u, S->bb_u, S->cc_u, 0);                      /* :427-429 */
+        if (poisson_spec(S, S->c_qu, S->c_u, 0, 0, 0, 1)) return 1;
+        assign_abc(S, S->aa_u, S->bb_u, S->cc_u, 0);                      /* :433-435 */
+        if (poisson_spec(S, S->c_qv, S->c_v, 0, 0, 0, 0)) return 1;
+        assign_abc(S, S->aa_w, S->bb_w, S->cc_w, 0);                      /* :438-440 */
+        if (poisson_spec(S, S->c_qw, S->c_w, 1, 0, 0, 0)) return 1;
+        cmplx_2_real(S, S->c_w, S->w);                                    /* :444-445 */
+        bc_uvw(S);
+        div_spec(S, S->c_u, S->c_v, S->w, S->c_qu);                       /* :449-450 */
+        for (size_t q = 0; q < ncplx(S); ++q) S->c_qu[q] = S->c_qu[q] / P->dt;
+        assign_abc(S, S->aa_p, S->bb_p, S->cc_p, 1);                      /* :452-454 */
+        if (poisson_spec(S, S->c_qu, S->c_phi, 0, 1, 0, 0)) return 1;
+        cmplx_2_real(S, S->c_phi, S->phi);                                /* :457-458 */
+        bc_p(S, S->phi);
+        new_p_spec(S);                                                    /* :461 */
+        uvw_spec(S);                                                      /* :464 */
+        truncation(S, S->c_u);                                            /* :467-470 */
+        truncation(S, S->c_v);
+        truncation(S, S->c_w);
+        truncation(S, S->c_p);
+        cmplx_2_real(S, S->c_u, S->u);                                    /* :473-478 */
+        cmplx_2_real(S, S->c_v, S->v);
+        cmplx_2_real(S, S->c_w, S->w);
+        cmplx_2_real(S, S->c_p, S->p);
+        bc_p(S, S->p);
+        bc_uvw(S);
+        if (P->isscalar == 1) {                                           /* :481-498 */
+            temperature_spec(S);
+            assign_abc(S, S->aa_t, S->bb_t, S->cc_t, 0);
+            if (poisson_spec(S, S->c_qu, S->c_t, 0, 0, 1, 0)) return 1;
+            truncation(S, S->c_t);
+            cmplx_2_real(S, S->c_t, S->t);
+            bc_t(S);
+        }
+        if (div_error_spec(S, 1)) return 1;                               /* :501-507 */
+    }
+    return 0;
+}
+
 /* ---- drive, src/module_navier_stokes.f90:31-218 (numerics only) -------- */
 /* start-up: mesh, coefficients, bulk velocity and boundary values (:53-129) */
 int ho_init(ho_state *S)
@@ -1073,6 +1472,27 @@ int ho_init(ho_state *S)
     if (S->prm.is_ri_var == 1) S->got = S->prm.ri_str;  /* update_ri(0) */
     if (S->prm.dp_opt == 2) ho_calc_ubar(S);
     ho_coefficients(S);
+    if (S->prm.cds == 3) {  /* :96-123: spectra of the initial fields, truncated */
+        real_2_cmplx(S, S->u, S->c_u);
+        real_2_cmplx(S, S->v, S->c_v);
+        real_2_cmplx(S, S->w, S->c_w);
+        real_2_cmplx(S, S->p, S->c_p);
+        real_2_cmplx(S, S->t, S->c_t);
+        real_2_cmplx(S, S->ff, S->c_ff);
+        real_2_cmplx(S, S->gg, S->c_gg);
+        real_2_cmplx(S, S->hh, S->c_hh);
+        real_2_cmplx(S, S->ss, S->c_ss);
+        truncation(S, S->c_u);
+        truncation(S, S->c_v);
+        truncation(S, S->c_w);
+        truncation(S, S->c_p);
+        truncation(S, S->c_t);
+        cmplx_2_real(S, S->c_u, S->u);
+        cmplx_2_real(S, S->c_v, S->v);
+        cmplx_2_real(S, S->c_w, S->w);
+        cmplx_2_real(S, S->c_p, S->p);
+        cmplx_2_real(S, S->c_t, S->t);
+    }
     bc_uvw(S);
     bc_p(S, S->p);
     bc_t(S);
@@ -1087,8 +1507,9 @@ int ho_steps(ho_state *S, int nsteps)
         S->time_sim += S->prm.dt;
         if (S->prm.is_ri_var == 1)
             S->got = S->prm.ri_str + (S->prm.ri_end - S->prm.ri_str) * 1.0 / S->prm.imax * S->stps;
-        if (S->prm.cds == 3) { snprintf(S->err, sizeof S->err, "cds=3 is not restated"); return 1; }
-        if (time_advancement_fd(S)) return 1;
+        if (S->prm.cds == 3) {
+            if (time_advancement_sp(S)) return 1;
+        } else if (time_advancement_fd(S)) return 1;
         if (S->prm.dp_opt == 2) ho_calc_ubar(S);
     }
     return 0;
@@ -1202,7 +1623,7 @@ int ho_solve(ho_state *S, int eq, double rk_c, const double *src, double *out)
 int ho_div_stats(ho_state *S, double *avg, double *mx)
 {
     double keep = S->div_hist_max;
-    int rc = div_error(S, 0);
+    int rc = S->prm.cds == 3 ? div_error_spec(S, 0) : div_error(S, 0);
     S->div_hist_max = keep;
     *avg = S->div_avg; *mx = S->div_max;
     return rc;
@@ -1239,4 +1660,53 @@ void ho_set_scalar(ho_state *S, int id, double v)
     case 4: S->got = v; break;
     default: break;
     }
+}
+/* spectral divergence of the current state as screen_div_error_spec forms it
+ * (interior, C order), for the tests */
+int ho_spec_divergence(ho_state *S, double *dst)
+{
+    if (S->prm.cds != 3) return 1;
+    div_spec(S, S->c_u, S->c_v, S->w, S->c_src);
+    ho_fft2d(S, S->c_src, 1);
+    size_t n = ncplx(S);
+    for (size_t q = 0; q < n; ++q) dst[q] = creal(S->c_src[q]);
+    return 0;
+}
+
+/* spectra of the spectral path (cds=3), C order (k, j, i) complex:
+ * 0 c_u 1 c_v 2 c_w 3 c_p 4 c_t 5 c_phi 6 c_ff 7 c_gg 8 c_hh 9 c_ss 10 c_qu 11 c_qv 12 c_qw */
+static cplx *spec_ptr(ho_state *S, int id)
+{
+    cplx *t[] = {S->c_u, S->c_v, S->c_w, S->c_p, S->c_t, S->c_phi, S->c_ff, S->c_gg, S->c_hh, S->c_ss,
+                 S->c_qu, S->c_qv, S->c_qw};
+    return (S->prm.cds == 3 && id >= 0 && id < 13) ? t[id] : NULL;
+}
+int ho_get_spectrum(ho_state *S, int id, cplx *dst)
+{
+    cplx *a = spec_ptr(S, id);
+    if (!a) return 1;
+    memcpy(dst, a, sizeof(cplx) * ncplx(S));
+    return 0;
+}
+int ho_set_spectrum(ho_state *S, int id, const cplx *src)
+{
+    cplx *a = spec_ptr(S, id);
+    if (!a) return 1;
+    memcpy(a, src, sizeof(cplx) * ncplx(S));
+    return 0;
+}
+/* get_momentum_cnvdiff_spec / get_temperature_cnvdiff_spec at substep t_its */
+int ho_stage_momentum_spec(ho_state *S, int t_its)
+{
+    if (S->prm.cds != 3) return 1;
+    assign_rk(S, t_its);
+    momentum_spec(S);
+    return 0;
+}
+int ho_stage_temperature_spec(ho_state *S, int t_its)
+{
+    if (S->prm.cds != 3) return 1;
+    assign_rk(S, t_its);
+    temperature_spec(S);
+    return 0;
 }

@@ -36,6 +36,8 @@ FIELD_IDS = {"u": 0, "v": 1, "w": 2, "p": 3, "t": 4, "phi": 5, "ff": 6, "gg": 7,
              "vhx": 16, "vhy": 17, "vhz": 18, "whx": 19, "why": 20, "thx": 21, "thy": 22,
              "thz": 23}
 EQ_IDS = {"u": 0, "v": 1, "w": 2, "p": 3, "t": 4}
+SPEC_IDS = {"u": 0, "v": 1, "w": 2, "p": 3, "t": 4, "phi": 5, "ff": 6, "gg": 7, "hh": 8, "ss": 9,
+            "qu": 10, "qv": 11, "qw": 12}
 
 _lib = None
 
@@ -71,6 +73,11 @@ def lib():
         L.ho_stage_bc_uvw.argtypes = [vp]
         L.ho_stage_temperature.argtypes = [vp, ip]
         L.ho_stage_damping.argtypes = [vp, ip]
+        L.ho_get_spectrum.argtypes = [vp, ip, dp]
+        L.ho_set_spectrum.argtypes = [vp, ip, dp]
+        L.ho_stage_momentum_spec.argtypes = [vp, ip]
+        L.ho_stage_temperature_spec.argtypes = [vp, ip]
+        L.ho_spec_divergence.argtypes = [vp, dp]
         L.ho_get_scalar.restype = C.c_double
         L.ho_get_scalar.argtypes = [vp, ip]
         L.ho_set_scalar.argtypes = [vp, ip, C.c_double]
@@ -162,6 +169,22 @@ class Oracle:
         out = np.zeros(self.shape)
         self._check(lib().ho_solve(self.h, EQ_IDS[eq], float(rk_c), _dptr(s), _dptr(out)))
         return out
+
+    def spectrum(self, name):
+        """C2C spectrum of the spectral path (cds=3), shape (nz, ny, nx),
+        forward-normalised as mpi_2d_fft leaves it."""
+        a = np.empty(self.shape, dtype=np.complex128)
+        self._check(lib().ho_get_spectrum(self.h, SPEC_IDS[name], a.ctypes.data_as(C.POINTER(C.c_double))))
+        return a
+
+    def set_spectrum(self, name, arr):
+        a = np.ascontiguousarray(arr, dtype=np.complex128)
+        self._check(lib().ho_set_spectrum(self.h, SPEC_IDS[name], a.ctypes.data_as(C.POINTER(C.c_double))))
+
+    def spec_divergence(self):
+        d = np.empty(self.shape)
+        self._check(lib().ho_spec_divergence(self.h, _dptr(d)))
+        return d
 
     def div_stats(self):
         a, m = C.c_double(), C.c_double()

@@ -1,0 +1,177 @@
+"""Pins of the oracle's spectral path (cds=3, src/module_spectral.f90) against
+mathematics that does not come from the oracle: exact Fourier derivatives of
+single modes, the identity of the horizontally uniform (0,0)-mode dynamics
+with the finite-difference path, the 2/3-rule truncation, and the discrete
+divergence of the projected state."""
+import numpy as np
+import pytest
+
+from hercules_amd import cases, grid
+from oracle.oracle import Oracle, lib
+from tests.helpers import oracle_kwargs, rel_l2
+
+
+def _oracle(prm, state):
+    o = Oracle(**oracle_kwargs(prm))
+    for k, v in state.items():
+        o.set(k, v)
+    o.init()
+    return o
+
+
+def _zero(prm):
+    z = np.zeros((prm.nz, prm.ny, prm.nx))
+    return dict(u=z.copy(), v=z.copy(), w=z.copy(), p=z.copy(), t=z.copy())
+
+
+def _phys(o, name):
+    """physical field of a C2C spectrum (forward-normalised)"""
+    return np.real(np.fft.ifft2(o.spectrum(name)) * (o.nx * o.ny))
+
+
+@pytest.mark.parametrize("issk", [0, 1])
+def test_spectral_convection_of_one_mode_is_exact(issk):
+    """u = A sin(kx x) (v = w = 0): d(uu)/dx = A^2 k sin(2kx) exactly, and the
+    skew-symmetric form adds u du/dx = A^2 k sin(2kx)/2 with weight 1/2
+    (src/module_spectral.f90:55-59,98-197): ff = dpx - A^2 k sin(2kx) for
+    issk=0 and dpx - 0.75 A^2 k sin(2kx) for issk=1."""
+    prm = cases.closed_channel(nx=16, ny=8, nz=9, zstretch=1.0, cds=3, issk=issk)
+    st = _zero(prm)
+    A, kx = 0.7, 2 * np.pi / prm.lx
+    x = np.arange(prm.nx) * prm.lx / prm.nx
+    st["u"] += A * np.sin(kx * x)[None, None, :]
+    o = _oracle(prm, st)
+    lib().ho_stage_momentum_spec(o.h, 1)
+    ff, gg, hh = _phys(o, "ff"), _phys(o, "gg"), _phys(o, "hh")
+    w = 1.0 if issk == 0 else 0.75
+    want = 1.0 - w * A * A * kx * np.sin(2 * kx * x)
+    assert np.abs(ff - want[None, None, :]).max() < 1e-13
+    assert np.abs(gg).max() < 1e-14 and np.abs(hh).max() < 1e-14
+
+
+@pytest.mark.parametrize("issk", [0, 1])
+def test_spectral_vertical_fluxes(issk):
+    """u = U(z) (x-independent) and w = W0 sin(kx x) on every face below the
+    top.  w equation: d(u_w w)/dx = u_w kx W0 cos(kx x), with u_w the l_t
+    interpolant of u to the w level (src/module_spectral.f90:1079-1085);
+    u equation: the flux difference of a1 w(k) - a2 w(k-1) over dz(k) with
+    a1, a2 the face values of u (:110-127).  issk=1 adds the advective forms
+    with weight 1/2 (:142-197, :449-496)."""
+    prm = cases.closed_channel(nx=16, ny=8, nz=9, zstretch=1.0, cds=3, issk=issk, isscalar=1, got=0.0,
+                               tbot=0.3, ttop=1.1)
+    st = _zero(prm)
+    m = grid.metrics(prm.nz, prm.lz, prm.zstretch, 0)
+    zc = m["zc"]
+    W0, kx = 0.3, 2 * np.pi / prm.lx
+    x = np.arange(prm.nx) * prm.lx / prm.nx
+    U = 1.0 + zc + zc ** 2
+    Tz = 0.3 + 0.4 * zc
+    st["u"] += U[:, None, None]
+    st["v"] += 2.0 * U[:, None, None]
+    st["t"] += Tz[:, None, None]
+    st["w"][:-1] += W0 * np.sin(kx * x)[None, None, :]
+    o = _oracle(prm, st)
+    lib().ho_stage_momentum_spec(o.h, 1)
+    hh, ff = _phys(o, "hh"), _phys(o, "ff")
+    mesh = o.mesh()
+    lt, lb = mesh["l_t"][2:-2], mesh["l_b"][2:-2]
+    dz, dzt = mesh["dz"][2:-2], mesh["dz_t"][2:-2]
+    # ghost values of u (no-slip walls, u_mrf = 0) and w
+    ug = np.concatenate([[-U[0]], U, [-U[-1]]])
+    wf = np.zeros((prm.nz, prm.nx))
+    wf[:-1] = W0 * np.sin(kx * x)[None, :]
+    wg = np.concatenate([np.zeros((1, prm.nx)), wf, -wf[-2:-1]], 0)  # w(k0-1)=0, w(ke+1)=-w(ke-1)
+    uw = ug[2:] * lt + ug[1:-1] * (1.0 - lt)            # u at the top face of each cell
+    ub = ug[:-2] * lb + ug[1:-1] * (1.0 - lb)           # u at the bottom face
+    wk, wkm, wkp = wg[1:-1], wg[:-2], wg[2:]
+    coe = 0.5 if issk else 1.0
+    tmp1 = (uw[:, None] * wk - ub[:, None] * wkm) / dz[:, None]
+    want_ff = 1.0 - coe * tmp1
+    wp, wm = 0.5 * (wk + wkp), 0.5 * (wk + wkm)
+    wwz = (wp ** 2 - wm ** 2) / dzt[:, None]
+    want_hh = -uw[:, None] * kx * W0 * np.cos(kx * x)[None, :] - coe * wwz
+    if issk:
+        want_ff -= coe * (uw - ub)[:, None] / dz[:, None] * (0.5 * (wk + wkm))
+        want_hh -= coe * wk * (wp - wm) / dzt[:, None]
+    assert np.abs(ff[:, 0, :] - want_ff).max() < 1e-12 * np.abs(want_ff).max()
+    gg = _phys(o, "gg")  # v = 2U: the same flux terms twice, no drive
+    assert np.abs(gg[:, 0, :] - 2.0 * (want_ff - 1.0)).max() < 1e-12 * np.abs(want_ff).max()
+    # temperature: ss = -(t_top w(k) - t_bot w(k-1))/dz(k) (src/module_spectral.f90:572-587)
+    lib().ho_stage_temperature_spec(o.h, 1)
+    tg = np.concatenate([[2 * prm.tbot - Tz[0]], Tz, [2 * prm.ttop - Tz[-1]]])
+    tt = tg[2:] * lt + tg[1:-1] * (1.0 - lt)
+    tb = tg[:-2] * lb + tg[1:-1] * (1.0 - lb)
+    want_ss = -(tt[:, None] * wk - tb[:, None] * wkm) / dz[:, None]
+    assert np.abs(_phys(o, "ss")[:, 0, :] - want_ss).max() < 1e-12 * np.abs(want_ss).max()
+    assert np.abs(hh[:-1, 0, :] - want_hh[:-1]).max() < 1e-12 * np.abs(want_hh).max()
+
+
+@pytest.mark.parametrize("kind", ["closed", "open_t"])
+def test_uniform_state_follows_the_fd_path(kind):
+    """A horizontally uniform state has only the (0,0) mode, where the
+    spectral and 4th-order symbols both vanish: cds=3 and cds=2 advance it
+    through the same z operators, Helmholtz rows and substep order."""
+    if kind == "closed":
+        prm = cases.closed_channel(nx=16, ny=16, nz=17, zstretch=1.0, cds=3, dp_opt=2, u_mrf=0.2)
+    else:
+        prm = cases.stratified_open_channel(nx=16, ny=16, nz=17, zstretch=1.0, cds=3)
+    m = grid.metrics(prm.nz, prm.lz, prm.zstretch, prm.ochannel)
+    st = _zero(prm)
+    st["u"] += (m["zc"] * (prm.lz - m["zc"]))[:, None, None] * 3.0
+    st["v"] += np.sin(m["zc"])[:, None, None]
+    st["t"] += (m["zc"] / prm.lz)[:, None, None]
+    o3 = _oracle(prm, st)
+    o2 = _oracle(prm.__class__(**{**prm.__dict__, "cds": 2}), st)
+    o3.steps(3)
+    o2.steps(3)
+    for n in ("u", "v", "w", "p") + (("t",) if prm.isscalar else ()):
+        a, b = o3.get(n), o2.get(n)
+        scale = max(np.abs(b).max(), 1.0)
+        assert np.abs(a - b).max() < 1e-12 * scale, n
+
+
+def test_spectral_step_truncates_and_projects():
+    prm = cases.closed_channel(nx=24, ny=16, nz=17, zstretch=1.0, cds=3, issk=1)
+    st = cases.laminar_noise(prm, seed=31)
+    o = _oracle(prm, st)
+    o.steps(2)
+    lc, mc = int(prm.nx / 3), int(prm.ny / 3)
+    for n in ("u", "v", "w", "p"):
+        S = np.fft.fft2(o.get(n))
+        keep = np.zeros(S.shape, bool)
+        l = np.fft.fftfreq(prm.nx, 1 / prm.nx).astype(int)
+        mm = np.fft.fftfreq(prm.ny, 1 / prm.ny).astype(int)
+        keep[:, (np.abs(mm) < mc)[:, None] & (np.abs(l) < lc)[None, :]] = True
+        assert np.abs(S[~keep]).max() < 1e-12 * np.abs(S).max(), n
+    d = o.spec_divergence()
+    u = o.get("u")
+    assert np.abs(d).max() < 1e-12 * np.abs(u).max() / (prm.lz / prm.nz)
+    # numpy's spectral divergence of the physical fields agrees
+    kx = 2 * np.pi / prm.lx * np.fft.fftfreq(prm.nx, 1 / prm.nx)
+    ky = 2 * np.pi / prm.ly * np.fft.fftfreq(prm.ny, 1 / prm.ny)
+    m = grid.metrics(prm.nz, prm.lz, prm.zstretch, 0)
+    D = np.fft.ifft2(1j * kx[None, None, :] * np.fft.fft2(u) + 1j * ky[None, :, None] * np.fft.fft2(o.get("v"))).real
+    w = o.get("w")
+    D += (w - np.concatenate([np.zeros((1,) + w.shape[1:]), w[:-1]], 0)) / m["dz"][:, None, None]
+    assert np.abs(D - d).max() < 1e-11 * np.abs(u).max() / (prm.lz / prm.nz)
+
+
+def test_spectral_translation_equivariance():
+    """Shifting the initial state by whole cells in x and y shifts the state
+    after a step by the same cells (no index bias in the transforms)."""
+    prm = cases.closed_channel(nx=16, ny=16, nz=9, zstretch=1.0, cds=3, issk=1)
+    st = cases.laminar_noise(prm, seed=32)
+    sh = {k: np.roll(np.roll(v, 3, axis=2), -5, axis=1) for k, v in st.items()}
+    o1, o2 = _oracle(prm, st), _oracle(prm, sh)
+    o1.steps(1)
+    o2.steps(1)
+    for n in ("u", "v", "w", "p"):
+        a = np.roll(np.roll(o1.get(n), 3, axis=2), -5, axis=1)
+        assert rel_l2(o2.get(n), a) < 1e-12, n
+
+
+def test_spectral_damping_is_rejected():
+    prm = cases.closed_channel(nx=16, ny=16, nz=9, cds=3, isdamp=1, nzdamp=3, cdamp=0.1)
+    o = _oracle(prm, cases.laminar_noise(prm, seed=1))
+    with pytest.raises(RuntimeError, match="no defined reference result"):
+        o.steps(1)
