@@ -49,6 +49,17 @@ def rhs_t_passes(substep):
     return 4 + (1 if substep > 1 else 0) + 2
 
 
+# Spectral substep (cds=3, skew-symmetric form, temperature), field passes per
+# launch slot: rhs = y inverse of the six derivative spectra (read c_u, c_v,
+# c_w, write 6) + products (read u, v, w, t and the 6, write 8); yfft_fwd = 8
+# spectra in place; column = read 8 products, c_u, c_v, c_w, c_p and the old
+# c_ff, c_gg, c_hh, write the 7 updated; finish = y and x inverses of c_u,
+# c_v, c_w; rhs_t = read u, v, w, t, write 3; yfft_t = 3 in place; column_t =
+# read 3, c_t, c_ss, write c_t, c_ss; finish_t = inverse of c_t.
+SPECTRAL_PASSES = {"rhs": 27, "yfft_fwd": 16, "column": 22, "finish": 12, "rhs_t": 7, "yfft_t": 6,
+                   "column_t": 7, "finish_t": 4}
+
+
 def build_case(workload):
     from hercules_amd import cases
     # wall_rows=1: every mode gets the implicit wall rows that the reference
@@ -60,6 +71,9 @@ def build_case(workload):
         st.pop("t")
     elif workload == "fd_t":
         prm = cases.stratified_open_channel(nx=256, ny=256, nz=257, wall_rows=1)
+        st = cases.loglaw_noise(prm, seed=2024)
+    elif workload == "sp":
+        prm = cases.stratified_open_channel(nx=256, ny=256, nz=257, cds=3, issk=1, wall_rows=1)
         st = cases.loglaw_noise(prm, seed=2024)
     elif workload == "poisson":
         prm = cases.closed_channel(nx=256, ny=256, nz=257)
@@ -127,7 +141,7 @@ def main():
     ap.add_argument("--gpus", type=int, default=1)
     ap.add_argument("--steps", type=int, default=50)
     ap.add_argument("--warmup", type=int, default=5)
-    ap.add_argument("--workload", default="fd", choices=["fd", "fd_t", "poisson", "copy"])
+    ap.add_argument("--workload", default="fd", choices=["fd", "fd_t", "sp", "poisson", "copy"])
     ap.add_argument("--cpu-steps", type=int, default=1, help="oracle steps for the CPU baseline")
     ap.add_argument("--no-cpu-baseline", action="store_true")
     ap.add_argument("--no-profile", action="store_true", help="skip the per-kernel event timing")
@@ -198,7 +212,9 @@ def main():
     kernels = {}
     nsub = 3 if prm.dts == 2 else 1
     for name, (ms, launches) in prof.items():
-        if name == "rhs":
+        if args.workload == "sp":
+            passes = SPECTRAL_PASSES[name]
+        elif name == "rhs":
             per_step = sum(algo_passes("rhs", s, with_t) for s in range(1, nsub + 1))
             passes = per_step / nsub
         elif name == "rhs_t":
@@ -229,7 +245,8 @@ def main():
                     "algo_bytes_per_launch": int(kd["algo_bytes"])}
     # whole step against the step's algorithmic bytes (1272 B/cell/step for FD
     # without T, 1608 with T: 159 / 201 FP64 passes per RK3 step)
-    step_passes = {"fd": 159, "fd_t": 201, "poisson": 10}[args.workload]
+    step_passes = {"fd": 159, "fd_t": 201, "sp": 3 * sum(SPECTRAL_PASSES.values()),
+                   "poisson": 10}[args.workload]
     step_gbs = step_passes * n * 8 / (ms_per_step * 1e-3) / 1e9
     step_roofline = {"passes_per_step": step_passes, "achieved": round(step_gbs, 1),
                      "peak": HBM_PEAK_GBS, "unit": "GB/s", "frac": round(step_gbs / HBM_PEAK_GBS, 4)}
@@ -244,6 +261,7 @@ def main():
     workload_name = {
         "fd": "FD RK3 step (cds=2), neutral closed channel, Re_tau=180, 256x256x257, zstretch=1, wall_rows=1",
         "fd_t": "FD RK3 step (cds=2) with temperature, stratified open channel, Re_tau=395, got=0.1, 256x256x257, zstretch=1, wall_rows=1",
+        "sp": "spectral RK3 step (cds=3, skew-symmetric, 2/3 rule) with temperature, stratified open channel, Re_tau=395, got=0.1, 256x256x257, zstretch=1, wall_rows=1",
         "poisson": "standalone pressure Poisson solve, 256x256x257, zstretch=1",
     }[args.workload]
     out = {

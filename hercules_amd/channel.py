@@ -25,7 +25,8 @@ FIELDS = {"u": 0, "v": 1, "w": 2, "p": 3, "t": 4, "ff": 5, "gg": 6, "hh": 7, "ss
 EQUATIONS = {"u": 0, "v": 1, "w": 2, "p": 3, "t": 4}
 LAYOUT_INTERIOR, LAYOUT_GHOSTED = 0, 1
 ABI_SYMBOLS = ("hc_last_error", "hc_create", "hc_destroy", "hc_set_field", "hc_get_field",
-               "hc_init", "hc_project", "hc_time_advancement_fd", "hc_drive_steps", "hc_poisson_solver_fft",
+               "hc_init", "hc_project", "hc_time_advancement_fd", "hc_time_advancement_sp", "hc_drive_steps",
+               "hc_poisson_solver_fft",
                "hc_calculate_ubar", "hc_screen_div_error", "hc_get_scalar", "hc_stream",
                "hc_field_device_ptr", "hc_synchronize", "hc_profile_enable", "hc_profile_read")
 KERNELS = {"rhs": 0, "yfft_fwd": 1, "column": 2, "yfft_inv": 3, "finish": 4, "rhs_t": 5,
@@ -59,6 +60,7 @@ def load_library(path: str = LIB_PATH):
     L.hc_init.argtypes = [vp]
     L.hc_project.argtypes = [vp]
     L.hc_time_advancement_fd.argtypes = [vp]
+    L.hc_time_advancement_sp.argtypes = [vp]
     L.hc_drive_steps.argtypes = [vp, ip]
     L.hc_poisson_solver_fft.argtypes = [vp, ip, C.c_double, vp, vp, ip]
     L.hc_calculate_ubar.argtypes = [vp, dp]
@@ -141,6 +143,9 @@ class Channel:
 
     def time_advancement_fd(self):
         self._check(self.L.hc_time_advancement_fd(self.h))
+
+    def time_advancement_sp(self):
+        self._check(self.L.hc_time_advancement_sp(self.h))
 
     def steps(self, n):
         self._check(self.L.hc_drive_steps(self.h, int(n)))

@@ -700,6 +700,8 @@ __global__ __launch_bounds__(64 * kColWaves) void k_column_mom(StepArgs A)
     col_store<L>(A.s3, tile, nz, plane, col0, m, dpn);
 }
 
+#include "hc_spectral.h"
+
 // Temperature Helmholtz solve in place on s0 (:335-337).
 template <int L>
 __global__ __launch_bounds__(64 * kColWaves) void k_column_t(StepArgs A)
@@ -822,17 +824,18 @@ __global__ __launch_bounds__(256) void k_div_partial(StepArgs A, double *part)
     for (size_t q = threadIdx.x; q < n; q += 256) {
         int j = (int)(q / A.nx), i = (int)(q - (size_t)j * A.nx);
         Pt P{A, i, j, k};
-        double dudx, dvdy;
+        double dudx = 0.0, dvdy = 0.0;
         if (CDS == 1) {
             dudx = (P.U(0, 0, 0) - P.U(-1, 0, 0)) / A.dx;
             dvdy = (P.V(0, 0, 0) - P.V(0, -1, 0)) / A.dy;
-        } else {
+        } else if (CDS == 2) {
             dudx = (-P.U(1, 0, 0) + 27.0 * P.U(0, 0, 0) - 27.0 * P.U(-1, 0, 0) + P.U(-2, 0, 0)) / 24.0 / A.dx;
             dvdy = (-P.V(0, 1, 0) + 27.0 * P.V(0, 0, 0) - 27.0 * P.V(0, -1, 0) + P.V(0, -2, 0)) / 24.0 / A.dy;
         }
         double w0 = A.w[P.at(0, 0, k)];
         double wm = k == 0 ? 0.0 : A.w[P.at(0, 0, k - 1)];
-        double d = dudx + dvdy + (w0 - wm) / dz;
+        // spectral path: the divergence field was formed and inverted beforehand
+        double d = CDS == 3 ? A.aux[P.at(0, 0, k)] : dudx + dvdy + (w0 - wm) / dz;
         s += fabs(d);
         mx = fmax(mx, fabs(d));
         double u0 = A.u[P.at(0, 0, k)], v0 = A.v[P.at(0, 0, k)];
@@ -900,6 +903,12 @@ struct hc_ctx {
     double2 *s[4];
     double *lev, *symx, *symy, *scal, *part, *stage;
     double *dmp_s, *dmp_m;  // Rayleigh damping: sigz per level, plane means per (field, level)
+    // spectral path (cds = 3): field and right-hand-side spectra, product
+    // spectra (sp[0..3] are s[0..3]), a physical scratch field
+    double2 *cu = nullptr, *cv = nullptr, *cw = nullptr, *cp = nullptr, *ct = nullptr;
+    double2 *cff = nullptr, *cgg = nullptr, *chh = nullptr, *css = nullptr, *sp[8] = {};
+    double *ptmp = nullptr;
+    bool spec_dirty = true;  // physical fields changed since the spectra were formed
     int kdamp0;             // first damped level (nz when isdamp = 0)
     double2 *tw_nl, *tw_nx, *tw_ny;
     double aU0, cUe, aT0, cTe;
@@ -1000,6 +1009,7 @@ static void build_levels(hc_ctx *c)
         int k = kk + 3;
         at(LV_DZ, kk) = dz[k]; at(LV_DZT, kk) = dzt[k]; at(LV_DZB, kk) = dzb[k]; at(LV_LT, kk) = lt[k];
         at(LV_IDZ, kk) = 1.0 / dz[k]; at(LV_IDZT, kk) = 1.0 / dzt[k]; at(LV_IDZB, kk) = 1.0 / dzb[k];
+        at(LV_LB, kk) = lb[k];
         // u (v): aa, cc of get_u_eqn_coeff (:200-201)
         double a = -0.5 * P.dt * P.nu / dz[k] / dzb[k], cc = -0.5 * P.dt * P.nu / dz[k] / dzt[k];
         at(LV_AU, kk) = kk == 0 ? 0.0 : a;
@@ -1045,7 +1055,7 @@ static void build_levels(hc_ctx *c)
 // Per-wavenumber symbols of the horizontal FD operators for wavenumber index
 // `m` of an n-point direction with spacing h (src/module_poisson_solver.f90:73-101,
 // 222-246; the stencils of src/module_cnvdiff.f90 and src/module_navier_stokes.f90:669-796).
-static void build_symbols(const hc_params &P, int n, int count, double h, std::vector<double> &S)
+static void build_symbols(const hc_params &P, int n, int count, double h, double len, std::vector<double> &S)
 {
     S.assign((size_t)SY_COUNT * count, 0.0);
     const double h2 = h * h;
@@ -1055,7 +1065,17 @@ static void build_symbols(const hc_params &P, int n, int count, double h, std::v
         double c1 = std::cos(th), c2 = std::cos(2.0 * th), c3 = std::cos(3.0 * th);
         double s1 = std::sin(th), s2 = std::sin(2.0 * th);
         double lap, pois, dre, dim, gre, gim;
-        if (P.cds == 1) {
+        if (P.cds == 3) {
+            // Fourier symbols of dev_fft_linear_cmplx and the spectral
+            // coefficients (src/module_spectral.f90:1305-1320,
+            // src/module_poisson_solver.f90:96-99,244-245); zero at the
+            // Nyquist wavenumber, which the reference removes
+            const double kw = m == -n / 2 ? 0.0 : 2.0 * kPi / len * m;
+            lap = -(kw * kw);
+            pois = lap;
+            dre = 0.0; dim = kw;
+            gre = 0.0; gim = kw;
+        } else if (P.cds == 1) {
             lap = (2.0 * c1 - 2.0) / h2;
             pois = lap;
             dre = (1.0 - c1) / h; dim = s1 / h;             // (1 - e^{-i th})/h
@@ -1112,6 +1132,13 @@ static StepArgs base_args(hc_ctx *c)
     A.idx = 1.0 / c->dx; A.idy = 1.0 / c->dy; A.idx2 = 1.0 / (c->dx * c->dx); A.idy2 = 1.0 / (c->dy * c->dy);
     A.idt = 1.0 / P.dt;
     A.dmp_s = c->dmp_s; A.dmp_m = c->dmp_m; A.kdamp0 = c->kdamp0;
+    A.cu = c->cu; A.cv = c->cv; A.cw = c->cw; A.cp = c->cp; A.ct = c->ct;
+    A.cff = c->cff; A.cgg = c->cgg; A.chh = c->chh; A.css = c->css;
+    for (int q = 0; q < 8; ++q) A.sp[q] = c->sp[q];
+    for (int q = 0; q < 6; ++q) A.sdv[q] = c->sp[q];
+    A.coe = P.issk == 1 ? 0.5 : 1.0;  // coe_sk, src/module_spectral.f90:55-59
+    A.issk = P.issk == 1;
+    A.lc = (int)(c->nx / 3.0); A.mc = (int)(c->ny / 3.0);
     return A;
 }
 
@@ -1159,12 +1186,14 @@ static int launch_yfft(hc_ctx *c, int sign, int nf, double2 *f0, double2 *f1, do
 
 static int launch_c2r(hc_ctx *c, int nf, C2RArgs &X)
 {
+    if (nf < 1 || nf > 4) return set_err(HC_ERR_ARG, "launch_c2r: %d fields", nf);
     X.ny = c->ny; X.tw_nl = c->tw_nl; X.tw_nx = c->tw_nx;
     int rows = rows_for(c->nx) / nf > 0 ? rows_for(c->nx) / nf : 1;
     dim3 g((c->ny + rows - 1) / rows, c->nz);
 #define HC_C(NX_)                                                                                  \
     if (nf == 4) hipLaunchKernelGGL((k_x_c2r<NX_, 4>), g, dim3(kRhsThreads), 0, c->stream, X);    \
     else if (nf == 3) hipLaunchKernelGGL((k_x_c2r<NX_, 3>), g, dim3(kRhsThreads), 0, c->stream, X); \
+    else if (nf == 2) hipLaunchKernelGGL((k_x_c2r<NX_, 2>), g, dim3(kRhsThreads), 0, c->stream, X); \
     else hipLaunchKernelGGL((k_x_c2r<NX_, 1>), g, dim3(kRhsThreads), 0, c->stream, X);
     HC_DISPATCH_N(c->nx, HC_C)
 #undef HC_C
@@ -1296,6 +1325,140 @@ static int substep(hc_ctx *c, int t_its)
     return 0;
 }
 
+// ---- spectral path (cds = 3) ------------------------------------------------
+static int launch_yio(hc_ctx *c, int sign, int nf, const double2 *const *src, double2 *const *dst, const int *mul)
+{
+    YIOArgs Y;
+    std::memset(&Y, 0, sizeof Y);
+    for (int q = 0; q < nf; ++q) { Y.src[q] = src[q]; Y.dst[q] = dst[q]; Y.mul[q] = mul[q]; }
+    Y.nl = c->nl; Y.ny = c->ny; Y.tw_ny = c->tw_ny; Y.symx = c->symx; Y.symy = c->symy;
+    dim3 g(c->nl / kYTile, c->nz, nf);
+#define HC_YIO(NY_)                                                                                  \
+    if (sign < 0) hipLaunchKernelGGL((k_yfft_io<NY_, -1>), g, dim3(256), 0, c->stream, Y);          \
+    else hipLaunchKernelGGL((k_yfft_io<NY_, +1>), g, dim3(256), 0, c->stream, Y);
+    HC_DISPATCH_N(c->ny, HC_YIO)
+#undef HC_YIO
+    return 0;
+}
+
+static int launch_truncate(hc_ctx *c, double2 *s)
+{
+    hipLaunchKernelGGL(k_truncate, dim3(1024), dim3(256), 0, c->stream, s, c->nl, c->ny, c->nz, (int)(c->nx / 3.0),
+                       (int)(c->ny / 3.0));
+    return 0;
+}
+
+// real_2_cmplx (src/module_spectral.f90:1157-1182): x R2C then y FFT, forward normalised
+static int spec_forward(hc_ctx *c, const double *phys, double2 *spec)
+{
+    if (int e = launch_r2c(c, phys, spec)) return e;
+    return launch_yfft(c, -1, 1, spec, nullptr, nullptr, nullptr);
+}
+
+// cmplx_2_real (:1188-1217) of up to three spectra, through scratch spectra
+static int spec_inverse(hc_ctx *c, int nf, const double2 *const *spec, double *const *phys)
+{
+    const int zero[3] = {0, 0, 0};
+    double2 *scr[3] = {c->sp[0], c->sp[1], c->sp[2]};
+    if (int e = launch_yio(c, +1, nf, spec, scr, zero)) return e;
+    C2RArgs X;
+    std::memset(&X, 0, sizeof X);
+    for (int q = 0; q < nf; ++q) { X.src[q] = scr[q]; X.dst[q] = phys[q]; }
+    return launch_c2r(c, nf, X);
+}
+
+// The spectral start-up of drive (src/module_navier_stokes.f90:96-123):
+// spectra of every field, 2/3-rule truncation of u, v, w, p, t, and the
+// truncated fields back in physical space.
+static int spec_init(hc_ctx *c)
+{
+    double *phys[9] = {c->u, c->v, c->w, c->p, c->t, c->fb[c->fcur][0], c->fb[c->fcur][1], c->fb[c->fcur][2],
+                       c->sb[c->scur]};
+    double2 *spec[9] = {c->cu, c->cv, c->cw, c->cp, c->ct, c->cff, c->cgg, c->chh, c->css};
+    for (int q = 0; q < 9; ++q)
+        if (int e = spec_forward(c, phys[q], spec[q])) return e;
+    for (int q = 0; q < 5; ++q)
+        if (int e = launch_truncate(c, spec[q])) return e;
+    const double2 *s3[3] = {c->cu, c->cv, c->cw};
+    double *p3[3] = {c->u, c->v, c->w};
+    if (int e = spec_inverse(c, 3, s3, p3)) return e;
+    const double2 *s2[2] = {c->cp, c->ct};
+    double *p2[2] = {c->p, c->t};
+    if (int e = spec_inverse(c, 2, s2, p2)) return e;
+    c->spec_dirty = false;
+    HIPCHK(hipGetLastError());
+    return 0;
+}
+
+// one substep of time_advancement_sp (src/module_navier_stokes.f90:393-509)
+static int substep_sp(hc_ctx *c, int t_its)
+{
+    const hc_params &P = c->prm;
+    if (c->spec_dirty)
+        if (int e = spec_init(c)) return e;
+    StepArgs A = base_args(c);
+    if (P.dts == 2) { A.ra = kRkc1[t_its - 1]; A.rb = kRkc2[t_its - 1]; A.rc = kRkc3[t_its - 1]; }
+    else { A.ra = 1.5; A.rb = -0.5; A.rc = 1.0; }
+    if (P.dp_opt == 2) hipLaunchKernelGGL(k_dpx_update, dim3(1), dim3(64), 0, c->stream, c->scal, P.u_mrf, P.nu * 1.0);
+    const int rows = rows_for(c->nx);
+    const dim3 grow((c->ny + rows - 1) / rows, c->nz);
+    const dim3 gcol((c->nl + kColWaves - 1) / kColWaves, c->ny);
+    PROF_BEGIN(c, HC_K_RHS);
+    if (P.issk == 1) {
+        // du/dx, du/dy, dv/dx, dv/dy, dw/dx, dw/dy in y-physical space
+        const double2 *src4[4] = {c->cu, c->cu, c->cv, c->cv};
+        double2 *dst4[4] = {c->sp[0], c->sp[1], c->sp[2], c->sp[3]};
+        const int mul4[4] = {1, 2, 1, 2};
+        if (int e = launch_yio(c, +1, 4, src4, dst4, mul4)) return e;
+        const double2 *src2[2] = {c->cw, c->cw};
+        double2 *dst2[2] = {c->sp[4], c->sp[5]};
+        const int mul2[2] = {1, 2};
+        if (int e = launch_yio(c, +1, 2, src2, dst2, mul2)) return e;
+    }
+#define HC_SM(NX_) hipLaunchKernelGGL((k_spec_mom<NX_>), grow, dim3(rows * NX_), 0, c->stream, A);
+    HC_DISPATCH_N(c->nx, HC_SM)
+#undef HC_SM
+    PROF_END(c, HC_K_RHS);
+    PROF_BEGIN(c, HC_K_YFFT_FWD);
+    if (int e = launch_yfft(c, -1, 4, c->sp[0], c->sp[1], c->sp[2], c->sp[3])) return e;
+    if (int e = launch_yfft(c, -1, 4, c->sp[4], c->sp[5], c->sp[6], c->sp[7])) return e;
+    PROF_END(c, HC_K_YFFT_FWD);
+    PROF_BEGIN(c, HC_K_COLUMN);
+#define HC_CSP(L_) hipLaunchKernelGGL((k_column_spec<L_>), gcol, dim3(64 * kColWaves), col_lds(c), c->stream, A);
+    HC_DISPATCH_L(c->Lcol, HC_CSP)
+#undef HC_CSP
+    PROF_END(c, HC_K_COLUMN);
+    PROF_BEGIN(c, HC_K_FINISH);
+    {
+        const double2 *s3[3] = {c->cu, c->cv, c->cw};
+        double *p3[3] = {c->u, c->v, c->w};
+        if (int e = spec_inverse(c, 3, s3, p3)) return e;
+    }
+    PROF_END(c, HC_K_FINISH);
+    if (P.isscalar == 1) {
+        PROF_BEGIN(c, HC_K_RHS_T);
+#define HC_ST(NX_) hipLaunchKernelGGL((k_spec_t<NX_>), grow, dim3(rows * NX_), 0, c->stream, A);
+        HC_DISPATCH_N(c->nx, HC_ST)
+#undef HC_ST
+        PROF_END(c, HC_K_RHS_T);
+        PROF_BEGIN(c, HC_K_YFFT_T);
+        if (int e = launch_yfft(c, -1, 3, c->sp[0], c->sp[1], c->sp[2], nullptr)) return e;
+        PROF_END(c, HC_K_YFFT_T);
+        PROF_BEGIN(c, HC_K_COLUMN_T);
+#define HC_CST(L_) hipLaunchKernelGGL((k_column_spec_t<L_>), gcol, dim3(64 * kColWaves), col_lds(c), c->stream, A);
+        HC_DISPATCH_L(c->Lcol, HC_CST)
+#undef HC_CST
+        PROF_END(c, HC_K_COLUMN_T);
+        PROF_BEGIN(c, HC_K_FINISH_T);
+        const double2 *s1[1] = {c->ct};
+        double *p1[1] = {c->t};
+        if (int e = spec_inverse(c, 1, s1, p1)) return e;
+        PROF_END(c, HC_K_FINISH_T);
+    }
+    HIPCHK(hipGetLastError());
+    return 0;
+}
+
 static int ubar_launch(hc_ctx *c)
 {
     hipLaunchKernelGGL(k_ubar_partial, dim3(c->nz), dim3(256), 0, c->stream, c->u, c->lev, c->nx, c->ny, c->nz, c->part);
@@ -1313,7 +1476,10 @@ extern "C" int hc_create(const hc_params *prm, hc_ctx **out)
     // initiate_parameters checks (src/module_parameters.f90:256-270)
     if (P.nx % 2 != 0 || P.ny % 2 != 0) return set_err(HC_ERR_ARG, "Please use even numbers for nx and ny! Quit!");
     if (P.nz < 3) return set_err(HC_ERR_ARG, "nz must be at least 3");
-    if (P.cds != 1 && P.cds != 2) return set_err(HC_ERR_UNSUPPORTED, "cds=%d: only the FD paths cds=1,2 are implemented", P.cds);
+    if (P.cds < 1 || P.cds > 3) return set_err(HC_ERR_ARG, "cds must be 1, 2 or 3");
+    if (P.cds == 3 && P.isdamp == 1)
+        return set_err(HC_ERR_UNSUPPORTED, "isdamp=1 with cds=3: rayleigh_damping_spec reduces a complex mean as one "
+                                           "real (src/module_spectral.f90:1372), so the reference result is undefined");
     if (P.dts != 1 && P.dts != 2) return set_err(HC_ERR_ARG, "dts must be 1 (AB2) or 2 (RK3)");
     if (P.dp_opt < 1 || P.dp_opt > 3) return set_err(HC_ERR_ARG, "dp_opt must be 1, 2 or 3");
     if (P.ochannel != 0 && P.ochannel != 1) return set_err(HC_ERR_ARG, "ochannel must be 0 or 1");
@@ -1347,8 +1513,8 @@ extern "C" int hc_create(const hc_params *prm, hc_ctx **out)
     for (int q = 0; q < 4 && !e; ++q) e = dalloc(c, (void **)&c->s[q], (size_t)c->nl * c->ny * c->nz * sizeof(double2));
     build_levels(c);
     std::vector<double> sx, sy;
-    build_symbols(P, c->nx, c->nl, c->dx, sx);
-    build_symbols(P, c->ny, c->ny, c->dy, sy);
+    build_symbols(P, c->nx, c->nl, c->dx, P.lx, sx);
+    build_symbols(P, c->ny, c->ny, c->dy, P.ly, sy);
     auto tnl = twiddles(c->nl), tnx = twiddles(c->nx), tny = twiddles(c->ny);
     if (!e) e = dalloc(c, (void **)&c->lev, c->h_lev.size() * sizeof(double));
     if (!e) e = dalloc(c, (void **)&c->symx, sx.size() * sizeof(double));
@@ -1360,6 +1526,14 @@ extern "C" int hc_create(const hc_params *prm, hc_ctx **out)
     if (!e) e = dalloc(c, (void **)&c->part, 3 * (size_t)c->nz * sizeof(double));
     if (!e) e = dalloc(c, (void **)&c->dmp_s, (size_t)c->nz * sizeof(double));
     if (!e) e = dalloc(c, (void **)&c->dmp_m, 5 * (size_t)c->nz * sizeof(double));
+    if (!e && P.cds == 3) {
+        const size_t hs = (size_t)c->nl * c->ny * c->nz * sizeof(double2);
+        double2 **sp[] = {&c->cu, &c->cv, &c->cw, &c->cp, &c->ct, &c->cff, &c->cgg, &c->chh, &c->css,
+                          &c->sp[4], &c->sp[5], &c->sp[6], &c->sp[7]};
+        for (auto pp : sp) if (!e) e = dalloc(c, (void **)pp, hs);
+        for (int q = 0; q < 4; ++q) c->sp[q] = c->s[q];
+        if (!e) e = dalloc(c, (void **)&c->ptmp, N * sizeof(double));
+    }
     if (e) { hc_destroy(c); return e; }
     // sponge strength sigz of rayleigh_damping (src/module_tools.f90:735-736)
     // on the top nzdamp+1 levels, zero below
@@ -1436,6 +1610,9 @@ extern "C" int hc_set_field(hc_ctx *c, int field, const double *src, int layout)
     } else {
         return set_err(HC_ERR_ARG, "unknown layout %d", layout);
     }
+    // spectral path: the spectra are formed again from the physical fields
+    // before the next substep, as at start-up
+    c->spec_dirty = true;
     HIPCHK(hipStreamSynchronize(c->stream));
     return 0;
 }
@@ -1445,6 +1622,12 @@ extern "C" int hc_get_field(hc_ctx *c, int field, double *dst, int layout)
     if (!c || !dst) return set_err(HC_ERR_ARG, "null argument");
     double *d = field_ptr(c, field);
     if (!d) return set_err(HC_ERR_ARG, "unknown field %d", field);
+    if (c->prm.cds == 3 && field == HC_P && !c->spec_dirty) {
+        // the spectral path keeps p as a spectrum: cmplx_2_real(c_p, p) (:476)
+        const double2 *s1[1] = {c->cp};
+        double *p1[1] = {c->p};
+        if (int e = spec_inverse(c, 1, s1, p1)) return e;
+    }
     const size_t N = (size_t)c->nx * c->ny * c->nz;
     if (layout == HC_LAYOUT_INTERIOR) {
         HIPCHK(hipMemcpyAsync(dst, d, N * sizeof(double), hipMemcpyDefault, c->stream));
@@ -1473,6 +1656,8 @@ extern "C" int hc_init(hc_ctx *c)
                        (size_t)(c->nz - 1) * plane, plane);
     if (P.dp_opt == 2)
         if (int e = ubar_launch(c)) return e;
+    if (P.cds == 3)
+        if (int e = spec_init(c)) return e;
     HIPCHK(hipGetLastError());
     HIPCHK(hipStreamSynchronize(c->stream));
     return 0;
@@ -1486,16 +1671,27 @@ static int one_step(hc_ctx *c)
     if (P.is_ri_var == 1) c->got = P.ri_str + (P.ri_end - P.ri_str) * 1.0 / P.imax * c->stps;
     int nsub = P.dts == 2 ? 3 : 1;
     for (int s = 1; s <= nsub; ++s)
-        if (int e = substep(c, s)) return e;
+        if (int e = P.cds == 3 ? substep_sp(c, s) : substep(c, s)) return e;
     return 0;
 }
 
 extern "C" int hc_time_advancement_fd(hc_ctx *c)
 {
     if (!c) return set_err(HC_ERR_ARG, "null context");
+    if (c->prm.cds == 3) return set_err(HC_ERR_ARG, "cds=3: the spectral path is hc_time_advancement_sp");
     int nsub = c->prm.dts == 2 ? 3 : 1;
     for (int s = 1; s <= nsub; ++s)
         if (int e = substep(c, s)) return e;
+    return 0;
+}
+
+extern "C" int hc_time_advancement_sp(hc_ctx *c)
+{
+    if (!c) return set_err(HC_ERR_ARG, "null context");
+    if (c->prm.cds != 3) return set_err(HC_ERR_ARG, "cds=%d: time_advancement_sp needs cds=3", c->prm.cds);
+    int nsub = c->prm.dts == 2 ? 3 : 1;
+    for (int s = 1; s <= nsub; ++s)
+        if (int e = substep_sp(c, s)) return e;
     return 0;
 }
 
@@ -1589,8 +1785,22 @@ extern "C" int hc_screen_div_error(hc_ctx *c, double *div_avg, double *div_max)
 {
     if (!c) return set_err(HC_ERR_ARG, "null context");
     StepArgs A = base_args(c);
-    if (c->prm.cds == 1) hipLaunchKernelGGL((k_div_partial<1>), dim3(c->nz), dim3(256), 0, c->stream, A, c->part);
-    else hipLaunchKernelGGL((k_div_partial<2>), dim3(c->nz), dim3(256), 0, c->stream, A, c->part);
+    if (c->prm.cds == 3) {
+        // screen_div_error_spec (src/module_spectral.f90:899-1007): the
+        // spectral divergence of (c_u, c_v, w) in physical space
+        if (c->spec_dirty)
+            if (int e = spec_init(c)) return e;
+        hipLaunchKernelGGL(k_spec_div, dim3(1024), dim3(256), 0, c->stream, A, c->sp[3]);
+        const double2 *s1[1] = {c->sp[3]};
+        double *p1[1] = {c->ptmp};
+        if (int e = spec_inverse(c, 1, s1, p1)) return e;
+        A.aux = c->ptmp;
+        hipLaunchKernelGGL((k_div_partial<3>), dim3(c->nz), dim3(256), 0, c->stream, A, c->part);
+    } else if (c->prm.cds == 1) {
+        hipLaunchKernelGGL((k_div_partial<1>), dim3(c->nz), dim3(256), 0, c->stream, A, c->part);
+    } else {
+        hipLaunchKernelGGL((k_div_partial<2>), dim3(c->nz), dim3(256), 0, c->stream, A, c->part);
+    }
     std::vector<double> h(3 * (size_t)c->nz);
     HIPCHK(hipMemcpyAsync(h.data(), c->part, h.size() * sizeof(double), hipMemcpyDeviceToHost, c->stream));
     HIPCHK(hipStreamSynchronize(c->stream));
@@ -1686,6 +1896,7 @@ extern "C" int hc_project(hc_ctx *c)
     X.src[0] = c->s[0]; X.src[1] = c->s[1]; X.src[2] = c->s[2];
     X.dst[0] = c->u; X.dst[1] = c->v; X.dst[2] = c->w;
     if (int e = launch_c2r(c, 3, X)) return e;
+    c->spec_dirty = true;
     HIPCHK(hipGetLastError());
     HIPCHK(hipStreamSynchronize(c->stream));
     return 0;

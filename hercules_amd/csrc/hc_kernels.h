@@ -23,6 +23,7 @@ enum {
     LV_AP, LV_CP, LV_DP, LV_DP00,
     LV_AT, LV_CT, LV_DT, LV_DT00,
     LV_IDZ, LV_IDZT, LV_IDZB,  // 1/dz, 1/dz_t, 1/dz_b
+    LV_LB,                     // l_b, bottom-face interpolation weight
     LV_COUNT
 };
 // per-wavenumber tables (x: nx/2 entries, y: ny entries), stacked
@@ -59,6 +60,14 @@ struct StepArgs {
     // fields an accessor reads damped (bit f for field f).
     const double *dmp_s, *dmp_m;
     int kdamp0, dmask;
+    // spectral path (cds = 3): state spectra, product spectra, y-inverted
+    // derivative spectra of the skew-symmetric terms, and the 2/3-rule limits
+    double2 *cu, *cv, *cw, *cp, *ct, *cff, *cgg, *chh, *css;
+    double2 *sp[8];
+    const double2 *sdv[6];
+    double coe;
+    int issk, lc, mc;
+    const double *aux;  // a physical field read by k_div_partial<3> (the spectral divergence)
 };
 
 HC_DEV int wrapi(int i, int n) { return i < 0 ? i + n : (i >= n ? i - n : i); }

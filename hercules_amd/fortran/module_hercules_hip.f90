@@ -6,6 +6,7 @@
 ! replace, so a caller written against the reference's modules links against
 ! these instead:
 !   time_advancement_fd     src/module_navier_stokes.f90:223-365
+!   time_advancement_sp     src/module_navier_stokes.f90:371-517
 !   poisson_solver_fft      src/module_poisson_solver.f90:483-658
 !   calculate_ubar          src/module_parameters.f90:513-546
 !   assign_rk_coeff         src/module_parameters.f90:618-641
@@ -64,6 +65,11 @@ module module_hercules_hip
             integer(c_int) :: rc
         end function
         function hc_time_advancement_fd(ctx) bind(C, name="hc_time_advancement_fd") result(rc)
+            import :: c_ptr, c_int
+            type(c_ptr), value :: ctx
+            integer(c_int) :: rc
+        end function
+        function hc_time_advancement_sp(ctx) bind(C, name="hc_time_advancement_sp") result(rc)
             import :: c_ptr, c_int
             type(c_ptr), value :: ctx
             integer(c_int) :: rc
@@ -174,6 +180,11 @@ contains
     ! src/module_navier_stokes.f90:223-365, on the device-resident state
     subroutine time_advancement_fd()
         call hip_check(hc_time_advancement_fd(hip_ctx), 'time_advancement_fd')
+    end subroutine
+
+    ! src/module_navier_stokes.f90:371-517 (cds=3), on the device-resident state
+    subroutine time_advancement_sp()
+        call hip_check(hc_time_advancement_sp(hip_ctx), 'time_advancement_sp')
     end subroutine
 
     ! src/module_poisson_solver.f90:483-658.  aa, bb, cc are accepted for

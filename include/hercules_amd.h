@@ -34,7 +34,7 @@ typedef struct hc_params {
     int nx, ny, nz;          /* &domain */
     int ochannel;            /* 0 closed, 1 open channel */
     int dp_opt;              /* 1 constant dp/dx, 2 constant bulk velocity, 3 Ekman */
-    int cds;                 /* 1 FD2, 2 FD4 (3 spectral: not yet supported) */
+    int cds;                 /* 1 FD2, 2 FD4, 3 spectral (2/3-rule dealiasing) */
     int dts;                 /* 1 AB2, 2 RK3 */
     int issk;                /* skew-symmetric form (spectral only) */
     int isdamp, nzdamp;      /* Rayleigh damping (not yet supported: must be 0) */
@@ -95,15 +95,25 @@ int hc_set_field(hc_ctx *ctx, int field, const double *src, int layout);
 int hc_get_field(hc_ctx *ctx, int field, double *dst, int layout);
 
 /* Start-up part of drive after the fields are read: bulk velocity for
- * dp_opt=2, Ri ramp start, boundary values (src/module_navier_stokes.f90:60-129). */
+ * dp_opt=2, Ri ramp start, boundary values (src/module_navier_stokes.f90:60-129);
+ * with cds=3 also the spectra of every field, their 2/3-rule truncation and
+ * the truncated physical fields (:96-123).  With cds=3 a later hc_set_field
+ * makes the next substep form the spectra again the same way. */
 int hc_init(hc_ctx *ctx);
 
 /* time_advancement_fd: one full step, all RK3 (or AB2) substeps
- * (src/module_navier_stokes.f90:223-365). */
+ * (src/module_navier_stokes.f90:223-365).  Requires cds = 1 or 2. */
 int hc_time_advancement_fd(hc_ctx *ctx);
 
-/* nsteps passes of the drive loop body: update_ri, time_advancement_fd and
- * calculate_ubar (src/module_navier_stokes.f90:139-165), output excluded. */
+/* time_advancement_sp: one full step of the spectral path
+ * (src/module_navier_stokes.f90:371-517, src/module_spectral.f90).
+ * Requires cds = 3; isdamp = 1 is rejected at hc_create because
+ * rayleigh_damping_spec has no defined result (src/module_spectral.f90:1372). */
+int hc_time_advancement_sp(hc_ctx *ctx);
+
+/* nsteps passes of the drive loop body: update_ri, time_advancement_fd or
+ * time_advancement_sp by cds, and calculate_ubar
+ * (src/module_navier_stokes.f90:139-165), output excluded. */
 int hc_drive_steps(hc_ctx *ctx, int nsteps);
 
 /* poisson_solver_fft for one equation (src/module_poisson_solver.f90:483-658)
