@@ -27,7 +27,7 @@ LAYOUT_INTERIOR, LAYOUT_GHOSTED = 0, 1
 ABI_SYMBOLS = ("hc_last_error", "hc_create", "hc_destroy", "hc_set_field", "hc_get_field",
                "hc_init", "hc_project", "hc_time_advancement_fd", "hc_time_advancement_sp", "hc_drive_steps",
                "hc_poisson_solver_fft",
-               "hc_calculate_ubar", "hc_screen_div_error", "hc_get_scalar", "hc_stream",
+               "hc_calculate_ubar", "hc_screen_div_error", "hc_wall_shear", "hc_get_scalar", "hc_stream",
                "hc_field_device_ptr", "hc_synchronize", "hc_profile_enable", "hc_profile_read")
 KERNELS = {"rhs": 0, "yfft_fwd": 1, "column": 2, "yfft_inv": 3, "finish": 4, "rhs_t": 5,
            "yfft_t": 6, "column_t": 7, "finish_t": 8, "solve_xr2c": 9, "solve_yfft": 10,
@@ -65,6 +65,7 @@ def load_library(path: str = LIB_PATH):
     L.hc_poisson_solver_fft.argtypes = [vp, ip, C.c_double, vp, vp, ip]
     L.hc_calculate_ubar.argtypes = [vp, dp]
     L.hc_screen_div_error.argtypes = [vp, dp, dp]
+    L.hc_wall_shear.argtypes = [vp, dp, dp]
     L.hc_get_scalar.argtypes = [vp, ip, dp]
     L.hc_stream.restype = vp
     L.hc_stream.argtypes = [vp]
@@ -175,6 +176,12 @@ class Channel:
         a, m = C.c_double(), C.c_double()
         self._check(self.L.hc_screen_div_error(self.h, C.byref(a), C.byref(m)))
         return a.value, m.value
+
+    def wall_shear(self):
+        """(tau_bottom, tau_top): plane-averaged nu du/dz at the walls."""
+        b, t = C.c_double(), C.c_double()
+        self._check(self.L.hc_wall_shear(self.h, C.byref(b), C.byref(t)))
+        return b.value, t.value
 
     def scalar(self, name):
         v = C.c_double()

@@ -362,10 +362,15 @@ struct C2RArgs {
     const double *dmp_s, *dmp_mean;
     int kdamp0;
 };
+// points per field of one inverse-x block
+constexpr int kC2RPoints = 1024;
+template <int NX, int F>
+constexpr int c2r_rows() { return kC2RPoints / NX / F > 0 ? kC2RPoints / NX / F : 1; }
+
 template <int NX, int F>
 __global__ __launch_bounds__(kRhsThreads) void k_x_c2r(C2RArgs A)
 {
-    constexpr int ROWS = rows_per_block<NX>() / F > 0 ? rows_per_block<NX>() / F : 1;
+    constexpr int ROWS = c2r_rows<NX, F>();
     constexpr int NL = NX / 2;
     constexpr int B = F * ROWS;
     __shared__ double2 sq_[B * NL];
@@ -490,6 +495,38 @@ HC_DEV void col_store(double2 *__restrict__ dst, double2 *tile, int nz, size_t p
         dst[(size_t)k * plane + col0 + c] = tile[col_tidx(k, c)];
     }
     __syncthreads();
+}
+
+// Multi-field transfers: fields f = 0..n-1 use sub-tiles f * nz * kColWaves
+// of the tile, so several fields share one global phase and one barrier.
+HC_DEV void col_fill(const double2 *__restrict__ src, double2 *sub, int nz, size_t plane, size_t col0)
+{
+    for (int q = threadIdx.x; q < nz * kColWaves; q += 64 * kColWaves) {
+        int k = q / kColWaves, c = q - k * kColWaves;
+        sub[col_tidx(k, c)] = src[(size_t)k * plane + col0 + c];
+    }
+}
+template <int L>
+HC_DEV void col_take(const double2 *sub, const ColMap &m, double2 (&d)[L])
+{
+    const int w = threadIdx.x >> 6;
+#pragma unroll
+    for (int r = 0; r < L; ++r) d[r] = r < m.cnt ? sub[col_tidx(m.k0 + r, w)] : czero();
+}
+template <int L>
+HC_DEV void col_put(double2 *sub, const ColMap &m, const double2 (&d)[L])
+{
+    const int w = threadIdx.x >> 6;
+#pragma unroll
+    for (int r = 0; r < L; ++r)
+        if (r < m.cnt) sub[col_tidx(m.k0 + r, w)] = d[r];
+}
+HC_DEV void col_drain(double2 *__restrict__ dst, const double2 *sub, int nz, size_t plane, size_t col0)
+{
+    for (int q = threadIdx.x; q < nz * kColWaves; q += 64 * kColWaves) {
+        int k = q / kColWaves, c = q - k * kColWaves;
+        dst[(size_t)k * plane + col0 + c] = sub[col_tidx(k, c)];
+    }
 }
 
 // value of level (k0+cnt-1) from register array v (runtime index)
@@ -649,9 +686,22 @@ __global__ __launch_bounds__(64 * kColWaves) void k_column_mom(StepArgs A)
     const bool is00 = (l == 0 && jm == 0);
     const size_t plane = (size_t)ny * nl, col0 = (size_t)jm * nl + (size_t)blockIdx.x * kColWaves;
     double2 duv[2][L], dw[1][L], dp[1][L];
-    col_load<L>(A.s0, tile, nz, plane, col0, m, duv[0]);
-    col_load<L>(A.s1, tile, nz, plane, col0, m, duv[1]);
-    col_load<L>(A.s2, tile, nz, plane, col0, m, dw[0]);
+    // all three inputs in one phase when the tile holds them (A.ntile sub-tiles)
+    const size_t sub = (size_t)nz * kColWaves;
+    if (A.ntile >= 3) {
+        col_fill(A.s0, tile, nz, plane, col0);
+        col_fill(A.s1, tile + sub, nz, plane, col0);
+        col_fill(A.s2, tile + 2 * sub, nz, plane, col0);
+        __syncthreads();
+        col_take<L>(tile, m, duv[0]);
+        col_take<L>(tile + sub, m, duv[1]);
+        col_take<L>(tile + 2 * sub, m, dw[0]);
+        __syncthreads();
+    } else {
+        col_load<L>(A.s0, tile, nz, plane, col0, m, duv[0]);
+        col_load<L>(A.s1, tile, nz, plane, col0, m, duv[1]);
+        col_load<L>(A.s2, tile, nz, plane, col0, m, dw[0]);
+    }
     double a[L], b[L], c[L];
     // u and v share the matrix (assign_abc(aa_u,...) for both, :291-299)
     assemble<L>(A, 0, l, jm, is00, m, a, b, c);
@@ -694,10 +744,22 @@ __global__ __launch_bounds__(64 * kColWaves) void k_column_mom(StepArgs A)
             dpn[r] = make_double2(ph.x - cnu * lap.x, ph.y - cnu * lap.y);
         }
     }
-    col_store<L>(A.s0, tile, nz, plane, col0, m, duv[0]);
-    col_store<L>(A.s1, tile, nz, plane, col0, m, duv[1]);
-    col_store<L>(A.s2, tile, nz, plane, col0, m, dw[0]);
-    col_store<L>(A.s3, tile, nz, plane, col0, m, dpn);
+    if (A.ntile >= 4) {
+        col_put<L>(tile, m, duv[0]);
+        col_put<L>(tile + sub, m, duv[1]);
+        col_put<L>(tile + 2 * sub, m, dw[0]);
+        col_put<L>(tile + 3 * sub, m, dpn);
+        __syncthreads();
+        col_drain(A.s0, tile, nz, plane, col0);
+        col_drain(A.s1, tile + sub, nz, plane, col0);
+        col_drain(A.s2, tile + 2 * sub, nz, plane, col0);
+        col_drain(A.s3, tile + 3 * sub, nz, plane, col0);
+    } else {
+        col_store<L>(A.s0, tile, nz, plane, col0, m, duv[0]);
+        col_store<L>(A.s1, tile, nz, plane, col0, m, duv[1]);
+        col_store<L>(A.s2, tile, nz, plane, col0, m, dw[0]);
+        col_store<L>(A.s3, tile, nz, plane, col0, m, dpn);
+    }
 }
 
 #include "hc_spectral.h"
@@ -799,6 +861,22 @@ __global__ __launch_bounds__(256) void k_ubar_partial(const double *__restrict__
         __syncthreads();
     }
     if (threadIdx.x == 0) part[k] = red[0] * lev[LV_DZ * nz + k];
+}
+// plane sums of one level per block: part[k] = sum of f over level k
+__global__ __launch_bounds__(256) void k_plane_sum(const double *__restrict__ f, int nx, int ny, int k0, double *part)
+{
+    __shared__ double red[256];
+    const int k = k0 + blockIdx.x;
+    double s = 0.0;
+    const size_t n = (size_t)nx * ny;
+    for (size_t q = threadIdx.x; q < n; q += 256) s += f[(size_t)k * n + q];
+    red[threadIdx.x] = s;
+    __syncthreads();
+    for (int w = 128; w > 0; w >>= 1) {
+        if (threadIdx.x < w) red[threadIdx.x] += red[threadIdx.x + w];
+        __syncthreads();
+    }
+    if (threadIdx.x == 0) part[blockIdx.x] = red[0];
 }
 __global__ void k_ubar_final(const double *part, int nz, double lz, int nx, int ny, double *scal)
 {
@@ -916,6 +994,7 @@ struct hc_ctx {
     int stps;
     std::vector<double> h_lev;
     std::vector<void *> allocs;
+    bool big_lds = false;  // column kernels may use more than 64 KiB of dynamic LDS
     // optional per-kernel timing with HIP events on the context stream
     bool prof = false;
     std::vector<hipEvent_t> ev_pool;
@@ -1168,7 +1247,18 @@ static StepArgs base_args(hc_ctx *c)
 static int rows_for(int nx) { return nx >= kRowPoints ? 1 : kRowPoints / nx; }
 
 // dynamic LDS of the column kernels: one nz x 8 complex tile
-static size_t col_lds(const hc_ctx *c) { return (size_t)c->nz * kColWaves * sizeof(double2); }
+// LDS of the column kernels: `nfields` sub-tiles of nz x kColWaves complex
+// when they fit in kColLds bytes, else one.
+constexpr size_t kColLds = 160 * 1024;
+static int col_tiles(const hc_ctx *c, int nfields)
+{
+    const size_t one = (size_t)c->nz * kColWaves * sizeof(double2);
+    return (c->big_lds && one * nfields <= kColLds) ? nfields : 1;
+}
+static size_t col_lds(const hc_ctx *c, int nfields = 1)
+{
+    return (size_t)col_tiles(c, nfields) * c->nz * kColWaves * sizeof(double2);
+}
 
 static int launch_yfft(hc_ctx *c, int sign, int nf, double2 *f0, double2 *f1, double2 *f2, double2 *f3)
 {
@@ -1188,7 +1278,7 @@ static int launch_c2r(hc_ctx *c, int nf, C2RArgs &X)
 {
     if (nf < 1 || nf > 4) return set_err(HC_ERR_ARG, "launch_c2r: %d fields", nf);
     X.ny = c->ny; X.tw_nl = c->tw_nl; X.tw_nx = c->tw_nx;
-    int rows = rows_for(c->nx) / nf > 0 ? rows_for(c->nx) / nf : 1;
+    int rows = kC2RPoints / c->nx / nf > 0 ? kC2RPoints / c->nx / nf : 1;
     dim3 g((c->ny + rows - 1) / rows, c->nz);
 #define HC_C(NX_)                                                                                  \
     if (nf == 4) hipLaunchKernelGGL((k_x_c2r<NX_, 4>), g, dim3(kRhsThreads), 0, c->stream, X);    \
@@ -1268,8 +1358,9 @@ static int substep(hc_ctx *c, int t_its)
     if (int e = launch_yfft(c, -1, 3, c->s[0], c->s[1], c->s[2], c->s[3])) return e;
     PROF_END(c, HC_K_YFFT_FWD);
     dim3 gcol((c->nl + kColWaves - 1) / kColWaves, c->ny);
+    A.ntile = col_tiles(c, 4);
     PROF_BEGIN(c, HC_K_COLUMN);
-#define HC_COL(L_) hipLaunchKernelGGL((k_column_mom<L_>), gcol, dim3(64 * kColWaves), col_lds(c), c->stream, A);
+#define HC_COL(L_) hipLaunchKernelGGL((k_column_mom<L_>), gcol, dim3(64 * kColWaves), col_lds(c, 4), c->stream, A);
     HC_DISPATCH_L(c->Lcol, HC_COL)
 #undef HC_COL
     PROF_END(c, HC_K_COLUMN);
@@ -1424,7 +1515,8 @@ static int substep_sp(hc_ctx *c, int t_its)
     if (int e = launch_yfft(c, -1, 4, c->sp[4], c->sp[5], c->sp[6], c->sp[7])) return e;
     PROF_END(c, HC_K_YFFT_FWD);
     PROF_BEGIN(c, HC_K_COLUMN);
-#define HC_CSP(L_) hipLaunchKernelGGL((k_column_spec<L_>), gcol, dim3(64 * kColWaves), col_lds(c), c->stream, A);
+    A.ntile = col_tiles(c, 4);
+#define HC_CSP(L_) hipLaunchKernelGGL((k_column_spec<L_>), gcol, dim3(64 * kColWaves), col_lds(c, 4), c->stream, A);
     HC_DISPATCH_L(c->Lcol, HC_CSP)
 #undef HC_CSP
     PROF_END(c, HC_K_COLUMN);
@@ -1505,6 +1597,20 @@ extern "C" int hc_create(const hc_params *prm, hc_ctx **out)
     hipError_t he = hipGetDevice(&c->device);
     if (he == hipSuccess) he = hipStreamCreateWithFlags(&c->stream, hipStreamNonBlocking);
     if (he != hipSuccess) { delete c; return set_err(HC_ERR_HIP, "HIP device/stream: %s", hipGetErrorString(he)); }
+    {
+        // allow the multi-field column tiles (up to kColLds of dynamic LDS)
+        hipError_t ae = hipSuccess;
+#define HC_ATTR(L_)                                                                                     \
+    ae = hipFuncSetAttribute(reinterpret_cast<const void *>(&k_column_mom<L_>),                        \
+                             hipFuncAttributeMaxDynamicSharedMemorySize, (int)kColLds);               \
+    if (ae == hipSuccess)                                                                               \
+        ae = hipFuncSetAttribute(reinterpret_cast<const void *>(&k_column_spec<L_>),                   \
+                                 hipFuncAttributeMaxDynamicSharedMemorySize, (int)kColLds);
+        HC_DISPATCH_L(L, HC_ATTR)
+#undef HC_ATTR
+        c->big_lds = ae == hipSuccess;
+        (void)hipGetLastError();
+    }
     const size_t N = (size_t)c->nx * c->ny * c->nz;
     int e = 0;
     double **phys[] = {&c->u, &c->v, &c->w, &c->p, &c->t, &c->fb[0][0], &c->fb[0][1], &c->fb[0][2],
@@ -1814,6 +1920,22 @@ extern "C" int hc_screen_div_error(hc_ctx *c, double *div_avg, double *div_max)
     if (div_avg) *div_avg = s / c->nx / c->ny / c->nz;
     if (div_max) *div_max = m;
     if (nan) return set_err(HC_ERR_NAN, "NaN error!");
+    return 0;
+}
+
+extern "C" int hc_wall_shear(hc_ctx *c, double *tau_bottom, double *tau_top)
+{
+    if (!c) return set_err(HC_ERR_ARG, "null context");
+    const hc_params &P = c->prm;
+    hipLaunchKernelGGL(k_plane_sum, dim3(1), dim3(256), 0, c->stream, c->u, c->nx, c->ny, 0, c->part);
+    hipLaunchKernelGGL(k_plane_sum, dim3(1), dim3(256), 0, c->stream, c->u, c->nx, c->ny, c->nz - 1, c->part + 1);
+    double h[2];
+    HIPCHK(hipMemcpyAsync(h, c->part, sizeof h, hipMemcpyDeviceToHost, c->stream));
+    HIPCHK(hipStreamSynchronize(c->stream));
+    const double plane = (double)c->nx * (double)c->ny;
+    const double dz0 = c->h_lev[(size_t)LV_DZ * c->nz], dze = c->h_lev[(size_t)LV_DZ * c->nz + c->nz - 1];
+    if (tau_bottom) *tau_bottom = 2.0 * P.nu * (h[0] / plane + P.u_mrf) / dz0;
+    if (tau_top) *tau_top = P.ochannel == 0 ? 2.0 * P.nu * (h[1] / plane + P.u_mrf) / dze : 0.0;
     return 0;
 }
 

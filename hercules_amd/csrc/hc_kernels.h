@@ -68,6 +68,7 @@ struct StepArgs {
     double coe;
     int issk, lc, mc;
     const double *aux;  // a physical field read by k_div_partial<3> (the spectral divergence)
+    int ntile;          // column kernels: number of nz x kColWaves sub-tiles in the LDS
 };
 
 HC_DEV int wrapi(int i, int n) { return i < 0 ? i + n : (i >= n ? i - n : i); }

@@ -281,6 +281,8 @@ HC_DEV void add_dzz_w(const StepArgs &A, const ColMap &m, const double2 (&c)[L],
 //   q  = c + dif (-kx^2-ky^2) c + dif Dzz c - rc dt grad_e p + rb dt f_old + ra dt f_new
 //   f_new = -u_mrf i kx c - coe (i kx X + i ky Y) + coe R
 // with X, Y, R the transformed products of k_spec_mom.  f_new replaces f_old.
+// With four LDS sub-tiles (A.ntile >= 4) the component moves in three
+// global phases: {c, X, Y, R} in, {f_old} in, {f_new} out.
 template <int L>
 HC_DEV void spec_assemble(const StepArgs &A, double2 *tile, const ColMap &m, size_t plane, size_t col0, int e,
                           bool is00, double kx, double ky, double hxy, const double2 (&cp)[L], double2 (&q)[L])
@@ -289,8 +291,21 @@ HC_DEV void spec_assemble(const StepArgs &A, double2 *tile, const ColMap &m, siz
     const double dt = A.dt, coe = A.coe;
     const double dif = A.rc * 0.5 * dt * A.nu;
     const double2 *cs = e == 0 ? A.cu : e == 1 ? A.cv : A.cw;
+    const int ix = e == 0 ? 0 : e == 1 ? 1 : 5, iy = e == 0 ? 1 : e == 1 ? 2 : 6, ir = e == 0 ? 3 : e == 1 ? 4 : 7;
+    double2 *fo = e == 0 ? A.cff : e == 1 ? A.cgg : A.chh;
+    const bool batch = A.ntile >= 4;
+    const size_t sub = (size_t)nz * kColWaves;
     double2 x[L], f[L];
-    col_load<L>(cs, tile, nz, plane, col0, m, x);
+    if (batch) {
+        col_fill(cs, tile, nz, plane, col0);
+        col_fill(A.sp[ix], tile + sub, nz, plane, col0);
+        col_fill(A.sp[iy], tile + 2 * sub, nz, plane, col0);
+        col_fill(A.sp[ir], tile + 3 * sub, nz, plane, col0);
+        __syncthreads();
+        col_take<L>(tile, m, x);
+    } else {
+        col_load<L>(cs, tile, nz, plane, col0, m, x);
+    }
 #pragma unroll
     for (int r = 0; r < L; ++r) {
         q[r] = make_double2(x[r].x + hxy * x[r].x, x[r].y + hxy * x[r].y);
@@ -298,14 +313,16 @@ HC_DEV void spec_assemble(const StepArgs &A, double2 *tile, const ColMap &m, siz
     }
     if (e == 2) add_dzz_w<L>(A, m, x, dif, q);
     else add_dzz<L>(A, m, e, is00, x, dif, q);
-    const int ix = e == 0 ? 0 : e == 1 ? 1 : 5, iy = e == 0 ? 1 : e == 1 ? 2 : 6, ir = e == 0 ? 3 : e == 1 ? 4 : 7;
-    col_load<L>(A.sp[ix], tile, nz, plane, col0, m, x);
+    if (batch) col_take<L>(tile + sub, m, x);
+    else col_load<L>(A.sp[ix], tile, nz, plane, col0, m, x);
 #pragma unroll
     for (int r = 0; r < L; ++r) f[r] = csub(f[r], times_i(x[r], coe * kx));
-    col_load<L>(A.sp[iy], tile, nz, plane, col0, m, x);
+    if (batch) col_take<L>(tile + 2 * sub, m, x);
+    else col_load<L>(A.sp[iy], tile, nz, plane, col0, m, x);
 #pragma unroll
     for (int r = 0; r < L; ++r) f[r] = csub(f[r], times_i(x[r], coe * ky));
-    col_load<L>(A.sp[ir], tile, nz, plane, col0, m, x);
+    if (batch) col_take<L>(tile + 3 * sub, m, x);
+    else col_load<L>(A.sp[ir], tile, nz, plane, col0, m, x);
 #pragma unroll
     for (int r = 0; r < L; ++r) f[r] = make_double2(f[r].x + coe * x[r].x, f[r].y + coe * x[r].y);
     // pressure gradient
@@ -325,15 +342,28 @@ HC_DEV void spec_assemble(const StepArgs &A, double2 *tile, const ColMap &m, siz
             q[r] = make_double2(q[r].x - g * (pn.x - cp[r].x), q[r].y - g * (pn.y - cp[r].y));
         }
     }
-    double2 *fo = e == 0 ? A.cff : e == 1 ? A.cgg : A.chh;
-    col_load<L>(fo, tile, nz, plane, col0, m, x);
+    if (batch) {
+        __syncthreads();
+        col_fill(fo, tile, nz, plane, col0);
+        __syncthreads();
+        col_take<L>(tile, m, x);
+    } else {
+        col_load<L>(fo, tile, nz, plane, col0, m, x);
+    }
     const double rb = A.rb * dt, ra = A.ra * dt;
 #pragma unroll
     for (int r = 0; r < L; ++r) {
         q[r] = make_double2(q[r].x + rb * x[r].x, q[r].y + rb * x[r].y);
         q[r] = make_double2(q[r].x + ra * f[r].x, q[r].y + ra * f[r].y);
     }
-    col_store<L>(fo, tile, nz, plane, col0, m, f);
+    if (batch) {
+        col_put<L>(tile + sub, m, f);
+        __syncthreads();
+        col_drain(fo, tile + sub, nz, plane, col0);
+        __syncthreads();
+    } else {
+        col_store<L>(fo, tile, nz, plane, col0, m, f);
+    }
 }
 
 // Spectral substep column: assembly of qu, qv, qw, the u, v, w Helmholtz
@@ -395,10 +425,23 @@ __global__ __launch_bounds__(64 * kColWaves) void k_column_spec(StepArgs A)
             duv[0][r] = duv[1][r] = dw[0][r] = cpr[r] = czero();
         }
     }
-    col_store<L>(A.cu, tile, nz, plane, col0, m, duv[0]);
-    col_store<L>(A.cv, tile, nz, plane, col0, m, duv[1]);
-    col_store<L>(A.cw, tile, nz, plane, col0, m, dw[0]);
-    col_store<L>(A.cp, tile, nz, plane, col0, m, cpr);
+    if (A.ntile >= 4) {
+        const size_t sub = (size_t)nz * kColWaves;
+        col_put<L>(tile, m, duv[0]);
+        col_put<L>(tile + sub, m, duv[1]);
+        col_put<L>(tile + 2 * sub, m, dw[0]);
+        col_put<L>(tile + 3 * sub, m, cpr);
+        __syncthreads();
+        col_drain(A.cu, tile, nz, plane, col0);
+        col_drain(A.cv, tile + sub, nz, plane, col0);
+        col_drain(A.cw, tile + 2 * sub, nz, plane, col0);
+        col_drain(A.cp, tile + 3 * sub, nz, plane, col0);
+    } else {
+        col_store<L>(A.cu, tile, nz, plane, col0, m, duv[0]);
+        col_store<L>(A.cv, tile, nz, plane, col0, m, duv[1]);
+        col_store<L>(A.cw, tile, nz, plane, col0, m, dw[0]);
+        col_store<L>(A.cp, tile, nz, plane, col0, m, cpr);
+    }
 }
 
 // Spectral temperature column (src/module_navier_stokes.f90:481-495):

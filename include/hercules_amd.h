@@ -137,6 +137,15 @@ int hc_calculate_ubar(hc_ctx *ctx, double *ubar);
  * reference's signed maximum, and HC_ERR_NAN when u, v or w hold a NaN. */
 int hc_screen_div_error(hc_ctx *ctx, double *div_avg, double *div_max);
 
+/* Plane-averaged wall shear stress nu * du/dz at the walls, from the no-slip
+ * ghost rule u(kstr3-1) = -u(kstr3) - 2 u_mrf (src/module_boundary.f90:52-53,
+ * 66) and dz_b(kstr3) = dz(kstr3), dz_t(kend3) = dz(kend3)
+ * (src/module_parameters.f90:469-492): tau_bottom = 2 nu (<u(kstr3)> + u_mrf) /
+ * dz(kstr3), tau_top likewise at kend3 for a closed channel and 0 for the
+ * free-slip top.  The reference has no such routine; its inputs are the
+ * module fields above. */
+int hc_wall_shear(hc_ctx *ctx, double *tau_bottom, double *tau_top);
+
 /* Scalars of the reference's module state: 0 dpx_drive, 1 ubar, 2 time_sim,
  * 3 got, 4 number of steps taken. */
 int hc_get_scalar(hc_ctx *ctx, int id, double *value);

@@ -18,7 +18,7 @@ DECK = """&domain
   nx = {nx}, ny = {ny}, nz = {nz}, zstretch = {zs!r}, ochannel = 0
 /
 &modeling
-  dp_opt = 1, cds = 2, issk = 1, dts = 2, isnoise = 0, noise_mag = 0.1
+  dp_opt = 1, cds = {cds}, issk = 1, dts = 2, isnoise = 0, noise_mag = 0.1
   isdamp = 0, nzdamp = 15, cdamp = 0.01
 /
 &constants
@@ -33,10 +33,13 @@ DECK = """&domain
 """
 
 
-def test_fortran_driver_matches_oracle(tmp_path):
+@pytest.mark.parametrize("cds", [2, 3])
+def test_fortran_driver_matches_oracle(tmp_path, cds):
+    """cds=2: time_advancement_fd; cds=3: time_advancement_sp (the driver's
+    step loop dispatches on cds as drive does)."""
     if not os.path.exists(FEXE):
         pytest.fail(f"Fortran host not built: {FEXE}")
-    prm = cases.closed_channel(nx=32, ny=32, nz=33, imax=4)
+    prm = cases.closed_channel(nx=32, ny=32, nz=33, imax=4, cds=cds, issk=1)
     st = cases.laminar_noise(prm, seed=21)
     with Channel(prm) as ch:
         for k, v in st.items():
@@ -46,7 +49,7 @@ def test_fortran_driver_matches_oracle(tmp_path):
     (tmp_path / "results").mkdir()
     (tmp_path / "parameters.input").write_text(DECK.format(
         lx=prm.lx, ly=prm.ly, nx=prm.nx, ny=prm.ny, nz=prm.nz, zs=prm.zstretch, nu=prm.nu,
-        alpha=prm.alpha, imax=prm.imax))
+        alpha=prm.alpha, imax=prm.imax, cds=cds))
     for k, v in st.items():  # Fortran (i,j,k) order == C order (nz, ny, nx)
         np.ascontiguousarray(v).tofile(tmp_path / "results" / f"init_{k}.dat")
     r = subprocess.run([FEXE], cwd=tmp_path, capture_output=True, text=True, timeout=120)

@@ -159,3 +159,24 @@ def test_config1_100_steps_match_oracle():
     assert abs(tw_g[0] - tw_o[0]) <= 1e-10 * abs(tw_o[0])
     assert abs(tw_g[1] - tw_o[1]) <= 1e-10 * abs(tw_o[1])
     ch.close()
+
+
+@pytest.mark.parametrize("kind,kw", [("closed", {"u_mrf": 0.25}), ("open_t", {})])
+def test_wall_shear_entry(kind, kw):
+    """hc_wall_shear against the closed form on the oracle's state after two
+    steps: 2 nu (<u_1> + u_mrf) / dz_1 (no-slip ghost rule), top likewise for a
+    closed channel and 0 under the free-slip top."""
+    prm, st = small_case(kind, **kw)
+    o = make_oracle(prm, st)
+    ch = _channel(prm, st)
+    o.steps(2)
+    ch.steps(2)
+    got = ch.wall_shear()
+    want = cases.wall_shear(o.get("u"), prm)
+    print(f"{kind}: wall shear gpu {got} oracle fields {want}")
+    assert abs(got[0] - want[0]) <= 1e-10 * abs(want[0])
+    if prm.ochannel == 0:
+        assert abs(got[1] - want[1]) <= 1e-10 * abs(want[1])
+    else:
+        assert got[1] == 0.0
+    ch.close()
