@@ -185,7 +185,7 @@ class Channel:
 
     def scalar(self, name):
         v = C.c_double()
-        ids = {"dpx_drive": 0, "ubar": 1, "time_sim": 2, "got": 3, "stps": 4}
+        ids = {"dpx_drive": 0, "ubar": 1, "time_sim": 2, "got": 3, "stps": 4, "col_tiles": 5}
         self._check(self.L.hc_get_scalar(self.h, ids[name], C.byref(v)))
         return v.value
 

@@ -142,16 +142,36 @@ __global__ __launch_bounds__(rhs2_threads<NX>()) void k_rhs_mom2(StepArgs A)
     __shared__ double sm[Tl::NROWS * NX];
     const int tid = threadIdx.x;
     const int nty = A.ny / TY;
+    // tile order: groups of G consecutive y tiles, levels fastest-changing
+    // across groups, so that an XCD's L2 holds both the y halos and the
+    // neighbouring levels of the tiles in flight
+    const int G = (nty % 16 == 0) ? 16 : (nty % 8 == 0) ? 8 : (nty % 4 == 0) ? 4 : 1;
     const int logical = xcd_remap(blockIdx.x, nty * A.nz);
-    const int jt = logical % nty, k = logical / nty, j0 = jt * TY;
+    const int grp = logical / (G * A.nz), rem = logical - grp * (G * A.nz);
+    const int k = rem / G, jt = grp * G + rem % G, j0 = jt * TY;
     const int i = tid % NX, jj = tid / NX, j = j0 + jj;
     const bool top = k == A.nz - 1;
-    for (int q = tid; q < Tl::NRAW * NX; q += T) {
-        int r = q / NX, x = q - r * NX;
-        sm[q] = tile2_raw<NX, TY>(A, r, x, j0, k);
+    // staging: every thread issues all of its loads before the first use
+    constexpr int NST = (Tl::NRAW * NX + T - 1) / T;
+    double stv[NST];
+#pragma unroll
+    for (int m = 0; m < NST; ++m) {
+        const int q = tid + m * T;
+        stv[m] = 0.0;
+        if ((Tl::NRAW * NX) % T == 0 || q < Tl::NRAW * NX) {
+            const int r = q / NX, x = q - r * NX;
+            const double *f = r < Tl::OV ? A.u : r < Tl::OW ? A.v : r < Tl::OP ? A.w : A.p;
+            const int jr = r < Tl::OP ? j0 - 3 + r % Tl::RU : j0 - 1 + (r - Tl::OP);
+            stv[m] = f[((unsigned)k * (unsigned)A.ny + (unsigned)wrapi(jr, A.ny)) * (unsigned)NX + (unsigned)x];
+        }
     }
     PointVals2 pv;
     pv.load(A, i, j, k);
+#pragma unroll
+    for (int m = 0; m < NST; ++m) {
+        const int q = tid + m * T;
+        if ((Tl::NRAW * NX) % T == 0 || q < Tl::NRAW * NX) sm[q] = stv[m];
+    }
     // v(k+1) under the why*vhz rows, loaded with the staging
     double vkp[PF];
 #pragma unroll
@@ -375,29 +395,54 @@ __global__ __launch_bounds__(kRhsThreads) void k_x_c2r(C2RArgs A)
     constexpr int B = F * ROWS;
     __shared__ double2 sq_[B * NL];
     const int k = blockIdx.y, j0 = blockIdx.x * ROWS, tid = threadIdx.x;
-    for (int p = tid; p < B * NL; p += kRhsThreads) {
-        int row = p / NL, l = p - row * NL, f = row / ROWS, jj = row - f * ROWS, j = j0 + jj;
-        double2 z = czero();
-        if (j < A.ny) {
+    // the rows and, for the accumulated fields, the old values are loaded
+    // before any is used
+    constexpr int NIN = (B * NL + kRhsThreads - 1) / kRhsThreads;
+    constexpr int NOUT = (B * NX + kRhsThreads - 1) / kRhsThreads;
+    double2 xl[NIN], xt[NIN];
+#pragma unroll
+    for (int m = 0; m < NIN; ++m) {
+        const int p = tid + m * kRhsThreads;
+        const int row = p / NL, l = p - row * NL, f = row / ROWS, jj = row - f * ROWS, j = j0 + jj;
+        xl[m] = czero();
+        xt[m] = czero();
+        if (((B * NL) % kRhsThreads == 0 || p < B * NL) && j < A.ny) {
             const double2 *s = A.src[f] + ((size_t)k * A.ny + j) * NL;
-            double2 xl = s[l];
-            double2 xt = l == 0 ? czero() : cconj(s[NL - l]);
-            z = c2r_pre(xl, xt, A.tw_nx[l]);
+            xl[m] = s[l];
+            if (l != 0) xt[m] = cconj(s[NL - l]);
         }
-        sq_[row * NL + l] = z;
+    }
+    double old[NOUT];
+#pragma unroll
+    for (int m = 0; m < NOUT; ++m) {
+        const int p = tid + m * kRhsThreads;
+        const int row = p / NX, i = p - row * NX, f = row / ROWS, jj = row - f * ROWS, j = j0 + jj;
+        old[m] = 0.0;
+        if (((B * NX) % kRhsThreads == 0 || p < B * NX) && j < A.ny && A.mode[f] != 0)
+            old[m] = A.dst[f][((size_t)k * A.ny + j) * NX + i];
+    }
+#pragma unroll
+    for (int m = 0; m < NIN; ++m) {
+        const int p = tid + m * kRhsThreads;
+        if ((B * NL) % kRhsThreads == 0 || p < B * NL) {
+            const int l = p % NL;
+            sq_[p] = c2r_pre(xl[m], xt[m], A.tw_nx[l]);
+        }
     }
     __syncthreads();
     lds_fft<NL, B, NL, 1, kRhsThreads, +1>(sq_, A.tw_nl, tid);
     const double *sqd = reinterpret_cast<const double *>(sq_);
-    for (int p = tid; p < B * NX; p += kRhsThreads) {
-        int row = p / NX, i = p - row * NX, f = row / ROWS, jj = row - f * ROWS, j = j0 + jj;
-        if (j < A.ny) {
+#pragma unroll
+    for (int m = 0; m < NOUT; ++m) {
+        const int p = tid + m * kRhsThreads;
+        const int row = p / NX, i = p - row * NX, f = row / ROWS, jj = row - f * ROWS, j = j0 + jj;
+        if (((B * NX) % kRhsThreads == 0 || p < B * NX) && j < A.ny) {
             size_t c = ((size_t)k * A.ny + j) * NX + i;
             double x = sqd[row * NX + i];
             if (A.mode[f] == 1) {
-                A.dst[f][c] += x;
+                A.dst[f][c] = old[m] + x;
             } else if (A.mode[f] == 2) {
-                double o = A.dst[f][c];
+                double o = old[m];
                 if (k >= A.kdamp0) o = o - A.dmp_s[k] * (o - A.dmp_mean[k]);
                 A.dst[f][c] = o + x;
             } else {
@@ -504,6 +549,30 @@ HC_DEV void col_fill(const double2 *__restrict__ src, double2 *sub, int nz, size
     for (int q = threadIdx.x; q < nz * kColWaves; q += 64 * kColWaves) {
         int k = q / kColWaves, c = q - k * kColWaves;
         sub[col_tidx(k, c)] = src[(size_t)k * plane + col0 + c];
+    }
+}
+// NF fields at once: every load is issued before the first LDS store
+// (nz <= 64 L, so each thread moves at most L elements per field)
+template <int L, int NF>
+HC_DEV void col_fill_n(const double2 *const (&src)[NF], double2 *tile, int nz, size_t plane, size_t col0)
+{
+    const size_t sub = (size_t)nz * kColWaves;
+    double2 v[NF][L];
+#pragma unroll
+    for (int it = 0; it < L; ++it) {
+        const int q = threadIdx.x + it * 64 * kColWaves;
+        const int k = q / kColWaves, c = q - k * kColWaves;
+#pragma unroll
+        for (int f = 0; f < NF; ++f) v[f][it] = q < nz * kColWaves ? src[f][(size_t)k * plane + col0 + c] : czero();
+    }
+#pragma unroll
+    for (int it = 0; it < L; ++it) {
+        const int q = threadIdx.x + it * 64 * kColWaves;
+        const int k = q / kColWaves, c = q - k * kColWaves;
+        if (q < nz * kColWaves) {
+#pragma unroll
+            for (int f = 0; f < NF; ++f) tile[f * sub + col_tidx(k, c)] = v[f][it];
+        }
     }
 }
 template <int L>
@@ -689,9 +758,8 @@ __global__ __launch_bounds__(64 * kColWaves) void k_column_mom(StepArgs A)
     // all three inputs in one phase when the tile holds them (A.ntile sub-tiles)
     const size_t sub = (size_t)nz * kColWaves;
     if (A.ntile >= 3) {
-        col_fill(A.s0, tile, nz, plane, col0);
-        col_fill(A.s1, tile + sub, nz, plane, col0);
-        col_fill(A.s2, tile + 2 * sub, nz, plane, col0);
+        const double2 *srcs[3] = {A.s0, A.s1, A.s2};
+        col_fill_n<L, 3>(srcs, tile, nz, plane, col0);
         __syncthreads();
         col_take<L>(tile, m, duv[0]);
         col_take<L>(tile + sub, m, duv[1]);
@@ -1953,6 +2021,7 @@ extern "C" int hc_get_scalar(hc_ctx *c, int id, double *value)
     case 2: *value = c->time_sim; return 0;
     case 3: *value = c->got; return 0;
     case 4: *value = c->stps; return 0;
+    case 5: *value = col_tiles(c, 4); return 0;
     default: return set_err(HC_ERR_ARG, "unknown scalar %d", id);
     }
 }

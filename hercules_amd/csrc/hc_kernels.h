@@ -668,32 +668,46 @@ struct PointVals2 {
     double u_km, v_km, u_kp, v_kp, w_kp, p_kp, t_k, t_kp, f0, g0, h0;
     double wkx[4];  // w(k-1) at x-1, x, x+1, x+2 (row j)
     double wky[3];  // w(k-1) at rows j-1, j+1, j+2 (column x)
-    HC_DEV void load(const StepArgs &a, int i, int j, int k)
+    // Every load is issued from a clamped level before any value is used;
+    // the wall rules of update_boundary_uvw / _p / _t
+    // (src/module_boundary.f90:33-196) are then applied by selects on the
+    // block-uniform level k.
+    HC_DEV void load(const StepArgs &a, int i, int j, int k, bool with_old = true)
     {
-        Pt g(a, i, j, k);
-        u_km = g.U(0, 0, -1);
-        v_km = g.V(0, 0, -1);
-        u_kp = g.U(0, 0, 1);
-        v_kp = g.V(0, 0, 1);
-        const bool in = k + 1 < a.nz;
-        w_kp = in ? g.W(0, 0, 1) : 0.0;
-        p_kp = g.P(0, 0, 1);
-        t_k = a.isscalar ? g.T(0, 0, 0) : 0.0;
-        t_kp = (a.isscalar && in) ? g.T(0, 0, 1) : 0.0;
-        f0 = g.f0();
-        g0 = g.g0();
-        h0 = g.h0();
-        if (k > 0) {
+        const int nx = a.nx, ny = a.ny, nz = a.nz;
+        const bool bot = k == 0, top = k == nz - 1;
+        const int km = bot ? 0 : k - 1, kp = top ? nz - 1 : k + 1;
+        auto at = [&](int kk, int jj, int ii) -> unsigned {
+            return ((unsigned)kk * (unsigned)ny + (unsigned)wrapi(jj, ny)) * (unsigned)nx + (unsigned)wrapi(ii, nx);
+        };
+        const unsigned ck = at(k, j, i), ckm = at(km, j, i), ckp = at(kp, j, i);
+        const double ukm = a.u[ckm], vkm = a.v[ckm], ukp = a.u[ckp], vkp = a.v[ckp];
+        const double wkp = a.w[ckp], pkp = a.p[ckp];
+        const double tk = a.isscalar ? a.t[ck] : 0.0, tkp = a.isscalar ? a.t[ckp] : 0.0;
+        double o0 = 0.0, o1 = 0.0, o2 = 0.0;
+        if (with_old && a.use_f0) { o0 = a.ff0[ck]; o1 = a.gg0[ck]; o2 = a.hh0[ck]; }
+        double wx[4], wy[3];
 #pragma unroll
-            for (int d = 0; d < 4; ++d) wkx[d] = g.W(d - 1, 0, -1);
-            wky[0] = g.W(0, -1, -1);
-            wky[1] = g.W(0, 1, -1);
-            wky[2] = g.W(0, 2, -1);
-        } else {
+        for (int d = 0; d < 4; ++d) wx[d] = a.w[at(km, j, i + d - 1)];
+        wy[0] = a.w[at(km, j - 1, i)];
+        wy[1] = a.w[at(km, j + 1, i)];
+        wy[2] = a.w[at(km, j + 2, i)];
+        const double um = a.u_mrf;
+        u_km = bot ? -ukm - 2.0 * um : ukm;
+        v_km = bot ? -vkm : vkm;
+        u_kp = top ? (a.ochannel == 0 ? -ukp - 2.0 * um : ukp) : ukp;
+        v_kp = top ? (a.ochannel == 0 ? -vkp : vkp) : vkp;
+        w_kp = (top || k + 1 == nz - 1) ? 0.0 : wkp;
+        p_kp = pkp;
+        t_k = tk;
+        t_kp = top ? 0.0 : tkp;
+        f0 = o0;
+        g0 = o1;
+        h0 = o2;
 #pragma unroll
-            for (int d = 0; d < 4; ++d) wkx[d] = 0.0;
-            wky[0] = wky[1] = wky[2] = 0.0;
-        }
+        for (int d = 0; d < 4; ++d) wkx[d] = bot ? 0.0 : wx[d];
+#pragma unroll
+        for (int d = 0; d < 3; ++d) wky[d] = bot ? 0.0 : wy[d];
     }
 };
 

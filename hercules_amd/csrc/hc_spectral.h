@@ -297,10 +297,8 @@ HC_DEV void spec_assemble(const StepArgs &A, double2 *tile, const ColMap &m, siz
     const size_t sub = (size_t)nz * kColWaves;
     double2 x[L], f[L];
     if (batch) {
-        col_fill(cs, tile, nz, plane, col0);
-        col_fill(A.sp[ix], tile + sub, nz, plane, col0);
-        col_fill(A.sp[iy], tile + 2 * sub, nz, plane, col0);
-        col_fill(A.sp[ir], tile + 3 * sub, nz, plane, col0);
+        const double2 *srcs[4] = {cs, A.sp[ix], A.sp[iy], A.sp[ir]};
+        col_fill_n<L, 4>(srcs, tile, nz, plane, col0);
         __syncthreads();
         col_take<L>(tile, m, x);
     } else {

@@ -147,7 +147,9 @@ int hc_screen_div_error(hc_ctx *ctx, double *div_avg, double *div_max);
 int hc_wall_shear(hc_ctx *ctx, double *tau_bottom, double *tau_top);
 
 /* Scalars of the reference's module state: 0 dpx_drive, 1 ubar, 2 time_sim,
- * 3 got, 4 number of steps taken. */
+ * 3 got, 4 number of steps taken; and 5, the number of fields the column
+ * kernels move per global phase (LDS sub-tiles, 1 or 4; no reference
+ * counterpart). */
 int hc_get_scalar(hc_ctx *ctx, int id, double *value);
 
 /* The HIP stream (hipStream_t) all work of this context is queued on, and
