@@ -140,7 +140,10 @@ __global__ __launch_bounds__(rhs2_threads<NX>()) void k_rhs_mom2(StepArgs A)
     static_assert(Tl::NRAW >= 3 * TY, "FFT rows must fit in the staging area");
     constexpr int NFWV = (TY + 3) * NX, PF = (NFWV + T - 1) / T;
     __shared__ double sm[Tl::NROWS * NX];
+    __shared__ double2 tws[NX];  // twiddles of the x transform: half-length, then R2C post
     const int tid = threadIdx.x;
+    double2 tw_a = czero(), tw_b = czero();
+    if (tid < NL) { tw_a = A.tw_nl[tid]; tw_b = A.tw_nx[tid]; }
     const int nty = A.ny / TY;
     // tile order: groups of G consecutive y tiles, levels fastest-changing
     // across groups, so that an XCD's L2 holds both the y halos and the
@@ -186,6 +189,7 @@ __global__ __launch_bounds__(rhs2_threads<NX>()) void k_rhs_mom2(StepArgs A)
     const double dpx = A.dp_opt == 2 ? A.scal[0] : A.dpx;
     const double um = A.u_mrf;
     const double lt = top ? 0.0 : A.lev[LV_LT * A.nz + k];
+    if (tid < NL) { tws[tid] = tw_a; tws[NL + tid] = tw_b; }
     __syncthreads();
     // ---- phase 1: interpolants and flux products of level k
     {
@@ -241,11 +245,11 @@ __global__ __launch_bounds__(rhs2_threads<NX>()) void k_rhs_mom2(StepArgs A)
     sm[(2 * TY + jj) * NX + i] = qw;
     __syncthreads();
     double2 *sq_ = reinterpret_cast<double2 *>(sm);
-    lds_fft<NL, 3 * TY, NL, 1, T, -1>(sq_, A.tw_nl, tid);
-    double2 *outs[3] = {A.s0, A.s1, A.s2};
+    lds_fft<NL, 3 * TY, NL, 1, T, -1>(sq_, tws, tid);
     for (int p = tid; p < 3 * TY * NL; p += T) {
         int row = p / NL, l = p - row * NL, fi = row / TY, r = row - fi * TY;
-        outs[fi][((size_t)k * A.ny + j0 + r) * NL + l] = r2c_post<NL>(sq_ + row * NL, l, A.tw_nx, A.norm);
+        double2 *o = fi == 0 ? A.s0 : fi == 1 ? A.s1 : A.s2;
+        o[((size_t)k * A.ny + j0 + r) * NL + l] = r2c_post<NL>(sq_ + row * NL, l, tws + NL, A.norm);
     }
 }
 
@@ -394,7 +398,10 @@ __global__ __launch_bounds__(kRhsThreads) void k_x_c2r(C2RArgs A)
     constexpr int NL = NX / 2;
     constexpr int B = F * ROWS;
     __shared__ double2 sq_[B * NL];
+    __shared__ double2 tws[2 * NL];  // half-length twiddles, then the C2R pre-twiddles
     const int k = blockIdx.y, j0 = blockIdx.x * ROWS, tid = threadIdx.x;
+    double2 tw_a = czero(), tw_b = czero();
+    if (tid < NL) { tw_a = A.tw_nl[tid]; tw_b = A.tw_nx[tid]; }
     // the rows and, for the accumulated fields, the old values are loaded
     // before any is used
     constexpr int NIN = (B * NL + kRhsThreads - 1) / kRhsThreads;
@@ -421,16 +428,18 @@ __global__ __launch_bounds__(kRhsThreads) void k_x_c2r(C2RArgs A)
         if (((B * NX) % kRhsThreads == 0 || p < B * NX) && j < A.ny && A.mode[f] != 0)
             old[m] = A.dst[f][((size_t)k * A.ny + j) * NX + i];
     }
+    if (tid < NL) { tws[tid] = tw_a; tws[NL + tid] = tw_b; }
+    __syncthreads();
 #pragma unroll
     for (int m = 0; m < NIN; ++m) {
         const int p = tid + m * kRhsThreads;
         if ((B * NL) % kRhsThreads == 0 || p < B * NL) {
             const int l = p % NL;
-            sq_[p] = c2r_pre(xl[m], xt[m], A.tw_nx[l]);
+            sq_[p] = c2r_pre(xl[m], xt[m], tws[NL + l]);
         }
     }
     __syncthreads();
-    lds_fft<NL, B, NL, 1, kRhsThreads, +1>(sq_, A.tw_nl, tid);
+    lds_fft<NL, B, NL, 1, kRhsThreads, +1>(sq_, tws, tid);
     const double *sqd = reinterpret_cast<const double *>(sq_);
 #pragma unroll
     for (int m = 0; m < NOUT; ++m) {
@@ -468,16 +477,24 @@ template <int NY, int SIGN>
 __global__ __launch_bounds__(256) void k_yfft(YArgs A)
 {
     __shared__ double2 tile[NY * kYPad];
+    __shared__ double2 twy[NY];
     double2 *fld = A.f[blockIdx.z];
     const int l0 = blockIdx.x * kYTile, k = blockIdx.y, tid = threadIdx.x;
-    for (int p = tid; p < NY * kYTile; p += 256) {
-        int j = p / kYTile, ll = p - j * kYTile;
-        double2 v = fld[((size_t)k * NY + j) * A.nl + l0 + ll];
-        if (SIGN > 0 && j == NY / 2) v = czero();
-        tile[j * kYPad + ll] = v;
+    for (int q = tid; q < NY; q += 256) twy[q] = A.tw_ny[q];
+    constexpr int NIT = (NY * kYTile + 255) / 256;
+    double2 v[NIT];
+#pragma unroll
+    for (int m = 0; m < NIT; ++m) {
+        const int p = tid + m * 256, j = p / kYTile, ll = p - j * kYTile;
+        v[m] = p < NY * kYTile ? fld[((size_t)k * NY + j) * A.nl + l0 + ll] : czero();
+    }
+#pragma unroll
+    for (int m = 0; m < NIT; ++m) {
+        const int p = tid + m * 256, j = p / kYTile, ll = p - j * kYTile;
+        if (p < NY * kYTile) tile[j * kYPad + ll] = (SIGN > 0 && j == NY / 2) ? czero() : v[m];
     }
     __syncthreads();
-    lds_fft<NY, kYTile, 1, kYPad, 256, SIGN>(tile, A.tw_ny, tid);
+    lds_fft<NY, kYTile, 1, kYPad, 256, SIGN>(tile, twy, tid);
     for (int p = tid; p < NY * kYTile; p += 256) {
         int j = p / kYTile, ll = p - j * kYTile;
         double2 v = tile[j * kYPad + ll];

@@ -36,16 +36,27 @@ __global__ __launch_bounds__(256) void k_yfft_io(YIOArgs A)
     double2 *dst = A.dst[f];
     const int mul = A.mul[f];
     const int l0 = blockIdx.x * kYTile, k = blockIdx.y, tid = threadIdx.x;
-    for (int p = tid; p < NY * kYTile; p += 256) {
-        int j = p / kYTile, ll = p - j * kYTile;
-        double2 v = src[((size_t)k * NY + j) * A.nl + l0 + ll];
+    __shared__ double2 twy[NY];
+    for (int q = tid; q < NY; q += 256) twy[q] = A.tw_ny[q];
+    constexpr int NIT = (NY * kYTile + 255) / 256;
+    double2 vv[NIT];
+#pragma unroll
+    for (int m = 0; m < NIT; ++m) {
+        const int p = tid + m * 256, j = p / kYTile, ll = p - j * kYTile;
+        vv[m] = p < NY * kYTile ? src[((size_t)k * NY + j) * A.nl + l0 + ll] : czero();
+    }
+#pragma unroll
+    for (int m = 0; m < NIT; ++m) {
+        const int p = tid + m * 256, j = p / kYTile, ll = p - j * kYTile;
+        if (p >= NY * kYTile) continue;
+        double2 v = vv[m];
         if (mul == 1) v = times_i(v, A.symx[SY_GIM * A.nl + l0 + ll]);
         else if (mul == 2) v = times_i(v, A.symy[SY_GIM * A.ny + j]);
         if (SIGN > 0 && j == NY / 2) v = czero();
         tile[j * kYPad + ll] = v;
     }
     __syncthreads();
-    lds_fft<NY, kYTile, 1, kYPad, 256, SIGN>(tile, A.tw_ny, tid);
+    lds_fft<NY, kYTile, 1, kYPad, 256, SIGN>(tile, twy, tid);
     for (int p = tid; p < NY * kYTile; p += 256) {
         int j = p / kYTile, ll = p - j * kYTile;
         double2 v = tile[j * kYPad + ll];
