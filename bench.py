@@ -83,6 +83,83 @@ def build_case(workload):
     return prm, st
 
 
+def ensemble_members(seed=2024):
+    """The 24-member ensemble: eight each at 128x128x129, 192x192x129 and
+    256x128x193; neutral closed channels, stratified open channels and
+    Ekman-type open channels (Coriolis drive with a top sponge), Re_tau
+    seeded uniform in [150, 400] and Ri uniform in [0, 0.2]."""
+    from hercules_amd import cases
+    rng = np.random.default_rng(seed)
+    shapes = [(128, 128, 129)] * 8 + [(192, 192, 129)] * 8 + [(256, 128, 193)] * 8
+    out = []
+    for q, (nx, ny, nz) in enumerate(shapes):
+        re_tau = rng.uniform(150.0, 400.0)
+        ri = rng.uniform(0.0, 0.2)
+        kind = q % 3
+        common = dict(nx=nx, ny=ny, nz=nz, nu=1.0 / re_tau, alpha=1.0 / re_tau / 0.71, wall_rows=1,
+                      dt=5e-4)
+        if kind == 0:
+            prm = cases.closed_channel(**common)
+        elif kind == 1:
+            prm = cases.stratified_open_channel(got=ri, **common)
+        else:
+            prm = cases.stratified_open_channel(got=ri, dp_opt=3, fc=0.5, ug=1.0, vg=0.0, isdamp=1,
+                                                nzdamp=nz // 8, cdamp=0.01, **common)
+        st = cases.loglaw_noise(prm, seed=seed + q)
+        if not prm.isscalar:
+            st.pop("t")
+        out.append((prm, st))
+    return out
+
+
+def run_ensemble(args):
+    """Configuration 4: aggregate throughput of the ensemble (one captured
+    graph per two steps, members concurrent) against the same members
+    stepped one after another."""
+    from hercules_amd import Channel, Ensemble
+    members = []
+    cells = 0
+    for prm, st in ensemble_members():
+        ch = Channel(prm)
+        for k, v in st.items():
+            ch.set_field(k, v)
+        ch.project()
+        ch.init()
+        members.append(ch)
+        cells += prm.nx * prm.ny * prm.nz
+    # sequential single-instance launches
+    for ch in members:
+        ch.steps(args.warmup)
+        ch.synchronize()
+    t0 = time.perf_counter()
+    for ch in members:
+        ch.steps(args.steps)
+        ch.synchronize()
+    seq = time.perf_counter() - t0
+    ens = Ensemble(members, use_graph=True)
+    ens.steps(2 * ((args.warmup + 1) // 2))
+    t0 = time.perf_counter()
+    ens.steps(args.steps)
+    par = time.perf_counter() - t0
+    graph = ens.graph_used
+    finite = all(bool(np.isfinite(ch.get_field("u")).all()) for ch in members)
+    ens.close()
+    for ch in members:
+        ch.close()
+    value = cells * args.steps / par / 1e6
+    out = {
+        "metric": METRIC, "value": round(value, 2), "unit": "Mcell-RK3-steps/s", "n_gpus": 1,
+        "steps": args.steps, "warmup": args.warmup, "ms_per_step": round(par / args.steps * 1e3, 4),
+        "higher_is_better": True, "scaling": "weak", "vs_baseline": None, "dtype": "f64",
+        "data": "synthetic (seeded log-law + noise states; no checkpoint)",
+        "config": {"workload": "ensemble of 24 channels (8 x 128x128x129, 8 x 192x192x129, 8 x 256x128x193; neutral, stratified, Ekman-type), one HIP graph per two steps",
+                   "members": len(members), "cells": cells, "precision": "fp64"},
+        "sequential_value": round(cells * args.steps / seq / 1e6, 2),
+        "speedup_vs_sequential": round(seq / par, 3), "graph_used": graph, "state_finite": finite,
+    }
+    print(json.dumps(out))
+
+
 def cpu_baseline(prm, st, workload, steps):
     """Oracle on this host's cores: `steps` full steps (or solves) of the same
     workload, start-up excluded."""
@@ -141,7 +218,7 @@ def main():
     ap.add_argument("--gpus", type=int, default=1)
     ap.add_argument("--steps", type=int, default=50)
     ap.add_argument("--warmup", type=int, default=5)
-    ap.add_argument("--workload", default="fd", choices=["fd", "fd_t", "sp", "poisson", "copy"])
+    ap.add_argument("--workload", default="fd", choices=["fd", "fd_t", "sp", "poisson", "ensemble", "copy"])
     ap.add_argument("--cpu-steps", type=int, default=1, help="oracle steps for the CPU baseline")
     ap.add_argument("--no-cpu-baseline", action="store_true")
     ap.add_argument("--no-profile", action="store_true", help="skip the per-kernel event timing")
@@ -153,6 +230,8 @@ def main():
 
     if args.workload == "copy":
         return copy_calibration(args)
+    if args.workload == "ensemble":
+        return run_ensemble(args)
 
     from hercules_amd import Channel
 

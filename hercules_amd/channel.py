@@ -28,7 +28,8 @@ ABI_SYMBOLS = ("hc_last_error", "hc_create", "hc_destroy", "hc_set_field", "hc_g
                "hc_init", "hc_project", "hc_time_advancement_fd", "hc_time_advancement_sp", "hc_drive_steps",
                "hc_poisson_solver_fft",
                "hc_calculate_ubar", "hc_screen_div_error", "hc_wall_shear", "hc_get_scalar", "hc_stream",
-               "hc_field_device_ptr", "hc_synchronize", "hc_profile_enable", "hc_profile_read")
+               "hc_field_device_ptr", "hc_synchronize", "hc_profile_enable", "hc_profile_read",
+               "hc_ensemble_create", "hc_ensemble_steps", "hc_ensemble_graph_used", "hc_ensemble_destroy")
 KERNELS = {"rhs": 0, "yfft_fwd": 1, "column": 2, "yfft_inv": 3, "finish": 4, "rhs_t": 5,
            "yfft_t": 6, "column_t": 7, "finish_t": 8, "solve_xr2c": 9, "solve_yfft": 10,
            "solve_column": 11, "solve_xc2r": 12}
@@ -66,6 +67,10 @@ def load_library(path: str = LIB_PATH):
     L.hc_calculate_ubar.argtypes = [vp, dp]
     L.hc_screen_div_error.argtypes = [vp, dp, dp]
     L.hc_wall_shear.argtypes = [vp, dp, dp]
+    L.hc_ensemble_create.argtypes = [C.POINTER(vp), ip, ip, C.POINTER(vp)]
+    L.hc_ensemble_steps.argtypes = [vp, ip]
+    L.hc_ensemble_graph_used.argtypes = [vp]
+    L.hc_ensemble_destroy.argtypes = [vp]
     L.hc_get_scalar.argtypes = [vp, ip, dp]
     L.hc_stream.restype = vp
     L.hc_stream.argtypes = [vp]
@@ -211,3 +216,44 @@ class Channel:
             if n.value:
                 out[name] = (ms.value, n.value)
         return out
+
+
+class Ensemble:
+    """Independent channels advanced together (hc_ensemble_*): each member on
+    its own stream, optionally replayed from one captured HIP graph."""
+
+    def __init__(self, members, use_graph=True):
+        self.L = load_library()
+        self.members = list(members)
+        arr = (C.c_void_p * len(self.members))(*[m.h.value for m in self.members])
+        h = C.c_void_p()
+        rc = self.L.hc_ensemble_create(arr, len(self.members), int(bool(use_graph)), C.byref(h))
+        if rc:
+            raise HerculesError(rc, self.L.hc_last_error().decode())
+        self.h = h
+
+    def steps(self, n):
+        rc = self.L.hc_ensemble_steps(self.h, int(n))
+        if rc:
+            raise HerculesError(rc, self.L.hc_last_error().decode())
+
+    @property
+    def graph_used(self):
+        return bool(self.L.hc_ensemble_graph_used(self.h))
+
+    def close(self):
+        if getattr(self, "h", None):
+            self.L.hc_ensemble_destroy(self.h)
+            self.h = None
+
+    def __enter__(self):
+        return self
+
+    def __exit__(self, *a):
+        self.close()
+
+    def __del__(self):
+        try:
+            self.close()
+        except Exception:
+            pass

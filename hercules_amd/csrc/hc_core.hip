@@ -2109,3 +2109,146 @@ extern "C" int hc_project(hc_ctx *c)
     HIPCHK(hipStreamSynchronize(c->stream));
     return 0;
 }
+
+// ---- ensembles --------------------------------------------------------------
+struct hc_ensemble {
+    std::vector<hc_ctx *> m;
+    bool use_graph = false, graph_used = false;
+    hipStream_t master = nullptr;
+    hipEvent_t fork = nullptr;
+    std::vector<hipEvent_t> join;
+    hipGraph_t graph = nullptr;
+    hipGraphExec_t exec = nullptr;
+};
+
+// one pass of the drive loop body for one member (hc_drive_steps' body)
+static int member_step(hc_ctx *c)
+{
+    if (int e = one_step(c)) return e;
+    if (c->prm.dp_opt == 2)
+        if (int e = ubar_launch(c)) return e;
+    return 0;
+}
+
+extern "C" int hc_ensemble_create(hc_ctx *const *members, int n, int use_graph, hc_ensemble **out)
+{
+    if (!members || n < 1 || !out) return set_err(HC_ERR_ARG, "null argument or empty ensemble");
+    *out = nullptr;
+    hc_ensemble *e = new hc_ensemble();
+    for (int q = 0; q < n; ++q) {
+        if (!members[q]) { delete e; return set_err(HC_ERR_ARG, "null member %d", q); }
+        e->m.push_back(members[q]);
+    }
+    e->use_graph = use_graph != 0;
+    hipError_t he = hipStreamCreateWithFlags(&e->master, hipStreamNonBlocking);
+    if (he == hipSuccess) he = hipEventCreateWithFlags(&e->fork, hipEventDisableTiming);
+    e->join.assign(n, nullptr);
+    for (int q = 0; q < n && he == hipSuccess; ++q) he = hipEventCreateWithFlags(&e->join[q], hipEventDisableTiming);
+    if (he != hipSuccess) {
+        hc_ensemble_destroy(e);
+        return set_err(HC_ERR_HIP, "ensemble streams/events: %s", hipGetErrorString(he));
+    }
+    *out = e;
+    return 0;
+}
+
+static bool ensemble_graphable(const hc_ensemble *e)
+{
+    for (const hc_ctx *c : e->m)
+        if (c->prm.is_ri_var == 1 || c->prof) return false;
+    return true;
+}
+
+// Capture two drive steps of every member into one graph: the master stream
+// forks to the member streams and joins them back.  The host-side counters
+// that one_step advances are restored afterwards; replays advance them.
+static int ensemble_capture(hc_ensemble *e)
+{
+    struct Saved { int stps, fcur, scur; double time_sim, got; };
+    std::vector<Saved> sv;
+    for (hc_ctx *c : e->m) {
+        if (c->prm.cds == 3 && c->spec_dirty)
+            if (int er = spec_init(c)) return er;
+        HIPCHK(hipStreamSynchronize(c->stream));
+        sv.push_back({c->stps, c->fcur, c->scur, c->time_sim, c->got});
+    }
+    HIPCHK(hipStreamBeginCapture(e->master, hipStreamCaptureModeThreadLocal));
+    int err = 0;
+    hipError_t he = hipEventRecord(e->fork, e->master);
+    for (size_t q = 0; q < e->m.size() && !err && he == hipSuccess; ++q) {
+        hc_ctx *c = e->m[q];
+        he = hipStreamWaitEvent(c->stream, e->fork, 0);
+        for (int s = 0; s < 2 && !err && he == hipSuccess; ++s) err = member_step(c);
+        if (!err && he == hipSuccess) he = hipEventRecord(e->join[q], c->stream);
+        if (!err && he == hipSuccess) he = hipStreamWaitEvent(e->master, e->join[q], 0);
+    }
+    hipGraph_t g = nullptr;
+    hipError_t ce = hipStreamEndCapture(e->master, &g);
+    for (size_t q = 0; q < e->m.size(); ++q) {
+        hc_ctx *c = e->m[q];
+        c->stps = sv[q].stps; c->fcur = sv[q].fcur; c->scur = sv[q].scur;
+        c->time_sim = sv[q].time_sim; c->got = sv[q].got;
+    }
+    if (err) { if (g) (void)hipGraphDestroy(g); return err; }
+    if (he != hipSuccess || ce != hipSuccess || !g) {
+        if (g) (void)hipGraphDestroy(g);
+        (void)hipGetLastError();
+        return set_err(HC_ERR_HIP, "ensemble capture: %s", hipGetErrorString(he != hipSuccess ? he : ce));
+    }
+    hipGraphExec_t x = nullptr;
+    hipError_t ie = hipGraphInstantiate(&x, g, nullptr, nullptr, 0);
+    if (ie != hipSuccess) {
+        (void)hipGraphDestroy(g);
+        return set_err(HC_ERR_HIP, "ensemble graph instantiate: %s", hipGetErrorString(ie));
+    }
+    e->graph = g;
+    e->exec = x;
+    return 0;
+}
+
+extern "C" int hc_ensemble_steps(hc_ensemble *e, int nsteps)
+{
+    if (!e) return set_err(HC_ERR_ARG, "null ensemble");
+    if (nsteps < 0) return set_err(HC_ERR_ARG, "nsteps < 0");
+    int done = 0;
+    e->graph_used = false;
+    if (e->use_graph && ensemble_graphable(e) && nsteps >= 2) {
+        if (!e->exec)
+            if (int er = ensemble_capture(e)) return er;
+        for (hc_ctx *c : e->m)
+            if (c->prm.cds == 3 && c->spec_dirty) return set_err(HC_ERR_ARG, "member changed after capture");
+        for (; done + 2 <= nsteps; done += 2) {
+            HIPCHK(hipGraphLaunch(e->exec, e->master));
+            for (hc_ctx *c : e->m) {
+                c->stps += 2;
+                c->time_sim += 2.0 * c->prm.dt;
+            }
+        }
+        HIPCHK(hipStreamSynchronize(e->master));
+        e->graph_used = true;
+    }
+    // remaining steps (or all of them): direct launches, member after member,
+    // each on its own stream so that the members still overlap
+    for (; done < nsteps; ++done)
+        for (hc_ctx *c : e->m)
+            if (int er = member_step(c)) return er;
+    for (hc_ctx *c : e->m) HIPCHK(hipStreamSynchronize(c->stream));
+    HIPCHK(hipGetLastError());
+    return 0;
+}
+
+extern "C" int hc_ensemble_graph_used(const hc_ensemble *e) { return e && e->graph_used ? 1 : 0; }
+
+extern "C" int hc_ensemble_destroy(hc_ensemble *e)
+{
+    if (!e) return 0;
+    if (e->master) (void)hipStreamSynchronize(e->master);
+    if (e->exec) (void)hipGraphExecDestroy(e->exec);
+    if (e->graph) (void)hipGraphDestroy(e->graph);
+    for (hipEvent_t v : e->join)
+        if (v) (void)hipEventDestroy(v);
+    if (e->fork) (void)hipEventDestroy(e->fork);
+    if (e->master) (void)hipStreamDestroy(e->master);
+    delete e;
+    return 0;
+}

@@ -184,6 +184,24 @@ enum hc_kernel {
 int hc_profile_enable(hc_ctx *ctx, int on);
 int hc_profile_read(hc_ctx *ctx, int kernel, double *total_ms, long *launches);
 
+/* Ensembles of independent channels (no counterpart in the reference, which
+ * runs one channel per job).  The members are contexts made by hc_create and
+ * prepared by hc_init; they may differ in grid, path (cds) and physics and
+ * stay owned by the caller.  hc_ensemble_steps advances every member by
+ * nsteps passes of the drive loop body (as hc_drive_steps), each member on
+ * its own stream so that members run concurrently.  With use_graph = 1 the
+ * members' launches for two steps are captured once into one HIP graph
+ * (fork/join over the member streams) and replayed; members with
+ * is_ri_var = 1 or kernel timing enabled make the ensemble fall back to
+ * direct launches.  Each member's result is identical to stepping it alone.
+ * Members must not be used on their own while the ensemble exists. */
+typedef struct hc_ensemble hc_ensemble;
+int hc_ensemble_create(hc_ctx *const *members, int n, int use_graph, hc_ensemble **out);
+int hc_ensemble_steps(hc_ensemble *ens, int nsteps);
+/* 1 when the last hc_ensemble_steps replayed the captured graph */
+int hc_ensemble_graph_used(const hc_ensemble *ens);
+int hc_ensemble_destroy(hc_ensemble *ens);
+
 #ifdef __cplusplus
 }
 #endif
