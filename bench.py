@@ -9,6 +9,13 @@ on a seeded synthetic state (Reichardt log-law mean + 10% Gaussian noise).
 ``poisson`` times the standalone pressure Poisson solve (seeded Gaussian
 right-hand side, plane means removed).
 
+``--precision mixed`` times the mixed variant (FP32 fields, stencils and
+FFTs; FP64 low-wavenumber column solves and bulk-flow update) instead.  For
+fd, fd_t and poisson the other precision is also run on the same state for
+the same number of steps: the line then carries the FP64/mixed speed ratio
+and the mixed variant's relative L2 error, wall-shear error and divergence
+against FP64 (``mixed`` object), unless ``--no-compare``.
+
 Prints one JSON line.  ``value`` is whole-job Mcell-RK3-steps/s
 (nx*ny*nz*steps / seconds / 1e6) with the state resident in HBM; the
 ``roofline`` object is computed for the kernel with the largest share of
@@ -119,8 +126,13 @@ def run_ensemble(args):
     from hercules_amd import Channel, Ensemble
     members = []
     cells = 0
+    mixed = args.precision == "mixed"
+    nmixed = 0
     for prm, st in ensemble_members():
-        ch = Channel(prm)
+        # the mixed variant has no Rayleigh damping: sponge members stay FP64
+        pr = 1 if (mixed and prm.isdamp == 0) else 0
+        nmixed += pr
+        ch = Channel(prm.replace(precision=pr))
         for k, v in st.items():
             ch.set_field(k, v)
         ch.project()
@@ -150,10 +162,12 @@ def run_ensemble(args):
     out = {
         "metric": METRIC, "value": round(value, 2), "unit": "Mcell-RK3-steps/s", "n_gpus": 1,
         "steps": args.steps, "warmup": args.warmup, "ms_per_step": round(par / args.steps * 1e3, 4),
-        "higher_is_better": True, "scaling": "weak", "vs_baseline": None, "dtype": "f64",
+        "higher_is_better": True, "scaling": "weak", "vs_baseline": None,
+        "dtype": "f32+f64" if mixed else "f64",
         "data": "synthetic (seeded log-law + noise states; no checkpoint)",
         "config": {"workload": "ensemble of 24 channels (8 x 128x128x129, 8 x 192x192x129, 8 x 256x128x193; neutral, stratified, Ekman-type), one HIP graph per two steps",
-                   "members": len(members), "cells": cells, "precision": "fp64"},
+                   "members": len(members), "cells": cells, "precision": args.precision,
+                   "mixed_members": nmixed},
         "sequential_value": round(cells * args.steps / seq / 1e6, 2),
         "speedup_vs_sequential": round(seq / par, 3), "graph_used": graph, "state_finite": finite,
     }
@@ -213,6 +227,114 @@ def copy_calibration(args):
                       "ms": round(ms, 4), "GBs": round(gbs, 1)}))
 
 
+class DeviceArray:
+    """An FP64 array in device memory (hipMalloc through ctypes)."""
+
+    _hip = None
+
+    def __init__(self, host):
+        import ctypes as C
+        if DeviceArray._hip is None:
+            for name in ("libamdhip64.so", "/opt/rocm/lib/libamdhip64.so"):
+                try:
+                    DeviceArray._hip = C.CDLL(name)
+                    break
+                except OSError:
+                    continue
+        H = DeviceArray._hip
+        self.shape, self.nbytes = host.shape, host.nbytes
+        p = C.c_void_p()
+        if H.hipMalloc(C.byref(p), C.c_size_t(self.nbytes)) != 0:
+            raise RuntimeError("hipMalloc failed")
+        self.ptr = p.value
+        a = np.ascontiguousarray(host, dtype=np.float64)
+        if H.hipMemcpy(C.c_void_p(self.ptr), C.c_void_p(a.ctypes.data), C.c_size_t(self.nbytes), 1) != 0:
+            raise RuntimeError("hipMemcpy to device failed")
+
+    def get(self):
+        import ctypes as C
+        out = np.empty(self.shape)
+        if DeviceArray._hip.hipMemcpy(C.c_void_p(out.ctypes.data), C.c_void_p(self.ptr),
+                                      C.c_size_t(self.nbytes), 2) != 0:
+            raise RuntimeError("hipMemcpy to host failed")
+        return out
+
+    def __del__(self):
+        import ctypes as C
+        if getattr(self, "ptr", None) and DeviceArray._hip is not None:
+            DeviceArray._hip.hipFree(C.c_void_p(self.ptr))
+            self.ptr = None
+
+
+def timed_run(prm, st, args, precision, profile):
+    """Warm up, then time args.steps steps (or solves) of one channel in the
+    given precision (0 FP64, 1 mixed), state resident on the device."""
+    from hercules_amd import Channel
+    ch = Channel(prm.replace(precision=precision))
+    if args.workload == "poisson":
+        rhs = np.ascontiguousarray(st["rhs"])
+        # device-resident FP64 source and result: two device arrays of the
+        # HIP runtime; the solver reads and writes them through the C ABI
+        src, dst = DeviceArray(rhs), DeviceArray(np.zeros_like(rhs))
+        step = lambda: ch.poisson_solver_fft_device("p", 1.0, src.ptr, dst.ptr)
+    else:
+        for k, v in st.items():
+            ch.set_field(k, v)
+        ch.init()
+        step = lambda: ch.steps(1)
+    for _ in range(args.warmup):
+        step()
+    ch.synchronize()
+    if profile:
+        ch.profile(True)
+    t0 = time.perf_counter()
+    for _ in range(args.steps):
+        step()
+    ch.synchronize()
+    elapsed = time.perf_counter() - t0
+    prof = ch.profile_read() if profile else {}
+    ch.profile(False)
+    res = {"ch": ch, "elapsed": elapsed, "prof": prof, "finite": True, "div": None}
+    if args.workload == "poisson":
+        res["phi"] = dst.get()
+        res["rhs"] = rhs
+        res["bufs"] = (src, dst)
+    else:
+        names = ["u", "v", "w", "p"] + (["t"] if prm.isscalar else [])
+        res["fields"] = {k: ch.get_field(k) for k in names}
+        res["finite"] = bool(all(np.isfinite(f).all() for f in res["fields"].values()))
+        try:
+            res["div"] = ch.screen_div_error()
+        except Exception as e:  # NaN in the state
+            res["div"] = str(e)
+        res["tau"] = ch.wall_shear()
+    return res
+
+
+def compare_precisions(prm, args, m, d, n):
+    """Mixed run m against FP64 run d after the same steps from the same state."""
+    def rel(a, b):
+        return float(np.linalg.norm(a - b) / max(np.linalg.norm(b), 1e-300))
+    ms_m = m["elapsed"] / args.steps * 1e3
+    ms_d = d["elapsed"] / args.steps * 1e3
+    out = {"ms_per_step_mixed": round(ms_m, 4), "ms_per_step_fp64": round(ms_d, 4),
+           "mixed_speedup_vs_fp64": round(ms_d / ms_m, 3)}
+    if args.workload == "poisson":
+        out["value_mixed"] = round(args.steps / m["elapsed"], 2)
+        out["unit"] = "Poisson solves/s"
+        out["rel_l2_phi"] = rel(m["phi"], d["phi"])
+    else:
+        out["value_mixed"] = round(n * args.steps / m["elapsed"] / 1e6, 2)
+        out["unit"] = "Mcell-RK3-steps/s"
+        out["after_steps"] = args.warmup + args.steps
+        out["rel_l2"] = {k: rel(m["fields"][k], d["fields"][k]) for k in d["fields"]}
+        tb_m, tb_d = m["tau"][0], d["tau"][0]
+        out["wall_shear_bottom"] = {"fp64": tb_d, "mixed": tb_m, "rel_err": abs(tb_m - tb_d) / abs(tb_d)}
+        out["div_fp64"] = d["div"]
+        out["div_mixed"] = m["div"]
+    return out
+
+
 def main():
     ap = argparse.ArgumentParser(description=__doc__.split("\n")[0])
     ap.add_argument("--gpus", type=int, default=1)
@@ -222,6 +344,10 @@ def main():
     ap.add_argument("--cpu-steps", type=int, default=1, help="oracle steps for the CPU baseline")
     ap.add_argument("--no-cpu-baseline", action="store_true")
     ap.add_argument("--no-profile", action="store_true", help="skip the per-kernel event timing")
+    ap.add_argument("--precision", default="fp64", choices=["fp64", "mixed"],
+                    help="fp64, or FP32 fields/stencils/FFTs with FP64 low-wavenumber solves")
+    ap.add_argument("--no-compare", action="store_true",
+                    help="skip the run in the other precision (speed ratio and mixed-vs-FP64 errors)")
     args = ap.parse_args()
     if args.gpus != 1:
         print(f"bench.py: this build runs on one GPU; --gpus {args.gpus} is not supported "
@@ -233,52 +359,34 @@ def main():
     if args.workload == "ensemble":
         return run_ensemble(args)
 
-    from hercules_amd import Channel
-
     prm, st = build_case(args.workload)
     n = prm.nx * prm.ny * prm.nz
-    ch = Channel(prm)
     with_t = args.workload == "fd_t"
-    if args.workload == "poisson":
-        import ctypes as C
-        rhs = np.ascontiguousarray(st["rhs"])
-        # device-resident source and result: reuse two field slots of the context
-        ch.set_field("u", rhs)
-        src_ptr, dst_ptr = ch.device_ptr("u"), ch.device_ptr("v")
-        step = lambda: ch.poisson_solver_fft_device("p", 1.0, src_ptr, dst_ptr)
-    else:
-        for k, v in st.items():
-            ch.set_field(k, v)
-        ch.project()  # seeded noise made discretely divergence-free
-        for k in ("u", "v", "w"):
-            st[k] = ch.get_field(k)
-        ch.init()
-        step = lambda: ch.steps(1)
-
-    for _ in range(args.warmup):
-        step()
-    ch.synchronize()
-    if not args.no_profile:
-        ch.profile(True)
-    t0 = time.perf_counter()
-    for _ in range(args.steps):
-        step()
-    ch.synchronize()
-    elapsed = time.perf_counter() - t0
-    prof = ch.profile_read() if not args.no_profile else {}
-    ch.profile(False)
-    ms_per_step = elapsed / args.steps * 1e3
-
-    # sanity of the final state
-    finite = True
-    div = None
+    mixed_main = args.precision == "mixed"
+    if mixed_main and args.workload == "sp":
+        print("bench.py: the mixed precision variant covers cds=1,2 (fd, fd_t, poisson)", file=sys.stderr)
+        sys.exit(2)
     if args.workload != "poisson":
-        u = ch.get_field("u")
-        finite = bool(np.isfinite(u).all())
-        try:
-            div = ch.screen_div_error()
-        except Exception as e:  # NaN in the state
-            div = str(e)
+        from hercules_amd import Channel
+        ch0 = Channel(prm)
+        for k, v in st.items():
+            ch0.set_field(k, v)
+        ch0.project()  # seeded noise made discretely divergence-free (FP64)
+        for k in ("u", "v", "w"):
+            st[k] = ch0.get_field(k)
+        ch0.close()
+    run = timed_run(prm, st, args, 1 if mixed_main else 0, profile=not args.no_profile)
+    ch, elapsed, prof = run["ch"], run["elapsed"], run["prof"]
+    ms_per_step = elapsed / args.steps * 1e3
+    finite, div = run["finite"], run["div"]
+    # the other precision on the same state and step count: its speed and
+    # the mixed variant's departure from FP64
+    other = None
+    if args.workload in ("fd", "fd_t", "poisson") and not args.no_compare:
+        alt = timed_run(prm, st, args, 0 if mixed_main else 1, profile=False)
+        m, d = (run, alt) if mixed_main else (alt, run)
+        other = compare_precisions(prm, args, m, d, n)
+        alt["ch"].close()
 
     if args.workload == "poisson":
         value = args.steps / elapsed
@@ -300,7 +408,7 @@ def main():
             passes = sum(rhs_t_passes(s) for s in range(1, nsub + 1)) / nsub
         else:
             passes = algo_passes(name, 1, with_t)
-        bytes_per_launch = passes * n * 8
+        bytes_per_launch = passes * n * (4 if mixed_main else 8)
         avg_ms = ms / launches
         kernels[name] = {"avg_ms": avg_ms, "launches": launches, "share": 0.0,
                          "algo_bytes": bytes_per_launch,
@@ -311,7 +419,8 @@ def main():
     # measured HBM bytes per launch (rocprofv3 FETCH_SIZE / WRITE_SIZE runs,
     # calibrated against a copy kernel), committed under profiles/
     traffic = {}
-    tpath = os.path.join(ROOT, "profiles", f"hbm_traffic_{args.workload}_{prm.nx}x{prm.ny}x{prm.nz}.json")
+    tag = args.workload + ("_mixed" if mixed_main else "")
+    tpath = os.path.join(ROOT, "profiles", f"hbm_traffic_{tag}_{prm.nx}x{prm.ny}x{prm.nz}.json")
     if os.path.exists(tpath):
         traffic = json.load(open(tpath)).get("bytes_per_launch", {})
     roofline = None
@@ -326,7 +435,7 @@ def main():
     # without T, 1608 with T: 159 / 201 FP64 passes per RK3 step)
     step_passes = {"fd": 159, "fd_t": 201, "sp": 3 * sum(SPECTRAL_PASSES.values()),
                    "poisson": 10}[args.workload]
-    step_gbs = step_passes * n * 8 / (ms_per_step * 1e-3) / 1e9
+    step_gbs = step_passes * n * (4 if mixed_main else 8) / (ms_per_step * 1e-3) / 1e9
     step_roofline = {"passes_per_step": step_passes, "achieved": round(step_gbs, 1),
                      "peak": HBM_PEAK_GBS, "unit": "GB/s", "frac": round(step_gbs / HBM_PEAK_GBS, 4)}
 
@@ -346,15 +455,17 @@ def main():
     out = {
         "metric": METRIC, "value": round(value, 2), "unit": unit, "n_gpus": 1, "steps": args.steps,
         "warmup": args.warmup, "ms_per_step": round(ms_per_step, 4), "higher_is_better": True,
-        "scaling": "weak", "vs_baseline": None, "dtype": "f64",
+        "scaling": "weak", "vs_baseline": None, "dtype": "f32+f64" if mixed_main else "f64",
         "data": "synthetic (seeded log-law + noise state; no checkpoint)",
         "config": {"workload": workload_name, "nx": prm.nx, "ny": prm.ny, "nz": prm.nz,
-                   "cells": n, "precision": "fp64"},
+                   "cells": n, "precision": args.precision},
         "roofline": roofline, "step_roofline": step_roofline, "cpu_baseline": cpu,
         "kernels": {k: {kk: (round(vv, 4) if isinstance(vv, float) else vv) for kk, vv in v.items()}
                     for k, v in kernels.items()},
         "state_finite": finite, "div": div,
     }
+    if other:
+        out["mixed" if not mixed_main else "fp64"] = other
     if cpu and "value" in cpu and cpu["value"]:
         out["gpu_over_cpu"] = round(value / cpu["value"], 1)
     ch.close()

@@ -16,11 +16,13 @@
 // plus, with temperature, the same chain for T.
 #include <hip/hip_runtime.h>
 
+#include <algorithm>
 #include <cmath>
 #include <cstdarg>
 #include <cstdio>
 #include <cstring>
 #include <string>
+#include <type_traits>
 #include <vector>
 
 #include "../../include/hercules_amd.h"
@@ -67,14 +69,15 @@ constexpr int rhs_ty()
 template <int NX>
 constexpr int rhs_threads() { return NX * rhs_ty<NX>() <= 512 ? NX * rhs_ty<NX>() : NX * rhs_ty<NX>() / 2; }
 
-template <int CDS, int NX>
-__global__ __launch_bounds__(rhs_threads<NX>()) void k_rhs_mom(StepArgs A)
+template <int CDS, int NX, class R>
+__global__ __launch_bounds__(rhs_threads<NX>()) void k_rhs_mom(StepArgsT<R> A)
 {
+    using C = cplx<R>;
     constexpr int TY = rhs_ty<NX>();
     constexpr int T = rhs_threads<NX>(), PP = NX * TY / T, NL = NX / 2;
     using Tl = RhsTile<NX, TY>;
     static_assert(Tl::NROWS * NX >= 3 * TY * NX, "FFT rows must fit in the staging area");
-    __shared__ double sm[Tl::NROWS * NX];
+    __shared__ __align__(16) R sm[Tl::NROWS * NX];
     const int tid = threadIdx.x;
     const int nty = A.ny / TY;
     const int logical = xcd_remap(blockIdx.x, nty * A.nz);
@@ -83,20 +86,20 @@ __global__ __launch_bounds__(rhs_threads<NX>()) void k_rhs_mom(StepArgs A)
         int r = q / NX, i = q - r * NX;
         sm[q] = rhs_stage_value<NX, TY>(A, r, i, j0, k);
     }
-    const double dpx = A.dp_opt == 2 ? A.scal[0] : A.dpx;
-    PointVals pv[PP];
+    const R dpx = A.dp_opt == 2 ? R(A.scal[0]) : A.dpx;
+    PointVals<R> pv[PP];
 #pragma unroll
     for (int pp = 0; pp < PP; ++pp) {
         const int pt = tid + pp * T;
         pv[pp].load(A, pt % NX, j0 + pt / NX, k);
     }
     __syncthreads();
-    double qs[PP][3];
+    R qs[PP][3];
 #pragma unroll
     for (int pp = 0; pp < PP; ++pp) {
         const int pt = tid + pp * T, i = pt % NX, jj = pt / NX, j = j0 + jj;
-        Ld<NX, TY> q(A, sm, i, jj, j, k, pv[pp]);
-        double f, g, h;
+        Ld<NX, TY, R> q(A, sm, i, jj, j, k, pv[pp]);
+        R f, g, h;
         momentum_point<CDS>(q, dpx, f, g, h, qs[pp][0], qs[pp][1], qs[pp][2]);
         const size_t c = ((size_t)k * A.ny + j) * NX + i;
         A.ff[c] = f;
@@ -112,9 +115,9 @@ __global__ __launch_bounds__(rhs_threads<NX>()) void k_rhs_mom(StepArgs A)
         sm[(2 * TY + jj) * NX + i] = qs[pp][2];
     }
     __syncthreads();
-    double2 *sq_ = reinterpret_cast<double2 *>(sm);
+    C *sq_ = reinterpret_cast<C *>(sm);
     lds_fft<NL, 3 * TY, NL, 1, T, -1>(sq_, A.tw_nl, tid);
-    double2 *outs[3] = {A.s0, A.s1, A.s2};
+    C *outs[3] = {A.s0, A.s1, A.s2};
     for (int p = tid; p < 3 * TY * NL; p += T) {
         int row = p / NL, l = p - row * NL, fi = row / TY, r = row - fi * TY;
         outs[fi][((size_t)k * A.ny + j0 + r) * NL + l] = r2c_post<NL>(sq_ + row * NL, l, A.tw_nx, A.norm);
@@ -131,18 +134,19 @@ constexpr int rhs2_ty() { return NX == 16 ? 16 : NX == 32 ? 32 : NX == 64 ? 16 :
 template <int NX>
 constexpr int rhs2_threads() { return NX * rhs2_ty<NX>(); }
 
-template <int CDS, int NX>
-__global__ __launch_bounds__(rhs2_threads<NX>()) void k_rhs_mom2(StepArgs A)
+template <int CDS, int NX, class R>
+__global__ __launch_bounds__(rhs2_threads<NX>()) void k_rhs_mom2(StepArgsT<R> A)
 {
+    using C = cplx<R>;
     constexpr int TY = rhs2_ty<NX>();
     constexpr int T = rhs2_threads<NX>(), NL = NX / 2;
     using Tl = Tile2<NX, TY>;
     static_assert(Tl::NRAW >= 3 * TY, "FFT rows must fit in the staging area");
     constexpr int NFWV = (TY + 3) * NX, PF = (NFWV + T - 1) / T;
-    __shared__ double sm[Tl::NROWS * NX];
-    __shared__ double2 tws[NX];  // twiddles of the x transform: half-length, then R2C post
+    __shared__ __align__(16) R sm[Tl::NROWS * NX];
+    __shared__ C tws[NX];  // twiddles of the x transform: half-length, then R2C post
     const int tid = threadIdx.x;
-    double2 tw_a = czero(), tw_b = czero();
+    C tw_a = czero_t<C>(), tw_b = czero_t<C>();
     if (tid < NL) { tw_a = A.tw_nl[tid]; tw_b = A.tw_nx[tid]; }
     const int nty = A.ny / TY;
     // tile order: groups of G consecutive y tiles, levels fastest-changing
@@ -156,19 +160,19 @@ __global__ __launch_bounds__(rhs2_threads<NX>()) void k_rhs_mom2(StepArgs A)
     const bool top = k == A.nz - 1;
     // staging: every thread issues all of its loads before the first use
     constexpr int NST = (Tl::NRAW * NX + T - 1) / T;
-    double stv[NST];
+    R stv[NST];
 #pragma unroll
     for (int m = 0; m < NST; ++m) {
         const int q = tid + m * T;
-        stv[m] = 0.0;
+        stv[m] = R(0.0);
         if ((Tl::NRAW * NX) % T == 0 || q < Tl::NRAW * NX) {
             const int r = q / NX, x = q - r * NX;
-            const double *f = r < Tl::OV ? A.u : r < Tl::OW ? A.v : r < Tl::OP ? A.w : A.p;
+            const R *f = r < Tl::OV ? A.u : r < Tl::OW ? A.v : r < Tl::OP ? A.w : A.p;
             const int jr = r < Tl::OP ? j0 - 3 + r % Tl::RU : j0 - 1 + (r - Tl::OP);
             stv[m] = f[((unsigned)k * (unsigned)A.ny + (unsigned)wrapi(jr, A.ny)) * (unsigned)NX + (unsigned)x];
         }
     }
-    PointVals2 pv;
+    PointVals2<R> pv;
     pv.load(A, i, j, k);
 #pragma unroll
     for (int m = 0; m < NST; ++m) {
@@ -176,28 +180,28 @@ __global__ __launch_bounds__(rhs2_threads<NX>()) void k_rhs_mom2(StepArgs A)
         if ((Tl::NRAW * NX) % T == 0 || q < Tl::NRAW * NX) sm[q] = stv[m];
     }
     // v(k+1) under the why*vhz rows, loaded with the staging
-    double vkp[PF];
+    R vkp[PF];
 #pragma unroll
     for (int m = 0; m < PF; ++m) {
         const int e = tid + m * T;
-        vkp[m] = 0.0;
+        vkp[m] = R(0.0);
         if ((NFWV % T == 0 || e < NFWV) && !top) {
             const int r = e / NX, x = e - r * NX;
             vkp[m] = A.v[((size_t)(k + 1) * A.ny + wrapi(j0 - 2 + r, A.ny)) * NX + x];
         }
     }
-    const double dpx = A.dp_opt == 2 ? A.scal[0] : A.dpx;
-    const double um = A.u_mrf;
-    const double lt = top ? 0.0 : A.lev[LV_LT * A.nz + k];
+    const R dpx = A.dp_opt == 2 ? R(A.scal[0]) : A.dpx;
+    const R um = A.u_mrf;
+    const R lt = top ? R(0.0) : A.lev[LV_LT * A.nz + k];
     if (tid < NL) { tws[tid] = tw_a; tws[NL + tid] = tw_b; }
     __syncthreads();
     // ---- phase 1: interpolants and flux products of level k
     {
-        const double *ru = sm + (Tl::OU + jj + 3) * NX, *rw = sm + (Tl::OW + jj + 3) * NX;
-        const double uhx = ixm<CDS, NX>(ru, i);
-        const double whx = top ? 0.0 : ixp<CDS, NX>(rw, i);
-        const double uc = ru[i];
-        const double uhz = top ? (A.ochannel == 0 ? -um : uc) : pv.u_kp * lt + uc * (1.0 - lt);
+        const R *ru = sm + (Tl::OU + jj + 3) * NX, *rw = sm + (Tl::OW + jj + 3) * NX;
+        const R uhx = ixm<CDS, NX>(ru, i);
+        const R whx = top ? R(0.0) : ixp<CDS, NX>(rw, i);
+        const R uc = ru[i];
+        const R uhz = top ? (A.ochannel == 0 ? -um : uc) : pv.u_kp * lt + uc * (R(1.0) - lt);
         sm[(Tl::OUHX + jj) * NX + i] = uhx;
         sm[(Tl::OWHX + jj) * NX + i] = whx;
         sm[(Tl::OFUW + jj) * NX + i] = uhz * whx;
@@ -213,8 +217,8 @@ __global__ __launch_bounds__(rhs2_threads<NX>()) void k_rhs_mom2(StepArgs A)
     }
     for (int e = tid; e < (TY + 3) * NX; e += T) {  // uhy*vhx, rows j0-2+r
         const int r = e / NX, x = e - r * NX;
-        const double uhy = iyp<CDS, NX>(sm + (Tl::OU + r + 1) * NX + x);
-        const double vhx = ixp<CDS, NX>(sm + (Tl::OV + r + 1) * NX, x);
+        const R uhy = iyp<CDS, NX>(sm + (Tl::OU + r + 1) * NX + x);
+        const R vhx = ixp<CDS, NX>(sm + (Tl::OV + r + 1) * NX, x);
         sm[(Tl::OFUV + r) * NX + x] = uhy * vhx;
     }
 #pragma unroll
@@ -222,17 +226,17 @@ __global__ __launch_bounds__(rhs2_threads<NX>()) void k_rhs_mom2(StepArgs A)
         const int e = tid + m * T;
         if (NFWV % T == 0 || e < NFWV) {
             const int r = e / NX, x = e - r * NX;
-            const double vc = sm[(Tl::OV + r + 1) * NX + x];
-            const double why = top ? 0.0 : iyp<CDS, NX>(sm + (Tl::OW + r + 1) * NX + x);
-            const double vhz = top ? (A.ochannel == 0 ? 0.0 : vc) : vkp[m] * lt + vc * (1.0 - lt);
+            const R vc = sm[(Tl::OV + r + 1) * NX + x];
+            const R why = top ? R(0.0) : iyp<CDS, NX>(sm + (Tl::OW + r + 1) * NX + x);
+            const R vhz = top ? (A.ochannel == 0 ? R(0.0) : vc) : vkp[m] * lt + vc * (R(1.0) - lt);
             sm[(Tl::OFWV + r) * NX + x] = vhz * why;
         }
     }
     __syncthreads();
     // ---- phase 2
-    double qu, qv, qw;
+    R qu, qv, qw;
     {
-        double f, g, h;
+        R f, g, h;
         momentum_tile<CDS, NX, TY>(A, sm, i, jj, k, pv, dpx, f, g, h, qu, qv, qw);
         const size_t c = ((size_t)k * A.ny + j) * NX + i;
         A.ff[c] = f;
@@ -244,11 +248,11 @@ __global__ __launch_bounds__(rhs2_threads<NX>()) void k_rhs_mom2(StepArgs A)
     sm[(1 * TY + jj) * NX + i] = qv;
     sm[(2 * TY + jj) * NX + i] = qw;
     __syncthreads();
-    double2 *sq_ = reinterpret_cast<double2 *>(sm);
+    C *sq_ = reinterpret_cast<C *>(sm);
     lds_fft<NL, 3 * TY, NL, 1, T, -1>(sq_, tws, tid);
     for (int p = tid; p < 3 * TY * NL; p += T) {
         int row = p / NL, l = p - row * NL, fi = row / TY, r = row - fi * TY;
-        double2 *o = fi == 0 ? A.s0 : fi == 1 ? A.s1 : A.s2;
+        C *o = fi == 0 ? A.s0 : fi == 1 ? A.s1 : A.s2;
         o[((size_t)k * A.ny + j0 + r) * NL + l] = r2c_post<NL>(sq_ + row * NL, l, tws + NL, A.norm);
     }
 }
@@ -326,19 +330,21 @@ __global__ __launch_bounds__(256) void k_damp_apply(double *f, const double *dmp
 
 // Temperature RHS + x R2C of qt.  Acc: Pt, or PtD in a damped substep.
 template <int CDS, int NX, class Acc>
-__global__ __launch_bounds__(kRhsThreads) void k_rhs_t(StepArgs A)
+__global__ __launch_bounds__(kRhsThreads) void k_rhs_t(StepArgsT<typename Acc::Real> A)
 {
+    using R = typename Acc::Real;
+    using C = cplx<R>;
     constexpr int ROWS = rows_per_block<NX>();
     constexpr int NL = NX / 2;
-    __shared__ double2 sq_[ROWS * NL];
-    double *sqd = reinterpret_cast<double *>(sq_);
+    __shared__ C sq_[ROWS * NL];
+    R *sqd = reinterpret_cast<R *>(sq_);
     const int k = blockIdx.y, j0 = blockIdx.x * ROWS, tid = threadIdx.x;
     for (int p = tid; p < ROWS * NX; p += kRhsThreads) {
         int jj = p / NX, i = p - jj * NX, j = j0 + jj;
-        double qt = 0.0;
+        R qt = R(0.0);
         if (j < A.ny) {
             Acc q(A, i, j, k);
-            double s;
+            R s;
             temperature_point<CDS>(q, s, qt);
             A.ss[((size_t)k * A.ny + j) * A.nx + i] = s;
         }
@@ -353,18 +359,19 @@ __global__ __launch_bounds__(kRhsThreads) void k_rhs_t(StepArgs A)
 }
 
 // Plain x R2C of one real field (standalone solve): src -> s0, scaled by norm.
-template <int NX>
-__global__ __launch_bounds__(kRhsThreads) void k_x_r2c(const double *__restrict__ src, double2 *dst, int ny,
-                                                       const double2 *tw_nl, const double2 *tw_nx, double norm)
+template <int NX, class R>
+__global__ __launch_bounds__(kRhsThreads) void k_x_r2c(const R *__restrict__ src, cplx<R> *dst, int ny,
+                                                       const cplx<R> *tw_nl, const cplx<R> *tw_nx, R norm)
 {
+    using C = cplx<R>;
     constexpr int ROWS = rows_per_block<NX>();
     constexpr int NL = NX / 2;
-    __shared__ double2 sq_[ROWS * NL];
-    double *sqd = reinterpret_cast<double *>(sq_);
+    __shared__ C sq_[ROWS * NL];
+    R *sqd = reinterpret_cast<R *>(sq_);
     const int k = blockIdx.y, j0 = blockIdx.x * ROWS, tid = threadIdx.x;
     for (int p = tid; p < ROWS * NX; p += kRhsThreads) {
         int jj = p / NX, i = p - jj * NX, j = j0 + jj;
-        sqd[jj * NX + i] = j < ny ? src[((size_t)k * ny + j) * NX + i] : 0.0;
+        sqd[jj * NX + i] = j < ny ? src[((size_t)k * ny + j) * NX + i] : R(0.0);
     }
     __syncthreads();
     lds_fft<NL, ROWS, NL, 1, kRhsThreads, -1>(sq_, tw_nl, tid);
@@ -377,54 +384,59 @@ __global__ __launch_bounds__(kRhsThreads) void k_x_r2c(const double *__restrict_
 // Inverse x transform (C2R) of F spectra rows into physical rows.
 // mode[f]: 0 store, 1 accumulate (dst += value), 2 accumulate onto the
 // damped old value on levels k >= kdamp0 (dst = dst - s (dst - mean) + value).
-struct C2RArgs {
-    const double2 *src[4];
-    double *dst[4];
+template <class R>
+struct C2RArgsT {
+    const cplx<R> *src[4];
+    R *dst[4];
     int mode[4];
     int ny;
-    const double2 *tw_nl, *tw_nx;
-    const double *dmp_s, *dmp_mean;
+    const cplx<R> *tw_nl, *tw_nx;
+    const R *dmp_s, *dmp_mean;
     int kdamp0;
 };
+using C2RArgs = C2RArgsT<double>;
 // points per field of one inverse-x block
 constexpr int kC2RPoints = 1024;
-template <int NX, int F>
-constexpr int c2r_rows() { return kC2RPoints / NX / F > 0 ? kC2RPoints / NX / F : 1; }
+template <class R>
+constexpr int c2r_points() { return kC2RPoints; }
+template <int NX, int F, class R = double>
+constexpr int c2r_rows() { return c2r_points<R>() / NX / F > 0 ? c2r_points<R>() / NX / F : 1; }
 
-template <int NX, int F>
-__global__ __launch_bounds__(kRhsThreads) void k_x_c2r(C2RArgs A)
+template <int NX, int F, class R>
+__global__ __launch_bounds__(kRhsThreads) void k_x_c2r(C2RArgsT<R> A)
 {
-    constexpr int ROWS = c2r_rows<NX, F>();
+    using C = cplx<R>;
+    constexpr int ROWS = c2r_rows<NX, F, R>();
     constexpr int NL = NX / 2;
     constexpr int B = F * ROWS;
-    __shared__ double2 sq_[B * NL];
-    __shared__ double2 tws[2 * NL];  // half-length twiddles, then the C2R pre-twiddles
+    __shared__ C sq_[B * NL];
+    __shared__ C tws[2 * NL];  // half-length twiddles, then the C2R pre-twiddles
     const int k = blockIdx.y, j0 = blockIdx.x * ROWS, tid = threadIdx.x;
-    double2 tw_a = czero(), tw_b = czero();
+    C tw_a = czero_t<C>(), tw_b = czero_t<C>();
     if (tid < NL) { tw_a = A.tw_nl[tid]; tw_b = A.tw_nx[tid]; }
     // the rows and, for the accumulated fields, the old values are loaded
     // before any is used
     constexpr int NIN = (B * NL + kRhsThreads - 1) / kRhsThreads;
     constexpr int NOUT = (B * NX + kRhsThreads - 1) / kRhsThreads;
-    double2 xl[NIN], xt[NIN];
+    C xl[NIN], xt[NIN];
 #pragma unroll
     for (int m = 0; m < NIN; ++m) {
         const int p = tid + m * kRhsThreads;
         const int row = p / NL, l = p - row * NL, f = row / ROWS, jj = row - f * ROWS, j = j0 + jj;
-        xl[m] = czero();
-        xt[m] = czero();
+        xl[m] = czero_t<C>();
+        xt[m] = czero_t<C>();
         if (((B * NL) % kRhsThreads == 0 || p < B * NL) && j < A.ny) {
-            const double2 *s = A.src[f] + ((size_t)k * A.ny + j) * NL;
+            const C *s = A.src[f] + ((size_t)k * A.ny + j) * NL;
             xl[m] = s[l];
             if (l != 0) xt[m] = cconj(s[NL - l]);
         }
     }
-    double old[NOUT];
+    R old[NOUT];
 #pragma unroll
     for (int m = 0; m < NOUT; ++m) {
         const int p = tid + m * kRhsThreads;
         const int row = p / NX, i = p - row * NX, f = row / ROWS, jj = row - f * ROWS, j = j0 + jj;
-        old[m] = 0.0;
+        old[m] = R(0.0);
         if (((B * NX) % kRhsThreads == 0 || p < B * NX) && j < A.ny && A.mode[f] != 0)
             old[m] = A.dst[f][((size_t)k * A.ny + j) * NX + i];
     }
@@ -440,18 +452,18 @@ __global__ __launch_bounds__(kRhsThreads) void k_x_c2r(C2RArgs A)
     }
     __syncthreads();
     lds_fft<NL, B, NL, 1, kRhsThreads, +1>(sq_, tws, tid);
-    const double *sqd = reinterpret_cast<const double *>(sq_);
+    const R *sqd = reinterpret_cast<const R *>(sq_);
 #pragma unroll
     for (int m = 0; m < NOUT; ++m) {
         const int p = tid + m * kRhsThreads;
         const int row = p / NX, i = p - row * NX, f = row / ROWS, jj = row - f * ROWS, j = j0 + jj;
         if (((B * NX) % kRhsThreads == 0 || p < B * NX) && j < A.ny) {
             size_t c = ((size_t)k * A.ny + j) * NX + i;
-            double x = sqd[row * NX + i];
+            R x = sqd[row * NX + i];
             if (A.mode[f] == 1) {
                 A.dst[f][c] = old[m] + x;
             } else if (A.mode[f] == 2) {
-                double o = old[m];
+                R o = old[m];
                 if (k >= A.kdamp0) o = o - A.dmp_s[k] * (o - A.dmp_mean[k]);
                 A.dst[f][c] = o + x;
             } else {
@@ -465,40 +477,45 @@ __global__ __launch_bounds__(kRhsThreads) void k_x_c2r(C2RArgs A)
 // level k, all j.  Forward zeroes the Nyquist row after the transform,
 // inverse before it (src/module_tools.f90:538-572,659-693).
 constexpr int kYTile = 8;
-struct YArgs {
-    double2 *f[4];
+template <class R>
+struct YArgsT {
+    cplx<R> *f[4];
     int nl, nz;
-    const double2 *tw_ny;
+    const cplx<R> *tw_ny;
 };
+using YArgs = YArgsT<double>;
 // LDS rows of the tile are padded by one complex so that the transform's
 // strided accesses spread over the banks.
 constexpr int kYPad = kYTile + 1;
-template <int NY, int SIGN>
-__global__ __launch_bounds__(256) void k_yfft(YArgs A)
+// TL consecutive l per block: 8 double2 or 16 float2, one 128-byte line per row.
+template <int NY, int SIGN, class R, int TL = kYTile>
+__global__ __launch_bounds__(256) void k_yfft(YArgsT<R> A)
 {
-    __shared__ double2 tile[NY * kYPad];
-    __shared__ double2 twy[NY];
-    double2 *fld = A.f[blockIdx.z];
-    const int l0 = blockIdx.x * kYTile, k = blockIdx.y, tid = threadIdx.x;
+    using C = cplx<R>;
+    constexpr int TP = TL + 1;
+    __shared__ C tile[NY * TP];
+    __shared__ C twy[NY];
+    C *fld = A.f[blockIdx.z];
+    const int l0 = blockIdx.x * TL, k = blockIdx.y, tid = threadIdx.x;
     for (int q = tid; q < NY; q += 256) twy[q] = A.tw_ny[q];
-    constexpr int NIT = (NY * kYTile + 255) / 256;
-    double2 v[NIT];
+    constexpr int NIT = (NY * TL + 255) / 256;
+    C v[NIT];
 #pragma unroll
     for (int m = 0; m < NIT; ++m) {
-        const int p = tid + m * 256, j = p / kYTile, ll = p - j * kYTile;
-        v[m] = p < NY * kYTile ? fld[((size_t)k * NY + j) * A.nl + l0 + ll] : czero();
+        const int p = tid + m * 256, j = p / TL, ll = p - j * TL;
+        v[m] = p < NY * TL ? fld[((size_t)k * NY + j) * A.nl + l0 + ll] : czero_t<C>();
     }
 #pragma unroll
     for (int m = 0; m < NIT; ++m) {
-        const int p = tid + m * 256, j = p / kYTile, ll = p - j * kYTile;
-        if (p < NY * kYTile) tile[j * kYPad + ll] = (SIGN > 0 && j == NY / 2) ? czero() : v[m];
+        const int p = tid + m * 256, j = p / TL, ll = p - j * TL;
+        if (p < NY * TL) tile[j * TP + ll] = (SIGN > 0 && j == NY / 2) ? czero_t<C>() : v[m];
     }
     __syncthreads();
-    lds_fft<NY, kYTile, 1, kYPad, 256, SIGN>(tile, twy, tid);
-    for (int p = tid; p < NY * kYTile; p += 256) {
-        int j = p / kYTile, ll = p - j * kYTile;
-        double2 v = tile[j * kYPad + ll];
-        if (SIGN < 0 && j == NY / 2) v = czero();
+    lds_fft<NY, TL, 1, TP, 256, SIGN>(tile, twy, tid);
+    for (int p = tid; p < NY * TL; p += 256) {
+        int j = p / TL, ll = p - j * TL;
+        C v = tile[j * TP + ll];
+        if (SIGN < 0 && j == NY / 2) v = czero_t<C>();
         fld[((size_t)k * NY + j) * A.nl + l0 + ll] = v;
     }
 }
@@ -528,9 +545,12 @@ HC_DEV ColMap col_map(int nz)
 // then hit distinct bank groups.
 HC_DEV int col_tidx(int k, int w) { return k * kColWaves + ((w + (k >> 2)) & (kColWaves - 1)); }
 
-template <int L>
-HC_DEV void col_load(const double2 *__restrict__ src, double2 *tile, int nz, size_t plane, size_t col0,
-                     const ColMap &m, double2 (&d)[L])
+// The tile holds the stored precision (CI: double2, or float2 in the mixed
+// variant); the registers of a column hold the solve's precision (CC), the
+// conversion happens between the two.
+template <int L, class CI, class CC>
+HC_DEV void col_load(const CI *__restrict__ src, CI *tile, int nz, size_t plane, size_t col0, const ColMap &m,
+                     CC (&d)[L])
 {
     const int w = threadIdx.x >> 6;
     for (int q = threadIdx.x; q < nz * kColWaves; q += 64 * kColWaves) {
@@ -539,18 +559,18 @@ HC_DEV void col_load(const double2 *__restrict__ src, double2 *tile, int nz, siz
     }
     __syncthreads();
 #pragma unroll
-    for (int r = 0; r < L; ++r) d[r] = r < m.cnt ? tile[col_tidx(m.k0 + r, w)] : czero();
+    for (int r = 0; r < L; ++r) d[r] = r < m.cnt ? ccast<CC>(tile[col_tidx(m.k0 + r, w)]) : czero_t<CC>();
     __syncthreads();
 }
 
-template <int L>
-HC_DEV void col_store(double2 *__restrict__ dst, double2 *tile, int nz, size_t plane, size_t col0,
-                      const ColMap &m, const double2 (&d)[L])
+template <int L, class CI, class CC>
+HC_DEV void col_store(CI *__restrict__ dst, CI *tile, int nz, size_t plane, size_t col0, const ColMap &m,
+                      const CC (&d)[L])
 {
     const int w = threadIdx.x >> 6;
 #pragma unroll
     for (int r = 0; r < L; ++r)
-        if (r < m.cnt) tile[col_tidx(m.k0 + r, w)] = d[r];
+        if (r < m.cnt) tile[col_tidx(m.k0 + r, w)] = ccast<CI>(d[r]);
     __syncthreads();
     for (int q = threadIdx.x; q < nz * kColWaves; q += 64 * kColWaves) {
         int k = q / kColWaves, c = q - k * kColWaves;
@@ -561,7 +581,8 @@ HC_DEV void col_store(double2 *__restrict__ dst, double2 *tile, int nz, size_t p
 
 // Multi-field transfers: fields f = 0..n-1 use sub-tiles f * nz * kColWaves
 // of the tile, so several fields share one global phase and one barrier.
-HC_DEV void col_fill(const double2 *__restrict__ src, double2 *sub, int nz, size_t plane, size_t col0)
+template <class CI>
+HC_DEV void col_fill(const CI *__restrict__ src, CI *sub, int nz, size_t plane, size_t col0)
 {
     for (int q = threadIdx.x; q < nz * kColWaves; q += 64 * kColWaves) {
         int k = q / kColWaves, c = q - k * kColWaves;
@@ -570,17 +591,18 @@ HC_DEV void col_fill(const double2 *__restrict__ src, double2 *sub, int nz, size
 }
 // NF fields at once: every load is issued before the first LDS store
 // (nz <= 64 L, so each thread moves at most L elements per field)
-template <int L, int NF>
-HC_DEV void col_fill_n(const double2 *const (&src)[NF], double2 *tile, int nz, size_t plane, size_t col0)
+template <int L, int NF, class CI>
+HC_DEV void col_fill_n(const CI *const (&src)[NF], CI *tile, int nz, size_t plane, size_t col0)
 {
     const size_t sub = (size_t)nz * kColWaves;
-    double2 v[NF][L];
+    CI v[NF][L];
 #pragma unroll
     for (int it = 0; it < L; ++it) {
         const int q = threadIdx.x + it * 64 * kColWaves;
         const int k = q / kColWaves, c = q - k * kColWaves;
 #pragma unroll
-        for (int f = 0; f < NF; ++f) v[f][it] = q < nz * kColWaves ? src[f][(size_t)k * plane + col0 + c] : czero();
+        for (int f = 0; f < NF; ++f)
+            v[f][it] = q < nz * kColWaves ? src[f][(size_t)k * plane + col0 + c] : czero_t<CI>();
     }
 #pragma unroll
     for (int it = 0; it < L; ++it) {
@@ -592,22 +614,23 @@ HC_DEV void col_fill_n(const double2 *const (&src)[NF], double2 *tile, int nz, s
         }
     }
 }
-template <int L>
-HC_DEV void col_take(const double2 *sub, const ColMap &m, double2 (&d)[L])
+template <int L, class CI, class CC>
+HC_DEV void col_take(const CI *sub, const ColMap &m, CC (&d)[L])
 {
     const int w = threadIdx.x >> 6;
 #pragma unroll
-    for (int r = 0; r < L; ++r) d[r] = r < m.cnt ? sub[col_tidx(m.k0 + r, w)] : czero();
+    for (int r = 0; r < L; ++r) d[r] = r < m.cnt ? ccast<CC>(sub[col_tidx(m.k0 + r, w)]) : czero_t<CC>();
 }
-template <int L>
-HC_DEV void col_put(double2 *sub, const ColMap &m, const double2 (&d)[L])
+template <int L, class CI, class CC>
+HC_DEV void col_put(CI *sub, const ColMap &m, const CC (&d)[L])
 {
     const int w = threadIdx.x >> 6;
 #pragma unroll
     for (int r = 0; r < L; ++r)
-        if (r < m.cnt) sub[col_tidx(m.k0 + r, w)] = d[r];
+        if (r < m.cnt) sub[col_tidx(m.k0 + r, w)] = ccast<CI>(d[r]);
 }
-HC_DEV void col_drain(double2 *__restrict__ dst, const double2 *sub, int nz, size_t plane, size_t col0)
+template <class CI>
+HC_DEV void col_drain(CI *__restrict__ dst, const CI *sub, int nz, size_t plane, size_t col0)
 {
     for (int q = threadIdx.x; q < nz * kColWaves; q += 64 * kColWaves) {
         int k = q / kColWaves, c = q - k * kColWaves;
@@ -616,31 +639,24 @@ HC_DEV void col_drain(double2 *__restrict__ dst, const double2 *sub, int nz, siz
 }
 
 // value of level (k0+cnt-1) from register array v (runtime index)
-template <int L>
-HC_DEV double2 last_of(const double2 (&v)[L], int cnt)
+template <int L, class C>
+HC_DEV C last_of(const C (&v)[L], int cnt)
 {
-    double2 r = czero();
+    C r = czero_t<C>();
 #pragma unroll
     for (int q = 0; q < L; ++q)
         if (q == cnt - 1) r = v[q];
     return r;
-}
-HC_DEV double2 shfl_up_c(double2 v, int d)
-{
-    return make_double2(__shfl_up(v.x, d, 64), __shfl_up(v.y, d, 64));
-}
-HC_DEV double2 shfl_dn_c(double2 v, int d)
-{
-    return make_double2(__shfl_down(v.x, d, 64), __shfl_down(v.y, d, 64));
 }
 
 // Assemble the rows of one equation for this lane's levels.  eq: 0 u/v,
 // 1 w, 2 p, 3 t.  The diagonal base of the (0,0) mode differs at the wall
 // rows because get_{u,w,t}_eqn_coeff zero the 1-D aa(k)/cc(k) while visiting
 // the first mode (src/module_poisson_solver.f90:257-272,361-366,455-469).
-template <int L>
-HC_DEV void assemble(const StepArgs &A, int eq, int l, int jm, bool is00, const ColMap &m, double (&a)[L],
-                     double (&b)[L], double (&c)[L])
+// The rows are formed in FP64 and rounded once to the solve's type Re.
+template <int L, class Re>
+HC_DEV void assemble(const StepArgs &A, int eq, int l, int jm, bool is00, const ColMap &m, Re (&a)[L],
+                     Re (&b)[L], Re (&c)[L])
 {
     const int nz = A.nz, nl = A.nl, ny = A.ny;
     const double rc = A.rc;
@@ -658,39 +674,40 @@ HC_DEV void assemble(const StepArgs &A, int eq, int l, int jm, bool is00, const 
     }
 #pragma unroll
     for (int r = 0; r < L; ++r) {
-        a[r] = 0.0; b[r] = 1.0; c[r] = 0.0;
+        a[r] = Re(0); b[r] = Re(1); c[r] = Re(0);
         if (r < m.cnt) {
             int k = m.k0 + r;
             if (eq == 1 && k == nz - 1) continue;  // w at kend3: trivial row, w = 0
-            a[r] = rc * A.lev[ta * nz + k];
-            c[r] = rc * A.lev[tc * nz + k];
+            a[r] = Re(rc * A.lev[ta * nz + k]);
+            c[r] = Re(rc * A.lev[tc * nz + k]);
             double dg = A.lev[(is00 ? td00 : td) * nz + k];
             if (eq == 2) {
-                b[r] = rc * (hxy + dg);
-                if (is00 && k == 0) { a[r] = 0.0; c[r] = 0.0; }  // delnull (:558-564)
+                b[r] = Re(rc * (hxy + dg));
+                if (is00 && k == 0) { a[r] = Re(0); c[r] = Re(0); }  // delnull (:558-564)
             } else {
-                b[r] = 1.0 + rc * (-hxy + dg);
+                b[r] = Re(1.0 + rc * (-hxy + dg));
             }
         }
     }
 }
 
 // (0,0)-mode right-hand-side fix-ups of poisson_solver_fft (:572-607)
-template <int L>
-HC_DEV void fixups(const StepArgs &A, int eq, const ColMap &m, double2 (&d)[L])
+template <int L, class C>
+HC_DEV void fixups(const StepArgs &A, int eq, const ColMap &m, C (&d)[L])
 {
+    using Re = real_of<C>;
 #pragma unroll
     for (int r = 0; r < L; ++r) {
         if (r >= m.cnt) continue;
         int k = m.k0 + r;
         if (eq == 0) {  // ubnd
-            if (k == 0) d[r].x += 2.0 * (A.rc * A.aU0) * A.u_mrf;
-            if (k == A.nz - 1 && A.ochannel == 0) d[r].x += 2.0 * (A.rc * A.cUe) * A.u_mrf;
+            if (k == 0) d[r].x += Re(2.0 * (A.rc * A.aU0) * A.u_mrf);
+            if (k == A.nz - 1 && A.ochannel == 0) d[r].x += Re(2.0 * (A.rc * A.cUe) * A.u_mrf);
         } else if (eq == 3) {  // tbnd
-            if (k == 0) d[r].x -= 2.0 * (A.rc * A.aT0) * A.tbot;
-            if (k == A.nz - 1) d[r].x -= 2.0 * (A.rc * A.cTe) * A.ttop;
+            if (k == 0) d[r].x -= Re(2.0 * (A.rc * A.aT0) * A.tbot);
+            if (k == A.nz - 1) d[r].x -= Re(2.0 * (A.rc * A.cTe) * A.ttop);
         } else if (eq == 2) {  // delnull
-            if (k == 0) d[r] = czero();
+            if (k == 0) d[r] = czero_t<C>();
         }
     }
 }
@@ -698,24 +715,25 @@ HC_DEV void fixups(const StepArgs &A, int eq, const ColMap &m, double2 (&d)[L])
 // Spectral divergence of (u, v, w) for this lane's levels, times `scale`:
 // D_x u + D_y v + (w(k) - w(k-1))/dz(k), w(-1) = 0 (get_p_eqn_src,
 // src/module_poisson_solver.f90:141-172, for periodic x, y).
-template <int L>
-HC_DEV void col_divergence(const StepArgs &A, const ColMap &m, int l, int jm, const double2 (&u)[L],
-                           const double2 (&v)[L], const double2 (&w)[L], double scale, double2 (&out)[L])
+template <int L, class C>
+HC_DEV void col_divergence(const StepArgs &A, const ColMap &m, int l, int jm, const C (&u)[L], const C (&v)[L],
+                           const C (&w)[L], real_of<C> scale, C (&out)[L])
 {
+    using Re = real_of<C>;
     const int nz = A.nz, nl = A.nl, ny = A.ny;
-    const double2 sdx = make_double2(A.symx[SY_DRE * nl + l], A.symx[SY_DIM * nl + l]);
-    const double2 sdy = make_double2(A.symy[SY_DRE * ny + jm], A.symy[SY_DIM * ny + jm]);
-    double2 wprev = shfl_up_c(last_of<L>(w, m.cnt), 1);
-    if (m.lane == 0) wprev = czero();
+    const C sdx = mk2<C>(Re(A.symx[SY_DRE * nl + l]), Re(A.symx[SY_DIM * nl + l]));
+    const C sdy = mk2<C>(Re(A.symy[SY_DRE * ny + jm]), Re(A.symy[SY_DIM * ny + jm]));
+    C wprev = shfl_up_c(last_of<L>(w, m.cnt), 1);
+    if (m.lane == 0) wprev = czero_t<C>();
 #pragma unroll
     for (int r = 0; r < L; ++r) {
-        out[r] = czero();
+        out[r] = czero_t<C>();
         if (r < m.cnt) {
             int k = m.k0 + r;
-            double2 wm = r == 0 ? wprev : w[r > 0 ? r - 1 : 0];
-            double idz = A.lev[LV_IDZ * nz + k];
-            double2 dv = cadd(cmul(sdx, u[r]), cmul(sdy, v[r]));
-            double2 dwz = csub(w[r], wm);
+            C wm = r == 0 ? wprev : w[r > 0 ? r - 1 : 0];
+            Re idz = Re(A.lev[LV_IDZ * nz + k]);
+            C dv = cadd(cmul(sdx, u[r]), cmul(sdy, v[r]));
+            C dwz = csub(w[r], wm);
             dv.x += dwz.x * idz;
             dv.y += dwz.y * idz;
             out[r] = cscale(dv, scale);
@@ -724,12 +742,12 @@ HC_DEV void col_divergence(const StepArgs &A, const ColMap &m, int l, int jm, co
 }
 
 // phi(k+1) seen from level r of this lane (Neumann copy at the top wall)
-template <int L>
-HC_DEV double2 col_next(const ColMap &m, int nz, const double2 (&ph)[L], int r, double2 pnext)
+template <int L, class C>
+HC_DEV C col_next(const ColMap &m, int nz, const C (&ph)[L], int r, C pnext)
 {
     int k = m.k0 + r;
     if (k >= nz - 1) return ph[r];
-    double2 cand = czero();
+    C cand = czero_t<C>();
 #pragma unroll
     for (int q = 0; q < L; ++q)
         if (q == r + 1) cand = ph[q];
@@ -738,14 +756,15 @@ HC_DEV double2 col_next(const ColMap &m, int nz, const double2 (&ph)[L], int r, 
 
 // Projection u -= cdt G_x phi, v -= cdt G_y phi, w(k) -= cdt (phi(k+1)-phi(k))/dz_t(k)
 // for k < kend3 (calculate_uvw, src/module_navier_stokes.f90:659-734).
-template <int L>
-HC_DEV void col_project(const StepArgs &A, const ColMap &m, int l, int jm, double cdt, const double2 (&ph)[L],
-                        double2 (&u)[L], double2 (&v)[L], double2 (&w)[L])
+template <int L, class C>
+HC_DEV void col_project(const StepArgs &A, const ColMap &m, int l, int jm, real_of<C> cdt, const C (&ph)[L],
+                        C (&u)[L], C (&v)[L], C (&w)[L])
 {
+    using Re = real_of<C>;
     const int nz = A.nz, nl = A.nl, ny = A.ny;
-    const double2 sgx = make_double2(A.symx[SY_GRE * nl + l], A.symx[SY_GIM * nl + l]);
-    const double2 sgy = make_double2(A.symy[SY_GRE * ny + jm], A.symy[SY_GIM * ny + jm]);
-    double2 pnext = shfl_dn_c(ph[0], 1);
+    const C sgx = mk2<C>(Re(A.symx[SY_GRE * nl + l]), Re(A.symx[SY_GIM * nl + l]));
+    const C sgy = mk2<C>(Re(A.symy[SY_GRE * ny + jm]), Re(A.symy[SY_GIM * ny + jm]));
+    C pnext = shfl_dn_c(ph[0], 1);
 #pragma unroll
     for (int r = 0; r < L; ++r) {
         if (r < m.cnt) {
@@ -753,29 +772,69 @@ HC_DEV void col_project(const StepArgs &A, const ColMap &m, int l, int jm, doubl
             u[r] = csub(u[r], cscale(cmul(sgx, ph[r]), cdt));
             v[r] = csub(v[r], cscale(cmul(sgy, ph[r]), cdt));
             if (k < nz - 1) {
-                double2 gz = csub(col_next<L>(m, nz, ph, r, pnext), ph[r]);
-                double f = cdt * A.lev[LV_IDZT * nz + k];
-                w[r] = make_double2(w[r].x - f * gz.x, w[r].y - f * gz.y);
+                C gz = csub(col_next<L>(m, nz, ph, r, pnext), ph[r]);
+                Re f = cdt * Re(A.lev[LV_IDZT * nz + k]);
+                w[r] = mk2<C>(w[r].x - f * gz.x, w[r].y - f * gz.y);
             }
         }
     }
 }
 
-template <int L>
-__global__ __launch_bounds__(64 * kColWaves) void k_column_mom(StepArgs A)
+// Spectra and column selection of a column-kernel launch.  The columns of
+// one block are kColWaves consecutive l at one y mode jm.  sel = 0: every
+// block of the grid; 1: only the low-wavenumber blocks (l < 8 * lo_l, |m|
+// <= lo_m), grid (lo_l, 2 lo_m + 1); 2: every block except those.  The mixed
+// precision variant solves the low-wavenumber blocks in FP64 and the rest
+// in FP32.
+template <class RIO>
+struct ColIO {
+    cplx<RIO> *s0, *s1, *s2, *s3;
+    int sel, lo_l, lo_m;
+};
+struct ColPos {
+    int bx, jm;
+    bool skip;
+};
+template <class RIO>
+HC_DEV ColPos col_pos(const ColIO<RIO> &io, int ny)
 {
+    ColPos q;
+    q.bx = blockIdx.x;
+    q.jm = blockIdx.y;
+    q.skip = false;
+    if (io.sel == 1) {
+        const int by = blockIdx.y;
+        q.jm = by <= io.lo_m ? by : ny - (2 * io.lo_m + 1) + by;
+    } else if (io.sel == 2) {
+        q.skip = q.bx < io.lo_l && (q.jm <= io.lo_m || q.jm >= ny - io.lo_m);
+    }
+    return q;
+}
+
+// One fused column pass of the substep: Helmholtz solves of u*, v*, w*, the
+// divergence source, the pressure Poisson solve, the projection and the
+// pressure increment.  RIO: stored precision of the spectra; RC: the
+// arithmetic of the solves.
+template <int L, class RIO, class RC>
+__global__ __launch_bounds__(64 * kColWaves) void k_column_mom(StepArgs A, ColIO<RIO> io)
+{
+    using CI = cplx<RIO>;
+    using CC = cplx<RC>;
+    extern __shared__ double2 tile_raw[];
+    CI *tile = reinterpret_cast<CI *>(tile_raw);
+    const ColPos pos = col_pos(io, A.ny);
+    if (pos.skip) return;  // block-uniform, before any barrier
     // nl is a multiple of kColWaves: every wavefront owns a column
-    extern __shared__ double2 tile[];
-    const int l = blockIdx.x * kColWaves + (threadIdx.x >> 6), jm = blockIdx.y;
+    const int l = pos.bx * kColWaves + (threadIdx.x >> 6), jm = pos.jm;
     const ColMap m = col_map(A.nz);
     const int nz = A.nz, nl = A.nl, ny = A.ny;
     const bool is00 = (l == 0 && jm == 0);
-    const size_t plane = (size_t)ny * nl, col0 = (size_t)jm * nl + (size_t)blockIdx.x * kColWaves;
-    double2 duv[2][L], dw[1][L], dp[1][L];
+    const size_t plane = (size_t)ny * nl, col0 = (size_t)jm * nl + (size_t)pos.bx * kColWaves;
+    CC duv[2][L], dw[1][L], dp[1][L];
     // all three inputs in one phase when the tile holds them (A.ntile sub-tiles)
     const size_t sub = (size_t)nz * kColWaves;
     if (A.ntile >= 3) {
-        const double2 *srcs[3] = {A.s0, A.s1, A.s2};
+        const CI *srcs[3] = {io.s0, io.s1, io.s2};
         col_fill_n<L, 3>(srcs, tile, nz, plane, col0);
         __syncthreads();
         col_take<L>(tile, m, duv[0]);
@@ -783,11 +842,11 @@ __global__ __launch_bounds__(64 * kColWaves) void k_column_mom(StepArgs A)
         col_take<L>(tile + 2 * sub, m, dw[0]);
         __syncthreads();
     } else {
-        col_load<L>(A.s0, tile, nz, plane, col0, m, duv[0]);
-        col_load<L>(A.s1, tile, nz, plane, col0, m, duv[1]);
-        col_load<L>(A.s2, tile, nz, plane, col0, m, dw[0]);
+        col_load<L>(io.s0, tile, nz, plane, col0, m, duv[0]);
+        col_load<L>(io.s1, tile, nz, plane, col0, m, duv[1]);
+        col_load<L>(io.s2, tile, nz, plane, col0, m, dw[0]);
     }
-    double a[L], b[L], c[L];
+    RC a[L], b[L], c[L];
     // u and v share the matrix (assign_abc(aa_u,...) for both, :291-299)
     assemble<L>(A, 0, l, jm, is00, m, a, b, c);
     if (is00) fixups<L>(A, 0, m, duv[0]);
@@ -795,38 +854,38 @@ __global__ __launch_bounds__(64 * kColWaves) void k_column_mom(StepArgs A)
     // w (:302-304): kend3 row is trivial and its right-hand side is unused
 #pragma unroll
     for (int r = 0; r < L; ++r)
-        if (r < m.cnt && m.k0 + r == nz - 1) dw[0][r] = czero();
+        if (r < m.cnt && m.k0 + r == nz - 1) dw[0][r] = czero_t<CC>();
     assemble<L>(A, 1, l, jm, is00, m, a, b, c);
     wave_tridiag<L, 1>(m.cnt, a, b, c, dw);
     // divergence source /dt (get_p_eqn_src, :127-176) in spectral form
-    col_divergence<L>(A, m, l, jm, duv[0], duv[1], dw[0], A.idt, dp[0]);
+    col_divergence<L>(A, m, l, jm, duv[0], duv[1], dw[0], RC(A.idt), dp[0]);
     // pressure Poisson (:312-314)
     assemble<L>(A, 2, l, jm, is00, m, a, b, c);
     if (is00) fixups<L>(A, 2, m, dp[0]);
     wave_tridiag_refined<L, 1>(m.cnt, a, b, c, dp);
     // projection (calculate_uvw, :659-734) and pressure increment
     // (calculate_new_p, :739-808)
-    const double cdt = A.rc * A.dt;
+    const RC cdt = RC(A.rc * A.dt);
     col_project<L>(A, m, l, jm, cdt, dp[0], duv[0], duv[1], dw[0]);
-    const double slap = A.symx[SY_L * nl + l] + A.symy[SY_L * ny + jm];
-    double2 pprev = shfl_up_c(last_of<L>(dp[0], m.cnt), 1);
-    double2 pnext = shfl_dn_c(dp[0][0], 1);
-    const double cnu = A.rc * 0.5 * A.dt * A.nu;
-    double2 dpn[L];
+    const RC slap = RC(A.symx[SY_L * nl + l] + A.symy[SY_L * ny + jm]);
+    CC pprev = shfl_up_c(last_of<L>(dp[0], m.cnt), 1);
+    CC pnext = shfl_dn_c(dp[0][0], 1);
+    const RC cnu = RC(A.rc * 0.5 * A.dt * A.nu);
+    CC dpn[L];
 #pragma unroll
     for (int r = 0; r < L; ++r) {
-        dpn[r] = czero();
+        dpn[r] = czero_t<CC>();
         if (r < m.cnt) {
             int k = m.k0 + r;
-            double2 ph = dp[0][r];
-            double2 pm = r == 0 ? (m.lane == 0 ? ph : pprev) : dp[0][r > 0 ? r - 1 : 0];
-            double2 pn = col_next<L>(m, nz, dp[0], r, pnext);
-            double idzt = A.lev[LV_IDZT * nz + k], idzb = A.lev[LV_IDZB * nz + k], idz = A.lev[LV_IDZ * nz + k];
-            double2 lz;
+            CC ph = dp[0][r];
+            CC pm = r == 0 ? (m.lane == 0 ? ph : pprev) : dp[0][r > 0 ? r - 1 : 0];
+            CC pn = col_next<L>(m, nz, dp[0], r, pnext);
+            RC idzt = RC(A.lev[LV_IDZT * nz + k]), idzb = RC(A.lev[LV_IDZB * nz + k]), idz = RC(A.lev[LV_IDZ * nz + k]);
+            CC lz;
             lz.x = ((pn.x - ph.x) * idzt - (ph.x - pm.x) * idzb) * idz;
             lz.y = ((pn.y - ph.y) * idzt - (ph.y - pm.y) * idzb) * idz;
-            double2 lap = make_double2(slap * ph.x + lz.x, slap * ph.y + lz.y);
-            dpn[r] = make_double2(ph.x - cnu * lap.x, ph.y - cnu * lap.y);
+            CC lap = mk2<CC>(slap * ph.x + lz.x, slap * ph.y + lz.y);
+            dpn[r] = mk2<CC>(ph.x - cnu * lap.x, ph.y - cnu * lap.y);
         }
     }
     if (A.ntile >= 4) {
@@ -835,90 +894,103 @@ __global__ __launch_bounds__(64 * kColWaves) void k_column_mom(StepArgs A)
         col_put<L>(tile + 2 * sub, m, dw[0]);
         col_put<L>(tile + 3 * sub, m, dpn);
         __syncthreads();
-        col_drain(A.s0, tile, nz, plane, col0);
-        col_drain(A.s1, tile + sub, nz, plane, col0);
-        col_drain(A.s2, tile + 2 * sub, nz, plane, col0);
-        col_drain(A.s3, tile + 3 * sub, nz, plane, col0);
+        col_drain(io.s0, tile, nz, plane, col0);
+        col_drain(io.s1, tile + sub, nz, plane, col0);
+        col_drain(io.s2, tile + 2 * sub, nz, plane, col0);
+        col_drain(io.s3, tile + 3 * sub, nz, plane, col0);
     } else {
-        col_store<L>(A.s0, tile, nz, plane, col0, m, duv[0]);
-        col_store<L>(A.s1, tile, nz, plane, col0, m, duv[1]);
-        col_store<L>(A.s2, tile, nz, plane, col0, m, dw[0]);
-        col_store<L>(A.s3, tile, nz, plane, col0, m, dpn);
+        col_store<L>(io.s0, tile, nz, plane, col0, m, duv[0]);
+        col_store<L>(io.s1, tile, nz, plane, col0, m, duv[1]);
+        col_store<L>(io.s2, tile, nz, plane, col0, m, dw[0]);
+        col_store<L>(io.s3, tile, nz, plane, col0, m, dpn);
     }
 }
 
 #include "hc_spectral.h"
 
 // Temperature Helmholtz solve in place on s0 (:335-337).
-template <int L>
-__global__ __launch_bounds__(64 * kColWaves) void k_column_t(StepArgs A)
+template <int L, class RIO, class RC>
+__global__ __launch_bounds__(64 * kColWaves) void k_column_t(StepArgs A, ColIO<RIO> io)
 {
-    extern __shared__ double2 tile[];
-    const int l = blockIdx.x * kColWaves + (threadIdx.x >> 6), jm = blockIdx.y;
+    using CI = cplx<RIO>;
+    extern __shared__ double2 tile_raw[];
+    CI *tile = reinterpret_cast<CI *>(tile_raw);
+    const ColPos pos = col_pos(io, A.ny);
+    if (pos.skip) return;
+    const int l = pos.bx * kColWaves + (threadIdx.x >> 6), jm = pos.jm;
     const ColMap m = col_map(A.nz);
     const bool is00 = (l == 0 && jm == 0);
-    const size_t plane = (size_t)A.ny * A.nl, col0 = (size_t)jm * A.nl + (size_t)blockIdx.x * kColWaves;
-    double2 d[1][L];
-    col_load<L>(A.s0, tile, A.nz, plane, col0, m, d[0]);
-    double a[L], b[L], c[L];
+    const size_t plane = (size_t)A.ny * A.nl, col0 = (size_t)jm * A.nl + (size_t)pos.bx * kColWaves;
+    cplx<RC> d[1][L];
+    col_load<L>(io.s0, tile, A.nz, plane, col0, m, d[0]);
+    RC a[L], b[L], c[L];
     assemble<L>(A, 3, l, jm, is00, m, a, b, c);
     if (is00) fixups<L>(A, 3, m, d[0]);
     wave_tridiag<L, 1>(m.cnt, a, b, c, d);
-    col_store<L>(A.s0, tile, A.nz, plane, col0, m, d[0]);
+    col_store<L>(io.s0, tile, A.nz, plane, col0, m, d[0]);
 }
 
 // Projection of (u, v, w) spectra in s0, s1, s2 onto discretely
 // divergence-free fields with the pressure operator at rk_c = 1:
 // phi = L^{-1} div u, u -= G phi (the operators of get_p_eqn_src,
 // poisson_solver_fft and calculate_uvw).  Used to prepare initial states.
-template <int L>
-__global__ __launch_bounds__(64 * kColWaves) void k_column_project(StepArgs A)
+template <int L, class RIO, class RC>
+__global__ __launch_bounds__(64 * kColWaves) void k_column_project(StepArgs A, ColIO<RIO> io)
 {
-    extern __shared__ double2 tile[];
-    const int l = blockIdx.x * kColWaves + (threadIdx.x >> 6), jm = blockIdx.y;
+    using CI = cplx<RIO>;
+    using CC = cplx<RC>;
+    extern __shared__ double2 tile_raw[];
+    CI *tile = reinterpret_cast<CI *>(tile_raw);
+    const ColPos pos = col_pos(io, A.ny);
+    if (pos.skip) return;
+    const int l = pos.bx * kColWaves + (threadIdx.x >> 6), jm = pos.jm;
     const ColMap m = col_map(A.nz);
     const bool is00 = (l == 0 && jm == 0);
-    const size_t plane = (size_t)A.ny * A.nl, col0 = (size_t)jm * A.nl + (size_t)blockIdx.x * kColWaves;
-    double2 u[1][L], v[1][L], w[1][L], ph[1][L];
-    col_load<L>(A.s0, tile, A.nz, plane, col0, m, u[0]);
-    col_load<L>(A.s1, tile, A.nz, plane, col0, m, v[0]);
-    col_load<L>(A.s2, tile, A.nz, plane, col0, m, w[0]);
+    const size_t plane = (size_t)A.ny * A.nl, col0 = (size_t)jm * A.nl + (size_t)pos.bx * kColWaves;
+    CC u[1][L], v[1][L], w[1][L], ph[1][L];
+    col_load<L>(io.s0, tile, A.nz, plane, col0, m, u[0]);
+    col_load<L>(io.s1, tile, A.nz, plane, col0, m, v[0]);
+    col_load<L>(io.s2, tile, A.nz, plane, col0, m, w[0]);
 #pragma unroll
     for (int r = 0; r < L; ++r)
-        if (r < m.cnt && m.k0 + r == A.nz - 1) w[0][r] = czero();
-    col_divergence<L>(A, m, l, jm, u[0], v[0], w[0], 1.0, ph[0]);
-    double a[L], b[L], c[L];
+        if (r < m.cnt && m.k0 + r == A.nz - 1) w[0][r] = czero_t<CC>();
+    col_divergence<L>(A, m, l, jm, u[0], v[0], w[0], RC(1), ph[0]);
+    RC a[L], b[L], c[L];
     assemble<L>(A, 2, l, jm, is00, m, a, b, c);
     if (is00) fixups<L>(A, 2, m, ph[0]);
     wave_tridiag_refined<L, 1>(m.cnt, a, b, c, ph);
-    col_project<L>(A, m, l, jm, 1.0, ph[0], u[0], v[0], w[0]);
-    col_store<L>(A.s0, tile, A.nz, plane, col0, m, u[0]);
-    col_store<L>(A.s1, tile, A.nz, plane, col0, m, v[0]);
-    col_store<L>(A.s2, tile, A.nz, plane, col0, m, w[0]);
+    col_project<L>(A, m, l, jm, RC(1), ph[0], u[0], v[0], w[0]);
+    col_store<L>(io.s0, tile, A.nz, plane, col0, m, u[0]);
+    col_store<L>(io.s1, tile, A.nz, plane, col0, m, v[0]);
+    col_store<L>(io.s2, tile, A.nz, plane, col0, m, w[0]);
 }
 
 // Standalone poisson_solver_fft for one equation, in place on s0.
 // eq: 0 u (ubnd), 1 v, 2 w (zstagger), 3 p (delnull), 4 t (tbnd).
-template <int L>
-__global__ __launch_bounds__(64 * kColWaves) void k_column_solve(StepArgs A, int eq)
+template <int L, class RIO, class RC>
+__global__ __launch_bounds__(64 * kColWaves) void k_column_solve(StepArgs A, ColIO<RIO> io, int eq)
 {
-    extern __shared__ double2 tile[];
-    const int l = blockIdx.x * kColWaves + (threadIdx.x >> 6), jm = blockIdx.y;
+    using CI = cplx<RIO>;
+    extern __shared__ double2 tile_raw[];
+    CI *tile = reinterpret_cast<CI *>(tile_raw);
+    const ColPos pos = col_pos(io, A.ny);
+    if (pos.skip) return;
+    const int l = pos.bx * kColWaves + (threadIdx.x >> 6), jm = pos.jm;
     const ColMap m = col_map(A.nz);
     const bool is00 = (l == 0 && jm == 0);
-    const size_t plane = (size_t)A.ny * A.nl, col0 = (size_t)jm * A.nl + (size_t)blockIdx.x * kColWaves;
-    double2 d[1][L];
-    col_load<L>(A.s0, tile, A.nz, plane, col0, m, d[0]);
+    const size_t plane = (size_t)A.ny * A.nl, col0 = (size_t)jm * A.nl + (size_t)pos.bx * kColWaves;
+    cplx<RC> d[1][L];
+    col_load<L>(io.s0, tile, A.nz, plane, col0, m, d[0]);
 #pragma unroll
     for (int r = 0; r < L; ++r)
-        if (eq == 2 && r < m.cnt && m.k0 + r == A.nz - 1) d[0][r] = czero();
+        if (eq == 2 && r < m.cnt && m.k0 + r == A.nz - 1) d[0][r] = czero_t<cplx<RC>>();
     const int meq = eq == 0 || eq == 1 ? 0 : (eq == 2 ? 1 : (eq == 3 ? 2 : 3));
-    double a[L], b[L], c[L];
+    RC a[L], b[L], c[L];
     assemble<L>(A, meq, l, jm, is00, m, a, b, c);
     if (is00 && eq != 1 && eq != 2) fixups<L>(A, meq, m, d[0]);
     if (meq == 2) wave_tridiag_refined<L, 1>(m.cnt, a, b, c, d);
     else wave_tridiag<L, 1>(m.cnt, a, b, c, d);
-    col_store<L>(A.s0, tile, A.nz, plane, col0, m, d[0]);
+    col_store<L>(io.s0, tile, A.nz, plane, col0, m, d[0]);
 }
 
 // ---------------------------------------------------------------------------
@@ -931,8 +1003,9 @@ __global__ void k_dpx_update(double *scal, double u_mrf, double coef)
 
 // calculate_ubar (src/module_parameters.f90:513-546): per-level partial sums
 // then a fixed-order final sum, so the result is deterministic.
-__global__ __launch_bounds__(256) void k_ubar_partial(const double *__restrict__ u, const double *lev, int nx,
-                                                      int ny, int nz, double *part)
+template <class R>
+__global__ __launch_bounds__(256) void k_ubar_partial(const R *__restrict__ u, const double *lev, int nx, int ny,
+                                                      int nz, double *part)
 {
     __shared__ double red[256];
     const int k = blockIdx.x;
@@ -948,7 +1021,8 @@ __global__ __launch_bounds__(256) void k_ubar_partial(const double *__restrict__
     if (threadIdx.x == 0) part[k] = red[0] * lev[LV_DZ * nz + k];
 }
 // plane sums of one level per block: part[k] = sum of f over level k
-__global__ __launch_bounds__(256) void k_plane_sum(const double *__restrict__ f, int nx, int ny, int k0, double *part)
+template <class R>
+__global__ __launch_bounds__(256) void k_plane_sum(const R *__restrict__ f, int nx, int ny, int k0, double *part)
 {
     __shared__ double red[256];
     const int k = k0 + blockIdx.x;
@@ -973,35 +1047,39 @@ __global__ void k_ubar_final(const double *part, int nz, double lz, int nx, int 
 }
 
 // screen_div_error values (src/module_io.f90:85-206): per-level sums of
-// |div| and max |div|, NaN flag.  cds as in the reference.
-template <int CDS>
-__global__ __launch_bounds__(256) void k_div_partial(StepArgs A, double *part)
+// |div| and max |div|, NaN flag.  cds as in the reference.  F holds the
+// fields in their stored precision; the divergence is formed in FP64 with
+// the metrics of D.
+template <int CDS, class R>
+__global__ __launch_bounds__(256) void k_div_partial(StepArgsT<R> F, StepArgs D, double *part)
 {
     __shared__ double rs[256], rm[256];
     __shared__ int rn[256];
     const int k = blockIdx.x;
     double s = 0.0, mx = 0.0;
     int nan = 0;
-    const size_t n = (size_t)A.nx * A.ny;
-    const double dz = A.lev[LV_DZ * A.nz + k];
+    const size_t n = (size_t)D.nx * D.ny;
+    const double dz = D.lev[LV_DZ * D.nz + k];
     for (size_t q = threadIdx.x; q < n; q += 256) {
-        int j = (int)(q / A.nx), i = (int)(q - (size_t)j * A.nx);
-        Pt P{A, i, j, k};
+        int j = (int)(q / D.nx), i = (int)(q - (size_t)j * D.nx);
+        PtT<R> P{F, i, j, k};
+        auto U = [&](int di) -> double { return (double)P.U(di, 0, 0); };
+        auto V = [&](int dj) -> double { return (double)P.V(0, dj, 0); };
         double dudx = 0.0, dvdy = 0.0;
         if (CDS == 1) {
-            dudx = (P.U(0, 0, 0) - P.U(-1, 0, 0)) / A.dx;
-            dvdy = (P.V(0, 0, 0) - P.V(0, -1, 0)) / A.dy;
+            dudx = (U(0) - U(-1)) / D.dx;
+            dvdy = (V(0) - V(-1)) / D.dy;
         } else if (CDS == 2) {
-            dudx = (-P.U(1, 0, 0) + 27.0 * P.U(0, 0, 0) - 27.0 * P.U(-1, 0, 0) + P.U(-2, 0, 0)) / 24.0 / A.dx;
-            dvdy = (-P.V(0, 1, 0) + 27.0 * P.V(0, 0, 0) - 27.0 * P.V(0, -1, 0) + P.V(0, -2, 0)) / 24.0 / A.dy;
+            dudx = (-U(1) + 27.0 * U(0) - 27.0 * U(-1) + U(-2)) / 24.0 / D.dx;
+            dvdy = (-V(1) + 27.0 * V(0) - 27.0 * V(-1) + V(-2)) / 24.0 / D.dy;
         }
-        double w0 = A.w[P.at(0, 0, k)];
-        double wm = k == 0 ? 0.0 : A.w[P.at(0, 0, k - 1)];
+        double w0 = F.w[P.at(0, 0, k)];
+        double wm = k == 0 ? 0.0 : (double)F.w[P.at(0, 0, k - 1)];
         // spectral path: the divergence field was formed and inverted beforehand
-        double d = CDS == 3 ? A.aux[P.at(0, 0, k)] : dudx + dvdy + (w0 - wm) / dz;
+        double d = CDS == 3 ? D.aux[P.at(0, 0, k)] : dudx + dvdy + (w0 - wm) / dz;
         s += fabs(d);
         mx = fmax(mx, fabs(d));
-        double u0 = A.u[P.at(0, 0, k)], v0 = A.v[P.at(0, 0, k)];
+        double u0 = F.u[P.at(0, 0, k)], v0 = F.v[P.at(0, 0, k)];
         if (u0 != u0 || v0 != v0 || w0 != w0) nan = 1;
     }
     rs[threadIdx.x] = s;
@@ -1023,38 +1101,75 @@ __global__ __launch_bounds__(256) void k_div_partial(StepArgs A, double *part)
     }
 }
 
-// interior <-> ghosted (nx+4, ny+4, nz+4) Fortran layout
-__global__ void k_pack_ghosted(const double *__restrict__ g, double *dst, int nx, int ny, int nz)
+// interior <-> ghosted (nx+4, ny+4, nz+4) Fortran layout; the interior
+// array is in the stored precision R
+template <class R>
+__global__ void k_pack_ghosted(const double *__restrict__ g, R *dst, int nx, int ny, int nz)
 {
     size_t n = (size_t)nx * ny * nz;
     for (size_t q = blockIdx.x * (size_t)blockDim.x + threadIdx.x; q < n; q += (size_t)gridDim.x * blockDim.x) {
         int i = (int)(q % nx), j = (int)((q / nx) % ny), k = (int)(q / ((size_t)nx * ny));
-        dst[q] = g[((size_t)(k + 2) * (ny + 4) + (j + 2)) * (nx + 4) + (i + 2)];
+        dst[q] = R(g[((size_t)(k + 2) * (ny + 4) + (j + 2)) * (nx + 4) + (i + 2)]);
     }
 }
-__global__ void k_unpack_ghosted(const double *__restrict__ src, double *g, int nx, int ny, int nz)
+template <class R>
+__global__ void k_unpack_ghosted(const R *__restrict__ src, double *g, int nx, int ny, int nz)
 {
     size_t n = (size_t)nx * ny * nz;
     for (size_t q = blockIdx.x * (size_t)blockDim.x + threadIdx.x; q < n; q += (size_t)gridDim.x * blockDim.x) {
         int i = (int)(q % nx), j = (int)((q / nx) % ny), k = (int)(q / ((size_t)nx * ny));
-        g[((size_t)(k + 2) * (ny + 4) + (j + 2)) * (nx + 4) + (i + 2)] = src[q];
+        g[((size_t)(k + 2) * (ny + 4) + (j + 2)) * (nx + 4) + (i + 2)] = (double)src[q];
     }
 }
-__global__ void k_zero_level(double *f, size_t off, size_t n)
+// element-wise precision conversion (rounded to nearest)
+template <class S, class D>
+__global__ void k_convert(const S *__restrict__ src, D *dst, size_t n)
+{
+    for (size_t q = blockIdx.x * (size_t)blockDim.x + threadIdx.x; q < n; q += (size_t)gridDim.x * blockDim.x)
+        dst[q] = D(src[q]);
+}
+template <class R>
+__global__ void k_zero_level(R *f, size_t off, size_t n)
 {
     size_t q = blockIdx.x * (size_t)blockDim.x + threadIdx.x;
-    if (q < n) f[off + q] = 0.0;
+    if (q < n) f[off + q] = R(0);
 }
 
 // ===========================================================================
 // host side
+
+// Low-wavenumber blocks of the mixed variant solved in FP64: l < kColWaves *
+// HC_LOWK_L and |ky index| <= HC_LOWK_M.
+#ifndef HC_LOWK_L
+#define HC_LOWK_L 1
+#endif
+#ifndef HC_LOWK_M
+#define HC_LOWK_M 8
+#endif
 static const double kPi = 3.1415926535897932;  // src/module_parameters.f90:32
 static const double kRkc1[3] = {8.0 / 15.0, 5.0 / 12.0, 3.0 / 4.0};  // :210-212
 static const double kRkc2[3] = {0.0, -17.0 / 60.0, -5.0 / 12.0};
 static const double kRkc3[3] = {8.0 / 15.0, 2.0 / 15.0, 1.0 / 3.0};
 
+// Device buffers of one stored precision: fields, right-hand-side
+// ping-pongs, the four half spectra, and the tables the row kernels read.
+template <class R>
+struct View {
+    R *u, *v, *w, *p, *t;
+    R *fb[2][3];
+    R *sb[2];
+    cplx<R> *s[4];
+    R *lev, *symx, *symy;
+    cplx<R> *tw_nl, *tw_nx, *tw_ny;
+};
+
 struct hc_ctx {
     hc_params prm;
+    // mixed precision (prm.precision = 1): fields, spectra and row kernels in
+    // FP32 (m32); column blocks with l < 8 lo_l and |m| <= lo_m solved in FP64
+    bool mixed = false;
+    View<float> m32 = {};
+    int lo_l = 0, lo_m = 0;
     int nx, ny, nz, nl, Lcol;
     double dx, dy;
     int device;
@@ -1306,6 +1421,57 @@ static StepArgs base_args(hc_ctx *c)
     return A;
 }
 
+template <class R> static View<R> view(hc_ctx *c);
+template <> View<double> view<double>(hc_ctx *c)
+{
+    View<double> V;
+    V.u = c->u; V.v = c->v; V.w = c->w; V.p = c->p; V.t = c->t;
+    for (int a = 0; a < 2; ++a)
+        for (int b = 0; b < 3; ++b) V.fb[a][b] = c->fb[a][b];
+    V.sb[0] = c->sb[0]; V.sb[1] = c->sb[1];
+    for (int q = 0; q < 4; ++q) V.s[q] = c->s[q];
+    V.lev = c->lev; V.symx = c->symx; V.symy = c->symy;
+    V.tw_nl = c->tw_nl; V.tw_nx = c->tw_nx; V.tw_ny = c->tw_ny;
+    return V;
+}
+template <> View<float> view<float>(hc_ctx *c) { return c->m32; }
+
+// Row-kernel arguments in the stored precision R: the pointers of V and the
+// scalars of A rounded to R (the FP64 device scalars stay shared).
+template <class R>
+static StepArgsT<R> rows_args(const StepArgs &A, const View<R> &V)
+{
+    StepArgsT<R> B;
+    std::memset(&B, 0, sizeof B);
+    B.u = V.u; B.v = V.v; B.w = V.w; B.p = V.p; B.t = V.t;
+    B.s0 = V.s[0]; B.s1 = V.s[1]; B.s2 = V.s[2]; B.s3 = V.s[3];
+    B.lev = V.lev; B.symx = V.symx; B.symy = V.symy;
+    B.tw_nl = V.tw_nl; B.tw_nx = V.tw_nx; B.tw_ny = V.tw_ny;
+    B.scal = A.scal;
+    B.nx = A.nx; B.ny = A.ny; B.nz = A.nz; B.nl = A.nl;
+    B.ochannel = A.ochannel; B.dp_opt = A.dp_opt; B.isscalar = A.isscalar; B.use_f0 = A.use_f0;
+    const double sc[] = {A.dx, A.dy, A.dx2, A.dy2, A.dt, A.nu, A.alpha, A.got, A.fc, A.ug, A.vg, A.u_mrf, A.t_ref,
+                         A.tbot, A.ttop, A.ra, A.rb, A.rc, A.dpx, A.norm, A.c24x, A.c12x2, A.c24y, A.c12y2, A.idx,
+                         A.idx2, A.idy, A.idy2, A.idt, A.aU0, A.cUe, A.aT0, A.cTe, A.coe};
+    R *dst[] = {&B.dx, &B.dy, &B.dx2, &B.dy2, &B.dt, &B.nu, &B.alpha, &B.got, &B.fc, &B.ug, &B.vg, &B.u_mrf, &B.t_ref,
+                &B.tbot, &B.ttop, &B.ra, &B.rb, &B.rc, &B.dpx, &B.norm, &B.c24x, &B.c12x2, &B.c24y, &B.c12y2, &B.idx,
+                &B.idx2, &B.idy, &B.idy2, &B.idt, &B.aU0, &B.cUe, &B.aT0, &B.cTe, &B.coe};
+    static_assert(sizeof sc / sizeof sc[0] == sizeof dst / sizeof dst[0], "scalar lists differ");
+    for (size_t q = 0; q < sizeof sc / sizeof sc[0]; ++q) *dst[q] = R(sc[q]);
+    B.kdamp0 = A.kdamp0; B.dmask = A.dmask;
+    B.issk = A.issk; B.lc = A.lc; B.mc = A.mc;
+    B.ntile = A.ntile;
+    return B;
+}
+template <>
+StepArgsT<double> rows_args<double>(const StepArgs &A, const View<double> &V)
+{
+    StepArgs B = A;
+    B.u = V.u; B.v = V.v; B.w = V.w; B.p = V.p; B.t = V.t;
+    B.s0 = V.s[0]; B.s1 = V.s[1]; B.s2 = V.s[2]; B.s3 = V.s[3];
+    return B;
+}
+
 // ---- template dispatch ----------------------------------------------------
 #define HC_DISPATCH_N(n, MACRO)                                                          \
     switch (n) {                                                                          \
@@ -1331,174 +1497,250 @@ static StepArgs base_args(hc_ctx *c)
 
 static int rows_for(int nx) { return nx >= kRowPoints ? 1 : kRowPoints / nx; }
 
-// dynamic LDS of the column kernels: one nz x 8 complex tile
 // LDS of the column kernels: `nfields` sub-tiles of nz x kColWaves complex
-// when they fit in kColLds bytes, else one.
+// (elem bytes each) when they fit in kColLds bytes, else one.
 constexpr size_t kColLds = 160 * 1024;
-static int col_tiles(const hc_ctx *c, int nfields)
+static int col_tiles(const hc_ctx *c, int nfields, size_t elem = sizeof(double2))
 {
-    const size_t one = (size_t)c->nz * kColWaves * sizeof(double2);
+    const size_t one = (size_t)c->nz * kColWaves * elem;
     return (c->big_lds && one * nfields <= kColLds) ? nfields : 1;
 }
-static size_t col_lds(const hc_ctx *c, int nfields = 1)
+static size_t col_lds(const hc_ctx *c, int nfields = 1, size_t elem = sizeof(double2))
 {
-    return (size_t)col_tiles(c, nfields) * c->nz * kColWaves * sizeof(double2);
+    return (size_t)col_tiles(c, nfields, elem) * c->nz * kColWaves * elem;
 }
 
-static int launch_yfft(hc_ctx *c, int sign, int nf, double2 *f0, double2 *f1, double2 *f2, double2 *f3)
+template <class T> struct nd { using type = T; };
+
+template <class C>
+static int launch_yfft(hc_ctx *c, int sign, int nf, C *f0, typename nd<C>::type *f1, typename nd<C>::type *f2,
+                       typename nd<C>::type *f3)
 {
-    YArgs Y;
+    using R = real_of<C>;
+    YArgsT<R> Y;
     Y.f[0] = f0; Y.f[1] = f1; Y.f[2] = f2; Y.f[3] = f3;
-    Y.nl = c->nl; Y.nz = c->nz; Y.tw_ny = c->tw_ny;
-    dim3 g(c->nl / kYTile, c->nz, nf);
-#define HC_Y(NY_)                                                                                 \
-    if (sign < 0) hipLaunchKernelGGL((k_yfft<NY_, -1>), g, dim3(256), 0, c->stream, Y);          \
-    else hipLaunchKernelGGL((k_yfft<NY_, +1>), g, dim3(256), 0, c->stream, Y);
+    Y.nl = c->nl; Y.nz = c->nz; Y.tw_ny = view<R>(c).tw_ny;
+    // FP32 spectra: 16 l per block (one cache line per row) where nl allows
+    const bool wide = sizeof(R) == 4 && c->nl % (2 * kYTile) == 0;
+    dim3 g(c->nl / (wide ? 2 * kYTile : kYTile), c->nz, nf);
+#define HC_Y(NY_)                                                                                          \
+    if (wide) {                                                                                            \
+        if (sign < 0) hipLaunchKernelGGL((k_yfft<NY_, -1, R, 2 * kYTile>), g, dim3(256), 0, c->stream, Y); \
+        else hipLaunchKernelGGL((k_yfft<NY_, +1, R, 2 * kYTile>), g, dim3(256), 0, c->stream, Y);          \
+    } else {                                                                                               \
+        if (sign < 0) hipLaunchKernelGGL((k_yfft<NY_, -1, R>), g, dim3(256), 0, c->stream, Y);             \
+        else hipLaunchKernelGGL((k_yfft<NY_, +1, R>), g, dim3(256), 0, c->stream, Y);                      \
+    }
     HC_DISPATCH_N(c->ny, HC_Y)
 #undef HC_Y
     return 0;
 }
 
-static int launch_c2r(hc_ctx *c, int nf, C2RArgs &X)
+template <class R>
+static int launch_c2r(hc_ctx *c, int nf, C2RArgsT<R> &X)
 {
     if (nf < 1 || nf > 4) return set_err(HC_ERR_ARG, "launch_c2r: %d fields", nf);
-    X.ny = c->ny; X.tw_nl = c->tw_nl; X.tw_nx = c->tw_nx;
-    int rows = kC2RPoints / c->nx / nf > 0 ? kC2RPoints / c->nx / nf : 1;
+    const View<R> V = view<R>(c);
+    X.ny = c->ny; X.tw_nl = V.tw_nl; X.tw_nx = V.tw_nx;
+    const int pts = c2r_points<R>();
+    int rows = pts / c->nx / nf > 0 ? pts / c->nx / nf : 1;
     dim3 g((c->ny + rows - 1) / rows, c->nz);
-#define HC_C(NX_)                                                                                  \
-    if (nf == 4) hipLaunchKernelGGL((k_x_c2r<NX_, 4>), g, dim3(kRhsThreads), 0, c->stream, X);    \
-    else if (nf == 3) hipLaunchKernelGGL((k_x_c2r<NX_, 3>), g, dim3(kRhsThreads), 0, c->stream, X); \
-    else if (nf == 2) hipLaunchKernelGGL((k_x_c2r<NX_, 2>), g, dim3(kRhsThreads), 0, c->stream, X); \
-    else hipLaunchKernelGGL((k_x_c2r<NX_, 1>), g, dim3(kRhsThreads), 0, c->stream, X);
+#define HC_C(NX_)                                                                                       \
+    if (nf == 4) hipLaunchKernelGGL((k_x_c2r<NX_, 4, R>), g, dim3(kRhsThreads), 0, c->stream, X);      \
+    else if (nf == 3) hipLaunchKernelGGL((k_x_c2r<NX_, 3, R>), g, dim3(kRhsThreads), 0, c->stream, X); \
+    else if (nf == 2) hipLaunchKernelGGL((k_x_c2r<NX_, 2, R>), g, dim3(kRhsThreads), 0, c->stream, X); \
+    else hipLaunchKernelGGL((k_x_c2r<NX_, 1, R>), g, dim3(kRhsThreads), 0, c->stream, X);
     HC_DISPATCH_N(c->nx, HC_C)
 #undef HC_C
     return 0;
 }
 
-static int launch_r2c(hc_ctx *c, const double *src, double2 *dst)
+template <class R>
+static int launch_r2c(hc_ctx *c, const R *src, cplx<R> *dst)
 {
+    const View<R> V = view<R>(c);
     int rows = rows_for(c->nx);
     dim3 g((c->ny + rows - 1) / rows, c->nz);
-    double norm = 1.0 / ((double)c->nx * (double)c->ny);
-#define HC_R(NX_) hipLaunchKernelGGL((k_x_r2c<NX_>), g, dim3(kRhsThreads), 0, c->stream, src, dst, c->ny, \
-                                     c->tw_nl, c->tw_nx, norm);
+    R norm = R(1.0 / ((double)c->nx * (double)c->ny));
+#define HC_R(NX_) hipLaunchKernelGGL((k_x_r2c<NX_, R>), g, dim3(kRhsThreads), 0, c->stream, src, dst, c->ny, \
+                                     (const cplx<R> *)V.tw_nl, (const cplx<R> *)V.tw_nx, norm);
     HC_DISPATCH_N(c->nx, HC_R)
 #undef HC_R
     return 0;
 }
 
-// one substep t_its (1..3 for RK3, 1 for AB2)
-static int substep(hc_ctx *c, int t_its)
+// Column kernels: kind 0 k_column_mom, 1 k_column_t, 2 k_column_project,
+// 3 k_column_solve (equation eq).
+template <int L, class RIO, class RC>
+static void col_kernel(int kind, dim3 g, size_t lds, hipStream_t st, const StepArgs &A, const ColIO<RIO> &io, int eq)
 {
+    const dim3 b(64 * kColWaves);
+    switch (kind) {
+    case 0: hipLaunchKernelGGL((k_column_mom<L, RIO, RC>), g, b, lds, st, A, io); break;
+    case 1: hipLaunchKernelGGL((k_column_t<L, RIO, RC>), g, b, lds, st, A, io); break;
+    case 2: hipLaunchKernelGGL((k_column_project<L, RIO, RC>), g, b, lds, st, A, io); break;
+    default: hipLaunchKernelGGL((k_column_solve<L, RIO, RC>), g, b, lds, st, A, io, eq); break;
+    }
+}
+// One column pass over spectra s[0..3] of V.  FP64 storage: every block in
+// FP64.  Mixed: the low-wavenumber blocks in FP64, then the rest in FP32.
+template <class R>
+static int launch_columns(hc_ctx *c, int kind, StepArgs A, const View<R> &V, int nfields, int eq = 0)
+{
+    ColIO<R> io;
+    io.s0 = V.s[0]; io.s1 = V.s[1]; io.s2 = V.s[2]; io.s3 = V.s[3];
+    io.lo_l = c->lo_l; io.lo_m = c->lo_m;
+    A.ntile = col_tiles(c, nfields, sizeof(cplx<R>));
+    const size_t lds = col_lds(c, nfields, sizeof(cplx<R>));
+    const dim3 gfull((c->nl + kColWaves - 1) / kColWaves, c->ny);
+    if constexpr (std::is_same<R, double>::value) {
+        io.sel = 0;
+#define HC_CK(L_) col_kernel<L_, double, double>(kind, gfull, lds, c->stream, A, io, eq);
+        HC_DISPATCH_L(c->Lcol, HC_CK)
+#undef HC_CK
+    } else {
+        if (c->lo_l > 0) {
+            io.sel = 1;
+            const dim3 glo(c->lo_l, 2 * c->lo_m + 1);
+#define HC_CK(L_) col_kernel<L_, float, double>(kind, glo, lds, c->stream, A, io, eq);
+            HC_DISPATCH_L(c->Lcol, HC_CK)
+#undef HC_CK
+        }
+        io.sel = c->lo_l > 0 ? 2 : 0;
+#define HC_CK(L_) col_kernel<L_, float, float>(kind, gfull, lds, c->stream, A, io, eq);
+        HC_DISPATCH_L(c->Lcol, HC_CK)
+#undef HC_CK
+    }
+    return 0;
+}
+
+// one substep t_its (1..3 for RK3, 1 for AB2) over the buffers of V (stored
+// precision R); Rayleigh damping only with FP64 storage
+template <class R>
+static int substep_t(hc_ctx *c, int t_its, const View<R> &V)
+{
+    using C = cplx<R>;
     const hc_params &P = c->prm;
     StepArgs A = base_args(c);
     if (P.dts == 2) { A.ra = kRkc1[t_its - 1]; A.rb = kRkc2[t_its - 1]; A.rc = kRkc3[t_its - 1]; }
     else { A.ra = 1.5; A.rb = -0.5; A.rc = 1.0; }
     A.use_f0 = A.rb != 0.0;
     int nxt = c->fcur ^ 1;
-    A.ff0 = c->fb[c->fcur][0]; A.gg0 = c->fb[c->fcur][1]; A.hh0 = c->fb[c->fcur][2];
-    A.ff = c->fb[nxt][0]; A.gg = c->fb[nxt][1]; A.hh = c->fb[nxt][2];
+    StepArgsT<R> B = rows_args<R>(A, V);
+    B.ff0 = V.fb[c->fcur][0]; B.gg0 = V.fb[c->fcur][1]; B.hh0 = V.fb[c->fcur][2];
+    B.ff = V.fb[nxt][0]; B.gg = V.fb[nxt][1]; B.hh = V.fb[nxt][2];
     if (P.dp_opt == 2) hipLaunchKernelGGL(k_dpx_update, dim3(1), dim3(64), 0, c->stream, c->scal, P.u_mrf, P.nu * 1.0);
     int rows = rows_for(c->nx);
     dim3 grow((c->ny + rows - 1) / rows, c->nz);
     const bool damp = P.isdamp == 1;
+    if (damp && !std::is_same<R, double>::value) return set_err(HC_ERR_UNSUPPORTED, "damping needs FP64 storage");
     PROF_BEGIN(c, HC_K_RHS);
-    if (damp) {
-        // rayleigh_damping of u, v, w, p, t at the start of the substep
-        // (src/module_navier_stokes.f90:258-266): plane means of the undamped
-        // fields; the stencils apply the damping as they read.
-        DampFields F = {{c->u, c->v, c->w, c->p, c->t}};
-        hipLaunchKernelGGL(k_damp_means, dim3(P.nzdamp + 1, 5), dim3(256), 0, c->stream, F, c->dmp_m, c->nx, c->ny,
-                           c->nz, c->kdamp0);
-        if (P.isscalar == 0) {
-            hipLaunchKernelGGL(k_damp_apply, dim3(256), dim3(256), 0, c->stream, c->t, c->dmp_s, c->dmp_m + 4 * c->nz,
-                               c->nx, c->ny, c->kdamp0, P.nzdamp + 1);
-        }
-        A.dmask = 31;
+    if constexpr (std::is_same<R, double>::value) {
+        if (damp) {
+            // rayleigh_damping of u, v, w, p, t at the start of the substep
+            // (src/module_navier_stokes.f90:258-266): plane means of the undamped
+            // fields; the stencils apply the damping as they read.
+            DampFields F = {{c->u, c->v, c->w, c->p, c->t}};
+            hipLaunchKernelGGL(k_damp_means, dim3(P.nzdamp + 1, 5), dim3(256), 0, c->stream, F, c->dmp_m, c->nx, c->ny,
+                               c->nz, c->kdamp0);
+            if (P.isscalar == 0) {
+                hipLaunchKernelGGL(k_damp_apply, dim3(256), dim3(256), 0, c->stream, c->t, c->dmp_s,
+                                   c->dmp_m + 4 * c->nz, c->nx, c->ny, c->kdamp0, P.nzdamp + 1);
+            }
+            B.dmask = 31;
 #define HC_RHSD(NX_)                                                                                         \
-    if (P.cds == 1) hipLaunchKernelGGL((k_rhs_mom_damped<1, NX_>), grow, dim3(kRhsThreads), 0, c->stream, A); \
-    else hipLaunchKernelGGL((k_rhs_mom_damped<2, NX_>), grow, dim3(kRhsThreads), 0, c->stream, A);
-        HC_DISPATCH_N(c->nx, HC_RHSD)
+    if (P.cds == 1) hipLaunchKernelGGL((k_rhs_mom_damped<1, NX_>), grow, dim3(kRhsThreads), 0, c->stream, B); \
+    else hipLaunchKernelGGL((k_rhs_mom_damped<2, NX_>), grow, dim3(kRhsThreads), 0, c->stream, B);
+            HC_DISPATCH_N(c->nx, HC_RHSD)
 #undef HC_RHSD
-    } else {
-#define HC_RHS(NX_)                                                                                   \
-    if constexpr (NX_ <= 256) {                                                                        \
-        constexpr int TY_ = rhs2_ty<NX_>();                                                            \
-        dim3 g1((unsigned)(c->ny / TY_ * c->nz));                                                      \
-        constexpr int T_ = rhs2_threads<NX_>();                                                        \
-        if (P.cds == 1) hipLaunchKernelGGL((k_rhs_mom2<1, NX_>), g1, dim3(T_), 0, c->stream, A);       \
-        else hipLaunchKernelGGL((k_rhs_mom2<2, NX_>), g1, dim3(T_), 0, c->stream, A);                  \
-    } else {                                                                                           \
-        constexpr int TY_ = rhs_ty<NX_>();                                                             \
-        dim3 g1((unsigned)(c->ny / TY_ * c->nz));                                                      \
-        constexpr int T_ = rhs_threads<NX_>();                                                         \
-        if (P.cds == 1) hipLaunchKernelGGL((k_rhs_mom<1, NX_>), g1, dim3(T_), 0, c->stream, A);        \
-        else hipLaunchKernelGGL((k_rhs_mom<2, NX_>), g1, dim3(T_), 0, c->stream, A);                   \
+        }
     }
-    HC_DISPATCH_N(c->nx, HC_RHS)
+    if (!damp) {
+#define HC_RHS(NX_)                                                                                     \
+    if constexpr (NX_ <= 256) {                                                                          \
+        constexpr int TY_ = rhs2_ty<NX_>();                                                              \
+        dim3 g1((unsigned)(c->ny / TY_ * c->nz));                                                        \
+        constexpr int T_ = rhs2_threads<NX_>();                                                          \
+        if (P.cds == 1) hipLaunchKernelGGL((k_rhs_mom2<1, NX_, R>), g1, dim3(T_), 0, c->stream, B);      \
+        else hipLaunchKernelGGL((k_rhs_mom2<2, NX_, R>), g1, dim3(T_), 0, c->stream, B);                 \
+    } else {                                                                                             \
+        constexpr int TY_ = rhs_ty<NX_>();                                                               \
+        dim3 g1((unsigned)(c->ny / TY_ * c->nz));                                                        \
+        constexpr int T_ = rhs_threads<NX_>();                                                           \
+        if (P.cds == 1) hipLaunchKernelGGL((k_rhs_mom<1, NX_, R>), g1, dim3(T_), 0, c->stream, B);       \
+        else hipLaunchKernelGGL((k_rhs_mom<2, NX_, R>), g1, dim3(T_), 0, c->stream, B);                  \
+    }
+        HC_DISPATCH_N(c->nx, HC_RHS)
 #undef HC_RHS
     }
     PROF_END(c, HC_K_RHS);
+    C *const *S = V.s;
     PROF_BEGIN(c, HC_K_YFFT_FWD);
-    if (int e = launch_yfft(c, -1, 3, c->s[0], c->s[1], c->s[2], c->s[3])) return e;
+    if (int e = launch_yfft(c, -1, 3, S[0], S[1], S[2], S[3])) return e;
     PROF_END(c, HC_K_YFFT_FWD);
-    dim3 gcol((c->nl + kColWaves - 1) / kColWaves, c->ny);
-    A.ntile = col_tiles(c, 4);
     PROF_BEGIN(c, HC_K_COLUMN);
-#define HC_COL(L_) hipLaunchKernelGGL((k_column_mom<L_>), gcol, dim3(64 * kColWaves), col_lds(c, 4), c->stream, A);
-    HC_DISPATCH_L(c->Lcol, HC_COL)
-#undef HC_COL
+    if (int e = launch_columns<R>(c, 0, A, V, 4)) return e;
     PROF_END(c, HC_K_COLUMN);
     PROF_BEGIN(c, HC_K_YFFT_INV);
-    if (int e = launch_yfft(c, +1, 4, c->s[0], c->s[1], c->s[2], c->s[3])) return e;
+    if (int e = launch_yfft(c, +1, 4, S[0], S[1], S[2], S[3])) return e;
     PROF_END(c, HC_K_YFFT_INV);
-    C2RArgs X;
+    C2RArgsT<R> X;
     std::memset(&X, 0, sizeof X);
-    X.src[0] = c->s[0]; X.src[1] = c->s[1]; X.src[2] = c->s[2]; X.src[3] = c->s[3];
-    X.dst[0] = c->u; X.dst[1] = c->v; X.dst[2] = c->w; X.dst[3] = c->p;
+    X.src[0] = S[0]; X.src[1] = S[1]; X.src[2] = S[2]; X.src[3] = S[3];
+    X.dst[0] = V.u; X.dst[1] = V.v; X.dst[2] = V.w; X.dst[3] = V.p;
     X.mode[3] = damp ? 2 : 1;  // calculate_new_p adds to the damped p
-    X.dmp_s = c->dmp_s; X.dmp_mean = c->dmp_m + 3 * c->nz; X.kdamp0 = c->kdamp0;
+    if constexpr (std::is_same<R, double>::value) { X.dmp_s = c->dmp_s; X.dmp_mean = c->dmp_m + 3 * c->nz; }
+    X.kdamp0 = c->kdamp0;
     PROF_BEGIN(c, HC_K_FINISH);
-    if (int e = launch_c2r(c, 4, X)) return e;
+    if (int e = launch_c2r<R>(c, 4, X)) return e;
     PROF_END(c, HC_K_FINISH);
     c->fcur = nxt;
     if (P.isscalar == 1) {
         int snx = c->scur ^ 1;
-        A.ss0 = c->sb[c->scur]; A.ss = c->sb[snx];
+        B.ss0 = V.sb[c->scur]; B.ss = V.sb[snx];
         PROF_BEGIN(c, HC_K_RHS_T);
-        A.dmask = 16;  // t read damped, the updated velocities as they are
-#define HC_RT(NX_)                                                                                          \
-    if (damp) {                                                                                             \
-        if (P.cds == 1) hipLaunchKernelGGL((k_rhs_t<1, NX_, PtD>), grow, dim3(kRhsThreads), 0, c->stream, A); \
-        else hipLaunchKernelGGL((k_rhs_t<2, NX_, PtD>), grow, dim3(kRhsThreads), 0, c->stream, A);          \
-    } else {                                                                                                \
-        if (P.cds == 1) hipLaunchKernelGGL((k_rhs_t<1, NX_, Pt>), grow, dim3(kRhsThreads), 0, c->stream, A);  \
-        else hipLaunchKernelGGL((k_rhs_t<2, NX_, Pt>), grow, dim3(kRhsThreads), 0, c->stream, A);           \
-    }
-        HC_DISPATCH_N(c->nx, HC_RT)
+        B.dmask = 16;  // t read damped, the updated velocities as they are
+        if constexpr (std::is_same<R, double>::value) {
+            if (damp) {
+#define HC_RT(NX_)                                                                                        \
+    if (P.cds == 1) hipLaunchKernelGGL((k_rhs_t<1, NX_, PtD>), grow, dim3(kRhsThreads), 0, c->stream, B); \
+    else hipLaunchKernelGGL((k_rhs_t<2, NX_, PtD>), grow, dim3(kRhsThreads), 0, c->stream, B);
+                HC_DISPATCH_N(c->nx, HC_RT)
 #undef HC_RT
+            }
+        }
+        if (!damp) {
+#define HC_RT(NX_)                                                                                             \
+    if (P.cds == 1) hipLaunchKernelGGL((k_rhs_t<1, NX_, PtT<R>>), grow, dim3(kRhsThreads), 0, c->stream, B);   \
+    else hipLaunchKernelGGL((k_rhs_t<2, NX_, PtT<R>>), grow, dim3(kRhsThreads), 0, c->stream, B);
+            HC_DISPATCH_N(c->nx, HC_RT)
+#undef HC_RT
+        }
         PROF_END(c, HC_K_RHS_T);
         PROF_BEGIN(c, HC_K_YFFT_T);
-        if (int e = launch_yfft(c, -1, 1, c->s[0], nullptr, nullptr, nullptr)) return e;
+        if (int e = launch_yfft(c, -1, 1, S[0], nullptr, nullptr, nullptr)) return e;
         PROF_END(c, HC_K_YFFT_T);
         PROF_BEGIN(c, HC_K_COLUMN_T);
-#define HC_COLT(L_) hipLaunchKernelGGL((k_column_t<L_>), gcol, dim3(64 * kColWaves), col_lds(c), c->stream, A);
-        HC_DISPATCH_L(c->Lcol, HC_COLT)
-#undef HC_COLT
+        if (int e = launch_columns<R>(c, 1, A, V, 1)) return e;
         PROF_END(c, HC_K_COLUMN_T);
         PROF_BEGIN(c, HC_K_YFFT_T);
-        if (int e = launch_yfft(c, +1, 1, c->s[0], nullptr, nullptr, nullptr)) return e;
+        if (int e = launch_yfft(c, +1, 1, S[0], nullptr, nullptr, nullptr)) return e;
         PROF_END(c, HC_K_YFFT_T);
-        C2RArgs XT;
+        C2RArgsT<R> XT;
         std::memset(&XT, 0, sizeof XT);
-        XT.src[0] = c->s[0]; XT.dst[0] = c->t;
+        XT.src[0] = S[0]; XT.dst[0] = V.t;
         PROF_BEGIN(c, HC_K_FINISH_T);
-        if (int e = launch_c2r(c, 1, XT)) return e;
+        if (int e = launch_c2r<R>(c, 1, XT)) return e;
         PROF_END(c, HC_K_FINISH_T);
         c->scur = snx;
     }
     HIPCHK(hipGetLastError());
     return 0;
+}
+
+static int substep(hc_ctx *c, int t_its)
+{
+    return c->mixed ? substep_t<float>(c, t_its, view<float>(c)) : substep_t<double>(c, t_its, view<double>(c));
 }
 
 // ---- spectral path (cds = 3) ------------------------------------------------
@@ -1638,7 +1880,12 @@ static int substep_sp(hc_ctx *c, int t_its)
 
 static int ubar_launch(hc_ctx *c)
 {
-    hipLaunchKernelGGL(k_ubar_partial, dim3(c->nz), dim3(256), 0, c->stream, c->u, c->lev, c->nx, c->ny, c->nz, c->part);
+    if (c->mixed)
+        hipLaunchKernelGGL(k_ubar_partial<float>, dim3(c->nz), dim3(256), 0, c->stream, c->m32.u, c->lev, c->nx, c->ny,
+                           c->nz, c->part);
+    else
+        hipLaunchKernelGGL(k_ubar_partial<double>, dim3(c->nz), dim3(256), 0, c->stream, c->u, c->lev, c->nx, c->ny,
+                           c->nz, c->part);
     hipLaunchKernelGGL(k_ubar_final, dim3(1), dim3(64), 0, c->stream, c->part, c->nz, c->prm.lz, c->nx, c->ny, c->scal);
     HIPCHK(hipGetLastError());
     return 0;
@@ -1666,6 +1913,9 @@ extern "C" int hc_create(const hc_params *prm, hc_ctx **out)
     if (P.isdamp == 1 && (P.nzdamp < 1 || P.nzdamp > P.nz - 1))
         return set_err(HC_ERR_ARG, "nzdamp=%d must be in 1..nz-1 when isdamp=1", P.nzdamp);
     if (P.wall_rows != 0 && P.wall_rows != 1) return set_err(HC_ERR_ARG, "wall_rows must be 0 or 1");
+    if (P.precision != HC_FP64 && P.precision != HC_MIXED) return set_err(HC_ERR_ARG, "precision must be 0 or 1");
+    if (P.precision == HC_MIXED && (P.cds == 3 || P.isdamp == 1))
+        return set_err(HC_ERR_UNSUPPORTED, "precision=1 (mixed) supports cds=1,2 without damping");
     if (!fft_len_ok(P.nx) || !fft_len_ok(P.ny)) return set_err(HC_ERR_UNSUPPORTED, "nx, ny must be one of 16,32,64,128,192,256,512");
     {
         int ty = P.nx == 16 ? 16 : P.nx == 32 ? 32 : P.nx == 64 ? 16 : P.nx == 128 ? 8 : P.nx == 192 ? 4 : P.nx == 256 ? 4 : 1;
@@ -1685,14 +1935,18 @@ extern "C" int hc_create(const hc_params *prm, hc_ctx **out)
     {
         // allow the multi-field column tiles (up to kColLds of dynamic LDS)
         hipError_t ae = hipSuccess;
-#define HC_ATTR(L_)                                                                                     \
-    ae = hipFuncSetAttribute(reinterpret_cast<const void *>(&k_column_mom<L_>),                        \
-                             hipFuncAttributeMaxDynamicSharedMemorySize, (int)kColLds);               \
-    if (ae == hipSuccess)                                                                               \
-        ae = hipFuncSetAttribute(reinterpret_cast<const void *>(&k_column_spec<L_>),                   \
-                                 hipFuncAttributeMaxDynamicSharedMemorySize, (int)kColLds);
+#define HC_ATTR1(K_)                                                                                   \
+    if (ae == hipSuccess)                                                                              \
+        ae = hipFuncSetAttribute(reinterpret_cast<const void *>(&K_), hipFuncAttributeMaxDynamicSharedMemorySize, \
+                                 (int)kColLds);
+#define HC_ATTR(L_)                                  \
+    HC_ATTR1((k_column_mom<L_, double, double>))     \
+    HC_ATTR1((k_column_mom<L_, float, float>))       \
+    HC_ATTR1((k_column_mom<L_, float, double>))      \
+    HC_ATTR1((k_column_spec<L_>))
         HC_DISPATCH_L(L, HC_ATTR)
 #undef HC_ATTR
+#undef HC_ATTR1
         c->big_lds = ae == hipSuccess;
         (void)hipGetLastError();
     }
@@ -1725,6 +1979,23 @@ extern "C" int hc_create(const hc_params *prm, hc_ctx **out)
         for (int q = 0; q < 4; ++q) c->sp[q] = c->s[q];
         if (!e) e = dalloc(c, (void **)&c->ptmp, N * sizeof(double));
     }
+    if (!e && P.precision == HC_MIXED) {
+        c->mixed = true;
+        c->lo_l = std::min(HC_LOWK_L, c->nl / kColWaves);
+        c->lo_m = std::min(HC_LOWK_M, c->ny / 2 - 1);
+        View<float> &M = c->m32;
+        float **phys32[] = {&M.u, &M.v, &M.w, &M.p, &M.t, &M.fb[0][0], &M.fb[0][1], &M.fb[0][2],
+                            &M.fb[1][0], &M.fb[1][1], &M.fb[1][2], &M.sb[0], &M.sb[1]};
+        for (auto pp : phys32) if (!e) e = dalloc(c, (void **)pp, N * sizeof(float));
+        for (int q = 0; q < 4 && !e; ++q)
+            e = dalloc(c, (void **)&M.s[q], (size_t)c->nl * c->ny * c->nz * sizeof(float2));
+        if (!e) e = dalloc(c, (void **)&M.lev, c->h_lev.size() * sizeof(float));
+        if (!e) e = dalloc(c, (void **)&M.symx, sx.size() * sizeof(float));
+        if (!e) e = dalloc(c, (void **)&M.symy, sy.size() * sizeof(float));
+        if (!e) e = dalloc(c, (void **)&M.tw_nl, tnl.size() * sizeof(float2));
+        if (!e) e = dalloc(c, (void **)&M.tw_nx, tnx.size() * sizeof(float2));
+        if (!e) e = dalloc(c, (void **)&M.tw_ny, tny.size() * sizeof(float2));
+    }
     if (e) { hc_destroy(c); return e; }
     // sponge strength sigz of rayleigh_damping (src/module_tools.f90:735-736)
     // on the top nzdamp+1 levels, zero below
@@ -1743,6 +2014,23 @@ extern "C" int hc_create(const hc_params *prm, hc_ctx **out)
     if (r == hipSuccess) r = hipMemcpyAsync(c->tw_ny, tny.data(), tny.size() * sizeof(double2), hipMemcpyHostToDevice, c->stream);
     if (r == hipSuccess) r = hipMemcpyAsync(c->scal, scal0, sizeof scal0, hipMemcpyHostToDevice, c->stream);
     if (r == hipSuccess) r = hipMemcpyAsync(c->dmp_s, hs.data(), hs.size() * sizeof(double), hipMemcpyHostToDevice, c->stream);
+    // FP32 copies of the tables for the mixed variant's row kernels
+    std::vector<float> flev(c->h_lev.begin(), c->h_lev.end()), fsx(sx.begin(), sx.end()), fsy(sy.begin(), sy.end());
+    auto f2 = [](const std::vector<double2> &t) {
+        std::vector<float2> o(t.size());
+        for (size_t q = 0; q < t.size(); ++q) o[q] = make_float2((float)t[q].x, (float)t[q].y);
+        return o;
+    };
+    std::vector<float2> fnl = f2(tnl), fnx = f2(tnx), fny = f2(tny);
+    if (c->mixed) {
+        View<float> &M = c->m32;
+        if (r == hipSuccess) r = hipMemcpyAsync(M.lev, flev.data(), flev.size() * sizeof(float), hipMemcpyHostToDevice, c->stream);
+        if (r == hipSuccess) r = hipMemcpyAsync(M.symx, fsx.data(), fsx.size() * sizeof(float), hipMemcpyHostToDevice, c->stream);
+        if (r == hipSuccess) r = hipMemcpyAsync(M.symy, fsy.data(), fsy.size() * sizeof(float), hipMemcpyHostToDevice, c->stream);
+        if (r == hipSuccess) r = hipMemcpyAsync(M.tw_nl, fnl.data(), fnl.size() * sizeof(float2), hipMemcpyHostToDevice, c->stream);
+        if (r == hipSuccess) r = hipMemcpyAsync(M.tw_nx, fnx.data(), fnx.size() * sizeof(float2), hipMemcpyHostToDevice, c->stream);
+        if (r == hipSuccess) r = hipMemcpyAsync(M.tw_ny, fny.data(), fny.size() * sizeof(float2), hipMemcpyHostToDevice, c->stream);
+    }
     if (r == hipSuccess) r = hipStreamSynchronize(c->stream);
     if (r != hipSuccess) { hc_destroy(c); return set_err(HC_ERR_HIP, "table upload: %s", hipGetErrorString(r)); }
     *out = c;
@@ -1761,18 +2049,20 @@ extern "C" int hc_destroy(hc_ctx *c)
     return 0;
 }
 
-static double *field_ptr(hc_ctx *c, int f)
+template <class R>
+static R *field_ptr(hc_ctx *c, int f)
 {
+    const View<R> V = view<R>(c);
     switch (f) {
-    case HC_U: return c->u;
-    case HC_V: return c->v;
-    case HC_W: return c->w;
-    case HC_P: return c->p;
-    case HC_T: return c->t;
-    case HC_FF: return c->fb[c->fcur][0];
-    case HC_GG: return c->fb[c->fcur][1];
-    case HC_HH: return c->fb[c->fcur][2];
-    case HC_SS: return c->sb[c->scur];
+    case HC_U: return V.u;
+    case HC_V: return V.v;
+    case HC_W: return V.w;
+    case HC_P: return V.p;
+    case HC_T: return V.t;
+    case HC_FF: return V.fb[c->fcur][0];
+    case HC_GG: return V.fb[c->fcur][1];
+    case HC_HH: return V.fb[c->fcur][2];
+    case HC_SS: return V.sb[c->scur];
     default: return nullptr;
     }
 }
@@ -1785,22 +2075,63 @@ static int ensure_stage(hc_ctx *c)
     return 0;
 }
 
-extern "C" int hc_set_field(hc_ctx *c, int field, const double *src, int layout)
+// Host or device FP64 data -> a field in the stored precision R.
+template <class R>
+static int set_field_t(hc_ctx *c, R *d, const double *src, int layout)
 {
-    if (!c || !src) return set_err(HC_ERR_ARG, "null argument");
-    double *d = field_ptr(c, field);
-    if (!d) return set_err(HC_ERR_ARG, "unknown field %d", field);
     const size_t N = (size_t)c->nx * c->ny * c->nz;
     if (layout == HC_LAYOUT_INTERIOR) {
-        HIPCHK(hipMemcpyAsync(d, src, N * sizeof(double), hipMemcpyDefault, c->stream));
+        if constexpr (std::is_same<R, double>::value) {
+            HIPCHK(hipMemcpyAsync(d, src, N * sizeof(double), hipMemcpyDefault, c->stream));
+        } else {
+            if (int e = ensure_stage(c)) return e;
+            HIPCHK(hipMemcpyAsync(c->stage, src, N * sizeof(double), hipMemcpyDefault, c->stream));
+            hipLaunchKernelGGL((k_convert<double, R>), dim3(1024), dim3(256), 0, c->stream, c->stage, d, N);
+        }
     } else if (layout == HC_LAYOUT_GHOSTED) {
         if (int e = ensure_stage(c)) return e;
         size_t n = (size_t)(c->nx + 4) * (c->ny + 4) * (c->nz + 4);
         HIPCHK(hipMemcpyAsync(c->stage, src, n * sizeof(double), hipMemcpyDefault, c->stream));
-        hipLaunchKernelGGL(k_pack_ghosted, dim3(1024), dim3(256), 0, c->stream, c->stage, d, c->nx, c->ny, c->nz);
+        hipLaunchKernelGGL((k_pack_ghosted<R>), dim3(1024), dim3(256), 0, c->stream, c->stage, d, c->nx, c->ny, c->nz);
     } else {
         return set_err(HC_ERR_ARG, "unknown layout %d", layout);
     }
+    return 0;
+}
+
+// A field in the stored precision R -> host or device FP64 data.
+template <class R>
+static int get_field_t(hc_ctx *c, const R *d, double *dst, int layout)
+{
+    const size_t N = (size_t)c->nx * c->ny * c->nz;
+    if (layout == HC_LAYOUT_INTERIOR) {
+        if constexpr (std::is_same<R, double>::value) {
+            HIPCHK(hipMemcpyAsync(dst, d, N * sizeof(double), hipMemcpyDefault, c->stream));
+        } else {
+            if (int e = ensure_stage(c)) return e;
+            hipLaunchKernelGGL((k_convert<R, double>), dim3(1024), dim3(256), 0, c->stream, d, c->stage, N);
+            HIPCHK(hipMemcpyAsync(dst, c->stage, N * sizeof(double), hipMemcpyDefault, c->stream));
+        }
+    } else if (layout == HC_LAYOUT_GHOSTED) {
+        if (int e = ensure_stage(c)) return e;
+        size_t n = (size_t)(c->nx + 4) * (c->ny + 4) * (c->nz + 4);
+        // keep the caller's ghost cells: bring the whole array, overwrite the interior
+        HIPCHK(hipMemcpyAsync(c->stage, dst, n * sizeof(double), hipMemcpyDefault, c->stream));
+        hipLaunchKernelGGL((k_unpack_ghosted<R>), dim3(1024), dim3(256), 0, c->stream, d, c->stage, c->nx, c->ny, c->nz);
+        HIPCHK(hipMemcpyAsync(dst, c->stage, n * sizeof(double), hipMemcpyDefault, c->stream));
+    } else {
+        return set_err(HC_ERR_ARG, "unknown layout %d", layout);
+    }
+    return 0;
+}
+
+extern "C" int hc_set_field(hc_ctx *c, int field, const double *src, int layout)
+{
+    if (!c || !src) return set_err(HC_ERR_ARG, "null argument");
+    if (field < HC_U || field > HC_SS) return set_err(HC_ERR_ARG, "unknown field %d", field);
+    int e = c->mixed ? set_field_t<float>(c, field_ptr<float>(c, field), src, layout)
+                     : set_field_t<double>(c, field_ptr<double>(c, field), src, layout);
+    if (e) return e;
     // spectral path: the spectra are formed again from the physical fields
     // before the next substep, as at start-up
     c->spec_dirty = true;
@@ -1811,27 +2142,16 @@ extern "C" int hc_set_field(hc_ctx *c, int field, const double *src, int layout)
 extern "C" int hc_get_field(hc_ctx *c, int field, double *dst, int layout)
 {
     if (!c || !dst) return set_err(HC_ERR_ARG, "null argument");
-    double *d = field_ptr(c, field);
-    if (!d) return set_err(HC_ERR_ARG, "unknown field %d", field);
+    if (field < HC_U || field > HC_SS) return set_err(HC_ERR_ARG, "unknown field %d", field);
     if (c->prm.cds == 3 && field == HC_P && !c->spec_dirty) {
         // the spectral path keeps p as a spectrum: cmplx_2_real(c_p, p) (:476)
         const double2 *s1[1] = {c->cp};
         double *p1[1] = {c->p};
         if (int e = spec_inverse(c, 1, s1, p1)) return e;
     }
-    const size_t N = (size_t)c->nx * c->ny * c->nz;
-    if (layout == HC_LAYOUT_INTERIOR) {
-        HIPCHK(hipMemcpyAsync(dst, d, N * sizeof(double), hipMemcpyDefault, c->stream));
-    } else if (layout == HC_LAYOUT_GHOSTED) {
-        if (int e = ensure_stage(c)) return e;
-        size_t n = (size_t)(c->nx + 4) * (c->ny + 4) * (c->nz + 4);
-        // keep the caller's ghost cells: bring the whole array, overwrite the interior
-        HIPCHK(hipMemcpyAsync(c->stage, dst, n * sizeof(double), hipMemcpyDefault, c->stream));
-        hipLaunchKernelGGL(k_unpack_ghosted, dim3(1024), dim3(256), 0, c->stream, d, c->stage, c->nx, c->ny, c->nz);
-        HIPCHK(hipMemcpyAsync(dst, c->stage, n * sizeof(double), hipMemcpyDefault, c->stream));
-    } else {
-        return set_err(HC_ERR_ARG, "unknown layout %d", layout);
-    }
+    int e = c->mixed ? get_field_t<float>(c, field_ptr<float>(c, field), dst, layout)
+                     : get_field_t<double>(c, field_ptr<double>(c, field), dst, layout);
+    if (e) return e;
     HIPCHK(hipStreamSynchronize(c->stream));
     return 0;
 }
@@ -1843,8 +2163,12 @@ extern "C" int hc_init(hc_ctx *c)
     if (P.is_ri_var == 1) c->got = P.ri_str;  // update_ri(0)
     // update_boundary_uvw forces w = 0 on the top face (src/module_boundary.f90:107)
     size_t plane = (size_t)c->nx * c->ny;
-    hipLaunchKernelGGL(k_zero_level, dim3((unsigned)((plane + 255) / 256)), dim3(256), 0, c->stream, c->w,
-                       (size_t)(c->nz - 1) * plane, plane);
+    if (c->mixed)
+        hipLaunchKernelGGL(k_zero_level<float>, dim3((unsigned)((plane + 255) / 256)), dim3(256), 0, c->stream,
+                           c->m32.w, (size_t)(c->nz - 1) * plane, plane);
+    else
+        hipLaunchKernelGGL(k_zero_level<double>, dim3((unsigned)((plane + 255) / 256)), dim3(256), 0, c->stream, c->w,
+                           (size_t)(c->nz - 1) * plane, plane);
     if (P.dp_opt == 2)
         if (int e = ubar_launch(c)) return e;
     if (P.cds == 3)
@@ -1898,69 +2222,68 @@ extern "C" int hc_drive_steps(hc_ctx *c, int nsteps)
     return 0;
 }
 
-extern "C" int hc_poisson_solver_fft(hc_ctx *c, int equation, double rk_c, const double *r_src, double *r_var, int layout)
+// poisson_solver_fft over the stored precision R.  src/dst: the caller's
+// FP64 arrays (device interior arrays are used in place when R is double).
+template <class R>
+static int poisson_t(hc_ctx *c, int equation, double rk_c, const double *r_src, double *r_var, int layout)
 {
-    if (!c || !r_src || !r_var) return set_err(HC_ERR_ARG, "null argument");
-    if (equation < HC_EQ_U || equation > HC_EQ_T) return set_err(HC_ERR_ARG, "unknown equation %d", equation);
-    if (layout != HC_LAYOUT_INTERIOR && layout != HC_LAYOUT_GHOSTED) return set_err(HC_ERR_ARG, "unknown layout %d", layout);
     const size_t N = (size_t)c->nx * c->ny * c->nz;
+    const View<R> V = view<R>(c);
     // the physical work buffer: the ss ping-pong slot not holding ss
-    double *work = c->sb[c->scur ^ 1];
+    R *work = V.sb[c->scur ^ 1];
     bool dev_io = false;
     hipPointerAttribute_t at;
     if (layout == HC_LAYOUT_INTERIOR && hipPointerGetAttributes(&at, r_src) == hipSuccess &&
         at.type == hipMemoryTypeDevice)
         dev_io = true;
     (void)hipGetLastError();
-    const double *src = r_src;
-    if (!dev_io) {
-        if (layout == HC_LAYOUT_INTERIOR) {
-            HIPCHK(hipMemcpyAsync(work, r_src, N * sizeof(double), hipMemcpyDefault, c->stream));
-        } else {
-            if (int e = ensure_stage(c)) return e;
-            size_t n = (size_t)(c->nx + 4) * (c->ny + 4) * (c->nz + 4);
-            HIPCHK(hipMemcpyAsync(c->stage, r_src, n * sizeof(double), hipMemcpyDefault, c->stream));
-            hipLaunchKernelGGL(k_pack_ghosted, dim3(1024), dim3(256), 0, c->stream, c->stage, work, c->nx, c->ny, c->nz);
-        }
-        src = work;
+    const R *src = work;
+    if (dev_io && std::is_same<R, double>::value) {
+        src = reinterpret_cast<const R *>(r_src);
+    } else if (dev_io) {
+        hipLaunchKernelGGL((k_convert<double, R>), dim3(1024), dim3(256), 0, c->stream, r_src, work, N);
+    } else {
+        if (int e = set_field_t<R>(c, work, r_src, layout)) return e;
     }
     PROF_BEGIN(c, HC_K_SOLVE_XR2C);
-    if (int e = launch_r2c(c, src, c->s[0])) return e;
+    if (int e = launch_r2c<R>(c, src, V.s[0])) return e;
     PROF_END(c, HC_K_SOLVE_XR2C);
     PROF_BEGIN(c, HC_K_SOLVE_YFFT);
-    if (int e = launch_yfft(c, -1, 1, c->s[0], nullptr, nullptr, nullptr)) return e;
+    if (int e = launch_yfft(c, -1, 1, V.s[0], nullptr, nullptr, nullptr)) return e;
     PROF_END(c, HC_K_SOLVE_YFFT);
     StepArgs A = base_args(c);
     A.rc = c->prm.dts == 2 ? rk_c : 1.0;
-    dim3 gcol((c->nl + kColWaves - 1) / kColWaves, c->ny);
     PROF_BEGIN(c, HC_K_SOLVE_COLUMN);
-#define HC_CS(L_) hipLaunchKernelGGL((k_column_solve<L_>), gcol, dim3(64 * kColWaves), col_lds(c), c->stream, A, equation);
-    HC_DISPATCH_L(c->Lcol, HC_CS)
-#undef HC_CS
+    if (int e = launch_columns<R>(c, 3, A, V, 1, equation)) return e;
     PROF_END(c, HC_K_SOLVE_COLUMN);
     PROF_BEGIN(c, HC_K_SOLVE_YFFT);
-    if (int e = launch_yfft(c, +1, 1, c->s[0], nullptr, nullptr, nullptr)) return e;
+    if (int e = launch_yfft(c, +1, 1, V.s[0], nullptr, nullptr, nullptr)) return e;
     PROF_END(c, HC_K_SOLVE_YFFT);
-    C2RArgs X;
+    C2RArgsT<R> X;
     std::memset(&X, 0, sizeof X);
-    X.src[0] = c->s[0];
-    X.dst[0] = dev_io ? r_var : work;
+    X.src[0] = V.s[0];
+    const bool direct = dev_io && std::is_same<R, double>::value;
+    X.dst[0] = direct ? reinterpret_cast<R *>(r_var) : work;
     PROF_BEGIN(c, HC_K_SOLVE_XC2R);
-    if (int e = launch_c2r(c, 1, X)) return e;
+    if (int e = launch_c2r<R>(c, 1, X)) return e;
     PROF_END(c, HC_K_SOLVE_XC2R);
-    if (!dev_io) {
-        if (layout == HC_LAYOUT_INTERIOR) {
-            HIPCHK(hipMemcpyAsync(r_var, work, N * sizeof(double), hipMemcpyDefault, c->stream));
-        } else {
-            size_t n = (size_t)(c->nx + 4) * (c->ny + 4) * (c->nz + 4);
-            HIPCHK(hipMemcpyAsync(c->stage, r_var, n * sizeof(double), hipMemcpyDefault, c->stream));
-            hipLaunchKernelGGL(k_unpack_ghosted, dim3(1024), dim3(256), 0, c->stream, work, c->stage, c->nx, c->ny, c->nz);
-            HIPCHK(hipMemcpyAsync(r_var, c->stage, n * sizeof(double), hipMemcpyDefault, c->stream));
-        }
+    if (dev_io && !direct) {
+        hipLaunchKernelGGL((k_convert<R, double>), dim3(1024), dim3(256), 0, c->stream, work, r_var, N);
+    } else if (!dev_io) {
+        if (int e = get_field_t<R>(c, work, r_var, layout)) return e;
         HIPCHK(hipStreamSynchronize(c->stream));
     }
     HIPCHK(hipGetLastError());
     return 0;
+}
+
+extern "C" int hc_poisson_solver_fft(hc_ctx *c, int equation, double rk_c, const double *r_src, double *r_var, int layout)
+{
+    if (!c || !r_src || !r_var) return set_err(HC_ERR_ARG, "null argument");
+    if (equation < HC_EQ_U || equation > HC_EQ_T) return set_err(HC_ERR_ARG, "unknown equation %d", equation);
+    if (layout != HC_LAYOUT_INTERIOR && layout != HC_LAYOUT_GHOSTED) return set_err(HC_ERR_ARG, "unknown layout %d", layout);
+    return c->mixed ? poisson_t<float>(c, equation, rk_c, r_src, r_var, layout)
+                    : poisson_t<double>(c, equation, rk_c, r_src, r_var, layout);
 }
 
 extern "C" int hc_calculate_ubar(hc_ctx *c, double *ubar)
@@ -1986,11 +2309,17 @@ extern "C" int hc_screen_div_error(hc_ctx *c, double *div_avg, double *div_max)
         double *p1[1] = {c->ptmp};
         if (int e = spec_inverse(c, 1, s1, p1)) return e;
         A.aux = c->ptmp;
-        hipLaunchKernelGGL((k_div_partial<3>), dim3(c->nz), dim3(256), 0, c->stream, A, c->part);
+        hipLaunchKernelGGL((k_div_partial<3, double>), dim3(c->nz), dim3(256), 0, c->stream, A, A, c->part);
+    } else if (c->mixed) {
+        StepArgsT<float> F = rows_args<float>(A, c->m32);
+        if (c->prm.cds == 1)
+            hipLaunchKernelGGL((k_div_partial<1, float>), dim3(c->nz), dim3(256), 0, c->stream, F, A, c->part);
+        else
+            hipLaunchKernelGGL((k_div_partial<2, float>), dim3(c->nz), dim3(256), 0, c->stream, F, A, c->part);
     } else if (c->prm.cds == 1) {
-        hipLaunchKernelGGL((k_div_partial<1>), dim3(c->nz), dim3(256), 0, c->stream, A, c->part);
+        hipLaunchKernelGGL((k_div_partial<1, double>), dim3(c->nz), dim3(256), 0, c->stream, A, A, c->part);
     } else {
-        hipLaunchKernelGGL((k_div_partial<2>), dim3(c->nz), dim3(256), 0, c->stream, A, c->part);
+        hipLaunchKernelGGL((k_div_partial<2, double>), dim3(c->nz), dim3(256), 0, c->stream, A, A, c->part);
     }
     std::vector<double> h(3 * (size_t)c->nz);
     HIPCHK(hipMemcpyAsync(h.data(), c->part, h.size() * sizeof(double), hipMemcpyDeviceToHost, c->stream));
@@ -2012,8 +2341,15 @@ extern "C" int hc_wall_shear(hc_ctx *c, double *tau_bottom, double *tau_top)
 {
     if (!c) return set_err(HC_ERR_ARG, "null context");
     const hc_params &P = c->prm;
-    hipLaunchKernelGGL(k_plane_sum, dim3(1), dim3(256), 0, c->stream, c->u, c->nx, c->ny, 0, c->part);
-    hipLaunchKernelGGL(k_plane_sum, dim3(1), dim3(256), 0, c->stream, c->u, c->nx, c->ny, c->nz - 1, c->part + 1);
+    if (c->mixed) {
+        hipLaunchKernelGGL(k_plane_sum<float>, dim3(1), dim3(256), 0, c->stream, c->m32.u, c->nx, c->ny, 0, c->part);
+        hipLaunchKernelGGL(k_plane_sum<float>, dim3(1), dim3(256), 0, c->stream, c->m32.u, c->nx, c->ny, c->nz - 1,
+                           c->part + 1);
+    } else {
+        hipLaunchKernelGGL(k_plane_sum<double>, dim3(1), dim3(256), 0, c->stream, c->u, c->nx, c->ny, 0, c->part);
+        hipLaunchKernelGGL(k_plane_sum<double>, dim3(1), dim3(256), 0, c->stream, c->u, c->nx, c->ny, c->nz - 1,
+                           c->part + 1);
+    }
     double h[2];
     HIPCHK(hipMemcpyAsync(h, c->part, sizeof h, hipMemcpyDeviceToHost, c->stream));
     HIPCHK(hipStreamSynchronize(c->stream));
@@ -2038,13 +2374,17 @@ extern "C" int hc_get_scalar(hc_ctx *c, int id, double *value)
     case 2: *value = c->time_sim; return 0;
     case 3: *value = c->got; return 0;
     case 4: *value = c->stps; return 0;
-    case 5: *value = col_tiles(c, 4); return 0;
+    case 5: *value = col_tiles(c, 4, c->mixed ? sizeof(float2) : sizeof(double2)); return 0;
     default: return set_err(HC_ERR_ARG, "unknown scalar %d", id);
     }
 }
 
 extern "C" void *hc_stream(hc_ctx *c) { return c ? (void *)c->stream : nullptr; }
-extern "C" double *hc_field_device_ptr(hc_ctx *c, int field) { return c ? field_ptr(c, field) : nullptr; }
+extern "C" void *hc_field_device_ptr(hc_ctx *c, int field)
+{
+    if (!c) return nullptr;
+    return c->mixed ? (void *)field_ptr<float>(c, field) : (void *)field_ptr<double>(c, field);
+}
 
 extern "C" int hc_synchronize(hc_ctx *c)
 {
@@ -2082,28 +2422,32 @@ extern "C" int hc_profile_read(hc_ctx *c, int kernel, double *total_ms, long *la
     return 0;
 }
 
+template <class R>
+static int project_t(hc_ctx *c)
+{
+    const View<R> V = view<R>(c);
+    size_t plane = (size_t)c->nx * c->ny;
+    hipLaunchKernelGGL(k_zero_level<R>, dim3((unsigned)((plane + 255) / 256)), dim3(256), 0, c->stream, V.w,
+                       (size_t)(c->nz - 1) * plane, plane);
+    if (int e = launch_r2c<R>(c, V.u, V.s[0])) return e;
+    if (int e = launch_r2c<R>(c, V.v, V.s[1])) return e;
+    if (int e = launch_r2c<R>(c, V.w, V.s[2])) return e;
+    if (int e = launch_yfft(c, -1, 3, V.s[0], V.s[1], V.s[2], nullptr)) return e;
+    StepArgs A = base_args(c);
+    A.rc = 1.0;
+    if (int e = launch_columns<R>(c, 2, A, V, 1)) return e;
+    if (int e = launch_yfft(c, +1, 3, V.s[0], V.s[1], V.s[2], nullptr)) return e;
+    C2RArgsT<R> X;
+    std::memset(&X, 0, sizeof X);
+    X.src[0] = V.s[0]; X.src[1] = V.s[1]; X.src[2] = V.s[2];
+    X.dst[0] = V.u; X.dst[1] = V.v; X.dst[2] = V.w;
+    return launch_c2r<R>(c, 3, X);
+}
+
 extern "C" int hc_project(hc_ctx *c)
 {
     if (!c) return set_err(HC_ERR_ARG, "null context");
-    size_t plane = (size_t)c->nx * c->ny;
-    hipLaunchKernelGGL(k_zero_level, dim3((unsigned)((plane + 255) / 256)), dim3(256), 0, c->stream, c->w,
-                       (size_t)(c->nz - 1) * plane, plane);
-    if (int e = launch_r2c(c, c->u, c->s[0])) return e;
-    if (int e = launch_r2c(c, c->v, c->s[1])) return e;
-    if (int e = launch_r2c(c, c->w, c->s[2])) return e;
-    if (int e = launch_yfft(c, -1, 3, c->s[0], c->s[1], c->s[2], nullptr)) return e;
-    StepArgs A = base_args(c);
-    A.rc = 1.0;
-    dim3 gcol((c->nl + kColWaves - 1) / kColWaves, c->ny);
-#define HC_PJ(L_) hipLaunchKernelGGL((k_column_project<L_>), gcol, dim3(64 * kColWaves), col_lds(c), c->stream, A);
-    HC_DISPATCH_L(c->Lcol, HC_PJ)
-#undef HC_PJ
-    if (int e = launch_yfft(c, +1, 3, c->s[0], c->s[1], c->s[2], nullptr)) return e;
-    C2RArgs X;
-    std::memset(&X, 0, sizeof X);
-    X.src[0] = c->s[0]; X.src[1] = c->s[1]; X.src[2] = c->s[2];
-    X.dst[0] = c->u; X.dst[1] = c->v; X.dst[2] = c->w;
-    if (int e = launch_c2r(c, 3, X)) return e;
+    if (int e = c->mixed ? project_t<float>(c) : project_t<double>(c)) return e;
     c->spec_dirty = true;
     HIPCHK(hipGetLastError());
     HIPCHK(hipStreamSynchronize(c->stream));

@@ -8,14 +8,34 @@
 
 #define HC_DEV __device__ __forceinline__
 
-HC_DEV double2 cadd(double2 a, double2 b) { return make_double2(a.x + b.x, a.y + b.y); }
-HC_DEV double2 csub(double2 a, double2 b) { return make_double2(a.x - b.x, a.y - b.y); }
-HC_DEV double2 cmul(double2 a, double2 b) { return make_double2(a.x * b.x - a.y * b.y, a.x * b.y + a.y * b.x); }
-HC_DEV double2 cscale(double2 a, double s) { return make_double2(a.x * s, a.y * s); }
-HC_DEV double2 cconj(double2 a) { return make_double2(a.x, -a.y); }
+// Complex arithmetic over the HIP vector types double2 and float2.
+template <class R> struct CplxOf;
+template <> struct CplxOf<double> { using type = double2; };
+template <> struct CplxOf<float> { using type = float2; };
+template <class R> using cplx = typename CplxOf<R>::type;
+template <class C> struct RealOf;
+template <> struct RealOf<double2> { using type = double; };
+template <> struct RealOf<float2> { using type = float; };
+template <class C> using real_of = typename RealOf<C>::type;
+
+template <class C> HC_DEV C mk2(real_of<C> x, real_of<C> y)
+{
+    C r;
+    r.x = x;
+    r.y = y;
+    return r;
+}
+template <class C> HC_DEV C cadd(C a, C b) { return mk2<C>(a.x + b.x, a.y + b.y); }
+template <class C> HC_DEV C csub(C a, C b) { return mk2<C>(a.x - b.x, a.y - b.y); }
+template <class C> HC_DEV C cmul(C a, C b) { return mk2<C>(a.x * b.x - a.y * b.y, a.x * b.y + a.y * b.x); }
+template <class C> HC_DEV C cscale(C a, real_of<C> s) { return mk2<C>(a.x * s, a.y * s); }
+template <class C> HC_DEV C cconj(C a) { return mk2<C>(a.x, -a.y); }
+template <class C> HC_DEV C czero_t() { return mk2<C>(real_of<C>(0), real_of<C>(0)); }
 HC_DEV double2 czero() { return make_double2(0.0, 0.0); }
 // a - s*b for real s
-HC_DEV double2 caxpy_neg(double2 a, double s, double2 b) { return make_double2(a.x - s * b.x, a.y - s * b.y); }
+template <class C> HC_DEV C caxpy_neg(C a, real_of<C> s, C b) { return mk2<C>(a.x - s * b.x, a.y - s * b.y); }
+// complex value converted to another precision
+template <class C, class S> HC_DEV C ccast(S a) { return mk2<C>(real_of<C>(a.x), real_of<C>(a.y)); }
 
 // 1/x for finite, normal, nonzero x: the hardware reciprocal estimate
 // (v_rcp_f64) refined by two Newton-Raphson steps r <- r + r(1 - x r),
@@ -29,6 +49,15 @@ HC_DEV double rcp64(double x)
     e = __builtin_fma(-x, r, 1.0);
     return __builtin_fma(r, e, r);
 }
+// FP32: v_rcp_f32 (1 ulp) and one Newton-Raphson step.
+HC_DEV float rcp32(float x)
+{
+    float r = __builtin_amdgcn_rcpf(x);
+    float e = __builtin_fmaf(-x, r, 1.0f);
+    return __builtin_fmaf(r, e, r);
+}
+HC_DEV double rcpr(double x) { return rcp64(x); }
+HC_DEV float rcpr(float x) { return rcp32(x); }
 
 // ---------------------------------------------------------------------------
 // FFT plans: N = 4^a 2^b 3^c, stages in that order (radix 4 first).
@@ -56,28 +85,30 @@ constexpr int fft_ns(int n, int st)  // product of the radices before stage st
 }
 
 // In-register R-point DFT, sign SIGN (-1 forward, +1 backward), unnormalised.
-template <int R, int SIGN>
-HC_DEV void dft_small(double2 *v)
+template <int R, int SIGN, class C>
+HC_DEV void dft_small(C *v)
 {
+    using Re = real_of<C>;
     if constexpr (R == 2) {
-        double2 a = v[0], b = v[1];
+        C a = v[0], b = v[1];
         v[0] = cadd(a, b);
         v[1] = csub(a, b);
     } else if constexpr (R == 4) {
-        double2 a0 = cadd(v[0], v[2]), a1 = csub(v[0], v[2]);
-        double2 a2 = cadd(v[1], v[3]), d = csub(v[1], v[3]);
+        C a0 = cadd(v[0], v[2]), a1 = csub(v[0], v[2]);
+        C a2 = cadd(v[1], v[3]), d = csub(v[1], v[3]);
         // a3 = (SIGN i) * d
-        double2 a3 = SIGN < 0 ? make_double2(d.y, -d.x) : make_double2(-d.y, d.x);
+        C a3 = SIGN < 0 ? mk2<C>(d.y, -d.x) : mk2<C>(-d.y, d.x);
         v[0] = cadd(a0, a2);
         v[2] = csub(a0, a2);
         v[1] = cadd(a1, a3);
         v[3] = csub(a1, a3);
     } else if constexpr (R == 3) {
-        const double h = 0.86602540378443864676;  // sqrt(3)/2
-        double2 t1 = cadd(v[1], v[2]), t2 = csub(v[1], v[2]);
-        double2 m = make_double2(v[0].x - 0.5 * t1.x, v[0].y - 0.5 * t1.y);
+        const Re h = Re(0.86602540378443864676);  // sqrt(3)/2
+        const Re half = Re(0.5);
+        C t1 = cadd(v[1], v[2]), t2 = csub(v[1], v[2]);
+        C m = mk2<C>(v[0].x - half * t1.x, v[0].y - half * t1.y);
         // s = (SIGN i) * h * t2
-        double2 s = SIGN < 0 ? make_double2(h * t2.y, -h * t2.x) : make_double2(-h * t2.y, h * t2.x);
+        C s = SIGN < 0 ? mk2<C>(h * t2.y, -h * t2.x) : mk2<C>(-h * t2.y, h * t2.x);
         v[0] = cadd(v[0], t1);
         v[1] = cadd(m, s);
         v[2] = csub(m, s);
@@ -87,9 +118,9 @@ HC_DEV void dft_small(double2 *v)
 // One Stockham stage over B transforms of length N held in LDS, element
 // (b, n) at lds[b*BS + n*ES].  All loads of the stage happen before the
 // barrier and all stores after it, so one buffer suffices.  tw[e] =
-// exp(-2 pi i e / N).
-template <int N, int B, int BS, int ES, int T, int SIGN, int ST>
-HC_DEV void fft_stages(double2 *lds, const double2 *__restrict__ tw, int tid)
+// exp(-2 pi i e / N).  C is double2 or float2.
+template <int N, int B, int BS, int ES, int T, int SIGN, int ST, class C>
+HC_DEV void fft_stages(C *lds, const C *__restrict__ tw, int tid)
 {
     if constexpr (ST < fft_nstages(N)) {
         constexpr int R = fft_radix(N, ST);
@@ -97,7 +128,7 @@ HC_DEV void fft_stages(double2 *lds, const double2 *__restrict__ tw, int tid)
         constexpr int M = N / R;
         constexpr int NB = B * M;
         constexpr int PER = (NB + T - 1) / T;
-        double2 v[PER][R];
+        C v[PER][R];
 #pragma unroll
         for (int q = 0; q < PER; ++q) {
             int bf = tid + q * T;
@@ -117,7 +148,7 @@ HC_DEV void fft_stages(double2 *lds, const double2 *__restrict__ tw, int tid)
                 if constexpr (NS > 1) {
 #pragma unroll
                     for (int r = 1; r < R; ++r) {
-                        double2 w = tw[r * e * (N / (NS * R))];
+                        C w = tw[r * e * (N / (NS * R))];
                         if (SIGN > 0) w.y = -w.y;
                         v[q][r] = cmul(v[q][r], w);
                     }
@@ -133,8 +164,8 @@ HC_DEV void fft_stages(double2 *lds, const double2 *__restrict__ tw, int tid)
     }
 }
 
-template <int N, int B, int BS, int ES, int T, int SIGN>
-HC_DEV void lds_fft(double2 *lds, const double2 *__restrict__ tw, int tid)
+template <int N, int B, int BS, int ES, int T, int SIGN, class C>
+HC_DEV void lds_fft(C *lds, const C *__restrict__ tw, int tid)
 {
     static_assert(fft_nstages(N) > 0, "FFT length must be 2^a 3^b");
     fft_stages<N, B, BS, ES, T, SIGN, 0>(lds, tw, tid);
@@ -152,31 +183,35 @@ HC_DEV void lds_fft(double2 *lds, const double2 *__restrict__ tw, int tid)
 // solved by parallel cyclic reduction through lane shuffles, after which the
 // interior is recovered.  Requires a_0 = 0 and c_{n-1} = 0.  NR complex
 // right-hand sides share the matrix; d is overwritten by the solution.
-HC_DEV double shfl_up1(double v, int d) { return __shfl_up(v, d, 64); }
-HC_DEV double shfl_dn1(double v, int d) { return __shfl_down(v, d, 64); }
+// Re (double or float) is the arithmetic of the whole solve.
+template <class Re> HC_DEV Re shfl_up1(Re v, int d) { return __shfl_up(v, d, 64); }
+template <class Re> HC_DEV Re shfl_dn1(Re v, int d) { return __shfl_down(v, d, 64); }
+template <class C> HC_DEV C shfl_up_c(C v, int d) { return mk2<C>(__shfl_up(v.x, d, 64), __shfl_up(v.y, d, 64)); }
+template <class C> HC_DEV C shfl_dn_c(C v, int d) { return mk2<C>(__shfl_down(v.x, d, 64), __shfl_down(v.y, d, 64)); }
 
-template <int L, int NR>
-HC_DEV void wave_tridiag(int cnt, const double (&a)[L], const double (&b)[L], const double (&c)[L],
-                         double2 (&d)[NR][L])
+template <int L, int NR, class Re>
+HC_DEV void wave_tridiag(int cnt, const Re (&a)[L], const Re (&b)[L], const Re (&c)[L], cplx<Re> (&d)[NR][L])
 {
+    using C = cplx<Re>;
+    const Re zero = Re(0), one = Re(1);
     const int lane = threadIdx.x & 63;
-    double cp[L], g[L], h[L];
+    Re cp[L], g[L], h[L];
     // forward sweep over the interior levels 1..cnt-1
 #pragma unroll
-    for (int r = 0; r < L; ++r) { cp[r] = 0.0; g[r] = 0.0; h[r] = 0.0; }
+    for (int r = 0; r < L; ++r) { cp[r] = zero; g[r] = zero; h[r] = zero; }
 #pragma unroll
     for (int r = 1; r < L; ++r) {
         if (r < cnt) {
-            double den = b[r] - (r > 1 ? a[r] * cp[r - 1] : 0.0);
-            double iv = rcp64(den);
+            Re den = b[r] - (r > 1 ? a[r] * cp[r - 1] : zero);
+            Re iv = rcpr(den);
             cp[r] = c[r] * iv;
-            double gr = r == 1 ? -a[1] : -a[r] * g[r - 1];
+            Re gr = r == 1 ? -a[1] : -a[r] * g[r - 1];
             g[r] = gr * iv;
-            double hr = (r == cnt - 1 ? -c[r] : 0.0) - (r > 1 ? a[r] * h[r - 1] : 0.0);
+            Re hr = (r == cnt - 1 ? -c[r] : zero) - (r > 1 ? a[r] * h[r - 1] : zero);
             h[r] = hr * iv;
 #pragma unroll
             for (int q = 0; q < NR; ++q) {
-                double2 y = d[q][r];
+                C y = d[q][r];
                 if (r > 1) y = caxpy_neg(y, a[r], d[q][r - 1]);
                 d[q][r] = cscale(y, iv);
             }
@@ -193,10 +228,10 @@ HC_DEV void wave_tridiag(int cnt, const double (&a)[L], const double (&b)[L], co
         }
     }
     // interface quantities: first interior (y1,g1,h1) and last level (yl,gl,hl)
-    double g1 = 0.0, h1 = 1.0, gl = 1.0, hl = 0.0;
-    double2 y1[NR], yl[NR];
+    Re g1 = zero, h1 = one, gl = one, hl = zero;
+    C y1[NR], yl[NR];
 #pragma unroll
-    for (int q = 0; q < NR; ++q) { y1[q] = czero(); yl[q] = czero(); }
+    for (int q = 0; q < NR; ++q) { y1[q] = czero_t<C>(); yl[q] = czero_t<C>(); }
     if (cnt >= 2) {
         g1 = g[1]; h1 = h[1];
 #pragma unroll
@@ -210,70 +245,69 @@ HC_DEV void wave_tridiag(int cnt, const double (&a)[L], const double (&b)[L], co
             }
     }
     // values of the previous lane's last level
-    double Gp = shfl_up1(gl, 1), Hp = shfl_up1(hl, 1);
-    double2 Yp[NR];
+    Re Gp = shfl_up1(gl, 1), Hp = shfl_up1(hl, 1);
+    C Yp[NR];
 #pragma unroll
-    for (int q = 0; q < NR; ++q) { Yp[q].x = shfl_up1(yl[q].x, 1); Yp[q].y = shfl_up1(yl[q].y, 1); }
+    for (int q = 0; q < NR; ++q) Yp[q] = shfl_up_c(yl[q], 1);
     if (lane == 0) {
-        Gp = 0.0; Hp = 0.0;
+        Gp = zero; Hp = zero;
 #pragma unroll
-        for (int q = 0; q < NR; ++q) Yp[q] = czero();
+        for (int q = 0; q < NR; ++q) Yp[q] = czero_t<C>();
     }
-    double al, be, ga;
-    double2 de[NR];
+    Re al, be, ga;
+    C de[NR];
     if (cnt >= 1) {
         al = a[0] * Gp;
         be = b[0] + a[0] * Hp + c[0] * g1;
         ga = c[0] * h1;
 #pragma unroll
         for (int q = 0; q < NR; ++q) {
-            double2 t = caxpy_neg(d[q][0], a[0], Yp[q]);
+            C t = caxpy_neg(d[q][0], a[0], Yp[q]);
             de[q] = caxpy_neg(t, c[0], y1[q]);
         }
     } else {
-        al = 0.0; be = 1.0; ga = 0.0;
+        al = zero; be = one; ga = zero;
 #pragma unroll
-        for (int q = 0; q < NR; ++q) de[q] = czero();
+        for (int q = 0; q < NR; ++q) de[q] = czero_t<C>();
     }
     // parallel cyclic reduction over the 64 lanes
 #pragma unroll
     for (int st = 1; st < 64; st <<= 1) {
-        double al_lo = shfl_up1(al, st), be_lo = shfl_up1(be, st), ga_lo = shfl_up1(ga, st);
-        double al_hi = shfl_dn1(al, st), be_hi = shfl_dn1(be, st), ga_hi = shfl_dn1(ga, st);
-        double2 de_lo[NR], de_hi[NR];
+        Re al_lo = shfl_up1(al, st), be_lo = shfl_up1(be, st), ga_lo = shfl_up1(ga, st);
+        Re al_hi = shfl_dn1(al, st), be_hi = shfl_dn1(be, st), ga_hi = shfl_dn1(ga, st);
+        C de_lo[NR], de_hi[NR];
 #pragma unroll
         for (int q = 0; q < NR; ++q) {
-            de_lo[q].x = shfl_up1(de[q].x, st); de_lo[q].y = shfl_up1(de[q].y, st);
-            de_hi[q].x = shfl_dn1(de[q].x, st); de_hi[q].y = shfl_dn1(de[q].y, st);
+            de_lo[q] = shfl_up_c(de[q], st);
+            de_hi[q] = shfl_dn_c(de[q], st);
         }
         if (lane < st) {
-            al_lo = 0.0; be_lo = 1.0; ga_lo = 0.0;
+            al_lo = zero; be_lo = one; ga_lo = zero;
 #pragma unroll
-            for (int q = 0; q < NR; ++q) de_lo[q] = czero();
+            for (int q = 0; q < NR; ++q) de_lo[q] = czero_t<C>();
         }
         if (lane + st >= 64) {
-            al_hi = 0.0; be_hi = 1.0; ga_hi = 0.0;
+            al_hi = zero; be_hi = one; ga_hi = zero;
 #pragma unroll
-            for (int q = 0; q < NR; ++q) de_hi[q] = czero();
+            for (int q = 0; q < NR; ++q) de_hi[q] = czero_t<C>();
         }
-        double k1 = al * rcp64(be_lo), k2 = ga * rcp64(be_hi);
-        double nal = -al_lo * k1, nga = -ga_hi * k2;
-        double nbe = be - ga_lo * k1 - al_hi * k2;
+        Re k1 = al * rcpr(be_lo), k2 = ga * rcpr(be_hi);
+        Re nal = -al_lo * k1, nga = -ga_hi * k2;
+        Re nbe = be - ga_lo * k1 - al_hi * k2;
 #pragma unroll
         for (int q = 0; q < NR; ++q) {
-            double2 t = caxpy_neg(de[q], k1, de_lo[q]);
+            C t = caxpy_neg(de[q], k1, de_lo[q]);
             de[q] = caxpy_neg(t, k2, de_hi[q]);
         }
         al = nal; be = nbe; ga = nga;
     }
-    double ib = rcp64(be);
-    double2 X[NR], Xn[NR];
+    Re ib = rcpr(be);
+    C X[NR], Xn[NR];
 #pragma unroll
     for (int q = 0; q < NR; ++q) {
         X[q] = cscale(de[q], ib);
-        Xn[q].x = shfl_dn1(X[q].x, 1);
-        Xn[q].y = shfl_dn1(X[q].y, 1);
-        if (lane == 63) Xn[q] = czero();
+        Xn[q] = shfl_dn_c(X[q], 1);
+        if (lane == 63) Xn[q] = czero_t<C>();
     }
     // recover the interior
 #pragma unroll
@@ -281,7 +315,7 @@ HC_DEV void wave_tridiag(int cnt, const double (&a)[L], const double (&b)[L], co
         if (r < cnt) {
 #pragma unroll
             for (int q = 0; q < NR; ++q) {
-                double2 y = d[q][r];
+                C y = d[q][r];
                 y.x += g[r] * X[q].x + h[r] * Xn[q].x;
                 y.y += g[r] * X[q].y + h[r] * Xn[q].y;
                 d[q][r] = y;
@@ -298,38 +332,38 @@ HC_DEV void wave_tridiag(int cnt, const double (&a)[L], const double (&b)[L], co
 // is 1e3-1e4 times that of a sequential Thomas sweep (its forward error is
 // comparable); one refinement step brings the residual back to the Thomas
 // level, which is what the divergence of the projected velocity measures.
-template <int L, int NR>
-HC_DEV void wave_tridiag_refined(int cnt, const double (&a)[L], const double (&b)[L], const double (&c)[L],
-                                 double2 (&d)[NR][L])
+template <int L, int NR, class Re>
+HC_DEV void wave_tridiag_refined(int cnt, const Re (&a)[L], const Re (&b)[L], const Re (&c)[L], cplx<Re> (&d)[NR][L])
 {
+    using C = cplx<Re>;
     const int lane = threadIdx.x & 63;
-    double2 d0[NR][L];
+    C d0[NR][L];
 #pragma unroll
     for (int q = 0; q < NR; ++q)
 #pragma unroll
         for (int r = 0; r < L; ++r) d0[q][r] = d[q][r];
     wave_tridiag<L, NR>(cnt, a, b, c, d);
-    double2 res[NR][L];
+    C res[NR][L];
 #pragma unroll
     for (int q = 0; q < NR; ++q) {
-        double2 last = czero();
+        C last = czero_t<C>();
 #pragma unroll
         for (int r = 0; r < L; ++r)
             if (r == cnt - 1) last = d[q][r];
-        double2 xprev = make_double2(__shfl_up(last.x, 1, 64), __shfl_up(last.y, 1, 64));
-        double2 xnext = make_double2(__shfl_down(d[q][0].x, 1, 64), __shfl_down(d[q][0].y, 1, 64));
-        if (lane == 0) xprev = czero();
-        if (lane == 63) xnext = czero();
+        C xprev = shfl_up_c(last, 1);
+        C xnext = shfl_dn_c(d[q][0], 1);
+        if (lane == 0) xprev = czero_t<C>();
+        if (lane == 63) xnext = czero_t<C>();
 #pragma unroll
         for (int r = 0; r < L; ++r) {
-            res[q][r] = czero();
+            res[q][r] = czero_t<C>();
             if (r < cnt) {
-                double2 xm = r == 0 ? xprev : d[q][r > 0 ? r - 1 : 0];
-                double2 xp = xnext;
+                C xm = r == 0 ? xprev : d[q][r > 0 ? r - 1 : 0];
+                C xp = xnext;
 #pragma unroll
                 for (int s = 0; s < L; ++s)
                     if (s == r + 1 && s < cnt) xp = d[q][s];
-                double2 x0 = d[q][r];
+                C x0 = d[q][r];
                 res[q][r].x = d0[q][r].x - (a[r] * xm.x + b[r] * x0.x + c[r] * xp.x);
                 res[q][r].y = d0[q][r].y - (a[r] * xm.y + b[r] * x0.y + c[r] * xp.y);
             }

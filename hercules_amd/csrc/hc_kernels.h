@@ -37,42 +37,50 @@ enum {
     SY_COUNT
 };
 
-struct StepArgs {
+// Arguments of the step kernels.  R is the storage and arithmetic type of
+// the fields, spectra and coefficients (double, or float in the mixed
+// precision variant); the device scalars of the mean pressure gradient and
+// bulk velocity stay FP64.
+template <class R>
+struct StepArgsT {
+    using Real = R;
+    using Cx = cplx<R>;
     // physical fields
-    const double *u, *v, *w, *p, *t, *ff0, *gg0, *hh0, *ss0;
-    double *ff, *gg, *hh, *ss;
-    double *uo, *vo, *wo, *po, *to;  // outputs of the finish kernels
+    const R *u, *v, *w, *p, *t, *ff0, *gg0, *hh0, *ss0;
+    R *ff, *gg, *hh, *ss;
+    R *uo, *vo, *wo, *po, *to;  // outputs of the finish kernels
     // spectra
-    double2 *s0, *s1, *s2, *s3;
+    Cx *s0, *s1, *s2, *s3;
     // tables
-    const double *lev, *symx, *symy;
-    const double2 *tw_nl, *tw_nx, *tw_ny;
+    const R *lev, *symx, *symy;
+    const Cx *tw_nl, *tw_nx, *tw_ny;
     const double *scal;  // device scalars: [0] dpx_drive, [1] ubar
     int nx, ny, nz, nl;
     int ochannel, dp_opt, isscalar, use_f0;
-    double dx, dy, dx2, dy2, dt, nu, alpha, got, fc, ug, vg, u_mrf, t_ref, tbot, ttop;
-    double ra, rb, rc, dpx, norm;
+    R dx, dy, dx2, dy2, dt, nu, alpha, got, fc, ug, vg, u_mrf, t_ref, tbot, ttop;
+    R ra, rb, rc, dpx, norm;
     // reciprocal stencil factors: 1/(24 dx), 1/(12 dx^2), 1/dx, 1/dx^2 (same in y), 1/dt
-    double c24x, c12x2, c24y, c12y2, idx, idx2, idy, idy2, idt;
-    double aU0, cUe, aT0, cTe;  // unmodified wall coefficients for the (0,0)-mode fix-ups
+    R c24x, c12x2, c24y, c12y2, idx, idx2, idy, idy2, idt;
+    R aU0, cUe, aT0, cTe;  // unmodified wall coefficients for the (0,0)-mode fix-ups
     // Rayleigh damping of the top levels k >= kdamp0: sponge strength per
     // level and plane means per (field u,v,w,p,t; level); dmask selects the
     // fields an accessor reads damped (bit f for field f).
-    const double *dmp_s, *dmp_m;
+    const R *dmp_s, *dmp_m;
     int kdamp0, dmask;
     // spectral path (cds = 3): state spectra, product spectra, y-inverted
     // derivative spectra of the skew-symmetric terms, and the 2/3-rule limits
-    double2 *cu, *cv, *cw, *cp, *ct, *cff, *cgg, *chh, *css;
-    double2 *sp[8];
-    const double2 *sdv[6];
-    double coe;
+    Cx *cu, *cv, *cw, *cp, *ct, *cff, *cgg, *chh, *css;
+    Cx *sp[8];
+    const Cx *sdv[6];
+    R coe;
     int issk, lc, mc;
-    const double *aux;  // a physical field read by k_div_partial<3> (the spectral divergence)
-    int ntile;          // column kernels: number of nz x kColWaves sub-tiles in the LDS
+    const R *aux;  // a physical field read by k_div_partial<3> (the spectral divergence)
+    int ntile;     // column kernels: number of nz x kColWaves sub-tiles in the LDS
 };
+using StepArgs = StepArgsT<double>;
 
 HC_DEV int wrapi(int i, int n) { return i < 0 ? i + n : (i >= n ? i - n : i); }
-HC_DEV double sq(double x) { return x * x; }
+template <class Re> HC_DEV Re sq(Re x) { return x * x; }
 
 // ---------------------------------------------------------------------------
 // Point accessors with the reference's ghost rules
@@ -83,142 +91,156 @@ HC_DEV double sq(double x) { return x * x; }
 template <class D>
 struct Interp {
     HC_DEV const D &d() const { return static_cast<const D &>(*this); }
-    template <int CDS> HC_DEV double uhx(int di, int dj, int dk) const
+    template <int CDS> HC_DEV auto uhx(int di, int dj, int dk) const
     {
         const D &q = d();
-        if (CDS == 1) return 0.5 * (q.U(di, dj, dk) + q.U(di - 1, dj, dk));
-        return (-q.U(di + 1, dj, dk) + 9.0 * q.U(di, dj, dk) + 9.0 * q.U(di - 1, dj, dk) - q.U(di - 2, dj, dk)) / 16.0;
+        using Re = typename D::Real;
+        if (CDS == 1) return Re(0.5) * (q.U(di, dj, dk) + q.U(di - 1, dj, dk));
+        return (-q.U(di + 1, dj, dk) + Re(9.0) * q.U(di, dj, dk) + Re(9.0) * q.U(di - 1, dj, dk) - q.U(di - 2, dj, dk)) / Re(16.0);
     }
-    template <int CDS> HC_DEV double uhy(int di, int dj, int dk) const
+    template <int CDS> HC_DEV auto uhy(int di, int dj, int dk) const
     {
         const D &q = d();
-        if (CDS == 1) return 0.5 * (q.U(di, dj + 1, dk) + q.U(di, dj, dk));
-        return (-q.U(di, dj + 2, dk) + 9.0 * q.U(di, dj + 1, dk) + 9.0 * q.U(di, dj, dk) - q.U(di, dj - 1, dk)) / 16.0;
+        using Re = typename D::Real;
+        if (CDS == 1) return Re(0.5) * (q.U(di, dj + 1, dk) + q.U(di, dj, dk));
+        return (-q.U(di, dj + 2, dk) + Re(9.0) * q.U(di, dj + 1, dk) + Re(9.0) * q.U(di, dj, dk) - q.U(di, dj - 1, dk)) / Re(16.0);
     }
-    template <int CDS> HC_DEV double vhx(int di, int dj, int dk) const
+    template <int CDS> HC_DEV auto vhx(int di, int dj, int dk) const
     {
         const D &q = d();
-        if (CDS == 1) return 0.5 * (q.V(di + 1, dj, dk) + q.V(di, dj, dk));
-        return (-q.V(di + 2, dj, dk) + 9.0 * q.V(di + 1, dj, dk) + 9.0 * q.V(di, dj, dk) - q.V(di - 1, dj, dk)) / 16.0;
+        using Re = typename D::Real;
+        if (CDS == 1) return Re(0.5) * (q.V(di + 1, dj, dk) + q.V(di, dj, dk));
+        return (-q.V(di + 2, dj, dk) + Re(9.0) * q.V(di + 1, dj, dk) + Re(9.0) * q.V(di, dj, dk) - q.V(di - 1, dj, dk)) / Re(16.0);
     }
-    template <int CDS> HC_DEV double vhy(int di, int dj, int dk) const
+    template <int CDS> HC_DEV auto vhy(int di, int dj, int dk) const
     {
         const D &q = d();
-        if (CDS == 1) return 0.5 * (q.V(di, dj, dk) + q.V(di, dj - 1, dk));
-        return (-q.V(di, dj + 1, dk) + 9.0 * q.V(di, dj, dk) + 9.0 * q.V(di, dj - 1, dk) - q.V(di, dj - 2, dk)) / 16.0;
+        using Re = typename D::Real;
+        if (CDS == 1) return Re(0.5) * (q.V(di, dj, dk) + q.V(di, dj - 1, dk));
+        return (-q.V(di, dj + 1, dk) + Re(9.0) * q.V(di, dj, dk) + Re(9.0) * q.V(di, dj - 1, dk) - q.V(di, dj - 2, dk)) / Re(16.0);
     }
-    template <int CDS> HC_DEV double whx(int di, int dj, int dk) const
+    template <int CDS> HC_DEV auto whx(int di, int dj, int dk) const
     {
         const D &q = d();
+        using Re = typename D::Real;
         int kk = q.k + dk;
-        if (kk < 0 || kk == q.A.nz - 1) return 0.0;
-        if (CDS == 1) return 0.5 * (q.W(di + 1, dj, dk) + q.W(di, dj, dk));
-        return (-q.W(di + 2, dj, dk) + 9.0 * q.W(di + 1, dj, dk) + 9.0 * q.W(di, dj, dk) - q.W(di - 1, dj, dk)) / 16.0;
+        if (kk < 0 || kk == q.A.nz - 1) return Re(0.0);
+        if (CDS == 1) return Re(0.5) * (q.W(di + 1, dj, dk) + q.W(di, dj, dk));
+        return (-q.W(di + 2, dj, dk) + Re(9.0) * q.W(di + 1, dj, dk) + Re(9.0) * q.W(di, dj, dk) - q.W(di - 1, dj, dk)) / Re(16.0);
     }
-    template <int CDS> HC_DEV double why(int di, int dj, int dk) const
+    template <int CDS> HC_DEV auto why(int di, int dj, int dk) const
     {
         const D &q = d();
+        using Re = typename D::Real;
         int kk = q.k + dk;
-        if (kk < 0 || kk == q.A.nz - 1) return 0.0;
-        if (CDS == 1) return 0.5 * (q.W(di, dj + 1, dk) + q.W(di, dj, dk));
-        return (-q.W(di, dj + 2, dk) + 9.0 * q.W(di, dj + 1, dk) + 9.0 * q.W(di, dj, dk) - q.W(di, dj - 1, dk)) / 16.0;
+        if (kk < 0 || kk == q.A.nz - 1) return Re(0.0);
+        if (CDS == 1) return Re(0.5) * (q.W(di, dj + 1, dk) + q.W(di, dj, dk));
+        return (-q.W(di, dj + 2, dk) + Re(9.0) * q.W(di, dj + 1, dk) + Re(9.0) * q.W(di, dj, dk) - q.W(di, dj - 1, dk)) / Re(16.0);
     }
-    HC_DEV double uhz(int di, int dj, int dk) const
+    HC_DEV auto uhz(int di, int dj, int dk) const
     {
         const D &q = d();
+        using Re = typename D::Real;
         int kk = q.k + dk;
         if (kk < 0) return -q.A.u_mrf;
         if (kk == q.A.nz - 1) return q.A.ochannel == 0 ? -q.A.u_mrf : q.U(di, dj, dk);
-        double lt = q.lev(LV_LT, kk);
-        return q.U(di, dj, dk + 1) * lt + q.U(di, dj, dk) * (1.0 - lt);
+        Re lt = q.lev(LV_LT, kk);
+        return q.U(di, dj, dk + 1) * lt + q.U(di, dj, dk) * (Re(1.0) - lt);
     }
-    HC_DEV double vhz(int di, int dj, int dk) const
+    HC_DEV auto vhz(int di, int dj, int dk) const
     {
         const D &q = d();
+        using Re = typename D::Real;
         int kk = q.k + dk;
-        if (kk < 0) return 0.0;
-        if (kk == q.A.nz - 1) return q.A.ochannel == 0 ? 0.0 : q.V(di, dj, dk);
-        double lt = q.lev(LV_LT, kk);
-        return q.V(di, dj, dk + 1) * lt + q.V(di, dj, dk) * (1.0 - lt);
+        if (kk < 0) return Re(0.0);
+        if (kk == q.A.nz - 1) return q.A.ochannel == 0 ? Re(0.0) : q.V(di, dj, dk);
+        Re lt = q.lev(LV_LT, kk);
+        return q.V(di, dj, dk + 1) * lt + q.V(di, dj, dk) * (Re(1.0) - lt);
     }
-    template <int CDS> HC_DEV double thx(int di, int dj, int dk) const
+    template <int CDS> HC_DEV auto thx(int di, int dj, int dk) const
     {
         const D &q = d();
-        if (CDS == 1) return 0.5 * (q.T(di + 1, dj, dk) + q.T(di, dj, dk));
-        return (-q.T(di + 2, dj, dk) + 9.0 * q.T(di + 1, dj, dk) + 9.0 * q.T(di, dj, dk) - q.T(di - 1, dj, dk)) / 16.0;
+        using Re = typename D::Real;
+        if (CDS == 1) return Re(0.5) * (q.T(di + 1, dj, dk) + q.T(di, dj, dk));
+        return (-q.T(di + 2, dj, dk) + Re(9.0) * q.T(di + 1, dj, dk) + Re(9.0) * q.T(di, dj, dk) - q.T(di - 1, dj, dk)) / Re(16.0);
     }
-    template <int CDS> HC_DEV double thy(int di, int dj, int dk) const
+    template <int CDS> HC_DEV auto thy(int di, int dj, int dk) const
     {
         const D &q = d();
-        if (CDS == 1) return 0.5 * (q.T(di, dj + 1, dk) + q.T(di, dj, dk));
-        return (-q.T(di, dj + 2, dk) + 9.0 * q.T(di, dj + 1, dk) + 9.0 * q.T(di, dj, dk) - q.T(di, dj - 1, dk)) / 16.0;
+        using Re = typename D::Real;
+        if (CDS == 1) return Re(0.5) * (q.T(di, dj + 1, dk) + q.T(di, dj, dk));
+        return (-q.T(di, dj + 2, dk) + Re(9.0) * q.T(di, dj + 1, dk) + Re(9.0) * q.T(di, dj, dk) - q.T(di, dj - 1, dk)) / Re(16.0);
     }
-    HC_DEV double thz(int dk) const
+    HC_DEV auto thz(int dk) const
     {
         const D &q = d();
+        using Re = typename D::Real;
         int kk = q.k + dk;
         if (kk < 0) return q.A.tbot;
         if (kk == q.A.nz - 1) return q.A.ttop;
-        double lt = q.lev(LV_LT, kk);
-        return q.T(0, 0, dk + 1) * lt + q.T(0, 0, dk) * (1.0 - lt);
+        Re lt = q.lev(LV_LT, kk);
+        return q.T(0, 0, dk + 1) * lt + q.T(0, 0, dk) * (Re(1.0) - lt);
     }
 };
 
 // Global-memory accessor: any offset, ghost values by the wall rules.
-struct Pt : Interp<Pt> {
-    const StepArgs &A;
+template <class R>
+struct PtT : Interp<PtT<R>> {
+    using Real = R;
+    const StepArgsT<R> &A;
     int i, j, k;
-    HC_DEV Pt(const StepArgs &a, int i_, int j_, int k_) : A(a), i(i_), j(j_), k(k_) {}
+    HC_DEV PtT(const StepArgsT<R> &a, int i_, int j_, int k_) : A(a), i(i_), j(j_), k(k_) {}
     HC_DEV size_t at(int di, int dj, int kk) const
     {
         return ((size_t)kk * A.ny + wrapi(j + dj, A.ny)) * A.nx + wrapi(i + di, A.nx);
     }
-    HC_DEV double U(int di, int dj, int dk) const
+    HC_DEV R U(int di, int dj, int dk) const
     {
         int kk = k + dk;
-        if (kk < 0) return -A.u[at(di, dj, 0)] - 2.0 * A.u_mrf;
+        if (kk < 0) return -A.u[at(di, dj, 0)] - R(2.0) * A.u_mrf;
         if (kk >= A.nz) {
-            double top = A.u[at(di, dj, A.nz - 1)];
-            return A.ochannel == 0 ? -top - 2.0 * A.u_mrf : top;
+            R top = A.u[at(di, dj, A.nz - 1)];
+            return A.ochannel == 0 ? -top - R(2.0) * A.u_mrf : top;
         }
         return A.u[at(di, dj, kk)];
     }
-    HC_DEV double V(int di, int dj, int dk) const
+    HC_DEV R V(int di, int dj, int dk) const
     {
         int kk = k + dk;
         if (kk < 0) return -A.v[at(di, dj, 0)];
         if (kk >= A.nz) {
-            double top = A.v[at(di, dj, A.nz - 1)];
+            R top = A.v[at(di, dj, A.nz - 1)];
             return A.ochannel == 0 ? -top : top;
         }
         return A.v[at(di, dj, kk)];
     }
-    HC_DEV double W(int di, int dj, int dk) const
+    HC_DEV R W(int di, int dj, int dk) const
     {
         int kk = k + dk;
-        if (kk < 0 || kk == A.nz - 1) return 0.0;
+        if (kk < 0 || kk == A.nz - 1) return R(0.0);
         if (kk >= A.nz) return -A.w[at(di, dj, A.nz - 2)];
         return A.w[at(di, dj, kk)];
     }
-    HC_DEV double P(int di, int dj, int dk) const
+    HC_DEV R P(int di, int dj, int dk) const
     {
         int kk = k + dk;
         kk = kk < 0 ? 0 : (kk >= A.nz ? A.nz - 1 : kk);
         return A.p[at(di, dj, kk)];
     }
-    HC_DEV double T(int di, int dj, int dk) const
+    HC_DEV R T(int di, int dj, int dk) const
     {
         int kk = k + dk;
-        if (kk < 0) return 2.0 * A.tbot - A.t[at(di, dj, 0)];
-        if (kk >= A.nz) return 2.0 * A.ttop - A.t[at(di, dj, A.nz - 1)];
+        if (kk < 0) return R(2.0) * A.tbot - A.t[at(di, dj, 0)];
+        if (kk >= A.nz) return R(2.0) * A.ttop - A.t[at(di, dj, A.nz - 1)];
         return A.t[at(di, dj, kk)];
     }
-    HC_DEV double lev(int tab, int kk) const { return A.lev[tab * A.nz + kk]; }
+    HC_DEV R lev(int tab, int kk) const { return A.lev[tab * A.nz + kk]; }
     HC_DEV size_t center() const { return at(0, 0, k); }
-    HC_DEV double f0() const { return A.use_f0 ? A.ff0[center()] : 0.0; }
-    HC_DEV double g0() const { return A.use_f0 ? A.gg0[center()] : 0.0; }
-    HC_DEV double h0() const { return A.use_f0 ? A.hh0[center()] : 0.0; }
+    HC_DEV R f0() const { return A.use_f0 ? A.ff0[center()] : R(0.0); }
+    HC_DEV R g0() const { return A.use_f0 ? A.gg0[center()] : R(0.0); }
+    HC_DEV R h0() const { return A.use_f0 ? A.hh0[center()] : R(0.0); }
 };
+using Pt = PtT<double>;
 
 // Global-memory accessor of a substep that starts with rayleigh_damping
 // (src/module_tools.f90:705-749, called at src/module_navier_stokes.f90:258-266).
@@ -231,6 +253,7 @@ struct Pt : Interp<Pt> {
 // (comm_bound in update_boundary_uvwhxyz, src/module_boundary.f90:202-355), so
 // an interpolant at a wrapped offset is evaluated at the wrapped centre.
 struct PtD : Interp<PtD> {
+    using Real = double;
     const StepArgs &A;
     int i, j, k;
     HC_DEV PtD(const StepArgs &a, int i_, int j_, int k_) : A(a), i(i_), j(j_), k(k_) {}
@@ -318,162 +341,168 @@ struct PtD : Interp<PtD> {
 // Momentum right-hand side at one point, get_momentum_cnvdiff
 // (src/module_cnvdiff.f90:35-374).  Returns ff,gg,hh and qu,qv,qw.
 template <int CDS, class Q>
-HC_DEV void momentum_point(const Q &q, double dpx, double &f, double &g, double &h, double &qu, double &qv, double &qw)
+HC_DEV void momentum_point(const Q &q, typename Q::Real dpx, typename Q::Real &f, typename Q::Real &g,
+                           typename Q::Real &h, typename Q::Real &qu, typename Q::Real &qv, typename Q::Real &qw)
 {
-    const StepArgs &A = q.A;
-    const double um = A.u_mrf;
+    using Re = typename Q::Real;
+    const auto &A = q.A;
+    const Re um = A.u_mrf;
     const int k = q.k;
-    const double idz = q.lev(LV_IDZ, k), idzt = q.lev(LV_IDZT, k), idzb = q.lev(LV_IDZB, k);
+    const Re idz = q.lev(LV_IDZ, k), idzt = q.lev(LV_IDZT, k), idzb = q.lev(LV_IDZB, k);
     // ---- U equation (:55-161)
     {
-        double uux, uvy, ux, uxx, uyy, dpdx;
+        Re uux, uvy, ux, uxx, uyy, dpdx;
         if (CDS == 1) {
             uux = (sq(q.template uhx<1>(1, 0, 0)) - sq(q.template uhx<1>(0, 0, 0))) * A.idx;
             uvy = (q.template uhy<1>(0, 0, 0) * q.template vhx<1>(0, 0, 0) - q.template uhy<1>(0, -1, 0) * q.template vhx<1>(0, -1, 0)) * A.idy;
             ux = (q.template uhx<1>(1, 0, 0) - q.template uhx<1>(0, 0, 0)) * A.idx;
-            uxx = (q.U(1, 0, 0) - 2.0 * q.U(0, 0, 0) + q.U(-1, 0, 0)) * A.idx2;
-            uyy = (q.U(0, 1, 0) - 2.0 * q.U(0, 0, 0) + q.U(0, -1, 0)) * A.idy2;
+            uxx = (q.U(1, 0, 0) - Re(2.0) * q.U(0, 0, 0) + q.U(-1, 0, 0)) * A.idx2;
+            uyy = (q.U(0, 1, 0) - Re(2.0) * q.U(0, 0, 0) + q.U(0, -1, 0)) * A.idy2;
             dpdx = (q.P(1, 0, 0) - q.P(0, 0, 0)) * A.idx;
         } else {
-            double h2 = q.template uhx<2>(2, 0, 0), h1 = q.template uhx<2>(1, 0, 0), h0 = q.template uhx<2>(0, 0, 0), hm = q.template uhx<2>(-1, 0, 0);
-            uux = (-sq(h2) + 27.0 * sq(h1) - 27.0 * sq(h0) + sq(hm)) * A.c24x;
-            uvy = (-q.template uhy<2>(0, 1, 0) * q.template vhx<2>(0, 1, 0) + 27.0 * q.template uhy<2>(0, 0, 0) * q.template vhx<2>(0, 0, 0) -
-                   27.0 * q.template uhy<2>(0, -1, 0) * q.template vhx<2>(0, -1, 0) + q.template uhy<2>(0, -2, 0) * q.template vhx<2>(0, -2, 0)) * A.c24y;
-            ux = (-h2 + 27.0 * h1 - 27.0 * h0 + hm) * A.c24x;
-            uxx = (-q.U(2, 0, 0) + 16.0 * q.U(1, 0, 0) - 30.0 * q.U(0, 0, 0) + 16.0 * q.U(-1, 0, 0) - q.U(-2, 0, 0)) * A.c12x2;
-            uyy = (-q.U(0, 2, 0) + 16.0 * q.U(0, 1, 0) - 30.0 * q.U(0, 0, 0) + 16.0 * q.U(0, -1, 0) - q.U(0, -2, 0)) * A.c12y2;
-            dpdx = (-q.P(2, 0, 0) + 27.0 * q.P(1, 0, 0) - 27.0 * q.P(0, 0, 0) + q.P(-1, 0, 0)) * A.c24x;
+            Re h2 = q.template uhx<2>(2, 0, 0), h1 = q.template uhx<2>(1, 0, 0), h0 = q.template uhx<2>(0, 0, 0), hm = q.template uhx<2>(-1, 0, 0);
+            uux = (-sq(h2) + Re(27.0) * sq(h1) - Re(27.0) * sq(h0) + sq(hm)) * A.c24x;
+            uvy = (-q.template uhy<2>(0, 1, 0) * q.template vhx<2>(0, 1, 0) + Re(27.0) * q.template uhy<2>(0, 0, 0) * q.template vhx<2>(0, 0, 0) -
+                   Re(27.0) * q.template uhy<2>(0, -1, 0) * q.template vhx<2>(0, -1, 0) + q.template uhy<2>(0, -2, 0) * q.template vhx<2>(0, -2, 0)) * A.c24y;
+            ux = (-h2 + Re(27.0) * h1 - Re(27.0) * h0 + hm) * A.c24x;
+            uxx = (-q.U(2, 0, 0) + Re(16.0) * q.U(1, 0, 0) - Re(30.0) * q.U(0, 0, 0) + Re(16.0) * q.U(-1, 0, 0) - q.U(-2, 0, 0)) * A.c12x2;
+            uyy = (-q.U(0, 2, 0) + Re(16.0) * q.U(0, 1, 0) - Re(30.0) * q.U(0, 0, 0) + Re(16.0) * q.U(0, -1, 0) - q.U(0, -2, 0)) * A.c12y2;
+            dpdx = (-q.P(2, 0, 0) + Re(27.0) * q.P(1, 0, 0) - Re(27.0) * q.P(0, 0, 0) + q.P(-1, 0, 0)) * A.c24x;
         }
-        double uwz = (q.uhz(0, 0, 0) * q.template whx<CDS>(0, 0, 0) - q.uhz(0, 0, -1) * q.template whx<CDS>(0, 0, -1)) * idz;
-        double u0 = q.U(0, 0, 0);
-        double uzz = ((q.U(0, 0, 1) - u0) * idzt - (u0 - q.U(0, 0, -1)) * idzb) * idz;
+        Re uwz = (q.uhz(0, 0, 0) * q.template whx<CDS>(0, 0, 0) - q.uhz(0, 0, -1) * q.template whx<CDS>(0, 0, -1)) * idz;
+        Re u0 = q.U(0, 0, 0);
+        Re uzz = ((q.U(0, 0, 1) - u0) * idzt - (u0 - q.U(0, 0, -1)) * idzb) * idz;
         if (A.dp_opt == 3)
-            f = -uux - uvy - uwz - um * ux + A.fc * (0.5 * (q.template vhx<CDS>(0, 0, 0) + q.template vhx<CDS>(0, -1, 0)) - A.vg);
+            f = -uux - uvy - uwz - um * ux + A.fc * (Re(0.5) * (q.template vhx<CDS>(0, 0, 0) + q.template vhx<CDS>(0, -1, 0)) - A.vg);
         else
             f = -uux - uvy - uwz + dpx - um * ux;
-        double f0 = q.f0();
-        qu = u0 + A.dt * (A.ra * f + A.rb * f0) + A.rc * 0.5 * A.dt * A.nu * (uxx + uyy + uzz) - A.rc * A.dt * dpdx;
+        Re f0 = q.f0();
+        qu = u0 + A.dt * (A.ra * f + A.rb * f0) + A.rc * Re(0.5) * A.dt * A.nu * (uxx + uyy + uzz) - A.rc * A.dt * dpdx;
     }
     // ---- V equation (:166-270)
     {
-        double vux, vvy, vx, vxx, vyy, dpdy;
+        Re vux, vvy, vx, vxx, vyy, dpdy;
         if (CDS == 1) {
             vux = (q.template vhx<1>(0, 0, 0) * q.template uhy<1>(0, 0, 0) - q.template vhx<1>(-1, 0, 0) * q.template uhy<1>(-1, 0, 0)) * A.idx;
             vvy = (sq(q.template vhy<1>(0, 1, 0)) - sq(q.template vhy<1>(0, 0, 0))) * A.idy;
             vx = (q.template vhx<1>(0, 0, 0) - q.template vhx<1>(-1, 0, 0)) * A.idx;
-            vxx = (q.V(1, 0, 0) - 2.0 * q.V(0, 0, 0) + q.V(-1, 0, 0)) * A.idx2;
-            vyy = (q.V(0, 1, 0) - 2.0 * q.V(0, 0, 0) + q.V(0, -1, 0)) * A.idy2;
+            vxx = (q.V(1, 0, 0) - Re(2.0) * q.V(0, 0, 0) + q.V(-1, 0, 0)) * A.idx2;
+            vyy = (q.V(0, 1, 0) - Re(2.0) * q.V(0, 0, 0) + q.V(0, -1, 0)) * A.idy2;
             dpdy = (q.P(0, 1, 0) - q.P(0, 0, 0)) * A.idy;
         } else {
-            double x1 = q.template vhx<2>(1, 0, 0), x0 = q.template vhx<2>(0, 0, 0), xm = q.template vhx<2>(-1, 0, 0), xm2 = q.template vhx<2>(-2, 0, 0);
-            vux = (-x1 * q.template uhy<2>(1, 0, 0) + 27.0 * x0 * q.template uhy<2>(0, 0, 0) - 27.0 * xm * q.template uhy<2>(-1, 0, 0) +
+            Re x1 = q.template vhx<2>(1, 0, 0), x0 = q.template vhx<2>(0, 0, 0), xm = q.template vhx<2>(-1, 0, 0), xm2 = q.template vhx<2>(-2, 0, 0);
+            vux = (-x1 * q.template uhy<2>(1, 0, 0) + Re(27.0) * x0 * q.template uhy<2>(0, 0, 0) - Re(27.0) * xm * q.template uhy<2>(-1, 0, 0) +
                    xm2 * q.template uhy<2>(-2, 0, 0)) * A.c24x;
-            vvy = (-sq(q.template vhy<2>(0, 2, 0)) + 27.0 * sq(q.template vhy<2>(0, 1, 0)) - 27.0 * sq(q.template vhy<2>(0, 0, 0)) +
+            vvy = (-sq(q.template vhy<2>(0, 2, 0)) + Re(27.0) * sq(q.template vhy<2>(0, 1, 0)) - Re(27.0) * sq(q.template vhy<2>(0, 0, 0)) +
                    sq(q.template vhy<2>(0, -1, 0))) * A.c24y;
-            vx = (-x1 + 27.0 * x0 - 27.0 * xm + xm2) * A.c24x;
-            vxx = (-q.V(2, 0, 0) + 16.0 * q.V(1, 0, 0) - 30.0 * q.V(0, 0, 0) + 16.0 * q.V(-1, 0, 0) - q.V(-2, 0, 0)) * A.c12x2;
-            vyy = (-q.V(0, 2, 0) + 16.0 * q.V(0, 1, 0) - 30.0 * q.V(0, 0, 0) + 16.0 * q.V(0, -1, 0) - q.V(0, -2, 0)) * A.c12y2;
-            dpdy = (-q.P(0, 2, 0) + 27.0 * q.P(0, 1, 0) - 27.0 * q.P(0, 0, 0) + q.P(0, -1, 0)) * A.c24y;
+            vx = (-x1 + Re(27.0) * x0 - Re(27.0) * xm + xm2) * A.c24x;
+            vxx = (-q.V(2, 0, 0) + Re(16.0) * q.V(1, 0, 0) - Re(30.0) * q.V(0, 0, 0) + Re(16.0) * q.V(-1, 0, 0) - q.V(-2, 0, 0)) * A.c12x2;
+            vyy = (-q.V(0, 2, 0) + Re(16.0) * q.V(0, 1, 0) - Re(30.0) * q.V(0, 0, 0) + Re(16.0) * q.V(0, -1, 0) - q.V(0, -2, 0)) * A.c12y2;
+            dpdy = (-q.P(0, 2, 0) + Re(27.0) * q.P(0, 1, 0) - Re(27.0) * q.P(0, 0, 0) + q.P(0, -1, 0)) * A.c24y;
         }
-        double vwz = (q.vhz(0, 0, 0) * q.template why<CDS>(0, 0, 0) - q.vhz(0, 0, -1) * q.template why<CDS>(0, 0, -1)) * idz;
-        double v0 = q.V(0, 0, 0);
-        double vzz = ((q.V(0, 0, 1) - v0) * idzt - (v0 - q.V(0, 0, -1)) * idzb) * idz;
+        Re vwz = (q.vhz(0, 0, 0) * q.template why<CDS>(0, 0, 0) - q.vhz(0, 0, -1) * q.template why<CDS>(0, 0, -1)) * idz;
+        Re v0 = q.V(0, 0, 0);
+        Re vzz = ((q.V(0, 0, 1) - v0) * idzt - (v0 - q.V(0, 0, -1)) * idzb) * idz;
         if (A.dp_opt == 3)
-            g = -vux - vvy - vwz - um * vx + A.fc * (A.ug - 0.5 * (q.template uhy<CDS>(0, 0, 0) + q.template uhy<CDS>(-1, 0, 0)) - um);
+            g = -vux - vvy - vwz - um * vx + A.fc * (A.ug - Re(0.5) * (q.template uhy<CDS>(0, 0, 0) + q.template uhy<CDS>(-1, 0, 0)) - um);
         else
             g = -vux - vvy - vwz - um * vx;
-        double g0 = q.g0();
-        qv = v0 + A.dt * (A.ra * g + A.rb * g0) + A.rc * 0.5 * A.dt * A.nu * (vxx + vyy + vzz) - A.rc * A.dt * dpdy;
+        Re g0 = q.g0();
+        qv = v0 + A.dt * (A.ra * g + A.rb * g0) + A.rc * Re(0.5) * A.dt * A.nu * (vxx + vyy + vzz) - A.rc * A.dt * dpdy;
     }
     // ---- W equation (:274-370), k < kend3
     if (k < A.nz - 1) {
-        double wux, wvy, wx, wxx, wyy;
+        Re wux, wvy, wx, wxx, wyy;
         if (CDS == 1) {
             wux = (q.template whx<1>(0, 0, 0) * q.uhz(0, 0, 0) - q.template whx<1>(-1, 0, 0) * q.uhz(-1, 0, 0)) * A.idx;
             wvy = (q.template why<1>(0, 0, 0) * q.vhz(0, 0, 0) - q.template why<1>(0, -1, 0) * q.vhz(0, -1, 0)) * A.idy;
             wx = (q.template whx<1>(0, 0, 0) - q.template whx<1>(-1, 0, 0)) * A.idx;
-            wxx = (q.W(1, 0, 0) - 2.0 * q.W(0, 0, 0) + q.W(-1, 0, 0)) * A.idx2;
-            wyy = (q.W(0, 1, 0) - 2.0 * q.W(0, 0, 0) + q.W(0, -1, 0)) * A.idy2;
+            wxx = (q.W(1, 0, 0) - Re(2.0) * q.W(0, 0, 0) + q.W(-1, 0, 0)) * A.idx2;
+            wyy = (q.W(0, 1, 0) - Re(2.0) * q.W(0, 0, 0) + q.W(0, -1, 0)) * A.idy2;
         } else {
-            double x1 = q.template whx<2>(1, 0, 0), x0 = q.template whx<2>(0, 0, 0), xm = q.template whx<2>(-1, 0, 0), xm2 = q.template whx<2>(-2, 0, 0);
-            wux = (-x1 * q.uhz(1, 0, 0) + 27.0 * x0 * q.uhz(0, 0, 0) - 27.0 * xm * q.uhz(-1, 0, 0) + xm2 * q.uhz(-2, 0, 0)) * A.c24x;
-            wvy = (-q.template why<2>(0, 1, 0) * q.vhz(0, 1, 0) + 27.0 * q.template why<2>(0, 0, 0) * q.vhz(0, 0, 0) -
-                   27.0 * q.template why<2>(0, -1, 0) * q.vhz(0, -1, 0) + q.template why<2>(0, -2, 0) * q.vhz(0, -2, 0)) * A.c24y;
-            wx = (-x1 + 27.0 * x0 - 27.0 * xm + xm2) * A.c24x;
-            wxx = (-q.W(2, 0, 0) + 16.0 * q.W(1, 0, 0) - 30.0 * q.W(0, 0, 0) + 16.0 * q.W(-1, 0, 0) - q.W(-2, 0, 0)) * A.c12x2;
-            wyy = (-q.W(0, 2, 0) + 16.0 * q.W(0, 1, 0) - 30.0 * q.W(0, 0, 0) + 16.0 * q.W(0, -1, 0) - q.W(0, -2, 0)) * A.c12y2;
+            Re x1 = q.template whx<2>(1, 0, 0), x0 = q.template whx<2>(0, 0, 0), xm = q.template whx<2>(-1, 0, 0), xm2 = q.template whx<2>(-2, 0, 0);
+            wux = (-x1 * q.uhz(1, 0, 0) + Re(27.0) * x0 * q.uhz(0, 0, 0) - Re(27.0) * xm * q.uhz(-1, 0, 0) + xm2 * q.uhz(-2, 0, 0)) * A.c24x;
+            wvy = (-q.template why<2>(0, 1, 0) * q.vhz(0, 1, 0) + Re(27.0) * q.template why<2>(0, 0, 0) * q.vhz(0, 0, 0) -
+                   Re(27.0) * q.template why<2>(0, -1, 0) * q.vhz(0, -1, 0) + q.template why<2>(0, -2, 0) * q.vhz(0, -2, 0)) * A.c24y;
+            wx = (-x1 + Re(27.0) * x0 - Re(27.0) * xm + xm2) * A.c24x;
+            wxx = (-q.W(2, 0, 0) + Re(16.0) * q.W(1, 0, 0) - Re(30.0) * q.W(0, 0, 0) + Re(16.0) * q.W(-1, 0, 0) - q.W(-2, 0, 0)) * A.c12x2;
+            wyy = (-q.W(0, 2, 0) + Re(16.0) * q.W(0, 1, 0) - Re(30.0) * q.W(0, 0, 0) + Re(16.0) * q.W(0, -1, 0) - q.W(0, -2, 0)) * A.c12y2;
         }
-        double w0 = q.W(0, 0, 0), wp = q.W(0, 0, 1), wm = q.W(0, 0, -1);
-        double a1 = sq(0.5 * (w0 + wp)), a2 = sq(0.5 * (w0 + wm));
-        double wwz = (a1 - a2) * idzt;
-        double wzz = ((wp - w0) * q.lev(LV_IDZ, k + 1) - (w0 - wm) * idz) * idzt;
+        Re w0 = q.W(0, 0, 0), wp = q.W(0, 0, 1), wm = q.W(0, 0, -1);
+        Re a1 = sq(Re(0.5) * (w0 + wp)), a2 = sq(Re(0.5) * (w0 + wm));
+        Re wwz = (a1 - a2) * idzt;
+        Re wzz = ((wp - w0) * q.lev(LV_IDZ, k + 1) - (w0 - wm) * idz) * idzt;
         if (A.isscalar) {
-            double lt = q.lev(LV_LT, k);
-            h = -wux - wvy - wwz - um * wx + A.got * ((1.0 - lt) * q.T(0, 0, 0) + lt * q.T(0, 0, 1) - A.t_ref);
+            Re lt = q.lev(LV_LT, k);
+            h = -wux - wvy - wwz - um * wx + A.got * ((Re(1.0) - lt) * q.T(0, 0, 0) + lt * q.T(0, 0, 1) - A.t_ref);
         } else {
             h = -wux - wvy - wwz - um * wx;
         }
-        double dpdz = (q.P(0, 0, 1) - q.P(0, 0, 0)) * idzt;
-        double h0 = q.h0();
-        qw = w0 + A.dt * (A.ra * h + A.rb * h0) + A.rc * 0.5 * A.dt * A.nu * (wxx + wyy + wzz) - A.rc * A.dt * dpdz;
+        Re dpdz = (q.P(0, 0, 1) - q.P(0, 0, 0)) * idzt;
+        Re h0 = q.h0();
+        qw = w0 + A.dt * (A.ra * h + A.rb * h0) + A.rc * Re(0.5) * A.dt * A.nu * (wxx + wyy + wzz) - A.rc * A.dt * dpdz;
     } else {
-        h = 0.0;
-        qw = 0.0;
+        h = Re(0.0);
+        qw = Re(0.0);
     }
 }
 
 // Temperature right-hand side at one point, get_temperature_cnvdiff
 // (src/module_cnvdiff.f90:381-473), evaluated with the updated u, v, w.
 template <int CDS, class Q>
-HC_DEV void temperature_point(const Q &q, double &s, double &qt)
+HC_DEV void temperature_point(const Q &q, typename Q::Real &s, typename Q::Real &qt)
 {
-    const StepArgs &A = q.A;
+    using Re = typename Q::Real;
+    const auto &A = q.A;
     const int k = q.k;
-    double tux, tvy, tx, txx, tyy;
+    Re tux, tvy, tx, txx, tyy;
     if (CDS == 1) {
         tux = (q.template thx<1>(0, 0, 0) * q.U(0, 0, 0) - q.template thx<1>(-1, 0, 0) * q.U(-1, 0, 0)) * A.idx;
         tvy = (q.template thy<1>(0, 0, 0) * q.V(0, 0, 0) - q.template thy<1>(0, -1, 0) * q.V(0, -1, 0)) * A.idy;
         tx = (q.template thx<1>(0, 0, 0) - q.template thx<1>(-1, 0, 0)) * A.idx;
-        txx = (q.T(1, 0, 0) - 2.0 * q.T(0, 0, 0) + q.T(-1, 0, 0)) * A.idx2;
-        tyy = (q.T(0, 1, 0) - 2.0 * q.T(0, 0, 0) + q.T(0, -1, 0)) * A.idy2;
+        txx = (q.T(1, 0, 0) - Re(2.0) * q.T(0, 0, 0) + q.T(-1, 0, 0)) * A.idx2;
+        tyy = (q.T(0, 1, 0) - Re(2.0) * q.T(0, 0, 0) + q.T(0, -1, 0)) * A.idy2;
     } else {
-        double x1 = q.template thx<2>(1, 0, 0), x0 = q.template thx<2>(0, 0, 0), xm = q.template thx<2>(-1, 0, 0), xm2 = q.template thx<2>(-2, 0, 0);
-        tux = (-x1 * q.U(1, 0, 0) + 27.0 * x0 * q.U(0, 0, 0) - 27.0 * xm * q.U(-1, 0, 0) + xm2 * q.U(-2, 0, 0)) * A.c24x;
-        tvy = (-q.template thy<2>(0, 1, 0) * q.V(0, 1, 0) + 27.0 * q.template thy<2>(0, 0, 0) * q.V(0, 0, 0) -
-               27.0 * q.template thy<2>(0, -1, 0) * q.V(0, -1, 0) + q.template thy<2>(0, -2, 0) * q.V(0, -2, 0)) * A.c24y;
-        tx = (-x1 + 27.0 * x0 - 27.0 * xm + xm2) * A.c24x;
-        txx = (-q.T(2, 0, 0) + 16.0 * q.T(1, 0, 0) - 30.0 * q.T(0, 0, 0) + 16.0 * q.T(-1, 0, 0) - q.T(-2, 0, 0)) * A.c12x2;
-        tyy = (-q.T(0, 2, 0) + 16.0 * q.T(0, 1, 0) - 30.0 * q.T(0, 0, 0) + 16.0 * q.T(0, -1, 0) - q.T(0, -2, 0)) * A.c12y2;
+        Re x1 = q.template thx<2>(1, 0, 0), x0 = q.template thx<2>(0, 0, 0), xm = q.template thx<2>(-1, 0, 0), xm2 = q.template thx<2>(-2, 0, 0);
+        tux = (-x1 * q.U(1, 0, 0) + Re(27.0) * x0 * q.U(0, 0, 0) - Re(27.0) * xm * q.U(-1, 0, 0) + xm2 * q.U(-2, 0, 0)) * A.c24x;
+        tvy = (-q.template thy<2>(0, 1, 0) * q.V(0, 1, 0) + Re(27.0) * q.template thy<2>(0, 0, 0) * q.V(0, 0, 0) -
+               Re(27.0) * q.template thy<2>(0, -1, 0) * q.V(0, -1, 0) + q.template thy<2>(0, -2, 0) * q.V(0, -2, 0)) * A.c24y;
+        tx = (-x1 + Re(27.0) * x0 - Re(27.0) * xm + xm2) * A.c24x;
+        txx = (-q.T(2, 0, 0) + Re(16.0) * q.T(1, 0, 0) - Re(30.0) * q.T(0, 0, 0) + Re(16.0) * q.T(-1, 0, 0) - q.T(-2, 0, 0)) * A.c12x2;
+        tyy = (-q.T(0, 2, 0) + Re(16.0) * q.T(0, 1, 0) - Re(30.0) * q.T(0, 0, 0) + Re(16.0) * q.T(0, -1, 0) - q.T(0, -2, 0)) * A.c12y2;
     }
-    double twz = (q.thz(0) * q.W(0, 0, 0) - q.thz(-1) * q.W(0, 0, -1)) * q.lev(LV_IDZ, k);
-    double t0 = q.T(0, 0, 0);
-    double tzz = ((q.T(0, 0, 1) - t0) * q.lev(LV_IDZT, k) - (t0 - q.T(0, 0, -1)) * q.lev(LV_IDZB, k)) * q.lev(LV_IDZ, k);
+    Re twz = (q.thz(0) * q.W(0, 0, 0) - q.thz(-1) * q.W(0, 0, -1)) * q.lev(LV_IDZ, k);
+    Re t0 = q.T(0, 0, 0);
+    Re tzz = ((q.T(0, 0, 1) - t0) * q.lev(LV_IDZT, k) - (t0 - q.T(0, 0, -1)) * q.lev(LV_IDZB, k)) * q.lev(LV_IDZ, k);
     s = -tux - tvy - twz - A.u_mrf * tx;
-    double s0 = A.use_f0 ? A.ss0[q.center()] : 0.0;
-    qt = t0 + A.dt * (A.ra * s + A.rb * s0) + A.rc * 0.5 * A.dt * A.alpha * (txx + tyy + tzz);
+    Re s0 = A.use_f0 ? A.ss0[q.center()] : Re(0.0);
+    qt = t0 + A.dt * (A.ra * s + A.rb * s0) + A.rc * Re(0.5) * A.dt * A.alpha * (txx + tyy + tzz);
 }
 
 // ---------------------------------------------------------------------------
 // Row transforms in LDS.  Rows of NX reals are packed as NX/2 complex.
 // R2C post-processing for rows already transformed by an NX/2-point FFT:
 // X(l) = (Z(l)+conj Z(NL-l))/2 + e^{-2 pi i l/NX} (Z(l)-conj Z(NL-l))/(2i)
-template <int NL>
-HC_DEV double2 r2c_post(const double2 *row, int l, const double2 *__restrict__ tw_nx, double norm)
+template <int NL, class C>
+HC_DEV C r2c_post(const C *row, int l, const C *__restrict__ tw_nx, real_of<C> norm)
 {
-    double2 z = row[l], zr = cconj(row[(NL - l) % NL]);
-    double2 e = make_double2(0.5 * (z.x + zr.x), 0.5 * (z.y + zr.y));
-    double2 dd = csub(z, zr);
-    double2 o = make_double2(0.5 * dd.y, -0.5 * dd.x);  // (Z - conj Z')/(2i)
-    double2 x = cadd(e, cmul(tw_nx[l], o));
+    using Re = real_of<C>;
+    const Re half = Re(0.5);
+    C z = row[l], zr = cconj(row[(NL - l) % NL]);
+    C e = mk2<C>(half * (z.x + zr.x), half * (z.y + zr.y));
+    C dd = csub(z, zr);
+    C o = mk2<C>(half * dd.y, -half * dd.x);  // (Z - conj Z')/(2i)
+    C x = cadd(e, cmul(tw_nx[l], o));
     return cscale(x, norm);
 }
 // C2R pre-processing: Z(l) = A + i e^{+2 pi i l/NX} B, A = X(l) + X~, B = X(l) - X~,
 // X~ = conj X(NL-l) (zero for l = 0, whose partner is the zeroed Nyquist bin).
-HC_DEV double2 c2r_pre(double2 xl, double2 xt, double2 twl)
+template <class C>
+HC_DEV C c2r_pre(C xl, C xt, C twl)
 {
-    double2 a = cadd(xl, xt), b = csub(xl, xt);
-    double2 e = cmul(cconj(twl), b);          // e^{+i theta} B
-    return make_double2(a.x - e.y, a.y + e.x);  // A + i e
+    C a = cadd(xl, xt), b = csub(xl, xt);
+    C e = cmul(cconj(twl), b);          // e^{+i theta} B
+    return mk2<C>(a.x - e.y, a.y + e.x);  // A + i e
 }
 
 // ---------------------------------------------------------------------------
@@ -500,8 +529,8 @@ HC_DEV int wrapx(int i)
 // Value of staged row r (of RhsTile) at column i, with the wall rules of
 // update_boundary_uvw / _p (src/module_boundary.f90:33-162) for the ghost
 // levels k-1 = -1 and k+1 = nz.
-template <int NX, int TY>
-HC_DEV double rhs_stage_value(const StepArgs &A, int r, int i, int j0, int k)
+template <int NX, int TY, class R>
+HC_DEV R rhs_stage_value(const StepArgsT<R> &A, int r, int i, int j0, int k)
 {
     using Tl = RhsTile<NX, TY>;
     const int ny = A.ny, nz = A.nz;
@@ -510,80 +539,82 @@ HC_DEV double rhs_stage_value(const StepArgs &A, int r, int i, int j0, int k)
     if (r < Tl::OW) return A.v[at(j0 - 3 + (r - Tl::OV), k)];
     if (r < Tl::OP) return A.w[at(j0 - 3 + (r - Tl::OW), k)];
     if (r < Tl::OWM) return A.p[at(j0 - 1 + (r - Tl::OP), k)];
-    if (r < Tl::OVP) return k == 0 ? 0.0 : A.w[at(j0 - 1 + (r - Tl::OWM), k - 1)];
+    if (r < Tl::OVP) return k == 0 ? R(0.0) : A.w[at(j0 - 1 + (r - Tl::OWM), k - 1)];
     if (r < Tl::OUP) {
         int jj = j0 - 2 + (r - Tl::OVP);
         if (k + 1 < nz) return A.v[at(jj, k + 1)];
-        double top = A.v[at(jj, nz - 1)];
+        R top = A.v[at(jj, nz - 1)];
         return A.ochannel == 0 ? -top : top;
     }
     int jj = j0 + (r - Tl::OUP);
     if (k + 1 < nz) return A.u[at(jj, k + 1)];
-    double top = A.u[at(jj, nz - 1)];
-    return A.ochannel == 0 ? -top - 2.0 * A.u_mrf : top;
+    R top = A.u[at(jj, nz - 1)];
+    return A.ochannel == 0 ? -top - R(2.0) * A.u_mrf : top;
 }
 
 // LDS accessor: the offsets momentum_point uses, read from the staged rows;
 // the single-point values at k-1 / k+1 come from registers.
 // Single-point values of one RHS point at levels k-1 / k+1 and the previous
 // substep's right-hand sides, loaded ahead of the LDS staging barrier.
+template <class R>
 struct PointVals {
-    double u_km, v_km, w_kp, p_kp, t_k, t_kp, f0, g0, h0;
-    HC_DEV void load(const StepArgs &a, int i, int j, int k)
+    R u_km, v_km, w_kp, p_kp, t_k, t_kp, f0, g0, h0;
+    HC_DEV void load(const StepArgsT<R> &a, int i, int j, int k)
     {
-        Pt g(a, i, j, k);
+        PtT<R> g(a, i, j, k);
         u_km = g.U(0, 0, -1);
         v_km = g.V(0, 0, -1);
         const bool in = k + 1 < a.nz;
-        w_kp = in ? g.W(0, 0, 1) : 0.0;
+        w_kp = in ? g.W(0, 0, 1) : R(0.0);
         p_kp = g.P(0, 0, 1);
-        t_k = a.isscalar ? g.T(0, 0, 0) : 0.0;
-        t_kp = (a.isscalar && in) ? g.T(0, 0, 1) : 0.0;
+        t_k = a.isscalar ? g.T(0, 0, 0) : R(0.0);
+        t_kp = (a.isscalar && in) ? g.T(0, 0, 1) : R(0.0);
         f0 = g.f0();
         g0 = g.g0();
         h0 = g.h0();
     }
 };
 
-template <int NX, int TY>
-struct Ld : Interp<Ld<NX, TY>> {
+template <int NX, int TY, class R>
+struct Ld : Interp<Ld<NX, TY, R>> {
+    using Real = R;
     using Tl = RhsTile<NX, TY>;
-    const StepArgs &A;
-    const double *sm;
+    const StepArgsT<R> &A;
+    const R *sm;
     int i, jj, j, k;
-    double u_km, v_km, w_kp, p_kp, t_k, t_kp, f0_, g0_, h0_;
-    HC_DEV Ld(const StepArgs &a, const double *s, int i_, int jj_, int j_, int k_, const PointVals &pv)
+    R u_km, v_km, w_kp, p_kp, t_k, t_kp, f0_, g0_, h0_;
+    HC_DEV Ld(const StepArgsT<R> &a, const R *s, int i_, int jj_, int j_, int k_, const PointVals<R> &pv)
         : A(a), sm(s), i(i_), jj(jj_), j(j_), k(k_), u_km(pv.u_km), v_km(pv.v_km), w_kp(pv.w_kp),
           p_kp(pv.p_kp), t_k(pv.t_k), t_kp(pv.t_kp), f0_(pv.f0), g0_(pv.g0), h0_(pv.h0) {}
-    HC_DEV double f0() const { return f0_; }
-    HC_DEV double g0() const { return g0_; }
-    HC_DEV double h0() const { return h0_; }
-    HC_DEV double row(int base, int r, int di) const { return sm[(base + r) * NX + wrapx<NX>(i + di)]; }
-    HC_DEV double U(int di, int dj, int dk) const
+    HC_DEV R f0() const { return f0_; }
+    HC_DEV R g0() const { return g0_; }
+    HC_DEV R h0() const { return h0_; }
+    HC_DEV R row(int base, int r, int di) const { return sm[(base + r) * NX + wrapx<NX>(i + di)]; }
+    HC_DEV R U(int di, int dj, int dk) const
     {
         if (dk == 0) return row(Tl::OU, jj + 3 + dj, di);
         if (dk > 0) return row(Tl::OUP, jj + dj, di);
         return u_km;
     }
-    HC_DEV double V(int di, int dj, int dk) const
+    HC_DEV R V(int di, int dj, int dk) const
     {
         if (dk == 0) return row(Tl::OV, jj + 3 + dj, di);
         if (dk > 0) return row(Tl::OVP, jj + 2 + dj, di);
         return v_km;
     }
-    HC_DEV double W(int di, int dj, int dk) const
+    HC_DEV R W(int di, int dj, int dk) const
     {
         if (dk == 0) return row(Tl::OW, jj + 3 + dj, di);
         if (dk < 0) return row(Tl::OWM, jj + 1 + dj, di);
         return w_kp;
     }
-    HC_DEV double P(int di, int dj, int dk) const
+    HC_DEV R P(int di, int dj, int dk) const
     {
         if (dk == 0) return row(Tl::OP, jj + 1 + dj, di);
         return p_kp;
     }
-    HC_DEV double T(int, int, int dk) const { return dk == 0 ? t_k : t_kp; }
-    HC_DEV double lev(int tab, int kk) const { return A.lev[tab * A.nz + kk]; }
+    HC_DEV R T(int, int, int dk) const { return dk == 0 ? t_k : t_kp; }
+    HC_DEV R lev(int tab, int kk) const { return A.lev[tab * A.nz + kk]; }
     HC_DEV size_t center() const { return ((size_t)k * A.ny + j) * NX + i; }
 };
 
@@ -622,8 +653,8 @@ struct Tile2 {
 };
 
 // Raw row r of a Tile2 at column i (interior values of level k).
-template <int NX, int TY>
-HC_DEV double tile2_raw(const StepArgs &A, int r, int i, int j0, int k)
+template <int NX, int TY, class R>
+HC_DEV R tile2_raw(const StepArgsT<R> &A, int r, int i, int j0, int k)
 {
     using Tl = Tile2<NX, TY>;
     auto at = [&](int jj) -> size_t { return ((size_t)k * A.ny + wrapi(jj, A.ny)) * NX + i; };
@@ -635,44 +666,45 @@ HC_DEV double tile2_raw(const StepArgs &A, int r, int i, int j0, int k)
 
 // x interpolants on one LDS row: to the face between i-1 and i (uhx form)
 // and between i and i+1 (vhx / whx form); y interpolants on a column of rows.
-template <int CDS, int NX>
-HC_DEV double ixm(const double *row, int i)
+template <int CDS, int NX, class R>
+HC_DEV R ixm(const R *row, int i)
 {
-    if (CDS == 1) return 0.5 * (row[wrapx<NX>(i)] + row[wrapx<NX>(i - 1)]);
-    return (-row[wrapx<NX>(i + 1)] + 9.0 * row[wrapx<NX>(i)] + 9.0 * row[wrapx<NX>(i - 1)] - row[wrapx<NX>(i - 2)]) / 16.0;
+    if (CDS == 1) return R(0.5) * (row[wrapx<NX>(i)] + row[wrapx<NX>(i - 1)]);
+    return (-row[wrapx<NX>(i + 1)] + R(9.0) * row[wrapx<NX>(i)] + R(9.0) * row[wrapx<NX>(i - 1)] - row[wrapx<NX>(i - 2)]) / R(16.0);
 }
-template <int CDS, int NX>
-HC_DEV double ixp(const double *row, int i)
+template <int CDS, int NX, class R>
+HC_DEV R ixp(const R *row, int i)
 {
-    if (CDS == 1) return 0.5 * (row[wrapx<NX>(i + 1)] + row[wrapx<NX>(i)]);
-    return (-row[wrapx<NX>(i + 2)] + 9.0 * row[wrapx<NX>(i + 1)] + 9.0 * row[wrapx<NX>(i)] - row[wrapx<NX>(i - 1)]) / 16.0;
+    if (CDS == 1) return R(0.5) * (row[wrapx<NX>(i + 1)] + row[wrapx<NX>(i)]);
+    return (-row[wrapx<NX>(i + 2)] + R(9.0) * row[wrapx<NX>(i + 1)] + R(9.0) * row[wrapx<NX>(i)] - row[wrapx<NX>(i - 1)]) / R(16.0);
 }
 // col points at row r of the staged block, i at column i: value at row r+d is col[d*NX]
-template <int CDS, int NX>
-HC_DEV double iym(const double *col)  // vhy form: between r-1 and r
+template <int CDS, int NX, class R>
+HC_DEV R iym(const R *col)  // vhy form: between r-1 and r
 {
-    if (CDS == 1) return 0.5 * (col[0] + col[-NX]);
-    return (-col[NX] + 9.0 * col[0] + 9.0 * col[-NX] - col[-2 * NX]) / 16.0;
+    if (CDS == 1) return R(0.5) * (col[0] + col[-NX]);
+    return (-col[NX] + R(9.0) * col[0] + R(9.0) * col[-NX] - col[-2 * NX]) / R(16.0);
 }
-template <int CDS, int NX>
-HC_DEV double iyp(const double *col)  // uhy / why form: between r and r+1
+template <int CDS, int NX, class R>
+HC_DEV R iyp(const R *col)  // uhy / why form: between r and r+1
 {
-    if (CDS == 1) return 0.5 * (col[NX] + col[0]);
-    return (-col[2 * NX] + 9.0 * col[NX] + 9.0 * col[0] - col[-NX]) / 16.0;
+    if (CDS == 1) return R(0.5) * (col[NX] + col[0]);
+    return (-col[2 * NX] + R(9.0) * col[NX] + R(9.0) * col[0] - col[-NX]) / R(16.0);
 }
 
 // Single-point values of one tile point, loaded ahead of the staging barrier:
 // the k-1 / k+1 neighbours, the previous right-hand sides and w at level k-1
 // where the k-1 interpolants whx, why need it.
+template <class R>
 struct PointVals2 {
-    double u_km, v_km, u_kp, v_kp, w_kp, p_kp, t_k, t_kp, f0, g0, h0;
-    double wkx[4];  // w(k-1) at x-1, x, x+1, x+2 (row j)
-    double wky[3];  // w(k-1) at rows j-1, j+1, j+2 (column x)
+    R u_km, v_km, u_kp, v_kp, w_kp, p_kp, t_k, t_kp, f0, g0, h0;
+    R wkx[4];  // w(k-1) at x-1, x, x+1, x+2 (row j)
+    R wky[3];  // w(k-1) at rows j-1, j+1, j+2 (column x)
     // Every load is issued from a clamped level before any value is used;
     // the wall rules of update_boundary_uvw / _p / _t
     // (src/module_boundary.f90:33-196) are then applied by selects on the
     // block-uniform level k.
-    HC_DEV void load(const StepArgs &a, int i, int j, int k, bool with_old = true)
+    HC_DEV void load(const StepArgsT<R> &a, int i, int j, int k, bool with_old = true)
     {
         const int nx = a.nx, ny = a.ny, nz = a.nz;
         const bool bot = k == 0, top = k == nz - 1;
@@ -681,166 +713,166 @@ struct PointVals2 {
             return ((unsigned)kk * (unsigned)ny + (unsigned)wrapi(jj, ny)) * (unsigned)nx + (unsigned)wrapi(ii, nx);
         };
         const unsigned ck = at(k, j, i), ckm = at(km, j, i), ckp = at(kp, j, i);
-        const double ukm = a.u[ckm], vkm = a.v[ckm], ukp = a.u[ckp], vkp = a.v[ckp];
-        const double wkp = a.w[ckp], pkp = a.p[ckp];
-        const double tk = a.isscalar ? a.t[ck] : 0.0, tkp = a.isscalar ? a.t[ckp] : 0.0;
-        double o0 = 0.0, o1 = 0.0, o2 = 0.0;
+        const R ukm = a.u[ckm], vkm = a.v[ckm], ukp = a.u[ckp], vkp = a.v[ckp];
+        const R wkp = a.w[ckp], pkp = a.p[ckp];
+        const R tk = a.isscalar ? a.t[ck] : R(0.0), tkp = a.isscalar ? a.t[ckp] : R(0.0);
+        R o0 = R(0.0), o1 = R(0.0), o2 = R(0.0);
         if (with_old && a.use_f0) { o0 = a.ff0[ck]; o1 = a.gg0[ck]; o2 = a.hh0[ck]; }
-        double wx[4], wy[3];
+        R wx[4], wy[3];
 #pragma unroll
         for (int d = 0; d < 4; ++d) wx[d] = a.w[at(km, j, i + d - 1)];
         wy[0] = a.w[at(km, j - 1, i)];
         wy[1] = a.w[at(km, j + 1, i)];
         wy[2] = a.w[at(km, j + 2, i)];
-        const double um = a.u_mrf;
-        u_km = bot ? -ukm - 2.0 * um : ukm;
+        const R um = a.u_mrf;
+        u_km = bot ? -ukm - R(2.0) * um : ukm;
         v_km = bot ? -vkm : vkm;
-        u_kp = top ? (a.ochannel == 0 ? -ukp - 2.0 * um : ukp) : ukp;
+        u_kp = top ? (a.ochannel == 0 ? -ukp - R(2.0) * um : ukp) : ukp;
         v_kp = top ? (a.ochannel == 0 ? -vkp : vkp) : vkp;
-        w_kp = (top || k + 1 == nz - 1) ? 0.0 : wkp;
+        w_kp = (top || k + 1 == nz - 1) ? R(0.0) : wkp;
         p_kp = pkp;
         t_k = tk;
-        t_kp = top ? 0.0 : tkp;
+        t_kp = top ? R(0.0) : tkp;
         f0 = o0;
         g0 = o1;
         h0 = o2;
 #pragma unroll
-        for (int d = 0; d < 4; ++d) wkx[d] = bot ? 0.0 : wx[d];
+        for (int d = 0; d < 4; ++d) wkx[d] = bot ? R(0.0) : wx[d];
 #pragma unroll
-        for (int d = 0; d < 3; ++d) wky[d] = bot ? 0.0 : wy[d];
+        for (int d = 0; d < 3; ++d) wky[d] = bot ? R(0.0) : wy[d];
     }
 };
 
 // Phase 2 of the two-phase tile at point (i, j0+jj): ff, gg, hh and
 // qu, qv, qw.  Same terms and coefficients as momentum_point.
-template <int CDS, int NX, int TY>
-HC_DEV void momentum_tile(const StepArgs &A, const double *sm, int i, int jj, int k, const PointVals2 &pv, double dpx,
-                          double &f, double &g, double &h, double &qu, double &qv, double &qw)
+template <int CDS, int NX, int TY, class R>
+HC_DEV void momentum_tile(const StepArgsT<R> &A, const R *sm, int i, int jj, int k, const PointVals2<R> &pv, R dpx,
+                          R &f, R &g, R &h, R &qu, R &qv, R &qw)
 {
     using Tl = Tile2<NX, TY>;
-    auto R = [&](int base, int r, int di) -> double { return sm[(base + r) * NX + wrapx<NX>(i + di)]; };
-    auto U = [&](int di, int dj) { return R(Tl::OU, jj + 3 + dj, di); };
-    auto V = [&](int di, int dj) { return R(Tl::OV, jj + 3 + dj, di); };
-    auto W = [&](int di, int dj) { return R(Tl::OW, jj + 3 + dj, di); };
-    auto P = [&](int di, int dj) { return R(Tl::OP, jj + 1 + dj, di); };
-    auto UHX = [&](int di) { return R(Tl::OUHX, jj, di); };
-    auto WHX = [&](int di) { return R(Tl::OWHX, jj, di); };
-    auto FUW = [&](int di) { return R(Tl::OFUW, jj, di); };
-    auto VHX = [&](int di, int dj) { return R(Tl::OVHX, jj + 1 + dj, di); };
-    auto VHY = [&](int dj) { return R(Tl::OVHY, jj + 1 + dj, 0); };
-    auto FUV = [&](int di, int dj) { return R(Tl::OFUV, jj + 2 + dj, di); };
-    auto FWV = [&](int dj) { return R(Tl::OFWV, jj + 2 + dj, 0); };
-    const double um = A.u_mrf;
-    const double idz = A.lev[LV_IDZ * A.nz + k], idzt = A.lev[LV_IDZT * A.nz + k], idzb = A.lev[LV_IDZB * A.nz + k];
-    const double u0 = U(0, 0), v0 = V(0, 0), w0 = W(0, 0);
+    auto RW = [&](int base, int r, int di) -> R { return sm[(base + r) * NX + wrapx<NX>(i + di)]; };
+    auto U = [&](int di, int dj) { return RW(Tl::OU, jj + 3 + dj, di); };
+    auto V = [&](int di, int dj) { return RW(Tl::OV, jj + 3 + dj, di); };
+    auto W = [&](int di, int dj) { return RW(Tl::OW, jj + 3 + dj, di); };
+    auto P = [&](int di, int dj) { return RW(Tl::OP, jj + 1 + dj, di); };
+    auto UHX = [&](int di) { return RW(Tl::OUHX, jj, di); };
+    auto WHX = [&](int di) { return RW(Tl::OWHX, jj, di); };
+    auto FUW = [&](int di) { return RW(Tl::OFUW, jj, di); };
+    auto VHX = [&](int di, int dj) { return RW(Tl::OVHX, jj + 1 + dj, di); };
+    auto VHY = [&](int dj) { return RW(Tl::OVHY, jj + 1 + dj, 0); };
+    auto FUV = [&](int di, int dj) { return RW(Tl::OFUV, jj + 2 + dj, di); };
+    auto FWV = [&](int dj) { return RW(Tl::OFWV, jj + 2 + dj, 0); };
+    const R um = A.u_mrf;
+    const R idz = A.lev[LV_IDZ * A.nz + k], idzt = A.lev[LV_IDZT * A.nz + k], idzb = A.lev[LV_IDZB * A.nz + k];
+    const R u0 = U(0, 0), v0 = V(0, 0), w0 = W(0, 0);
     // k-1 flux products at the point (uhz*whx, vhz*why), wall rules at k-1 = -1
-    double fuw_km, fwv_km;
+    R fuw_km, fwv_km;
     if (k == 0) {
-        fuw_km = -um * 0.0;
-        fwv_km = 0.0 * 0.0;
+        fuw_km = -um * R(0.0);
+        fwv_km = R(0.0) * R(0.0);
     } else {
-        const double lt = A.lev[LV_LT * A.nz + k - 1];
-        const double uhz = u0 * lt + pv.u_km * (1.0 - lt);
-        const double vhz = v0 * lt + pv.v_km * (1.0 - lt);
-        double whx, why;
+        const R lt = A.lev[LV_LT * A.nz + k - 1];
+        const R uhz = u0 * lt + pv.u_km * (R(1.0) - lt);
+        const R vhz = v0 * lt + pv.v_km * (R(1.0) - lt);
+        R whx, why;
         if (CDS == 1) {
-            whx = 0.5 * (pv.wkx[2] + pv.wkx[1]);
-            why = 0.5 * (pv.wky[1] + pv.wkx[1]);
+            whx = R(0.5) * (pv.wkx[2] + pv.wkx[1]);
+            why = R(0.5) * (pv.wky[1] + pv.wkx[1]);
         } else {
-            whx = (-pv.wkx[3] + 9.0 * pv.wkx[2] + 9.0 * pv.wkx[1] - pv.wkx[0]) / 16.0;
-            why = (-pv.wky[2] + 9.0 * pv.wky[1] + 9.0 * pv.wkx[1] - pv.wky[0]) / 16.0;
+            whx = (-pv.wkx[3] + R(9.0) * pv.wkx[2] + R(9.0) * pv.wkx[1] - pv.wkx[0]) / R(16.0);
+            why = (-pv.wky[2] + R(9.0) * pv.wky[1] + R(9.0) * pv.wkx[1] - pv.wky[0]) / R(16.0);
         }
         fuw_km = uhz * whx;
         fwv_km = vhz * why;
     }
     // ---- U equation (:55-161)
     {
-        double uux, uvy, ux, uxx, uyy, dpdx;
+        R uux, uvy, ux, uxx, uyy, dpdx;
         if (CDS == 1) {
-            const double h1 = UHX(1), h0 = UHX(0);
+            const R h1 = UHX(1), h0 = UHX(0);
             uux = (sq(h1) - sq(h0)) * A.idx;
             uvy = (FUV(0, 0) - FUV(0, -1)) * A.idy;
             ux = (h1 - h0) * A.idx;
-            uxx = (U(1, 0) - 2.0 * u0 + U(-1, 0)) * A.idx2;
-            uyy = (U(0, 1) - 2.0 * u0 + U(0, -1)) * A.idy2;
+            uxx = (U(1, 0) - R(2.0) * u0 + U(-1, 0)) * A.idx2;
+            uyy = (U(0, 1) - R(2.0) * u0 + U(0, -1)) * A.idy2;
             dpdx = (P(1, 0) - P(0, 0)) * A.idx;
         } else {
-            const double h2 = UHX(2), h1 = UHX(1), h0 = UHX(0), hm = UHX(-1);
-            uux = (-sq(h2) + 27.0 * sq(h1) - 27.0 * sq(h0) + sq(hm)) * A.c24x;
-            uvy = (-FUV(0, 1) + 27.0 * FUV(0, 0) - 27.0 * FUV(0, -1) + FUV(0, -2)) * A.c24y;
-            ux = (-h2 + 27.0 * h1 - 27.0 * h0 + hm) * A.c24x;
-            uxx = (-U(2, 0) + 16.0 * U(1, 0) - 30.0 * u0 + 16.0 * U(-1, 0) - U(-2, 0)) * A.c12x2;
-            uyy = (-U(0, 2) + 16.0 * U(0, 1) - 30.0 * u0 + 16.0 * U(0, -1) - U(0, -2)) * A.c12y2;
-            dpdx = (-P(2, 0) + 27.0 * P(1, 0) - 27.0 * P(0, 0) + P(-1, 0)) * A.c24x;
+            const R h2 = UHX(2), h1 = UHX(1), h0 = UHX(0), hm = UHX(-1);
+            uux = (-sq(h2) + R(27.0) * sq(h1) - R(27.0) * sq(h0) + sq(hm)) * A.c24x;
+            uvy = (-FUV(0, 1) + R(27.0) * FUV(0, 0) - R(27.0) * FUV(0, -1) + FUV(0, -2)) * A.c24y;
+            ux = (-h2 + R(27.0) * h1 - R(27.0) * h0 + hm) * A.c24x;
+            uxx = (-U(2, 0) + R(16.0) * U(1, 0) - R(30.0) * u0 + R(16.0) * U(-1, 0) - U(-2, 0)) * A.c12x2;
+            uyy = (-U(0, 2) + R(16.0) * U(0, 1) - R(30.0) * u0 + R(16.0) * U(0, -1) - U(0, -2)) * A.c12y2;
+            dpdx = (-P(2, 0) + R(27.0) * P(1, 0) - R(27.0) * P(0, 0) + P(-1, 0)) * A.c24x;
         }
-        const double uwz = (FUW(0) - fuw_km) * idz;
-        const double uzz = ((pv.u_kp - u0) * idzt - (u0 - pv.u_km) * idzb) * idz;
+        const R uwz = (FUW(0) - fuw_km) * idz;
+        const R uzz = ((pv.u_kp - u0) * idzt - (u0 - pv.u_km) * idzb) * idz;
         if (A.dp_opt == 3)
-            f = -uux - uvy - uwz - um * ux + A.fc * (0.5 * (VHX(0, 0) + VHX(0, -1)) - A.vg);
+            f = -uux - uvy - uwz - um * ux + A.fc * (R(0.5) * (VHX(0, 0) + VHX(0, -1)) - A.vg);
         else
             f = -uux - uvy - uwz + dpx - um * ux;
-        qu = u0 + A.dt * (A.ra * f + A.rb * pv.f0) + A.rc * 0.5 * A.dt * A.nu * (uxx + uyy + uzz) - A.rc * A.dt * dpdx;
+        qu = u0 + A.dt * (A.ra * f + A.rb * pv.f0) + A.rc * R(0.5) * A.dt * A.nu * (uxx + uyy + uzz) - A.rc * A.dt * dpdx;
     }
     // ---- V equation (:166-270)
     {
-        double vux, vvy, vx, vxx, vyy, dpdy;
+        R vux, vvy, vx, vxx, vyy, dpdy;
         if (CDS == 1) {
             vux = (FUV(0, 0) - FUV(-1, 0)) * A.idx;
             vvy = (sq(VHY(1)) - sq(VHY(0))) * A.idy;
             vx = (VHX(0, 0) - VHX(-1, 0)) * A.idx;
-            vxx = (V(1, 0) - 2.0 * v0 + V(-1, 0)) * A.idx2;
-            vyy = (V(0, 1) - 2.0 * v0 + V(0, -1)) * A.idy2;
+            vxx = (V(1, 0) - R(2.0) * v0 + V(-1, 0)) * A.idx2;
+            vyy = (V(0, 1) - R(2.0) * v0 + V(0, -1)) * A.idy2;
             dpdy = (P(0, 1) - P(0, 0)) * A.idy;
         } else {
-            vux = (-FUV(1, 0) + 27.0 * FUV(0, 0) - 27.0 * FUV(-1, 0) + FUV(-2, 0)) * A.c24x;
-            vvy = (-sq(VHY(2)) + 27.0 * sq(VHY(1)) - 27.0 * sq(VHY(0)) + sq(VHY(-1))) * A.c24y;
-            vx = (-VHX(1, 0) + 27.0 * VHX(0, 0) - 27.0 * VHX(-1, 0) + VHX(-2, 0)) * A.c24x;
-            vxx = (-V(2, 0) + 16.0 * V(1, 0) - 30.0 * v0 + 16.0 * V(-1, 0) - V(-2, 0)) * A.c12x2;
-            vyy = (-V(0, 2) + 16.0 * V(0, 1) - 30.0 * v0 + 16.0 * V(0, -1) - V(0, -2)) * A.c12y2;
-            dpdy = (-P(0, 2) + 27.0 * P(0, 1) - 27.0 * P(0, 0) + P(0, -1)) * A.c24y;
+            vux = (-FUV(1, 0) + R(27.0) * FUV(0, 0) - R(27.0) * FUV(-1, 0) + FUV(-2, 0)) * A.c24x;
+            vvy = (-sq(VHY(2)) + R(27.0) * sq(VHY(1)) - R(27.0) * sq(VHY(0)) + sq(VHY(-1))) * A.c24y;
+            vx = (-VHX(1, 0) + R(27.0) * VHX(0, 0) - R(27.0) * VHX(-1, 0) + VHX(-2, 0)) * A.c24x;
+            vxx = (-V(2, 0) + R(16.0) * V(1, 0) - R(30.0) * v0 + R(16.0) * V(-1, 0) - V(-2, 0)) * A.c12x2;
+            vyy = (-V(0, 2) + R(16.0) * V(0, 1) - R(30.0) * v0 + R(16.0) * V(0, -1) - V(0, -2)) * A.c12y2;
+            dpdy = (-P(0, 2) + R(27.0) * P(0, 1) - R(27.0) * P(0, 0) + P(0, -1)) * A.c24y;
         }
-        const double vwz = (FWV(0) - fwv_km) * idz;
-        const double vzz = ((pv.v_kp - v0) * idzt - (v0 - pv.v_km) * idzb) * idz;
+        const R vwz = (FWV(0) - fwv_km) * idz;
+        const R vzz = ((pv.v_kp - v0) * idzt - (v0 - pv.v_km) * idzb) * idz;
         if (A.dp_opt == 3) {
             // uhy at (i, j) and (i-1, j), from the staged u rows
-            const double *cu = sm + (Tl::OU + jj + 3) * NX;
-            const double a0 = iyp<CDS, NX>(cu + wrapx<NX>(i)), a1 = iyp<CDS, NX>(cu + wrapx<NX>(i - 1));
-            g = -vux - vvy - vwz - um * vx + A.fc * (A.ug - 0.5 * (a0 + a1) - um);
+            const R *cu = sm + (Tl::OU + jj + 3) * NX;
+            const R a0 = iyp<CDS, NX>(cu + wrapx<NX>(i)), a1 = iyp<CDS, NX>(cu + wrapx<NX>(i - 1));
+            g = -vux - vvy - vwz - um * vx + A.fc * (A.ug - R(0.5) * (a0 + a1) - um);
         } else {
             g = -vux - vvy - vwz - um * vx;
         }
-        qv = v0 + A.dt * (A.ra * g + A.rb * pv.g0) + A.rc * 0.5 * A.dt * A.nu * (vxx + vyy + vzz) - A.rc * A.dt * dpdy;
+        qv = v0 + A.dt * (A.ra * g + A.rb * pv.g0) + A.rc * R(0.5) * A.dt * A.nu * (vxx + vyy + vzz) - A.rc * A.dt * dpdy;
     }
     // ---- W equation (:274-370), k < kend3
     if (k < A.nz - 1) {
-        double wux, wvy, wx, wxx, wyy;
+        R wux, wvy, wx, wxx, wyy;
         if (CDS == 1) {
             wux = (FUW(0) - FUW(-1)) * A.idx;
             wvy = (FWV(0) - FWV(-1)) * A.idy;
             wx = (WHX(0) - WHX(-1)) * A.idx;
-            wxx = (W(1, 0) - 2.0 * w0 + W(-1, 0)) * A.idx2;
-            wyy = (W(0, 1) - 2.0 * w0 + W(0, -1)) * A.idy2;
+            wxx = (W(1, 0) - R(2.0) * w0 + W(-1, 0)) * A.idx2;
+            wyy = (W(0, 1) - R(2.0) * w0 + W(0, -1)) * A.idy2;
         } else {
-            wux = (-FUW(1) + 27.0 * FUW(0) - 27.0 * FUW(-1) + FUW(-2)) * A.c24x;
-            wvy = (-FWV(1) + 27.0 * FWV(0) - 27.0 * FWV(-1) + FWV(-2)) * A.c24y;
-            wx = (-WHX(1) + 27.0 * WHX(0) - 27.0 * WHX(-1) + WHX(-2)) * A.c24x;
-            wxx = (-W(2, 0) + 16.0 * W(1, 0) - 30.0 * w0 + 16.0 * W(-1, 0) - W(-2, 0)) * A.c12x2;
-            wyy = (-W(0, 2) + 16.0 * W(0, 1) - 30.0 * w0 + 16.0 * W(0, -1) - W(0, -2)) * A.c12y2;
+            wux = (-FUW(1) + R(27.0) * FUW(0) - R(27.0) * FUW(-1) + FUW(-2)) * A.c24x;
+            wvy = (-FWV(1) + R(27.0) * FWV(0) - R(27.0) * FWV(-1) + FWV(-2)) * A.c24y;
+            wx = (-WHX(1) + R(27.0) * WHX(0) - R(27.0) * WHX(-1) + WHX(-2)) * A.c24x;
+            wxx = (-W(2, 0) + R(16.0) * W(1, 0) - R(30.0) * w0 + R(16.0) * W(-1, 0) - W(-2, 0)) * A.c12x2;
+            wyy = (-W(0, 2) + R(16.0) * W(0, 1) - R(30.0) * w0 + R(16.0) * W(0, -1) - W(0, -2)) * A.c12y2;
         }
-        const double wp = pv.w_kp, wm = pv.wkx[1];
-        const double a1 = sq(0.5 * (w0 + wp)), a2 = sq(0.5 * (w0 + wm));
-        const double wwz = (a1 - a2) * idzt;
-        const double wzz = ((wp - w0) * A.lev[LV_IDZ * A.nz + k + 1] - (w0 - wm) * idz) * idzt;
+        const R wp = pv.w_kp, wm = pv.wkx[1];
+        const R a1 = sq(R(0.5) * (w0 + wp)), a2 = sq(R(0.5) * (w0 + wm));
+        const R wwz = (a1 - a2) * idzt;
+        const R wzz = ((wp - w0) * A.lev[LV_IDZ * A.nz + k + 1] - (w0 - wm) * idz) * idzt;
         if (A.isscalar) {
-            const double lt = A.lev[LV_LT * A.nz + k];
-            h = -wux - wvy - wwz - um * wx + A.got * ((1.0 - lt) * pv.t_k + lt * pv.t_kp - A.t_ref);
+            const R lt = A.lev[LV_LT * A.nz + k];
+            h = -wux - wvy - wwz - um * wx + A.got * ((R(1.0) - lt) * pv.t_k + lt * pv.t_kp - A.t_ref);
         } else {
             h = -wux - wvy - wwz - um * wx;
         }
-        const double dpdz = (pv.p_kp - P(0, 0)) * idzt;
-        qw = w0 + A.dt * (A.ra * h + A.rb * pv.h0) + A.rc * 0.5 * A.dt * A.nu * (wxx + wyy + wzz) - A.rc * A.dt * dpdz;
+        const R dpdz = (pv.p_kp - P(0, 0)) * idzt;
+        qw = w0 + A.dt * (A.ra * h + A.rb * pv.h0) + A.rc * R(0.5) * A.dt * A.nu * (wxx + wyy + wzz) - A.rc * A.dt * dpdz;
     } else {
-        h = 0.0;
-        qw = 0.0;
+        h = R(0.0);
+        qw = R(0.0);
     }
 }

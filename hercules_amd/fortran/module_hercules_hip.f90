@@ -25,6 +25,7 @@ module module_hercules_hip
         real(c_double) :: lx, ly, lz, zstretch, cdamp, nu, alpha, got, fc, ug, vg, dt, &
                           u_mrf, t_ref, tbot, ttop, ri_str, ri_end
         integer(c_int) :: wall_rows
+        integer(c_int) :: precision   ! 0 FP64, 1 mixed FP32 + FP64 low wavenumbers
     end type
 
     integer(c_int), parameter :: HC_U = 0, HC_V = 1, HC_W = 2, HC_P = 3, HC_T = 4

@@ -20,7 +20,7 @@ DBL_FIELDS = ("lx", "ly", "lz", "zstretch", "cdamp", "nu", "alpha", "got", "fc",
 class CParams(C.Structure):
     """ctypes image of ``hc_params`` (include/hercules_amd.h)."""
     _fields_ = ([(n, C.c_int) for n in INT_FIELDS] + [(n, C.c_double) for n in DBL_FIELDS] +
-                [("wall_rows", C.c_int)])
+                [("wall_rows", C.c_int), ("precision", C.c_int)])
 
 
 @dataclass
@@ -79,6 +79,9 @@ class Params:
     # extension, no namelist counterpart: 1 gives every mode the implicit
     # wall rows the reference gives the (0,0) mode only
     wall_rows: int = 0
+    # extension, no namelist counterpart: 0 FP64, 1 mixed FP32 + FP64 low
+    # wavenumbers (hc_params.precision)
+    precision: int = 0
 
     def validate(self):
         """The checks of initiate_parameters (src/module_parameters.f90:256-270)."""
@@ -97,6 +100,7 @@ class Params:
         for n in DBL_FIELDS:
             setattr(p, n, float(getattr(self, n)))
         p.wall_rows = int(self.wall_rows)
+        p.precision = int(self.precision)
         return p
 
     def as_dict(self):

@@ -37,7 +37,7 @@ typedef struct hc_params {
     int cds;                 /* 1 FD2, 2 FD4, 3 spectral (2/3-rule dealiasing) */
     int dts;                 /* 1 AB2, 2 RK3 */
     int issk;                /* skew-symmetric form (spectral only) */
-    int isdamp, nzdamp;      /* Rayleigh damping (not yet supported: must be 0) */
+    int isdamp, nzdamp;      /* Rayleigh damping */
     int isscalar;            /* temperature on/off */
     int is_ri_var;           /* Richardson number ramp */
     int imax;                /* number of steps (used by the Ri ramp) */
@@ -52,7 +52,18 @@ typedef struct hc_params {
      * the explicit half at the wall; 0 is unstable when
      * rk_c*dt*nu/(2 dz_min^2) > 1. */
     int wall_rows;
+    /* Arithmetic of the step (no namelist counterpart; the reference is FP64
+     * only).  HC_FP64 (0): everything in FP64.  HC_MIXED (1): fields, spectra,
+     * right-hand-side stencils, FFTs and the tridiagonal solves of all but the
+     * lowest wavenumbers in FP32; the column solves of the low-wavenumber
+     * blocks (kx index l < 8, |ky index| <= 8, which contain the kx = ky = 0
+     * mean mode and its Neumann compatibility condition) and the mean
+     * pressure gradient / bulk velocity update in FP64.  HC_MIXED needs
+     * cds = 1 or 2 and isdamp = 0 (HC_ERR_UNSUPPORTED otherwise). */
+    int precision;
 } hc_params;
+
+enum hc_precision { HC_FP64 = 0, HC_MIXED = 1 };
 
 typedef struct hc_ctx hc_ctx;
 
@@ -153,9 +164,10 @@ int hc_wall_shear(hc_ctx *ctx, double *tau_bottom, double *tau_top);
 int hc_get_scalar(hc_ctx *ctx, int id, double *value);
 
 /* The HIP stream (hipStream_t) all work of this context is queued on, and
- * the device address of a field's interior (nx, ny, nz) array. */
+ * the device address of a field's interior (nx, ny, nz) array: double, or
+ * float with precision = HC_MIXED. */
 void *hc_stream(hc_ctx *ctx);
-double *hc_field_device_ptr(hc_ctx *ctx, int field);
+void *hc_field_device_ptr(hc_ctx *ctx, int field);
 
 /* Wait for all queued work of the context. */
 int hc_synchronize(hc_ctx *ctx);
