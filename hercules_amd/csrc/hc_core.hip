@@ -133,9 +133,17 @@ template <int NX>
 constexpr int rhs2_ty() { return NX == 16 ? 16 : NX == 32 ? 32 : NX == 64 ? 16 : NX == 128 ? 8 : 4; }
 template <int NX>
 constexpr int rhs2_threads() { return NX * rhs2_ty<NX>(); }
+// waves per SIMD the register allocation must allow: with FP32 tiles two
+// 1024-thread blocks fit in the LDS of one CU when each thread keeps to 64
+// VGPRs
+#ifndef HC_RHS2_F32_WAVES
+#define HC_RHS2_F32_WAVES 8
+#endif
+template <int NX, class R>
+constexpr int rhs2_waves() { return sizeof(R) == 4 ? HC_RHS2_F32_WAVES : 1; }
 
 template <int CDS, int NX, class R>
-__global__ __launch_bounds__(rhs2_threads<NX>()) void k_rhs_mom2(StepArgsT<R> A)
+__global__ __launch_bounds__(rhs2_threads<NX>(), (rhs2_waves<NX, R>())) void k_rhs_mom2(StepArgsT<R> A)
 {
     using C = cplx<R>;
     constexpr int TY = rhs2_ty<NX>();
@@ -397,8 +405,11 @@ struct C2RArgsT {
 using C2RArgs = C2RArgsT<double>;
 // points per field of one inverse-x block
 constexpr int kC2RPoints = 1024;
+#ifndef HC_C2R_POINTS_F32
+#define HC_C2R_POINTS_F32 1024
+#endif
 template <class R>
-constexpr int c2r_points() { return kC2RPoints; }
+constexpr int c2r_points() { return sizeof(R) == 4 ? HC_C2R_POINTS_F32 : kC2RPoints; }
 template <int NX, int F, class R = double>
 constexpr int c2r_rows() { return c2r_points<R>() / NX / F > 0 ? c2r_points<R>() / NX / F : 1; }
 
