@@ -353,7 +353,7 @@ __global__ __launch_bounds__(kRhsThreads) void k_rhs_t(StepArgsT<typename Acc::R
         if (j < A.ny) {
             Acc q(A, i, j, k);
             R s;
-            temperature_point<CDS>(q, s, qt);
+            temperature_point<CDS>(q, s, qt, A.use_f0 ? A.ss0[q.center()] : R(0));
             A.ss[((size_t)k * A.ny + j) * A.nx + i] = s;
         }
         sqd[jj * NX + i] = qt;
@@ -363,6 +363,73 @@ __global__ __launch_bounds__(kRhsThreads) void k_rhs_t(StepArgsT<typename Acc::R
     for (int p = tid; p < ROWS * NL; p += kRhsThreads) {
         int jj = p / NL, l = p - jj * NL, j = j0 + jj;
         if (j < A.ny) A.s0[((size_t)k * A.ny + j) * NL + l] = r2c_post<NL>(sq_ + jj * NL, l, A.tw_nx, A.norm);
+    }
+}
+
+// Tiled temperature RHS + x R2C of qt, one point per thread: full x rows
+// j0..j0+TY-1 of level k; the t, u, v rows the stencils reach are staged in
+// LDS once (all loads issued before the first use), then the same LDS holds
+// the qt rows for the FFT.  Undamped substeps, nx <= 256.
+template <int CDS, int NX, class R>
+__global__ __launch_bounds__(rhs2_threads<NX>(), (rhs2_waves<NX, R>())) void k_rhs_t2(StepArgsT<R> A)
+{
+    using C = cplx<R>;
+    constexpr int TY = rhs2_ty<NX>(), T = rhs2_threads<NX>(), NL = NX / 2;
+    using Q = LdT<NX, TY, R>;
+    static_assert(Q::NROWS >= TY, "FFT rows must fit in the staging area");
+    __shared__ __align__(16) R sm[Q::NROWS * NX];
+    __shared__ C tws[NX];  // twiddles of the x transform: half-length, then R2C post
+    const int tid = threadIdx.x;
+    C tw_a = czero_t<C>(), tw_b = czero_t<C>();
+    if (tid < NL) { tw_a = A.tw_nl[tid]; tw_b = A.tw_nx[tid]; }
+    const int nty = A.ny / TY, nz = A.nz;
+    const int logical = xcd_remap(blockIdx.x, nty * nz);
+    const int jt = logical % nty, k = logical / nty, j0 = jt * TY;
+    const int i = tid % NX, jj = tid / NX, j = j0 + jj;
+    constexpr int NST = (Q::NROWS * NX + T - 1) / T;
+    R stv[NST];
+#pragma unroll
+    for (int m = 0; m < NST; ++m) {
+        const int q = tid + m * T;
+        stv[m] = R(0);
+        if ((Q::NROWS * NX) % T == 0 || q < Q::NROWS * NX) {
+            const int r = q / NX, x = q - r * NX;
+            const R *f = r < Q::OU ? A.t : r < Q::OV ? A.u : A.v;
+            const int jr = r < Q::OU ? j0 - 3 + r : r < Q::OV ? j0 + (r - Q::OU) : j0 - 2 + (r - Q::OV);
+            stv[m] = f[((unsigned)k * (unsigned)A.ny + (unsigned)wrapi(jr, A.ny)) * (unsigned)NX + (unsigned)x];
+        }
+    }
+    const size_t plane = (size_t)A.ny * NX, ck = (size_t)k * plane + (size_t)j * NX + i;
+    const R tkm = k > 0 ? A.t[ck - plane] : R(0), tkp = k < nz - 1 ? A.t[ck + plane] : R(0);
+    const R wk = k < nz - 1 ? A.w[ck] : R(0), wkm = k > 0 ? A.w[ck - plane] : R(0);
+    const R s0 = A.use_f0 ? A.ss0[ck] : R(0);
+#pragma unroll
+    for (int m = 0; m < NST; ++m) {
+        const int q = tid + m * T;
+        if ((Q::NROWS * NX) % T == 0 || q < Q::NROWS * NX) sm[q] = stv[m];
+    }
+    if (tid < NL) { tws[tid] = tw_a; tws[NL + tid] = tw_b; }
+    __syncthreads();
+    R qt;
+    {
+        Q q(A, sm, i, jj, j, k);
+        const R tc = sm[(Q::OT + jj + 3) * NX + i];
+        q.t_km = k == 0 ? R(2.0) * A.tbot - tc : tkm;
+        q.t_kp = k == nz - 1 ? R(2.0) * A.ttop - tc : tkp;
+        q.w_k = wk;
+        q.w_km = wkm;
+        R s;
+        temperature_point<CDS>(q, s, qt, s0);
+        A.ss[ck] = s;
+    }
+    __syncthreads();  // staging no longer read: reuse it for the qt rows
+    sm[jj * NX + i] = qt;
+    __syncthreads();
+    C *sq_ = reinterpret_cast<C *>(sm);
+    lds_fft<NL, TY, NL, 1, T, -1>(sq_, tws, tid);
+    for (int p = tid; p < TY * NL; p += T) {
+        const int row = p / NL, l = p - row * NL;
+        A.s0[((size_t)k * A.ny + j0 + row) * NL + l] = r2c_post<NL>(sq_ + row * NL, l, tws + NL, A.norm);
     }
 }
 
@@ -1721,9 +1788,16 @@ static int substep_t(hc_ctx *c, int t_its, const View<R> &V)
             }
         }
         if (!damp) {
-#define HC_RT(NX_)                                                                                             \
-    if (P.cds == 1) hipLaunchKernelGGL((k_rhs_t<1, NX_, PtT<R>>), grow, dim3(kRhsThreads), 0, c->stream, B);   \
-    else hipLaunchKernelGGL((k_rhs_t<2, NX_, PtT<R>>), grow, dim3(kRhsThreads), 0, c->stream, B);
+#define HC_RT(NX_)                                                                                               \
+    if constexpr (NX_ <= 256) {                                                                                  \
+        dim3 g1((unsigned)(c->ny / rhs2_ty<NX_>() * c->nz));                                                     \
+        constexpr int T_ = rhs2_threads<NX_>();                                                                  \
+        if (P.cds == 1) hipLaunchKernelGGL((k_rhs_t2<1, NX_, R>), g1, dim3(T_), 0, c->stream, B);                \
+        else hipLaunchKernelGGL((k_rhs_t2<2, NX_, R>), g1, dim3(T_), 0, c->stream, B);                           \
+    } else {                                                                                                     \
+        if (P.cds == 1) hipLaunchKernelGGL((k_rhs_t<1, NX_, PtT<R>>), grow, dim3(kRhsThreads), 0, c->stream, B); \
+        else hipLaunchKernelGGL((k_rhs_t<2, NX_, PtT<R>>), grow, dim3(kRhsThreads), 0, c->stream, B);            \
+    }
             HC_DISPATCH_N(c->nx, HC_RT)
 #undef HC_RT
         }

@@ -450,7 +450,7 @@ HC_DEV void momentum_point(const Q &q, typename Q::Real dpx, typename Q::Real &f
 // Temperature right-hand side at one point, get_temperature_cnvdiff
 // (src/module_cnvdiff.f90:381-473), evaluated with the updated u, v, w.
 template <int CDS, class Q>
-HC_DEV void temperature_point(const Q &q, typename Q::Real &s, typename Q::Real &qt)
+HC_DEV void temperature_point(const Q &q, typename Q::Real &s, typename Q::Real &qt, typename Q::Real s0)
 {
     using Re = typename Q::Real;
     const auto &A = q.A;
@@ -475,7 +475,6 @@ HC_DEV void temperature_point(const Q &q, typename Q::Real &s, typename Q::Real 
     Re t0 = q.T(0, 0, 0);
     Re tzz = ((q.T(0, 0, 1) - t0) * q.lev(LV_IDZT, k) - (t0 - q.T(0, 0, -1)) * q.lev(LV_IDZB, k)) * q.lev(LV_IDZ, k);
     s = -tux - tvy - twz - A.u_mrf * tx;
-    Re s0 = A.use_f0 ? A.ss0[q.center()] : Re(0.0);
     qt = t0 + A.dt * (A.ra * s + A.rb * s0) + A.rc * Re(0.5) * A.dt * A.alpha * (txx + tyy + tzz);
 }
 
@@ -876,3 +875,31 @@ HC_DEV void momentum_tile(const StepArgsT<R> &A, const R *sm, int i, int jj, int
         qw = R(0.0);
     }
 }
+
+// LDS accessor of the tiled temperature right-hand side: t rows j0-3 ..
+// j0+TY+2, u rows j0 .. j0+TY-1 and v rows j0-2 .. j0+TY of level k, plus
+// the point's t and w at k-1 / k+1 with the wall rules of update_boundary_t
+// and update_boundary_uvw (src/module_boundary.f90:33-196) already applied.
+// Offers the offsets temperature_point reads.
+template <int NX, int TY, class R>
+struct LdT : Interp<LdT<NX, TY, R>> {
+    using Real = R;
+    static constexpr int OT = 0, OU = TY + 6, OV = OU + TY, NROWS = OV + TY + 3;
+    const StepArgsT<R> &A;
+    const R *sm;
+    int i, jj, j, k;
+    R t_km, t_kp, w_k, w_km;
+    HC_DEV LdT(const StepArgsT<R> &a, const R *s, int i_, int jj_, int j_, int k_)
+        : A(a), sm(s), i(i_), jj(jj_), j(j_), k(k_) {}
+    HC_DEV R row(int base, int r, int di) const { return sm[(base + r) * NX + wrapx<NX>(i + di)]; }
+    HC_DEV R U(int di, int, int) const { return row(OU, jj, di); }
+    HC_DEV R V(int di, int dj, int) const { return row(OV, jj + 2 + dj, di); }
+    HC_DEV R W(int, int, int dk) const { return dk == 0 ? w_k : w_km; }
+    HC_DEV R T(int di, int dj, int dk) const
+    {
+        if (dk == 0) return row(OT, jj + 3 + dj, di);
+        return dk > 0 ? t_kp : t_km;
+    }
+    HC_DEV R lev(int tab, int kk) const { return A.lev[tab * A.nz + kk]; }
+    HC_DEV size_t center() const { return ((size_t)k * A.ny + j) * NX + i; }
+};
