@@ -80,7 +80,9 @@ def build_case(workload):
         prm = cases.stratified_open_channel(nx=256, ny=256, nz=257, wall_rows=1)
         st = cases.loglaw_noise(prm, seed=2024)
     elif workload == "sp":
-        prm = cases.stratified_open_channel(nx=256, ny=256, nz=257, cds=3, issk=1, wall_rows=1)
+        # configuration 3's top condition: constant temperature gradient
+        prm = cases.stratified_open_channel(nx=256, ny=256, nz=257, cds=3, issk=1, wall_rows=1, tbc_top=1,
+                                            dtdz_top=1.0)
         st = cases.loglaw_noise(prm, seed=2024)
     elif workload == "poisson":
         prm = cases.closed_channel(nx=256, ny=256, nz=257)
@@ -449,7 +451,7 @@ def main():
     workload_name = {
         "fd": "FD RK3 step (cds=2), neutral closed channel, Re_tau=180, 256x256x257, zstretch=1, wall_rows=1",
         "fd_t": "FD RK3 step (cds=2) with temperature, stratified open channel, Re_tau=395, got=0.1, 256x256x257, zstretch=1, wall_rows=1",
-        "sp": "spectral RK3 step (cds=3, skew-symmetric, 2/3 rule) with temperature, stratified open channel, Re_tau=395, got=0.1, 256x256x257, zstretch=1, wall_rows=1",
+        "sp": "spectral RK3 step (cds=3, skew-symmetric, 2/3 rule) with temperature, stratified open channel, Re_tau=395, got=0.1, top dT/dz=1, 256x256x257, zstretch=1, wall_rows=1",
         "poisson": "standalone pressure Poisson solve, 256x256x257, zstretch=1",
     }[args.workload]
     out = {

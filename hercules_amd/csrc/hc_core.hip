@@ -415,7 +415,7 @@ __global__ __launch_bounds__(rhs2_threads<NX>(), (rhs2_waves<NX, R>())) void k_r
         Q q(A, sm, i, jj, j, k);
         const R tc = sm[(Q::OT + jj + 3) * NX + i];
         q.t_km = k == 0 ? R(2.0) * A.tbot - tc : tkm;
-        q.t_kp = k == nz - 1 ? R(2.0) * A.ttop - tc : tkp;
+        q.t_kp = k == nz - 1 ? (A.tbc_top ? tc + A.gdz_top : R(2.0) * A.ttop - tc) : tkp;
         q.w_k = wk;
         q.w_km = wkm;
         R s;
@@ -783,7 +783,8 @@ HC_DEV void fixups(const StepArgs &A, int eq, const ColMap &m, C (&d)[L])
             if (k == A.nz - 1 && A.ochannel == 0) d[r].x += Re(2.0 * (A.rc * A.cUe) * A.u_mrf);
         } else if (eq == 3) {  // tbnd
             if (k == 0) d[r].x -= Re(2.0 * (A.rc * A.aT0) * A.tbot);
-            if (k == A.nz - 1) d[r].x -= Re(2.0 * (A.rc * A.cTe) * A.ttop);
+            if (k == A.nz - 1)  // Dirichlet ghost 2 ttop - t, or gradient ghost t + g dz_t
+                d[r].x -= A.tbc_top ? Re((A.rc * A.cTe) * A.gdz_top) : Re(2.0 * (A.rc * A.cTe) * A.ttop);
         } else if (eq == 2) {  // delnull
             if (k == 0) d[r] = czero_t<C>();
         }
@@ -1375,13 +1376,14 @@ static void build_levels(hc_ctx *c)
         if (kk == 0) { d = -cc; d00 = -2.0 * a - cc; }
         if (kk == nz - 1) { d = -a; d00 = P.ochannel == 0 ? -a - 2.0 * cc : -a; }
         at(LV_DU, kk) = P.wall_rows ? d00 : d; at(LV_DU00, kk) = d00;
-        // t: get_t_eqn_coeff (:398-399), Dirichlet at both walls
+        // t: get_t_eqn_coeff (:398-399), Dirichlet at both walls; with
+        // tbc_top = 1 the top row takes the gradient ghost t + g dz_t
         a = -0.5 * P.dt * P.alpha / dz[k] / dzb[k]; cc = -0.5 * P.dt * P.alpha / dz[k] / dzt[k];
         at(LV_AT, kk) = kk == 0 ? 0.0 : a;
         at(LV_CT, kk) = kk == nz - 1 ? 0.0 : cc;
         d = d00 = -a - cc;
         if (kk == 0) { d = -cc; d00 = -2.0 * a - cc; }
-        if (kk == nz - 1) { d = -a; d00 = -a - 2.0 * cc; }
+        if (kk == nz - 1) { d = -a; d00 = P.tbc_top == 1 ? -a : -a - 2.0 * cc; }
         at(LV_DT, kk) = P.wall_rows ? d00 : d; at(LV_DT00, kk) = d00;
         // p: get_p_eqn_coeff (:51-52), Neumann at both walls
         a = 1.0 / dz[k] / dzb[k]; cc = 1.0 / dz[k] / dzt[k];
@@ -1484,6 +1486,8 @@ static StepArgs base_args(hc_ctx *c)
     A.dpx = 1.0;
     A.norm = 1.0 / ((double)c->nx * (double)c->ny);
     A.aU0 = c->aU0; A.cUe = c->cUe; A.aT0 = c->aT0; A.cTe = c->cTe;
+    A.tbc_top = P.tbc_top;
+    A.gdz_top = P.dtdz_top * c->h_lev[(size_t)LV_DZT * c->nz + c->nz - 1];
     A.c24x = 1.0 / (24.0 * c->dx); A.c24y = 1.0 / (24.0 * c->dy);
     A.c12x2 = 1.0 / (12.0 * c->dx * c->dx); A.c12y2 = 1.0 / (12.0 * c->dy * c->dy);
     A.idx = 1.0 / c->dx; A.idy = 1.0 / c->dy; A.idx2 = 1.0 / (c->dx * c->dx); A.idy2 = 1.0 / (c->dy * c->dy);
@@ -1530,13 +1534,13 @@ static StepArgsT<R> rows_args(const StepArgs &A, const View<R> &V)
     B.ochannel = A.ochannel; B.dp_opt = A.dp_opt; B.isscalar = A.isscalar; B.use_f0 = A.use_f0;
     const double sc[] = {A.dx, A.dy, A.dx2, A.dy2, A.dt, A.nu, A.alpha, A.got, A.fc, A.ug, A.vg, A.u_mrf, A.t_ref,
                          A.tbot, A.ttop, A.ra, A.rb, A.rc, A.dpx, A.norm, A.c24x, A.c12x2, A.c24y, A.c12y2, A.idx,
-                         A.idx2, A.idy, A.idy2, A.idt, A.aU0, A.cUe, A.aT0, A.cTe, A.coe};
+                         A.idx2, A.idy, A.idy2, A.idt, A.aU0, A.cUe, A.aT0, A.cTe, A.coe, A.gdz_top};
     R *dst[] = {&B.dx, &B.dy, &B.dx2, &B.dy2, &B.dt, &B.nu, &B.alpha, &B.got, &B.fc, &B.ug, &B.vg, &B.u_mrf, &B.t_ref,
                 &B.tbot, &B.ttop, &B.ra, &B.rb, &B.rc, &B.dpx, &B.norm, &B.c24x, &B.c12x2, &B.c24y, &B.c12y2, &B.idx,
-                &B.idx2, &B.idy, &B.idy2, &B.idt, &B.aU0, &B.cUe, &B.aT0, &B.cTe, &B.coe};
+                &B.idx2, &B.idy, &B.idy2, &B.idt, &B.aU0, &B.cUe, &B.aT0, &B.cTe, &B.coe, &B.gdz_top};
     static_assert(sizeof sc / sizeof sc[0] == sizeof dst / sizeof dst[0], "scalar lists differ");
     for (size_t q = 0; q < sizeof sc / sizeof sc[0]; ++q) *dst[q] = R(sc[q]);
-    B.kdamp0 = A.kdamp0; B.dmask = A.dmask;
+    B.kdamp0 = A.kdamp0; B.dmask = A.dmask; B.tbc_top = A.tbc_top;
     B.issk = A.issk; B.lc = A.lc; B.mc = A.mc;
     B.ntile = A.ntile;
     return B;
@@ -1999,6 +2003,7 @@ extern "C" int hc_create(const hc_params *prm, hc_ctx **out)
         return set_err(HC_ERR_ARG, "nzdamp=%d must be in 1..nz-1 when isdamp=1", P.nzdamp);
     if (P.wall_rows != 0 && P.wall_rows != 1) return set_err(HC_ERR_ARG, "wall_rows must be 0 or 1");
     if (P.precision != HC_FP64 && P.precision != HC_MIXED) return set_err(HC_ERR_ARG, "precision must be 0 or 1");
+    if (P.tbc_top != 0 && P.tbc_top != 1) return set_err(HC_ERR_ARG, "tbc_top must be 0 or 1");
     if (P.precision == HC_MIXED && (P.cds == 3 || P.isdamp == 1))
         return set_err(HC_ERR_UNSUPPORTED, "precision=1 (mixed) supports cds=1,2 without damping");
     if (!fft_len_ok(P.nx) || !fft_len_ok(P.ny)) return set_err(HC_ERR_UNSUPPORTED, "nx, ny must be one of 16,32,64,128,192,256,512");

@@ -62,6 +62,10 @@ struct StepArgsT {
     // reciprocal stencil factors: 1/(24 dx), 1/(12 dx^2), 1/dx, 1/dx^2 (same in y), 1/dt
     R c24x, c12x2, c24y, c12y2, idx, idx2, idy, idy2, idt;
     R aU0, cUe, aT0, cTe;  // unmodified wall coefficients for the (0,0)-mode fix-ups
+    // top temperature condition: tbc_top 0 t = ttop, 1 dT/dz given; gdz_top =
+    // dtdz_top * dz_t(kend3), the ghost's offset t(kend3+1) - t(kend3)
+    int tbc_top;
+    R gdz_top;
     // Rayleigh damping of the top levels k >= kdamp0: sponge strength per
     // level and plane means per (field u,v,w,p,t; level); dmask selects the
     // fields an accessor reads damped (bit f for field f).
@@ -177,7 +181,7 @@ struct Interp {
         using Re = typename D::Real;
         int kk = q.k + dk;
         if (kk < 0) return q.A.tbot;
-        if (kk == q.A.nz - 1) return q.A.ttop;
+        if (kk == q.A.nz - 1) return q.A.tbc_top ? q.T(0, 0, dk) + Re(0.5) * q.A.gdz_top : q.A.ttop;
         Re lt = q.lev(LV_LT, kk);
         return q.T(0, 0, dk + 1) * lt + q.T(0, 0, dk) * (Re(1.0) - lt);
     }
@@ -231,7 +235,10 @@ struct PtT : Interp<PtT<R>> {
     {
         int kk = k + dk;
         if (kk < 0) return R(2.0) * A.tbot - A.t[at(di, dj, 0)];
-        if (kk >= A.nz) return R(2.0) * A.ttop - A.t[at(di, dj, A.nz - 1)];
+        if (kk >= A.nz) {
+            const R te = A.t[at(di, dj, A.nz - 1)];
+            return A.tbc_top ? te + A.gdz_top : R(2.0) * A.ttop - te;
+        }
         return A.t[at(di, dj, kk)];
     }
     HC_DEV R lev(int tab, int kk) const { return A.lev[tab * A.nz + kk]; }
@@ -309,7 +316,10 @@ struct PtD : Interp<PtD> {
     {
         int kk = k + dk;
         if (kk < 0) return 2.0 * A.tbot - raw(A.t, i + di, j + dj, 0);
-        if (kk >= A.nz) return 2.0 * A.ttop - raw(A.t, i + di, j + dj, A.nz - 1);
+        if (kk >= A.nz) {
+            const double te = raw(A.t, i + di, j + dj, A.nz - 1);
+            return A.tbc_top ? te + A.gdz_top : 2.0 * A.ttop - te;
+        }
         return val(4, A.t, di, dj, kk);
     }
     HC_DEV PtD centre(int di, int dj) const { return PtD(A, wrapi(i + di, A.nx), wrapi(j + dj, A.ny), k); }

@@ -210,7 +210,8 @@ __global__ __launch_bounds__(1024) void k_spec_t(StepArgs A)
 // column, with the wall ghosts of update_boundary_uvw / _p / _t
 // (src/module_boundary.f90:33-196) as they appear in spectral space: a ghost
 // -f(k0) - 2 u_mrf becomes -c(k0) - 2 u_mrf in the (0,0) mode and -c(k0)
-// elsewhere.  kind: 0 u, 1 v, 2 p (Neumann), 3 t (Dirichlet).
+// elsewhere.  kind: 0 u, 1 v, 2 p (Neumann), 3 t (Dirichlet, or the
+// gradient ghost at the top with tbc_top = 1).
 HC_DEV double2 zghost(const StepArgs &A, int kind, bool bottom, double2 c, bool is00)
 {
     switch (kind) {
@@ -225,6 +226,8 @@ HC_DEV double2 zghost(const StepArgs &A, int kind, bool bottom, double2 c, bool 
     case 2:
         return c;
     default: {
+        if (!bottom && A.tbc_top)  // gradient ghost t + g dz_t
+            return make_double2(c.x + (is00 ? A.gdz_top : 0.0), c.y);
         const double s = is00 ? 2.0 * (bottom ? A.tbot : A.ttop) : 0.0;
         return make_double2(s - c.x, -c.y);
     }

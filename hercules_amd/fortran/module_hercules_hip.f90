@@ -26,6 +26,8 @@ module module_hercules_hip
                           u_mrf, t_ref, tbot, ttop, ri_str, ri_end
         integer(c_int) :: wall_rows
         integer(c_int) :: precision   ! 0 FP64, 1 mixed FP32 + FP64 low wavenumbers
+        integer(c_int) :: tbc_top     ! 0 t = ttop at the top wall, 1 dT/dz = dtdz_top
+        real(c_double) :: dtdz_top
     end type
 
     integer(c_int), parameter :: HC_U = 0, HC_V = 1, HC_W = 2, HC_P = 3, HC_T = 4

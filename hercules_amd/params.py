@@ -20,7 +20,8 @@ DBL_FIELDS = ("lx", "ly", "lz", "zstretch", "cdamp", "nu", "alpha", "got", "fc",
 class CParams(C.Structure):
     """ctypes image of ``hc_params`` (include/hercules_amd.h)."""
     _fields_ = ([(n, C.c_int) for n in INT_FIELDS] + [(n, C.c_double) for n in DBL_FIELDS] +
-                [("wall_rows", C.c_int), ("precision", C.c_int)])
+                [("wall_rows", C.c_int), ("precision", C.c_int), ("tbc_top", C.c_int),
+                 ("dtdz_top", C.c_double)])
 
 
 @dataclass
@@ -82,6 +83,10 @@ class Params:
     # extension, no namelist counterpart: 0 FP64, 1 mixed FP32 + FP64 low
     # wavenumbers (hc_params.precision)
     precision: int = 0
+    # extension, no namelist counterpart: top temperature condition, 0 t = ttop
+    # (the reference), 1 dT/dz = dtdz_top (hc_params.tbc_top, dtdz_top)
+    tbc_top: int = 0
+    dtdz_top: float = 0.0
 
     def validate(self):
         """The checks of initiate_parameters (src/module_parameters.f90:256-270)."""
@@ -101,6 +106,8 @@ class Params:
             setattr(p, n, float(getattr(self, n)))
         p.wall_rows = int(self.wall_rows)
         p.precision = int(self.precision)
+        p.tbc_top = int(self.tbc_top)
+        p.dtdz_top = float(self.dtdz_top)
         return p
 
     def as_dict(self):

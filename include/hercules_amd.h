@@ -61,6 +61,13 @@ typedef struct hc_params {
      * pressure gradient / bulk velocity update in FP64.  HC_MIXED needs
      * cds = 1 or 2 and isdamp = 0 (HC_ERR_UNSUPPORTED otherwise). */
     int precision;
+    /* Top temperature condition (no namelist counterpart; the reference is
+     * Dirichlet at both walls, src/module_boundary.f90:179-180).  0: t = ttop
+     * at the top wall, as the reference.  1: constant gradient dT/dz =
+     * dtdz_top at the top wall (ghost t(kend3+1) = t(kend3) + dtdz_top *
+     * dz_t(kend3)); ttop is then unused.  The bottom stays t = tbot. */
+    int tbc_top;
+    double dtdz_top;
 } hc_params;
 
 enum hc_precision { HC_FP64 = 0, HC_MIXED = 1 };

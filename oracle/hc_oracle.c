@@ -38,6 +38,11 @@ typedef struct ho_params {
     double lx, ly, lz, zstretch, cdamp;
     double nu, alpha, got, fc, ug, vg, dt, u_mrf, t_ref, tbot, ttop, ri_str, ri_end;
     int wall_rows; /* 1: every mode gets the wall rows the reference gives mode (0,0) only */
+    int precision; /* unused here: the oracle is FP64 */
+    /* extension (no reference counterpart): 1 = top temperature gradient
+     * dtdz_top instead of t = ttop */
+    int tbc_top;
+    double dtdz_top;
 } ho_params;
 
 typedef struct ho_state {
@@ -153,6 +158,7 @@ ho_state *ho_create(const ho_params *P)
     if (P->nx % 2 || P->ny % 2 || P->nz < 3) { ho_destroy(S); return NULL; }
     /* the damped levels kend3-nzdamp..kend3 must be interior levels */
     if (P->isdamp == 1 && (P->nzdamp < 1 || P->nzdamp > P->nz - 1)) { ho_destroy(S); return NULL; }
+    if (P->tbc_top != 0 && P->tbc_top != 1) { ho_destroy(S); return NULL; }
     S->istr3 = 3; S->iend3 = P->nx + 2;
     S->jstr3 = 3; S->jend3 = P->ny + 2;
     S->kstr3 = 3; S->kend3 = P->nz + 2;
@@ -337,11 +343,14 @@ static void coeff_uvt(ho_state *S, double *aa, cplx *bb, double *cc, double coef
             for (int i = 1; i <= P->nx; ++i) {
                 cplx kx = kxv[i - 1], ky = kyv[j - 1];
                 cplx *b = &B3(S, bb, i, j, k);
+                /* top wall row: Dirichlet (t, closed-channel u) or Neumann
+                 * (open-channel u, t with tbc_top = 1) */
+                const int top_dir = is_t ? P->tbc_top != 1 : P->ochannel == 0;
                 if (P->wall_rows == 1) {
                     /* extension: the first mode's wall rows for every mode */
                     *b = -kx - ky - a0 - c0;
                     if (k == S->kstr3) *b -= a0;
-                    if (k == S->kend3) *b += (is_t || P->ochannel == 0) ? -c0 : c0;
+                    if (k == S->kend3) *b += top_dir ? -c0 : c0;
                     if (k == S->kstr3) V1(aa, k) = 0.0;
                     if (k == S->kend3) V1(cc, k) = 0.0;
                     continue;
@@ -349,7 +358,7 @@ static void coeff_uvt(ho_state *S, double *aa, cplx *bb, double *cc, double coef
                 *b = -kx - ky - V1(aa, k) - V1(cc, k);
                 if (k == S->kstr3) { *b -= V1(aa, k); V1(aa, k) = 0.0; }
                 if (k == S->kend3) {
-                    if (is_t || P->ochannel == 0) { *b -= V1(cc, k); V1(cc, k) = 0.0; }
+                    if (top_dir) { *b -= V1(cc, k); V1(cc, k) = 0.0; }
                     else                          { *b += V1(cc, k); V1(cc, k) = 0.0; }
                 }
             }
@@ -446,12 +455,15 @@ static void bc_p(ho_state *S, double *p)
     if (S->prm.cds != 3) comm_bound(S, p);
 }
 
-/* update_boundary_t, src/module_boundary.f90:167-196 */
+/* update_boundary_t, src/module_boundary.f90:167-196.  With tbc_top = 1
+ * the top ghost carries the gradient instead: t(ke+1) = t(ke) + g dz_t(ke). */
 static void bc_t(ho_state *S)
 {
+    const double gdz = S->prm.dtdz_top * V1(S->dz_t, S->kend3);
     for (int j = S->jstr3; j <= S->jend3; ++j)
         for (int i = S->istr3; i <= S->iend3; ++i) {
-            F3(S, S->t, i, j, S->kend3 + 1) = 2.0 * S->prm.ttop - F3(S, S->t, i, j, S->kend3);
+            const double te = F3(S, S->t, i, j, S->kend3);
+            F3(S, S->t, i, j, S->kend3 + 1) = S->prm.tbc_top == 1 ? te + gdz : 2.0 * S->prm.ttop - te;
             F3(S, S->t, i, j, S->kstr3 - 1) = 2.0 * S->prm.tbot - F3(S, S->t, i, j, S->kstr3);
         }
     if (S->prm.cds != 3) comm_bound(S, S->t);
@@ -504,17 +516,28 @@ static void bc_uvwh(ho_state *S)
     }
 }
 
-/* update_boundary_thxyz, src/module_boundary.f90:311-355 */
+/* update_boundary_thxyz, src/module_boundary.f90:311-355.  With tbc_top = 1
+ * the top values follow the gradient ghost: thx, thy at ke+1 shift by
+ * g dz_t(ke), thz(ke) is the face value t(ke) + g dz_t(ke)/2. */
 static void bc_th(ho_state *S)
 {
     double tt = S->prm.ttop, tb = S->prm.tbot;
     int ks = S->kstr3, ke = S->kend3;
+    const int neu = S->prm.tbc_top == 1;
+    const double gdz = S->prm.dtdz_top * V1(S->dz_t, ke);
     for (int j = S->jstr3; j <= S->jend3; ++j)
         for (int i = S->istr3; i <= S->iend3; ++i) {
-            F3(S, S->thx, i, j, ke + 1) = -F3(S, S->thx, i, j, ke) + 2.0 * tt;
-            F3(S, S->thy, i, j, ke + 1) = -F3(S, S->thy, i, j, ke) + 2.0 * tt;
-            F3(S, S->thz, i, j, ke) = tt;
-            F3(S, S->thz, i, j, ke + 1) = -F3(S, S->thz, i, j, ke - 1) + 2.0 * tt;
+            if (neu) {
+                F3(S, S->thx, i, j, ke + 1) = F3(S, S->thx, i, j, ke) + gdz;
+                F3(S, S->thy, i, j, ke + 1) = F3(S, S->thy, i, j, ke) + gdz;
+                F3(S, S->thz, i, j, ke) = F3(S, S->t, i, j, ke) + 0.5 * gdz;
+                F3(S, S->thz, i, j, ke + 1) = 2.0 * F3(S, S->thz, i, j, ke) - F3(S, S->thz, i, j, ke - 1);
+            } else {
+                F3(S, S->thx, i, j, ke + 1) = -F3(S, S->thx, i, j, ke) + 2.0 * tt;
+                F3(S, S->thy, i, j, ke + 1) = -F3(S, S->thy, i, j, ke) + 2.0 * tt;
+                F3(S, S->thz, i, j, ke) = tt;
+                F3(S, S->thz, i, j, ke + 1) = -F3(S, S->thz, i, j, ke - 1) + 2.0 * tt;
+            }
             F3(S, S->thx, i, j, ks - 1) = -F3(S, S->thx, i, j, ks) + 2.0 * tb;
             F3(S, S->thy, i, j, ks - 1) = -F3(S, S->thy, i, j, ks) + 2.0 * tb;
             F3(S, S->thz, i, j, ks - 1) = tb;
@@ -857,7 +880,10 @@ static int spectral_thomas(ho_state *S, int zstagger, int delnull, int tbnd, int
         double aa_t_zstr = -0.5 * S->rk_c * P->dt * P->alpha / V1(S->dz, ks) / V1(S->dz_b, ks);
         C3(S, S->c_src, 1, 1, 1) = C3(S, S->c_src, 1, 1, 1) - 2.0 * aa_t_zstr * P->tbot;
         double cc_t_zend = -0.5 * S->rk_c * P->dt * P->alpha / V1(S->dz, ke) / V1(S->dz_t, ke);
-        C3(S, S->c_src, 1, 1, nz) = C3(S, S->c_src, 1, 1, nz) - 2.0 * cc_t_zend * P->ttop;
+        if (P->tbc_top == 1) /* ghost t + g dz_t: its constant part moves to the source */
+            C3(S, S->c_src, 1, 1, nz) = C3(S, S->c_src, 1, 1, nz) - cc_t_zend * P->dtdz_top * V1(S->dz_t, ke);
+        else
+            C3(S, S->c_src, 1, 1, nz) = C3(S, S->c_src, 1, 1, nz) - 2.0 * cc_t_zend * P->ttop;
     } else if (tbnd != 0) { snprintf(S->err, sizeof S->err, "tbnd flag error!"); return 1; }
     if (ubnd == 1) {
         double aa_u_zstr = -0.5 * S->rk_c * P->dt * P->nu / V1(S->dz, ks) / V1(S->dz_b, ks);

@@ -28,7 +28,8 @@ _DBL_FIELDS = ("lx", "ly", "lz", "zstretch", "cdamp", "nu", "alpha", "got", "fc"
 
 class OParams(C.Structure):
     _fields_ = ([(n, C.c_int) for n in _INT_FIELDS] + [(n, C.c_double) for n in _DBL_FIELDS] +
-                [("wall_rows", C.c_int)])
+                [("wall_rows", C.c_int), ("precision", C.c_int), ("tbc_top", C.c_int),
+                 ("dtdz_top", C.c_double)])
 
 
 FIELD_IDS = {"u": 0, "v": 1, "w": 2, "p": 3, "t": 4, "phi": 5, "ff": 6, "gg": 7, "hh": 8,
@@ -96,6 +97,8 @@ def make_params(**kw) -> OParams:
     for n in _DBL_FIELDS:
         setattr(p, n, float(kw.get(n, 0.0)))
     p.wall_rows = int(kw.get("wall_rows", 0))
+    p.tbc_top = int(kw.get("tbc_top", 0))
+    p.dtdz_top = float(kw.get("dtdz_top", 0.0))
     return p
 
 

@@ -7,7 +7,7 @@ from hercules_amd.params import Params
 ORACLE_KEYS = ("nx", "ny", "nz", "ochannel", "dp_opt", "cds", "dts", "issk", "isdamp", "nzdamp",
                "isscalar", "is_ri_var", "imax", "lx", "ly", "lz", "zstretch", "cdamp", "nu",
                "alpha", "got", "fc", "ug", "vg", "dt", "u_mrf", "t_ref", "tbot", "ttop",
-               "ri_str", "ri_end", "wall_rows")
+               "ri_str", "ri_end", "wall_rows", "tbc_top", "dtdz_top")
 
 
 def oracle_kwargs(prm: Params):

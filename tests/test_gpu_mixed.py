@@ -55,6 +55,7 @@ CASES = [
     ("closed", dict(nx=64, ny=32, nz=65, dp_opt=2), 40),
     ("closed", dict(nx=32, ny=64, nz=65, dp_opt=3, fc=0.5, ug=1.0, vg=0.2), 40),
     ("open_t", dict(nx=64, ny=64, nz=65), 40),
+    ("open_t", dict(nx=64, ny=64, nz=65, tbc_top=1, dtdz_top=0.5), 40),
     ("open_t", dict(nx=128, ny=64, nz=129, is_ri_var=1, ri_str=0.0, ri_end=0.2, imax=40), 40),
     # nx = 512 takes the one-pass right-hand-side kernel
     ("closed", dict(nx=512, ny=16, nz=33, dt=2.5e-4), 10),

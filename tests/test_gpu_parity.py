@@ -75,6 +75,11 @@ def test_steps_shapes_match_oracle(shape):
     ("closed", {"isdamp": 1, "nzdamp": 8, "cdamp": 0.3}),
     ("open_t", {"isdamp": 1, "nzdamp": 10, "cdamp": 0.2}),
     ("closed", {"isdamp": 1, "nzdamp": 6, "cdamp": 1.0, "cds": 1, "dp_opt": 3, "fc": 0.5, "ug": 1.0}),
+    # top temperature gradient instead of t = ttop (extension)
+    ("open_t", {"tbc_top": 1, "dtdz_top": 0.5}),
+    ("open_t", {"tbc_top": 1, "dtdz_top": -0.8, "cds": 1, "wall_rows": 1}),
+    ("closed", {"isscalar": 1, "got": 0.05, "tbc_top": 1, "dtdz_top": -0.3}),
+    ("open_t", {"tbc_top": 1, "dtdz_top": 0.5, "isdamp": 1, "nzdamp": 8, "cdamp": 0.2}),
 ])
 def test_steps_match_oracle(kind, kw):
     prm, st = small_case(kind, **kw)
@@ -180,3 +185,16 @@ def test_wall_shear_entry(kind, kw):
     else:
         assert got[1] == 0.0
     ch.close()
+
+
+@pytest.mark.parametrize("wall_rows", [0, 1])
+def test_temperature_solve_with_top_gradient_matches_oracle(wall_rows):
+    """poisson_solver_fft for t (tbnd) with the top gradient row and ghost."""
+    prm = cases.stratified_open_channel(nx=32, ny=32, nz=65, tbc_top=1, dtdz_top=0.4, wall_rows=wall_rows)
+    src = np.random.default_rng(9).standard_normal((prm.nz, prm.ny, prm.nx))
+    ref = make_oracle(prm, {}).solve("t", 8.0 / 15.0, src)
+    with Channel(prm) as ch:
+        got = ch.poisson_solver_fft("t", 8.0 / 15.0, src)
+    err = rel_l2(got, ref)
+    print(f"t solve, top gradient, wall_rows {wall_rows}: rel L2 {err:.3e}")
+    assert err < 1e-13

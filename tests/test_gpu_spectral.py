@@ -26,6 +26,7 @@ def _pair(prm, st):
     ("closed", {"issk": 1, "dts": 1}),
     ("open_t", {"issk": 1}),
     ("open_t", {"issk": 0, "is_ri_var": 1, "ri_str": 0.05, "ri_end": 0.2, "imax": 10}),
+    ("open_t", {"issk": 1, "tbc_top": 1, "dtdz_top": 0.5}),
 ])
 def test_spectral_steps_match_oracle(kind, kw):
     prm, st = small_case(kind, cds=3, **kw)

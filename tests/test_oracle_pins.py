@@ -369,3 +369,67 @@ def test_damping_rejects_levels_outside_the_interior():
     prm = cases.closed_channel(nx=16, ny=16, nz=9, isdamp=1, nzdamp=9)
     with pytest.raises(ValueError):
         Oracle(**oracle_kwargs(prm))
+
+
+# ---- top temperature gradient (tbc_top = 1): an extension with no reference
+# function, pinned by an exact steady state and by mirror symmetry.
+
+@pytest.mark.parametrize("cds", [1, 2, 3])
+def test_top_gradient_linear_profile_is_steady(cds):
+    """t = tbot + g z with the gradient g imposed at the top and t = tbot at
+    the bottom is an exact discrete steady state of the 2nd-order z operator
+    on the stretched grid (the ghosts extend the line), while the mean drive
+    accelerates u(z) and buoyancy is balanced hydrostatically."""
+    g, tb = 0.7, 0.2
+    prm = cases.stratified_open_channel(nx=16, ny=16, nz=33, cds=cds, issk=1, tbc_top=1, dtdz_top=g, tbot=tb,
+                                        ttop=123.0, wall_rows=1, got=0.1)
+    m = grid.metrics(prm.nz, prm.lz, prm.zstretch, prm.ochannel)
+    shape = (prm.nz, prm.ny, prm.nx)
+    t0 = np.broadcast_to((tb + g * m["zc"])[:, None, None], shape).copy()
+    z = np.zeros(shape)
+    o = make_oracle(prm, dict(u=z, v=z, w=z, p=z, t=t0))
+    o.steps(5)
+    assert np.abs(o.get("t") - t0).max() < 1e-12 * np.abs(t0).max()
+    assert np.abs(o.get("w")).max() < 1e-12
+    assert np.abs(o.get("u")).max() > 1e-4  # the drive did act
+
+
+@pytest.mark.parametrize("cds", [2, 3])
+def test_top_zero_gradient_is_the_mirror_of_a_closed_channel(cds):
+    """An open channel of height h (free-slip top, dT/dz = 0 at the top) is the
+    lower half of a closed channel of height 2h whose state is mirror-symmetric
+    about the centre plane (u, v, p, t even, w odd) with t = tbot at both
+    walls: the two grids coincide there (get_mesh_param), so after any number
+    of steps the open channel equals the closed channel's lower half."""
+    m_lev = 16
+    common = dict(nx=16, ny=16, cds=cds, issk=1, nu=1.0 / 150.0, alpha=1.0 / 150.0 / 0.71, got=0.0, isscalar=1,
+                  tbot=0.3, wall_rows=1, dt=5e-4, dp_opt=1)
+    full = cases.closed_channel(nz=2 * m_lev, lz=2.0, ttop=0.3, **common)
+    half = cases.stratified_open_channel(nz=m_lev, lz=1.0, tbc_top=1, dtdz_top=0.0, ttop=-5.0, **common)
+    rng = np.random.default_rng(4)
+    sh = (m_lev, 16, 16)
+    zc = grid.metrics(m_lev, 1.0, half.zstretch, 1)["zc"][:, None, None]
+    u_l = 10.0 * zc * (2.0 - zc) + 0.3 * rng.standard_normal(sh)
+    v_l = 0.3 * rng.standard_normal(sh)
+    t_l = 0.3 + 0.5 * zc + 0.1 * rng.standard_normal(sh)
+    w_l = 0.1 * rng.standard_normal(sh)
+    w_l[-1] = 0.0  # the centre face of the odd w
+    even = lambda a: np.concatenate([a, a[::-1]], 0)
+    odd_w = np.concatenate([w_l, -w_l[-2::-1], np.zeros((1,) + sh[1:])], 0)
+    z = np.zeros((2 * m_lev,) + sh[1:])
+    of = make_oracle(full, dict(u=even(u_l), v=even(v_l), w=odd_w, p=z, t=even(t_l)))
+    oh = make_oracle(half, dict(u=u_l, v=v_l, w=w_l, p=z[:m_lev], t=t_l))
+    # the odd w of the full channel: w(k) of the faces above the centre is
+    # -w(2m-2-k); its top face (the wall) is 0
+    assert np.array_equal(odd_w[m_lev - 1], np.zeros(sh[1:]))
+    of.steps(6)
+    oh.steps(6)
+    for n in ("u", "v", "w", "p", "t"):
+        a, b = oh.get(n), of.get(n)[:m_lev]
+        assert rel_l2(a, b) < 1e-10, (n, rel_l2(a, b))
+
+
+def test_top_gradient_rejects_unknown_flag():
+    prm = cases.stratified_open_channel(nx=8, ny=8, nz=9, tbc_top=2)
+    with pytest.raises(ValueError):
+        Oracle(**oracle_kwargs(prm))
