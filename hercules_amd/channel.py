@@ -29,7 +29,9 @@ ABI_SYMBOLS = ("hc_last_error", "hc_create", "hc_destroy", "hc_set_field", "hc_g
                "hc_poisson_solver_fft",
                "hc_calculate_ubar", "hc_screen_div_error", "hc_wall_shear", "hc_get_scalar", "hc_stream",
                "hc_field_device_ptr", "hc_synchronize", "hc_profile_enable", "hc_profile_read",
-               "hc_ensemble_create", "hc_ensemble_steps", "hc_ensemble_graph_used", "hc_ensemble_destroy")
+               "hc_ensemble_create", "hc_ensemble_steps", "hc_ensemble_graph_used", "hc_ensemble_destroy",
+               "hc_substep_fd", "hc_set_adjoint", "hc_get_adjoint", "hc_adjoint_substep_fd",
+               "hc_wall_shear_gradient")
 KERNELS = {"rhs": 0, "yfft_fwd": 1, "column": 2, "yfft_inv": 3, "finish": 4, "rhs_t": 5,
            "yfft_t": 6, "column_t": 7, "finish_t": 8, "solve_xr2c": 9, "solve_yfft": 10,
            "solve_column": 11, "solve_xc2r": 12}
@@ -77,6 +79,11 @@ def load_library(path: str = LIB_PATH):
     L.hc_field_device_ptr.restype = vp
     L.hc_field_device_ptr.argtypes = [vp, ip]
     L.hc_synchronize.argtypes = [vp]
+    L.hc_substep_fd.argtypes = [vp, ip]
+    L.hc_set_adjoint.argtypes = [vp, ip, vp, ip]
+    L.hc_get_adjoint.argtypes = [vp, ip, vp, ip]
+    L.hc_adjoint_substep_fd.argtypes = [vp, ip]
+    L.hc_wall_shear_gradient.argtypes = [vp, ip, ip, dp]
     L.hc_profile_enable.argtypes = [vp, ip]
     L.hc_profile_read.argtypes = [vp, ip, dp, C.POINTER(C.c_long)]
     _lib = L
@@ -171,6 +178,35 @@ class Channel:
         self._check(self.L.hc_poisson_solver_fft(self.h, EQUATIONS[equation], float(rk_c),
                                                  C.c_void_p(src_ptr), C.c_void_p(dst_ptr),
                                                  LAYOUT_INTERIOR))
+
+    def substep(self, t_its):
+        """One substep of time_advancement_fd (t_its = 1..3 for RK3)."""
+        self._check(self.L.hc_substep_fd(self.h, int(t_its)))
+
+    # ---- discrete adjoint
+    def set_adjoint(self, name, arr, layout=LAYOUT_INTERIOR):
+        shape = self.shape if layout == LAYOUT_INTERIOR else self.ghost_shape
+        a = np.ascontiguousarray(arr, dtype=np.float64)
+        if a.shape != shape:
+            raise ValueError(f"{name}: shape {a.shape} != {shape}")
+        self._check(self.L.hc_set_adjoint(self.h, FIELDS[name], _ptr(a), layout))
+
+    def get_adjoint(self, name, layout=LAYOUT_INTERIOR):
+        shape = self.shape if layout == LAYOUT_INTERIOR else self.ghost_shape
+        a = np.zeros(shape)
+        self._check(self.L.hc_get_adjoint(self.h, FIELDS[name], _ptr(a), layout))
+        return a
+
+    def adjoint_substep(self, t_its):
+        """Transposed Jacobian of substep t_its at the current (input) state."""
+        self._check(self.L.hc_adjoint_substep_fd(self.h, int(t_its)))
+
+    def wall_shear_gradient(self, nsteps, ckpt_every=20):
+        """Bottom wall shear after nsteps steps and, in the adjoint fields
+        u, v, w, p, its gradient with respect to the current state."""
+        v = C.c_double()
+        self._check(self.L.hc_wall_shear_gradient(self.h, int(nsteps), int(ckpt_every), C.byref(v)))
+        return v.value
 
     def calculate_ubar(self):
         v = C.c_double()

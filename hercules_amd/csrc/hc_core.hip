@@ -1072,6 +1072,8 @@ __global__ __launch_bounds__(64 * kColWaves) void k_column_solve(StepArgs A, Col
     col_store<L>(io.s0, tile, A.nz, plane, col0, m, d[0]);
 }
 
+#include "hc_adjoint.h"
+
 // ---------------------------------------------------------------------------
 // small kernels
 // dpx_drive update of dp_opt=2 (update_dpx_drive, src/module_parameters.f90:594-613)
@@ -1207,6 +1209,12 @@ __global__ void k_convert(const S *__restrict__ src, D *dst, size_t n)
     for (size_t q = blockIdx.x * (size_t)blockDim.x + threadIdx.x; q < n; q += (size_t)gridDim.x * blockDim.x)
         dst[q] = D(src[q]);
 }
+// f[off + q] = val for q < n
+__global__ void k_fill(double *f, size_t off, size_t n, double val)
+{
+    size_t q = blockIdx.x * (size_t)blockDim.x + threadIdx.x;
+    if (q < n) f[off + q] = val;
+}
 template <class R>
 __global__ void k_zero_level(R *f, size_t off, size_t n)
 {
@@ -1273,6 +1281,12 @@ struct hc_ctx {
     int stps;
     std::vector<double> h_lev;
     std::vector<void *> allocs;
+    // discrete adjoint: adjoints of u, v, w, p; of f, g, h (ping-pong); of
+    // the Helmholtz sources q (physical scratch)
+    double *adj[4] = {};
+    double *adf[2][3] = {};
+    double *adq[3] = {};
+    int adcur = 0;
     bool big_lds = false;  // column kernels may use more than 64 KiB of dynamic LDS
     // optional per-kernel timing with HIP events on the context stream
     bool prof = false;
@@ -2684,5 +2698,214 @@ extern "C" int hc_ensemble_destroy(hc_ensemble *e)
     if (e->fork) (void)hipEventDestroy(e->fork);
     if (e->master) (void)hipStreamDestroy(e->master);
     delete e;
+    return 0;
+}
+
+// ---- discrete adjoint ---------------------------------------------------------
+static int adjoint_supported(const hc_ctx *c)
+{
+    const hc_params &P = c->prm;
+    if (c->mixed || P.cds == 3 || P.isscalar == 1 || P.isdamp == 1 || P.dp_opt == 2)
+        return set_err(HC_ERR_UNSUPPORTED, "the adjoint covers FP64, cds=1,2, isscalar=0, isdamp=0, dp_opt=1,3");
+    return 0;
+}
+
+static int ensure_adjoint(hc_ctx *c)
+{
+    if (c->adj[0]) return 0;
+    const size_t bytes = (size_t)c->nx * c->ny * c->nz * sizeof(double);
+    int e = 0;
+    for (int q = 0; q < 4 && !e; ++q) e = dalloc(c, (void **)&c->adj[q], bytes);
+    for (int a = 0; a < 2; ++a)
+        for (int q = 0; q < 3 && !e; ++q) e = dalloc(c, (void **)&c->adf[a][q], bytes);
+    for (int q = 0; q < 3 && !e; ++q) e = dalloc(c, (void **)&c->adq[q], bytes);
+    return e;
+}
+
+static double *adjoint_ptr(hc_ctx *c, int f)
+{
+    switch (f) {
+    case HC_U: return c->adj[0];
+    case HC_V: return c->adj[1];
+    case HC_W: return c->adj[2];
+    case HC_P: return c->adj[3];
+    case HC_FF: return c->adf[c->adcur][0];
+    case HC_GG: return c->adf[c->adcur][1];
+    case HC_HH: return c->adf[c->adcur][2];
+    default: return nullptr;
+    }
+}
+
+// Adjoint of substep t_its at the forward state held in the context (the
+// substep's input u, v, w, p): the adjoints of its outputs (u', v', w', p'
+// and f' = this substep's f) become those of its inputs (u, v, w, p and
+// f_old).
+static int adjoint_substep(hc_ctx *c, int t_its)
+{
+    const hc_params &P = c->prm;
+    StepArgs A = base_args(c);
+    if (P.dts == 2) { A.ra = kRkc1[t_its - 1]; A.rb = kRkc2[t_its - 1]; A.rc = kRkc3[t_its - 1]; }
+    else { A.ra = 1.5; A.rb = -0.5; A.rc = 1.0; }
+    for (int q = 0; q < 4; ++q)
+        if (int e = launch_r2c<double>(c, c->adj[q], c->s[q])) return e;
+    if (int e = launch_yfft(c, -1, 4, c->s[0], c->s[1], c->s[2], c->s[3])) return e;
+    A.s0 = c->s[0]; A.s1 = c->s[1]; A.s2 = c->s[2]; A.s3 = c->s[3];
+    dim3 gcol((c->nl + kColWaves - 1) / kColWaves, c->ny);
+#define HC_AC(L_) hipLaunchKernelGGL((k_adj_column<L_>), gcol, dim3(64 * kColWaves), col_lds(c), c->stream, A);
+    HC_DISPATCH_L(c->Lcol, HC_AC)
+#undef HC_AC
+    if (int e = launch_yfft(c, +1, 3, c->s[0], c->s[1], c->s[2], nullptr)) return e;
+    C2RArgs X;
+    std::memset(&X, 0, sizeof X);
+    for (int q = 0; q < 3; ++q) { X.src[q] = c->s[q]; X.dst[q] = c->adq[q]; }
+    if (int e = launch_c2r<double>(c, 3, X)) return e;
+    const int cur = c->adcur, nxt = cur ^ 1;
+    AdjIO io;
+    io.qu = c->adq[0]; io.qv = c->adq[1]; io.qw = c->adq[2];
+    io.fu = c->adf[cur][0]; io.fv = c->adf[cur][1]; io.fw = c->adf[cur][2];
+    io.au = c->adj[0]; io.av = c->adj[1]; io.aw = c->adj[2]; io.ap = c->adj[3];
+    io.ou = c->adf[nxt][0]; io.ov = c->adf[nxt][1]; io.ow = c->adf[nxt][2];
+    if (P.cds == 1) hipLaunchKernelGGL((k_adj_rhs<1>), dim3(2048), dim3(256), 0, c->stream, A, io);
+    else hipLaunchKernelGGL((k_adj_rhs<2>), dim3(2048), dim3(256), 0, c->stream, A, io);
+    c->adcur = nxt;
+    HIPCHK(hipGetLastError());
+    return 0;
+}
+
+extern "C" int hc_substep_fd(hc_ctx *c, int t_its)
+{
+    if (!c) return set_err(HC_ERR_ARG, "null context");
+    if (c->prm.cds == 3) return set_err(HC_ERR_ARG, "cds=3: no finite-difference substep");
+    const int nsub = c->prm.dts == 2 ? 3 : 1;
+    if (t_its < 1 || t_its > nsub) return set_err(HC_ERR_ARG, "t_its=%d outside 1..%d", t_its, nsub);
+    return substep(c, t_its);
+}
+
+extern "C" int hc_set_adjoint(hc_ctx *c, int field, const double *src, int layout)
+{
+    if (!c || !src) return set_err(HC_ERR_ARG, "null argument");
+    if (int e = adjoint_supported(c)) return e;
+    if (int e = ensure_adjoint(c)) return e;
+    double *d = adjoint_ptr(c, field);
+    if (!d) return set_err(HC_ERR_ARG, "no adjoint of field %d", field);
+    if (int e = set_field_t<double>(c, d, src, layout)) return e;
+    HIPCHK(hipStreamSynchronize(c->stream));
+    return 0;
+}
+
+extern "C" int hc_get_adjoint(hc_ctx *c, int field, double *dst, int layout)
+{
+    if (!c || !dst) return set_err(HC_ERR_ARG, "null argument");
+    if (int e = adjoint_supported(c)) return e;
+    if (int e = ensure_adjoint(c)) return e;
+    double *d = adjoint_ptr(c, field);
+    if (!d) return set_err(HC_ERR_ARG, "no adjoint of field %d", field);
+    if (int e = get_field_t<double>(c, d, dst, layout)) return e;
+    HIPCHK(hipStreamSynchronize(c->stream));
+    return 0;
+}
+
+extern "C" int hc_adjoint_substep_fd(hc_ctx *c, int t_its)
+{
+    if (!c) return set_err(HC_ERR_ARG, "null context");
+    if (int e = adjoint_supported(c)) return e;
+    const int nsub = c->prm.dts == 2 ? 3 : 1;
+    if (t_its < 1 || t_its > nsub) return set_err(HC_ERR_ARG, "t_its=%d outside 1..%d", t_its, nsub);
+    if (int e = ensure_adjoint(c)) return e;
+    if (int e = adjoint_substep(c, t_its)) return e;
+    HIPCHK(hipStreamSynchronize(c->stream));
+    return 0;
+}
+
+// Gradient of the bottom wall shear after nsteps steps with respect to the
+// initial u, v, w, p: forward run with a checkpoint of (u, v, w, p) every
+// ckpt_every steps, then segment by segment (last first) the segment's
+// substep inputs are recomputed from its checkpoint and the substep adjoints
+// applied in reverse order.  RK3 only (its first substep does not read
+// f_old, so a step's input is (u, v, w, p)).
+extern "C" int hc_wall_shear_gradient(hc_ctx *c, int nsteps, int ckpt_every, double *loss)
+{
+    if (!c) return set_err(HC_ERR_ARG, "null context");
+    if (int e = adjoint_supported(c)) return e;
+    const hc_params &P = c->prm;
+    if (P.dts != 2) return set_err(HC_ERR_UNSUPPORTED, "the gradient driver needs RK3 (dts=2)");
+    if (nsteps < 1 || ckpt_every < 1) return set_err(HC_ERR_ARG, "nsteps and ckpt_every must be >= 1");
+    if (int e = ensure_adjoint(c)) return e;
+    const size_t N = (size_t)c->nx * c->ny * c->nz, fb = N * sizeof(double);
+    const int nseg = (nsteps + ckpt_every - 1) / ckpt_every, nsub = 3;
+    const int seglen = std::min(ckpt_every, nsteps);
+    std::vector<void *> mem;
+    auto alloc = [&](size_t bytes) -> double * {
+        void *p = nullptr;
+        if (hipMalloc(&p, bytes) != hipSuccess) return nullptr;
+        mem.push_back(p);
+        return (double *)p;
+    };
+    auto release = [&]() {
+        (void)hipStreamSynchronize(c->stream);
+        for (void *p : mem) (void)hipFree(p);
+        mem.clear();
+    };
+    double *ck = alloc((size_t)nseg * 4 * fb);
+    double *seg = alloc((size_t)seglen * nsub * 4 * fb);
+    if (!ck || !seg) { release(); return set_err(HC_ERR_HIP, "checkpoint storage (%zu MB) not available",
+                                                 ((size_t)nseg * 4 + (size_t)seglen * nsub * 4) * fb >> 20); }
+    double *fld[4] = {c->u, c->v, c->w, c->p};
+    auto save = [&](double *dst) -> hipError_t {
+        hipError_t r = hipSuccess;
+        for (int q = 0; q < 4 && r == hipSuccess; ++q)
+            r = hipMemcpyAsync(dst + q * N, fld[q], fb, hipMemcpyDeviceToDevice, c->stream);
+        return r;
+    };
+    auto load = [&](const double *src) -> hipError_t {
+        hipError_t r = hipSuccess;
+        for (int q = 0; q < 4 && r == hipSuccess; ++q)
+            r = hipMemcpyAsync(fld[q], src + q * N, fb, hipMemcpyDeviceToDevice, c->stream);
+        return r;
+    };
+    const int stps0 = c->stps, fcur0 = c->fcur;
+    const double time0 = c->time_sim, got0 = c->got;
+    int err = 0;
+    hipError_t he = hipSuccess;
+    // forward with checkpoints
+    for (int n = 0; n < nsteps && !err && he == hipSuccess; ++n) {
+        if (n % ckpt_every == 0) he = save(ck + (size_t)(n / ckpt_every) * 4 * N);
+        if (he == hipSuccess) err = one_step(c);
+    }
+    double tau[2] = {0.0, 0.0};
+    if (!err && he == hipSuccess) err = hc_wall_shear(c, &tau[0], &tau[1]);
+    // seed: d tau_bottom / d u(level 0) = 2 nu / (dz(kstr3) nx ny)
+    if (!err && he == hipSuccess) {
+        for (int q = 0; q < 4 && he == hipSuccess; ++q) he = hipMemsetAsync(c->adj[q], 0, fb, c->stream);
+        for (int a = 0; a < 2; ++a)
+            for (int q = 0; q < 3 && he == hipSuccess; ++q) he = hipMemsetAsync(c->adf[a][q], 0, fb, c->stream);
+        const size_t plane = (size_t)c->nx * c->ny;
+        const double seed = 2.0 * P.nu / (c->h_lev[(size_t)LV_DZ * c->nz] * (double)plane);
+        hipLaunchKernelGGL(k_fill, dim3((unsigned)((plane + 255) / 256)), dim3(256), 0, c->stream, c->adj[0],
+                           (size_t)0, plane, seed);
+    }
+    // backward over the segments
+    for (int sg = nseg - 1; sg >= 0 && !err && he == hipSuccess; --sg) {
+        const int n0 = sg * ckpt_every, n1 = std::min(nsteps, n0 + ckpt_every);
+        he = load(ck + (size_t)sg * 4 * N);
+        int idx = 0;
+        for (int n = n0; n < n1 && !err && he == hipSuccess; ++n)
+            for (int s = 1; s <= nsub && !err && he == hipSuccess; ++s) {
+                he = save(seg + (size_t)(idx++) * 4 * N);
+                if (he == hipSuccess) err = substep(c, s);
+            }
+        for (int q = idx - 1; q >= 0 && !err && he == hipSuccess; --q) {
+            he = load(seg + (size_t)q * 4 * N);
+            if (he == hipSuccess) err = adjoint_substep(c, q % nsub + 1);
+        }
+    }
+    // leave the initial state in the context
+    if (!err && he == hipSuccess) he = load(ck);
+    if (he == hipSuccess) he = hipStreamSynchronize(c->stream);
+    release();
+    c->stps = stps0; c->fcur = fcur0; c->time_sim = time0; c->got = got0;
+    if (err) return err;
+    if (he != hipSuccess) return set_err(HC_ERR_HIP, "gradient: %s", hipGetErrorString(he));
+    if (loss) *loss = tau[0];
     return 0;
 }

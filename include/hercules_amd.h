@@ -203,6 +203,38 @@ enum hc_kernel {
 int hc_profile_enable(hc_ctx *ctx, int on);
 int hc_profile_read(hc_ctx *ctx, int kernel, double *total_ms, long *launches);
 
+/* One substep t_its (1..3 for RK3, 1 for AB2) of time_advancement_fd: one
+ * pass of its substep loop (src/module_navier_stokes.f90:247-355).  With
+ * cds = 1 or 2. */
+int hc_substep_fd(hc_ctx *ctx, int t_its);
+
+/* Discrete adjoint of the finite-difference step (no counterpart in the
+ * reference).  Covers FP64, cds = 1 or 2, isscalar = 0, isdamp = 0,
+ * dp_opt = 1 or 3 (HC_ERR_UNSUPPORTED otherwise).  The context holds
+ * adjoint fields for HC_U, HC_V, HC_W, HC_P and, for HC_FF, HC_GG, HC_HH,
+ * the adjoint of the stored right-hand sides f, g, h (the next substep's
+ * old right-hand sides); hc_set_adjoint / hc_get_adjoint move them like
+ * hc_set_field / hc_get_field.
+ *
+ * hc_adjoint_substep_fd: with the context's fields equal to the input state
+ * of substep t_its (u, v, w, p at its start), replaces the adjoints of the
+ * substep's outputs (u, v, w, p after it and its f, g, h) by the adjoints of
+ * its inputs (u, v, w, p and the f, g, h it read as old right-hand sides),
+ * i.e. applies the transposed Jacobian of hc_substep_fd.  The fields are not
+ * changed.
+ *
+ * hc_wall_shear_gradient: loss = plane-averaged bottom wall shear (as
+ * hc_wall_shear) after nsteps RK3 steps from the context's current state;
+ * on return the adjoint fields HC_U, HC_V, HC_W, HC_P hold d loss / d
+ * (initial u, v, w, p) and the context's fields hold the initial state
+ * again.  The forward run keeps (u, v, w, p) every ckpt_every steps; the
+ * backward sweep recomputes each segment's substep inputs from its
+ * checkpoint (device memory: (nsteps/ckpt_every + 3 ckpt_every) x 4 fields). */
+int hc_set_adjoint(hc_ctx *ctx, int field, const double *src, int layout);
+int hc_get_adjoint(hc_ctx *ctx, int field, double *dst, int layout);
+int hc_adjoint_substep_fd(hc_ctx *ctx, int t_its);
+int hc_wall_shear_gradient(hc_ctx *ctx, int nsteps, int ckpt_every, double *loss);
+
 /* Ensembles of independent channels (no counterpart in the reference, which
  * runs one channel per job).  The members are contexts made by hc_create and
  * prepared by hc_init; they may differ in grid, path (cds) and physics and
