@@ -9,6 +9,10 @@ on a seeded synthetic state (Reichardt log-law mean + 10% Gaussian noise).
 ``poisson`` times the standalone pressure Poisson solve (seeded Gaussian
 right-hand side, plane means removed).
 
+``--workload adjoint`` times the gradient of the wall shear after --steps
+steps by the discrete adjoint (checkpoint every --ckpt steps) against the
+forward run, on 128x128x129.
+
 ``--precision mixed`` times the mixed variant (FP32 fields, stencils and
 FFTs; FP64 low-wavenumber column solves and bulk-flow update) instead.  For
 fd, fd_t and poisson the other precision is also run on the same state for
@@ -172,6 +176,81 @@ def run_ensemble(args):
                    "mixed_members": nmixed},
         "sequential_value": round(cells * args.steps / seq / 1e6, 2),
         "speedup_vs_sequential": round(seq / par, 3), "graph_used": graph, "state_finite": finite,
+    }
+    print(json.dumps(out))
+
+
+def run_adjoint(args):
+    """Configuration 5: gradient of the bottom wall shear after K = --steps
+    RK3 steps (128x128x129 neutral closed channel, Re_tau = 180) with respect
+    to the initial velocity, by the hand-written adjoint with a checkpoint
+    every --ckpt steps; against the forward run of the same K steps."""
+    from hercules_amd import Channel, cases
+    prm = cases.closed_channel(nx=128, ny=128, nz=129, wall_rows=1)
+    st = cases.loglaw_noise(prm, seed=2024, rel_amp=0.05)
+    st.pop("t")
+    n = prm.nx * prm.ny * prm.nz
+    K, ck = args.steps, args.ckpt
+    ch = Channel(prm)
+    for k, v in st.items():
+        ch.set_field(k, v)
+    ch.project()
+    x0 = {k: ch.get_field(k) for k in ("u", "v", "w", "p")}
+    ch.init()
+    ch.wall_shear_gradient(min(K, ck), ck)  # warm-up
+    ch.synchronize()
+    # forward K steps
+    t0 = time.perf_counter()
+    ch.steps(K)
+    ch.synchronize()
+    t_fwd = time.perf_counter() - t0
+    j_fwd = ch.wall_shear()[0]
+    for k, v in x0.items():
+        ch.set_field(k, v)
+    ch.init()
+    t0 = time.perf_counter()
+    loss = ch.wall_shear_gradient(K, ck)
+    ch.synchronize()
+    t_grad = time.perf_counter() - t0
+    g = {k: ch.get_adjoint(k) for k in ("u", "v", "w")}
+    # one central-difference check along a seeded divergence-free direction
+    rng = np.random.default_rng(7)
+    for k in ("u", "v", "w"):
+        a = rng.standard_normal(x0[k].shape)
+        if k == "w":
+            a[-1] = 0.0
+        ch.set_field(k, a)
+    ch.project()
+    d = {k: ch.get_field(k) for k in ("u", "v", "w")}
+    eps = 1e-6 * np.sqrt(sum(np.vdot(x0[k], x0[k]) for k in d) / sum(np.vdot(d[k], d[k]) for k in d))
+    js = []
+    for sgn in (1.0, -1.0):
+        for k, v in x0.items():
+            ch.set_field(k, v + sgn * eps * d[k] if k in d else v)
+        ch.init()
+        ch.steps(K)
+        js.append(ch.wall_shear()[0])
+    fd = (js[0] - js[1]) / (2 * eps)
+    ad = sum(float(np.vdot(g[k], d[k])) for k in d)
+    ch.close()
+    nseg = (K + ck - 1) // ck
+    ckpt_bytes = (nseg * 4 + 3 * min(ck, K) * 4) * n * 8
+    ctx_bytes = (13 + 13) * n * 8 + 4 * (prm.nx // 2) * prm.ny * prm.nz * 16
+    value = n * K / t_grad / 1e6
+    out = {
+        "metric": METRIC, "value": round(value, 2), "unit": "Mcell-RK3-steps/s (gradient)", "n_gpus": 1,
+        "steps": K, "warmup": 1, "ms_per_step": round(t_grad / K * 1e3, 4), "higher_is_better": True,
+        "scaling": "weak", "vs_baseline": None, "dtype": "f64",
+        "data": "synthetic (seeded log-law + 5% noise, projected; no checkpoint file)",
+        "config": {"workload": f"wall-shear gradient after {K} RK3 steps, adjoint with a checkpoint every {ck} steps, "
+                               "neutral closed channel Re_tau=180, 128x128x129, cds=2, wall_rows=1",
+                   "nx": prm.nx, "ny": prm.ny, "nz": prm.nz, "cells": n, "precision": "fp64"},
+        "forward_ms_per_step": round(t_fwd / K * 1e3, 4),
+        "adjoint_over_forward": round(t_grad / t_fwd, 3),
+        "backward_over_forward": round((t_grad - 2 * t_fwd) / t_fwd, 3),
+        "loss": loss, "loss_forward_run": j_fwd,
+        "fd_check": {"eps_rel": 1e-6, "fd": fd, "adjoint": ad, "rel_err": abs(fd - ad) / abs(ad)},
+        "peak_hbm_bytes_estimate": int(ckpt_bytes + ctx_bytes),
     }
     print(json.dumps(out))
 
@@ -342,7 +421,9 @@ def main():
     ap.add_argument("--gpus", type=int, default=1)
     ap.add_argument("--steps", type=int, default=50)
     ap.add_argument("--warmup", type=int, default=5)
-    ap.add_argument("--workload", default="fd", choices=["fd", "fd_t", "sp", "poisson", "ensemble", "copy"])
+    ap.add_argument("--workload", default="fd", choices=["fd", "fd_t", "sp", "poisson", "ensemble", "adjoint",
+                                                          "copy"])
+    ap.add_argument("--ckpt", type=int, default=20, help="adjoint workload: steps between checkpoints")
     ap.add_argument("--cpu-steps", type=int, default=1, help="oracle steps for the CPU baseline")
     ap.add_argument("--no-cpu-baseline", action="store_true")
     ap.add_argument("--no-profile", action="store_true", help="skip the per-kernel event timing")
@@ -360,6 +441,8 @@ def main():
         return copy_calibration(args)
     if args.workload == "ensemble":
         return run_ensemble(args)
+    if args.workload == "adjoint":
+        return run_adjoint(args)
 
     prm, st = build_case(args.workload)
     n = prm.nx * prm.ny * prm.nz

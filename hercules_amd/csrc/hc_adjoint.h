@@ -9,9 +9,11 @@
 // Its adjoint runs the same transforms: for real translation-invariant maps
 // IFFT M FFT, the transpose is IFFT M^H FFT with M^H the conjugate transpose
 // of each column's map (k_adj_column).  The right-hand side is quadratic in
-// (u, v, w); its transpose at the forward state is formed by gathering, for
-// each point, every face interpolant and flux product that reads it
-// (k_adj_rhs).  Affine constants (wall values, u_mrf, the drive) drop out.
+// (u, v, w); its transpose at the forward state is formed in three passes:
+// the total adjoints of f, g, h (k_adj_fbar), the adjoint of every face
+// interpolant at every point (k_adj_faces), and for each point the gather of
+// every face interpolant, diffusion and gradient term that reads it
+// (k_adj_gather).  Affine constants (wall values, u_mrf, the drive) drop out.
 #pragma once
 
 // Forward row (a, b, c) of level k of one column equation, as assemble()
@@ -169,7 +171,10 @@ struct AdjIO {
     const double *fu, *fv, *fw;   // adjoints of this substep's f (next substep's f_old)
     double *au, *av, *aw, *ap;    // out: adjoints of the substep's u, v, w; p in/out
     double *ou, *ov, *ow;         // out: adjoints of f_old
+    double *fb[3];                // scratch: total adjoints of f, g, h
+    double *face[8];              // scratch: adjoints of the face interpolants
 };
+enum { FB_UHX = 0, FB_UHY, FB_VHX, FB_VHY, FB_WHX, FB_WHY, FB_UHZ, FB_VHZ };
 
 // Stencil tables: interpolation to the face "behind" a point (I-, e.g. uhx
 // from u), to the face "ahead" (I+, e.g. vhx from v), and the derivative
@@ -195,181 +200,211 @@ template <> struct AdjStencil<2> {
     static constexpr double wd[4] = {-1.0, 27.0, -27.0, 1.0};
 };
 
-// Transpose of the substep's explicit part at the forward state (u, v, w)
-// of A: one point per thread, gathering every contribution to that point.
+// Periodic-wrap field access and the stencils of the transposed right-hand
+// side at the forward state of A.
 template <int CDS>
-__global__ __launch_bounds__(256) void k_adj_rhs(StepArgs A, AdjIO io)
-{
+struct AdjOps {
     using St = AdjStencil<CDS>;
+    const StepArgs &A;
+    int nx, ny, nz;
+    HC_DEV AdjOps(const StepArgs &a) : A(a), nx(a.nx), ny(a.ny), nz(a.nz) {}
+    HC_DEV size_t at(int i, int j, int k) const { return ((size_t)k * ny + wrapi(j, ny)) * nx + wrapi(i, nx); }
+    HC_DEV double lev(int t, int k) const { return A.lev[t * nz + k]; }
+    HC_DEV double f(const double *p, int i, int j, int k) const { return p[at(i, j, k)]; }
+    HC_DEV double U(int i, int j, int k) const { return A.u[at(i, j, k)]; }
+    HC_DEV double V(int i, int j, int k) const { return A.v[at(i, j, k)]; }
+    HC_DEV double W(int i, int j, int k) const { return (k < 0 || k >= nz - 1) ? 0.0 : A.w[at(i, j, k)]; }
+    // I- and I+ of a field along axis ax (0 x, 1 y)
+    template <class F> HC_DEV double Im(F g, int i, int j, int k, int ax) const
+    {
+        double s = 0.0;
+#pragma unroll
+        for (int q = 0; q < St::NI; ++q) s += St::wi[q] * (ax == 0 ? g(i + St::tm[q], j, k) : g(i, j + St::tm[q], k));
+        return s;
+    }
+    template <class F> HC_DEV double Ip(F g, int i, int j, int k, int ax) const
+    {
+        double s = 0.0;
+#pragma unroll
+        for (int q = 0; q < St::NI; ++q) s += St::wi[q] * (ax == 0 ? g(i + St::tp[q], j, k) : g(i, j + St::tp[q], k));
+        return s;
+    }
+    // transposes of I- and I+
+    template <class F> HC_DEV double ImT(F g, int i, int j, int k, int ax) const
+    {
+        double s = 0.0;
+#pragma unroll
+        for (int q = 0; q < St::NI; ++q) s += St::wi[q] * (ax == 0 ? g(i - St::tm[q], j, k) : g(i, j - St::tm[q], k));
+        return s;
+    }
+    template <class F> HC_DEV double IpT(F g, int i, int j, int k, int ax) const
+    {
+        double s = 0.0;
+#pragma unroll
+        for (int q = 0; q < St::NI; ++q) s += St::wi[q] * (ax == 0 ? g(i - St::tp[q], j, k) : g(i, j - St::tp[q], k));
+        return s;
+    }
+    // adjoint of a quantity at S / P offsets: sc * sum_s w_s fbar(point - s)
+    HC_DEV double DS(const double *p, int i, int j, int k, int ax, double sc) const
+    {
+        double s = 0.0;
+#pragma unroll
+        for (int q = 0; q < St::ND; ++q) s += St::wd[q] * (ax == 0 ? f(p, i - St::sS[q], j, k) : f(p, i, j - St::sS[q], k));
+        return s * sc;
+    }
+    HC_DEV double DP(const double *p, int i, int j, int k, int ax, double sc) const
+    {
+        double s = 0.0;
+#pragma unroll
+        for (int q = 0; q < St::ND; ++q) s += St::wd[q] * (ax == 0 ? f(p, i - St::sP[q], j, k) : f(p, i, j - St::sP[q], k));
+        return s * sc;
+    }
+    // z flux divergence adjoint at face k: cell k (+1/dz), cell k+1 (-1/dz)
+    HC_DEV double EZ(const double *p, int i, int j, int k) const
+    {
+        return f(p, i, j, k) * lev(LV_IDZ, k) - (k + 1 <= nz - 1 ? f(p, i, j, k + 1) * lev(LV_IDZ, k + 1) : 0.0);
+    }
+};
+
+// Pass 1: total adjoints of f, g, h (this substep's f feeds q with ra and
+// the stored f_old' of the next substep) and the adjoints of f_old.
+__global__ __launch_bounds__(256) void k_adj_fbar(StepArgs A, AdjIO io)
+{
+    const size_t plane = (size_t)A.nx * A.ny, n = plane * A.nz, top = plane * (A.nz - 1);
+    const double dra = A.dt * A.ra, drb = A.dt * A.rb;
+    for (size_t pt = blockIdx.x * (size_t)blockDim.x + threadIdx.x; pt < n; pt += (size_t)gridDim.x * blockDim.x) {
+        const double qu = io.qu[pt], qv = io.qv[pt], qw = pt < top ? io.qw[pt] : 0.0;
+        io.fb[0][pt] = io.fu[pt] + dra * qu;
+        io.fb[1][pt] = io.fv[pt] + dra * qv;
+        io.fb[2][pt] = pt < top ? io.fw[pt] + dra * qw : 0.0;
+        io.ou[pt] = drb * qu;
+        io.ov[pt] = drb * qv;
+        io.ow[pt] = drb * qw;
+    }
+}
+
+// Pass 2: adjoints of the face interpolants at every point (sign: f = -uux
+// - ...).  Faces in z (uhz, vhz) and the w faces (whx, why) exist on levels
+// 0..nz-2; the top level is written 0.
+template <int CDS>
+__global__ __launch_bounds__(256) void k_adj_faces(StepArgs A, AdjIO io)
+{
+    const AdjOps<CDS> o(A);
     const int nx = A.nx, ny = A.ny, nz = A.nz;
     const size_t n = (size_t)nx * ny * nz;
     const double cx = CDS == 1 ? A.idx : A.c24x, cy = CDS == 1 ? A.idy : A.c24y;
     const double um = A.u_mrf, fc = A.dp_opt == 3 ? A.fc : 0.0;
-    const double dra = A.dt * A.ra, drb = A.dt * A.rb, cnu = A.rc * 0.5 * A.dt * A.nu, crp = A.rc * A.dt;
-    auto idx3 = [&](int i, int j, int k) -> size_t {
-        return ((size_t)k * ny + wrapi(j, ny)) * nx + wrapi(i, nx);
-    };
-    auto lev = [&](int t, int k) { return A.lev[t * nz + k]; };
-    auto U = [&](int i, int j, int k) { return A.u[idx3(i, j, k)]; };
-    auto V = [&](int i, int j, int k) { return A.v[idx3(i, j, k)]; };
-    auto W = [&](int i, int j, int k) { return (k < 0 || k >= nz - 1) ? 0.0 : A.w[idx3(i, j, k)]; };
-    // adjoints of f, g, h: this substep's f feeds q (ra) and the stored f_old'
-    auto FU = [&](int i, int j, int k) { const size_t c = idx3(i, j, k); return io.fu[c] + dra * io.qu[c]; };
-    auto FV = [&](int i, int j, int k) { const size_t c = idx3(i, j, k); return io.fv[c] + dra * io.qv[c]; };
-    auto FW = [&](int i, int j, int k) {
-        if (k >= nz - 1) return 0.0;
-        const size_t c = idx3(i, j, k);
-        return io.fw[c] + dra * io.qw[c];
-    };
-    // interpolants
-    auto Im = [&](auto F, int i, int j, int k, int ax) {  // I-, axis ax (0 x, 1 y)
-        double s = 0.0;
-#pragma unroll
-        for (int q = 0; q < St::NI; ++q) s += St::wi[q] * (ax == 0 ? F(i + St::tm[q], j, k) : F(i, j + St::tm[q], k));
-        return s;
-    };
-    auto Ip = [&](auto F, int i, int j, int k, int ax) {  // I+
-        double s = 0.0;
-#pragma unroll
-        for (int q = 0; q < St::NI; ++q) s += St::wi[q] * (ax == 0 ? F(i + St::tp[q], j, k) : F(i, j + St::tp[q], k));
-        return s;
-    };
-    // adjoint of a quantity at S / P offsets: sum_s w_s fbar(point - s)
-    auto DS = [&](auto F, int i, int j, int k, int ax, double sc) {
-        double s = 0.0;
-#pragma unroll
-        for (int q = 0; q < St::ND; ++q) s += St::wd[q] * (ax == 0 ? F(i - St::sS[q], j, k) : F(i, j - St::sS[q], k));
-        return s * sc;
-    };
-    auto DP = [&](auto F, int i, int j, int k, int ax, double sc) {
-        double s = 0.0;
-#pragma unroll
-        for (int q = 0; q < St::ND; ++q) s += St::wd[q] * (ax == 0 ? F(i - St::sP[q], j, k) : F(i, j - St::sP[q], k));
-        return s * sc;
-    };
-    auto uhx = [&](int i, int j, int k) { return Im(U, i, j, k, 0); };
-    auto uhy = [&](int i, int j, int k) { return Ip(U, i, j, k, 1); };
-    auto vhx = [&](int i, int j, int k) { return Ip(V, i, j, k, 0); };
-    auto vhy = [&](int i, int j, int k) { return Im(V, i, j, k, 1); };
-    auto whx = [&](int i, int j, int k) { return Ip(W, i, j, k, 0); };
-    auto why = [&](int i, int j, int k) { return Ip(W, i, j, k, 1); };
-    auto uhz = [&](int i, int j, int k) {  // faces 0 <= k <= nz-2
-        const double lt = lev(LV_LT, k);
-        return U(i, j, k + 1) * lt + U(i, j, k) * (1.0 - lt);
-    };
-    auto vhz = [&](int i, int j, int k) {
-        const double lt = lev(LV_LT, k);
-        return V(i, j, k + 1) * lt + V(i, j, k) * (1.0 - lt);
-    };
-    // z flux divergence adjoint at face k: cell k (+1/dz), cell k+1 (-1/dz)
-    auto EZ = [&](auto F, int i, int j, int k) {
-        return F(i, j, k) * lev(LV_IDZ, k) - (k + 1 <= nz - 1 ? F(i, j, k + 1) * lev(LV_IDZ, k + 1) : 0.0);
-    };
-    // adjoints of the face interpolants (sign: f = -uux - ...)
-    auto b_uhx = [&](int i, int j, int k) {
-        return -(2.0 * uhx(i, j, k) + um) * DS(FU, i, j, k, 0, cx);
-    };
-    auto b_uhy = [&](int i, int j, int k) {
-        double r = -vhx(i, j, k) * (DP(FU, i, j, k, 1, cy) + DP(FV, i, j, k, 0, cx));
-        if (fc != 0.0) r -= 0.5 * fc * (FV(i, j, k) + FV(i + 1, j, k));
-        return r;
-    };
-    auto b_vhx = [&](int i, int j, int k) {
-        double r = -uhy(i, j, k) * DP(FU, i, j, k, 1, cy) - (uhy(i, j, k) + um) * DP(FV, i, j, k, 0, cx);
-        if (fc != 0.0) r += 0.5 * fc * (FU(i, j, k) + FU(i, j + 1, k));
-        return r;
-    };
-    auto b_vhy = [&](int i, int j, int k) { return -2.0 * vhy(i, j, k) * DS(FV, i, j, k, 1, cy); };
-    auto b_whx = [&](int i, int j, int k) {  // levels 0..nz-2
-        return -uhz(i, j, k) * EZ(FU, i, j, k) - (uhz(i, j, k) + um) * DP(FW, i, j, k, 0, cx);
-    };
-    auto b_why = [&](int i, int j, int k) {
-        return -vhz(i, j, k) * EZ(FV, i, j, k) - vhz(i, j, k) * DP(FW, i, j, k, 1, cy);
-    };
-    auto b_uhz = [&](int i, int j, int k) {  // faces 0..nz-2
-        return -whx(i, j, k) * (EZ(FU, i, j, k) + DP(FW, i, j, k, 0, cx));
-    };
-    auto b_vhz = [&](int i, int j, int k) {
-        return -why(i, j, k) * (EZ(FV, i, j, k) + DP(FW, i, j, k, 1, cy));
-    };
-    // transposes of I- and I+ applied to an adjoint face field
-    auto ImT = [&](auto B, int i, int j, int k, int ax) {
-        double s = 0.0;
-#pragma unroll
-        for (int q = 0; q < St::NI; ++q) s += St::wi[q] * (ax == 0 ? B(i - St::tm[q], j, k) : B(i, j - St::tm[q], k));
-        return s;
-    };
-    auto IpT = [&](auto B, int i, int j, int k, int ax) {
-        double s = 0.0;
-#pragma unroll
-        for (int q = 0; q < St::NI; ++q) s += St::wi[q] * (ax == 0 ? B(i - St::tp[q], j, k) : B(i, j - St::tp[q], k));
-        return s;
-    };
+    const double *FU = io.fb[0], *FV = io.fb[1], *FW = io.fb[2];
+    auto Uf = [&](int i, int j, int k) { return o.U(i, j, k); };
+    auto Vf = [&](int i, int j, int k) { return o.V(i, j, k); };
+    auto Wf = [&](int i, int j, int k) { return o.W(i, j, k); };
+    for (size_t pt = blockIdx.x * (size_t)blockDim.x + threadIdx.x; pt < n; pt += (size_t)gridDim.x * blockDim.x) {
+        const int i = (int)(pt % nx), j = (int)((pt / nx) % ny), k = (int)(pt / ((size_t)nx * ny));
+        const double uhx = o.Im(Uf, i, j, k, 0), uhy = o.Ip(Uf, i, j, k, 1);
+        const double vhx = o.Ip(Vf, i, j, k, 0), vhy = o.Im(Vf, i, j, k, 1);
+        const double dpu_y = o.DP(FU, i, j, k, 1, cy), dpv_x = o.DP(FV, i, j, k, 0, cx);
+        double b_uhy = -vhx * (dpu_y + dpv_x);
+        double b_vhx = -uhy * dpu_y - (uhy + um) * dpv_x;
+        if (fc != 0.0) {
+            b_uhy -= 0.5 * fc * (o.f(FV, i, j, k) + o.f(FV, i + 1, j, k));
+            b_vhx += 0.5 * fc * (o.f(FU, i, j, k) + o.f(FU, i, j + 1, k));
+        }
+        io.face[FB_UHX][pt] = -(2.0 * uhx + um) * o.DS(FU, i, j, k, 0, cx);
+        io.face[FB_UHY][pt] = b_uhy;
+        io.face[FB_VHX][pt] = b_vhx;
+        io.face[FB_VHY][pt] = -2.0 * vhy * o.DS(FV, i, j, k, 1, cy);
+        double bwx = 0.0, bwy = 0.0, buz = 0.0, bvz = 0.0;
+        if (k <= nz - 2) {
+            const double lt = o.lev(LV_LT, k);
+            const double uhz = o.U(i, j, k + 1) * lt + o.U(i, j, k) * (1.0 - lt);
+            const double vhz = o.V(i, j, k + 1) * lt + o.V(i, j, k) * (1.0 - lt);
+            const double whx = o.Ip(Wf, i, j, k, 0), why = o.Ip(Wf, i, j, k, 1);
+            const double ezu = o.EZ(FU, i, j, k), ezv = o.EZ(FV, i, j, k);
+            const double dpw_x = o.DP(FW, i, j, k, 0, cx), dpw_y = o.DP(FW, i, j, k, 1, cy);
+            bwx = -uhz * ezu - (uhz + um) * dpw_x;
+            bwy = -vhz * ezv - vhz * dpw_y;
+            buz = -whx * (ezu + dpw_x);
+            bvz = -why * (ezv + dpw_y);
+        }
+        io.face[FB_WHX][pt] = bwx;
+        io.face[FB_WHY][pt] = bwy;
+        io.face[FB_UHZ][pt] = buz;
+        io.face[FB_VHZ][pt] = bvz;
+    }
+}
+
+// Pass 3: gather of the transposed interpolations, explicit diffusion,
+// pressure gradients and the w self-advection at every point.
+template <int CDS>
+__global__ __launch_bounds__(256) void k_adj_gather(StepArgs A, AdjIO io)
+{
+    const AdjOps<CDS> o(A);
+    const int nx = A.nx, ny = A.ny, nz = A.nz;
+    const size_t pl = (size_t)nx * ny, n = pl * nz;
+    const double cx = CDS == 1 ? A.idx : A.c24x, cy = CDS == 1 ? A.idy : A.c24y;
+    const double cnu = A.rc * 0.5 * A.dt * A.nu, crp = A.rc * A.dt;
+    const double *FW = io.fb[2];
+    auto face = [&](int id) { return [&, id](int i, int j, int k) { return o.f(io.face[id], i, j, k); }; };
     // symmetric horizontal second differences
-    auto Lh = [&](const double *f, int i, int j, int k) {
-        auto F = [&](int x, int y) { return f[idx3(x, y, k)]; };
+    auto Lh = [&](const double *g, int i, int j, int k) {
+        auto F = [&](int x, int y) { return g[o.at(x, y, k)]; };
         if (CDS == 1)
             return (F(i + 1, j) - 2.0 * F(i, j) + F(i - 1, j)) * A.idx2 + (F(i, j + 1) - 2.0 * F(i, j) + F(i, j - 1)) * A.idy2;
         return (-F(i + 2, j) + 16.0 * F(i + 1, j) - 30.0 * F(i, j) + 16.0 * F(i - 1, j) - F(i - 2, j)) * A.c12x2 +
                (-F(i, j + 2) + 16.0 * F(i, j + 1) - 30.0 * F(i, j) + 16.0 * F(i, j - 1) - F(i, j - 2)) * A.c12y2;
     };
     for (size_t pt = blockIdx.x * (size_t)blockDim.x + threadIdx.x; pt < n; pt += (size_t)gridDim.x * blockDim.x) {
-        const int i = (int)(pt % nx), j = (int)((pt / nx) % ny), k = (int)(pt / ((size_t)nx * ny));
+        const int i = (int)(pt % nx), j = (int)((pt / nx) % ny), k = (int)(pt / pl);
         const double qu = io.qu[pt], qv = io.qv[pt], qw = k < nz - 1 ? io.qw[pt] : 0.0;
-        const double idz = lev(LV_IDZ, k), idzt = lev(LV_IDZT, k), idzb = lev(LV_IDZB, k);
+        const double idz = o.lev(LV_IDZ, k), idzt = o.lev(LV_IDZT, k), idzb = o.lev(LV_IDZB, k);
         // ---- u and v: identity, explicit diffusion (transposed z operator with
         // the no-slip / free-slip ghosts), interpolant transposes
         double dgz = -(idzt + idzb);
         if (k == 0) dgz = -(idzt + 2.0 * idzb);
         if (k == nz - 1) dgz = A.ochannel == 0 ? -(2.0 * idzt + idzb) : -idzb;
-        auto LzT = [&](const double *f) {
-            double s = f[pt] * dgz * idz;
-            if (k >= 1) s += f[pt - (size_t)nx * ny] * lev(LV_IDZT, k - 1) * lev(LV_IDZ, k - 1);
-            if (k <= nz - 2) s += f[pt + (size_t)nx * ny] * lev(LV_IDZB, k + 1) * lev(LV_IDZ, k + 1);
+        auto LzT = [&](const double *g) {
+            double s = g[pt] * dgz * idz;
+            if (k >= 1) s += g[pt - pl] * o.lev(LV_IDZT, k - 1) * o.lev(LV_IDZ, k - 1);
+            if (k <= nz - 2) s += g[pt + pl] * o.lev(LV_IDZB, k + 1) * o.lev(LV_IDZ, k + 1);
             return s;
         };
         double au = qu + cnu * (Lh(io.qu, i, j, k) + LzT(io.qu));
         double av = qv + cnu * (Lh(io.qv, i, j, k) + LzT(io.qv));
-        au += ImT(b_uhx, i, j, k, 0) + IpT(b_uhy, i, j, k, 1);
-        av += IpT(b_vhx, i, j, k, 0) + ImT(b_vhy, i, j, k, 1);
+        au += o.ImT(face(FB_UHX), i, j, k, 0) + o.IpT(face(FB_UHY), i, j, k, 1);
+        av += o.IpT(face(FB_VHX), i, j, k, 0) + o.ImT(face(FB_VHY), i, j, k, 1);
         if (k <= nz - 2) {
-            const double lt = lev(LV_LT, k);
-            au += b_uhz(i, j, k) * (1.0 - lt);
-            av += b_vhz(i, j, k) * (1.0 - lt);
+            const double lt = o.lev(LV_LT, k);
+            au += io.face[FB_UHZ][pt] * (1.0 - lt);
+            av += io.face[FB_VHZ][pt] * (1.0 - lt);
         }
         if (k >= 1) {
-            const double lt = lev(LV_LT, k - 1);
-            au += b_uhz(i, j, k - 1) * lt;
-            av += b_vhz(i, j, k - 1) * lt;
+            const double lt = o.lev(LV_LT, k - 1);
+            au += io.face[FB_UHZ][pt - pl] * lt;
+            av += io.face[FB_VHZ][pt - pl] * lt;
         }
         // ---- w (levels 0..nz-2; w at the top face is fixed)
         double aw = 0.0;
         if (k <= nz - 2) {
-            const size_t pl = (size_t)nx * ny;
-            double s = qw * (-(lev(LV_IDZ, k + 1) + idz) * idzt);
-            if (k >= 1) s += io.qw[pt - pl] * idz * lev(LV_IDZT, k - 1);
-            if (k + 1 <= nz - 2) s += io.qw[pt + pl] * lev(LV_IDZ, k + 1) * lev(LV_IDZT, k + 1);
+            double s = qw * (-(o.lev(LV_IDZ, k + 1) + idz) * idzt);
+            if (k >= 1) s += io.qw[pt - pl] * idz * o.lev(LV_IDZT, k - 1);
+            if (k + 1 <= nz - 2) s += io.qw[pt + pl] * o.lev(LV_IDZ, k + 1) * o.lev(LV_IDZT, k + 1);
             aw = qw + cnu * (Lh(io.qw, i, j, k) + s);
-            aw += IpT(b_whx, i, j, k, 0) + IpT(b_why, i, j, k, 1);
+            aw += o.IpT(face(FB_WHX), i, j, k, 0) + o.IpT(face(FB_WHY), i, j, k, 1);
             // wwz = ((w(k)+w(k+1))^2/4 - (w(k)+w(k-1))^2/4) / dz_t(k)
-            const double w0 = W(i, j, k);
-            const double c1 = 0.5 * (w0 + W(i, j, k + 1)), c2 = 0.5 * (w0 + W(i, j, k - 1));
-            double r = FW(i, j, k) * (c1 - c2) * idzt;
-            if (k >= 1) r += FW(i, j, k - 1) * 0.5 * (W(i, j, k - 1) + w0) * lev(LV_IDZT, k - 1);
-            if (k + 1 <= nz - 2) r -= FW(i, j, k + 1) * 0.5 * (W(i, j, k + 1) + w0) * lev(LV_IDZT, k + 1);
+            const double w0 = o.W(i, j, k);
+            const double c1 = 0.5 * (w0 + o.W(i, j, k + 1)), c2 = 0.5 * (w0 + o.W(i, j, k - 1));
+            double r = FW[pt] * (c1 - c2) * idzt;
+            if (k >= 1) r += FW[pt - pl] * 0.5 * (o.W(i, j, k - 1) + w0) * o.lev(LV_IDZT, k - 1);
+            if (k + 1 <= nz - 2) r -= FW[pt + pl] * 0.5 * (o.W(i, j, k + 1) + w0) * o.lev(LV_IDZT, k + 1);
             aw -= r;
         }
         // ---- p: the pressure gradients of q
-        double ap = io.ap[pt] - crp * (DS([&](int x, int y, int z) { return io.qu[idx3(x, y, z)]; }, i, j, k, 0, cx) +
-                                       DS([&](int x, int y, int z) { return io.qv[idx3(x, y, z)]; }, i, j, k, 1, cy));
-        if (k >= 1) ap -= crp * io.qw[pt - (size_t)nx * ny] * lev(LV_IDZT, k - 1);
+        double ap = io.ap[pt] - crp * (o.DS(io.qu, i, j, k, 0, cx) + o.DS(io.qv, i, j, k, 1, cy));
+        if (k >= 1) ap -= crp * io.qw[pt - pl] * o.lev(LV_IDZT, k - 1);
         if (k <= nz - 2) ap += crp * qw * idzt;
         io.au[pt] = au;
         io.av[pt] = av;
         io.aw[pt] = aw;
         io.ap[pt] = ap;
-        io.ou[pt] = drb * qu;
-        io.ov[pt] = drb * qv;
-        io.ow[pt] = drb * qw;
     }
 }

@@ -1286,6 +1286,7 @@ struct hc_ctx {
     double *adj[4] = {};
     double *adf[2][3] = {};
     double *adq[3] = {};
+    double *adfb[3] = {}, *adface[8] = {};  // scratch of the transposed right-hand side
     int adcur = 0;
     bool big_lds = false;  // column kernels may use more than 64 KiB of dynamic LDS
     // optional per-kernel timing with HIP events on the context stream
@@ -2719,6 +2720,8 @@ static int ensure_adjoint(hc_ctx *c)
     for (int a = 0; a < 2; ++a)
         for (int q = 0; q < 3 && !e; ++q) e = dalloc(c, (void **)&c->adf[a][q], bytes);
     for (int q = 0; q < 3 && !e; ++q) e = dalloc(c, (void **)&c->adq[q], bytes);
+    for (int q = 0; q < 3 && !e; ++q) e = dalloc(c, (void **)&c->adfb[q], bytes);
+    for (int q = 0; q < 8 && !e; ++q) e = dalloc(c, (void **)&c->adface[q], bytes);
     return e;
 }
 
@@ -2765,8 +2768,17 @@ static int adjoint_substep(hc_ctx *c, int t_its)
     io.fu = c->adf[cur][0]; io.fv = c->adf[cur][1]; io.fw = c->adf[cur][2];
     io.au = c->adj[0]; io.av = c->adj[1]; io.aw = c->adj[2]; io.ap = c->adj[3];
     io.ou = c->adf[nxt][0]; io.ov = c->adf[nxt][1]; io.ow = c->adf[nxt][2];
-    if (P.cds == 1) hipLaunchKernelGGL((k_adj_rhs<1>), dim3(2048), dim3(256), 0, c->stream, A, io);
-    else hipLaunchKernelGGL((k_adj_rhs<2>), dim3(2048), dim3(256), 0, c->stream, A, io);
+    for (int q = 0; q < 3; ++q) io.fb[q] = c->adfb[q];
+    for (int q = 0; q < 8; ++q) io.face[q] = c->adface[q];
+    const dim3 gp(2048), bp(256);
+    hipLaunchKernelGGL(k_adj_fbar, gp, bp, 0, c->stream, A, io);
+    if (P.cds == 1) {
+        hipLaunchKernelGGL((k_adj_faces<1>), gp, bp, 0, c->stream, A, io);
+        hipLaunchKernelGGL((k_adj_gather<1>), gp, bp, 0, c->stream, A, io);
+    } else {
+        hipLaunchKernelGGL((k_adj_faces<2>), gp, bp, 0, c->stream, A, io);
+        hipLaunchKernelGGL((k_adj_gather<2>), gp, bp, 0, c->stream, A, io);
+    }
     c->adcur = nxt;
     HIPCHK(hipGetLastError());
     return 0;
