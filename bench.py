@@ -13,6 +13,9 @@ right-hand side, plane means removed).
 steps by the discrete adjoint (checkpoint every --ckpt steps) against the
 forward run, on 128x128x129.
 
+Before timing, hc_autotune picks the launch shapes (bit-identical results;
+``--no-autotune`` keeps the defaults).
+
 ``--precision mixed`` times the mixed variant (FP32 fields, stencils and
 FFTs; FP64 low-wavenumber column solves and bulk-flow update) instead.  For
 fd, fd_t and poisson the other precision is also run on the same state for
@@ -143,6 +146,8 @@ def run_ensemble(args):
             ch.set_field(k, v)
         ch.project()
         ch.init()
+        if args.autotune:
+            ch.autotune(steps_per_trial=1)
         members.append(ch)
         cells += prm.nx * prm.ny * prm.nz
     # sequential single-instance launches
@@ -362,6 +367,8 @@ def timed_run(prm, st, args, precision, profile):
         for k, v in st.items():
             ch.set_field(k, v)
         ch.init()
+        if getattr(args, "autotune", False):
+            ch.tuning = ch.autotune(steps_per_trial=2)
         step = lambda: ch.steps(1)
     for _ in range(args.warmup):
         step()
@@ -424,6 +431,9 @@ def main():
     ap.add_argument("--workload", default="fd", choices=["fd", "fd_t", "sp", "poisson", "ensemble", "adjoint",
                                                           "copy"])
     ap.add_argument("--ckpt", type=int, default=20, help="adjoint workload: steps between checkpoints")
+    ap.add_argument("--no-autotune", dest="autotune", action="store_false",
+                    help="keep the default launch shapes (otherwise hc_autotune picks them before timing; "
+                         "fd, fd_t, ensemble)")
     ap.add_argument("--cpu-steps", type=int, default=1, help="oracle steps for the CPU baseline")
     ap.add_argument("--no-cpu-baseline", action="store_true")
     ap.add_argument("--no-profile", action="store_true", help="skip the per-kernel event timing")
@@ -549,6 +559,8 @@ def main():
                     for k, v in kernels.items()},
         "state_finite": finite, "div": div,
     }
+    if getattr(ch, "tuning", None):
+        out["config"]["launch_shapes"] = ch.tuning
     if other:
         out["mixed" if not mixed_main else "fp64"] = other
     if cpu and "value" in cpu and cpu["value"]:

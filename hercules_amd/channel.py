@@ -31,7 +31,7 @@ ABI_SYMBOLS = ("hc_last_error", "hc_create", "hc_destroy", "hc_set_field", "hc_g
                "hc_field_device_ptr", "hc_synchronize", "hc_profile_enable", "hc_profile_read",
                "hc_ensemble_create", "hc_ensemble_steps", "hc_ensemble_graph_used", "hc_ensemble_destroy",
                "hc_substep_fd", "hc_set_adjoint", "hc_get_adjoint", "hc_adjoint_substep_fd",
-               "hc_wall_shear_gradient")
+               "hc_wall_shear_gradient", "hc_autotune", "hc_set_tuning")
 KERNELS = {"rhs": 0, "yfft_fwd": 1, "column": 2, "yfft_inv": 3, "finish": 4, "rhs_t": 5,
            "yfft_t": 6, "column_t": 7, "finish_t": 8, "solve_xr2c": 9, "solve_yfft": 10,
            "solve_column": 11, "solve_xc2r": 12}
@@ -80,6 +80,8 @@ def load_library(path: str = LIB_PATH):
     L.hc_field_device_ptr.argtypes = [vp, ip]
     L.hc_synchronize.argtypes = [vp]
     L.hc_substep_fd.argtypes = [vp, ip]
+    L.hc_autotune.argtypes = [vp, ip]
+    L.hc_set_tuning.argtypes = [vp, ip, ip]
     L.hc_set_adjoint.argtypes = [vp, ip, vp, ip]
     L.hc_get_adjoint.argtypes = [vp, ip, vp, ip]
     L.hc_adjoint_substep_fd.argtypes = [vp, ip]
@@ -179,6 +181,17 @@ class Channel:
                                                  C.c_void_p(src_ptr), C.c_void_p(dst_ptr),
                                                  LAYOUT_INTERIOR))
 
+    def autotune(self, steps_per_trial=2):
+        """Pick the fastest bit-identical launch shapes for this channel
+        (hc_autotune); returns the choices."""
+        self._check(self.L.hc_autotune(self.h, int(steps_per_trial)))
+        return {k: int(self.scalar("tune_" + k)) for k in ("rgroup", "ytile", "c2r_pts", "col_remap")}
+
+    def set_tuning(self, rgroup=None, ytile=None, c2r_pts=None, col_remap=None):
+        for knob, v in enumerate((rgroup, ytile, c2r_pts, col_remap)):
+            if v is not None:
+                self._check(self.L.hc_set_tuning(self.h, knob, int(v)))
+
     def substep(self, t_its):
         """One substep of time_advancement_fd (t_its = 1..3 for RK3)."""
         self._check(self.L.hc_substep_fd(self.h, int(t_its)))
@@ -226,7 +239,8 @@ class Channel:
 
     def scalar(self, name):
         v = C.c_double()
-        ids = {"dpx_drive": 0, "ubar": 1, "time_sim": 2, "got": 3, "stps": 4, "col_tiles": 5}
+        ids = {"dpx_drive": 0, "ubar": 1, "time_sim": 2, "got": 3, "stps": 4, "col_tiles": 5,
+               "tune_rgroup": 6, "tune_ytile": 7, "tune_c2r_pts": 8, "tune_col_remap": 9}
         self._check(self.L.hc_get_scalar(self.h, ids[name], C.byref(v)))
         return v.value
 

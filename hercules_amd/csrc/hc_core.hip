@@ -160,7 +160,7 @@ __global__ __launch_bounds__(rhs2_threads<NX>(), (rhs2_waves<NX, R>())) void k_r
     // tile order: groups of G consecutive y tiles, levels fastest-changing
     // across groups, so that an XCD's L2 holds both the y halos and the
     // neighbouring levels of the tiles in flight
-    const int G = (nty % 16 == 0) ? 16 : (nty % 8 == 0) ? 8 : (nty % 4 == 0) ? 4 : 1;
+    const int G = A.rgroup > 0 ? A.rgroup : (nty % 16 == 0) ? 16 : (nty % 8 == 0) ? 8 : (nty % 4 == 0) ? 4 : 1;
     const int logical = xcd_remap(blockIdx.x, nty * A.nz);
     const int grp = logical / (G * A.nz), rem = logical - grp * (G * A.nz);
     const int k = rem / G, jt = grp * G + rem % G, j0 = jt * TY;
@@ -472,19 +472,14 @@ struct C2RArgsT {
 using C2RArgs = C2RArgsT<double>;
 // points per field of one inverse-x block
 constexpr int kC2RPoints = 1024;
-#ifndef HC_C2R_POINTS_F32
-#define HC_C2R_POINTS_F32 1024
-#endif
-template <class R>
-constexpr int c2r_points() { return sizeof(R) == 4 ? HC_C2R_POINTS_F32 : kC2RPoints; }
-template <int NX, int F, class R = double>
-constexpr int c2r_rows() { return c2r_points<R>() / NX / F > 0 ? c2r_points<R>() / NX / F : 1; }
+template <int NX, int F, int PTS>
+constexpr int c2r_rows() { return PTS / NX / F > 0 ? PTS / NX / F : 1; }
 
-template <int NX, int F, class R>
+template <int NX, int F, class R, int PTS = kC2RPoints>
 __global__ __launch_bounds__(kRhsThreads) void k_x_c2r(C2RArgsT<R> A)
 {
     using C = cplx<R>;
-    constexpr int ROWS = c2r_rows<NX, F, R>();
+    constexpr int ROWS = c2r_rows<NX, F, PTS>();
     constexpr int NL = NX / 2;
     constexpr int B = F * ROWS;
     __shared__ C sq_[B * NL];
@@ -869,6 +864,7 @@ template <class RIO>
 struct ColIO {
     cplx<RIO> *s0, *s1, *s2, *s3;
     int sel, lo_l, lo_m;
+    int remap;  // 1: XCD-contiguous block order (neighbouring l blocks share an L2)
 };
 struct ColPos {
     int bx, jm;
@@ -881,6 +877,12 @@ HC_DEV ColPos col_pos(const ColIO<RIO> &io, int ny)
     q.bx = blockIdx.x;
     q.jm = blockIdx.y;
     q.skip = false;
+    if (io.remap && io.sel != 1) {
+        const int nb = gridDim.x * gridDim.y;
+        const int logical = xcd_remap(blockIdx.x + gridDim.x * blockIdx.y, nb);
+        q.bx = logical % gridDim.x;
+        q.jm = logical / gridDim.x;
+    }
     if (io.sel == 1) {
         const int by = blockIdx.y;
         q.jm = by <= io.lo_m ? by : ny - (2 * io.lo_m + 1) + by;
@@ -1256,6 +1258,13 @@ struct hc_ctx {
     // FP32 (m32); column blocks with l < 8 lo_l and |m| <= lo_m solved in FP64
     bool mixed = false;
     View<float> m32 = {};
+    // launch shapes chosen by hc_autotune (the defaults otherwise)
+    struct {
+        int rgroup = 0;     // RHS tile order group (0: largest of 16, 8, 4, 1 dividing ny/TY)
+        int ytile = 0;      // y transform columns per block (0: 8 for FP64, 16 for FP32)
+        int c2r_pts = 1024; // inverse-x points per field per block
+        int col_remap = 0;  // XCD-contiguous column block order
+    } tune;
     int lo_l = 0, lo_m = 0;
     int nx, ny, nz, nl, Lcol;
     double dx, dy;
@@ -1515,6 +1524,7 @@ static StepArgs base_args(hc_ctx *c)
     A.coe = P.issk == 1 ? 0.5 : 1.0;  // coe_sk, src/module_spectral.f90:55-59
     A.issk = P.issk == 1;
     A.lc = (int)(c->nx / 3.0); A.mc = (int)(c->ny / 3.0);
+    A.rgroup = c->tune.rgroup;
     return A;
 }
 
@@ -1558,6 +1568,7 @@ static StepArgsT<R> rows_args(const StepArgs &A, const View<R> &V)
     B.kdamp0 = A.kdamp0; B.dmask = A.dmask; B.tbc_top = A.tbc_top;
     B.issk = A.issk; B.lc = A.lc; B.mc = A.mc;
     B.ntile = A.ntile;
+    B.rgroup = A.rgroup;
     return B;
 }
 template <>
@@ -1617,8 +1628,10 @@ static int launch_yfft(hc_ctx *c, int sign, int nf, C *f0, typename nd<C>::type 
     YArgsT<R> Y;
     Y.f[0] = f0; Y.f[1] = f1; Y.f[2] = f2; Y.f[3] = f3;
     Y.nl = c->nl; Y.nz = c->nz; Y.tw_ny = view<R>(c).tw_ny;
-    // FP32 spectra: 16 l per block (one cache line per row) where nl allows
-    const bool wide = sizeof(R) == 4 && c->nl % (2 * kYTile) == 0;
+    // FP32 spectra: 16 l per block (one cache line per row) where nl allows;
+    // the autotuner may choose either width for either precision
+    const int tl = c->tune.ytile ? c->tune.ytile : (sizeof(R) == 4 ? 2 * kYTile : kYTile);
+    const bool wide = tl == 2 * kYTile && c->nl % (2 * kYTile) == 0;
     dim3 g(c->nl / (wide ? 2 * kYTile : kYTile), c->nz, nf);
 #define HC_Y(NY_)                                                                                          \
     if (wide) {                                                                                            \
@@ -1639,16 +1652,21 @@ static int launch_c2r(hc_ctx *c, int nf, C2RArgsT<R> &X)
     if (nf < 1 || nf > 4) return set_err(HC_ERR_ARG, "launch_c2r: %d fields", nf);
     const View<R> V = view<R>(c);
     X.ny = c->ny; X.tw_nl = V.tw_nl; X.tw_nx = V.tw_nx;
-    const int pts = c2r_points<R>();
+    const int pts = c->tune.c2r_pts;
     int rows = pts / c->nx / nf > 0 ? pts / c->nx / nf : 1;
     dim3 g((c->ny + rows - 1) / rows, c->nz);
-#define HC_C(NX_)                                                                                       \
-    if (nf == 4) hipLaunchKernelGGL((k_x_c2r<NX_, 4, R>), g, dim3(kRhsThreads), 0, c->stream, X);      \
-    else if (nf == 3) hipLaunchKernelGGL((k_x_c2r<NX_, 3, R>), g, dim3(kRhsThreads), 0, c->stream, X); \
-    else if (nf == 2) hipLaunchKernelGGL((k_x_c2r<NX_, 2, R>), g, dim3(kRhsThreads), 0, c->stream, X); \
-    else hipLaunchKernelGGL((k_x_c2r<NX_, 1, R>), g, dim3(kRhsThreads), 0, c->stream, X);
+#define HC_C1(NX_, P_)                                                                                      \
+    if (nf == 4) hipLaunchKernelGGL((k_x_c2r<NX_, 4, R, P_>), g, dim3(kRhsThreads), 0, c->stream, X);      \
+    else if (nf == 3) hipLaunchKernelGGL((k_x_c2r<NX_, 3, R, P_>), g, dim3(kRhsThreads), 0, c->stream, X); \
+    else if (nf == 2) hipLaunchKernelGGL((k_x_c2r<NX_, 2, R, P_>), g, dim3(kRhsThreads), 0, c->stream, X); \
+    else hipLaunchKernelGGL((k_x_c2r<NX_, 1, R, P_>), g, dim3(kRhsThreads), 0, c->stream, X);
+#define HC_C(NX_)                                 \
+    if (pts == 512) { HC_C1(NX_, 512) }           \
+    else if (pts == 2048) { HC_C1(NX_, 2048) }    \
+    else { HC_C1(NX_, 1024) }
     HC_DISPATCH_N(c->nx, HC_C)
 #undef HC_C
+#undef HC_C1
     return 0;
 }
 
@@ -1687,6 +1705,7 @@ static int launch_columns(hc_ctx *c, int kind, StepArgs A, const View<R> &V, int
     ColIO<R> io;
     io.s0 = V.s[0]; io.s1 = V.s[1]; io.s2 = V.s[2]; io.s3 = V.s[3];
     io.lo_l = c->lo_l; io.lo_m = c->lo_m;
+    io.remap = c->tune.col_remap;
     A.ntile = col_tiles(c, nfields, sizeof(cplx<R>));
     const size_t lds = col_lds(c, nfields, sizeof(cplx<R>));
     const dim3 gfull((c->nl + kColWaves - 1) / kColWaves, c->ny);
@@ -2480,6 +2499,10 @@ extern "C" int hc_get_scalar(hc_ctx *c, int id, double *value)
     case 3: *value = c->got; return 0;
     case 4: *value = c->stps; return 0;
     case 5: *value = col_tiles(c, 4, c->mixed ? sizeof(float2) : sizeof(double2)); return 0;
+    case 6: *value = c->tune.rgroup; return 0;
+    case 7: *value = c->tune.ytile; return 0;
+    case 8: *value = c->tune.c2r_pts; return 0;
+    case 9: *value = c->tune.col_remap; return 0;
     default: return set_err(HC_ERR_ARG, "unknown scalar %d", id);
     }
 }
@@ -2920,4 +2943,143 @@ extern "C" int hc_wall_shear_gradient(hc_ctx *c, int nsteps, int ckpt_every, dou
     if (he != hipSuccess) return set_err(HC_ERR_HIP, "gradient: %s", hipGetErrorString(he));
     if (loss) *loss = tau[0];
     return 0;
+}
+
+// ---- autotuner ------------------------------------------------------------------
+// Launch shapes that leave every result bit-identical (each tile, transform
+// row or column is computed the same way whatever the block shape): the RHS
+// tile-order group, the y-transform columns per block, the inverse-x points
+// per block and the column-kernel block order.  Coordinate descent over the
+// knobs, each trial timing whole steps from the same saved state with HIP
+// events; a candidate replaces the current choice when it is more than 1%
+// faster.  The state and step counters are restored afterwards.
+extern "C" int hc_autotune(hc_ctx *c, int steps_per_trial)
+{
+    if (!c) return set_err(HC_ERR_ARG, "null context");
+    if (steps_per_trial < 1) steps_per_trial = 2;
+    const hc_params &P = c->prm;
+    const size_t N = (size_t)c->nx * c->ny * c->nz;
+    const size_t eb = c->mixed ? sizeof(float) : sizeof(double);
+    std::vector<void *> fields;
+    if (c->mixed) {
+        View<float> &M = c->m32;
+        void *f[] = {M.u, M.v, M.w, M.p, M.t, M.fb[0][0], M.fb[0][1], M.fb[0][2], M.fb[1][0], M.fb[1][1], M.fb[1][2],
+                     M.sb[0], M.sb[1]};
+        fields.assign(f, f + 13);
+    } else {
+        void *f[] = {c->u, c->v, c->w, c->p, c->t, c->fb[0][0], c->fb[0][1], c->fb[0][2], c->fb[1][0], c->fb[1][1],
+                     c->fb[1][2], c->sb[0], c->sb[1]};
+        fields.assign(f, f + 13);
+    }
+    if (P.cds == 3) return 0;  // the spectral path's launch shapes are fixed
+    void *save = nullptr;
+    HIPCHK(hipMalloc(&save, fields.size() * N * eb + 8 * sizeof(double)));
+    char *sv = (char *)save;
+    hipError_t he = hipSuccess;
+    for (size_t q = 0; q < fields.size() && he == hipSuccess; ++q)
+        he = hipMemcpyAsync(sv + q * N * eb, fields[q], N * eb, hipMemcpyDeviceToDevice, c->stream);
+    if (he == hipSuccess)
+        he = hipMemcpyAsync(sv + fields.size() * N * eb, c->scal, 8 * sizeof(double), hipMemcpyDeviceToDevice, c->stream);
+    const int stps0 = c->stps, fcur0 = c->fcur, scur0 = c->scur;
+    const double time0 = c->time_sim, got0 = c->got;
+    auto restore = [&]() -> hipError_t {
+        hipError_t r = hipSuccess;
+        for (size_t q = 0; q < fields.size() && r == hipSuccess; ++q)
+            r = hipMemcpyAsync(fields[q], sv + q * N * eb, N * eb, hipMemcpyDeviceToDevice, c->stream);
+        if (r == hipSuccess)
+            r = hipMemcpyAsync(c->scal, sv + fields.size() * N * eb, 8 * sizeof(double), hipMemcpyDeviceToDevice,
+                               c->stream);
+        c->stps = stps0; c->fcur = fcur0; c->scur = scur0; c->time_sim = time0; c->got = got0;
+        return r;
+    };
+    hipEvent_t e0 = nullptr, e1 = nullptr;
+    if (he == hipSuccess) he = hipEventCreate(&e0);
+    if (he == hipSuccess) he = hipEventCreate(&e1);
+    int err = 0;
+    auto step = [&]() -> int {
+        if (int e = one_step(c)) return e;
+        if (P.dp_opt == 2) return ubar_launch(c);
+        return 0;
+    };
+    auto trial = [&](float &ms) -> int {
+        if ((he = restore()) != hipSuccess) return HC_ERR_HIP;
+        if (int e = step()) return e;
+        if ((he = hipEventRecord(e0, c->stream)) != hipSuccess) return HC_ERR_HIP;
+        for (int n = 0; n < steps_per_trial; ++n)
+            if (int e = step()) return e;
+        if ((he = hipEventRecord(e1, c->stream)) != hipSuccess) return HC_ERR_HIP;
+        if ((he = hipEventSynchronize(e1)) != hipSuccess) return HC_ERR_HIP;
+        if ((he = hipEventElapsedTime(&ms, e0, e1)) != hipSuccess) return HC_ERR_HIP;
+        return 0;
+    };
+    // candidates per knob
+    const int ty = P.nx == 16 ? 16 : P.nx == 32 ? 32 : P.nx == 64 ? 16 : P.nx == 128 ? 8 : P.nx == 192 ? 4 :
+                   P.nx == 256 ? 4 : 1;
+    const int nty = c->ny / ty;
+    std::vector<int> groups = {0};
+    if (P.nx <= 256 && P.isdamp == 0)
+        for (int g : {1, 2, 4, 8, 16, 32})
+            if (nty % g == 0 && g <= nty) groups.push_back(g);
+    std::vector<int> ytiles = {0, kYTile};
+    if (c->nl % (2 * kYTile) == 0) ytiles.push_back(2 * kYTile);
+    int *knob[4] = {&c->tune.rgroup, &c->tune.ytile, &c->tune.c2r_pts, &c->tune.col_remap};
+    std::vector<int> cand[4] = {groups, ytiles, {1024, 512, 2048}, {0, 1}};
+    float best = 0.0f;
+    if (he == hipSuccess) err = trial(best);
+    for (int kb = 0; kb < 4 && !err && he == hipSuccess; ++kb) {
+        const int keep = *knob[kb];
+        int chosen = keep;
+        for (int v : cand[kb]) {
+            if (v == keep) continue;
+            *knob[kb] = v;
+            float ms = 0.0f;
+            if ((err = trial(ms)) || he != hipSuccess) break;
+            if (ms < 0.99f * best) { best = ms; chosen = v; }
+        }
+        *knob[kb] = chosen;
+    }
+    if (he == hipSuccess) he = restore();
+    if (he == hipSuccess) he = hipStreamSynchronize(c->stream);
+    if (e0) (void)hipEventDestroy(e0);
+    if (e1) (void)hipEventDestroy(e1);
+    (void)hipFree(save);
+    if (err) return err == HC_ERR_HIP && he != hipSuccess ? set_err(HC_ERR_HIP, "autotune: %s", hipGetErrorString(he)) : err;
+    if (he != hipSuccess) return set_err(HC_ERR_HIP, "autotune: %s", hipGetErrorString(he));
+    return 0;
+}
+
+// Set one launch shape directly (knob 0 RHS tile-order group, 1 y-transform
+// columns per block, 2 inverse-x points per block, 3 column block order),
+// e.g. to reuse a previous hc_autotune choice.
+extern "C" int hc_set_tuning(hc_ctx *c, int knob, int value)
+{
+    if (!c) return set_err(HC_ERR_ARG, "null context");
+    switch (knob) {
+    case 0: {
+        const hc_params &P = c->prm;
+        const int ty = P.nx == 16 ? 16 : P.nx == 32 ? 32 : P.nx == 64 ? 16 : P.nx == 128 ? 8 : P.nx == 192 ? 4 :
+                       P.nx == 256 ? 4 : 1;
+        const int nty = c->ny / ty;
+        if (value < 0 || (value > 0 && (nty % value != 0)))
+            return set_err(HC_ERR_ARG, "RHS tile group %d must divide %d", value, nty);
+        c->tune.rgroup = value;
+        return 0;
+    }
+    case 1:
+        if (value != 0 && value != kYTile && value != 2 * kYTile)
+            return set_err(HC_ERR_ARG, "y-transform columns per block must be 0, %d or %d", kYTile, 2 * kYTile);
+        c->tune.ytile = value;
+        return 0;
+    case 2:
+        if (value != 512 && value != 1024 && value != 2048)
+            return set_err(HC_ERR_ARG, "inverse-x points per block must be 512, 1024 or 2048");
+        c->tune.c2r_pts = value;
+        return 0;
+    case 3:
+        if (value != 0 && value != 1) return set_err(HC_ERR_ARG, "column block order must be 0 or 1");
+        c->tune.col_remap = value;
+        return 0;
+    default:
+        return set_err(HC_ERR_ARG, "unknown launch-shape knob %d", knob);
+    }
 }

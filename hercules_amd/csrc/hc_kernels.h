@@ -80,6 +80,7 @@ struct StepArgsT {
     int issk, lc, mc;
     const R *aux;  // a physical field read by k_div_partial<3> (the spectral divergence)
     int ntile;     // column kernels: number of nz x kColWaves sub-tiles in the LDS
+    int rgroup;    // RHS tile order: y tiles per group (0: the largest of 16, 8, 4, 1 dividing ny/TY)
 };
 using StepArgs = StepArgsT<double>;
 
