@@ -188,3 +188,36 @@ def test_adjoint_rejects_unsupported(kw):
     with pytest.raises(HerculesError, match="status 2"):
         ch.adjoint_substep(1)
     ch.close()
+
+
+@pytest.mark.parametrize("kind,kw", [("closed", {}), ("closed", {"cds": 1}), ("open", {})])
+def test_gradient_matches_torch_autograd_reference(kind, kw):
+    """The hand-written adjoint against torch.autograd through the FP64
+    torch-ops restatement of the step (tests/torch_reference.py, itself
+    equal to the oracle to round-off)."""
+    torch = pytest.importorskip("torch")
+    from tests.torch_reference import TorchChannel
+    if kind == "closed":
+        prm = cases.closed_channel(nx=16, ny=16, nz=17, wall_rows=1, **kw)
+    else:
+        prm = _params("open", nx=16, ny=16, nz=17, wall_rows=1, **kw)
+    x = _state(prm, seed=12)
+    x = {k: x[k] for k in ("u", "v", "w", "p")}
+    nsteps = 3
+    tc = TorchChannel(prm)
+    xt = {k: torch.tensor(v, requires_grad=True) for k, v in x.items()}
+    jt = tc.wall_shear(tc.steps(xt, nsteps))
+    jt.backward()
+    ch = Channel(prm)
+    _set(ch, x)
+    ch.init()
+    loss = ch.wall_shear_gradient(nsteps, ckpt_every=2)
+    print(kind, kw, f"loss gpu {loss:.15e} torch {float(jt):.15e}")
+    assert abs(loss - float(jt)) <= 1e-12 * abs(loss)
+    for k in ("u", "v", "w", "p"):
+        g = ch.get_adjoint(k)
+        ref = xt[k].grad.numpy()
+        err = np.linalg.norm(g - ref) / np.linalg.norm(ref)
+        print(f"   d loss / d {k}: rel L2 {err:.2e}")
+        assert err < 1e-9, (k, err)
+    ch.close()
