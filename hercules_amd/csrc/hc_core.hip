@@ -2902,10 +2902,23 @@ extern "C" int hc_wall_shear_gradient(hc_ctx *c, int nsteps, int ckpt_every, dou
     const double time0 = c->time_sim, got0 = c->got;
     int err = 0;
     hipError_t he = hipSuccess;
-    // forward with checkpoints
+    // forward with checkpoints; the last segment's substep inputs are kept
+    // as they are computed, so the backward sweep starts without recomputing
+    const int last0 = (nseg - 1) * ckpt_every;
+    int idx_last = 0;
     for (int n = 0; n < nsteps && !err && he == hipSuccess; ++n) {
         if (n % ckpt_every == 0) he = save(ck + (size_t)(n / ckpt_every) * 4 * N);
-        if (he == hipSuccess) err = one_step(c);
+        if (n < last0) {
+            if (he == hipSuccess) err = one_step(c);
+            continue;
+        }
+        c->stps += 1;  // the bookkeeping of one_step
+        c->time_sim += P.dt;
+        if (P.is_ri_var == 1) c->got = P.ri_str + (P.ri_end - P.ri_str) * 1.0 / P.imax * c->stps;
+        for (int s = 1; s <= nsub && !err && he == hipSuccess; ++s) {
+            he = save(seg + (size_t)(idx_last++) * 4 * N);
+            if (he == hipSuccess) err = substep(c, s);
+        }
     }
     double tau[2] = {0.0, 0.0};
     if (!err && he == hipSuccess) err = hc_wall_shear(c, &tau[0], &tau[1]);
@@ -2919,19 +2932,25 @@ extern "C" int hc_wall_shear_gradient(hc_ctx *c, int nsteps, int ckpt_every, dou
         hipLaunchKernelGGL(k_fill, dim3((unsigned)((plane + 255) / 256)), dim3(256), 0, c->stream, c->adj[0],
                            (size_t)0, plane, seed);
     }
-    // backward over the segments
+    // backward over the segments; the transposed substeps read the stored
+    // inputs in place (the context's field pointers are pointed at them)
     for (int sg = nseg - 1; sg >= 0 && !err && he == hipSuccess; --sg) {
         const int n0 = sg * ckpt_every, n1 = std::min(nsteps, n0 + ckpt_every);
-        he = load(ck + (size_t)sg * 4 * N);
-        int idx = 0;
-        for (int n = n0; n < n1 && !err && he == hipSuccess; ++n)
-            for (int s = 1; s <= nsub && !err && he == hipSuccess; ++s) {
-                he = save(seg + (size_t)(idx++) * 4 * N);
-                if (he == hipSuccess) err = substep(c, s);
-            }
+        int idx = idx_last;
+        if (sg != nseg - 1) {
+            he = load(ck + (size_t)sg * 4 * N);
+            idx = 0;
+            for (int n = n0; n < n1 && !err && he == hipSuccess; ++n)
+                for (int s = 1; s <= nsub && !err && he == hipSuccess; ++s) {
+                    he = save(seg + (size_t)(idx++) * 4 * N);
+                    if (he == hipSuccess) err = substep(c, s);
+                }
+        }
         for (int q = idx - 1; q >= 0 && !err && he == hipSuccess; --q) {
-            he = load(seg + (size_t)q * 4 * N);
-            if (he == hipSuccess) err = adjoint_substep(c, q % nsub + 1);
+            double *st = seg + (size_t)q * 4 * N;
+            c->u = st; c->v = st + N; c->w = st + 2 * N; c->p = st + 3 * N;
+            err = adjoint_substep(c, q % nsub + 1);
+            c->u = fld[0]; c->v = fld[1]; c->w = fld[2]; c->p = fld[3];
         }
     }
     // leave the initial state in the context
