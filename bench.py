@@ -14,11 +14,11 @@ steps by the discrete adjoint (checkpoint every --ckpt steps) against the
 forward run, on 128x128x129.
 
 Before timing, hc_autotune picks the launch shapes (bit-identical results;
-``--no-autotune`` keeps the defaults).
+``--no-autotune`` keeps the defaults, ``--launch-shapes`` sets given ones).
 
 ``--precision mixed`` times the mixed variant (FP32 fields, stencils and
 FFTs; FP64 low-wavenumber column solves and bulk-flow update) instead.  For
-fd, fd_t and poisson the other precision is also run on the same state for
+fd, fd_t, sp and poisson the other precision is also run on the same state for
 the same number of steps: the line then carries the FP64/mixed speed ratio
 and the mixed variant's relative L2 error, wall-shear error and divergence
 against FP64 (``mixed`` object), unless ``--no-compare``.
@@ -367,7 +367,10 @@ def timed_run(prm, st, args, precision, profile):
         for k, v in st.items():
             ch.set_field(k, v)
         ch.init()
-        if getattr(args, "autotune", False):
+        if getattr(args, "launch_shapes", None):
+            ch.set_tuning(**args.launch_shapes)
+            ch.tuning = dict(args.launch_shapes)
+        elif getattr(args, "autotune", False):
             ch.tuning = ch.autotune(steps_per_trial=2)
         step = lambda: ch.steps(1)
     for _ in range(args.warmup):
@@ -434,6 +437,9 @@ def main():
     ap.add_argument("--no-autotune", dest="autotune", action="store_false",
                     help="keep the default launch shapes (otherwise hc_autotune picks them before timing; "
                          "fd, fd_t, ensemble)")
+    ap.add_argument("--launch-shapes", default=None,
+                    help="fixed launch shapes instead of autotuning, e.g. rgroup=16,ytile=8,c2r_pts=1024,col_remap=1 "
+                         "(the config.launch_shapes an autotuned run printed; fd, fd_t, sp)")
     ap.add_argument("--cpu-steps", type=int, default=1, help="oracle steps for the CPU baseline")
     ap.add_argument("--no-cpu-baseline", action="store_true")
     ap.add_argument("--no-profile", action="store_true", help="skip the per-kernel event timing")
@@ -442,6 +448,14 @@ def main():
     ap.add_argument("--no-compare", action="store_true",
                     help="skip the run in the other precision (speed ratio and mixed-vs-FP64 errors)")
     args = ap.parse_args()
+    if args.launch_shapes:
+        try:
+            args.launch_shapes = {k: int(v) for k, v in (kv.split("=") for kv in args.launch_shapes.split(","))}
+        except ValueError:
+            ap.error("--launch-shapes takes knob=value pairs separated by commas")
+        unknown = set(args.launch_shapes) - {"rgroup", "ytile", "c2r_pts", "col_remap"}
+        if unknown:
+            ap.error(f"unknown launch-shape knobs {sorted(unknown)}")
     if args.gpus != 1:
         print(f"bench.py: this build runs on one GPU; --gpus {args.gpus} is not supported "
               "(the channel is resident in one MI355X's HBM)", file=sys.stderr)
@@ -458,9 +472,6 @@ def main():
     n = prm.nx * prm.ny * prm.nz
     with_t = args.workload == "fd_t"
     mixed_main = args.precision == "mixed"
-    if mixed_main and args.workload == "sp":
-        print("bench.py: the mixed precision variant covers cds=1,2 (fd, fd_t, poisson)", file=sys.stderr)
-        sys.exit(2)
     if args.workload != "poisson":
         from hercules_amd import Channel
         ch0 = Channel(prm)
@@ -477,7 +488,7 @@ def main():
     # the other precision on the same state and step count: its speed and
     # the mixed variant's departure from FP64
     other = None
-    if args.workload in ("fd", "fd_t", "poisson") and not args.no_compare:
+    if args.workload in ("fd", "fd_t", "sp", "poisson") and not args.no_compare:
         alt = timed_run(prm, st, args, 0 if mixed_main else 1, profile=False)
         m, d = (run, alt) if mixed_main else (alt, run)
         other = compare_precisions(prm, args, m, d, n)

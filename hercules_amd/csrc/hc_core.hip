@@ -1159,7 +1159,7 @@ __global__ __launch_bounds__(256) void k_div_partial(StepArgsT<R> F, StepArgs D,
         double w0 = F.w[P.at(0, 0, k)];
         double wm = k == 0 ? 0.0 : (double)F.w[P.at(0, 0, k - 1)];
         // spectral path: the divergence field was formed and inverted beforehand
-        double d = CDS == 3 ? D.aux[P.at(0, 0, k)] : dudx + dvdy + (w0 - wm) / dz;
+        double d = CDS == 3 ? (double)F.aux[P.at(0, 0, k)] : dudx + dvdy + (w0 - wm) / dz;
         s += fabs(d);
         mx = fmax(mx, fabs(d));
         double u0 = F.u[P.at(0, 0, k)], v0 = F.v[P.at(0, 0, k)];
@@ -1250,6 +1250,10 @@ struct View {
     cplx<R> *s[4];
     R *lev, *symx, *symy;
     cplx<R> *tw_nl, *tw_nx, *tw_ny;
+    // spectral path (cds = 3): field and right-hand-side spectra, product
+    // spectra (sp[0..3] are s[0..3]), a physical scratch field
+    cplx<R> *cu, *cv, *cw, *cp, *ct, *cff, *cgg, *chh, *css, *sp[8];
+    R *ptmp;
 };
 
 struct hc_ctx {
@@ -1539,6 +1543,10 @@ template <> View<double> view<double>(hc_ctx *c)
     for (int q = 0; q < 4; ++q) V.s[q] = c->s[q];
     V.lev = c->lev; V.symx = c->symx; V.symy = c->symy;
     V.tw_nl = c->tw_nl; V.tw_nx = c->tw_nx; V.tw_ny = c->tw_ny;
+    V.cu = c->cu; V.cv = c->cv; V.cw = c->cw; V.cp = c->cp; V.ct = c->ct;
+    V.cff = c->cff; V.cgg = c->cgg; V.chh = c->chh; V.css = c->css;
+    for (int q = 0; q < 8; ++q) V.sp[q] = c->sp[q];
+    V.ptmp = c->ptmp;
     return V;
 }
 template <> View<float> view<float>(hc_ctx *c) { return c->m32; }
@@ -1554,6 +1562,8 @@ static StepArgsT<R> rows_args(const StepArgs &A, const View<R> &V)
     B.s0 = V.s[0]; B.s1 = V.s[1]; B.s2 = V.s[2]; B.s3 = V.s[3];
     B.lev = V.lev; B.symx = V.symx; B.symy = V.symy;
     B.tw_nl = V.tw_nl; B.tw_nx = V.tw_nx; B.tw_ny = V.tw_ny;
+    B.cu = V.cu; B.cv = V.cv; B.cw = V.cw; B.cp = V.cp; B.ct = V.ct;
+    B.cff = V.cff; B.cgg = V.cgg; B.chh = V.chh; B.css = V.css;
     B.scal = A.scal;
     B.nx = A.nx; B.ny = A.ny; B.nz = A.nz; B.nl = A.nl;
     B.ochannel = A.ochannel; B.dp_opt = A.dp_opt; B.isscalar = A.isscalar; B.use_f0 = A.use_f0;
@@ -1867,138 +1877,194 @@ static int substep(hc_ctx *c, int t_its)
 }
 
 // ---- spectral path (cds = 3) ------------------------------------------------
-static int launch_yio(hc_ctx *c, int sign, int nf, const double2 *const *src, double2 *const *dst, const int *mul)
+template <class R>
+static int launch_yio(hc_ctx *c, int sign, int nf, const cplx<R> *const *src, cplx<R> *const *dst, const int *mul)
 {
-    YIOArgs Y;
+    const View<R> V = view<R>(c);
+    YIOArgsT<R> Y;
     std::memset(&Y, 0, sizeof Y);
     for (int q = 0; q < nf; ++q) { Y.src[q] = src[q]; Y.dst[q] = dst[q]; Y.mul[q] = mul[q]; }
-    Y.nl = c->nl; Y.ny = c->ny; Y.tw_ny = c->tw_ny; Y.symx = c->symx; Y.symy = c->symy;
+    Y.nl = c->nl; Y.ny = c->ny; Y.tw_ny = V.tw_ny; Y.symx = V.symx; Y.symy = V.symy;
     dim3 g(c->nl / kYTile, c->nz, nf);
-#define HC_YIO(NY_)                                                                                  \
-    if (sign < 0) hipLaunchKernelGGL((k_yfft_io<NY_, -1>), g, dim3(256), 0, c->stream, Y);          \
-    else hipLaunchKernelGGL((k_yfft_io<NY_, +1>), g, dim3(256), 0, c->stream, Y);
+#define HC_YIO(NY_)                                                                                     \
+    if (sign < 0) hipLaunchKernelGGL((k_yfft_io<NY_, -1, R>), g, dim3(256), 0, c->stream, Y);          \
+    else hipLaunchKernelGGL((k_yfft_io<NY_, +1, R>), g, dim3(256), 0, c->stream, Y);
     HC_DISPATCH_N(c->ny, HC_YIO)
 #undef HC_YIO
     return 0;
 }
 
-static int launch_truncate(hc_ctx *c, double2 *s)
+template <class C>
+static int launch_truncate(hc_ctx *c, C *s)
 {
-    hipLaunchKernelGGL(k_truncate, dim3(1024), dim3(256), 0, c->stream, s, c->nl, c->ny, c->nz, (int)(c->nx / 3.0),
-                       (int)(c->ny / 3.0));
+    hipLaunchKernelGGL((k_truncate<C>), dim3(1024), dim3(256), 0, c->stream, s, c->nl, c->ny, c->nz,
+                       (int)(c->nx / 3.0), (int)(c->ny / 3.0));
     return 0;
 }
 
 // real_2_cmplx (src/module_spectral.f90:1157-1182): x R2C then y FFT, forward normalised
-static int spec_forward(hc_ctx *c, const double *phys, double2 *spec)
+template <class R>
+static int spec_forward(hc_ctx *c, const R *phys, cplx<R> *spec)
 {
-    if (int e = launch_r2c(c, phys, spec)) return e;
+    if (int e = launch_r2c<R>(c, phys, spec)) return e;
     return launch_yfft(c, -1, 1, spec, nullptr, nullptr, nullptr);
 }
 
 // cmplx_2_real (:1188-1217) of up to three spectra, through scratch spectra
-static int spec_inverse(hc_ctx *c, int nf, const double2 *const *spec, double *const *phys)
+template <class R>
+static int spec_inverse(hc_ctx *c, int nf, const cplx<R> *const *spec, R *const *phys)
 {
+    const View<R> V = view<R>(c);
     const int zero[3] = {0, 0, 0};
-    double2 *scr[3] = {c->sp[0], c->sp[1], c->sp[2]};
-    if (int e = launch_yio(c, +1, nf, spec, scr, zero)) return e;
-    C2RArgs X;
+    cplx<R> *scr[3] = {V.sp[0], V.sp[1], V.sp[2]};
+    if (int e = launch_yio<R>(c, +1, nf, spec, scr, zero)) return e;
+    C2RArgsT<R> X;
     std::memset(&X, 0, sizeof X);
     for (int q = 0; q < nf; ++q) { X.src[q] = scr[q]; X.dst[q] = phys[q]; }
-    return launch_c2r(c, nf, X);
+    return launch_c2r<R>(c, nf, X);
 }
 
 // The spectral start-up of drive (src/module_navier_stokes.f90:96-123):
 // spectra of every field, 2/3-rule truncation of u, v, w, p, t, and the
 // truncated fields back in physical space.
-static int spec_init(hc_ctx *c)
+template <class R>
+static int spec_init_t(hc_ctx *c, const View<R> &V)
 {
-    double *phys[9] = {c->u, c->v, c->w, c->p, c->t, c->fb[c->fcur][0], c->fb[c->fcur][1], c->fb[c->fcur][2],
-                       c->sb[c->scur]};
-    double2 *spec[9] = {c->cu, c->cv, c->cw, c->cp, c->ct, c->cff, c->cgg, c->chh, c->css};
+    R *phys[9] = {V.u, V.v, V.w, V.p, V.t, V.fb[c->fcur][0], V.fb[c->fcur][1], V.fb[c->fcur][2], V.sb[c->scur]};
+    cplx<R> *spec[9] = {V.cu, V.cv, V.cw, V.cp, V.ct, V.cff, V.cgg, V.chh, V.css};
     for (int q = 0; q < 9; ++q)
-        if (int e = spec_forward(c, phys[q], spec[q])) return e;
+        if (int e = spec_forward<R>(c, phys[q], spec[q])) return e;
     for (int q = 0; q < 5; ++q)
         if (int e = launch_truncate(c, spec[q])) return e;
-    const double2 *s3[3] = {c->cu, c->cv, c->cw};
-    double *p3[3] = {c->u, c->v, c->w};
-    if (int e = spec_inverse(c, 3, s3, p3)) return e;
-    const double2 *s2[2] = {c->cp, c->ct};
-    double *p2[2] = {c->p, c->t};
-    if (int e = spec_inverse(c, 2, s2, p2)) return e;
+    const cplx<R> *s3[3] = {V.cu, V.cv, V.cw};
+    R *p3[3] = {V.u, V.v, V.w};
+    if (int e = spec_inverse<R>(c, 3, s3, p3)) return e;
+    const cplx<R> *s2[2] = {V.cp, V.ct};
+    R *p2[2] = {V.p, V.t};
+    if (int e = spec_inverse<R>(c, 2, s2, p2)) return e;
     c->spec_dirty = false;
     HIPCHK(hipGetLastError());
     return 0;
 }
+static int spec_init(hc_ctx *c)
+{
+    return c->mixed ? spec_init_t<float>(c, view<float>(c)) : spec_init_t<double>(c, view<double>(c));
+}
+
+// Spectral column pass (kind 0: k_column_spec, 1: k_column_spec_t) over
+// the spectra of V; with FP32 storage the low-wavenumber blocks are solved
+// in FP64 first, as launch_columns does.
+template <int L, class RIO, class RC>
+static void spec_col_kernel(int kind, dim3 g, size_t lds, hipStream_t st, const StepArgs &A, const SpecIO<RIO> &io)
+{
+    const dim3 b(64 * kColWaves);
+    if (kind == 0) hipLaunchKernelGGL((k_column_spec<L, RIO, RC>), g, b, lds, st, A, io);
+    else hipLaunchKernelGGL((k_column_spec_t<L, RIO, RC>), g, b, lds, st, A, io);
+}
+template <class R>
+static int launch_spec_columns(hc_ctx *c, int kind, StepArgs A, const View<R> &V)
+{
+    SpecIO<R> io;
+    io.cu = V.cu; io.cv = V.cv; io.cw = V.cw; io.cp = V.cp; io.ct = V.ct;
+    io.cff = V.cff; io.cgg = V.cgg; io.chh = V.chh; io.css = V.css;
+    for (int q = 0; q < 8; ++q) io.sp[q] = V.sp[q];
+    io.lo_l = c->lo_l; io.lo_m = c->lo_m; io.remap = c->tune.col_remap;
+    const int nfields = kind == 0 ? 4 : 1;
+    A.ntile = col_tiles(c, nfields, sizeof(cplx<R>));
+    const size_t lds = col_lds(c, nfields, sizeof(cplx<R>));
+    const dim3 gfull((c->nl + kColWaves - 1) / kColWaves, c->ny);
+    if constexpr (std::is_same<R, double>::value) {
+        io.sel = 0;
+#define HC_CK(L_) spec_col_kernel<L_, double, double>(kind, gfull, lds, c->stream, A, io);
+        HC_DISPATCH_L(c->Lcol, HC_CK)
+#undef HC_CK
+    } else {
+        if (c->lo_l > 0) {
+            io.sel = 1;
+            const dim3 glo(c->lo_l, 2 * c->lo_m + 1);
+#define HC_CK(L_) spec_col_kernel<L_, float, double>(kind, glo, lds, c->stream, A, io);
+            HC_DISPATCH_L(c->Lcol, HC_CK)
+#undef HC_CK
+        }
+        io.sel = c->lo_l > 0 ? 2 : 0;
+#define HC_CK(L_) spec_col_kernel<L_, float, float>(kind, gfull, lds, c->stream, A, io);
+        HC_DISPATCH_L(c->Lcol, HC_CK)
+#undef HC_CK
+    }
+    return 0;
+}
 
 // one substep of time_advancement_sp (src/module_navier_stokes.f90:393-509)
-static int substep_sp(hc_ctx *c, int t_its)
+template <class R>
+static int substep_sp_t(hc_ctx *c, int t_its, const View<R> &V)
 {
+    using C = cplx<R>;
     const hc_params &P = c->prm;
     if (c->spec_dirty)
-        if (int e = spec_init(c)) return e;
+        if (int e = spec_init_t<R>(c, V)) return e;
     StepArgs A = base_args(c);
     if (P.dts == 2) { A.ra = kRkc1[t_its - 1]; A.rb = kRkc2[t_its - 1]; A.rc = kRkc3[t_its - 1]; }
     else { A.ra = 1.5; A.rb = -0.5; A.rc = 1.0; }
+    StepArgsT<R> B = rows_args<R>(A, V);
+    for (int q = 0; q < 8; ++q) B.sp[q] = V.sp[q];
+    for (int q = 0; q < 6; ++q) B.sdv[q] = V.sp[q];
     if (P.dp_opt == 2) hipLaunchKernelGGL(k_dpx_update, dim3(1), dim3(64), 0, c->stream, c->scal, P.u_mrf, P.nu * 1.0);
     const int rows = rows_for(c->nx);
     const dim3 grow((c->ny + rows - 1) / rows, c->nz);
-    const dim3 gcol((c->nl + kColWaves - 1) / kColWaves, c->ny);
     PROF_BEGIN(c, HC_K_RHS);
     if (P.issk == 1) {
         // du/dx, du/dy, dv/dx, dv/dy, dw/dx, dw/dy in y-physical space
-        const double2 *src4[4] = {c->cu, c->cu, c->cv, c->cv};
-        double2 *dst4[4] = {c->sp[0], c->sp[1], c->sp[2], c->sp[3]};
+        const C *src4[4] = {V.cu, V.cu, V.cv, V.cv};
+        C *dst4[4] = {V.sp[0], V.sp[1], V.sp[2], V.sp[3]};
         const int mul4[4] = {1, 2, 1, 2};
-        if (int e = launch_yio(c, +1, 4, src4, dst4, mul4)) return e;
-        const double2 *src2[2] = {c->cw, c->cw};
-        double2 *dst2[2] = {c->sp[4], c->sp[5]};
+        if (int e = launch_yio<R>(c, +1, 4, src4, dst4, mul4)) return e;
+        const C *src2[2] = {V.cw, V.cw};
+        C *dst2[2] = {V.sp[4], V.sp[5]};
         const int mul2[2] = {1, 2};
-        if (int e = launch_yio(c, +1, 2, src2, dst2, mul2)) return e;
+        if (int e = launch_yio<R>(c, +1, 2, src2, dst2, mul2)) return e;
     }
-#define HC_SM(NX_) hipLaunchKernelGGL((k_spec_mom<NX_>), grow, dim3(rows * NX_), 0, c->stream, A);
+#define HC_SM(NX_) hipLaunchKernelGGL((k_spec_mom<NX_, R>), grow, dim3(rows * NX_), 0, c->stream, B);
     HC_DISPATCH_N(c->nx, HC_SM)
 #undef HC_SM
     PROF_END(c, HC_K_RHS);
     PROF_BEGIN(c, HC_K_YFFT_FWD);
-    if (int e = launch_yfft(c, -1, 4, c->sp[0], c->sp[1], c->sp[2], c->sp[3])) return e;
-    if (int e = launch_yfft(c, -1, 4, c->sp[4], c->sp[5], c->sp[6], c->sp[7])) return e;
+    if (int e = launch_yfft(c, -1, 4, V.sp[0], V.sp[1], V.sp[2], V.sp[3])) return e;
+    if (int e = launch_yfft(c, -1, 4, V.sp[4], V.sp[5], V.sp[6], V.sp[7])) return e;
     PROF_END(c, HC_K_YFFT_FWD);
     PROF_BEGIN(c, HC_K_COLUMN);
-    A.ntile = col_tiles(c, 4);
-#define HC_CSP(L_) hipLaunchKernelGGL((k_column_spec<L_>), gcol, dim3(64 * kColWaves), col_lds(c, 4), c->stream, A);
-    HC_DISPATCH_L(c->Lcol, HC_CSP)
-#undef HC_CSP
+    if (int e = launch_spec_columns<R>(c, 0, A, V)) return e;
     PROF_END(c, HC_K_COLUMN);
     PROF_BEGIN(c, HC_K_FINISH);
     {
-        const double2 *s3[3] = {c->cu, c->cv, c->cw};
-        double *p3[3] = {c->u, c->v, c->w};
-        if (int e = spec_inverse(c, 3, s3, p3)) return e;
+        const C *s3[3] = {V.cu, V.cv, V.cw};
+        R *p3[3] = {V.u, V.v, V.w};
+        if (int e = spec_inverse<R>(c, 3, s3, p3)) return e;
     }
     PROF_END(c, HC_K_FINISH);
     if (P.isscalar == 1) {
         PROF_BEGIN(c, HC_K_RHS_T);
-#define HC_ST(NX_) hipLaunchKernelGGL((k_spec_t<NX_>), grow, dim3(rows * NX_), 0, c->stream, A);
+#define HC_ST(NX_) hipLaunchKernelGGL((k_spec_t<NX_, R>), grow, dim3(rows * NX_), 0, c->stream, B);
         HC_DISPATCH_N(c->nx, HC_ST)
 #undef HC_ST
         PROF_END(c, HC_K_RHS_T);
         PROF_BEGIN(c, HC_K_YFFT_T);
-        if (int e = launch_yfft(c, -1, 3, c->sp[0], c->sp[1], c->sp[2], nullptr)) return e;
+        if (int e = launch_yfft(c, -1, 3, V.sp[0], V.sp[1], V.sp[2], nullptr)) return e;
         PROF_END(c, HC_K_YFFT_T);
         PROF_BEGIN(c, HC_K_COLUMN_T);
-#define HC_CST(L_) hipLaunchKernelGGL((k_column_spec_t<L_>), gcol, dim3(64 * kColWaves), col_lds(c), c->stream, A);
-        HC_DISPATCH_L(c->Lcol, HC_CST)
-#undef HC_CST
+        if (int e = launch_spec_columns<R>(c, 1, A, V)) return e;
         PROF_END(c, HC_K_COLUMN_T);
         PROF_BEGIN(c, HC_K_FINISH_T);
-        const double2 *s1[1] = {c->ct};
-        double *p1[1] = {c->t};
-        if (int e = spec_inverse(c, 1, s1, p1)) return e;
+        const C *s1[1] = {V.ct};
+        R *p1[1] = {V.t};
+        if (int e = spec_inverse<R>(c, 1, s1, p1)) return e;
         PROF_END(c, HC_K_FINISH_T);
     }
     HIPCHK(hipGetLastError());
     return 0;
+}
+static int substep_sp(hc_ctx *c, int t_its)
+{
+    return c->mixed ? substep_sp_t<float>(c, t_its, view<float>(c)) : substep_sp_t<double>(c, t_its, view<double>(c));
 }
 
 static int ubar_launch(hc_ctx *c)
@@ -2038,8 +2104,8 @@ extern "C" int hc_create(const hc_params *prm, hc_ctx **out)
     if (P.wall_rows != 0 && P.wall_rows != 1) return set_err(HC_ERR_ARG, "wall_rows must be 0 or 1");
     if (P.precision != HC_FP64 && P.precision != HC_MIXED) return set_err(HC_ERR_ARG, "precision must be 0 or 1");
     if (P.tbc_top != 0 && P.tbc_top != 1) return set_err(HC_ERR_ARG, "tbc_top must be 0 or 1");
-    if (P.precision == HC_MIXED && (P.cds == 3 || P.isdamp == 1))
-        return set_err(HC_ERR_UNSUPPORTED, "precision=1 (mixed) supports cds=1,2 without damping");
+    if (P.precision == HC_MIXED && P.isdamp == 1)
+        return set_err(HC_ERR_UNSUPPORTED, "precision=1 (mixed) does not cover Rayleigh damping");
     if (!fft_len_ok(P.nx) || !fft_len_ok(P.ny)) return set_err(HC_ERR_UNSUPPORTED, "nx, ny must be one of 16,32,64,128,192,256,512");
     {
         int ty = P.nx == 16 ? 16 : P.nx == 32 ? 32 : P.nx == 64 ? 16 : P.nx == 128 ? 8 : P.nx == 192 ? 4 : P.nx == 256 ? 4 : 1;
@@ -2067,7 +2133,9 @@ extern "C" int hc_create(const hc_params *prm, hc_ctx **out)
     HC_ATTR1((k_column_mom<L_, double, double>))     \
     HC_ATTR1((k_column_mom<L_, float, float>))       \
     HC_ATTR1((k_column_mom<L_, float, double>))      \
-    HC_ATTR1((k_column_spec<L_>))
+    HC_ATTR1((k_column_spec<L_, double, double>))    \
+    HC_ATTR1((k_column_spec<L_, float, float>))      \
+    HC_ATTR1((k_column_spec<L_, float, double>))
         HC_DISPATCH_L(L, HC_ATTR)
 #undef HC_ATTR
 #undef HC_ATTR1
@@ -2119,6 +2187,14 @@ extern "C" int hc_create(const hc_params *prm, hc_ctx **out)
         if (!e) e = dalloc(c, (void **)&M.tw_nl, tnl.size() * sizeof(float2));
         if (!e) e = dalloc(c, (void **)&M.tw_nx, tnx.size() * sizeof(float2));
         if (!e) e = dalloc(c, (void **)&M.tw_ny, tny.size() * sizeof(float2));
+        if (!e && P.cds == 3) {
+            const size_t hs = (size_t)c->nl * c->ny * c->nz * sizeof(float2);
+            float2 **sp[] = {&M.cu, &M.cv, &M.cw, &M.cp, &M.ct, &M.cff, &M.cgg, &M.chh, &M.css,
+                             &M.sp[4], &M.sp[5], &M.sp[6], &M.sp[7]};
+            for (auto pp : sp) if (!e) e = dalloc(c, (void **)pp, hs);
+            for (int q = 0; q < 4; ++q) M.sp[q] = M.s[q];
+            if (!e) e = dalloc(c, (void **)&M.ptmp, N * sizeof(float));
+        }
     }
     if (e) { hc_destroy(c); return e; }
     // sponge strength sigz of rayleigh_damping (src/module_tools.f90:735-736)
@@ -2269,9 +2345,17 @@ extern "C" int hc_get_field(hc_ctx *c, int field, double *dst, int layout)
     if (field < HC_U || field > HC_SS) return set_err(HC_ERR_ARG, "unknown field %d", field);
     if (c->prm.cds == 3 && field == HC_P && !c->spec_dirty) {
         // the spectral path keeps p as a spectrum: cmplx_2_real(c_p, p) (:476)
-        const double2 *s1[1] = {c->cp};
-        double *p1[1] = {c->p};
-        if (int e = spec_inverse(c, 1, s1, p1)) return e;
+        int e = 0;
+        if (c->mixed) {
+            const float2 *s1[1] = {c->m32.cp};
+            float *p1[1] = {c->m32.p};
+            e = spec_inverse<float>(c, 1, s1, p1);
+        } else {
+            const double2 *s1[1] = {c->cp};
+            double *p1[1] = {c->p};
+            e = spec_inverse<double>(c, 1, s1, p1);
+        }
+        if (e) return e;
     }
     int e = c->mixed ? get_field_t<float>(c, field_ptr<float>(c, field), dst, layout)
                      : get_field_t<double>(c, field_ptr<double>(c, field), dst, layout);
@@ -2428,12 +2512,25 @@ extern "C" int hc_screen_div_error(hc_ctx *c, double *div_avg, double *div_max)
         // spectral divergence of (c_u, c_v, w) in physical space
         if (c->spec_dirty)
             if (int e = spec_init(c)) return e;
-        hipLaunchKernelGGL(k_spec_div, dim3(1024), dim3(256), 0, c->stream, A, c->sp[3]);
-        const double2 *s1[1] = {c->sp[3]};
-        double *p1[1] = {c->ptmp};
-        if (int e = spec_inverse(c, 1, s1, p1)) return e;
-        A.aux = c->ptmp;
-        hipLaunchKernelGGL((k_div_partial<3, double>), dim3(c->nz), dim3(256), 0, c->stream, A, A, c->part);
+        if (c->mixed) {
+            View<float> &M = c->m32;
+            hipLaunchKernelGGL((k_spec_div<float2>), dim3(1024), dim3(256), 0, c->stream, A, (const float2 *)M.cu,
+                               (const float2 *)M.cv, (const float2 *)M.cw, M.sp[3]);
+            const float2 *s1[1] = {M.sp[3]};
+            float *p1[1] = {M.ptmp};
+            if (int e = spec_inverse<float>(c, 1, s1, p1)) return e;
+            StepArgsT<float> F = rows_args<float>(A, M);
+            F.aux = M.ptmp;
+            hipLaunchKernelGGL((k_div_partial<3, float>), dim3(c->nz), dim3(256), 0, c->stream, F, A, c->part);
+        } else {
+            hipLaunchKernelGGL((k_spec_div<double2>), dim3(1024), dim3(256), 0, c->stream, A, (const double2 *)c->cu,
+                               (const double2 *)c->cv, (const double2 *)c->cw, c->sp[3]);
+            const double2 *s1[1] = {c->sp[3]};
+            double *p1[1] = {c->ptmp};
+            if (int e = spec_inverse<double>(c, 1, s1, p1)) return e;
+            A.aux = c->ptmp;
+            hipLaunchKernelGGL((k_div_partial<3, double>), dim3(c->nz), dim3(256), 0, c->stream, A, A, c->part);
+        }
     } else if (c->mixed) {
         StepArgsT<float> F = rows_args<float>(A, c->m32);
         if (c->prm.cds == 1)

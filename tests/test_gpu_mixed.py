@@ -1,5 +1,5 @@
 """Mixed precision variant (hc_params.precision = 1): fields, spectra,
-right-hand-side stencils and FFTs in FP32; the column solves of the
+right-hand-side stencils and FFTs in FP32 (FD and spectral paths); the column solves of the
 low-wavenumber blocks (with the kx = ky = 0 mean mode) and the mean pressure
 gradient / bulk velocity update in FP64.  The reference is FP64 only, so the
 variant is held against the FP64 path of this library (itself equal to the
@@ -59,6 +59,12 @@ CASES = [
     ("open_t", dict(nx=128, ny=64, nz=129, is_ri_var=1, ri_str=0.0, ri_end=0.2, imax=40), 40),
     # nx = 512 takes the one-pass right-hand-side kernel
     ("closed", dict(nx=512, ny=16, nz=33, dt=2.5e-4), 10),
+    # spectral path (cds = 3): spectra in FP32, low-wavenumber columns in FP64
+    ("closed", dict(nx=64, ny=64, nz=65, cds=3, issk=1), 40),
+    ("closed", dict(nx=32, ny=32, nz=33, cds=3, issk=0, dts=1), 40),
+    ("closed", dict(nx=64, ny=32, nz=65, cds=3, issk=1, dp_opt=2, u_mrf=0.3), 40),
+    ("open_t", dict(nx=64, ny=64, nz=65, cds=3, issk=1, tbc_top=1, dtdz_top=0.5), 40),
+    ("open_t", dict(nx=128, ny=64, nz=129, cds=3, issk=1, is_ri_var=1, ri_str=0.0, ri_end=0.2, imax=40), 40),
 ]
 
 
@@ -131,7 +137,7 @@ def test_mixed_projection():
     assert out[1][1][1] < 40 * EPS32 * umax / (prm.lx / prm.nx)
 
 
-@pytest.mark.parametrize("kw", [dict(cds=3, issk=1), dict(isdamp=1, nzdamp=8, cdamp=0.1)])
+@pytest.mark.parametrize("kw", [dict(isdamp=1, nzdamp=8, cdamp=0.1), dict(cds=3, issk=1, isdamp=1, nzdamp=8, cdamp=0.1)])
 def test_mixed_rejects_unsupported(kw):
     prm, _ = small_case("closed", nx=32, ny=32, nz=33, **kw)
     with pytest.raises(HerculesError, match="status 2"):
@@ -140,7 +146,8 @@ def test_mixed_rejects_unsupported(kw):
 
 def test_mixed_ensemble_members_equal_single_runs():
     specs = [("closed", dict(nx=32, ny=32, nz=33)), ("open_t", dict(nx=64, ny=32, nz=65)),
-             ("closed", dict(nx=16, ny=16, nz=129, dp_opt=3, fc=0.5, ug=1.0, vg=0.2))]
+             ("closed", dict(nx=16, ny=16, nz=129, dp_opt=3, fc=0.5, ug=1.0, vg=0.2)),
+             ("open_t", dict(nx=32, ny=32, nz=33, cds=3, issk=1))]
     cs = [small_case(kind, **kw) for kind, kw in specs]
 
     def make(prm, st):
