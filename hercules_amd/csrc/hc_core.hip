@@ -23,6 +23,7 @@
 #include <cstring>
 #include <string>
 #include <type_traits>
+#include <functional>
 #include <vector>
 
 #include "../../include/hercules_amd.h"
@@ -265,9 +266,10 @@ __global__ __launch_bounds__(rhs2_threads<NX>(), (rhs2_waves<NX, R>())) void k_r
     }
 }
 
-// Momentum RHS + x R2C of qu,qv,qw through the global-memory accessor of a
-// damped substep (PtD).  Block: rows_per_block full rows of level k.
-template <int CDS, int NX>
+// Momentum RHS + x R2C of qu,qv,qw through a global-memory accessor: PtD in
+// a damped substep, Pt with the 6th-order z operators (ZO = 6).  Block:
+// rows_per_block full rows of level k.
+template <int CDS, int NX, class Acc = PtD, int ZO = 2>
 __global__ __launch_bounds__(kRhsThreads) void k_rhs_mom_damped(StepArgs A)
 {
     constexpr int ROWS = rows_per_block<NX>();
@@ -280,9 +282,9 @@ __global__ __launch_bounds__(kRhsThreads) void k_rhs_mom_damped(StepArgs A)
         int jj = p / NX, i = p - jj * NX, j = j0 + jj;
         double qu = 0.0, qv = 0.0, qw = 0.0;
         if (j < A.ny) {
-            PtD q(A, i, j, k);
+            Acc q(A, i, j, k);
             double f, g, h;
-            momentum_point<CDS>(q, dpx, f, g, h, qu, qv, qw);
+            momentum_point<CDS, ZO>(q, dpx, f, g, h, qu, qv, qw);
             const size_t c = ((size_t)k * A.ny + j) * NX + i;
             A.ff[c] = f;
             A.gg[c] = g;
@@ -337,7 +339,7 @@ __global__ __launch_bounds__(256) void k_damp_apply(double *f, const double *dmp
 }
 
 // Temperature RHS + x R2C of qt.  Acc: Pt, or PtD in a damped substep.
-template <int CDS, int NX, class Acc>
+template <int CDS, int NX, class Acc, int ZO = 2>
 __global__ __launch_bounds__(kRhsThreads) void k_rhs_t(StepArgsT<typename Acc::Real> A)
 {
     using R = typename Acc::Real;
@@ -353,7 +355,7 @@ __global__ __launch_bounds__(kRhsThreads) void k_rhs_t(StepArgsT<typename Acc::R
         if (j < A.ny) {
             Acc q(A, i, j, k);
             R s;
-            temperature_point<CDS>(q, s, qt, A.use_f0 ? A.ss0[q.center()] : R(0));
+            temperature_point<CDS, ZO>(q, s, qt, A.use_f0 ? A.ss0[q.center()] : R(0));
             A.ss[((size_t)k * A.ny + j) * A.nx + i] = s;
         }
         sqd[jj * NX + i] = qt;
@@ -1074,6 +1076,7 @@ __global__ __launch_bounds__(64 * kColWaves) void k_column_solve(StepArgs A, Col
     col_store<L>(io.s0, tile, A.nz, plane, col0, m, d[0]);
 }
 
+#include "hc_zorder6.h"
 #include "hc_adjoint.h"
 
 // ---------------------------------------------------------------------------
@@ -1158,8 +1161,18 @@ __global__ __launch_bounds__(256) void k_div_partial(StepArgsT<R> F, StepArgs D,
         }
         double w0 = F.w[P.at(0, 0, k)];
         double wm = k == 0 ? 0.0 : (double)F.w[P.at(0, 0, k - 1)];
+        double dwdz = (w0 - wm) / dz;
+        if (D.zorder == 6) {  // ZO_D over the faces, zero at the walls
+            const size_t e = (size_t)ZO_D * D.nz + k;
+            const int s0 = D.zst[e];
+            dwdz = 0.0;
+            for (int jz = 0; jz < 6; ++jz) {
+                const int dd = s0 + jz;
+                if (dd > 0 && dd < D.nz) dwdz += D.zwt[e * kZW + jz] * (double)F.w[P.at(0, 0, dd - 1)];
+            }
+        }
         // spectral path: the divergence field was formed and inverted beforehand
-        double d = CDS == 3 ? (double)F.aux[P.at(0, 0, k)] : dudx + dvdy + (w0 - wm) / dz;
+        double d = CDS == 3 ? (double)F.aux[P.at(0, 0, k)] : dudx + dvdy + dwdz;
         s += fabs(d);
         mx = fmax(mx, fabs(d));
         double u0 = F.u[P.at(0, 0, k)], v0 = F.v[P.at(0, 0, k)];
@@ -1280,6 +1293,11 @@ struct hc_ctx {
     int fcur, scur;
     double2 *s[4];
     double *lev, *symx, *symy, *scal, *part, *stage;
+    // 6th-order z operators (zorder = 6): stencil starts and weights
+    int *zst = nullptr;
+    double *zwt = nullptr;
+    std::vector<int> h_zst;
+    std::vector<double> h_zwt;
     double *dmp_s, *dmp_m;  // Rayleigh damping: sigz per level, plane means per (field, level)
     // spectral path (cds = 3): field and right-hand-side spectra, product
     // spectra (sp[0..3] are s[0..3]), a physical scratch field
@@ -1439,6 +1457,121 @@ static void build_levels(hc_ctx *c)
     c->cTe = -0.5 * P.dt * P.alpha / dz[ke] / dzt[ke];
 }
 
+// ---- 6th-order z operators (zorder = 6) ------------------------------------
+// Weights w of the stencil sum_j w_j data_j for the derivative `der` at z0,
+// data_j the value (kind 0) or the first derivative (kind 1) at pts[j]: the
+// n weights that make the stencil exact for every polynomial of degree < n,
+// from the n x n system over the monomials ((z - z0)/h)^i, solved with
+// partial pivoting in long double.
+static bool fd_weights(double z0, const double *pts, const int *kind, int n, int der, double *w)
+{
+    long double h = 0.0L;
+    for (int j = 0; j < n; ++j) h = std::max(h, std::fabs((long double)pts[j] - z0));
+    if (h == 0.0L) return false;
+    long double M[kZW][kZW + 1];
+    for (int i = 0; i < n; ++i) {
+        for (int j = 0; j < n; ++j) {
+            const long double sj = ((long double)pts[j] - z0) / h;
+            if (kind[j] == 0) M[i][j] = std::pow(sj, (long double)i);
+            else M[i][j] = i == 0 ? 0.0L : i * std::pow(sj, (long double)(i - 1)) / h;
+        }
+        long double f = 1.0L;
+        for (int q = 2; q <= der; ++q) f *= q;
+        M[i][n] = i == der ? f / std::pow(h, (long double)der) : 0.0L;
+    }
+    for (int col = 0; col < n; ++col) {
+        int piv = col;
+        for (int r = col + 1; r < n; ++r)
+            if (std::fabs(M[r][col]) > std::fabs(M[piv][col])) piv = r;
+        if (M[piv][col] == 0.0L) return false;
+        if (piv != col)
+            for (int q = 0; q <= n; ++q) std::swap(M[piv][q], M[col][q]);
+        for (int r = 0; r < n; ++r) {
+            if (r == col) continue;
+            const long double f = M[r][col] / M[col][col];
+            for (int q = col; q <= n; ++q) M[r][q] -= f * M[col][q];
+        }
+    }
+    for (int j = 0; j < n; ++j) w[j] = (double)(M[j][n] / M[j][j]);
+    return true;
+}
+
+// Stencil tables of the ZOp operators on the stretched grid: for each target
+// the n consecutive data entries centred on it (shifted inwards at the walls),
+// and their weights.  ZO_P is the product ZO_D o ZO_G restricted to the
+// interior faces (w = 0 at the walls).
+static bool build_zops(hc_ctx *c)
+{
+    const hc_params &P = c->prm;
+    const int nz = P.nz;
+    // faces z_f(0..nz) of get_mesh_param (src/module_parameters.f90:445-466)
+    std::vector<double> zf(nz + 1), zc(nz);
+    {
+        double zmin = 0.0, zmax = P.lz, zmean, a;
+        if (P.ochannel == 0) { zmean = 0.5 * (zmin + zmax); a = (2.0 + nz) * 0.5; }
+        else { zmean = zmin + zmax; a = (1.0 + nz) * 1.0; }
+        for (int k = 1; k <= nz + 1; ++k)
+            zf[k - 1] = zmean + (zmin - zmean) / std::tanh(2.0 * P.zstretch) * std::tanh(2.0 * P.zstretch * (k - a) / (1.0 - a));
+        for (int k = 0; k < nz; ++k) zc[k] = 0.5 * (zf[k] + zf[k + 1]);
+    }
+    c->h_zst.assign((size_t)ZO_COUNT * nz, 0);
+    c->h_zwt.assign((size_t)ZO_COUNT * nz * kZW, 0.0);
+    // data sets: C (wall datum, centres, wall datum), F (faces), P (centres)
+    auto cpos = [&](int d) { return d == 0 ? zf[0] : d == nz + 1 ? zf[nz] : zc[d - 1]; };
+    auto row = [&](int op, int target, int ndata, double center, int n, double z0, int der, bool ntop,
+                   const std::function<double(int)> &pos) -> bool {
+        int lo = (int)std::ceil(center - 0.5 * n);
+        lo = std::max(0, std::min(lo, ndata - n));
+        double pts[kZW], w[kZW];
+        int kind[kZW];
+        for (int j = 0; j < n; ++j) {
+            pts[j] = pos(lo + j);
+            kind[j] = (ntop && lo + j == nz + 1) ? 1 : 0;
+        }
+        if (!fd_weights(z0, pts, kind, n, der, w)) return false;
+        const size_t e = (size_t)op * nz + target;
+        c->h_zst[e] = lo;
+        for (int j = 0; j < n; ++j) c->h_zwt[e * kZW + j] = w[j];
+        return true;
+    };
+    auto fpos = [&](int d) { return zf[d]; };
+    auto ppos = [&](int d) { return zc[d]; };
+    bool ok = true;
+    for (int k = 0; k < nz && ok; ++k) {
+        ok = ok && row(ZO_C2_DD, k, nz + 2, k + 1.0, 7, zc[k], 2, false, cpos);
+        ok = ok && row(ZO_C2_DN, k, nz + 2, k + 1.0, 7, zc[k], 2, true, cpos);
+        ok = ok && row(ZO_D, k, nz + 1, k + 0.5, 6, zc[k], 1, false, fpos);
+        ok = ok && row(ZO_IFC, k, nz + 1, k + 0.5, 6, zc[k], 0, false, fpos);
+        if (k < nz - 1) {  // interior face kw = k, at z_f(k + 1)
+            ok = ok && row(ZO_F2, k, nz + 1, k + 1.0, 7, zf[k + 1], 2, false, fpos);
+            ok = ok && row(ZO_G, k, nz, k + 0.5, 6, zf[k + 1], 1, false, ppos);
+            ok = ok && row(ZO_I_DD, k, nz + 2, k + 1.5, 6, zf[k + 1], 0, false, cpos);
+            ok = ok && row(ZO_I_DN, k, nz + 2, k + 1.5, 6, zf[k + 1], 0, true, cpos);
+        }
+    }
+    if (!ok) return false;
+    // ZO_P row k = sum over the interior faces d of D(k, d) G(d - 1, .)
+    std::vector<double> dense(nz);
+    for (int k = 0; k < nz; ++k) {
+        std::fill(dense.begin(), dense.end(), 0.0);
+        const size_t ed = (size_t)ZO_D * nz + k;
+        for (int j = 0; j < 6; ++j) {
+            const int d = c->h_zst[ed] + j;
+            if (d == 0 || d == nz) continue;
+            const size_t eg = (size_t)ZO_G * nz + (d - 1);
+            for (int i = 0; i < 6; ++i) dense[c->h_zst[eg] + i] += c->h_zwt[ed * kZW + j] * c->h_zwt[eg * kZW + i];
+        }
+        int first = nz, last = -1;
+        for (int q = 0; q < nz; ++q)
+            if (dense[q] != 0.0) { first = std::min(first, q); last = q; }
+        if (last < first || last - first + 1 > kZW) return false;
+        const size_t e = (size_t)ZO_P * nz + k;
+        c->h_zst[e] = first;
+        for (int q = first; q <= last; ++q) c->h_zwt[e * kZW + (q - first)] = dense[q];
+    }
+    return true;
+}
+
 // Per-wavenumber symbols of the horizontal FD operators for wavenumber index
 // `m` of an n-point direction with spacing h (src/module_poisson_solver.f90:73-101,
 // 222-246; the stencils of src/module_cnvdiff.f90 and src/module_navier_stokes.f90:669-796).
@@ -1529,6 +1662,8 @@ static StepArgs base_args(hc_ctx *c)
     A.issk = P.issk == 1;
     A.lc = (int)(c->nx / 3.0); A.mc = (int)(c->ny / 3.0);
     A.rgroup = c->tune.rgroup;
+    A.zorder = P.zorder == 6 ? 6 : 2;
+    A.zst = c->zst; A.zwt = c->zwt;
     return A;
 }
 
@@ -1700,6 +1835,17 @@ template <int L, class RIO, class RC>
 static void col_kernel(int kind, dim3 g, size_t lds, hipStream_t st, const StepArgs &A, const ColIO<RIO> &io, int eq)
 {
     const dim3 b(64 * kColWaves);
+    if constexpr (std::is_same<RIO, double>::value && std::is_same<RC, double>::value) {
+        if (A.zorder == 6) {  // 6th-order z operators (FP64)
+            switch (kind) {
+            case 0: hipLaunchKernelGGL((k_column_mom6<L>), g, b, lds, st, A, io); break;
+            case 1: hipLaunchKernelGGL((k_column_t6<L>), g, b, lds, st, A, io); break;
+            case 2: hipLaunchKernelGGL((k_column_project6<L>), g, b, lds, st, A, io); break;
+            default: hipLaunchKernelGGL((k_column_solve6<L>), g, b, lds, st, A, io, eq); break;
+            }
+            return;
+        }
+    }
     switch (kind) {
     case 0: hipLaunchKernelGGL((k_column_mom<L, RIO, RC>), g, b, lds, st, A, io); break;
     case 1: hipLaunchKernelGGL((k_column_t<L, RIO, RC>), g, b, lds, st, A, io); break;
@@ -1781,7 +1927,18 @@ static int substep_t(hc_ctx *c, int t_its, const View<R> &V)
 #undef HC_RHSD
         }
     }
-    if (!damp) {
+    const bool z6 = P.zorder == 6;
+    if constexpr (std::is_same<R, double>::value) {
+        if (z6) {  // 6th-order z operators: the global-memory point kernel
+            B.dmask = 0;
+#define HC_RHSZ(NX_)                                                                                              \
+    if (P.cds == 1) hipLaunchKernelGGL((k_rhs_mom_damped<1, NX_, Pt, 6>), grow, dim3(kRhsThreads), 0, c->stream, B); \
+    else hipLaunchKernelGGL((k_rhs_mom_damped<2, NX_, Pt, 6>), grow, dim3(kRhsThreads), 0, c->stream, B);
+            HC_DISPATCH_N(c->nx, HC_RHSZ)
+#undef HC_RHSZ
+        }
+    }
+    if (!damp && !z6) {
 #define HC_RHS(NX_)                                                                                     \
     if constexpr (NX_ <= 256) {                                                                          \
         constexpr int TY_ = rhs2_ty<NX_>();                                                              \
@@ -1827,6 +1984,14 @@ static int substep_t(hc_ctx *c, int t_its, const View<R> &V)
         PROF_BEGIN(c, HC_K_RHS_T);
         B.dmask = 16;  // t read damped, the updated velocities as they are
         if constexpr (std::is_same<R, double>::value) {
+            if (z6) {
+                B.dmask = 0;
+#define HC_RTZ(NX_)                                                                                                    \
+    if (P.cds == 1) hipLaunchKernelGGL((k_rhs_t<1, NX_, Pt, 6>), grow, dim3(kRhsThreads), 0, c->stream, B);             \
+    else hipLaunchKernelGGL((k_rhs_t<2, NX_, Pt, 6>), grow, dim3(kRhsThreads), 0, c->stream, B);
+                HC_DISPATCH_N(c->nx, HC_RTZ)
+#undef HC_RTZ
+            }
             if (damp) {
 #define HC_RT(NX_)                                                                                        \
     if (P.cds == 1) hipLaunchKernelGGL((k_rhs_t<1, NX_, PtD>), grow, dim3(kRhsThreads), 0, c->stream, B); \
@@ -1835,7 +2000,7 @@ static int substep_t(hc_ctx *c, int t_its, const View<R> &V)
 #undef HC_RT
             }
         }
-        if (!damp) {
+        if (!damp && !z6) {
 #define HC_RT(NX_)                                                                                               \
     if constexpr (NX_ <= 256) {                                                                                  \
         dim3 g1((unsigned)(c->ny / rhs2_ty<NX_>() * c->nz));                                                     \
@@ -2106,6 +2271,13 @@ extern "C" int hc_create(const hc_params *prm, hc_ctx **out)
     if (P.tbc_top != 0 && P.tbc_top != 1) return set_err(HC_ERR_ARG, "tbc_top must be 0 or 1");
     if (P.precision == HC_MIXED && P.isdamp == 1)
         return set_err(HC_ERR_UNSUPPORTED, "precision=1 (mixed) does not cover Rayleigh damping");
+    if (P.zorder != 0 && P.zorder != 2 && P.zorder != 6) return set_err(HC_ERR_ARG, "zorder must be 2 or 6");
+    if (P.zorder == 6) {
+        if (P.precision != HC_FP64) return set_err(HC_ERR_UNSUPPORTED, "zorder=6 needs precision=0 (FP64)");
+        if (P.isdamp == 1) return set_err(HC_ERR_UNSUPPORTED, "zorder=6 does not cover Rayleigh damping");
+        if (P.cds == 3) return set_err(HC_ERR_UNSUPPORTED, "zorder=6 covers the FD path (cds=1, 2)");
+        if (P.nz < 16) return set_err(HC_ERR_UNSUPPORTED, "zorder=6 needs nz >= 16");
+    }
     if (!fft_len_ok(P.nx) || !fft_len_ok(P.ny)) return set_err(HC_ERR_UNSUPPORTED, "nx, ny must be one of 16,32,64,128,192,256,512");
     {
         int ty = P.nx == 16 ? 16 : P.nx == 32 ? 32 : P.nx == 64 ? 16 : P.nx == 128 ? 8 : P.nx == 192 ? 4 : P.nx == 256 ? 4 : 1;
@@ -2135,7 +2307,11 @@ extern "C" int hc_create(const hc_params *prm, hc_ctx **out)
     HC_ATTR1((k_column_mom<L_, float, double>))      \
     HC_ATTR1((k_column_spec<L_, double, double>))    \
     HC_ATTR1((k_column_spec<L_, float, float>))      \
-    HC_ATTR1((k_column_spec<L_, float, double>))
+    HC_ATTR1((k_column_spec<L_, float, double>))    \
+    HC_ATTR1((k_column_mom6<L_>))                    \
+    HC_ATTR1((k_column_t6<L_>))                      \
+    HC_ATTR1((k_column_project6<L_>))                \
+    HC_ATTR1((k_column_solve6<L_>))
         HC_DISPATCH_L(L, HC_ATTR)
 #undef HC_ATTR
 #undef HC_ATTR1
@@ -2170,6 +2346,11 @@ extern "C" int hc_create(const hc_params *prm, hc_ctx **out)
         for (auto pp : sp) if (!e) e = dalloc(c, (void **)pp, hs);
         for (int q = 0; q < 4; ++q) c->sp[q] = c->s[q];
         if (!e) e = dalloc(c, (void **)&c->ptmp, N * sizeof(double));
+    }
+    if (!e && P.zorder == 6) {
+        if (!build_zops(c)) e = set_err(HC_ERR_ARG, "6th-order z stencils: singular system");
+        if (!e) e = dalloc(c, (void **)&c->zst, c->h_zst.size() * sizeof(int));
+        if (!e) e = dalloc(c, (void **)&c->zwt, c->h_zwt.size() * sizeof(double));
     }
     if (!e && P.precision == HC_MIXED) {
         c->mixed = true;
@@ -2214,6 +2395,10 @@ extern "C" int hc_create(const hc_params *prm, hc_ctx **out)
     if (r == hipSuccess) r = hipMemcpyAsync(c->tw_ny, tny.data(), tny.size() * sizeof(double2), hipMemcpyHostToDevice, c->stream);
     if (r == hipSuccess) r = hipMemcpyAsync(c->scal, scal0, sizeof scal0, hipMemcpyHostToDevice, c->stream);
     if (r == hipSuccess) r = hipMemcpyAsync(c->dmp_s, hs.data(), hs.size() * sizeof(double), hipMemcpyHostToDevice, c->stream);
+    if (r == hipSuccess && c->zst)
+        r = hipMemcpyAsync(c->zst, c->h_zst.data(), c->h_zst.size() * sizeof(int), hipMemcpyHostToDevice, c->stream);
+    if (r == hipSuccess && c->zwt)
+        r = hipMemcpyAsync(c->zwt, c->h_zwt.data(), c->h_zwt.size() * sizeof(double), hipMemcpyHostToDevice, c->stream);
     // FP32 copies of the tables for the mixed variant's row kernels
     std::vector<float> flev(c->h_lev.begin(), c->h_lev.end()), fsx(sx.begin(), sx.end()), fsy(sy.begin(), sy.end());
     auto f2 = [](const std::vector<double2> &t) {
@@ -2826,8 +3011,9 @@ extern "C" int hc_ensemble_destroy(hc_ensemble *e)
 static int adjoint_supported(const hc_ctx *c)
 {
     const hc_params &P = c->prm;
-    if (c->mixed || P.cds == 3 || P.isscalar == 1 || P.isdamp == 1 || P.dp_opt == 2)
-        return set_err(HC_ERR_UNSUPPORTED, "the adjoint covers FP64, cds=1,2, isscalar=0, isdamp=0, dp_opt=1,3");
+    if (c->mixed || P.cds == 3 || P.isscalar == 1 || P.isdamp == 1 || P.dp_opt == 2 || P.zorder == 6)
+        return set_err(HC_ERR_UNSUPPORTED,
+                       "the adjoint covers FP64, cds=1,2, isscalar=0, isdamp=0, dp_opt=1,3, 2nd-order z");
     return 0;
 }
 

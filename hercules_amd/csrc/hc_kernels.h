@@ -37,6 +37,30 @@ enum {
     SY_COUNT
 };
 
+// 6th-order z operators (zorder = 6).  Each row holds up to kZW weights of
+// consecutive data entries starting at zst.  Data indices:
+//   centre data with wall values ("C"): 0 bottom datum (at the bottom wall
+//     face), 1..nz levels 0..nz-1, nz+1 top datum (the top wall value, or the
+//     wall gradient for a Neumann top);
+//   face data ("F"): 0..nz the faces z_f(0..nz), i.e. w levels -1..nz-1,
+//     whose two wall entries are zero;
+//   plain centres ("P"): 0..nz-1 the levels.
+// Rows: levels k for centre targets, w levels kw = 0..nz-2 for face targets.
+enum ZOp {
+    ZO_C2_DD = 0,  // d2/dz2 at centres from C data, value at both walls (7 points)
+    ZO_C2_DN,      // the same with a Neumann top (top datum = wall gradient)
+    ZO_F2,         // d2/dz2 at interior faces from F data (7 points)
+    ZO_G,          // d/dz at interior faces from P data (6 points)
+    ZO_D,          // d/dz at centres from F data (6 points)
+    ZO_I_DD,       // centre-to-face interpolation from C data (6 points)
+    ZO_I_DN,       // the same with a Neumann top
+    ZO_IFC,        // face-to-centre interpolation from F data (6 points)
+    ZO_P,          // ZO_D o ZO_G: the pressure operator on P data (up to kZW points)
+    ZO_COUNT
+};
+constexpr int kZW = 16;
+
+
 // Arguments of the step kernels.  R is the storage and arithmetic type of
 // the fields, spectra and coefficients (double, or float in the mixed
 // precision variant); the device scalars of the mean pressure gradient and
@@ -81,6 +105,11 @@ struct StepArgsT {
     const R *aux;  // a physical field read by k_div_partial<3> (the spectral divergence)
     int ntile;     // column kernels: number of nz x kColWaves sub-tiles in the LDS
     int rgroup;    // RHS tile order: y tiles per group (0: the largest of 16, 8, 4, 1 dividing ny/TY)
+    // 6th-order z operators (zorder = 6, FP64): per operator and target row,
+    // the first data index of the stencil (zst) and kZW weights (zwt)
+    int zorder;
+    const int *zst;
+    const double *zwt;
 };
 using StepArgs = StepArgsT<double>;
 
@@ -185,6 +214,230 @@ struct Interp {
         if (kk == q.A.nz - 1) return q.A.tbc_top ? q.T(0, 0, dk) + Re(0.5) * q.A.gdz_top : q.A.ttop;
         Re lt = q.lev(LV_LT, kk);
         return q.T(0, 0, dk + 1) * lt + q.T(0, 0, dk) * (Re(1.0) - lt);
+    }
+
+    // ---- 6th-order z (zorder = 6): stencils of the ZOp tables over the
+    // data of u, v, t (with their wall values), w (with its zero wall
+    // values) and p.  Only levels inside 0..nz-1 are read.
+    HC_DEV const double *zw(int op, int row, int &s) const
+    {
+        const D &q = d();
+        const size_t e = (size_t)op * q.A.nz + row;
+        s = q.A.zst[e];
+        return q.A.zwt + e * kZW;
+    }
+    // centre data (C convention) of u, v, t at column offset (di, dj)
+    HC_DEV auto udat(int di, int dj, int dd) const
+    {
+        const D &q = d();
+        using Re = typename D::Real;
+        if (dd == 0) return Re(-q.A.u_mrf);
+        if (dd == q.A.nz + 1) return q.A.ochannel == 0 ? Re(-q.A.u_mrf) : Re(0.0);  // value, or du/dz = 0
+        return q.U(di, dj, dd - 1 - q.k);
+    }
+    HC_DEV auto vdat(int di, int dj, int dd) const
+    {
+        const D &q = d();
+        using Re = typename D::Real;
+        if (dd == 0 || dd == q.A.nz + 1) return Re(0.0);
+        return q.V(di, dj, dd - 1 - q.k);
+    }
+    HC_DEV auto tdat(int dd) const
+    {
+        const D &q = d();
+        using Re = typename D::Real;
+        if (dd == 0) return Re(q.A.tbot);
+        if (dd == q.A.nz + 1)  // ttop, or dT/dz = gdz_top / dz_t(kend3)
+            return q.A.tbc_top ? Re(q.A.gdz_top * q.lev(LV_IDZT, q.A.nz - 1)) : Re(q.A.ttop);
+        return q.T(0, 0, dd - 1 - q.k);
+    }
+    // face data (F convention) of w
+    HC_DEV auto wdat(int di, int dj, int dd) const
+    {
+        const D &q = d();
+        using Re = typename D::Real;
+        if (dd == 0 || dd == q.A.nz) return Re(0.0);
+        return q.W(di, dj, dd - 1 - q.k);
+    }
+    // u, v, t at the face of w level k + dk (interior faces), 6 points
+    HC_DEV auto uhz6(int di, int dj, int dk) const
+    {
+        const D &q = d();
+        using Re = typename D::Real;
+        const int kw = q.k + dk;
+        if (kw < 0 || kw >= q.A.nz - 1) return uhz(di, dj, dk);  // wall faces: the wall rule
+        int s;
+        const double *w = zw(q.A.ochannel ? ZO_I_DN : ZO_I_DD, kw, s);
+        Re acc = Re(0.0);
+#pragma unroll
+        for (int j = 0; j < 6; ++j) acc += Re(w[j]) * udat(di, dj, s + j);
+        return acc;
+    }
+    HC_DEV auto vhz6(int di, int dj, int dk) const
+    {
+        const D &q = d();
+        using Re = typename D::Real;
+        const int kw = q.k + dk;
+        if (kw < 0 || kw >= q.A.nz - 1) return vhz(di, dj, dk);
+        int s;
+        const double *w = zw(q.A.ochannel ? ZO_I_DN : ZO_I_DD, kw, s);
+        Re acc = Re(0.0);
+#pragma unroll
+        for (int j = 0; j < 6; ++j) acc += Re(w[j]) * vdat(di, dj, s + j);
+        return acc;
+    }
+    HC_DEV auto thz6(int dk) const
+    {
+        const D &q = d();
+        using Re = typename D::Real;
+        const int kw = q.k + dk;
+        if (kw < 0 || kw >= q.A.nz - 1) return thz(dk);
+        int s;
+        const double *w = zw(q.A.tbc_top ? ZO_I_DN : ZO_I_DD, kw, s);
+        Re acc = Re(0.0);
+#pragma unroll
+        for (int j = 0; j < 6; ++j) acc += Re(w[j]) * tdat(s + j);
+        return acc;
+    }
+    template <int ZO> HC_DEV auto uhzO(int di, int dj, int dk) const
+    {
+        if constexpr (ZO == 6) return uhz6(di, dj, dk);
+        else return uhz(di, dj, dk);
+    }
+    template <int ZO> HC_DEV auto vhzO(int di, int dj, int dk) const
+    {
+        if constexpr (ZO == 6) return vhz6(di, dj, dk);
+        else return vhz(di, dj, dk);
+    }
+    // w at centre level c (absolute), 6 points of face data
+    HC_DEV auto whz6(int c) const
+    {
+        using Re = typename D::Real;
+        int s;
+        const double *w = zw(ZO_IFC, c, s);
+        Re acc = Re(0.0);
+#pragma unroll
+        for (int j = 0; j < 6; ++j) acc += Re(w[j]) * wdat(0, 0, s + j);
+        return acc;
+    }
+    // d(u w)/dz, d(v w)/dz, d(t w)/dz at this centre level: the face fluxes
+    // (6th-order interpolant times the x / y interpolated w) differentiated
+    // by ZO_D; the wall fluxes vanish with w
+    template <int CDS> HC_DEV auto uwz6() const
+    {
+        const D &q = d();
+        using Re = typename D::Real;
+        int s;
+        const double *w = zw(ZO_D, q.k, s);
+        Re acc = Re(0.0);
+#pragma unroll
+        for (int j = 0; j < 6; ++j) {
+            const int dd = s + j;
+            if (dd == 0 || dd == q.A.nz) continue;
+            const int dk = dd - 1 - q.k;
+            acc += Re(w[j]) * uhz6(0, 0, dk) * q.template whx<CDS>(0, 0, dk);
+        }
+        return acc;
+    }
+    template <int CDS> HC_DEV auto vwz6() const
+    {
+        const D &q = d();
+        using Re = typename D::Real;
+        int s;
+        const double *w = zw(ZO_D, q.k, s);
+        Re acc = Re(0.0);
+#pragma unroll
+        for (int j = 0; j < 6; ++j) {
+            const int dd = s + j;
+            if (dd == 0 || dd == q.A.nz) continue;
+            const int dk = dd - 1 - q.k;
+            acc += Re(w[j]) * vhz6(0, 0, dk) * q.template why<CDS>(0, 0, dk);
+        }
+        return acc;
+    }
+    HC_DEV auto twz6() const
+    {
+        const D &q = d();
+        using Re = typename D::Real;
+        int s;
+        const double *w = zw(ZO_D, q.k, s);
+        Re acc = Re(0.0);
+#pragma unroll
+        for (int j = 0; j < 6; ++j) {
+            const int dd = s + j;
+            if (dd == 0 || dd == q.A.nz) continue;
+            const int dk = dd - 1 - q.k;
+            acc += Re(w[j]) * thz6(dk) * q.W(0, 0, dk);
+        }
+        return acc;
+    }
+    // second derivatives at this level: u, v, t at centres, w at its face
+    HC_DEV auto uzz6() const
+    {
+        const D &q = d();
+        using Re = typename D::Real;
+        int s;
+        const double *w = zw(q.A.ochannel ? ZO_C2_DN : ZO_C2_DD, q.k, s);
+        Re acc = Re(0.0);
+#pragma unroll
+        for (int j = 0; j < 7; ++j) acc += Re(w[j]) * udat(0, 0, s + j);
+        return acc;
+    }
+    HC_DEV auto vzz6() const
+    {
+        const D &q = d();
+        using Re = typename D::Real;
+        int s;
+        const double *w = zw(q.A.ochannel ? ZO_C2_DN : ZO_C2_DD, q.k, s);
+        Re acc = Re(0.0);
+#pragma unroll
+        for (int j = 0; j < 7; ++j) acc += Re(w[j]) * vdat(0, 0, s + j);
+        return acc;
+    }
+    HC_DEV auto tzz6() const
+    {
+        const D &q = d();
+        using Re = typename D::Real;
+        int s;
+        const double *w = zw(q.A.tbc_top ? ZO_C2_DN : ZO_C2_DD, q.k, s);
+        Re acc = Re(0.0);
+#pragma unroll
+        for (int j = 0; j < 7; ++j) acc += Re(w[j]) * tdat(s + j);
+        return acc;
+    }
+    HC_DEV auto wzz6() const
+    {
+        const D &q = d();
+        using Re = typename D::Real;
+        int s;
+        const double *w = zw(ZO_F2, q.k, s);
+        Re acc = Re(0.0);
+#pragma unroll
+        for (int j = 0; j < 7; ++j) acc += Re(w[j]) * wdat(0, 0, s + j);
+        return acc;
+    }
+    // d(w^2)/dz at this face from w interpolated to the centres, and dp/dz
+    HC_DEV auto wwz6() const
+    {
+        const D &q = d();
+        using Re = typename D::Real;
+        int s;
+        const double *w = zw(ZO_G, q.k, s);
+        Re acc = Re(0.0);
+#pragma unroll
+        for (int j = 0; j < 6; ++j) acc += Re(w[j]) * sq(whz6(s + j));
+        return acc;
+    }
+    HC_DEV auto pz6() const
+    {
+        const D &q = d();
+        using Re = typename D::Real;
+        int s;
+        const double *w = zw(ZO_G, q.k, s);
+        Re acc = Re(0.0);
+#pragma unroll
+        for (int j = 0; j < 6; ++j) acc += Re(w[j]) * q.P(0, 0, s + j - q.k);
+        return acc;
     }
 };
 
@@ -351,7 +604,7 @@ struct PtD : Interp<PtD> {
 // ---------------------------------------------------------------------------
 // Momentum right-hand side at one point, get_momentum_cnvdiff
 // (src/module_cnvdiff.f90:35-374).  Returns ff,gg,hh and qu,qv,qw.
-template <int CDS, class Q>
+template <int CDS, int ZO = 2, class Q>
 HC_DEV void momentum_point(const Q &q, typename Q::Real dpx, typename Q::Real &f, typename Q::Real &g,
                            typename Q::Real &h, typename Q::Real &qu, typename Q::Real &qv, typename Q::Real &qw)
 {
@@ -380,9 +633,14 @@ HC_DEV void momentum_point(const Q &q, typename Q::Real dpx, typename Q::Real &f
             uyy = (-q.U(0, 2, 0) + Re(16.0) * q.U(0, 1, 0) - Re(30.0) * q.U(0, 0, 0) + Re(16.0) * q.U(0, -1, 0) - q.U(0, -2, 0)) * A.c12y2;
             dpdx = (-q.P(2, 0, 0) + Re(27.0) * q.P(1, 0, 0) - Re(27.0) * q.P(0, 0, 0) + q.P(-1, 0, 0)) * A.c24x;
         }
-        Re uwz = (q.uhz(0, 0, 0) * q.template whx<CDS>(0, 0, 0) - q.uhz(0, 0, -1) * q.template whx<CDS>(0, 0, -1)) * idz;
-        Re u0 = q.U(0, 0, 0);
-        Re uzz = ((q.U(0, 0, 1) - u0) * idzt - (u0 - q.U(0, 0, -1)) * idzb) * idz;
+        Re u0 = q.U(0, 0, 0), uwz, uzz;
+        if constexpr (ZO == 6) {
+            uwz = q.template uwz6<CDS>();
+            uzz = q.uzz6();
+        } else {
+            uwz = (q.uhz(0, 0, 0) * q.template whx<CDS>(0, 0, 0) - q.uhz(0, 0, -1) * q.template whx<CDS>(0, 0, -1)) * idz;
+            uzz = ((q.U(0, 0, 1) - u0) * idzt - (u0 - q.U(0, 0, -1)) * idzb) * idz;
+        }
         if (A.dp_opt == 3)
             f = -uux - uvy - uwz - um * ux + A.fc * (Re(0.5) * (q.template vhx<CDS>(0, 0, 0) + q.template vhx<CDS>(0, -1, 0)) - A.vg);
         else
@@ -411,9 +669,14 @@ HC_DEV void momentum_point(const Q &q, typename Q::Real dpx, typename Q::Real &f
             vyy = (-q.V(0, 2, 0) + Re(16.0) * q.V(0, 1, 0) - Re(30.0) * q.V(0, 0, 0) + Re(16.0) * q.V(0, -1, 0) - q.V(0, -2, 0)) * A.c12y2;
             dpdy = (-q.P(0, 2, 0) + Re(27.0) * q.P(0, 1, 0) - Re(27.0) * q.P(0, 0, 0) + q.P(0, -1, 0)) * A.c24y;
         }
-        Re vwz = (q.vhz(0, 0, 0) * q.template why<CDS>(0, 0, 0) - q.vhz(0, 0, -1) * q.template why<CDS>(0, 0, -1)) * idz;
-        Re v0 = q.V(0, 0, 0);
-        Re vzz = ((q.V(0, 0, 1) - v0) * idzt - (v0 - q.V(0, 0, -1)) * idzb) * idz;
+        Re v0 = q.V(0, 0, 0), vwz, vzz;
+        if constexpr (ZO == 6) {
+            vwz = q.template vwz6<CDS>();
+            vzz = q.vzz6();
+        } else {
+            vwz = (q.vhz(0, 0, 0) * q.template why<CDS>(0, 0, 0) - q.vhz(0, 0, -1) * q.template why<CDS>(0, 0, -1)) * idz;
+            vzz = ((q.V(0, 0, 1) - v0) * idzt - (v0 - q.V(0, 0, -1)) * idzb) * idz;
+        }
         if (A.dp_opt == 3)
             g = -vux - vvy - vwz - um * vx + A.fc * (A.ug - Re(0.5) * (q.template uhy<CDS>(0, 0, 0) + q.template uhy<CDS>(-1, 0, 0)) - um);
         else
@@ -425,31 +688,40 @@ HC_DEV void momentum_point(const Q &q, typename Q::Real dpx, typename Q::Real &f
     if (k < A.nz - 1) {
         Re wux, wvy, wx, wxx, wyy;
         if (CDS == 1) {
-            wux = (q.template whx<1>(0, 0, 0) * q.uhz(0, 0, 0) - q.template whx<1>(-1, 0, 0) * q.uhz(-1, 0, 0)) * A.idx;
-            wvy = (q.template why<1>(0, 0, 0) * q.vhz(0, 0, 0) - q.template why<1>(0, -1, 0) * q.vhz(0, -1, 0)) * A.idy;
+            wux = (q.template whx<1>(0, 0, 0) * q.template uhzO<ZO>(0, 0, 0) - q.template whx<1>(-1, 0, 0) * q.template uhzO<ZO>(-1, 0, 0)) * A.idx;
+            wvy = (q.template why<1>(0, 0, 0) * q.template vhzO<ZO>(0, 0, 0) - q.template why<1>(0, -1, 0) * q.template vhzO<ZO>(0, -1, 0)) * A.idy;
             wx = (q.template whx<1>(0, 0, 0) - q.template whx<1>(-1, 0, 0)) * A.idx;
             wxx = (q.W(1, 0, 0) - Re(2.0) * q.W(0, 0, 0) + q.W(-1, 0, 0)) * A.idx2;
             wyy = (q.W(0, 1, 0) - Re(2.0) * q.W(0, 0, 0) + q.W(0, -1, 0)) * A.idy2;
         } else {
             Re x1 = q.template whx<2>(1, 0, 0), x0 = q.template whx<2>(0, 0, 0), xm = q.template whx<2>(-1, 0, 0), xm2 = q.template whx<2>(-2, 0, 0);
-            wux = (-x1 * q.uhz(1, 0, 0) + Re(27.0) * x0 * q.uhz(0, 0, 0) - Re(27.0) * xm * q.uhz(-1, 0, 0) + xm2 * q.uhz(-2, 0, 0)) * A.c24x;
-            wvy = (-q.template why<2>(0, 1, 0) * q.vhz(0, 1, 0) + Re(27.0) * q.template why<2>(0, 0, 0) * q.vhz(0, 0, 0) -
-                   Re(27.0) * q.template why<2>(0, -1, 0) * q.vhz(0, -1, 0) + q.template why<2>(0, -2, 0) * q.vhz(0, -2, 0)) * A.c24y;
+            wux = (-x1 * q.template uhzO<ZO>(1, 0, 0) + Re(27.0) * x0 * q.template uhzO<ZO>(0, 0, 0) - Re(27.0) * xm * q.template uhzO<ZO>(-1, 0, 0) + xm2 * q.template uhzO<ZO>(-2, 0, 0)) * A.c24x;
+            wvy = (-q.template why<2>(0, 1, 0) * q.template vhzO<ZO>(0, 1, 0) + Re(27.0) * q.template why<2>(0, 0, 0) * q.template vhzO<ZO>(0, 0, 0) -
+                   Re(27.0) * q.template why<2>(0, -1, 0) * q.template vhzO<ZO>(0, -1, 0) + q.template why<2>(0, -2, 0) * q.template vhzO<ZO>(0, -2, 0)) * A.c24y;
             wx = (-x1 + Re(27.0) * x0 - Re(27.0) * xm + xm2) * A.c24x;
             wxx = (-q.W(2, 0, 0) + Re(16.0) * q.W(1, 0, 0) - Re(30.0) * q.W(0, 0, 0) + Re(16.0) * q.W(-1, 0, 0) - q.W(-2, 0, 0)) * A.c12x2;
             wyy = (-q.W(0, 2, 0) + Re(16.0) * q.W(0, 1, 0) - Re(30.0) * q.W(0, 0, 0) + Re(16.0) * q.W(0, -1, 0) - q.W(0, -2, 0)) * A.c12y2;
         }
-        Re w0 = q.W(0, 0, 0), wp = q.W(0, 0, 1), wm = q.W(0, 0, -1);
-        Re a1 = sq(Re(0.5) * (w0 + wp)), a2 = sq(Re(0.5) * (w0 + wm));
-        Re wwz = (a1 - a2) * idzt;
-        Re wzz = ((wp - w0) * q.lev(LV_IDZ, k + 1) - (w0 - wm) * idz) * idzt;
-        if (A.isscalar) {
+        Re w0 = q.W(0, 0, 0), wwz, wzz, dpdz, th;
+        if constexpr (ZO == 6) {
+            wwz = q.wwz6();
+            wzz = q.wzz6();
+            dpdz = q.pz6();
+            th = A.isscalar ? q.thz6(0) : Re(0.0);
+        } else {
+            Re wp = q.W(0, 0, 1), wm = q.W(0, 0, -1);
+            Re a1 = sq(Re(0.5) * (w0 + wp)), a2 = sq(Re(0.5) * (w0 + wm));
+            wwz = (a1 - a2) * idzt;
+            wzz = ((wp - w0) * q.lev(LV_IDZ, k + 1) - (w0 - wm) * idz) * idzt;
+            dpdz = (q.P(0, 0, 1) - q.P(0, 0, 0)) * idzt;
             Re lt = q.lev(LV_LT, k);
-            h = -wux - wvy - wwz - um * wx + A.got * ((Re(1.0) - lt) * q.T(0, 0, 0) + lt * q.T(0, 0, 1) - A.t_ref);
+            th = A.isscalar ? (Re(1.0) - lt) * q.T(0, 0, 0) + lt * q.T(0, 0, 1) : Re(0.0);
+        }
+        if (A.isscalar) {
+            h = -wux - wvy - wwz - um * wx + A.got * (th - A.t_ref);
         } else {
             h = -wux - wvy - wwz - um * wx;
         }
-        Re dpdz = (q.P(0, 0, 1) - q.P(0, 0, 0)) * idzt;
         Re h0 = q.h0();
         qw = w0 + A.dt * (A.ra * h + A.rb * h0) + A.rc * Re(0.5) * A.dt * A.nu * (wxx + wyy + wzz) - A.rc * A.dt * dpdz;
     } else {
@@ -460,7 +732,7 @@ HC_DEV void momentum_point(const Q &q, typename Q::Real dpx, typename Q::Real &f
 
 // Temperature right-hand side at one point, get_temperature_cnvdiff
 // (src/module_cnvdiff.f90:381-473), evaluated with the updated u, v, w.
-template <int CDS, class Q>
+template <int CDS, int ZO = 2, class Q>
 HC_DEV void temperature_point(const Q &q, typename Q::Real &s, typename Q::Real &qt, typename Q::Real s0)
 {
     using Re = typename Q::Real;
@@ -482,9 +754,14 @@ HC_DEV void temperature_point(const Q &q, typename Q::Real &s, typename Q::Real 
         txx = (-q.T(2, 0, 0) + Re(16.0) * q.T(1, 0, 0) - Re(30.0) * q.T(0, 0, 0) + Re(16.0) * q.T(-1, 0, 0) - q.T(-2, 0, 0)) * A.c12x2;
         tyy = (-q.T(0, 2, 0) + Re(16.0) * q.T(0, 1, 0) - Re(30.0) * q.T(0, 0, 0) + Re(16.0) * q.T(0, -1, 0) - q.T(0, -2, 0)) * A.c12y2;
     }
-    Re twz = (q.thz(0) * q.W(0, 0, 0) - q.thz(-1) * q.W(0, 0, -1)) * q.lev(LV_IDZ, k);
-    Re t0 = q.T(0, 0, 0);
-    Re tzz = ((q.T(0, 0, 1) - t0) * q.lev(LV_IDZT, k) - (t0 - q.T(0, 0, -1)) * q.lev(LV_IDZB, k)) * q.lev(LV_IDZ, k);
+    Re t0 = q.T(0, 0, 0), twz, tzz;
+    if constexpr (ZO == 6) {
+        twz = q.twz6();
+        tzz = q.tzz6();
+    } else {
+        twz = (q.thz(0) * q.W(0, 0, 0) - q.thz(-1) * q.W(0, 0, -1)) * q.lev(LV_IDZ, k);
+        tzz = ((q.T(0, 0, 1) - t0) * q.lev(LV_IDZT, k) - (t0 - q.T(0, 0, -1)) * q.lev(LV_IDZB, k)) * q.lev(LV_IDZ, k);
+    }
     s = -tux - tvy - twz - A.u_mrf * tx;
     qt = t0 + A.dt * (A.ra * s + A.rb * s0) + A.rc * Re(0.5) * A.dt * A.alpha * (txx + tyy + tzz);
 }

@@ -1,0 +1,387 @@
+// 6th-order wall-normal operators (hc_params.zorder = 6), column side.
+//
+// The reference is 2nd order in z (src/module_cnvdiff.f90:89,121,
+// src/module_poisson_solver.f90:32-122).  With zorder = 6 the implicit
+// operators of a column are
+//   u, v, t:  I + rc (-hxy - kappa Z)      Z = ZO_C2_* (7-point d2/dz2, centres)
+//   w:        I + rc (-hxy - kappa Z)      Z = ZO_F2   (7-point d2/dz2, faces)
+//   p:        rc (hxy + ZO_P)              ZO_P = ZO_D o ZO_G
+// with hxy the horizontal symbol the 2nd-order rows use.  These are banded
+// (up to 16 diagonals for ZO_P) and not tridiagonal; each is solved exactly
+// by a Chebyshev iteration preconditioned with the 2nd-order tridiagonal
+// operator of the same equation (wave_tridiag): the preconditioned spectrum
+// lies in [0.95, 1.8] on the stretched channel grids, so kZ6Iters
+// iterations reduce the residual by ~1e-15.  Operator products go through
+// the wavefront's slice of the column tile in LDS.  FP64 only.
+#pragma once
+
+constexpr int kZ6Iters = 20;
+
+// y = Z x for the ZOp `op` on this wavefront's column.  Data index d of
+// the table maps to column level d - off; levels outside 0..ndata-1 are
+// zero (the homogeneous wall data).  Rows k >= nrows give y = 0.
+template <int L, int W>
+HC_DEV void z6_apply(const StepArgs &A, int op, int off, int ndata, int nrows, const ColMap &m, double2 *buf,
+                     const double2 (&x)[L], double2 (&y)[L])
+{
+    const int nz = A.nz;
+#pragma unroll
+    for (int r = 0; r < L; ++r)
+        if (r < m.cnt) buf[m.k0 + r] = x[r];
+    __syncthreads();
+#pragma unroll
+    for (int r = 0; r < L; ++r) {
+        double2 acc = make_double2(0.0, 0.0);
+        const int k = m.k0 + r;
+        if (r < m.cnt && k < nrows) {
+            const size_t e = (size_t)op * nz + k;
+            const int s = A.zst[e];
+            const double *w = A.zwt + e * kZW;
+#pragma unroll
+            for (int j = 0; j < W; ++j) {
+                const int lv = s + j - off;
+                if (lv >= 0 && lv < ndata) {
+                    const double2 v = buf[lv];
+                    acc.x += w[j] * v.x;
+                    acc.y += w[j] * v.y;
+                }
+            }
+        }
+        y[r] = acc;
+    }
+    __syncthreads();
+}
+
+// Operator of equation eq (0 u/v, 1 w, 2 p, 3 t) of column (l, jm) applied
+// to x: the matrix the Chebyshev iteration inverts.
+// In the (0,0) mode the pressure operator is singular (Neumann walls); as in
+// the 2nd-order solve, its level-0 row is replaced by the preconditioner's
+// row b0 phi(0) = 0 (delnull, src/module_poisson_solver.f90:558-564), which
+// selects the solution with phi(0) = 0.
+struct Z6Op {
+    int eq, op;
+    double hxy, kappa, rc, b0;
+    bool delnull;
+};
+HC_DEV Z6Op z6_op(const StepArgs &A, int eq, int l, int jm)
+{
+    Z6Op o;
+    o.eq = eq;
+    o.rc = A.rc;
+    o.b0 = 0.0;
+    o.delnull = false;
+    const int nl = A.nl, ny = A.ny;
+    switch (eq) {
+    case 0:
+        o.op = A.ochannel ? ZO_C2_DN : ZO_C2_DD;
+        o.hxy = A.symx[SY_HU * nl + l] + A.symy[SY_HU * ny + jm];
+        o.kappa = 0.5 * A.dt * A.nu;
+        break;
+    case 1:
+        o.op = ZO_F2;
+        o.hxy = A.symx[SY_HU * nl + l] + A.symy[SY_HU * ny + jm];
+        o.kappa = 0.5 * A.dt * A.nu;
+        break;
+    case 2:
+        o.op = ZO_P;
+        o.hxy = A.symx[SY_PX * nl + l] + A.symy[SY_PX * ny + jm];
+        o.kappa = 0.0;
+        break;
+    default:
+        o.op = A.tbc_top ? ZO_C2_DN : ZO_C2_DD;
+        o.hxy = A.symx[SY_HT * nl + l] + A.symy[SY_HT * ny + jm];
+        o.kappa = 0.5 * A.dt * A.alpha;
+        break;
+    }
+    return o;
+}
+template <int L>
+HC_DEV void z6_mul(const StepArgs &A, const Z6Op &o, const ColMap &m, double2 *buf, const double2 (&x)[L],
+                   double2 (&y)[L])
+{
+    const int nz = A.nz;
+    if (o.eq == 2) {
+        z6_apply<L, kZW>(A, o.op, 0, nz, nz, m, buf, x, y);
+#pragma unroll
+        for (int r = 0; r < L; ++r) y[r] = make_double2(o.rc * (o.hxy * x[r].x + y[r].x), o.rc * (o.hxy * x[r].y + y[r].y));
+        if (o.delnull && m.lane == 0) y[0] = make_double2(o.b0 * x[0].x, o.b0 * x[0].y);
+        return;
+    }
+    const bool face = o.eq == 1;
+    // w: data are the w levels 0..nz-2 (the top wall level nz-1 is zero)
+    z6_apply<L, 7>(A, o.op, 1, face ? nz - 1 : nz, face ? nz - 1 : nz, m, buf, x, y);
+    const double dg = 1.0 - o.rc * o.hxy, ck = o.rc * o.kappa;
+#pragma unroll
+    for (int r = 0; r < L; ++r) {
+        if (face && m.k0 + r == nz - 1) { y[r] = x[r]; continue; }  // trivial row: w = 0 at the top wall
+        y[r] = make_double2(dg * x[r].x - ck * y[r].x, dg * x[r].y - ck * y[r].y);
+    }
+}
+
+// Solve o x = rhs in place (x holds rhs on entry), preconditioned with the
+// tridiagonal rows a, b, c of the 2nd-order operator of the same equation.
+template <int L>
+HC_DEV void z6_solve(const StepArgs &A, const Z6Op &o, const ColMap &m, double2 *buf, const double (&a)[L],
+                     const double (&b)[L], const double (&c)[L], double2 (&x)[L])
+{
+    const double lo = 0.9, hi = 1.85, th = 0.5 * (hi + lo), de = 0.5 * (hi - lo), s1 = th / de;
+    double2 r[L], t[L], d[1][L];
+#pragma unroll
+    for (int q = 0; q < L; ++q) { r[q] = x[q]; d[0][q] = x[q]; }
+    wave_tridiag<L, 1>(m.cnt, a, b, c, d);
+#pragma unroll
+    for (int q = 0; q < L; ++q) x[q] = d[0][q];
+    z6_mul<L>(A, o, m, buf, x, t);
+#pragma unroll
+    for (int q = 0; q < L; ++q) { r[q].x -= t[q].x; r[q].y -= t[q].y; d[0][q] = r[q]; }
+    wave_tridiag<L, 1>(m.cnt, a, b, c, d);
+#pragma unroll
+    for (int q = 0; q < L; ++q) { d[0][q].x /= th; d[0][q].y /= th; }
+    double rho0 = 1.0 / s1;
+    for (int it = 0; it < kZ6Iters; ++it) {
+        double2 z[1][L];
+#pragma unroll
+        for (int q = 0; q < L; ++q) { x[q].x += d[0][q].x; x[q].y += d[0][q].y; }
+        z6_mul<L>(A, o, m, buf, d[0], t);
+#pragma unroll
+        for (int q = 0; q < L; ++q) { r[q].x -= t[q].x; r[q].y -= t[q].y; z[0][q] = r[q]; }
+        wave_tridiag<L, 1>(m.cnt, a, b, c, z);
+        const double rho1 = 1.0 / (2.0 * s1 - rho0), cd = rho1 * rho0, cz = 2.0 * rho1 / de;
+#pragma unroll
+        for (int q = 0; q < L; ++q) {
+            d[0][q].x = cd * d[0][q].x + cz * z[0][q].x;
+            d[0][q].y = cd * d[0][q].y + cz * z[0][q].y;
+        }
+        rho0 = rho1;
+    }
+}
+
+// (0,0)-mode wall data of the u and t equations: the Crank-Nicolson
+// implicit half of the wall terms, rc kappa w_wall (wall value or gradient),
+// for the rows whose stencils reach a wall datum.
+template <int L>
+HC_DEV void z6_fixups(const StepArgs &A, const Z6Op &o, const ColMap &m, double2 (&d)[L])
+{
+    const int nz = A.nz;
+    double bot, top;
+    bool has_top;
+    if (o.eq == 0) {
+        bot = -A.u_mrf;
+        top = -A.u_mrf;
+        has_top = A.ochannel == 0;  // open top: du/dz = 0
+    } else if (o.eq == 3) {
+        bot = A.tbot;
+        top = A.tbc_top ? A.gdz_top * A.lev[LV_IDZT * nz + nz - 1] : A.ttop;
+        has_top = true;
+    } else {
+        return;
+    }
+    const double ck = o.rc * o.kappa;
+#pragma unroll
+    for (int r = 0; r < L; ++r) {
+        if (r >= m.cnt) continue;
+        const int k = m.k0 + r;
+        const size_t e = (size_t)o.op * nz + k;
+        const int s = A.zst[e];
+        const double *w = A.zwt + e * kZW;
+        if (s == 0) d[r].x += ck * w[0] * bot;
+        const int jt = nz + 1 - s;
+        if (has_top && jt < 7) d[r].x += ck * w[jt] * top;
+    }
+}
+
+// Pressure solve of a column in place (rhs in phi on entry).
+template <int L>
+HC_DEV void z6_pressure(const StepArgs &A, const ColMap &m, double2 *buf, int l, int jm, bool is00, double2 (&phi)[L])
+{
+    double a[L], b[L], c[L];
+    Z6Op o = z6_op(A, 2, l, jm);
+    assemble<L>(A, 2, l, jm, is00, m, a, b, c);
+    if (is00) {
+        fixups<L>(A, 2, m, phi);
+        o.delnull = true;
+        o.b0 = b[0];
+    }
+    z6_solve<L>(A, o, m, buf, a, b, c, phi);
+}
+
+// Divergence of (u, v, w) times `scale`: horizontal symbols, ZO_D in z.
+template <int L>
+HC_DEV void z6_divergence(const StepArgs &A, const ColMap &m, double2 *buf, int l, int jm, const double2 (&u)[L],
+                          const double2 (&v)[L], const double2 (&w)[L], double scale, double2 (&out)[L])
+{
+    const int nz = A.nz, nl = A.nl, ny = A.ny;
+    z6_apply<L, 6>(A, ZO_D, 1, nz - 1, nz, m, buf, w, out);
+    const double2 sdx = make_double2(A.symx[SY_DRE * nl + l], A.symx[SY_DIM * nl + l]);
+    const double2 sdy = make_double2(A.symy[SY_DRE * ny + jm], A.symy[SY_DIM * ny + jm]);
+#pragma unroll
+    for (int r = 0; r < L; ++r) {
+        const double2 dv = cadd(cadd(cmul(sdx, u[r]), cmul(sdy, v[r])), out[r]);
+        out[r] = r < m.cnt ? make_double2(dv.x * scale, dv.y * scale) : make_double2(0.0, 0.0);
+    }
+}
+
+// u, v, w -= cdt grad phi: horizontal symbols, ZO_G at the interior faces.
+template <int L>
+HC_DEV void z6_project(const StepArgs &A, const ColMap &m, double2 *buf, int l, int jm, double cdt,
+                       const double2 (&phi)[L], double2 (&u)[L], double2 (&v)[L], double2 (&w)[L])
+{
+    const int nz = A.nz, nl = A.nl, ny = A.ny;
+    double2 gz[L];
+    z6_apply<L, 6>(A, ZO_G, 0, nz, nz - 1, m, buf, phi, gz);
+    const double2 sgx = make_double2(A.symx[SY_GRE * nl + l], A.symx[SY_GIM * nl + l]);
+    const double2 sgy = make_double2(A.symy[SY_GRE * ny + jm], A.symy[SY_GIM * ny + jm]);
+#pragma unroll
+    for (int r = 0; r < L; ++r) {
+        if (r >= m.cnt) continue;
+        u[r] = csub(u[r], cscale(cmul(sgx, phi[r]), cdt));
+        v[r] = csub(v[r], cscale(cmul(sgy, phi[r]), cdt));
+        if (m.k0 + r < nz - 1) { w[r].x -= cdt * gz[r].x; w[r].y -= cdt * gz[r].y; }
+    }
+}
+
+// FD substep column with the 6th-order z operators: the steps of
+// k_column_mom (u, v, w Helmholtz solves, divergence, pressure Poisson,
+// projection, pressure increment) with the z operators of the ZOp tables.
+template <int L>
+__global__ __launch_bounds__(64 * kColWaves) void k_column_mom6(StepArgs A, ColIO<double> io)
+{
+    extern __shared__ double2 tile_raw[];
+    double2 *tile = tile_raw;
+    const ColPos pos = col_pos(io, A.ny);
+    if (pos.skip) return;
+    const int wave = threadIdx.x >> 6;
+    const int l = pos.bx * kColWaves + wave, jm = pos.jm;
+    const ColMap m = col_map(A.nz);
+    const int nz = A.nz, nl = A.nl, ny = A.ny;
+    const bool is00 = (l == 0 && jm == 0);
+    const size_t plane = (size_t)ny * nl, col0 = (size_t)jm * nl + (size_t)pos.bx * kColWaves;
+    double2 duv[2][L], dw[L], dp[L], t[L];
+    col_load<L>(io.s0, tile, nz, plane, col0, m, duv[0]);
+    col_load<L>(io.s1, tile, nz, plane, col0, m, duv[1]);
+    col_load<L>(io.s2, tile, nz, plane, col0, m, dw);
+    __syncthreads();
+    double2 *buf = tile + (size_t)wave * nz;  // this wavefront's column of the tile
+    double a[L], b[L], c[L];
+    // u and v: every mode takes the (0,0) rows in the preconditioner
+    Z6Op o = z6_op(A, 0, l, jm);
+    assemble<L>(A, 0, l, jm, true, m, a, b, c);
+    if (is00) z6_fixups<L>(A, o, m, duv[0]);
+    z6_solve<L>(A, o, m, buf, a, b, c, duv[0]);
+    z6_solve<L>(A, o, m, buf, a, b, c, duv[1]);
+    // w; the top-wall row is trivial
+#pragma unroll
+    for (int r = 0; r < L; ++r)
+        if (r < m.cnt && m.k0 + r == nz - 1) dw[r] = make_double2(0.0, 0.0);
+    o = z6_op(A, 1, l, jm);
+    assemble<L>(A, 1, l, jm, true, m, a, b, c);
+    z6_solve<L>(A, o, m, buf, a, b, c, dw);
+    // divergence / dt (get_p_eqn_src) with ZO_D over the interior faces
+    z6_divergence<L>(A, m, buf, l, jm, duv[0], duv[1], dw, A.idt, dp);
+    // pressure: rc (hxy + D G) phi = div / dt
+    z6_pressure<L>(A, m, buf, l, jm, is00, dp);
+    // projection (calculate_uvw) with ZO_G for w
+    z6_project<L>(A, m, buf, l, jm, A.rc * A.dt, dp, duv[0], duv[1], dw);
+    // pressure increment phi - rc dt nu / 2 (hxy_L + D G) phi
+    z6_apply<L, kZW>(A, ZO_P, 0, nz, nz, m, buf, dp, t);
+    const double slap = A.symx[SY_L * nl + l] + A.symy[SY_L * ny + jm];
+    const double cnu = A.rc * 0.5 * A.dt * A.nu;
+#pragma unroll
+    for (int r = 0; r < L; ++r) {
+        t[r] = make_double2(dp[r].x - cnu * (slap * dp[r].x + t[r].x), dp[r].y - cnu * (slap * dp[r].y + t[r].y));
+        if (r >= m.cnt) t[r] = make_double2(0.0, 0.0);
+    }
+    col_store<L>(io.s0, tile, nz, plane, col0, m, duv[0]);
+    col_store<L>(io.s1, tile, nz, plane, col0, m, duv[1]);
+    col_store<L>(io.s2, tile, nz, plane, col0, m, dw);
+    col_store<L>(io.s3, tile, nz, plane, col0, m, t);
+}
+
+// Temperature Helmholtz solve with the 6th-order z operator, in place on s0.
+template <int L>
+__global__ __launch_bounds__(64 * kColWaves) void k_column_t6(StepArgs A, ColIO<double> io)
+{
+    extern __shared__ double2 tile_raw[];
+    double2 *tile = tile_raw;
+    const ColPos pos = col_pos(io, A.ny);
+    if (pos.skip) return;
+    const int wave = threadIdx.x >> 6;
+    const int l = pos.bx * kColWaves + wave, jm = pos.jm;
+    const ColMap m = col_map(A.nz);
+    const bool is00 = (l == 0 && jm == 0);
+    const size_t plane = (size_t)A.ny * A.nl, col0 = (size_t)jm * A.nl + (size_t)pos.bx * kColWaves;
+    double2 d[L];
+    col_load<L>(io.s0, tile, A.nz, plane, col0, m, d);
+    __syncthreads();
+    double2 *buf = tile + (size_t)wave * A.nz;
+    double a[L], b[L], c[L];
+    const Z6Op o = z6_op(A, 3, l, jm);
+    assemble<L>(A, 3, l, jm, true, m, a, b, c);
+    if (is00) z6_fixups<L>(A, o, m, d);
+    z6_solve<L>(A, o, m, buf, a, b, c, d);
+    col_store<L>(io.s0, tile, A.nz, plane, col0, m, d);
+}
+
+// Projection onto divergence-free fields (rk_c = 1) with the 6th-order
+// operators: phi = (D G)^-1 div u, u -= G phi.
+template <int L>
+__global__ __launch_bounds__(64 * kColWaves) void k_column_project6(StepArgs A, ColIO<double> io)
+{
+    extern __shared__ double2 tile_raw[];
+    double2 *tile = tile_raw;
+    const ColPos pos = col_pos(io, A.ny);
+    if (pos.skip) return;
+    const int wave = threadIdx.x >> 6;
+    const int l = pos.bx * kColWaves + wave, jm = pos.jm;
+    const ColMap m = col_map(A.nz);
+    const bool is00 = (l == 0 && jm == 0);
+    const size_t plane = (size_t)A.ny * A.nl, col0 = (size_t)jm * A.nl + (size_t)pos.bx * kColWaves;
+    double2 u[L], v[L], w[L], ph[L];
+    col_load<L>(io.s0, tile, A.nz, plane, col0, m, u);
+    col_load<L>(io.s1, tile, A.nz, plane, col0, m, v);
+    col_load<L>(io.s2, tile, A.nz, plane, col0, m, w);
+#pragma unroll
+    for (int r = 0; r < L; ++r)
+        if (r < m.cnt && m.k0 + r == A.nz - 1) w[r] = make_double2(0.0, 0.0);
+    double2 *buf = tile + (size_t)wave * A.nz;
+    z6_divergence<L>(A, m, buf, l, jm, u, v, w, 1.0, ph);
+    z6_pressure<L>(A, m, buf, l, jm, is00, ph);
+    z6_project<L>(A, m, buf, l, jm, 1.0, ph, u, v, w);
+    col_store<L>(io.s0, tile, A.nz, plane, col0, m, u);
+    col_store<L>(io.s1, tile, A.nz, plane, col0, m, v);
+    col_store<L>(io.s2, tile, A.nz, plane, col0, m, w);
+}
+
+// Standalone solve of one equation with the 6th-order z operators, in place
+// on s0.  eq: 0 u (wall value -u_mrf), 1 v, 2 w, 3 p, 4 t (tbot, ttop or
+// the top gradient), as k_column_solve.
+template <int L>
+__global__ __launch_bounds__(64 * kColWaves) void k_column_solve6(StepArgs A, ColIO<double> io, int eq)
+{
+    extern __shared__ double2 tile_raw[];
+    double2 *tile = tile_raw;
+    const ColPos pos = col_pos(io, A.ny);
+    if (pos.skip) return;
+    const int wave = threadIdx.x >> 6;
+    const int l = pos.bx * kColWaves + wave, jm = pos.jm;
+    const ColMap m = col_map(A.nz);
+    const bool is00 = (l == 0 && jm == 0);
+    const size_t plane = (size_t)A.ny * A.nl, col0 = (size_t)jm * A.nl + (size_t)pos.bx * kColWaves;
+    double2 d[L];
+    col_load<L>(io.s0, tile, A.nz, plane, col0, m, d);
+#pragma unroll
+    for (int r = 0; r < L; ++r)
+        if (eq == 2 && r < m.cnt && m.k0 + r == A.nz - 1) d[r] = make_double2(0.0, 0.0);
+    double2 *buf = tile + (size_t)wave * A.nz;
+    const int meq = eq == 0 || eq == 1 ? 0 : (eq == 2 ? 1 : (eq == 3 ? 2 : 3));
+    if (meq == 2) {
+        z6_pressure<L>(A, m, buf, l, jm, is00, d);
+    } else {
+        double a[L], b[L], c[L];
+        const Z6Op o = z6_op(A, meq, l, jm);
+        assemble<L>(A, meq, l, jm, true, m, a, b, c);
+        if (is00 && (eq == 0 || eq == 4)) z6_fixups<L>(A, o, m, d);
+        z6_solve<L>(A, o, m, buf, a, b, c, d);
+    }
+    col_store<L>(io.s0, tile, A.nz, plane, col0, m, d);
+}

@@ -69,7 +69,7 @@ program hercules_hip
     prm%nu = nu; prm%alpha = alpha; prm%got = got; prm%fc = fc; prm%ug = ug; prm%vg = vg
     prm%dt = dt; prm%u_mrf = u_mrf; prm%t_ref = t_ref; prm%tbot = tbot; prm%ttop = ttop
     prm%ri_str = ri_str; prm%ri_end = ri_end; prm%wall_rows = 0; prm%precision = 0
-    prm%tbc_top = 0; prm%dtdz_top = 0.0d0
+    prm%tbc_top = 0; prm%dtdz_top = 0.0d0; prm%zorder = 2
     call hip_create(prm)
 
     allocate(f(nx + 4, ny + 4, nz + 4), g(nx, ny, nz))

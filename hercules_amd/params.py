@@ -21,7 +21,7 @@ class CParams(C.Structure):
     """ctypes image of ``hc_params`` (include/hercules_amd.h)."""
     _fields_ = ([(n, C.c_int) for n in INT_FIELDS] + [(n, C.c_double) for n in DBL_FIELDS] +
                 [("wall_rows", C.c_int), ("precision", C.c_int), ("tbc_top", C.c_int),
-                 ("dtdz_top", C.c_double)])
+                 ("dtdz_top", C.c_double), ("zorder", C.c_int)])
 
 
 @dataclass
@@ -87,6 +87,8 @@ class Params:
     # (the reference), 1 dT/dz = dtdz_top (hc_params.tbc_top, dtdz_top)
     tbc_top: int = 0
     dtdz_top: float = 0.0
+    # order of the z operators: 2 (the reference) or 6 (hc_params.zorder)
+    zorder: int = 2
 
     def validate(self):
         """The checks of initiate_parameters (src/module_parameters.f90:256-270)."""
@@ -108,6 +110,7 @@ class Params:
         p.precision = int(self.precision)
         p.tbc_top = int(self.tbc_top)
         p.dtdz_top = float(self.dtdz_top)
+        p.zorder = int(self.zorder)
         return p
 
     def as_dict(self):

@@ -68,6 +68,19 @@ typedef struct hc_params {
      * dz_t(kend3)); ttop is then unused.  The bottom stays t = tbot. */
     int tbc_top;
     double dtdz_top;
+    /* Order of the wall-normal (z) operators (no namelist counterpart; the
+     * reference is 2nd order in z, src/module_cnvdiff.f90:89,121).  0 or 2:
+     * as the reference.  6: every z derivative, z interpolation and implicit
+     * z operator of the FD step (cds = 1, 2) from 7-point (second
+     * derivatives) and 6-point (first derivatives, interpolation) Lagrange
+     * stencils on the stretched grid, one-sided near the walls with the wall
+     * values as data; the pressure equation is the product of the discrete
+     * z divergence and gradient, so the projected velocity is divergence-free
+     * to round-off; the implicit solves are exact (Chebyshev iteration
+     * preconditioned by the 2nd-order tridiagonal operator).  zorder = 6
+     * needs precision = HC_FP64, isdamp = 0, cds = 1 or 2 and nz >= 16
+     * (HC_ERR_UNSUPPORTED otherwise). */
+    int zorder;
 } hc_params;
 
 enum hc_precision { HC_FP64 = 0, HC_MIXED = 1 };

@@ -1,0 +1,130 @@
+"""6th-order z operators (hc_params.zorder = 6) on the GPU, through the C
+ABI, against the FP64 torch-ops restatement of the step with the dense
+operators of tests/zops6.py (tests/torch_reference.py; those operators are
+pinned on the CPU by tests/test_zorder6.py).  The reference is 2nd order in
+z, so there is no oracle for this extension; needs an MI355X."""
+import numpy as np
+import pytest
+
+torch = pytest.importorskip("torch")
+
+from hercules_amd import Channel, cases  # noqa: E402
+from hercules_amd.channel import HerculesError  # noqa: E402
+from tests.helpers import rel_l2  # noqa: E402
+from tests.torch_reference import TorchChannel  # noqa: E402
+
+pytestmark = pytest.mark.gpu
+
+
+def _case(kind, nz, **kw):
+    if kind == "closed":
+        prm = cases.closed_channel(nx=16, ny=16, nz=nz, wall_rows=1, zorder=6, **kw)
+        st = cases.laminar_noise(prm, seed=5)
+        st.pop("t", None)
+    else:
+        prm = cases.stratified_open_channel(nx=16, ny=16, nz=nz, wall_rows=1, zorder=6, nu=1.0 / 180.0, **kw)
+        st = cases.loglaw_noise(prm, seed=5)
+        if not prm.isscalar:
+            st.pop("t", None)
+    return prm, st
+
+
+CASES = [
+    ("closed", 33, {}),
+    ("closed", 65, {"cds": 1}),
+    ("closed", 129, {}),
+    ("open", 33, {"isscalar": 0, "got": 0.0}),
+    ("open", 65, {}),
+    ("open", 33, {"tbc_top": 1, "dtdz_top": 0.5}),
+    ("open", 129, {"cds": 1, "tbc_top": 1, "dtdz_top": 1.0}),
+]
+
+
+@pytest.mark.parametrize("kind,nz,kw", CASES)
+def test_z6_steps_match_torch_reference(kind, nz, kw):
+    prm, st = _case(kind, nz, **kw)
+    tc = TorchChannel(prm)
+    ref = tc.steps({k: torch.tensor(v) for k, v in st.items()}, 2)
+    ch = Channel(prm)
+    for k, v in st.items():
+        ch.set_field(k, v)
+    ch.init()
+    ch.steps(2)
+    names = ["u", "v", "w", "p"] + (["t"] if prm.isscalar else [])
+    # p: its (0,0) mode solves the Neumann pressure operator with the delnull
+    # row, whose condition number grows as 1/dz_min^2 (1.4e6 at nz = 129):
+    # two exact solvers agree to ~1e-16 times that, relative to p
+    tol = {"p": 1e-9}
+    for n in names:
+        err = rel_l2(ch.get_field(n), ref[n].numpy())
+        print(kind, nz, kw, n, f"{err:.2e}")
+        assert err < tol.get(n, 1e-12), (n, err)
+    # divergence with the 6th-order z divergence: round-off of the projection
+    _, dmax = ch.screen_div_error()
+    scale = np.abs(ref["u"].numpy()).max() / float(tc.dz.min())
+    print("   max |div|", dmax, "scale", scale)
+    assert dmax < 1e-12 * scale
+    ch.close()
+
+
+@pytest.mark.parametrize("eq", ["u", "v", "w", "p", "t"])
+@pytest.mark.parametrize("nz", [33, 257])
+def test_z6_standalone_solve_matches_dense(eq, nz):
+    """poisson_solver_fft with the 6th-order operators against the dense
+    per-mode solve: (I - beta (lap_xy + Z)) x = q (+ wall data of u, t in the
+    (0,0) mode), rc (lap_xy + D G) phi = s with phi(0) = 0 in the (0,0) mode."""
+    prm = cases.stratified_open_channel(nx=16, ny=32, nz=nz, wall_rows=1, zorder=6, nu=1.0 / 180.0,
+                                        tbc_top=1 if eq == "t" else 0, dtdz_top=0.7)
+    rng = np.random.default_rng(9)
+    src = rng.standard_normal((prm.nz, prm.ny, prm.nx))
+    rk_c = 2.0 / 15.0
+    ch = Channel(prm)
+    got = ch.poisson_solver_fft(eq, rk_c, src)
+    ch.close()
+    tc = TorchChannel(prm)
+    s = torch.tensor(src)
+    if eq == "p":
+        ref = tc.poisson(s / rk_c)
+    elif eq == "t":
+        beta = rk_c * 0.5 * prm.dt * prm.alpha
+        ref = tc.helm(s + beta * tc.col(tc.bt), tc.Lt, beta)
+    else:
+        beta = rk_c * 0.5 * prm.dt * prm.nu
+        ref = tc.helm(s, tc.Lw if eq == "w" else tc.Lu, beta, trivial_top=eq == "w")
+    err = rel_l2(got, ref.numpy())
+    print(eq, nz, f"{err:.2e}")
+    # p: the conditioning of the (0,0)-mode pressure operator (6e6 at nz = 257)
+    assert err < (1e-9 if eq == "p" else 1e-11), err
+
+
+def test_z6_projection_is_divergence_free():
+    prm, st = _case("closed", 129)
+    ch = Channel(prm)
+    for k, v in st.items():
+        ch.set_field(k, v)
+    before = ch.screen_div_error()[1]
+    ch.project()
+    after = ch.screen_div_error()[1]
+    u = ch.get_field("u")
+    scale = np.abs(u).max() / (prm.lz / prm.nz)
+    print("div before", before, "after", after)
+    assert before > 1e-6 * scale and after < 1e-12 * scale
+    ch.close()
+
+
+@pytest.mark.parametrize("kw", [dict(precision=1), dict(isdamp=1, nzdamp=8, cdamp=0.1),
+                                dict(cds=3, issk=1), dict(nz=9), dict(zorder=4)])
+def test_z6_rejects_unsupported(kw):
+    base = dict(nx=16, ny=16, nz=33, zorder=6)
+    base.update(kw)
+    prm = cases.closed_channel(**base)
+    with pytest.raises(HerculesError):
+        Channel(prm)
+
+
+def test_z6_adjoint_rejected():
+    prm, st = _case("closed", 33)
+    ch = Channel(prm)
+    with pytest.raises(HerculesError, match="status 2"):
+        ch.wall_shear_gradient(2, 1)
+    ch.close()

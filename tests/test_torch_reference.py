@@ -34,3 +34,17 @@ def test_torch_step_matches_oracle(kind, kw):
         err = rel_l2(out[n].numpy(), o.get(n))
         print(kind, kw, n, f"{err:.2e}")
         assert err < 1e-11, (n, err)
+
+
+@pytest.mark.parametrize("kw", [{}, {"tbc_top": 1, "dtdz_top": 0.5}, {"cds": 1, "ochannel": 0}])
+def test_torch_step_with_temperature_matches_oracle(kw):
+    prm = cases.stratified_open_channel(nx=16, ny=16, nz=17, wall_rows=1, nu=1.0 / 180.0, **kw)
+    st = cases.loglaw_noise(prm, seed=6)
+    o = make_oracle(prm, st)
+    o.steps(2)
+    tc = TorchChannel(prm)
+    out = tc.steps({k: torch.tensor(v) for k, v in st.items()}, 2)
+    for n in ("u", "v", "w", "p", "t"):
+        err = rel_l2(out[n].numpy(), o.get(n))
+        print(kw, n, f"{err:.2e}")
+        assert err < 1e-11, (n, err)

@@ -8,8 +8,13 @@ src/module_poisson_solver.f90 and src/module_navier_stokes.f90:247-355,
 523-808 for the solves, projection and pressure update) in their physical
 form: explicit stencils with torch.roll and wall ghosts, implicit z operators
 as dense per-mode matrices after a 2-D FFT.  Scope: cds = 1 or 2, no
-temperature, no damping, dp_opt = 1, u_mrf = 0, wall_rows = 1 (every mode
-gets the wall rows), RK3.
+damping, dp_opt = 1, u_mrf = 0, wall_rows = 1 (every mode gets the wall
+rows), RK3; temperature (get_temperature_cnvdiff, src/module_cnvdiff.f90:
+381-473, with the Dirichlet or constant-gradient top) without the Ri ramp.
+
+With prm.zorder = 6 every z derivative, z interpolation and implicit z
+operator is the dense 6th-order matrix of tests/zops6.py (the library's
+zorder = 6 extension; the reference is 2nd order in z).
 """
 from __future__ import annotations
 
@@ -19,19 +24,23 @@ import numpy as np
 import torch
 
 from hercules_amd import grid
+from tests.zops6 import zops
 
 RK3 = [(8.0 / 15.0, 0.0, 8.0 / 15.0), (5.0 / 12.0, -17.0 / 60.0, 2.0 / 15.0), (3.0 / 4.0, -5.0 / 12.0, 1.0 / 3.0)]
 
 
 class TorchChannel:
     def __init__(self, prm):
-        assert prm.cds in (1, 2) and prm.isscalar == 0 and prm.isdamp == 0 and prm.dp_opt == 1
+        assert prm.cds in (1, 2) and prm.isdamp == 0 and prm.dp_opt == 1 and prm.is_ri_var == 0
         assert prm.u_mrf == 0.0 and prm.wall_rows == 1 and prm.dts == 2
         self.p = prm
         m = grid.metrics(prm.nz, prm.lz, prm.zstretch, prm.ochannel)
         t = lambda a: torch.tensor(a, dtype=torch.float64)
         self.dz, self.dzt, self.dzb, self.lt = t(m["dz"]), t(m["dz_t"]), t(m["dz_b"]), t(m["l_t"])
         self.dx, self.dy = prm.lx / prm.nx, prm.ly / prm.ny
+        self.z6 = getattr(prm, "zorder", 2) == 6
+        if self.z6:
+            self.Z = {k: t(v) for k, v in zops(prm.nz, prm.lz, prm.zstretch, prm.ochannel).items()}
         self._symbols()
 
     # ---- horizontal stencils (axis 2 = x, axis 1 = y) ---------------------
@@ -137,8 +146,36 @@ class TorchChannel:
             if k < nz - 2:
                 Lw[k, k + 1] = 1 / (dz[k + 1] * dzt[k])
             Lw[k, k] = -(1 / (dz[k + 1] * dzt[k]) + 1 / (dz[k] * dzt[k]))
+        # t: Dirichlet bottom (ghost 2 tbot - t), top Dirichlet or constant gradient
+        Lt = L.copy()
+        Lt[0, 0] -= 2 / (dz[0] * dzb[0])
+        bt = np.zeros(nz)
+        bt[0] = 2 * p.tbot / (dz[0] * dzb[0])
+        if p.tbc_top == 0:
+            Lt[-1, -1] -= 2 / (dz[-1] * dzt[-1])
+            bt[-1] = 2 * p.ttop / (dz[-1] * dzt[-1])
+        else:
+            bt[-1] = p.dtdz_top * dzt[-1] / (dz[-1] * dzt[-1])
         t = lambda a: torch.tensor(a, dtype=torch.float64)
-        self.Lu, self.Lp, self.Lw = t(Lu), t(Lp), t(Lw)
+        self.Lu, self.Lp, self.Lw, self.Lt, self.bt = t(Lu), t(Lp), t(Lw), t(Lt), t(bt)
+        if self.z6:
+            Z = self.Z
+            c2u = Z["C2_DN"] if p.ochannel else Z["C2_DD"]
+            c2t = Z["C2_DN"] if p.tbc_top else Z["C2_DD"]
+            self.Lu = c2u[:, 1:-1]
+            Lw6 = torch.zeros((nz, nz), dtype=torch.float64)
+            Lw6[:-1, :-1] = Z["F2"][:, 1:-1]
+            self.Lw = Lw6
+            self.Lp = Z["P"]
+            self.Lt = c2t[:, 1:-1]
+            self.bt = c2t[:, 0] * p.tbot + c2t[:, -1] * self.t_top_datum()
+
+    def t_top_datum(self):  # the top wall value, or the gradient
+        return self.p.dtdz_top if self.p.tbc_top else self.p.ttop
+
+    @staticmethod
+    def zmul(M, f):  # z operator on (levels, ny, nx)
+        return torch.tensordot(M, f, dims=([1], [0]))
 
     def fft(self, f):
         return torch.fft.fft2(f) / (self.p.nx * self.p.ny) * self.keep
@@ -197,24 +234,59 @@ class TorchChannel:
         uhz = torch.cat([uhz[:-1], (zero if p.ochannel == 0 else u[-1])[None]], 0)
         vhz = torch.cat([vhz[:-1], (zero if p.ochannel == 0 else v[-1])[None]], 0)
         idz = col(1 / self.dz)
-        fuw = uhz * whx
-        fvw = vhz * why
-        f = (-self.d_s(uhx * uhx, 2, dx) - self.d_p(uhy * vhx, 1, dy)
-             - (fuw - self.dn(fuw, zero)) * idz + 1.0)  # dp_opt = 1: dpx_drive = 1
-        g = (-self.d_p(vhx * uhy, 2, dx) - self.d_s(vhy * vhy, 1, dy) - (fvw - self.dn(fvw, zero)) * idz)
-        # w equation on faces 0..nz-2
-        wp, wm = self.up(wz, zero), self.dn(wz, zero)
         idzt = col(1 / self.dzt)
-        h = (-self.d_p(whx * uhz, 2, dx) - self.d_p(why * vhz, 1, dy)
-             - ((0.5 * (wz + wp)) ** 2 - (0.5 * (wz + wm)) ** 2) * idzt)
-        # explicit diffusion and pressure gradients
-        uzz = ((self.up(u, top_u) - u) / col(self.dzt) - (u - self.dn(u, -u[0])) / col(self.dzb)) * idz
-        vzz = ((self.up(v, top_v) - v) / col(self.dzt) - (v - self.dn(v, -v[0])) / col(self.dzb)) * idz
-        dzn = torch.cat([self.dz[1:], self.dz[-1:]])
-        wzz = ((wp - wz) / col(dzn) - (wz - wm) * idz) * idzt
+        tt = s.get("t")
+        if p.isscalar:  # buoyancy at the w faces from the old t
+            if self.z6:
+                tdat = torch.cat([torch.full_like(zero, p.tbot)[None], tt,
+                                  torch.full_like(zero, self.t_top_datum())[None]], 0)
+                ti = self.Z["I_DN"] if p.tbc_top else self.Z["I_DD"]
+                thz_i = self.zmul(ti, tdat)
+            else:
+                thz_i = ((1 - lt) * tt + lt * self.up(tt, tt[-1]))[:-1]
+            buoy = p.got * (thz_i - p.t_ref)
+        else:
+            buoy = 0.0
+        if self.z6:
+            Z = self.Z
+            c2u = Z["C2_DN"] if p.ochannel else Z["C2_DD"]
+            ciu = Z["I_DN"] if p.ochannel else Z["I_DD"]
+            udat = torch.cat([zero[None], u, zero[None]], 0)   # u_mrf = 0: wall value 0 (du/dz = 0 open top)
+            vdat = torch.cat([zero[None], v, zero[None]], 0)
+            wdat = torch.cat([zero[None], wz[:-1], zero[None]], 0)
+            uhz_i, vhz_i = self.zmul(ciu, udat), self.zmul(ciu, vdat)
+            whx_i, why_i = whx[:-1], why[:-1]
+            fuw = torch.cat([zero[None], uhz_i * whx_i, zero[None]], 0)
+            fvw = torch.cat([zero[None], vhz_i * why_i, zero[None]], 0)
+            f = -self.d_s(uhx * uhx, 2, dx) - self.d_p(uhy * vhx, 1, dy) - self.zmul(Z["D"], fuw) + 1.0
+            g = -self.d_p(vhx * uhy, 2, dx) - self.d_s(vhy * vhy, 1, dy) - self.zmul(Z["D"], fvw)
+            whz = self.zmul(Z["IFC"], wdat)
+            hi = (-self.d_p(whx_i * uhz_i, 2, dx) - self.d_p(why_i * vhz_i, 1, dy)
+                  - self.zmul(Z["G"], whz * whz) + buoy)
+            h = torch.cat([hi, zero[None]], 0)
+            uzz, vzz = self.zmul(c2u, udat), self.zmul(c2u, vdat)
+            wzz = torch.cat([self.zmul(Z["F2"], wdat), zero[None]], 0)
+            gzp = torch.cat([self.zmul(Z["G"], pr), zero[None]], 0)
+        else:
+            fuw = uhz * whx
+            fvw = vhz * why
+            f = (-self.d_s(uhx * uhx, 2, dx) - self.d_p(uhy * vhx, 1, dy)
+                 - (fuw - self.dn(fuw, zero)) * idz + 1.0)  # dp_opt = 1: dpx_drive = 1
+            g = (-self.d_p(vhx * uhy, 2, dx) - self.d_s(vhy * vhy, 1, dy) - (fvw - self.dn(fvw, zero)) * idz)
+            # w equation on faces 0..nz-2
+            wp, wm = self.up(wz, zero), self.dn(wz, zero)
+            h = (-self.d_p(whx * uhz, 2, dx) - self.d_p(why * vhz, 1, dy)
+                 - ((0.5 * (wz + wp)) ** 2 - (0.5 * (wz + wm)) ** 2) * idzt)
+            if p.isscalar:
+                h = h + torch.cat([buoy, zero[None]], 0)
+            # explicit diffusion and pressure gradients
+            uzz = ((self.up(u, top_u) - u) / col(self.dzt) - (u - self.dn(u, -u[0])) / col(self.dzb)) * idz
+            vzz = ((self.up(v, top_v) - v) / col(self.dzt) - (v - self.dn(v, -v[0])) / col(self.dzb)) * idz
+            dzn = torch.cat([self.dz[1:], self.dz[-1:]])
+            wzz = ((wp - wz) / col(dzn) - (wz - wm) * idz) * idzt
+            gzp = (self.up(pr, pr[-1]) - pr) * idzt
         gxp = self.d_s(pr, 2, dx)
         gyp = self.d_s(pr, 1, dy)
-        gzp = (self.up(pr, pr[-1]) - pr) * idzt
         f0, g0, h0 = s["f"], s["g"], s["h"]
         cn = rc * 0.5 * dt * nu
         qu = u + dt * (ra * f + rb * f0) + cn * (self.lap_h(u) + uzz) - rc * dt * gxp
@@ -226,16 +298,60 @@ class TorchChannel:
         vs = self.helm(qv, self.Lu, beta)
         ws = self.helm(qw, self.Lw, beta, trivial_top=True)
         # divergence (get_p_eqn_src), pressure Poisson, projection, new p
-        div = (self.d_p(us, 2, dx) + self.d_p(vs, 1, dy) + (ws - self.dn(ws, zero)) * idz) / dt
+        if self.z6:
+            dwz = self.zmul(self.Z["D"][:, 1:-1], ws[:-1])
+        else:
+            dwz = (ws - self.dn(ws, zero)) * idz
+        div = (self.d_p(us, 2, dx) + self.d_p(vs, 1, dy) + dwz) / dt
         phi = self.poisson(div / rc)
         cdt = rc * dt
         un = us - cdt * self.d_s(phi, 2, dx)
         vn = vs - cdt * self.d_s(phi, 1, dy)
-        gz = (self.up(phi, phi[-1]) - phi) * idzt
+        if self.z6:
+            gz = torch.cat([self.zmul(self.Z["G"], phi), zero[None]], 0)
+            lz = self.zmul(self.Z["P"], phi)
+        else:
+            gz = (self.up(phi, phi[-1]) - phi) * idzt
+            lz = ((self.up(phi, phi[-1]) - phi) / col(self.dzt) - (phi - self.dn(phi, phi[0])) / col(self.dzb)) * idz
         wn = torch.cat([(ws - cdt * gz)[:-1], zero[None]], 0)
-        lz = ((self.up(phi, phi[-1]) - phi) / col(self.dzt) - (phi - self.dn(phi, phi[0])) / col(self.dzb)) * idz
         pn = pr + phi - cn * (self.lap_h(phi) + lz)
-        return dict(u=un, v=vn, w=wn, p=pn, f=f, g=g, h=torch.cat([h[:-1], zero[None]], 0))
+        out = dict(u=un, v=vn, w=wn, p=pn, f=f, g=g, h=torch.cat([h[:-1], zero[None]], 0))
+        if p.isscalar:
+            out["t"], out["s"] = self.temperature(tt, s["s"], un, vn, wn, ra, rb, rc)
+        return out
+
+    def temperature(self, t, s0, u, v, w, ra, rb, rc):
+        """get_temperature_cnvdiff with the updated velocities, then the t
+        Helmholtz solve with the wall data in the (0,0) mode."""
+        p = self.p
+        dx, dy, dt = self.dx, self.dy, p.dt
+        col = self.col
+        zero = torch.zeros_like(t[0])
+        tux = self.d_p(self.ip(t, 2) * u, 2, dx)
+        tvy = self.d_p(self.ip(t, 1) * v, 1, dy)
+        if self.z6:
+            tdat = torch.cat([torch.full_like(zero, p.tbot)[None], t,
+                              torch.full_like(zero, self.t_top_datum())[None]], 0)
+            ti = self.Z["I_DN"] if p.tbc_top else self.Z["I_DD"]
+            c2t = self.Z["C2_DN"] if p.tbc_top else self.Z["C2_DD"]
+            ft = torch.cat([zero[None], self.zmul(ti, tdat) * w[:-1], zero[None]], 0)
+            twz = self.zmul(self.Z["D"], ft)
+            tzz = self.zmul(c2t, tdat)
+        else:
+            lt = col(self.lt)
+            gdz = p.dtdz_top * float(self.dzt[-1])
+            ttop_g = t[-1] + gdz if p.tbc_top else 2 * p.ttop - t[-1]
+            tbot_g = 2 * p.tbot - t[0]
+            thz = self.up(t, ttop_g) * lt + t * (1 - lt)
+            top_face = t[-1] + 0.5 * gdz if p.tbc_top else torch.full_like(zero, p.ttop)
+            thz = torch.cat([thz[:-1], top_face[None]], 0)
+            ft = thz * w
+            twz = (ft - self.dn(ft, zero)) / col(self.dz)
+            tzz = ((self.up(t, ttop_g) - t) / col(self.dzt) - (t - self.dn(t, tbot_g)) / col(self.dzb)) / col(self.dz)
+        snew = -tux - tvy - twz
+        ct = rc * 0.5 * dt * p.alpha
+        qt = t + dt * (ra * snew + rb * s0) + ct * (self.lap_h(t) + tzz) + ct * col(self.bt)
+        return self.helm(qt, self.Lt, ct), snew
 
     def steps(self, st, n):
         s = dict(st)
@@ -243,6 +359,8 @@ class TorchChannel:
         s.setdefault("f", z)
         s.setdefault("g", z)
         s.setdefault("h", z)
+        if self.p.isscalar:
+            s.setdefault("s", z)
         for _ in range(n):
             for t in (1, 2, 3):
                 s = self.substep(s, t)
