@@ -1547,6 +1547,9 @@ static bool build_zops(hc_ctx *c)
             ok = ok && row(ZO_G, k, nz, k + 0.5, 6, zf[k + 1], 1, false, ppos);
             ok = ok && row(ZO_I_DD, k, nz + 2, k + 1.5, 6, zf[k + 1], 0, false, cpos);
             ok = ok && row(ZO_I_DN, k, nz + 2, k + 1.5, 6, zf[k + 1], 0, true, cpos);
+        } else {  // the top wall face (row nz - 1 of the interpolations)
+            ok = ok && row(ZO_I_DD, k, nz + 2, k + 1.5, 6, zf[nz], 0, false, cpos);
+            ok = ok && row(ZO_I_DN, k, nz + 2, k + 1.5, 6, zf[nz], 0, true, cpos);
         }
     }
     if (!ok) return false;
@@ -2123,6 +2126,13 @@ template <int L, class RIO, class RC>
 static void spec_col_kernel(int kind, dim3 g, size_t lds, hipStream_t st, const StepArgs &A, const SpecIO<RIO> &io)
 {
     const dim3 b(64 * kColWaves);
+    if constexpr (std::is_same<RIO, double>::value && std::is_same<RC, double>::value) {
+        if (A.zorder == 6) {  // 6th-order z operators (FP64)
+            if (kind == 0) hipLaunchKernelGGL((k_column_spec6<L>), g, b, lds, st, A, io);
+            else hipLaunchKernelGGL((k_column_spec_t6<L>), g, b, lds, st, A, io);
+            return;
+        }
+    }
     if (kind == 0) hipLaunchKernelGGL((k_column_spec<L, RIO, RC>), g, b, lds, st, A, io);
     else hipLaunchKernelGGL((k_column_spec_t<L, RIO, RC>), g, b, lds, st, A, io);
 }
@@ -2188,7 +2198,13 @@ static int substep_sp_t(hc_ctx *c, int t_its, const View<R> &V)
         const int mul2[2] = {1, 2};
         if (int e = launch_yio<R>(c, +1, 2, src2, dst2, mul2)) return e;
     }
-#define HC_SM(NX_) hipLaunchKernelGGL((k_spec_mom<NX_, R>), grow, dim3(rows * NX_), 0, c->stream, B);
+#define HC_SM(NX_)                                                                                       \
+    if constexpr (std::is_same<R, double>::value) {                                                      \
+        if (A.zorder == 6) hipLaunchKernelGGL((k_spec_mom<NX_, R, 6>), grow, dim3(rows * NX_), 0, c->stream, B); \
+        else hipLaunchKernelGGL((k_spec_mom<NX_, R>), grow, dim3(rows * NX_), 0, c->stream, B);          \
+    } else {                                                                                             \
+        hipLaunchKernelGGL((k_spec_mom<NX_, R>), grow, dim3(rows * NX_), 0, c->stream, B);               \
+    }
     HC_DISPATCH_N(c->nx, HC_SM)
 #undef HC_SM
     PROF_END(c, HC_K_RHS);
@@ -2208,7 +2224,13 @@ static int substep_sp_t(hc_ctx *c, int t_its, const View<R> &V)
     PROF_END(c, HC_K_FINISH);
     if (P.isscalar == 1) {
         PROF_BEGIN(c, HC_K_RHS_T);
-#define HC_ST(NX_) hipLaunchKernelGGL((k_spec_t<NX_, R>), grow, dim3(rows * NX_), 0, c->stream, B);
+#define HC_ST(NX_)                                                                                     \
+    if constexpr (std::is_same<R, double>::value) {                                                    \
+        if (A.zorder == 6) hipLaunchKernelGGL((k_spec_t<NX_, R, 6>), grow, dim3(rows * NX_), 0, c->stream, B); \
+        else hipLaunchKernelGGL((k_spec_t<NX_, R>), grow, dim3(rows * NX_), 0, c->stream, B);          \
+    } else {                                                                                           \
+        hipLaunchKernelGGL((k_spec_t<NX_, R>), grow, dim3(rows * NX_), 0, c->stream, B);               \
+    }
         HC_DISPATCH_N(c->nx, HC_ST)
 #undef HC_ST
         PROF_END(c, HC_K_RHS_T);
@@ -2275,7 +2297,6 @@ extern "C" int hc_create(const hc_params *prm, hc_ctx **out)
     if (P.zorder == 6) {
         if (P.precision != HC_FP64) return set_err(HC_ERR_UNSUPPORTED, "zorder=6 needs precision=0 (FP64)");
         if (P.isdamp == 1) return set_err(HC_ERR_UNSUPPORTED, "zorder=6 does not cover Rayleigh damping");
-        if (P.cds == 3) return set_err(HC_ERR_UNSUPPORTED, "zorder=6 covers the FD path (cds=1, 2)");
         if (P.nz < 16) return set_err(HC_ERR_UNSUPPORTED, "zorder=6 needs nz >= 16");
     }
     if (!fft_len_ok(P.nx) || !fft_len_ok(P.ny)) return set_err(HC_ERR_UNSUPPORTED, "nx, ny must be one of 16,32,64,128,192,256,512");
@@ -2311,7 +2332,9 @@ extern "C" int hc_create(const hc_params *prm, hc_ctx **out)
     HC_ATTR1((k_column_mom6<L_>))                    \
     HC_ATTR1((k_column_t6<L_>))                      \
     HC_ATTR1((k_column_project6<L_>))                \
-    HC_ATTR1((k_column_solve6<L_>))
+    HC_ATTR1((k_column_solve6<L_>))                  \
+    HC_ATTR1((k_column_spec6<L_>))                   \
+    HC_ATTR1((k_column_spec_t6<L_>))
         HC_DISPATCH_L(L, HC_ATTR)
 #undef HC_ATTR
 #undef HC_ATTR1

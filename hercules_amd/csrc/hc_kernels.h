@@ -52,8 +52,8 @@ enum ZOp {
     ZO_F2,         // d2/dz2 at interior faces from F data (7 points)
     ZO_G,          // d/dz at interior faces from P data (6 points)
     ZO_D,          // d/dz at centres from F data (6 points)
-    ZO_I_DD,       // centre-to-face interpolation from C data (6 points)
-    ZO_I_DN,       // the same with a Neumann top
+    ZO_I_DD,       // centre-to-face interpolation from C data (6 points); row nz-1: the top wall face
+    ZO_I_DN,       // the same with a Neumann top (row nz-1 extrapolates to the wall)
     ZO_IFC,        // face-to-centre interpolation from F data (6 points)
     ZO_P,          // ZO_D o ZO_G: the pressure operator on P data (up to kZW points)
     ZO_COUNT
@@ -265,7 +265,7 @@ struct Interp {
         const D &q = d();
         using Re = typename D::Real;
         const int kw = q.k + dk;
-        if (kw < 0 || kw >= q.A.nz - 1) return uhz(di, dj, dk);  // wall faces: the wall rule
+        if (kw < 0 || kw >= q.A.nz - 1) return q.uhz(di, dj, dk);  // wall faces: the wall rule
         int s;
         const double *w = zw(q.A.ochannel ? ZO_I_DN : ZO_I_DD, kw, s);
         Re acc = Re(0.0);
@@ -278,7 +278,7 @@ struct Interp {
         const D &q = d();
         using Re = typename D::Real;
         const int kw = q.k + dk;
-        if (kw < 0 || kw >= q.A.nz - 1) return vhz(di, dj, dk);
+        if (kw < 0 || kw >= q.A.nz - 1) return q.vhz(di, dj, dk);
         int s;
         const double *w = zw(q.A.ochannel ? ZO_I_DN : ZO_I_DD, kw, s);
         Re acc = Re(0.0);
@@ -291,7 +291,7 @@ struct Interp {
         const D &q = d();
         using Re = typename D::Real;
         const int kw = q.k + dk;
-        if (kw < 0 || kw >= q.A.nz - 1) return thz(dk);
+        if (kw < 0 || kw >= q.A.nz - 1) return q.thz(dk);
         int s;
         const double *w = zw(q.A.tbc_top ? ZO_I_DN : ZO_I_DD, kw, s);
         Re acc = Re(0.0);
@@ -299,15 +299,42 @@ struct Interp {
         for (int j = 0; j < 6; ++j) acc += Re(w[j]) * tdat(s + j);
         return acc;
     }
+    // u, v at the face of w level kw = -1..nz-1 (both walls included), for
+    // the advective z terms of the spectral path
+    HC_DEV auto uf6(int kw) const
+    {
+        const D &q = d();
+        using Re = typename D::Real;
+        if (kw < 0) return Re(-q.A.u_mrf);
+        int s;
+        const double *w = zw(q.A.ochannel ? ZO_I_DN : ZO_I_DD, kw, s);
+        Re acc = Re(0.0);
+#pragma unroll
+        for (int j = 0; j < 6; ++j) acc += Re(w[j]) * udat(0, 0, s + j);
+        return acc;
+    }
+    HC_DEV auto vf6(int kw) const
+    {
+        const D &q = d();
+        using Re = typename D::Real;
+        if (kw < 0) return Re(0.0);
+        int s;
+        const double *w = zw(q.A.ochannel ? ZO_I_DN : ZO_I_DD, kw, s);
+        Re acc = Re(0.0);
+#pragma unroll
+        for (int j = 0; j < 6; ++j) acc += Re(w[j]) * vdat(0, 0, s + j);
+        return acc;
+    }
     template <int ZO> HC_DEV auto uhzO(int di, int dj, int dk) const
     {
-        if constexpr (ZO == 6) return uhz6(di, dj, dk);
-        else return uhz(di, dj, dk);
+        // through the accessor, which may override the interpolant (PtD)
+        if constexpr (ZO == 6) return d().uhz6(di, dj, dk);
+        else return d().uhz(di, dj, dk);
     }
     template <int ZO> HC_DEV auto vhzO(int di, int dj, int dk) const
     {
-        if constexpr (ZO == 6) return vhz6(di, dj, dk);
-        else return vhz(di, dj, dk);
+        if constexpr (ZO == 6) return d().vhz6(di, dj, dk);
+        else return d().vhz(di, dj, dk);
     }
     // w at centre level c (absolute), 6 points of face data
     HC_DEV auto whz6(int c) const

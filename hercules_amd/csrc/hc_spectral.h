@@ -83,7 +83,7 @@ __global__ __launch_bounds__(256) void k_yfft_io(YIOArgsT<R> A)
 //   R_u = tmp2/coe - duwdz - [issk](u dudx + v dudy + w dudz)   (:98-197)
 //   R_v = tmp2/coe - dvwdz - [issk](u dvdx + v dvdy + w dvdz)   (:279-339)
 //   R_w = buoy/coe - dwwdz - [issk](u dwdx + v dwdy + w dwdz)   (:426-496)
-template <int NX, class R>
+template <int NX, class R, int ZO = 2>
 __global__ __launch_bounds__(1024) void k_spec_mom(StepArgsT<R> A)
 {
     using C = cplx<R>;
@@ -114,7 +114,58 @@ __global__ __launch_bounds__(1024) void k_spec_mom(StepArgsT<R> A)
     R out[8];
 #pragma unroll
     for (int f = 0; f < 8; ++f) out[f] = R(0.0);
-    if (in) {
+    if (in && ZO == 6) {
+        // 6th-order z (zorder = 6): the same products and terms with the
+        // face values, z derivatives and z interpolants of the ZOp tables
+        PtT<R> g(A, i, j, k);
+        const R u = g.U(0, 0, 0), v = g.V(0, 0, 0), w = g.W(0, 0, 0);
+        const R coe = A.coe;
+        const R dpx = A.dp_opt == 2 ? R(A.scal[0]) : A.dpx;
+        const R a1 = g.uf6(k), a1v = g.vf6(k);  // u, v at this level's w face
+        out[0] = u * u;
+        out[1] = u * v;
+        out[2] = v * v;
+        out[5] = a1 * w;
+        out[6] = a1v * w;
+        const R tmp2u = A.dp_opt == 3 ? A.fc * (v - A.vg) : dpx;
+        const R tmp2v = A.dp_opt == 3 ? A.fc * (A.ug - u - A.u_mrf) : R(0.0);
+        // d(u w)/dz, d(v w)/dz and du/dz, dv/dz at this level from the face values
+        int s0;
+        const double *wd = g.zw(ZO_D, k, s0);
+        R uwz = R(0.0), vwz = R(0.0), duz = R(0.0), dvz = R(0.0);
+#pragma unroll
+        for (int q = 0; q < 6; ++q) {
+            const int kw = s0 + q - 1;
+            const R uf = g.uf6(kw), vf = g.vf6(kw);
+            const R wf = (kw < 0 || kw >= A.nz - 1) ? R(0.0) : g.W(0, 0, kw - k);
+            uwz += R(wd[q]) * uf * wf;
+            vwz += R(wd[q]) * vf * wf;
+            duz += R(wd[q]) * uf;
+            dvz += R(wd[q]) * vf;
+        }
+        R ru = R(1.0) / coe * tmp2u - uwz;
+        R rv = R(1.0) / coe * tmp2v - vwz;
+        R rw = k < A.nz - 1 ? -g.wwz6() : R(0.0);
+        if (A.isscalar && k < A.nz - 1) rw += R(1.0) / coe * (A.got * (g.thz6(0) - A.t_ref));
+        if (A.issk) {
+            auto D = [&](int f) { return sd[(f * ROWS + jj) * NX + i]; };
+            const R wc = g.whz6(k);  // w at this level's centre
+            ru -= u * D(0) + v * D(1) + duz * wc;
+            rv -= u * D(2) + v * D(3) + dvz * wc;
+            if (k < A.nz - 1) {
+                int sg;
+                const double *wg = g.zw(ZO_G, k, sg);
+                R dwz = R(0.0);
+#pragma unroll
+                for (int q = 0; q < 6; ++q) dwz += R(wg[q]) * g.whz6(sg + q);
+                rw -= a1 * D(4) + a1v * D(5) + w * dwz;
+            }
+        }
+        out[3] = ru;
+        out[4] = rv;
+        out[7] = rw;
+    }
+    if (in && ZO != 6) {
         PtT<R> g(A, i, j, k);
         const R u = g.U(0, 0, 0), up = g.U(0, 0, 1), um_ = g.U(0, 0, -1);
         const R v = g.V(0, 0, 0), vp = g.V(0, 0, 1), vm = g.V(0, 0, -1);
@@ -176,7 +227,7 @@ __global__ __launch_bounds__(1024) void k_spec_mom(StepArgsT<R> A)
 // Physical-space part of get_temperature_cnvdiff_spec
 // (src/module_spectral.f90:509-595) with the updated u, v, w: sp[0] = u*t
 // (d/dx), sp[1] = t*v (d/dy), sp[2] = -dTwdz.
-template <int NX, class R>
+template <int NX, class R, int ZO = 2>
 __global__ __launch_bounds__(1024) void k_spec_t(StepArgsT<R> A)
 {
     using C = cplx<R>;
@@ -197,7 +248,8 @@ __global__ __launch_bounds__(1024) void k_spec_t(StepArgsT<R> A)
         const R a1 = tp * lt + t * (R(1.0) - lt), a2 = tm * lb + t * (R(1.0) - lb);
         out[0] = u * t;
         out[1] = t * v;
-        out[2] = -((a1 * w - a2 * wm) / A.lev[LV_DZ * A.nz + k]);
+        if constexpr (ZO == 6) out[2] = -g.twz6();
+        else out[2] = -((a1 * w - a2 * wm) / A.lev[LV_DZ * A.nz + k]);
     }
 #pragma unroll
     for (int f = 0; f < 3; ++f) sd[(f * ROWS + jj) * NX + i] = out[f];
@@ -575,11 +627,23 @@ __global__ __launch_bounds__(256) void k_spec_div(StepArgs A, const C *cu, const
         const int l = (int)(q % A.nl), j = (int)((q / A.nl) % A.ny), k = (int)(q / plane);
         const double kx = A.symx[SY_GIM * A.nl + l], ky = A.symy[SY_GIM * A.ny + j];
         double2 d = cadd(times_i(ccast<double2>(cu[q]), kx), times_i(ccast<double2>(cv[q]), ky));
-        const double2 w0 = ccast<double2>(cw[q]);
-        const double2 wm = k == 0 ? czero() : ccast<double2>(cw[q - plane]);
-        const double idz = A.lev[LV_IDZ * A.nz + k];
-        d.x += (w0.x - wm.x) * idz;
-        d.y += (w0.y - wm.y) * idz;
+        if (A.zorder == 6) {  // ZO_D over the faces, zero at the walls
+            const size_t e = (size_t)ZO_D * A.nz + k;
+            const int s0 = A.zst[e];
+            for (int jz = 0; jz < 6; ++jz) {
+                const int lv = s0 + jz - 1;
+                if (lv < 0 || lv >= A.nz - 1) continue;
+                const double2 wv = ccast<double2>(cw[q + (size_t)(lv - k) * plane]);
+                d.x += A.zwt[e * kZW + jz] * wv.x;
+                d.y += A.zwt[e * kZW + jz] * wv.y;
+            }
+        } else {
+            const double2 w0 = ccast<double2>(cw[q]);
+            const double2 wm = k == 0 ? czero() : ccast<double2>(cw[q - plane]);
+            const double idz = A.lev[LV_IDZ * A.nz + k];
+            d.x += (w0.x - wm.x) * idz;
+            d.y += (w0.y - wm.y) * idz;
+        }
         dst[q] = ccast<C>(d);
     }
 }

@@ -385,3 +385,179 @@ __global__ __launch_bounds__(64 * kColWaves) void k_column_solve6(StepArgs A, Co
     }
     col_store<L>(io.s0, tile, A.nz, plane, col0, m, d);
 }
+
+// ---- spectral path (cds = 3) with the 6th-order z operators ---------------
+// One velocity component of the spectral column assembly (spec_assemble)
+// with ZO_C2_* / ZO_F2 for the explicit half of the z diffusion (plus the
+// (0,0)-mode wall data of u) and ZO_G for the w pressure gradient.
+template <int L>
+HC_DEV void spec_assemble6(const StepArgs &A, const SpecIO<double> &io, double2 *tile, double2 *buf, const ColMap &m,
+                           size_t plane, size_t col0, int e, int l, int jm, bool is00, double kx, double ky,
+                           double hxy, const double2 (&cp)[L], double2 (&q)[L])
+{
+    const int nz = A.nz;
+    const double coe = A.coe;
+    const double dif = A.rc * 0.5 * A.dt * A.nu;
+    const double2 *cs = e == 0 ? io.cu : e == 1 ? io.cv : io.cw;
+    const int ix = e == 0 ? 0 : e == 1 ? 1 : 5, iy = e == 0 ? 1 : e == 1 ? 2 : 6, ir = e == 0 ? 3 : e == 1 ? 4 : 7;
+    double2 *fo = e == 0 ? io.cff : e == 1 ? io.cgg : io.chh;
+    double2 x[L], f[L], t[L];
+    col_load<L>(cs, tile, nz, plane, col0, m, x);
+    const double um = A.u_mrf;
+#pragma unroll
+    for (int r = 0; r < L; ++r) {
+        q[r] = make_double2(x[r].x + hxy * x[r].x, x[r].y + hxy * x[r].y);
+        f[r] = times_i(x[r], -um * kx);
+    }
+    const Z6Op o = z6_op(A, e == 2 ? 1 : 0, l, jm);
+    if (e == 2) z6_apply<L, 7>(A, ZO_F2, 1, nz - 1, nz - 1, m, buf, x, t);
+    else z6_apply<L, 7>(A, o.op, 1, nz, nz, m, buf, x, t);
+#pragma unroll
+    for (int r = 0; r < L; ++r) { q[r].x += dif * t[r].x; q[r].y += dif * t[r].y; }
+    if (is00 && e == 0) z6_fixups<L>(A, o, m, q);
+    col_load<L>(io.sp[ix], tile, nz, plane, col0, m, x);
+#pragma unroll
+    for (int r = 0; r < L; ++r) f[r] = csub(f[r], times_i(x[r], coe * kx));
+    col_load<L>(io.sp[iy], tile, nz, plane, col0, m, x);
+#pragma unroll
+    for (int r = 0; r < L; ++r) f[r] = csub(f[r], times_i(x[r], coe * ky));
+    col_load<L>(io.sp[ir], tile, nz, plane, col0, m, x);
+#pragma unroll
+    for (int r = 0; r < L; ++r) f[r] = make_double2(f[r].x + coe * x[r].x, f[r].y + coe * x[r].y);
+    const double cdt = A.rc * A.dt;
+    if (e < 2) {
+        const double kk = e == 0 ? kx : ky;
+#pragma unroll
+        for (int r = 0; r < L; ++r) q[r] = csub(q[r], times_i(cp[r], cdt * kk));
+    } else {
+        z6_apply<L, 6>(A, ZO_G, 0, nz, nz - 1, m, buf, cp, t);
+#pragma unroll
+        for (int r = 0; r < L; ++r) { q[r].x -= cdt * t[r].x; q[r].y -= cdt * t[r].y; }
+    }
+    col_load<L>(fo, tile, nz, plane, col0, m, x);
+    const double rb = A.rb * A.dt, ra = A.ra * A.dt;
+#pragma unroll
+    for (int r = 0; r < L; ++r) {
+        q[r] = make_double2(q[r].x + rb * x[r].x + ra * f[r].x, q[r].y + rb * x[r].y + ra * f[r].y);
+    }
+    col_store<L>(fo, tile, nz, plane, col0, m, f);
+}
+
+// Spectral substep column (k_column_spec) with the 6th-order z operators.
+// Every wavefront runs the solves (the operator products synchronise the
+// block); columns outside the 2/3 rule are zeroed at the end.
+template <int L>
+__global__ __launch_bounds__(64 * kColWaves) void k_column_spec6(StepArgs A, SpecIO<double> io)
+{
+    extern __shared__ double2 tile_raw[];
+    double2 *tile = tile_raw;
+    const ColPos pos = spec_pos(io, A.ny);
+    if (pos.skip) return;
+    const int wave = threadIdx.x >> 6;
+    const int l = pos.bx * kColWaves + wave, jm = pos.jm;
+    const ColMap m = col_map(A.nz);
+    const int nz = A.nz, nl = A.nl, ny = A.ny;
+    const bool is00 = (l == 0 && jm == 0);
+    const int mw = jm < ny / 2 ? jm : jm - ny;
+    const bool kept = l < A.lc && mw < A.mc && mw > -A.mc;
+    const size_t plane = (size_t)ny * nl, col0 = (size_t)jm * nl + (size_t)pos.bx * kColWaves;
+    const double kx = A.symx[SY_GIM * nl + l], ky = A.symy[SY_GIM * ny + jm];
+    const double hxy = A.rc * (A.symx[SY_HU * nl + l] + A.symy[SY_HU * ny + jm]);
+    double2 cpr[L], duv[2][L], dw[L], dp[L], t[L];
+    col_load<L>(io.cp, tile, nz, plane, col0, m, cpr);
+    double2 *buf = tile + (size_t)wave * nz;
+    spec_assemble6<L>(A, io, tile, buf, m, plane, col0, 0, l, jm, is00, kx, ky, hxy, cpr, duv[0]);
+    spec_assemble6<L>(A, io, tile, buf, m, plane, col0, 1, l, jm, is00, kx, ky, hxy, cpr, duv[1]);
+    spec_assemble6<L>(A, io, tile, buf, m, plane, col0, 2, l, jm, is00, kx, ky, hxy, cpr, dw);
+    double a[L], b[L], c[L];
+    Z6Op o = z6_op(A, 0, l, jm);
+    assemble<L>(A, 0, l, jm, true, m, a, b, c);
+    if (is00) z6_fixups<L>(A, o, m, duv[0]);
+    z6_solve<L>(A, o, m, buf, a, b, c, duv[0]);
+    z6_solve<L>(A, o, m, buf, a, b, c, duv[1]);
+#pragma unroll
+    for (int r = 0; r < L; ++r)
+        if (r < m.cnt && m.k0 + r == nz - 1) dw[r] = make_double2(0.0, 0.0);
+    o = z6_op(A, 1, l, jm);
+    assemble<L>(A, 1, l, jm, true, m, a, b, c);
+    z6_solve<L>(A, o, m, buf, a, b, c, dw);
+    z6_divergence<L>(A, m, buf, l, jm, duv[0], duv[1], dw, A.idt, dp);
+    z6_pressure<L>(A, m, buf, l, jm, is00, dp);
+    // new pressure (:461): p + phi - rc dt nu / 2 (hxy_L + D G) phi, then the projection (:464)
+    z6_apply<L, kZW>(A, ZO_P, 0, nz, nz, m, buf, dp, t);
+    const double slap = A.symx[SY_L * nl + l] + A.symy[SY_L * ny + jm];
+    const double cnu = A.rc * 0.5 * A.dt * A.nu;
+#pragma unroll
+    for (int r = 0; r < L; ++r) {
+        if (r >= m.cnt) continue;
+        cpr[r].x += dp[r].x - cnu * (slap * dp[r].x + t[r].x);
+        cpr[r].y += dp[r].y - cnu * (slap * dp[r].y + t[r].y);
+    }
+    z6_project<L>(A, m, buf, l, jm, A.rc * A.dt, dp, duv[0], duv[1], dw);
+    if (!kept) {
+#pragma unroll
+        for (int r = 0; r < L; ++r) duv[0][r] = duv[1][r] = dw[r] = cpr[r] = make_double2(0.0, 0.0);
+    }
+    col_store<L>(io.cu, tile, nz, plane, col0, m, duv[0]);
+    col_store<L>(io.cv, tile, nz, plane, col0, m, duv[1]);
+    col_store<L>(io.cw, tile, nz, plane, col0, m, dw);
+    col_store<L>(io.cp, tile, nz, plane, col0, m, cpr);
+}
+
+// Spectral temperature column (k_column_spec_t) with the 6th-order z operator.
+template <int L>
+__global__ __launch_bounds__(64 * kColWaves) void k_column_spec_t6(StepArgs A, SpecIO<double> io)
+{
+    extern __shared__ double2 tile_raw[];
+    double2 *tile = tile_raw;
+    const ColPos pos = spec_pos(io, A.ny);
+    if (pos.skip) return;
+    const int wave = threadIdx.x >> 6;
+    const int l = pos.bx * kColWaves + wave, jm = pos.jm;
+    const ColMap m = col_map(A.nz);
+    const int nz = A.nz, nl = A.nl, ny = A.ny;
+    const bool is00 = (l == 0 && jm == 0);
+    const int mw = jm < ny / 2 ? jm : jm - ny;
+    const bool kept = l < A.lc && mw < A.mc && mw > -A.mc;
+    const size_t plane = (size_t)ny * nl, col0 = (size_t)jm * nl + (size_t)pos.bx * kColWaves;
+    const double kx = A.symx[SY_GIM * nl + l], ky = A.symy[SY_GIM * ny + jm];
+    const double hxy = A.rc * (A.symx[SY_HT * nl + l] + A.symy[SY_HT * ny + jm]);
+    const double dif = A.rc * 0.5 * A.dt * A.alpha;
+    const double um = A.u_mrf;
+    double2 x[L], f[L], d[L], t[L];
+    col_load<L>(io.ct, tile, nz, plane, col0, m, x);
+    double2 *buf = tile + (size_t)wave * nz;
+#pragma unroll
+    for (int r = 0; r < L; ++r) {
+        d[r] = make_double2(x[r].x + hxy * x[r].x, x[r].y + hxy * x[r].y);
+        f[r] = times_i(x[r], -um * kx);
+    }
+    const Z6Op o = z6_op(A, 3, l, jm);
+    z6_apply<L, 7>(A, o.op, 1, nz, nz, m, buf, x, t);
+#pragma unroll
+    for (int r = 0; r < L; ++r) { d[r].x += dif * t[r].x; d[r].y += dif * t[r].y; }
+    if (is00) z6_fixups<L>(A, o, m, d);  // explicit half of the wall data
+    col_load<L>(io.sp[0], tile, nz, plane, col0, m, x);
+#pragma unroll
+    for (int r = 0; r < L; ++r) f[r] = csub(f[r], times_i(x[r], kx));
+    col_load<L>(io.sp[1], tile, nz, plane, col0, m, x);
+#pragma unroll
+    for (int r = 0; r < L; ++r) f[r] = csub(f[r], times_i(x[r], ky));
+    col_load<L>(io.sp[2], tile, nz, plane, col0, m, x);
+#pragma unroll
+    for (int r = 0; r < L; ++r) f[r] = cadd(f[r], x[r]);
+    col_load<L>(io.css, tile, nz, plane, col0, m, x);
+    const double rb = A.rb * A.dt, ra = A.ra * A.dt;
+#pragma unroll
+    for (int r = 0; r < L; ++r) d[r] = make_double2(d[r].x + rb * x[r].x + ra * f[r].x, d[r].y + rb * x[r].y + ra * f[r].y);
+    col_store<L>(io.css, tile, nz, plane, col0, m, f);
+    double a[L], b[L], c[L];
+    assemble<L>(A, 3, l, jm, true, m, a, b, c);
+    if (is00) z6_fixups<L>(A, o, m, d);  // implicit half
+    z6_solve<L>(A, o, m, buf, a, b, c, d);
+    if (!kept) {
+#pragma unroll
+        for (int r = 0; r < L; ++r) d[r] = make_double2(0.0, 0.0);
+    }
+    col_store<L>(io.ct, tile, nz, plane, col0, m, d);
+}

@@ -113,7 +113,7 @@ def test_z6_projection_is_divergence_free():
 
 
 @pytest.mark.parametrize("kw", [dict(precision=1), dict(isdamp=1, nzdamp=8, cdamp=0.1),
-                                dict(cds=3, issk=1), dict(nz=9), dict(zorder=4)])
+                                dict(cds=3, issk=1, precision=1), dict(nz=9), dict(zorder=4)])
 def test_z6_rejects_unsupported(kw):
     base = dict(nx=16, ny=16, nz=33, zorder=6)
     base.update(kw)
@@ -127,4 +127,38 @@ def test_z6_adjoint_rejected():
     ch = Channel(prm)
     with pytest.raises(HerculesError, match="status 2"):
         ch.wall_shear_gradient(2, 1)
+    ch.close()
+
+
+SPECTRAL_CASES = [
+    ("closed", 33, {"issk": 1}),
+    ("closed", 65, {"issk": 0, "dp_opt": 3, "fc": 0.5, "ug": 1.0, "vg": 0.2}),
+    ("open", 33, {"issk": 1}),
+    ("open", 65, {"issk": 1, "tbc_top": 1, "dtdz_top": 0.5}),
+    ("open", 129, {"issk": 0}),
+]
+
+
+@pytest.mark.parametrize("kind,nz,kw", SPECTRAL_CASES)
+def test_z6_spectral_steps_match_torch_reference(kind, nz, kw):
+    """Spectral path (cds = 3) with the 6th-order z operators against the
+    torch restatement of time_advancement_sp (tests/torch_spectral.py, equal
+    to the C oracle at 2nd order) with the same dense operators."""
+    from tests.torch_spectral import TorchSpectral
+    prm, st = _case(kind, nz, cds=3, **kw)
+    ref = TorchSpectral(prm).steps(st, 2)
+    ch = Channel(prm)
+    for k, v in st.items():
+        ch.set_field(k, v)
+    ch.init()
+    ch.steps(2)
+    names = ["u", "v", "w", "p"] + (["t"] if prm.isscalar else [])
+    tol = {"p": 1e-9}
+    for n in names:
+        err = rel_l2(ch.get_field(n), ref[n].numpy())
+        print(kind, nz, kw, n, f"{err:.2e}")
+        assert err < tol.get(n, 1e-12), (n, err)
+    _, dmax = ch.screen_div_error()
+    scale = np.abs(ref["u"].numpy()).max() / (prm.lz / prm.nz)
+    assert dmax < 1e-12 * scale, dmax
     ch.close()

@@ -48,3 +48,31 @@ def test_torch_step_with_temperature_matches_oracle(kw):
         err = rel_l2(out[n].numpy(), o.get(n))
         print(kw, n, f"{err:.2e}")
         assert err < 1e-11, (n, err)
+
+
+@pytest.mark.parametrize("kind,kw", [
+    ("closed", {"issk": 1}),
+    ("closed", {"issk": 0, "dp_opt": 3, "fc": 0.5, "ug": 1.0, "vg": 0.2}),
+    ("open", {"issk": 1}),
+    ("open", {"issk": 1, "tbc_top": 1, "dtdz_top": 0.5}),
+    ("open", {"issk": 0}),
+])
+def test_torch_spectral_step_matches_oracle(kind, kw):
+    """The torch restatement of the spectral substep (tests/torch_spectral.py)
+    against the C oracle's time_advancement_sp: the same fields after two steps."""
+    from tests.torch_spectral import TorchSpectral
+    if kind == "closed":
+        prm = cases.closed_channel(nx=16, ny=16, nz=17, wall_rows=1, cds=3, **kw)
+        st = cases.laminar_noise(prm, seed=4)
+        st.pop("t", None)
+    else:
+        prm = cases.stratified_open_channel(nx=16, ny=16, nz=17, wall_rows=1, cds=3, nu=1.0 / 180.0, **kw)
+        st = cases.loglaw_noise(prm, seed=4)
+    o = make_oracle(prm, st)
+    o.steps(2)
+    ref = TorchSpectral(prm).steps(st, 2)
+    names = ["u", "v", "w", "p"] + (["t"] if prm.isscalar else [])
+    for n in names:
+        err = rel_l2(ref[n].numpy(), o.get(n))
+        print(kind, kw, n, f"{err:.2e}")
+        assert err < 1e-11, (n, err)

@@ -61,6 +61,8 @@ def test_stencils_exact_for_polynomials(grid):
             check(Z["I_DD"], cdat, f(zf[1:-1]))
             check(Z["I_DN"], cdat_n, f(zf[1:-1]))
             check(Z["IFC"], fdat, f(zc))
+            check(Z["I_DD_top"][None], cdat, f(zf[-1:]))
+            check(Z["I_DN_top"][None], cdat_n, f(zf[-1:]))
     # a 7th-degree polynomial is not reproduced: the stencils are not wider
     f7 = lambda z: _poly(s(z), 7)
     cdat = np.concatenate([[f7(zf[0])], f7(zc), [f7(zf[-1])]])

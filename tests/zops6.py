@@ -70,6 +70,14 @@ def zops(nz, lz, zstretch, ochannel):
     build("G", nz - 1, nz, zc, 6, 1, lambda k: zf[k + 1], lambda k: k + 0.5)
     build("D", nz, nz + 1, zf, 6, 1, lambda k: zc[k], lambda k: k + 0.5)
     build("IFC", nz, nz + 1, zf, 6, 0, lambda k: zc[k], lambda k: k + 0.5)
+    # interpolation to the top wall face z_f(nz): the wall value itself, or
+    # (Neumann top) the extrapolation through the wall gradient
+    for top, ntop in (("DD", False), ("DN", True)):
+        lo = window(nz + 2, nz + 0.5, 6)
+        idx = list(range(lo, lo + 6))
+        row = np.zeros(nz + 2)
+        row[idx] = fd_weights(zf[-1], cpos[idx], 0, [5] if ntop else [])
+        ops["I_%s_top" % top] = row
     ops["P"] = ops["D"][:, 1:-1] @ ops["G"]
     ops["zf"], ops["zc"] = zf, zc
     return ops
