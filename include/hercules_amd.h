@@ -58,8 +58,9 @@ typedef struct hc_params {
      * lowest wavenumbers in FP32; the column solves of the low-wavenumber
      * blocks (kx index l < 8, |ky index| <= 8, which contain the kx = ky = 0
      * mean mode and its Neumann compatibility condition) and the mean
-     * pressure gradient / bulk velocity update in FP64.  HC_MIXED needs
-     * cds = 1 or 2 and isdamp = 0 (HC_ERR_UNSUPPORTED otherwise). */
+     * pressure gradient / bulk velocity update in FP64.  HC_MIXED covers
+     * the FD (cds = 1, 2) and spectral (cds = 3) paths and needs isdamp = 0
+     * (HC_ERR_UNSUPPORTED otherwise). */
     int precision;
     /* Top temperature condition (no namelist counterpart; the reference is
      * Dirichlet at both walls, src/module_boundary.f90:179-180).  0: t = ttop
@@ -71,15 +72,16 @@ typedef struct hc_params {
     /* Order of the wall-normal (z) operators (no namelist counterpart; the
      * reference is 2nd order in z, src/module_cnvdiff.f90:89,121).  0 or 2:
      * as the reference.  6: every z derivative, z interpolation and implicit
-     * z operator of the FD step (cds = 1, 2) from 7-point (second
-     * derivatives) and 6-point (first derivatives, interpolation) Lagrange
+     * z operator of the step (FD, cds = 1, 2, and spectral, cds = 3) from
+     * 7-point (second derivatives) and 6-point (first derivatives, interpolation) Lagrange
      * stencils on the stretched grid, one-sided near the walls with the wall
      * values as data; the pressure equation is the product of the discrete
      * z divergence and gradient, so the projected velocity is divergence-free
      * to round-off; the implicit solves are exact (Chebyshev iteration
      * preconditioned by the 2nd-order tridiagonal operator).  zorder = 6
-     * needs precision = HC_FP64, isdamp = 0, cds = 1 or 2 and nz >= 16
-     * (HC_ERR_UNSUPPORTED otherwise). */
+     * needs precision = HC_FP64, isdamp = 0 and nz >= 16, and has no
+     * adjoint (HC_ERR_UNSUPPORTED otherwise); every mode takes the
+     * consistent wall rows (wall_rows is not used). */
     int zorder;
 } hc_params;
 
