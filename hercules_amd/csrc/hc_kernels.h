@@ -357,7 +357,7 @@ struct Interp {
         int s;
         const double *w = zw(ZO_D, q.k, s);
         Re acc = Re(0.0);
-#pragma unroll
+#pragma unroll 1
         for (int j = 0; j < 6; ++j) {
             const int dd = s + j;
             if (dd == 0 || dd == q.A.nz) continue;
@@ -373,7 +373,7 @@ struct Interp {
         int s;
         const double *w = zw(ZO_D, q.k, s);
         Re acc = Re(0.0);
-#pragma unroll
+#pragma unroll 1
         for (int j = 0; j < 6; ++j) {
             const int dd = s + j;
             if (dd == 0 || dd == q.A.nz) continue;
@@ -389,7 +389,7 @@ struct Interp {
         int s;
         const double *w = zw(ZO_D, q.k, s);
         Re acc = Re(0.0);
-#pragma unroll
+#pragma unroll 1
         for (int j = 0; j < 6; ++j) {
             const int dd = s + j;
             if (dd == 0 || dd == q.A.nz) continue;
@@ -451,7 +451,7 @@ struct Interp {
         int s;
         const double *w = zw(ZO_G, q.k, s);
         Re acc = Re(0.0);
-#pragma unroll
+#pragma unroll 1
         for (int j = 0; j < 6; ++j) acc += Re(w[j]) * sq(whz6(s + j));
         return acc;
     }

@@ -133,7 +133,7 @@ __global__ __launch_bounds__(1024) void k_spec_mom(StepArgsT<R> A)
         int s0;
         const double *wd = g.zw(ZO_D, k, s0);
         R uwz = R(0.0), vwz = R(0.0), duz = R(0.0), dvz = R(0.0);
-#pragma unroll
+#pragma unroll 1
         for (int q = 0; q < 6; ++q) {
             const int kw = s0 + q - 1;
             const R uf = g.uf6(kw), vf = g.vf6(kw);
@@ -156,7 +156,7 @@ __global__ __launch_bounds__(1024) void k_spec_mom(StepArgsT<R> A)
                 int sg;
                 const double *wg = g.zw(ZO_G, k, sg);
                 R dwz = R(0.0);
-#pragma unroll
+#pragma unroll 1
                 for (int q = 0; q < 6; ++q) dwz += R(wg[q]) * g.whz6(sg + q);
                 rw -= a1 * D(4) + a1v * D(5) + w * dwz;
             }

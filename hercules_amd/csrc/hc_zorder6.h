@@ -12,7 +12,8 @@
 // operator of the same equation (wave_tridiag): the preconditioned spectrum
 // lies in [0.95, 1.8] on the stretched channel grids, so kZ6Iters
 // iterations reduce the residual by ~1e-15.  Operator products go through
-// the wavefront's slice of the column tile in LDS.  FP64 only.
+// the wavefront's own column of a second LDS sub-tile (the first stages the
+// global loads and stores), ordered within the wavefront.  FP64 only.
 #pragma once
 
 constexpr int kZ6Iters = 20;
@@ -24,11 +25,17 @@ template <int L, int W>
 HC_DEV void z6_apply(const StepArgs &A, int op, int off, int ndata, int nrows, const ColMap &m, double2 *buf,
                      const double2 (&x)[L], double2 (&y)[L])
 {
+    // buf is this wavefront's own slice of the tile: ordering the LDS
+    // accesses within the wavefront is enough (no block barrier)
     const int nz = A.nz;
+    __builtin_amdgcn_fence(__ATOMIC_RELEASE, "wavefront");
+    __builtin_amdgcn_wave_barrier();
 #pragma unroll
     for (int r = 0; r < L; ++r)
         if (r < m.cnt) buf[m.k0 + r] = x[r];
-    __syncthreads();
+    __builtin_amdgcn_fence(__ATOMIC_RELEASE, "wavefront");
+    __builtin_amdgcn_wave_barrier();
+    __builtin_amdgcn_fence(__ATOMIC_ACQUIRE, "wavefront");
 #pragma unroll
     for (int r = 0; r < L; ++r) {
         double2 acc = make_double2(0.0, 0.0);
@@ -49,7 +56,9 @@ HC_DEV void z6_apply(const StepArgs &A, int op, int off, int ndata, int nrows, c
         }
         y[r] = acc;
     }
-    __syncthreads();
+    __builtin_amdgcn_fence(__ATOMIC_RELEASE, "wavefront");
+    __builtin_amdgcn_wave_barrier();
+    __builtin_amdgcn_fence(__ATOMIC_ACQUIRE, "wavefront");
 }
 
 // Operator of equation eq (0 u/v, 1 w, 2 p, 3 t) of column (l, jm) applied
@@ -120,31 +129,36 @@ HC_DEV void z6_mul(const StepArgs &A, const Z6Op &o, const ColMap &m, double2 *b
 
 // Solve o x = rhs in place (x holds rhs on entry), preconditioned with the
 // tridiagonal rows a, b, c of the 2nd-order operator of the same equation.
+// Live vectors: x, the residual r, the search direction d and one product.
 template <int L>
 HC_DEV void z6_solve(const StepArgs &A, const Z6Op &o, const ColMap &m, double2 *buf, const double (&a)[L],
                      const double (&b)[L], const double (&c)[L], double2 (&x)[L])
 {
     const double lo = 0.9, hi = 1.85, th = 0.5 * (hi + lo), de = 0.5 * (hi - lo), s1 = th / de;
-    double2 r[L], t[L], d[1][L];
+    double2 r[L], d[1][L];
 #pragma unroll
     for (int q = 0; q < L; ++q) { r[q] = x[q]; d[0][q] = x[q]; }
     wave_tridiag<L, 1>(m.cnt, a, b, c, d);
 #pragma unroll
     for (int q = 0; q < L; ++q) x[q] = d[0][q];
-    z6_mul<L>(A, o, m, buf, x, t);
+    {
+        double2 z[1][L];
+        z6_mul<L>(A, o, m, buf, x, z[0]);
 #pragma unroll
-    for (int q = 0; q < L; ++q) { r[q].x -= t[q].x; r[q].y -= t[q].y; d[0][q] = r[q]; }
-    wave_tridiag<L, 1>(m.cnt, a, b, c, d);
+        for (int q = 0; q < L; ++q) { r[q].x -= z[0][q].x; r[q].y -= z[0][q].y; z[0][q] = r[q]; }
+        wave_tridiag<L, 1>(m.cnt, a, b, c, z);
 #pragma unroll
-    for (int q = 0; q < L; ++q) { d[0][q].x /= th; d[0][q].y /= th; }
+        for (int q = 0; q < L; ++q) { d[0][q].x = z[0][q].x / th; d[0][q].y = z[0][q].y / th; }
+    }
     double rho0 = 1.0 / s1;
+#pragma unroll 1
     for (int it = 0; it < kZ6Iters; ++it) {
         double2 z[1][L];
 #pragma unroll
         for (int q = 0; q < L; ++q) { x[q].x += d[0][q].x; x[q].y += d[0][q].y; }
-        z6_mul<L>(A, o, m, buf, d[0], t);
+        z6_mul<L>(A, o, m, buf, d[0], z[0]);
 #pragma unroll
-        for (int q = 0; q < L; ++q) { r[q].x -= t[q].x; r[q].y -= t[q].y; z[0][q] = r[q]; }
+        for (int q = 0; q < L; ++q) { r[q].x -= z[0][q].x; r[q].y -= z[0][q].y; z[0][q] = r[q]; }
         wave_tridiag<L, 1>(m.cnt, a, b, c, z);
         const double rho1 = 1.0 / (2.0 * s1 - rho0), cd = rho1 * rho0, cz = 2.0 * rho1 / de;
 #pragma unroll
@@ -243,6 +257,8 @@ HC_DEV void z6_project(const StepArgs &A, const ColMap &m, double2 *buf, int l, 
 // FD substep column with the 6th-order z operators: the steps of
 // k_column_mom (u, v, w Helmholtz solves, divergence, pressure Poisson,
 // projection, pressure increment) with the z operators of the ZOp tables.
+// One field is live in registers at a time: the solved u*, v*, w* go back to
+// their spectra and are read again for the divergence and the projection.
 template <int L>
 __global__ __launch_bounds__(64 * kColWaves) void k_column_mom6(StepArgs A, ColIO<double> io)
 {
@@ -256,45 +272,81 @@ __global__ __launch_bounds__(64 * kColWaves) void k_column_mom6(StepArgs A, ColI
     const int nz = A.nz, nl = A.nl, ny = A.ny;
     const bool is00 = (l == 0 && jm == 0);
     const size_t plane = (size_t)ny * nl, col0 = (size_t)jm * nl + (size_t)pos.bx * kColWaves;
-    double2 duv[2][L], dw[L], dp[L], t[L];
-    col_load<L>(io.s0, tile, nz, plane, col0, m, duv[0]);
-    col_load<L>(io.s1, tile, nz, plane, col0, m, duv[1]);
-    col_load<L>(io.s2, tile, nz, plane, col0, m, dw);
-    __syncthreads();
-    double2 *buf = tile + (size_t)wave * nz;  // this wavefront's column of the tile
-    double a[L], b[L], c[L];
-    // u and v: every mode takes the (0,0) rows in the preconditioner
-    Z6Op o = z6_op(A, 0, l, jm);
-    assemble<L>(A, 0, l, jm, true, m, a, b, c);
-    if (is00) z6_fixups<L>(A, o, m, duv[0]);
-    z6_solve<L>(A, o, m, buf, a, b, c, duv[0]);
-    z6_solve<L>(A, o, m, buf, a, b, c, duv[1]);
-    // w; the top-wall row is trivial
+    double2 *buf = tile + (size_t)(kColWaves + wave) * nz;  // this wavefront's column of the second sub-tile
+    double2 x[L];
+    {
+        double a[L], b[L], c[L];
+        // u and v: one operator; every mode takes the (0,0) rows in the preconditioner
+        Z6Op o = z6_op(A, 0, l, jm);
+        assemble<L>(A, 0, l, jm, true, m, a, b, c);
+        col_load<L>(io.s0, tile, nz, plane, col0, m, x);
+        if (is00) z6_fixups<L>(A, o, m, x);
+        z6_solve<L>(A, o, m, buf, a, b, c, x);
+        col_store<L>(io.s0, tile, nz, plane, col0, m, x);
+        col_load<L>(io.s1, tile, nz, plane, col0, m, x);
+        z6_solve<L>(A, o, m, buf, a, b, c, x);
+        col_store<L>(io.s1, tile, nz, plane, col0, m, x);
+        // w; the top-wall row is trivial
+        col_load<L>(io.s2, tile, nz, plane, col0, m, x);
 #pragma unroll
-    for (int r = 0; r < L; ++r)
-        if (r < m.cnt && m.k0 + r == nz - 1) dw[r] = make_double2(0.0, 0.0);
-    o = z6_op(A, 1, l, jm);
-    assemble<L>(A, 1, l, jm, true, m, a, b, c);
-    z6_solve<L>(A, o, m, buf, a, b, c, dw);
+        for (int r = 0; r < L; ++r)
+            if (r < m.cnt && m.k0 + r == nz - 1) x[r] = make_double2(0.0, 0.0);
+        o = z6_op(A, 1, l, jm);
+        assemble<L>(A, 1, l, jm, true, m, a, b, c);
+        z6_solve<L>(A, o, m, buf, a, b, c, x);
+        col_store<L>(io.s2, tile, nz, plane, col0, m, x);
+    }
     // divergence / dt (get_p_eqn_src) with ZO_D over the interior faces
-    z6_divergence<L>(A, m, buf, l, jm, duv[0], duv[1], dw, A.idt, dp);
+    double2 dp[L];
+    z6_apply<L, 6>(A, ZO_D, 1, nz - 1, nz, m, buf, x, dp);
+    {
+        const double2 sdx = make_double2(A.symx[SY_DRE * nl + l], A.symx[SY_DIM * nl + l]);
+        const double2 sdy = make_double2(A.symy[SY_DRE * ny + jm], A.symy[SY_DIM * ny + jm]);
+        col_load<L>(io.s0, tile, nz, plane, col0, m, x);
+#pragma unroll
+        for (int r = 0; r < L; ++r) dp[r] = cadd(dp[r], cmul(sdx, x[r]));
+        col_load<L>(io.s1, tile, nz, plane, col0, m, x);
+#pragma unroll
+        for (int r = 0; r < L; ++r) {
+            const double2 dv = cadd(dp[r], cmul(sdy, x[r]));
+            dp[r] = r < m.cnt ? make_double2(dv.x * A.idt, dv.y * A.idt) : make_double2(0.0, 0.0);
+        }
+    }
     // pressure: rc (hxy + D G) phi = div / dt
     z6_pressure<L>(A, m, buf, l, jm, is00, dp);
-    // projection (calculate_uvw) with ZO_G for w
-    z6_project<L>(A, m, buf, l, jm, A.rc * A.dt, dp, duv[0], duv[1], dw);
-    // pressure increment phi - rc dt nu / 2 (hxy_L + D G) phi
-    z6_apply<L, kZW>(A, ZO_P, 0, nz, nz, m, buf, dp, t);
+    // projection (calculate_uvw): u, v by the horizontal gradient symbols, w by ZO_G
+    const double cdt = A.rc * A.dt;
+    {
+        double2 g[L];
+        z6_apply<L, 6>(A, ZO_G, 0, nz, nz - 1, m, buf, dp, g);
+        col_load<L>(io.s2, tile, nz, plane, col0, m, x);
+#pragma unroll
+        for (int r = 0; r < L; ++r)
+            if (r < m.cnt && m.k0 + r < nz - 1) { x[r].x -= cdt * g[r].x; x[r].y -= cdt * g[r].y; }
+        col_store<L>(io.s2, tile, nz, plane, col0, m, x);
+    }
+    {
+        const double2 sgx = make_double2(A.symx[SY_GRE * nl + l], A.symx[SY_GIM * nl + l]);
+        col_load<L>(io.s0, tile, nz, plane, col0, m, x);
+#pragma unroll
+        for (int r = 0; r < L; ++r) x[r] = csub(x[r], cscale(cmul(sgx, dp[r]), cdt));
+        col_store<L>(io.s0, tile, nz, plane, col0, m, x);
+        const double2 sgy = make_double2(A.symy[SY_GRE * ny + jm], A.symy[SY_GIM * ny + jm]);
+        col_load<L>(io.s1, tile, nz, plane, col0, m, x);
+#pragma unroll
+        for (int r = 0; r < L; ++r) x[r] = csub(x[r], cscale(cmul(sgy, dp[r]), cdt));
+        col_store<L>(io.s1, tile, nz, plane, col0, m, x);
+    }
+    // pressure increment phi - rc dt nu / 2 (hxy_L + D G) phi (calculate_new_p)
+    z6_apply<L, kZW>(A, ZO_P, 0, nz, nz, m, buf, dp, x);
     const double slap = A.symx[SY_L * nl + l] + A.symy[SY_L * ny + jm];
     const double cnu = A.rc * 0.5 * A.dt * A.nu;
 #pragma unroll
     for (int r = 0; r < L; ++r) {
-        t[r] = make_double2(dp[r].x - cnu * (slap * dp[r].x + t[r].x), dp[r].y - cnu * (slap * dp[r].y + t[r].y));
-        if (r >= m.cnt) t[r] = make_double2(0.0, 0.0);
+        x[r] = make_double2(dp[r].x - cnu * (slap * dp[r].x + x[r].x), dp[r].y - cnu * (slap * dp[r].y + x[r].y));
+        if (r >= m.cnt) x[r] = make_double2(0.0, 0.0);
     }
-    col_store<L>(io.s0, tile, nz, plane, col0, m, duv[0]);
-    col_store<L>(io.s1, tile, nz, plane, col0, m, duv[1]);
-    col_store<L>(io.s2, tile, nz, plane, col0, m, dw);
-    col_store<L>(io.s3, tile, nz, plane, col0, m, t);
+    col_store<L>(io.s3, tile, nz, plane, col0, m, x);
 }
 
 // Temperature Helmholtz solve with the 6th-order z operator, in place on s0.
@@ -313,7 +365,7 @@ __global__ __launch_bounds__(64 * kColWaves) void k_column_t6(StepArgs A, ColIO<
     double2 d[L];
     col_load<L>(io.s0, tile, A.nz, plane, col0, m, d);
     __syncthreads();
-    double2 *buf = tile + (size_t)wave * A.nz;
+    double2 *buf = tile + (size_t)(kColWaves + wave) * A.nz;
     double a[L], b[L], c[L];
     const Z6Op o = z6_op(A, 3, l, jm);
     assemble<L>(A, 3, l, jm, true, m, a, b, c);
@@ -343,7 +395,7 @@ __global__ __launch_bounds__(64 * kColWaves) void k_column_project6(StepArgs A, 
 #pragma unroll
     for (int r = 0; r < L; ++r)
         if (r < m.cnt && m.k0 + r == A.nz - 1) w[r] = make_double2(0.0, 0.0);
-    double2 *buf = tile + (size_t)wave * A.nz;
+    double2 *buf = tile + (size_t)(kColWaves + wave) * A.nz;
     z6_divergence<L>(A, m, buf, l, jm, u, v, w, 1.0, ph);
     z6_pressure<L>(A, m, buf, l, jm, is00, ph);
     z6_project<L>(A, m, buf, l, jm, 1.0, ph, u, v, w);
@@ -372,7 +424,7 @@ __global__ __launch_bounds__(64 * kColWaves) void k_column_solve6(StepArgs A, Co
 #pragma unroll
     for (int r = 0; r < L; ++r)
         if (eq == 2 && r < m.cnt && m.k0 + r == A.nz - 1) d[r] = make_double2(0.0, 0.0);
-    double2 *buf = tile + (size_t)wave * A.nz;
+    double2 *buf = tile + (size_t)(kColWaves + wave) * A.nz;
     const int meq = eq == 0 || eq == 1 ? 0 : (eq == 2 ? 1 : (eq == 3 ? 2 : 3));
     if (meq == 2) {
         z6_pressure<L>(A, m, buf, l, jm, is00, d);
@@ -393,7 +445,7 @@ __global__ __launch_bounds__(64 * kColWaves) void k_column_solve6(StepArgs A, Co
 template <int L>
 HC_DEV void spec_assemble6(const StepArgs &A, const SpecIO<double> &io, double2 *tile, double2 *buf, const ColMap &m,
                            size_t plane, size_t col0, int e, int l, int jm, bool is00, double kx, double ky,
-                           double hxy, const double2 (&cp)[L], double2 (&q)[L])
+                           double hxy, double2 (&q)[L])
 {
     const int nz = A.nz;
     const double coe = A.coe;
@@ -425,12 +477,13 @@ HC_DEV void spec_assemble6(const StepArgs &A, const SpecIO<double> &io, double2 
 #pragma unroll
     for (int r = 0; r < L; ++r) f[r] = make_double2(f[r].x + coe * x[r].x, f[r].y + coe * x[r].y);
     const double cdt = A.rc * A.dt;
+    col_load<L>(io.cp, tile, nz, plane, col0, m, x);  // the pressure of the substep's start
     if (e < 2) {
         const double kk = e == 0 ? kx : ky;
 #pragma unroll
-        for (int r = 0; r < L; ++r) q[r] = csub(q[r], times_i(cp[r], cdt * kk));
+        for (int r = 0; r < L; ++r) q[r] = csub(q[r], times_i(x[r], cdt * kk));
     } else {
-        z6_apply<L, 6>(A, ZO_G, 0, nz, nz - 1, m, buf, cp, t);
+        z6_apply<L, 6>(A, ZO_G, 0, nz, nz - 1, m, buf, x, t);
 #pragma unroll
         for (int r = 0; r < L; ++r) { q[r].x -= cdt * t[r].x; q[r].y -= cdt * t[r].y; }
     }
@@ -445,7 +498,9 @@ HC_DEV void spec_assemble6(const StepArgs &A, const SpecIO<double> &io, double2 
 
 // Spectral substep column (k_column_spec) with the 6th-order z operators.
 // Every wavefront runs the solves (the operator products synchronise the
-// block); columns outside the 2/3 rule are zeroed at the end.
+// block); columns outside the 2/3 rule are zeroed as they are stored.  One
+// field is live in registers at a time: the solved u*, v*, w* go back to
+// c_u, c_v, c_w and are read again for the divergence and the projection.
 template <int L>
 __global__ __launch_bounds__(64 * kColWaves) void k_column_spec6(StepArgs A, SpecIO<double> io)
 {
@@ -463,45 +518,82 @@ __global__ __launch_bounds__(64 * kColWaves) void k_column_spec6(StepArgs A, Spe
     const size_t plane = (size_t)ny * nl, col0 = (size_t)jm * nl + (size_t)pos.bx * kColWaves;
     const double kx = A.symx[SY_GIM * nl + l], ky = A.symy[SY_GIM * ny + jm];
     const double hxy = A.rc * (A.symx[SY_HU * nl + l] + A.symy[SY_HU * ny + jm]);
-    double2 cpr[L], duv[2][L], dw[L], dp[L], t[L];
-    col_load<L>(io.cp, tile, nz, plane, col0, m, cpr);
-    double2 *buf = tile + (size_t)wave * nz;
-    spec_assemble6<L>(A, io, tile, buf, m, plane, col0, 0, l, jm, is00, kx, ky, hxy, cpr, duv[0]);
-    spec_assemble6<L>(A, io, tile, buf, m, plane, col0, 1, l, jm, is00, kx, ky, hxy, cpr, duv[1]);
-    spec_assemble6<L>(A, io, tile, buf, m, plane, col0, 2, l, jm, is00, kx, ky, hxy, cpr, dw);
-    double a[L], b[L], c[L];
-    Z6Op o = z6_op(A, 0, l, jm);
-    assemble<L>(A, 0, l, jm, true, m, a, b, c);
-    if (is00) z6_fixups<L>(A, o, m, duv[0]);
-    z6_solve<L>(A, o, m, buf, a, b, c, duv[0]);
-    z6_solve<L>(A, o, m, buf, a, b, c, duv[1]);
+    double2 *buf = tile + (size_t)(kColWaves + wave) * nz;
+    double2 x[L];
+    {
+        double a[L], b[L], c[L];
+        Z6Op o = z6_op(A, 0, l, jm);
+        assemble<L>(A, 0, l, jm, true, m, a, b, c);
+        spec_assemble6<L>(A, io, tile, buf, m, plane, col0, 0, l, jm, is00, kx, ky, hxy, x);
+        if (is00) z6_fixups<L>(A, o, m, x);
+        z6_solve<L>(A, o, m, buf, a, b, c, x);
+        col_store<L>(io.cu, tile, nz, plane, col0, m, x);
+        spec_assemble6<L>(A, io, tile, buf, m, plane, col0, 1, l, jm, is00, kx, ky, hxy, x);
+        z6_solve<L>(A, o, m, buf, a, b, c, x);
+        col_store<L>(io.cv, tile, nz, plane, col0, m, x);
+        spec_assemble6<L>(A, io, tile, buf, m, plane, col0, 2, l, jm, is00, kx, ky, hxy, x);
 #pragma unroll
-    for (int r = 0; r < L; ++r)
-        if (r < m.cnt && m.k0 + r == nz - 1) dw[r] = make_double2(0.0, 0.0);
-    o = z6_op(A, 1, l, jm);
-    assemble<L>(A, 1, l, jm, true, m, a, b, c);
-    z6_solve<L>(A, o, m, buf, a, b, c, dw);
-    z6_divergence<L>(A, m, buf, l, jm, duv[0], duv[1], dw, A.idt, dp);
+        for (int r = 0; r < L; ++r)
+            if (r < m.cnt && m.k0 + r == nz - 1) x[r] = make_double2(0.0, 0.0);
+        o = z6_op(A, 1, l, jm);
+        assemble<L>(A, 1, l, jm, true, m, a, b, c);
+        z6_solve<L>(A, o, m, buf, a, b, c, x);
+        col_store<L>(io.cw, tile, nz, plane, col0, m, x);
+    }
+    // divergence / dt with ZO_D (get_div_spec)
+    double2 dp[L];
+    z6_apply<L, 6>(A, ZO_D, 1, nz - 1, nz, m, buf, x, dp);
+    {
+        const double2 sdx = make_double2(A.symx[SY_DRE * nl + l], A.symx[SY_DIM * nl + l]);
+        const double2 sdy = make_double2(A.symy[SY_DRE * ny + jm], A.symy[SY_DIM * ny + jm]);
+        col_load<L>(io.cu, tile, nz, plane, col0, m, x);
+#pragma unroll
+        for (int r = 0; r < L; ++r) dp[r] = cadd(dp[r], cmul(sdx, x[r]));
+        col_load<L>(io.cv, tile, nz, plane, col0, m, x);
+#pragma unroll
+        for (int r = 0; r < L; ++r) {
+            const double2 dv = cadd(dp[r], cmul(sdy, x[r]));
+            dp[r] = r < m.cnt ? make_double2(dv.x * A.idt, dv.y * A.idt) : make_double2(0.0, 0.0);
+        }
+    }
     z6_pressure<L>(A, m, buf, l, jm, is00, dp);
-    // new pressure (:461): p + phi - rc dt nu / 2 (hxy_L + D G) phi, then the projection (:464)
-    z6_apply<L, kZW>(A, ZO_P, 0, nz, nz, m, buf, dp, t);
-    const double slap = A.symx[SY_L * nl + l] + A.symy[SY_L * ny + jm];
-    const double cnu = A.rc * 0.5 * A.dt * A.nu;
+    const double2 zero = make_double2(0.0, 0.0);
+    // new pressure (:461): p + phi - rc dt nu / 2 (hxy_L + D G) phi
+    {
+        double2 t[L];
+        z6_apply<L, kZW>(A, ZO_P, 0, nz, nz, m, buf, dp, t);
+        const double slap = A.symx[SY_L * nl + l] + A.symy[SY_L * ny + jm];
+        const double cnu = A.rc * 0.5 * A.dt * A.nu;
+        col_load<L>(io.cp, tile, nz, plane, col0, m, x);
 #pragma unroll
-    for (int r = 0; r < L; ++r) {
-        if (r >= m.cnt) continue;
-        cpr[r].x += dp[r].x - cnu * (slap * dp[r].x + t[r].x);
-        cpr[r].y += dp[r].y - cnu * (slap * dp[r].y + t[r].y);
-    }
-    z6_project<L>(A, m, buf, l, jm, A.rc * A.dt, dp, duv[0], duv[1], dw);
-    if (!kept) {
+        for (int r = 0; r < L; ++r) {
+            x[r].x += dp[r].x - cnu * (slap * dp[r].x + t[r].x);
+            x[r].y += dp[r].y - cnu * (slap * dp[r].y + t[r].y);
+            if (!kept || r >= m.cnt) x[r] = zero;
+        }
+        col_store<L>(io.cp, tile, nz, plane, col0, m, x);
+        // projection (:464): w by ZO_G
+        z6_apply<L, 6>(A, ZO_G, 0, nz, nz - 1, m, buf, dp, t);
+        col_load<L>(io.cw, tile, nz, plane, col0, m, x);
+        const double cdt = A.rc * A.dt;
 #pragma unroll
-        for (int r = 0; r < L; ++r) duv[0][r] = duv[1][r] = dw[r] = cpr[r] = make_double2(0.0, 0.0);
+        for (int r = 0; r < L; ++r) {
+            if (r < m.cnt && m.k0 + r < nz - 1) { x[r].x -= cdt * t[r].x; x[r].y -= cdt * t[r].y; }
+            if (!kept) x[r] = zero;
+        }
+        col_store<L>(io.cw, tile, nz, plane, col0, m, x);
     }
-    col_store<L>(io.cu, tile, nz, plane, col0, m, duv[0]);
-    col_store<L>(io.cv, tile, nz, plane, col0, m, duv[1]);
-    col_store<L>(io.cw, tile, nz, plane, col0, m, dw);
-    col_store<L>(io.cp, tile, nz, plane, col0, m, cpr);
+    const double cdt = A.rc * A.dt;
+    const double2 sgx = make_double2(A.symx[SY_GRE * nl + l], A.symx[SY_GIM * nl + l]);
+    col_load<L>(io.cu, tile, nz, plane, col0, m, x);
+#pragma unroll
+    for (int r = 0; r < L; ++r) x[r] = kept ? csub(x[r], cscale(cmul(sgx, dp[r]), cdt)) : zero;
+    col_store<L>(io.cu, tile, nz, plane, col0, m, x);
+    const double2 sgy = make_double2(A.symy[SY_GRE * ny + jm], A.symy[SY_GIM * ny + jm]);
+    col_load<L>(io.cv, tile, nz, plane, col0, m, x);
+#pragma unroll
+    for (int r = 0; r < L; ++r) x[r] = kept ? csub(x[r], cscale(cmul(sgy, dp[r]), cdt)) : zero;
+    col_store<L>(io.cv, tile, nz, plane, col0, m, x);
 }
 
 // Spectral temperature column (k_column_spec_t) with the 6th-order z operator.
@@ -526,7 +618,7 @@ __global__ __launch_bounds__(64 * kColWaves) void k_column_spec_t6(StepArgs A, S
     const double um = A.u_mrf;
     double2 x[L], f[L], d[L], t[L];
     col_load<L>(io.ct, tile, nz, plane, col0, m, x);
-    double2 *buf = tile + (size_t)wave * nz;
+    double2 *buf = tile + (size_t)(kColWaves + wave) * nz;
 #pragma unroll
     for (int r = 0; r < L; ++r) {
         d[r] = make_double2(x[r].x + hxy * x[r].x, x[r].y + hxy * x[r].y);
