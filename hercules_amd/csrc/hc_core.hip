@@ -1293,9 +1293,11 @@ struct hc_ctx {
     int fcur, scur;
     double2 *s[4];
     double *lev, *symx, *symy, *scal, *part, *stage;
-    // 6th-order z operators (zorder = 6): stencil starts and weights
+    // 6th-order z operators (zorder = 6): stencil starts and weights, and
+    // the interpolants the right-hand-side kernels read (k_zinterp6)
     int *zst = nullptr;
     double *zwt = nullptr;
+    double *zbuf[4] = {nullptr, nullptr, nullptr, nullptr};
     std::vector<int> h_zst;
     std::vector<double> h_zwt;
     double *dmp_s, *dmp_m;  // Rayleigh damping: sigz per level, plane means per (field, level)
@@ -1893,6 +1895,16 @@ static int launch_columns(hc_ctx *c, int kind, StepArgs A, const View<R> &V, int
     return 0;
 }
 
+// 6th-order interpolants of the current fields into zbuf, and their
+// addresses in the row-kernel arguments B
+template <class R>
+static void zinterp6(hc_ctx *c, const StepArgs &A, StepArgsT<R> &B)
+{
+    hipLaunchKernelGGL(k_zinterp6, dim3(2048), dim3(256), 0, c->stream, A, c->zbuf[0], c->zbuf[1], c->zbuf[2],
+                       c->zbuf[3]);
+    B.zfu = c->zbuf[0]; B.zfv = c->zbuf[1]; B.zcw = c->zbuf[2]; B.zft = c->zbuf[3];
+}
+
 // one substep t_its (1..3 for RK3, 1 for AB2) over the buffers of V (stored
 // precision R); Rayleigh damping only with FP64 storage
 template <class R>
@@ -1938,6 +1950,7 @@ static int substep_t(hc_ctx *c, int t_its, const View<R> &V)
     if constexpr (std::is_same<R, double>::value) {
         if (z6) {  // 6th-order z operators: the global-memory point kernel
             B.dmask = 0;
+            zinterp6<R>(c, A, B);
 #define HC_RHSZ(NX_)                                                                                              \
     if (P.cds == 1) hipLaunchKernelGGL((k_rhs_mom_damped<1, NX_, Pt, 6>), grow, dim3(kRhsThreads), 0, c->stream, B); \
     else hipLaunchKernelGGL((k_rhs_mom_damped<2, NX_, Pt, 6>), grow, dim3(kRhsThreads), 0, c->stream, B);
@@ -2193,6 +2206,8 @@ static int substep_sp_t(hc_ctx *c, int t_its, const View<R> &V)
     const int rows = rows_for(c->nx);
     const dim3 grow((c->ny + rows - 1) / rows, c->nz);
     PROF_BEGIN(c, HC_K_RHS);
+    if constexpr (std::is_same<R, double>::value)
+        if (A.zorder == 6) zinterp6<R>(c, A, B);
     if (P.issk == 1) {
         // du/dx, du/dy, dv/dx, dv/dy, dw/dx, dw/dy in y-physical space
         const C *src4[4] = {V.cu, V.cu, V.cv, V.cv};
@@ -2380,6 +2395,8 @@ extern "C" int hc_create(const hc_params *prm, hc_ctx **out)
         if (!build_zops(c)) e = set_err(HC_ERR_ARG, "6th-order z stencils: singular system");
         if (!e) e = dalloc(c, (void **)&c->zst, c->h_zst.size() * sizeof(int));
         if (!e) e = dalloc(c, (void **)&c->zwt, c->h_zwt.size() * sizeof(double));
+        for (int q = 0; q < 4 && !e; ++q)
+            if (q < 3 || P.isscalar == 1) e = dalloc(c, (void **)&c->zbuf[q], N * sizeof(double));
     }
     if (!e && P.precision == HC_MIXED) {
         c->mixed = true;

@@ -110,6 +110,11 @@ struct StepArgsT {
     int zorder;
     const int *zst;
     const double *zwt;
+    // 6th-order interpolants of the substep's fields, formed once by
+    // k_zinterp6 (null: the accessors evaluate them): u, v at the w face of
+    // each level (the top wall face included), w at the centres, t at the
+    // interior w faces
+    const double *zfu, *zfv, *zcw, *zft;
 };
 using StepArgs = StepArgsT<double>;
 
@@ -266,6 +271,7 @@ struct Interp {
         using Re = typename D::Real;
         const int kw = q.k + dk;
         if (kw < 0 || kw >= q.A.nz - 1) return q.uhz(di, dj, dk);  // wall faces: the wall rule
+        if (q.A.zfu) return Re(q.A.zfu[q.at(di, dj, kw)]);
         int s;
         const double *w = zw(q.A.ochannel ? ZO_I_DN : ZO_I_DD, kw, s);
         Re acc = Re(0.0);
@@ -279,6 +285,7 @@ struct Interp {
         using Re = typename D::Real;
         const int kw = q.k + dk;
         if (kw < 0 || kw >= q.A.nz - 1) return q.vhz(di, dj, dk);
+        if (q.A.zfv) return Re(q.A.zfv[q.at(di, dj, kw)]);
         int s;
         const double *w = zw(q.A.ochannel ? ZO_I_DN : ZO_I_DD, kw, s);
         Re acc = Re(0.0);
@@ -292,6 +299,7 @@ struct Interp {
         using Re = typename D::Real;
         const int kw = q.k + dk;
         if (kw < 0 || kw >= q.A.nz - 1) return q.thz(dk);
+        if (q.A.zft) return Re(q.A.zft[q.at(0, 0, kw)]);
         int s;
         const double *w = zw(q.A.tbc_top ? ZO_I_DN : ZO_I_DD, kw, s);
         Re acc = Re(0.0);
@@ -306,6 +314,7 @@ struct Interp {
         const D &q = d();
         using Re = typename D::Real;
         if (kw < 0) return Re(-q.A.u_mrf);
+        if (q.A.zfu) return Re(q.A.zfu[q.at(0, 0, kw)]);
         int s;
         const double *w = zw(q.A.ochannel ? ZO_I_DN : ZO_I_DD, kw, s);
         Re acc = Re(0.0);
@@ -318,6 +327,7 @@ struct Interp {
         const D &q = d();
         using Re = typename D::Real;
         if (kw < 0) return Re(0.0);
+        if (q.A.zfv) return Re(q.A.zfv[q.at(0, 0, kw)]);
         int s;
         const double *w = zw(q.A.ochannel ? ZO_I_DN : ZO_I_DD, kw, s);
         Re acc = Re(0.0);
@@ -340,6 +350,7 @@ struct Interp {
     HC_DEV auto whz6(int c) const
     {
         using Re = typename D::Real;
+        if (d().A.zcw) return Re(d().A.zcw[d().at(0, 0, c)]);
         int s;
         const double *w = zw(ZO_IFC, c, s);
         Re acc = Re(0.0);

@@ -18,6 +18,23 @@
 
 constexpr int kZ6Iters = 20;
 
+// The 6th-order interpolants of the substep's fields, once per point, for
+// the right-hand-side kernels: u, v at the w face of level k (the top wall
+// face included), w at the centre of level k, t at the interior w face k.
+// A carries no interpolant buffers, so the accessor evaluates them.
+__global__ __launch_bounds__(256) void k_zinterp6(StepArgs A, double *fu, double *fv, double *cw, double *ft)
+{
+    const size_t n = (size_t)A.nx * A.ny * A.nz, plane = (size_t)A.nx * A.ny;
+    for (size_t q = blockIdx.x * (size_t)blockDim.x + threadIdx.x; q < n; q += (size_t)gridDim.x * blockDim.x) {
+        const int k = (int)(q / plane), j = (int)((q - (size_t)k * plane) / A.nx), i = (int)(q % A.nx);
+        const Pt g(A, i, j, k);
+        fu[q] = g.uf6(k);
+        fv[q] = g.vf6(k);
+        cw[q] = g.whz6(k);
+        if (ft) ft[q] = k < A.nz - 1 ? g.thz6(0) : 0.0;
+    }
+}
+
 // y = Z x for the ZOp `op` on this wavefront's column.  Data index d of
 // the table maps to column level d - off; levels outside 0..ndata-1 are
 // zero (the homogeneous wall data).  Rows k >= nrows give y = 0.
