@@ -31,7 +31,9 @@ ABI_SYMBOLS = ("hc_last_error", "hc_create", "hc_destroy", "hc_set_field", "hc_g
                "hc_field_device_ptr", "hc_synchronize", "hc_profile_enable", "hc_profile_read",
                "hc_ensemble_create", "hc_ensemble_steps", "hc_ensemble_graph_used", "hc_ensemble_destroy",
                "hc_substep_fd", "hc_set_adjoint", "hc_get_adjoint", "hc_adjoint_substep_fd",
-               "hc_wall_shear_gradient", "hc_autotune", "hc_set_tuning")
+               "hc_wall_shear_gradient", "hc_autotune", "hc_set_tuning", "hc_mean_fields", "hc_get_mean_field")
+# statistics of calculate_mean_fields, in the order of output_mean_fields
+MEANS = ("u", "v", "w", "t", "uu", "vv", "ww", "uw", "vw", "tt", "tw")
 KERNELS = {"rhs": 0, "yfft_fwd": 1, "column": 2, "yfft_inv": 3, "finish": 4, "rhs_t": 5,
            "yfft_t": 6, "column_t": 7, "finish_t": 8, "solve_xr2c": 9, "solve_yfft": 10,
            "solve_column": 11, "solve_xc2r": 12}
@@ -82,6 +84,8 @@ def load_library(path: str = LIB_PATH):
     L.hc_substep_fd.argtypes = [vp, ip]
     L.hc_autotune.argtypes = [vp, ip]
     L.hc_set_tuning.argtypes = [vp, ip, ip]
+    L.hc_mean_fields.argtypes = [vp, ip]
+    L.hc_get_mean_field.argtypes = [vp, ip, vp, ip]
     L.hc_set_adjoint.argtypes = [vp, ip, vp, ip]
     L.hc_get_adjoint.argtypes = [vp, ip, vp, ip]
     L.hc_adjoint_substep_fd.argtypes = [vp, ip]
@@ -142,6 +146,17 @@ class Channel:
         shape = self.shape if layout == LAYOUT_INTERIOR else self.ghost_shape
         a = np.zeros(shape) if out is None else out
         self._check(self.L.hc_get_field(self.h, FIELDS[name], _ptr(a), layout))
+        return a
+
+    # ---- mean statistics (calculate_mean_fields / output_mean_fields)
+    def mean_fields(self, on=True):
+        """Zero the running sums and add every following step (on), or stop adding."""
+        self._check(self.L.hc_mean_fields(self.h, 1 if on else 0))
+
+    def get_mean(self, name, layout=LAYOUT_INTERIOR):
+        shape = self.shape if layout == LAYOUT_INTERIOR else self.ghost_shape
+        a = np.zeros(shape)
+        self._check(self.L.hc_get_mean_field(self.h, MEANS.index(name), _ptr(a), layout))
         return a
 
     def set_state(self, **fields):
@@ -240,7 +255,8 @@ class Channel:
     def scalar(self, name):
         v = C.c_double()
         ids = {"dpx_drive": 0, "ubar": 1, "time_sim": 2, "got": 3, "stps": 4, "col_tiles": 5,
-               "tune_rgroup": 6, "tune_ytile": 7, "tune_c2r_pts": 8, "tune_col_remap": 9}
+               "tune_rgroup": 6, "tune_ytile": 7, "tune_c2r_pts": 8, "tune_col_remap": 9,
+               "mean_steps": 10}
         self._check(self.L.hc_get_scalar(self.h, ids[name], C.byref(v)))
         return v.value
 

@@ -1197,6 +1197,57 @@ __global__ __launch_bounds__(256) void k_div_partial(StepArgsT<R> F, StepArgs D,
     }
 }
 
+// ---- calculate_mean_fields (src/module_tools.f90:362-404), FD, and
+// calculate_mean_fields_spec (src/module_spectral.f90:1012-1052).
+// The FD sums take u and v from the x / y interpolants uhx, vhy of the
+// fields at the start of the step's last substep (get_interp_fields_uvwhxyz
+// at src/module_navier_stokes.f90:274 is the last to set them before the
+// sums at :358-361); Acc: PtT, or PtD in a damped substep.
+template <int CDS, class Acc>
+__global__ __launch_bounds__(256) void k_mean_interp(StepArgsT<typename Acc::Real> A, double *uhx, double *vhy)
+{
+    const size_t n = (size_t)A.nx * A.ny * A.nz, plane = (size_t)A.nx * A.ny;
+    for (size_t q = blockIdx.x * (size_t)blockDim.x + threadIdx.x; q < n; q += (size_t)gridDim.x * blockDim.x) {
+        const int k = (int)(q / plane), j = (int)((q - (size_t)k * plane) / A.nx), i = (int)(q % A.nx);
+        const Acc g(A, i, j, k);
+        uhx[q] = (double)g.template uhx<CDS>(0, 0, 0);
+        vhy[q] = (double)g.template vhy<CDS>(0, 0, 0);
+    }
+}
+// Running sums of u + u_mrf, v, w, t, (u + u_mrf)^2, v^2, w^2, w_c (uh + u_mrf),
+// w_c vh, t^2, t w_c with w_c = (w(k) + w(k-1))/2, w(-1) = 0; uh, vh: the
+// interpolants (FD) or u, v (spectral, uhx = vhy = null).  FP64 sums.
+struct MeanSums {
+    double *m[HC_MEAN_COUNT];
+};
+template <class R>
+__global__ __launch_bounds__(256) void k_mean_accum(const R *u, const R *v, const R *w, const R *t, const double *uhx,
+                                                    const double *vhy, MeanSums S, int nx, int ny, int nz, double um)
+{
+    const size_t n = (size_t)nx * ny * nz, plane = (size_t)nx * ny;
+    for (size_t q = blockIdx.x * (size_t)blockDim.x + threadIdx.x; q < n; q += (size_t)gridDim.x * blockDim.x) {
+        const double uu = (double)u[q], vv = (double)v[q], ww = (double)w[q], tt = (double)t[q];
+        const double wc = 0.5 * (ww + (q >= plane ? (double)w[q - plane] : 0.0));
+        const double uh = uhx ? uhx[q] : uu, vh = vhy ? vhy[q] : vv;
+        S.m[HC_MEAN_UU][q] += (uu + um) * (uu + um);
+        S.m[HC_MEAN_VV][q] += vv * vv;
+        S.m[HC_MEAN_WW][q] += ww * ww;
+        S.m[HC_MEAN_UW][q] += wc * (uh + um);
+        S.m[HC_MEAN_VW][q] += wc * vh;
+        S.m[HC_MEAN_TT][q] += tt * tt;
+        S.m[HC_MEAN_TW][q] += tt * wc;
+        S.m[HC_MEAN_U][q] += uu + um;
+        S.m[HC_MEAN_V][q] += vv;
+        S.m[HC_MEAN_W][q] += ww;
+        S.m[HC_MEAN_T][q] += tt;
+    }
+}
+__global__ __launch_bounds__(256) void k_scale_copy(const double *src, double *dst, size_t n, double s)
+{
+    for (size_t q = blockIdx.x * (size_t)blockDim.x + threadIdx.x; q < n; q += (size_t)gridDim.x * blockDim.x)
+        dst[q] = src[q] * s;
+}
+
 // interior <-> ghosted (nx+4, ny+4, nz+4) Fortran layout; the interior
 // array is in the stored precision R
 template <class R>
@@ -1298,6 +1349,13 @@ struct hc_ctx {
     int *zst = nullptr;
     double *zwt = nullptr;
     double *zbuf[4] = {nullptr, nullptr, nullptr, nullptr};
+    // calculate_mean_fields: running sums (HC_MEAN_COUNT fields), the x / y
+    // interpolants of u, v taken at the start of a step's last substep (FD),
+    // and the number of steps summed
+    double *mean[HC_MEAN_COUNT] = {};
+    double *mean_uhx = nullptr, *mean_vhy = nullptr;
+    bool mean_on = false;
+    long mean_steps = 0;
     std::vector<int> h_zst;
     std::vector<double> h_zwt;
     double *dmp_s, *dmp_m;  // Rayleigh damping: sigz per level, plane means per (field, level)
@@ -1977,6 +2035,21 @@ static int substep_t(hc_ctx *c, int t_its, const View<R> &V)
 #undef HC_RHS
     }
     PROF_END(c, HC_K_RHS);
+    if (c->mean_on && t_its == (P.dts == 2 ? 3 : 1)) {
+        // the interpolants the statistics of this step take: the fields at the
+        // start of the last substep (damped as the stencils read them)
+        const dim3 gm(1024), bm(256);
+        if constexpr (std::is_same<R, double>::value) {
+            if (damp) {
+                if (P.cds == 1) hipLaunchKernelGGL((k_mean_interp<1, PtD>), gm, bm, 0, c->stream, B, c->mean_uhx, c->mean_vhy);
+                else hipLaunchKernelGGL((k_mean_interp<2, PtD>), gm, bm, 0, c->stream, B, c->mean_uhx, c->mean_vhy);
+            }
+        }
+        if (!damp) {
+            if (P.cds == 1) hipLaunchKernelGGL((k_mean_interp<1, PtT<R>>), gm, bm, 0, c->stream, B, c->mean_uhx, c->mean_vhy);
+            else hipLaunchKernelGGL((k_mean_interp<2, PtT<R>>), gm, bm, 0, c->stream, B, c->mean_uhx, c->mean_vhy);
+        }
+    }
     C *const *S = V.s;
     PROF_BEGIN(c, HC_K_YFFT_FWD);
     if (int e = launch_yfft(c, -1, 3, S[0], S[1], S[2], S[3])) return e;
@@ -2617,6 +2690,26 @@ extern "C" int hc_init(hc_ctx *c)
     return 0;
 }
 
+// calculate_mean_fields at the end of a step (src/module_navier_stokes.f90:358-361, 510-513)
+static int mean_accumulate(hc_ctx *c)
+{
+    if (!c->mean_on) return 0;
+    MeanSums S;
+    for (int q = 0; q < HC_MEAN_COUNT; ++q) S.m[q] = c->mean[q];
+    const double *uh = c->prm.cds == 3 ? nullptr : c->mean_uhx, *vh = c->prm.cds == 3 ? nullptr : c->mean_vhy;
+    if (c->mixed) {
+        const View<float> &M = c->m32;
+        hipLaunchKernelGGL(k_mean_accum<float>, dim3(2048), dim3(256), 0, c->stream, M.u, M.v, M.w, M.t, uh, vh, S,
+                           c->nx, c->ny, c->nz, c->prm.u_mrf);
+    } else {
+        hipLaunchKernelGGL(k_mean_accum<double>, dim3(2048), dim3(256), 0, c->stream, c->u, c->v, c->w, c->t, uh, vh,
+                           S, c->nx, c->ny, c->nz, c->prm.u_mrf);
+    }
+    c->mean_steps += 1;
+    HIPCHK(hipGetLastError());
+    return 0;
+}
+
 static int one_step(hc_ctx *c)
 {
     const hc_params &P = c->prm;
@@ -2626,7 +2719,7 @@ static int one_step(hc_ctx *c)
     int nsub = P.dts == 2 ? 3 : 1;
     for (int s = 1; s <= nsub; ++s)
         if (int e = P.cds == 3 ? substep_sp(c, s) : substep(c, s)) return e;
-    return 0;
+    return mean_accumulate(c);
 }
 
 extern "C" int hc_time_advancement_fd(hc_ctx *c)
@@ -2636,7 +2729,7 @@ extern "C" int hc_time_advancement_fd(hc_ctx *c)
     int nsub = c->prm.dts == 2 ? 3 : 1;
     for (int s = 1; s <= nsub; ++s)
         if (int e = substep(c, s)) return e;
-    return 0;
+    return mean_accumulate(c);
 }
 
 extern "C" int hc_time_advancement_sp(hc_ctx *c)
@@ -2646,7 +2739,7 @@ extern "C" int hc_time_advancement_sp(hc_ctx *c)
     int nsub = c->prm.dts == 2 ? 3 : 1;
     for (int s = 1; s <= nsub; ++s)
         if (int e = substep_sp(c, s)) return e;
-    return 0;
+    return mean_accumulate(c);
 }
 
 extern "C" int hc_drive_steps(hc_ctx *c, int nsteps)
@@ -2658,6 +2751,43 @@ extern "C" int hc_drive_steps(hc_ctx *c, int nsteps)
             if (int e = ubar_launch(c)) return e;
     }
     HIPCHK(hipGetLastError());
+    return 0;
+}
+
+extern "C" int hc_mean_fields(hc_ctx *c, int on)
+{
+    if (!c) return set_err(HC_ERR_ARG, "null context");
+    if (on != 0 && on != 1) return set_err(HC_ERR_ARG, "on must be 0 or 1");
+    if (on) {
+        const size_t N = (size_t)c->nx * c->ny * c->nz;
+        if (!c->mean[0]) {
+            for (int q = 0; q < HC_MEAN_COUNT; ++q)
+                if (int e = dalloc(c, (void **)&c->mean[q], N * sizeof(double))) return e;
+            if (c->prm.cds != 3) {
+                if (int e = dalloc(c, (void **)&c->mean_uhx, N * sizeof(double))) return e;
+                if (int e = dalloc(c, (void **)&c->mean_vhy, N * sizeof(double))) return e;
+            }
+        } else {
+            for (int q = 0; q < HC_MEAN_COUNT; ++q) HIPCHK(hipMemsetAsync(c->mean[q], 0, N * sizeof(double), c->stream));
+        }
+        c->mean_steps = 0;
+    }
+    c->mean_on = on == 1;
+    HIPCHK(hipStreamSynchronize(c->stream));
+    return 0;
+}
+
+extern "C" int hc_get_mean_field(hc_ctx *c, int stat, double *dst, int layout)
+{
+    if (!c || !dst) return set_err(HC_ERR_ARG, "null argument");
+    if (stat < 0 || stat >= HC_MEAN_COUNT) return set_err(HC_ERR_ARG, "statistic %d", stat);
+    if (!c->mean[0] || c->mean_steps == 0) return set_err(HC_ERR_ARG, "no statistics accumulated (hc_mean_fields)");
+    const size_t N = (size_t)c->nx * c->ny * c->nz;
+    double *tmp = c->sb[c->scur ^ 1];  // the free ss slot
+    hipLaunchKernelGGL(k_scale_copy, dim3(1024), dim3(256), 0, c->stream, c->mean[stat], tmp, N,
+                       1.0 / (double)c->mean_steps);
+    if (int e = get_field_t<double>(c, tmp, dst, layout)) return e;
+    HIPCHK(hipStreamSynchronize(c->stream));
     return 0;
 }
 
@@ -2831,6 +2961,7 @@ extern "C" int hc_get_scalar(hc_ctx *c, int id, double *value)
     case 7: *value = c->tune.ytile; return 0;
     case 8: *value = c->tune.c2r_pts; return 0;
     case 9: *value = c->tune.col_remap; return 0;
+    case 10: *value = (double)c->mean_steps; return 0;
     default: return set_err(HC_ERR_ARG, "unknown scalar %d", id);
     }
 }

@@ -229,6 +229,26 @@ enum hc_kernel {
     HC_K_SOLVE_XC2R = 12,
     HC_KERNEL_COUNT = 13
 };
+/* Mean statistics, calculate_mean_fields (src/module_tools.f90:362-404) and
+ * calculate_mean_fields_spec (src/module_spectral.f90:1012-1052): pointwise
+ * running sums over the steps, in the order of output_mean_fields
+ * (src/module_io.f90:546-598).  u includes u_mrf; w_c = (w(k) + w(k-1))/2.
+ * In the FD path uw and vw take u, v from their x / y interpolants at the
+ * start of the step's last substep, as the reference does. */
+enum hc_mean {
+    HC_MEAN_U = 0, HC_MEAN_V, HC_MEAN_W, HC_MEAN_T, HC_MEAN_UU, HC_MEAN_VV, HC_MEAN_WW, HC_MEAN_UW,
+    HC_MEAN_VW, HC_MEAN_TT, HC_MEAN_TW, HC_MEAN_COUNT
+};
+/* on = 1: zero the sums and add every following step to them (at the end of
+ * hc_time_advancement_fd / _sp, hc_drive_steps and ensemble steps); the
+ * first call allocates 11 fields (13 in the FD path).  on = 0: stop adding,
+ * keep the sums.  hc_get_scalar id 10 returns the number of steps summed. */
+int hc_mean_fields(hc_ctx *ctx, int on);
+/* output_mean_fields: the sum of statistic `stat` divided by the number of
+ * steps summed (the reference divides by the step count stps of its run,
+ * the same number when the statistics start with the run). */
+int hc_get_mean_field(hc_ctx *ctx, int stat, double *dst, int layout);
+
 int hc_profile_enable(hc_ctx *ctx, int on);
 int hc_profile_read(hc_ctx *ctx, int kernel, double *total_ms, long *launches);
 

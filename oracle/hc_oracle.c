@@ -64,6 +64,10 @@ typedef struct ho_state {
     cplx *c_u, *c_v, *c_w, *c_p, *c_t, *c_phi, *c_ff, *c_gg, *c_hh, *c_ss, *c_qu, *c_qv, *c_qw;
     double div_avg, div_max, div_hist_max, time_sim;
     int stps;
+    /* calculate_mean_fields running sums (interior arrays, allocated on
+     * request) and the number of steps they hold */
+    double *mean[11];
+    int mean_steps;
     char err[256];
 } ho_state;
 
@@ -205,6 +209,7 @@ void ho_destroy(ho_state *S)
     cplx *cs[] = {S->c_u, S->c_v, S->c_w, S->c_p, S->c_t, S->c_phi, S->c_ff, S->c_gg, S->c_hh, S->c_ss,
                   S->c_qu, S->c_qv, S->c_qw};
     for (size_t q = 0; q < sizeof(cs) / sizeof(cs[0]); ++q) free(cs[q]);
+    for (int q = 0; q < 11; ++q) free(S->mean[q]);
     free(S);
 }
 
@@ -1388,6 +1393,8 @@ static void assign_rk(ho_state *S, int t_its)
 }
 
 /* ---- A16: time_advancement_fd, src/module_navier_stokes.f90:223-365 ---- */
+static void mean_fields(ho_state *S);
+
 static int time_advancement_fd(ho_state *S)
 {
     const ho_params *P = &S->prm;
@@ -1434,7 +1441,46 @@ static int time_advancement_fd(ho_state *S)
         calc_new_p(S);                                            /* :353 */
         bc_p(S, S->p);                                            /* :354 */
     }
+    mean_fields(S);                                               /* :358-361 */
     return 0;
+}
+
+/* ---- calculate_mean_fields, src/module_tools.f90:362-404 (FD) and
+ * calculate_mean_fields_spec, src/module_spectral.f90:1012-1052: pointwise
+ * running sums of u + u_mrf, v, w, t, their squares, and w at the centre
+ * (0.5 (w(k) + w(k-1))) times u, v, t.  The FD form takes u and v from the
+ * x / y interpolants uhx, vhy, which still hold the values interpolated at
+ * the start of the step's last substep (get_interp_fields_uvwhxyz, :274);
+ * the spectral form takes u and v themselves.  Order of the sums:
+ * u, v, w, t, uu, vv, ww, uw, vw, tt, tw. */
+static void mean_fields(ho_state *S)
+{
+    if (!S->mean[0]) return;
+    const ho_params *P = &S->prm;
+    const double um = P->u_mrf;
+    const int nx = P->nx, ny = P->ny;
+    for (int k = S->kstr3; k <= S->kend3; ++k)
+        for (int j = S->jstr3; j <= S->jend3; ++j)
+            for (int i = S->istr3; i <= S->iend3; ++i) {
+                const size_t q = ((size_t)(k - S->kstr3) * ny + (j - S->jstr3)) * nx + (i - S->istr3);
+                const double u = F3(S, S->u, i, j, k), v = F3(S, S->v, i, j, k), w = F3(S, S->w, i, j, k);
+                const double t = F3(S, S->t, i, j, k);
+                const double wc = 0.5 * (w + F3(S, S->w, i, j, k - 1));
+                const double uh = P->cds == 3 ? u : F3(S, S->uhx, i, j, k);
+                const double vh = P->cds == 3 ? v : F3(S, S->vhy, i, j, k);
+                S->mean[4][q] += (u + um) * (u + um);
+                S->mean[5][q] += v * v;
+                S->mean[6][q] += w * w;
+                S->mean[7][q] += wc * (uh + um);
+                S->mean[8][q] += wc * vh;
+                S->mean[9][q] += t * t;
+                S->mean[10][q] += t * wc;
+                S->mean[0][q] += u + um;
+                S->mean[1][q] += v;
+                S->mean[2][q] += w;
+                S->mean[3][q] += t;
+            }
+    S->mean_steps += 1;
 }
 
 /* ---- A18: time_advancement_sp, src/module_navier_stokes.f90:371-517 ---- */
@@ -1487,6 +1533,7 @@ static int time_advancement_sp(ho_state *S)
         }
         if (div_error_spec(S, 1)) return 1;                               /* :501-507 */
     }
+    mean_fields(S);                                                       /* :510-513 */
     return 0;
 }
 
@@ -1538,6 +1585,32 @@ int ho_steps(ho_state *S, int nsteps)
         } else if (time_advancement_fd(S)) return 1;
         if (S->prm.dp_opt == 2) ho_calc_ubar(S);
     }
+    return 0;
+}
+
+/* statistics from now on: zeroed running sums of calculate_mean_fields */
+int ho_mean_enable(ho_state *S)
+{
+    const size_t n = (size_t)S->prm.nx * S->prm.ny * S->prm.nz;
+    for (int q = 0; q < 11; ++q) {
+        if (!S->mean[q]) S->mean[q] = dalloc(n);
+        if (!S->mean[q]) { snprintf(S->err, sizeof S->err, "out of memory"); return 1; }
+        memset(S->mean[q], 0, n * sizeof(double));
+    }
+    S->mean_steps = 0;
+    return 0;
+}
+
+/* output_mean_fields (src/module_io.f90:546-598): the running sum divided
+ * by the number of steps it holds */
+int ho_get_mean(ho_state *S, int id, double *dst)
+{
+    if (id < 0 || id >= 11 || !S->mean[id] || S->mean_steps == 0) {
+        snprintf(S->err, sizeof S->err, "no statistics %d", id);
+        return 1;
+    }
+    const size_t n = (size_t)S->prm.nx * S->prm.ny * S->prm.nz;
+    for (size_t q = 0; q < n; ++q) dst[q] = S->mean[id][q] / S->mean_steps;
     return 0;
 }
 

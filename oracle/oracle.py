@@ -80,6 +80,8 @@ def lib():
         L.ho_stage_temperature_spec.argtypes = [vp, ip]
         L.ho_spec_divergence.argtypes = [vp, dp]
         L.ho_get_scalar.restype = C.c_double
+        L.ho_mean_enable.argtypes = [vp]
+        L.ho_get_mean.argtypes = [vp, ip, dp]
         L.ho_get_scalar.argtypes = [vp, ip]
         L.ho_set_scalar.argtypes = [vp, ip, C.c_double]
         _lib = L
@@ -149,6 +151,17 @@ class Oracle:
 
     def init(self):
         self._check(lib().ho_init(self.h))
+
+    # calculate_mean_fields statistics, in the order of output_mean_fields
+    MEANS = ("u", "v", "w", "t", "uu", "vv", "ww", "uw", "vw", "tt", "tw")
+
+    def mean_enable(self):
+        self._check(lib().ho_mean_enable(self.h))
+
+    def get_mean(self, name):
+        a = np.empty(self.shape, dtype=np.float64)
+        self._check(lib().ho_get_mean(self.h, self.MEANS.index(name), _dptr(a)))
+        return a
 
     def steps(self, n):
         self._check(lib().ho_steps(self.h, int(n)))
