@@ -7,8 +7,10 @@
 ! the 2DECOMP decomp_2d_write_one format read at src/module_tools.f90:273-286),
 ! runs imax steps of the drive loop (src/module_navier_stokes.f90:139-210)
 ! on the GPU and writes results/bak_{u,v,w,p,t}.dat every ibackup steps
-! (src/module_io.f90:311-360).  Time series, mean fields and slices of the
-! reference are not produced.
+! (src/module_io.f90:311-360) and the mean fields results/mean_*.dat every
+! imeanfl steps (calculate_mean_fields summed on the GPU from the first
+! step, written as output_mean_fields does, src/module_io.f90:546-598).
+! Time series and slices of the reference are not produced.
 !*************************************************************************
 program hercules_hip
 
@@ -36,7 +38,9 @@ program hercules_hip
     type(hc_params) :: prm
     real(c_double), allocatable, target :: f(:,:,:), g(:,:,:)
     real(c_double) :: div_avg, div_max
-    integer :: stps, indx_bk, nsteps_env
+    integer :: stps, indx_bk, indx_m, nsteps_env, q
+    character(len=2), dimension(11), parameter :: mean_names = &
+        (/ 'u ', 'v ', 'w ', 't ', 'uu', 'vv', 'ww', 'uw', 'vw', 'tt', 'tw' /)
     character(len=32) :: arg
 
     open(10, file='parameters.input')
@@ -83,8 +87,10 @@ program hercules_hip
         call read_one('results/init_t.dat', 0.d0, HC_T)
     endif
     call hip_init()
+    call hip_check(hc_mean_fields(hip_ctx, 1_c_int), 'hc_mean_fields')
 
     indx_bk = 1
+    indx_m = 1
     do stps = 1, nsteps_env
         call hip_check(hc_drive_steps(hip_ctx, 1_c_int), 'drive step')
         call hip_check(hc_screen_div_error(hip_ctx, div_avg, div_max), 'screen_div_error')
@@ -99,6 +105,14 @@ program hercules_hip
             indx_bk = 0
         endif
         indx_bk = indx_bk + 1
+        if (indx_m / imeanfl .eq. 1) then
+            print *, 'Writing Mean Fields...'
+            do q = 1, HC_MEAN_COUNT
+                call write_mean('results/mean_' // trim(mean_names(q)) // '.dat', q - 1)
+            enddo
+            indx_m = 0
+        endif
+        indx_m = indx_m + 1
     enddo
 
     call hip_destroy()
@@ -125,6 +139,17 @@ contains
         integer :: u_
         call hip_download(field, f)
         g = f(3:nx + 2, 3:ny + 2, 3:nz + 2) + shift
+        open(newunit=u_, file=fname, access='stream', form='unformatted', status='replace')
+        write(u_) g
+        close(u_)
+    end subroutine
+
+    subroutine write_mean(fname, stat)
+        character(len=*), intent(in) :: fname
+        integer, intent(in) :: stat
+        integer :: u_
+        call hip_check(hc_get_mean_field(hip_ctx, int(stat, c_int), c_loc(g), HC_LAYOUT_INTERIOR), &
+                       'hc_get_mean_field')
         open(newunit=u_, file=fname, access='stream', form='unformatted', status='replace')
         write(u_) g
         close(u_)

@@ -32,6 +32,8 @@ module module_hercules_hip
     end type
 
     integer(c_int), parameter :: HC_U = 0, HC_V = 1, HC_W = 2, HC_P = 3, HC_T = 4
+    ! statistics of calculate_mean_fields, in the order of output_mean_fields
+    integer(c_int), parameter :: HC_MEAN_COUNT = 11
     integer(c_int), parameter :: HC_LAYOUT_INTERIOR = 0, HC_LAYOUT_GHOSTED = 1
     integer(c_int), parameter :: HC_EQ_U = 0, HC_EQ_V = 1, HC_EQ_W = 2, HC_EQ_P = 3, HC_EQ_T = 4
 
@@ -96,6 +98,18 @@ module module_hercules_hip
             import :: c_ptr, c_int, c_double
             type(c_ptr), value :: ctx
             real(c_double), intent(out) :: ubar
+            integer(c_int) :: rc
+        end function
+        function hc_mean_fields(ctx, on) bind(C, name="hc_mean_fields") result(rc)
+            import :: c_ptr, c_int
+            type(c_ptr), value :: ctx
+            integer(c_int), value :: on
+            integer(c_int) :: rc
+        end function
+        function hc_get_mean_field(ctx, stat, dst, layout) bind(C, name="hc_get_mean_field") result(rc)
+            import :: c_ptr, c_int
+            type(c_ptr), value :: ctx, dst
+            integer(c_int), value :: stat, layout
             integer(c_int) :: rc
         end function
         function hc_screen_div_error(ctx, div_avg, div_max) bind(C, name="hc_screen_div_error") result(rc)

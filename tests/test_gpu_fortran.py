@@ -1,6 +1,6 @@
 """The Fortran 90 host (hercules_amd/fortran) drives the HIP library through
 ISO_C_BINDING: parameters.input and results/init_*.dat in, results/bak_*.dat
-out, compared with the CPU oracle (needs an MI355X)."""
+and results/mean_*.dat out, compared with the CPU oracle (needs an MI355X)."""
 import os
 import subprocess
 
@@ -27,7 +27,7 @@ DECK = """&domain
   tbot = -0.5, ttop = 0.5, is_ri_var = 0, ri_str = 0.0, ri_end = 0.0
 /
 &io
-  restart = 1, istmsr = 0, ibackup = {imax}, iinstfl = 100000, imeanfl = 100000
+  restart = 1, istmsr = 0, ibackup = {imax}, iinstfl = 100000, imeanfl = {imax}
   isxy2d = 0, xy2d_id = 10, isxz2d = 0, xz2d_id = 10, isyz2d = 0, yz2d_id = 10, intv_2d = 1000
 /
 """
@@ -56,7 +56,14 @@ def test_fortran_driver_matches_oracle(tmp_path, cds):
     print(r.stdout[-2000:], r.stderr[-2000:])
     assert r.returncode == 0
     o = make_oracle(prm, st)
+    o.mean_enable()
     o.steps(prm.imax)
+    # output_mean_fields at step imax: the sums over imax steps / imax
+    for k in ("u", "uu", "uw", "vw", "ww"):
+        got = np.fromfile(tmp_path / "results" / f"mean_{k}.dat").reshape(prm.nz, prm.ny, prm.nx)
+        err = rel_l2(got, o.get_mean(k))
+        print("mean", k, f"{err:.2e}")
+        assert err < 1e-10, (k, err)
     for k in ("u", "v", "w", "p"):
         got = np.fromfile(tmp_path / "results" / f"bak_{k}.dat").reshape(prm.nz, prm.ny, prm.nx)
         err = rel_l2(got, o.get(k))
