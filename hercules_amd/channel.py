@@ -31,7 +31,8 @@ ABI_SYMBOLS = ("hc_last_error", "hc_create", "hc_destroy", "hc_set_field", "hc_g
                "hc_field_device_ptr", "hc_synchronize", "hc_profile_enable", "hc_profile_read",
                "hc_ensemble_create", "hc_ensemble_steps", "hc_ensemble_graph_used", "hc_ensemble_destroy",
                "hc_substep_fd", "hc_set_adjoint", "hc_get_adjoint", "hc_adjoint_substep_fd",
-               "hc_wall_shear_gradient", "hc_autotune", "hc_set_tuning", "hc_mean_fields", "hc_get_mean_field")
+               "hc_wall_shear_gradient", "hc_autotune", "hc_set_tuning", "hc_mean_fields", "hc_get_mean_field",
+               "hc_get_mesh")
 # statistics of calculate_mean_fields, in the order of output_mean_fields
 MEANS = ("u", "v", "w", "t", "uu", "vv", "ww", "uw", "vw", "tt", "tw")
 KERNELS = {"rhs": 0, "yfft_fwd": 1, "column": 2, "yfft_inv": 3, "finish": 4, "rhs_t": 5,
@@ -86,6 +87,7 @@ def load_library(path: str = LIB_PATH):
     L.hc_set_tuning.argtypes = [vp, ip, ip]
     L.hc_mean_fields.argtypes = [vp, ip]
     L.hc_get_mean_field.argtypes = [vp, ip, vp, ip]
+    L.hc_get_mesh.argtypes = [vp, vp, vp, vp, vp]
     L.hc_set_adjoint.argtypes = [vp, ip, vp, ip]
     L.hc_get_adjoint.argtypes = [vp, ip, vp, ip]
     L.hc_adjoint_substep_fd.argtypes = [vp, ip]
@@ -147,6 +149,12 @@ class Channel:
         a = np.zeros(shape) if out is None else out
         self._check(self.L.hc_get_field(self.h, FIELDS[name], _ptr(a), layout))
         return a
+
+    def mesh(self):
+        """(zc, dz, l_t, l_b) of the nz levels, the columns of results/geo.info."""
+        out = [np.zeros(self.prm.nz) for _ in range(4)]
+        self._check(self.L.hc_get_mesh(self.h, *(_ptr(a) for a in out)))
+        return tuple(out)
 
     # ---- mean statistics (calculate_mean_fields / output_mean_fields)
     def mean_fields(self, on=True):

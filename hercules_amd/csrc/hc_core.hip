@@ -1457,6 +1457,18 @@ static void mesh_1based(const hc_params &P, std::vector<double> &dz, std::vector
     }
 }
 
+// The faces z_f(0..nz) of get_mesh_param (src/module_parameters.f90:445-466).
+static void mesh_faces(const hc_params &P, std::vector<double> &zf)
+{
+    const int nz = P.nz;
+    zf.assign(nz + 1, 0.0);
+    double zmin = 0.0, zmax = P.lz, zmean, a;
+    if (P.ochannel == 0) { zmean = 0.5 * (zmin + zmax); a = (2.0 + nz) * 0.5; }
+    else { zmean = zmin + zmax; a = (1.0 + nz) * 1.0; }
+    for (int k = 1; k <= nz + 1; ++k)
+        zf[k - 1] = zmean + (zmin - zmean) / std::tanh(2.0 * P.zstretch) * std::tanh(2.0 * P.zstretch * (k - a) / (1.0 - a));
+}
+
 // Per-level tables: metrics and the 1-D coefficients of get_{u,w,p,t}_eqn_coeff
 // (src/module_poisson_solver.f90:32-477) with their wall rows, for general
 // modes and for the (0,0) mode.
@@ -1564,16 +1576,10 @@ static bool build_zops(hc_ctx *c)
 {
     const hc_params &P = c->prm;
     const int nz = P.nz;
-    // faces z_f(0..nz) of get_mesh_param (src/module_parameters.f90:445-466)
-    std::vector<double> zf(nz + 1), zc(nz);
-    {
-        double zmin = 0.0, zmax = P.lz, zmean, a;
-        if (P.ochannel == 0) { zmean = 0.5 * (zmin + zmax); a = (2.0 + nz) * 0.5; }
-        else { zmean = zmin + zmax; a = (1.0 + nz) * 1.0; }
-        for (int k = 1; k <= nz + 1; ++k)
-            zf[k - 1] = zmean + (zmin - zmean) / std::tanh(2.0 * P.zstretch) * std::tanh(2.0 * P.zstretch * (k - a) / (1.0 - a));
-        for (int k = 0; k < nz; ++k) zc[k] = 0.5 * (zf[k] + zf[k + 1]);
-    }
+    std::vector<double> zf, zc;
+    mesh_faces(P, zf);
+    zc.resize(nz);
+    for (int k = 0; k < nz; ++k) zc[k] = 0.5 * (zf[k] + zf[k + 1]);
     c->h_zst.assign((size_t)ZO_COUNT * nz, 0);
     c->h_zwt.assign((size_t)ZO_COUNT * nz * kZW, 0.0);
     // data sets: C (wall datum, centres, wall datum), F (faces), P (centres)
@@ -2751,6 +2757,20 @@ extern "C" int hc_drive_steps(hc_ctx *c, int nsteps)
             if (int e = ubar_launch(c)) return e;
     }
     HIPCHK(hipGetLastError());
+    return 0;
+}
+
+extern "C" int hc_get_mesh(hc_ctx *c, double *zc, double *dz, double *l_t, double *l_b)
+{
+    if (!c || !zc || !dz || !l_t || !l_b) return set_err(HC_ERR_ARG, "null argument");
+    std::vector<double> zf;
+    mesh_faces(c->prm, zf);
+    for (int k = 0; k < c->nz; ++k) {
+        zc[k] = 0.5 * (zf[k + 1] + zf[k]);
+        dz[k] = c->h_lev[(size_t)LV_DZ * c->nz + k];
+        l_t[k] = c->h_lev[(size_t)LV_LT * c->nz + k];
+        l_b[k] = c->h_lev[(size_t)LV_LB * c->nz + k];
+    }
     return 0;
 }
 

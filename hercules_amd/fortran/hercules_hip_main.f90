@@ -7,7 +7,10 @@
 ! the 2DECOMP decomp_2d_write_one format read at src/module_tools.f90:273-286),
 ! runs imax steps of the drive loop (src/module_navier_stokes.f90:139-210)
 ! on the GPU and writes results/bak_{u,v,w,p,t}.dat every ibackup steps
-! (src/module_io.f90:311-360) and the mean fields results/mean_*.dat every
+! (src/module_io.f90:311-360; with AB2, dts = 1, also the right-hand-side
+! history bak_{ff,gg,hh,ss}.dat, which a restart reads back from
+! init_{ff,gg,hh,ss}.dat, src/module_tools.f90:292-307), the grid
+! results/geo.info (src/module_parameters.f90:495-504) and the mean fields results/mean_*.dat every
 ! imeanfl steps (calculate_mean_fields summed on the GPU from the first
 ! step, written as output_mean_fields does, src/module_io.f90:546-598).
 ! Time series and slices of the reference are not produced.
@@ -37,6 +40,7 @@ program hercules_hip
 
     type(hc_params) :: prm
     real(c_double), allocatable, target :: f(:,:,:), g(:,:,:)
+    real(c_double), allocatable, target :: zc(:), dzk(:), l_t(:), l_b(:)
     real(c_double) :: div_avg, div_max
     integer :: stps, indx_bk, indx_m, nsteps_env, q
     character(len=2), dimension(11), parameter :: mean_names = &
@@ -85,8 +89,25 @@ program hercules_hip
         call read_one('results/init_w.dat', 0.d0, HC_W)
         call read_one('results/init_p.dat', 0.d0, HC_P)
         call read_one('results/init_t.dat', 0.d0, HC_T)
+        ! for RK3, the f, g, h, s of previous steps are not needed
+        if (dts .ne. 2) then
+            call read_one('results/init_ff.dat', 0.d0, HC_FF)
+            call read_one('results/init_gg.dat', 0.d0, HC_GG)
+            call read_one('results/init_hh.dat', 0.d0, HC_HH)
+            call read_one('results/init_ss.dat', 0.d0, HC_SS)
+        endif
     endif
     call hip_init()
+
+    ! the grid, as get_mesh_param writes it
+    allocate(zc(nz), dzk(nz), l_t(nz), l_b(nz))
+    call hip_check(hc_get_mesh(hip_ctx, c_loc(zc), c_loc(dzk), c_loc(l_t), c_loc(l_b)), 'hc_get_mesh')
+    write(*,*) 'Writing Geo Info to Files.'
+    open(11, file='results/geo.info')
+    do q = 1, nz
+        write(11, '(4e14.6)') zc(q), dzk(q), l_t(q), l_b(q)
+    enddo
+    close(11)
     call hip_check(hc_mean_fields(hip_ctx, 1_c_int), 'hc_mean_fields')
 
     indx_bk = 1
@@ -102,6 +123,12 @@ program hercules_hip
             call write_one('results/bak_w.dat', 0.d0, HC_W)
             call write_one('results/bak_p.dat', 0.d0, HC_P)
             call write_one('results/bak_t.dat', 0.d0, HC_T)
+            if (dts .ne. 2) then
+                call write_one('results/bak_ff.dat', 0.d0, HC_FF)
+                call write_one('results/bak_gg.dat', 0.d0, HC_GG)
+                call write_one('results/bak_hh.dat', 0.d0, HC_HH)
+                call write_one('results/bak_ss.dat', 0.d0, HC_SS)
+            endif
             indx_bk = 0
         endif
         indx_bk = indx_bk + 1

@@ -32,6 +32,7 @@ module module_hercules_hip
     end type
 
     integer(c_int), parameter :: HC_U = 0, HC_V = 1, HC_W = 2, HC_P = 3, HC_T = 4
+    integer(c_int), parameter :: HC_FF = 5, HC_GG = 6, HC_HH = 7, HC_SS = 8
     ! statistics of calculate_mean_fields, in the order of output_mean_fields
     integer(c_int), parameter :: HC_MEAN_COUNT = 11
     integer(c_int), parameter :: HC_LAYOUT_INTERIOR = 0, HC_LAYOUT_GHOSTED = 1
@@ -98,6 +99,11 @@ module module_hercules_hip
             import :: c_ptr, c_int, c_double
             type(c_ptr), value :: ctx
             real(c_double), intent(out) :: ubar
+            integer(c_int) :: rc
+        end function
+        function hc_get_mesh(ctx, zc, dz, l_t, l_b) bind(C, name="hc_get_mesh") result(rc)
+            import :: c_ptr, c_int
+            type(c_ptr), value :: ctx, zc, dz, l_t, l_b
             integer(c_int) :: rc
         end function
         function hc_mean_fields(ctx, on) bind(C, name="hc_mean_fields") result(rc)

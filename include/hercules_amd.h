@@ -229,6 +229,11 @@ enum hc_kernel {
     HC_K_SOLVE_XC2R = 12,
     HC_KERNEL_COUNT = 13
 };
+/* The wall-normal grid as get_mesh_param writes it to results/geo.info
+ * (src/module_parameters.f90:495-504): cell-centre z, dz, l_t, l_b of the
+ * nz levels (arrays of nz values). */
+int hc_get_mesh(hc_ctx *ctx, double *zc, double *dz, double *l_t, double *l_b);
+
 /* Mean statistics, calculate_mean_fields (src/module_tools.f90:362-404) and
  * calculate_mean_fields_spec (src/module_spectral.f90:1012-1052): pointwise
  * running sums over the steps, in the order of output_mean_fields

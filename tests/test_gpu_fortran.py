@@ -18,7 +18,7 @@ DECK = """&domain
   nx = {nx}, ny = {ny}, nz = {nz}, zstretch = {zs!r}, ochannel = 0
 /
 &modeling
-  dp_opt = 1, cds = {cds}, issk = 1, dts = 2, isnoise = 0, noise_mag = 0.1
+  dp_opt = 1, cds = {cds}, issk = 1, dts = {dts}, isnoise = 0, noise_mag = 0.1
   isdamp = 0, nzdamp = 15, cdamp = 0.01
 /
 &constants
@@ -49,7 +49,7 @@ def test_fortran_driver_matches_oracle(tmp_path, cds):
     (tmp_path / "results").mkdir()
     (tmp_path / "parameters.input").write_text(DECK.format(
         lx=prm.lx, ly=prm.ly, nx=prm.nx, ny=prm.ny, nz=prm.nz, zs=prm.zstretch, nu=prm.nu,
-        alpha=prm.alpha, imax=prm.imax, cds=cds))
+        alpha=prm.alpha, imax=prm.imax, cds=cds, dts=2))
     for k, v in st.items():  # Fortran (i,j,k) order == C order (nz, ny, nx)
         np.ascontiguousarray(v).tofile(tmp_path / "results" / f"init_{k}.dat")
     r = subprocess.run([FEXE], cwd=tmp_path, capture_output=True, text=True, timeout=120)
@@ -69,3 +69,60 @@ def test_fortran_driver_matches_oracle(tmp_path, cds):
         err = rel_l2(got, o.get(k))
         print(f"fortran host {k}: rel L2 {err:.3e}")
         assert err < 1e-10
+
+    # results/geo.info: z, dz, l_t, l_b of each level, format (4e14.6)
+    geo = np.loadtxt(tmp_path / "results" / "geo.info")
+    from hercules_amd import grid
+    m = grid.metrics(prm.nz, prm.lz, prm.zstretch, prm.ochannel)
+    for col, ref in enumerate((m["zc"], m["dz"], m["l_t"], m["l_b"])):
+        assert np.allclose(geo[:, col], ref, rtol=1e-5, atol=1e-12), col
+
+
+def _deck(tmp_path, prm, dts):
+    (tmp_path / "results").mkdir(exist_ok=True)
+    (tmp_path / "parameters.input").write_text(DECK.format(
+        lx=prm.lx, ly=prm.ly, nx=prm.nx, ny=prm.ny, nz=prm.nz, zs=prm.zstretch, nu=prm.nu,
+        alpha=prm.alpha, imax=prm.imax, cds=prm.cds, dts=dts))
+
+
+@pytest.mark.parametrize("dts", [2, 1])
+def test_fortran_restart_is_bitwise(tmp_path, dts):
+    """Resume: 4 steps in one run equal, bit for bit, 2 steps, the backup
+    files read back as init files (src/module_tools.f90:260-316), and 2 more
+    steps.  With AB2 (dts = 1) the backup carries the right-hand-side history
+    ff, gg, hh, ss (src/module_io.f90:340-356)."""
+    if not os.path.exists(FEXE):
+        pytest.fail(f"Fortran host not built: {FEXE}")
+    prm = cases.closed_channel(nx=32, ny=32, nz=33, imax=4, cds=2, issk=1)
+    st = cases.laminar_noise(prm, seed=5)
+    with Channel(prm) as ch:
+        for k, v in st.items():
+            ch.set_field(k, v)
+        ch.project()
+        st = {k: ch.get_field(k) for k in ("u", "v", "w", "p", "t")}
+    runs = {}
+    for name in ("one", "two"):
+        d = tmp_path / name
+        d.mkdir()
+        _deck(d, prm, dts)
+        for k, v in st.items():
+            np.ascontiguousarray(v).tofile(d / "results" / f"init_{k}.dat")
+        if dts != 2:  # an AB2 restart also reads the right-hand-side history
+            for k in ("ff", "gg", "hh", "ss"):
+                np.zeros_like(st["u"]).tofile(d / "results" / f"init_{k}.dat")
+        runs[name] = d
+    r = subprocess.run([FEXE, "4"], cwd=runs["one"], capture_output=True, text=True, timeout=120)
+    assert r.returncode == 0, r.stdout[-2000:] + r.stderr[-2000:]
+    d = runs["two"]
+    r = subprocess.run([FEXE, "2"], cwd=d, capture_output=True, text=True, timeout=120)
+    assert r.returncode == 0, r.stdout[-2000:] + r.stderr[-2000:]
+    names = ["u", "v", "w", "p", "t"] + (["ff", "gg", "hh", "ss"] if dts != 2 else [])
+    for k in names:
+        (d / "results" / f"bak_{k}.dat").replace(d / "results" / f"init_{k}.dat")
+    r = subprocess.run([FEXE, "2"], cwd=d, capture_output=True, text=True, timeout=120)
+    assert r.returncode == 0, r.stdout[-2000:] + r.stderr[-2000:]
+    for k in names:
+        a = np.fromfile(runs["one"] / "results" / f"bak_{k}.dat")
+        b = np.fromfile(d / "results" / f"bak_{k}.dat")
+        assert np.array_equal(a, b), (k, np.abs(a - b).max())
+    assert (d / "results" / "bak_ff.dat").exists() == (dts != 2)
