@@ -1242,6 +1242,29 @@ __global__ __launch_bounds__(256) void k_mean_accum(const R *u, const R *v, cons
         S.m[HC_MEAN_T][q] += tt;
     }
 }
+// CFL number of screen_cpu_time (src/module_io.f90:32-80): per level, the
+// largest dt (|u|/dx + |v|/dy + |w|/dz(k)) over the plane (u relative to
+// the moving frame, point values as the reference takes them).
+template <class R>
+__global__ __launch_bounds__(256) void k_cfl_partial(const R *u, const R *v, const R *w, const double *lev, int nx,
+                                                     int ny, int nz, double dt, double dx, double dy, double *part)
+{
+    __shared__ double rm[256];
+    const int k = blockIdx.x;
+    const size_t n = (size_t)nx * ny, o = (size_t)k * n;
+    const double idz = 1.0 / lev[LV_DZ * nz + k];
+    double m = 0.0;
+    for (size_t q = threadIdx.x; q < n; q += 256)
+        m = fmax(m, dt * (fabs((double)u[o + q]) / dx + fabs((double)v[o + q]) / dy + fabs((double)w[o + q]) * idz));
+    rm[threadIdx.x] = m;
+    __syncthreads();
+    for (int s = 128; s > 0; s >>= 1) {
+        if (threadIdx.x < s) rm[threadIdx.x] = fmax(rm[threadIdx.x], rm[threadIdx.x + s]);
+        __syncthreads();
+    }
+    if (threadIdx.x == 0) part[k] = rm[0];
+}
+
 __global__ __launch_bounds__(256) void k_scale_copy(const double *src, double *dst, size_t n, double s)
 {
     for (size_t q = blockIdx.x * (size_t)blockDim.x + threadIdx.x; q < n; q += (size_t)gridDim.x * blockDim.x)
@@ -2757,6 +2780,27 @@ extern "C" int hc_drive_steps(hc_ctx *c, int nsteps)
             if (int e = ubar_launch(c)) return e;
     }
     HIPCHK(hipGetLastError());
+    return 0;
+}
+
+extern "C" int hc_screen_cfl(hc_ctx *c, double *cfl_max)
+{
+    if (!c || !cfl_max) return set_err(HC_ERR_ARG, "null argument");
+    const hc_params &P = c->prm;
+    if (c->mixed) {
+        const View<float> &M = c->m32;
+        hipLaunchKernelGGL(k_cfl_partial<float>, dim3(c->nz), dim3(256), 0, c->stream, M.u, M.v, M.w, c->lev, c->nx,
+                           c->ny, c->nz, P.dt, c->dx, c->dy, c->part);
+    } else {
+        hipLaunchKernelGGL(k_cfl_partial<double>, dim3(c->nz), dim3(256), 0, c->stream, c->u, c->v, c->w, c->lev,
+                           c->nx, c->ny, c->nz, P.dt, c->dx, c->dy, c->part);
+    }
+    std::vector<double> h(c->nz);
+    HIPCHK(hipMemcpyAsync(h.data(), c->part, h.size() * sizeof(double), hipMemcpyDeviceToHost, c->stream));
+    HIPCHK(hipStreamSynchronize(c->stream));
+    double m = 0.0;
+    for (double x : h) m = std::max(m, x);
+    *cfl_max = m;
     return 0;
 }
 

@@ -41,7 +41,8 @@ program hercules_hip
     type(hc_params) :: prm
     real(c_double), allocatable, target :: f(:,:,:), g(:,:,:)
     real(c_double), allocatable, target :: zc(:), dzk(:), l_t(:), l_b(:)
-    real(c_double) :: div_avg, div_max
+    real(c_double) :: div_avg, div_max, cfl_max, ri_now, ubar_now
+    real :: cpu_t
     integer :: stps, indx_bk, indx_m, nsteps_env, q
     character(len=2), dimension(11), parameter :: mean_names = &
         (/ 'u ', 'v ', 'w ', 't ', 'uu', 'vv', 'ww', 'uw', 'vw', 'tt', 'tw' /)
@@ -113,10 +114,28 @@ program hercules_hip
     indx_bk = 1
     indx_m = 1
     do stps = 1, nsteps_env
+        ! screen_cpu_time (src/module_io.f90:32-80): CFL of the fields entering the step
+        call hip_check(hc_screen_cfl(hip_ctx, cfl_max), 'screen_cfl')
+        call cpu_time(cpu_t)
+        print *, ' '
+        write (*, 184) stps * dt, cpu_t, cfl_max
+184     format(' Time = ', es12.5, '; CPU Time = ', es11.4, '; CFL max = ', f5.2)
         call hip_check(hc_drive_steps(hip_ctx, 1_c_int), 'drive step')
+        ! screen_div_error after the last substep (:85-206); the library reports
+        ! the largest |div| where the reference keeps a signed running value
         call hip_check(hc_screen_div_error(hip_ctx, div_avg, div_max), 'screen_div_error')
-        write (*, 105) stps, div_avg, div_max
-105     format(' Step ', i8, ' DIV-ERR FINAL: DIV_AVG = ', es11.4, ';  |DIV|_MAX = ', es11.4)
+        write (*, 105) div_avg, div_max
+105     format(' DIV-ERR FINAL: DIV_AVG = ', es11.4, ';  |DIV|_MAX = ', es11.4)
+        if (is_ri_var .eq. 1) then
+            call hip_check(hc_get_scalar(hip_ctx, 3_c_int, ri_now), 'get_scalar')
+            write (*, 258) ri_now
+258         format(' Ri_b = ', es12.5)
+        endif
+        if (dp_opt .eq. 2) then
+            call hip_check(hc_calculate_ubar(hip_ctx, ubar_now), 'calculate_ubar')
+            write (*, 259) ubar_now + u_mrf
+259         format(' Ubar = ', es12.5)
+        endif
         if (indx_bk / ibackup .eq. 1 .or. stps .eq. nsteps_env) then
             call write_one('results/bak_u.dat', u_mrf, HC_U)
             call write_one('results/bak_v.dat', 0.d0, HC_V)

@@ -101,6 +101,12 @@ module module_hercules_hip
             real(c_double), intent(out) :: ubar
             integer(c_int) :: rc
         end function
+        function hc_screen_cfl(ctx, cfl_max) bind(C, name="hc_screen_cfl") result(rc)
+            import :: c_ptr, c_int, c_double
+            type(c_ptr), value :: ctx
+            real(c_double), intent(out) :: cfl_max
+            integer(c_int) :: rc
+        end function
         function hc_get_mesh(ctx, zc, dz, l_t, l_b) bind(C, name="hc_get_mesh") result(rc)
             import :: c_ptr, c_int
             type(c_ptr), value :: ctx, zc, dz, l_t, l_b

@@ -229,6 +229,11 @@ enum hc_kernel {
     HC_K_SOLVE_XC2R = 12,
     HC_KERNEL_COUNT = 13
 };
+/* The CFL number screen_cpu_time prints (src/module_io.f90:32-80): the
+ * largest dt (|u|/dx + |v|/dy + |w|/dz(k)) over the domain, u relative to
+ * the moving frame. */
+int hc_screen_cfl(hc_ctx *ctx, double *cfl_max);
+
 /* The wall-normal grid as get_mesh_param writes it to results/geo.info
  * (src/module_parameters.f90:495-504): cell-centre z, dz, l_t, l_b of the
  * nz levels (arrays of nz values). */

@@ -55,6 +55,8 @@ def test_fortran_driver_matches_oracle(tmp_path, cds):
     r = subprocess.run([FEXE], cwd=tmp_path, capture_output=True, text=True, timeout=120)
     print(r.stdout[-2000:], r.stderr[-2000:])
     assert r.returncode == 0
+    # screen_cpu_time and screen_div_error lines, one each per step
+    assert r.stdout.count("CFL max =") == prm.imax and r.stdout.count("DIV-ERR FINAL") == prm.imax
     o = make_oracle(prm, st)
     o.mean_enable()
     o.steps(prm.imax)
