@@ -32,7 +32,7 @@ ABI_SYMBOLS = ("hc_last_error", "hc_create", "hc_destroy", "hc_set_field", "hc_g
                "hc_ensemble_create", "hc_ensemble_steps", "hc_ensemble_graph_used", "hc_ensemble_destroy",
                "hc_substep_fd", "hc_set_adjoint", "hc_get_adjoint", "hc_adjoint_substep_fd",
                "hc_wall_shear_gradient", "hc_autotune", "hc_set_tuning", "hc_mean_fields", "hc_get_mean_field",
-               "hc_get_mesh", "hc_screen_cfl")
+               "hc_get_mesh", "hc_screen_cfl", "hc_get_column")
 # statistics of calculate_mean_fields, in the order of output_mean_fields
 MEANS = ("u", "v", "w", "t", "uu", "vv", "ww", "uw", "vw", "tt", "tw")
 KERNELS = {"rhs": 0, "yfft_fwd": 1, "column": 2, "yfft_inv": 3, "finish": 4, "rhs_t": 5,
@@ -89,6 +89,7 @@ def load_library(path: str = LIB_PATH):
     L.hc_get_mean_field.argtypes = [vp, ip, vp, ip]
     L.hc_get_mesh.argtypes = [vp, vp, vp, vp, vp]
     L.hc_screen_cfl.argtypes = [vp, dp]
+    L.hc_get_column.argtypes = [vp, ip, ip, ip, vp]
     L.hc_set_adjoint.argtypes = [vp, ip, vp, ip]
     L.hc_get_adjoint.argtypes = [vp, ip, vp, ip]
     L.hc_adjoint_substep_fd.argtypes = [vp, ip]
@@ -149,6 +150,12 @@ class Channel:
         shape = self.shape if layout == LAYOUT_INTERIOR else self.ghost_shape
         a = np.zeros(shape) if out is None else out
         self._check(self.L.hc_get_field(self.h, FIELDS[name], _ptr(a), layout))
+        return a
+
+    def get_column(self, name, i, j):
+        """The z column (i, j) of a field (interior indices)."""
+        a = np.zeros(self.prm.nz)
+        self._check(self.L.hc_get_column(self.h, FIELDS[name], int(i), int(j), _ptr(a)))
         return a
 
     def screen_cfl(self):

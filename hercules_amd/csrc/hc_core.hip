@@ -1265,6 +1265,13 @@ __global__ __launch_bounds__(256) void k_cfl_partial(const R *u, const R *v, con
     if (threadIdx.x == 0) part[k] = rm[0];
 }
 
+// one z column (i, j) of a field, in FP64
+template <class R>
+__global__ __launch_bounds__(256) void k_get_column(const R *f, size_t plane, size_t off, int nz, double *dst)
+{
+    for (int k = threadIdx.x; k < nz; k += 256) dst[k] = (double)f[(size_t)k * plane + off];
+}
+
 __global__ __launch_bounds__(256) void k_scale_copy(const double *src, double *dst, size_t n, double s)
 {
     for (size_t q = blockIdx.x * (size_t)blockDim.x + threadIdx.x; q < n; q += (size_t)gridDim.x * blockDim.x)
@@ -2780,6 +2787,31 @@ extern "C" int hc_drive_steps(hc_ctx *c, int nsteps)
             if (int e = ubar_launch(c)) return e;
     }
     HIPCHK(hipGetLastError());
+    return 0;
+}
+
+extern "C" int hc_get_column(hc_ctx *c, int field, int i, int j, double *dst)
+{
+    if (!c || !dst) return set_err(HC_ERR_ARG, "null argument");
+    if (i < 0 || i >= c->nx || j < 0 || j >= c->ny) return set_err(HC_ERR_ARG, "column (%d, %d) outside the grid", i, j);
+    const size_t plane = (size_t)c->nx * c->ny, off = (size_t)j * c->nx + i;
+    if (c->prm.cds == 3 && field == HC_P && !c->spec_dirty) {  // p lives as a spectrum: form it first
+        std::vector<double> full(plane * c->nz);
+        if (int e = hc_get_field(c, HC_P, full.data(), HC_LAYOUT_INTERIOR)) return e;
+        for (int k = 0; k < c->nz; ++k) dst[k] = full[(size_t)k * plane + off];
+        return 0;
+    }
+    if (c->mixed) {
+        const float *f = field_ptr<float>(c, field);
+        if (!f) return set_err(HC_ERR_ARG, "no field %d", field);
+        hipLaunchKernelGGL(k_get_column<float>, dim3(1), dim3(256), 0, c->stream, f, plane, off, c->nz, c->part);
+    } else {
+        const double *f = field_ptr<double>(c, field);
+        if (!f) return set_err(HC_ERR_ARG, "no field %d", field);
+        hipLaunchKernelGGL(k_get_column<double>, dim3(1), dim3(256), 0, c->stream, f, plane, off, c->nz, c->part);
+    }
+    HIPCHK(hipMemcpyAsync(dst, c->part, (size_t)c->nz * sizeof(double), hipMemcpyDeviceToHost, c->stream));
+    HIPCHK(hipStreamSynchronize(c->stream));
     return 0;
 }
 

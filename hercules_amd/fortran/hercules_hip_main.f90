@@ -12,8 +12,11 @@
 ! init_{ff,gg,hh,ss}.dat, src/module_tools.f90:292-307), the grid
 ! results/geo.info (src/module_parameters.f90:495-504) and the mean fields results/mean_*.dat every
 ! imeanfl steps (calculate_mean_fields summed on the GPU from the first
-! step, written as output_mean_fields does, src/module_io.f90:546-598).
-! Time series and slices of the reference are not produced.
+! step, written as output_mean_fields does, src/module_io.f90:546-598),
+! the time series results/tmsr_{u,v,w,t,p}.dat every step when istmsr = 1
+! (:211-305), the instantaneous fields results/inst_{u,v,w,p,t}NNN.dat every
+! iinstfl steps (:366-403) and the x-y, x-z, y-z slices
+! results/slices/{xy,xz,yz}2d_* every intv_2d steps (:408-542).
 !*************************************************************************
 program hercules_hip
 
@@ -43,11 +46,14 @@ program hercules_hip
     real(c_double), allocatable, target :: zc(:), dzk(:), l_t(:), l_b(:)
     real(c_double) :: div_avg, div_max, cfl_max, ri_now, ubar_now
     real :: cpu_t
-    integer :: stps, indx_bk, indx_m, nsteps_env, q
+    integer :: stps, indx_bk, indx_m, indx_ins, indx_slc, nsteps_env, q
+    real(c_double), allocatable, target :: col(:)
     character(len=2), dimension(11), parameter :: mean_names = &
         (/ 'u ', 'v ', 'w ', 't ', 'uu', 'vv', 'ww', 'uw', 'vw', 'tt', 'tw' /)
     character(len=32) :: arg
 
+    ! slice lists end at their first zero entry
+    xy2d_id(:) = 0; xz2d_id(:) = 0; yz2d_id(:) = 0
     open(10, file='parameters.input')
     read(10, nml=domain)
     read(10, nml=modeling)
@@ -111,8 +117,19 @@ program hercules_hip
     close(11)
     call hip_check(hc_mean_fields(hip_ctx, 1_c_int), 'hc_mean_fields')
 
+    if (istmsr .eq. 1) then  ! the time-series files start empty (src/module_navier_stokes.f90:82-93)
+        open(99, file='results/tmsr_u.dat', status='replace'); close(99)
+        open(98, file='results/tmsr_v.dat', status='replace'); close(98)
+        open(97, file='results/tmsr_w.dat', status='replace'); close(97)
+        open(96, file='results/tmsr_t.dat', status='replace'); close(96)
+        open(95, file='results/tmsr_p.dat', status='replace'); close(95)
+    endif
+    allocate(col(nz))
+
     indx_bk = 1
     indx_m = 1
+    indx_ins = 1
+    indx_slc = 1
     do stps = 1, nsteps_env
         ! screen_cpu_time (src/module_io.f90:32-80): CFL of the fields entering the step
         call hip_check(hc_screen_cfl(hip_ctx, cfl_max), 'screen_cfl')
@@ -135,6 +152,14 @@ program hercules_hip
             call hip_check(hc_calculate_ubar(hip_ctx, ubar_now), 'calculate_ubar')
             write (*, 259) ubar_now + u_mrf
 259         format(' Ubar = ', es12.5)
+        endif
+        ! time series of the column at the first interior point (ghst+1, ghst+1)
+        if (istmsr .eq. 1) then
+            call write_series('results/tmsr_u.dat', HC_U, u_mrf, stps * dt)
+            call write_series('results/tmsr_v.dat', HC_V, 0.d0, stps * dt)
+            call write_series('results/tmsr_w.dat', HC_W, 0.d0, stps * dt)
+            call write_series('results/tmsr_t.dat', HC_T, 0.d0, stps * dt)
+            call write_series('results/tmsr_p.dat', HC_P, 0.d0, stps * dt)
         endif
         if (indx_bk / ibackup .eq. 1 .or. stps .eq. nsteps_env) then
             call write_one('results/bak_u.dat', u_mrf, HC_U)
@@ -159,6 +184,24 @@ program hercules_hip
             indx_m = 0
         endif
         indx_m = indx_m + 1
+        if (indx_ins / iinstfl .eq. 1) then
+            print *, 'Writing Instantaneous Fields...'
+            write (arg, '(i3.3)') stps / iinstfl
+            call write_one('results/inst_u' // trim(arg) // '.dat', u_mrf, HC_U)
+            call write_one('results/inst_v' // trim(arg) // '.dat', 0.d0, HC_V)
+            call write_one('results/inst_w' // trim(arg) // '.dat', 0.d0, HC_W)
+            call write_one('results/inst_p' // trim(arg) // '.dat', 0.d0, HC_P)
+            call write_one('results/inst_t' // trim(arg) // '.dat', 0.d0, HC_T)
+            indx_ins = 0
+        endif
+        indx_ins = indx_ins + 1
+        if (isxy2d .eq. 1 .or. isxz2d .eq. 1 .or. isyz2d .eq. 1) then
+            if (indx_slc / intv_2d .eq. 1) then
+                call write_slices(stps / intv_2d)
+                indx_slc = 0
+            endif
+        endif
+        indx_slc = indx_slc + 1
     enddo
 
     call hip_destroy()
@@ -189,6 +232,86 @@ contains
         write(u_) g
         close(u_)
     end subroutine
+
+    ! one line: time, then the column's nz values (formats of output_time_series)
+    subroutine write_series(fname, field, shift, time)
+        character(len=*), intent(in) :: fname
+        integer(c_int), intent(in) :: field
+        real(c_double), intent(in) :: shift, time
+        integer :: u_, k
+        call hip_check(hc_get_column(hip_ctx, field, 0_c_int, 0_c_int, c_loc(col)), 'hc_get_column')
+        open(newunit=u_, file=fname, status='old', position='append', action='write')
+        do k = 1, nz
+            if (k .eq. 1) then
+                write(u_, '(2f14.8)', advance='no') time, col(k) + shift
+            elseif (k .eq. nz) then
+                write(u_, '(1f14.8)') col(k) + shift
+            else
+                write(u_, '(1f14.8)', advance='no') col(k) + shift
+            endif
+        enddo
+        close(u_)
+    end subroutine
+
+    ! output_2d_slices: planes of u + u_mrf, v, w, p, t at the listed indices
+    subroutine write_slices(nout)
+        integer, intent(in) :: nout
+        character(len=6) :: ff1
+        character(len=4) :: ii1
+        character(len=1), dimension(5), parameter :: fn = (/ 'u', 'v', 'w', 'p', 't' /)
+        integer(c_int), dimension(5), parameter :: fid = (/ HC_U, HC_V, HC_W, HC_P, HC_T /)
+        integer :: n, m, nxy, nxz, nyz, u_
+        real(c_double) :: shift
+        write (ff1, '(i6.6)') nout
+        nxy = count_ids(xy2d_id); nxz = count_ids(xz2d_id); nyz = count_ids(yz2d_id)
+        if (isxy2d .eq. 1) print *, 'Writing X-Y Slices...'
+        if (isxz2d .eq. 1) print *, 'Writing X-Z Slices...'
+        if (isyz2d .eq. 1) print *, 'Writing Y-Z Slices...'
+        do m = 1, 5
+            shift = 0.d0
+            if (m .eq. 1) shift = u_mrf
+            call hip_download(fid(m), f)
+            g = f(3:nx + 2, 3:ny + 2, 3:nz + 2) + shift
+            if (isxy2d .eq. 1) then
+                do n = 1, nxy
+                    write (ii1, '(i4.4)') xy2d_id(n)
+                    open(newunit=u_, file='results/slices/xy2d_' // fn(m) // '_k' // ii1 // '_t' // ff1, &
+                         access='stream', form='unformatted', status='replace')
+                    write(u_) g(:, :, xy2d_id(n))
+                    close(u_)
+                enddo
+            endif
+            if (isxz2d .eq. 1) then
+                do n = 1, nxz
+                    write (ii1, '(i4.4)') xz2d_id(n)
+                    open(newunit=u_, file='results/slices/xz2d_' // fn(m) // '_j' // ii1 // '_t' // ff1, &
+                         access='stream', form='unformatted', status='replace')
+                    write(u_) g(:, xz2d_id(n), :)
+                    close(u_)
+                enddo
+            endif
+            if (isyz2d .eq. 1) then
+                do n = 1, nyz
+                    write (ii1, '(i4.4)') yz2d_id(n)
+                    open(newunit=u_, file='results/slices/yz2d_' // fn(m) // '_i' // ii1 // '_t' // ff1, &
+                         access='stream', form='unformatted', status='replace')
+                    write(u_) g(yz2d_id(n), :, :)
+                    close(u_)
+                enddo
+            endif
+        enddo
+    end subroutine
+
+    ! entries before the first zero (output_2d_slices counts the list this way)
+    integer function count_ids(ids)
+        integer, dimension(:), intent(in) :: ids
+        integer :: n
+        count_ids = 0
+        do n = 1, size(ids)
+            if (ids(n) .eq. 0) exit
+            count_ids = n
+        enddo
+    end function
 
     subroutine write_mean(fname, stat)
         character(len=*), intent(in) :: fname

@@ -101,6 +101,12 @@ module module_hercules_hip
             real(c_double), intent(out) :: ubar
             integer(c_int) :: rc
         end function
+        function hc_get_column(ctx, field, i, j, dst) bind(C, name="hc_get_column") result(rc)
+            import :: c_ptr, c_int
+            type(c_ptr), value :: ctx, dst
+            integer(c_int), value :: field, i, j
+            integer(c_int) :: rc
+        end function
         function hc_screen_cfl(ctx, cfl_max) bind(C, name="hc_screen_cfl") result(rc)
             import :: c_ptr, c_int, c_double
             type(c_ptr), value :: ctx

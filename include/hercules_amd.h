@@ -229,6 +229,10 @@ enum hc_kernel {
     HC_K_SOLVE_XC2R = 12,
     HC_KERNEL_COUNT = 13
 };
+/* One z column (interior indices i, j, 0-based) of a field into dst (nz
+ * values): the profile output_time_series writes (src/module_io.f90:211-305). */
+int hc_get_column(hc_ctx *ctx, int field, int i, int j, double *dst);
+
 /* The CFL number screen_cpu_time prints (src/module_io.f90:32-80): the
  * largest dt (|u|/dx + |v|/dy + |w|/dz(k)) over the domain, u relative to
  * the moving frame. */

@@ -128,3 +128,56 @@ def test_fortran_restart_is_bitwise(tmp_path, dts):
         b = np.fromfile(d / "results" / f"bak_{k}.dat")
         assert np.array_equal(a, b), (k, np.abs(a - b).max())
     assert (d / "results" / "bak_ff.dat").exists() == (dts != 2)
+
+
+def test_fortran_outputs_series_instants_slices(tmp_path):
+    """Time series (istmsr = 1, src/module_io.f90:211-305), instantaneous
+    fields every iinstfl steps (:366-403) and x-y / x-z / y-z slices every
+    intv_2d steps (:408-542), against the oracle's fields."""
+    if not os.path.exists(FEXE):
+        pytest.fail(f"Fortran host not built: {FEXE}")
+    prm = cases.closed_channel(nx=32, ny=32, nz=33, imax=4, cds=2, issk=1)
+    st = cases.laminar_noise(prm, seed=8)
+    with Channel(prm) as ch:
+        for k, v in st.items():
+            ch.set_field(k, v)
+        ch.project()
+        st = {k: ch.get_field(k) for k in ("u", "v", "w", "p", "t")}
+    (tmp_path / "results" / "slices").mkdir(parents=True)
+    deck = DECK.format(lx=prm.lx, ly=prm.ly, nx=prm.nx, ny=prm.ny, nz=prm.nz, zs=prm.zstretch, nu=prm.nu,
+                       alpha=prm.alpha, imax=prm.imax, cds=2, dts=2)
+    deck = deck.replace("restart = 1, istmsr = 0, ibackup = 4, iinstfl = 100000, imeanfl = 4",
+                        "restart = 1, istmsr = 1, ibackup = 4, iinstfl = 2, imeanfl = 4")
+    deck = deck.replace("isxy2d = 0, xy2d_id = 10, isxz2d = 0, xz2d_id = 10, isyz2d = 0, yz2d_id = 10, intv_2d = 1000",
+                        "isxy2d = 1, xy2d_id = 3, 10, isxz2d = 1, xz2d_id = 5, isyz2d = 1, yz2d_id = 7, intv_2d = 2")
+    assert "istmsr = 1" in deck and "intv_2d = 2" in deck
+    (tmp_path / "parameters.input").write_text(deck)
+    for k, v in st.items():
+        np.ascontiguousarray(v).tofile(tmp_path / "results" / f"init_{k}.dat")
+    r = subprocess.run([FEXE], cwd=tmp_path, capture_output=True, text=True, timeout=120)
+    assert r.returncode == 0, r.stdout[-2000:] + r.stderr[-2000:]
+    o = make_oracle(prm, st)
+    snaps = []
+    for _ in range(prm.imax):
+        o.steps(1)
+        snaps.append({k: o.get(k) for k in ("u", "v", "w", "p", "t")})
+    res = tmp_path / "results"
+    # time series: one line per step, time then the (0, 0) column
+    ser = np.loadtxt(res / "tmsr_u.dat")
+    assert ser.shape == (prm.imax, prm.nz + 1)
+    assert np.allclose(ser[:, 0], prm.dt * np.arange(1, prm.imax + 1), atol=1e-8)
+    for n in range(prm.imax):
+        assert np.allclose(ser[n, 1:], snaps[n]["u"][:, 0, 0], atol=2e-8), n
+    # instantaneous fields after steps 2 and 4
+    for tag, n in (("001", 1), ("002", 3)):
+        got = np.fromfile(res / f"inst_u{tag}.dat").reshape(prm.nz, prm.ny, prm.nx)
+        assert rel_l2(got, snaps[n]["u"]) < 1e-10, tag
+    # slices after step 4 (output 2)
+    u4 = snaps[3]["u"]
+    xy = np.fromfile(res / "slices" / "xy2d_u_k0003_t000002").reshape(prm.ny, prm.nx)
+    assert rel_l2(xy, u4[2]) < 1e-10
+    assert (res / "slices" / "xy2d_w_k0010_t000002").exists()
+    xz = np.fromfile(res / "slices" / "xz2d_u_j0005_t000002").reshape(prm.nz, prm.nx)
+    assert rel_l2(xz, u4[:, 4, :]) < 1e-10
+    yz = np.fromfile(res / "slices" / "yz2d_u_i0007_t000002").reshape(prm.nz, prm.ny)
+    assert rel_l2(yz, u4[:, :, 6]) < 1e-10
