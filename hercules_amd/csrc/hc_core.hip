@@ -3146,6 +3146,7 @@ struct hc_ensemble {
     std::vector<hipEvent_t> join;
     hipGraph_t graph = nullptr;
     hipGraphExec_t exec = nullptr;
+    std::vector<char> cap_mean;  // each member's statistics switch when the graph was captured
 };
 
 // one pass of the drive loop body for one member (hc_drive_steps' body)
@@ -3191,13 +3192,15 @@ static bool ensemble_graphable(const hc_ensemble *e)
 // that one_step advances are restored afterwards; replays advance them.
 static int ensemble_capture(hc_ensemble *e)
 {
-    struct Saved { int stps, fcur, scur; double time_sim, got; };
+    struct Saved { int stps, fcur, scur; double time_sim, got; long mean_steps; };
     std::vector<Saved> sv;
+    e->cap_mean.clear();
     for (hc_ctx *c : e->m) {
         if (c->prm.cds == 3 && c->spec_dirty)
             if (int er = spec_init(c)) return er;
         HIPCHK(hipStreamSynchronize(c->stream));
-        sv.push_back({c->stps, c->fcur, c->scur, c->time_sim, c->got});
+        sv.push_back({c->stps, c->fcur, c->scur, c->time_sim, c->got, c->mean_steps});
+        e->cap_mean.push_back(c->mean_on ? 1 : 0);
     }
     HIPCHK(hipStreamBeginCapture(e->master, hipStreamCaptureModeThreadLocal));
     int err = 0;
@@ -3214,7 +3217,7 @@ static int ensemble_capture(hc_ensemble *e)
     for (size_t q = 0; q < e->m.size(); ++q) {
         hc_ctx *c = e->m[q];
         c->stps = sv[q].stps; c->fcur = sv[q].fcur; c->scur = sv[q].scur;
-        c->time_sim = sv[q].time_sim; c->got = sv[q].got;
+        c->time_sim = sv[q].time_sim; c->got = sv[q].got; c->mean_steps = sv[q].mean_steps;
     }
     if (err) { if (g) (void)hipGraphDestroy(g); return err; }
     if (he != hipSuccess || ce != hipSuccess || !g) {
@@ -3240,6 +3243,16 @@ extern "C" int hc_ensemble_steps(hc_ensemble *e, int nsteps)
     int done = 0;
     e->graph_used = false;
     if (e->use_graph && ensemble_graphable(e) && nsteps >= 2) {
+        // a member's statistics switch changed since the capture: capture again
+        bool stale = false;
+        for (size_t q = 0; q < e->m.size() && e->exec; ++q)
+            stale = stale || (e->cap_mean[q] != (e->m[q]->mean_on ? 1 : 0));
+        if (stale) {
+            (void)hipGraphExecDestroy(e->exec);
+            (void)hipGraphDestroy(e->graph);
+            e->exec = nullptr;
+            e->graph = nullptr;
+        }
         if (!e->exec)
             if (int er = ensemble_capture(e)) return er;
         for (hc_ctx *c : e->m)
@@ -3249,6 +3262,7 @@ extern "C" int hc_ensemble_steps(hc_ensemble *e, int nsteps)
             for (hc_ctx *c : e->m) {
                 c->stps += 2;
                 c->time_sim += 2.0 * c->prm.dt;
+                if (c->mean_on) c->mean_steps += 2;
             }
         }
         HIPCHK(hipStreamSynchronize(e->master));

@@ -84,3 +84,38 @@ def test_means_stop_restart_and_errors():
     assert ch.scalar("mean_steps") == 1
     assert np.allclose(ch.get_mean("u"), ch.get_field("u"), rtol=0, atol=1e-13 * np.abs(u2).max())
     ch.close()
+
+
+def test_means_of_ensemble_members_equal_single_runs():
+    """Statistics accumulated inside the replayed ensemble graph equal those
+    of the same channels stepped alone (same kernels, bit for bit), also when
+    a member's statistics are switched on after the first capture."""
+    from hercules_amd import Ensemble
+    specs = [("closed", dict(nx=32, ny=32, nz=33)), ("open_t", dict(nx=32, ny=32, nz=33, cds=3, issk=1))]
+    cs = [small_case(kind, **kw) for kind, kw in specs]
+
+    def make(prm, st):
+        ch = Channel(prm)
+        for k, v in st.items():
+            ch.set_field(k, v)
+        ch.init()
+        return ch
+
+    singles = [make(p, s) for p, s in cs]
+    for ch in singles:
+        ch.steps(2)
+        ch.mean_fields(True)
+        ch.steps(4)
+    members = [make(p, s) for p, s in cs]
+    with Ensemble(members, use_graph=True) as ens:
+        ens.steps(2)  # captured without statistics
+        for ch in members:
+            ch.mean_fields(True)
+        ens.steps(4)  # captured again with them
+        assert ens.graph_used
+    for (prm, _), a, b in zip(cs, singles, members):
+        assert a.scalar("mean_steps") == b.scalar("mean_steps") == 4
+        for n in MEANS:
+            assert np.array_equal(a.get_mean(n), b.get_mean(n)), (prm.cds, n)
+    for ch in singles + members:
+        ch.close()
