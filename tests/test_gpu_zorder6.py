@@ -162,3 +162,51 @@ def test_z6_spectral_steps_match_torch_reference(kind, nz, kw):
     scale = np.abs(ref["u"].numpy()).max() / (prm.lz / prm.nz)
     assert dmax < 1e-12 * scale, dmax
     ch.close()
+
+
+def test_z6_ensemble_members_equal_single_runs():
+    """6th-order members inside a replayed ensemble graph (the interpolant
+    pre-pass and the Chebyshev columns captured) equal their single runs."""
+    from hercules_amd import Ensemble
+    specs = [("closed", 33, {}), ("open", 65, {"cds": 3, "issk": 1})]
+    cs = [_case(kind, nz, **kw) for kind, nz, kw in specs]
+
+    def make(prm, st):
+        ch = Channel(prm)
+        for k, v in st.items():
+            ch.set_field(k, v)
+        ch.init()
+        return ch
+
+    singles = [make(p, s) for p, s in cs]
+    for ch in singles:
+        ch.steps(4)
+    members = [make(p, s) for p, s in cs]
+    with Ensemble(members, use_graph=True) as ens:
+        ens.steps(4)
+        assert ens.graph_used
+    for (prm, _), a, b in zip(cs, singles, members):
+        for n in ["u", "v", "w", "p"] + (["t"] if prm.isscalar else []):
+            assert np.array_equal(a.get_field(n), b.get_field(n)), (prm.cds, n)
+    for ch in singles + members:
+        ch.close()
+
+
+@pytest.mark.parametrize("kw", [{"dts": 1}, {"dp_opt": 2}, {"dp_opt": 3, "fc": 0.5, "ug": 1.0, "vg": 0.2}])
+def test_z6_other_flags_stay_divergence_free(kw):
+    """AB2, the bulk-velocity drive and the Coriolis drive with the 6th-order
+    operators (no restatement covers these flags): finite fields and a
+    divergence at round-off after 5 steps."""
+    prm, st = _case("closed", 65, **kw)
+    ch = Channel(prm)
+    for k, v in st.items():
+        ch.set_field(k, v)
+    ch.project()
+    ch.init()
+    ch.steps(5)
+    u = ch.get_field("u")
+    assert np.isfinite(u).all()
+    _, dmax = ch.screen_div_error()
+    scale = np.abs(u).max() / (prm.lz / prm.nz)
+    assert dmax < 1e-12 * scale, dmax
+    ch.close()
