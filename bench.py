@@ -87,7 +87,7 @@ def build_case(workload):
         prm = cases.stratified_open_channel(nx=256, ny=256, nz=257, wall_rows=1)
         st = cases.loglaw_noise(prm, seed=2024)
     elif workload == "sp":
-        # configuration 3's top condition: constant temperature gradient
+        # stratified spectral case with a constant temperature gradient at the top
         prm = cases.stratified_open_channel(nx=256, ny=256, nz=257, cds=3, issk=1, wall_rows=1, tbc_top=1,
                                             dtdz_top=1.0)
         st = cases.loglaw_noise(prm, seed=2024)
@@ -129,7 +129,7 @@ def ensemble_members(seed=2024):
 
 
 def run_ensemble(args):
-    """Configuration 4: aggregate throughput of the ensemble (one captured
+    """Aggregate throughput of the 24-channel ensemble (one captured
     graph per two steps, members concurrent) against the same members
     stepped one after another."""
     from hercules_amd import Channel, Ensemble
@@ -186,7 +186,7 @@ def run_ensemble(args):
 
 
 def run_adjoint(args):
-    """Configuration 5: gradient of the bottom wall shear after K = --steps
+    """Gradient of the bottom wall shear after K = --steps
     RK3 steps (128x128x129 neutral closed channel, Re_tau = 180) with respect
     to the initial velocity, by the hand-written adjoint with a checkpoint
     every --ckpt steps; against the forward run of the same K steps."""
