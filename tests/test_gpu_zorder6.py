@@ -37,6 +37,8 @@ CASES = [
     ("open", 65, {}),
     ("open", 33, {"tbc_top": 1, "dtdz_top": 0.5}),
     ("open", 129, {"cds": 1, "tbc_top": 1, "dtdz_top": 1.0}),
+    # nz = 257: five levels per lane in the column kernels
+    ("closed", 257, {}),
 ]
 
 
@@ -136,6 +138,7 @@ SPECTRAL_CASES = [
     ("open", 33, {"issk": 1}),
     ("open", 65, {"issk": 1, "tbc_top": 1, "dtdz_top": 0.5}),
     ("open", 129, {"issk": 0}),
+    ("open", 257, {"issk": 1, "tbc_top": 1, "dtdz_top": 0.5}),
 ]
 
 
