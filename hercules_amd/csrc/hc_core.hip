@@ -2428,6 +2428,9 @@ extern "C" int hc_create(const hc_params *prm, hc_ctx **out)
         if (P.precision != HC_FP64) return set_err(HC_ERR_UNSUPPORTED, "zorder=6 needs precision=0 (FP64)");
         if (P.isdamp == 1) return set_err(HC_ERR_UNSUPPORTED, "zorder=6 does not cover Rayleigh damping");
         if (P.nz < 16) return set_err(HC_ERR_UNSUPPORTED, "zorder=6 needs nz >= 16");
+        // two column sub-tiles (staging, operator products) of nz x 8 complex in the LDS
+        if ((size_t)2 * P.nz * kColWaves * sizeof(double2) > kColLds)
+            return set_err(HC_ERR_UNSUPPORTED, "zorder=6 needs nz <= %d", (int)(kColLds / (2 * kColWaves * sizeof(double2))));
     }
     if (!fft_len_ok(P.nx) || !fft_len_ok(P.ny)) return set_err(HC_ERR_UNSUPPORTED, "nx, ny must be one of 16,32,64,128,192,256,512");
     {

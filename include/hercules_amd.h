@@ -79,7 +79,7 @@ typedef struct hc_params {
      * z divergence and gradient, so the projected velocity is divergence-free
      * to round-off; the implicit solves are exact (Chebyshev iteration
      * preconditioned by the 2nd-order tridiagonal operator).  zorder = 6
-     * needs precision = HC_FP64, isdamp = 0 and nz >= 16, and has no
+     * needs precision = HC_FP64, isdamp = 0 and 16 <= nz <= 640, and has no
      * adjoint (HC_ERR_UNSUPPORTED otherwise); every mode takes the
      * consistent wall rows (wall_rows is not used). */
     int zorder;
