@@ -466,7 +466,7 @@ struct C2RArgsT {
     const cplx<R> *src[4];
     R *dst[4];
     int mode[4];
-    int ny;
+    int ny, nz;
     const cplx<R> *tw_nl, *tw_nx;
     const R *dmp_s, *dmp_mean;
     int kdamp0;
@@ -477,52 +477,68 @@ constexpr int kC2RPoints = 1024;
 template <int NX, int F, int PTS>
 constexpr int c2r_rows() { return PTS / NX / F > 0 ? PTS / NX / F : 1; }
 
-template <int NX, int F, class R, int PTS = kC2RPoints>
-__global__ __launch_bounds__(kRhsThreads) void k_x_c2r(C2RArgsT<R> A)
+// Field of a row: wave-uniform when a wavefront's 64 lanes share the row
+// (the row length is a multiple of 64), and then taken from a scalar register
+// so that the field's pointer is selected per wavefront.
+template <int LEN, int ROWS>
+HC_DEV int c2r_field(int p)
+{
+    const int f = p / LEN / ROWS;
+    if constexpr (LEN % 64 == 0) return __builtin_amdgcn_readfirstlane(f);
+    return f;
+}
+
+// The spectra rows and, for the accumulated fields, the old values of one
+// tile (ROWS rows of level k, F fields).  Every call issues the same loads,
+// so that the compiler's wait for one buffer never covers the loads of the
+// other: a tile past the last one reads the last tile (never used), a field
+// that is only stored reads its first entry in place of the old row, and xt
+// holds the mirrored entry (NL - l) mod NL as loaded (its conjugate is taken
+// where it is used).
+template <int NX, int F, class R, int ROWS, int NIN, int NOUT>
+HC_DEV void c2r_fetch(const C2RArgsT<R> &A, int tile, int njt, int ntiles, int tid, cplx<R> (&xl)[NIN],
+                      cplx<R> (&xt)[NIN], R (&old)[NOUT])
 {
     using C = cplx<R>;
-    constexpr int ROWS = c2r_rows<NX, F, PTS>();
-    constexpr int NL = NX / 2;
-    constexpr int B = F * ROWS;
-    __shared__ C sq_[B * NL];
-    __shared__ C tws[2 * NL];  // half-length twiddles, then the C2R pre-twiddles
-    const int k = blockIdx.y, j0 = blockIdx.x * ROWS, tid = threadIdx.x;
-    C tw_a = czero_t<C>(), tw_b = czero_t<C>();
-    if (tid < NL) { tw_a = A.tw_nl[tid]; tw_b = A.tw_nx[tid]; }
-    // the rows and, for the accumulated fields, the old values are loaded
-    // before any is used
-    constexpr int NIN = (B * NL + kRhsThreads - 1) / kRhsThreads;
-    constexpr int NOUT = (B * NX + kRhsThreads - 1) / kRhsThreads;
-    C xl[NIN], xt[NIN];
+    constexpr int NL = NX / 2, B = F * ROWS;
+    const int t = tile < ntiles ? tile : ntiles - 1;
+    const int k = t / njt, j0 = (t - k * njt) * ROWS;
 #pragma unroll
     for (int m = 0; m < NIN; ++m) {
-        const int p = tid + m * kRhsThreads;
-        const int row = p / NL, l = p - row * NL, f = row / ROWS, jj = row - f * ROWS, j = j0 + jj;
-        xl[m] = czero_t<C>();
-        xt[m] = czero_t<C>();
-        if (((B * NL) % kRhsThreads == 0 || p < B * NL) && j < A.ny) {
-            const C *s = A.src[f] + ((size_t)k * A.ny + j) * NL;
-            xl[m] = s[l];
-            if (l != 0) xt[m] = cconj(s[NL - l]);
-        }
+        const int p = (B * NL) % kRhsThreads == 0 ? tid + m * kRhsThreads : min(tid + m * kRhsThreads, B * NL - 1);
+        const int row = p / NL, l = p - row * NL, f = c2r_field<NL, ROWS>(p), jj = row - f * ROWS;
+        const int j = min(j0 + jj, A.ny - 1);
+        const C *s = A.src[f] + ((size_t)k * A.ny + j) * NL;
+        xl[m] = s[l];
+        xt[m] = s[l == 0 ? 0 : NL - l];
     }
-    R old[NOUT];
 #pragma unroll
     for (int m = 0; m < NOUT; ++m) {
-        const int p = tid + m * kRhsThreads;
-        const int row = p / NX, i = p - row * NX, f = row / ROWS, jj = row - f * ROWS, j = j0 + jj;
-        old[m] = R(0.0);
-        if (((B * NX) % kRhsThreads == 0 || p < B * NX) && j < A.ny && A.mode[f] != 0)
-            old[m] = A.dst[f][((size_t)k * A.ny + j) * NX + i];
+        const int p = (B * NX) % kRhsThreads == 0 ? tid + m * kRhsThreads : min(tid + m * kRhsThreads, B * NX - 1);
+        const int row = p / NX, i = p - row * NX, f = c2r_field<NX, ROWS>(p), jj = row - f * ROWS;
+        const int j = min(j0 + jj, A.ny - 1);
+        const size_t c = A.mode[f] != 0 ? ((size_t)k * A.ny + j) * NX + i : 0;
+        old[m] = A.dst[f][c];
     }
-    if (tid < NL) { tws[tid] = tw_a; tws[NL + tid] = tw_b; }
-    __syncthreads();
+}
+
+// One tile of the inverse x transform from fetched rows: pre-twiddle into
+// LDS, half-length inverse FFT, store or accumulate.
+template <int NX, int F, class R, int ROWS, int NIN, int NOUT>
+HC_DEV void c2r_tile(const C2RArgsT<R> &A, int tile, int njt, int tid, const cplx<R> (&xl)[NIN],
+                     const cplx<R> (&xt)[NIN], const R (&old)[NOUT], cplx<R> *sq_, const cplx<R> *tws)
+{
+    using C = cplx<R>;
+    constexpr int NL = NX / 2, B = F * ROWS;
+    const int k = tile / njt, j0 = (tile - k * njt) * ROWS;
+    __syncthreads();  // twiddles stored; the previous tile's rows read
 #pragma unroll
     for (int m = 0; m < NIN; ++m) {
         const int p = tid + m * kRhsThreads;
         if ((B * NL) % kRhsThreads == 0 || p < B * NL) {
             const int l = p % NL;
-            sq_[p] = c2r_pre(xl[m], xt[m], tws[NL + l]);
+            const C t = l != 0 ? cconj(xt[m]) : czero_t<C>();
+            sq_[p] = c2r_pre(xl[m], t, tws[NL + l]);
         }
     }
     __syncthreads();
@@ -531,7 +547,7 @@ __global__ __launch_bounds__(kRhsThreads) void k_x_c2r(C2RArgsT<R> A)
 #pragma unroll
     for (int m = 0; m < NOUT; ++m) {
         const int p = tid + m * kRhsThreads;
-        const int row = p / NX, i = p - row * NX, f = row / ROWS, jj = row - f * ROWS, j = j0 + jj;
+        const int row = p / NX, i = p - row * NX, f = c2r_field<NX, ROWS>(p), jj = row - f * ROWS, j = j0 + jj;
         if (((B * NX) % kRhsThreads == 0 || p < B * NX) && j < A.ny) {
             size_t c = ((size_t)k * A.ny + j) * NX + i;
             R x = sqd[row * NX + i];
@@ -545,6 +561,36 @@ __global__ __launch_bounds__(kRhsThreads) void k_x_c2r(C2RArgsT<R> A)
                 A.dst[f][c] = x;
             }
         }
+    }
+}
+
+// Blocks loop over the tiles with a grid stride, two tiles per trip with two
+// register buffers: the next tile's spectra and old values are loaded before
+// the current tile's transform, so that the loads of one tile overlap the LDS
+// work of the previous one.
+template <int NX, int F, class R, int PTS = kC2RPoints>
+__global__ __launch_bounds__(kRhsThreads) void k_x_c2r(C2RArgsT<R> A)
+{
+    using C = cplx<R>;
+    constexpr int ROWS = c2r_rows<NX, F, PTS>();
+    constexpr int NL = NX / 2;
+    constexpr int B = F * ROWS;
+    __shared__ C sq_[B * NL];
+    __shared__ C tws[2 * NL];  // half-length twiddles, then the C2R pre-twiddles
+    const int tid = threadIdx.x;
+    const int njt = (A.ny + ROWS - 1) / ROWS, ntiles = njt * A.nz, G = gridDim.x;
+    if (tid < NL) { tws[tid] = A.tw_nl[tid]; tws[NL + tid] = A.tw_nx[tid]; }
+    constexpr int NIN = (B * NL + kRhsThreads - 1) / kRhsThreads;
+    constexpr int NOUT = (B * NX + kRhsThreads - 1) / kRhsThreads;
+    C xa[NIN], ta[NIN], xb[NIN], tb[NIN];
+    R oa[NOUT], ob[NOUT];
+    c2r_fetch<NX, F, R, ROWS>(A, blockIdx.x, njt, ntiles, tid, xa, ta, oa);
+    for (int tile = blockIdx.x; tile < ntiles; tile += 2 * G) {
+        c2r_fetch<NX, F, R, ROWS>(A, tile + G, njt, ntiles, tid, xb, tb, ob);
+        c2r_tile<NX, F, R, ROWS>(A, tile, njt, tid, xa, ta, oa, sq_, tws);
+        if (tile + G >= ntiles) break;
+        c2r_fetch<NX, F, R, ROWS>(A, tile + 2 * G, njt, ntiles, tid, xa, ta, oa);
+        c2r_tile<NX, F, R, ROWS>(A, tile + G, njt, tid, xb, tb, ob, sq_, tws);
     }
 }
 
@@ -1367,6 +1413,7 @@ struct hc_ctx {
     int nx, ny, nz, nl, Lcol;
     double dx, dy;
     int device;
+    int ncu = 256;  // compute units of the device
     hipStream_t stream;
     double *u, *v, *w, *p, *t;
     double *fb[2][3];  // ff, gg, hh ping-pong
@@ -1895,10 +1942,12 @@ static int launch_c2r(hc_ctx *c, int nf, C2RArgsT<R> &X)
 {
     if (nf < 1 || nf > 4) return set_err(HC_ERR_ARG, "launch_c2r: %d fields", nf);
     const View<R> V = view<R>(c);
-    X.ny = c->ny; X.tw_nl = V.tw_nl; X.tw_nx = V.tw_nx;
+    X.ny = c->ny; X.nz = c->nz; X.tw_nl = V.tw_nl; X.tw_nx = V.tw_nx;
     const int pts = c->tune.c2r_pts;
     int rows = pts / c->nx / nf > 0 ? pts / c->nx / nf : 1;
-    dim3 g((c->ny + rows - 1) / rows, c->nz);
+    // grid-stride blocks: as many as the CUs hold at once (8 per CU), at most one per tile
+    const long ntiles = (long)((c->ny + rows - 1) / rows) * c->nz;
+    dim3 g((unsigned)std::min<long>(ntiles, 8L * c->ncu));
 #define HC_C1(NX_, P_)                                                                                      \
     if (nf == 4) hipLaunchKernelGGL((k_x_c2r<NX_, 4, R, P_>), g, dim3(kRhsThreads), 0, c->stream, X);      \
     else if (nf == 3) hipLaunchKernelGGL((k_x_c2r<NX_, 3, R, P_>), g, dim3(kRhsThreads), 0, c->stream, X); \
@@ -2446,6 +2495,7 @@ extern "C" int hc_create(const hc_params *prm, hc_ctx **out)
     c->got = P.got;
     c->time_sim = 0.0; c->stps = 0; c->fcur = 0; c->scur = 0;
     hipError_t he = hipGetDevice(&c->device);
+    if (he == hipSuccess) he = hipDeviceGetAttribute(&c->ncu, hipDeviceAttributeMultiprocessorCount, c->device);
     if (he == hipSuccess) he = hipStreamCreateWithFlags(&c->stream, hipStreamNonBlocking);
     if (he != hipSuccess) { delete c; return set_err(HC_ERR_HIP, "HIP device/stream: %s", hipGetErrorString(he)); }
     {
