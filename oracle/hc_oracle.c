@@ -1740,6 +1740,11 @@ void ho_stage_damping(ho_state *S, int id)
     if (id >= 0 && id < 5) rayleigh_damping(S, f[id]);
 }
 void ho_stage_temperature(ho_state *S, int t_its) { assign_rk(S, t_its); interp_t(S); bc_th(S); temperature(S, S->qu); }
+/* calculate_new_p at substep t_its on the current phi and p, after the
+ * Neumann ghosts of phi (update_boundary_p at :316, then :353) */
+void ho_stage_new_p(ho_state *S, int t_its) { assign_rk(S, t_its); bc_p(S, S->phi); calc_new_p(S); }
+/* calculate_uvw at substep t_its on the current phi and u, v, w (:316-320) */
+void ho_stage_project(ho_state *S, int t_its) { assign_rk(S, t_its); bc_p(S, S->phi); calc_uvw(S); bc_uvw(S); }
 double ho_get_scalar(ho_state *S, int id)
 {
     switch (id) {

@@ -74,6 +74,8 @@ def lib():
         L.ho_stage_bc_uvw.argtypes = [vp]
         L.ho_stage_temperature.argtypes = [vp, ip]
         L.ho_stage_damping.argtypes = [vp, ip]
+        L.ho_stage_new_p.argtypes = [vp, ip]
+        L.ho_stage_project.argtypes = [vp, ip]
         L.ho_get_spectrum.argtypes = [vp, ip, dp]
         L.ho_set_spectrum.argtypes = [vp, ip, dp]
         L.ho_stage_momentum_spec.argtypes = [vp, ip]
