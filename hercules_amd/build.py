@@ -30,10 +30,50 @@ def _run(cmd, cwd=None):
     return r
 
 
-def build_hip(verbose=False):
-    src = os.path.join(PKG, "csrc", "hc_core.hip")
+# Translation units of the library: each kernel family compiles on its own,
+# in parallel, and the objects are linked into one shared library.
+SOURCES = ["hc_api.hip", "hc_xfer.hip", "hc_rhs.hip", "hc_rhs_t.hip", "hc_columns.hip", "hc_spec.hip",
+           "hc_spec_col.hip", "hc_z6.hip", "hc_adj.hip"]
+CSRC = os.path.join(PKG, "csrc")
+OBJ = os.path.join(PKG, "_obj")
+
+
+def _stale(src, obj, deps):
+    if not os.path.exists(obj):
+        return True
+    t = os.path.getmtime(obj)
+    return any(os.path.getmtime(d) > t for d in [src, *deps])
+
+
+def build_hip(verbose=False, incremental=False, jobs=None):
+    """Compile every unit (or, with incremental=True, those whose source or any
+    header changed since their object was built) and link the library."""
+    os.makedirs(OBJ, exist_ok=True)
+    deps = [os.path.join(CSRC, f) for f in os.listdir(CSRC) if f.endswith(".h")]
+    deps.append(os.path.join(ROOT, "include", "hercules_amd.h"))
+    objs, todo = [], []
+    for f in SOURCES:
+        src = os.path.join(CSRC, f)
+        obj = os.path.join(OBJ, f.replace(".hip", ".o"))
+        objs.append(obj)
+        if not incremental or _stale(src, obj, deps):
+            todo.append((src, obj))
+    flags = [x for x in HIP_FLAGS if x != "-shared"]
+
+    def one(so):
+        src, obj = so
+        tmp = obj + ".tmp.o"
+        _run([HIPCC, *flags, "-c", "-o", tmp, src])
+        os.replace(tmp, obj)
+        return src
+
+    from concurrent.futures import ThreadPoolExecutor
+    with ThreadPoolExecutor(max_workers=jobs or min(len(todo) or 1, os.cpu_count() or 4)) as ex:
+        for src in ex.map(one, todo):
+            if verbose:
+                print("compiled", os.path.basename(src))
     tmp = LIB + ".tmp"
-    _run([HIPCC, *HIP_FLAGS, "-o", tmp, src])
+    _run([HIPCC, "--offload-arch=gfx950", "-shared", "-fPIC", "-o", tmp, *objs])
     os.replace(tmp, LIB)
     if verbose:
         print("built", LIB)
@@ -66,4 +106,8 @@ def build_all(verbose=False):
 
 
 if __name__ == "__main__":
-    build_all(verbose=True)
+    import sys
+    if "--incremental" in sys.argv:
+        build_hip(verbose=True, incremental=True)
+    else:
+        build_all(verbose=True)

@@ -1,0 +1,332 @@
+// Momentum right-hand side of the FD substep (get_momentum_cnvdiff,
+// src/module_cnvdiff.f90:35-374) fused with the x-direction R2C of qu, qv,
+// qw: the two-phase LDS tile (nx <= 256), the staged-row kernel (nx = 512),
+// the global-memory kernel of damped substeps and of the 6th-order z
+// operators, and the Rayleigh damping means.
+#include "hc_host.h"
+
+// Momentum RHS + x R2C of qu,qv,qw.  Block: full rows j0..j0+TY-1 of level
+// k, one point per thread; every row the stencils reach is staged in LDS
+// once (coalesced), then the same LDS holds the q rows for the FFT.
+template <int NX>
+constexpr int rhs_ty()
+{
+    return NX == 16 ? 16 : NX == 32 ? 32 : NX == 64 ? 16 : NX == 128 ? 8 : NX == 192 ? 4 : NX == 256 ? 4 : 1;
+}
+
+template <int NX>
+constexpr int rhs_threads() { return NX * rhs_ty<NX>() <= 512 ? NX * rhs_ty<NX>() : NX * rhs_ty<NX>() / 2; }
+
+template <int CDS, int NX, class R>
+__global__ __launch_bounds__(rhs_threads<NX>()) void k_rhs_mom(StepArgsT<R> A)
+{
+    using C = cplx<R>;
+    constexpr int TY = rhs_ty<NX>();
+    constexpr int T = rhs_threads<NX>(), PP = NX * TY / T, NL = NX / 2;
+    using Tl = RhsTile<NX, TY>;
+    static_assert(Tl::NROWS * NX >= 3 * TY * NX, "FFT rows must fit in the staging area");
+    __shared__ __align__(16) R sm[Tl::NROWS * NX];
+    const int tid = threadIdx.x;
+    const int nty = A.ny / TY;
+    const int logical = xcd_remap(blockIdx.x, nty * A.nz);
+    const int jt = logical % nty, k = logical / nty, j0 = jt * TY;
+    for (int q = tid; q < Tl::NROWS * NX; q += T) {
+        int r = q / NX, i = q - r * NX;
+        sm[q] = rhs_stage_value<NX, TY>(A, r, i, j0, k);
+    }
+    const R dpx = A.dp_opt == 2 ? R(A.scal[0]) : A.dpx;
+    PointVals<R> pv[PP];
+#pragma unroll
+    for (int pp = 0; pp < PP; ++pp) {
+        const int pt = tid + pp * T;
+        pv[pp].load(A, pt % NX, j0 + pt / NX, k);
+    }
+    __syncthreads();
+    R qs[PP][3];
+#pragma unroll
+    for (int pp = 0; pp < PP; ++pp) {
+        const int pt = tid + pp * T, i = pt % NX, jj = pt / NX, j = j0 + jj;
+        Ld<NX, TY, R> q(A, sm, i, jj, j, k, pv[pp]);
+        R f, g, h;
+        momentum_point<CDS>(q, dpx, f, g, h, qs[pp][0], qs[pp][1], qs[pp][2]);
+        const size_t c = ((size_t)k * A.ny + j) * NX + i;
+        A.ff[c] = f;
+        A.gg[c] = g;
+        A.hh[c] = h;
+    }
+    __syncthreads();  // staging no longer read: reuse it for the q rows
+#pragma unroll
+    for (int pp = 0; pp < PP; ++pp) {
+        const int pt = tid + pp * T, i = pt % NX, jj = pt / NX;
+        sm[(0 * TY + jj) * NX + i] = qs[pp][0];
+        sm[(1 * TY + jj) * NX + i] = qs[pp][1];
+        sm[(2 * TY + jj) * NX + i] = qs[pp][2];
+    }
+    __syncthreads();
+    C *sq_ = reinterpret_cast<C *>(sm);
+    lds_fft<NL, 3 * TY, NL, 1, T, -1>(sq_, A.tw_nl, tid);
+    C *outs[3] = {A.s0, A.s1, A.s2};
+    for (int p = tid; p < 3 * TY * NL; p += T) {
+        int row = p / NL, l = p - row * NL, fi = row / TY, r = row - fi * TY;
+        outs[fi][((size_t)k * A.ny + j0 + r) * NL + l] = r2c_post<NL>(sq_ + row * NL, l, A.tw_nx, A.norm);
+    }
+}
+
+
+template <int CDS, int NX, class R>
+__global__ __launch_bounds__(rhs2_threads<NX>(), (rhs2_waves<NX, R>())) void k_rhs_mom2(StepArgsT<R> A)
+{
+    using C = cplx<R>;
+    constexpr int TY = rhs2_ty<NX>();
+    constexpr int T = rhs2_threads<NX>(), NL = NX / 2;
+    using Tl = Tile2<NX, TY>;
+    static_assert(Tl::NRAW >= 3 * TY, "FFT rows must fit in the staging area");
+    constexpr int NFWV = (TY + 3) * NX, PF = (NFWV + T - 1) / T;
+    __shared__ __align__(16) R sm[Tl::NROWS * NX];
+    __shared__ C tws[NX];  // twiddles of the x transform: half-length, then R2C post
+    const int tid = threadIdx.x;
+    C tw_a = czero_t<C>(), tw_b = czero_t<C>();
+    if (tid < NL) { tw_a = A.tw_nl[tid]; tw_b = A.tw_nx[tid]; }
+    const int nty = A.ny / TY;
+    // tile order: groups of G consecutive y tiles, levels fastest-changing
+    // across groups, so that an XCD's L2 holds both the y halos and the
+    // neighbouring levels of the tiles in flight
+    const int G = A.rgroup > 0 ? A.rgroup : (nty % 16 == 0) ? 16 : (nty % 8 == 0) ? 8 : (nty % 4 == 0) ? 4 : 1;
+    const int logical = xcd_remap(blockIdx.x, nty * A.nz);
+    const int grp = logical / (G * A.nz), rem = logical - grp * (G * A.nz);
+    const int k = rem / G, jt = grp * G + rem % G, j0 = jt * TY;
+    const int i = tid % NX, jj = tid / NX, j = j0 + jj;
+    const bool top = k == A.nz - 1;
+    // staging: every thread issues all of its loads before the first use
+    constexpr int NST = (Tl::NRAW * NX + T - 1) / T;
+    R stv[NST];
+#pragma unroll
+    for (int m = 0; m < NST; ++m) {
+        const int q = tid + m * T;
+        stv[m] = R(0.0);
+        if ((Tl::NRAW * NX) % T == 0 || q < Tl::NRAW * NX) {
+            const int r = q / NX, x = q - r * NX;
+            const R *f = r < Tl::OV ? A.u : r < Tl::OW ? A.v : r < Tl::OP ? A.w : A.p;
+            const int jr = r < Tl::OP ? j0 - 3 + r % Tl::RU : j0 - 1 + (r - Tl::OP);
+            stv[m] = f[((unsigned)k * (unsigned)A.ny + (unsigned)wrapi(jr, A.ny)) * (unsigned)NX + (unsigned)x];
+        }
+    }
+    PointVals2<R> pv;
+    pv.load(A, i, j, k);
+#pragma unroll
+    for (int m = 0; m < NST; ++m) {
+        const int q = tid + m * T;
+        if ((Tl::NRAW * NX) % T == 0 || q < Tl::NRAW * NX) sm[q] = stv[m];
+    }
+    // v(k+1) under the why*vhz rows, loaded with the staging
+    R vkp[PF];
+#pragma unroll
+    for (int m = 0; m < PF; ++m) {
+        const int e = tid + m * T;
+        vkp[m] = R(0.0);
+        if ((NFWV % T == 0 || e < NFWV) && !top) {
+            const int r = e / NX, x = e - r * NX;
+            vkp[m] = A.v[((size_t)(k + 1) * A.ny + wrapi(j0 - 2 + r, A.ny)) * NX + x];
+        }
+    }
+    const R dpx = A.dp_opt == 2 ? R(A.scal[0]) : A.dpx;
+    const R um = A.u_mrf;
+    const R lt = top ? R(0.0) : A.lev[LV_LT * A.nz + k];
+    if (tid < NL) { tws[tid] = tw_a; tws[NL + tid] = tw_b; }
+    __syncthreads();
+    // ---- phase 1: interpolants and flux products of level k
+    {
+        const R *ru = sm + (Tl::OU + jj + 3) * NX, *rw = sm + (Tl::OW + jj + 3) * NX;
+        const R uhx = ixm<CDS, NX>(ru, i);
+        const R whx = top ? R(0.0) : ixp<CDS, NX>(rw, i);
+        const R uc = ru[i];
+        const R uhz = top ? (A.ochannel == 0 ? -um : uc) : pv.u_kp * lt + uc * (R(1.0) - lt);
+        sm[(Tl::OUHX + jj) * NX + i] = uhx;
+        sm[(Tl::OWHX + jj) * NX + i] = whx;
+        sm[(Tl::OFUW + jj) * NX + i] = uhz * whx;
+    }
+    for (int e = tid; e < Tl::NH1 * NX; e += T) {
+        const int r = e / NX, x = e - r * NX;
+        if (r < TY + 1) {  // vhx, rows j0-1+r
+            sm[(Tl::OVHX + r) * NX + x] = ixp<CDS, NX>(sm + (Tl::OV + r + 2) * NX, x);
+        } else {           // vhy, rows j0-1+r'
+            const int r2 = r - (TY + 1);
+            sm[(Tl::OVHY + r2) * NX + x] = iym<CDS, NX>(sm + (Tl::OV + r2 + 2) * NX + x);
+        }
+    }
+    for (int e = tid; e < (TY + 3) * NX; e += T) {  // uhy*vhx, rows j0-2+r
+        const int r = e / NX, x = e - r * NX;
+        const R uhy = iyp<CDS, NX>(sm + (Tl::OU + r + 1) * NX + x);
+        const R vhx = ixp<CDS, NX>(sm + (Tl::OV + r + 1) * NX, x);
+        sm[(Tl::OFUV + r) * NX + x] = uhy * vhx;
+    }
+#pragma unroll
+    for (int m = 0; m < PF; ++m) {  // why*vhz, rows j0-2+r
+        const int e = tid + m * T;
+        if (NFWV % T == 0 || e < NFWV) {
+            const int r = e / NX, x = e - r * NX;
+            const R vc = sm[(Tl::OV + r + 1) * NX + x];
+            const R why = top ? R(0.0) : iyp<CDS, NX>(sm + (Tl::OW + r + 1) * NX + x);
+            const R vhz = top ? (A.ochannel == 0 ? R(0.0) : vc) : vkp[m] * lt + vc * (R(1.0) - lt);
+            sm[(Tl::OFWV + r) * NX + x] = vhz * why;
+        }
+    }
+    __syncthreads();
+    // ---- phase 2
+    R qu, qv, qw;
+    {
+        R f, g, h;
+        momentum_tile<CDS, NX, TY>(A, sm, i, jj, k, pv, dpx, f, g, h, qu, qv, qw);
+        const size_t c = ((size_t)k * A.ny + j) * NX + i;
+        A.ff[c] = f;
+        A.gg[c] = g;
+        A.hh[c] = h;
+    }
+    __syncthreads();  // staging no longer read: reuse it for the q rows
+    sm[(0 * TY + jj) * NX + i] = qu;
+    sm[(1 * TY + jj) * NX + i] = qv;
+    sm[(2 * TY + jj) * NX + i] = qw;
+    __syncthreads();
+    C *sq_ = reinterpret_cast<C *>(sm);
+    lds_fft<NL, 3 * TY, NL, 1, T, -1>(sq_, tws, tid);
+    for (int p = tid; p < 3 * TY * NL; p += T) {
+        int row = p / NL, l = p - row * NL, fi = row / TY, r = row - fi * TY;
+        C *o = fi == 0 ? A.s0 : fi == 1 ? A.s1 : A.s2;
+        o[((size_t)k * A.ny + j0 + r) * NL + l] = r2c_post<NL>(sq_ + row * NL, l, tws + NL, A.norm);
+    }
+}
+
+// Momentum RHS + x R2C of qu,qv,qw through a global-memory accessor: PtD in
+// a damped substep, Pt with the 6th-order z operators (ZO = 6).  Block:
+// rows_per_block full rows of level k.
+template <int CDS, int NX, class Acc = PtD, int ZO = 2>
+__global__ __launch_bounds__(kRhsThreads) void k_rhs_mom_damped(StepArgs A)
+{
+    constexpr int ROWS = rows_per_block<NX>();
+    constexpr int NL = NX / 2;
+    __shared__ double2 sq_[3 * ROWS * NL];
+    double *sqd = reinterpret_cast<double *>(sq_);
+    const int k = blockIdx.y, j0 = blockIdx.x * ROWS, tid = threadIdx.x;
+    const double dpx = A.dp_opt == 2 ? A.scal[0] : A.dpx;
+    for (int p = tid; p < ROWS * NX; p += kRhsThreads) {
+        int jj = p / NX, i = p - jj * NX, j = j0 + jj;
+        double qu = 0.0, qv = 0.0, qw = 0.0;
+        if (j < A.ny) {
+            Acc q(A, i, j, k);
+            double f, g, h;
+            momentum_point<CDS, ZO>(q, dpx, f, g, h, qu, qv, qw);
+            const size_t c = ((size_t)k * A.ny + j) * NX + i;
+            A.ff[c] = f;
+            A.gg[c] = g;
+            A.hh[c] = h;
+        }
+        sqd[(0 * ROWS + jj) * NX + i] = qu;
+        sqd[(1 * ROWS + jj) * NX + i] = qv;
+        sqd[(2 * ROWS + jj) * NX + i] = qw;
+    }
+    __syncthreads();
+    lds_fft<NL, 3 * ROWS, NL, 1, kRhsThreads, -1>(sq_, A.tw_nl, tid);
+    double2 *outs[3] = {A.s0, A.s1, A.s2};
+    for (int p = tid; p < 3 * ROWS * NL; p += kRhsThreads) {
+        int row = p / NL, l = p - row * NL, fi = row / ROWS, jj = row - fi * ROWS, j = j0 + jj;
+        if (j < A.ny) outs[fi][((size_t)k * A.ny + j) * NL + l] = r2c_post<NL>(sq_ + row * NL, l, A.tw_nx, A.norm);
+    }
+}
+
+// Plane means of the damped levels, one block per (level, field):
+// the mean_local of rayleigh_damping (src/module_tools.f90:718-732).
+struct DampFields {
+    const double *f[5];
+};
+__global__ __launch_bounds__(256) void k_damp_means(DampFields F, double *dmp_m, int nx, int ny, int nz, int kdamp0)
+{
+    __shared__ double red[256];
+    const int kk = kdamp0 + blockIdx.x, fid = blockIdx.y;
+    const double *plane = F.f[fid] + (size_t)kk * nx * ny;
+    double s = 0.0;
+    for (int q = threadIdx.x; q < nx * ny; q += 256) s += plane[q];
+    red[threadIdx.x] = s;
+    __syncthreads();
+    for (int w = 128; w > 0; w >>= 1) {
+        if (threadIdx.x < w) red[threadIdx.x] += red[threadIdx.x + w];
+        __syncthreads();
+    }
+    if (threadIdx.x == 0) dmp_m[fid * nz + kk] = red[0] / nx / ny;
+}
+
+// In-place damping of one field's interior (the temperature when no
+// temperature equation overwrites it later in the substep).
+__global__ __launch_bounds__(256) void k_damp_apply(double *f, const double *dmp_s, const double *mean, int nx,
+                                                     int ny, int kdamp0, int nlev)
+{
+    const size_t n = (size_t)nx * ny * nlev;
+    for (size_t q = blockIdx.x * (size_t)blockDim.x + threadIdx.x; q < n; q += (size_t)gridDim.x * blockDim.x) {
+        const int kk = kdamp0 + (int)(q / ((size_t)nx * ny));
+        const size_t c = (size_t)kdamp0 * nx * ny + q;
+        const double v = f[c], s = dmp_s[kk], m = mean[kk];
+        f[c] = v - s * (v - m);
+    }
+}
+
+
+// The momentum right-hand side of one substep into B.ff, gg, hh and the x
+// spectra of qu, qv, qw (B.s0..s2).  A damped substep (isdamp = 1) first
+// forms the plane means of the undamped fields (and damps a passive t in
+// place); the stencils then apply the damping as they read.
+template <class R>
+int rhs_momentum(hc_ctx *c, const StepArgs &A, StepArgsT<R> &B)
+{
+    const hc_params &P = c->prm;
+    const int rows = rows_for(c->nx);
+    const dim3 grow((c->ny + rows - 1) / rows, c->nz);
+    const bool damp = P.isdamp == 1, z6 = P.zorder == 6;
+    if constexpr (std::is_same<R, double>::value) {
+        if (damp) {
+            // rayleigh_damping of u, v, w, p, t at the start of the substep
+            // (src/module_navier_stokes.f90:258-266)
+            DampFields F = {{c->u, c->v, c->w, c->p, c->t}};
+            hipLaunchKernelGGL(k_damp_means, dim3(P.nzdamp + 1, 5), dim3(256), 0, c->stream, F, c->dmp_m, c->nx, c->ny,
+                               c->nz, c->kdamp0);
+            if (P.isscalar == 0) {
+                hipLaunchKernelGGL(k_damp_apply, dim3(256), dim3(256), 0, c->stream, c->t, c->dmp_s,
+                                   c->dmp_m + 4 * c->nz, c->nx, c->ny, c->kdamp0, P.nzdamp + 1);
+            }
+            B.dmask = 31;
+#define HC_RHSD(NX_)                                                                                         \
+    if (P.cds == 1) hipLaunchKernelGGL((k_rhs_mom_damped<1, NX_>), grow, dim3(kRhsThreads), 0, c->stream, B); \
+    else hipLaunchKernelGGL((k_rhs_mom_damped<2, NX_>), grow, dim3(kRhsThreads), 0, c->stream, B);
+            HC_DISPATCH_N(c->nx, HC_RHSD)
+#undef HC_RHSD
+        } else if (z6) {  // 6th-order z operators: the global-memory point kernel
+            B.dmask = 0;
+            zinterp6(c, A, B);
+#define HC_RHSZ(NX_)                                                                                              \
+    if (P.cds == 1) hipLaunchKernelGGL((k_rhs_mom_damped<1, NX_, Pt, 6>), grow, dim3(kRhsThreads), 0, c->stream, B); \
+    else hipLaunchKernelGGL((k_rhs_mom_damped<2, NX_, Pt, 6>), grow, dim3(kRhsThreads), 0, c->stream, B);
+            HC_DISPATCH_N(c->nx, HC_RHSZ)
+#undef HC_RHSZ
+        }
+    }
+    if (!damp && !z6) {
+#define HC_RHS(NX_)                                                                                     \
+    if constexpr (NX_ <= 256) {                                                                          \
+        constexpr int TY_ = rhs2_ty<NX_>();                                                              \
+        dim3 g1((unsigned)(c->ny / TY_ * c->nz));                                                        \
+        constexpr int T_ = rhs2_threads<NX_>();                                                          \
+        if (P.cds == 1) hipLaunchKernelGGL((k_rhs_mom2<1, NX_, R>), g1, dim3(T_), 0, c->stream, B);      \
+        else hipLaunchKernelGGL((k_rhs_mom2<2, NX_, R>), g1, dim3(T_), 0, c->stream, B);                 \
+    } else {                                                                                             \
+        constexpr int TY_ = rhs_ty<NX_>();                                                               \
+        dim3 g1((unsigned)(c->ny / TY_ * c->nz));                                                        \
+        constexpr int T_ = rhs_threads<NX_>();                                                           \
+        if (P.cds == 1) hipLaunchKernelGGL((k_rhs_mom<1, NX_, R>), g1, dim3(T_), 0, c->stream, B);       \
+        else hipLaunchKernelGGL((k_rhs_mom<2, NX_, R>), g1, dim3(T_), 0, c->stream, B);                  \
+    }
+        HC_DISPATCH_N(c->nx, HC_RHS)
+#undef HC_RHS
+    }
+    return 0;
+}
+
+template int rhs_momentum<double>(hc_ctx *, const StepArgs &, StepArgsT<double> &);
+template int rhs_momentum<float>(hc_ctx *, const StepArgs &, StepArgsT<float> &);
