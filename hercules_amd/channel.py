@@ -32,7 +32,9 @@ ABI_SYMBOLS = ("hc_last_error", "hc_create", "hc_destroy", "hc_set_field", "hc_g
                "hc_ensemble_create", "hc_ensemble_steps", "hc_ensemble_graph_used", "hc_ensemble_destroy",
                "hc_substep_fd", "hc_set_adjoint", "hc_get_adjoint", "hc_adjoint_substep_fd",
                "hc_wall_shear_gradient", "hc_autotune", "hc_set_tuning", "hc_mean_fields", "hc_get_mean_field",
-               "hc_get_mesh", "hc_screen_cfl", "hc_get_column")
+               "hc_get_mesh", "hc_screen_cfl", "hc_get_column", "hc_calculate_ubar_of", "hc_update_ri",
+               "hc_screen_div_error_ref", "hc_screen_substeps", "hc_get_screen", "hc_get_momentum_cnvdiff",
+               "hc_get_temperature_cnvdiff", "hc_get_p_eqn_src", "hc_calculate_uvw", "hc_calculate_new_p")
 # statistics of calculate_mean_fields, in the order of output_mean_fields
 MEANS = ("u", "v", "w", "t", "uu", "vv", "ww", "uw", "vw", "tt", "tw")
 KERNELS = {"rhs": 0, "yfft_fwd": 1, "column": 2, "yfft_inv": 3, "finish": 4, "rhs_t": 5,
@@ -96,6 +98,17 @@ def load_library(path: str = LIB_PATH):
     L.hc_wall_shear_gradient.argtypes = [vp, ip, ip, dp]
     L.hc_profile_enable.argtypes = [vp, ip]
     L.hc_profile_read.argtypes = [vp, ip, dp, C.POINTER(C.c_long)]
+    L.hc_calculate_ubar_of.argtypes = [vp, vp, ip, dp]
+    L.hc_update_ri.argtypes = [vp, ip]
+    L.hc_screen_div_error_ref.argtypes = [vp, ip, dp, dp, dp]
+    L.hc_screen_substeps.argtypes = [vp, ip]
+    L.hc_get_screen.argtypes = [vp, ip, dp, dp, dp]
+    d = C.c_double
+    L.hc_get_momentum_cnvdiff.argtypes = [vp, d, d, d] + [vp] * 14 + [ip]
+    L.hc_get_temperature_cnvdiff.argtypes = [vp, d, d, d] + [vp] * 7 + [ip]
+    L.hc_get_p_eqn_src.argtypes = [vp] + [vp] * 4 + [ip]
+    L.hc_calculate_uvw.argtypes = [vp, d] + [vp] * 4 + [ip]
+    L.hc_calculate_new_p.argtypes = [vp, d, vp, vp, ip]
     _lib = L
     return L
 
@@ -263,10 +276,90 @@ class Channel:
         self._check(self.L.hc_calculate_ubar(self.h, C.byref(v)))
         return v.value
 
+    def calculate_ubar_of(self, u, layout=LAYOUT_INTERIOR):
+        """calculate_ubar(u): bulk velocity of the given u, kept as the module's ubar."""
+        a = self._arr(u, layout)
+        v = C.c_double()
+        self._check(self.L.hc_calculate_ubar_of(self.h, _ptr(a), layout, C.byref(v)))
+        return v.value
+
+    def update_ri(self, stps):
+        """update_ri(stps): got = ri_str + (ri_end - ri_str) / imax * stps."""
+        self._check(self.L.hc_update_ri(self.h, int(stps)))
+
     def screen_div_error(self):
+        """(mean |div|, largest |div|) of the current fields."""
         a, m = C.c_double(), C.c_double()
         self._check(self.L.hc_screen_div_error(self.h, C.byref(a), C.byref(m)))
         return a.value, m.value
+
+    def screen_div_error_ref(self, isfinal=1):
+        """The reference's (div_avg, signed div_max, div_hist_max)."""
+        a, m, h = C.c_double(), C.c_double(), C.c_double()
+        self._check(self.L.hc_screen_div_error_ref(self.h, int(isfinal), C.byref(a), C.byref(m), C.byref(h)))
+        return a.value, m.value, h.value
+
+    def screen_substeps(self, on=True):
+        """screen_div_error after every substep; NaN stops the step (HerculesError code 4)."""
+        self._check(self.L.hc_screen_substeps(self.h, 1 if on else 0))
+
+    def get_screen(self, substep):
+        """(div_avg, div_max, div_hist_max) recorded after `substep` of the last step."""
+        a, m, h = C.c_double(), C.c_double(), C.c_double()
+        self._check(self.L.hc_get_screen(self.h, int(substep), C.byref(a), C.byref(m), C.byref(h)))
+        return a.value, m.value, h.value
+
+    # ---- per-stage subroutines with the reference's argument lists
+    def _arr(self, x, layout):
+        shape = self.shape if layout == LAYOUT_INTERIOR else self.ghost_shape
+        a = np.ascontiguousarray(x, dtype=np.float64)
+        if a.shape != shape:
+            raise ValueError(f"shape {a.shape} != {shape}")
+        return a
+
+    def _out(self, layout):
+        return np.zeros(self.shape if layout == LAYOUT_INTERIOR else self.ghost_shape)
+
+    def get_momentum_cnvdiff(self, rk, u, v, w, p, t, ff0, gg0, hh0, layout=LAYOUT_INTERIOR):
+        """get_momentum_cnvdiff with rk = (rk_a, rk_b, rk_c); returns ff, gg, hh, qu, qv, qw."""
+        ins = [self._arr(x, layout) for x in (u, v, w, p)]
+        tt = None if t is None else self._arr(t, layout)
+        olds = [self._arr(x, layout) for x in (ff0, gg0, hh0)]
+        outs = [self._out(layout) for _ in range(6)]
+        ff, gg, hh, qu, qv, qw = outs
+        self._check(self.L.hc_get_momentum_cnvdiff(
+            self.h, float(rk[0]), float(rk[1]), float(rk[2]), *(_ptr(a) for a in ins),
+            None if tt is None else _ptr(tt), _ptr(ff), _ptr(olds[0]), _ptr(gg), _ptr(olds[1]), _ptr(hh),
+            _ptr(olds[2]), _ptr(qu), _ptr(qv), _ptr(qw), layout))
+        return ff, gg, hh, qu, qv, qw
+
+    def get_temperature_cnvdiff(self, rk, u, v, w, t, ss0, layout=LAYOUT_INTERIOR):
+        """get_temperature_cnvdiff; returns ss, qt."""
+        ins = [self._arr(x, layout) for x in (u, v, w, t)]
+        s0 = self._arr(ss0, layout)
+        ss, qt = self._out(layout), self._out(layout)
+        self._check(self.L.hc_get_temperature_cnvdiff(self.h, float(rk[0]), float(rk[1]), float(rk[2]),
+                                                      *(_ptr(a) for a in ins), _ptr(ss), _ptr(s0), _ptr(qt), layout))
+        return ss, qt
+
+    def get_p_eqn_src(self, u, v, w, layout=LAYOUT_INTERIOR):
+        ins = [self._arr(x, layout) for x in (u, v, w)]
+        qp = self._out(layout)
+        self._check(self.L.hc_get_p_eqn_src(self.h, *(_ptr(a) for a in ins), _ptr(qp), layout))
+        return qp
+
+    def calculate_uvw(self, rk_c, u, v, w, phi, layout=LAYOUT_INTERIOR):
+        """calculate_uvw: returns the projected (u, v, w) (copies of the inputs updated)."""
+        uvw = [self._arr(x, layout).copy() for x in (u, v, w)]
+        ph = self._arr(phi, layout)
+        self._check(self.L.hc_calculate_uvw(self.h, float(rk_c), *(_ptr(a) for a in uvw), _ptr(ph), layout))
+        return tuple(uvw)
+
+    def calculate_new_p(self, rk_c, phi, p, layout=LAYOUT_INTERIOR):
+        ph = self._arr(phi, layout)
+        pp = self._arr(p, layout).copy()
+        self._check(self.L.hc_calculate_new_p(self.h, float(rk_c), _ptr(ph), _ptr(pp), layout))
+        return pp
 
     def wall_shear(self):
         """(tau_bottom, tau_top): plane-averaged nu du/dz at the walls."""

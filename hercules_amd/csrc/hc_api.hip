@@ -646,6 +646,15 @@ static int substep(hc_ctx *c, int t_its)
     return c->mixed ? substep_t<float>(c, t_its, view<float>(c)) : substep_t<double>(c, t_its, view<double>(c));
 }
 
+int ubar_of(hc_ctx *c, const double *u)
+{
+    hipLaunchKernelGGL(k_ubar_partial<double>, dim3(c->nz), dim3(256), 0, c->stream, u, c->lev, c->nx, c->ny, c->nz,
+                       c->part);
+    hipLaunchKernelGGL(k_ubar_final, dim3(1), dim3(64), 0, c->stream, c->part, c->nz, c->prm.lz, c->nx, c->ny, c->scal);
+    HIPCHK(hipGetLastError());
+    return 0;
+}
+
 static int ubar_launch(hc_ctx *c)
 {
     if (c->mixed)
@@ -914,6 +923,9 @@ static int get_field_t(hc_ctx *c, const R *d, double *dst, int layout)
     return 0;
 }
 
+int put_f64(hc_ctx *c, double *dev, const double *src, int layout) { return set_field_t<double>(c, dev, src, layout); }
+int get_f64(hc_ctx *c, const double *dev, double *dst, int layout) { return get_field_t<double>(c, dev, dst, layout); }
+
 extern "C" int hc_set_field(hc_ctx *c, int field, const double *src, int layout)
 {
     if (!c || !src) return set_err(HC_ERR_ARG, "null argument");
@@ -995,36 +1007,69 @@ static int mean_accumulate(hc_ctx *c)
     return 0;
 }
 
+// The substeps of one step (time_advancement_fd / _sp) and
+// calculate_mean_fields; with the screen on, screen_div_error(u,v,w,1,.)
+// after each substep (src/module_navier_stokes.f90:344-350, 505-509) into
+// record slot t_its.
+static int step_body(hc_ctx *c)
+{
+    const hc_params &P = c->prm;
+    const int nsub = P.dts == 2 ? 3 : 1;
+    for (int s = 1; s <= nsub; ++s) {
+        if (int e = P.cds == 3 ? substep_sp(c, s) : substep(c, s)) return e;
+        if (c->screen)
+            if (int e = screen_queue(c, s, 1)) return e;
+    }
+    return mean_accumulate(c);
+}
+
+// After a screened step: the reference stops at the end of the first
+// substep whose fields hold a NaN ('NaN error!', src/module_io.f90:135-140).
+static int screen_check(hc_ctx *c)
+{
+    if (!c->screen) return 0;
+    double rec[16];
+    HIPCHK(hipMemcpyAsync(rec, c->scr + 4, sizeof rec, hipMemcpyDeviceToHost, c->stream));
+    HIPCHK(hipStreamSynchronize(c->stream));
+    const int nsub = c->prm.dts == 2 ? 3 : 1;
+    for (int s = 1; s <= nsub; ++s)
+        if (rec[4 * s + 3] != 0.0)
+            return set_err(HC_ERR_NAN, "NaN error! (substep %d of step %d)", s, c->stps);
+    return 0;
+}
+
 static int one_step(hc_ctx *c)
 {
     const hc_params &P = c->prm;
     c->stps += 1;
     c->time_sim += P.dt;
     if (P.is_ri_var == 1) c->got = P.ri_str + (P.ri_end - P.ri_str) * 1.0 / P.imax * c->stps;
-    int nsub = P.dts == 2 ? 3 : 1;
-    for (int s = 1; s <= nsub; ++s)
-        if (int e = P.cds == 3 ? substep_sp(c, s) : substep(c, s)) return e;
-    return mean_accumulate(c);
+    return step_body(c);
 }
 
 extern "C" int hc_time_advancement_fd(hc_ctx *c)
 {
     if (!c) return set_err(HC_ERR_ARG, "null context");
     if (c->prm.cds == 3) return set_err(HC_ERR_ARG, "cds=3: the spectral path is hc_time_advancement_sp");
-    int nsub = c->prm.dts == 2 ? 3 : 1;
-    for (int s = 1; s <= nsub; ++s)
-        if (int e = substep(c, s)) return e;
-    return mean_accumulate(c);
+    if (int e = step_body(c)) return e;
+    return screen_check(c);
 }
 
 extern "C" int hc_time_advancement_sp(hc_ctx *c)
 {
     if (!c) return set_err(HC_ERR_ARG, "null context");
     if (c->prm.cds != 3) return set_err(HC_ERR_ARG, "cds=%d: time_advancement_sp needs cds=3", c->prm.cds);
-    int nsub = c->prm.dts == 2 ? 3 : 1;
-    for (int s = 1; s <= nsub; ++s)
-        if (int e = substep_sp(c, s)) return e;
-    return mean_accumulate(c);
+    if (int e = step_body(c)) return e;
+    return screen_check(c);
+}
+
+extern "C" int hc_update_ri(hc_ctx *c, int stps)
+{
+    if (!c) return set_err(HC_ERR_ARG, "null context");
+    const hc_params &P = c->prm;
+    if (P.imax < 1) return set_err(HC_ERR_ARG, "update_ri needs imax >= 1");
+    c->got = P.ri_str + (P.ri_end - P.ri_str) * 1.0 / P.imax * stps;
+    return 0;
 }
 
 extern "C" int hc_drive_steps(hc_ctx *c, int nsteps)
@@ -1034,9 +1079,43 @@ extern "C" int hc_drive_steps(hc_ctx *c, int nsteps)
         if (int e = one_step(c)) return e;
         if (c->prm.dp_opt == 2)
             if (int e = ubar_launch(c)) return e;
+        if (int e = screen_check(c)) return e;
     }
     HIPCHK(hipGetLastError());
     return 0;
+}
+
+extern "C" int hc_screen_substeps(hc_ctx *c, int on)
+{
+    if (!c) return set_err(HC_ERR_ARG, "null context");
+    if (on != 0 && on != 1) return set_err(HC_ERR_ARG, "on must be 0 or 1");
+    if (on)
+        if (int e = screen_alloc(c)) return e;
+    c->screen = on;
+    return 0;
+}
+
+extern "C" int hc_get_screen(hc_ctx *c, int substep, double *div_avg, double *div_max, double *div_hist_max)
+{
+    if (!c) return set_err(HC_ERR_ARG, "null context");
+    if (substep < 0 || substep > 3) return set_err(HC_ERR_ARG, "substep must be 0..3");
+    if (!c->scr) return set_err(HC_ERR_ARG, "no screen has run on this context");
+    double rec[4];
+    HIPCHK(hipMemcpyAsync(rec, c->scr + 4 + 4 * substep, sizeof rec, hipMemcpyDeviceToHost, c->stream));
+    HIPCHK(hipStreamSynchronize(c->stream));
+    if (div_avg) *div_avg = rec[0];
+    if (div_max) *div_max = rec[1];
+    if (div_hist_max) *div_hist_max = rec[2];
+    if (rec[3] != 0.0) return set_err(HC_ERR_NAN, "NaN error!");
+    return 0;
+}
+
+extern "C" int hc_screen_div_error_ref(hc_ctx *c, int isfinal, double *div_avg, double *div_max, double *div_hist_max)
+{
+    if (!c) return set_err(HC_ERR_ARG, "null context");
+    if (isfinal != 0 && isfinal != 1) return set_err(HC_ERR_ARG, "isfinal must be 0 or 1");
+    if (int e = screen_queue(c, 0, isfinal)) return e;
+    return hc_get_screen(c, 0, div_avg, div_max, div_hist_max);
 }
 
 extern "C" int hc_get_column(hc_ctx *c, int field, int i, int j, double *dst)
@@ -1389,7 +1468,11 @@ struct hc_ensemble {
 // one pass of the drive loop body for one member (hc_drive_steps' body)
 static int member_step(hc_ctx *c)
 {
-    if (int e = one_step(c)) return e;
+    const int scr = c->screen;
+    c->screen = 0;
+    const int e0 = one_step(c);
+    c->screen = scr;
+    if (e0) return e0;
     if (c->prm.dp_opt == 2)
         if (int e = ubar_launch(c)) return e;
     return 0;
@@ -1630,6 +1713,7 @@ extern "C" int hc_wall_shear_gradient(hc_ctx *c, int nsteps, int ckpt_every, dou
     if (P.dts != 2) return set_err(HC_ERR_UNSUPPORTED, "the gradient driver needs RK3 (dts=2)");
     if (nsteps < 1 || ckpt_every < 1) return set_err(HC_ERR_ARG, "nsteps and ckpt_every must be >= 1");
     if (int e = ensure_adjoint(c)) return e;
+    QuietSteps quiet(c);
     const size_t N = (size_t)c->nx * c->ny * c->nz, fb = N * sizeof(double);
     const int nseg = (nsteps + ckpt_every - 1) / ckpt_every, nsub = 3;
     const int seglen = std::min(ckpt_every, nsteps);
@@ -1755,6 +1839,7 @@ extern "C" int hc_autotune(hc_ctx *c, int steps_per_trial)
         fields.assign(f, f + 13);
     }
     if (P.cds == 3) return 0;  // the spectral path's launch shapes are fixed
+    QuietSteps quiet(c);
     void *save = nullptr;
     HIPCHK(hipMalloc(&save, fields.size() * N * eb + 8 * sizeof(double)));
     char *sv = (char *)save;

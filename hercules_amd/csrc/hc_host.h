@@ -125,6 +125,30 @@ struct hc_ctx {
     size_t ev_used = 0;
     double prof_ms[HC_KERNEL_COUNT] = {};
     long prof_n[HC_KERNEL_COUNT] = {};
+    // divergence screen (screen_div_error after every substep when screen =
+    // 1): scr[0] div_hist_max, scr[4 + 4 s ..] the record of slot s (div_avg,
+    // div_max, div_hist_max, NaN flag; slot 0 on request, 1..3 the substeps);
+    // scr_rows / scr_blk the per-row and per-block partials
+    int screen = 0;
+    double *scr = nullptr, *scr_rows = nullptr, *scr_blk = nullptr;
+};
+
+// Steps taken for tuning or for a gradient's forward run leave the
+// statistics and the divergence screen of the context untouched.
+struct QuietSteps {
+    hc_ctx *c;
+    bool mean;
+    int screen;
+    explicit QuietSteps(hc_ctx *c_) : c(c_), mean(c_->mean_on), screen(c_->screen)
+    {
+        c->mean_on = false;
+        c->screen = 0;
+    }
+    ~QuietSteps()
+    {
+        c->mean_on = mean;
+        c->screen = screen;
+    }
 };
 
 // Record a start (begin=true) or stop event for kernel `kid` when profiling.
@@ -310,3 +334,14 @@ int adjoint_substep(hc_ctx *c, int t_its);
 
 // small kernels (hc_api.hip)
 void dpx_update_launch(hc_ctx *c);
+// calculate_ubar of an FP64 interior field into the device scalar ubar
+int ubar_of(hc_ctx *c, const double *u);
+// FP64 field transfers between a caller's array (host or device, `layout`)
+// and a device interior array
+int put_f64(hc_ctx *c, double *dev, const double *src, int layout);
+int get_f64(hc_ctx *c, const double *dev, double *dst, int layout);
+
+// Divergence screen (hc_screen.hip): queue screen_div_error of the current
+// fields into record slot `slot`; fin = 1 updates div_hist_max.
+int screen_alloc(hc_ctx *c);
+int screen_queue(hc_ctx *c, int slot, int fin);

@@ -1229,3 +1229,42 @@ struct LdT : Interp<LdT<NX, TY, R>> {
     HC_DEV R lev(int tab, int kk) const { return A.lev[tab * A.nz + kk]; }
     HC_DEV size_t center() const { return ((size_t)k * A.ny + j) * NX + i; }
 };
+
+// ---------------------------------------------------------------------------
+// Discrete divergence at interior point (i, j, k) as screen_div_error forms it
+// (src/module_io.f90:103-128): x and y differences of order cds, the z
+// difference (w(k) - w(k-1))/dz(k) with w = 0 below the first level; with
+// zorder = 6 the z derivative is the face-to-centre operator ZO_D.  F holds
+// the fields in their stored precision R, D the FP64 metrics; the spectral
+// path (CDS = 3) reads the divergence formed beforehand from F.aux.
+template <int CDS, class R>
+HC_DEV double div_at(const StepArgsT<R> &F, const StepArgs &D, int i, int j, int k)
+{
+    PtT<R> P{F, i, j, k};
+    if (CDS == 3) return (double)F.aux[P.at(0, 0, k)];
+    auto U = [&](int di) -> double { return (double)P.U(di, 0, 0); };
+    auto V = [&](int dj) -> double { return (double)P.V(0, dj, 0); };
+    double dudx, dvdy;
+    if (CDS == 1) {
+        dudx = (U(0) - U(-1)) / D.dx;
+        dvdy = (V(0) - V(-1)) / D.dy;
+    } else {
+        dudx = (-U(1) + 27.0 * U(0) - 27.0 * U(-1) + U(-2)) / 24.0 / D.dx;
+        dvdy = (-V(1) + 27.0 * V(0) - 27.0 * V(-1) + V(-2)) / 24.0 / D.dy;
+    }
+    double dwdz;
+    if (D.zorder == 6) {
+        const size_t e = (size_t)ZO_D * D.nz + k;
+        const int s0 = D.zst[e];
+        dwdz = 0.0;
+        for (int jz = 0; jz < 6; ++jz) {
+            const int dd = s0 + jz;
+            if (dd > 0 && dd < D.nz) dwdz += D.zwt[e * kZW + jz] * (double)F.w[P.at(0, 0, dd - 1)];
+        }
+    } else {
+        const double w0 = F.w[P.at(0, 0, k)];
+        const double wm = k == 0 ? 0.0 : (double)F.w[P.at(0, 0, k - 1)];
+        dwdz = (w0 - wm) / D.lev[LV_DZ * D.nz + k];
+    }
+    return dudx + dvdy + dwdz;
+}

@@ -44,7 +44,7 @@ program hercules_hip
     type(hc_params) :: prm
     real(c_double), allocatable, target :: f(:,:,:), g(:,:,:)
     real(c_double), allocatable, target :: zc(:), dzk(:), l_t(:), l_b(:)
-    real(c_double) :: div_avg, div_max, cfl_max, ri_now, ubar_now
+    real(c_double) :: cfl_max
     real :: cpu_t
     integer :: stps, indx_bk, indx_m, indx_ins, indx_slc, nsteps_env, q
     real(c_double), allocatable, target :: col(:)
@@ -137,22 +137,16 @@ program hercules_hip
         print *, ' '
         write (*, 184) stps * dt, cpu_t, cfl_max
 184     format(' Time = ', es12.5, '; CPU Time = ', es11.4, '; CFL max = ', f5.2)
-        call hip_check(hc_drive_steps(hip_ctx, 1_c_int), 'drive step')
-        ! screen_div_error after the last substep (:85-206); the library reports
-        ! the largest |div| where the reference keeps a signed running value
-        call hip_check(hc_screen_div_error(hip_ctx, div_avg, div_max), 'screen_div_error')
-        write (*, 105) div_avg, div_max
-105     format(' DIV-ERR FINAL: DIV_AVG = ', es11.4, ';  |DIV|_MAX = ', es11.4)
-        if (is_ri_var .eq. 1) then
-            call hip_check(hc_get_scalar(hip_ctx, 3_c_int, ri_now), 'get_scalar')
-            write (*, 258) ri_now
-258         format(' Ri_b = ', es12.5)
+        ! the drive loop body (src/module_navier_stokes.f90:143-165) through the
+        ! drop-in module: the step prints the divergence lines of every substep
+        ! and stops on a NaN, as screen_div_error does
+        if (is_ri_var .eq. 1) call update_ri(stps)
+        if (cds .eq. 3) then
+            call time_advancement_sp()
+        else
+            call time_advancement_fd()
         endif
-        if (dp_opt .eq. 2) then
-            call hip_check(hc_calculate_ubar(hip_ctx, ubar_now), 'calculate_ubar')
-            write (*, 259) ubar_now + u_mrf
-259         format(' Ubar = ', es12.5)
-        endif
+        if (dp_opt .eq. 2) call hip_calculate_ubar()
         ! time series of the column at the first interior point (ghst+1, ghst+1)
         if (istmsr .eq. 1) then
             call write_series('results/tmsr_u.dat', HC_U, u_mrf, stps * dt)

@@ -8,11 +8,23 @@
 !   time_advancement_fd     src/module_navier_stokes.f90:223-365
 !   time_advancement_sp     src/module_navier_stokes.f90:371-517
 !   poisson_solver_fft      src/module_poisson_solver.f90:483-658
+!   get_momentum_cnvdiff    src/module_cnvdiff.f90:35-374
+!   get_temperature_cnvdiff src/module_cnvdiff.f90:381-473
+!   get_p_eqn_src           src/module_poisson_solver.f90:127-176
+!   calculate_uvw           src/module_navier_stokes.f90:659-734
+!   calculate_new_p         src/module_navier_stokes.f90:739-808
+!   screen_div_error        src/module_io.f90:85-206
 !   calculate_ubar          src/module_parameters.f90:513-546
+!   update_ri               src/module_parameters.f90:580-590
 !   assign_rk_coeff         src/module_parameters.f90:618-641
-! Fields live on the GPU between calls; hip_upload / hip_download move the
-! reference's ghosted (nx+4, ny+4, nz+4) arrays across.  A nonzero status
-! prints the library's message and stops, as the reference does on bad flags.
+! time_advancement_fd / _sp run on the device-resident fields, which
+! hip_upload / hip_download move across as the reference's ghosted
+! (nx+4, ny+4, nz+4) arrays; like the reference they screen the divergence
+! after every substep, print its lines and stop on a NaN.  The per-stage
+! routines take their array arguments as data (copied in and out on each
+! call), for callers that keep the reference's own substep loop.  A nonzero
+! status prints the library's message and stops, as the reference does on
+! bad flags.
 !*************************************************************************
 module module_hercules_hip
 
@@ -148,6 +160,76 @@ module module_hercules_hip
             type(c_ptr), value :: ctx
             integer(c_int) :: rc
         end function
+        function hc_calculate_ubar_of(ctx, u, layout, ubar) bind(C, name="hc_calculate_ubar_of") result(rc)
+            import :: c_ptr, c_int, c_double
+            type(c_ptr), value :: ctx, u
+            integer(c_int), value :: layout
+            real(c_double), intent(out) :: ubar
+            integer(c_int) :: rc
+        end function
+        function hc_update_ri(ctx, stps) bind(C, name="hc_update_ri") result(rc)
+            import :: c_ptr, c_int
+            type(c_ptr), value :: ctx
+            integer(c_int), value :: stps
+            integer(c_int) :: rc
+        end function
+        function hc_screen_substeps(ctx, on) bind(C, name="hc_screen_substeps") result(rc)
+            import :: c_ptr, c_int
+            type(c_ptr), value :: ctx
+            integer(c_int), value :: on
+            integer(c_int) :: rc
+        end function
+        function hc_get_screen(ctx, substep, div_avg, div_max, div_hist_max) bind(C, name="hc_get_screen") result(rc)
+            import :: c_ptr, c_int, c_double
+            type(c_ptr), value :: ctx
+            integer(c_int), value :: substep
+            real(c_double), intent(out) :: div_avg, div_max, div_hist_max
+            integer(c_int) :: rc
+        end function
+        function hc_screen_div_error_ref(ctx, isfinal, div_avg, div_max, div_hist_max) &
+                 bind(C, name="hc_screen_div_error_ref") result(rc)
+            import :: c_ptr, c_int, c_double
+            type(c_ptr), value :: ctx
+            integer(c_int), value :: isfinal
+            real(c_double), intent(out) :: div_avg, div_max, div_hist_max
+            integer(c_int) :: rc
+        end function
+        function hc_get_momentum_cnvdiff(ctx, rk_a, rk_b, rk_c, u, v, w, p, t, ff, ff0, gg, gg0, hh, hh0, &
+                                         qu, qv, qw, layout) bind(C, name="hc_get_momentum_cnvdiff") result(rc)
+            import :: c_ptr, c_int, c_double
+            type(c_ptr), value :: ctx, u, v, w, p, t, ff, ff0, gg, gg0, hh, hh0, qu, qv, qw
+            real(c_double), value :: rk_a, rk_b, rk_c
+            integer(c_int), value :: layout
+            integer(c_int) :: rc
+        end function
+        function hc_get_temperature_cnvdiff(ctx, rk_a, rk_b, rk_c, u, v, w, t, ss, ss0, qt, layout) &
+                 bind(C, name="hc_get_temperature_cnvdiff") result(rc)
+            import :: c_ptr, c_int, c_double
+            type(c_ptr), value :: ctx, u, v, w, t, ss, ss0, qt
+            real(c_double), value :: rk_a, rk_b, rk_c
+            integer(c_int), value :: layout
+            integer(c_int) :: rc
+        end function
+        function hc_get_p_eqn_src(ctx, u, v, w, qp, layout) bind(C, name="hc_get_p_eqn_src") result(rc)
+            import :: c_ptr, c_int
+            type(c_ptr), value :: ctx, u, v, w, qp
+            integer(c_int), value :: layout
+            integer(c_int) :: rc
+        end function
+        function hc_calculate_uvw(ctx, rk_c, u, v, w, phi, layout) bind(C, name="hc_calculate_uvw") result(rc)
+            import :: c_ptr, c_int, c_double
+            type(c_ptr), value :: ctx, u, v, w, phi
+            real(c_double), value :: rk_c
+            integer(c_int), value :: layout
+            integer(c_int) :: rc
+        end function
+        function hc_calculate_new_p(ctx, rk_c, phi, p, layout) bind(C, name="hc_calculate_new_p") result(rc)
+            import :: c_ptr, c_int, c_double
+            type(c_ptr), value :: ctx, phi, p
+            real(c_double), value :: rk_c
+            integer(c_int), value :: layout
+            integer(c_int) :: rc
+        end function
         function c_strlen(s) bind(C, name="strlen") result(n)
             import :: c_ptr, c_size_t
             type(c_ptr), value :: s
@@ -158,6 +240,12 @@ module module_hercules_hip
     ! the device-resident channel and the RK coefficient of the current substep
     type(c_ptr), save :: hip_ctx = c_null_ptr
     real(c_double), save :: hip_rk_a = 0.d0, hip_rk_b = 0.d0, hip_rk_c = 1.d0
+    ! the parameters the channel was created with, and the module's bulk
+    ! velocity (src/module_parameters.f90:205) as calculate_ubar leaves it
+    type(hc_params), save :: hip_prm
+    real(c_double), save :: ubar = 0.d0
+    ! divergence lines after every substep of time_advancement_fd / _sp
+    logical, save :: hip_screen_on = .true.
 
 contains
 
@@ -179,6 +267,64 @@ contains
     subroutine hip_create(prm)
         type(hc_params), intent(in) :: prm
         call hip_check(hc_create(prm, hip_ctx), 'hc_create')
+        hip_prm = prm
+        call hip_screen(hip_screen_on)
+    end subroutine
+
+    ! screen_div_error after every substep of the device step (on by default)
+    subroutine hip_screen(on)
+        logical, intent(in) :: on
+        integer(c_int) :: flag
+        flag = 0
+        if (on) flag = 1
+        call hip_check(hc_screen_substeps(hip_ctx, flag), 'hc_screen_substeps')
+        hip_screen_on = on
+    end subroutine
+
+    ! the lines screen_div_error prints (src/module_io.f90:160-199)
+    subroutine hip_print_div(div_avg, div_max, div_hist_max, isfinal, ishistout)
+        real(c_double), intent(in) :: div_avg, div_max, div_hist_max
+        integer, intent(in) :: isfinal, ishistout
+        real(c_double) :: v
+        if (isfinal .eq. 0) then
+            write (*, 104) div_avg, div_max
+104         format(' DIV-ERR INIT : DIV_AVG = ', es11.4, ';  DIV_MAX = ', es11.4)
+        elseif (ishistout .eq. 0) then
+            write (*, 105) div_avg, div_max
+105         format(' DIV-ERR FINAL: DIV_AVG = ', es11.4, ';  DIV_MAX = ', es11.4)
+        else
+            write (*, 115) div_avg, div_max
+115         format(' DIV-ERR FINAL: DIV_AVG = ', es11.4, ';  DIV_MAX = ', es11.4)
+            write (*, 106) div_hist_max
+106         format(' DIV-ERR HIST_MAX = ', es11.4)
+            if (hip_prm%is_ri_var .eq. 1) then
+                call hip_check(hc_get_scalar(hip_ctx, 3_c_int, v), 'hc_get_scalar')
+                write (*, 258) v
+258             format(' Ri_b = ', es12.5)
+            endif
+            if (hip_prm%dp_opt .eq. 2) then
+                call hip_check(hc_get_scalar(hip_ctx, 1_c_int, v), 'hc_get_scalar')
+                write (*, 259) v + hip_prm%u_mrf
+259             format(' Ubar = ', es12.5)
+            endif
+        endif
+    end subroutine
+
+    ! the screens recorded after each substep of the last device step
+    subroutine hip_print_substeps()
+        integer :: s, nsub
+        real(c_double) :: a, m, h
+        if (.not. hip_screen_on) return
+        nsub = 3
+        if (hip_prm%dts .ne. 2) nsub = 1
+        do s = 1, nsub
+            call hip_check(hc_get_screen(hip_ctx, int(s, c_int), a, m, h), 'screen_div_error')
+            if (hip_prm%dts .ne. 2 .or. s .eq. 3) then
+                call hip_print_div(a, m, h, 1, 1)
+            else
+                call hip_print_div(a, m, h, 1, 0)
+            endif
+        enddo
     end subroutine
 
     subroutine hip_destroy()
@@ -216,11 +362,81 @@ contains
     ! src/module_navier_stokes.f90:223-365, on the device-resident state
     subroutine time_advancement_fd()
         call hip_check(hc_time_advancement_fd(hip_ctx), 'time_advancement_fd')
+        call hip_print_substeps()
     end subroutine
 
     ! src/module_navier_stokes.f90:371-517 (cds=3), on the device-resident state
     subroutine time_advancement_sp()
         call hip_check(hc_time_advancement_sp(hip_ctx), 'time_advancement_sp')
+        call hip_print_substeps()
+    end subroutine
+
+    ! src/module_parameters.f90:580-590
+    subroutine update_ri(stps)
+        integer, intent(in) :: stps
+        call hip_check(hc_update_ri(hip_ctx, int(stps, c_int)), 'update_ri')
+    end subroutine
+
+    ! src/module_io.f90:85-206 on the given velocity (which becomes the
+    ! channel's u, v, w); prints the reference's lines and stops on a NaN
+    subroutine screen_div_error(u, v, w, isfinal, ishistout)
+        real(c_double), dimension(:,:,:), intent(in), target, contiguous :: u, v, w
+        integer, intent(in) :: isfinal, ishistout
+        real(c_double) :: a, m, h
+        call hip_upload(HC_U, u)
+        call hip_upload(HC_V, v)
+        call hip_upload(HC_W, w)
+        call hip_check(hc_screen_div_error_ref(hip_ctx, int(isfinal, c_int), a, m, h), 'screen_div_error')
+        call hip_print_div(a, m, h, isfinal, ishistout)
+    end subroutine
+
+    ! src/module_cnvdiff.f90:35-374.  The interpolants uhx .. why are formed
+    ! by the library from u, v, w (the caller's copies are not read).
+    subroutine get_momentum_cnvdiff(u, v, w, p, t, uhx, uhy, uhz, vhx, vhy, vhz, whx, why, &
+                                    ff, ff0, gg, gg0, hh, hh0, qu, qv, qw)
+        real(c_double), dimension(:,:,:), intent(in), target, contiguous :: u, v, w, p, t, ff0, gg0, hh0
+        real(c_double), dimension(:,:,:), intent(in) :: uhx, uhy, uhz, vhx, vhy, vhz, whx, why
+        real(c_double), dimension(:,:,:), intent(inout), target, contiguous :: ff, gg, hh, qu, qv, qw
+        call hip_check(hc_get_momentum_cnvdiff(hip_ctx, hip_rk_a, hip_rk_b, hip_rk_c, c_loc(u), c_loc(v), &
+                                               c_loc(w), c_loc(p), c_loc(t), c_loc(ff), c_loc(ff0), c_loc(gg), &
+                                               c_loc(gg0), c_loc(hh), c_loc(hh0), c_loc(qu), c_loc(qv), &
+                                               c_loc(qw), HC_LAYOUT_GHOSTED), 'get_momentum_cnvdiff')
+        if (size(uhx) + size(uhy) + size(uhz) + size(vhx) + size(vhy) + size(vhz) + size(whx) + size(why) < 0) stop
+    end subroutine
+
+    ! src/module_cnvdiff.f90:381-473 (thx, thy, thz formed by the library)
+    subroutine get_temperature_cnvdiff(u, v, w, t, thx, thy, thz, ss, ss0, qt)
+        real(c_double), dimension(:,:,:), intent(in), target, contiguous :: u, v, w, t, ss0
+        real(c_double), dimension(:,:,:), intent(in) :: thx, thy, thz
+        real(c_double), dimension(:,:,:), intent(inout), target, contiguous :: ss, qt
+        call hip_check(hc_get_temperature_cnvdiff(hip_ctx, hip_rk_a, hip_rk_b, hip_rk_c, c_loc(u), c_loc(v), &
+                                                  c_loc(w), c_loc(t), c_loc(ss), c_loc(ss0), c_loc(qt), &
+                                                  HC_LAYOUT_GHOSTED), 'get_temperature_cnvdiff')
+        if (size(thx) + size(thy) + size(thz) < 0) stop
+    end subroutine
+
+    ! src/module_poisson_solver.f90:127-176
+    subroutine get_p_eqn_src(u, v, w, qp)
+        real(c_double), dimension(:,:,:), intent(in), target, contiguous :: u, v, w
+        real(c_double), dimension(:,:,:), intent(inout), target, contiguous :: qp
+        call hip_check(hc_get_p_eqn_src(hip_ctx, c_loc(u), c_loc(v), c_loc(w), c_loc(qp), HC_LAYOUT_GHOSTED), &
+                       'get_p_eqn_src')
+    end subroutine
+
+    ! src/module_navier_stokes.f90:659-734
+    subroutine calculate_uvw(u, v, w, phi)
+        real(c_double), dimension(:,:,:), intent(inout), target, contiguous :: u, v, w
+        real(c_double), dimension(:,:,:), intent(in), target, contiguous :: phi
+        call hip_check(hc_calculate_uvw(hip_ctx, hip_rk_c, c_loc(u), c_loc(v), c_loc(w), c_loc(phi), &
+                                        HC_LAYOUT_GHOSTED), 'calculate_uvw')
+    end subroutine
+
+    ! src/module_navier_stokes.f90:739-808
+    subroutine calculate_new_p(phi, p)
+        real(c_double), dimension(:,:,:), intent(in), target, contiguous :: phi
+        real(c_double), dimension(:,:,:), intent(inout), target, contiguous :: p
+        call hip_check(hc_calculate_new_p(hip_ctx, hip_rk_c, c_loc(phi), c_loc(p), HC_LAYOUT_GHOSTED), &
+                       'calculate_new_p')
     end subroutine
 
     ! src/module_poisson_solver.f90:483-658.  aa, bb, cc are accepted for
@@ -254,9 +470,15 @@ contains
         if (size(aa) < 0 .or. size(bb) < 0 .or. size(cc) < 0) stop
     end subroutine
 
-    ! src/module_parameters.f90:513-546
-    subroutine calculate_ubar(ubar)
-        real(c_double), intent(out) :: ubar
+    ! src/module_parameters.f90:513-546: the module's ubar from the given u
+    ! (which the next dp_opt=2 update of dpx_drive reads)
+    subroutine calculate_ubar(u)
+        real(c_double), dimension(:,:,:), intent(in), target, contiguous :: u
+        call hip_check(hc_calculate_ubar_of(hip_ctx, c_loc(u), HC_LAYOUT_GHOSTED, ubar), 'calculate_ubar')
+    end subroutine
+
+    ! the module's ubar from the device-resident u (the coarse path)
+    subroutine hip_calculate_ubar()
         call hip_check(hc_calculate_ubar(hip_ctx, ubar), 'calculate_ubar')
     end subroutine
 

@@ -163,12 +163,44 @@ int hc_poisson_solver_fft(hc_ctx *ctx, int equation, double rk_c, const double *
  * rk_c = 1) and calculate_uvw.  Used to build seeded initial states. */
 int hc_project(hc_ctx *ctx);
 
-/* calculate_ubar (src/module_parameters.f90:513-546) on the current u. */
+/* calculate_ubar (src/module_parameters.f90:513-546) on the context's u. */
 int hc_calculate_ubar(hc_ctx *ctx, double *ubar);
+/* calculate_ubar(u) with the reference's argument: the bulk velocity of the
+ * given u (FP64, `layout`), stored as the module's ubar (read by the dp_opt=2
+ * update of the next substep) and returned in *ubar (may be null).  The
+ * context's own u is not changed. */
+int hc_calculate_ubar_of(hc_ctx *ctx, const double *u, int layout, double *ubar);
 
-/* screen_div_error values (src/module_io.f90:85-206): mean |div|, the
- * reference's signed maximum, and HC_ERR_NAN when u, v or w hold a NaN. */
+/* update_ri (src/module_parameters.f90:580-590): got = ri_str + (ri_end -
+ * ri_str) / imax * stps.  hc_drive_steps applies it by itself when
+ * is_ri_var = 1; a caller looping over hc_time_advancement_fd / _sp calls it
+ * before each step, as the reference's drive loop does (src/module_navier_stokes.f90:143-145). */
+int hc_update_ri(hc_ctx *ctx, int stps);
+
+/* Mean |div| and the LARGEST |div| of the current fields (FP64 metrics), and
+ * HC_ERR_NAN when u, v or w hold a NaN.  The reference keeps a signed
+ * running value instead; hc_screen_div_error_ref returns that. */
 int hc_screen_div_error(hc_ctx *ctx, double *div_avg, double *div_max);
+
+/* screen_div_error(u, v, w, isfinal, .) exactly as the reference computes it
+ * (src/module_io.f90:85-206): div_avg = sum |d| / (nx ny nz) and the signed
+ * values of its scan in (k, j, i) order, div_max (from 0: at each point with
+ * |d| > div_max, div_max = d) and div_hist_max (the same rule, kept in the
+ * context across calls and updated only when isfinal = 1).  HC_ERR_NAN when
+ * u, v or w hold a NaN (the reference prints 'NaN error!' and stops). */
+int hc_screen_div_error_ref(hc_ctx *ctx, int isfinal, double *div_avg, double *div_max, double *div_hist_max);
+
+/* Screen every substep (on = 1; default 0): time_advancement_fd / _sp call
+ * screen_div_error(u,v,w,1,.) after each substep (src/module_navier_stokes.f90:344-350,
+ * 505-509), which updates div_hist_max and stops on a NaN.  With the screen on,
+ * hc_time_advancement_fd / _sp and each step of hc_drive_steps wait for the
+ * step and return HC_ERR_NAN when a substep left a NaN in u, v or w; the
+ * values of the last step's substeps are read with hc_get_screen (substep
+ * 1..3; 0 reads the last hc_screen_div_error_ref).  Ensemble steps, the
+ * autotuner and the gradient driver do not screen.  Cost at 256x256x257:
+ * one read of u, v, w per substep. */
+int hc_screen_substeps(hc_ctx *ctx, int on);
+int hc_get_screen(hc_ctx *ctx, int substep, double *div_avg, double *div_max, double *div_hist_max);
 
 /* Plane-averaged wall shear stress nu * du/dz at the walls, from the no-slip
  * ghost rule u(kstr3-1) = -u(kstr3) - 2 u_mrf (src/module_boundary.f90:52-53,
@@ -297,6 +329,39 @@ int hc_set_adjoint(hc_ctx *ctx, int field, const double *src, int layout);
 int hc_get_adjoint(hc_ctx *ctx, int field, double *dst, int layout);
 int hc_adjoint_substep_fd(hc_ctx *ctx, int t_its);
 int hc_wall_shear_gradient(hc_ctx *ctx, int nsteps, int ckpt_every, double *loss);
+
+/* Per-stage entry points with the reference's argument lists, for callers
+ * that keep the reference's own time_advancement_fd loop on the host.  Every
+ * array is a whole FP64 field in `layout` (host or device memory) and crosses
+ * the ABI on each call; inputs become the context's module fields (u, v, w,
+ * p, t, the old right-hand sides), as the reference's arguments are its
+ * module arrays.  Ghost cells are not read: the wall rules and the periodic
+ * wrap of update_boundary_* and comm_bound are applied on the fly.  FP64,
+ * cds = 1 or 2, 2nd-order z (HC_ERR_UNSUPPORTED / HC_ERR_ARG otherwise).
+ * The RK coefficients are the module's rk_a, rk_b, rk_c of assign_rk_coeff
+ * (src/module_parameters.f90:618-641).
+ *
+ * get_momentum_cnvdiff(u,v,w,p,t,uhx,...,why,ff,ff0,gg,gg0,hh,hh0,qu,qv,qw)
+ * (src/module_cnvdiff.f90:35-374), including its update_dpx_drive (:51); the
+ * interpolants uhx..why are formed inside.  t may be null when isscalar = 0.
+ * hh and qw are written as 0 on the top level, which the reference skips. */
+int hc_get_momentum_cnvdiff(hc_ctx *ctx, double rk_a, double rk_b, double rk_c, const double *u, const double *v,
+                            const double *w, const double *p, const double *t, double *ff, const double *ff0,
+                            double *gg, const double *gg0, double *hh, const double *hh0, double *qu, double *qv,
+                            double *qw, int layout);
+/* get_temperature_cnvdiff(u,v,w,t,thx,thy,thz,ss,ss0,qt)
+ * (src/module_cnvdiff.f90:381-473), the interpolants thx..thz formed inside. */
+int hc_get_temperature_cnvdiff(hc_ctx *ctx, double rk_a, double rk_b, double rk_c, const double *u, const double *v,
+                               const double *w, const double *t, double *ss, const double *ss0, double *qt,
+                               int layout);
+/* get_p_eqn_src(u,v,w,qp) (src/module_poisson_solver.f90:127-176). */
+int hc_get_p_eqn_src(hc_ctx *ctx, const double *u, const double *v, const double *w, double *qp, int layout);
+/* calculate_uvw(u,v,w,phi) (src/module_navier_stokes.f90:659-734): u, v, w
+ * updated in place. */
+int hc_calculate_uvw(hc_ctx *ctx, double rk_c, double *u, double *v, double *w, const double *phi, int layout);
+/* calculate_new_p(phi,p) (src/module_navier_stokes.f90:739-808): p updated
+ * in place. */
+int hc_calculate_new_p(hc_ctx *ctx, double rk_c, const double *phi, double *p, int layout);
 
 /* Ensembles of independent channels (no counterpart in the reference, which
  * runs one channel per job).  The members are contexts made by hc_create and

@@ -1627,7 +1627,7 @@ static double *field_ptr(ho_state *S, int id)
 {
     double *f[] = {S->u, S->v, S->w, S->p, S->t, S->phi, S->ff, S->gg, S->hh, S->ss,
                    S->qu, S->qv, S->qw, S->uhx, S->uhy, S->uhz, S->vhx, S->vhy, S->vhz, S->whx,
-                   S->why, S->thx, S->thy, S->thz};
+                   S->why, S->thx, S->thy, S->thz, S->ff0, S->gg0, S->hh0, S->ss0};
     if (id < 0 || id >= (int)(sizeof(f) / sizeof(f[0]))) return NULL;
     return f[id];
 }
@@ -1726,6 +1726,21 @@ int ho_div_stats(ho_state *S, double *avg, double *mx)
     S->div_hist_max = keep;
     *avg = S->div_avg; *mx = S->div_max;
     return rc;
+}
+
+/* screen_div_error(u,v,w,isfinal,.) as the reference runs it: div_avg, the
+ * signed div_max of this call and the running div_hist_max (src/module_io.f90:85-206) */
+int ho_screen(ho_state *S, int isfinal, double *avg, double *mx, double *hist)
+{
+    if (S->prm.cds != 3) bc_uvw(S);  /* the callers refresh the ghosts first (:306, :320) */
+    int rc = S->prm.cds == 3 ? div_error_spec(S, isfinal) : div_error(S, isfinal);
+    *avg = S->div_avg; *mx = S->div_max; *hist = S->div_hist_max;
+    return rc;
+}
+/* the values of the last screen the step itself ran (the final substep's) */
+void ho_last_screen(ho_state *S, double *avg, double *mx, double *hist)
+{
+    *avg = S->div_avg; *mx = S->div_max; *hist = S->div_hist_max;
 }
 
 /* Stage-level entry points for per-function parity tests.  Each runs the

@@ -35,7 +35,7 @@ class OParams(C.Structure):
 FIELD_IDS = {"u": 0, "v": 1, "w": 2, "p": 3, "t": 4, "phi": 5, "ff": 6, "gg": 7, "hh": 8,
              "ss": 9, "qu": 10, "qv": 11, "qw": 12, "uhx": 13, "uhy": 14, "uhz": 15,
              "vhx": 16, "vhy": 17, "vhz": 18, "whx": 19, "why": 20, "thx": 21, "thy": 22,
-             "thz": 23}
+             "thz": 23, "ff0": 24, "gg0": 25, "hh0": 26, "ss0": 27}
 EQ_IDS = {"u": 0, "v": 1, "w": 2, "p": 3, "t": 4}
 SPEC_IDS = {"u": 0, "v": 1, "w": 2, "p": 3, "t": 4, "phi": 5, "ff": 6, "gg": 7, "hh": 8, "ss": 9,
             "qu": 10, "qv": 11, "qw": 12}
@@ -64,6 +64,8 @@ def lib():
         L.ho_get_coeff.argtypes = [vp, ip, dp, dp, dp]
         L.ho_solve.argtypes = [vp, ip, C.c_double, dp, dp]
         L.ho_div_stats.argtypes = [vp, dp, dp]
+        L.ho_screen.argtypes = [vp, ip, dp, dp, dp]
+        L.ho_last_screen.argtypes = [vp, dp, dp, dp]
         L.ho_fft1d.argtypes = [dp, ip, ip]
         L.ho_mesh.argtypes = [vp]
         L.ho_coefficients.argtypes = [vp]
@@ -208,6 +210,18 @@ class Oracle:
         a, m = C.c_double(), C.c_double()
         self._check(lib().ho_div_stats(self.h, C.byref(a), C.byref(m)))
         return a.value, m.value
+
+    def screen(self, isfinal=1):
+        """screen_div_error: (div_avg, signed div_max, div_hist_max)."""
+        a, m, h = C.c_double(), C.c_double(), C.c_double()
+        self._check(lib().ho_screen(self.h, int(isfinal), C.byref(a), C.byref(m), C.byref(h)))
+        return a.value, m.value, h.value
+
+    def last_screen(self):
+        """(div_avg, div_max, div_hist_max) of the last screen the steps ran."""
+        a, m, h = C.c_double(), C.c_double(), C.c_double()
+        lib().ho_last_screen(self.h, C.byref(a), C.byref(m), C.byref(h))
+        return a.value, m.value, h.value
 
     def scalar(self, name):
         return lib().ho_get_scalar(self.h, {"dpx_drive": 0, "ubar": 1, "div_hist_max": 2,

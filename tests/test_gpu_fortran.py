@@ -55,8 +55,10 @@ def test_fortran_driver_matches_oracle(tmp_path, cds):
     r = subprocess.run([FEXE], cwd=tmp_path, capture_output=True, text=True, timeout=120)
     print(r.stdout[-2000:], r.stderr[-2000:])
     assert r.returncode == 0
-    # screen_cpu_time and screen_div_error lines, one each per step
-    assert r.stdout.count("CFL max =") == prm.imax and r.stdout.count("DIV-ERR FINAL") == prm.imax
+    # screen_cpu_time once per step; screen_div_error after every substep,
+    # with the history line after the last one
+    assert r.stdout.count("CFL max =") == prm.imax and r.stdout.count("DIV-ERR FINAL") == 3 * prm.imax
+    assert r.stdout.count("DIV-ERR HIST_MAX") == prm.imax
     o = make_oracle(prm, st)
     o.mean_enable()
     o.steps(prm.imax)
@@ -181,3 +183,77 @@ def test_fortran_outputs_series_instants_slices(tmp_path):
     assert rel_l2(xz, u4[:, 4, :]) < 1e-10
     yz = np.fromfile(res / "slices" / "yz2d_u_i0007_t000002").reshape(prm.nz, prm.ny)
     assert rel_l2(yz, u4[:, :, 6]) < 1e-10
+
+
+DECK_T = """&domain
+  p_row = 1, p_col = 1, lx = {lx!r}, ly = {ly!r}, lz = {lz!r}
+  nx = {nx}, ny = {ny}, nz = {nz}, zstretch = {zs!r}, ochannel = {och}
+/
+&modeling
+  dp_opt = {dp}, cds = {cds}, issk = 1, dts = {dts}, isnoise = 0, noise_mag = 0.1
+  isdamp = 0, nzdamp = 15, cdamp = 0.01
+/
+&constants
+  nu = {nu!r}, alpha = {alpha!r}, got = {got!r}, fc = 0.0, ug = 0.0, vg = 0.0
+  dt = 1.0d-3, imax = {imax}, u_mrf = {mrf!r}, isscalar = {iss}, t_ref = 0
+  tbot = 0.0, ttop = 1.0, is_ri_var = {ri}, ri_str = 0.05, ri_end = 0.15
+/
+&io
+  restart = 1, istmsr = 0, ibackup = {imax}, iinstfl = 100000, imeanfl = 100000
+  isxy2d = 0, xy2d_id = 10, isxz2d = 0, xz2d_id = 10, isyz2d = 0, yz2d_id = 10, intv_2d = 1000
+/
+"""
+
+
+@pytest.mark.parametrize("kind", ["closed_cds2", "open_t_dp2_ri", "closed_cds1_ab2_mrf"])
+def test_host_stage_loop_equals_device_step(tmp_path, kind):
+    """A Fortran caller running the reference's substep loop on the host with
+    the per-stage subroutines (tests/fortran/stage_loop.f90) gives the fields
+    of the fused device step run by the driver on the same deck."""
+    from hercules_amd.build import STAGE_EXE
+    if not os.path.exists(FEXE) or not os.path.exists(STAGE_EXE):
+        pytest.fail("Fortran host programs not built")
+    if kind == "closed_cds2":
+        prm = cases.closed_channel(nx=32, ny=32, nz=33, imax=3)
+        st = cases.laminar_noise(prm, seed=31)
+        dts, ri = 2, 0
+    elif kind == "open_t_dp2_ri":
+        prm = cases.stratified_open_channel(nx=32, ny=16, nz=33, imax=3, dp_opt=2, is_ri_var=1,
+                                            ri_str=0.05, ri_end=0.15)
+        st = cases.loglaw_noise(prm, seed=32)
+        dts, ri = 2, 1
+    else:
+        prm = cases.closed_channel(nx=16, ny=32, nz=17, imax=3, cds=1, dts=1, u_mrf=0.25)
+        st = cases.laminar_noise(prm, seed=33)
+        dts, ri = 1, 0
+    with Channel(prm) as ch:
+        for k, v in st.items():
+            ch.set_field(k, v)
+        ch.project()
+        st = {k: ch.get_field(k) for k in ("u", "v", "w", "p", "t")}
+    (tmp_path / "results").mkdir()
+    (tmp_path / "parameters.input").write_text(DECK_T.format(
+        lx=prm.lx, ly=prm.ly, lz=prm.lz, nx=prm.nx, ny=prm.ny, nz=prm.nz, zs=prm.zstretch, och=prm.ochannel,
+        dp=prm.dp_opt, cds=prm.cds, dts=dts, nu=prm.nu, alpha=prm.alpha, got=prm.got, imax=prm.imax,
+        mrf=prm.u_mrf, iss=prm.isscalar, ri=ri))
+    for k, v in st.items():  # init files hold u + u_mrf
+        (np.ascontiguousarray(v) + (prm.u_mrf if k == "u" else 0.0)).tofile(tmp_path / "results" / f"init_{k}.dat")
+    if dts == 1:  # AB2 restarts read the right-hand-side history too
+        for k in ("ff", "gg", "hh", "ss"):
+            np.zeros((prm.nz, prm.ny, prm.nx)).tofile(tmp_path / "results" / f"init_{k}.dat")
+    nsub = 3 if dts == 2 else 1
+    for exe in (FEXE, STAGE_EXE):
+        r = subprocess.run([exe], cwd=tmp_path, capture_output=True, text=True, timeout=120)
+        print(exe, r.stdout[-1500:], r.stderr[-1500:])
+        assert r.returncode == 0
+        assert r.stdout.count("DIV-ERR FINAL") == nsub * prm.imax
+        assert r.stdout.count("HIST_MAX") == prm.imax
+        if ri:
+            assert r.stdout.count("Ri_b =") == prm.imax
+    names = ("u", "v", "w", "p", "t") if prm.isscalar else ("u", "v", "w", "p")
+    for k in names:
+        a = np.fromfile(tmp_path / "results" / f"bak_{k}.dat")
+        b = np.fromfile(tmp_path / "results" / f"stage_{k}.dat")
+        err = rel_l2(b, a)
+        print(kind, k, f"{err:.2e}")
+        assert err < 1e-11, (k, err)

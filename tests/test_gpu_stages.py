@@ -1,0 +1,123 @@
+"""Per-stage drop-ins with the reference's argument lists (get_momentum_cnvdiff,
+get_temperature_cnvdiff, get_p_eqn_src, calculate_uvw, calculate_new_p,
+calculate_ubar(u)) through the C ABI, against the oracle's own stages on the
+same seeded, non-solenoidal fields."""
+import numpy as np
+import pytest
+
+from hercules_amd import Channel, cases
+from hercules_amd.channel import LAYOUT_GHOSTED, LAYOUT_INTERIOR
+from tests.helpers import make_oracle, rel_l2
+
+pytestmark = pytest.mark.gpu
+
+RK = [(8 / 15, 0.0, 8 / 15), (5 / 12, -17 / 60, 2 / 15), (3 / 4, -5 / 12, 1 / 3)]
+
+
+def ghost(a):
+    """interior (nz, ny, nx) -> ghosted (nz+4, ny+4, nx+4) with junk ghosts"""
+    g = np.full((a.shape[0] + 4, a.shape[1] + 4, a.shape[2] + 4), 7.25)
+    g[2:-2, 2:-2, 2:-2] = a
+    return g
+
+
+def unghost(g):
+    return g[2:-2, 2:-2, 2:-2]
+
+
+CASES = {
+    "closed_cds2": lambda: cases.closed_channel(nx=32, ny=32, nz=33),
+    "closed_cds1_mrf": lambda: cases.closed_channel(nx=32, ny=16, nz=17, cds=1, u_mrf=0.3),
+    "open_t_dp2": lambda: cases.stratified_open_channel(nx=32, ny=32, nz=33, dp_opt=2),
+    "open_t_coriolis": lambda: cases.stratified_open_channel(nx=16, ny=32, nz=17, dp_opt=3, fc=0.5, ug=1.0, vg=0.2),
+}
+
+
+def state(prm, seed):
+    rng = np.random.default_rng(seed)
+    sh = (prm.nz, prm.ny, prm.nx)
+    st = {k: rng.standard_normal(sh) for k in ("u", "v", "w", "p", "t", "ff0", "gg0", "hh0", "ss0", "phi")}
+    st["w"][-1] = 0.0
+    return st
+
+
+@pytest.mark.parametrize("name", list(CASES))
+@pytest.mark.parametrize("t_its", [1, 2])
+@pytest.mark.parametrize("layout", [LAYOUT_INTERIOR, LAYOUT_GHOSTED])
+def test_momentum_and_temperature_stages(name, t_its, layout):
+    prm = CASES[name]()
+    st = state(prm, 100 + t_its)
+    o = make_oracle(prm, {k: st[k] for k in ("u", "v", "w", "p", "t")})
+    ch = Channel(prm)
+    ch.init()
+    wrap = ghost if layout == LAYOUT_GHOSTED else (lambda a: a)
+    back = unghost if layout == LAYOUT_GHOSTED else (lambda a: a)
+    if prm.dp_opt == 2:
+        # module ubar from calculate_ubar(u) on both sides
+        ub = ch.calculate_ubar_of(wrap(st["u"]), layout)
+        from oracle.oracle import lib
+        lib().ho_calc_ubar(o.h)
+        assert ub == pytest.approx(o.scalar("ubar"), rel=1e-13)
+    for k in ("ff0", "gg0", "hh0"):
+        o.set(k, st[k])
+    o.set("ss0", st["ss0"])
+    from oracle.oracle import lib
+    lib().ho_stage_momentum(o.h, t_its)
+    out = ch.get_momentum_cnvdiff(RK[t_its - 1], *(wrap(st[k]) for k in ("u", "v", "w", "p")),
+                                  wrap(st["t"]) if prm.isscalar else None,
+                                  *(wrap(st[k]) for k in ("ff0", "gg0", "hh0")), layout=layout)
+    names = ("ff", "gg", "hh", "qu", "qv", "qw")
+    for n, g in zip(names, out):
+        r = o.get(n)
+        g = back(g)
+        if n in ("hh", "qw"):  # the reference writes levels below kend3 only
+            g, r = g[:-1], r[:-1]
+        e = rel_l2(g, r)
+        print(name, t_its, n, f"{e:.2e}")
+        assert e < 1e-12, (n, e)
+    if layout == LAYOUT_GHOSTED:  # the caller's ghost cells are kept
+        assert np.all(out[0][:2] == 7.25)
+    if prm.dp_opt == 2:
+        assert ch.scalar("dpx_drive") == pytest.approx(o.scalar("dpx_drive"), rel=1e-14, abs=1e-300)
+    if prm.isscalar:
+        # T after the velocity update: feed the stage the same (new) u, v, w
+        lib().ho_stage_temperature(o.h, t_its)
+        ss, qt = ch.get_temperature_cnvdiff(RK[t_its - 1], *(wrap(st[k]) for k in ("u", "v", "w", "t")),
+                                            wrap(st["ss0"]), layout=layout)
+        for n, g in (("ss", ss), ("qu", qt)):
+            e = rel_l2(back(g), o.get(n))
+            print(name, t_its, "T", n, f"{e:.2e}")
+            assert e < 1e-12, (n, e)
+    ch.close()
+
+
+@pytest.mark.parametrize("name", list(CASES))
+def test_p_src_project_new_p_stages(name):
+    prm = CASES[name]()
+    st = state(prm, 7)
+    t_its = 2
+    o = make_oracle(prm, {k: st[k] for k in ("u", "v", "w", "p", "t")})
+    ch = Channel(prm)
+    ch.init()
+    from oracle.oracle import lib
+    lib().ho_stage_bc_uvw(o.h)
+    lib().ho_stage_p_src(o.h)
+    qp = ch.get_p_eqn_src(st["u"], st["v"], st["w"])
+    e = rel_l2(qp, o.get("qu"))
+    assert e < 1e-12, e
+    o.set("phi", st["phi"])
+    lib().ho_stage_project(o.h, t_its)
+    u, v, w = ch.calculate_uvw(RK[t_its - 1][2], st["u"], st["v"], st["w"], st["phi"])
+    for n, g in (("u", u), ("v", v), ("w", w)):
+        r = o.get(n)
+        if n == "w":
+            g, r = g[:-1], r[:-1]  # w(kend3) is the wall value the caller's BC update sets
+        e = rel_l2(g, r)
+        assert e < 1e-13, (n, e)
+    o.set("phi", st["phi"])
+    o.set("p", st["p"])
+    lib().ho_stage_new_p(o.h, t_its)
+    p = ch.calculate_new_p(RK[t_its - 1][2], st["phi"], st["p"])
+    e = rel_l2(p, o.get("p"))
+    assert e < 1e-13, e
+    ch.close()
