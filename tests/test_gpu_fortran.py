@@ -218,7 +218,7 @@ def test_host_stage_loop_equals_device_step(tmp_path, kind):
         st = cases.laminar_noise(prm, seed=31)
         dts, ri = 2, 0
     elif kind == "open_t_dp2_ri":
-        prm = cases.stratified_open_channel(nx=32, ny=16, nz=33, imax=3, dp_opt=2, is_ri_var=1,
+        prm = cases.stratified_open_channel(nx=32, ny=32, nz=33, imax=3, dp_opt=2, is_ri_var=1,
                                             ri_str=0.05, ri_end=0.15)
         st = cases.loglaw_noise(prm, seed=32)
         dts, ri = 2, 1
