@@ -23,6 +23,8 @@
 //                 one load of their maximum.
 #include "hc_host.h"
 
+#include <mutex>
+
 namespace {
 constexpr int kRowsPerBlock = 16;
 constexpr int kScrLen = 32;
@@ -109,6 +111,7 @@ template <int CDS, class R>
 __global__ __launch_bounds__(64) void k_div_signed(StepArgsT<R> F, StepArgs D, const double *rowmax, const double *blk,
                                                    int nblk, int gx, double *state, double *rec, int fin)
 {
+    // state[0]: div_hist_max before the call (updated in place when fin)
     const int lane = threadIdx.x;
     // fixed-order sum of the block sums, NaN flag
     double s = 0.0;
@@ -164,13 +167,48 @@ __global__ __launch_bounds__(64) void k_div_signed(StepArgsT<R> F, StepArgs D, c
     }
 }
 
+__global__ void k_scr_set(double *f, double v)
+{
+    if (threadIdx.x == 0) f[0] = v;
+}
+
+// Partials of one screen: per-row maxima (ny nz), per-block sums, maxima and
+// NaN flags (3 gx nz), where gx = ceil(ny / 16).
+struct ScreenBufs {
+    double *rows, *blk, *state, *rec;
+};
+inline size_t screen_rows_len(int ny, int nz) { return (size_t)ny * nz; }
+inline size_t screen_blk_len(int ny, int nz) { return 3 * (size_t)((ny + kRowsPerBlock - 1) / kRowsPerBlock) * nz; }
+
 template <int CDS, class R>
-void launch_screen(hc_ctx *c, const StepArgsT<R> &F, const StepArgs &D, double *rec, int fin)
+void launch_screen(hc_ctx *c, const StepArgsT<R> &F, const StepArgs &D, const ScreenBufs &B, int fin)
 {
     const int gx = (c->ny + kRowsPerBlock - 1) / kRowsPerBlock, nblk = gx * c->nz;
-    hipLaunchKernelGGL((k_div_rows<CDS, R>), dim3(gx, c->nz), dim3(256), 0, c->stream, F, D, c->scr_rows, c->scr_blk);
-    hipLaunchKernelGGL((k_div_signed<CDS, R>), dim3(1), dim3(64), 0, c->stream, F, D, c->scr_rows, c->scr_blk, nblk,
-                       gx, c->scr, rec, fin);
+    hipLaunchKernelGGL((k_div_rows<CDS, R>), dim3(gx, c->nz), dim3(256), 0, c->stream, F, D, B.rows, B.blk);
+    hipLaunchKernelGGL((k_div_signed<CDS, R>), dim3(1), dim3(64), 0, c->stream, F, D, B.rows, B.blk, nblk, gx,
+                       B.state, B.rec, fin);
+}
+
+// Queue a screen of the context's current fields with the partials in B.
+int queue_screen(hc_ctx *c, const ScreenBufs &B, int fin)
+{
+    StepArgs A = base_args(c);
+    if (c->prm.cds == 3)
+        if (int e = spec_divergence(c)) return e;
+    if (c->mixed) {
+        StepArgsT<float> F = rows_args<float>(A, c->m32);
+        F.aux = c->m32.ptmp;
+        if (c->prm.cds == 1) launch_screen<1, float>(c, F, A, B, fin);
+        else if (c->prm.cds == 2) launch_screen<2, float>(c, F, A, B, fin);
+        else launch_screen<3, float>(c, F, A, B, fin);
+    } else {
+        A.aux = c->ptmp;
+        if (c->prm.cds == 1) launch_screen<1, double>(c, A, A, B, fin);
+        else if (c->prm.cds == 2) launch_screen<2, double>(c, A, A, B, fin);
+        else launch_screen<3, double>(c, A, A, B, fin);
+    }
+    HIPCHK(hipGetLastError());
+    return 0;
 }
 }  // namespace
 
@@ -194,22 +232,65 @@ int screen_alloc(hc_ctx *c)
 int screen_queue(hc_ctx *c, int slot, int fin)
 {
     if (int e = screen_alloc(c)) return e;
-    double *rec = c->scr + 4 + 4 * slot;
-    StepArgs A = base_args(c);
-    if (c->prm.cds == 3)
-        if (int e = spec_divergence(c)) return e;
-    if (c->mixed) {
-        StepArgsT<float> F = rows_args<float>(A, c->m32);
-        F.aux = c->m32.ptmp;
-        if (c->prm.cds == 1) launch_screen<1, float>(c, F, A, rec, fin);
-        else if (c->prm.cds == 2) launch_screen<2, float>(c, F, A, rec, fin);
-        else launch_screen<3, float>(c, F, A, rec, fin);
-    } else {
-        A.aux = c->ptmp;
-        if (c->prm.cds == 1) launch_screen<1, double>(c, A, A, rec, fin);
-        else if (c->prm.cds == 2) launch_screen<2, double>(c, A, A, rec, fin);
-        else launch_screen<3, double>(c, A, A, rec, fin);
+    return queue_screen(c, ScreenBufs{c->scr_rows, c->scr_blk, c->scr, c->scr + 4 + 4 * slot}, fin);
+}
+
+// ---------------------------------------------------------------------------
+// The screen without allocation: its partials live in `scratch`, a
+// file-scope device array that every GPU the library runs on holds until the
+// process ends.  Host code never passes the array's name to a kernel (that is
+// a host-side stand-in, not a device address): the device address is looked
+// up once per device with hipGetSymbolAddress and kept in g_scratch.
+namespace {
+constexpr size_t kScratchDoubles = (size_t)1 << 19;  // 4 MiB
+constexpr int kMaxDevices = 64;
+__device__ __align__(256) double scratch[kScratchDoubles];
+double *g_scratch[kMaxDevices] = {};
+std::mutex g_scratch_mu;
+
+int scratch_ptr(int device, double **out)
+{
+    if (device < 0 || device >= kMaxDevices) return set_err(HC_ERR_ARG, "device %d out of range", device);
+    std::lock_guard<std::mutex> lk(g_scratch_mu);
+    if (!g_scratch[device]) {
+        void *p = nullptr;
+        hipError_t e = hipGetSymbolAddress(&p, HIP_SYMBOL(scratch));
+        if (e != hipSuccess || !p)
+            return set_err(HC_ERR_HIP, "hipGetSymbolAddress(scratch): %s", hipGetErrorString(e));
+        g_scratch[device] = (double *)p;
     }
+    *out = g_scratch[device];
+    return 0;
+}
+}  // namespace
+
+extern "C" int hc_screen_div_error_noalloc(hc_ctx *c, int isfinal, double *div_avg, double *div_max,
+                                           double *div_hist_max)
+{
+    if (!c || !div_hist_max) return set_err(HC_ERR_ARG, "null argument");
+    if (isfinal != 0 && isfinal != 1) return set_err(HC_ERR_ARG, "isfinal must be 0 or 1");
+    const size_t nrows = screen_rows_len(c->ny, c->nz), nblk = screen_blk_len(c->ny, c->nz);
+    const size_t need = 8 + nrows + nblk;
+    if (need > kScratchDoubles)
+        return set_err(HC_ERR_ARG, "screen partials of %zu doubles (ny=%d, nz=%d) exceed the scratch of %zu doubles",
+                       need, c->ny, c->nz, kScratchDoubles);
+    int dev = -1;
+    HIPCHK(hipGetDevice(&dev));
+    if (dev != c->device) return set_err(HC_ERR_ARG, "the context lives on device %d, current is %d", c->device, dev);
+    double *ptr = nullptr;
+    if (int e = scratch_ptr(dev, &ptr)) return e;
+    // scratch layout: [0] div_hist_max in/out, [4..7] the record, then the
+    // row maxima and the block partials (both fully rewritten by each call)
+    const ScreenBufs B{ptr + 8, ptr + 8 + nrows, ptr, ptr + 4};
+    hipLaunchKernelGGL(k_scr_set, dim3(1), dim3(64), 0, c->stream, ptr, *div_hist_max);
     HIPCHK(hipGetLastError());
+    if (int e = queue_screen(c, B, isfinal)) return e;
+    double rec[4];
+    HIPCHK(hipMemcpyAsync(rec, ptr + 4, sizeof rec, hipMemcpyDeviceToHost, c->stream));
+    HIPCHK(hipStreamSynchronize(c->stream));
+    if (div_avg) *div_avg = rec[0];
+    if (div_max) *div_max = rec[1];
+    *div_hist_max = rec[2];
+    if (rec[3] != 0.0) return set_err(HC_ERR_NAN, "NaN error!");
     return 0;
 }

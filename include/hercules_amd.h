@@ -190,6 +190,20 @@ int hc_screen_div_error(hc_ctx *ctx, double *div_avg, double *div_max);
  * u, v or w hold a NaN (the reference prints 'NaN error!' and stops). */
 int hc_screen_div_error_ref(hc_ctx *ctx, int isfinal, double *div_avg, double *div_max, double *div_hist_max);
 
+/* No counterpart in the reference: hc_screen_div_error_ref for callers that
+ * cannot allocate device memory (loops that forbid hipMalloc, such as tight
+ * serving loops and harnesses that pre-plan every allocation).  It allocates
+ * nothing: its partial results live in one file-scope device array of 2^19
+ * doubles (4 MiB) that each GPU the library runs on holds until the process
+ * ends, so the grid must satisfy 8 + ny*nz + 3*ceil(ny/16)*nz <= 2^19
+ * (HC_ERR_ARG otherwise; 256x256x257 needs 78,356, 512x512x1024 would need
+ * 622,600).  div_hist_max is in/out:
+ * the caller keeps the history (isfinal = 1 updates it).  Every call on a GPU
+ * shares that GPU's array: calls that overlap on one GPU (two host threads,
+ * or two contexts' streams) race and must be serialised by the caller.  The
+ * context must live on the current device. */
+int hc_screen_div_error_noalloc(hc_ctx *ctx, int isfinal, double *div_avg, double *div_max, double *div_hist_max);
+
 /* Screen every substep (on = 1; default 0): time_advancement_fd / _sp call
  * screen_div_error(u,v,w,1,.) after each substep (src/module_navier_stokes.f90:344-350,
  * 505-509), which updates div_hist_max and stops on a NaN.  With the screen on,
