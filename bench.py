@@ -100,32 +100,9 @@ def build_case(workload):
 
 
 def ensemble_members(seed=2024):
-    """The 24-member ensemble: eight each at 128x128x129, 192x192x129 and
-    256x128x193; neutral closed channels, stratified open channels and
-    Ekman-type open channels (Coriolis drive with a top sponge), Re_tau
-    seeded uniform in [150, 400] and Ri uniform in [0, 0.2]."""
+    """The 24-member ensemble of hercules_amd.cases.ensemble_24."""
     from hercules_amd import cases
-    rng = np.random.default_rng(seed)
-    shapes = [(128, 128, 129)] * 8 + [(192, 192, 129)] * 8 + [(256, 128, 193)] * 8
-    out = []
-    for q, (nx, ny, nz) in enumerate(shapes):
-        re_tau = rng.uniform(150.0, 400.0)
-        ri = rng.uniform(0.0, 0.2)
-        kind = q % 3
-        common = dict(nx=nx, ny=ny, nz=nz, nu=1.0 / re_tau, alpha=1.0 / re_tau / 0.71, wall_rows=1,
-                      dt=5e-4)
-        if kind == 0:
-            prm = cases.closed_channel(**common)
-        elif kind == 1:
-            prm = cases.stratified_open_channel(got=ri, **common)
-        else:
-            prm = cases.stratified_open_channel(got=ri, dp_opt=3, fc=0.5, ug=1.0, vg=0.0, isdamp=1,
-                                                nzdamp=nz // 8, cdamp=0.01, **common)
-        st = cases.loglaw_noise(prm, seed=seed + q)
-        if not prm.isscalar:
-            st.pop("t")
-        out.append((prm, st))
-    return out
+    return cases.ensemble_24(seed)
 
 
 def run_ensemble(args):
@@ -190,12 +167,29 @@ def run_adjoint(args):
     RK3 steps (128x128x129 neutral closed channel, Re_tau = 180) with respect
     to the initial velocity, by the hand-written adjoint with a checkpoint
     every --ckpt steps; against the forward run of the same K steps."""
+    r = adjoint_case(args.steps, args.ckpt)
+    K, n, t_grad = r.pop("K"), r.pop("cells"), r.pop("t_grad")
+    value = n * K / t_grad / 1e6
+    out = {
+        "metric": METRIC, "value": round(value, 2), "unit": "Mcell-RK3-steps/s (gradient)", "n_gpus": 1,
+        "steps": K, "warmup": 1, "ms_per_step": round(t_grad / K * 1e3, 4), "higher_is_better": True,
+        "scaling": "weak", "vs_baseline": None, "dtype": "f64",
+        "data": "synthetic (seeded log-law + 5% noise, projected; no checkpoint file)",
+        "config": r.pop("config"),
+    }
+    out.update(r)
+    print(json.dumps(out))
+
+
+def adjoint_case(K, ck):
+    """The adjoint workload (128x128x129, gradient of the wall shear after K
+    steps, checkpoint every ck): gradient and forward times, one central
+    difference check, the HBM held."""
     from hercules_amd import Channel, cases
     prm = cases.closed_channel(nx=128, ny=128, nz=129, wall_rows=1)
     st = cases.loglaw_noise(prm, seed=2024, rel_amp=0.05)
     st.pop("t")
     n = prm.nx * prm.ny * prm.nz
-    K, ck = args.steps, args.ckpt
     ch = Channel(prm)
     for k, v in st.items():
         ch.set_field(k, v)
@@ -241,15 +235,12 @@ def run_adjoint(args):
     nseg = (K + ck - 1) // ck
     ckpt_bytes = (nseg * 4 + 3 * min(ck, K) * 4) * n * 8
     ctx_bytes = (13 + 13) * n * 8 + 4 * (prm.nx // 2) * prm.ny * prm.nz * 16
-    value = n * K / t_grad / 1e6
-    out = {
-        "metric": METRIC, "value": round(value, 2), "unit": "Mcell-RK3-steps/s (gradient)", "n_gpus": 1,
-        "steps": K, "warmup": 1, "ms_per_step": round(t_grad / K * 1e3, 4), "higher_is_better": True,
-        "scaling": "weak", "vs_baseline": None, "dtype": "f64",
-        "data": "synthetic (seeded log-law + 5% noise, projected; no checkpoint file)",
+    return {
+        "K": K, "cells": n, "t_grad": t_grad,
         "config": {"workload": f"wall-shear gradient after {K} RK3 steps, adjoint with a checkpoint every {ck} steps, "
                                "neutral closed channel Re_tau=180, 128x128x129, cds=2, wall_rows=1",
                    "nx": prm.nx, "ny": prm.ny, "nz": prm.nz, "cells": n, "precision": "fp64"},
+        "gradient_ms_per_step": round(t_grad / K * 1e3, 4),
         "forward_ms_per_step": round(t_fwd / K * 1e3, 4),
         "adjoint_over_forward": round(t_grad / t_fwd, 3),
         "backward_over_forward": round((t_grad - 2 * t_fwd) / t_fwd, 3),
@@ -257,7 +248,63 @@ def run_adjoint(args):
         "fd_check": {"eps_rel": 1e-6, "fd": fd, "adjoint": ad, "rel_err": abs(fd - ad) / abs(ad)},
         "peak_hbm_bytes_estimate": int(ckpt_bytes + ctx_bytes),
     }
-    print(json.dumps(out))
+
+
+def config3_case(steps, warmup, precision=0):
+    """The spectral step of the stratified open channel with temperature,
+    6th-order z operators and a constant top temperature gradient at
+    256x256x257 (cds=3, skew-symmetric, 2/3 rule; Re_tau=395, got=0.1):
+    time per step, kernel timings and the roofline of the dominant kernel."""
+    from hercules_amd import Channel
+    prm, st = build_case("sp")
+    prm = prm.replace(zorder=6, precision=precision)
+    n = prm.nx * prm.ny * prm.nz
+    ch = Channel(prm.replace(precision=0))
+    for k, v in st.items():
+        ch.set_field(k, v)
+    ch.project()
+    for k in ("u", "v", "w"):
+        st[k] = ch.get_field(k)
+    ch.close()
+    ch = Channel(prm)
+    for k, v in st.items():
+        ch.set_field(k, v)
+    ch.init()
+    ch.steps(warmup)
+    ch.synchronize()
+    ch.profile(True)
+    t0 = time.perf_counter()
+    ch.steps(steps)
+    ch.synchronize()
+    el = time.perf_counter() - t0
+    prof = ch.profile_read()
+    ch.profile(False)
+    fields = {k: ch.get_field(k) for k in ("u", "v", "w", "t")}
+    div = ch.screen_div_error()
+    tau = ch.wall_shear()[0]
+    ch.close()
+    eb = 4 if precision else 8
+    kernels = {}
+    for name, (ms, launches) in prof.items():
+        b = SPECTRAL_PASSES[name] * n * eb
+        kernels[name] = {"avg_ms": round(ms / launches, 4), "launches": launches,
+                         "GBs": round(b / (ms / launches * 1e-3) / 1e9, 1), "algo_bytes": b}
+    dom = max(kernels, key=lambda k: kernels[k]["avg_ms"] * kernels[k]["launches"])
+    kd = kernels[dom]
+    ms_step = el / steps * 1e3
+    step_gbs = 3 * sum(SPECTRAL_PASSES.values()) * n * eb / (ms_step * 1e-3) / 1e9
+    return {"workload": "spectral RK3 step (cds=3, skew-symmetric, 2/3 rule) with temperature, 6th-order z "
+                        "operators, top dT/dz=1, stratified open channel, Re_tau=395, got=0.1, 256x256x257",
+            "precision": "mixed" if precision else "fp64", "steps": steps, "warmup": warmup,
+            "ms_per_step": round(ms_step, 4), "value": round(n * steps / el / 1e6, 2), "unit": "Mcell-RK3-steps/s",
+            "roofline": {"bound": "hbm", "kernel": dom, "achieved": kd["GBs"], "peak": HBM_PEAK_GBS, "unit": "GB/s",
+                         "frac": round(kd["GBs"] / HBM_PEAK_GBS, 4), "avg_ms": kd["avg_ms"],
+                         "algo_bytes_per_launch": int(kd["algo_bytes"])},
+            "step_roofline": {"passes_per_step": 3 * sum(SPECTRAL_PASSES.values()), "achieved": round(step_gbs, 1),
+                              "frac": round(step_gbs / HBM_PEAK_GBS, 4)},
+            "kernels": {k: {kk: vv for kk, vv in v.items() if kk != "algo_bytes"} for k, v in kernels.items()},
+            "state_finite": bool(all(np.isfinite(f).all() for f in fields.values())), "div": div,
+            "wall_shear_bottom": tau, "_fields": fields}
 
 
 def cpu_baseline(prm, st, workload, steps):
@@ -449,6 +496,9 @@ def main():
                     help="fp64, or FP32 fields/stencils/FFTs with FP64 low-wavenumber solves")
     ap.add_argument("--no-compare", action="store_true",
                     help="skip the run in the other precision (speed ratio and mixed-vs-FP64 errors)")
+    ap.add_argument("--no-extras", action="store_true",
+                    help="fd: skip the adjoint (128x128x129, 400 steps, checkpoint every 20) and the "
+                         "6th-order-z spectral step (256x256x257) that the default line also carries")
     args = ap.parse_args()
     if args.zorder == 6 and (args.precision == "mixed" or args.workload in ("ensemble", "adjoint", "copy")):
         ap.error("--zorder 6 covers fd, fd_t, sp and poisson in FP64")
@@ -587,6 +637,27 @@ def main():
     if cpu and "value" in cpu and cpu["value"]:
         out["gpu_over_cpu"] = round(value / cpu["value"], 1)
     ch.close()
+    if args.workload == "fd" and not args.no_extras and args.zorder == 2 and not mixed_main:
+        # the metric's third part: the adjoint/forward cost ratio (128x128x129,
+        # 400 steps, checkpoint every 20), and the 6th-order-z spectral step
+        a = adjoint_case(400, 20)
+        for key in ("K", "cells", "t_grad"):
+            a.pop(key)
+        out["adjoint"] = a
+        c3 = config3_case(5, 1, 0)
+        f64 = c3.pop("_fields")
+        out["config3"] = c3
+        try:
+            c3m = config3_case(5, 1, 1)
+            fm = c3m.pop("_fields")
+            c3m["rel_l2_vs_fp64"] = {k: float(np.linalg.norm(fm[k] - f64[k]) / np.linalg.norm(f64[k]))
+                                     for k in f64}
+            c3m["wall_shear_rel_err"] = abs(c3m["wall_shear_bottom"] - c3["wall_shear_bottom"]) / abs(
+                c3["wall_shear_bottom"])
+            c3m["speedup_vs_fp64"] = round(c3["ms_per_step"] / c3m["ms_per_step"], 3)
+            out["config3_mixed"] = c3m
+        except Exception as e:  # not every build carries the mixed 6th-order path
+            out["config3_mixed"] = {"error": str(e)}
     print(json.dumps(out))
 
 

@@ -103,3 +103,32 @@ def wall_shear(u, prm: Params):
     bot = 2.0 * prm.nu * (u[0].mean() + prm.u_mrf) / dz[0]
     top = 2.0 * prm.nu * (u[-1].mean() + prm.u_mrf) / dz[-1] if prm.ochannel == 0 else 0.0
     return bot, top
+
+
+def ensemble_24(seed=2024):
+    """The 24-member ensemble: eight each at 128x128x129, 192x192x129 and
+    256x128x193; neutral closed channels, stratified open channels and
+    Ekman-type open channels (Coriolis drive with a top sponge), Re_tau
+    seeded uniform in [150, 400] and Ri uniform in [0, 0.2].  Returns
+    (params, state) pairs."""
+    rng = np.random.default_rng(seed)
+    shapes = [(128, 128, 129)] * 8 + [(192, 192, 129)] * 8 + [(256, 128, 193)] * 8
+    out = []
+    for q, (nx, ny, nz) in enumerate(shapes):
+        re_tau = rng.uniform(150.0, 400.0)
+        ri = rng.uniform(0.0, 0.2)
+        kind = q % 3
+        common = dict(nx=nx, ny=ny, nz=nz, nu=1.0 / re_tau, alpha=1.0 / re_tau / 0.71, wall_rows=1,
+                      dt=5e-4)
+        if kind == 0:
+            prm = closed_channel(**common)
+        elif kind == 1:
+            prm = stratified_open_channel(got=ri, **common)
+        else:
+            prm = stratified_open_channel(got=ri, dp_opt=3, fc=0.5, ug=1.0, vg=0.0, isdamp=1,
+                                          nzdamp=nz // 8, cdamp=0.01, **common)
+        st = loglaw_noise(prm, seed=seed + q)
+        if not prm.isscalar:
+            st.pop("t")
+        out.append((prm, st))
+    return out
