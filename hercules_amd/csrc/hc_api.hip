@@ -472,6 +472,28 @@ static bool build_zops(hc_ctx *c)
         c->h_zst[e] = first;
         for (int q = first; q <= last; ++q) c->h_zwt[e * kZW + (q - first)] = dense[q];
     }
+    // band rows of the implicit operators on their unknowns: entry (k, k + d)
+    // at d + kBandMax; data outside the unknowns (wall data) are dropped.
+    // C2 on the levels (data index = level + 1), F2 on the interior faces
+    // (data index = w level + 1, nz - 1 unknowns), P on the levels.
+    c->h_zband.assign((size_t)ZB_COUNT * nz * kBandRow, 0.0);
+    const int bops[ZB_COUNT] = {ZO_C2_DD, ZO_C2_DN, ZO_F2, ZO_P};
+    const int bhalf[ZB_COUNT] = {5, 5, 5, 7};  // half bands the solver is instantiated for
+    for (int b = 0; b < ZB_COUNT; ++b) {
+        const int op = bops[b];
+        const int n = op == ZO_F2 ? nz - 1 : nz, off = op == ZO_P ? 0 : 1, W = op == ZO_P ? kZW : 7;
+        for (int k = 0; k < n; ++k) {
+            const size_t e = (size_t)op * nz + k;
+            for (int j = 0; j < W; ++j) {
+                const double w = c->h_zwt[e * kZW + j];
+                const int lv = c->h_zst[e] + j - off;
+                if (w == 0.0 || lv < 0 || lv >= n) continue;
+                const int d = lv - k;
+                if (d < -bhalf[b] || d > bhalf[b]) return false;
+                c->h_zband[((size_t)b * nz + k) * kBandRow + kBandMax + d] += w;
+            }
+        }
+    }
     return true;
 }
 
@@ -567,6 +589,8 @@ StepArgs base_args(hc_ctx *c)
     A.rgroup = c->tune.rgroup;
     A.zorder = P.zorder == 6 ? 6 : 2;
     A.zst = c->zst; A.zwt = c->zwt;
+    A.zband = c->zband; A.zscr = c->zscr;
+    A.zspc = c->zbuf[0] ? reinterpret_cast<double2 *>(c->zbuf[0]) : nullptr;
     return A;
 }
 
@@ -762,6 +786,8 @@ extern "C" int hc_create(const hc_params *prm, hc_ctx **out)
         if (!build_zops(c)) e = set_err(HC_ERR_ARG, "6th-order z stencils: singular system");
         if (!e) e = dalloc(c, (void **)&c->zst, c->h_zst.size() * sizeof(int));
         if (!e) e = dalloc(c, (void **)&c->zwt, c->h_zwt.size() * sizeof(double));
+        if (!e) e = dalloc(c, (void **)&c->zband, c->h_zband.size() * sizeof(double));
+        if (!e) e = dalloc(c, (void **)&c->zscr, (size_t)12 * c->nz * c->nl * c->ny * sizeof(double));
         for (int q = 0; q < 4 && !e; ++q)
             if (q < 3 || P.isscalar == 1) e = dalloc(c, (void **)&c->zbuf[q], N * sizeof(double));
     }
@@ -812,6 +838,9 @@ extern "C" int hc_create(const hc_params *prm, hc_ctx **out)
         r = hipMemcpyAsync(c->zst, c->h_zst.data(), c->h_zst.size() * sizeof(int), hipMemcpyHostToDevice, c->stream);
     if (r == hipSuccess && c->zwt)
         r = hipMemcpyAsync(c->zwt, c->h_zwt.data(), c->h_zwt.size() * sizeof(double), hipMemcpyHostToDevice, c->stream);
+    if (r == hipSuccess && c->zband)
+        r = hipMemcpyAsync(c->zband, c->h_zband.data(), c->h_zband.size() * sizeof(double), hipMemcpyHostToDevice,
+                           c->stream);
     // FP32 copies of the tables for the mixed variant's row kernels
     std::vector<float> flev(c->h_lev.begin(), c->h_lev.end()), fsx(sx.begin(), sx.end()), fsy(sy.begin(), sy.end());
     auto f2 = [](const std::vector<double2> &t) {

@@ -87,6 +87,10 @@ struct hc_ctx {
     int *zst = nullptr;
     double *zwt = nullptr;
     double *zbuf[4] = {nullptr, nullptr, nullptr, nullptr};
+    // direct banded solves: band rows of ZO_C2_DD, ZO_C2_DN, ZO_F2, ZO_P
+    // (kBandRow per level) and the U-row scratch (12 nz nl ny doubles)
+    double *zband = nullptr, *zscr = nullptr;
+    std::vector<double> h_zband;
     // calculate_mean_fields: running sums (HC_MEAN_COUNT fields), the x / y
     // interpolants of u, v taken at the start of a step's last substep (FD),
     // and the number of steps summed

@@ -59,6 +59,12 @@ enum ZOp {
     ZO_COUNT
 };
 constexpr int kZW = 16;
+// Band rows of the implicit 6th-order operators on their unknowns, for the
+// direct solves: nz rows of kBandRow entries each, entry (k, k + d) at
+// d + kBandMax.  ZB_F2 has nz - 1 rows (the interior faces).
+constexpr int kBandMax = 7;
+constexpr int kBandRow = 2 * kBandMax + 1;
+enum { ZB_C2_DD = 0, ZB_C2_DN, ZB_F2, ZB_P, ZB_COUNT };
 
 
 // Arguments of the step kernels.  R is the storage and arithmetic type of
@@ -115,6 +121,11 @@ struct StepArgsT {
     // each level (the top wall face included), w at the centres, t at the
     // interior w faces
     const double *zfu, *zfv, *zcw, *zft;
+    // direct banded solves of the 6th-order systems: band tables of the
+    // operators (ZB_*), scratch for the U rows, and a half spectrum of scratch
+    const double *zband;
+    double *zscr;
+    cplx<double> *zspc;
 };
 using StepArgs = StepArgsT<double>;
 

@@ -6,17 +6,17 @@
 //   u, v, t:  I + rc (-hxy - kappa Z)      Z = ZO_C2_* (7-point d2/dz2, centres)
 //   w:        I + rc (-hxy - kappa Z)      Z = ZO_F2   (7-point d2/dz2, faces)
 //   p:        rc (hxy + ZO_P)              ZO_P = ZO_D o ZO_G
-// with hxy the horizontal symbol the 2nd-order rows use.  These are banded
-// (up to 16 diagonals for ZO_P) and not tridiagonal; each is solved exactly
-// by a Chebyshev iteration preconditioned with the 2nd-order tridiagonal
-// operator of the same equation (wave_tridiag): the preconditioned spectrum
-// lies in [0.95, 1.8] on the stretched channel grids, so kZ6Iters
-// iterations reduce the residual by ~1e-15.  Operator products go through
-// the wavefront's own column of a second LDS sub-tile (the first stages the
-// global loads and stores), ordered within the wavefront.  FP64 only.
+// with hxy the horizontal symbol the 2nd-order rows use.  They are banded
+// (half band 5, and 7 for ZO_P) and solved directly, one thread per column,
+// by k_z6_band (hc_band6.h).  A column pass of the step is therefore a
+// sequence of launches: right-hand sides (spectral path), the Helmholtz
+// solves, the divergence, the pressure solve, then the projection and the
+// pressure increment.  The explicit operator products of the divergence,
+// projection and increment run one column per wavefront through the
+// wavefront's own slice of an LDS sub-tile.  FP64 only.
 #pragma once
 
-constexpr int kZ6Iters = 20;
+#include "hc_band6.h"
 
 // The 6th-order interpolants of the substep's fields, once per point, for
 // the right-hand-side kernels: u, v at the w face of level k (the top wall
@@ -78,12 +78,8 @@ HC_DEV void z6_apply(const StepArgs &A, int op, int off, int ndata, int nrows, c
     __builtin_amdgcn_fence(__ATOMIC_ACQUIRE, "wavefront");
 }
 
-// Operator of equation eq (0 u/v, 1 w, 2 p, 3 t) of column (l, jm) applied
-// to x: the matrix the Chebyshev iteration inverts.
-// In the (0,0) mode the pressure operator is singular (Neumann walls); as in
-// the 2nd-order solve, its level-0 row is replaced by the preconditioner's
-// row b0 phi(0) = 0 (delnull, src/module_poisson_solver.f90:558-564), which
-// selects the solution with phi(0) = 0.
+// Operator of equation eq (0 u/v, 1 w, 2 p, 3 t) of column (l, jm): its
+// ZOp table, horizontal symbol and diffusivity.
 struct Z6Op {
     int eq, op;
     double hxy, kappa, rc, b0;
@@ -121,72 +117,6 @@ HC_DEV Z6Op z6_op(const StepArgs &A, int eq, int l, int jm)
     }
     return o;
 }
-template <int L>
-HC_DEV void z6_mul(const StepArgs &A, const Z6Op &o, const ColMap &m, double2 *buf, const double2 (&x)[L],
-                   double2 (&y)[L])
-{
-    const int nz = A.nz;
-    if (o.eq == 2) {
-        z6_apply<L, kZW>(A, o.op, 0, nz, nz, m, buf, x, y);
-#pragma unroll
-        for (int r = 0; r < L; ++r) y[r] = make_double2(o.rc * (o.hxy * x[r].x + y[r].x), o.rc * (o.hxy * x[r].y + y[r].y));
-        if (o.delnull && m.lane == 0) y[0] = make_double2(o.b0 * x[0].x, o.b0 * x[0].y);
-        return;
-    }
-    const bool face = o.eq == 1;
-    // w: data are the w levels 0..nz-2 (the top wall level nz-1 is zero)
-    z6_apply<L, 7>(A, o.op, 1, face ? nz - 1 : nz, face ? nz - 1 : nz, m, buf, x, y);
-    const double dg = 1.0 - o.rc * o.hxy, ck = o.rc * o.kappa;
-#pragma unroll
-    for (int r = 0; r < L; ++r) {
-        if (face && m.k0 + r == nz - 1) { y[r] = x[r]; continue; }  // trivial row: w = 0 at the top wall
-        y[r] = make_double2(dg * x[r].x - ck * y[r].x, dg * x[r].y - ck * y[r].y);
-    }
-}
-
-// Solve o x = rhs in place (x holds rhs on entry), preconditioned with the
-// tridiagonal rows a, b, c of the 2nd-order operator of the same equation.
-// Live vectors: x, the residual r, the search direction d and one product.
-template <int L>
-HC_DEV void z6_solve(const StepArgs &A, const Z6Op &o, const ColMap &m, double2 *buf, const double (&a)[L],
-                     const double (&b)[L], const double (&c)[L], double2 (&x)[L])
-{
-    const double lo = 0.9, hi = 1.85, th = 0.5 * (hi + lo), de = 0.5 * (hi - lo), s1 = th / de;
-    double2 r[L], d[1][L];
-#pragma unroll
-    for (int q = 0; q < L; ++q) { r[q] = x[q]; d[0][q] = x[q]; }
-    wave_tridiag<L, 1>(m.cnt, a, b, c, d);
-#pragma unroll
-    for (int q = 0; q < L; ++q) x[q] = d[0][q];
-    {
-        double2 z[1][L];
-        z6_mul<L>(A, o, m, buf, x, z[0]);
-#pragma unroll
-        for (int q = 0; q < L; ++q) { r[q].x -= z[0][q].x; r[q].y -= z[0][q].y; z[0][q] = r[q]; }
-        wave_tridiag<L, 1>(m.cnt, a, b, c, z);
-#pragma unroll
-        for (int q = 0; q < L; ++q) { d[0][q].x = z[0][q].x / th; d[0][q].y = z[0][q].y / th; }
-    }
-    double rho0 = 1.0 / s1;
-#pragma unroll 1
-    for (int it = 0; it < kZ6Iters; ++it) {
-        double2 z[1][L];
-#pragma unroll
-        for (int q = 0; q < L; ++q) { x[q].x += d[0][q].x; x[q].y += d[0][q].y; }
-        z6_mul<L>(A, o, m, buf, d[0], z[0]);
-#pragma unroll
-        for (int q = 0; q < L; ++q) { r[q].x -= z[0][q].x; r[q].y -= z[0][q].y; z[0][q] = r[q]; }
-        wave_tridiag<L, 1>(m.cnt, a, b, c, z);
-        const double rho1 = 1.0 / (2.0 * s1 - rho0), cd = rho1 * rho0, cz = 2.0 * rho1 / de;
-#pragma unroll
-        for (int q = 0; q < L; ++q) {
-            d[0][q].x = cd * d[0][q].x + cz * z[0][q].x;
-            d[0][q].y = cd * d[0][q].y + cz * z[0][q].y;
-        }
-        rho0 = rho1;
-    }
-}
-
 // (0,0)-mode wall data of the u and t equations: the Crank-Nicolson
 // implicit half of the wall terms, rc kappa w_wall (wall value or gradient),
 // for the rows whose stencils reach a wall datum.
@@ -219,21 +149,6 @@ HC_DEV void z6_fixups(const StepArgs &A, const Z6Op &o, const ColMap &m, double2
         const int jt = nz + 1 - s;
         if (has_top && jt < 7) d[r].x += ck * w[jt] * top;
     }
-}
-
-// Pressure solve of a column in place (rhs in phi on entry).
-template <int L>
-HC_DEV void z6_pressure(const StepArgs &A, const ColMap &m, double2 *buf, int l, int jm, bool is00, double2 (&phi)[L])
-{
-    double a[L], b[L], c[L];
-    Z6Op o = z6_op(A, 2, l, jm);
-    assemble<L>(A, 2, l, jm, is00, m, a, b, c);
-    if (is00) {
-        fixups<L>(A, 2, m, phi);
-        o.delnull = true;
-        o.b0 = b[0];
-    }
-    z6_solve<L>(A, o, m, buf, a, b, c, phi);
 }
 
 // Divergence of (u, v, w) times `scale`: horizontal symbols, ZO_D in z.
@@ -271,13 +186,40 @@ HC_DEV void z6_project(const StepArgs &A, const ColMap &m, double2 *buf, int l, 
     }
 }
 
-// FD substep column with the 6th-order z operators: the steps of
-// k_column_mom (u, v, w Helmholtz solves, divergence, pressure Poisson,
-// projection, pressure increment) with the z operators of the ZOp tables.
-// One field is live in registers at a time: the solved u*, v*, w* go back to
-// their spectra and are read again for the divergence and the projection.
+// Divergence of (s0, s1, s2) = (u, v, w) times `scale` into dst (ZO_D over
+// the interior faces, w(nz-1) taken as the top wall's zero): get_p_eqn_src
+// with the 6th-order z operator.
 template <int L>
-__global__ __launch_bounds__(64 * kColWaves) void k_column_mom6(StepArgs A, ColIO<double> io)
+__global__ __launch_bounds__(64 * kColWaves) void k_z6_div(StepArgs A, ColIO<double> io, double2 *dst, double scale)
+{
+    extern __shared__ double2 tile_raw[];
+    double2 *tile = tile_raw;
+    const ColPos pos = col_pos(io, A.ny);
+    if (pos.skip) return;
+    const int wave = threadIdx.x >> 6;
+    const int l = pos.bx * kColWaves + wave, jm = pos.jm;
+    const ColMap m = col_map(A.nz);
+    const size_t plane = (size_t)A.ny * A.nl, col0 = (size_t)jm * A.nl + (size_t)pos.bx * kColWaves;
+    double2 *buf = tile + (size_t)(kColWaves + wave) * A.nz;
+    double2 u[L], v[L], w[L], d[L];
+    col_load<L>(io.s0, tile, A.nz, plane, col0, m, u);
+    col_load<L>(io.s1, tile, A.nz, plane, col0, m, v);
+    col_load<L>(io.s2, tile, A.nz, plane, col0, m, w);
+#pragma unroll
+    for (int r = 0; r < L; ++r)
+        if (r < m.cnt && m.k0 + r == A.nz - 1) w[r] = make_double2(0.0, 0.0);
+    z6_divergence<L>(A, m, buf, l, jm, u, v, w, scale, d);
+    col_store<L>(dst, tile, A.nz, plane, col0, m, d);
+}
+
+// Projection u, v, w -= cdt grad phi (calculate_uvw) with phi in `phi`;
+// mode 0 (FD substep): cdt = rc dt, then the pressure increment
+// phi - rc dt nu / 2 (hxy_L + ZO_P) phi (calculate_new_p) replaces phi;
+// mode 1 (hc_project): cdt = 1; mode 2 (spectral substep): cdt = rc dt, the
+// increment is added to c_p (s3), and the columns outside the 2/3 rule are
+// zeroed.
+template <int L>
+__global__ __launch_bounds__(64 * kColWaves) void k_z6_proj(StepArgs A, ColIO<double> io, double2 *phi, int mode)
 {
     extern __shared__ double2 tile_raw[];
     double2 *tile = tile_raw;
@@ -287,172 +229,45 @@ __global__ __launch_bounds__(64 * kColWaves) void k_column_mom6(StepArgs A, ColI
     const int l = pos.bx * kColWaves + wave, jm = pos.jm;
     const ColMap m = col_map(A.nz);
     const int nz = A.nz, nl = A.nl, ny = A.ny;
-    const bool is00 = (l == 0 && jm == 0);
     const size_t plane = (size_t)ny * nl, col0 = (size_t)jm * nl + (size_t)pos.bx * kColWaves;
-    double2 *buf = tile + (size_t)(kColWaves + wave) * nz;  // this wavefront's column of the second sub-tile
+    double2 *buf = tile + (size_t)(kColWaves + wave) * nz;
+    const int mw = jm < ny / 2 ? jm : jm - ny;
+    const bool kept = mode != 2 || (l < A.lc && mw < A.mc && mw > -A.mc);
+    const double2 zero = make_double2(0.0, 0.0);
+    double2 ph[L], u[L], v[L], w[L];
+    col_load<L>(phi, tile, nz, plane, col0, m, ph);
+    col_load<L>(io.s0, tile, nz, plane, col0, m, u);
+    col_load<L>(io.s1, tile, nz, plane, col0, m, v);
+    col_load<L>(io.s2, tile, nz, plane, col0, m, w);
+    const double cdt = mode == 1 ? 1.0 : A.rc * A.dt;
+    z6_project<L>(A, m, buf, l, jm, cdt, ph, u, v, w);
+    if (!kept) {
+#pragma unroll
+        for (int r = 0; r < L; ++r) u[r] = v[r] = w[r] = zero;
+    }
+    col_store<L>(io.s0, tile, nz, plane, col0, m, u);
+    col_store<L>(io.s1, tile, nz, plane, col0, m, v);
+    col_store<L>(io.s2, tile, nz, plane, col0, m, w);
+    if (mode == 1) return;
     double2 x[L];
-    {
-        double a[L], b[L], c[L];
-        // u and v: one operator; every mode takes the (0,0) rows in the preconditioner
-        Z6Op o = z6_op(A, 0, l, jm);
-        assemble<L>(A, 0, l, jm, true, m, a, b, c);
-        col_load<L>(io.s0, tile, nz, plane, col0, m, x);
-        if (is00) z6_fixups<L>(A, o, m, x);
-        z6_solve<L>(A, o, m, buf, a, b, c, x);
-        col_store<L>(io.s0, tile, nz, plane, col0, m, x);
-        col_load<L>(io.s1, tile, nz, plane, col0, m, x);
-        z6_solve<L>(A, o, m, buf, a, b, c, x);
-        col_store<L>(io.s1, tile, nz, plane, col0, m, x);
-        // w; the top-wall row is trivial
-        col_load<L>(io.s2, tile, nz, plane, col0, m, x);
-#pragma unroll
-        for (int r = 0; r < L; ++r)
-            if (r < m.cnt && m.k0 + r == nz - 1) x[r] = make_double2(0.0, 0.0);
-        o = z6_op(A, 1, l, jm);
-        assemble<L>(A, 1, l, jm, true, m, a, b, c);
-        z6_solve<L>(A, o, m, buf, a, b, c, x);
-        col_store<L>(io.s2, tile, nz, plane, col0, m, x);
-    }
-    // divergence / dt (get_p_eqn_src) with ZO_D over the interior faces
-    double2 dp[L];
-    z6_apply<L, 6>(A, ZO_D, 1, nz - 1, nz, m, buf, x, dp);
-    {
-        const double2 sdx = make_double2(A.symx[SY_DRE * nl + l], A.symx[SY_DIM * nl + l]);
-        const double2 sdy = make_double2(A.symy[SY_DRE * ny + jm], A.symy[SY_DIM * ny + jm]);
-        col_load<L>(io.s0, tile, nz, plane, col0, m, x);
-#pragma unroll
-        for (int r = 0; r < L; ++r) dp[r] = cadd(dp[r], cmul(sdx, x[r]));
-        col_load<L>(io.s1, tile, nz, plane, col0, m, x);
-#pragma unroll
-        for (int r = 0; r < L; ++r) {
-            const double2 dv = cadd(dp[r], cmul(sdy, x[r]));
-            dp[r] = r < m.cnt ? make_double2(dv.x * A.idt, dv.y * A.idt) : make_double2(0.0, 0.0);
-        }
-    }
-    // pressure: rc (hxy + D G) phi = div / dt
-    z6_pressure<L>(A, m, buf, l, jm, is00, dp);
-    // projection (calculate_uvw): u, v by the horizontal gradient symbols, w by ZO_G
-    const double cdt = A.rc * A.dt;
-    {
-        double2 g[L];
-        z6_apply<L, 6>(A, ZO_G, 0, nz, nz - 1, m, buf, dp, g);
-        col_load<L>(io.s2, tile, nz, plane, col0, m, x);
-#pragma unroll
-        for (int r = 0; r < L; ++r)
-            if (r < m.cnt && m.k0 + r < nz - 1) { x[r].x -= cdt * g[r].x; x[r].y -= cdt * g[r].y; }
-        col_store<L>(io.s2, tile, nz, plane, col0, m, x);
-    }
-    {
-        const double2 sgx = make_double2(A.symx[SY_GRE * nl + l], A.symx[SY_GIM * nl + l]);
-        col_load<L>(io.s0, tile, nz, plane, col0, m, x);
-#pragma unroll
-        for (int r = 0; r < L; ++r) x[r] = csub(x[r], cscale(cmul(sgx, dp[r]), cdt));
-        col_store<L>(io.s0, tile, nz, plane, col0, m, x);
-        const double2 sgy = make_double2(A.symy[SY_GRE * ny + jm], A.symy[SY_GIM * ny + jm]);
-        col_load<L>(io.s1, tile, nz, plane, col0, m, x);
-#pragma unroll
-        for (int r = 0; r < L; ++r) x[r] = csub(x[r], cscale(cmul(sgy, dp[r]), cdt));
-        col_store<L>(io.s1, tile, nz, plane, col0, m, x);
-    }
-    // pressure increment phi - rc dt nu / 2 (hxy_L + D G) phi (calculate_new_p)
-    z6_apply<L, kZW>(A, ZO_P, 0, nz, nz, m, buf, dp, x);
+    z6_apply<L, kZW>(A, ZO_P, 0, nz, nz, m, buf, ph, x);
     const double slap = A.symx[SY_L * nl + l] + A.symy[SY_L * ny + jm];
     const double cnu = A.rc * 0.5 * A.dt * A.nu;
 #pragma unroll
-    for (int r = 0; r < L; ++r) {
-        x[r] = make_double2(dp[r].x - cnu * (slap * dp[r].x + x[r].x), dp[r].y - cnu * (slap * dp[r].y + x[r].y));
-        if (r >= m.cnt) x[r] = make_double2(0.0, 0.0);
-    }
-    col_store<L>(io.s3, tile, nz, plane, col0, m, x);
-}
-
-// Temperature Helmholtz solve with the 6th-order z operator, in place on s0.
-template <int L>
-__global__ __launch_bounds__(64 * kColWaves) void k_column_t6(StepArgs A, ColIO<double> io)
-{
-    extern __shared__ double2 tile_raw[];
-    double2 *tile = tile_raw;
-    const ColPos pos = col_pos(io, A.ny);
-    if (pos.skip) return;
-    const int wave = threadIdx.x >> 6;
-    const int l = pos.bx * kColWaves + wave, jm = pos.jm;
-    const ColMap m = col_map(A.nz);
-    const bool is00 = (l == 0 && jm == 0);
-    const size_t plane = (size_t)A.ny * A.nl, col0 = (size_t)jm * A.nl + (size_t)pos.bx * kColWaves;
-    double2 d[L];
-    col_load<L>(io.s0, tile, A.nz, plane, col0, m, d);
-    __syncthreads();
-    double2 *buf = tile + (size_t)(kColWaves + wave) * A.nz;
-    double a[L], b[L], c[L];
-    const Z6Op o = z6_op(A, 3, l, jm);
-    assemble<L>(A, 3, l, jm, true, m, a, b, c);
-    if (is00) z6_fixups<L>(A, o, m, d);
-    z6_solve<L>(A, o, m, buf, a, b, c, d);
-    col_store<L>(io.s0, tile, A.nz, plane, col0, m, d);
-}
-
-// Projection onto divergence-free fields (rk_c = 1) with the 6th-order
-// operators: phi = (D G)^-1 div u, u -= G phi.
-template <int L>
-__global__ __launch_bounds__(64 * kColWaves) void k_column_project6(StepArgs A, ColIO<double> io)
-{
-    extern __shared__ double2 tile_raw[];
-    double2 *tile = tile_raw;
-    const ColPos pos = col_pos(io, A.ny);
-    if (pos.skip) return;
-    const int wave = threadIdx.x >> 6;
-    const int l = pos.bx * kColWaves + wave, jm = pos.jm;
-    const ColMap m = col_map(A.nz);
-    const bool is00 = (l == 0 && jm == 0);
-    const size_t plane = (size_t)A.ny * A.nl, col0 = (size_t)jm * A.nl + (size_t)pos.bx * kColWaves;
-    double2 u[L], v[L], w[L], ph[L];
-    col_load<L>(io.s0, tile, A.nz, plane, col0, m, u);
-    col_load<L>(io.s1, tile, A.nz, plane, col0, m, v);
-    col_load<L>(io.s2, tile, A.nz, plane, col0, m, w);
-#pragma unroll
     for (int r = 0; r < L; ++r)
-        if (r < m.cnt && m.k0 + r == A.nz - 1) w[r] = make_double2(0.0, 0.0);
-    double2 *buf = tile + (size_t)(kColWaves + wave) * A.nz;
-    z6_divergence<L>(A, m, buf, l, jm, u, v, w, 1.0, ph);
-    z6_pressure<L>(A, m, buf, l, jm, is00, ph);
-    z6_project<L>(A, m, buf, l, jm, 1.0, ph, u, v, w);
-    col_store<L>(io.s0, tile, A.nz, plane, col0, m, u);
-    col_store<L>(io.s1, tile, A.nz, plane, col0, m, v);
-    col_store<L>(io.s2, tile, A.nz, plane, col0, m, w);
-}
-
-// Standalone solve of one equation with the 6th-order z operators, in place
-// on s0.  eq: 0 u (wall value -u_mrf), 1 v, 2 w, 3 p, 4 t (tbot, ttop or
-// the top gradient), as k_column_solve.
-template <int L>
-__global__ __launch_bounds__(64 * kColWaves) void k_column_solve6(StepArgs A, ColIO<double> io, int eq)
-{
-    extern __shared__ double2 tile_raw[];
-    double2 *tile = tile_raw;
-    const ColPos pos = col_pos(io, A.ny);
-    if (pos.skip) return;
-    const int wave = threadIdx.x >> 6;
-    const int l = pos.bx * kColWaves + wave, jm = pos.jm;
-    const ColMap m = col_map(A.nz);
-    const bool is00 = (l == 0 && jm == 0);
-    const size_t plane = (size_t)A.ny * A.nl, col0 = (size_t)jm * A.nl + (size_t)pos.bx * kColWaves;
-    double2 d[L];
-    col_load<L>(io.s0, tile, A.nz, plane, col0, m, d);
+        x[r] = make_double2(ph[r].x - cnu * (slap * ph[r].x + x[r].x), ph[r].y - cnu * (slap * ph[r].y + x[r].y));
+    if (mode == 0) {
 #pragma unroll
-    for (int r = 0; r < L; ++r)
-        if (eq == 2 && r < m.cnt && m.k0 + r == A.nz - 1) d[r] = make_double2(0.0, 0.0);
-    double2 *buf = tile + (size_t)(kColWaves + wave) * A.nz;
-    const int meq = eq == 0 || eq == 1 ? 0 : (eq == 2 ? 1 : (eq == 3 ? 2 : 3));
-    if (meq == 2) {
-        z6_pressure<L>(A, m, buf, l, jm, is00, d);
-    } else {
-        double a[L], b[L], c[L];
-        const Z6Op o = z6_op(A, meq, l, jm);
-        assemble<L>(A, meq, l, jm, true, m, a, b, c);
-        if (is00 && (eq == 0 || eq == 4)) z6_fixups<L>(A, o, m, d);
-        z6_solve<L>(A, o, m, buf, a, b, c, d);
+        for (int r = 0; r < L; ++r)
+            if (r >= m.cnt) x[r] = zero;
+        col_store<L>(io.s3, tile, nz, plane, col0, m, x);
+        return;
     }
-    col_store<L>(io.s0, tile, A.nz, plane, col0, m, d);
+    double2 p[L];
+    col_load<L>(io.s3, tile, nz, plane, col0, m, p);
+#pragma unroll
+    for (int r = 0; r < L; ++r) p[r] = (kept && r < m.cnt) ? cadd(p[r], x[r]) : zero;
+    col_store<L>(io.s3, tile, nz, plane, col0, m, p);
 }
 
 // ---- spectral path (cds = 3) with the 6th-order z operators ---------------
@@ -513,13 +328,12 @@ HC_DEV void spec_assemble6(const StepArgs &A, const SpecIO<double> &io, double2 
     col_store<L>(fo, tile, nz, plane, col0, m, f);
 }
 
-// Spectral substep column (k_column_spec) with the 6th-order z operators.
-// Every wavefront runs the solves (the operator products synchronise the
-// block); columns outside the 2/3 rule are zeroed as they are stored.  One
-// field is live in registers at a time: the solved u*, v*, w* go back to
-// c_u, c_v, c_w and are read again for the divergence and the projection.
+// Right-hand sides of the spectral substep's u, v, w Helmholtz systems
+// (get_momentum_cnvdiff_spec with the 6th-order z operators), in place of
+// c_u, c_v, c_w; w's top wall level is zero.  The new right-hand sides f
+// replace the old ones.
 template <int L>
-__global__ __launch_bounds__(64 * kColWaves) void k_column_spec6(StepArgs A, SpecIO<double> io)
+__global__ __launch_bounds__(64 * kColWaves) void k_spec6_rhs(StepArgs A, SpecIO<double> io)
 {
     extern __shared__ double2 tile_raw[];
     double2 *tile = tile_raw;
@@ -530,92 +344,27 @@ __global__ __launch_bounds__(64 * kColWaves) void k_column_spec6(StepArgs A, Spe
     const ColMap m = col_map(A.nz);
     const int nz = A.nz, nl = A.nl, ny = A.ny;
     const bool is00 = (l == 0 && jm == 0);
-    const int mw = jm < ny / 2 ? jm : jm - ny;
-    const bool kept = l < A.lc && mw < A.mc && mw > -A.mc;
     const size_t plane = (size_t)ny * nl, col0 = (size_t)jm * nl + (size_t)pos.bx * kColWaves;
     const double kx = A.symx[SY_GIM * nl + l], ky = A.symy[SY_GIM * ny + jm];
     const double hxy = A.rc * (A.symx[SY_HU * nl + l] + A.symy[SY_HU * ny + jm]);
     double2 *buf = tile + (size_t)(kColWaves + wave) * nz;
-    double2 x[L];
-    {
-        double a[L], b[L], c[L];
-        Z6Op o = z6_op(A, 0, l, jm);
-        assemble<L>(A, 0, l, jm, true, m, a, b, c);
-        spec_assemble6<L>(A, io, tile, buf, m, plane, col0, 0, l, jm, is00, kx, ky, hxy, x);
-        if (is00) z6_fixups<L>(A, o, m, x);
-        z6_solve<L>(A, o, m, buf, a, b, c, x);
-        col_store<L>(io.cu, tile, nz, plane, col0, m, x);
-        spec_assemble6<L>(A, io, tile, buf, m, plane, col0, 1, l, jm, is00, kx, ky, hxy, x);
-        z6_solve<L>(A, o, m, buf, a, b, c, x);
-        col_store<L>(io.cv, tile, nz, plane, col0, m, x);
-        spec_assemble6<L>(A, io, tile, buf, m, plane, col0, 2, l, jm, is00, kx, ky, hxy, x);
+    double2 q[L];
+    spec_assemble6<L>(A, io, tile, buf, m, plane, col0, 0, l, jm, is00, kx, ky, hxy, q);
+    col_store<L>(io.cu, tile, nz, plane, col0, m, q);
+    spec_assemble6<L>(A, io, tile, buf, m, plane, col0, 1, l, jm, is00, kx, ky, hxy, q);
+    col_store<L>(io.cv, tile, nz, plane, col0, m, q);
+    spec_assemble6<L>(A, io, tile, buf, m, plane, col0, 2, l, jm, is00, kx, ky, hxy, q);
 #pragma unroll
-        for (int r = 0; r < L; ++r)
-            if (r < m.cnt && m.k0 + r == nz - 1) x[r] = make_double2(0.0, 0.0);
-        o = z6_op(A, 1, l, jm);
-        assemble<L>(A, 1, l, jm, true, m, a, b, c);
-        z6_solve<L>(A, o, m, buf, a, b, c, x);
-        col_store<L>(io.cw, tile, nz, plane, col0, m, x);
-    }
-    // divergence / dt with ZO_D (get_div_spec)
-    double2 dp[L];
-    z6_apply<L, 6>(A, ZO_D, 1, nz - 1, nz, m, buf, x, dp);
-    {
-        const double2 sdx = make_double2(A.symx[SY_DRE * nl + l], A.symx[SY_DIM * nl + l]);
-        const double2 sdy = make_double2(A.symy[SY_DRE * ny + jm], A.symy[SY_DIM * ny + jm]);
-        col_load<L>(io.cu, tile, nz, plane, col0, m, x);
-#pragma unroll
-        for (int r = 0; r < L; ++r) dp[r] = cadd(dp[r], cmul(sdx, x[r]));
-        col_load<L>(io.cv, tile, nz, plane, col0, m, x);
-#pragma unroll
-        for (int r = 0; r < L; ++r) {
-            const double2 dv = cadd(dp[r], cmul(sdy, x[r]));
-            dp[r] = r < m.cnt ? make_double2(dv.x * A.idt, dv.y * A.idt) : make_double2(0.0, 0.0);
-        }
-    }
-    z6_pressure<L>(A, m, buf, l, jm, is00, dp);
-    const double2 zero = make_double2(0.0, 0.0);
-    // new pressure (:461): p + phi - rc dt nu / 2 (hxy_L + D G) phi
-    {
-        double2 t[L];
-        z6_apply<L, kZW>(A, ZO_P, 0, nz, nz, m, buf, dp, t);
-        const double slap = A.symx[SY_L * nl + l] + A.symy[SY_L * ny + jm];
-        const double cnu = A.rc * 0.5 * A.dt * A.nu;
-        col_load<L>(io.cp, tile, nz, plane, col0, m, x);
-#pragma unroll
-        for (int r = 0; r < L; ++r) {
-            x[r].x += dp[r].x - cnu * (slap * dp[r].x + t[r].x);
-            x[r].y += dp[r].y - cnu * (slap * dp[r].y + t[r].y);
-            if (!kept || r >= m.cnt) x[r] = zero;
-        }
-        col_store<L>(io.cp, tile, nz, plane, col0, m, x);
-        // projection (:464): w by ZO_G
-        z6_apply<L, 6>(A, ZO_G, 0, nz, nz - 1, m, buf, dp, t);
-        col_load<L>(io.cw, tile, nz, plane, col0, m, x);
-        const double cdt = A.rc * A.dt;
-#pragma unroll
-        for (int r = 0; r < L; ++r) {
-            if (r < m.cnt && m.k0 + r < nz - 1) { x[r].x -= cdt * t[r].x; x[r].y -= cdt * t[r].y; }
-            if (!kept) x[r] = zero;
-        }
-        col_store<L>(io.cw, tile, nz, plane, col0, m, x);
-    }
-    const double cdt = A.rc * A.dt;
-    const double2 sgx = make_double2(A.symx[SY_GRE * nl + l], A.symx[SY_GIM * nl + l]);
-    col_load<L>(io.cu, tile, nz, plane, col0, m, x);
-#pragma unroll
-    for (int r = 0; r < L; ++r) x[r] = kept ? csub(x[r], cscale(cmul(sgx, dp[r]), cdt)) : zero;
-    col_store<L>(io.cu, tile, nz, plane, col0, m, x);
-    const double2 sgy = make_double2(A.symy[SY_GRE * ny + jm], A.symy[SY_GIM * ny + jm]);
-    col_load<L>(io.cv, tile, nz, plane, col0, m, x);
-#pragma unroll
-    for (int r = 0; r < L; ++r) x[r] = kept ? csub(x[r], cscale(cmul(sgy, dp[r]), cdt)) : zero;
-    col_store<L>(io.cv, tile, nz, plane, col0, m, x);
+    for (int r = 0; r < L; ++r)
+        if (r < m.cnt && m.k0 + r == nz - 1) q[r] = make_double2(0.0, 0.0);
+    col_store<L>(io.cw, tile, nz, plane, col0, m, q);
 }
 
-// Spectral temperature column (k_column_spec_t) with the 6th-order z operator.
+// Right-hand side of the spectral temperature system
+// (get_temperature_cnvdiff_spec with the 6th-order z operator) in place of
+// c_t; the new right-hand side s replaces the old one in c_ss.
 template <int L>
-__global__ __launch_bounds__(64 * kColWaves) void k_column_spec_t6(StepArgs A, SpecIO<double> io)
+__global__ __launch_bounds__(64 * kColWaves) void k_spec6_t_rhs(StepArgs A, SpecIO<double> io)
 {
     extern __shared__ double2 tile_raw[];
     double2 *tile = tile_raw;
@@ -626,8 +375,6 @@ __global__ __launch_bounds__(64 * kColWaves) void k_column_spec_t6(StepArgs A, S
     const ColMap m = col_map(A.nz);
     const int nz = A.nz, nl = A.nl, ny = A.ny;
     const bool is00 = (l == 0 && jm == 0);
-    const int mw = jm < ny / 2 ? jm : jm - ny;
-    const bool kept = l < A.lc && mw < A.mc && mw > -A.mc;
     const size_t plane = (size_t)ny * nl, col0 = (size_t)jm * nl + (size_t)pos.bx * kColWaves;
     const double kx = A.symx[SY_GIM * nl + l], ky = A.symy[SY_GIM * ny + jm];
     const double hxy = A.rc * (A.symx[SY_HT * nl + l] + A.symy[SY_HT * ny + jm]);
@@ -660,13 +407,5 @@ __global__ __launch_bounds__(64 * kColWaves) void k_column_spec_t6(StepArgs A, S
 #pragma unroll
     for (int r = 0; r < L; ++r) d[r] = make_double2(d[r].x + rb * x[r].x + ra * f[r].x, d[r].y + rb * x[r].y + ra * f[r].y);
     col_store<L>(io.css, tile, nz, plane, col0, m, f);
-    double a[L], b[L], c[L];
-    assemble<L>(A, 3, l, jm, true, m, a, b, c);
-    if (is00) z6_fixups<L>(A, o, m, d);  // implicit half
-    z6_solve<L>(A, o, m, buf, a, b, c, d);
-    if (!kept) {
-#pragma unroll
-        for (int r = 0; r < L; ++r) d[r] = make_double2(0.0, 0.0);
-    }
     col_store<L>(io.ct, tile, nz, plane, col0, m, d);
 }
