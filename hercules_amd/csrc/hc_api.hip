@@ -477,17 +477,23 @@ static bool build_zops(hc_ctx *c)
     // C2 on the levels (data index = level + 1), F2 on the interior faces
     // (data index = w level + 1, nz - 1 unknowns), P on the levels.
     c->h_zband.assign((size_t)ZB_COUNT * nz * kBandRow, 0.0);
-    const int bops[ZB_COUNT] = {ZO_C2_DD, ZO_C2_DN, ZO_F2, ZO_P};
-    const int bhalf[ZB_COUNT] = {5, 5, 5, 7};  // half bands the solver is instantiated for
+    const int bops[ZB_COUNT] = {ZO_C2_DD, ZO_C2_DN, ZO_F2, ZO_P, ZO_G, ZO_D};
+    // half bands the solver is instantiated for (5 and 7); the explicit G and D
+    // are applied over the full stored band
+    const int bhalf[ZB_COUNT] = {5, 5, 5, 7, kBandMax, kBandMax};
     for (int b = 0; b < ZB_COUNT; ++b) {
         const int op = bops[b];
-        const int n = op == ZO_F2 ? nz - 1 : nz, off = op == ZO_P ? 0 : 1, W = op == ZO_P ? kZW : 7;
-        for (int k = 0; k < n; ++k) {
+        // rows, the columns' range and the data index of column 0
+        const int nrow = (op == ZO_F2 || op == ZO_G) ? nz - 1 : nz;
+        const int ncol = (op == ZO_F2 || op == ZO_D) ? nz - 1 : nz;
+        const int off = (op == ZO_P || op == ZO_G) ? 0 : 1;
+        const int W = op == ZO_P ? kZW : (op == ZO_G || op == ZO_D) ? 6 : 7;
+        for (int k = 0; k < nrow; ++k) {
             const size_t e = (size_t)op * nz + k;
             for (int j = 0; j < W; ++j) {
                 const double w = c->h_zwt[e * kZW + j];
                 const int lv = c->h_zst[e] + j - off;
-                if (w == 0.0 || lv < 0 || lv >= n) continue;
+                if (w == 0.0 || lv < 0 || lv >= ncol) continue;
                 const int d = lv - k;
                 if (d < -bhalf[b] || d > bhalf[b]) return false;
                 c->h_zband[((size_t)b * nz + k) * kBandRow + kBandMax + d] += w;
@@ -723,9 +729,6 @@ extern "C" int hc_create(const hc_params *prm, hc_ctx **out)
         if (P.precision != HC_FP64) return set_err(HC_ERR_UNSUPPORTED, "zorder=6 needs precision=0 (FP64)");
         if (P.isdamp == 1) return set_err(HC_ERR_UNSUPPORTED, "zorder=6 does not cover Rayleigh damping");
         if (P.nz < 16) return set_err(HC_ERR_UNSUPPORTED, "zorder=6 needs nz >= 16");
-        // two column sub-tiles (staging, operator products) of nz x 8 complex in the LDS
-        if ((size_t)2 * P.nz * kColWaves * sizeof(double2) > kColLds)
-            return set_err(HC_ERR_UNSUPPORTED, "zorder=6 needs nz <= %d", (int)(kColLds / (2 * kColWaves * sizeof(double2))));
     }
     if (!fft_len_ok(P.nx) || !fft_len_ok(P.ny)) return set_err(HC_ERR_UNSUPPORTED, "nx, ny must be one of 16,32,64,128,192,256,512");
     {

@@ -211,3 +211,264 @@ __global__ __launch_bounds__(256) void k_z6_band(StepArgs A, BandArgs B)
         if (nr > 1) S.f[1][o] = zero_out ? make_double2(0.0, 0.0) : r[1];
     }
 }
+
+// ---------------------------------------------------------------------------
+// Explicit column operations of the 6th-order step, one thread per (l, m)
+// column streaming along z.  A thread keeps a window of kWin levels of the
+// fields that an operator reaches (levels k-7 .. k+7 around the row k being
+// formed) and applies band rows (ZB_* tables, read as block-uniform values)
+// to it; the fields read at the row itself pass through a delay of kBandMax
+// levels.  Loads run kPF levels ahead of the window through a staging ring
+// indexed by the unrolled loop position, so the wavefront never waits on a
+// load issued less than kPF levels earlier.
+constexpr int kWin = 2 * kBandMax + 1;
+
+// Iterations i = 0, 1, ...: window field w enters level i, point fields
+// enter level i - kBandMax, and row k = i - kBandMax (0 <= k < nrows) is
+// formed by row(k, win, pts) with win[kBandMax + t] = level k + t (zero
+// outside 0 .. wlev[f] - 1).
+template <int N>
+struct FieldSet {
+    const double2 *p[N > 0 ? N : 1];
+    int n[N > 0 ? N : 1];  // levels of a window field
+};
+template <int NW, int NP, int PF, class Row>
+HC_DEV void z6_stream(size_t plane, size_t col, int nrows, const FieldSet<NW> ws, const FieldSet<NP> ps, Row &&row)
+{
+    const auto &wlev = ws.n;
+    const double2 zero = make_double2(0.0, 0.0);
+    double2 win[NW][kWin], fw[NW][PF], fp[PF][NP > 0 ? NP : 1];
+#pragma unroll
+    for (int f = 0; f < NW; ++f)
+#pragma unroll
+        for (int t = 0; t < kWin; ++t) win[f][t] = zero;
+#pragma unroll
+    for (int u = 0; u < PF; ++u) {
+#pragma unroll
+        for (int f = 0; f < NW; ++f) fw[f][u] = u < wlev[f] ? ws.p[f][(size_t)u * plane + col] : zero;
+        const int lp = u - kBandMax;
+#pragma unroll
+        for (int p = 0; p < NP; ++p) fp[u][p] = (lp >= 0 && lp < nrows) ? ps.p[p][(size_t)lp * plane + col] : zero;
+    }
+    for (int i0 = 0; i0 < nrows + kBandMax; i0 += PF) {
+#pragma unroll
+        for (int u = 0; u < PF; ++u) {
+            const int i = i0 + u;
+            double2 pts[NP > 0 ? NP : 1];
+#pragma unroll
+            for (int f = 0; f < NW; ++f) {
+#pragma unroll
+                for (int t = 0; t < kWin - 1; ++t) win[f][t] = win[f][t + 1];
+                win[f][kWin - 1] = fw[f][u];
+            }
+#pragma unroll
+            for (int p = 0; p < NP; ++p) pts[p] = fp[u][p];
+            // the loads of iteration i + PF
+            const int li = i + PF, lp = li - kBandMax;
+#pragma unroll
+            for (int f = 0; f < NW; ++f) fw[f][u] = li < wlev[f] ? ws.p[f][(size_t)li * plane + col] : zero;
+#pragma unroll
+            for (int p = 0; p < NP; ++p) fp[u][p] = (lp >= 0 && lp < nrows) ? ps.p[p][(size_t)lp * plane + col] : zero;
+            const int k = i - kBandMax;
+            if (k >= 0 && k < nrows) row(k, win, pts);
+        }
+    }
+}
+
+// sum_t band(k, t) x(k + t) over the stored band
+HC_DEV double2 band_dot(const double *brow, const double2 (&x)[kWin])
+{
+    double2 acc = make_double2(0.0, 0.0);
+#pragma unroll
+    for (int t = 0; t < kWin; ++t) {
+        const double b = brow[t];
+        acc.x += b * x[t].x;
+        acc.y += b * x[t].y;
+    }
+    return acc;
+}
+HC_DEV const double *band_row(const StepArgs &A, int band, int k)
+{
+    return A.zband + ((size_t)band * A.nz + k) * kBandRow;
+}
+
+struct ColIdx {
+    int l, jm;
+    size_t col, plane;
+};
+HC_DEV bool col_idx(const StepArgs &A, ColIdx &c)
+{
+    const int ncol = A.nl * A.ny;
+    const int q = blockIdx.x * 256 + threadIdx.x;
+    if (q >= ncol) return false;
+    c.l = q % A.nl;
+    c.jm = q / A.nl;
+    c.col = (size_t)q;
+    c.plane = (size_t)ncol;
+    return true;
+}
+
+// Divergence D_x u + D_y v + D_z w of a column times `scale` into dst
+// (get_p_eqn_src / get_div_spec with ZO_D over the interior faces).
+__global__ __launch_bounds__(256) void k_z6s_div(StepArgs A, const double2 *u, const double2 *v, const double2 *w,
+                                                 double2 *dst, double scale)
+{
+    ColIdx c;
+    if (!col_idx(A, c)) return;
+    const int nl = A.nl, ny = A.ny, nz = A.nz;
+    const double2 sdx = make_double2(A.symx[SY_DRE * nl + c.l], A.symx[SY_DIM * nl + c.l]);
+    const double2 sdy = make_double2(A.symy[SY_DRE * ny + c.jm], A.symy[SY_DIM * ny + c.jm]);
+    const FieldSet<1> ws = {{w}, {nz - 1}};
+    const FieldSet<2> ps = {{u, v}, {0, 0}};
+    z6_stream<1, 2, 8>(c.plane, c.col, nz, ws, ps, [&](int k, const double2 (&win)[1][kWin], const double2 (&p)[2]) __attribute__((always_inline)) {
+        const double2 dz = band_dot(band_row(A, ZB_D, k), win[0]);
+        const double2 d = cadd(cadd(cmul(sdx, p[0]), cmul(sdy, p[1])), dz);
+        dst[(size_t)k * c.plane + c.col] = make_double2(d.x * scale, d.y * scale);
+    });
+}
+
+// Projection u, v, w -= cdt grad phi (calculate_uvw, ZO_G at the interior
+// faces).  mode 0 (FD substep, cdt = rc dt): phi is then replaced by the
+// pressure increment phi - rc dt nu / 2 (hxy_L + ZO_P) phi
+// (calculate_new_p); mode 1 (projection of a state, cdt = 1); mode 2
+// (spectral substep, cdt = rc dt): the increment is added to c_p in `pinc`
+// and the columns outside the 2/3 rule are zeroed.
+__global__ __launch_bounds__(256) void k_z6s_proj(StepArgs A, double2 *u, double2 *v, double2 *w, double2 *phi,
+                                                  double2 *pinc, int mode)
+{
+    ColIdx c;
+    if (!col_idx(A, c)) return;
+    const int nl = A.nl, ny = A.ny, nz = A.nz;
+    const double2 sgx = make_double2(A.symx[SY_GRE * nl + c.l], A.symx[SY_GIM * nl + c.l]);
+    const double2 sgy = make_double2(A.symy[SY_GRE * ny + c.jm], A.symy[SY_GIM * ny + c.jm]);
+    const double slap = A.symx[SY_L * nl + c.l] + A.symy[SY_L * ny + c.jm];
+    const double cdt = mode == 1 ? 1.0 : A.rc * A.dt, cnu = A.rc * 0.5 * A.dt * A.nu;
+    const int mw = c.jm < ny / 2 ? c.jm : c.jm - ny;
+    const bool kept = mode != 2 || (c.l < A.lc && mw < A.mc && mw > -A.mc);
+    const double2 zero = make_double2(0.0, 0.0);
+    const FieldSet<1> ws = {{phi}, {nz}};
+    const FieldSet<4> ps = {{u, v, w, mode == 2 ? pinc : phi}, {0, 0, 0, 0}};
+    z6_stream<1, 4, 6>(c.plane, c.col, nz, ws, ps, [&](int k, const double2 (&win)[1][kWin], const double2 (&p)[4]) __attribute__((always_inline)) {
+        const size_t o = (size_t)k * c.plane + c.col;
+        const double2 ph = win[0][kBandMax];
+        double2 uu = csub(p[0], cscale(cmul(sgx, ph), cdt));
+        double2 vv = csub(p[1], cscale(cmul(sgy, ph), cdt));
+        double2 ww = p[2];
+        if (k < nz - 1) {
+            const double2 g = band_dot(band_row(A, ZB_G, k), win[0]);
+            ww = make_double2(ww.x - cdt * g.x, ww.y - cdt * g.y);
+        }
+        u[o] = kept ? uu : zero;
+        v[o] = kept ? vv : zero;
+        w[o] = kept ? ww : zero;
+        if (mode == 1) return;
+        const double2 pz = band_dot(band_row(A, ZB_P, k), win[0]);
+        const double2 inc = make_double2(ph.x - cnu * (slap * ph.x + pz.x), ph.y - cnu * (slap * ph.y + pz.y));
+        if (mode == 0) phi[o] = inc;
+        else pinc[o] = kept ? cadd(p[3], inc) : zero;
+    });
+}
+
+// Right-hand sides of the spectral substep's Helmholtz systems
+// (get_momentum_cnvdiff_spec, src/module_spectral.f90:35-504, with the
+// 6th-order z operators), in place of c_u, c_v (blockIdx.y = 0, 1) or c_w
+// (k_z6s_rhs_w):
+//   q = c + hxy c + dif Z c - cdt grad_e p + rb dt f_old + ra dt f
+//   f = -u_mrf i kx c - coe (i kx X + i ky Y) + coe R
+// with X, Y, R the transformed products of the component; f replaces f_old.
+// The (0,0) mode adds the explicit half of the wall data of u.
+__global__ __launch_bounds__(256) void k_z6s_rhs_uv(StepArgs A, SpecIO<double> io)
+{
+    ColIdx c;
+    if (!col_idx(A, c)) return;
+    const int e = blockIdx.y;
+    const int nl = A.nl, ny = A.ny, nz = A.nz;
+    const double kx = A.symx[SY_GIM * nl + c.l], ky = A.symy[SY_GIM * ny + c.jm];
+    const double hxy = A.rc * (A.symx[SY_HU * nl + c.l] + A.symy[SY_HU * ny + c.jm]);
+    const double dif = A.rc * 0.5 * A.dt * A.nu, coe = A.coe, um = A.u_mrf, cdt = A.rc * A.dt;
+    const double rb = A.rb * A.dt, ra = A.ra * A.dt, kk = e == 0 ? kx : ky;
+    const bool is00 = e == 0 && c.l == 0 && c.jm == 0;
+    const int band = A.ochannel ? ZB_C2_DN : ZB_C2_DD, op = A.ochannel ? ZO_C2_DN : ZO_C2_DD;
+    double2 *cs = e == 0 ? io.cu : io.cv, *fo = e == 0 ? io.cff : io.cgg;
+    const FieldSet<1> ws = {{cs}, {nz}};
+    const FieldSet<5> ps = {{io.sp[e == 0 ? 0 : 1], io.sp[e == 0 ? 1 : 2], io.sp[e == 0 ? 3 : 4], io.cp, fo}, {0, 0, 0, 0, 0}};
+    z6_stream<1, 5, 4>(c.plane, c.col, nz, ws, ps, [&](int k, const double2 (&win)[1][kWin], const double2 (&p)[5]) __attribute__((always_inline)) {
+        const double2 x = win[0][kBandMax];
+        const double2 t = band_dot(band_row(A, band, k), win[0]);
+        double2 q = make_double2(x.x + hxy * x.x + dif * t.x, x.y + hxy * x.y + dif * t.y);
+        if (is00) q.x += z6_wall_rhs(A, 0, op, dif, k);
+        double2 f = times_i(x, -um * kx);
+        f = csub(f, times_i(p[0], coe * kx));
+        f = csub(f, times_i(p[1], coe * ky));
+        f = make_double2(f.x + coe * p[2].x, f.y + coe * p[2].y);
+        q = csub(q, times_i(p[3], cdt * kk));
+        q = make_double2(q.x + rb * p[4].x + ra * f.x, q.y + rb * p[4].y + ra * f.y);
+        const size_t o = (size_t)k * c.plane + c.col;
+        fo[o] = f;
+        cs[o] = q;
+    });
+}
+__global__ __launch_bounds__(256) void k_z6s_rhs_w(StepArgs A, SpecIO<double> io)
+{
+    ColIdx c;
+    if (!col_idx(A, c)) return;
+    const int nl = A.nl, ny = A.ny, nz = A.nz;
+    const double kx = A.symx[SY_GIM * nl + c.l], ky = A.symy[SY_GIM * ny + c.jm];
+    const double hxy = A.rc * (A.symx[SY_HU * nl + c.l] + A.symy[SY_HU * ny + c.jm]);
+    const double dif = A.rc * 0.5 * A.dt * A.nu, coe = A.coe, um = A.u_mrf, cdt = A.rc * A.dt;
+    const double rb = A.rb * A.dt, ra = A.ra * A.dt;
+    const double2 zero = make_double2(0.0, 0.0);
+    const FieldSet<2> ws = {{io.cw, io.cp}, {nz - 1, nz}};
+    const FieldSet<4> ps = {{io.sp[5], io.sp[6], io.sp[7], io.chh}, {0, 0, 0, 0}};
+    z6_stream<2, 4, 4>(c.plane, c.col, nz, ws, ps, [&](int k, const double2 (&win)[2][kWin], const double2 (&p)[4]) __attribute__((always_inline)) {
+        const size_t o = (size_t)k * c.plane + c.col;
+        const double2 x = win[0][kBandMax];
+        double2 f = times_i(x, -um * kx);
+        f = csub(f, times_i(p[0], coe * kx));
+        f = csub(f, times_i(p[1], coe * ky));
+        f = make_double2(f.x + coe * p[2].x, f.y + coe * p[2].y);
+        io.chh[o] = f;
+        if (k == nz - 1) {  // the top wall face
+            io.cw[o] = zero;
+            return;
+        }
+        const double2 t = band_dot(band_row(A, ZB_F2, k), win[0]);
+        const double2 g = band_dot(band_row(A, ZB_G, k), win[1]);
+        double2 q = make_double2(x.x + hxy * x.x + dif * t.x - cdt * g.x, x.y + hxy * x.y + dif * t.y - cdt * g.y);
+        q = make_double2(q.x + rb * p[3].x + ra * f.x, q.y + rb * p[3].y + ra * f.y);
+        io.cw[o] = q;
+    });
+}
+
+// Right-hand side of the spectral temperature system
+// (get_temperature_cnvdiff_spec with the 6th-order z operator) in place of
+// c_t; the new s replaces the old one in c_ss; the (0,0) mode adds the
+// explicit half of the wall data.
+__global__ __launch_bounds__(256) void k_z6s_rhs_t(StepArgs A, SpecIO<double> io)
+{
+    ColIdx c;
+    if (!col_idx(A, c)) return;
+    const int nl = A.nl, ny = A.ny, nz = A.nz;
+    const double kx = A.symx[SY_GIM * nl + c.l], ky = A.symy[SY_GIM * ny + c.jm];
+    const double hxy = A.rc * (A.symx[SY_HT * nl + c.l] + A.symy[SY_HT * ny + c.jm]);
+    const double dif = A.rc * 0.5 * A.dt * A.alpha, um = A.u_mrf;
+    const double rb = A.rb * A.dt, ra = A.ra * A.dt;
+    const bool is00 = c.l == 0 && c.jm == 0;
+    const int band = A.tbc_top ? ZB_C2_DN : ZB_C2_DD, op = A.tbc_top ? ZO_C2_DN : ZO_C2_DD;
+    const FieldSet<1> ws = {{io.ct}, {nz}};
+    const FieldSet<4> ps = {{io.sp[0], io.sp[1], io.sp[2], io.css}, {0, 0, 0, 0}};
+    z6_stream<1, 4, 6>(c.plane, c.col, nz, ws, ps, [&](int k, const double2 (&win)[1][kWin], const double2 (&p)[4]) __attribute__((always_inline)) {
+        const double2 x = win[0][kBandMax];
+        const double2 t = band_dot(band_row(A, band, k), win[0]);
+        double2 d = make_double2(x.x + hxy * x.x + dif * t.x, x.y + hxy * x.y + dif * t.y);
+        if (is00) d.x += z6_wall_rhs(A, 3, op, dif, k);
+        double2 f = times_i(x, -um * kx);
+        f = csub(f, times_i(p[0], kx));
+        f = csub(f, times_i(p[1], ky));
+        f = cadd(f, p[2]);
+        d = make_double2(d.x + rb * p[3].x + ra * f.x, d.y + rb * p[3].y + ra * f.y);
+        const size_t o = (size_t)k * c.plane + c.col;
+        io.css[o] = f;
+        io.ct[o] = d;
+    });
+}

@@ -211,10 +211,6 @@ int launch_columns(hc_ctx *c, int kind, StepArgs A, const View<R> &V, int nfield
     io.s0 = V.s[0]; io.s1 = V.s[1]; io.s2 = V.s[2]; io.s3 = V.s[3];
     io.lo_l = c->lo_l; io.lo_m = c->lo_m;
     io.remap = c->tune.col_remap;
-    if (A.zorder == 6) {  // the 6th-order kernels: one staging sub-tile and one for the operator products
-        nfields = 2;
-        if (col_tiles(c, 2, sizeof(cplx<R>)) < 2) return set_err(HC_ERR_UNSUPPORTED, "zorder=6: nz=%d too large", c->nz);
-    }
     A.ntile = col_tiles(c, nfields, sizeof(cplx<R>));
     const size_t lds = col_lds(c, nfields, sizeof(cplx<R>));
     const dim3 gfull((c->nl + kColWaves - 1) / kColWaves, c->ny);

@@ -64,7 +64,9 @@ constexpr int kZW = 16;
 // d + kBandMax.  ZB_F2 has nz - 1 rows (the interior faces).
 constexpr int kBandMax = 7;
 constexpr int kBandRow = 2 * kBandMax + 1;
-enum { ZB_C2_DD = 0, ZB_C2_DN, ZB_F2, ZB_P, ZB_COUNT };
+// ZB_G: faces kw = 0..nz-2 from the levels; ZB_D: levels from the interior
+// faces (w levels 0..nz-2, the wall faces being zero).
+enum { ZB_C2_DD = 0, ZB_C2_DN, ZB_F2, ZB_P, ZB_G, ZB_D, ZB_COUNT };
 
 
 // Arguments of the step kernels.  R is the storage and arithmetic type of

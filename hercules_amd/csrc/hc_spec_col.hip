@@ -31,9 +31,7 @@ int launch_spec_columns(hc_ctx *c, int kind, StepArgs A, const View<R> &V)
     io.cff = V.cff; io.cgg = V.cgg; io.chh = V.chh; io.css = V.css;
     for (int q = 0; q < 8; ++q) io.sp[q] = V.sp[q];
     io.lo_l = c->lo_l; io.lo_m = c->lo_m; io.remap = c->tune.col_remap;
-    const int nfields = A.zorder == 6 ? 2 : (kind == 0 ? 4 : 1);
-    if (A.zorder == 6 && col_tiles(c, 2, sizeof(cplx<R>)) < 2)
-        return set_err(HC_ERR_UNSUPPORTED, "zorder=6: nz=%d too large", c->nz);
+    const int nfields = kind == 0 ? 4 : 1;
     A.ntile = col_tiles(c, nfields, sizeof(cplx<R>));
     const size_t lds = col_lds(c, nfields, sizeof(cplx<R>));
     const dim3 gfull((c->nl + kColWaves - 1) / kColWaves, c->ny);

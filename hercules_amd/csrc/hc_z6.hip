@@ -42,30 +42,30 @@ void band_launch(hipStream_t st, const StepArgs &A, const BandArgs &B, int nsys)
 // 6 doubles per level and column for half band 5
 size_t band_off(const StepArgs &A) { return (size_t)6 * A.nz * A.nl * A.ny; }
 
-template <int L>
-void col6_seq(int kind, dim3 g, size_t lds, hipStream_t st, const StepArgs &A, const ColIO<double> &io, int eq)
+dim3 stream_grid(const StepArgs &A, int ny = 1) { return dim3((unsigned)((A.nl * A.ny + 255) / 256), (unsigned)ny); }
+
+void col6_seq(int kind, hipStream_t st, const StepArgs &A, const ColIO<double> &io, int eq)
 {
-    const dim3 b(64 * kColWaves);
     BandArgs B;
     switch (kind) {
     case 0:  // FD substep: u, v and w Helmholtz solves, divergence, pressure, projection and increment
         B.s[0] = band_sys(io.s0, io.s1, A.zscr, 0, 1, 0);
         B.s[1] = band_sys(io.s2, nullptr, A.zscr + band_off(A), 1, 0, 0);
         band_launch(st, A, B, 2);
-        hipLaunchKernelGGL(k_z6_div<L>, g, b, lds, st, A, io, io.s3, A.idt);
+        hipLaunchKernelGGL(k_z6s_div, stream_grid(A), dim3(256), 0, st, A, io.s0, io.s1, io.s2, io.s3, A.idt);
         B.s[0] = band_sys(io.s3, nullptr, A.zscr, 2, 1, 0);
         band_launch(st, A, B, 1);
-        hipLaunchKernelGGL(k_z6_proj<L>, g, b, lds, st, A, io, io.s3, 0);
+        hipLaunchKernelGGL(k_z6s_proj, stream_grid(A), dim3(256), 0, st, A, io.s0, io.s1, io.s2, io.s3, nullptr, 0);
         break;
     case 1:  // temperature
         B.s[0] = band_sys(io.s0, nullptr, A.zscr, 3, 1, 0);
         band_launch(st, A, B, 1);
         break;
     case 2:  // projection onto divergence-free fields (rk_c = 1)
-        hipLaunchKernelGGL(k_z6_div<L>, g, b, lds, st, A, io, io.s3, 1.0);
+        hipLaunchKernelGGL(k_z6s_div, stream_grid(A), dim3(256), 0, st, A, io.s0, io.s1, io.s2, io.s3, 1.0);
         B.s[0] = band_sys(io.s3, nullptr, A.zscr, 2, 1, 0);
         band_launch(st, A, B, 1);
-        hipLaunchKernelGGL(k_z6_proj<L>, g, b, lds, st, A, io, io.s3, 1);
+        hipLaunchKernelGGL(k_z6s_proj, stream_grid(A), dim3(256), 0, st, A, io.s0, io.s1, io.s2, io.s3, nullptr, 1);
         break;
     default: {  // one equation: 0 u, 1 v, 2 w, 3 p, 4 t; wall data for u and t, delnull for p
         const int meq = eq == 0 || eq == 1 ? 0 : (eq == 2 ? 1 : (eq == 3 ? 2 : 3));
@@ -76,66 +76,36 @@ void col6_seq(int kind, dim3 g, size_t lds, hipStream_t st, const StepArgs &A, c
     }
 }
 
-template <int L>
-void spec6_seq(int kind, dim3 g, size_t lds, hipStream_t st, const StepArgs &A, const SpecIO<double> &io)
+void spec6_seq(int kind, hipStream_t st, const StepArgs &A, const SpecIO<double> &io)
 {
-    const dim3 b(64 * kColWaves);
     BandArgs B;
     if (kind == 1) {  // temperature: right-hand side, solve, 2/3 rule
-        hipLaunchKernelGGL(k_spec6_t_rhs<L>, g, b, lds, st, A, io);
+        hipLaunchKernelGGL(k_z6s_rhs_t, stream_grid(A), dim3(256), 0, st, A, io);
         B.s[0] = band_sys(io.ct, nullptr, A.zscr, 3, 1, 1);
         band_launch(st, A, B, 1);
         return;
     }
-    hipLaunchKernelGGL(k_spec6_rhs<L>, g, b, lds, st, A, io);
+    hipLaunchKernelGGL(k_z6s_rhs_uv, stream_grid(A, 2), dim3(256), 0, st, A, io);
+    hipLaunchKernelGGL(k_z6s_rhs_w, stream_grid(A), dim3(256), 0, st, A, io);
     B.s[0] = band_sys(io.cu, io.cv, A.zscr, 0, 1, 0);
     B.s[1] = band_sys(io.cw, nullptr, A.zscr + band_off(A), 1, 0, 0);
     band_launch(st, A, B, 2);
-    ColIO<double> cio;
-    cio.s0 = io.cu; cio.s1 = io.cv; cio.s2 = io.cw; cio.s3 = io.cp;
-    cio.sel = io.sel; cio.lo_l = io.lo_l; cio.lo_m = io.lo_m; cio.remap = io.remap;
-    hipLaunchKernelGGL(k_z6_div<L>, g, b, lds, st, A, cio, A.zspc, A.idt);
+    hipLaunchKernelGGL(k_z6s_div, stream_grid(A), dim3(256), 0, st, A, io.cu, io.cv, io.cw, A.zspc, A.idt);
     B.s[0] = band_sys(A.zspc, nullptr, A.zscr, 2, 1, 0);
     band_launch(st, A, B, 1);
-    hipLaunchKernelGGL(k_z6_proj<L>, g, b, lds, st, A, cio, A.zspc, 2);
+    hipLaunchKernelGGL(k_z6s_proj, stream_grid(A), dim3(256), 0, st, A, io.cu, io.cv, io.cw, A.zspc, io.cp, 2);
 }
 }  // namespace
 
-void launch_col6(int kind, int L, dim3 g, size_t lds, hipStream_t st, const StepArgs &A, const ColIO<double> &io, int eq)
+void launch_col6(int kind, int, dim3, size_t, hipStream_t st, const StepArgs &A, const ColIO<double> &io, int eq)
 {
-#define HC_K(L_) case L_: col6_seq<L_>(kind, g, lds, st, A, io, eq); break;
-    switch (L) { HC_K(2) HC_K(3) HC_K(4) HC_K(5) HC_K(8) HC_K(16) default: break; }
-#undef HC_K
+    col6_seq(kind, st, A, io, eq);
 }
 
-void launch_spec_col6(int kind, int L, dim3 g, size_t lds, hipStream_t st, const StepArgs &A, const SpecIO<double> &io)
+void launch_spec_col6(int kind, int, dim3, size_t, hipStream_t st, const StepArgs &A, const SpecIO<double> &io)
 {
-#define HC_K(L_) case L_: spec6_seq<L_>(kind, g, lds, st, A, io); break;
-    switch (L) { HC_K(2) HC_K(3) HC_K(4) HC_K(5) HC_K(8) HC_K(16) default: break; }
-#undef HC_K
+    spec6_seq(kind, st, A, io);
 }
 
-hipError_t z6_set_lds(int L, int bytes)
-{
-    hipError_t e = hipSuccess;
-#define HC_A1(K_)                                                                                                   \
-    if (e == hipSuccess)                                                                                            \
-        e = hipFuncSetAttribute(reinterpret_cast<const void *>(&K_), hipFuncAttributeMaxDynamicSharedMemorySize, bytes);
-#define HC_A(L_)                   \
-    HC_A1((k_z6_div<L_>))          \
-    HC_A1((k_z6_proj<L_>))         \
-    HC_A1((k_spec6_rhs<L_>))       \
-    HC_A1((k_spec6_t_rhs<L_>))
-    switch (L) {
-    case 2: HC_A(2) break;
-    case 3: HC_A(3) break;
-    case 4: HC_A(4) break;
-    case 5: HC_A(5) break;
-    case 8: HC_A(8) break;
-    case 16: HC_A(16) break;
-    default: return hipErrorInvalidValue;
-    }
-#undef HC_A
-#undef HC_A1
-    return e;
-}
+// the 6th-order column kernels use no dynamic LDS
+hipError_t z6_set_lds(int, int) { return hipSuccess; }

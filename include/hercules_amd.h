@@ -77,9 +77,9 @@ typedef struct hc_params {
      * stencils on the stretched grid, one-sided near the walls with the wall
      * values as data; the pressure equation is the product of the discrete
      * z divergence and gradient, so the projected velocity is divergence-free
-     * to round-off; the implicit solves are exact (Chebyshev iteration
-     * preconditioned by the 2nd-order tridiagonal operator).  zorder = 6
-     * needs precision = HC_FP64, isdamp = 0 and 16 <= nz <= 640, and has no
+     * to round-off; the implicit solves are exact (banded LU, one thread per
+     * horizontal wavenumber).  zorder = 6 needs precision = HC_FP64,
+     * isdamp = 0 and 16 <= nz <= 1024, and has no
      * adjoint (HC_ERR_UNSUPPORTED otherwise); every mode takes the
      * consistent wall rows (wall_rows is not used). */
     int zorder;

@@ -115,7 +115,7 @@ def test_z6_projection_is_divergence_free():
 
 
 @pytest.mark.parametrize("kw", [dict(precision=1), dict(isdamp=1, nzdamp=8, cdamp=0.1),
-                                dict(cds=3, issk=1, precision=1), dict(nz=9), dict(nz=641), dict(zorder=4)])
+                                dict(cds=3, issk=1, precision=1), dict(nz=9), dict(nz=1025), dict(zorder=4)])
 def test_z6_rejects_unsupported(kw):
     base = dict(nx=16, ny=16, nz=33, zorder=6)
     base.update(kw)
