@@ -410,6 +410,9 @@ static bool build_zops(hc_ctx *c)
 {
     const hc_params &P = c->prm;
     const int nz = P.nz;
+    // second derivatives from zorder + 1 data, first derivatives and
+    // interpolation from zorder data
+    const int n2 = P.zorder + 1, n1 = P.zorder;
     std::vector<double> zf, zc;
     mesh_faces(P, zf);
     zc.resize(nz);
@@ -438,18 +441,18 @@ static bool build_zops(hc_ctx *c)
     auto ppos = [&](int d) { return zc[d]; };
     bool ok = true;
     for (int k = 0; k < nz && ok; ++k) {
-        ok = ok && row(ZO_C2_DD, k, nz + 2, k + 1.0, 7, zc[k], 2, false, cpos);
-        ok = ok && row(ZO_C2_DN, k, nz + 2, k + 1.0, 7, zc[k], 2, true, cpos);
-        ok = ok && row(ZO_D, k, nz + 1, k + 0.5, 6, zc[k], 1, false, fpos);
-        ok = ok && row(ZO_IFC, k, nz + 1, k + 0.5, 6, zc[k], 0, false, fpos);
+        ok = ok && row(ZO_C2_DD, k, nz + 2, k + 1.0, n2, zc[k], 2, false, cpos);
+        ok = ok && row(ZO_C2_DN, k, nz + 2, k + 1.0, n2, zc[k], 2, true, cpos);
+        ok = ok && row(ZO_D, k, nz + 1, k + 0.5, n1, zc[k], 1, false, fpos);
+        ok = ok && row(ZO_IFC, k, nz + 1, k + 0.5, n1, zc[k], 0, false, fpos);
         if (k < nz - 1) {  // interior face kw = k, at z_f(k + 1)
-            ok = ok && row(ZO_F2, k, nz + 1, k + 1.0, 7, zf[k + 1], 2, false, fpos);
-            ok = ok && row(ZO_G, k, nz, k + 0.5, 6, zf[k + 1], 1, false, ppos);
-            ok = ok && row(ZO_I_DD, k, nz + 2, k + 1.5, 6, zf[k + 1], 0, false, cpos);
-            ok = ok && row(ZO_I_DN, k, nz + 2, k + 1.5, 6, zf[k + 1], 0, true, cpos);
+            ok = ok && row(ZO_F2, k, nz + 1, k + 1.0, n2, zf[k + 1], 2, false, fpos);
+            ok = ok && row(ZO_G, k, nz, k + 0.5, n1, zf[k + 1], 1, false, ppos);
+            ok = ok && row(ZO_I_DD, k, nz + 2, k + 1.5, n1, zf[k + 1], 0, false, cpos);
+            ok = ok && row(ZO_I_DN, k, nz + 2, k + 1.5, n1, zf[k + 1], 0, true, cpos);
         } else {  // the top wall face (row nz - 1 of the interpolations)
-            ok = ok && row(ZO_I_DD, k, nz + 2, k + 1.5, 6, zf[nz], 0, false, cpos);
-            ok = ok && row(ZO_I_DN, k, nz + 2, k + 1.5, 6, zf[nz], 0, true, cpos);
+            ok = ok && row(ZO_I_DD, k, nz + 2, k + 1.5, n1, zf[nz], 0, false, cpos);
+            ok = ok && row(ZO_I_DN, k, nz + 2, k + 1.5, n1, zf[nz], 0, true, cpos);
         }
     }
     if (!ok) return false;
@@ -458,11 +461,11 @@ static bool build_zops(hc_ctx *c)
     for (int k = 0; k < nz; ++k) {
         std::fill(dense.begin(), dense.end(), 0.0);
         const size_t ed = (size_t)ZO_D * nz + k;
-        for (int j = 0; j < 6; ++j) {
+        for (int j = 0; j < n1; ++j) {
             const int d = c->h_zst[ed] + j;
             if (d == 0 || d == nz) continue;
             const size_t eg = (size_t)ZO_G * nz + (d - 1);
-            for (int i = 0; i < 6; ++i) dense[c->h_zst[eg] + i] += c->h_zwt[ed * kZW + j] * c->h_zwt[eg * kZW + i];
+            for (int i = 0; i < n1; ++i) dense[c->h_zst[eg] + i] += c->h_zwt[ed * kZW + j] * c->h_zwt[eg * kZW + i];
         }
         int first = nz, last = -1;
         for (int q = 0; q < nz; ++q)
@@ -487,7 +490,7 @@ static bool build_zops(hc_ctx *c)
         const int nrow = (op == ZO_F2 || op == ZO_G) ? nz - 1 : nz;
         const int ncol = (op == ZO_F2 || op == ZO_D) ? nz - 1 : nz;
         const int off = (op == ZO_P || op == ZO_G) ? 0 : 1;
-        const int W = op == ZO_P ? kZW : (op == ZO_G || op == ZO_D) ? 6 : 7;
+        const int W = op == ZO_P ? kZW : (op == ZO_G || op == ZO_D) ? n1 : n2;
         for (int k = 0; k < nrow; ++k) {
             const size_t e = (size_t)op * nz + k;
             for (int j = 0; j < W; ++j) {
@@ -593,8 +596,10 @@ StepArgs base_args(hc_ctx *c)
     A.issk = P.issk == 1;
     A.lc = (int)(c->nx / 3.0); A.mc = (int)(c->ny / 3.0);
     A.rgroup = c->tune.rgroup;
-    A.zorder = P.zorder == 6 ? 6 : 2;
+    A.zorder = (P.zorder == 4 || P.zorder == 6) ? 6 : 2;  // 6: the table-driven z operators
     A.zst = c->zst; A.zwt = c->zwt;
+    A.zn1 = P.zorder == 4 ? 4 : 6;
+    A.zn2 = A.zn1 + 1;
     A.zband = c->zband; A.zscr = c->zscr;
     A.zspc = c->zbuf[0] ? reinterpret_cast<double2 *>(c->zbuf[0]) : nullptr;
     return A;
@@ -724,11 +729,13 @@ extern "C" int hc_create(const hc_params *prm, hc_ctx **out)
     if (P.tbc_top != 0 && P.tbc_top != 1) return set_err(HC_ERR_ARG, "tbc_top must be 0 or 1");
     if (P.precision == HC_MIXED && P.isdamp == 1)
         return set_err(HC_ERR_UNSUPPORTED, "precision=1 (mixed) does not cover Rayleigh damping");
-    if (P.zorder != 0 && P.zorder != 2 && P.zorder != 6) return set_err(HC_ERR_ARG, "zorder must be 2 or 6");
-    if (P.zorder == 6) {
-        if (P.precision != HC_FP64) return set_err(HC_ERR_UNSUPPORTED, "zorder=6 needs precision=0 (FP64)");
-        if (P.isdamp == 1) return set_err(HC_ERR_UNSUPPORTED, "zorder=6 does not cover Rayleigh damping");
-        if (P.nz < 16) return set_err(HC_ERR_UNSUPPORTED, "zorder=6 needs nz >= 16");
+    if (P.zorder != 0 && P.zorder != 2 && P.zorder != 4 && P.zorder != 6)
+        return set_err(HC_ERR_ARG, "zorder must be 2, 4 or 6");
+    const bool hiz = P.zorder == 4 || P.zorder == 6;
+    if (hiz) {
+        if (P.precision != HC_FP64) return set_err(HC_ERR_UNSUPPORTED, "zorder=%d needs precision=0 (FP64)", P.zorder);
+        if (P.isdamp == 1) return set_err(HC_ERR_UNSUPPORTED, "zorder=%d does not cover Rayleigh damping", P.zorder);
+        if (P.nz < 16) return set_err(HC_ERR_UNSUPPORTED, "zorder=%d needs nz >= 16", P.zorder);
     }
     if (!fft_len_ok(P.nx) || !fft_len_ok(P.ny)) return set_err(HC_ERR_UNSUPPORTED, "nx, ny must be one of 16,32,64,128,192,256,512");
     {
@@ -785,8 +792,8 @@ extern "C" int hc_create(const hc_params *prm, hc_ctx **out)
         for (int q = 0; q < 4; ++q) c->sp[q] = c->s[q];
         if (!e) e = dalloc(c, (void **)&c->ptmp, N * sizeof(double));
     }
-    if (!e && P.zorder == 6) {
-        if (!build_zops(c)) e = set_err(HC_ERR_ARG, "6th-order z stencils: singular system");
+    if (!e && hiz) {
+        if (!build_zops(c)) e = set_err(HC_ERR_ARG, "order-%d z stencils: singular system", P.zorder);
         if (!e) e = dalloc(c, (void **)&c->zst, c->h_zst.size() * sizeof(int));
         if (!e) e = dalloc(c, (void **)&c->zwt, c->h_zwt.size() * sizeof(double));
         if (!e) e = dalloc(c, (void **)&c->zband, c->h_zband.size() * sizeof(double));
@@ -1650,7 +1657,7 @@ extern "C" int hc_ensemble_destroy(hc_ensemble *e)
 static int adjoint_supported(const hc_ctx *c)
 {
     const hc_params &P = c->prm;
-    if (c->mixed || P.cds == 3 || P.isscalar == 1 || P.isdamp == 1 || P.dp_opt == 2 || P.zorder == 6)
+    if (c->mixed || P.cds == 3 || P.isscalar == 1 || P.isdamp == 1 || P.dp_opt == 2 || P.zorder == 4 || P.zorder == 6)
         return set_err(HC_ERR_UNSUPPORTED,
                        "the adjoint covers FP64, cds=1,2, isscalar=0, isdamp=0, dp_opt=1,3, 2nd-order z");
     return 0;

@@ -116,6 +116,7 @@ struct StepArgsT {
     // 6th-order z operators (zorder = 6, FP64): per operator and target row,
     // the first data index of the stencil (zst) and kZW weights (zwt)
     int zorder;
+    int zn1, zn2;  // stencil widths: first derivatives / interpolation, second derivatives
     const int *zst;
     const double *zwt;
     // 6th-order interpolants of the substep's fields, formed once by
@@ -237,6 +238,8 @@ struct Interp {
     // ---- 6th-order z (zorder = 6): stencils of the ZOp tables over the
     // data of u, v, t (with their wall values), w (with its zero wall
     // values) and p.  Only levels inside 0..nz-1 are read.
+    // width of a stencil (wide: second derivatives); table entries beyond it are zero
+    HC_DEV int zn(bool wide) const { return wide ? d().A.zn2 : d().A.zn1; }
     HC_DEV const double *zw(int op, int row, int &s) const
     {
         const D &q = d();
@@ -289,7 +292,8 @@ struct Interp {
         const double *w = zw(q.A.ochannel ? ZO_I_DN : ZO_I_DD, kw, s);
         Re acc = Re(0.0);
 #pragma unroll
-        for (int j = 0; j < 6; ++j) acc += Re(w[j]) * udat(di, dj, s + j);
+        for (int j = 0; j < 6; ++j)
+            if (j < zn(false)) acc += Re(w[j]) * udat(di, dj, s + j);
         return acc;
     }
     HC_DEV auto vhz6(int di, int dj, int dk) const
@@ -303,7 +307,8 @@ struct Interp {
         const double *w = zw(q.A.ochannel ? ZO_I_DN : ZO_I_DD, kw, s);
         Re acc = Re(0.0);
 #pragma unroll
-        for (int j = 0; j < 6; ++j) acc += Re(w[j]) * vdat(di, dj, s + j);
+        for (int j = 0; j < 6; ++j)
+            if (j < zn(false)) acc += Re(w[j]) * vdat(di, dj, s + j);
         return acc;
     }
     HC_DEV auto thz6(int dk) const
@@ -317,7 +322,8 @@ struct Interp {
         const double *w = zw(q.A.tbc_top ? ZO_I_DN : ZO_I_DD, kw, s);
         Re acc = Re(0.0);
 #pragma unroll
-        for (int j = 0; j < 6; ++j) acc += Re(w[j]) * tdat(s + j);
+        for (int j = 0; j < 6; ++j)
+            if (j < zn(false)) acc += Re(w[j]) * tdat(s + j);
         return acc;
     }
     // u, v at the face of w level kw = -1..nz-1 (both walls included), for
@@ -332,7 +338,8 @@ struct Interp {
         const double *w = zw(q.A.ochannel ? ZO_I_DN : ZO_I_DD, kw, s);
         Re acc = Re(0.0);
 #pragma unroll
-        for (int j = 0; j < 6; ++j) acc += Re(w[j]) * udat(0, 0, s + j);
+        for (int j = 0; j < 6; ++j)
+            if (j < zn(false)) acc += Re(w[j]) * udat(0, 0, s + j);
         return acc;
     }
     HC_DEV auto vf6(int kw) const
@@ -345,7 +352,8 @@ struct Interp {
         const double *w = zw(q.A.ochannel ? ZO_I_DN : ZO_I_DD, kw, s);
         Re acc = Re(0.0);
 #pragma unroll
-        for (int j = 0; j < 6; ++j) acc += Re(w[j]) * vdat(0, 0, s + j);
+        for (int j = 0; j < 6; ++j)
+            if (j < zn(false)) acc += Re(w[j]) * vdat(0, 0, s + j);
         return acc;
     }
     template <int ZO> HC_DEV auto uhzO(int di, int dj, int dk) const
@@ -368,7 +376,8 @@ struct Interp {
         const double *w = zw(ZO_IFC, c, s);
         Re acc = Re(0.0);
 #pragma unroll
-        for (int j = 0; j < 6; ++j) acc += Re(w[j]) * wdat(0, 0, s + j);
+        for (int j = 0; j < 6; ++j)
+            if (j < zn(false)) acc += Re(w[j]) * wdat(0, 0, s + j);
         return acc;
     }
     // d(u w)/dz, d(v w)/dz, d(t w)/dz at this centre level: the face fluxes
@@ -383,6 +392,7 @@ struct Interp {
         Re acc = Re(0.0);
 #pragma unroll 1
         for (int j = 0; j < 6; ++j) {
+            if (j >= zn(false)) break;
             const int dd = s + j;
             if (dd == 0 || dd == q.A.nz) continue;
             const int dk = dd - 1 - q.k;
@@ -399,6 +409,7 @@ struct Interp {
         Re acc = Re(0.0);
 #pragma unroll 1
         for (int j = 0; j < 6; ++j) {
+            if (j >= zn(false)) break;
             const int dd = s + j;
             if (dd == 0 || dd == q.A.nz) continue;
             const int dk = dd - 1 - q.k;
@@ -415,6 +426,7 @@ struct Interp {
         Re acc = Re(0.0);
 #pragma unroll 1
         for (int j = 0; j < 6; ++j) {
+            if (j >= zn(false)) break;
             const int dd = s + j;
             if (dd == 0 || dd == q.A.nz) continue;
             const int dk = dd - 1 - q.k;
@@ -431,7 +443,8 @@ struct Interp {
         const double *w = zw(q.A.ochannel ? ZO_C2_DN : ZO_C2_DD, q.k, s);
         Re acc = Re(0.0);
 #pragma unroll
-        for (int j = 0; j < 7; ++j) acc += Re(w[j]) * udat(0, 0, s + j);
+        for (int j = 0; j < 7; ++j)
+            if (j < zn(true)) acc += Re(w[j]) * udat(0, 0, s + j);
         return acc;
     }
     HC_DEV auto vzz6() const
@@ -442,7 +455,8 @@ struct Interp {
         const double *w = zw(q.A.ochannel ? ZO_C2_DN : ZO_C2_DD, q.k, s);
         Re acc = Re(0.0);
 #pragma unroll
-        for (int j = 0; j < 7; ++j) acc += Re(w[j]) * vdat(0, 0, s + j);
+        for (int j = 0; j < 7; ++j)
+            if (j < zn(true)) acc += Re(w[j]) * vdat(0, 0, s + j);
         return acc;
     }
     HC_DEV auto tzz6() const
@@ -453,7 +467,8 @@ struct Interp {
         const double *w = zw(q.A.tbc_top ? ZO_C2_DN : ZO_C2_DD, q.k, s);
         Re acc = Re(0.0);
 #pragma unroll
-        for (int j = 0; j < 7; ++j) acc += Re(w[j]) * tdat(s + j);
+        for (int j = 0; j < 7; ++j)
+            if (j < zn(true)) acc += Re(w[j]) * tdat(s + j);
         return acc;
     }
     HC_DEV auto wzz6() const
@@ -464,7 +479,8 @@ struct Interp {
         const double *w = zw(ZO_F2, q.k, s);
         Re acc = Re(0.0);
 #pragma unroll
-        for (int j = 0; j < 7; ++j) acc += Re(w[j]) * wdat(0, 0, s + j);
+        for (int j = 0; j < 7; ++j)
+            if (j < zn(true)) acc += Re(w[j]) * wdat(0, 0, s + j);
         return acc;
     }
     // d(w^2)/dz at this face from w interpolated to the centres, and dp/dz
@@ -476,7 +492,8 @@ struct Interp {
         const double *w = zw(ZO_G, q.k, s);
         Re acc = Re(0.0);
 #pragma unroll 1
-        for (int j = 0; j < 6; ++j) acc += Re(w[j]) * sq(whz6(s + j));
+        for (int j = 0; j < 6; ++j)
+            if (j < zn(false)) acc += Re(w[j]) * sq(whz6(s + j));
         return acc;
     }
     HC_DEV auto pz6() const
@@ -487,7 +504,8 @@ struct Interp {
         const double *w = zw(ZO_G, q.k, s);
         Re acc = Re(0.0);
 #pragma unroll
-        for (int j = 0; j < 6; ++j) acc += Re(w[j]) * q.P(0, 0, s + j - q.k);
+        for (int j = 0; j < 6; ++j)
+            if (j < zn(false)) acc += Re(w[j]) * q.P(0, 0, s + j - q.k);
         return acc;
     }
 };
@@ -1270,7 +1288,7 @@ HC_DEV double div_at(const StepArgsT<R> &F, const StepArgs &D, int i, int j, int
         const size_t e = (size_t)ZO_D * D.nz + k;
         const int s0 = D.zst[e];
         dwdz = 0.0;
-        for (int jz = 0; jz < 6; ++jz) {
+        for (int jz = 0; jz < D.zn1; ++jz) {
             const int dd = s0 + jz;
             if (dd > 0 && dd < D.nz) dwdz += D.zwt[e * kZW + jz] * (double)F.w[P.at(0, 0, dd - 1)];
         }

@@ -279,7 +279,7 @@ int rhs_momentum(hc_ctx *c, const StepArgs &A, StepArgsT<R> &B)
     const hc_params &P = c->prm;
     const int rows = rows_for(c->nx);
     const dim3 grow((c->ny + rows - 1) / rows, c->nz);
-    const bool damp = P.isdamp == 1, z6 = P.zorder == 6;
+    const bool damp = P.isdamp == 1, z6 = P.zorder == 4 || P.zorder == 6;
     if constexpr (std::is_same<R, double>::value) {
         if (damp) {
             // rayleigh_damping of u, v, w, p, t at the start of the substep

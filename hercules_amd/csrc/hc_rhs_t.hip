@@ -127,7 +127,7 @@ int rhs_temperature(hc_ctx *c, StepArgsT<R> &B)
     const hc_params &P = c->prm;
     const int rows = rows_for(c->nx);
     const dim3 grow((c->ny + rows - 1) / rows, c->nz);
-    const bool damp = P.isdamp == 1, z6 = P.zorder == 6;
+    const bool damp = P.isdamp == 1, z6 = P.zorder == 4 || P.zorder == 6;
     B.dmask = 16;  // t read damped, the updated velocities as they are
     if constexpr (std::is_same<R, double>::value) {
         if (z6) {

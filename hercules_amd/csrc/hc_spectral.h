@@ -630,7 +630,7 @@ __global__ __launch_bounds__(256) void k_spec_div(StepArgs A, const C *cu, const
         if (A.zorder == 6) {  // ZO_D over the faces, zero at the walls
             const size_t e = (size_t)ZO_D * A.nz + k;
             const int s0 = A.zst[e];
-            for (int jz = 0; jz < 6; ++jz) {
+            for (int jz = 0; jz < A.zn1; ++jz) {
                 const int lv = s0 + jz - 1;
                 if (lv < 0 || lv >= A.nz - 1) continue;
                 const double2 wv = ccast<double2>(cw[q + (size_t)(lv - k) * plane]);

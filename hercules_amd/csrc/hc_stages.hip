@@ -125,7 +125,7 @@ int stage_supported(const hc_ctx *c)
     const hc_params &P = c->prm;
     if (c->mixed) return set_err(HC_ERR_UNSUPPORTED, "the per-stage entry points are FP64 (precision=0)");
     if (P.cds == 3) return set_err(HC_ERR_ARG, "cds=3: the finite-difference stages need cds=1 or 2");
-    if (P.zorder == 6) return set_err(HC_ERR_UNSUPPORTED, "the per-stage entry points use the reference's 2nd-order z");
+    if (P.zorder == 4 || P.zorder == 6) return set_err(HC_ERR_UNSUPPORTED, "the per-stage entry points use the reference's 2nd-order z");
     return 0;
 }
 

@@ -40,7 +40,7 @@ module module_hercules_hip
         integer(c_int) :: precision   ! 0 FP64, 1 mixed FP32 + FP64 low wavenumbers
         integer(c_int) :: tbc_top     ! 0 t = ttop at the top wall, 1 dT/dz = dtdz_top
         real(c_double) :: dtdz_top
-        integer(c_int) :: zorder      ! 2 (or 0) as the reference, 6 sixth-order z operators
+        integer(c_int) :: zorder      ! 2 (or 0) as the reference, 4 or 6: 4th- or 6th-order z operators
     end type
 
     integer(c_int), parameter :: HC_U = 0, HC_V = 1, HC_W = 2, HC_P = 3, HC_T = 4

@@ -87,7 +87,7 @@ class Params:
     # (the reference), 1 dT/dz = dtdz_top (hc_params.tbc_top, dtdz_top)
     tbc_top: int = 0
     dtdz_top: float = 0.0
-    # order of the z operators: 2 (the reference) or 6 (hc_params.zorder)
+    # order of the z operators: 2 (the reference), 4 or 6 (hc_params.zorder)
     zorder: int = 2
 
     def validate(self):

@@ -71,14 +71,15 @@ typedef struct hc_params {
     double dtdz_top;
     /* Order of the wall-normal (z) operators (no namelist counterpart; the
      * reference is 2nd order in z, src/module_cnvdiff.f90:89,121).  0 or 2:
-     * as the reference.  6: every z derivative, z interpolation and implicit
-     * z operator of the step (FD, cds = 1, 2, and spectral, cds = 3) from
-     * 7-point (second derivatives) and 6-point (first derivatives, interpolation) Lagrange
+     * as the reference.  6 (or 4): every z derivative, z interpolation and
+     * implicit z operator of the step (FD, cds = 1, 2, and spectral, cds = 3)
+     * from 7-point (5-point) second-derivative and 6-point (4-point)
+     * first-derivative and interpolation Lagrange
      * stencils on the stretched grid, one-sided near the walls with the wall
      * values as data; the pressure equation is the product of the discrete
      * z divergence and gradient, so the projected velocity is divergence-free
      * to round-off; the implicit solves are exact (banded LU, one thread per
-     * horizontal wavenumber).  zorder = 6 needs precision = HC_FP64,
+     * horizontal wavenumber).  zorder = 4 or 6 needs precision = HC_FP64,
      * isdamp = 0 and 16 <= nz <= 1024, and has no
      * adjoint (HC_ERR_UNSUPPORTED otherwise); every mode takes the
      * consistent wall rows (wall_rows is not used). */

@@ -16,13 +16,13 @@ from tests.torch_reference import TorchChannel  # noqa: E402
 pytestmark = pytest.mark.gpu
 
 
-def _case(kind, nz, **kw):
+def _case(kind, nz, zorder=6, **kw):
     if kind == "closed":
-        prm = cases.closed_channel(nx=16, ny=16, nz=nz, wall_rows=1, zorder=6, **kw)
+        prm = cases.closed_channel(nx=16, ny=16, nz=nz, wall_rows=1, zorder=zorder, **kw)
         st = cases.laminar_noise(prm, seed=5)
         st.pop("t", None)
     else:
-        prm = cases.stratified_open_channel(nx=16, ny=16, nz=nz, wall_rows=1, zorder=6, nu=1.0 / 180.0, **kw)
+        prm = cases.stratified_open_channel(nx=16, ny=16, nz=nz, wall_rows=1, zorder=zorder, nu=1.0 / 180.0, **kw)
         st = cases.loglaw_noise(prm, seed=5)
         if not prm.isscalar:
             st.pop("t", None)
@@ -39,6 +39,10 @@ CASES = [
     ("open", 129, {"cds": 1, "tbc_top": 1, "dtdz_top": 1.0}),
     # nz = 257: five levels per lane in the column kernels
     ("closed", 257, {}),
+    # 4th-order z operators (zorder = 4): 5- and 4-point stencils
+    ("closed", 65, {"zorder": 4}),
+    ("open", 65, {"zorder": 4}),
+    ("open", 33, {"zorder": 4, "cds": 1, "tbc_top": 1, "dtdz_top": 0.5}),
 ]
 
 
@@ -70,12 +74,12 @@ def test_z6_steps_match_torch_reference(kind, nz, kw):
 
 
 @pytest.mark.parametrize("eq", ["u", "v", "w", "p", "t"])
-@pytest.mark.parametrize("nz", [33, 257])
-def test_z6_standalone_solve_matches_dense(eq, nz):
+@pytest.mark.parametrize("nz,zorder", [(33, 6), (257, 6), (65, 4)])
+def test_z6_standalone_solve_matches_dense(eq, nz, zorder):
     """poisson_solver_fft with the 6th-order operators against the dense
     per-mode solve: (I - beta (lap_xy + Z)) x = q (+ wall data of u, t in the
     (0,0) mode), rc (lap_xy + D G) phi = s with phi(0) = 0 in the (0,0) mode."""
-    prm = cases.stratified_open_channel(nx=16, ny=32, nz=nz, wall_rows=1, zorder=6, nu=1.0 / 180.0,
+    prm = cases.stratified_open_channel(nx=16, ny=32, nz=nz, wall_rows=1, zorder=zorder, nu=1.0 / 180.0,
                                         tbc_top=1 if eq == "t" else 0, dtdz_top=0.7)
     rng = np.random.default_rng(9)
     src = rng.standard_normal((prm.nz, prm.ny, prm.nx))
@@ -99,8 +103,9 @@ def test_z6_standalone_solve_matches_dense(eq, nz):
     assert err < (1e-9 if eq == "p" else 1e-11), err
 
 
-def test_z6_projection_is_divergence_free():
-    prm, st = _case("closed", 129)
+@pytest.mark.parametrize("zorder", [6, 4])
+def test_z6_projection_is_divergence_free(zorder):
+    prm, st = _case("closed", 129, zorder=zorder)
     ch = Channel(prm)
     for k, v in st.items():
         ch.set_field(k, v)
@@ -115,7 +120,7 @@ def test_z6_projection_is_divergence_free():
 
 
 @pytest.mark.parametrize("kw", [dict(precision=1), dict(isdamp=1, nzdamp=8, cdamp=0.1),
-                                dict(cds=3, issk=1, precision=1), dict(nz=9), dict(nz=1025), dict(zorder=4)])
+                                dict(cds=3, issk=1, precision=1), dict(nz=9), dict(nz=1025), dict(zorder=3)])
 def test_z6_rejects_unsupported(kw):
     base = dict(nx=16, ny=16, nz=33, zorder=6)
     base.update(kw)
@@ -139,6 +144,8 @@ SPECTRAL_CASES = [
     ("open", 65, {"issk": 1, "tbc_top": 1, "dtdz_top": 0.5}),
     ("open", 129, {"issk": 0}),
     ("open", 257, {"issk": 1, "tbc_top": 1, "dtdz_top": 0.5}),
+    ("open", 65, {"issk": 1, "zorder": 4}),
+    ("closed", 33, {"issk": 0, "zorder": 4}),
 ]
 
 

@@ -38,9 +38,9 @@ class TorchChannel:
         t = lambda a: torch.tensor(a, dtype=torch.float64)
         self.dz, self.dzt, self.dzb, self.lt = t(m["dz"]), t(m["dz_t"]), t(m["dz_b"]), t(m["l_t"])
         self.dx, self.dy = prm.lx / prm.nx, prm.ly / prm.ny
-        self.z6 = getattr(prm, "zorder", 2) == 6
+        self.z6 = getattr(prm, "zorder", 2) in (4, 6)
         if self.z6:
-            self.Z = {k: t(v) for k, v in zops(prm.nz, prm.lz, prm.zstretch, prm.ochannel).items()}
+            self.Z = {k: t(v) for k, v in zops(prm.nz, prm.lz, prm.zstretch, prm.ochannel, prm.zorder).items()}
         self._symbols()
 
     # ---- horizontal stencils (axis 2 = x, axis 1 = y) ---------------------

@@ -34,9 +34,9 @@ class TorchSpectral:
         m = grid.metrics(nz, prm.lz, prm.zstretch, prm.ochannel)
         t = lambda a: torch.tensor(np.asarray(a), dtype=torch.float64)
         self.dz, self.dzt, self.dzb, self.lt, self.lb = (t(m[k]) for k in ("dz", "dz_t", "dz_b", "l_t", "l_b"))
-        self.z6 = getattr(prm, "zorder", 2) == 6
+        self.z6 = getattr(prm, "zorder", 2) in (4, 6)
         if self.z6:
-            self.Z = {k: t(v) for k, v in zops(nz, prm.lz, prm.zstretch, prm.ochannel).items()}
+            self.Z = {k: t(v) for k, v in zops(nz, prm.lz, prm.zstretch, prm.ochannel, prm.zorder).items()}
         mx = np.fft.fftfreq(nx, 1.0 / nx)
         my = np.fft.fftfreq(ny, 1.0 / ny)
         kx = 2 * math.pi / prm.lx * mx

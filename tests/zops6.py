@@ -48,7 +48,10 @@ def window(ndata, center, n):
     return max(0, min(lo, ndata - n))
 
 
-def zops(nz, lz, zstretch, ochannel):
+def zops(nz, lz, zstretch, ochannel, order=6):
+    """Dense operators of the order-`order` z stencils (order 6: 7- and
+    6-point, order 4: 5- and 4-point stencils)."""
+    n2, n1 = order + 1, order
     zf = grid.faces(nz, lz, zstretch, ochannel)
     zc = 0.5 * (zf[1:] + zf[:-1])
     cpos = np.concatenate([[zf[0]], zc, [zf[-1]]])
@@ -64,19 +67,19 @@ def zops(nz, lz, zstretch, ochannel):
         ops[name] = M
 
     for top, ntop in (("DD", False), ("DN", True)):
-        build("C2_" + top, nz, nz + 2, cpos, 7, 2, lambda k: zc[k], lambda k: k + 1.0, ntop)
-        build("I_" + top, nz - 1, nz + 2, cpos, 6, 0, lambda k: zf[k + 1], lambda k: k + 1.5, ntop)
-    build("F2", nz - 1, nz + 1, zf, 7, 2, lambda k: zf[k + 1], lambda k: k + 1.0)
-    build("G", nz - 1, nz, zc, 6, 1, lambda k: zf[k + 1], lambda k: k + 0.5)
-    build("D", nz, nz + 1, zf, 6, 1, lambda k: zc[k], lambda k: k + 0.5)
-    build("IFC", nz, nz + 1, zf, 6, 0, lambda k: zc[k], lambda k: k + 0.5)
+        build("C2_" + top, nz, nz + 2, cpos, n2, 2, lambda k: zc[k], lambda k: k + 1.0, ntop)
+        build("I_" + top, nz - 1, nz + 2, cpos, n1, 0, lambda k: zf[k + 1], lambda k: k + 1.5, ntop)
+    build("F2", nz - 1, nz + 1, zf, n2, 2, lambda k: zf[k + 1], lambda k: k + 1.0)
+    build("G", nz - 1, nz, zc, n1, 1, lambda k: zf[k + 1], lambda k: k + 0.5)
+    build("D", nz, nz + 1, zf, n1, 1, lambda k: zc[k], lambda k: k + 0.5)
+    build("IFC", nz, nz + 1, zf, n1, 0, lambda k: zc[k], lambda k: k + 0.5)
     # interpolation to the top wall face z_f(nz): the wall value itself, or
     # (Neumann top) the extrapolation through the wall gradient
     for top, ntop in (("DD", False), ("DN", True)):
-        lo = window(nz + 2, nz + 0.5, 6)
-        idx = list(range(lo, lo + 6))
+        lo = window(nz + 2, nz + 0.5, n1)
+        idx = list(range(lo, lo + n1))
         row = np.zeros(nz + 2)
-        row[idx] = fd_weights(zf[-1], cpos[idx], 0, [5] if ntop else [])
+        row[idx] = fd_weights(zf[-1], cpos[idx], 0, [n1 - 1] if ntop else [])
         ops["I_%s_top" % top] = row
     ops["P"] = ops["D"][:, 1:-1] @ ops["G"]
     ops["zf"], ops["zc"] = zf, zc
