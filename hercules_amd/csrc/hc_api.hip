@@ -602,6 +602,7 @@ StepArgs base_args(hc_ctx *c)
     A.zn2 = A.zn1 + 1;
     A.zband = c->zband; A.zscr = c->zscr;
     A.zspc = c->zbuf[0] ? reinterpret_cast<double2 *>(c->zbuf[0]) : nullptr;
+    A.spund = c->spund;
     return A;
 }
 
@@ -713,9 +714,6 @@ extern "C" int hc_create(const hc_params *prm, hc_ctx **out)
     if (P.nx % 2 != 0 || P.ny % 2 != 0) return set_err(HC_ERR_ARG, "Please use even numbers for nx and ny! Quit!");
     if (P.nz < 3) return set_err(HC_ERR_ARG, "nz must be at least 3");
     if (P.cds < 1 || P.cds > 3) return set_err(HC_ERR_ARG, "cds must be 1, 2 or 3");
-    if (P.cds == 3 && P.isdamp == 1)
-        return set_err(HC_ERR_UNSUPPORTED, "isdamp=1 with cds=3: rayleigh_damping_spec reduces a complex mean as one "
-                                           "real (src/module_spectral.f90:1372), so the reference result is undefined");
     if (P.dts != 1 && P.dts != 2) return set_err(HC_ERR_ARG, "dts must be 1 (AB2) or 2 (RK3)");
     if (P.dp_opt < 1 || P.dp_opt > 3) return set_err(HC_ERR_ARG, "dp_opt must be 1, 2 or 3");
     if (P.ochannel != 0 && P.ochannel != 1) return set_err(HC_ERR_ARG, "ochannel must be 0 or 1");
@@ -790,6 +788,9 @@ extern "C" int hc_create(const hc_params *prm, hc_ctx **out)
                           &c->sp[4], &c->sp[5], &c->sp[6], &c->sp[7]};
         for (auto pp : sp) if (!e) e = dalloc(c, (void **)pp, hs);
         for (int q = 0; q < 4; ++q) c->sp[q] = c->s[q];
+        // undamped spectra of u, v, w, p, t on the nzdamp + 1 damped levels
+        if (!e && P.isdamp == 1)
+            e = dalloc(c, (void **)&c->spund, 5 * (size_t)(P.nzdamp + 1) * c->nl * c->ny * sizeof(double2));
         if (!e) e = dalloc(c, (void **)&c->ptmp, N * sizeof(double));
     }
     if (!e && hiz) {

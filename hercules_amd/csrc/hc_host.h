@@ -101,6 +101,7 @@ struct hc_ctx {
     std::vector<int> h_zst;
     std::vector<double> h_zwt;
     double *dmp_s, *dmp_m;  // Rayleigh damping: sigz per level, plane means per (field, level)
+    double2 *spund;         // spectral damping: undamped spectra of u, v, w, p, t on levels >= kdamp0
     // spectral path (cds = 3): field and right-hand-side spectra, product
     // spectra (sp[0..3] are s[0..3]), a physical scratch field
     double2 *cu = nullptr, *cv = nullptr, *cw = nullptr, *cp = nullptr, *ct = nullptr;

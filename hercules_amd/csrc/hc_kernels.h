@@ -129,6 +129,10 @@ struct StepArgsT {
     const double *zband;
     double *zscr;
     cplx<double> *zspc;
+    // spectral Rayleigh damping (cds = 3, FP64): the spectra of u, v, w, p, t
+    // on levels kdamp0..nz-1 as they were before the damping, [field][level][l,m]
+    // (null without damping)
+    const cplx<double> *spund;
 };
 using StepArgs = StepArgsT<double>;
 

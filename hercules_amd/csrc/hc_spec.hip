@@ -80,6 +80,66 @@ int spec_init(hc_ctx *c)
     return c->mixed ? spec_init_t<float>(c, view<float>(c)) : spec_init_t<double>(c, view<double>(c));
 }
 
+// rayleigh_damping_spec (src/module_spectral.f90:1345-1391) of five spectra:
+// on the top levels k >= kdamp0 every mode relaxes toward the level's mean
+// over the modes, c <- c - sigz(k) (c - M(k)), M(k) = (1/(nx ny)) sum over
+// all (l, m) of c(l, m, k).  The reference reduces that complex mean with a
+// one-element real MPI_ALLREDUCE (:1372), which leaves its imaginary part
+// unset; here the mean is taken whole.  For the spectrum of a real field
+// (Hermitian, Nyquist modes zero) it is real: the (0, m) modes once and the
+// l > 0 modes of the half spectrum twice, real parts.
+struct SpecDampArgs {
+    double2 *f[5];
+};
+__global__ __launch_bounds__(256) void k_spec_damp_means(SpecDampArgs F, double *dmp_m, int nl, int ny, int nz,
+                                                         int kdamp0, double inv_nxny)
+{
+    __shared__ double red[256];
+    const int k = kdamp0 + blockIdx.x, fid = blockIdx.y;
+    const double2 *plane = F.f[fid] + (size_t)k * nl * ny;
+    double s = 0.0;
+    for (int q = threadIdx.x; q < nl * ny; q += 256) {
+        const int l = q % nl;
+        s += (l == 0 ? 1.0 : 2.0) * plane[q].x;
+    }
+    red[threadIdx.x] = s;
+    __syncthreads();
+    for (int w = 128; w > 0; w >>= 1) {
+        if (threadIdx.x < w) red[threadIdx.x] += red[threadIdx.x + w];
+        __syncthreads();
+    }
+    if (threadIdx.x == 0) dmp_m[fid * nz + k] = red[0] * inv_nxny;
+}
+// The undamped values go to und ([field][level - kdamp0][l, m]) first: the
+// reference forms the z diffusion and the w pressure gradient from the
+// physical fields, which the damping leaves alone.
+__global__ __launch_bounds__(256) void k_spec_damp_apply(SpecDampArgs F, double2 *und, const double *dmp_s,
+                                                         const double *dmp_m, int nl, int ny, int nz, int kdamp0)
+{
+    const size_t plane = (size_t)nl * ny, n = plane * (size_t)(nz - kdamp0);
+    const int fid = blockIdx.y;
+    double2 *f = F.f[fid] + (size_t)kdamp0 * plane;
+    double2 *u = und + (size_t)fid * n;
+    for (size_t q = blockIdx.x * (size_t)blockDim.x + threadIdx.x; q < n; q += (size_t)gridDim.x * blockDim.x) {
+        const int k = kdamp0 + (int)(q / plane);
+        const double sg = dmp_s[k], m = dmp_m[fid * nz + k];
+        const double2 v = f[q];
+        u[q] = v;
+        f[q] = make_double2(v.x - sg * (v.x - m), v.y - sg * v.y);
+    }
+}
+static int spec_damp(hc_ctx *c)
+{
+    SpecDampArgs F = {{c->cu, c->cv, c->cw, c->cp, c->ct}};
+    const int nlev = c->nz - c->kdamp0;
+    hipLaunchKernelGGL(k_spec_damp_means, dim3(nlev, 5), dim3(256), 0, c->stream, F, c->dmp_m, c->nl, c->ny, c->nz,
+                       c->kdamp0, 1.0 / ((double)c->nx * (double)c->ny));
+    hipLaunchKernelGGL(k_spec_damp_apply, dim3(1024, 5), dim3(256), 0, c->stream, F, c->spund, c->dmp_s, c->dmp_m,
+                       c->nl, c->ny, c->nz, c->kdamp0);
+    HIPCHK(hipGetLastError());
+    return 0;
+}
+
 // one substep of time_advancement_sp (src/module_navier_stokes.f90:393-509)
 template <class R>
 static int substep_sp_t(hc_ctx *c, int t_its, const View<R> &V)
@@ -88,6 +148,12 @@ static int substep_sp_t(hc_ctx *c, int t_its, const View<R> &V)
     const hc_params &P = c->prm;
     if (c->spec_dirty)
         if (int e = spec_init_t<R>(c, V)) return e;
+    // the spectra of u, v, w, p, t are damped; the physical fields keep their
+    // values (src/module_navier_stokes.f90:406-414).  Damping is FP64 only
+    // (the mixed variant rejects isdamp = 1 at creation).
+    if constexpr (std::is_same<R, double>::value)
+        if (P.isdamp == 1)
+            if (int e = spec_damp(c)) return e;
     StepArgs A = base_args(c);
     if (P.dts == 2) { A.ra = kRkc1[t_its - 1]; A.rb = kRkc2[t_its - 1]; A.rc = kRkc3[t_its - 1]; }
     else { A.ra = 1.5; A.rb = -0.5; A.rc = 1.0; }

@@ -368,6 +368,23 @@ HC_DEV ColPos spec_pos(const SpecIO<RIO> &io, int ny)
     return col_pos(c, ny);
 }
 
+// Levels k >= kdamp0 of field fid (0 u, 1 v, 2 w, 3 p, 4 t) as they were
+// before the spectral Rayleigh damping of the substep.  The reference damps
+// only the spectra; its z diffusion (src/module_spectral.f90:83, 225, 367,
+// 547) and w pressure gradient (:382) are formed from the physical fields,
+// which keep the undamped values.  No-op without damping.
+template <int L, class CC>
+HC_DEV void spec_undamped(const StepArgs &A, const ColMap &m, size_t plane, size_t col, int fid, CC (&x)[L])
+{
+    if (!A.spund) return;
+    const cplx<double> *u = A.spund + (size_t)fid * (A.nz - A.kdamp0) * plane;
+#pragma unroll
+    for (int r = 0; r < L; ++r) {
+        const int k = m.k0 + r;
+        if (r < m.cnt && k >= A.kdamp0) x[r] = ccast<CC>(u[(size_t)(k - A.kdamp0) * plane + col]);
+    }
+}
+
 // One velocity component of get_momentum_cnvdiff_spec assembled in spectral
 // space (e: 0 u, 1 v, 2 w):
 //   q  = c + dif (-kx^2-ky^2) c + dif Dzz c - rc dt grad_e p + rb dt f_old + ra dt f_new
@@ -404,6 +421,7 @@ HC_DEV void spec_assemble(const StepArgs &A, const SpecIO<real_of<CI>> &io, CI *
         q[r] = mk2<CC>(x[r].x + hxy * x[r].x, x[r].y + hxy * x[r].y);
         f[r] = times_i(x[r], -um * kx);
     }
+    spec_undamped<L>(A, m, plane, col0 + (threadIdx.x >> 6), e, x);
     if (e == 2) add_dzz_w<L>(A, m, x, dif, q);
     else add_dzz<L>(A, m, e, is00, x, dif, q);
     if (batch) col_take<L>(tile + sub, m, x);
@@ -487,7 +505,15 @@ __global__ __launch_bounds__(64 * kColWaves) void k_column_spec(StepArgs A, Spec
     col_load<L>(io.cp, tile, nz, plane, col0, m, cpr);
     spec_assemble<L>(A, io, tile, m, plane, col0, 0, is00, kx, ky, hxy, cpr, duv[0]);
     spec_assemble<L>(A, io, tile, m, plane, col0, 1, is00, kx, ky, hxy, cpr, duv[1]);
-    spec_assemble<L>(A, io, tile, m, plane, col0, 2, is00, kx, ky, hxy, cpr, dw[0]);
+    if (A.spund) {
+        // w pressure gradient from the undamped p; the damped one returns for
+        // the new pressure
+        spec_undamped<L>(A, m, plane, col0 + (threadIdx.x >> 6), 3, cpr);
+        spec_assemble<L>(A, io, tile, m, plane, col0, 2, is00, kx, ky, hxy, cpr, dw[0]);
+        col_load<L>(io.cp, tile, nz, plane, col0, m, cpr);
+    } else {
+        spec_assemble<L>(A, io, tile, m, plane, col0, 2, is00, kx, ky, hxy, cpr, dw[0]);
+    }
     if (kept) {
         RC a[L], b[L], c[L];
         assemble<L>(A, 0, l, jm, is00, m, a, b, c);
@@ -573,6 +599,7 @@ __global__ __launch_bounds__(64 * kColWaves) void k_column_spec_t(StepArgs A, Sp
         d[0][r] = mk2<CC>(x[r].x + hxy * x[r].x, x[r].y + hxy * x[r].y);
         f[r] = times_i(x[r], -um * kx);
     }
+    spec_undamped<L>(A, m, plane, col0 + (threadIdx.x >> 6), 4, x);
     add_dzz<L>(A, m, 3, is00, x, dif, d[0]);
     col_load<L>(io.sp[0], tile, nz, plane, col0, m, x);
 #pragma unroll

@@ -1484,16 +1484,39 @@ static void mean_fields(ho_state *S)
 }
 
 /* ---- A18: time_advancement_sp, src/module_navier_stokes.f90:371-517 ---- */
+/* rayleigh_damping_spec, src/module_spectral.f90:1345-1391: on levels
+ * kend3-nzdamp..kend3 every mode relaxes toward the mean over the modes of
+ * the level.  The reference passes that complex mean to a one-element real
+ * MPI_ALLREDUCE (:1372), which sets only its real part; here (one rank) the
+ * whole complex mean is used, mean_all = mean_local / nprc. */
+static void rayleigh_damping_spec(ho_state *S, cplx *c)
+{
+    const ho_params *P = &S->prm;
+    for (int k = S->kend3 - P->nzdamp; k <= S->kend3; ++k) {
+        cplx mean_local = 0.0;
+        for (int j = S->jstr3; j <= S->jend3; ++j)
+            for (int i = S->istr3; i <= S->iend3; ++i) mean_local += C3(S, c, i - 2, j - 2, k - 2);
+        mean_local = mean_local / P->nx / P->ny;
+        const cplx mean_all = mean_local / 1.0;
+        const double zd = (k - S->kend3 + P->nzdamp) * 1.0 / P->nzdamp;
+        const double sigz = P->cdamp * 0.5 * (1.0 - cos(PI * zd));
+        for (int j = S->jstr3; j <= S->jend3; ++j)
+            for (int i = S->istr3; i <= S->iend3; ++i)
+                C3(S, c, i - 2, j - 2, k - 2) = C3(S, c, i - 2, j - 2, k - 2) - sigz * (C3(S, c, i - 2, j - 2, k - 2) - mean_all);
+    }
+}
+
 static int time_advancement_sp(ho_state *S)
 {
     const ho_params *P = &S->prm;
     int t_its_max = P->dts == 2 ? 3 : 1;
     for (int t_its = 1; t_its <= t_its_max; ++t_its) {
-        if (P->isdamp == 1) {
-            /* rayleigh_damping_spec reduces a complex mean as one real
-             * (src/module_spectral.f90:1372): its imaginary part is never set */
-            snprintf(S->err, sizeof S->err, "isdamp=1 with cds=3 has no defined reference result");
-            return 1;
+        if (P->isdamp == 1) {                                             /* :406-414 */
+            rayleigh_damping_spec(S, S->c_u);
+            rayleigh_damping_spec(S, S->c_v);
+            rayleigh_damping_spec(S, S->c_w);
+            rayleigh_damping_spec(S, S->c_p);
+            rayleigh_damping_spec(S, S->c_t);
         }
         assign_rk(S, t_its);                                              /* :417 */
         momentum_spec(S);                                                 /* :420-421 */
@@ -1812,6 +1835,14 @@ int ho_set_spectrum(ho_state *S, int id, const cplx *src)
     cplx *a = spec_ptr(S, id);
     if (!a) return 1;
     memcpy(a, src, sizeof(cplx) * ncplx(S));
+    return 0;
+}
+/* rayleigh_damping_spec of spectrum id (0 c_u .. 4 c_t) */
+int ho_stage_damping_spec(ho_state *S, int id)
+{
+    cplx *a = spec_ptr(S, id);
+    if (!a || id > 4) return 1;
+    rayleigh_damping_spec(S, a);
     return 0;
 }
 /* get_momentum_cnvdiff_spec / get_temperature_cnvdiff_spec at substep t_its */

@@ -81,6 +81,7 @@ def lib():
         L.ho_get_spectrum.argtypes = [vp, ip, dp]
         L.ho_set_spectrum.argtypes = [vp, ip, dp]
         L.ho_stage_momentum_spec.argtypes = [vp, ip]
+        L.ho_stage_damping_spec.argtypes = [vp, ip]
         L.ho_stage_temperature_spec.argtypes = [vp, ip]
         L.ho_spec_divergence.argtypes = [vp, dp]
         L.ho_get_scalar.restype = C.c_double

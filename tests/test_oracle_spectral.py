@@ -170,8 +170,33 @@ def test_spectral_translation_equivariance():
         assert rel_l2(o2.get(n), a) < 1e-12, n
 
 
-def test_spectral_damping_is_rejected():
-    prm = cases.closed_channel(nx=16, ny=16, nz=9, cds=3, isdamp=1, nzdamp=3, cdamp=0.1)
+def test_spectral_damping_relaxes_toward_the_origin_value():
+    """rayleigh_damping_spec (src/module_spectral.f90:1345-1391) with the whole
+    complex mean over the modes (the reference's real reduction at :1372 drops
+    its imaginary part): c <- c - sigz (c - M), sigz = cdamp (1 - cos(pi zd))/2
+    on levels kend3-nzdamp..kend3.  For the spectrum of a real field M is the
+    mean of the modes, which the inverse transform fixes: M = f(0,0)/(nx ny)
+    (forward transform normalised by nx ny, Nyquist modes zero); the damped
+    spectrum stays that of a real field."""
+    from oracle.oracle import lib
+    prm = cases.closed_channel(nx=16, ny=8, nz=9, cds=3, isdamp=1, nzdamp=3, cdamp=0.3)
     o = _oracle(prm, cases.laminar_noise(prm, seed=1))
-    with pytest.raises(RuntimeError, match="no defined reference result"):
-        o.steps(1)
+    rng = np.random.default_rng(4)
+    f = rng.standard_normal((prm.nz, prm.ny, prm.nx))
+    c = np.fft.fft2(f) / (prm.nx * prm.ny)
+    c[:, prm.ny // 2, :] = 0.0
+    c[:, :, prm.nx // 2] = 0.0
+    fr = np.fft.ifft2(c).real * prm.nx * prm.ny  # the field without its Nyquist modes
+    o.set_spectrum("u", c)
+    assert lib().ho_stage_damping_spec(o.h, 0) == 0
+    got = o.spectrum("u")
+    kd = prm.nz - 1 - prm.nzdamp
+    for k in range(prm.nz):
+        if k < kd:
+            assert np.array_equal(got[k], c[k])
+            continue
+        sig = prm.cdamp * 0.5 * (1.0 - np.cos(np.pi * (k - kd) / prm.nzdamp))
+        m = fr[k, 0, 0] / (prm.nx * prm.ny)
+        assert np.allclose(got[k], c[k] - sig * (c[k] - m), rtol=0, atol=1e-14 * np.abs(c[k]).max())
+        back = np.fft.ifft2(got[k])
+        assert np.abs(back.imag).max() < 1e-14 * np.abs(back.real).max()
