@@ -10,7 +10,7 @@
 // dphidz, dphidz2) are applied to the spectra column by column, with the
 // wall ghosts of update_boundary_uvw / _p / _t carried by the (0,0) mode.
 //
-// Included by hc_core.hip after the column helpers.
+// Included by hc_spec.hip and hc_spec_col.hip after the column helpers.
 #pragma once
 
 // ---------------------------------------------------------------------------

@@ -1,6 +1,6 @@
 """6th-order wall-normal operators (hc_params.zorder = 6) as dense numpy
 matrices — test infrastructure, restating the stencil rule the library
-builds its tables by (build_zops in hercules_amd/csrc/hc_core.hip):
+builds its tables by (build_zops in hercules_amd/csrc/hc_api.hip):
 
 * second derivatives from 7 data, first derivatives and interpolation from
   6 data: n consecutive entries of the data set centred on the target,
