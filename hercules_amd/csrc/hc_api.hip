@@ -124,7 +124,7 @@ __global__ __launch_bounds__(256) void k_div_partial(StepArgsT<R> F, StepArgs D,
             const size_t e = (size_t)ZO_D * D.nz + k;
             const int s0 = D.zst[e];
             dwdz = 0.0;
-            for (int jz = 0; jz < 6; ++jz) {
+            for (int jz = 0; jz < D.zn1; ++jz) {
                 const int dd = s0 + jz;
                 if (dd > 0 && dd < D.nz) dwdz += D.zwt[e * kZW + jz] * (double)F.w[P.at(0, 0, dd - 1)];
             }
@@ -731,7 +731,6 @@ extern "C" int hc_create(const hc_params *prm, hc_ctx **out)
         return set_err(HC_ERR_ARG, "zorder must be 2, 4 or 6");
     const bool hiz = P.zorder == 4 || P.zorder == 6;
     if (hiz) {
-        if (P.precision != HC_FP64) return set_err(HC_ERR_UNSUPPORTED, "zorder=%d needs precision=0 (FP64)", P.zorder);
         if (P.isdamp == 1) return set_err(HC_ERR_UNSUPPORTED, "zorder=%d does not cover Rayleigh damping", P.zorder);
         if (P.nz < 16) return set_err(HC_ERR_UNSUPPORTED, "zorder=%d needs nz >= 16", P.zorder);
     }

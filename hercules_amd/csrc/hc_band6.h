@@ -23,16 +23,20 @@
 #pragma once
 
 
+// CI: the stored complex type of the right-hand sides (double2, or float2 in
+// the mixed variant); the factorisation and the solve run in FP64.
+template <class CI>
 struct BandSys {
-    double2 *f[2];  // right-hand sides (k, j, l), solved in place
+    CI *f[2];       // right-hand sides (k, j, l), solved in place
     double *scr;    // U rows: PB + 1 doubles per level and column, level-major
     int nr;         // right-hand sides sharing the matrix: 1 or 2
     int eq;         // 0 u/v, 1 w, 2 p, 3 t
     int fix;        // apply the (0,0)-mode wall data / delnull row
     int trunc;      // zero the columns outside the 2/3 rule on output
 };
+template <class CI>
 struct BandArgs {
-    BandSys s[2];  // one system per blockIdx.y
+    BandSys<CI> s[2];  // one system per blockIdx.y
 };
 
 // Wall data of the (0,0) mode for row k of the u (eq 0) or t (eq 3)
@@ -63,10 +67,10 @@ HC_DEV double z6_wall_rhs(const StepArgs &A, int eq, int op, double ck, int k)
     return v;
 }
 
-template <int PB>
-__global__ __launch_bounds__(256) void k_z6_band(StepArgs A, BandArgs B)
+template <int PB, class CI>
+__global__ __launch_bounds__(256) void k_z6_band(StepArgs A, BandArgs<CI> B)
 {
-    const BandSys S = B.s[blockIdx.y];
+    const BandSys<CI> S = B.s[blockIdx.y];
     const int nl = A.nl, ny = A.ny, nz = A.nz;
     const int ncol = nl * ny;
     const int col = blockIdx.x * 256 + threadIdx.x;
@@ -125,8 +129,8 @@ __global__ __launch_bounds__(256) void k_z6_band(StepArgs A, BandArgs B)
         a[PB] += cdiag;
         const size_t o = (size_t)k * plane + col;
         double2 r[2];
-        r[0] = S.f[0][o];
-        r[1] = nr > 1 ? S.f[1][o] : make_double2(0.0, 0.0);
+        r[0] = ccast<double2>(S.f[0][o]);
+        r[1] = nr > 1 ? ccast<double2>(S.f[1][o]) : make_double2(0.0, 0.0);
         if (is00) {
             if (pres) {
                 if (k == 0) {  // delnull: b0 phi(0) = 0 (src/module_poisson_solver.f90:558-564)
@@ -154,8 +158,8 @@ __global__ __launch_bounds__(256) void k_z6_band(StepArgs A, BandArgs B)
 #pragma unroll
         for (int c = 1; c <= PB; ++c) srow[(size_t)(c - 1) * plane] = a[PB + c];
         srow[(size_t)PB * plane] = dk;
-        S.f[0][o] = r[0];
-        if (nr > 1) S.f[1][o] = r[1];
+        S.f[0][o] = ccast<CI>(r[0]);
+        if (nr > 1) S.f[1][o] = ccast<CI>(r[1]);
 #pragma unroll
         for (int t = 0; t < PB - 1; ++t) {
             dv[t] = dv[t + 1];
@@ -176,10 +180,11 @@ __global__ __launch_bounds__(256) void k_z6_band(StepArgs A, BandArgs B)
     double2 xs[2][PB];  // x at k+1 .. k+PB
 #pragma unroll
     for (int c = 0; c < PB; ++c) xs[0][c] = xs[1][c] = make_double2(0.0, 0.0);
+    const CI zero = czero_t<CI>();
     if (face) {  // w at the top wall face
         const size_t o = (size_t)(nz - 1) * plane + col;
-        S.f[0][o] = make_double2(0.0, 0.0);
-        if (nr > 1) S.f[1][o] = make_double2(0.0, 0.0);
+        S.f[0][o] = zero;
+        if (nr > 1) S.f[1][o] = zero;
     }
     for (int k = n - 1; k >= 0; --k) {
         const size_t o = (size_t)k * plane + col;
@@ -189,8 +194,8 @@ __global__ __launch_bounds__(256) void k_z6_band(StepArgs A, BandArgs B)
         for (int c = 0; c < PB; ++c) u[c] = srow[(size_t)c * plane];
         const double dk = srow[(size_t)PB * plane];
         double2 r[2];
-        r[0] = S.f[0][o];
-        r[1] = nr > 1 ? S.f[1][o] : make_double2(0.0, 0.0);
+        r[0] = ccast<double2>(S.f[0][o]);
+        r[1] = nr > 1 ? ccast<double2>(S.f[1][o]) : make_double2(0.0, 0.0);
 #pragma unroll
         for (int c = 0; c < PB; ++c) {
             r[0].x -= u[c] * xs[0][c].x;
@@ -207,8 +212,8 @@ __global__ __launch_bounds__(256) void k_z6_band(StepArgs A, BandArgs B)
         }
         xs[0][0] = r[0];
         xs[1][0] = r[1];
-        S.f[0][o] = zero_out ? make_double2(0.0, 0.0) : r[0];
-        if (nr > 1) S.f[1][o] = zero_out ? make_double2(0.0, 0.0) : r[1];
+        S.f[0][o] = zero_out ? zero : ccast<CI>(r[0]);
+        if (nr > 1) S.f[1][o] = zero_out ? zero : ccast<CI>(r[1]);
     }
 }
 
@@ -226,18 +231,26 @@ constexpr int kWin = 2 * kBandMax + 1;
 // Iterations i = 0, 1, ...: window field w enters level i, point fields
 // enter level i - kBandMax, and row k = i - kBandMax (0 <= k < nrows) is
 // formed by row(k, win, pts) with win[kBandMax + t] = level k + t (zero
-// outside 0 .. wlev[f] - 1).
-template <int N>
+// outside 0 .. wlev[f] - 1).  Fields are stored as CI (double2 or float2)
+// and held in FP64.
+template <int N, class CI = double2>
 struct FieldSet {
-    const double2 *p[N > 0 ? N : 1];
+    const CI *p[N > 0 ? N : 1];
     int n[N > 0 ? N : 1];  // levels of a window field
 };
-template <int NW, int NP, int PF, class Row>
-HC_DEV void z6_stream(size_t plane, size_t col, int nrows, const FieldSet<NW> ws, const FieldSet<NP> ps, Row &&row)
+template <int NW, int NP, int PF, class CW, class CP, class Row>
+HC_DEV void z6_stream(size_t plane, size_t col, int nrows, const FieldSet<NW, CW> ws, const FieldSet<NP, CP> ps,
+                      Row &&row)
 {
     const auto &wlev = ws.n;
     const double2 zero = make_double2(0.0, 0.0);
-    double2 win[NW][kWin], fw[NW][PF], fp[PF][NP > 0 ? NP : 1];
+    const CW zw_ = czero_t<CW>();
+    const CP zp_ = czero_t<CP>();
+    // the staging ring holds the stored type: widening to FP64 happens when a
+    // level leaves the ring, so no conversion waits on a load in flight
+    double2 win[NW][kWin];
+    CW fw[NW][PF];
+    CP fp[PF][NP > 0 ? NP : 1];
 #pragma unroll
     for (int f = 0; f < NW; ++f)
 #pragma unroll
@@ -245,10 +258,10 @@ HC_DEV void z6_stream(size_t plane, size_t col, int nrows, const FieldSet<NW> ws
 #pragma unroll
     for (int u = 0; u < PF; ++u) {
 #pragma unroll
-        for (int f = 0; f < NW; ++f) fw[f][u] = u < wlev[f] ? ws.p[f][(size_t)u * plane + col] : zero;
+        for (int f = 0; f < NW; ++f) fw[f][u] = u < wlev[f] ? ws.p[f][(size_t)u * plane + col] : zw_;
         const int lp = u - kBandMax;
 #pragma unroll
-        for (int p = 0; p < NP; ++p) fp[u][p] = (lp >= 0 && lp < nrows) ? ps.p[p][(size_t)lp * plane + col] : zero;
+        for (int p = 0; p < NP; ++p) fp[u][p] = (lp >= 0 && lp < nrows) ? ps.p[p][(size_t)lp * plane + col] : zp_;
     }
     for (int i0 = 0; i0 < nrows + kBandMax; i0 += PF) {
 #pragma unroll
@@ -259,16 +272,16 @@ HC_DEV void z6_stream(size_t plane, size_t col, int nrows, const FieldSet<NW> ws
             for (int f = 0; f < NW; ++f) {
 #pragma unroll
                 for (int t = 0; t < kWin - 1; ++t) win[f][t] = win[f][t + 1];
-                win[f][kWin - 1] = fw[f][u];
+                win[f][kWin - 1] = ccast<double2>(fw[f][u]);
             }
 #pragma unroll
-            for (int p = 0; p < NP; ++p) pts[p] = fp[u][p];
+            for (int p = 0; p < NP; ++p) pts[p] = ccast<double2>(fp[u][p]);
             // the loads of iteration i + PF
             const int li = i + PF, lp = li - kBandMax;
 #pragma unroll
-            for (int f = 0; f < NW; ++f) fw[f][u] = li < wlev[f] ? ws.p[f][(size_t)li * plane + col] : zero;
+            for (int f = 0; f < NW; ++f) fw[f][u] = li < wlev[f] ? ws.p[f][(size_t)li * plane + col] : zw_;
 #pragma unroll
-            for (int p = 0; p < NP; ++p) fp[u][p] = (lp >= 0 && lp < nrows) ? ps.p[p][(size_t)lp * plane + col] : zero;
+            for (int p = 0; p < NP; ++p) fp[u][p] = (lp >= 0 && lp < nrows) ? ps.p[p][(size_t)lp * plane + col] : zp_;
             const int k = i - kBandMax;
             if (k >= 0 && k < nrows) row(k, win, pts);
         }
@@ -308,18 +321,20 @@ HC_DEV bool col_idx(const StepArgs &A, ColIdx &c)
     return true;
 }
 
-// Divergence D_x u + D_y v + D_z w of a column times `scale` into dst
-// (get_p_eqn_src / get_div_spec with ZO_D over the interior faces).
-__global__ __launch_bounds__(256) void k_z6s_div(StepArgs A, const double2 *u, const double2 *v, const double2 *w,
-                                                 double2 *dst, double scale)
+// Divergence D_x u + D_y v + D_z w of a column times `scale` into the FP64
+// spectrum dst (get_p_eqn_src / get_div_spec with ZO_D over the interior
+// faces).
+template <class CI>
+__global__ __launch_bounds__(256) void k_z6s_div(StepArgs A, const CI *u, const CI *v, const CI *w, double2 *dst,
+                                                 double scale)
 {
     ColIdx c;
     if (!col_idx(A, c)) return;
     const int nl = A.nl, ny = A.ny, nz = A.nz;
     const double2 sdx = make_double2(A.symx[SY_DRE * nl + c.l], A.symx[SY_DIM * nl + c.l]);
     const double2 sdy = make_double2(A.symy[SY_DRE * ny + c.jm], A.symy[SY_DIM * ny + c.jm]);
-    const FieldSet<1> ws = {{w}, {nz - 1}};
-    const FieldSet<2> ps = {{u, v}, {0, 0}};
+    const FieldSet<1, CI> ws = {{w}, {nz - 1}};
+    const FieldSet<2, CI> ps = {{u, v}, {0, 0}};
     z6_stream<1, 2, 8>(c.plane, c.col, nz, ws, ps, [&](int k, const double2 (&win)[1][kWin], const double2 (&p)[2]) __attribute__((always_inline)) {
         const double2 dz = band_dot(band_row(A, ZB_D, k), win[0]);
         const double2 d = cadd(cadd(cmul(sdx, p[0]), cmul(sdy, p[1])), dz);
@@ -328,13 +343,14 @@ __global__ __launch_bounds__(256) void k_z6s_div(StepArgs A, const double2 *u, c
 }
 
 // Projection u, v, w -= cdt grad phi (calculate_uvw, ZO_G at the interior
-// faces).  mode 0 (FD substep, cdt = rc dt): phi is then replaced by the
-// pressure increment phi - rc dt nu / 2 (hxy_L + ZO_P) phi
-// (calculate_new_p); mode 1 (projection of a state, cdt = 1); mode 2
-// (spectral substep, cdt = rc dt): the increment is added to c_p in `pinc`
-// and the columns outside the 2/3 rule are zeroed.
-__global__ __launch_bounds__(256) void k_z6s_proj(StepArgs A, double2 *u, double2 *v, double2 *w, double2 *phi,
-                                                  double2 *pinc, int mode)
+// faces), phi an FP64 spectrum.  mode 0 (FD substep, cdt = rc dt): the
+// pressure increment phi - rc dt nu / 2 (hxy_L + ZO_P) phi (calculate_new_p)
+// goes to `pinc`; mode 1 (projection of a state, cdt = 1); mode 2 (spectral
+// substep, cdt = rc dt): the increment is added to c_p in `pinc` and the
+// columns outside the 2/3 rule are zeroed.
+template <class CI>
+__global__ __launch_bounds__(256) void k_z6s_proj(StepArgs A, CI *u, CI *v, CI *w, const double2 *phi, CI *pinc,
+                                                  int mode)
 {
     ColIdx c;
     if (!col_idx(A, c)) return;
@@ -345,9 +361,10 @@ __global__ __launch_bounds__(256) void k_z6s_proj(StepArgs A, double2 *u, double
     const double cdt = mode == 1 ? 1.0 : A.rc * A.dt, cnu = A.rc * 0.5 * A.dt * A.nu;
     const int mw = c.jm < ny / 2 ? c.jm : c.jm - ny;
     const bool kept = mode != 2 || (c.l < A.lc && mw < A.mc && mw > -A.mc);
-    const double2 zero = make_double2(0.0, 0.0);
+    const CI zero = czero_t<CI>();
     const FieldSet<1> ws = {{phi}, {nz}};
-    const FieldSet<4> ps = {{u, v, w, mode == 2 ? pinc : phi}, {0, 0, 0, 0}};
+    // the fourth point field is read only in mode 2 (c_p); u stands in otherwise
+    const FieldSet<4, CI> ps = {{u, v, w, mode == 2 ? pinc : u}, {0, 0, 0, 0}};
     z6_stream<1, 4, 6>(c.plane, c.col, nz, ws, ps, [&](int k, const double2 (&win)[1][kWin], const double2 (&p)[4]) __attribute__((always_inline)) {
         const size_t o = (size_t)k * c.plane + c.col;
         const double2 ph = win[0][kBandMax];
@@ -358,14 +375,14 @@ __global__ __launch_bounds__(256) void k_z6s_proj(StepArgs A, double2 *u, double
             const double2 g = band_dot(band_row(A, ZB_G, k), win[0]);
             ww = make_double2(ww.x - cdt * g.x, ww.y - cdt * g.y);
         }
-        u[o] = kept ? uu : zero;
-        v[o] = kept ? vv : zero;
-        w[o] = kept ? ww : zero;
+        u[o] = kept ? ccast<CI>(uu) : zero;
+        v[o] = kept ? ccast<CI>(vv) : zero;
+        w[o] = kept ? ccast<CI>(ww) : zero;
         if (mode == 1) return;
         const double2 pz = band_dot(band_row(A, ZB_P, k), win[0]);
         const double2 inc = make_double2(ph.x - cnu * (slap * ph.x + pz.x), ph.y - cnu * (slap * ph.y + pz.y));
-        if (mode == 0) phi[o] = inc;
-        else pinc[o] = kept ? cadd(p[3], inc) : zero;
+        if (mode == 0) pinc[o] = ccast<CI>(inc);
+        else pinc[o] = kept ? ccast<CI>(cadd(p[3], inc)) : zero;
     });
 }
 
@@ -377,7 +394,8 @@ __global__ __launch_bounds__(256) void k_z6s_proj(StepArgs A, double2 *u, double
 //   f = -u_mrf i kx c - coe (i kx X + i ky Y) + coe R
 // with X, Y, R the transformed products of the component; f replaces f_old.
 // The (0,0) mode adds the explicit half of the wall data of u.
-__global__ __launch_bounds__(256) void k_z6s_rhs_uv(StepArgs A, SpecIO<double> io)
+template <class CI>
+__global__ __launch_bounds__(256) void k_z6s_rhs_uv(StepArgs A, SpecIO<real_of<CI>> io)
 {
     ColIdx c;
     if (!col_idx(A, c)) return;
@@ -389,9 +407,10 @@ __global__ __launch_bounds__(256) void k_z6s_rhs_uv(StepArgs A, SpecIO<double> i
     const double rb = A.rb * A.dt, ra = A.ra * A.dt, kk = e == 0 ? kx : ky;
     const bool is00 = e == 0 && c.l == 0 && c.jm == 0;
     const int band = A.ochannel ? ZB_C2_DN : ZB_C2_DD, op = A.ochannel ? ZO_C2_DN : ZO_C2_DD;
-    double2 *cs = e == 0 ? io.cu : io.cv, *fo = e == 0 ? io.cff : io.cgg;
-    const FieldSet<1> ws = {{cs}, {nz}};
-    const FieldSet<5> ps = {{io.sp[e == 0 ? 0 : 1], io.sp[e == 0 ? 1 : 2], io.sp[e == 0 ? 3 : 4], io.cp, fo}, {0, 0, 0, 0, 0}};
+    CI *cs = e == 0 ? io.cu : io.cv, *fo = e == 0 ? io.cff : io.cgg;
+    const FieldSet<1, CI> ws = {{cs}, {nz}};
+    const FieldSet<5, CI> ps = {{io.sp[e == 0 ? 0 : 1], io.sp[e == 0 ? 1 : 2], io.sp[e == 0 ? 3 : 4], io.cp, fo},
+                                {0, 0, 0, 0, 0}};
     z6_stream<1, 5, 4>(c.plane, c.col, nz, ws, ps, [&](int k, const double2 (&win)[1][kWin], const double2 (&p)[5]) __attribute__((always_inline)) {
         const double2 x = win[0][kBandMax];
         const double2 t = band_dot(band_row(A, band, k), win[0]);
@@ -404,11 +423,12 @@ __global__ __launch_bounds__(256) void k_z6s_rhs_uv(StepArgs A, SpecIO<double> i
         q = csub(q, times_i(p[3], cdt * kk));
         q = make_double2(q.x + rb * p[4].x + ra * f.x, q.y + rb * p[4].y + ra * f.y);
         const size_t o = (size_t)k * c.plane + c.col;
-        fo[o] = f;
-        cs[o] = q;
+        fo[o] = ccast<CI>(f);
+        cs[o] = ccast<CI>(q);
     });
 }
-__global__ __launch_bounds__(256) void k_z6s_rhs_w(StepArgs A, SpecIO<double> io)
+template <class CI>
+__global__ __launch_bounds__(256) void k_z6s_rhs_w(StepArgs A, SpecIO<real_of<CI>> io)
 {
     ColIdx c;
     if (!col_idx(A, c)) return;
@@ -417,9 +437,9 @@ __global__ __launch_bounds__(256) void k_z6s_rhs_w(StepArgs A, SpecIO<double> io
     const double hxy = A.rc * (A.symx[SY_HU * nl + c.l] + A.symy[SY_HU * ny + c.jm]);
     const double dif = A.rc * 0.5 * A.dt * A.nu, coe = A.coe, um = A.u_mrf, cdt = A.rc * A.dt;
     const double rb = A.rb * A.dt, ra = A.ra * A.dt;
-    const double2 zero = make_double2(0.0, 0.0);
-    const FieldSet<2> ws = {{io.cw, io.cp}, {nz - 1, nz}};
-    const FieldSet<4> ps = {{io.sp[5], io.sp[6], io.sp[7], io.chh}, {0, 0, 0, 0}};
+    const CI zero = czero_t<CI>();
+    const FieldSet<2, CI> ws = {{io.cw, io.cp}, {nz - 1, nz}};
+    const FieldSet<4, CI> ps = {{io.sp[5], io.sp[6], io.sp[7], io.chh}, {0, 0, 0, 0}};
     z6_stream<2, 4, 4>(c.plane, c.col, nz, ws, ps, [&](int k, const double2 (&win)[2][kWin], const double2 (&p)[4]) __attribute__((always_inline)) {
         const size_t o = (size_t)k * c.plane + c.col;
         const double2 x = win[0][kBandMax];
@@ -427,7 +447,7 @@ __global__ __launch_bounds__(256) void k_z6s_rhs_w(StepArgs A, SpecIO<double> io
         f = csub(f, times_i(p[0], coe * kx));
         f = csub(f, times_i(p[1], coe * ky));
         f = make_double2(f.x + coe * p[2].x, f.y + coe * p[2].y);
-        io.chh[o] = f;
+        io.chh[o] = ccast<CI>(f);
         if (k == nz - 1) {  // the top wall face
             io.cw[o] = zero;
             return;
@@ -436,7 +456,7 @@ __global__ __launch_bounds__(256) void k_z6s_rhs_w(StepArgs A, SpecIO<double> io
         const double2 g = band_dot(band_row(A, ZB_G, k), win[1]);
         double2 q = make_double2(x.x + hxy * x.x + dif * t.x - cdt * g.x, x.y + hxy * x.y + dif * t.y - cdt * g.y);
         q = make_double2(q.x + rb * p[3].x + ra * f.x, q.y + rb * p[3].y + ra * f.y);
-        io.cw[o] = q;
+        io.cw[o] = ccast<CI>(q);
     });
 }
 
@@ -444,7 +464,8 @@ __global__ __launch_bounds__(256) void k_z6s_rhs_w(StepArgs A, SpecIO<double> io
 // (get_temperature_cnvdiff_spec with the 6th-order z operator) in place of
 // c_t; the new s replaces the old one in c_ss; the (0,0) mode adds the
 // explicit half of the wall data.
-__global__ __launch_bounds__(256) void k_z6s_rhs_t(StepArgs A, SpecIO<double> io)
+template <class CI>
+__global__ __launch_bounds__(256) void k_z6s_rhs_t(StepArgs A, SpecIO<real_of<CI>> io)
 {
     ColIdx c;
     if (!col_idx(A, c)) return;
@@ -455,8 +476,8 @@ __global__ __launch_bounds__(256) void k_z6s_rhs_t(StepArgs A, SpecIO<double> io
     const double rb = A.rb * A.dt, ra = A.ra * A.dt;
     const bool is00 = c.l == 0 && c.jm == 0;
     const int band = A.tbc_top ? ZB_C2_DN : ZB_C2_DD, op = A.tbc_top ? ZO_C2_DN : ZO_C2_DD;
-    const FieldSet<1> ws = {{io.ct}, {nz}};
-    const FieldSet<4> ps = {{io.sp[0], io.sp[1], io.sp[2], io.css}, {0, 0, 0, 0}};
+    const FieldSet<1, CI> ws = {{io.ct}, {nz}};
+    const FieldSet<4, CI> ps = {{io.sp[0], io.sp[1], io.sp[2], io.css}, {0, 0, 0, 0}};
     z6_stream<1, 4, 6>(c.plane, c.col, nz, ws, ps, [&](int k, const double2 (&win)[1][kWin], const double2 (&p)[4]) __attribute__((always_inline)) {
         const double2 x = win[0][kBandMax];
         const double2 t = band_dot(band_row(A, band, k), win[0]);
@@ -468,7 +489,7 @@ __global__ __launch_bounds__(256) void k_z6s_rhs_t(StepArgs A, SpecIO<double> io
         f = cadd(f, p[2]);
         d = make_double2(d.x + rb * p[3].x + ra * f.x, d.y + rb * p[3].y + ra * f.y);
         const size_t o = (size_t)k * c.plane + c.col;
-        io.css[o] = f;
-        io.ct[o] = d;
+        io.css[o] = ccast<CI>(f);
+        io.ct[o] = ccast<CI>(d);
     });
 }

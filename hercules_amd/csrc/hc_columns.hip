@@ -189,12 +189,6 @@ template <int L, class RIO, class RC>
 static void col_kernel(int kind, dim3 g, size_t lds, hipStream_t st, const StepArgs &A, const ColIO<RIO> &io, int eq)
 {
     const dim3 b(64 * kColWaves);
-    if constexpr (std::is_same<RIO, double>::value && std::is_same<RC, double>::value) {
-        if (A.zorder == 6) {  // 6th-order z operators (FP64)
-            launch_col6(kind, L, g, lds, st, A, io, eq);
-            return;
-        }
-    }
     switch (kind) {
     case 0: hipLaunchKernelGGL((k_column_mom<L, RIO, RC>), g, b, lds, st, A, io); break;
     case 1: hipLaunchKernelGGL((k_column_t<L, RIO, RC>), g, b, lds, st, A, io); break;
@@ -211,6 +205,10 @@ int launch_columns(hc_ctx *c, int kind, StepArgs A, const View<R> &V, int nfield
     io.s0 = V.s[0]; io.s1 = V.s[1]; io.s2 = V.s[2]; io.s3 = V.s[3];
     io.lo_l = c->lo_l; io.lo_m = c->lo_m;
     io.remap = c->tune.col_remap;
+    if (A.zorder == 6) {  // 6th/4th-order z operators: every column in FP64 arithmetic
+        launch_col6<R>(kind, c->stream, A, io, eq);
+        return 0;
+    }
     A.ntile = col_tiles(c, nfields, sizeof(cplx<R>));
     const size_t lds = col_lds(c, nfields, sizeof(cplx<R>));
     const dim3 gfull((c->nl + kColWaves - 1) / kColWaves, c->ny);

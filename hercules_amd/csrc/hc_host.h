@@ -244,6 +244,10 @@ inline StepArgsT<R> rows_args(const StepArgs &A, const View<R> &V)
     B.issk = A.issk; B.lc = A.lc; B.mc = A.mc;
     B.ntile = A.ntile;
     B.rgroup = A.rgroup;
+    // the high-order z tables (FP64 in every variant)
+    B.zorder = A.zorder; B.zn1 = A.zn1; B.zn2 = A.zn2;
+    B.zst = A.zst; B.zwt = A.zwt; B.zband = A.zband; B.zscr = A.zscr; B.zspc = A.zspc;
+    B.spund = A.spund;
     return B;
 }
 template <>
@@ -328,10 +332,13 @@ template <class R> int spec_inverse(hc_ctx *c, int nf, const cplx<R> *const *spe
 int spec_divergence(hc_ctx *c);
 hipError_t spec_set_lds(int L, int bytes);
 
-// 6th-order z operators (hc_z6.hip)
-void zinterp6(hc_ctx *c, const StepArgs &A, StepArgs &B);
-void launch_col6(int kind, int L, dim3 g, size_t lds, hipStream_t st, const StepArgs &A, const ColIO<double> &io,
-                 int eq);
+// 6th- and 4th-order z operators (hc_z6.hip): the interpolant pre-pass of
+// the right-hand side, and the column passes (kind as in launch_columns /
+// launch_spec_columns) over spectra stored as R
+template <class RIO> struct SpecIO;  // hc_spectral.h
+template <class R> void zinterp6(hc_ctx *c, StepArgsT<R> &B);
+template <class R> void launch_col6(int kind, hipStream_t st, const StepArgs &A, const ColIO<R> &io, int eq);
+template <class R> void launch_spec_col6(int kind, hipStream_t st, const StepArgs &A, const SpecIO<R> &io);
 hipError_t z6_set_lds(int L, int bytes);
 
 // Discrete adjoint of one FD substep (hc_adj.hip)

@@ -113,7 +113,7 @@ struct StepArgsT {
     const R *aux;  // a physical field read by k_div_partial<3> (the spectral divergence)
     int ntile;     // column kernels: number of nz x kColWaves sub-tiles in the LDS
     int rgroup;    // RHS tile order: y tiles per group (0: the largest of 16, 8, 4, 1 dividing ny/TY)
-    // 6th-order z operators (zorder = 6, FP64): per operator and target row,
+    // 6th-order z operators (zorder = 6; also 4): per operator and target row,
     // the first data index of the stencil (zst) and kZW weights (zwt)
     int zorder;
     int zn1, zn2;  // stencil widths: first derivatives / interpolation, second derivatives
@@ -123,7 +123,7 @@ struct StepArgsT {
     // k_zinterp6 (null: the accessors evaluate them): u, v at the w face of
     // each level (the top wall face included), w at the centres, t at the
     // interior w faces
-    const double *zfu, *zfv, *zcw, *zft;
+    const R *zfu, *zfv, *zcw, *zft;
     // direct banded solves of the 6th-order systems: band tables of the
     // operators (ZB_*), scratch for the U rows, and a half spectrum of scratch
     const double *zband;

@@ -200,20 +200,22 @@ __global__ __launch_bounds__(rhs2_threads<NX>(), (rhs2_waves<NX, R>())) void k_r
 // a damped substep, Pt with the 6th-order z operators (ZO = 6).  Block:
 // rows_per_block full rows of level k.
 template <int CDS, int NX, class Acc = PtD, int ZO = 2>
-__global__ __launch_bounds__(kRhsThreads) void k_rhs_mom_damped(StepArgs A)
+__global__ __launch_bounds__(kRhsThreads) void k_rhs_mom_damped(StepArgsT<typename Acc::Real> A)
 {
+    using R = typename Acc::Real;
+    using C = cplx<R>;
     constexpr int ROWS = rows_per_block<NX>();
     constexpr int NL = NX / 2;
-    __shared__ double2 sq_[3 * ROWS * NL];
-    double *sqd = reinterpret_cast<double *>(sq_);
+    __shared__ C sq_[3 * ROWS * NL];
+    R *sqd = reinterpret_cast<R *>(sq_);
     const int k = blockIdx.y, j0 = blockIdx.x * ROWS, tid = threadIdx.x;
-    const double dpx = A.dp_opt == 2 ? A.scal[0] : A.dpx;
+    const R dpx = A.dp_opt == 2 ? R(A.scal[0]) : A.dpx;
     for (int p = tid; p < ROWS * NX; p += kRhsThreads) {
         int jj = p / NX, i = p - jj * NX, j = j0 + jj;
-        double qu = 0.0, qv = 0.0, qw = 0.0;
+        R qu = R(0.0), qv = R(0.0), qw = R(0.0);
         if (j < A.ny) {
             Acc q(A, i, j, k);
-            double f, g, h;
+            R f, g, h;
             momentum_point<CDS, ZO>(q, dpx, f, g, h, qu, qv, qw);
             const size_t c = ((size_t)k * A.ny + j) * NX + i;
             A.ff[c] = f;
@@ -226,7 +228,7 @@ __global__ __launch_bounds__(kRhsThreads) void k_rhs_mom_damped(StepArgs A)
     }
     __syncthreads();
     lds_fft<NL, 3 * ROWS, NL, 1, kRhsThreads, -1>(sq_, A.tw_nl, tid);
-    double2 *outs[3] = {A.s0, A.s1, A.s2};
+    C *outs[3] = {A.s0, A.s1, A.s2};
     for (int p = tid; p < 3 * ROWS * NL; p += kRhsThreads) {
         int row = p / NL, l = p - row * NL, fi = row / ROWS, jj = row - fi * ROWS, j = j0 + jj;
         if (j < A.ny) outs[fi][((size_t)k * A.ny + j) * NL + l] = r2c_post<NL>(sq_ + row * NL, l, A.tw_nx, A.norm);
@@ -297,15 +299,16 @@ int rhs_momentum(hc_ctx *c, const StepArgs &A, StepArgsT<R> &B)
     else hipLaunchKernelGGL((k_rhs_mom_damped<2, NX_>), grow, dim3(kRhsThreads), 0, c->stream, B);
             HC_DISPATCH_N(c->nx, HC_RHSD)
 #undef HC_RHSD
-        } else if (z6) {  // 6th-order z operators: the global-memory point kernel
-            B.dmask = 0;
-            zinterp6(c, A, B);
-#define HC_RHSZ(NX_)                                                                                              \
-    if (P.cds == 1) hipLaunchKernelGGL((k_rhs_mom_damped<1, NX_, Pt, 6>), grow, dim3(kRhsThreads), 0, c->stream, B); \
-    else hipLaunchKernelGGL((k_rhs_mom_damped<2, NX_, Pt, 6>), grow, dim3(kRhsThreads), 0, c->stream, B);
-            HC_DISPATCH_N(c->nx, HC_RHSZ)
-#undef HC_RHSZ
         }
+    }
+    if (z6 && !damp) {  // 6th/4th-order z operators: the global-memory point kernel
+        B.dmask = 0;
+        zinterp6<R>(c, B);
+#define HC_RHSZ(NX_)                                                                                                   \
+    if (P.cds == 1) hipLaunchKernelGGL((k_rhs_mom_damped<1, NX_, PtT<R>, 6>), grow, dim3(kRhsThreads), 0, c->stream, B); \
+    else hipLaunchKernelGGL((k_rhs_mom_damped<2, NX_, PtT<R>, 6>), grow, dim3(kRhsThreads), 0, c->stream, B);
+        HC_DISPATCH_N(c->nx, HC_RHSZ)
+#undef HC_RHSZ
     }
     if (!damp && !z6) {
 #define HC_RHS(NX_)                                                                                     \

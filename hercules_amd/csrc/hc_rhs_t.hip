@@ -129,15 +129,15 @@ int rhs_temperature(hc_ctx *c, StepArgsT<R> &B)
     const dim3 grow((c->ny + rows - 1) / rows, c->nz);
     const bool damp = P.isdamp == 1, z6 = P.zorder == 4 || P.zorder == 6;
     B.dmask = 16;  // t read damped, the updated velocities as they are
-    if constexpr (std::is_same<R, double>::value) {
-        if (z6) {
-            B.dmask = 0;
-#define HC_RTZ(NX_)                                                                                                    \
-    if (P.cds == 1) hipLaunchKernelGGL((k_rhs_t<1, NX_, Pt, 6>), grow, dim3(kRhsThreads), 0, c->stream, B);             \
-    else hipLaunchKernelGGL((k_rhs_t<2, NX_, Pt, 6>), grow, dim3(kRhsThreads), 0, c->stream, B);
-            HC_DISPATCH_N(c->nx, HC_RTZ)
+    if (z6) {
+        B.dmask = 0;
+#define HC_RTZ(NX_)                                                                                                  \
+    if (P.cds == 1) hipLaunchKernelGGL((k_rhs_t<1, NX_, PtT<R>, 6>), grow, dim3(kRhsThreads), 0, c->stream, B);       \
+    else hipLaunchKernelGGL((k_rhs_t<2, NX_, PtT<R>, 6>), grow, dim3(kRhsThreads), 0, c->stream, B);
+        HC_DISPATCH_N(c->nx, HC_RTZ)
 #undef HC_RTZ
-        }
+    }
+    if constexpr (std::is_same<R, double>::value) {
         if (damp) {
 #define HC_RT(NX_)                                                                                        \
     if (P.cds == 1) hipLaunchKernelGGL((k_rhs_t<1, NX_, PtD>), grow, dim3(kRhsThreads), 0, c->stream, B); \

@@ -164,8 +164,7 @@ static int substep_sp_t(hc_ctx *c, int t_its, const View<R> &V)
     const int rows = rows_for(c->nx);
     const dim3 grow((c->ny + rows - 1) / rows, c->nz);
     PROF_BEGIN(c, HC_K_RHS);
-    if constexpr (std::is_same<R, double>::value)
-        if (A.zorder == 6) zinterp6(c, A, B);
+    if (A.zorder == 6) zinterp6<R>(c, B);
     if (P.issk == 1) {
         // du/dx, du/dy, dv/dx, dv/dy, dw/dx, dw/dy in y-physical space
         const C *src4[4] = {V.cu, V.cu, V.cv, V.cv};
@@ -177,13 +176,9 @@ static int substep_sp_t(hc_ctx *c, int t_its, const View<R> &V)
         const int mul2[2] = {1, 2};
         if (int e = launch_yio<R>(c, +1, 2, src2, dst2, mul2)) return e;
     }
-#define HC_SM(NX_)                                                                                       \
-    if constexpr (std::is_same<R, double>::value) {                                                      \
-        if (A.zorder == 6) hipLaunchKernelGGL((k_spec_mom<NX_, R, 6>), grow, dim3(rows * NX_), 0, c->stream, B); \
-        else hipLaunchKernelGGL((k_spec_mom<NX_, R>), grow, dim3(rows * NX_), 0, c->stream, B);          \
-    } else {                                                                                             \
-        hipLaunchKernelGGL((k_spec_mom<NX_, R>), grow, dim3(rows * NX_), 0, c->stream, B);               \
-    }
+#define HC_SM(NX_)                                                                                   \
+    if (A.zorder == 6) hipLaunchKernelGGL((k_spec_mom<NX_, R, 6>), grow, dim3(rows * NX_), 0, c->stream, B); \
+    else hipLaunchKernelGGL((k_spec_mom<NX_, R>), grow, dim3(rows * NX_), 0, c->stream, B);
     HC_DISPATCH_N(c->nx, HC_SM)
 #undef HC_SM
     PROF_END(c, HC_K_RHS);
@@ -203,13 +198,9 @@ static int substep_sp_t(hc_ctx *c, int t_its, const View<R> &V)
     PROF_END(c, HC_K_FINISH);
     if (P.isscalar == 1) {
         PROF_BEGIN(c, HC_K_RHS_T);
-#define HC_ST(NX_)                                                                                     \
-    if constexpr (std::is_same<R, double>::value) {                                                    \
-        if (A.zorder == 6) hipLaunchKernelGGL((k_spec_t<NX_, R, 6>), grow, dim3(rows * NX_), 0, c->stream, B); \
-        else hipLaunchKernelGGL((k_spec_t<NX_, R>), grow, dim3(rows * NX_), 0, c->stream, B);          \
-    } else {                                                                                           \
-        hipLaunchKernelGGL((k_spec_t<NX_, R>), grow, dim3(rows * NX_), 0, c->stream, B);               \
-    }
+#define HC_ST(NX_)                                                                                 \
+    if (A.zorder == 6) hipLaunchKernelGGL((k_spec_t<NX_, R, 6>), grow, dim3(rows * NX_), 0, c->stream, B); \
+    else hipLaunchKernelGGL((k_spec_t<NX_, R>), grow, dim3(rows * NX_), 0, c->stream, B);
         HC_DISPATCH_N(c->nx, HC_ST)
 #undef HC_ST
         PROF_END(c, HC_K_RHS_T);

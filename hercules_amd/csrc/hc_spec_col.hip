@@ -4,9 +4,6 @@
 
 #include "hc_spectral.h"
 
-// the 6th-order spectral column kernels (hc_z6.hip)
-void launch_spec_col6(int kind, int L, dim3 g, size_t lds, hipStream_t st, const StepArgs &A, const SpecIO<double> &io);
-
 // Spectral column pass (kind 0: k_column_spec, 1: k_column_spec_t) over
 // the spectra of V; with FP32 storage the low-wavenumber blocks are solved
 // in FP64 first, as launch_columns does.
@@ -14,12 +11,6 @@ template <int L, class RIO, class RC>
 static void spec_col_kernel(int kind, dim3 g, size_t lds, hipStream_t st, const StepArgs &A, const SpecIO<RIO> &io)
 {
     const dim3 b(64 * kColWaves);
-    if constexpr (std::is_same<RIO, double>::value && std::is_same<RC, double>::value) {
-        if (A.zorder == 6) {  // 6th-order z operators (FP64)
-            launch_spec_col6(kind, L, g, lds, st, A, io);
-            return;
-        }
-    }
     if (kind == 0) hipLaunchKernelGGL((k_column_spec<L, RIO, RC>), g, b, lds, st, A, io);
     else hipLaunchKernelGGL((k_column_spec_t<L, RIO, RC>), g, b, lds, st, A, io);
 }
@@ -31,6 +22,10 @@ int launch_spec_columns(hc_ctx *c, int kind, StepArgs A, const View<R> &V)
     io.cff = V.cff; io.cgg = V.cgg; io.chh = V.chh; io.css = V.css;
     for (int q = 0; q < 8; ++q) io.sp[q] = V.sp[q];
     io.lo_l = c->lo_l; io.lo_m = c->lo_m; io.remap = c->tune.col_remap;
+    if (A.zorder == 6) {  // 6th/4th-order z operators: every column in FP64 arithmetic
+        launch_spec_col6<R>(kind, c->stream, A, io);
+        return 0;
+    }
     const int nfields = kind == 0 ? 4 : 1;
     A.ntile = col_tiles(c, nfields, sizeof(cplx<R>));
     const size_t lds = col_lds(c, nfields, sizeof(cplx<R>));

@@ -134,7 +134,7 @@ __global__ __launch_bounds__(1024) void k_spec_mom(StepArgsT<R> A)
         const double *wd = g.zw(ZO_D, k, s0);
         R uwz = R(0.0), vwz = R(0.0), duz = R(0.0), dvz = R(0.0);
 #pragma unroll 1
-        for (int q = 0; q < 6; ++q) {
+        for (int q = 0; q < A.zn1; ++q) {  // faces s0-1 .. s0+zn1-2, all within -1..nz-1
             const int kw = s0 + q - 1;
             const R uf = g.uf6(kw), vf = g.vf6(kw);
             const R wf = (kw < 0 || kw >= A.nz - 1) ? R(0.0) : g.W(0, 0, kw - k);
@@ -157,7 +157,7 @@ __global__ __launch_bounds__(1024) void k_spec_mom(StepArgsT<R> A)
                 const double *wg = g.zw(ZO_G, k, sg);
                 R dwz = R(0.0);
 #pragma unroll 1
-                for (int q = 0; q < 6; ++q) dwz += R(wg[q]) * g.whz6(sg + q);
+                for (int q = 0; q < A.zn1; ++q) dwz += R(wg[q]) * g.whz6(sg + q);
                 rw -= a1 * D(4) + a1v * D(5) + w * dwz;
             }
         }

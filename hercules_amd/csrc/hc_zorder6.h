@@ -12,7 +12,8 @@
 // launches, each one thread per (l, m) column streaming along z:
 // right-hand sides (spectral path), the Helmholtz solves, the divergence,
 // the pressure solve, then the projection and the pressure increment.
-// FP64 only.
+// Arithmetic is FP64; the spectra are stored in FP64 or (mixed variant)
+// FP32, the pressure correction phi always in FP64.
 #pragma once
 
 #include "hc_band6.h"
@@ -20,16 +21,18 @@
 // The 6th-order interpolants of the substep's fields, once per point, for
 // the right-hand-side kernels: u, v at the w face of level k (the top wall
 // face included), w at the centre of level k, t at the interior w face k.
-// A carries no interpolant buffers, so the accessor evaluates them.
-__global__ __launch_bounds__(256) void k_zinterp6(StepArgs A, double *fu, double *fv, double *cw, double *ft)
+// A carries no interpolant buffers, so the accessor evaluates them (in the
+// stored type R).
+template <class R>
+__global__ __launch_bounds__(256) void k_zinterp6(StepArgsT<R> A, R *fu, R *fv, R *cw, R *ft)
 {
     const size_t n = (size_t)A.nx * A.ny * A.nz, plane = (size_t)A.nx * A.ny;
     for (size_t q = blockIdx.x * (size_t)blockDim.x + threadIdx.x; q < n; q += (size_t)gridDim.x * blockDim.x) {
         const int k = (int)(q / plane), j = (int)((q - (size_t)k * plane) / A.nx), i = (int)(q % A.nx);
-        const Pt g(A, i, j, k);
+        const PtT<R> g(A, i, j, k);
         fu[q] = g.uf6(k);
         fv[q] = g.vf6(k);
         cw[q] = g.whz6(k);
-        if (ft) ft[q] = k < A.nz - 1 ? g.thz6(0) : 0.0;
+        if (ft) ft[q] = k < A.nz - 1 ? g.thz6(0) : R(0.0);
     }
 }

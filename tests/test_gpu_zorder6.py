@@ -119,8 +119,8 @@ def test_z6_projection_is_divergence_free(zorder):
     ch.close()
 
 
-@pytest.mark.parametrize("kw", [dict(precision=1), dict(isdamp=1, nzdamp=8, cdamp=0.1),
-                                dict(cds=3, issk=1, precision=1), dict(nz=9), dict(nz=1025), dict(zorder=3)])
+@pytest.mark.parametrize("kw", [dict(isdamp=1, nzdamp=8, cdamp=0.1), dict(cds=3, isdamp=1, nzdamp=8, cdamp=0.1),
+                                dict(nz=9), dict(nz=1025), dict(zorder=3)])
 def test_z6_rejects_unsupported(kw):
     base = dict(nx=16, ny=16, nz=33, zorder=6)
     base.update(kw)
