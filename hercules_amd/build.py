@@ -33,7 +33,7 @@ def _run(cmd, cwd=None):
 # Translation units of the library: each kernel family compiles on its own,
 # in parallel, and the objects are linked into one shared library.
 SOURCES = ["hc_api.hip", "hc_xfer.hip", "hc_rhs.hip", "hc_rhs_t.hip", "hc_columns.hip", "hc_spec.hip",
-           "hc_spec_col.hip", "hc_z6.hip", "hc_adj.hip", "hc_screen.hip", "hc_stages.hip"]
+           "hc_spec_col.hip", "hc_z6.hip", "hc_adj.hip", "hc_screen.hip", "hc_stages.hip", "hc_stages_sp.hip"]
 CSRC = os.path.join(PKG, "csrc")
 OBJ = os.path.join(PKG, "_obj")
 
@@ -85,6 +85,8 @@ FDIR = os.path.join(PKG, "fortran")
 FEXE = os.path.join(FDIR, "hercules_hip")
 STAGE_SRC = os.path.join(ROOT, "tests", "fortran", "stage_loop.f90")
 STAGE_EXE = os.path.join(ROOT, "tests", "fortran", "stage_loop")
+STAGE_SP_SRC = os.path.join(ROOT, "tests", "fortran", "stage_loop_sp.f90")
+STAGE_SP_EXE = os.path.join(ROOT, "tests", "fortran", "stage_loop_sp")
 
 
 def build_fortran(verbose=False):
@@ -99,12 +101,13 @@ def build_fortran(verbose=False):
           "-L" + PKG, "-lhercules_amd", "-Wl,-rpath," + PKG, "-Wl,-rpath,$ORIGIN/.."])
     if verbose:
         print("built", FEXE)
-    # the tests' host-loop caller of the per-stage subroutines
-    if os.path.exists(STAGE_SRC):
-        _run([FC, "-O2", STAGE_SRC, obj, "-module-dir", out, "-I", out, "-o", STAGE_EXE,
-              "-L" + PKG, "-lhercules_amd", "-Wl,-rpath," + PKG])
-        if verbose:
-            print("built", STAGE_EXE)
+    # the tests' host-loop callers of the per-stage subroutines (FD and spectral)
+    for src, exe in ((STAGE_SRC, STAGE_EXE), (STAGE_SP_SRC, STAGE_SP_EXE)):
+        if os.path.exists(src):
+            _run([FC, "-O2", src, obj, "-module-dir", out, "-I", out, "-o", exe,
+                  "-L" + PKG, "-lhercules_amd", "-Wl,-rpath," + PKG])
+            if verbose:
+                print("built", exe)
     return FEXE
 
 

@@ -34,7 +34,10 @@ ABI_SYMBOLS = ("hc_last_error", "hc_create", "hc_destroy", "hc_set_field", "hc_g
                "hc_wall_shear_gradient", "hc_autotune", "hc_set_tuning", "hc_mean_fields", "hc_get_mean_field",
                "hc_get_mesh", "hc_screen_cfl", "hc_get_column", "hc_calculate_ubar_of", "hc_update_ri",
                "hc_screen_div_error_ref", "hc_screen_substeps", "hc_get_screen", "hc_get_momentum_cnvdiff",
-               "hc_get_temperature_cnvdiff", "hc_get_p_eqn_src", "hc_calculate_uvw", "hc_calculate_new_p")
+               "hc_get_temperature_cnvdiff", "hc_get_p_eqn_src", "hc_calculate_uvw", "hc_calculate_new_p",
+               "hc_get_momentum_cnvdiff_spec", "hc_get_temperature_cnvdiff_spec", "hc_poisson_solver_fft_spec",
+               "hc_get_div_spec", "hc_calculate_uvw_spec", "hc_calculate_new_p_spec", "hc_real_2_cmplx",
+               "hc_cmplx_2_real", "hc_spectral_truncation", "hc_rayleigh_damping_spec")
 # statistics of calculate_mean_fields, in the order of output_mean_fields
 MEANS = ("u", "v", "w", "t", "uu", "vv", "ww", "uw", "vw", "tt", "tw")
 KERNELS = {"rhs": 0, "yfft_fwd": 1, "column": 2, "yfft_inv": 3, "finish": 4, "rhs_t": 5,
@@ -109,6 +112,16 @@ def load_library(path: str = LIB_PATH):
     L.hc_get_p_eqn_src.argtypes = [vp] + [vp] * 4 + [ip]
     L.hc_calculate_uvw.argtypes = [vp, d] + [vp] * 4 + [ip]
     L.hc_calculate_new_p.argtypes = [vp, d, vp, vp, ip]
+    L.hc_get_momentum_cnvdiff_spec.argtypes = [vp, d, d, d] + [vp] * 15 + [ip]
+    L.hc_get_temperature_cnvdiff_spec.argtypes = [vp, d, d, d] + [vp] * 7 + [ip]
+    L.hc_poisson_solver_fft_spec.argtypes = [vp, ip, d, vp, vp]
+    L.hc_get_div_spec.argtypes = [vp] + [vp] * 4 + [ip]
+    L.hc_calculate_uvw_spec.argtypes = [vp, d] + [vp] * 5 + [ip]
+    L.hc_calculate_new_p_spec.argtypes = [vp, d, vp, vp, vp, ip]
+    L.hc_real_2_cmplx.argtypes = [vp, vp, vp, ip]
+    L.hc_cmplx_2_real.argtypes = [vp, vp, vp, ip]
+    L.hc_spectral_truncation.argtypes = [vp, vp]
+    L.hc_rayleigh_damping_spec.argtypes = [vp, vp]
     _lib = L
     return L
 
@@ -360,6 +373,88 @@ class Channel:
         pp = self._arr(p, layout).copy()
         self._check(self.L.hc_calculate_new_p(self.h, float(rk_c), _ptr(ph), _ptr(pp), layout))
         return pp
+
+    # ---- spectral per-stage subroutines: complex arrays (nz, ny, nx), i.e.
+    # the reference's c(nx, ny, nz) work arrays
+    def _spec(self, c):
+        a = np.ascontiguousarray(c, dtype=np.complex128)
+        if a.shape != self.shape:
+            raise ValueError(f"spectrum shape {a.shape} != {self.shape}")
+        return a
+
+    def get_momentum_cnvdiff_spec(self, rk, u, v, w, p, t, c_u, c_v, c_w, c_p, c_ff, c_gg, c_hh,
+                                  layout=LAYOUT_INTERIOR):
+        """get_momentum_cnvdiff_spec; returns c_ff, c_gg, c_hh (new), c_qu, c_qv, c_qw."""
+        ins = [self._arr(x, layout) for x in (u, v, w, p)]
+        tt = None if t is None else self._arr(t, layout)
+        cs = [self._spec(x) for x in (c_u, c_v, c_w, c_p)]
+        fs = [self._spec(x).copy() for x in (c_ff, c_gg, c_hh)]
+        qs = [np.zeros(self.shape, dtype=np.complex128) for _ in range(3)]
+        self._check(self.L.hc_get_momentum_cnvdiff_spec(
+            self.h, float(rk[0]), float(rk[1]), float(rk[2]), *(_ptr(a) for a in ins),
+            None if tt is None else _ptr(tt), *(_ptr(a) for a in cs), *(_ptr(a) for a in fs),
+            *(_ptr(a) for a in qs), layout))
+        return (*fs, *qs)
+
+    def get_temperature_cnvdiff_spec(self, rk, u, v, w, t, c_t, c_ss, layout=LAYOUT_INTERIOR):
+        """get_temperature_cnvdiff_spec; returns c_ss (new), c_qt."""
+        ins = [self._arr(x, layout) for x in (u, v, w, t)]
+        ct = self._spec(c_t)
+        ss = self._spec(c_ss).copy()
+        qt = np.zeros(self.shape, dtype=np.complex128)
+        self._check(self.L.hc_get_temperature_cnvdiff_spec(self.h, float(rk[0]), float(rk[1]), float(rk[2]),
+                                                           *(_ptr(a) for a in ins), _ptr(ct), _ptr(ss), _ptr(qt),
+                                                           layout))
+        return ss, qt
+
+    def poisson_solver_fft_spec(self, equation, rk_c, c_src):
+        src = self._spec(c_src)
+        out = np.zeros(self.shape, dtype=np.complex128)
+        self._check(self.L.hc_poisson_solver_fft_spec(self.h, EQUATIONS[equation], float(rk_c), _ptr(src), _ptr(out)))
+        return out
+
+    def get_div_spec(self, c_u, c_v, w, layout=LAYOUT_INTERIOR):
+        cu, cv = self._spec(c_u), self._spec(c_v)
+        ww = self._arr(w, layout)
+        out = np.zeros(self.shape, dtype=np.complex128)
+        self._check(self.L.hc_get_div_spec(self.h, _ptr(cu), _ptr(cv), _ptr(ww), _ptr(out), layout))
+        return out
+
+    def calculate_uvw_spec(self, rk_c, phi, c_phi, c_u, c_v, c_w, layout=LAYOUT_INTERIOR):
+        """calculate_uvw_spec: returns updated copies of c_u, c_v, c_w."""
+        ph, cph = self._arr(phi, layout), self._spec(c_phi)
+        uvw = [self._spec(x).copy() for x in (c_u, c_v, c_w)]
+        self._check(self.L.hc_calculate_uvw_spec(self.h, float(rk_c), _ptr(ph), _ptr(cph), *(_ptr(a) for a in uvw),
+                                                 layout))
+        return tuple(uvw)
+
+    def calculate_new_p_spec(self, rk_c, phi, c_phi, c_p, layout=LAYOUT_INTERIOR):
+        ph, cph = self._arr(phi, layout), self._spec(c_phi)
+        cp = self._spec(c_p).copy()
+        self._check(self.L.hc_calculate_new_p_spec(self.h, float(rk_c), _ptr(ph), _ptr(cph), _ptr(cp), layout))
+        return cp
+
+    def real_2_cmplx(self, r, layout=LAYOUT_INTERIOR):
+        rr = self._arr(r, layout)
+        out = np.zeros(self.shape, dtype=np.complex128)
+        self._check(self.L.hc_real_2_cmplx(self.h, _ptr(rr), _ptr(out), layout))
+        return out
+
+    def cmplx_2_real(self, c):
+        cc = self._spec(c)
+        out = np.zeros(self.shape)
+        self._check(self.L.hc_cmplx_2_real(self.h, _ptr(cc), _ptr(out), LAYOUT_INTERIOR))
+        return out
+
+    def spectral_truncation(self, c):
+        cc = self._spec(c).copy()
+        self._check(self.L.hc_spectral_truncation(self.h, _ptr(cc)))
+        return cc
+
+    def rayleigh_damping_spec(self, c):
+        cc = self._spec(c).copy()
+        self._check(self.L.hc_rayleigh_damping_spec(self.h, _ptr(cc)))
+        return cc
 
     def wall_shear(self):
         """(tau_bottom, tau_top): plane-averaged nu du/dz at the walls."""

@@ -330,6 +330,12 @@ template <class R> int spec_inverse(hc_ctx *c, int nf, const cplx<R> *const *spe
 // screen_div_error_spec: the spectral divergence of (c_u, c_v, w) in
 // physical space, into the scratch field ptmp of the stored precision
 int spec_divergence(hc_ctx *c);
+// FP64 spectral helpers of the per-stage entry points (hc_spec.hip): the
+// products of the momentum (which = 0) or temperature (1) right-hand side and
+// their spectra in sp[], and real_2_cmplx of one field
+int spec_products(hc_ctx *c, const StepArgs &A, int which);
+int spec_forward64(hc_ctx *c, const double *phys, double2 *spec);
+int spec_damp_one(hc_ctx *c, double2 *f);
 hipError_t spec_set_lds(int L, int bytes);
 
 // 6th- and 4th-order z operators (hc_z6.hip): the interpolant pre-pass of

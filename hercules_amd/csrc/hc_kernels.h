@@ -112,6 +112,7 @@ struct StepArgsT {
     int issk, lc, mc;
     const R *aux;  // a physical field read by k_div_partial<3> (the spectral divergence)
     int ntile;     // column kernels: number of nz x kColWaves sub-tiles in the LDS
+    int stage;     // spectral column kernels: 1 = assemble the right-hand sides only (per-stage entries)
     int rgroup;    // RHS tile order: y tiles per group (0: the largest of 16, 8, 4, 1 dividing ny/TY)
     // 6th-order z operators (zorder = 6; also 4): per operator and target row,
     // the first data index of the stencil (zst) and kZW weights (zwt)

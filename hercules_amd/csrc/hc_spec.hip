@@ -140,29 +140,37 @@ static int spec_damp(hc_ctx *c)
     return 0;
 }
 
-// one substep of time_advancement_sp (src/module_navier_stokes.f90:393-509)
+// The physical-space products of a substep and their spectra
+// (get_momentum_cnvdiff_spec, src/module_spectral.f90:35-504, which = 0:
+// the y-inverted derivative spectra of the skew-symmetric form, k_spec_mom,
+// the y FFT of the eight products; get_temperature_cnvdiff_spec, :509-595,
+// which = 1: k_spec_t and the y FFT of its three products).
 template <class R>
-static int substep_sp_t(hc_ctx *c, int t_its, const View<R> &V)
+static int spec_products_t(hc_ctx *c, const StepArgs &A, const View<R> &V, int which)
 {
     using C = cplx<R>;
     const hc_params &P = c->prm;
-    if (c->spec_dirty)
-        if (int e = spec_init_t<R>(c, V)) return e;
-    // the spectra of u, v, w, p, t are damped; the physical fields keep their
-    // values (src/module_navier_stokes.f90:406-414).  Damping is FP64 only
-    // (the mixed variant rejects isdamp = 1 at creation).
-    if constexpr (std::is_same<R, double>::value)
-        if (P.isdamp == 1)
-            if (int e = spec_damp(c)) return e;
-    StepArgs A = base_args(c);
-    if (P.dts == 2) { A.ra = kRkc1[t_its - 1]; A.rb = kRkc2[t_its - 1]; A.rc = kRkc3[t_its - 1]; }
-    else { A.ra = 1.5; A.rb = -0.5; A.rc = 1.0; }
     StepArgsT<R> B = rows_args<R>(A, V);
     for (int q = 0; q < 8; ++q) B.sp[q] = V.sp[q];
     for (int q = 0; q < 6; ++q) B.sdv[q] = V.sp[q];
-    if (P.dp_opt == 2) dpx_update_launch(c);
     const int rows = rows_for(c->nx);
     const dim3 grow((c->ny + rows - 1) / rows, c->nz);
+    if (which == 1) {
+        PROF_BEGIN(c, HC_K_RHS_T);
+        // t is unchanged since the substep's start: its face interpolant of
+        // the momentum pre-pass still holds (the other buffers do not)
+        if (A.zorder == 6) B.zft = reinterpret_cast<const R *>(c->zbuf[3]);
+#define HC_ST(NX_)                                                                                 \
+    if (A.zorder == 6) hipLaunchKernelGGL((k_spec_t<NX_, R, 6>), grow, dim3(rows * NX_), 0, c->stream, B); \
+    else hipLaunchKernelGGL((k_spec_t<NX_, R>), grow, dim3(rows * NX_), 0, c->stream, B);
+        HC_DISPATCH_N(c->nx, HC_ST)
+#undef HC_ST
+        PROF_END(c, HC_K_RHS_T);
+        PROF_BEGIN(c, HC_K_YFFT_T);
+        if (int e = launch_yfft(c, -1, 3, V.sp[0], V.sp[1], V.sp[2], nullptr)) return e;
+        PROF_END(c, HC_K_YFFT_T);
+        return 0;
+    }
     PROF_BEGIN(c, HC_K_RHS);
     if (A.zorder == 6) zinterp6<R>(c, B);
     if (P.issk == 1) {
@@ -186,6 +194,47 @@ static int substep_sp_t(hc_ctx *c, int t_its, const View<R> &V)
     if (int e = launch_yfft(c, -1, 4, V.sp[0], V.sp[1], V.sp[2], V.sp[3])) return e;
     if (int e = launch_yfft(c, -1, 4, V.sp[4], V.sp[5], V.sp[6], V.sp[7])) return e;
     PROF_END(c, HC_K_YFFT_FWD);
+    return 0;
+}
+int spec_products(hc_ctx *c, const StepArgs &A, int which)
+{
+    return spec_products_t<double>(c, A, view<double>(c), which);
+}
+int spec_forward64(hc_ctx *c, const double *phys, double2 *spec) { return spec_forward<double>(c, phys, spec); }
+
+// rayleigh_damping_spec of one half spectrum (FP64, isdamp = 1); the
+// undamped levels go to the side buffer's first field
+int spec_damp_one(hc_ctx *c, double2 *f)
+{
+    SpecDampArgs F = {{f, nullptr, nullptr, nullptr, nullptr}};
+    const int nlev = c->nz - c->kdamp0;
+    hipLaunchKernelGGL(k_spec_damp_means, dim3(nlev, 1), dim3(256), 0, c->stream, F, c->dmp_m, c->nl, c->ny, c->nz,
+                       c->kdamp0, 1.0 / ((double)c->nx * (double)c->ny));
+    hipLaunchKernelGGL(k_spec_damp_apply, dim3(1024, 1), dim3(256), 0, c->stream, F, c->spund, c->dmp_s, c->dmp_m,
+                       c->nl, c->ny, c->nz, c->kdamp0);
+    HIPCHK(hipGetLastError());
+    return 0;
+}
+
+// one substep of time_advancement_sp (src/module_navier_stokes.f90:393-509)
+template <class R>
+static int substep_sp_t(hc_ctx *c, int t_its, const View<R> &V)
+{
+    using C = cplx<R>;
+    const hc_params &P = c->prm;
+    if (c->spec_dirty)
+        if (int e = spec_init_t<R>(c, V)) return e;
+    // the spectra of u, v, w, p, t are damped; the physical fields keep their
+    // values (src/module_navier_stokes.f90:406-414).  Damping is FP64 only
+    // (the mixed variant rejects isdamp = 1 at creation).
+    if constexpr (std::is_same<R, double>::value)
+        if (P.isdamp == 1)
+            if (int e = spec_damp(c)) return e;
+    StepArgs A = base_args(c);
+    if (P.dts == 2) { A.ra = kRkc1[t_its - 1]; A.rb = kRkc2[t_its - 1]; A.rc = kRkc3[t_its - 1]; }
+    else { A.ra = 1.5; A.rb = -0.5; A.rc = 1.0; }
+    if (P.dp_opt == 2) dpx_update_launch(c);
+    if (int e = spec_products_t<R>(c, A, V, 0)) return e;
     PROF_BEGIN(c, HC_K_COLUMN);
     if (int e = launch_spec_columns<R>(c, 0, A, V)) return e;
     PROF_END(c, HC_K_COLUMN);
@@ -197,16 +246,7 @@ static int substep_sp_t(hc_ctx *c, int t_its, const View<R> &V)
     }
     PROF_END(c, HC_K_FINISH);
     if (P.isscalar == 1) {
-        PROF_BEGIN(c, HC_K_RHS_T);
-#define HC_ST(NX_)                                                                                 \
-    if (A.zorder == 6) hipLaunchKernelGGL((k_spec_t<NX_, R, 6>), grow, dim3(rows * NX_), 0, c->stream, B); \
-    else hipLaunchKernelGGL((k_spec_t<NX_, R>), grow, dim3(rows * NX_), 0, c->stream, B);
-        HC_DISPATCH_N(c->nx, HC_ST)
-#undef HC_ST
-        PROF_END(c, HC_K_RHS_T);
-        PROF_BEGIN(c, HC_K_YFFT_T);
-        if (int e = launch_yfft(c, -1, 3, V.sp[0], V.sp[1], V.sp[2], nullptr)) return e;
-        PROF_END(c, HC_K_YFFT_T);
+        if (int e = spec_products_t<R>(c, A, V, 1)) return e;
         PROF_BEGIN(c, HC_K_COLUMN_T);
         if (int e = launch_spec_columns<R>(c, 1, A, V)) return e;
         PROF_END(c, HC_K_COLUMN_T);

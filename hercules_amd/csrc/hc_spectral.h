@@ -514,6 +514,12 @@ __global__ __launch_bounds__(64 * kColWaves) void k_column_spec(StepArgs A, Spec
     } else {
         spec_assemble<L>(A, io, tile, m, plane, col0, 2, is00, kx, ky, hxy, cpr, dw[0]);
     }
+    if (A.stage) {  // get_momentum_cnvdiff_spec alone: c_qu, c_qv, c_qw in place of c_u, c_v, c_w
+        col_store<L>(io.cu, tile, nz, plane, col0, m, duv[0]);
+        col_store<L>(io.cv, tile, nz, plane, col0, m, duv[1]);
+        col_store<L>(io.cw, tile, nz, plane, col0, m, dw[0]);
+        return;
+    }
     if (kept) {
         RC a[L], b[L], c[L];
         assemble<L>(A, 0, l, jm, is00, m, a, b, c);
@@ -618,6 +624,10 @@ __global__ __launch_bounds__(64 * kColWaves) void k_column_spec_t(StepArgs A, Sp
         d[0][r] = mk2<CC>(d[0][r].x + ra * f[r].x, d[0][r].y + ra * f[r].y);
     }
     col_store<L>(io.css, tile, nz, plane, col0, m, f);
+    if (A.stage) {  // get_temperature_cnvdiff_spec alone: c_qt in place of c_t
+        col_store<L>(io.ct, tile, nz, plane, col0, m, d[0]);
+        return;
+    }
     if (kept) {
         RC a[L], b[L], c[L];
         assemble<L>(A, 3, l, jm, is00, m, a, b, c);

@@ -230,6 +230,75 @@ module module_hercules_hip
             integer(c_int), value :: layout
             integer(c_int) :: rc
         end function
+        function hc_get_momentum_cnvdiff_spec(ctx, rk_a, rk_b, rk_c, u, v, w, p, t, c_u, c_v, c_w, c_p, &
+                                              c_ff, c_gg, c_hh, c_qu, c_qv, c_qw, layout) &
+                 bind(C, name="hc_get_momentum_cnvdiff_spec") result(rc)
+            import :: c_ptr, c_int, c_double
+            type(c_ptr), value :: ctx, u, v, w, p, t, c_u, c_v, c_w, c_p, c_ff, c_gg, c_hh, c_qu, c_qv, c_qw
+            real(c_double), value :: rk_a, rk_b, rk_c
+            integer(c_int), value :: layout
+            integer(c_int) :: rc
+        end function
+        function hc_get_temperature_cnvdiff_spec(ctx, rk_a, rk_b, rk_c, u, v, w, t, c_t, c_ss, c_qt, layout) &
+                 bind(C, name="hc_get_temperature_cnvdiff_spec") result(rc)
+            import :: c_ptr, c_int, c_double
+            type(c_ptr), value :: ctx, u, v, w, t, c_t, c_ss, c_qt
+            real(c_double), value :: rk_a, rk_b, rk_c
+            integer(c_int), value :: layout
+            integer(c_int) :: rc
+        end function
+        function hc_poisson_solver_fft_spec(ctx, equation, rk_c, c_src, c_var) &
+                 bind(C, name="hc_poisson_solver_fft_spec") result(rc)
+            import :: c_ptr, c_int, c_double
+            type(c_ptr), value :: ctx, c_src, c_var
+            integer(c_int), value :: equation
+            real(c_double), value :: rk_c
+            integer(c_int) :: rc
+        end function
+        function hc_get_div_spec(ctx, c_u, c_v, w, c_div, layout) bind(C, name="hc_get_div_spec") result(rc)
+            import :: c_ptr, c_int
+            type(c_ptr), value :: ctx, c_u, c_v, w, c_div
+            integer(c_int), value :: layout
+            integer(c_int) :: rc
+        end function
+        function hc_calculate_uvw_spec(ctx, rk_c, phi, c_phi, c_u, c_v, c_w, layout) &
+                 bind(C, name="hc_calculate_uvw_spec") result(rc)
+            import :: c_ptr, c_int, c_double
+            type(c_ptr), value :: ctx, phi, c_phi, c_u, c_v, c_w
+            real(c_double), value :: rk_c
+            integer(c_int), value :: layout
+            integer(c_int) :: rc
+        end function
+        function hc_calculate_new_p_spec(ctx, rk_c, phi, c_phi, c_p, layout) &
+                 bind(C, name="hc_calculate_new_p_spec") result(rc)
+            import :: c_ptr, c_int, c_double
+            type(c_ptr), value :: ctx, phi, c_phi, c_p
+            real(c_double), value :: rk_c
+            integer(c_int), value :: layout
+            integer(c_int) :: rc
+        end function
+        function hc_real_2_cmplx(ctx, r_in, c_out, layout) bind(C, name="hc_real_2_cmplx") result(rc)
+            import :: c_ptr, c_int
+            type(c_ptr), value :: ctx, r_in, c_out
+            integer(c_int), value :: layout
+            integer(c_int) :: rc
+        end function
+        function hc_cmplx_2_real(ctx, c_in, r_out, layout) bind(C, name="hc_cmplx_2_real") result(rc)
+            import :: c_ptr, c_int
+            type(c_ptr), value :: ctx, c_in, r_out
+            integer(c_int), value :: layout
+            integer(c_int) :: rc
+        end function
+        function hc_spectral_truncation(ctx, c_inout) bind(C, name="hc_spectral_truncation") result(rc)
+            import :: c_ptr, c_int
+            type(c_ptr), value :: ctx, c_inout
+            integer(c_int) :: rc
+        end function
+        function hc_rayleigh_damping_spec(ctx, c_inout) bind(C, name="hc_rayleigh_damping_spec") result(rc)
+            import :: c_ptr, c_int
+            type(c_ptr), value :: ctx, c_inout
+            integer(c_int) :: rc
+        end function
         function c_strlen(s) bind(C, name="strlen") result(n)
             import :: c_ptr, c_size_t
             type(c_ptr), value :: s
@@ -449,6 +518,18 @@ contains
         real(c_double), dimension(:,:,:), intent(out), target, contiguous :: r_var
         integer, intent(in) :: zstagger, delnull, tbnd, ubnd
         integer(c_int) :: eq
+        eq = flags_equation(zstagger, delnull, tbnd, ubnd)
+        ! the reference zeroes r_var first (:527)
+        r_var(:,:,:) = 0.d0
+        call hip_check(hc_poisson_solver_fft(hip_ctx, eq, hip_rk_c, c_loc(r_src), c_loc(r_var), &
+                                             HC_LAYOUT_GHOSTED), 'poisson_solver_fft')
+        if (size(aa) < 0 .or. size(bb) < 0 .or. size(cc) < 0) stop
+    end subroutine
+
+    ! the equation a flag tuple of poisson_solver_fft(_spec) selects
+    function flags_equation(zstagger, delnull, tbnd, ubnd) result(eq)
+        integer, intent(in) :: zstagger, delnull, tbnd, ubnd
+        integer(c_int) :: eq
         if (zstagger .eq. 1 .and. delnull .eq. 0 .and. tbnd .eq. 0 .and. ubnd .eq. 0) then
             eq = HC_EQ_W
         elseif (zstagger .eq. 0 .and. delnull .eq. 1 .and. tbnd .eq. 0 .and. ubnd .eq. 0) then
@@ -463,11 +544,95 @@ contains
             print *, 'Invalid zstagger/delnull/tbnd/ubnd flags!'
             stop
         endif
-        ! the reference zeroes r_var first (:527)
-        r_var(:,:,:) = 0.d0
-        call hip_check(hc_poisson_solver_fft(hip_ctx, eq, hip_rk_c, c_loc(r_src), c_loc(r_var), &
-                                             HC_LAYOUT_GHOSTED), 'poisson_solver_fft')
+    end function
+
+    ! ---- the spectral substep's stages (src/module_spectral.f90); the complex
+    ! work arrays are the reference's c(nx, ny, nz), the real ones its ghosted
+    ! module arrays
+
+    ! get_momentum_cnvdiff_spec (:35-504)
+    subroutine get_momentum_cnvdiff_spec(u, v, w, p, t, c_u, c_v, c_w, c_p, c_ff, c_gg, c_hh, c_qu, c_qv, c_qw)
+        real(c_double), dimension(:,:,:), intent(in), target, contiguous :: u, v, w, p, t
+        complex(c_double_complex), dimension(:,:,:), intent(in), target, contiguous :: c_u, c_v, c_w, c_p
+        complex(c_double_complex), dimension(:,:,:), intent(inout), target, contiguous :: c_ff, c_gg, c_hh
+        complex(c_double_complex), dimension(:,:,:), intent(out), target, contiguous :: c_qu, c_qv, c_qw
+        call hip_check(hc_get_momentum_cnvdiff_spec(hip_ctx, hip_rk_a, hip_rk_b, hip_rk_c, c_loc(u), c_loc(v), &
+                                                    c_loc(w), c_loc(p), c_loc(t), c_loc(c_u), c_loc(c_v), &
+                                                    c_loc(c_w), c_loc(c_p), c_loc(c_ff), c_loc(c_gg), c_loc(c_hh), &
+                                                    c_loc(c_qu), c_loc(c_qv), c_loc(c_qw), HC_LAYOUT_GHOSTED), &
+                       'get_momentum_cnvdiff_spec')
+    end subroutine
+
+    ! get_temperature_cnvdiff_spec (:509-595)
+    subroutine get_temperature_cnvdiff_spec(u, v, w, t, c_t, c_ss, c_qt)
+        real(c_double), dimension(:,:,:), intent(in), target, contiguous :: u, v, w, t
+        complex(c_double_complex), dimension(:,:,:), intent(in), target, contiguous :: c_t
+        complex(c_double_complex), dimension(:,:,:), intent(inout), target, contiguous :: c_ss
+        complex(c_double_complex), dimension(:,:,:), intent(out), target, contiguous :: c_qt
+        call hip_check(hc_get_temperature_cnvdiff_spec(hip_ctx, hip_rk_a, hip_rk_b, hip_rk_c, c_loc(u), c_loc(v), &
+                                                       c_loc(w), c_loc(t), c_loc(c_t), c_loc(c_ss), c_loc(c_qt), &
+                                                       HC_LAYOUT_GHOSTED), 'get_temperature_cnvdiff_spec')
+    end subroutine
+
+    ! poisson_solver_fft_spec (:602-746); the coefficient arrays are held by the library
+    subroutine poisson_solver_fft_spec(aa, bb, cc, c_src_in, c_var_out, zstagger, delnull, tbnd, ubnd)
+        real(c_double), dimension(:,:,:), intent(inout) :: aa, cc
+        complex(c_double_complex), dimension(:,:,:), intent(inout) :: bb
+        complex(c_double_complex), dimension(:,:,:), intent(in), target, contiguous :: c_src_in
+        complex(c_double_complex), dimension(:,:,:), intent(out), target, contiguous :: c_var_out
+        integer, intent(in) :: zstagger, delnull, tbnd, ubnd
+        call hip_check(hc_poisson_solver_fft_spec(hip_ctx, flags_equation(zstagger, delnull, tbnd, ubnd), hip_rk_c, &
+                                                  c_loc(c_src_in), c_loc(c_var_out)), 'poisson_solver_fft_spec')
         if (size(aa) < 0 .or. size(bb) < 0 .or. size(cc) < 0) stop
+    end subroutine
+
+    ! get_div_spec (:752-795)
+    subroutine get_div_spec(c_u, c_v, w, c_div)
+        complex(c_double_complex), dimension(:,:,:), intent(in), target, contiguous :: c_u, c_v
+        real(c_double), dimension(:,:,:), intent(in), target, contiguous :: w
+        complex(c_double_complex), dimension(:,:,:), intent(out), target, contiguous :: c_div
+        call hip_check(hc_get_div_spec(hip_ctx, c_loc(c_u), c_loc(c_v), c_loc(w), c_loc(c_div), HC_LAYOUT_GHOSTED), &
+                       'get_div_spec')
+    end subroutine
+
+    ! calculate_uvw_spec (:801-849)
+    subroutine calculate_uvw_spec(phi, c_phi, c_u, c_v, c_w)
+        real(c_double), dimension(:,:,:), intent(in), target, contiguous :: phi
+        complex(c_double_complex), dimension(:,:,:), intent(in), target, contiguous :: c_phi
+        complex(c_double_complex), dimension(:,:,:), intent(inout), target, contiguous :: c_u, c_v, c_w
+        call hip_check(hc_calculate_uvw_spec(hip_ctx, hip_rk_c, c_loc(phi), c_loc(c_phi), c_loc(c_u), c_loc(c_v), &
+                                             c_loc(c_w), HC_LAYOUT_GHOSTED), 'calculate_uvw_spec')
+    end subroutine
+
+    ! calculate_new_p_spec (:854-894)
+    subroutine calculate_new_p_spec(phi, c_phi, c_p)
+        real(c_double), dimension(:,:,:), intent(in), target, contiguous :: phi
+        complex(c_double_complex), dimension(:,:,:), intent(in), target, contiguous :: c_phi
+        complex(c_double_complex), dimension(:,:,:), intent(inout), target, contiguous :: c_p
+        call hip_check(hc_calculate_new_p_spec(hip_ctx, hip_rk_c, c_loc(phi), c_loc(c_phi), c_loc(c_p), &
+                                               HC_LAYOUT_GHOSTED), 'calculate_new_p_spec')
+    end subroutine
+
+    ! real_2_cmplx / cmplx_2_real (:1157-1217)
+    subroutine real_2_cmplx(r_in, c_out)
+        real(c_double), dimension(:,:,:), intent(in), target, contiguous :: r_in
+        complex(c_double_complex), dimension(:,:,:), intent(out), target, contiguous :: c_out
+        call hip_check(hc_real_2_cmplx(hip_ctx, c_loc(r_in), c_loc(c_out), HC_LAYOUT_GHOSTED), 'real_2_cmplx')
+    end subroutine
+    subroutine cmplx_2_real(c_in, r_out)
+        complex(c_double_complex), dimension(:,:,:), intent(in), target, contiguous :: c_in
+        real(c_double), dimension(:,:,:), intent(inout), target, contiguous :: r_out
+        call hip_check(hc_cmplx_2_real(hip_ctx, c_loc(c_in), c_loc(r_out), HC_LAYOUT_GHOSTED), 'cmplx_2_real')
+    end subroutine
+
+    ! spectral_truncation (:1222-1268) and rayleigh_damping_spec (:1345-1391)
+    subroutine spectral_truncation(c_inout3)
+        complex(c_double_complex), dimension(:,:,:), intent(inout), target, contiguous :: c_inout3
+        call hip_check(hc_spectral_truncation(hip_ctx, c_loc(c_inout3)), 'spectral_truncation')
+    end subroutine
+    subroutine rayleigh_damping_spec(c_varin)
+        complex(c_double_complex), dimension(:,:,:), intent(inout), target, contiguous :: c_varin
+        call hip_check(hc_rayleigh_damping_spec(hip_ctx, c_loc(c_varin)), 'rayleigh_damping_spec')
     end subroutine
 
     ! src/module_parameters.f90:513-546: the module's ubar from the given u

@@ -141,8 +141,10 @@ int hc_time_advancement_fd(hc_ctx *ctx);
 
 /* time_advancement_sp: one full step of the spectral path
  * (src/module_navier_stokes.f90:371-517, src/module_spectral.f90).
- * Requires cds = 3; isdamp = 1 is rejected at hc_create because
- * rayleigh_damping_spec has no defined result (src/module_spectral.f90:1372). */
+ * Requires cds = 3.  With isdamp = 1, rayleigh_damping_spec relaxes the five
+ * spectra toward their level's whole complex mean over the modes, where the
+ * reference reduces that mean as one real (src/module_spectral.f90:1372); for
+ * the spectra of real fields the mean is real. */
 int hc_time_advancement_sp(hc_ctx *ctx);
 
 /* nsteps passes of the drive loop body: update_ri, time_advancement_fd or
@@ -377,6 +379,55 @@ int hc_calculate_uvw(hc_ctx *ctx, double rk_c, double *u, double *v, double *w, 
 /* calculate_new_p(phi,p) (src/module_navier_stokes.f90:739-808): p updated
  * in place. */
 int hc_calculate_new_p(hc_ctx *ctx, double rk_c, const double *phi, double *p, int layout);
+
+/* Per-stage entry points of the spectral substep (cds = 3, FP64, 2nd-order
+ * z), for callers that keep the reference's time_advancement_sp loop
+ * (src/module_navier_stokes.f90:417-464).  The complex arguments c_* are the
+ * reference's work arrays c(nx, ny, nz): full C2C spectra, interleaved
+ * (re, im) doubles, x index fastest, forward transform divided by nx ny.
+ * They hold the spectra of real fields; the x and y Nyquist modes are not
+ * carried (read as 0, written as 0) and the modes above nx/2 are written as
+ * the conjugate mirror.  Real arrays are whole fields in `layout`.  Terms the
+ * reference forms from its real arguments (the z diffusion and dp/dz, dw/dz,
+ * the z derivatives of phi) are formed from those arguments here too.
+ *
+ * get_momentum_cnvdiff_spec(u,v,w,p,t,c_u,c_v,c_w,c_p,c_ff,c_gg,c_hh,c_qu,c_qv,c_qw)
+ * (src/module_spectral.f90:35-504), with its update_dpx (:51): c_ff, c_gg,
+ * c_hh hold the old right-hand sides on entry and the new ones on return.
+ * t may be null when isscalar = 0. */
+int hc_get_momentum_cnvdiff_spec(hc_ctx *ctx, double rk_a, double rk_b, double rk_c, const double *u,
+                                 const double *v, const double *w, const double *p, const double *t,
+                                 const double *c_u, const double *c_v, const double *c_w, const double *c_p,
+                                 double *c_ff, double *c_gg, double *c_hh, double *c_qu, double *c_qv, double *c_qw,
+                                 int layout);
+/* get_temperature_cnvdiff_spec(u,v,w,t,c_t,c_ss,c_qt) (src/module_spectral.f90:509-595);
+ * c_ss old on entry, new on return. */
+int hc_get_temperature_cnvdiff_spec(hc_ctx *ctx, double rk_a, double rk_b, double rk_c, const double *u,
+                                    const double *v, const double *w, const double *t, const double *c_t,
+                                    double *c_ss, double *c_qt, int layout);
+/* poisson_solver_fft_spec(aa,bb,cc,c_src,c_var,zstagger,delnull,tbnd,ubnd)
+ * (src/module_spectral.f90:602-746), the flag tuple given as the equation
+ * of hc_poisson_solver_fft and the coefficients assigned at rk_c. */
+int hc_poisson_solver_fft_spec(hc_ctx *ctx, int equation, double rk_c, const double *c_src, double *c_var);
+/* get_div_spec(c_u,c_v,w,c_div) (src/module_spectral.f90:752-795). */
+int hc_get_div_spec(hc_ctx *ctx, const double *c_u, const double *c_v, const double *w, double *c_div, int layout);
+/* calculate_uvw_spec(phi,c_phi,c_u,c_v,c_w) (src/module_spectral.f90:801-849):
+ * c_u, c_v, c_w updated in place. */
+int hc_calculate_uvw_spec(hc_ctx *ctx, double rk_c, const double *phi, const double *c_phi, double *c_u,
+                          double *c_v, double *c_w, int layout);
+/* calculate_new_p_spec(phi,c_phi,c_p) (src/module_spectral.f90:854-894):
+ * c_p updated in place. */
+int hc_calculate_new_p_spec(hc_ctx *ctx, double rk_c, const double *phi, const double *c_phi, double *c_p,
+                            int layout);
+/* The utilities time_advancement_sp calls between its stages:
+ * real_2_cmplx(r_in,c_out) and cmplx_2_real(c_in,r_out)
+ * (src/module_spectral.f90:1157-1217; cmplx_2_real writes the interior of
+ * r_out), spectral_truncation(c) (:1222-1268) and rayleigh_damping_spec(c)
+ * (:1345-1391, with the whole complex mean; needs isdamp = 1). */
+int hc_real_2_cmplx(hc_ctx *ctx, const double *r_in, double *c_out, int layout);
+int hc_cmplx_2_real(hc_ctx *ctx, const double *c_in, double *r_out, int layout);
+int hc_spectral_truncation(hc_ctx *ctx, double *c_inout);
+int hc_rayleigh_damping_spec(hc_ctx *ctx, double *c_inout);
 
 /* Ensembles of independent channels (no counterpart in the reference, which
  * runs one channel per job).  The members are contexts made by hc_create and

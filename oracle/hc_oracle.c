@@ -1860,3 +1860,44 @@ int ho_stage_temperature_spec(ho_state *S, int t_its)
     temperature_spec(S);
     return 0;
 }
+/* poisson_solver_fft_spec of one equation (flags as in ho_solve) with the
+ * coefficients assigned at rk_c: c_qu -> c_phi */
+int ho_stage_poisson_spec(ho_state *S, int eq, double rk_c)
+{
+    if (S->prm.cds != 3) return 1;
+    S->rk_c = rk_c;
+    cplx *src = S->c_qu, *out = S->c_phi;
+    switch (eq) {
+    case 0: assign_abc(S, S->aa_u, S->bb_u, S->cc_u, 0); return poisson_spec(S, src, out, 0, 0, 0, 1);
+    case 1: assign_abc(S, S->aa_u, S->bb_u, S->cc_u, 0); return poisson_spec(S, src, out, 0, 0, 0, 0);
+    case 2: assign_abc(S, S->aa_w, S->bb_w, S->cc_w, 0); return poisson_spec(S, src, out, 1, 0, 0, 0);
+    case 3: assign_abc(S, S->aa_p, S->bb_p, S->cc_p, 1); return poisson_spec(S, src, out, 0, 1, 0, 0);
+    case 4: assign_abc(S, S->aa_t, S->bb_t, S->cc_t, 0); return poisson_spec(S, src, out, 0, 0, 1, 0);
+    default: snprintf(S->err, sizeof S->err, "bad equation id"); return 1;
+    }
+}
+/* get_div_spec(c_u, c_v, w, c_qu) */
+int ho_stage_div_spec(ho_state *S)
+{
+    if (S->prm.cds != 3) return 1;
+    div_spec(S, S->c_u, S->c_v, S->w, S->c_qu);
+    return 0;
+}
+/* calculate_uvw_spec / calculate_new_p_spec at substep t_its, with phi's
+ * ghosts from update_boundary_p as time_advancement_sp sets them (:457-458) */
+int ho_stage_uvw_spec(ho_state *S, int t_its)
+{
+    if (S->prm.cds != 3) return 1;
+    assign_rk(S, t_its);
+    bc_p(S, S->phi);
+    uvw_spec(S);
+    return 0;
+}
+int ho_stage_new_p_spec(ho_state *S, int t_its)
+{
+    if (S->prm.cds != 3) return 1;
+    assign_rk(S, t_its);
+    bc_p(S, S->phi);
+    new_p_spec(S);
+    return 0;
+}

@@ -83,6 +83,10 @@ def lib():
         L.ho_stage_momentum_spec.argtypes = [vp, ip]
         L.ho_stage_damping_spec.argtypes = [vp, ip]
         L.ho_stage_temperature_spec.argtypes = [vp, ip]
+        L.ho_stage_poisson_spec.argtypes = [vp, ip, C.c_double]
+        L.ho_stage_div_spec.argtypes = [vp]
+        L.ho_stage_uvw_spec.argtypes = [vp, ip]
+        L.ho_stage_new_p_spec.argtypes = [vp, ip]
         L.ho_spec_divergence.argtypes = [vp, dp]
         L.ho_get_scalar.restype = C.c_double
         L.ho_mean_enable.argtypes = [vp]

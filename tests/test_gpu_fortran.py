@@ -257,3 +257,55 @@ def test_host_stage_loop_equals_device_step(tmp_path, kind):
         err = rel_l2(b, a)
         print(kind, k, f"{err:.2e}")
         assert err < 1e-11, (k, err)
+
+
+@pytest.mark.parametrize("kind", ["closed_sp_skew", "open_t_sp_dp2_damped", "closed_sp_adv_ab2"])
+def test_host_spectral_stage_loop_equals_device_step(tmp_path, kind):
+    """A Fortran caller running the reference's spectral substep loop and
+    start-up on the host with the spectral per-stage subroutines and complex
+    work arrays (tests/fortran/stage_loop_sp.f90) gives the fields of the
+    fused device step time_advancement_sp run by the driver on the same deck."""
+    from hercules_amd.build import STAGE_SP_EXE
+    if not os.path.exists(FEXE) or not os.path.exists(STAGE_SP_EXE):
+        pytest.fail("Fortran host programs not built")
+    issk, damp = 1, "isdamp = 0, nzdamp = 15, cdamp = 0.01"
+    if kind == "closed_sp_skew":
+        prm = cases.closed_channel(nx=32, ny=32, nz=33, imax=3, cds=3)
+        st = cases.laminar_noise(prm, seed=34)
+        dts = 2
+    elif kind == "open_t_sp_dp2_damped":
+        prm = cases.stratified_open_channel(nx=32, ny=32, nz=33, imax=3, cds=3, dp_opt=2, isdamp=1, nzdamp=6,
+                                            cdamp=0.2)
+        st = cases.loglaw_noise(prm, seed=35)
+        dts, damp = 2, "isdamp = 1, nzdamp = 6, cdamp = 0.2"
+    else:
+        prm = cases.closed_channel(nx=16, ny=32, nz=17, imax=3, cds=3, dts=1)
+        st = cases.laminar_noise(prm, seed=36)
+        dts, issk = 1, 0
+    with Channel(prm) as ch:
+        for k, v in st.items():
+            ch.set_field(k, v)
+        ch.project()
+        st = {k: ch.get_field(k) for k in ("u", "v", "w", "p", "t")}
+    (tmp_path / "results").mkdir()
+    deck = DECK_T.replace("issk = 1", f"issk = {issk}").replace("isdamp = 0, nzdamp = 15, cdamp = 0.01", damp)
+    (tmp_path / "parameters.input").write_text(deck.format(
+        lx=prm.lx, ly=prm.ly, lz=prm.lz, nx=prm.nx, ny=prm.ny, nz=prm.nz, zs=prm.zstretch, och=prm.ochannel,
+        dp=prm.dp_opt, cds=prm.cds, dts=dts, nu=prm.nu, alpha=prm.alpha, got=prm.got, imax=prm.imax,
+        mrf=prm.u_mrf, iss=prm.isscalar, ri=0))
+    for k, v in st.items():
+        (np.ascontiguousarray(v) + (prm.u_mrf if k == "u" else 0.0)).tofile(tmp_path / "results" / f"init_{k}.dat")
+    if dts == 1:
+        for k in ("ff", "gg", "hh", "ss"):
+            np.zeros((prm.nz, prm.ny, prm.nx)).tofile(tmp_path / "results" / f"init_{k}.dat")
+    for exe in (FEXE, STAGE_SP_EXE):
+        r = subprocess.run([exe], cwd=tmp_path, capture_output=True, text=True, timeout=120)
+        print(exe, r.stdout[-1500:], r.stderr[-1500:])
+        assert r.returncode == 0
+    names = ("u", "v", "w", "p", "t") if prm.isscalar else ("u", "v", "w", "p")
+    for k in names:
+        a = np.fromfile(tmp_path / "results" / f"bak_{k}.dat")
+        b = np.fromfile(tmp_path / "results" / f"stage_{k}.dat")
+        err = rel_l2(b, a)
+        print(kind, k, f"{err:.2e}")
+        assert err < 1e-11, (k, err)
