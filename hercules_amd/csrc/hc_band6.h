@@ -27,7 +27,8 @@
 // the mixed variant); the factorisation and the solve run in FP64.
 template <class CI>
 struct BandSys {
-    CI *f[2];       // right-hand sides (k, j, l), solved in place
+    const CI *in[2];  // right-hand sides (k, j, l)
+    CI *f[2];         // solutions (may be in[]); hold the eliminated right-hand sides in between
     double *scr;    // U rows: PB + 1 doubles per level and column, level-major
     int nr;         // right-hand sides sharing the matrix: 1 or 2
     int eq;         // 0 u/v, 1 w, 2 p, 3 t
@@ -129,8 +130,8 @@ __global__ __launch_bounds__(256) void k_z6_band(StepArgs A, BandArgs<CI> B)
         a[PB] += cdiag;
         const size_t o = (size_t)k * plane + col;
         double2 r[2];
-        r[0] = ccast<double2>(S.f[0][o]);
-        r[1] = nr > 1 ? ccast<double2>(S.f[1][o]) : make_double2(0.0, 0.0);
+        r[0] = ccast<double2>(S.in[0][o]);
+        r[1] = nr > 1 ? ccast<double2>(S.in[1][o]) : make_double2(0.0, 0.0);
         if (is00) {
             if (pres) {
                 if (k == 0) {  // delnull: b0 phi(0) = 0 (src/module_poisson_solver.f90:558-564)
@@ -228,11 +229,16 @@ __global__ __launch_bounds__(256) void k_z6_band(StepArgs A, BandArgs<CI> B)
 // load issued less than kPF levels earlier.
 constexpr int kWin = 2 * kBandMax + 1;
 
-// Iterations i = 0, 1, ...: window field w enters level i, point fields
-// enter level i - kBandMax, and row k = i - kBandMax (0 <= k < nrows) is
+// A thread forms the rows kb .. ke-1 of its column's z chunk (blockIdx.z of
+// gridDim.z equal chunks), so that a launch has several wavefronts per SIMD.
+// Iterations i = kb - kBandMax, ...: window field w enters level i, point
+// fields enter level i - kBandMax, and row k = i - kBandMax (kb <= k < ke) is
 // formed by row(k, win, pts) with win[kBandMax + t] = level k + t (zero
-// outside 0 .. wlev[f] - 1).  Fields are stored as CI (double2 or float2)
-// and held in FP64.
+// outside 0 .. wlev[f] - 1; a chunk reads kBandMax levels beyond each end).
+// Fields are stored as CI (double2 or float2) and held in FP64.  A kernel
+// writes only its point fields' rows or fields it does not read.
+constexpr int kStreamRowsPerChunk = 32;
+inline int stream_chunks(int nz) { return nz / kStreamRowsPerChunk < 1 ? 1 : (nz / kStreamRowsPerChunk > 8 ? 8 : nz / kStreamRowsPerChunk); }
 template <int N, class CI = double2>
 struct FieldSet {
     const CI *p[N > 0 ? N : 1];
@@ -246,6 +252,9 @@ HC_DEV void z6_stream(size_t plane, size_t col, int nrows, const FieldSet<NW, CW
     const double2 zero = make_double2(0.0, 0.0);
     const CW zw_ = czero_t<CW>();
     const CP zp_ = czero_t<CP>();
+    const int csz = (nrows + (int)gridDim.z - 1) / (int)gridDim.z;
+    const int kb = (int)blockIdx.z * csz, ke = kb + csz < nrows ? kb + csz : nrows;
+    const int ib = kb - kBandMax;
     // the staging ring holds the stored type: widening to FP64 happens when a
     // level leaves the ring, so no conversion waits on a load in flight
     double2 win[NW][kWin];
@@ -257,13 +266,13 @@ HC_DEV void z6_stream(size_t plane, size_t col, int nrows, const FieldSet<NW, CW
         for (int t = 0; t < kWin; ++t) win[f][t] = zero;
 #pragma unroll
     for (int u = 0; u < PF; ++u) {
+        const int li = ib + u, lp = li - kBandMax;
 #pragma unroll
-        for (int f = 0; f < NW; ++f) fw[f][u] = u < wlev[f] ? ws.p[f][(size_t)u * plane + col] : zw_;
-        const int lp = u - kBandMax;
+        for (int f = 0; f < NW; ++f) fw[f][u] = (li >= 0 && li < wlev[f]) ? ws.p[f][(size_t)li * plane + col] : zw_;
 #pragma unroll
-        for (int p = 0; p < NP; ++p) fp[u][p] = (lp >= 0 && lp < nrows) ? ps.p[p][(size_t)lp * plane + col] : zp_;
+        for (int p = 0; p < NP; ++p) fp[u][p] = (lp >= kb && lp < ke) ? ps.p[p][(size_t)lp * plane + col] : zp_;
     }
-    for (int i0 = 0; i0 < nrows + kBandMax; i0 += PF) {
+    for (int i0 = ib; i0 < ke + kBandMax; i0 += PF) {
 #pragma unroll
         for (int u = 0; u < PF; ++u) {
             const int i = i0 + u;
@@ -279,11 +288,12 @@ HC_DEV void z6_stream(size_t plane, size_t col, int nrows, const FieldSet<NW, CW
             // the loads of iteration i + PF
             const int li = i + PF, lp = li - kBandMax;
 #pragma unroll
-            for (int f = 0; f < NW; ++f) fw[f][u] = li < wlev[f] ? ws.p[f][(size_t)li * plane + col] : zw_;
+            for (int f = 0; f < NW; ++f)
+                fw[f][u] = (li >= 0 && li < wlev[f]) ? ws.p[f][(size_t)li * plane + col] : zw_;
 #pragma unroll
-            for (int p = 0; p < NP; ++p) fp[u][p] = (lp >= 0 && lp < nrows) ? ps.p[p][(size_t)lp * plane + col] : zp_;
+            for (int p = 0; p < NP; ++p) fp[u][p] = (lp >= kb && lp < ke) ? ps.p[p][(size_t)lp * plane + col] : zp_;
             const int k = i - kBandMax;
-            if (k >= 0 && k < nrows) row(k, win, pts);
+            if (k >= kb && k < ke) row(k, win, pts);
         }
     }
 }
@@ -388,8 +398,8 @@ __global__ __launch_bounds__(256) void k_z6s_proj(StepArgs A, CI *u, CI *v, CI *
 
 // Right-hand sides of the spectral substep's Helmholtz systems
 // (get_momentum_cnvdiff_spec, src/module_spectral.f90:35-504, with the
-// 6th-order z operators), in place of c_u, c_v (blockIdx.y = 0, 1) or c_w
-// (k_z6s_rhs_w):
+// 6th-order z operators) for u, v (blockIdx.y = 0, 1; into sp[3], sp[4]) or
+// w (k_z6s_rhs_w; into sp[7]):
 //   q = c + hxy c + dif Z c - cdt grad_e p + rb dt f_old + ra dt f
 //   f = -u_mrf i kx c - coe (i kx X + i ky Y) + coe R
 // with X, Y, R the transformed products of the component; f replaces f_old.
@@ -407,7 +417,9 @@ __global__ __launch_bounds__(256) void k_z6s_rhs_uv(StepArgs A, SpecIO<real_of<C
     const double rb = A.rb * A.dt, ra = A.ra * A.dt, kk = e == 0 ? kx : ky;
     const bool is00 = e == 0 && c.l == 0 && c.jm == 0;
     const int band = A.ochannel ? ZB_C2_DN : ZB_C2_DD, op = A.ochannel ? ZO_C2_DN : ZO_C2_DD;
-    CI *cs = e == 0 ? io.cu : io.cv, *fo = e == 0 ? io.cff : io.cgg;
+    // q goes over R_u / R_v (read at the same row by the same thread): c_u,
+    // c_v stay intact for the neighbouring z chunks' windows
+    CI *cs = e == 0 ? io.cu : io.cv, *fo = e == 0 ? io.cff : io.cgg, *qo = e == 0 ? io.sp[3] : io.sp[4];
     const FieldSet<1, CI> ws = {{cs}, {nz}};
     const FieldSet<5, CI> ps = {{io.sp[e == 0 ? 0 : 1], io.sp[e == 0 ? 1 : 2], io.sp[e == 0 ? 3 : 4], io.cp, fo},
                                 {0, 0, 0, 0, 0}};
@@ -424,7 +436,7 @@ __global__ __launch_bounds__(256) void k_z6s_rhs_uv(StepArgs A, SpecIO<real_of<C
         q = make_double2(q.x + rb * p[4].x + ra * f.x, q.y + rb * p[4].y + ra * f.y);
         const size_t o = (size_t)k * c.plane + c.col;
         fo[o] = ccast<CI>(f);
-        cs[o] = ccast<CI>(q);
+        qo[o] = ccast<CI>(q);
     });
 }
 template <class CI>
@@ -437,7 +449,6 @@ __global__ __launch_bounds__(256) void k_z6s_rhs_w(StepArgs A, SpecIO<real_of<CI
     const double hxy = A.rc * (A.symx[SY_HU * nl + c.l] + A.symy[SY_HU * ny + c.jm]);
     const double dif = A.rc * 0.5 * A.dt * A.nu, coe = A.coe, um = A.u_mrf, cdt = A.rc * A.dt;
     const double rb = A.rb * A.dt, ra = A.ra * A.dt;
-    const CI zero = czero_t<CI>();
     const FieldSet<2, CI> ws = {{io.cw, io.cp}, {nz - 1, nz}};
     const FieldSet<4, CI> ps = {{io.sp[5], io.sp[6], io.sp[7], io.chh}, {0, 0, 0, 0}};
     z6_stream<2, 4, 4>(c.plane, c.col, nz, ws, ps, [&](int k, const double2 (&win)[2][kWin], const double2 (&p)[4]) __attribute__((always_inline)) {
@@ -448,22 +459,19 @@ __global__ __launch_bounds__(256) void k_z6s_rhs_w(StepArgs A, SpecIO<real_of<CI
         f = csub(f, times_i(p[1], coe * ky));
         f = make_double2(f.x + coe * p[2].x, f.y + coe * p[2].y);
         io.chh[o] = ccast<CI>(f);
-        if (k == nz - 1) {  // the top wall face
-            io.cw[o] = zero;
-            return;
-        }
+        if (k == nz - 1) return;  // the top wall face: no equation
         const double2 t = band_dot(band_row(A, ZB_F2, k), win[0]);
         const double2 g = band_dot(band_row(A, ZB_G, k), win[1]);
         double2 q = make_double2(x.x + hxy * x.x + dif * t.x - cdt * g.x, x.y + hxy * x.y + dif * t.y - cdt * g.y);
         q = make_double2(q.x + rb * p[3].x + ra * f.x, q.y + rb * p[3].y + ra * f.y);
-        io.cw[o] = ccast<CI>(q);
+        io.sp[7][o] = ccast<CI>(q);  // over R_w, as for u and v
     });
 }
 
 // Right-hand side of the spectral temperature system
-// (get_temperature_cnvdiff_spec with the 6th-order z operator) in place of
-// c_t; the new s replaces the old one in c_ss; the (0,0) mode adds the
-// explicit half of the wall data.
+// (get_temperature_cnvdiff_spec with the 6th-order z operator) into sp[2];
+// the new s replaces the old one in c_ss; the (0,0) mode adds the explicit
+// half of the wall data.
 template <class CI>
 __global__ __launch_bounds__(256) void k_z6s_rhs_t(StepArgs A, SpecIO<real_of<CI>> io)
 {
@@ -490,6 +498,6 @@ __global__ __launch_bounds__(256) void k_z6s_rhs_t(StepArgs A, SpecIO<real_of<CI
         d = make_double2(d.x + rb * p[3].x + ra * f.x, d.y + rb * p[3].y + ra * f.y);
         const size_t o = (size_t)k * c.plane + c.col;
         io.css[o] = ccast<CI>(f);
-        io.ct[o] = ccast<CI>(d);
+        io.sp[2][o] = ccast<CI>(d);  // over the z flux term
     });
 }

@@ -21,10 +21,13 @@ template void zinterp6<double>(hc_ctx *, StepArgsT<double> &);
 template void zinterp6<float>(hc_ctx *, StepArgsT<float> &);
 
 namespace {
+// right-hand sides in0/in1 (null: in place), solutions f0/f1
 template <class CI>
-BandSys<CI> band_sys(CI *f0, CI *f1, double *scr, int eq, int fix, int trunc)
+BandSys<CI> band_sys(const CI *in0, const CI *in1, CI *f0, CI *f1, double *scr, int eq, int fix, int trunc)
 {
     BandSys<CI> b;
+    b.in[0] = in0 ? in0 : f0;
+    b.in[1] = in1 ? in1 : f1;
     b.f[0] = f0;
     b.f[1] = f1;
     b.scr = scr;
@@ -51,7 +54,7 @@ void band_launch(hipStream_t st, const StepArgs &A, const BandArgs<CI> &B, int n
 void pressure_solve(hipStream_t st, const StepArgs &A)
 {
     BandArgs<double2> B;
-    B.s[0] = band_sys<double2>(A.zspc, nullptr, A.zscr, 2, 1, 0);
+    B.s[0] = band_sys<double2>(nullptr, nullptr, A.zspc, nullptr, A.zscr, 2, 1, 0);
     band_launch(st, A, B, 1);
 }
 
@@ -59,7 +62,11 @@ void pressure_solve(hipStream_t st, const StepArgs &A)
 // 6 doubles per level and column for half band 5
 size_t band_off(const StepArgs &A) { return (size_t)6 * A.nz * A.nl * A.ny; }
 
-dim3 stream_grid(const StepArgs &A, int ny = 1) { return dim3((unsigned)((A.nl * A.ny + 255) / 256), (unsigned)ny); }
+// one thread per (l, m) column and z chunk
+dim3 stream_grid(const StepArgs &A, int ny = 1)
+{
+    return dim3((unsigned)((A.nl * A.ny + 255) / 256), (unsigned)ny, (unsigned)stream_chunks(A.nz));
+}
 
 template <class R>
 void col6_seq(int kind, hipStream_t st, const StepArgs &A, const ColIO<R> &io, int eq)
@@ -68,15 +75,15 @@ void col6_seq(int kind, hipStream_t st, const StepArgs &A, const ColIO<R> &io, i
     BandArgs<CI> B;
     switch (kind) {
     case 0:  // FD substep: u, v and w Helmholtz solves, divergence, pressure, projection and increment (into s3)
-        B.s[0] = band_sys<CI>(io.s0, io.s1, A.zscr, 0, 1, 0);
-        B.s[1] = band_sys<CI>(io.s2, nullptr, A.zscr + band_off(A), 1, 0, 0);
+        B.s[0] = band_sys<CI>(nullptr, nullptr, io.s0, io.s1, A.zscr, 0, 1, 0);
+        B.s[1] = band_sys<CI>(nullptr, nullptr, io.s2, nullptr, A.zscr + band_off(A), 1, 0, 0);
         band_launch(st, A, B, 2);
         hipLaunchKernelGGL((k_z6s_div<CI>), stream_grid(A), dim3(256), 0, st, A, io.s0, io.s1, io.s2, A.zspc, A.idt);
         pressure_solve(st, A);
         hipLaunchKernelGGL((k_z6s_proj<CI>), stream_grid(A), dim3(256), 0, st, A, io.s0, io.s1, io.s2, A.zspc, io.s3, 0);
         break;
     case 1:  // temperature
-        B.s[0] = band_sys<CI>(io.s0, nullptr, A.zscr, 3, 1, 0);
+        B.s[0] = band_sys<CI>(nullptr, nullptr, io.s0, nullptr, A.zscr, 3, 1, 0);
         band_launch(st, A, B, 1);
         break;
     case 2:  // projection onto divergence-free fields (rk_c = 1)
@@ -87,7 +94,7 @@ void col6_seq(int kind, hipStream_t st, const StepArgs &A, const ColIO<R> &io, i
         break;
     default: {  // one equation: 0 u, 1 v, 2 w, 3 p, 4 t; wall data for u and t, delnull for p
         const int meq = eq == 0 || eq == 1 ? 0 : (eq == 2 ? 1 : (eq == 3 ? 2 : 3));
-        B.s[0] = band_sys<CI>(io.s0, nullptr, A.zscr, meq, eq == 0 || eq == 3 || eq == 4, 0);
+        B.s[0] = band_sys<CI>(nullptr, nullptr, io.s0, nullptr, A.zscr, meq, eq == 0 || eq == 3 || eq == 4, 0);
         band_launch(st, A, B, 1);
         break;
     }
@@ -101,14 +108,14 @@ void spec6_seq(int kind, hipStream_t st, const StepArgs &A, const SpecIO<R> &io)
     BandArgs<CI> B;
     if (kind == 1) {  // temperature: right-hand side, solve, 2/3 rule
         hipLaunchKernelGGL((k_z6s_rhs_t<CI>), stream_grid(A), dim3(256), 0, st, A, io);
-        B.s[0] = band_sys<CI>(io.ct, nullptr, A.zscr, 3, 1, 1);
+        B.s[0] = band_sys<CI>(io.sp[2], nullptr, io.ct, nullptr, A.zscr, 3, 1, 1);
         band_launch(st, A, B, 1);
         return;
     }
     hipLaunchKernelGGL((k_z6s_rhs_uv<CI>), stream_grid(A, 2), dim3(256), 0, st, A, io);
     hipLaunchKernelGGL((k_z6s_rhs_w<CI>), stream_grid(A), dim3(256), 0, st, A, io);
-    B.s[0] = band_sys<CI>(io.cu, io.cv, A.zscr, 0, 1, 0);
-    B.s[1] = band_sys<CI>(io.cw, nullptr, A.zscr + band_off(A), 1, 0, 0);
+    B.s[0] = band_sys<CI>(io.sp[3], io.sp[4], io.cu, io.cv, A.zscr, 0, 1, 0);
+    B.s[1] = band_sys<CI>(io.sp[7], nullptr, io.cw, nullptr, A.zscr + band_off(A), 1, 0, 0);
     band_launch(st, A, B, 2);
     hipLaunchKernelGGL((k_z6s_div<CI>), stream_grid(A), dim3(256), 0, st, A, io.cu, io.cv, io.cw, A.zspc, A.idt);
     pressure_solve(st, A);
