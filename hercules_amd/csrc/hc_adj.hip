@@ -14,8 +14,11 @@ int adjoint_substep(hc_ctx *c, int t_its)
     StepArgs A = base_args(c);
     if (P.dts == 2) { A.ra = kRkc1[t_its - 1]; A.rb = kRkc2[t_its - 1]; A.rc = kRkc3[t_its - 1]; }
     else { A.ra = 1.5; A.rb = -0.5; A.rc = 1.0; }
-    for (int q = 0; q < 4; ++q)
-        if (int e = launch_r2c<double>(c, c->adj[q], c->s[q])) return e;
+    {
+        const double *src4[4] = {c->adj[0], c->adj[1], c->adj[2], c->adj[3]};
+        double2 *dst4[4] = {c->s[0], c->s[1], c->s[2], c->s[3]};
+        if (int e = launch_r2c_n<double>(c, 4, src4, dst4)) return e;
+    }
     if (int e = launch_yfft(c, -1, 4, c->s[0], c->s[1], c->s[2], c->s[3])) return e;
     A.s0 = c->s[0]; A.s1 = c->s[1]; A.s2 = c->s[2]; A.s3 = c->s[3];
     dim3 gcol((c->nl + kColWaves - 1) / kColWaves, c->ny);

@@ -267,19 +267,17 @@ struct AdjOps {
 };
 
 // Pass 1: total adjoints of f, g, h (this substep's f feeds q with ra and
-// the stored f_old' of the next substep) and the adjoints of f_old.
+// the stored f_old' of the next substep).
 __global__ __launch_bounds__(256) void k_adj_fbar(StepArgs A, AdjIO io)
 {
     const size_t plane = (size_t)A.nx * A.ny, n = plane * A.nz, top = plane * (A.nz - 1);
     const double dra = A.dt * A.ra, drb = A.dt * A.rb;
+    (void)drb;  // the f_old adjoints rb dt q are written by k_adj_gather
     for (size_t pt = blockIdx.x * (size_t)blockDim.x + threadIdx.x; pt < n; pt += (size_t)gridDim.x * blockDim.x) {
         const double qu = io.qu[pt], qv = io.qv[pt], qw = pt < top ? io.qw[pt] : 0.0;
         io.fb[0][pt] = io.fu[pt] + dra * qu;
         io.fb[1][pt] = io.fv[pt] + dra * qv;
         io.fb[2][pt] = pt < top ? io.fw[pt] + dra * qw : 0.0;
-        io.ou[pt] = drb * qu;
-        io.ov[pt] = drb * qv;
-        io.ow[pt] = drb * qw;
     }
 }
 
@@ -406,5 +404,10 @@ __global__ __launch_bounds__(256) void k_adj_gather(StepArgs A, AdjIO io)
         io.av[pt] = av;
         io.aw[pt] = aw;
         io.ap[pt] = ap;
+        // adjoints of f_old (this substep's q took rb dt f_old)
+        const double drb = A.dt * A.rb;
+        io.ou[pt] = drb * qu;
+        io.ov[pt] = drb * qv;
+        io.ow[pt] = drb * qw;
     }
 }

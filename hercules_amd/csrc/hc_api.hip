@@ -1467,9 +1467,11 @@ static int project_t(hc_ctx *c)
     size_t plane = (size_t)c->nx * c->ny;
     hipLaunchKernelGGL(k_zero_level<R>, dim3((unsigned)((plane + 255) / 256)), dim3(256), 0, c->stream, V.w,
                        (size_t)(c->nz - 1) * plane, plane);
-    if (int e = launch_r2c<R>(c, V.u, V.s[0])) return e;
-    if (int e = launch_r2c<R>(c, V.v, V.s[1])) return e;
-    if (int e = launch_r2c<R>(c, V.w, V.s[2])) return e;
+    {
+        const R *src3[3] = {V.u, V.v, V.w};
+        cplx<R> *dst3[3] = {V.s[0], V.s[1], V.s[2]};
+        if (int e = launch_r2c_n<R>(c, 3, src3, dst3)) return e;
+    }
     if (int e = launch_yfft(c, -1, 3, V.s[0], V.s[1], V.s[2], nullptr)) return e;
     StepArgs A = base_args(c);
     A.rc = 1.0;

@@ -307,6 +307,7 @@ int launch_yfft(hc_ctx *c, int sign, int nf, C *f0, typename nd<C>::type *f1, ty
                 typename nd<C>::type *f3);
 template <class R> int launch_c2r(hc_ctx *c, int nf, C2RArgsT<R> &X);
 template <class R> int launch_r2c(hc_ctx *c, const R *src, cplx<R> *dst);
+template <class R> int launch_r2c_n(hc_ctx *c, int nf, const R *const *src, cplx<R> *const *dst);
 
 // Momentum right-hand side of one substep with the x R2C of qu, qv, qw
 // (hc_rhs.hip): Rayleigh damping means first when isdamp = 1, then the

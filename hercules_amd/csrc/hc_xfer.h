@@ -44,6 +44,12 @@ struct C2RArgsT {
     int kdamp0;
 };
 using C2RArgs = C2RArgsT<double>;
+// fields of one batched x R2C launch (blockIdx.z selects the field)
+template <class R>
+struct R2CFields {
+    const R *src[4];
+    cplx<R> *dst[4];
+};
 // points per field of one inverse-x block
 constexpr int kC2RPoints = 1024;
 template <int NX, int F, int PTS>

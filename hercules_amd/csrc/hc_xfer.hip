@@ -6,8 +6,8 @@
 
 // Plain x R2C of one real field (standalone solve): src -> s0, scaled by norm.
 template <int NX, class R>
-__global__ __launch_bounds__(kRhsThreads) void k_x_r2c(const R *__restrict__ src, cplx<R> *dst, int ny,
-                                                       const cplx<R> *tw_nl, const cplx<R> *tw_nx, R norm)
+__global__ __launch_bounds__(kRhsThreads) void k_x_r2c(R2CFields<R> F, int ny, const cplx<R> *tw_nl,
+                                                       const cplx<R> *tw_nx, R norm)
 {
     using C = cplx<R>;
     constexpr int ROWS = rows_per_block<NX>();
@@ -15,6 +15,8 @@ __global__ __launch_bounds__(kRhsThreads) void k_x_r2c(const R *__restrict__ src
     __shared__ C sq_[ROWS * NL];
     R *sqd = reinterpret_cast<R *>(sq_);
     const int k = blockIdx.y, j0 = blockIdx.x * ROWS, tid = threadIdx.x;
+    const R *__restrict__ src = F.src[blockIdx.z];  // field blockIdx.z
+    C *dst = F.dst[blockIdx.z];
     for (int p = tid; p < ROWS * NX; p += kRhsThreads) {
         int jj = p / NX, i = p - jj * NX, j = j0 + jj;
         sqd[jj * NX + i] = j < ny ? src[((size_t)k * ny + j) * NX + i] : R(0.0);
@@ -229,18 +231,32 @@ int launch_c2r(hc_ctx *c, int nf, C2RArgsT<R> &X)
     return 0;
 }
 
+// x R2C of nf (1..4) fields in one launch
 template <class R>
-int launch_r2c(hc_ctx *c, const R *src, cplx<R> *dst)
+int launch_r2c_n(hc_ctx *c, int nf, const R *const *src, cplx<R> *const *dst)
 {
+    if (nf < 1 || nf > 4) return set_err(HC_ERR_ARG, "launch_r2c_n: %d fields", nf);
     const View<R> V = view<R>(c);
+    R2CFields<R> F;
+    for (int q = 0; q < 4; ++q) {
+        F.src[q] = q < nf ? src[q] : nullptr;
+        F.dst[q] = q < nf ? dst[q] : nullptr;
+    }
     int rows = rows_for(c->nx);
-    dim3 g((c->ny + rows - 1) / rows, c->nz);
+    dim3 g((c->ny + rows - 1) / rows, c->nz, nf);
     R norm = R(1.0 / ((double)c->nx * (double)c->ny));
-#define HC_R(NX_) hipLaunchKernelGGL((k_x_r2c<NX_, R>), g, dim3(kRhsThreads), 0, c->stream, src, dst, c->ny, \
+#define HC_R(NX_) hipLaunchKernelGGL((k_x_r2c<NX_, R>), g, dim3(kRhsThreads), 0, c->stream, F, c->ny, \
                                      (const cplx<R> *)V.tw_nl, (const cplx<R> *)V.tw_nx, norm);
     HC_DISPATCH_N(c->nx, HC_R)
 #undef HC_R
     return 0;
+}
+template <class R>
+int launch_r2c(hc_ctx *c, const R *src, cplx<R> *dst)
+{
+    const R *s1[1] = {src};
+    cplx<R> *d1[1] = {dst};
+    return launch_r2c_n<R>(c, 1, s1, d1);
 }
 
 template int launch_yfft<double2>(hc_ctx *, int, int, double2 *, double2 *, double2 *, double2 *);
@@ -248,4 +264,6 @@ template int launch_yfft<float2>(hc_ctx *, int, int, float2 *, float2 *, float2 
 template int launch_c2r<double>(hc_ctx *, int, C2RArgsT<double> &);
 template int launch_c2r<float>(hc_ctx *, int, C2RArgsT<float> &);
 template int launch_r2c<double>(hc_ctx *, const double *, double2 *);
+template int launch_r2c_n<double>(hc_ctx *, int, const double *const *, double2 *const *);
+template int launch_r2c_n<float>(hc_ctx *, int, const float *const *, float2 *const *);
 template int launch_r2c<float>(hc_ctx *, const float *, float2 *);
