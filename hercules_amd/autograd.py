@@ -6,20 +6,47 @@ started from the velocity fields ``u0, v0, w0`` (torch tensors of shape
 (nz, ny, nx)) as a differentiable scalar.  Its backward is the library's
 hand-written adjoint (hc_wall_shear_gradient: checkpointed forward run,
 recomputation per segment, transposed substeps); the channel's pressure and
-other fields are taken as they are and not differentiated.
+other fields are taken as they are and not differentiated.  Tensors on the
+GPU stay there: they are copied device-to-device into the channel and the
+gradients are written into new device tensors.
 """
 from __future__ import annotations
 
 import torch
 
 
+def _to_channel(channel, name, t):
+    """A (nz, ny, nx) tensor into the channel's field: device tensors are
+    copied device-to-device on the library's stream after torch's stream has
+    produced them, host tensors through numpy."""
+    t = t.detach().to(torch.float64).contiguous()
+    if t.is_cuda:
+        torch.cuda.current_stream(t.device).synchronize()
+        channel.set_field_from(name, t.data_ptr())
+    else:
+        channel.set_field(name, t.numpy())
+
+
+def _adjoint(channel, name, like):
+    """The adjoint field `name` as a tensor on like's device (written in
+    place by the library when that device is a GPU)."""
+    if like.is_cuda:
+        g = torch.empty(like.shape, dtype=torch.float64, device=like.device)
+        channel.get_adjoint_into(name, g.data_ptr())  # returns after its stream finished
+        return g.to(like.dtype)
+    return torch.from_numpy(channel.get_adjoint(name)).to(like.dtype)
+
+
 class _WallShear(torch.autograd.Function):
+    """The forward runs the checkpointed gradient computation at once (the
+    adjoint needs the forward's checkpoints), so backward only scales it."""
+
     @staticmethod
     def forward(ctx, u0, v0, w0, channel, nsteps, ckpt_every):
         for name, t in (("u", u0), ("v", v0), ("w", w0)):
-            channel.set_field(name, t.detach().to("cpu", torch.float64).numpy())
+            _to_channel(channel, name, t)
         loss = channel.wall_shear_gradient(nsteps, ckpt_every)
-        grads = [torch.from_numpy(channel.get_adjoint(n)).to(u0.device, u0.dtype) for n in ("u", "v", "w")]
+        grads = [_adjoint(channel, n, u0) for n in ("u", "v", "w")]
         ctx.save_for_backward(*grads)
         return torch.tensor(loss, dtype=u0.dtype, device=u0.device)
 

@@ -267,6 +267,15 @@ class Channel:
             raise ValueError(f"{name}: shape {a.shape} != {shape}")
         self._check(self.L.hc_set_adjoint(self.h, FIELDS[name], _ptr(a), layout))
 
+    def set_field_from(self, name, ptr, layout=LAYOUT_INTERIOR):
+        """hc_set_field from raw memory (host or device address of a float64
+        array of the layout's shape)."""
+        self._check(self.L.hc_set_field(self.h, FIELDS[name], C.c_void_p(int(ptr)), layout))
+
+    def get_adjoint_into(self, name, ptr, layout=LAYOUT_INTERIOR):
+        """hc_get_adjoint into raw memory (host or device address)."""
+        self._check(self.L.hc_get_adjoint(self.h, FIELDS[name], C.c_void_p(int(ptr)), layout))
+
     def get_adjoint(self, name, layout=LAYOUT_INTERIOR):
         shape = self.shape if layout == LAYOUT_INTERIOR else self.ghost_shape
         a = np.zeros(shape)

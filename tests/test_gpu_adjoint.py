@@ -181,6 +181,29 @@ def test_torch_autograd_binding_returns_the_adjoint_gradient():
     ref.close()
 
 
+def test_torch_binding_keeps_device_tensors_on_the_device():
+    """The same gradient from tensors on the GPU (device-to-device transfers)
+    as from host tensors, bit for bit; the gradients come back on the GPU."""
+    torch = pytest.importorskip("torch")
+    from hercules_amd.autograd import wall_shear_loss
+    prm = _params("closed", nx=32, ny=32, nz=33)
+    x = _state(prm, seed=10)
+    out = {}
+    for dev in ("cpu", "cuda"):
+        ch = Channel(prm)
+        _set(ch, {k: x[k] for k in ("u", "v", "w", "p")})
+        ch.init()
+        ts = [torch.tensor(x[k], device=dev, requires_grad=True) for k in ("u", "v", "w")]
+        loss = wall_shear_loss(ch, *ts, nsteps=3, ckpt_every=2)
+        loss.backward()
+        assert all(t.grad.device.type == dev for t in ts)
+        out[dev] = (float(loss), [t.grad.cpu().numpy() for t in ts])
+        ch.close()
+    assert out["cpu"][0] == out["cuda"][0]
+    for a, b in zip(out["cpu"][1], out["cuda"][1]):
+        assert np.array_equal(a, b)
+
+
 @pytest.mark.parametrize("kw", [dict(isscalar=1), dict(cds=3, issk=1), dict(precision=1), dict(dp_opt=2)])
 def test_adjoint_rejects_unsupported(kw):
     prm = cases.closed_channel(nx=32, ny=32, nz=33, **kw)
