@@ -734,9 +734,10 @@ extern "C" int hc_create(const hc_params *prm, hc_ctx **out)
         if (P.isdamp == 1) return set_err(HC_ERR_UNSUPPORTED, "zorder=%d does not cover Rayleigh damping", P.zorder);
         if (P.nz < 16) return set_err(HC_ERR_UNSUPPORTED, "zorder=%d needs nz >= 16", P.zorder);
     }
-    if (!fft_len_ok(P.nx) || !fft_len_ok(P.ny)) return set_err(HC_ERR_UNSUPPORTED, "nx, ny must be one of 16,32,64,128,192,256,512");
+    if (!fft_len_ok(P.nx) || !fft_len_ok(P.ny))
+        return set_err(HC_ERR_UNSUPPORTED, "nx, ny must be one of 16,32,48,64,96,128,192,256,384,512");
     {
-        int ty = P.nx == 16 ? 16 : P.nx == 32 ? 32 : P.nx == 64 ? 16 : P.nx == 128 ? 8 : P.nx == 192 ? 4 : P.nx == 256 ? 4 : 1;
+        const int ty = rhs_tile_rows(P.nx);
         if (P.ny % ty != 0) return set_err(HC_ERR_UNSUPPORTED, "ny=%d must be a multiple of %d for nx=%d", P.ny, ty, P.nx);
     }
     int L = col_levels(P.nz);

@@ -179,7 +179,17 @@ inline void prof_mark(hc_ctx *c, int kid, bool begin)
 #define PROF_BEGIN(c, kid) prof_mark(c, kid, true)
 #define PROF_END(c, kid) prof_mark(c, kid, false)
 
-inline bool fft_len_ok(int n) { return n == 16 || n == 32 || n == 64 || n == 128 || n == 192 || n == 256 || n == 512; }
+inline bool fft_len_ok(int n)
+{
+    return n == 16 || n == 32 || n == 48 || n == 64 || n == 96 || n == 128 || n == 192 || n == 256 || n == 384 ||
+           n == 512;
+}
+// rows of a right-hand-side tile for nx (rhs2_ty for nx <= 256, rhs_ty above):
+// ny must be a multiple of it
+inline int rhs_tile_rows(int nx)
+{
+    return nx == 16 ? 16 : nx == 32 ? 32 : nx == 48 ? 16 : nx == 64 ? 16 : nx == 96 ? 8 : nx == 128 ? 8 : nx <= 256 ? 4 : 1;
+}
 
 inline int col_levels(int nz)
 {
@@ -264,10 +274,13 @@ inline StepArgsT<double> rows_args<double>(const StepArgs &A, const View<double>
     switch (n) {                                                                          \
     case 16: MACRO(16); break;                                                            \
     case 32: MACRO(32); break;                                                            \
+    case 48: MACRO(48); break;                                                            \
     case 64: MACRO(64); break;                                                            \
+    case 96: MACRO(96); break;                                                            \
     case 128: MACRO(128); break;                                                          \
     case 192: MACRO(192); break;                                                          \
     case 256: MACRO(256); break;                                                          \
+    case 384: MACRO(384); break;                                                          \
     case 512: MACRO(512); break;                                                          \
     default: return set_err(HC_ERR_UNSUPPORTED, "unsupported transform length %d", n);   \
     }

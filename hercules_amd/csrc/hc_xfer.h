@@ -18,7 +18,7 @@ constexpr int rows_per_block() { return NX >= kRowPoints ? 1 : kRowPoints / NX; 
 // point's right-hand side from those rows (momentum_tile).  The staging area
 // then holds the q rows for the x FFT.
 template <int NX>
-constexpr int rhs2_ty() { return NX == 16 ? 16 : NX == 32 ? 32 : NX == 64 ? 16 : NX == 128 ? 8 : 4; }
+constexpr int rhs2_ty() { return NX == 16 ? 16 : NX == 32 ? 32 : NX == 48 ? 16 : NX == 64 ? 16 : NX == 96 ? 8 : NX == 128 ? 8 : 4; }
 template <int NX>
 constexpr int rhs2_threads() { return NX * rhs2_ty<NX>(); }
 // waves per SIMD the register allocation must allow: with FP32 tiles two

@@ -26,7 +26,7 @@ def _oracle(prm, state=None, init=True):
 
 
 # ---------------------------------------------------------------- FFT (A11)
-@pytest.mark.parametrize("n", [8, 12, 16, 48, 64, 96, 192, 256])
+@pytest.mark.parametrize("n", [8, 12, 16, 24, 48, 64, 96, 192, 256, 384])
 def test_fft_matches_numpy(n):
     x = RNG.standard_normal(n) + 1j * RNG.standard_normal(n)
     assert np.abs(fft1d(x, -1) - np.fft.fft(x)).max() < 1e-13 * np.abs(x).sum()
