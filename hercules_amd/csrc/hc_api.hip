@@ -480,23 +480,26 @@ static bool build_zops(hc_ctx *c)
     // C2 on the levels (data index = level + 1), F2 on the interior faces
     // (data index = w level + 1, nz - 1 unknowns), P on the levels.
     c->h_zband.assign((size_t)ZB_COUNT * nz * kBandRow, 0.0);
-    const int bops[ZB_COUNT] = {ZO_C2_DD, ZO_C2_DN, ZO_F2, ZO_P, ZO_G, ZO_D};
+    const int bops[ZB_COUNT] = {ZO_C2_DD, ZO_C2_DN, ZO_F2, ZO_P, ZO_G, ZO_D, ZO_I_DD, ZO_I_DN, ZO_IFC};
     // half bands the solver is instantiated for (5 and 7); the explicit G and D
-    // are applied over the full stored band
-    const int bhalf[ZB_COUNT] = {5, 5, 5, 7, kBandMax, kBandMax};
+    // are applied over the full stored band, the interpolations over kIntHW
+    const int bhalf[ZB_COUNT] = {5, 5, 5, 7, kBandMax, kBandMax, kIntHW, kIntHW, kIntHW};
     for (int b = 0; b < ZB_COUNT; ++b) {
         const int op = bops[b];
+        const bool interp = op == ZO_I_DD || op == ZO_I_DN || op == ZO_IFC;
         // rows, the columns' range and the data index of column 0
         const int nrow = (op == ZO_F2 || op == ZO_G) ? nz - 1 : nz;
-        const int ncol = (op == ZO_F2 || op == ZO_D) ? nz - 1 : nz;
+        const int ncol = (op == ZO_F2 || op == ZO_D || op == ZO_IFC) ? nz - 1 : nz;
         const int off = (op == ZO_P || op == ZO_G) ? 0 : 1;
-        const int W = op == ZO_P ? kZW : (op == ZO_G || op == ZO_D) ? n1 : n2;
+        const int W = op == ZO_P ? kZW : (op == ZO_G || op == ZO_D || interp) ? n1 : n2;
+        // the interpolations keep the wall data (columns -1 and ncol)
+        const int lo = interp ? -1 : 0, hi = interp ? ncol : ncol - 1;
         for (int k = 0; k < nrow; ++k) {
             const size_t e = (size_t)op * nz + k;
             for (int j = 0; j < W; ++j) {
                 const double w = c->h_zwt[e * kZW + j];
                 const int lv = c->h_zst[e] + j - off;
-                if (w == 0.0 || lv < 0 || lv >= ncol) continue;
+                if (w == 0.0 || lv < lo || lv > hi) continue;
                 const int d = lv - k;
                 if (d < -bhalf[b] || d > bhalf[b]) return false;
                 c->h_zband[((size_t)b * nz + k) * kBandRow + kBandMax + d] += w;

@@ -65,8 +65,12 @@ constexpr int kZW = 16;
 constexpr int kBandMax = 7;
 constexpr int kBandRow = 2 * kBandMax + 1;
 // ZB_G: faces kw = 0..nz-2 from the levels; ZB_D: levels from the interior
-// faces (w levels 0..nz-2, the wall faces being zero).
-enum { ZB_C2_DD = 0, ZB_C2_DN, ZB_F2, ZB_P, ZB_G, ZB_D, ZB_COUNT };
+// faces (w levels 0..nz-2, the wall faces being zero).  ZB_IU_DD, ZB_IU_DN
+// (ZO_I_DD, ZO_I_DN: faces kw = 0..nz-1 from the levels, the wall data at
+// levels -1 and nz kept) and ZB_IFC (ZO_IFC: levels from the faces) are the
+// explicit interpolations, within kIntHW of the diagonal.
+enum { ZB_C2_DD = 0, ZB_C2_DN, ZB_F2, ZB_P, ZB_G, ZB_D, ZB_IU_DD, ZB_IU_DN, ZB_IFC, ZB_COUNT };
+constexpr int kIntHW = 5;
 
 
 // Arguments of the step kernels.  R is the storage and arithmetic type of

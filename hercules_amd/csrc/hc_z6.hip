@@ -14,7 +14,8 @@ void zinterp6(hc_ctx *c, StepArgsT<R> &B)
     I.zfu = I.zfv = I.zcw = I.zft = nullptr;  // the accessors evaluate the stencils
     R *b[4];
     for (int q = 0; q < 4; ++q) b[q] = reinterpret_cast<R *>(c->zbuf[q]);
-    hipLaunchKernelGGL((k_zinterp6<R>), dim3(2048), dim3(256), 0, c->stream, I, b[0], b[1], b[2], b[3]);
+    const dim3 grid((unsigned)(((size_t)B.nx * B.ny + 255) / 256), 4, (unsigned)stream_chunks(B.nz));
+    hipLaunchKernelGGL((k_zinterp6<R>), grid, dim3(256), 0, c->stream, I, b[0], b[1], b[2], b[3]);
     B.zfu = b[0]; B.zfv = b[1]; B.zcw = b[2]; B.zft = b[3];
 }
 template void zinterp6<double>(hc_ctx *, StepArgsT<double> &);

@@ -21,18 +21,78 @@
 // The 6th-order interpolants of the substep's fields, once per point, for
 // the right-hand-side kernels: u, v at the w face of level k (the top wall
 // face included), w at the centre of level k, t at the interior w face k.
-// A carries no interpolant buffers, so the accessor evaluates them (in the
-// stored type R).
+// One thread per (i, j) column, field (blockIdx.y: u, v, w, t) and z chunk
+// (blockIdx.z of gridDim.z) streams along z with a window of kIntWin levels,
+// k-kIntHW .. k+kIntHW around the row k being formed, and applies the band
+// rows ZB_IU_* and ZB_IFC (block-uniform values).  The window holds the wall
+// data of the C convention at levels -1 and nz (u: -u_mrf, and -u_mrf or
+// du/dz = 0 at the top; v: 0; t: tbot, and ttop or the wall gradient) and
+// zero beyond; w is zero outside its interior faces 0..nz-2.  The band rows
+// list the stencil weights in the stencils' own order with zeros between
+// them, so each value equals the accessor's sum term for term, in the stored
+// type R.
+constexpr int kIntWin = 2 * kIntHW + 1;
 template <class R>
 __global__ __launch_bounds__(256) void k_zinterp6(StepArgsT<R> A, R *fu, R *fv, R *cw, R *ft)
 {
-    const size_t n = (size_t)A.nx * A.ny * A.nz, plane = (size_t)A.nx * A.ny;
-    for (size_t q = blockIdx.x * (size_t)blockDim.x + threadIdx.x; q < n; q += (size_t)gridDim.x * blockDim.x) {
-        const int k = (int)(q / plane), j = (int)((q - (size_t)k * plane) / A.nx), i = (int)(q % A.nx);
-        const PtT<R> g(A, i, j, k);
-        fu[q] = g.uf6(k);
-        fv[q] = g.vf6(k);
-        cw[q] = g.whz6(k);
-        if (ft) ft[q] = k < A.nz - 1 ? g.thz6(0) : R(0.0);
+    constexpr int PF = 8;
+    const int nz = A.nz, f = (int)blockIdx.y;
+    const size_t plane = (size_t)A.nx * A.ny;
+    const size_t col = blockIdx.x * (size_t)256 + threadIdx.x;
+    if (col >= plane || (f == 3 && !ft)) return;
+    const int csz = (nz + (int)gridDim.z - 1) / (int)gridDim.z;
+    const int kb = (int)blockIdx.z * csz, ke = kb + csz < nz ? kb + csz : nz;
+    const int ib = kb - kIntHW;
+    const R zero = R(0.0);
+    // the field's data, its stored levels lo .. nlev-1, wall data at levels
+    // -1 and nz, band and output
+    const R *src;
+    R *dst, wb = zero, wt = zero;
+    int nlev = nz, band = A.ochannel ? ZB_IU_DN : ZB_IU_DD;
+    if (f == 0) {
+        src = A.u; dst = fu;
+        wb = R(-A.u_mrf);
+        wt = A.ochannel == 0 ? R(-A.u_mrf) : zero;
+    } else if (f == 1) {
+        src = A.v; dst = fv;
+    } else if (f == 2) {
+        src = A.w; dst = cw; nlev = nz - 1; band = ZB_IFC;
+    } else {
+        src = A.t; dst = ft;
+        wb = A.tbot;
+        wt = A.tbc_top ? R(A.gdz_top * A.lev[LV_IDZT * nz + nz - 1]) : A.ttop;
+        band = A.tbc_top ? ZB_IU_DN : ZB_IU_DD;
+    }
+    if (f == 2) wb = wt = zero;  // w: zero at both wall faces
+    // level li (block-uniform: the selection branches on scalars)
+    auto load = [&](int li) __attribute__((always_inline)) {
+        return (li >= 0 && li < nlev) ? src[(size_t)li * plane + col] : (li == -1 ? wb : (li == nz ? wt : zero));
+    };
+    R win[kIntWin];
+    R ring[PF];
+#pragma unroll
+    for (int t = 0; t < kIntWin; ++t) win[t] = zero;
+#pragma unroll
+    for (int u = 0; u < PF; ++u) ring[u] = load(ib + u);
+    // iteration i: level i enters the window; row k = i - kIntHW is formed
+    for (int i0 = ib; i0 < ke + kIntHW; i0 += PF) {
+#pragma unroll
+        for (int u = 0; u < PF; ++u) {
+            const int i = i0 + u;
+#pragma unroll
+            for (int t = 0; t < kIntWin - 1; ++t) win[t] = win[t + 1];
+            win[kIntWin - 1] = ring[u];
+            ring[u] = load(i + PF);
+            const int k = i - kIntHW;
+            if (k >= kb && k < ke) {
+                R acc = zero;
+                if (f != 3 || k < nz - 1) {
+                    const double *b = A.zband + ((size_t)band * nz + k) * kBandRow + kBandMax - kIntHW;
+#pragma unroll
+                    for (int t = 0; t < kIntWin; ++t) acc += R(b[t]) * win[t];
+                }
+                dst[(size_t)k * plane + col] = acc;
+            }
+        }
     }
 }
