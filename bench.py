@@ -487,7 +487,7 @@ def main():
     ap.add_argument("--zorder", type=int, default=2, choices=[2, 4, 6],
                     help="order of the z operators (4, 6: the high-order extension; fd, fd_t, sp, poisson)")
     ap.add_argument("--launch-shapes", default=None,
-                    help="fixed launch shapes instead of autotuning, e.g. rgroup=16,ytile=8,c2r_pts=1024,col_remap=1 "
+                    help="fixed launch shapes instead of autotuning, e.g. rgroup=16,ytile=8,c2r_pts=1024,col_remap=1,rhs_xt=2 "
                          "(the config.launch_shapes an autotuned run printed; fd, fd_t, sp)")
     ap.add_argument("--cpu-steps", type=int, default=1, help="oracle steps for the CPU baseline")
     ap.add_argument("--no-cpu-baseline", action="store_true")
@@ -507,7 +507,7 @@ def main():
             args.launch_shapes = {k: int(v) for k, v in (kv.split("=") for kv in args.launch_shapes.split(","))}
         except ValueError:
             ap.error("--launch-shapes takes knob=value pairs separated by commas")
-        unknown = set(args.launch_shapes) - {"rgroup", "ytile", "c2r_pts", "col_remap"}
+        unknown = set(args.launch_shapes) - {"rgroup", "ytile", "c2r_pts", "col_remap", "rhs_xt"}
         if unknown:
             ap.error(f"unknown launch-shape knobs {sorted(unknown)}")
     if args.gpus != 1:

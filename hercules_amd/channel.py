@@ -248,10 +248,10 @@ class Channel:
         """Pick the fastest bit-identical launch shapes for this channel
         (hc_autotune); returns the choices."""
         self._check(self.L.hc_autotune(self.h, int(steps_per_trial)))
-        return {k: int(self.scalar("tune_" + k)) for k in ("rgroup", "ytile", "c2r_pts", "col_remap")}
+        return {k: int(self.scalar("tune_" + k)) for k in ("rgroup", "ytile", "c2r_pts", "col_remap", "rhs_xt")}
 
-    def set_tuning(self, rgroup=None, ytile=None, c2r_pts=None, col_remap=None):
-        for knob, v in enumerate((rgroup, ytile, c2r_pts, col_remap)):
+    def set_tuning(self, rgroup=None, ytile=None, c2r_pts=None, col_remap=None, rhs_xt=None):
+        for knob, v in enumerate((rgroup, ytile, c2r_pts, col_remap, rhs_xt)):
             if v is not None:
                 self._check(self.L.hc_set_tuning(self.h, knob, int(v)))
 
@@ -475,7 +475,7 @@ class Channel:
         v = C.c_double()
         ids = {"dpx_drive": 0, "ubar": 1, "time_sim": 2, "got": 3, "stps": 4, "col_tiles": 5,
                "tune_rgroup": 6, "tune_ytile": 7, "tune_c2r_pts": 8, "tune_col_remap": 9,
-               "mean_steps": 10}
+               "mean_steps": 10, "tune_rhs_xt": 11}
         self._check(self.L.hc_get_scalar(self.h, ids[name], C.byref(v)))
         return v.value
 

@@ -591,6 +591,7 @@ StepArgs base_args(hc_ctx *c)
     A.idx = 1.0 / c->dx; A.idy = 1.0 / c->dy; A.idx2 = 1.0 / (c->dx * c->dx); A.idy2 = 1.0 / (c->dy * c->dy);
     A.idt = 1.0 / P.dt;
     A.dmp_s = c->dmp_s; A.dmp_m = c->dmp_m; A.kdamp0 = c->kdamp0;
+    A.klo = 0; A.nlev = c->nz;
     A.cu = c->cu; A.cv = c->cv; A.cw = c->cw; A.cp = c->cp; A.ct = c->ct;
     A.cff = c->cff; A.cgg = c->cgg; A.chh = c->chh; A.css = c->css;
     for (int q = 0; q < 8; ++q) A.sp[q] = c->sp[q];
@@ -1417,6 +1418,7 @@ extern "C" int hc_get_scalar(hc_ctx *c, int id, double *value)
     case 8: *value = c->tune.c2r_pts; return 0;
     case 9: *value = c->tune.col_remap; return 0;
     case 10: *value = (double)c->mean_steps; return 0;
+    case 11: *value = c->tune.rhs_xt; return 0;
     default: return set_err(HC_ERR_ARG, "unknown scalar %d", id);
     }
 }
@@ -1930,16 +1932,20 @@ extern "C" int hc_autotune(hc_ctx *c, int steps_per_trial)
                    P.nx == 256 ? 4 : 1;
     const int nty = c->ny / ty;
     std::vector<int> groups = {0};
-    if (P.nx <= 256 && P.isdamp == 0)
+    if (P.nx <= 256)
         for (int g : {1, 2, 4, 8, 16, 32})
             if (nty % g == 0 && g <= nty) groups.push_back(g);
     std::vector<int> ytiles = {0, kYTile};
     if (c->nl % (2 * kYTile) == 0) ytiles.push_back(2 * kYTile);
-    int *knob[4] = {&c->tune.rgroup, &c->tune.ytile, &c->tune.c2r_pts, &c->tune.col_remap};
-    std::vector<int> cand[4] = {groups, ytiles, {1024, 512, 2048}, {0, 1}};
+    std::vector<int> xts = {0};
+    if (P.zorder == 2)
+        for (int sh = 1; sh <= 4; ++sh)
+            if (rhs_xt_fits(sh, c->nx, c->ny)) xts.push_back(sh);
+    int *knob[5] = {&c->tune.rgroup, &c->tune.ytile, &c->tune.c2r_pts, &c->tune.col_remap, &c->tune.rhs_xt};
+    std::vector<int> cand[5] = {groups, ytiles, {1024, 512, 2048}, {0, 1}, xts};
     float best = 0.0f;
     if (he == hipSuccess) err = trial(best);
-    for (int kb = 0; kb < 4 && !err && he == hipSuccess; ++kb) {
+    for (int kb = 0; kb < 5 && !err && he == hipSuccess; ++kb) {
         const int keep = *knob[kb];
         int chosen = keep;
         for (int v : cand[kb]) {
@@ -1962,7 +1968,8 @@ extern "C" int hc_autotune(hc_ctx *c, int steps_per_trial)
 }
 
 // Set one launch shape directly (knob 0 RHS tile-order group, 1 y-transform
-// columns per block, 2 inverse-x points per block, 3 column block order),
+// columns per block, 2 inverse-x points per block, 3 column block order,
+// 4 momentum tile shape),
 // e.g. to reuse a previous hc_autotune choice.
 extern "C" int hc_set_tuning(hc_ctx *c, int knob, int value)
 {
@@ -1991,6 +1998,12 @@ extern "C" int hc_set_tuning(hc_ctx *c, int knob, int value)
     case 3:
         if (value != 0 && value != 1) return set_err(HC_ERR_ARG, "column block order must be 0 or 1");
         c->tune.col_remap = value;
+        return 0;
+    case 4:
+        if (value != 0 && !rhs_xt_fits(value, c->nx, c->ny))
+            return set_err(HC_ERR_ARG, "momentum tile shape %d must be 0, or 1-4 with tiles dividing %dx%d", value,
+                           c->nx, c->ny);
+        c->tune.rhs_xt = value;
         return 0;
     default:
         return set_err(HC_ERR_ARG, "unknown launch-shape knob %d", knob);

@@ -69,6 +69,8 @@ struct hc_ctx {
         int ytile = 0;      // y transform columns per block (0: 8 for FP64, 16 for FP32)
         int c2r_pts = 1024; // inverse-x points per field per block
         int col_remap = 0;  // XCD-contiguous column block order
+        int rhs_xt = 0;     // FD momentum right-hand side: 0 whole rows with the x R2C fused, 1-4 x-y tiles
+                            // (32x8, 64x8, 32x16, 64x4) and a separate in-place R2C
     } tune;
     int lo_l = 0, lo_m = 0;
     int nx, ny, nz, nl, Lcol;
@@ -251,6 +253,7 @@ inline StepArgsT<R> rows_args(const StepArgs &A, const View<R> &V)
     static_assert(sizeof sc / sizeof sc[0] == sizeof dst / sizeof dst[0], "scalar lists differ");
     for (size_t q = 0; q < sizeof sc / sizeof sc[0]; ++q) *dst[q] = R(sc[q]);
     B.kdamp0 = A.kdamp0; B.dmask = A.dmask; B.tbc_top = A.tbc_top;
+    B.klo = A.klo; B.nlev = A.nlev;
     B.issk = A.issk; B.lc = A.lc; B.mc = A.mc;
     B.ntile = A.ntile;
     B.rgroup = A.rgroup;
@@ -320,7 +323,10 @@ int launch_yfft(hc_ctx *c, int sign, int nf, C *f0, typename nd<C>::type *f1, ty
                 typename nd<C>::type *f3);
 template <class R> int launch_c2r(hc_ctx *c, int nf, C2RArgsT<R> &X);
 template <class R> int launch_r2c(hc_ctx *c, const R *src, cplx<R> *dst);
-template <class R> int launch_r2c_n(hc_ctx *c, int nf, const R *const *src, cplx<R> *const *dst);
+template <class R>
+int launch_r2c_n(hc_ctx *c, int nf, const R *const *src, cplx<R> *const *dst, int k0 = 0, int nk = -1);
+// whether tile shape 1-4 of the x-y tiled momentum right-hand side divides the grid
+bool rhs_xt_fits(int shape, int nx, int ny);
 
 // Momentum right-hand side of one substep with the x R2C of qu, qv, qw
 // (hc_rhs.hip): Rayleigh damping means first when isdamp = 1, then the

@@ -107,6 +107,10 @@ struct StepArgsT {
     // fields an accessor reads damped (bit f for field f).
     const R *dmp_s, *dmp_m;
     int kdamp0, dmask;
+    // levels klo .. klo+nlev-1 of a right-hand-side launch: all levels, or in
+    // a damped substep the levels below the sponge (the tiled kernels) and
+    // the levels that read a damped value (the accessor kernels)
+    int klo, nlev;
     // spectral path (cds = 3): state spectra, product spectra, y-inverted
     // derivative spectra of the skew-symmetric terms, and the 2/3-rule limits
     Cx *cu, *cv, *cw, *cp, *ct, *cff, *cgg, *chh, *css;
@@ -1043,6 +1047,19 @@ HC_DEV R ixp(const R *row, int i)
     if (CDS == 1) return R(0.5) * (row[wrapx<NX>(i + 1)] + row[wrapx<NX>(i)]);
     return (-row[wrapx<NX>(i + 2)] + R(9.0) * row[wrapx<NX>(i + 1)] + R(9.0) * row[wrapx<NX>(i)] - row[wrapx<NX>(i - 1)]) / R(16.0);
 }
+// The same on a row indexed directly (a tile that carries its x halos).
+template <int CDS, class R>
+HC_DEV R ixm_d(const R *row, int i)
+{
+    if (CDS == 1) return R(0.5) * (row[i] + row[i - 1]);
+    return (-row[i + 1] + R(9.0) * row[i] + R(9.0) * row[i - 1] - row[i - 2]) / R(16.0);
+}
+template <int CDS, class R>
+HC_DEV R ixp_d(const R *row, int i)
+{
+    if (CDS == 1) return R(0.5) * (row[i + 1] + row[i]);
+    return (-row[i + 2] + R(9.0) * row[i + 1] + R(9.0) * row[i] - row[i - 1]) / R(16.0);
+}
 // col points at row r of the staged block, i at column i: value at row r+d is col[d*NX]
 template <int CDS, int NX, class R>
 HC_DEV R iym(const R *col)  // vhy form: between r-1 and r
@@ -1109,13 +1126,17 @@ struct PointVals2 {
 };
 
 // Phase 2 of the two-phase tile at point (i, j0+jj): ff, gg, hh and
-// qu, qv, qw.  Same terms and coefficients as momentum_point.
-template <int CDS, int NX, int TY, class R>
+// qu, qv, qw.  Same terms and coefficients as momentum_point.  The LDS rows
+// have NX entries; with WRAP they are whole periodic x rows and i is the
+// global x index, otherwise they hold a tile with its x halos and i is the
+// point's index within the row.
+template <int CDS, int NX, int TY, class R, bool WRAP = true>
 HC_DEV void momentum_tile(const StepArgsT<R> &A, const R *sm, int i, int jj, int k, const PointVals2<R> &pv, R dpx,
                           R &f, R &g, R &h, R &qu, R &qv, R &qw)
 {
     using Tl = Tile2<NX, TY>;
-    auto RW = [&](int base, int r, int di) -> R { return sm[(base + r) * NX + wrapx<NX>(i + di)]; };
+    auto X = [](int x) { if constexpr (WRAP) return wrapx<NX>(x); else return x; };
+    auto RW = [&](int base, int r, int di) -> R { return sm[(base + r) * NX + X(i + di)]; };
     auto U = [&](int di, int dj) { return RW(Tl::OU, jj + 3 + dj, di); };
     auto V = [&](int di, int dj) { return RW(Tl::OV, jj + 3 + dj, di); };
     auto W = [&](int di, int dj) { return RW(Tl::OW, jj + 3 + dj, di); };
@@ -1201,7 +1222,7 @@ HC_DEV void momentum_tile(const StepArgsT<R> &A, const R *sm, int i, int jj, int
         if (A.dp_opt == 3) {
             // uhy at (i, j) and (i-1, j), from the staged u rows
             const R *cu = sm + (Tl::OU + jj + 3) * NX;
-            const R a0 = iyp<CDS, NX>(cu + wrapx<NX>(i)), a1 = iyp<CDS, NX>(cu + wrapx<NX>(i - 1));
+            const R a0 = iyp<CDS, NX>(cu + X(i)), a1 = iyp<CDS, NX>(cu + X(i - 1));
             g = -vux - vvy - vwz - um * vx + A.fc * (A.ug - R(0.5) * (a0 + a1) - um);
         } else {
             g = -vux - vvy - vwz - um * vx;

@@ -28,8 +28,8 @@ __global__ __launch_bounds__(rhs_threads<NX>()) void k_rhs_mom(StepArgsT<R> A)
     __shared__ __align__(16) R sm[Tl::NROWS * NX];
     const int tid = threadIdx.x;
     const int nty = A.ny / TY;
-    const int logical = xcd_remap(blockIdx.x, nty * A.nz);
-    const int jt = logical % nty, k = logical / nty, j0 = jt * TY;
+    const int logical = xcd_remap(blockIdx.x, nty * A.nlev);
+    const int jt = logical % nty, k = A.klo + logical / nty, j0 = jt * TY;
     for (int q = tid; q < Tl::NROWS * NX; q += T) {
         int r = q / NX, i = q - r * NX;
         sm[q] = rhs_stage_value<NX, TY>(A, r, i, j0, k);
@@ -92,9 +92,9 @@ __global__ __launch_bounds__(rhs2_threads<NX>(), (rhs2_waves<NX, R>())) void k_r
     // across groups, so that an XCD's L2 holds both the y halos and the
     // neighbouring levels of the tiles in flight
     const int G = A.rgroup > 0 ? A.rgroup : (nty % 16 == 0) ? 16 : (nty % 8 == 0) ? 8 : (nty % 4 == 0) ? 4 : 1;
-    const int logical = xcd_remap(blockIdx.x, nty * A.nz);
-    const int grp = logical / (G * A.nz), rem = logical - grp * (G * A.nz);
-    const int k = rem / G, jt = grp * G + rem % G, j0 = jt * TY;
+    const int logical = xcd_remap(blockIdx.x, nty * A.nlev);
+    const int grp = logical / (G * A.nlev), rem = logical - grp * (G * A.nlev);
+    const int k = A.klo + rem / G, jt = grp * G + rem % G, j0 = jt * TY;
     const int i = tid % NX, jj = tid / NX, j = j0 + jj;
     const bool top = k == A.nz - 1;
     // staging: every thread issues all of its loads before the first use
@@ -196,6 +196,142 @@ __global__ __launch_bounds__(rhs2_threads<NX>(), (rhs2_waves<NX, R>())) void k_r
     }
 }
 
+// Momentum right-hand side on a TX x TY tile of level k (x tiles fastest,
+// then y tiles, then levels, in XCD-contiguous order), without the x
+// transform: the same phases, terms and wall rules as k_rhs_mom2 on LDS rows
+// of S = TX + 6 points (columns i0-3 .. i0+TX+2; a derived row holds the
+// columns that phase 2 reads).  ff, gg, hh are stored; qu, qv, qw go as
+// physical rows into the storage of their spectra (a row of nx values takes
+// the bytes of its nx/2 modes), which launch_r2c_n then transforms in place.
+// The small LDS tile lets several blocks share a CU, so that one block's
+// loads overlap another's stencils.
+template <int CDS, int TX, int TY, class R>
+__global__ __launch_bounds__(TX * TY, 4) void k_rhs_mom_xt(StepArgsT<R> A)
+{
+    constexpr int T = TX * TY, S = TX + 6, HX = 3;
+    using Tl = Tile2<S, TY>;
+    constexpr int NST = (Tl::NRAW * S + T - 1) / T;
+    constexpr int NUK = TY * (TX + 3), PU = (NUK + T - 1) / T;   // uhx/whx/fuw columns 1 .. TX+3
+    constexpr int NVK = (TY + 3) * TX, PV = (NVK + T - 1) / T;   // fwv rows j0-2 .. j0+TY, own columns
+    __shared__ __align__(16) R sm[Tl::NROWS * S];
+    const int tid = threadIdx.x, nx = A.nx, ny = A.ny;
+    const int ntx = nx / TX, nty = ny / TY;
+    const int logical = xcd_remap(blockIdx.x, ntx * nty * A.nlev);
+    const int it = logical % ntx, rem = logical / ntx, jt = rem % nty, k = A.klo + rem / nty;
+    const int i0 = it * TX, j0 = jt * TY;
+    const int il = tid % TX, jj = tid / TX;
+    const int i = i0 + il, j = j0 + jj, xl = il + HX;
+    const bool top = k == A.nz - 1;
+    auto gx = [&](int x) -> unsigned { return (unsigned)wrapi(i0 - HX + x, nx); };
+    // staging: every thread issues all of its loads before the first use
+    R stv[NST];
+#pragma unroll
+    for (int m = 0; m < NST; ++m) {
+        const int q = tid + m * T;
+        stv[m] = R(0.0);
+        if ((Tl::NRAW * S) % T == 0 || q < Tl::NRAW * S) {
+            const int r = q / S, x = q - r * S;
+            const R *f = r < Tl::OV ? A.u : r < Tl::OW ? A.v : r < Tl::OP ? A.w : A.p;
+            const int jr = r < Tl::OP ? j0 - 3 + r % Tl::RU : j0 - 1 + (r - Tl::OP);
+            stv[m] = f[((unsigned)k * (unsigned)ny + (unsigned)wrapi(jr, ny)) * (unsigned)nx + gx(x)];
+        }
+    }
+    PointVals2<R> pv;
+    pv.load(A, i, j, k);
+    // u(k+1) under the uhz*whx columns and v(k+1) under the why*vhz rows
+    R ukp[PU], vkp[PV];
+#pragma unroll
+    for (int m = 0; m < PU; ++m) {
+        const int e = tid + m * T;
+        ukp[m] = R(0.0);
+        if ((NUK % T == 0 || e < NUK) && !top) {
+            const int r = e / (TX + 3), x = 1 + e % (TX + 3);
+            ukp[m] = A.u[((unsigned)(k + 1) * (unsigned)ny + (unsigned)(j0 + r)) * (unsigned)nx + gx(x)];
+        }
+    }
+#pragma unroll
+    for (int m = 0; m < PV; ++m) {
+        const int e = tid + m * T;
+        vkp[m] = R(0.0);
+        if ((NVK % T == 0 || e < NVK) && !top) {
+            const int r = e / TX, x = e % TX;
+            vkp[m] = A.v[((unsigned)(k + 1) * (unsigned)ny + (unsigned)wrapi(j0 - 2 + r, ny)) * (unsigned)nx +
+                         (unsigned)(i0 + x)];
+        }
+    }
+#pragma unroll
+    for (int m = 0; m < NST; ++m) {
+        const int q = tid + m * T;
+        if ((Tl::NRAW * S) % T == 0 || q < Tl::NRAW * S) sm[q] = stv[m];
+    }
+    const R dpx = A.dp_opt == 2 ? R(A.scal[0]) : A.dpx;
+    const R um = A.u_mrf;
+    const R lt = top ? R(0.0) : A.lev[LV_LT * A.nz + k];
+    __syncthreads();
+    // ---- phase 1: interpolants and flux products of level k
+    for (int e = tid; e < TY * (TX + 3); e += T) {  // uhx, rows j0 .. j0+TY-1, columns 2 .. TX+4
+        const int r = e / (TX + 3), x = 2 + e % (TX + 3);
+        sm[(Tl::OUHX + r) * S + x] = ixm_d<CDS>(sm + (Tl::OU + r + 3) * S, x);
+    }
+#pragma unroll
+    for (int m = 0; m < PU; ++m) {  // whx and uhz*whx, rows j0 .. j0+TY-1, columns 1 .. TX+3
+        const int e = tid + m * T;
+        if (NUK % T == 0 || e < NUK) {
+            const int r = e / (TX + 3), x = 1 + e % (TX + 3);
+            const R whx = top ? R(0.0) : ixp_d<CDS>(sm + (Tl::OW + r + 3) * S, x);
+            const R uc = sm[(Tl::OU + r + 3) * S + x];
+            const R uhz = top ? (A.ochannel == 0 ? -um : uc) : ukp[m] * lt + uc * (R(1.0) - lt);
+            sm[(Tl::OWHX + r) * S + x] = whx;
+            sm[(Tl::OFUW + r) * S + x] = uhz * whx;
+        }
+    }
+    for (int e = tid; e < (TY + 1) * (TX + 3); e += T) {  // vhx, rows j0-1 .. j0+TY-1, columns 1 .. TX+3
+        const int r = e / (TX + 3), x = 1 + e % (TX + 3);
+        sm[(Tl::OVHX + r) * S + x] = ixp_d<CDS>(sm + (Tl::OV + r + 2) * S, x);
+    }
+    for (int e = tid; e < (TY + 3) * TX; e += T) {  // vhy, rows j0-1 .. j0+TY+1, own columns
+        const int r = e / TX, x = HX + e % TX;
+        sm[(Tl::OVHY + r) * S + x] = iym<CDS, S>(sm + (Tl::OV + r + 2) * S + x);
+    }
+    for (int e = tid; e < (TY + 3) * (TX + 3); e += T) {  // uhy*vhx, rows j0-2 .. j0+TY, columns 1 .. TX+3
+        const int r = e / (TX + 3), x = 1 + e % (TX + 3);
+        const R uhy = iyp<CDS, S>(sm + (Tl::OU + r + 1) * S + x);
+        const R vhx = ixp_d<CDS>(sm + (Tl::OV + r + 1) * S, x);
+        sm[(Tl::OFUV + r) * S + x] = uhy * vhx;
+    }
+#pragma unroll
+    for (int m = 0; m < PV; ++m) {  // why*vhz, rows j0-2 .. j0+TY, own columns
+        const int e = tid + m * T;
+        if (NVK % T == 0 || e < NVK) {
+            const int r = e / TX, x = HX + e % TX;
+            const R vc = sm[(Tl::OV + r + 1) * S + x];
+            const R why = top ? R(0.0) : iyp<CDS, S>(sm + (Tl::OW + r + 1) * S + x);
+            const R vhz = top ? (A.ochannel == 0 ? R(0.0) : vc) : vkp[m] * lt + vc * (R(1.0) - lt);
+            sm[(Tl::OFWV + r) * S + x] = vhz * why;
+        }
+    }
+    __syncthreads();
+    // ---- phase 2
+    R f, g, h, qu, qv, qw;
+    momentum_tile<CDS, S, TY, R, false>(A, sm, xl, jj, k, pv, dpx, f, g, h, qu, qv, qw);
+    const size_t c = ((size_t)k * ny + j) * nx + i;
+    A.ff[c] = f;
+    A.gg[c] = g;
+    A.hh[c] = h;
+    reinterpret_cast<R *>(A.s0)[c] = qu;
+    reinterpret_cast<R *>(A.s1)[c] = qv;
+    reinterpret_cast<R *>(A.s2)[c] = qw;
+}
+
+// Tile shapes of k_rhs_mom_xt (launch-shape knob rhs_xt; 0 is the fused
+// whole-row kernel)
+constexpr int kRhsXtTX[5] = {0, 32, 64, 32, 64};
+constexpr int kRhsXtTY[5] = {0, 8, 8, 16, 4};
+bool rhs_xt_fits(int shape, int nx, int ny)
+{
+    return shape >= 1 && shape <= 4 && nx % kRhsXtTX[shape] == 0 && ny % kRhsXtTY[shape] == 0;
+}
+
 // Momentum RHS + x R2C of qu,qv,qw through a global-memory accessor: PtD in
 // a damped substep, Pt with the 6th-order z operators (ZO = 6).  Block:
 // rows_per_block full rows of level k.
@@ -208,7 +344,7 @@ __global__ __launch_bounds__(kRhsThreads) void k_rhs_mom_damped(StepArgsT<typena
     constexpr int NL = NX / 2;
     __shared__ C sq_[3 * ROWS * NL];
     R *sqd = reinterpret_cast<R *>(sq_);
-    const int k = blockIdx.y, j0 = blockIdx.x * ROWS, tid = threadIdx.x;
+    const int k = A.klo + (int)blockIdx.y, j0 = blockIdx.x * ROWS, tid = threadIdx.x;
     const R dpx = A.dp_opt == 2 ? R(A.scal[0]) : A.dpx;
     for (int p = tid; p < ROWS * NX; p += kRhsThreads) {
         int jj = p / NX, i = p - jj * NX, j = j0 + jj;
@@ -274,14 +410,17 @@ __global__ __launch_bounds__(256) void k_damp_apply(double *f, const double *dmp
 // The momentum right-hand side of one substep into B.ff, gg, hh and the x
 // spectra of qu, qv, qw (B.s0..s2).  A damped substep (isdamp = 1) first
 // forms the plane means of the undamped fields (and damps a passive t in
-// place); the stencils then apply the damping as they read.
+// place); the levels that read a damped value (kdamp0-1 and up: the
+// stencils reach one level down) then go through the accessor kernel that
+// applies the damping as it reads, and the levels below through the tiled
+// kernels of an undamped substep.
 template <class R>
 int rhs_momentum(hc_ctx *c, const StepArgs &A, StepArgsT<R> &B)
 {
     const hc_params &P = c->prm;
     const int rows = rows_for(c->nx);
-    const dim3 grow((c->ny + rows - 1) / rows, c->nz);
     const bool damp = P.isdamp == 1, z6 = P.zorder == 4 || P.zorder == 6;
+    const int ksplit = damp ? std::max(0, c->kdamp0 - 1) : c->nz;
     if constexpr (std::is_same<R, double>::value) {
         if (damp) {
             // rayleigh_damping of u, v, w, p, t at the start of the substep
@@ -294,9 +433,12 @@ int rhs_momentum(hc_ctx *c, const StepArgs &A, StepArgsT<R> &B)
                                    c->dmp_m + 4 * c->nz, c->nx, c->ny, c->kdamp0, P.nzdamp + 1);
             }
             B.dmask = 31;
-#define HC_RHSD(NX_)                                                                                         \
-    if (P.cds == 1) hipLaunchKernelGGL((k_rhs_mom_damped<1, NX_>), grow, dim3(kRhsThreads), 0, c->stream, B); \
-    else hipLaunchKernelGGL((k_rhs_mom_damped<2, NX_>), grow, dim3(kRhsThreads), 0, c->stream, B);
+            B.klo = ksplit;
+            B.nlev = c->nz - ksplit;
+            const dim3 gd((c->ny + rows - 1) / rows, B.nlev);
+#define HC_RHSD(NX_)                                                                                       \
+    if (P.cds == 1) hipLaunchKernelGGL((k_rhs_mom_damped<1, NX_>), gd, dim3(kRhsThreads), 0, c->stream, B); \
+    else hipLaunchKernelGGL((k_rhs_mom_damped<2, NX_>), gd, dim3(kRhsThreads), 0, c->stream, B);
             HC_DISPATCH_N(c->nx, HC_RHSD)
 #undef HC_RHSD
         }
@@ -304,31 +446,56 @@ int rhs_momentum(hc_ctx *c, const StepArgs &A, StepArgsT<R> &B)
     if (z6 && !damp) {  // 6th/4th-order z operators: the global-memory point kernel
         B.dmask = 0;
         zinterp6<R>(c, B);
+        const dim3 grow((c->ny + rows - 1) / rows, c->nz);
 #define HC_RHSZ(NX_)                                                                                                   \
     if (P.cds == 1) hipLaunchKernelGGL((k_rhs_mom_damped<1, NX_, PtT<R>, 6>), grow, dim3(kRhsThreads), 0, c->stream, B); \
     else hipLaunchKernelGGL((k_rhs_mom_damped<2, NX_, PtT<R>, 6>), grow, dim3(kRhsThreads), 0, c->stream, B);
         HC_DISPATCH_N(c->nx, HC_RHSZ)
 #undef HC_RHSZ
     }
-    if (!damp && !z6) {
+    int err = 0;
+    if (!z6 && ksplit > 0) {
+        B.klo = 0;
+        B.nlev = ksplit;
+        if (rhs_xt_fits(c->tune.rhs_xt, c->nx, c->ny)) {
+            const int sh = c->tune.rhs_xt;
+            const dim3 g((unsigned)(c->nx / kRhsXtTX[sh] * (c->ny / kRhsXtTY[sh]) * ksplit));
+#define HC_XT(TX_, TY_)                                                                                      \
+    if (P.cds == 1) hipLaunchKernelGGL((k_rhs_mom_xt<1, TX_, TY_, R>), g, dim3(TX_ * TY_), 0, c->stream, B); \
+    else hipLaunchKernelGGL((k_rhs_mom_xt<2, TX_, TY_, R>), g, dim3(TX_ * TY_), 0, c->stream, B);
+            switch (sh) {
+            case 1: HC_XT(32, 8) break;
+            case 2: HC_XT(64, 8) break;
+            case 3: HC_XT(32, 16) break;
+            default: HC_XT(64, 4) break;
+            }
+#undef HC_XT
+            const R *src[3] = {reinterpret_cast<const R *>(B.s0), reinterpret_cast<const R *>(B.s1),
+                               reinterpret_cast<const R *>(B.s2)};
+            cplx<R> *dst[3] = {B.s0, B.s1, B.s2};
+            err = launch_r2c_n<R>(c, 3, src, dst, 0, ksplit);
+        } else {
 #define HC_RHS(NX_)                                                                                     \
     if constexpr (NX_ <= 256) {                                                                          \
         constexpr int TY_ = rhs2_ty<NX_>();                                                              \
-        dim3 g1((unsigned)(c->ny / TY_ * c->nz));                                                        \
+        dim3 g1((unsigned)(c->ny / TY_ * ksplit));                                                       \
         constexpr int T_ = rhs2_threads<NX_>();                                                          \
         if (P.cds == 1) hipLaunchKernelGGL((k_rhs_mom2<1, NX_, R>), g1, dim3(T_), 0, c->stream, B);      \
         else hipLaunchKernelGGL((k_rhs_mom2<2, NX_, R>), g1, dim3(T_), 0, c->stream, B);                 \
     } else {                                                                                             \
         constexpr int TY_ = rhs_ty<NX_>();                                                               \
-        dim3 g1((unsigned)(c->ny / TY_ * c->nz));                                                        \
+        dim3 g1((unsigned)(c->ny / TY_ * ksplit));                                                       \
         constexpr int T_ = rhs_threads<NX_>();                                                           \
         if (P.cds == 1) hipLaunchKernelGGL((k_rhs_mom<1, NX_, R>), g1, dim3(T_), 0, c->stream, B);       \
         else hipLaunchKernelGGL((k_rhs_mom<2, NX_, R>), g1, dim3(T_), 0, c->stream, B);                  \
     }
-        HC_DISPATCH_N(c->nx, HC_RHS)
+            HC_DISPATCH_N(c->nx, HC_RHS)
 #undef HC_RHS
+        }
     }
-    return 0;
+    B.klo = 0;
+    B.nlev = c->nz;
+    return err;
 }
 
 template int rhs_momentum<double>(hc_ctx *, const StepArgs &, StepArgsT<double> &);

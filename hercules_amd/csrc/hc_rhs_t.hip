@@ -13,7 +13,7 @@ __global__ __launch_bounds__(kRhsThreads) void k_rhs_t(StepArgsT<typename Acc::R
     constexpr int NL = NX / 2;
     __shared__ C sq_[ROWS * NL];
     R *sqd = reinterpret_cast<R *>(sq_);
-    const int k = blockIdx.y, j0 = blockIdx.x * ROWS, tid = threadIdx.x;
+    const int k = A.klo + (int)blockIdx.y, j0 = blockIdx.x * ROWS, tid = threadIdx.x;
     for (int p = tid; p < ROWS * NX; p += kRhsThreads) {
         int jj = p / NX, i = p - jj * NX, j = j0 + jj;
         R qt = R(0.0);
@@ -50,8 +50,8 @@ __global__ __launch_bounds__(rhs2_threads<NX>(), (rhs2_waves<NX, R>())) void k_r
     C tw_a = czero_t<C>(), tw_b = czero_t<C>();
     if (tid < NL) { tw_a = A.tw_nl[tid]; tw_b = A.tw_nx[tid]; }
     const int nty = A.ny / TY, nz = A.nz;
-    const int logical = xcd_remap(blockIdx.x, nty * nz);
-    const int jt = logical % nty, k = logical / nty, j0 = jt * TY;
+    const int logical = xcd_remap(blockIdx.x, nty * A.nlev);
+    const int jt = logical % nty, k = A.klo + logical / nty, j0 = jt * TY;
     const int i = tid % NX, jj = tid / NX, j = j0 + jj;
     constexpr int NST = (Q::NROWS * NX + T - 1) / T;
     R stv[NST];
@@ -120,36 +120,46 @@ __global__ __launch_bounds__(256) void k_mean_interp(StepArgsT<typename Acc::Rea
 
 
 // The temperature right-hand side of one substep into B.ss and the x
-// spectrum of qt (B.s0), with the updated velocities (t read damped).
+// spectrum of qt (B.s0), with the updated velocities (t read damped).  In a
+// damped substep the levels that read a damped t (kdamp0-1 and up) go
+// through the accessor kernel, the levels below through the tiled kernel.
 template <class R>
 int rhs_temperature(hc_ctx *c, StepArgsT<R> &B)
 {
     const hc_params &P = c->prm;
     const int rows = rows_for(c->nx);
-    const dim3 grow((c->ny + rows - 1) / rows, c->nz);
     const bool damp = P.isdamp == 1, z6 = P.zorder == 4 || P.zorder == 6;
+    const int ksplit = damp ? std::max(0, c->kdamp0 - 1) : c->nz;
     B.dmask = 16;  // t read damped, the updated velocities as they are
     if (z6) {
         B.dmask = 0;
+        const dim3 grow((c->ny + rows - 1) / rows, c->nz);
 #define HC_RTZ(NX_)                                                                                                  \
     if (P.cds == 1) hipLaunchKernelGGL((k_rhs_t<1, NX_, PtT<R>, 6>), grow, dim3(kRhsThreads), 0, c->stream, B);       \
     else hipLaunchKernelGGL((k_rhs_t<2, NX_, PtT<R>, 6>), grow, dim3(kRhsThreads), 0, c->stream, B);
         HC_DISPATCH_N(c->nx, HC_RTZ)
 #undef HC_RTZ
+        return 0;
     }
     if constexpr (std::is_same<R, double>::value) {
         if (damp) {
-#define HC_RT(NX_)                                                                                        \
-    if (P.cds == 1) hipLaunchKernelGGL((k_rhs_t<1, NX_, PtD>), grow, dim3(kRhsThreads), 0, c->stream, B); \
-    else hipLaunchKernelGGL((k_rhs_t<2, NX_, PtD>), grow, dim3(kRhsThreads), 0, c->stream, B);
+            B.klo = ksplit;
+            B.nlev = c->nz - ksplit;
+            const dim3 gd((c->ny + rows - 1) / rows, B.nlev);
+#define HC_RT(NX_)                                                                                      \
+    if (P.cds == 1) hipLaunchKernelGGL((k_rhs_t<1, NX_, PtD>), gd, dim3(kRhsThreads), 0, c->stream, B); \
+    else hipLaunchKernelGGL((k_rhs_t<2, NX_, PtD>), gd, dim3(kRhsThreads), 0, c->stream, B);
             HC_DISPATCH_N(c->nx, HC_RT)
 #undef HC_RT
         }
     }
-    if (!damp && !z6) {
+    if (ksplit > 0) {
+        B.klo = 0;
+        B.nlev = ksplit;
+        const dim3 grow((c->ny + rows - 1) / rows, ksplit);
 #define HC_RT(NX_)                                                                                               \
     if constexpr (NX_ <= 256) {                                                                                  \
-        dim3 g1((unsigned)(c->ny / rhs2_ty<NX_>() * c->nz));                                                     \
+        dim3 g1((unsigned)(c->ny / rhs2_ty<NX_>() * ksplit));                                                    \
         constexpr int T_ = rhs2_threads<NX_>();                                                                  \
         if (P.cds == 1) hipLaunchKernelGGL((k_rhs_t2<1, NX_, R>), g1, dim3(T_), 0, c->stream, B);                \
         else hipLaunchKernelGGL((k_rhs_t2<2, NX_, R>), g1, dim3(T_), 0, c->stream, B);                           \
@@ -160,6 +170,8 @@ int rhs_temperature(hc_ctx *c, StepArgsT<R> &B)
         HC_DISPATCH_N(c->nx, HC_RT)
 #undef HC_RT
     }
+    B.klo = 0;
+    B.nlev = c->nz;
     return 0;
 }
 

@@ -15,7 +15,9 @@ __global__ __launch_bounds__(kRhsThreads) void k_x_r2c(R2CFields<R> F, int ny, c
     __shared__ C sq_[ROWS * NL];
     R *sqd = reinterpret_cast<R *>(sq_);
     const int k = blockIdx.y, j0 = blockIdx.x * ROWS, tid = threadIdx.x;
-    const R *__restrict__ src = F.src[blockIdx.z];  // field blockIdx.z
+    // field blockIdx.z; src may be dst (an in-place transform): a block reads
+    // all of its rows before it writes any
+    const R *src = F.src[blockIdx.z];
     C *dst = F.dst[blockIdx.z];
     for (int p = tid; p < ROWS * NX; p += kRhsThreads) {
         int jj = p / NX, i = p - jj * NX, j = j0 + jj;
@@ -231,19 +233,23 @@ int launch_c2r(hc_ctx *c, int nf, C2RArgsT<R> &X)
     return 0;
 }
 
-// x R2C of nf (1..4) fields in one launch
+// x R2C of nf (1..4) fields in one launch, levels k0 .. k0+nk-1 (nk < 0:
+// to the top)
 template <class R>
-int launch_r2c_n(hc_ctx *c, int nf, const R *const *src, cplx<R> *const *dst)
+int launch_r2c_n(hc_ctx *c, int nf, const R *const *src, cplx<R> *const *dst, int k0, int nk)
 {
     if (nf < 1 || nf > 4) return set_err(HC_ERR_ARG, "launch_r2c_n: %d fields", nf);
+    if (nk < 0) nk = c->nz - k0;
+    if (nk <= 0) return 0;
     const View<R> V = view<R>(c);
     R2CFields<R> F;
+    const size_t pr = (size_t)k0 * c->nx * c->ny, pc = (size_t)k0 * c->nl * c->ny;
     for (int q = 0; q < 4; ++q) {
-        F.src[q] = q < nf ? src[q] : nullptr;
-        F.dst[q] = q < nf ? dst[q] : nullptr;
+        F.src[q] = q < nf ? src[q] + pr : nullptr;
+        F.dst[q] = q < nf ? dst[q] + pc : nullptr;
     }
     int rows = rows_for(c->nx);
-    dim3 g((c->ny + rows - 1) / rows, c->nz, nf);
+    dim3 g((c->ny + rows - 1) / rows, nk, nf);
     R norm = R(1.0 / ((double)c->nx * (double)c->ny));
 #define HC_R(NX_) hipLaunchKernelGGL((k_x_r2c<NX_, R>), g, dim3(kRhsThreads), 0, c->stream, F, c->ny, \
                                      (const cplx<R> *)V.tw_nl, (const cplx<R> *)V.tw_nx, norm);
@@ -264,6 +270,6 @@ template int launch_yfft<float2>(hc_ctx *, int, int, float2 *, float2 *, float2 
 template int launch_c2r<double>(hc_ctx *, int, C2RArgsT<double> &);
 template int launch_c2r<float>(hc_ctx *, int, C2RArgsT<float> &);
 template int launch_r2c<double>(hc_ctx *, const double *, double2 *);
-template int launch_r2c_n<double>(hc_ctx *, int, const double *const *, double2 *const *);
-template int launch_r2c_n<float>(hc_ctx *, int, const float *const *, float2 *const *);
+template int launch_r2c_n<double>(hc_ctx *, int, const double *const *, double2 *const *, int, int);
+template int launch_r2c_n<float>(hc_ctx *, int, const float *const *, float2 *const *, int, int);
 template int launch_r2c<float>(hc_ctx *, const float *, float2 *);

@@ -231,20 +231,23 @@ int hc_wall_shear(hc_ctx *ctx, double *tau_bottom, double *tau_top);
 /* Scalars of the reference's module state: 0 dpx_drive, 1 ubar, 2 time_sim,
  * 3 got, 4 number of steps taken; and, with no reference counterpart, 5 the
  * number of fields the column kernels move per global phase (LDS sub-tiles,
- * 1 or 4) and 6-9 the launch shapes of hc_autotune (RHS tile-order group,
- * y-transform columns per block, inverse-x points per block, column block
- * order; 0 means the default). */
+ * 1 or 4), 6-9 and 11 the launch shapes of hc_autotune (RHS tile-order
+ * group, y-transform columns per block, inverse-x points per block, column
+ * block order, momentum tile shape; 0 means the default) and 10 the number
+ * of steps in the device statistics. */
 int hc_get_scalar(hc_ctx *ctx, int id, double *value);
 
 /* Autotuner (no counterpart in the reference).  Times whole steps of this
  * context from its current state for each candidate launch shape of the
  * finite-difference kernels (RHS tile-order group, y-transform columns per
- * block, inverse-x points per block, column block order; coordinate
+ * block, inverse-x points per block, column block order, momentum tile
+ * shape: 0 whole x rows with the x transform fused, 1-4 tiles of 32x8, 64x8,
+ * 32x16, 64x4 points and a separate in-place x transform; coordinate
  * descent, steps_per_trial steps per trial) and keeps the fastest.  Every
  * candidate gives bit-identical results.  The state and step counters are
  * restored; the spectral path (cds = 3) keeps its fixed shapes. */
 int hc_autotune(hc_ctx *ctx, int steps_per_trial);
-/* Set one of those launch shapes directly (knob 0..3 in the order above;
+/* Set one of those launch shapes directly (knob 0..4 in the order above;
  * HC_ERR_ARG for a shape the kernels do not accept). */
 int hc_set_tuning(hc_ctx *ctx, int knob, int value);
 
