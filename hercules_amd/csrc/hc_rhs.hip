@@ -205,7 +205,14 @@ __global__ __launch_bounds__(rhs2_threads<NX>(), (rhs2_waves<NX, R>())) void k_r
 // the bytes of its nx/2 modes), which launch_r2c_n then transforms in place.
 // The small LDS tile lets several blocks share a CU, so that one block's
 // loads overlap another's stencils.
-template <int CDS, int TX, int TY, class R>
+// DAMP (levels that read the sponge of a damped substep, FP64): every value
+// is read as rayleigh_damping leaves it (src/module_tools.f90:705-749): a
+// level k >= kdamp0 damped at the domain's interior (i, j), undamped at a
+// wrapped x / y position (the stale halo) and in a z ghost.  That equals the
+// reference for the points two or more columns and rows from the domain's
+// x / y edges, whose interpolants all sit at interior centres;
+// k_rhs_mom_edge then recomputes the points nearer the edges.
+template <int CDS, int TX, int TY, class R, bool DAMP = false>
 __global__ __launch_bounds__(TX * TY, 4) void k_rhs_mom_xt(StepArgsT<R> A)
 {
     constexpr int T = TX * TY, S = TX + 6, HX = 3;
@@ -223,6 +230,16 @@ __global__ __launch_bounds__(TX * TY, 4) void k_rhs_mom_xt(StepArgsT<R> A)
     const int i = i0 + il, j = j0 + jj, xl = il + HX;
     const bool top = k == A.nz - 1;
     auto gx = [&](int x) -> unsigned { return (unsigned)wrapi(i0 - HX + x, nx); };
+    // field fid of level kk as the damped substep reads it (interior point)
+    auto dmp = [&](int fid, int kk, R v) __attribute__((always_inline)) {
+        if constexpr (DAMP) {
+            if (kk >= A.kdamp0 && ((A.dmask >> fid) & 1)) {
+                const R s = A.dmp_s[kk], m = A.dmp_m[fid * A.nz + kk];
+                v = v - s * (v - m);
+            }
+        }
+        return v;
+    };
     // staging: every thread issues all of its loads before the first use
     R stv[NST];
 #pragma unroll
@@ -258,6 +275,43 @@ __global__ __launch_bounds__(TX * TY, 4) void k_rhs_mom_xt(StepArgsT<R> A)
             vkp[m] = A.v[((unsigned)(k + 1) * (unsigned)ny + (unsigned)wrapi(j0 - 2 + r, ny)) * (unsigned)nx +
                          (unsigned)(i0 + x)];
         }
+    }
+    if constexpr (DAMP) {
+        // the staged rows: damped at the domain's interior positions only
+#pragma unroll
+        for (int m = 0; m < NST; ++m) {
+            const int q = tid + m * T, r = q / S, x = q - r * S;
+            const int fid = r < Tl::OV ? 0 : r < Tl::OW ? 1 : r < Tl::OP ? 2 : 3;
+            const int jr = r < Tl::OP ? j0 - 3 + r % Tl::RU : j0 - 1 + (r - Tl::OP), xg = i0 - HX + x;
+            if (xg >= 0 && xg < nx && jr >= 0 && jr < ny) stv[m] = dmp(fid, k, stv[m]);
+        }
+#pragma unroll
+        for (int m = 0; m < PU; ++m) {
+            const int e = tid + m * T, xg = i0 - HX + 1 + e % (TX + 3);
+            if (!top && xg >= 0 && xg < nx) ukp[m] = dmp(0, k + 1, ukp[m]);
+        }
+#pragma unroll
+        for (int m = 0; m < PV; ++m) {
+            const int jr = j0 - 2 + (tid + m * T) / TX;
+            if (!top && jr >= 0 && jr < ny) vkp[m] = dmp(1, k + 1, vkp[m]);
+        }
+        // the point's z neighbours (wall ghosts stay as loaded)
+        if (k >= 1) {
+            pv.u_km = dmp(0, k - 1, pv.u_km);
+            pv.v_km = dmp(1, k - 1, pv.v_km);
+#pragma unroll
+            for (int d = 0; d < 4; ++d) pv.wkx[d] = dmp(2, k - 1, pv.wkx[d]);
+#pragma unroll
+            for (int d = 0; d < 3; ++d) pv.wky[d] = dmp(2, k - 1, pv.wky[d]);
+        }
+        if (!top) {
+            pv.u_kp = dmp(0, k + 1, pv.u_kp);
+            pv.v_kp = dmp(1, k + 1, pv.v_kp);
+            if (k + 1 < A.nz - 1) pv.w_kp = dmp(2, k + 1, pv.w_kp);
+            pv.p_kp = dmp(3, k + 1, pv.p_kp);
+            if (A.isscalar) pv.t_kp = dmp(4, k + 1, pv.t_kp);
+        }
+        if (A.isscalar) pv.t_k = dmp(4, k, pv.t_k);
     }
 #pragma unroll
     for (int m = 0; m < NST; ++m) {
@@ -321,6 +375,41 @@ __global__ __launch_bounds__(TX * TY, 4) void k_rhs_mom_xt(StepArgsT<R> A)
     reinterpret_cast<R *>(A.s0)[c] = qu;
     reinterpret_cast<R *>(A.s1)[c] = qv;
     reinterpret_cast<R *>(A.s2)[c] = qw;
+}
+
+// The points of a damped substep's sponge levels within two columns or rows
+// of the domain's x / y edges (x in 0,1,nx-2,nx-1 or y in 0,1,ny-2,ny-1):
+// their stencils read stale halos and interpolants evaluated at wrapped
+// centres, which the accessor PtD reproduces.  Writes ff, gg, hh and the
+// physical qu, qv, qw over the values k_rhs_mom_xt<..., DAMP> left there.
+template <int CDS>
+__global__ __launch_bounds__(256) void k_rhs_mom_edge(StepArgs A)
+{
+    const int nx = A.nx, ny = A.ny, per = 4 * nx + 4 * (ny - 4);
+    const int t = (int)(blockIdx.x * 256 + threadIdx.x);
+    if (t >= per * A.nlev) return;
+    const int k = A.klo + t / per, e = t % per;
+    int i, j;
+    if (e < 4 * nx) {
+        const int r = e / nx;
+        j = r < 2 ? r : ny - 4 + r;
+        i = e - r * nx;
+    } else {
+        const int u = e - 4 * nx, q = u % 4;
+        j = 2 + u / 4;
+        i = q < 2 ? q : nx - 4 + q;
+    }
+    PtD q(A, i, j, k);
+    double f, g, h, qu, qv, qw;
+    const double dpx = A.dp_opt == 2 ? A.scal[0] : A.dpx;
+    momentum_point<CDS, 2>(q, dpx, f, g, h, qu, qv, qw);
+    const size_t c = ((size_t)k * ny + j) * nx + i;
+    A.ff[c] = f;
+    A.gg[c] = g;
+    A.hh[c] = h;
+    reinterpret_cast<double *>(A.s0)[c] = qu;
+    reinterpret_cast<double *>(A.s1)[c] = qv;
+    reinterpret_cast<double *>(A.s2)[c] = qw;
 }
 
 // Tile shapes of k_rhs_mom_xt (launch-shape knob rhs_xt; 0 is the fused
@@ -435,12 +524,39 @@ int rhs_momentum(hc_ctx *c, const StepArgs &A, StepArgsT<R> &B)
             B.dmask = 31;
             B.klo = ksplit;
             B.nlev = c->nz - ksplit;
-            const dim3 gd((c->ny + rows - 1) / rows, B.nlev);
+            // the sponge levels: x-y tiles reading damped values, then the
+            // points near the x / y edges through the accessor, then the x
+            // transform in place; the accessor kernel alone where no tile
+            // shape divides the grid
+            const int sh = c->tune.rhs_xt > 0 ? c->tune.rhs_xt : 1;
+            if (rhs_xt_fits(sh, c->nx, c->ny) && c->nx >= 4 && c->ny >= 4) {
+                const dim3 g((unsigned)(c->nx / kRhsXtTX[sh] * (c->ny / kRhsXtTY[sh]) * B.nlev));
+#define HC_XTD(TX_, TY_)                                                                                                 \
+    if (P.cds == 1) hipLaunchKernelGGL((k_rhs_mom_xt<1, TX_, TY_, R, true>), g, dim3(TX_ * TY_), 0, c->stream, B);       \
+    else hipLaunchKernelGGL((k_rhs_mom_xt<2, TX_, TY_, R, true>), g, dim3(TX_ * TY_), 0, c->stream, B);
+                switch (sh) {
+                case 1: HC_XTD(32, 8) break;
+                case 2: HC_XTD(64, 8) break;
+                case 3: HC_XTD(32, 16) break;
+                default: HC_XTD(64, 4) break;
+                }
+#undef HC_XTD
+                const int per = 4 * c->nx + 4 * (c->ny - 4);
+                const dim3 ge((unsigned)(((size_t)per * B.nlev + 255) / 256));
+                if (P.cds == 1) hipLaunchKernelGGL((k_rhs_mom_edge<1>), ge, dim3(256), 0, c->stream, B);
+                else hipLaunchKernelGGL((k_rhs_mom_edge<2>), ge, dim3(256), 0, c->stream, B);
+                const R *src[3] = {reinterpret_cast<const R *>(B.s0), reinterpret_cast<const R *>(B.s1),
+                                   reinterpret_cast<const R *>(B.s2)};
+                cplx<R> *dst[3] = {B.s0, B.s1, B.s2};
+                if (int e = launch_r2c_n<R>(c, 3, src, dst, ksplit, B.nlev)) return e;
+            } else {
+                const dim3 gd((c->ny + rows - 1) / rows, B.nlev);
 #define HC_RHSD(NX_)                                                                                       \
     if (P.cds == 1) hipLaunchKernelGGL((k_rhs_mom_damped<1, NX_>), gd, dim3(kRhsThreads), 0, c->stream, B); \
     else hipLaunchKernelGGL((k_rhs_mom_damped<2, NX_>), gd, dim3(kRhsThreads), 0, c->stream, B);
-            HC_DISPATCH_N(c->nx, HC_RHSD)
+                HC_DISPATCH_N(c->nx, HC_RHSD)
 #undef HC_RHSD
+            }
         }
     }
     if (z6 && !damp) {  // 6th/4th-order z operators: the global-memory point kernel

@@ -80,6 +80,11 @@ def test_steps_shapes_match_oracle(shape):
     ("open_t", {"tbc_top": 1, "dtdz_top": -0.8, "cds": 1, "wall_rows": 1}),
     ("closed", {"isscalar": 1, "got": 0.05, "tbc_top": 1, "dtdz_top": -0.3}),
     ("open_t", {"tbc_top": 1, "dtdz_top": 0.5, "isdamp": 1, "nzdamp": 8, "cdamp": 0.2}),
+    # sponge levels through x-y tiles with points clear of the x / y edges
+    # (k_rhs_mom_xt<..., DAMP>) and the edge points through the accessor
+    ("open_t", {"nx": 64, "ny": 64, "nz": 65, "isdamp": 1, "nzdamp": 12, "cdamp": 0.2}),
+    ("closed", {"nx": 128, "ny": 32, "nz": 33, "isdamp": 1, "nzdamp": 8, "cdamp": 0.3, "dp_opt": 3, "fc": 0.5,
+                "ug": 1.0, "vg": 0.2}),
 ])
 def test_steps_match_oracle(kind, kw):
     prm, st = small_case(kind, **kw)
