@@ -223,8 +223,15 @@ __global__ __launch_bounds__(TX * TY, 4) void k_rhs_mom_xt(StepArgsT<R> A)
     __shared__ __align__(16) R sm[Tl::NROWS * S];
     const int tid = threadIdx.x, nx = A.nx, ny = A.ny;
     const int ntx = nx / TX, nty = ny / TY;
+    // tile order: groups of G rows of y tiles swept level by level, x tiles
+    // fastest, so that an XCD's L2 holds the y halos and the neighbouring
+    // levels of the tiles in flight
+    int G = A.rgroup > 0 ? A.rgroup : 8;
+    if (nty % G) G = nty % 8 == 0 ? 8 : nty % 4 == 0 ? 4 : 1;
+    const int per = G * ntx * A.nlev;
     const int logical = xcd_remap(blockIdx.x, ntx * nty * A.nlev);
-    const int it = logical % ntx, rem = logical / ntx, jt = rem % nty, k = A.klo + rem / nty;
+    const int grp = logical / per, rem = logical - grp * per, r2 = rem % (G * ntx);
+    const int k = A.klo + rem / (G * ntx), jt = grp * G + r2 / ntx, it = r2 % ntx;
     const int i0 = it * TX, j0 = jt * TY;
     const int il = tid % TX, jj = tid / TX;
     const int i = i0 + il, j = j0 + jj, xl = il + HX;
