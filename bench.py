@@ -584,7 +584,12 @@ def main():
     tag = args.workload + ("_mixed" if mixed_main else "") + (f"_z{args.zorder}" if args.zorder != 2 else "")
     tpath = os.path.join(ROOT, "profiles", f"hbm_traffic_{tag}_{prm.nx}x{prm.ny}x{prm.nz}.json")
     if os.path.exists(tpath):
-        traffic = json.load(open(tpath)).get("bytes_per_launch", {})
+        tj = json.load(open(tpath))
+        traffic = dict(tj.get("bytes_per_launch", {}))
+        # the right-hand-side stage's bytes depend on the momentum tile shape
+        shapes = getattr(ch, "tuning", None) or {}
+        if "launch_shapes" in tj and shapes.get("rhs_xt", 0) != tj["launch_shapes"].get("rhs_xt", 0):
+            traffic.pop("rhs", None)
     roofline = None
     if kernels:
         dom = max(kernels, key=lambda k: kernels[k]["avg_ms"] * kernels[k]["launches"])
