@@ -9,10 +9,25 @@ recomputation per segment, transposed substeps); the channel's pressure and
 other fields are taken as they are and not differentiated.  Tensors on the
 GPU stay there: they are copied device-to-device into the channel and the
 gradients are written into new device tensors.
+
+torch ships its own HIP runtime under the soname the library links
+(libamdhip64.so.7), and a process keeps the first one it loads: import torch
+(or this module) before the first Channel is created, or torch may find no
+GPU when it first touches one.
 """
 from __future__ import annotations
 
-import torch
+import sys
+import warnings
+
+from . import channel as _channel
+
+if "torch" not in sys.modules and _channel._lib is not None:
+    warnings.warn("hercules_amd.autograd: torch is imported after the HIP library was loaded; torch's GPU "
+                  "tensors may then find no device. Import torch before creating the first Channel.",
+                  RuntimeWarning, stacklevel=2)
+
+import torch  # noqa: E402
 
 
 def _to_channel(channel, name, t):

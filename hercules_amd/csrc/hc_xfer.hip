@@ -4,30 +4,62 @@
 // src/module_tools.f90:518-700, on one GPU).
 #include "hc_host.h"
 
-// Plain x R2C of one real field (standalone solve): src -> s0, scaled by norm.
+// x R2C of up to four real fields, scaled by norm.  Blocks of one field
+// (blockIdx.z) loop over its tiles of ROWS rows with a grid stride, two tiles
+// per trip with two register buffers: the next tile's rows are loaded before
+// the current tile's transform.  src may be dst (an in-place transform): a
+// tile is read and written only by its own block (a fetch past the last tile
+// rereads the last one and is never used).
 template <int NX, class R>
-__global__ __launch_bounds__(kRhsThreads) void k_x_r2c(R2CFields<R> F, int ny, const cplx<R> *tw_nl,
+__global__ __launch_bounds__(kRhsThreads) void k_x_r2c(R2CFields<R> F, int ny, int nk, const cplx<R> *tw_nl,
                                                        const cplx<R> *tw_nx, R norm)
 {
     using C = cplx<R>;
     constexpr int ROWS = rows_per_block<NX>();
     constexpr int NL = NX / 2;
+    constexpr int NIN = (ROWS * NX + kRhsThreads - 1) / kRhsThreads;
     __shared__ C sq_[ROWS * NL];
+    __shared__ C tws[2 * NL];  // half-length twiddles, then the R2C post-twiddles
     R *sqd = reinterpret_cast<R *>(sq_);
-    const int k = blockIdx.y, j0 = blockIdx.x * ROWS, tid = threadIdx.x;
-    // field blockIdx.z; src may be dst (an in-place transform): a block reads
-    // all of its rows before it writes any
+    const int tid = threadIdx.x, G = gridDim.x;
     const R *src = F.src[blockIdx.z];
     C *dst = F.dst[blockIdx.z];
-    for (int p = tid; p < ROWS * NX; p += kRhsThreads) {
-        int jj = p / NX, i = p - jj * NX, j = j0 + jj;
-        sqd[jj * NX + i] = j < ny ? src[((size_t)k * ny + j) * NX + i] : R(0.0);
-    }
-    __syncthreads();
-    lds_fft<NL, ROWS, NL, 1, kRhsThreads, -1>(sq_, tw_nl, tid);
-    for (int p = tid; p < ROWS * NL; p += kRhsThreads) {
-        int jj = p / NL, l = p - jj * NL, j = j0 + jj;
-        if (j < ny) dst[((size_t)k * ny + j) * NL + l] = r2c_post<NL>(sq_ + jj * NL, l, tw_nx, norm);
+    const int njt = (ny + ROWS - 1) / ROWS, ntiles = njt * nk;
+    for (int q = tid; q < NL; q += kRhsThreads) { tws[q] = tw_nl[q]; tws[NL + q] = tw_nx[q]; }
+    auto fetch = [&](int tile, R (&x)[NIN]) __attribute__((always_inline)) {
+        const int t = tile < ntiles ? tile : ntiles - 1;
+        const int k = t / njt, j0 = (t - k * njt) * ROWS;
+#pragma unroll
+        for (int m = 0; m < NIN; ++m) {
+            const int p = (ROWS * NX) % kRhsThreads == 0 ? tid + m * kRhsThreads
+                                                          : min(tid + m * kRhsThreads, ROWS * NX - 1);
+            const int jj = p / NX, i = p - jj * NX, j = min(j0 + jj, ny - 1);
+            x[m] = src[((size_t)k * ny + j) * NX + i];
+        }
+    };
+    auto work = [&](int tile, const R (&x)[NIN]) __attribute__((always_inline)) {
+        const int k = tile / njt, j0 = (tile - k * njt) * ROWS;
+        __syncthreads();  // twiddles stored; the previous tile's rows read
+#pragma unroll
+        for (int m = 0; m < NIN; ++m) {
+            const int p = tid + m * kRhsThreads;
+            if ((ROWS * NX) % kRhsThreads == 0 || p < ROWS * NX) sqd[p] = j0 + p / NX < ny ? x[m] : R(0.0);
+        }
+        __syncthreads();
+        lds_fft<NL, ROWS, NL, 1, kRhsThreads, -1>(sq_, tws, tid);
+        for (int p = tid; p < ROWS * NL; p += kRhsThreads) {
+            const int jj = p / NL, l = p - jj * NL, j = j0 + jj;
+            if (j < ny) dst[((size_t)k * ny + j) * NL + l] = r2c_post<NL>(sq_ + jj * NL, l, tws + NL, norm);
+        }
+    };
+    R xa[NIN], xb[NIN];
+    fetch(blockIdx.x, xa);
+    for (int tile = blockIdx.x; tile < ntiles; tile += 2 * G) {
+        fetch(tile + G, xb);
+        work(tile, xa);
+        if (tile + G >= ntiles) break;
+        fetch(tile + 2 * G, xa);
+        work(tile + G, xb);
     }
 }
 
@@ -248,10 +280,13 @@ int launch_r2c_n(hc_ctx *c, int nf, const R *const *src, cplx<R> *const *dst, in
         F.src[q] = q < nf ? src[q] + pr : nullptr;
         F.dst[q] = q < nf ? dst[q] + pc : nullptr;
     }
-    int rows = rows_for(c->nx);
-    dim3 g((c->ny + rows - 1) / rows, nk, nf);
+    const int rows = rows_for(c->nx);
+    // grid-stride blocks: as many per field as the CUs hold at once (8 per
+    // CU over the fields), at most one per tile
+    const long ntiles = (long)((c->ny + rows - 1) / rows) * nk;
+    dim3 g((unsigned)std::min<long>(ntiles, std::max<long>(1, 8L * c->ncu / nf)), 1, nf);
     R norm = R(1.0 / ((double)c->nx * (double)c->ny));
-#define HC_R(NX_) hipLaunchKernelGGL((k_x_r2c<NX_, R>), g, dim3(kRhsThreads), 0, c->stream, F, c->ny, \
+#define HC_R(NX_) hipLaunchKernelGGL((k_x_r2c<NX_, R>), g, dim3(kRhsThreads), 0, c->stream, F, c->ny, nk, \
                                      (const cplx<R> *)V.tw_nl, (const cplx<R> *)V.tw_nx, norm);
     HC_DISPATCH_N(c->nx, HC_R)
 #undef HC_R

@@ -10,8 +10,11 @@ divergence-free directions, and checkpoint spacing must not change it."""
 import numpy as np
 import pytest
 
-from hercules_amd import Channel, cases
-from hercules_amd.channel import HerculesError
+# torch before the library: the process keeps the first HIP runtime it loads
+torch = pytest.importorskip("torch")
+
+from hercules_amd import Channel, cases  # noqa: E402
+from hercules_amd.channel import HerculesError  # noqa: E402
 
 pytestmark = pytest.mark.gpu
 
@@ -184,7 +187,6 @@ def test_torch_autograd_binding_returns_the_adjoint_gradient():
 def test_torch_binding_keeps_device_tensors_on_the_device():
     """The same gradient from tensors on the GPU (device-to-device transfers)
     as from host tensors, bit for bit; the gradients come back on the GPU."""
-    torch = pytest.importorskip("torch")
     from hercules_amd.autograd import wall_shear_loss
     prm = _params("closed", nx=32, ny=32, nz=33)
     x = _state(prm, seed=10)
