@@ -208,11 +208,7 @@ struct AdjOps {
     const StepArgs &A;
     int nx, ny, nz;
     HC_DEV AdjOps(const StepArgs &a) : A(a), nx(a.nx), ny(a.ny), nz(a.nz) {}
-    // 32-bit index arithmetic: a field of one GPU holds fewer than 2^32 points
-    HC_DEV size_t at(int i, int j, int k) const
-    {
-        return ((unsigned)k * (unsigned)ny + (unsigned)wrapi(j, ny)) * (unsigned)nx + (unsigned)wrapi(i, nx);
-    }
+    HC_DEV size_t at(int i, int j, int k) const { return ((size_t)k * ny + wrapi(j, ny)) * nx + wrapi(i, nx); }
     HC_DEV double lev(int t, int k) const { return A.lev[t * nz + k]; }
     HC_DEV double f(const double *p, int i, int j, int k) const { return p[at(i, j, k)]; }
     HC_DEV double U(int i, int j, int k) const { return A.u[at(i, j, k)]; }
@@ -357,8 +353,7 @@ __global__ __launch_bounds__(256) void k_adj_gather(StepArgs A, AdjIO io)
                (-F(i, j + 2) + 16.0 * F(i, j + 1) - 30.0 * F(i, j) + 16.0 * F(i, j - 1) - F(i, j - 2)) * A.c12y2;
     };
     for (size_t pt = blockIdx.x * (size_t)blockDim.x + threadIdx.x; pt < n; pt += (size_t)gridDim.x * blockDim.x) {
-        const unsigned p32 = (unsigned)pt, row = p32 / (unsigned)nx;
-        const int i = (int)(p32 - row * (unsigned)nx), j = (int)(row % (unsigned)ny), k = (int)(row / (unsigned)ny);
+        const int i = (int)(pt % nx), j = (int)((pt / nx) % ny), k = (int)(pt / pl);
         const double qu = io.qu[pt], qv = io.qv[pt], qw = k < nz - 1 ? io.qw[pt] : 0.0;
         const double idz = o.lev(LV_IDZ, k), idzt = o.lev(LV_IDZT, k), idzb = o.lev(LV_IDZB, k);
         // ---- u and v: identity, explicit diffusion (transposed z operator with
