@@ -122,16 +122,41 @@ __global__ __launch_bounds__(256) void k_z6_band(StepArgs A, BandArgs<CI> B)
 #pragma unroll
         for (int c = 0; c < PB; ++c) U[t][c] = 0.0;
     }
+    // The right-hand sides (forward) and the U rows with the partial
+    // solutions (backward) of level k are loaded PF levels ahead into rings
+    // indexed by the unrolled loop position, in the stored type, so that a
+    // column's chain of levels waits less on its loads (a launch has about
+    // one wavefront per SIMD).  With FP64 spectra the kernel moves its bytes
+    // near the HBM rate already, and the one-level ring keeps its registers
+    // (two wavefronts per SIMD); FP32 spectra take four levels.
+    constexpr int PF = sizeof(CI) == 8 ? (PB >= 7 ? 2 : 4) : 1;
+    const CI zero = czero_t<CI>();
+    CI rq[PF][2];
+#pragma unroll
+    for (int u = 0; u < PF; ++u) {
+        const size_t o = (size_t)u * plane + col;
+        rq[u][0] = u < n ? S.in[0][o] : zero;
+        rq[u][1] = (u < n && nr > 1) ? S.in[1][o] : zero;
+    }
     // ---- forward elimination
-    for (int k = 0; k < n; ++k) {
+    for (int k0 = 0; k0 < n; k0 += PF) {
+#pragma unroll
+      for (int u = 0; u < PF; ++u) {
+        const int k = k0 + u;
+        if (k >= n) break;
         double a[2 * PB + 1];
 #pragma unroll
         for (int d = -PB; d <= PB; ++d) a[d + PB] = coff * bt[(size_t)k * kBandRow + d];
         a[PB] += cdiag;
         const size_t o = (size_t)k * plane + col;
         double2 r[2];
-        r[0] = ccast<double2>(S.in[0][o]);
-        r[1] = nr > 1 ? ccast<double2>(S.in[1][o]) : make_double2(0.0, 0.0);
+        r[0] = ccast<double2>(rq[u][0]);
+        r[1] = ccast<double2>(rq[u][1]);
+        if (k + PF < n) {
+            const size_t on = (size_t)(k + PF) * plane + col;
+            rq[u][0] = S.in[0][on];
+            if (nr > 1) rq[u][1] = S.in[1][on];
+        }
         if (is00) {
             if (pres) {
                 if (k == 0) {  // delnull: b0 phi(0) = 0 (src/module_poisson_solver.f90:558-564)
@@ -174,6 +199,7 @@ __global__ __launch_bounds__(256) void k_z6_band(StepArgs A, BandArgs<CI> B)
         y[1][PB - 1] = r[1];
 #pragma unroll
         for (int c = 0; c < PB; ++c) U[PB - 1][c] = a[PB + 1 + c];
+      }
     }
     // ---- back substitution
     const int mw = jm < ny / 2 ? jm : jm - ny;
@@ -181,22 +207,38 @@ __global__ __launch_bounds__(256) void k_z6_band(StepArgs A, BandArgs<CI> B)
     double2 xs[2][PB];  // x at k+1 .. k+PB
 #pragma unroll
     for (int c = 0; c < PB; ++c) xs[0][c] = xs[1][c] = make_double2(0.0, 0.0);
-    const CI zero = czero_t<CI>();
     if (face) {  // w at the top wall face
         const size_t o = (size_t)(nz - 1) * plane + col;
         S.f[0][o] = zero;
         if (nr > 1) S.f[1][o] = zero;
     }
-    for (int k = n - 1; k >= 0; --k) {
+    double ub[PF][PB + 1];
+    CI fq[PF][2];
+    auto ld_back = [&](int k, double (&uu)[PB + 1], CI (&ff)[2]) __attribute__((always_inline)) {
         const size_t o = (size_t)k * plane + col;
         const double *srow = S.scr + (size_t)k * (PB + 1) * plane + col;
+#pragma unroll
+        for (int c = 0; c <= PB; ++c) uu[c] = srow[(size_t)c * plane];
+        ff[0] = S.f[0][o];
+        ff[1] = nr > 1 ? S.f[1][o] : zero;
+    };
+#pragma unroll
+    for (int u = 0; u < PF; ++u)
+        if (n - 1 - u >= 0) ld_back(n - 1 - u, ub[u], fq[u]);
+    for (int k0 = n - 1; k0 >= 0; k0 -= PF) {
+#pragma unroll
+      for (int q = 0; q < PF; ++q) {
+        const int k = k0 - q;
+        if (k < 0) break;
+        const size_t o = (size_t)k * plane + col;
         double u[PB];
 #pragma unroll
-        for (int c = 0; c < PB; ++c) u[c] = srow[(size_t)c * plane];
-        const double dk = srow[(size_t)PB * plane];
+        for (int c = 0; c < PB; ++c) u[c] = ub[q][c];
+        const double dk = ub[q][PB];
         double2 r[2];
-        r[0] = ccast<double2>(S.f[0][o]);
-        r[1] = nr > 1 ? ccast<double2>(S.f[1][o]) : make_double2(0.0, 0.0);
+        r[0] = ccast<double2>(fq[q][0]);
+        r[1] = ccast<double2>(fq[q][1]);
+        if (k - PF >= 0) ld_back(k - PF, ub[q], fq[q]);
 #pragma unroll
         for (int c = 0; c < PB; ++c) {
             r[0].x -= u[c] * xs[0][c].x;
@@ -215,6 +257,7 @@ __global__ __launch_bounds__(256) void k_z6_band(StepArgs A, BandArgs<CI> B)
         xs[1][0] = r[1];
         S.f[0][o] = zero_out ? zero : ccast<CI>(r[0]);
         if (nr > 1) S.f[1][o] = zero_out ? zero : ccast<CI>(r[1]);
+      }
     }
 }
 
