@@ -64,6 +64,25 @@ def test_steps_shapes_match_oracle(shape):
     ch.close()
 
 
+@pytest.mark.parametrize("shapes", [dict(rhs_xt=0), dict(rhs_xt=1), dict(rhs_xt=2, c2r_pts=512)])
+def test_grid_stride_transforms_over_many_tiles_match_oracle(shapes):
+    """256x256x25: the grid-stride x transforms run several trips per block
+    (inverse x: 6400 tiles over 2048 blocks; the in-place R2C of the tiled
+    right-hand side: 1600 tiles per field over 682 blocks), with the fused and
+    the x-y tiled momentum right-hand side."""
+    prm, st = small_case("closed", nx=256, ny=256, nz=25)
+    o = make_oracle(prm, st)
+    ch = _channel(prm, st)
+    ch.set_tuning(**shapes)
+    o.steps(2)
+    ch.steps(2)
+    for n in ["u", "v", "w", "p"]:
+        err = rel_l2(ch.get_field(n), o.get(n))
+        print(f"{shapes} {n}: rel L2 {err:.3e}")
+        assert err < 1e-10, (n, err)
+    ch.close()
+
+
 @pytest.mark.parametrize("kind,kw", [
     ("closed", {}),
     ("closed", {"cds": 1}),
