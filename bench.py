@@ -13,8 +13,10 @@ right-hand side, plane means removed).
 steps by the discrete adjoint (checkpoint every --ckpt steps) against the
 forward run, on 128x128x129.
 
-Before timing, hc_autotune picks the launch shapes (bit-identical results;
-``--no-autotune`` keeps the defaults, ``--launch-shapes`` sets given ones).
+Before timing, hc_autotune picks the launch shapes (bit-identical results,
+except the x-y tiled momentum shapes, which agree with the fused kernel to
+round-off; ``--no-autotune`` keeps the defaults, ``--launch-shapes`` sets
+given ones).
 
 ``--precision mixed`` times the mixed variant (FP32 fields, stencils and
 FFTs; FP64 low-wavenumber column solves and bulk-flow update) instead.  For
