@@ -81,3 +81,26 @@ def test_bench_refuses_more_than_one_gpu():
     r = subprocess.run([sys.executable, os.path.join(ROOT, "bench.py"), "--gpus", "2"],
                        capture_output=True, text=True, timeout=120)
     assert r.returncode == 2 and "one GPU" in r.stderr
+
+
+def _run_py(code):
+    env = dict(os.environ)
+    env["PYTHONPATH"] = ROOT + os.pathsep + env.get("PYTHONPATH", "")
+    return subprocess.run([sys.executable, "-W", "always", "-c", code], capture_output=True, text=True, env=env,
+                          cwd=ROOT, timeout=300)
+
+
+@pytest.mark.skipif(not os.path.exists(LIB_PATH), reason="HIP library not built")
+def test_torch_binding_warns_when_the_library_was_loaded_first():
+    """torch ships a HIP runtime under the library's soname; the binding warns
+    when the library's runtime was loaded before torch, and is silent when
+    torch came first (no GPU is touched: loading the library creates no
+    context)."""
+    pytest.importorskip("torch")
+    late = _run_py("from hercules_amd.channel import load_library; load_library(); import hercules_amd.autograd")
+    assert late.returncode == 0, late.stderr
+    assert "torch is imported after the HIP library" in late.stderr
+    early = _run_py("import torch; from hercules_amd.channel import load_library; load_library(); "
+                    "import hercules_amd.autograd")
+    assert early.returncode == 0, early.stderr
+    assert "torch is imported after the HIP library" not in early.stderr
