@@ -590,7 +590,10 @@ def main():
         traffic = dict(tj.get("bytes_per_launch", {}))
         # the right-hand-side stage's bytes depend on the momentum tile shape
         shapes = getattr(ch, "tuning", None) or {}
-        if "launch_shapes" in tj and shapes.get("rhs_xt", 0) != tj["launch_shapes"].get("rhs_xt", 0):
+        by_shape = tj.get("rhs_by_shape", {})
+        if str(shapes.get("rhs_xt", 0)) in by_shape:
+            traffic["rhs"] = by_shape[str(shapes.get("rhs_xt", 0))]
+        elif "launch_shapes" in tj and shapes.get("rhs_xt", 0) != tj["launch_shapes"].get("rhs_xt", 0):
             traffic.pop("rhs", None)
     roofline = None
     if kernels:
