@@ -472,16 +472,16 @@ __global__ __launch_bounds__(kRhsThreads) void k_rhs_mom_damped(StepArgsT<typena
 struct DampFields {
     const double *f[5];
 };
-__global__ __launch_bounds__(256) void k_damp_means(DampFields F, double *dmp_m, int nx, int ny, int nz, int kdamp0)
+__global__ __launch_bounds__(1024) void k_damp_means(DampFields F, double *dmp_m, int nx, int ny, int nz, int kdamp0)
 {
-    __shared__ double red[256];
+    __shared__ double red[1024];
     const int kk = kdamp0 + blockIdx.x, fid = blockIdx.y;
     const double *plane = F.f[fid] + (size_t)kk * nx * ny;
     double s = 0.0;
-    for (int q = threadIdx.x; q < nx * ny; q += 256) s += plane[q];
+    for (int q = threadIdx.x; q < nx * ny; q += 1024) s += plane[q];
     red[threadIdx.x] = s;
     __syncthreads();
-    for (int w = 128; w > 0; w >>= 1) {
+    for (int w = 512; w > 0; w >>= 1) {
         if (threadIdx.x < w) red[threadIdx.x] += red[threadIdx.x + w];
         __syncthreads();
     }
@@ -522,7 +522,7 @@ int rhs_momentum(hc_ctx *c, const StepArgs &A, StepArgsT<R> &B)
             // rayleigh_damping of u, v, w, p, t at the start of the substep
             // (src/module_navier_stokes.f90:258-266)
             DampFields F = {{c->u, c->v, c->w, c->p, c->t}};
-            hipLaunchKernelGGL(k_damp_means, dim3(P.nzdamp + 1, 5), dim3(256), 0, c->stream, F, c->dmp_m, c->nx, c->ny,
+            hipLaunchKernelGGL(k_damp_means, dim3(P.nzdamp + 1, 5), dim3(1024), 0, c->stream, F, c->dmp_m, c->nx, c->ny,
                                c->nz, c->kdamp0);
             if (P.isscalar == 0) {
                 hipLaunchKernelGGL(k_damp_apply, dim3(256), dim3(256), 0, c->stream, c->t, c->dmp_s,
